@@ -1,0 +1,882 @@
+/*
+ * afivo_oracle.c -- CPU restatement of afivo's FAS multigrid hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: this is the checker the HIP path is compared
+ * against (tests, the smoke check and the CPU baseline of bench.py).  The
+ * product library never links or calls it.
+ *
+ * Each function restates one reference routine, cited as file:line in the
+ * reference's dimension-generic sources (src/m_aX_*.f90).  Floating-point
+ * expressions keep the reference's left-to-right evaluation order and are
+ * compiled without contraction (-ffp-contract=off), so that results can be
+ * compared bit for bit with the reference built by flang -O2.
+ *
+ * Box arrays use the Fortran layout of box%cc(0:n+1, 0:n+1[, 0:n+1], iv):
+ * x fastest.
+ */
+#include "afivo_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+struct orc_tree {
+    int dim, nc, n_var, nch, nnb;
+    int min_lvl, highest_lvl, n_boxes, max_id;
+    double dr_base;
+    /* per id, 0..max_id */
+    int *lvl, *ix, *parent, *children, *nbr, *ncell, *tag;
+    double *dr;
+    size_t *off;           /* offset (doubles) of the box array in data[iv] */
+    size_t n_values;
+    double **data;         /* data[iv-1] */
+    int *rec_ids;          /* ids in record order */
+    int *lvl_start;        /* record-order ids of level l: [lvl_start[l-min], lvl_start[l-min+1]) */
+    int *bc_type;          /* per (id, nb) */
+    double **bc_val;       /* per (id, nb), NULL means b = 0 */
+};
+
+#define TAG_UNSET 0
+#define TAG_NORMAL 1   /* mg_normal_box, m_aX_mg.f90:14 */
+#define TAG_CEPS 3     /* mg_ceps_box, m_aX_mg.f90:16 */
+#define TAG_VEPS 4     /* mg_veps_box, m_aX_mg.f90:17 */
+
+static int box_len(const orc_tree *t, int n)
+{
+    int m = n + 2;
+    return t->dim == 2 ? m * m : m * m * m;
+}
+
+static double *cc(const orc_tree *t, int id, int iv)
+{
+    return t->data[iv - 1] + t->off[id];
+}
+
+/* Linear index of cell (i, j, k) in a box with n cells per dimension. */
+static inline int ix3(int n, int i, int j, int k)
+{
+    return i + (n + 2) * (j + (n + 2) * k);
+}
+
+/* --------------------------------------------------------------------- */
+/* tree set-up                                                            */
+
+orc_tree *orc_create(int dim, int nc, int n_var, double dr_base, int n_boxes,
+                     const int32_t *recs)
+{
+    if ((dim != 2 && dim != 3) || nc < 2 || (nc & 1) || n_var < 1 || n_boxes < 1)
+        return NULL;
+    orc_tree *t = calloc(1, sizeof(*t));
+    t->dim = dim;
+    t->nc = nc;
+    t->n_var = n_var;
+    t->nch = 1 << dim;
+    t->nnb = 2 * dim;
+    t->dr_base = dr_base;
+    t->n_boxes = n_boxes;
+    int reclen = 3 + dim + t->nch + t->nnb;
+    int max_id = 0, min_lvl = 1 << 30, hi_lvl = -(1 << 30);
+    for (int r = 0; r < n_boxes; r++) {
+        const int32_t *q = recs + (size_t)r * reclen;
+        if (q[0] > max_id) max_id = q[0];
+        if (q[1] < min_lvl) min_lvl = q[1];
+        if (q[1] > hi_lvl) hi_lvl = q[1];
+    }
+    t->max_id = max_id;
+    t->min_lvl = min_lvl;
+    t->highest_lvl = hi_lvl;
+    size_t n = (size_t)max_id + 1;
+    t->lvl = calloc(n, sizeof(int));
+    t->ix = calloc(n * 3, sizeof(int));
+    t->parent = calloc(n, sizeof(int));
+    t->children = calloc(n * 8, sizeof(int));
+    t->nbr = calloc(n * 6, sizeof(int));
+    t->ncell = calloc(n, sizeof(int));
+    t->tag = calloc(n, sizeof(int));
+    t->dr = calloc(n, sizeof(double));
+    t->off = calloc(n, sizeof(size_t));
+    t->rec_ids = calloc((size_t)n_boxes, sizeof(int));
+    t->bc_type = calloc(n * 6, sizeof(int));
+    t->bc_val = calloc(n * 6, sizeof(double *));
+    size_t off = 0;
+    for (int r = 0; r < n_boxes; r++) {
+        const int32_t *q = recs + (size_t)r * reclen;
+        int id = q[0];
+        t->rec_ids[r] = id;
+        t->lvl[id] = q[1];
+        for (int d = 0; d < dim; d++) t->ix[id * 3 + d] = q[2 + d];
+        t->parent[id] = q[2 + dim];
+        for (int c = 0; c < t->nch; c++) t->children[id * 8 + c] = q[3 + dim + c];
+        for (int f = 0; f < t->nnb; f++) {
+            t->nbr[id * 6 + f] = q[3 + dim + t->nch + f];
+            t->bc_type[id * 6 + f] = ORC_BC_DIRICHLET;   /* a$D_bc_dirichlet_zero */
+        }
+        int l = q[1];
+        /* a$D_n_cell, m_aX_utils.f90:676-686; box%dr, m_aX_core.f90:196,764 */
+        t->ncell[id] = l >= 1 ? nc : nc / (1 << (1 - l));
+        t->dr[id] = dr_base * pow(0.5, (double)(l - 1));
+        t->off[id] = off;
+        off += (size_t)box_len(t, t->ncell[id]);
+    }
+    t->n_values = off;
+    t->data = calloc((size_t)n_var, sizeof(double *));
+    for (int v = 0; v < n_var; v++) t->data[v] = calloc(off, sizeof(double));
+    int nl = hi_lvl - min_lvl + 1;
+    t->lvl_start = calloc((size_t)nl + 1, sizeof(int));
+    /* records are required in level order */
+    int r = 0;
+    for (int l = min_lvl; l <= hi_lvl; l++) {
+        t->lvl_start[l - min_lvl] = r;
+        while (r < n_boxes && t->lvl[t->rec_ids[r]] == l) r++;
+    }
+    t->lvl_start[nl] = r;
+    if (r != n_boxes) {
+        orc_destroy(t);
+        return NULL;
+    }
+    return t;
+}
+
+void orc_destroy(orc_tree *t)
+{
+    if (!t) return;
+    if (t->data)
+        for (int v = 0; v < t->n_var; v++) free(t->data[v]);
+    if (t->bc_val)
+        for (size_t i = 0; i < ((size_t)t->max_id + 1) * 6; i++) free(t->bc_val[i]);
+    free(t->data); free(t->bc_val); free(t->bc_type);
+    free(t->lvl); free(t->ix); free(t->parent); free(t->children); free(t->nbr);
+    free(t->ncell); free(t->tag); free(t->dr); free(t->off); free(t->rec_ids);
+    free(t->lvl_start);
+    free(t);
+}
+
+int orc_n_values(const orc_tree *t) { return (int)t->n_values; }
+
+int orc_set_cc(orc_tree *t, int iv, const double *packed)
+{
+    if (iv < 1 || iv > t->n_var) return -1;
+    memcpy(t->data[iv - 1], packed, t->n_values * sizeof(double));
+    return 0;
+}
+
+int orc_get_cc(const orc_tree *t, int iv, double *packed)
+{
+    if (iv < 1 || iv > t->n_var) return -1;
+    memcpy(packed, t->data[iv - 1], t->n_values * sizeof(double));
+    return 0;
+}
+
+int orc_set_bc(orc_tree *t, int id, int nb, int bc_type, const double *b)
+{
+    if (id < 1 || id > t->max_id || nb < 1 || nb > t->nnb) return -1;
+    if (bc_type != ORC_BC_DIRICHLET && bc_type != ORC_BC_NEUMANN) return -1;
+    size_t k = (size_t)id * 6 + (nb - 1);
+    t->bc_type[k] = bc_type;
+    free(t->bc_val[k]);
+    t->bc_val[k] = NULL;
+    if (b) {
+        int n = t->ncell[id];
+        int m = t->dim == 2 ? n : n * n;
+        t->bc_val[k] = malloc((size_t)m * sizeof(double));
+        memcpy(t->bc_val[k], b, (size_t)m * sizeof(double));
+    }
+    return 0;
+}
+
+static int n_ids(const orc_tree *t, int lvl)
+{
+    if (lvl < t->min_lvl || lvl > t->highest_lvl) return 0;
+    return t->lvl_start[lvl - t->min_lvl + 1] - t->lvl_start[lvl - t->min_lvl];
+}
+
+static int level_id(const orc_tree *t, int lvl, int i)
+{
+    return t->rec_ids[t->lvl_start[lvl - t->min_lvl] + i];
+}
+
+/* a$D_has_children, m_aX_t.f90:355-358 */
+static int has_children(const orc_tree *t, int id) { return t->children[id * 8] != 0; }
+
+/* a$D_get_child_offset, m_aX_t.f90:362-374 */
+static void child_offset(const orc_tree *t, int id, int off[3])
+{
+    off[0] = off[1] = off[2] = 0;
+    if (t->lvl[id] > 1)
+        for (int d = 0; d < t->dim; d++)
+            off[d] = ((t->ix[id * 3 + d] - 1) & 1) * (t->ncell[id] >> 1);
+}
+
+/* --------------------------------------------------------------------- */
+/* ghost cells                                                            */
+
+/* Cell (i, j, k) addressed through a permutation: a[dn] is the normal
+ * coordinate of face direction dn, a[t1], a[t2] the tangential ones. */
+static inline int face_ix(int n, int dn, int g, int t1v, int t2v)
+{
+    int a[3] = {0, 0, 0};
+    a[dn] = g;
+    a[dn == 0 ? 1 : 0] = t1v;
+    a[dn == 2 ? 1 : 2] = t2v;
+    return ix3(n, a[0], a[1], a[2]);
+}
+
+/* sides_from_nb, m_aX_gc.f90:381-415: copy the neighbour's adjacent layer. */
+static void sides_from_nb(orc_tree *t, int id, int nb_id, int f, int iv)
+{
+    int n = t->ncell[id], dn = f / 2, high = f % 2;
+    double *c = cc(t, id, iv);
+    const double *s = cc(t, nb_id, iv);
+    int g = high ? n + 1 : 0, src = high ? 1 : n;
+    int n2 = t->dim == 3 ? n : 1;
+    for (int b = 1; b <= n2; b++)
+        for (int a = 1; a <= n; a++) {
+            int bb = t->dim == 3 ? b : 0;
+            c[face_ix(n, dn, g, a, bb)] = s[face_ix(n, dn, src, a, bb)];
+        }
+}
+
+/* a$D_prolong0 restricted to one ghost layer (a$D_gc_prolong0,
+ * m_aX_gc.f90:135-151; a$D_prolong0, m_aX_prolong.f90:39-108). */
+static void gc_prolong0(orc_tree *t, int id, int f, int iv)
+{
+    int n = t->ncell[id], dn = f / 2, high = f % 2;
+    int p = t->parent[id], np = t->ncell[p];
+    int off[3];
+    child_offset(t, id, off);
+    double *c = cc(t, id, iv);
+    const double *pc = cc(t, p, iv);
+    int lo[3] = {1, 1, 1}, hi[3] = {n, n, n};
+    if (t->dim == 2) lo[2] = hi[2] = 0;
+    lo[dn] = hi[dn] = high * (n + 1);
+    for (int k = lo[2]; k <= hi[2]; k++) {
+        int kc = t->dim == 3 ? off[2] + ((k + 1) >> 1) : 0;
+        for (int j = lo[1]; j <= hi[1]; j++) {
+            int jc = off[1] + ((j + 1) >> 1);
+            for (int i = lo[0]; i <= hi[0]; i++) {
+                int ic = off[0] + ((i + 1) >> 1);
+                c[ix3(n, i, j, k)] = pc[ix3(np, ic, jc, kc)];
+            }
+        }
+    }
+}
+
+/* mg$D_sides_rb, m_aX_mg.f90:262-350 */
+static void sides_rb(orc_tree *t, int id, int f, int iv)
+{
+    int n = t->ncell[id], dn = f / 2, high = f % 2;
+    int ixn = high ? n : 1, dix = high ? -1 : 1;
+    gc_prolong0(t, id, f, iv);
+    double *c = cc(t, id, iv);
+    if (t->dim == 2) {
+        for (int tt = 1; tt <= n; tt++) {
+            int dt = -1 + 2 * (tt & 1);
+            int i, j, di, dj;
+            if (dn == 0) { i = ixn; di = dix; j = tt; dj = dt; }
+            else         { j = ixn; dj = dix; i = tt; di = dt; }
+            int g = dn == 0 ? ix3(n, i - di, j, 0) : ix3(n, i, j - dj, 0);
+            c[g] = 0.5 * c[g] + c[ix3(n, i, j, 0)] -
+                   0.25 * (c[ix3(n, i + di, j, 0)] + c[ix3(n, i, j + dj, 0)]);
+        }
+    } else {
+        for (int b = 1; b <= n; b++)
+            for (int a = 1; a <= n; a++) {
+                int idx[3], dd[3];
+                /* tangential coordinates in increasing dimension order */
+                int t1 = dn == 0 ? 1 : 0, t2 = dn == 2 ? 1 : 2;
+                idx[dn] = ixn; dd[dn] = dix;
+                idx[t1] = a; dd[t1] = -1 + 2 * (a & 1);
+                idx[t2] = b; dd[t2] = -1 + 2 * (b & 1);
+                int gi[3] = {idx[0], idx[1], idx[2]};
+                gi[dn] -= dd[dn];
+                int g = ix3(n, gi[0], gi[1], gi[2]);
+                c[g] = 0.5 * c[g] + 0.75 * c[ix3(n, idx[0], idx[1], idx[2])] -
+                       0.25 * c[ix3(n, idx[0] + dd[0], idx[1] + dd[1], idx[2] + dd[2])];
+            }
+    }
+}
+
+/* sides_bc (captured boundary values, e.g. a$D_bc_dirichlet_zero,
+ * m_aX_gc.f90:371-379 via a$D_set_box_gc, m_aX_t.f90:436-505) followed by
+ * bc_to_gc, m_aX_gc.f90:71-132. */
+static void sides_bc(orc_tree *t, int id, int f, int iv)
+{
+    int n = t->ncell[id], dn = f / 2, high = f % 2;
+    size_t k = (size_t)id * 6 + f;
+    double c1, c2;
+    if (t->bc_type[k] == ORC_BC_DIRICHLET) {
+        c1 = -1;
+        c2 = 2;
+    } else {
+        c1 = 1;
+        c2 = t->dr[id] * (high ? 1 : -1);   /* a$D_neighb_high_pm */
+    }
+    const double *b = t->bc_val[k];
+    double *c = cc(t, id, iv);
+    int g = high ? n + 1 : 0, in = high ? n : 1;
+    int n2 = t->dim == 3 ? n : 1;
+    for (int bb = 1; bb <= n2; bb++)
+        for (int a = 1; a <= n; a++) {
+            int z = t->dim == 3 ? bb : 0;
+            int gi = face_ix(n, dn, g, a, z);
+            c[gi] = b ? b[(a - 1) + n * (bb - 1)] : 0.0;
+            c[gi] = c2 * c[gi] + c1 * c[face_ix(n, dn, in, a, z)];
+        }
+}
+
+/* a$D_gc_box, m_aX_gc.f90:50-69, with subr_rb = mg$D_sides_rb */
+static void gc_box(orc_tree *t, int id, int iv)
+{
+    for (int f = 0; f < t->nnb; f++) {
+        int nb_id = t->nbr[id * 6 + f];
+        if (nb_id > 0)
+            sides_from_nb(t, id, nb_id, f, iv);
+        else if (nb_id == 0)
+            sides_rb(t, id, f, iv);
+        else
+            sides_bc(t, id, f, iv);
+    }
+}
+
+/* fill_gc_phi, m_aX_mg.f90:352-365 (over all boxes of a level) */
+int orc_fill_gc_level(orc_tree *t, int iv, int lvl)
+{
+    for (int i = 0; i < n_ids(t, lvl); i++) gc_box(t, level_id(t, lvl, i), iv);
+    return 0;
+}
+
+/* --------------------------------------------------------------------- */
+/* transfer operators                                                     */
+
+/* a$D_restrict_box, m_aX_restrict.f90:67-132 (Cartesian branch) */
+static void restrict_box(orc_tree *t, int c_id, int p_id, int iv, int iv_to)
+{
+    int n = t->ncell[c_id], np = t->ncell[p_id], hnc = n >> 1;
+    int off[3];
+    child_offset(t, c_id, off);
+    const double *c = cc(t, c_id, iv);
+    double *p = cc(t, p_id, iv_to);
+    if (t->dim == 2) {
+        for (int j = 1; j <= hnc; j++)
+            for (int i = 1; i <= hnc; i++) {
+                int f = 2 * i - 1, g = 2 * j - 1;
+                double s = c[ix3(n, f, g, 0)];
+                s += c[ix3(n, f + 1, g, 0)];
+                s += c[ix3(n, f, g + 1, 0)];
+                s += c[ix3(n, f + 1, g + 1, 0)];
+                p[ix3(np, off[0] + i, off[1] + j, 0)] = 0.25 * s;
+            }
+    } else {
+        for (int k = 1; k <= hnc; k++)
+            for (int j = 1; j <= hnc; j++)
+                for (int i = 1; i <= hnc; i++) {
+                    int f = 2 * i - 1, g = 2 * j - 1, h = 2 * k - 1;
+                    double s = c[ix3(n, f, g, h)];
+                    s += c[ix3(n, f + 1, g, h)];
+                    s += c[ix3(n, f, g + 1, h)];
+                    s += c[ix3(n, f + 1, g + 1, h)];
+                    s += c[ix3(n, f, g, h + 1)];
+                    s += c[ix3(n, f + 1, g, h + 1)];
+                    s += c[ix3(n, f, g + 1, h + 1)];
+                    s += c[ix3(n, f + 1, g + 1, h + 1)];
+                    p[ix3(np, off[0] + i, off[1] + j, off[2] + k)] = 0.125 * s;
+                }
+    }
+}
+
+/* a$D_prolong2, m_aX_prolong.f90:243-348 */
+static void prolong2(orc_tree *t, int p_id, int c_id, int iv, int ivc, int add)
+{
+    int n = t->ncell[c_id], np = t->ncell[p_id], hnc = n >> 1;
+    int off[3];
+    child_offset(t, c_id, off);
+    const double *p = cc(t, p_id, iv);
+    double *c = cc(t, c_id, ivc);
+    if (!add) {
+        int kk = t->dim == 3 ? n : 0;
+        for (int k = t->dim == 3 ? 1 : 0; k <= kk; k++)
+            for (int j = 1; j <= n; j++)
+                for (int i = 1; i <= n; i++) c[ix3(n, i, j, k)] = 0;
+    }
+    if (t->dim == 2) {
+        for (int j = 1; j <= hnc; j++) {
+            int jc = j + off[1], jf = 2 * j - 1;
+            for (int i = 1; i <= hnc; i++) {
+                int ic = i + off[0], i_f = 2 * i - 1;
+                double f0 = p[ix3(np, ic, jc, 0)];
+                double fx = 0.125 * (p[ix3(np, ic + 1, jc, 0)] - p[ix3(np, ic - 1, jc, 0)]);
+                double fy = 0.125 * (p[ix3(np, ic, jc + 1, 0)] - p[ix3(np, ic, jc - 1, 0)]);
+                double fxx = 0.03125 * (p[ix3(np, ic - 1, jc, 0)] - 2 * f0 + p[ix3(np, ic + 1, jc, 0)]);
+                double fyy = 0.03125 * (p[ix3(np, ic, jc - 1, 0)] - 2 * f0 + p[ix3(np, ic, jc + 1, 0)]);
+                double f2 = fxx + fyy;
+                double *q;
+                q = &c[ix3(n, i_f, jf, 0)];         *q = f0 - fx - fy + f2 + *q;
+                q = &c[ix3(n, i_f + 1, jf, 0)];     *q = f0 + fx - fy + f2 + *q;
+                q = &c[ix3(n, i_f, jf + 1, 0)];     *q = f0 - fx + fy + f2 + *q;
+                q = &c[ix3(n, i_f + 1, jf + 1, 0)]; *q = f0 + fx + fy + f2 + *q;
+            }
+        }
+    } else {
+        for (int k = 1; k <= hnc; k++) {
+            int kc = k + off[2], kf = 2 * k - 1;
+            for (int j = 1; j <= hnc; j++) {
+                int jc = j + off[1], jf = 2 * j - 1;
+                for (int i = 1; i <= hnc; i++) {
+                    int ic = i + off[0], i_f = 2 * i - 1;
+                    double f0 = p[ix3(np, ic, jc, kc)];
+                    double fx = 0.125 * (p[ix3(np, ic + 1, jc, kc)] - p[ix3(np, ic - 1, jc, kc)]);
+                    double fy = 0.125 * (p[ix3(np, ic, jc + 1, kc)] - p[ix3(np, ic, jc - 1, kc)]);
+                    double fz = 0.125 * (p[ix3(np, ic, jc, kc + 1)] - p[ix3(np, ic, jc, kc - 1)]);
+                    double fxx = 0.03125 * (p[ix3(np, ic - 1, jc, kc)] - 2 * f0 + p[ix3(np, ic + 1, jc, kc)]);
+                    double fyy = 0.03125 * (p[ix3(np, ic, jc - 1, kc)] - 2 * f0 + p[ix3(np, ic, jc + 1, kc)]);
+                    double fzz = 0.03125 * (p[ix3(np, ic, jc, kc - 1)] - 2 * f0 + p[ix3(np, ic, jc, kc + 1)]);
+                    double f2 = fxx + fyy + fzz;
+                    for (int s = 0; s < 8; s++) {
+                        int sx = s & 1, sy = (s >> 1) & 1, sz = (s >> 2) & 1;
+                        double v = sx ? f0 + fx : f0 - fx;
+                        v = sy ? v + fy : v - fy;
+                        v = sz ? v + fz : v - fz;
+                        double *q = &c[ix3(n, i_f + sx, jf + sy, kf + sz)];
+                        *q = v + f2 + *q;
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* --------------------------------------------------------------------- */
+/* operators                                                              */
+
+/* mg$D_set_box_tag, m_aX_mg.f90:667-706 (eps part; no level set) */
+static void set_box_tag(orc_tree *t, int id, const orc_mg_opts *o)
+{
+    const double *e = cc(t, id, o->i_eps);
+    int len = box_len(t, t->ncell[id]);
+    double a = e[0], b = e[0];
+    for (int q = 1; q < len; q++) {
+        if (e[q] < a) a = e[q];
+        if (e[q] > b) b = e[q];
+    }
+    int is_deps = b > a, is_eps = 0;
+    if (!is_deps) is_eps = (a < 1 || a > 1);
+    t->tag[id] = TAG_NORMAL;
+    if (is_eps) t->tag[id] = TAG_CEPS;
+    if (is_deps) t->tag[id] = TAG_VEPS;
+}
+
+static int use_lpld(orc_tree *t, int id, const orc_mg_opts *o)
+{
+    if (o->op != 1) return 0;
+    if (t->tag[id] == TAG_UNSET) set_box_tag(t, id, o);   /* lazy, m_aX_mg.f90:567,602,655 */
+    return t->tag[id] != TAG_NORMAL;
+}
+
+/* mg$D_box_gsrb_lpl, m_aX_mg.f90:718-760 */
+static void gsrb_lpl(orc_tree *t, int id, int rb, const orc_mg_opts *o)
+{
+    int n = t->ncell[id];
+    double dx2 = t->dr[id] * t->dr[id];
+    double *p = cc(t, id, o->i_phi);
+    const double *r = cc(t, id, o->i_rhs);
+    const double sixth = 1 / 6.0;
+    if (t->dim == 2) {
+        for (int j = 1; j <= n; j++) {
+            int i0 = 2 - ((rb ^ j) & 1);
+            for (int i = i0; i <= n; i += 2) {
+                int q = ix3(n, i, j, 0);
+                p[q] = 0.25 * (p[q + 1] + p[q - 1] + p[q + (n + 2)] + p[q - (n + 2)] -
+                               dx2 * r[q]);
+            }
+        }
+    } else {
+        int s = n + 2, s2 = s * s;
+        for (int k = 1; k <= n; k++)
+            for (int j = 1; j <= n; j++) {
+                int i0 = 2 - ((rb ^ (k + j)) & 1);
+                for (int i = i0; i <= n; i += 2) {
+                    int q = ix3(n, i, j, k);
+                    p[q] = sixth * (p[q + 1] + p[q - 1] + p[q + s] + p[q - s] + p[q + s2] +
+                                    p[q - s2] - dx2 * r[q]);
+                }
+            }
+    }
+}
+
+/* mg$D_box_gsrb_lpld, m_aX_mg.f90:810-863 */
+static void gsrb_lpld(orc_tree *t, int id, int rb, const orc_mg_opts *o)
+{
+    int n = t->ncell[id];
+    double dx2 = t->dr[id] * t->dr[id];
+    double *p = cc(t, id, o->i_phi);
+    const double *r = cc(t, id, o->i_rhs);
+    const double *e = cc(t, id, o->i_eps);
+    int s = n + 2, s2 = s * s;
+    int nn = 2 * t->dim;
+    int dq[6] = {-1, 1, -s, s, -s2, s2};
+    int kk = t->dim == 3 ? n : 0;
+    for (int k = t->dim == 3 ? 1 : 0; k <= kk; k++)
+        for (int j = 1; j <= n; j++) {
+            int i0 = 2 - ((rb ^ (t->dim == 3 ? k + j : j)) & 1);
+            for (int i = i0; i <= n; i += 2) {
+                int q = ix3(n, i, j, k);
+                double a0 = e[q], c[6] = {0, 0, 0, 0, 0, 0}, su, sc;
+                for (int m = 0; m < nn; m++) c[m] = 2 * a0 * e[q + dq[m]] / (a0 + e[q + dq[m]]);
+                su = c[0] * p[q + dq[0]];
+                for (int m = 1; m < nn; m++) su += c[m] * p[q + dq[m]];
+                sc = c[0];
+                for (int m = 1; m < nn; m++) sc += c[m];
+                p[q] = (su - dx2 * r[q]) / sc;
+            }
+        }
+}
+
+/* mg$D_box_lpl, m_aX_mg.f90:763-797 */
+static void box_lpl(orc_tree *t, int id, int i_out, const orc_mg_opts *o)
+{
+    int n = t->ncell[id];
+    double inv_dr_sq = 1 / (t->dr[id] * t->dr[id]);
+    const double *p = cc(t, id, o->i_phi);
+    double *out = cc(t, id, i_out);
+    int s = n + 2, s2 = s * s;
+    if (t->dim == 2) {
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                int q = ix3(n, i, j, 0);
+                out[q] = inv_dr_sq * (p[q - 1] + p[q + 1] + p[q - s] + p[q + s] - 4 * p[q]);
+            }
+    } else {
+        for (int k = 1; k <= n; k++)
+            for (int j = 1; j <= n; j++)
+                for (int i = 1; i <= n; i++) {
+                    int q = ix3(n, i, j, k);
+                    out[q] = inv_dr_sq * (p[q - 1] + p[q + 1] + p[q - s] + p[q + s] +
+                                          p[q - s2] + p[q + s2] - 6 * p[q]);
+                }
+    }
+}
+
+/* mg$D_box_lpld, m_aX_mg.f90:866-916 */
+static void box_lpld(orc_tree *t, int id, int i_out, const orc_mg_opts *o)
+{
+    int n = t->ncell[id];
+    double inv_dr_sq = 1 / (t->dr[id] * t->dr[id]);
+    const double *p = cc(t, id, o->i_phi);
+    const double *e = cc(t, id, o->i_eps);
+    double *out = cc(t, id, i_out);
+    int s = n + 2, s2 = s * s;
+    int nn = 2 * t->dim;
+    int dq[6] = {-1, 1, -s, s, -s2, s2};
+    int kk = t->dim == 3 ? n : 0;
+    for (int k = t->dim == 3 ? 1 : 0; k <= kk; k++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                int q = ix3(n, i, j, k);
+                double u0 = p[q], a0 = e[q], sum;
+                double a = e[q + dq[0]];
+                sum = a0 * a / (a0 + a) * (p[q + dq[0]] - u0);
+                for (int m = 1; m < nn; m++) {
+                    a = e[q + dq[m]];
+                    sum += a0 * a / (a0 + a) * (p[q + dq[m]] - u0);
+                }
+                out[q] = inv_dr_sq * 2 * sum;
+            }
+}
+
+/* mg$D_box_corr_lpld, m_aX_mg.f90:918-985 */
+static void corr_lpld(orc_tree *t, int p_id, int c_id, const orc_mg_opts *o)
+{
+    int n = t->ncell[c_id], np = t->ncell[p_id];
+    int off[3];
+    child_offset(t, c_id, off);
+    const double *pc = cc(t, p_id, o->i_tmp);
+    const double *pe = cc(t, p_id, o->i_eps);
+    double *c = cc(t, c_id, o->i_phi);
+    const double third = 1 / 3.0;
+    int kk = t->dim == 3 ? n : 0;
+    for (int k = t->dim == 3 ? 1 : 0; k <= kk; k++) {
+        int kc1 = 0, kc2 = 0;
+        if (t->dim == 3) {
+            kc1 = off[2] + ((k + 1) >> 1);
+            kc2 = kc1 + 1 - 2 * (k & 1);
+        }
+        for (int j = 1; j <= n; j++) {
+            int jc1 = off[1] + ((j + 1) >> 1), jc2 = jc1 + 1 - 2 * (j & 1);
+            for (int i = 1; i <= n; i++) {
+                int ic1 = off[0] + ((i + 1) >> 1), ic2 = ic1 + 1 - 2 * (i & 1);
+                int q0 = ix3(np, ic1, jc1, kc1);
+                int qn[3] = {ix3(np, ic2, jc1, kc1), ix3(np, ic1, jc2, kc1), ix3(np, ic1, jc1, kc2)};
+                double u0 = pc[q0], a0 = pe[q0];
+                double *q = &c[ix3(n, i, j, k)];
+                if (t->dim == 2) {
+                    double s = (a0 * u0 + pe[qn[0]] * pc[qn[0]]) / (a0 + pe[qn[0]]);
+                    s += (a0 * u0 + pe[qn[1]] * pc[qn[1]]) / (a0 + pe[qn[1]]);
+                    *q = *q + 0.5 * s;
+                } else {
+                    double s = (a0 * u0 + pe[qn[0]] * (1.5 * pc[qn[0]] - 0.5 * u0)) / (a0 + pe[qn[0]]);
+                    for (int m = 1; m < 3; m++)
+                        s += (a0 * u0 + pe[qn[m]] * (1.5 * pc[qn[m]] - 0.5 * u0)) / (a0 + pe[qn[m]]);
+                    *q = *q + third * s;
+                }
+            }
+        }
+    }
+}
+
+/* mg%box_gsrb: mg$D_box_gsrb_lpl or mg$D_auto_gsrb (m_aX_mg.f90:562-590) */
+static void box_gsrb(orc_tree *t, int id, int rb, const orc_mg_opts *o)
+{
+    if (use_lpld(t, id, o))
+        gsrb_lpld(t, id, rb, o);
+    else
+        gsrb_lpl(t, id, rb, o);
+}
+
+/* mg%box_op: mg$D_box_lpl or mg$D_auto_op (m_aX_mg.f90:593-627) */
+static void box_op(orc_tree *t, int id, int i_out, const orc_mg_opts *o)
+{
+    if (use_lpld(t, id, o))
+        box_lpld(t, id, i_out, o);
+    else
+        box_lpl(t, id, i_out, o);
+}
+
+/* mg%box_corr: mg$D_box_corr_lpl (m_aX_mg.f90:708-715) or mg$D_auto_corr
+ * (m_aX_mg.f90:646-665); the tag is that of the child. */
+static void box_corr(orc_tree *t, int p_id, int c_id, const orc_mg_opts *o)
+{
+    if (use_lpld(t, c_id, o))
+        corr_lpld(t, p_id, c_id, o);
+    else
+        prolong2(t, p_id, c_id, o->i_tmp, o->i_phi, 1);
+}
+
+/* residual_box, m_aX_mg.f90:545-559 */
+static void residual_box(orc_tree *t, int id, const orc_mg_opts *o)
+{
+    box_op(t, id, o->i_tmp, o);
+    int n = t->ncell[id];
+    double *tm = cc(t, id, o->i_tmp);
+    const double *r = cc(t, id, o->i_rhs);
+    int kk = t->dim == 3 ? n : 0;
+    for (int k = t->dim == 3 ? 1 : 0; k <= kk; k++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                int q = ix3(n, i, j, k);
+                tm[q] = r[q] - tm[q];
+            }
+}
+
+/* --------------------------------------------------------------------- */
+/* multigrid stages                                                       */
+
+/* gsrb_boxes, m_aX_mg.f90:397-420 */
+int orc_gsrb_level(orc_tree *t, const orc_mg_opts *o, int lvl, int n_cycle)
+{
+    int nb = n_ids(t, lvl);
+    for (int rb = 1; rb <= 2 * n_cycle; rb++) {
+        for (int i = 0; i < nb; i++) box_gsrb(t, level_id(t, lvl, i), rb, o);
+        for (int i = 0; i < nb; i++) gc_box(t, level_id(t, lvl, i), o->i_phi);
+    }
+    return 0;
+}
+
+int orc_residual_level(orc_tree *t, const orc_mg_opts *o, int lvl)
+{
+    for (int i = 0; i < n_ids(t, lvl); i++) residual_box(t, level_id(t, lvl, i), o);
+    return 0;
+}
+
+int orc_restrict_level(orc_tree *t, int iv, int lvl)
+{
+    for (int i = 0; i < n_ids(t, lvl); i++) {
+        int id = level_id(t, lvl, i);
+        restrict_box(t, id, t->parent[id], iv, iv);
+    }
+    return 0;
+}
+
+int orc_prolong2_level(orc_tree *t, int iv, int iv_to, int lvl, int add)
+{
+    for (int i = 0; i < n_ids(t, lvl); i++) {
+        int id = level_id(t, lvl, i);
+        prolong2(t, t->parent[id], id, iv, iv_to, add);
+    }
+    return 0;
+}
+
+static void copy_all(orc_tree *t, int id, int from, int to)
+{
+    memcpy(cc(t, id, to), cc(t, id, from), (size_t)box_len(t, t->ncell[id]) * sizeof(double));
+}
+
+/* correct_children, m_aX_mg.f90:368-395, over the parents of level lvl-1 */
+int orc_correct_children_level(orc_tree *t, const orc_mg_opts *o, int lvl)
+{
+    int pl = lvl - 1;
+    for (int i = 0; i < n_ids(t, pl); i++) {
+        int id = level_id(t, pl, i);
+        if (!has_children(t, id)) continue;
+        int len = box_len(t, t->ncell[id]);
+        double *tm = cc(t, id, o->i_tmp);
+        const double *ph = cc(t, id, o->i_phi);
+        for (int q = 0; q < len; q++) tm[q] = ph[q] - tm[q];
+        for (int c = 0; c < t->nch; c++) {
+            int c_id = t->children[id * 8 + c];
+            if (c_id == 0) continue;
+            box_corr(t, id, c_id, o);
+        }
+    }
+    return 0;
+}
+
+/* update_coarse, m_aX_mg.f90:424-480 */
+int orc_update_coarse_level(orc_tree *t, const orc_mg_opts *o, int lvl)
+{
+    int nb = n_ids(t, lvl);
+    for (int i = 0; i < nb; i++) {
+        int id = level_id(t, lvl, i), p_id = t->parent[id];
+        int n = t->ncell[id], len = box_len(t, n);
+        double *tm = cc(t, id, o->i_tmp);
+        double *save = malloc((size_t)len * sizeof(double));
+        memcpy(save, tm, (size_t)len * sizeof(double));   /* interior is what matters */
+        residual_box(t, id, o);
+        restrict_box(t, id, p_id, o->i_tmp, o->i_tmp);      /* box_rstr = lpl */
+        restrict_box(t, id, p_id, o->i_phi, o->i_phi);
+        int kk = t->dim == 3 ? n : 0;
+        for (int k = t->dim == 3 ? 1 : 0; k <= kk; k++)
+            for (int j = 1; j <= n; j++)
+                for (int ii = 1; ii <= n; ii++) {
+                    int q = ix3(n, ii, j, k);
+                    tm[q] = save[q];
+                }
+        free(save);
+    }
+    orc_fill_gc_level(t, o->i_phi, lvl - 1);
+    for (int i = 0; i < n_ids(t, lvl - 1); i++) {
+        int id = level_id(t, lvl - 1, i);
+        if (!has_children(t, id)) continue;
+        box_op(t, id, o->i_rhs, o);
+        int len = box_len(t, t->ncell[id]);
+        double *r = cc(t, id, o->i_rhs), *tm = cc(t, id, o->i_tmp);
+        for (int q = 0; q < len; q++) r[q] = r[q] + tm[q];   /* a$D_box_add_cc */
+        copy_all(t, id, o->i_phi, o->i_tmp);                 /* a$D_box_copy_cc */
+    }
+    return 0;
+}
+
+/* set_coarse_phi_rhs, m_aX_mg.f90:484-517 */
+static void set_coarse_phi_rhs(orc_tree *t, int lvl, const orc_mg_opts *o)
+{
+    if (lvl == t->highest_lvl) orc_fill_gc_level(t, o->i_phi, lvl);
+    for (int i = 0; i < n_ids(t, lvl); i++) {
+        int id = level_id(t, lvl, i), p_id = t->parent[id];
+        residual_box(t, id, o);
+        restrict_box(t, id, p_id, o->i_tmp, o->i_tmp);
+        restrict_box(t, id, p_id, o->i_phi, o->i_phi);
+    }
+    orc_fill_gc_level(t, o->i_phi, lvl - 1);
+    for (int i = 0; i < n_ids(t, lvl - 1); i++) {
+        int id = level_id(t, lvl - 1, i);
+        if (!has_children(t, id)) continue;
+        box_op(t, id, o->i_rhs, o);
+        int len = box_len(t, t->ncell[id]);
+        double *r = cc(t, id, o->i_rhs), *tm = cc(t, id, o->i_tmp);
+        for (int q = 0; q < len; q++) r[q] = r[q] + tm[q];
+    }
+}
+
+/* init_phi_rhs, m_aX_mg.f90:520-543 */
+static void init_phi_rhs(orc_tree *t, const orc_mg_opts *o)
+{
+    for (int lvl = t->highest_lvl; lvl >= t->min_lvl + 1; lvl--)
+        for (int i = 0; i < n_ids(t, lvl); i++) {
+            int id = level_id(t, lvl, i);
+            memset(cc(t, id, o->i_phi), 0, (size_t)box_len(t, t->ncell[id]) * sizeof(double));
+            restrict_box(t, id, t->parent[id], o->i_rhs, o->i_rhs);
+        }
+}
+
+static int check_opts(const orc_tree *t, const orc_mg_opts *o)
+{
+    if (o->i_phi < 1 || o->i_phi > t->n_var || o->i_rhs < 1 || o->i_rhs > t->n_var ||
+        o->i_tmp < 1 || o->i_tmp > t->n_var)
+        return -1;
+    if (o->op == 1 && (o->i_eps < 1 || o->i_eps > t->n_var)) return -1;
+    if (o->n_cycle_down < 0 || o->n_cycle_up < 0 || o->n_cycle_base < 0) return -1;
+    return 0;
+}
+
+/* mg$D_fas_vcycle, m_aX_mg.f90:208-255 */
+int orc_fas_vcycle(orc_tree *t, const orc_mg_opts *o, int highest_lvl, int set_residual)
+{
+    if (check_opts(t, o)) return -1;
+    int min_lvl = t->min_lvl;
+    for (int lvl = highest_lvl; lvl >= min_lvl + 1; lvl--) {
+        orc_gsrb_level(t, o, lvl, o->n_cycle_down);
+        orc_update_coarse_level(t, o, lvl);
+    }
+    orc_gsrb_level(t, o, min_lvl, o->n_cycle_base);
+    for (int lvl = min_lvl + 1; lvl <= highest_lvl; lvl++) {
+        orc_correct_children_level(t, o, lvl);
+        orc_fill_gc_level(t, o->i_phi, lvl);
+        orc_gsrb_level(t, o, lvl, o->n_cycle_up);
+    }
+    if (set_residual)
+        for (int lvl = min_lvl; lvl <= highest_lvl; lvl++) orc_residual_level(t, o, lvl);
+    return 0;
+}
+
+/* mg$D_fas_fmg, m_aX_mg.f90:164-203 */
+int orc_fas_fmg(orc_tree *t, const orc_mg_opts *o, int set_residual, int have_guess)
+{
+    if (check_opts(t, o)) return -1;
+    int min_lvl = t->min_lvl;
+    if (have_guess) {
+        for (int lvl = t->highest_lvl; lvl >= min_lvl + 1; lvl--) set_coarse_phi_rhs(t, lvl, o);
+    } else {
+        init_phi_rhs(t, o);
+    }
+    for (int lvl = min_lvl; lvl <= t->highest_lvl; lvl++) {
+        for (int i = 0; i < n_ids(t, lvl); i++)
+            copy_all(t, level_id(t, lvl, i), o->i_phi, o->i_tmp);   /* a$D_boxes_copy_cc */
+        if (lvl > min_lvl) {
+            orc_correct_children_level(t, o, lvl);
+            orc_fill_gc_level(t, o->i_phi, lvl);
+        }
+        orc_fas_vcycle(t, o, lvl, set_residual && lvl == t->highest_lvl);
+    }
+    return 0;
+}
+
+/* a$D_tree_max_cc / a$D_tree_min_cc, m_aX_utils.f90:468-571 */
+static double tree_ext(const orc_tree *t, int iv, int include_parents, int want_max)
+{
+    double m = want_max ? -DBL_MAX : DBL_MAX;
+    for (int lvl = t->min_lvl; lvl <= t->highest_lvl; lvl++)
+        for (int i = 0; i < n_ids(t, lvl); i++) {
+            int id = level_id(t, lvl, i);
+            if (!include_parents && has_children(t, id)) continue;
+            int n = t->ncell[id];
+            const double *c = cc(t, id, iv);
+            int kk = t->dim == 3 ? n : 0;
+            for (int k = t->dim == 3 ? 1 : 0; k <= kk; k++)
+                for (int j = 1; j <= n; j++)
+                    for (int ii = 1; ii <= n; ii++) {
+                        double v = c[ix3(n, ii, j, k)];
+                        if (want_max ? v > m : v < m) m = v;
+                    }
+        }
+    return m;
+}
+
+double orc_tree_max_cc(const orc_tree *t, int iv, int include_parents)
+{
+    return tree_ext(t, iv, include_parents, 1);
+}
+
+double orc_tree_min_cc(const orc_tree *t, int iv, int include_parents)
+{
+    return tree_ext(t, iv, include_parents, 0);
+}

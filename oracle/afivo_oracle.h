@@ -1,0 +1,63 @@
+/*
+ * afivo_oracle.h -- CPU restatement of afivo's FAS multigrid path (the oracle).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * implementation is compared against; the product library never links or
+ * calls it.  Every routine cites the reference routine it restates.
+ *
+ * Tree input is the same flat record format the product C ABI takes:
+ * one int32 record per box,
+ *     [id, lvl, ix(dim), parent, children(2^dim), neighbors(2*dim)]
+ * with afivo's conventions (1-based ids; a neighbour id > 0 is a box, 0 is a
+ * refinement boundary, < 0 a physical boundary; child 0 means no child).
+ * Cell data of variable iv is packed in record order, each box contributing
+ * its whole cc(0:nc_l+1, ..., iv) array in Fortran (x fastest) order, where
+ * nc_l = nc for lvl >= 1 and nc / 2^(1-lvl) below.
+ */
+#ifndef AFIVO_ORACLE_H
+#define AFIVO_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_tree orc_tree;
+
+typedef struct orc_mg_opts {
+    int32_t i_phi, i_rhs, i_tmp, i_eps;     /* 1-based variable indices */
+    int32_t n_cycle_down, n_cycle_up, n_cycle_base;
+    int32_t op;                             /* 0: lpl set, 1: auto (lpld) set */
+} orc_mg_opts;
+
+enum { ORC_BC_DIRICHLET = -1, ORC_BC_NEUMANN = -2 };
+
+orc_tree *orc_create(int dim, int nc, int n_var, double dr_base, int n_boxes,
+                     const int32_t *recs);
+void orc_destroy(orc_tree *t);
+int orc_n_values(const orc_tree *t);          /* doubles per packed variable */
+int orc_set_cc(orc_tree *t, int iv, const double *packed);
+int orc_get_cc(const orc_tree *t, int iv, double *packed);
+/* Physical boundary of box `id` in direction nb (1-based): bc_type and the
+ * boundary values b (nc_l^(dim-1), x fastest), or NULL for b = 0. */
+int orc_set_bc(orc_tree *t, int id, int nb, int bc_type, const double *b);
+
+int orc_fas_fmg(orc_tree *t, const orc_mg_opts *o, int set_residual, int have_guess);
+int orc_fas_vcycle(orc_tree *t, const orc_mg_opts *o, int highest_lvl, int set_residual);
+double orc_tree_max_cc(const orc_tree *t, int iv, int include_parents);
+double orc_tree_min_cc(const orc_tree *t, int iv, int include_parents);
+
+/* Single stages over one level, for stage-level parity checks. */
+int orc_fill_gc_level(orc_tree *t, int iv, int lvl);
+int orc_gsrb_level(orc_tree *t, const orc_mg_opts *o, int lvl, int n_cycle);
+int orc_residual_level(orc_tree *t, const orc_mg_opts *o, int lvl);
+int orc_restrict_level(orc_tree *t, int iv, int lvl);          /* lvl -> lvl-1 */
+int orc_prolong2_level(orc_tree *t, int iv, int iv_to, int lvl, int add); /* lvl-1 -> lvl */
+int orc_correct_children_level(orc_tree *t, const orc_mg_opts *o, int lvl);
+int orc_update_coarse_level(orc_tree *t, const orc_mg_opts *o, int lvl);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
