@@ -1,0 +1,96 @@
+"""ctypes binding of libafivo_amd.so (the C ABI of include/afivo_amd.h).
+
+The library must have been built (``python -m afivo_amd.build``); loading
+fails loudly otherwise -- there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libafivo_amd.so")
+
+P = ctypes.c_void_p
+I32 = ctypes.c_int32
+I64 = ctypes.c_int64
+F64 = ctypes.c_double
+I32P = ctypes.POINTER(ctypes.c_int32)
+I64P = ctypes.POINTER(ctypes.c_int64)
+F64P = ctypes.POINTER(ctypes.c_double)
+
+AFG_OP_LPL, AFG_OP_AUTO = 0, 1
+AFG_BC_DIRICHLET, AFG_BC_NEUMANN = -1, -2
+AFG_PREC_FP64, AFG_PREC_MIXED = 0, 1
+
+
+class MgOpts(ctypes.Structure):
+    """afg_mg_opts: the fields of mg$D_t (reference src/m_aX_mg.f90:20-52)."""
+    _fields_ = [("i_phi", I32), ("i_rhs", I32), ("i_tmp", I32), ("i_eps", I32),
+                ("n_cycle_down", I32), ("n_cycle_up", I32), ("n_cycle_base", I32),
+                ("op", I32), ("initialized", I32), ("reserved", I32 * 7)]
+
+    def __init__(self, i_phi=-1, i_rhs=-1, i_tmp=-1, i_eps=-1, n_cycle_down=-1,
+                 n_cycle_up=-1, n_cycle_base=-1, op=AFG_OP_LPL):
+        super().__init__(i_phi, i_rhs, i_tmp, i_eps, n_cycle_down, n_cycle_up, n_cycle_base,
+                         op, 0)
+
+
+# name -> (restype, argtypes); every symbol of include/afivo_amd.h
+SIGNATURES = {
+    "afg_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(P)]),
+    "afg_ctx_destroy": (ctypes.c_int, [P]),
+    "afg_last_error": (ctypes.c_char_p, [P]),
+    "afg_sync": (ctypes.c_int, [P]),
+    "afg_set_tree": (ctypes.c_int, [P, ctypes.c_int, I32P, F64]),
+    "afg_set_bc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, F64P]),
+    "afg_packed_size": (ctypes.c_int, [P, ctypes.c_int, I32P, I64P]),
+    "afg_upload_cc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, I32P, F64P]),
+    "afg_download_cc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, I32P, F64P]),
+    "afg_copy_cc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
+    "afg_set_cc": (ctypes.c_int, [P, ctypes.c_int, F64]),
+    "afg_mg_init": (ctypes.c_int, [ctypes.POINTER(MgOpts)]),
+    "afg_fas_fmg": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int, ctypes.c_int]),
+    "afg_fas_vcycle": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int, ctypes.c_int]),
+    "afg_levels": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "afg_tree_max_cc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, F64P]),
+    "afg_tree_min_cc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, F64P]),
+    "afg_fill_gc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
+    "afg_gsrb_level": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int, ctypes.c_int]),
+    "afg_residual_level": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int]),
+    "afg_restrict_level": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
+    "afg_prolong2_level": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "afg_correct_children": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int]),
+    "afg_update_coarse": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int]),
+    "afg_set_graphs": (ctypes.c_int, [P, ctypes.c_int]),
+    "afg_set_timing": (ctypes.c_int, [P, ctypes.c_int]),
+    "afg_timing": (ctypes.c_int, [P, I64P, F64P, F64P]),
+    "afg_vcycle_cell_updates": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int, F64P]),
+    "afg_device_bytes": (ctypes.c_int, [P, I64P]),
+}
+
+_lib = None
+
+
+class AfgError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libafivo_amd.so, raising if it is missing (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise AfgError(f"{LIB_PATH} is missing: build it with `python -m afivo_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, ctx=None):
+    if rc != 0:
+        msg = load().afg_last_error(ctx)
+        raise AfgError(f"afivo_amd error {rc}: {msg.decode() if msg else ''}")

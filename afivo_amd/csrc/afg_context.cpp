@@ -1,0 +1,1174 @@
+// afg_context.cpp -- host side of the afivo_amd C ABI: tree tables, device
+// pools, data movement and the orchestration of the FAS multigrid cycles.
+//
+// The cycle orchestration restates m_aX_mg.f90 (mg$D_fas_fmg :164-203,
+// mg$D_fas_vcycle :208-255 and the stages they call) stage by stage; every
+// stage is one or a few kernel launches over all boxes of a level.
+
+#include "../../include/afivo_amd.h"
+#include "afg_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+using namespace afg;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct Level {
+    int lvl = 0, nc = 0, slot0 = 0, n = 0;
+    double dr = 0, dx2 = 0, inv_dr_sq = 0;
+    int parents_off = 0, n_parents = 0;
+    int leaves_off = 0, n_leaves = 0;
+    int all_off = 0;
+};
+
+struct BcRec {
+    int slot, face, type;
+    int voff;   // -1: b = 0
+};
+
+// An instantiated V-cycle graph; in timing mode it also owns the event pairs
+// recorded around its colour-pass launches and their algorithmic bytes.
+struct GraphRec {
+    hipGraphExec_t exec = nullptr;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    std::vector<double> bytes;
+};
+
+void destroy_graph(GraphRec &g)
+{
+    if (g.exec) hipGraphExecDestroy(g.exec);
+    for (auto &e : g.ev) {
+        hipEventDestroy(e.first);
+        hipEventDestroy(e.second);
+    }
+    g.exec = nullptr;
+    g.ev.clear();
+    g.bytes.clear();
+}
+
+uint64_t spread2(uint64_t x)   // 2D interleave (bit_space_1, m_morton.f90)
+{
+    x &= 0xffffffffULL;
+    x = (x | (x << 16)) & 0x0000ffff0000ffffULL;
+    x = (x | (x << 8)) & 0x00ff00ff00ff00ffULL;
+    x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0fULL;
+    x = (x | (x << 2)) & 0x3333333333333333ULL;
+    x = (x | (x << 1)) & 0x5555555555555555ULL;
+    return x;
+}
+
+uint64_t spread3(uint64_t x)   // 3D interleave (bit_space_2, m_morton.f90)
+{
+    x &= 0x1fffffULL;
+    x = (x | (x << 32)) & 0x001f00000000ffffULL;
+    x = (x | (x << 16)) & 0x001f0000ff0000ffULL;
+    x = (x | (x << 8)) & 0x100f00f00f00f00fULL;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ULL;
+    x = (x | (x << 2)) & 0x1249249249249249ULL;
+    return x;
+}
+
+}  // namespace
+
+struct afg_ctx {
+    int dim = 0, nc = 0, n_var = 0, nch = 0, nnb = 0, stride = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // tree
+    bool have_tree = false;
+    int min_lvl = 0, highest_lvl = -1, n_slots = 0, max_id = 0;
+    double dr_base = 0;
+    std::vector<Level> levels;           // index lvl - min_lvl
+    std::vector<int32_t> id2slot;        // -1 if absent
+    std::vector<int32_t> slot2id, slot_lvl, slot_nc;
+    std::vector<int32_t> h_nb;           // slot * nnb + face (afivo ids)
+    std::vector<BcRec> bc;
+    std::vector<int32_t> bc_of;          // slot * nnb + face -> bc record or -1
+    std::vector<double> bc_vals;
+    bool bc_dirty = false;
+
+    // device
+    int32_t *d_nb = nullptr, *d_parent = nullptr, *d_choff = nullptr, *d_lists = nullptr;
+    int32_t *d_bc_type = nullptr, *d_bc_voff = nullptr;
+    double *d_bc_vals = nullptr;
+    size_t bc_vals_cap = 0, bc_cap = 0;
+    int8_t *d_tag = nullptr;
+    std::vector<double *> d_var;
+    double *d_partial = nullptr;
+    int partial_cap = 0;
+    double *d_out = nullptr, *h_out = nullptr;
+    double *h_stage = nullptr;
+    size_t stage_cap = 0;
+
+    // execution controls
+    bool use_graphs = true;
+    std::map<std::tuple<int, int, int, int, int, int, int, int, int, int>, GraphRec> graphs;
+    bool capturing = false;
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_used, ev_free;
+    std::vector<double> ev_bytes;
+    // events recorded into the graph being captured (timing mode)
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> cap_ev;
+    std::vector<double> cap_bytes;
+    int64_t t_launches = 0;
+    double t_ms = 0, t_bytes = 0;
+
+    int fail(int code, const char *fmt, ...)
+    {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        err = buf;
+        return code;
+    }
+};
+
+#define HIP_TRY(ctx, expr)                                                            \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return (ctx)->fail(AFG_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+namespace {
+
+DevTree dev_tree(const afg_ctx *c)
+{
+    DevTree T;
+    T.stride = c->stride;
+    T.nb = c->d_nb;
+    T.parent = c->d_parent;
+    T.choff = c->d_choff;
+    T.bc_type = c->d_bc_type;
+    T.bc_voff = c->d_bc_voff;
+    T.bc_vals = c->d_bc_vals;
+    T.tag = c->d_tag;
+    return T;
+}
+
+LevelArgs level_args(const afg_ctx *c, int lvl)
+{
+    const Level &L = c->levels[lvl - c->min_lvl];
+    LevelArgs a;
+    a.slot0 = L.slot0;
+    a.n = L.n;
+    a.nc = L.nc;
+    a.pnc = lvl > c->min_lvl ? c->levels[lvl - 1 - c->min_lvl].nc : 0;
+    a.dr = L.dr;
+    a.dx2 = L.dx2;
+    a.inv_dr_sq = L.inv_dr_sq;
+    return a;
+}
+
+ListArgs list_args(const afg_ctx *c, int lvl, int which)   // 0 parents, 1 leaves, 2 all
+{
+    const Level &L = c->levels[lvl - c->min_lvl];
+    ListArgs a;
+    const int off = which == 0 ? L.parents_off : which == 1 ? L.leaves_off : L.all_off;
+    a.slots = c->d_lists + off;
+    a.n = which == 0 ? L.n_parents : which == 1 ? L.n_leaves : L.n;
+    a.nc = L.nc;
+    a.dr = L.dr;
+    a.dx2 = L.dx2;
+    a.inv_dr_sq = L.inv_dr_sq;
+    return a;
+}
+
+int box_len(int dim, int n) { return dim == 2 ? (n + 2) * (n + 2) : (n + 2) * (n + 2) * (n + 2); }
+
+void free_tree(afg_ctx *c)
+{
+    for (auto &g : c->graphs) destroy_graph(g.second);
+    c->graphs.clear();
+    hipFree(c->d_nb);
+    hipFree(c->d_parent);
+    hipFree(c->d_choff);
+    hipFree(c->d_lists);
+    hipFree(c->d_bc_type);
+    hipFree(c->d_bc_voff);
+    hipFree(c->d_bc_vals);
+    hipFree(c->d_tag);
+    hipFree(c->d_partial);
+    for (double *p : c->d_var) hipFree(p);
+    c->d_nb = c->d_parent = c->d_choff = c->d_lists = c->d_bc_type = c->d_bc_voff = nullptr;
+    c->d_bc_vals = nullptr;
+    c->d_tag = nullptr;
+    c->d_partial = nullptr;
+    c->partial_cap = 0;
+    c->bc_vals_cap = c->bc_cap = 0;
+    std::fill(c->d_var.begin(), c->d_var.end(), nullptr);
+    c->have_tree = false;
+}
+
+int check_ctx(afg_ctx *c)
+{
+    if (!c) return AFG_ERR_ARG;
+    if (!c->have_tree) return c->fail(AFG_ERR_STATE, "no tree set (call afg_set_tree first)");
+    HIP_TRY(c, hipSetDevice(c->device));
+    return 0;
+}
+
+int check_var(afg_ctx *c, int iv)
+{
+    if (iv < 1 || iv > c->n_var) return c->fail(AFG_ERR_ARG, "variable index %d out of range 1..%d", iv, c->n_var);
+    return 0;
+}
+
+// Device pool of variable iv, allocated (and zeroed) on first use.
+int ensure_var(afg_ctx *c, int iv, double **out)
+{
+    if (int e = check_var(c, iv)) return e;
+    double *&p = c->d_var[iv - 1];
+    if (!p) {
+        if (c->capturing) return c->fail(AFG_ERR_STATE, "variable %d first used inside a graph capture", iv);
+        const size_t bytes = (size_t)c->n_slots * c->stride * sizeof(double);
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return c->fail(AFG_ERR_MEMORY, "hipMalloc(%zu) for variable %d: %s", bytes, iv,
+                           hipGetErrorString(e));
+        }
+        HIP_TRY(c, hipMemsetAsync(p, 0, bytes, c->stream));
+    }
+    if (out) *out = p;
+    return 0;
+}
+
+int upload_bc(afg_ctx *c)
+{
+    if (!c->bc_dirty) return 0;
+    const size_t nrec = std::max<size_t>(c->bc.size(), 1);
+    std::vector<int32_t> ty(nrec, -1), vo(nrec, -1);
+    for (size_t r = 0; r < c->bc.size(); r++) {
+        ty[r] = c->bc[r].type;
+        vo[r] = c->bc[r].voff;
+    }
+    if (nrec > c->bc_cap) {
+        hipFree(c->d_bc_type);
+        hipFree(c->d_bc_voff);
+        c->d_bc_type = c->d_bc_voff = nullptr;
+        HIP_TRY(c, hipMalloc(&c->d_bc_type, nrec * sizeof(int32_t)));
+        HIP_TRY(c, hipMalloc(&c->d_bc_voff, nrec * sizeof(int32_t)));
+        c->bc_cap = nrec;
+        for (auto &g : c->graphs) destroy_graph(g.second);
+        c->graphs.clear();
+    }
+    const size_t nv = std::max<size_t>(c->bc_vals.size(), 1);
+    if (nv > c->bc_vals_cap) {
+        hipFree(c->d_bc_vals);
+        c->d_bc_vals = nullptr;
+        HIP_TRY(c, hipMalloc(&c->d_bc_vals, nv * sizeof(double)));
+        c->bc_vals_cap = nv;
+        for (auto &g : c->graphs) destroy_graph(g.second);
+        c->graphs.clear();
+    }
+    HIP_TRY(c, hipMemcpyAsync(c->d_bc_type, ty.data(), nrec * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->d_bc_voff, vo.data(), nrec * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    if (!c->bc_vals.empty())
+        HIP_TRY(c, hipMemcpyAsync(c->d_bc_vals, c->bc_vals.data(), c->bc_vals.size() * sizeof(double),
+                                  hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->bc_dirty = false;
+    return 0;
+}
+
+int ensure_stage(afg_ctx *c, size_t n_doubles)
+{
+    if (n_doubles <= c->stage_cap) return 0;
+    if (c->h_stage) hipHostFree(c->h_stage);
+    c->h_stage = nullptr;
+    HIP_TRY(c, hipHostMalloc((void **)&c->h_stage, n_doubles * sizeof(double), hipHostMallocDefault));
+    c->stage_cap = n_doubles;
+    return 0;
+}
+
+// ---- timing of the colour-pass kernel ----
+
+// Algorithmic bytes of one colour pass over a level: half of a full sweep's
+// (3 + 2*dim/nc) * 8 per cell (read phi with its face ghosts, read rhs, write
+// phi), plus (1 + 2*dim/nc) * 8 per cell of eps reads for the auto operators.
+double gsrb_pass_bytes(const afg_ctx *c, const LevelArgs &L, const MgVars &V)
+{
+    const double cells = (double)L.n * (c->dim == 2 ? L.nc * L.nc : L.nc * L.nc * L.nc);
+    const double g = 2.0 * c->dim / L.nc;
+    double per_cell = (3 + g) * 8.0;
+    if (V.op_auto) per_cell += (1 + g) * 8.0;
+    return 0.5 * cells * per_cell;
+}
+
+void timed_gsrb(afg_ctx *c, const DevTree &T, const LevelArgs &L, const MgVars &V, int rb, int stored)
+{
+    if (!c->timing) {
+        launch_gsrb_pass(c->dim, c->stream, T, L, V, rb, stored);
+        return;
+    }
+    std::pair<hipEvent_t, hipEvent_t> ev;
+    if (!c->capturing && !c->ev_free.empty()) {
+        ev = c->ev_free.back();
+        c->ev_free.pop_back();
+    } else {
+        hipEventCreate(&ev.first);
+        hipEventCreate(&ev.second);
+    }
+    hipEventRecord(ev.first, c->stream);
+    launch_gsrb_pass(c->dim, c->stream, T, L, V, rb, stored);
+    hipEventRecord(ev.second, c->stream);
+    if (c->capturing) {
+        c->cap_ev.push_back(ev);
+        c->cap_bytes.push_back(gsrb_pass_bytes(c, L, V));
+    } else {
+        c->ev_used.push_back(ev);
+        c->ev_bytes.push_back(gsrb_pass_bytes(c, L, V));
+    }
+}
+
+int collect_timing(afg_ctx *c)
+{
+    if (c->ev_used.empty()) return 0;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (size_t k = 0; k < c->ev_used.size(); k++) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, c->ev_used[k].first, c->ev_used[k].second);
+        c->t_ms += ms;
+        c->t_bytes += c->ev_bytes[k];
+        c->t_launches++;
+        c->ev_free.push_back(c->ev_used[k]);
+    }
+    c->ev_used.clear();
+    c->ev_bytes.clear();
+    return 0;
+}
+
+// ---- stages (m_aX_mg.f90) ----
+
+struct Mg {
+    afg_ctx *c;
+    DevTree T;
+    MgVars V;
+    const afg_mg_opts *o;
+};
+
+int make_mg(afg_ctx *c, const afg_mg_opts *o, Mg *m)
+{
+    if (!o) return c->fail(AFG_ERR_ARG, "null options");
+    if (!o->initialized) return c->fail(AFG_ERR_STATE, "check_mg: you haven't called mg%d_init_mg", c->dim);
+    if (o->op != AFG_OP_LPL && o->op != AFG_OP_AUTO) return c->fail(AFG_ERR_UNSUPPORTED, "operator %d not supported on the device", o->op);
+    m->c = c;
+    m->o = o;
+    if (int e = ensure_var(c, o->i_phi, &m->V.phi)) return e;
+    if (int e = ensure_var(c, o->i_rhs, &m->V.rhs)) return e;
+    if (int e = ensure_var(c, o->i_tmp, &m->V.tmp)) return e;
+    m->V.eps = nullptr;
+    m->V.op_auto = o->op == AFG_OP_AUTO;
+    if (m->V.op_auto) {
+        double *e = nullptr;
+        if (int r = ensure_var(c, o->i_eps, &e)) return r;
+        m->V.eps = e;
+    }
+    if (int e = upload_bc(c)) return e;
+    m->T = dev_tree(c);
+    return 0;
+}
+
+// Lazy tags of the auto operators (m_aX_mg.f90:567,602,655): boxes whose tag
+// is unset get it now; eps is not changed by the solver, so this equals the
+// reference's first-use evaluation.
+void set_tags(Mg &m)
+{
+    if (!m.V.op_auto) return;
+    for (int l = m.c->min_lvl; l <= m.c->highest_lvl; l++)
+        launch_set_tags(m.c->dim, m.c->stream, m.T, level_args(m.c, l), m.V.eps);
+}
+
+void fill_gc(Mg &m, double *var, int lvl)
+{
+    launch_fill_gc(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), var);
+}
+
+// gsrb_boxes (m_aX_mg.f90:397-420): 2*n_cycle colour passes, each followed in
+// the reference by a ghost fill.  For even nc the passes compute face ghosts
+// on the fly, so only the fill after the last pass is launched; the first
+// pass reads the ghosts in memory, as the reference does.
+void gsrb_level(Mg &m, int lvl, int n_cycle)
+{
+    const LevelArgs L = level_args(m.c, lvl);
+    const bool odd = L.nc & 1;
+    for (int rb = 1; rb <= 2 * n_cycle; rb++) {
+        timed_gsrb(m.c, m.T, L, m.V, rb, (rb == 1 || odd) ? 1 : 0);
+        if (odd || rb == 2 * n_cycle) launch_fill_gc(m.c->dim, m.c->stream, m.T, L, m.V.phi);
+    }
+}
+
+// update_coarse (m_aX_mg.f90:424-480)
+void update_coarse(Mg &m, int lvl)
+{
+    launch_update_coarse_child(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), m.V, 1);
+    fill_gc(m, m.V.phi, lvl - 1);
+    launch_update_coarse_parent(m.c->dim, m.c->stream, m.T, list_args(m.c, lvl - 1, 0), m.V, 1);
+}
+
+// set_coarse_phi_rhs (m_aX_mg.f90:484-517)
+void set_coarse_phi_rhs(Mg &m, int lvl)
+{
+    if (lvl == m.c->highest_lvl) fill_gc(m, m.V.phi, lvl);
+    launch_update_coarse_child(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), m.V, 0);
+    fill_gc(m, m.V.phi, lvl - 1);
+    launch_update_coarse_parent(m.c->dim, m.c->stream, m.T, list_args(m.c, lvl - 1, 0), m.V, 0);
+}
+
+// init_phi_rhs (m_aX_mg.f90:520-543)
+void init_phi_rhs(Mg &m)
+{
+    for (int lvl = m.c->highest_lvl; lvl >= m.c->min_lvl + 1; lvl--) {
+        const LevelArgs L = level_args(m.c, lvl);
+        launch_fill_value(m.c->dim, m.c->stream, m.T, L.slot0, L.n, L.nc, m.V.phi, 0.0);
+        launch_restrict(m.c->dim, m.c->stream, m.T, L, m.V.rhs, m.V.rhs);
+    }
+}
+
+// correct_children over the parents of lvl-1 (m_aX_mg.f90:368-395)
+void correct_children(Mg &m, int lvl)
+{
+    launch_diff_phi_tmp(m.c->dim, m.c->stream, m.T, list_args(m.c, lvl - 1, 0), m.V);
+    launch_correct(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), m.V);
+}
+
+void residual_level(Mg &m, int lvl)
+{
+    launch_residual(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), m.V);
+}
+
+// mg$D_fas_vcycle (m_aX_mg.f90:208-255)
+void vcycle_body(Mg &m, int highest_lvl, int set_residual)
+{
+    afg_ctx *c = m.c;
+    for (int lvl = highest_lvl; lvl >= c->min_lvl + 1; lvl--) {
+        gsrb_level(m, lvl, m.o->n_cycle_down);
+        update_coarse(m, lvl);
+    }
+    gsrb_level(m, c->min_lvl, m.o->n_cycle_base);
+    for (int lvl = c->min_lvl + 1; lvl <= highest_lvl; lvl++) {
+        correct_children(m, lvl);
+        fill_gc(m, m.V.phi, lvl);
+        gsrb_level(m, lvl, m.o->n_cycle_up);
+    }
+    if (set_residual)
+        for (int lvl = c->min_lvl; lvl <= highest_lvl; lvl++) residual_level(m, lvl);
+}
+
+// Run a V-cycle, from a cached HIP graph when enabled.  In timing mode the
+// graph carries event records around its colour passes; the stream is then
+// synchronised after each launch to read them before the next replay.
+int run_vcycle(Mg &m, int highest_lvl, int set_residual)
+{
+    afg_ctx *c = m.c;
+    if (!c->use_graphs) {
+        vcycle_body(m, highest_lvl, set_residual);
+        return 0;
+    }
+    const afg_mg_opts *o = m.o;
+    auto key = std::make_tuple(highest_lvl, set_residual, o->i_phi, o->i_rhs, o->i_tmp, o->i_eps,
+                               o->op, o->n_cycle_down * 1000000 + o->n_cycle_up * 1000, o->n_cycle_base,
+                               c->timing ? 1 : 0);
+    auto it = c->graphs.find(key);
+    if (it == c->graphs.end()) {
+        hipGraph_t g = nullptr;
+        c->cap_ev.clear();
+        c->cap_bytes.clear();
+        HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        c->capturing = true;
+        vcycle_body(m, highest_lvl, set_residual);
+        c->capturing = false;
+        HIP_TRY(c, hipStreamEndCapture(c->stream, &g));
+        GraphRec rec;
+        hipError_t e = hipGraphInstantiate(&rec.exec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        rec.ev.swap(c->cap_ev);
+        rec.bytes.swap(c->cap_bytes);
+        if (e != hipSuccess) {
+            destroy_graph(rec);
+            return c->fail(AFG_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+        }
+        it = c->graphs.emplace(key, std::move(rec)).first;
+    }
+    HIP_TRY(c, hipGraphLaunch(it->second.exec, c->stream));
+    if (c->timing && !it->second.ev.empty()) {
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        const GraphRec &g = it->second;
+        for (size_t k = 0; k < g.ev.size(); k++) {
+            float ms = 0;
+            hipEventElapsedTime(&ms, g.ev[k].first, g.ev[k].second);
+            c->t_ms += ms;
+            c->t_bytes += g.bytes[k];
+            c->t_launches++;
+        }
+    }
+    return 0;
+}
+
+int check_level(afg_ctx *c, int lvl, bool need_parent)
+{
+    if (lvl < c->min_lvl || lvl > c->highest_lvl)
+        return c->fail(AFG_ERR_ARG, "level %d outside %d..%d", lvl, c->min_lvl, c->highest_lvl);
+    if (need_parent && lvl == c->min_lvl)
+        return c->fail(AFG_ERR_ARG, "level %d has no parent level", lvl);
+    return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+
+extern "C" {
+
+int afg_ctx_create(int dim, int n_cell, int n_var, afg_ctx **out)
+{
+    if (!out) return AFG_ERR_ARG;
+    *out = nullptr;
+    if (dim != 2 && dim != 3) {
+        g_create_error = "dim must be 2 or 3";
+        return AFG_ERR_ARG;
+    }
+    // a$D_init checks (m_aX_core.f90:61-63)
+    if (n_cell < 2 || (n_cell & 1) || n_cell > 1000) {
+        g_create_error = "n_cell should be even, >= 2 and <= 1000";
+        return AFG_ERR_ARG;
+    }
+    if (n_var <= 0) {
+        g_create_error = "n_var_cell should be > 0";
+        return AFG_ERR_ARG;
+    }
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        g_create_error = std::string("hipGetDevice: ") + hipGetErrorString(e);
+        return AFG_ERR_HIP;
+    }
+    afg_ctx *c = new afg_ctx;
+    c->dim = dim;
+    c->nc = n_cell;
+    c->n_var = n_var;
+    c->nch = 1 << dim;
+    c->nnb = 2 * dim;
+    c->device = dev;
+    // each box array padded to a multiple of 16 doubles (128 B)
+    c->stride = (box_len(dim, n_cell) + 15) / 16 * 16;
+    c->d_var.assign(n_var, nullptr);
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&c->d_out, 8 * sizeof(double));
+    if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_out, 8 * sizeof(double), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        g_create_error = std::string("HIP set-up: ") + hipGetErrorString(e);
+        afg_ctx_destroy(c);
+        return AFG_ERR_HIP;
+    }
+    *out = c;
+    return 0;
+}
+
+int afg_ctx_destroy(afg_ctx *c)
+{
+    if (!c) return 0;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    free_tree(c);
+    for (auto &ev : c->ev_used) {
+        hipEventDestroy(ev.first);
+        hipEventDestroy(ev.second);
+    }
+    for (auto &ev : c->ev_free) {
+        hipEventDestroy(ev.first);
+        hipEventDestroy(ev.second);
+    }
+    hipFree(c->d_out);
+    if (c->h_out) hipHostFree(c->h_out);
+    if (c->h_stage) hipHostFree(c->h_stage);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+const char *afg_last_error(const afg_ctx *c)
+{
+    return c ? c->err.c_str() : g_create_error.c_str();
+}
+
+int afg_sync(afg_ctx *c)
+{
+    if (!c) return AFG_ERR_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int afg_set_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base)
+{
+    if (!c) return AFG_ERR_ARG;
+    if (n_boxes < 1 || !recs) return c->fail(AFG_ERR_ARG, "empty tree");
+    if (!(dr_base > 0)) return c->fail(AFG_ERR_ARG, "dr_base must be > 0");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    free_tree(c);
+    const int dim = c->dim, nch = c->nch, nnb = c->nnb;
+    const int rl = 3 + dim + nch + nnb;
+    int max_id = 0, min_lvl = 1 << 30, hi_lvl = -(1 << 30);
+    for (int r = 0; r < n_boxes; r++) {
+        const int32_t *q = recs + (size_t)r * rl;
+        if (q[0] < 1) return c->fail(AFG_ERR_ARG, "record %d: box id %d < 1", r, q[0]);
+        max_id = std::max(max_id, q[0]);
+        min_lvl = std::min(min_lvl, q[1]);
+        hi_lvl = std::max(hi_lvl, q[1]);
+    }
+    if (hi_lvl - min_lvl > 60) return c->fail(AFG_ERR_ARG, "level range %d..%d too large", min_lvl, hi_lvl);
+    std::vector<int32_t> rec_of(max_id + 1, -1);
+    for (int r = 0; r < n_boxes; r++) {
+        const int id = recs[(size_t)r * rl];
+        if (rec_of[id] >= 0) return c->fail(AFG_ERR_ARG, "box id %d appears twice", id);
+        rec_of[id] = r;
+    }
+    // group by level, Morton order within a level (the key of a$D_tidy_up,
+    // m_aX_core.f90:272-279: morton_from_ix$D(ix - 1))
+    const int nl = hi_lvl - min_lvl + 1;
+    std::vector<std::vector<std::pair<uint64_t, int>>> per(nl);
+    for (int r = 0; r < n_boxes; r++) {
+        const int32_t *q = recs + (size_t)r * rl;
+        uint64_t key;
+        if (dim == 2)
+            key = spread2(q[2] - 1) | (spread2(q[3] - 1) << 1);
+        else
+            key = spread3(q[2] - 1) | (spread3(q[3] - 1) << 1) | (spread3(q[4] - 1) << 2);
+        per[q[1] - min_lvl].push_back({key, r});
+    }
+    c->levels.assign(nl, Level());
+    c->id2slot.assign(max_id + 1, -1);
+    c->slot2id.assign(n_boxes, 0);
+    c->slot_lvl.assign(n_boxes, 0);
+    c->slot_nc.assign(n_boxes, 0);
+    int slot = 0;
+    for (int li = 0; li < nl; li++) {
+        const int lvl = min_lvl + li;
+        if (per[li].empty()) return c->fail(AFG_ERR_ARG, "level %d has no boxes", lvl);
+        std::stable_sort(per[li].begin(), per[li].end(),
+                         [](const std::pair<uint64_t, int> &a, const std::pair<uint64_t, int> &b) { return a.first < b.first; });
+        Level &L = c->levels[li];
+        L.lvl = lvl;
+        // a$D_n_cell (m_aX_utils.f90:676-686)
+        L.nc = lvl >= 1 ? c->nc : c->nc >> (1 - lvl);
+        if (L.nc < 1 || (lvl < 1 && (c->nc % (1 << (1 - lvl))) != 0))
+            return c->fail(AFG_ERR_ARG, "level %d: n_cell %d cannot be coarsened to it", lvl, c->nc);
+        if (L.nc < 2 || ((L.nc & 1) && lvl > min_lvl))
+            return c->fail(AFG_ERR_UNSUPPORTED, "level %d: n_cell %d not supported (need >= 2, even above the lowest level)", lvl, L.nc);
+        // box%dr = dr_base * 0.5**(lvl-1) (m_aX_core.f90:196, halved per child :764)
+        L.dr = dr_base * std::pow(0.5, lvl - 1);
+        L.dx2 = L.dr * L.dr;
+        L.inv_dr_sq = 1 / (L.dr * L.dr);
+        L.slot0 = slot;
+        L.n = (int)per[li].size();
+        for (auto &kr : per[li]) {
+            const int id = recs[(size_t)kr.second * rl];
+            c->id2slot[id] = slot;
+            c->slot2id[slot] = id;
+            c->slot_lvl[slot] = lvl;
+            c->slot_nc[slot] = L.nc;
+            slot++;
+        }
+    }
+    c->n_slots = slot;
+    c->min_lvl = min_lvl;
+    c->highest_lvl = hi_lvl;
+    c->max_id = max_id;
+    c->dr_base = dr_base;
+
+    // per-slot tables
+    std::vector<int32_t> nbv((size_t)slot * nnb), par(slot, -1), choff(slot, 0);
+    c->h_nb.assign((size_t)slot * nnb, 0);
+    c->bc.clear();
+    c->bc_vals.clear();
+    c->bc_of.assign((size_t)slot * nnb, -1);
+    std::vector<int32_t> lists;
+    for (int s = 0; s < slot; s++) {
+        const int32_t *q = recs + (size_t)rec_of[c->slot2id[s]] * rl;
+        const int lvl = q[1];
+        const int pid = q[2 + dim];
+        if (lvl > min_lvl) {
+            if (pid < 1 || pid > max_id || c->id2slot[pid] < 0)
+                return c->fail(AFG_ERR_ARG, "box %d: parent %d missing", q[0], pid);
+            const int ps = c->id2slot[pid];
+            if (c->slot_lvl[ps] != lvl - 1) return c->fail(AFG_ERR_ARG, "box %d: parent %d not on level %d", q[0], pid, lvl - 1);
+            par[s] = ps;
+            // a$D_get_child_offset (m_aX_t.f90:362-374)
+            int co = 0;
+            if (lvl > 1)
+                for (int d = 0; d < dim; d++) co |= (((q[2 + d] - 1) & 1) * (c->nc >> 1)) << (10 * d);
+            choff[s] = co;
+        }
+        for (int f = 0; f < nnb; f++) {
+            const int nid = q[3 + dim + nch + f];
+            c->h_nb[(size_t)s * nnb + f] = nid;
+            if (nid > 0) {
+                if (nid > max_id || c->id2slot[nid] < 0) return c->fail(AFG_ERR_ARG, "box %d: neighbour %d missing", q[0], nid);
+                const int ns = c->id2slot[nid];
+                if (c->slot_lvl[ns] != lvl) return c->fail(AFG_ERR_ARG, "box %d: neighbour %d on another level", q[0], nid);
+                nbv[(size_t)s * nnb + f] = ns;
+            } else if (nid == 0) {
+                if (lvl == min_lvl) return c->fail(AFG_ERR_ARG, "box %d: refinement boundary on the lowest level", q[0]);
+                nbv[(size_t)s * nnb + f] = NB_REFINE;
+            } else {
+                const int rec = (int)c->bc.size();
+                c->bc.push_back({s, f, AFG_BC_DIRICHLET, -1});   // a$D_bc_dirichlet_zero
+                c->bc_of[(size_t)s * nnb + f] = rec;
+                nbv[(size_t)s * nnb + f] = NB_BC0 - rec;
+            }
+        }
+    }
+    for (int li = 0; li < nl; li++) {
+        Level &L = c->levels[li];
+        L.parents_off = (int)lists.size();
+        for (int s = L.slot0; s < L.slot0 + L.n; s++) {
+            const int32_t *q = recs + (size_t)rec_of[c->slot2id[s]] * rl;
+            if (q[3 + dim] != 0) lists.push_back(s);   // a$D_has_children
+        }
+        L.n_parents = (int)lists.size() - L.parents_off;
+        L.leaves_off = (int)lists.size();
+        for (int s = L.slot0; s < L.slot0 + L.n; s++) {
+            const int32_t *q = recs + (size_t)rec_of[c->slot2id[s]] * rl;
+            if (q[3 + dim] == 0) lists.push_back(s);
+        }
+        L.n_leaves = (int)lists.size() - L.leaves_off;
+        L.all_off = (int)lists.size();
+        for (int s = L.slot0; s < L.slot0 + L.n; s++) lists.push_back(s);
+    }
+    // every box of a level above the lowest must be the child of a parent of
+    // the level below (all children of a refined box exist)
+    for (int li = 1; li < nl; li++) {
+        long long expect = 0;
+        const Level &P = c->levels[li - 1];
+        for (int k = 0; k < P.n_parents; k++) {
+            const int ps = lists[P.parents_off + k];
+            const int32_t *q = recs + (size_t)rec_of[c->slot2id[ps]] * rl;
+            for (int ch = 0; ch < nch; ch++) {
+                const int cid = q[3 + dim + ch];
+                if (cid == 0) continue;
+                if (cid > max_id || c->id2slot[cid] < 0 || par[c->id2slot[cid]] != ps)
+                    return c->fail(AFG_ERR_ARG, "box %d: child %d inconsistent", q[0], cid);
+                expect++;
+            }
+        }
+        if (expect != c->levels[li].n)
+            return c->fail(AFG_ERR_ARG, "level %d: %d boxes but %lld children of level %d", P.lvl + 1, c->levels[li].n, expect, P.lvl);
+    }
+
+    const size_t sn = (size_t)slot;
+    HIP_TRY(c, hipMalloc(&c->d_nb, sn * nnb * sizeof(int32_t)));
+    HIP_TRY(c, hipMalloc(&c->d_parent, sn * sizeof(int32_t)));
+    HIP_TRY(c, hipMalloc(&c->d_choff, sn * sizeof(int32_t)));
+    HIP_TRY(c, hipMalloc(&c->d_lists, std::max<size_t>(lists.size(), 1) * sizeof(int32_t)));
+    HIP_TRY(c, hipMalloc(&c->d_tag, sn * sizeof(int8_t)));
+    HIP_TRY(c, hipMemcpy(c->d_nb, nbv.data(), sn * nnb * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_parent, par.data(), sn * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_choff, choff.data(), sn * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_lists, lists.data(), lists.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemset(c->d_tag, 0, sn * sizeof(int8_t)));
+    int maxp = 0;
+    for (int li = 0; li < nl; li++) {
+        const Level &L = c->levels[li];
+        ListArgs a;
+        a.n = L.n;
+        a.nc = L.nc;
+        maxp += extreme_blocks(dim, a);
+    }
+    c->partial_cap = std::max(maxp, 1);
+    HIP_TRY(c, hipMalloc(&c->d_partial, c->partial_cap * sizeof(double)));
+    c->bc_dirty = true;
+    c->have_tree = true;
+    if (int e = upload_bc(c)) return e;
+    return 0;
+}
+
+int afg_set_bc(afg_ctx *c, int id, int nb, int bc_type, const double *b)
+{
+    if (int e = check_ctx(c)) return e;
+    if (id < 1 || id > c->max_id || c->id2slot[id] < 0) return c->fail(AFG_ERR_ARG, "no box with id %d", id);
+    if (nb < 1 || nb > c->nnb) return c->fail(AFG_ERR_ARG, "neighbour direction %d out of range", nb);
+    if (bc_type != AFG_BC_DIRICHLET && bc_type != AFG_BC_NEUMANN)
+        return c->fail(AFG_ERR_ARG, "fill_bc: unknown boundary condition %d", bc_type);
+    const int s = c->id2slot[id];
+    const int rec = c->bc_of[(size_t)s * c->nnb + nb - 1];
+    if (rec < 0) return c->fail(AFG_ERR_ARG, "box %d direction %d is not a physical boundary", id, nb);
+    BcRec &r = c->bc[rec];
+    r.type = bc_type;
+    if (b) {
+        const int n = c->slot_nc[s];
+        const int m = c->dim == 2 ? n : n * n;
+        bool zero = true;
+        for (int k = 0; k < m; k++) zero = zero && b[k] == 0.0 && !std::signbit(b[k]);
+        if (zero) {
+            r.voff = -1;
+        } else {
+            if (r.voff < 0) {
+                r.voff = (int)c->bc_vals.size();
+                c->bc_vals.resize(c->bc_vals.size() + m);
+            }
+            std::memcpy(&c->bc_vals[r.voff], b, m * sizeof(double));
+        }
+    } else {
+        r.voff = -1;
+    }
+    c->bc_dirty = true;
+    return 0;
+}
+
+int afg_packed_size(const afg_ctx *cc, int n_ids, const int32_t *ids, int64_t *n_values)
+{
+    afg_ctx *c = const_cast<afg_ctx *>(cc);
+    if (int e = check_ctx(c)) return e;
+    int64_t n = 0;
+    for (int k = 0; k < n_ids; k++) {
+        const int id = ids[k];
+        if (id < 1 || id > c->max_id || c->id2slot[id] < 0) return c->fail(AFG_ERR_ARG, "no box with id %d", id);
+        n += box_len(c->dim, c->slot_nc[c->id2slot[id]]);
+    }
+    *n_values = n;
+    return 0;
+}
+
+// Copy the contiguous runs of marked slots between device pool and staging.
+static int copy_runs(afg_ctx *c, double *dev, const std::vector<char> &mark, bool to_device)
+{
+    int s = 0;
+    while (s < c->n_slots) {
+        if (!mark[s]) {
+            s++;
+            continue;
+        }
+        int e = s;
+        while (e < c->n_slots && mark[e]) e++;
+        const size_t off = (size_t)s * c->stride, len = (size_t)(e - s) * c->stride * sizeof(double);
+        if (to_device)
+            HIP_TRY(c, hipMemcpyAsync(dev + off, c->h_stage + off, len, hipMemcpyHostToDevice, c->stream));
+        else
+            HIP_TRY(c, hipMemcpyAsync(c->h_stage + off, dev + off, len, hipMemcpyDeviceToHost, c->stream));
+        s = e;
+    }
+    return 0;
+}
+
+int afg_upload_cc(afg_ctx *c, int iv, int n_ids, const int32_t *ids, const double *packed)
+{
+    if (int e = check_ctx(c)) return e;
+    double *dev = nullptr;
+    if (int e = ensure_var(c, iv, &dev)) return e;
+    if (int e = ensure_stage(c, (size_t)c->n_slots * c->stride)) return e;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));   // staging may be in use
+    std::vector<char> mark(c->n_slots, 0);
+    size_t pos = 0;
+    for (int k = 0; k < n_ids; k++) {
+        const int id = ids[k];
+        if (id < 1 || id > c->max_id || c->id2slot[id] < 0) return c->fail(AFG_ERR_ARG, "no box with id %d", id);
+        const int s = c->id2slot[id];
+        const int len = box_len(c->dim, c->slot_nc[s]);
+        std::memcpy(c->h_stage + (size_t)s * c->stride, packed + pos, len * sizeof(double));
+        if (len < c->stride) std::memset(c->h_stage + (size_t)s * c->stride + len, 0, (c->stride - len) * sizeof(double));
+        pos += len;
+        mark[s] = 1;
+    }
+    if (int e = copy_runs(c, dev, mark, true)) return e;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int afg_download_cc(afg_ctx *c, int iv, int n_ids, const int32_t *ids, double *packed)
+{
+    if (int e = check_ctx(c)) return e;
+    double *dev = nullptr;
+    if (int e = ensure_var(c, iv, &dev)) return e;
+    if (int e = ensure_stage(c, (size_t)c->n_slots * c->stride)) return e;
+    std::vector<char> mark(c->n_slots, 0);
+    for (int k = 0; k < n_ids; k++) {
+        const int id = ids[k];
+        if (id < 1 || id > c->max_id || c->id2slot[id] < 0) return c->fail(AFG_ERR_ARG, "no box with id %d", id);
+        mark[c->id2slot[id]] = 1;
+    }
+    if (int e = copy_runs(c, dev, mark, false)) return e;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    size_t pos = 0;
+    for (int k = 0; k < n_ids; k++) {
+        const int s = c->id2slot[ids[k]];
+        const int len = box_len(c->dim, c->slot_nc[s]);
+        std::memcpy(packed + pos, c->h_stage + (size_t)s * c->stride, len * sizeof(double));
+        pos += len;
+    }
+    return 0;
+}
+
+int afg_copy_cc(afg_ctx *c, int iv_from, int iv_to)
+{
+    if (int e = check_ctx(c)) return e;
+    double *a = nullptr, *b = nullptr;
+    if (int e = ensure_var(c, iv_from, &a)) return e;
+    if (int e = ensure_var(c, iv_to, &b)) return e;
+    const DevTree T = dev_tree(c);
+    for (const Level &L : c->levels) launch_copy_boxes(c->dim, c->stream, T, L.slot0, L.n, L.nc, a, b);
+    return 0;
+}
+
+int afg_set_cc(afg_ctx *c, int iv, double value)
+{
+    if (int e = check_ctx(c)) return e;
+    double *a = nullptr;
+    if (int e = ensure_var(c, iv, &a)) return e;
+    const DevTree T = dev_tree(c);
+    for (const Level &L : c->levels) launch_fill_value(c->dim, c->stream, T, L.slot0, L.n, L.nc, a, value);
+    return 0;
+}
+
+int afg_mg_init(afg_mg_opts *o)
+{
+    if (!o) return AFG_ERR_ARG;
+    // mg$D_init_mg (m_aX_mg.f90:130-154)
+    if (o->i_phi < 1 || o->i_tmp < 1 || o->i_rhs < 1) {
+        g_create_error = o->i_phi < 1 ? "mg_init_mg: i_phi not set" : o->i_tmp < 1 ? "mg_init_mg: i_tmp not set" : "mg_init_mg: i_rhs not set";
+        return AFG_ERR_ARG;
+    }
+    if (o->op == AFG_OP_AUTO && o->i_eps < 1) {
+        g_create_error = "mg_init_mg: auto operators need i_eps";
+        return AFG_ERR_ARG;
+    }
+    if (o->n_cycle_down < 0) o->n_cycle_down = 2;
+    if (o->n_cycle_up < 0) o->n_cycle_up = 2;
+    if (o->n_cycle_base < 0) o->n_cycle_base = 2;
+    o->initialized = 1;
+    return 0;
+}
+
+int afg_levels(const afg_ctx *cc, int *min_lvl, int *highest_lvl)
+{
+    afg_ctx *c = const_cast<afg_ctx *>(cc);
+    if (int e = check_ctx(c)) return e;
+    if (min_lvl) *min_lvl = c->min_lvl;
+    if (highest_lvl) *highest_lvl = c->highest_lvl;
+    return 0;
+}
+
+int afg_fas_vcycle(afg_ctx *c, const afg_mg_opts *o, int highest_lvl, int set_residual)
+{
+    if (int e = check_ctx(c)) return e;
+    if (int e = check_level(c, highest_lvl, false)) return e;
+    Mg m;
+    if (int e = make_mg(c, o, &m)) return e;
+    set_tags(m);
+    return run_vcycle(m, highest_lvl, set_residual ? 1 : 0);
+}
+
+int afg_fas_fmg(afg_ctx *c, const afg_mg_opts *o, int set_residual, int have_guess)
+{
+    if (int e = check_ctx(c)) return e;
+    Mg m;
+    if (int e = make_mg(c, o, &m)) return e;
+    set_tags(m);
+    // mg$D_fas_fmg (m_aX_mg.f90:164-203)
+    if (have_guess) {
+        for (int lvl = c->highest_lvl; lvl >= c->min_lvl + 1; lvl--) set_coarse_phi_rhs(m, lvl);
+    } else {
+        init_phi_rhs(m);
+    }
+    for (int lvl = c->min_lvl; lvl <= c->highest_lvl; lvl++) {
+        const LevelArgs L = level_args(c, lvl);
+        launch_copy_boxes(c->dim, c->stream, m.T, L.slot0, L.n, L.nc, m.V.phi, m.V.tmp);
+        if (lvl > c->min_lvl) {
+            correct_children(m, lvl);
+            fill_gc(m, m.V.phi, lvl);
+        }
+        if (int e = run_vcycle(m, lvl, (set_residual && lvl == c->highest_lvl) ? 1 : 0)) return e;
+    }
+    return 0;
+}
+
+static int tree_extreme(afg_ctx *c, int iv, int include_parents, int want_max, double *out)
+{
+    if (int e = check_ctx(c)) return e;
+    if (!out) return c->fail(AFG_ERR_ARG, "null output");
+    double *v = nullptr;
+    if (int e = ensure_var(c, iv, &v)) return e;
+    const DevTree T = dev_tree(c);
+    int used = 0;
+    for (int lvl = c->min_lvl; lvl <= c->highest_lvl; lvl++) {
+        const ListArgs P = list_args(c, lvl, include_parents ? 2 : 1);
+        launch_extreme_list(c->dim, c->stream, T, P, v, want_max, c->d_partial + used);
+        used += extreme_blocks(c->dim, P);
+    }
+    if (used == 0) {
+        *out = want_max ? -1.7976931348623157e308 : 1.7976931348623157e308;
+        return 0;
+    }
+    launch_reduce_partials(c->stream, c->d_partial, used, want_max, c->d_out);
+    HIP_TRY(c, hipMemcpyAsync(c->h_out, c->d_out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    *out = c->h_out[0];
+    return 0;
+}
+
+int afg_tree_max_cc(afg_ctx *c, int iv, int include_parents, double *out)
+{
+    return tree_extreme(c, iv, include_parents, 1, out);
+}
+
+int afg_tree_min_cc(afg_ctx *c, int iv, int include_parents, double *out)
+{
+    return tree_extreme(c, iv, include_parents, 0, out);
+}
+
+int afg_fill_gc(afg_ctx *c, int iv, int lvl)
+{
+    if (int e = check_ctx(c)) return e;
+    if (int e = check_level(c, lvl, false)) return e;
+    double *v = nullptr;
+    if (int e = ensure_var(c, iv, &v)) return e;
+    if (int e = upload_bc(c)) return e;
+    launch_fill_gc(c->dim, c->stream, dev_tree(c), level_args(c, lvl), v);
+    return 0;
+}
+
+int afg_gsrb_level(afg_ctx *c, const afg_mg_opts *o, int lvl, int n_cycle)
+{
+    if (int e = check_ctx(c)) return e;
+    if (int e = check_level(c, lvl, false)) return e;
+    if (n_cycle < 0) return c->fail(AFG_ERR_ARG, "n_cycle < 0");
+    Mg m;
+    if (int e = make_mg(c, o, &m)) return e;
+    set_tags(m);
+    gsrb_level(m, lvl, n_cycle);
+    return 0;
+}
+
+int afg_residual_level(afg_ctx *c, const afg_mg_opts *o, int lvl)
+{
+    if (int e = check_ctx(c)) return e;
+    if (int e = check_level(c, lvl, false)) return e;
+    Mg m;
+    if (int e = make_mg(c, o, &m)) return e;
+    set_tags(m);
+    residual_level(m, lvl);
+    return 0;
+}
+
+int afg_restrict_level(afg_ctx *c, int iv, int lvl)
+{
+    if (int e = check_ctx(c)) return e;
+    if (int e = check_level(c, lvl, true)) return e;
+    double *v = nullptr;
+    if (int e = ensure_var(c, iv, &v)) return e;
+    launch_restrict(c->dim, c->stream, dev_tree(c), level_args(c, lvl), v, v);
+    return 0;
+}
+
+int afg_prolong2_level(afg_ctx *c, int iv, int iv_to, int lvl, int add)
+{
+    if (int e = check_ctx(c)) return e;
+    if (int e = check_level(c, lvl, true)) return e;
+    double *a = nullptr, *b = nullptr;
+    if (int e = ensure_var(c, iv, &a)) return e;
+    if (int e = ensure_var(c, iv_to, &b)) return e;
+    if (a == b) return c->fail(AFG_ERR_ARG, "prolong2: source and destination must differ");
+    launch_prolong2(c->dim, c->stream, dev_tree(c), level_args(c, lvl), a, b, add ? 1 : 0);
+    return 0;
+}
+
+int afg_correct_children(afg_ctx *c, const afg_mg_opts *o, int lvl)
+{
+    if (int e = check_ctx(c)) return e;
+    if (int e = check_level(c, lvl, true)) return e;
+    Mg m;
+    if (int e = make_mg(c, o, &m)) return e;
+    set_tags(m);
+    correct_children(m, lvl);
+    return 0;
+}
+
+int afg_update_coarse(afg_ctx *c, const afg_mg_opts *o, int lvl)
+{
+    if (int e = check_ctx(c)) return e;
+    if (int e = check_level(c, lvl, true)) return e;
+    Mg m;
+    if (int e = make_mg(c, o, &m)) return e;
+    set_tags(m);
+    update_coarse(m, lvl);
+    return 0;
+}
+
+int afg_set_graphs(afg_ctx *c, int enabled)
+{
+    if (!c) return AFG_ERR_ARG;
+    c->use_graphs = enabled != 0;
+    return 0;
+}
+
+int afg_set_timing(afg_ctx *c, int enabled)
+{
+    if (!c) return AFG_ERR_ARG;
+    if (int e = collect_timing(c)) return e;
+    c->timing = enabled != 0;
+    if (c->timing) {
+        c->t_launches = 0;
+        c->t_ms = 0;
+        c->t_bytes = 0;
+    }
+    return 0;
+}
+
+int afg_timing(afg_ctx *c, int64_t *n_launches, double *ms, double *bytes)
+{
+    if (!c) return AFG_ERR_ARG;
+    if (int e = collect_timing(c)) return e;
+    if (n_launches) *n_launches = c->t_launches;
+    if (ms) *ms = c->t_ms;
+    if (bytes) *bytes = c->t_bytes;
+    return 0;
+}
+
+int afg_vcycle_cell_updates(const afg_ctx *cc, const afg_mg_opts *o, int highest_lvl, double *out)
+{
+    afg_ctx *c = const_cast<afg_ctx *>(cc);
+    if (int e = check_ctx(c)) return e;
+    if (!o || !out) return c->fail(AFG_ERR_ARG, "null argument");
+    if (int e = check_level(c, highest_lvl, false)) return e;
+    double n = 0;
+    for (int lvl = c->min_lvl; lvl <= highest_lvl; lvl++) {
+        const Level &L = c->levels[lvl - c->min_lvl];
+        const double cells = (double)L.n * (c->dim == 2 ? L.nc * L.nc : L.nc * L.nc * L.nc);
+        const int sweeps = lvl == c->min_lvl ? o->n_cycle_base : o->n_cycle_down + o->n_cycle_up;
+        n += cells * sweeps;
+    }
+    *out = n;
+    return 0;
+}
+
+int afg_device_bytes(const afg_ctx *cc, int64_t *out)
+{
+    afg_ctx *c = const_cast<afg_ctx *>(cc);
+    if (!c || !out) return AFG_ERR_ARG;
+    int64_t b = 0;
+    for (double *p : c->d_var)
+        if (p) b += (int64_t)c->n_slots * c->stride * 8;
+    b += (int64_t)c->n_slots * (c->nnb + 3) * 4 + c->n_slots;
+    *out = b;
+    return 0;
+}
+
+}  // extern "C"

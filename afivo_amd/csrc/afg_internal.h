@@ -1,0 +1,95 @@
+// afg_internal.h -- shared declarations of the HIP kernels and the host
+// orchestration of the afivo_amd multigrid library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace afg {
+
+// Encoding of the per-slot neighbour table (the device form of
+// box%neighbors, m_aX_t.f90:163):
+//   >= 0        slot of the same-level neighbour
+//   NB_REFINE   no neighbour at this level: refinement boundary (a5_no_box)
+//   <= NB_BC0   physical boundary, boundary record index NB_BC0 - value
+constexpr int32_t NB_REFINE = -1;
+constexpr int32_t NB_BC0 = -2;
+
+// Tags of mg$D_set_box_tag (m_aX_mg.f90:13-17, 667-706).
+constexpr int8_t TAG_UNSET = 0;
+constexpr int8_t TAG_NORMAL = 1;
+constexpr int8_t TAG_CEPS = 3;
+constexpr int8_t TAG_VEPS = 4;
+
+// Device topology of one tree.  Slots of a level are contiguous and in Morton
+// order; every box array of a variable occupies `stride` doubles at
+// var + slot * stride and holds cc(0:nc_l+1, ...) in Fortran order.
+struct DevTree {
+    int stride;
+    const int32_t *nb;        // [slot * 2*dim + face]
+    const int32_t *parent;    // parent slot, -1 if none
+    const int32_t *choff;     // child offset in the parent: x | y << 10 | z << 20
+    const int32_t *bc_type;   // per boundary record
+    const int32_t *bc_voff;   // offset of the boundary values, -1 for b = 0
+    const double *bc_vals;
+    int8_t *tag;              // per slot (auto operators)
+};
+
+// One level: slots [slot0, slot0 + n), nc cells per dimension, cell size dr;
+// pnc is the cell count of the parent level.
+struct LevelArgs {
+    int slot0, n, nc, pnc;
+    double dr;          // box%dr of the level
+    double dx2;         // dr * dr  (dr**2, m_aX_mg.f90:730)
+    double inv_dr_sq;   // 1 / (dr * dr)  (m_aX_mg.f90:772)
+};
+
+// A list of slots (parents or leaves of a level), all with nc cells.
+struct ListArgs {
+    const int32_t *slots;
+    int n, nc;
+    double dr, dx2, inv_dr_sq;
+};
+
+// Variables of the multigrid options, as device pointers.
+struct MgVars {
+    double *phi, *rhs, *tmp;
+    const double *eps;   // nullptr unless the auto operators are used
+    int op_auto;
+};
+
+// ---- launchers (afg_kernels.hip) ----
+void launch_fill_gc(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, double *var);
+void launch_gsrb_pass(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
+                      const MgVars &V, int rb, int stored_ghosts);
+void launch_residual(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
+                     const MgVars &V);
+void launch_update_coarse_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
+                                const MgVars &V, int keep_child_tmp);
+void launch_update_coarse_parent(int dim, hipStream_t s, const DevTree &T, const ListArgs &P,
+                                 const MgVars &V, int copy_tmp);
+void launch_diff_phi_tmp(int dim, hipStream_t s, const DevTree &T, const ListArgs &P,
+                         const MgVars &V);
+void launch_correct(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
+                    const MgVars &V);
+void launch_restrict(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
+                     const double *from, double *to);
+void launch_prolong2(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
+                     const double *from, double *to, int add);
+void launch_fill_value(int dim, hipStream_t s, const DevTree &T, int slot0, int n_slots,
+                       int nc, double *var, double value);
+void launch_copy_boxes(int dim, hipStream_t s, const DevTree &T, int slot0, int n_slots, int nc,
+                       const double *from, double *to);
+void launch_set_tags(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
+                     const double *eps);
+// Extreme (max or min) of the interiors of the boxes of a slot list: one
+// partial per block into `partial`; returns the number of partials written.
+int extreme_blocks(int dim, const ListArgs &P);
+void launch_extreme_list(int dim, hipStream_t s, const DevTree &T, const ListArgs &P,
+                         const double *var, int want_max, double *partial);
+// out[0] = extreme of partial[0..n).
+void launch_reduce_partials(hipStream_t s, const double *partial, int n, int want_max,
+                            double *out);
+
+}  // namespace afg

@@ -1,0 +1,190 @@
+"""Python host API over the C ABI, mirroring afivo's multigrid interface.
+
+Names follow the reference module ``m_a$D_mg`` (``src/m_aX_mg.f90:87-125``) and
+the tree reductions of ``m_a$D_utils``; ``DeviceTree`` is the device mirror of
+an ``a$D_t`` (topology plus the cell variables).  Errors raise
+:class:`afivo_amd._lib.AfgError` with the library's message, where the
+reference stops with a message.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import AFG_BC_DIRICHLET, AFG_BC_NEUMANN, AFG_OP_AUTO, AFG_OP_LPL, MgOpts, check
+from .treefile import TreeState
+
+__all__ = ["DeviceTree", "MgOpts", "mg_init_mg", "mg_fas_fmg", "mg_fas_vcycle",
+           "tree_max_cc", "tree_min_cc", "AFG_OP_LPL", "AFG_OP_AUTO",
+           "AFG_BC_DIRICHLET", "AFG_BC_NEUMANN"]
+
+
+def _i32(a):
+    return a.ctypes.data_as(_lib.I32P)
+
+
+def _f64(a):
+    return a.ctypes.data_as(_lib.F64P)
+
+
+class DeviceTree:
+    """A tree whose cell data lives in device memory (one afg_ctx)."""
+
+    def __init__(self, dim, n_cell, n_var, records, dr_base):
+        L = _lib.load()
+        h = ctypes.c_void_p()
+        check(L.afg_ctx_create(dim, n_cell, n_var, ctypes.byref(h)))
+        self.h = h
+        self.dim, self.nc, self.n_var = dim, n_cell, n_var
+        self.records = np.ascontiguousarray(records, dtype=np.int32)
+        self.ids = np.ascontiguousarray(self.records[:, 0])
+        check(L.afg_set_tree(self.h, self.records.shape[0], _i32(self.records), float(dr_base)), self.h)
+        lo, hi = ctypes.c_int(), ctypes.c_int()
+        check(L.afg_levels(self.h, ctypes.byref(lo), ctypes.byref(hi)), self.h)
+        self.min_lvl, self.highest_lvl = lo.value, hi.value
+        n = ctypes.c_int64()
+        check(L.afg_packed_size(self.h, self.ids.size, _i32(self.ids), ctypes.byref(n)), self.h)
+        self.n_values = n.value
+
+    @classmethod
+    def from_state(cls, st: TreeState, n_var=None):
+        t = cls(st.dim, st.nc, n_var or max(st.n_var, len(st.data)), st.recs, st.dr_base)
+        for iv, a in enumerate(st.data, start=1):
+            t.upload(iv, a)
+        return t
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.load().afg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _c(self, rc):
+        check(rc, self.h)
+
+    # data movement
+    def upload(self, iv, packed, ids=None):
+        ids = self.ids if ids is None else np.ascontiguousarray(ids, dtype=np.int32)
+        a = np.ascontiguousarray(packed, dtype=np.float64)
+        self._c(_lib.load().afg_upload_cc(self.h, iv, ids.size, _i32(ids), _f64(a)))
+
+    def download(self, iv, ids=None):
+        ids = self.ids if ids is None else np.ascontiguousarray(ids, dtype=np.int32)
+        n = self.n_values if ids is self.ids else self.packed_size(ids)
+        out = np.empty(n, dtype=np.float64)
+        self._c(_lib.load().afg_download_cc(self.h, iv, ids.size, _i32(ids), _f64(out)))
+        return out
+
+    def packed_size(self, ids):
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        n = ctypes.c_int64()
+        self._c(_lib.load().afg_packed_size(self.h, ids.size, _i32(ids), ctypes.byref(n)))
+        return n.value
+
+    def copy_cc(self, iv_from, iv_to):
+        self._c(_lib.load().afg_copy_cc(self.h, iv_from, iv_to))
+
+    def set_cc(self, iv, value):
+        self._c(_lib.load().afg_set_cc(self.h, iv, float(value)))
+
+    def set_bc(self, id_, nb, bc_type, values=None):
+        v = None
+        if values is not None:
+            arr = np.ascontiguousarray(values, dtype=np.float64)
+            v = _f64(arr)
+        self._c(_lib.load().afg_set_bc(self.h, int(id_), int(nb), int(bc_type), v))
+
+    def sync(self):
+        self._c(_lib.load().afg_sync(self.h))
+
+    # single stages
+    def fill_gc(self, iv, lvl):
+        self._c(_lib.load().afg_fill_gc(self.h, iv, lvl))
+
+    def gsrb_level(self, mg, lvl, n_cycle):
+        self._c(_lib.load().afg_gsrb_level(self.h, ctypes.byref(mg), lvl, n_cycle))
+
+    def residual_level(self, mg, lvl):
+        self._c(_lib.load().afg_residual_level(self.h, ctypes.byref(mg), lvl))
+
+    def restrict_level(self, iv, lvl):
+        self._c(_lib.load().afg_restrict_level(self.h, iv, lvl))
+
+    def prolong2_level(self, iv, iv_to, lvl, add):
+        self._c(_lib.load().afg_prolong2_level(self.h, iv, iv_to, lvl, int(add)))
+
+    def correct_children(self, mg, lvl):
+        self._c(_lib.load().afg_correct_children(self.h, ctypes.byref(mg), lvl))
+
+    def update_coarse(self, mg, lvl):
+        self._c(_lib.load().afg_update_coarse(self.h, ctypes.byref(mg), lvl))
+
+    # controls and measurements
+    def set_graphs(self, enabled):
+        self._c(_lib.load().afg_set_graphs(self.h, int(enabled)))
+
+    def set_timing(self, enabled):
+        self._c(_lib.load().afg_set_timing(self.h, int(enabled)))
+
+    def timing(self):
+        n, ms, b = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+        self._c(_lib.load().afg_timing(self.h, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)))
+        return n.value, ms.value, b.value
+
+    def vcycle_cell_updates(self, mg, highest_lvl=None):
+        out = ctypes.c_double()
+        hl = self.highest_lvl if highest_lvl is None else highest_lvl
+        self._c(_lib.load().afg_vcycle_cell_updates(self.h, ctypes.byref(mg), hl, ctypes.byref(out)))
+        return out.value
+
+    def device_bytes(self):
+        out = ctypes.c_int64()
+        self._c(_lib.load().afg_device_bytes(self.h, ctypes.byref(out)))
+        return out.value
+
+
+def mg_init_mg(mg: MgOpts) -> MgOpts:
+    """mg$D_init_mg (m_aX_mg.f90:130-154)."""
+    check(_lib.load().afg_mg_init(ctypes.byref(mg)))
+    return mg
+
+
+def mg_fas_fmg(tree: DeviceTree, mg: MgOpts, set_residual: bool, have_guess: bool):
+    """mg$D_fas_fmg (m_aX_mg.f90:164-203)."""
+    tree._c(_lib.load().afg_fas_fmg(tree.h, ctypes.byref(mg), int(set_residual), int(have_guess)))
+
+
+def mg_fas_vcycle(tree: DeviceTree, mg: MgOpts, highest_lvl: int, set_residual: bool):
+    """mg$D_fas_vcycle (m_aX_mg.f90:208-255)."""
+    tree._c(_lib.load().afg_fas_vcycle(tree.h, ctypes.byref(mg), highest_lvl, int(set_residual)))
+
+
+def tree_max_cc(tree: DeviceTree, iv: int, include_parents: bool = False) -> float:
+    """a$D_tree_max_cc (m_aX_utils.f90:468-518)."""
+    out = ctypes.c_double()
+    tree._c(_lib.load().afg_tree_max_cc(tree.h, iv, int(include_parents), ctypes.byref(out)))
+    return out.value
+
+
+def tree_min_cc(tree: DeviceTree, iv: int, include_parents: bool = False) -> float:
+    """a$D_tree_min_cc (m_aX_utils.f90:522-571)."""
+    out = ctypes.c_double()
+    tree._c(_lib.load().afg_tree_min_cc(tree.h, iv, int(include_parents), ctypes.byref(out)))
+    return out.value
+
+
+def max_abs_cc(tree: DeviceTree, iv: int) -> float:
+    """max(|min|, |max|) over leaf interiors, as examples/poisson_basic_2d.f90:90-92."""
+    return max(abs(tree_min_cc(tree, iv)), abs(tree_max_cc(tree, iv)))
+
+
+def default_mg(eps: bool = False, n_down=2, n_up=2, n_base=2):
+    """Options with phi, rhs, tmp (and eps) in variables 1, 2, 3 (4), as the
+    problems of afivo_amd.problems store them."""
+    mg = MgOpts(i_phi=1, i_rhs=2, i_tmp=3, i_eps=4 if eps else -1, n_cycle_down=n_down,
+                n_cycle_up=n_up, n_cycle_base=n_base, op=AFG_OP_AUTO if eps else AFG_OP_LPL)
+    return mg_init_mg(mg)

@@ -1,0 +1,166 @@
+"""Benchmark of the FAS multigrid hot path on one MI355X.
+
+Workload: the 3D FP64 Poisson configuration -- unit cube, 8^3-cell boxes,
+uniform refinement to level 4 and adaptive refinement to level 7 around four
+Gaussians (sigma 0.03, centres uniform in [0.2, 0.8]^3 from seed 1), threshold
+dr^2 max|rhs| > 1e-3, giving 2.63e6 leaf cells (3.0e6 cells on all levels);
+Dirichlet zero, 7-point Laplacian.  One step is one whole solve: FMG followed
+by 8 V-cycles with 2+2 GSRB sweeps and 50 sweeps on the coarsest level, the
+residual stored after every cycle (as the reference examples do).
+
+`value` is cell-updates/s over the timed steps (a cell-update is one cell
+relaxed by one full red+black sweep; FMG's cycles count), with the tree and
+its data resident in device memory before timing starts.  The roofline object
+is for the dominant kernel, the GSRB colour pass, timed with HIP events on
+the context's stream during the timed steps.  The CPU baseline runs the
+reference's own solver (built from its sources into oracle/_ref) on the same
+tree on this host's cores.
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: all cores up to 16")
+    ap.add_argument("--no-timing", action="store_true",
+                    help="skip the per-kernel event timing (roofline is then null)")
+    return ap.parse_args()
+
+
+def cpu_baseline(p, cell_updates_per_solve, threads):
+    """The reference solver, built from the reference sources (oracle/_ref),
+    on the same tree: median solve time of 5 solves after one warm-up solve."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from afivo_amd import problems as P
+    if not O.ref_available(3):
+        return None
+    import tempfile
+    work = tempfile.mkdtemp(prefix="afg_cpu_")
+    cfg = P.harness_cfg(p, n_time=5, dump_input=False, dump_output=False)
+    prefix, _ = O.run_ref(3, cfg, workdir=work, threads=threads, timeout=900)
+    n_thr, t = O.read_time(prefix)
+    med = float(np.median(t))
+    return {"value": cell_updates_per_solve / med, "unit": "cell-updates/s", "cores": int(n_thr),
+            "kind": "reference",
+            "sample": f"same tree and options, whole solve (FMG + {p.n_vcycle} V-cycles), "
+                      f"median of {len(t)} solves after one warm-up, OpenMP threads={n_thr}",
+            "s_per_solve": med}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported "
+              "(the multi-GPU design is described in DESIGN.md, not built)", file=sys.stderr)
+        return 2
+
+    from afivo_amd import mg as M
+    from afivo_amd import problems as P
+
+    t0 = time.time()
+    p = P.c2(seed=a.seed)
+    st = p.state
+    build_s = time.time() - t0
+    dt = M.DeviceTree.from_state(st, 3)
+    mg = M.default_mg(n_base=p.n_base)
+
+    # cell-updates of one solve: FMG runs a V-cycle up to every level
+    cu_solve = sum(dt.vcycle_cell_updates(mg, lvl) for lvl in range(dt.min_lvl, dt.highest_lvl + 1))
+    cu_solve += p.n_vcycle * dt.vcycle_cell_updates(mg)
+
+    def solve():
+        M.mg_fas_fmg(dt, mg, True, False)
+        for _ in range(p.n_vcycle):
+            M.mg_fas_vcycle(dt, mg, dt.highest_lvl, True)
+
+    for _ in range(a.warmup):
+        solve()
+    dt.sync()
+    rel_res = M.max_abs_cc(dt, 3) / max(abs(M.tree_min_cc(dt, 2)), abs(M.tree_max_cc(dt, 2)))
+
+    if not a.no_timing:
+        dt.set_timing(True)
+    dt.sync()
+    t1 = time.perf_counter()
+    for _ in range(a.steps):
+        solve()
+    dt.sync()
+    t2 = time.perf_counter()
+    elapsed = t2 - t1
+    n_launch, k_ms, k_bytes = dt.timing()
+    dt.set_timing(False)
+
+    value = cu_solve * a.steps / elapsed
+    out = {
+        "metric": "cell-updates/s",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * elapsed / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "3D FP64 Poisson (poisson_basic_3d form), 8^3 boxes, uniform to level 4, "
+                        "adaptive to level 7 around 4 Gaussians (sigma 0.03, seed %d), "
+                        "threshold dr^2 max|rhs| > %g; step = FMG + %d V-cycles, 2+2 GSRB sweeps, "
+                        "%d base sweeps" % (a.seed, p.thr, p.n_vcycle, p.n_base),
+            "boxes": int(st.n_boxes),
+            "leaf_cells": int(p.tree.n_leaf_cells()),
+            "cells_all_levels": int((st.ncell().astype(np.int64) ** 3).sum()),
+            "levels": [int(st.min_lvl), int(st.highest_lvl)],
+            "cell_updates_per_step": cu_solve,
+        },
+        "solves_per_s": a.steps / elapsed,
+        "rel_residual_after_step": rel_res,
+        "host_tree_build_s": build_s,
+    }
+    if n_launch and k_ms > 0:
+        achieved = k_bytes / (k_ms * 1e-3) / 1e9
+        out["roofline"] = {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "k_gsrb_pass (GSRB colour pass, all levels)",
+            "launches": n_launch, "avg_launch_us": 1e3 * k_ms / n_launch,
+            "alg_bytes_per_launch": k_bytes / n_launch,
+            "share_of_step_time": (k_ms * 1e-3) / elapsed,
+        }
+    else:
+        out["roofline"] = None
+    if not a.no_cpu_baseline:
+        thr = a.cpu_threads or min(os.cpu_count() or 1, 16)
+        out["cpu_baseline"] = cpu_baseline(p, cu_solve, thr)
+    else:
+        out["cpu_baseline"] = None
+    dt.close()
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,170 @@
+/*
+ * afivo_amd.h -- C ABI of the MI355X (gfx950) FAS multigrid library.
+ *
+ * The library is a drop-in for afivo's multigrid path: the Fortran module
+ * m_a2_mg / m_a3_mg of this project (afivo_amd/fortran) keeps the reference's
+ * public names and calls these entry points through ISO_C_BINDING.  All
+ * arguments are plain C types; every entry point returns 0 on success and a
+ * non-zero afg_status otherwise, with the message available from
+ * afg_last_error().
+ *
+ * Conventions follow the reference (src/m_aX_t.f90:153-207):
+ *   - box ids are 1-based; a neighbour id > 0 is a box, 0 (a5_no_box) a
+ *     refinement boundary and < 0 a physical boundary; a child id 0 means
+ *     no child;
+ *   - variable indices iv are 1-based, as box%cc(..., iv);
+ *   - a box on level lvl has nc_l = nc (lvl >= 1) or nc / 2^(1-lvl) cells per
+ *     dimension and stores cc(0:nc_l+1, ...) with one ghost layer, x fastest.
+ *
+ * Threading: a context is used by one host thread at a time.  Work is queued
+ * on the context's own HIP stream; entry points that return data to the host
+ * synchronise that stream.
+ */
+#ifndef AFIVO_AMD_H
+#define AFIVO_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct afg_ctx afg_ctx;
+
+enum afg_status {
+    AFG_OK = 0,
+    AFG_ERR_ARG = 1,        /* invalid argument */
+    AFG_ERR_STATE = 2,      /* call out of order (e.g. no tree set) */
+    AFG_ERR_HIP = 3,        /* HIP runtime error */
+    AFG_ERR_MEMORY = 4,     /* device memory exhausted / over budget */
+    AFG_ERR_UNSUPPORTED = 5 /* operator or configuration not implemented on device */
+};
+
+/* Box operator families of m_aX_mg (mg$D_t procedure pointers). */
+enum afg_op {
+    AFG_OP_LPL = 0,   /* mg$D_box_lpl / gsrb_lpl / corr_lpl / rstr_lpl   (m_aX_mg.f90:708-808) */
+    AFG_OP_AUTO = 1   /* mg$D_auto_op / auto_gsrb / auto_corr with rstr_lpl (m_aX_mg.f90:562-665) */
+};
+
+/* Physical boundary types (a5_bc_dirichlet, a5_bc_neumann; m_aX_t.f90:45-48). */
+enum { AFG_BC_DIRICHLET = -1, AFG_BC_NEUMANN = -2 };
+
+/* Arithmetic of the relaxation, transfer and correction stages. */
+enum afg_precision {
+    AFG_PREC_FP64 = 0,   /* every stage in FP64 (reference arithmetic) */
+    AFG_PREC_MIXED = 1   /* FP32 GSRB/restrict/prolong inside FP64 defect correction */
+};
+
+/* The multigrid options of mg$D_t (m_aX_mg.f90:20-52).  Unset values are -1,
+ * filled in by afg_mg_init exactly as mg$D_init_mg does (m_aX_mg.f90:130-154). */
+typedef struct afg_mg_opts {
+    int32_t i_phi, i_rhs, i_tmp;     /* required */
+    int32_t i_eps;                   /* AFG_OP_AUTO only, else -1 */
+    int32_t n_cycle_down, n_cycle_up, n_cycle_base;
+    int32_t op;                      /* enum afg_op */
+    int32_t initialized;             /* set by afg_mg_init */
+    int32_t reserved[7];
+} afg_mg_opts;
+
+/* ------------------------------------------------------------------ */
+/* context                                                             */
+
+/* Create a context on the current HIP device for trees of dimension dim
+ * (2 or 3) with n_cell cells per box dimension and n_var cell variables.
+ * Replaces the device side of a$D_init (m_aX_core.f90:24-131). */
+int afg_ctx_create(int dim, int n_cell, int n_var, afg_ctx **out);
+int afg_ctx_destroy(afg_ctx *ctx);
+/* Message of the last failed call on ctx (or of afg_ctx_create when ctx is NULL). */
+const char *afg_last_error(const afg_ctx *ctx);
+/* Wait for all work queued on the context's stream. */
+int afg_sync(afg_ctx *ctx);
+
+/* ------------------------------------------------------------------ */
+/* tree                                                                */
+
+/* Set the tree topology: n_boxes int32 records
+ *   [id, lvl, ix(dim), parent, children(2^dim), neighbors(2*dim)]
+ * (box$D_t components, m_aX_t.f90:153-177), in any order.  dr_base is the
+ * cell size at level 1 (a$D_t%dr_base).  Builds the per-level tables in
+ * Morton order (m_morton.f90:33-59), allocates the device pools and resets
+ * every physical boundary to Dirichlet zero.  Existing device data is lost. */
+int afg_set_tree(afg_ctx *ctx, int n_boxes, const int32_t *records, double dr_base);
+
+/* Boundary condition of box `id` in direction nb (1-based, a$D_neighb_*):
+ * bc_type AFG_BC_DIRICHLET or AFG_BC_NEUMANN and the boundary values b written
+ * by the user's sides_bc (nc_l^(dim-1) values, first tangential index
+ * fastest), or NULL for b = 0.  The ghost value is c2*b + c1*interior as in
+ * bc_to_gc (m_aX_gc.f90:71-132). */
+int afg_set_bc(afg_ctx *ctx, int id, int nb, int bc_type, const double *b);
+
+/* Number of doubles of one variable packed over the given ids (each box its
+ * whole cc(0:nc_l+1, ..., iv) array, Fortran order). */
+int afg_packed_size(const afg_ctx *ctx, int n_ids, const int32_t *ids, int64_t *n_values);
+/* Copy variable iv of boxes ids[0..n_ids) from host (packed) to the device,
+ * and back. */
+int afg_upload_cc(afg_ctx *ctx, int iv, int n_ids, const int32_t *ids, const double *packed);
+int afg_download_cc(afg_ctx *ctx, int iv, int n_ids, const int32_t *ids, double *packed);
+/* Whole-array device copy cc(..., iv_to) = cc(..., iv_from) on every box, and
+ * cc(..., iv) = value on every box (a$D_box_copy_cc / a$D_box_clear_cc). */
+int afg_copy_cc(afg_ctx *ctx, int iv_from, int iv_to);
+int afg_set_cc(afg_ctx *ctx, int iv, double value);
+
+/* ------------------------------------------------------------------ */
+/* multigrid (m_aX_mg)                                                 */
+
+/* mg$D_init_mg (m_aX_mg.f90:130-154): check and default the options. */
+int afg_mg_init(afg_mg_opts *opts);
+/* mg$D_fas_fmg (m_aX_mg.f90:164-203). */
+int afg_fas_fmg(afg_ctx *ctx, const afg_mg_opts *opts, int set_residual, int have_guess);
+/* mg$D_fas_vcycle (m_aX_mg.f90:208-255). */
+int afg_fas_vcycle(afg_ctx *ctx, const afg_mg_opts *opts, int highest_lvl, int set_residual);
+/* Highest level present in the tree (a$D_t%highest_lvl) and the lowest
+ * (lbound(tree%lvls)). */
+int afg_levels(const afg_ctx *ctx, int *min_lvl, int *highest_lvl);
+
+/* a$D_tree_max_cc / a$D_tree_min_cc (m_aX_utils.f90:468-571): extreme of
+ * cc(..., iv) over leaf interiors (or all boxes if include_parents). */
+int afg_tree_max_cc(afg_ctx *ctx, int iv, int include_parents, double *out);
+int afg_tree_min_cc(afg_ctx *ctx, int iv, int include_parents, double *out);
+
+/* Single stages over one level (the building blocks of the cycles above,
+ * exported for stage-level checks and for callers that drive their own
+ * cycles). */
+/* fill_gc_phi (m_aX_mg.f90:352-365) = a$D_gc_box over a level with
+ * mg$D_sides_rb and the boundary table (m_aX_gc.f90:50-69). */
+int afg_fill_gc(afg_ctx *ctx, int iv, int lvl);
+/* gsrb_boxes (m_aX_mg.f90:397-420). */
+int afg_gsrb_level(afg_ctx *ctx, const afg_mg_opts *opts, int lvl, int n_cycle);
+/* residual_box over a level (m_aX_mg.f90:545-559). */
+int afg_residual_level(afg_ctx *ctx, const afg_mg_opts *opts, int lvl);
+/* a$D_restrict_box of every box of lvl into its parent (m_aX_restrict.f90:67-132). */
+int afg_restrict_level(afg_ctx *ctx, int iv, int lvl);
+/* a$D_prolong2 from the parent into every box of lvl (m_aX_prolong.f90:243-348). */
+int afg_prolong2_level(afg_ctx *ctx, int iv, int iv_to, int lvl, int add);
+/* correct_children over the parents of lvl-1 (m_aX_mg.f90:368-395). */
+int afg_correct_children(afg_ctx *ctx, const afg_mg_opts *opts, int lvl);
+/* update_coarse (m_aX_mg.f90:424-480). */
+int afg_update_coarse(afg_ctx *ctx, const afg_mg_opts *opts, int lvl);
+
+/* ------------------------------------------------------------------ */
+/* execution controls                                                  */
+
+/* Replay whole V-cycles from HIP graphs (default on). */
+int afg_set_graphs(afg_ctx *ctx, int enabled);
+/* Per-stage device timing: while enabled, each launch of the GSRB colour-pass
+ * kernel is bracketed with HIP events on the context stream; afg_timing reads
+ * the number of launches, their summed duration (ms) and their summed
+ * algorithmic bytes. */
+int afg_set_timing(afg_ctx *ctx, int enabled);
+int afg_timing(afg_ctx *ctx, int64_t *n_launches, double *ms, double *bytes);
+/* Number of cell-updates (cells relaxed by one full red+black sweep) one
+ * V-cycle with these options performs, summed over levels. */
+int afg_vcycle_cell_updates(const afg_ctx *ctx, const afg_mg_opts *opts, int highest_lvl, double *out);
+/* Device bytes held by the context (pools and tables). */
+int afg_device_bytes(const afg_ctx *ctx, int64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,71 @@
+"""The C-ABI library loads and exports every entry point include/afivo_amd.h
+declares (no device compute: these run without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from afivo_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "afivo_amd.h")
+
+
+def declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(afg_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    assert "afg_fas_fmg" in names and "afg_fas_vcycle" in names
+    assert len(names) >= 25
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    missing = [n for n in declared() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_every_declared_symbol():
+    assert set(declared()) <= set(_lib.SIGNATURES)
+
+
+def test_mg_init_defaults_and_errors():
+    """afg_mg_init restates mg$D_init_mg (m_aX_mg.f90:130-154): defaults 2/2/2
+    and a stop for missing indices (here: an error code and message)."""
+    L = _lib.load()
+    o = _lib.MgOpts(i_phi=1, i_rhs=2, i_tmp=3)
+    assert L.afg_mg_init(ctypes.byref(o)) == 0
+    assert (o.n_cycle_down, o.n_cycle_up, o.n_cycle_base, o.initialized) == (2, 2, 2, 1)
+    bad = _lib.MgOpts(i_phi=1, i_rhs=2)
+    assert L.afg_mg_init(ctypes.byref(bad)) != 0
+    assert b"i_tmp not set" in L.afg_last_error(None)
+    bad = _lib.MgOpts(i_phi=1, i_rhs=2, i_tmp=3, op=_lib.AFG_OP_AUTO)
+    assert L.afg_mg_init(ctypes.byref(bad)) != 0
+
+
+def test_ctx_create_argument_checks():
+    """a$D_init's checks (m_aX_core.f90:61-63) come before any device call."""
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    assert L.afg_ctx_create(2, 7, 3, ctypes.byref(h)) == 1
+    assert b"even" in L.afg_last_error(None)
+    assert L.afg_ctx_create(4, 8, 3, ctypes.byref(h)) == 1
+    assert L.afg_ctx_create(2, 8, 0, ctypes.byref(h)) == 1
+
+
+def test_null_context_is_rejected():
+    L = _lib.load()
+    assert L.afg_sync(None) != 0
+    assert L.afg_fas_fmg(None, None, 1, 0) != 0
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "nope.so"))
+    monkeypatch.setattr(_lib, "_lib", None)
+    with pytest.raises(_lib.AfgError):
+        _lib.load()

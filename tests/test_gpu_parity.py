@@ -1,0 +1,249 @@
+"""HIP path against the oracle, through the C ABI, bit for bit.
+
+Stage tests start every variable (ghosts and corners included) from random
+values and compare all variables after one stage on one level; solve tests
+run FMG and V-cycles on the benchmark trees and on the reference's own golden
+fixtures.  The comparison is bitwise (sign of zero included): the kernels keep
+the reference's operation order and are compiled without FMA contraction.
+"""
+import zlib
+
+import numpy as np
+import pytest
+
+import oracle as O
+from afivo_amd import mg as M
+from afivo_amd import problems as P
+from afivo_amd import tree as T
+from helpers import assert_bitexact, random_state
+from test_oracle import CASES, load_golden, opts_of
+
+pytestmark = pytest.mark.gpu
+
+
+def pair(st, n_var, data):
+    dt = M.DeviceTree(st.dim, st.nc, n_var, st.recs, st.dr_base)
+    ot = O.OracleTree(st, n_var=n_var)
+    for iv, a in enumerate(data, start=1):
+        dt.upload(iv, a)
+        ot.set(iv, a)
+    return dt, ot
+
+
+def compare_all(dt, ot, n_var, what):
+    for iv in range(1, n_var + 1):
+        assert_bitexact(dt.download(iv), ot.get(iv), f"{what} var {iv}")
+
+
+def problem(kind):
+    if kind == "2d_nc8":
+        return P.small_2d(nc=8)
+    if kind == "2d_nc4":
+        return P.small_2d(nc=4, l_uniform=3, l_max=6)
+    if kind == "2d_nc16":
+        return P.small_2d(nc=16, l_uniform=2, l_max=4)
+    if kind == "3d_nc8":
+        return P.small_3d(nc=8)
+    if kind == "3d_nc4_eps":
+        return P.small_3d(nc=4, l_uniform=2, l_max=4, eps_hi=4.0)
+    if kind == "2d_nc8_eps":
+        return P.make_problem("e2", 2, 8, P.C1_GAUSS, 3, 5, 1e-3, n_base=4, n_vcycle=2, eps_hi=7.0)
+    raise KeyError(kind)
+
+
+KINDS = ["2d_nc8", "2d_nc4", "2d_nc16", "3d_nc8", "3d_nc4_eps", "2d_nc8_eps"]
+STAGES = ["fill_gc", "gsrb", "residual", "restrict", "prolong2", "prolong2_add", "correct", "update_coarse"]
+
+
+def stage_state(p, seed):
+    st = p.state
+    n_var = len(st.data)
+    data = random_state(st, n_var, seed)
+    if n_var == 4:              # keep eps positive and piecewise constant
+        data[3] = st.data[3].copy()
+    return n_var, data
+
+
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("stage", STAGES)
+def test_stage_bitexact(kind, stage):
+    p = problem(kind)
+    st = p.state
+    n_var, data = stage_state(p, seed=zlib.crc32(f"{kind}{stage}".encode()) % 1000)
+    eps = n_var == 4
+    mg = M.default_mg(eps=eps)
+    om = O.default_opts(eps=eps)
+    L = O.lib()
+    for lvl in range(st.min_lvl, st.highest_lvl + 1):
+        dt, ot = pair(st, n_var, data)
+        has_parent = lvl > st.min_lvl
+        if stage == "fill_gc":
+            dt.fill_gc(1, lvl)
+            L.orc_fill_gc_level(ot.h, 1, lvl)
+        elif stage == "gsrb":
+            dt.gsrb_level(mg, lvl, 2)
+            L.orc_gsrb_level(ot.h, om, lvl, 2)
+        elif stage == "residual":
+            dt.residual_level(mg, lvl)
+            L.orc_residual_level(ot.h, om, lvl)
+        elif not has_parent:
+            continue
+        elif stage == "restrict":
+            dt.restrict_level(2, lvl)
+            L.orc_restrict_level(ot.h, 2, lvl)
+        elif stage == "prolong2":
+            dt.prolong2_level(1, 3, lvl, False)
+            L.orc_prolong2_level(ot.h, 1, 3, lvl, 0)
+        elif stage == "prolong2_add":
+            dt.prolong2_level(3, 1, lvl, True)
+            L.orc_prolong2_level(ot.h, 3, 1, lvl, 1)
+        elif stage == "correct":
+            dt.correct_children(mg, lvl)
+            L.orc_correct_children_level(ot.h, om, lvl)
+        elif stage == "update_coarse":
+            dt.update_coarse(mg, lvl)
+            L.orc_update_coarse_level(ot.h, om, lvl)
+        compare_all(dt, ot, n_var, f"{kind} {stage} lvl {lvl}")
+        dt.close()
+
+
+def solve_both(st, n_var, mg, om, n_fmg, n_vcycle, dt=None):
+    dt = dt or M.DeviceTree.from_state(st, n_var)
+    ot = O.OracleTree(st, n_var=n_var)
+    rd, ro = [], []
+    for it in range(n_fmg):
+        M.mg_fas_fmg(dt, mg, True, it > 0)
+        ot.fas_fmg(om, True, it > 0)
+        rd.append(M.max_abs_cc(dt, 3))
+        ro.append(ot.max_abs(3))
+    for _ in range(n_vcycle):
+        M.mg_fas_vcycle(dt, mg, dt.highest_lvl, True)
+        ot.fas_vcycle(om)
+        rd.append(M.max_abs_cc(dt, 3))
+        ro.append(ot.max_abs(3))
+    return dt, ot, np.array(rd), np.array(ro)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_golden_solve_bitexact(name):
+    """The reference's own output (golden fixture) reproduced on the device."""
+    cfg, st_in, st_out, res_ref = load_golden(name)
+    c = cfg["cfg"]
+    eps = c.get("use_eps", False)
+    mg = M.default_mg(eps=eps, n_down=c.get("n_down", 2), n_up=c.get("n_up", 2), n_base=c["n_base"])
+    dt = M.DeviceTree.from_state(st_in)
+    res = []
+    for it in range(c.get("n_fmg", 1)):
+        M.mg_fas_fmg(dt, mg, True, it > 0)
+        res.append(M.max_abs_cc(dt, 3))
+    for _ in range(c.get("n_vcycle", 0)):
+        M.mg_fas_vcycle(dt, mg, dt.highest_lvl, True)
+        res.append(M.max_abs_cc(dt, 3))
+    for iv in range(1, st_out.n_var + 1):
+        assert_bitexact(dt.download(iv), st_out.data[iv - 1], f"{name} var {iv}")
+    assert np.array_equal(np.array(res), res_ref)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_solve_bitexact(kind):
+    p = problem(kind)
+    eps = p.eps_hi is not None
+    mg = M.default_mg(eps=eps, n_base=p.n_base)
+    om = O.default_opts(2, 2, p.n_base, eps=eps)
+    dt, ot, rd, ro = solve_both(p.state, len(p.state.data), mg, om, 2, 2)
+    compare_all(dt, ot, len(p.state.data), kind)
+    assert np.array_equal(rd, ro)
+
+
+def test_c1_full_config_bitexact():
+    """The 2D benchmark configuration: FMG + 10 V-cycles, 2+2 sweeps, 50 base."""
+    p = P.c1()
+    mg = M.default_mg(n_base=50)
+    om = O.default_opts(2, 2, 50)
+    dt, ot, rd, ro = solve_both(p.state, 3, mg, om, 1, 10)
+    compare_all(dt, ot, 3, "c1")
+    assert np.array_equal(rd, ro)
+    assert rd[-1] < 1e-9
+
+
+def test_c2_full_config_bitexact():
+    """The 3D benchmark configuration (2.6e6 leaf cells): FMG + 8 V-cycles."""
+    p = P.c2()
+    mg = M.default_mg(n_base=50)
+    om = O.default_opts(2, 2, 50)
+    dt, ot, rd, ro = solve_both(p.state, 3, mg, om, 1, 8)
+    compare_all(dt, ot, 3, "c2")
+    assert np.array_equal(rd, ro)
+    # per-cycle residual reduction factors
+    f = rd[1:] / rd[:-1]
+    assert (f < 0.2).all(), f
+
+
+def test_graphs_and_direct_launches_agree():
+    p = P.small_3d()
+    mg = M.default_mg(n_base=p.n_base)
+    out = []
+    for graphs in (True, False):
+        dt = M.DeviceTree.from_state(p.state)
+        dt.set_graphs(graphs)
+        M.mg_fas_fmg(dt, mg, True, False)
+        for _ in range(3):
+            M.mg_fas_vcycle(dt, mg, dt.highest_lvl, True)
+        out.append([dt.download(iv) for iv in (1, 2, 3)])
+    for a, b in zip(*out):
+        assert_bitexact(a, b, "graph vs direct")
+
+
+def test_timing_mode_is_exact():
+    """Event timing (graphs with event records) leaves results unchanged."""
+    p = P.small_2d()
+    mg = M.default_mg(n_base=p.n_base)
+    om = O.default_opts(2, 2, p.n_base)
+    dt = M.DeviceTree.from_state(p.state)
+    dt.set_timing(True)
+    dt, ot, rd, ro = solve_both(p.state, 3, mg, om, 1, 3, dt=dt)
+    n, ms, b = dt.timing()
+    assert n > 0 and ms > 0 and b > 0
+    compare_all(dt, ot, 3, "timing")
+
+
+def test_boundary_values_and_neumann():
+    """Non-zero Dirichlet values and Neumann faces from the boundary table."""
+    p = P.small_2d()
+    st = p.state
+    dt = M.DeviceTree.from_state(st)
+    ot = O.OracleTree(st)
+    rng = np.random.default_rng(7)
+    nc_of = dict(zip(st.ids.tolist(), st.ncell().tolist()))
+    for r in range(st.n_boxes):
+        for f in range(4):
+            if st.neighbors[r, f] < 0:
+                id_ = int(st.ids[r])
+                b = rng.standard_normal(nc_of[id_])
+                typ = -2 if (f == 0 and r % 2 == 0) else -1
+                dt.set_bc(id_, f + 1, typ, b)
+                ot.set_bc(id_, f + 1, typ, b)
+    mg = M.default_mg(n_base=p.n_base)
+    om = O.default_opts(2, 2, p.n_base)
+    dt, ot, rd, ro = solve_both(st, 3, mg, om, 1, 2, dt=dt)
+    ot2 = ot
+    compare_all(dt, ot2, 3, "bc")
+
+
+def test_periodic_base():
+    """Two base boxes periodic in x (neighbours through the base list)."""
+    t = T.Tree(2, 8, 1.0 / 16, coarsen_to=2)
+    t.set_base([[1, 1], [2, 1]], [[2, 2, -1, -1], [1, 1, -1, -1]])
+    gs = P.Gaussians(np.array([1.0]), np.array([0.1]), np.array([[0.9, 0.5]]))
+    for _ in range(10):
+        if t.adjust_refinement(P.gauss_refinement(gs, 2, 4, 1e-3)) == 0:
+            break
+    st = t.state()
+    rhs = P.set_rhs(t, st, gs)
+    z = np.zeros_like(rhs)
+    st.data = [z.copy(), rhs, z.copy()]
+    st.n_var = 3
+    mg = M.default_mg(n_base=8)
+    om = O.default_opts(2, 2, 8)
+    dt, ot, rd, ro = solve_both(st, 3, mg, om, 1, 2)
+    compare_all(dt, ot, 3, "periodic")
