@@ -106,6 +106,7 @@ struct afg_ctx {
     double *d_bc_vals = nullptr;
     size_t bc_vals_cap = 0, bc_cap = 0;
     int8_t *d_tag = nullptr;
+    BottomLevels *d_bottom = nullptr;
     std::vector<double *> d_var;
     double *d_partial = nullptr;
     int partial_cap = 0;
@@ -114,6 +115,8 @@ struct afg_ctx {
     size_t stage_cap = 0;
 
     // execution controls
+    int bottom_lvl = -1000;             // highest level run by the bottom kernel
+    int64_t bottom_cells = 8192;        // cell limit per level for the bottom kernel
     bool use_graphs = true;
     std::map<std::tuple<int, int, int, int, int, int, int, int, int, int>, GraphRec> graphs;
     bool capturing = false;
@@ -203,6 +206,8 @@ void free_tree(afg_ctx *c)
     hipFree(c->d_bc_voff);
     hipFree(c->d_bc_vals);
     hipFree(c->d_tag);
+    hipFree(c->d_bottom);
+    c->d_bottom = nullptr;
     hipFree(c->d_partial);
     for (double *p : c->d_var) hipFree(p);
     c->d_nb = c->d_parent = c->d_choff = c->d_lists = c->d_bc_type = c->d_bc_voff = nullptr;
@@ -325,9 +330,11 @@ void timed_gsrb(afg_ctx *c, const DevTree &T, const LevelArgs &L, const MgVars &
         hipEventCreate(&ev.first);
         hipEventCreate(&ev.second);
     }
-    hipEventRecord(ev.first, c->stream);
+    // inside a capture the events must become event-record nodes of the graph
+    const unsigned flags = c->capturing ? hipEventRecordExternal : 0;
+    hipEventRecordWithFlags(ev.first, c->stream, flags);
     launch_gsrb_pass(c->dim, c->stream, T, L, V, rb, stored);
-    hipEventRecord(ev.second, c->stream);
+    hipEventRecordWithFlags(ev.second, c->stream, flags);
     if (c->capturing) {
         c->cap_ev.push_back(ev);
         c->cap_bytes.push_back(gsrb_pass_bytes(c, L, V));
@@ -401,16 +408,15 @@ void fill_gc(Mg &m, double *var, int lvl)
 }
 
 // gsrb_boxes (m_aX_mg.f90:397-420): 2*n_cycle colour passes, each followed in
-// the reference by a ghost fill.  For even nc the passes compute face ghosts
-// on the fly, so only the fill after the last pass is launched; the first
-// pass reads the ghosts in memory, as the reference does.
+// the reference by a ghost fill.  nc is even on every level, so the passes
+// compute face ghosts on the fly and only the fill after the last pass is
+// launched; the first pass reads the ghosts in memory, as the reference does.
 void gsrb_level(Mg &m, int lvl, int n_cycle)
 {
     const LevelArgs L = level_args(m.c, lvl);
-    const bool odd = L.nc & 1;
     for (int rb = 1; rb <= 2 * n_cycle; rb++) {
-        timed_gsrb(m.c, m.T, L, m.V, rb, (rb == 1 || odd) ? 1 : 0);
-        if (odd || rb == 2 * n_cycle) launch_fill_gc(m.c->dim, m.c->stream, m.T, L, m.V.phi);
+        timed_gsrb(m.c, m.T, L, m.V, rb, rb == 1 ? 1 : 0);
+        if (rb == 2 * n_cycle) launch_fill_gc(m.c->dim, m.c->stream, m.T, L, m.V.phi);
     }
 }
 
@@ -453,10 +459,42 @@ void residual_level(Mg &m, int lvl)
     launch_residual(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), m.V);
 }
 
-// mg$D_fas_vcycle (m_aX_mg.f90:208-255)
+BottomArgs bottom_args(Mg &m, int top, int mode)
+{
+    afg_ctx *c = m.c;
+    BottomArgs B;
+    B.levels = c->d_bottom;
+    B.hl = top - c->min_lvl;
+    B.n_down = m.o->n_cycle_down;
+    B.n_up = m.o->n_cycle_up;
+    B.n_base = m.o->n_cycle_base;
+    B.mode = mode;
+    return B;
+}
+
+// mg$D_fas_vcycle (m_aX_mg.f90:208-255).  Levels up to c->bottom_lvl run in
+// the one-workgroup bottom kernel; the levels above, stage by stage.
 void vcycle_body(Mg &m, int highest_lvl, int set_residual)
 {
     afg_ctx *c = m.c;
+    const int lb = std::min(c->bottom_lvl, highest_lvl);
+    if (lb >= c->min_lvl) {
+        for (int lvl = highest_lvl; lvl > lb; lvl--) {
+            gsrb_level(m, lvl, m.o->n_cycle_down);
+            update_coarse(m, lvl);
+        }
+        launch_bottom(c->dim, c->stream, m.T, m.V, bottom_args(m, lb, 0));
+        for (int lvl = lb + 1; lvl <= highest_lvl; lvl++) {
+            correct_children(m, lvl);
+            fill_gc(m, m.V.phi, lvl);
+            gsrb_level(m, lvl, m.o->n_cycle_up);
+        }
+        if (set_residual) {
+            launch_bottom(c->dim, c->stream, m.T, m.V, bottom_args(m, lb, 1));
+            for (int lvl = lb + 1; lvl <= highest_lvl; lvl++) residual_level(m, lvl);
+        }
+        return;
+    }
     for (int lvl = highest_lvl; lvl >= c->min_lvl + 1; lvl--) {
         gsrb_level(m, lvl, m.o->n_cycle_down);
         update_coarse(m, lvl);
@@ -527,6 +565,28 @@ int check_level(afg_ctx *c, int lvl, bool need_parent)
         return c->fail(AFG_ERR_ARG, "level %d outside %d..%d", lvl, c->min_lvl, c->highest_lvl);
     if (need_parent && lvl == c->min_lvl)
         return c->fail(AFG_ERR_ARG, "level %d has no parent level", lvl);
+    return 0;
+}
+
+// The bottom kernel takes the lowest levels while each holds at most
+// bottom_cells cells; their tables go to device memory.
+int choose_bottom(afg_ctx *c)
+{
+    c->bottom_lvl = -1000;
+    BottomLevels h;
+    std::memset(&h, 0, sizeof h);
+    const int nl = (int)c->levels.size();
+    for (int li = 0; li < nl && li < MAX_BOTTOM_LEVELS; li++) {
+        const Level &L = c->levels[li];
+        const int64_t cells = (int64_t)L.n * (c->dim == 2 ? L.nc * L.nc : L.nc * L.nc * L.nc);
+        if (cells > c->bottom_cells) break;
+        c->bottom_lvl = L.lvl;
+        h.lv[li] = level_args(c, L.lvl);
+        h.par[li] = list_args(c, L.lvl, 0);
+    }
+    HIP_TRY(c, hipMemcpy(c->d_bottom, &h, sizeof h, hipMemcpyHostToDevice));
+    for (auto &g : c->graphs) destroy_graph(g.second);
+    c->graphs.clear();
     return 0;
 }
 
@@ -672,8 +732,8 @@ int afg_set_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base)
         L.nc = lvl >= 1 ? c->nc : c->nc >> (1 - lvl);
         if (L.nc < 1 || (lvl < 1 && (c->nc % (1 << (1 - lvl))) != 0))
             return c->fail(AFG_ERR_ARG, "level %d: n_cell %d cannot be coarsened to it", lvl, c->nc);
-        if (L.nc < 2 || ((L.nc & 1) && lvl > min_lvl))
-            return c->fail(AFG_ERR_UNSUPPORTED, "level %d: n_cell %d not supported (need >= 2, even above the lowest level)", lvl, L.nc);
+        if (L.nc < 2 || (L.nc & 1))
+            return c->fail(AFG_ERR_UNSUPPORTED, "level %d: n_cell %d not supported (coarse levels need an even n_cell >= 2)", lvl, L.nc);
         // box%dr = dr_base * 0.5**(lvl-1) (m_aX_core.f90:196, halved per child :764)
         L.dr = dr_base * std::pow(0.5, lvl - 1);
         L.dx2 = L.dr * L.dr;
@@ -794,9 +854,11 @@ int afg_set_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base)
         maxp += extreme_blocks(dim, a);
     }
     c->partial_cap = std::max(maxp, 1);
+    HIP_TRY(c, hipMalloc(&c->d_bottom, sizeof(BottomLevels)));
+    c->have_tree = true;
+    if (int e = choose_bottom(c)) return e;
     HIP_TRY(c, hipMalloc(&c->d_partial, c->partial_cap * sizeof(double)));
     c->bc_dirty = true;
-    c->have_tree = true;
     if (int e = upload_bc(c)) return e;
     return 0;
 }
@@ -1110,6 +1172,15 @@ int afg_update_coarse(afg_ctx *c, const afg_mg_opts *o, int lvl)
     set_tags(m);
     update_coarse(m, lvl);
     return 0;
+}
+
+int afg_set_bottom_cells(afg_ctx *c, int64_t cells)
+{
+    if (int e = check_ctx(c)) return e;
+    if (cells < 0) return c->fail(AFG_ERR_ARG, "bottom cell limit must be >= 0");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->bottom_cells = cells;
+    return choose_bottom(c);
 }
 
 int afg_set_graphs(afg_ctx *c, int enabled)

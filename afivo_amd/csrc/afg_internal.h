@@ -59,7 +59,27 @@ struct MgVars {
     int op_auto;
 };
 
+// The lower part of a V-cycle -- every level from `hl` down to the lowest
+// and back up -- run by one workgroup in one launch (levels with few cells,
+// where per-stage launches would cost more than the work).  Levels are
+// indexed relative to the lowest one.
+constexpr int MAX_BOTTOM_LEVELS = 12;
+constexpr int BOTTOM_BLOCK = 1024;
+// Level tables of the bottom kernel, kept in device memory (index: level
+// relative to the lowest one).
+struct BottomLevels {
+    LevelArgs lv[MAX_BOTTOM_LEVELS];
+    ListArgs par[MAX_BOTTOM_LEVELS];   // parents of each level
+};
+struct BottomArgs {
+    const BottomLevels *levels;   // device pointer
+    int hl;                       // highest level of this call (relative index)
+    int n_down, n_up, n_base;
+    int mode;                     // 0: V-cycle part below and at hl; 1: residual on levels 0..hl
+};
+
 // ---- launchers (afg_kernels.hip) ----
+void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVars &V, const BottomArgs &B);
 void launch_fill_gc(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, double *var);
 void launch_gsrb_pass(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
                       const MgVars &V, int rb, int stored_ghosts);

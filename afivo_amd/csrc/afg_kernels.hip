@@ -99,13 +99,12 @@ __device__ __forceinline__ double ghost_value(const DevTree &T, const double *__
 // one thread per face ghost cell: n * 2*DIM * nc^(DIM-1)
 
 template <int DIM, int NC>
-__global__ __launch_bounds__(BLOCK) void k_fill_gc(DevTree T, LevelArgs L, double *var)
+__device__ __forceinline__ void fill_gc_body(const DevTree &T, const LevelArgs &L, double *var,
+                                             int gid)
 {
     const int nc = NC ? NC : L.nc;
     const int per_face = DIM == 2 ? nc : nc * nc;
     const int per_box = 2 * DIM * per_face;
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= L.n * per_box) return;
     const int b = gid / per_box;
     const int r = gid - b * per_box;
     const int f = r / per_face;
@@ -127,6 +126,19 @@ __global__ __launch_bounds__(BLOCK) void k_fill_gc(DevTree T, LevelArgs L, doubl
     box[gi + s * (gj + s * gk)] = g;
 }
 
+template <int DIM>
+__host__ __device__ inline int fill_gc_threads(int n, int nc)
+{
+    return n * 2 * DIM * (DIM == 2 ? nc : nc * nc);
+}
+
+template <int DIM, int NC>
+__global__ __launch_bounds__(BLOCK) void k_fill_gc(DevTree T, LevelArgs L, double *var)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < fill_gc_threads<DIM>(L.n, NC ? NC : L.nc)) fill_gc_body<DIM, NC>(T, L, var, gid);
+}
+
 // ---------------------------------------------------------------------------
 // one colour pass of GSRB over a level: the body of gsrb_boxes
 // (m_aX_mg.f90:404-411) with mg$D_box_gsrb_lpl (m_aX_mg.f90:718-760) or, for
@@ -141,14 +153,12 @@ __global__ __launch_bounds__(BLOCK) void k_fill_gc(DevTree T, LevelArgs L, doubl
 // this pass because nc is even.  One thread per cell of colour rb.
 
 template <int DIM, int NC>
-__global__ __launch_bounds__(BLOCK) void k_gsrb_pass(DevTree T, LevelArgs L, MgVars V, int rb,
-                                                     int stored)
+__device__ __forceinline__ void gsrb_body(const DevTree &T, const LevelArgs &L, const MgVars &V,
+                                          int rb, int stored, int gid)
 {
     const int nc = NC ? NC : L.nc;
     const int hn = nc >> 1;
     const int half = DIM == 2 ? nc * hn : nc * nc * hn;
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= L.n * half) return;
     const int b = gid / half;
     const int c = gid - b * half;
     const int r = c / hn;
@@ -205,6 +215,20 @@ __global__ __launch_bounds__(BLOCK) void k_gsrb_pass(DevTree T, LevelArgs L, MgV
     }
 }
 
+template <int DIM>
+__host__ __device__ inline int gsrb_threads(int n, int nc)
+{
+    return n * ipow(nc, DIM) / 2;
+}
+
+template <int DIM, int NC>
+__global__ __launch_bounds__(BLOCK) void k_gsrb_pass(DevTree T, LevelArgs L, MgVars V, int rb,
+                                                     int stored)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < gsrb_threads<DIM>(L.n, NC ? NC : L.nc)) gsrb_body<DIM, NC>(T, L, V, rb, stored, gid);
+}
+
 // ---------------------------------------------------------------------------
 // operator A phi at interior cell q of a box: mg$D_box_lpl (m_aX_mg.f90:763-797)
 // or mg$D_box_lpld (m_aX_mg.f90:866-916) for tagged boxes.
@@ -235,12 +259,11 @@ __device__ __forceinline__ double box_op_at(const double *__restrict__ ph,
 // residual_box (m_aX_mg.f90:545-559) over a level: tmp = rhs - A phi
 
 template <int DIM, int NC>
-__global__ __launch_bounds__(BLOCK) void k_residual(DevTree T, LevelArgs L, MgVars V)
+__device__ __forceinline__ void residual_body(const DevTree &T, const LevelArgs &L, const MgVars &V,
+                                              int gid)
 {
     const int nc = NC ? NC : L.nc;
     const int per_box = ipow(nc, DIM);
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= L.n * per_box) return;
     const int b = gid / per_box;
     const int c = gid - b * per_box;
     const int i = c % nc + 1;
@@ -256,6 +279,13 @@ __global__ __launch_bounds__(BLOCK) void k_residual(DevTree T, LevelArgs L, MgVa
     V.tmp[base + q] = V.rhs[base + q] - a;
 }
 
+template <int DIM, int NC>
+__global__ __launch_bounds__(BLOCK) void k_residual(DevTree T, LevelArgs L, MgVars V)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < L.n * ipow(NC ? NC : L.nc, DIM)) residual_body<DIM, NC>(T, L, V, gid);
+}
+
 // ---------------------------------------------------------------------------
 // update_coarse, child part (m_aX_mg.f90:444-465), fused: per coarse cell of a
 // child's octant, compute the residual of its 2^DIM fine cells, restrict it
@@ -266,14 +296,13 @@ __global__ __launch_bounds__(BLOCK) void k_residual(DevTree T, LevelArgs L, MgVa
 // stays in the child's tmp.
 
 template <int DIM, int NC>
-__global__ __launch_bounds__(BLOCK) void k_update_coarse_child(DevTree T, LevelArgs L, MgVars V,
-                                                               int keep_child_tmp)
+__device__ __forceinline__ void update_coarse_child_body(const DevTree &T, const LevelArgs &L,
+                                                         const MgVars &V, int keep_child_tmp,
+                                                         int gid)
 {
     const int nc = NC ? NC : L.nc;
     const int hn = nc >> 1;
     const int per_box = ipow(hn, DIM);
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= L.n * per_box) return;
     const int b = gid / per_box;
     const int c = gid - b * per_box;
     const int ii = c % hn + 1;
@@ -309,20 +338,27 @@ __global__ __launch_bounds__(BLOCK) void k_update_coarse_child(DevTree T, LevelA
     V.phi[pb + pq] = w * sp;
 }
 
+template <int DIM, int NC>
+__global__ __launch_bounds__(BLOCK) void k_update_coarse_child(DevTree T, LevelArgs L, MgVars V,
+                                                               int keep_child_tmp)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < L.n * ipow((NC ? NC : L.nc) / 2, DIM))
+        update_coarse_child_body<DIM, NC>(T, L, V, keep_child_tmp, gid);
+}
+
 // update_coarse, parent part (m_aX_mg.f90:472-479): rhs = A phi + tmp on the
 // interior, rhs = rhs + tmp on ghost and corner cells (a$D_box_add_cc over the
 // whole array), then tmp = phi (a$D_box_copy_cc) if copy_tmp.
 // One thread per cell of the whole (nc+2)^DIM array of each parent.
 
 template <int DIM, int NC>
-__global__ __launch_bounds__(BLOCK) void k_update_coarse_parent(DevTree T, ListArgs P, MgVars V,
-                                                                int copy_tmp)
+__device__ __forceinline__ void update_coarse_parent_body(const DevTree &T, const ListArgs &P,
+                                                          const MgVars &V, int copy_tmp, int gid)
 {
     const int nc = NC ? NC : P.nc;
     const int s = nc + 2;
     const int per_box = ipow(s, DIM);
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= P.n * per_box) return;
     const int b = gid / per_box;
     const int q = gid - b * per_box;
     const int i = q % s;
@@ -343,20 +379,34 @@ __global__ __launch_bounds__(BLOCK) void k_update_coarse_parent(DevTree T, ListA
     if (copy_tmp) V.tmp[base + q] = V.phi[base + q];
 }
 
+template <int DIM, int NC>
+__global__ __launch_bounds__(BLOCK) void k_update_coarse_parent(DevTree T, ListArgs P, MgVars V,
+                                                                int copy_tmp)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < P.n * ipow((NC ? NC : P.nc) + 2, DIM))
+        update_coarse_parent_body<DIM, NC>(T, P, V, copy_tmp, gid);
+}
+
 // correct_children, parent part (m_aX_mg.f90:380-381): tmp = phi - tmp over
 // the whole array.
 
 template <int DIM, int NC>
-__global__ __launch_bounds__(BLOCK) void k_diff_phi_tmp(DevTree T, ListArgs P, MgVars V)
+__device__ __forceinline__ void diff_body(const DevTree &T, const ListArgs &P, const MgVars &V, int gid)
 {
     const int nc = NC ? NC : P.nc;
     const int per_box = ipow(nc + 2, DIM);
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= P.n * per_box) return;
     const int b = gid / per_box;
     const int q = gid - b * per_box;
     const size_t base = (size_t)P.slots[b] * T.stride;
     V.tmp[base + q] = V.phi[base + q] - V.tmp[base + q];
+}
+
+template <int DIM, int NC>
+__global__ __launch_bounds__(BLOCK) void k_diff_phi_tmp(DevTree T, ListArgs P, MgVars V)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < P.n * ipow((NC ? NC : P.nc) + 2, DIM)) diff_body<DIM, NC>(T, P, V, gid);
 }
 
 // Quadratic prolongation of the coarse cell (ic, jc, kc) of parent array p:
@@ -398,13 +448,12 @@ struct Prolong2 {
 // the child's octant, updating its 2^DIM fine cells.
 
 template <int DIM, int NC>
-__global__ __launch_bounds__(BLOCK) void k_correct(DevTree T, LevelArgs L, MgVars V)
+__device__ __forceinline__ void correct_body(const DevTree &T, const LevelArgs &L, const MgVars &V,
+                                             int gid)
 {
     const int nc = NC ? NC : L.nc;
     const int hn = nc >> 1;
     const int per_box = ipow(hn, DIM);
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= L.n * per_box) return;
     const int b = gid / per_box;
     const int c = gid - b * per_box;
     const int ii = c % hn + 1;
@@ -460,6 +509,13 @@ __global__ __launch_bounds__(BLOCK) void k_correct(DevTree T, LevelArgs L, MgVar
             *t = P2.at(m) + *t;
         }
     }
+}
+
+template <int DIM, int NC>
+__global__ __launch_bounds__(BLOCK) void k_correct(DevTree T, LevelArgs L, MgVars V)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < L.n * ipow((NC ? NC : L.nc) / 2, DIM)) correct_body<DIM, NC>(T, L, V, gid);
 }
 
 // a$D_restrict_box (m_aX_restrict.f90:67-132) of every box of a level into
@@ -641,6 +697,115 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_partials(const double *partial
     if (threadIdx.x == 0) out[0] = v;
 }
 
+// ---------------------------------------------------------------------------
+// bottom of the V-cycle in one workgroup
+//
+// Runs, for levels hl .. lowest, exactly the stage sequence of the per-level
+// path (gsrb_boxes, update_coarse, the base relaxation, correct_children,
+// fill_gc_phi; m_aX_mg.f90:218-241) with the same thread bodies, separated by
+// workgroup barriers instead of kernel boundaries.
+
+template <int N>
+struct IC {
+    static constexpr int value = N;
+};
+
+template <typename F>
+__device__ __forceinline__ void nc_dispatch(int nc, F &&f)
+{
+    switch (nc) {
+    case 2: f(IC<2>()); break;
+    case 4: f(IC<4>()); break;
+    case 8: f(IC<8>()); break;
+    case 16: f(IC<16>()); break;
+    default: f(IC<0>()); break;
+    }
+}
+
+template <int DIM>
+__global__ __launch_bounds__(BOTTOM_BLOCK) void k_bottom(DevTree T, MgVars V, BottomArgs A)
+{
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const BottomLevels &B = *A.levels;
+
+    auto fill = [&](int li) {
+        const LevelArgs &L = B.lv[li];
+        nc_dispatch(L.nc, [&](auto ncv) {
+            constexpr int NC = decltype(ncv)::value;
+            const int tot = fill_gc_threads<DIM>(L.n, L.nc);
+            for (int g = tid; g < tot; g += nt) fill_gc_body<DIM, NC>(T, L, V.phi, g);
+        });
+        __syncthreads();
+    };
+    auto gsrb = [&](int li, int n_cycle) {
+        const LevelArgs &L = B.lv[li];
+        for (int rb = 1; rb <= 2 * n_cycle; rb++) {
+            nc_dispatch(L.nc, [&](auto ncv) {
+                constexpr int NC = decltype(ncv)::value;
+                const int tot = gsrb_threads<DIM>(L.n, L.nc);
+                for (int g = tid; g < tot; g += nt) gsrb_body<DIM, NC>(T, L, V, rb, rb == 1, g);
+            });
+            __syncthreads();
+        }
+        if (n_cycle > 0) fill(li);
+    };
+    auto update_coarse = [&](int li) {
+        const LevelArgs &L = B.lv[li];
+        nc_dispatch(L.nc, [&](auto ncv) {
+            constexpr int NC = decltype(ncv)::value;
+            const int tot = L.n * ipow(L.nc / 2, DIM);
+            for (int g = tid; g < tot; g += nt) update_coarse_child_body<DIM, NC>(T, L, V, 1, g);
+        });
+        __syncthreads();
+        fill(li - 1);
+        const ListArgs &P = B.par[li - 1];
+        nc_dispatch(P.nc, [&](auto ncv) {
+            constexpr int NC = decltype(ncv)::value;
+            const int tot = P.n * ipow(P.nc + 2, DIM);
+            for (int g = tid; g < tot; g += nt) update_coarse_parent_body<DIM, NC>(T, P, V, 1, g);
+        });
+        __syncthreads();
+    };
+    auto correct_children = [&](int li) {
+        const ListArgs &P = B.par[li - 1];
+        nc_dispatch(P.nc, [&](auto ncv) {
+            constexpr int NC = decltype(ncv)::value;
+            const int tot = P.n * ipow(P.nc + 2, DIM);
+            for (int g = tid; g < tot; g += nt) diff_body<DIM, NC>(T, P, V, g);
+        });
+        __syncthreads();
+        const LevelArgs &L = B.lv[li];
+        nc_dispatch(L.nc, [&](auto ncv) {
+            constexpr int NC = decltype(ncv)::value;
+            const int tot = L.n * ipow(L.nc / 2, DIM);
+            for (int g = tid; g < tot; g += nt) correct_body<DIM, NC>(T, L, V, g);
+        });
+        __syncthreads();
+    };
+
+    if (A.mode == 1) {
+        for (int li = 0; li <= A.hl; li++) {
+            const LevelArgs &L = B.lv[li];
+            nc_dispatch(L.nc, [&](auto ncv) {
+                constexpr int NC = decltype(ncv)::value;
+                const int tot = L.n * ipow(L.nc, DIM);
+                for (int g = tid; g < tot; g += nt) residual_body<DIM, NC>(T, L, V, g);
+            });
+        }
+        return;
+    }
+    for (int li = A.hl; li >= 1; li--) {
+        gsrb(li, A.n_down);
+        update_coarse(li);
+    }
+    gsrb(0, A.n_base);
+    for (int li = 1; li <= A.hl; li++) {
+        correct_children(li);
+        fill(li);
+        gsrb(li, A.n_up);
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -786,6 +951,14 @@ void launch_reduce_partials(hipStream_t s, const double *partial, int n, int wan
                             double *out)
 {
     k_reduce_partials<<<1, BLOCK, 0, s>>>(partial, n, want_max, out);
+}
+
+void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVars &V, const BottomArgs &B)
+{
+    if (dim == 2)
+        k_bottom<2><<<1, BOTTOM_BLOCK, 0, s>>>(T, V, B);
+    else
+        k_bottom<3><<<1, BOTTOM_BLOCK, 0, s>>>(T, V, B);
 }
 
 }  // namespace afg

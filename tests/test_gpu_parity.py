@@ -107,9 +107,9 @@ def test_stage_bitexact(kind, stage):
         dt.close()
 
 
-def solve_both(st, n_var, mg, om, n_fmg, n_vcycle, dt=None):
+def solve_both(st, n_var, mg, om, n_fmg, n_vcycle, dt=None, ot=None):
     dt = dt or M.DeviceTree.from_state(st, n_var)
-    ot = O.OracleTree(st, n_var=n_var)
+    ot = ot or O.OracleTree(st, n_var=n_var)
     rd, ro = [], []
     for it in range(n_fmg):
         M.mg_fas_fmg(dt, mg, True, it > 0)
@@ -225,9 +225,9 @@ def test_boundary_values_and_neumann():
                 ot.set_bc(id_, f + 1, typ, b)
     mg = M.default_mg(n_base=p.n_base)
     om = O.default_opts(2, 2, p.n_base)
-    dt, ot, rd, ro = solve_both(st, 3, mg, om, 1, 2, dt=dt)
-    ot2 = ot
-    compare_all(dt, ot2, 3, "bc")
+    dt, ot, rd, ro = solve_both(st, 3, mg, om, 1, 2, dt=dt, ot=ot)
+    compare_all(dt, ot, 3, "bc")
+    assert np.array_equal(rd, ro)
 
 
 def test_periodic_base():
@@ -247,3 +247,17 @@ def test_periodic_base():
     om = O.default_opts(2, 2, 8)
     dt, ot, rd, ro = solve_both(st, 3, mg, om, 1, 2)
     compare_all(dt, ot, 3, "periodic")
+
+
+@pytest.mark.parametrize("cells", [0, 512, 1 << 20])
+def test_bottom_kernel_limit_is_exact(cells):
+    """The one-workgroup bottom kernel and per-level launches give the same
+    bits, whichever levels the bottom kernel takes."""
+    p = P.small_3d()
+    mg = M.default_mg(n_base=p.n_base)
+    om = O.default_opts(2, 2, p.n_base)
+    dt = M.DeviceTree.from_state(p.state)
+    dt.set_bottom_cells(cells)
+    dt, ot, rd, ro = solve_both(p.state, 3, mg, om, 1, 2, dt=dt)
+    compare_all(dt, ot, 3, f"bottom {cells}")
+    assert np.array_equal(rd, ro)
