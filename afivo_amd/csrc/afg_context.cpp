@@ -106,7 +106,8 @@ struct afg_ctx {
     double *d_bc_vals = nullptr;
     size_t bc_vals_cap = 0, bc_cap = 0;
     int8_t *d_tag = nullptr;
-    BottomLevels *d_bottom = nullptr;
+    BottomLevels *d_bottom[2] = {nullptr, nullptr};   // 3 and 4 variables in LDS
+    int8_t *d_is_parent = nullptr;
     std::vector<double *> d_var;
     double *d_partial = nullptr;
     int partial_cap = 0;
@@ -115,7 +116,9 @@ struct afg_ctx {
     size_t stage_cap = 0;
 
     // execution controls
-    int bottom_lvl = -1000;             // highest level run by the bottom kernel
+    int bottom_lvl[2] = {-1000, -1000};  // highest level run by the bottom kernel (3 / 4 variables)
+    size_t bottom_lds[2] = {0, 0};       // its dynamic LDS bytes
+    int64_t lds_limit = 65536;           // LDS bytes one workgroup may use
     int64_t bottom_cells = 8192;        // cell limit per level for the bottom kernel
     bool use_graphs = true;
     std::map<std::tuple<int, int, int, int, int, int, int, int, int, int>, GraphRec> graphs;
@@ -161,6 +164,7 @@ DevTree dev_tree(const afg_ctx *c)
     T.bc_voff = c->d_bc_voff;
     T.bc_vals = c->d_bc_vals;
     T.tag = c->d_tag;
+    T.is_parent = c->d_is_parent;
     return T;
 }
 
@@ -206,8 +210,11 @@ void free_tree(afg_ctx *c)
     hipFree(c->d_bc_voff);
     hipFree(c->d_bc_vals);
     hipFree(c->d_tag);
-    hipFree(c->d_bottom);
-    c->d_bottom = nullptr;
+    hipFree(c->d_bottom[0]);
+    hipFree(c->d_bottom[1]);
+    c->d_bottom[0] = c->d_bottom[1] = nullptr;
+    hipFree(c->d_is_parent);
+    c->d_is_parent = nullptr;
     hipFree(c->d_partial);
     for (double *p : c->d_var) hipFree(p);
     c->d_nb = c->d_parent = c->d_choff = c->d_lists = c->d_bc_type = c->d_bc_voff = nullptr;
@@ -316,10 +323,10 @@ double gsrb_pass_bytes(const afg_ctx *c, const LevelArgs &L, const MgVars &V)
     return 0.5 * cells * per_cell;
 }
 
-void timed_gsrb(afg_ctx *c, const DevTree &T, const LevelArgs &L, const MgVars &V, int rb, int stored)
+void timed_gsrb(afg_ctx *c, const DevTree &T, const LevelArgs &L, const MgVars &V, int rb, int fly)
 {
     if (!c->timing) {
-        launch_gsrb_pass(c->dim, c->stream, T, L, V, rb, stored);
+        launch_gsrb_pass(c->dim, c->stream, T, L, V, rb, fly);
         return;
     }
     std::pair<hipEvent_t, hipEvent_t> ev;
@@ -333,7 +340,7 @@ void timed_gsrb(afg_ctx *c, const DevTree &T, const LevelArgs &L, const MgVars &
     // inside a capture the events must become event-record nodes of the graph
     const unsigned flags = c->capturing ? hipEventRecordExternal : 0;
     hipEventRecordWithFlags(ev.first, c->stream, flags);
-    launch_gsrb_pass(c->dim, c->stream, T, L, V, rb, stored);
+    launch_gsrb_pass(c->dim, c->stream, T, L, V, rb, fly);
     hipEventRecordWithFlags(ev.second, c->stream, flags);
     if (c->capturing) {
         c->cap_ev.push_back(ev);
@@ -407,34 +414,43 @@ void fill_gc(Mg &m, double *var, int lvl)
     launch_fill_gc(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), var);
 }
 
-// gsrb_boxes (m_aX_mg.f90:397-420): 2*n_cycle colour passes, each followed in
-// the reference by a ghost fill.  nc is even on every level, so the passes
-// compute face ghosts on the fly and only the fill after the last pass is
-// launched; the first pass reads the ghosts in memory, as the reference does.
-void gsrb_level(Mg &m, int lvl, int n_cycle)
+// The colour passes of gsrb_boxes (m_aX_mg.f90:397-420) on one level.  The
+// reference fills the ghost layer after every pass; nc is even on every
+// level, so passes 2.. compute the face ghosts they read on the fly (the
+// values that fill would store) and only the ghost fill after the last pass
+// remains, which the caller launches or fuses into the next stage.  The
+// first pass reads the ghosts in memory (first_fly = 0), as the reference
+// does, or computes them when a ghost fill precedes it (first_fly = 1).
+void passes(Mg &m, int lvl, int n_cycle, int first_fly)
 {
     const LevelArgs L = level_args(m.c, lvl);
-    for (int rb = 1; rb <= 2 * n_cycle; rb++) {
-        timed_gsrb(m.c, m.T, L, m.V, rb, rb == 1 ? 1 : 0);
-        if (rb == 2 * n_cycle) launch_fill_gc(m.c->dim, m.c->stream, m.T, L, m.V.phi);
-    }
+    for (int rb = 1; rb <= 2 * n_cycle; rb++) timed_gsrb(m.c, m.T, L, m.V, rb, rb == 1 ? first_fly : 1);
 }
 
-// update_coarse (m_aX_mg.f90:424-480)
-void update_coarse(Mg &m, int lvl)
+// gsrb_boxes as the reference has it: passes, then the last ghost fill.
+void gsrb_level(Mg &m, int lvl, int n_cycle)
 {
-    launch_update_coarse_child(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), m.V, 1);
-    fill_gc(m, m.V.phi, lvl - 1);
-    launch_update_coarse_parent(m.c->dim, m.c->stream, m.T, list_args(m.c, lvl - 1, 0), m.V, 1);
+    passes(m, lvl, n_cycle, 0);
+    if (n_cycle > 0) fill_gc(m, m.V.phi, lvl);
+}
+
+// update_coarse (m_aX_mg.f90:424-480) after `n_cycle` relaxations of lvl: the
+// child stage reads lvl's face ghosts as the fill after the last relaxation
+// stores them (that fill is not launched: nothing else reads those ghosts
+// before the upward sweep fills lvl again), and the ghost fill of lvl-1 runs
+// inside the parent stage.
+void update_coarse(Mg &m, int lvl, int relaxed)
+{
+    launch_uc_child(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), m.V, 1, relaxed ? 1 : 0);
+    launch_uc_parent(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl - 1), m.V, 1);
 }
 
 // set_coarse_phi_rhs (m_aX_mg.f90:484-517)
 void set_coarse_phi_rhs(Mg &m, int lvl)
 {
     if (lvl == m.c->highest_lvl) fill_gc(m, m.V.phi, lvl);
-    launch_update_coarse_child(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), m.V, 0);
-    fill_gc(m, m.V.phi, lvl - 1);
-    launch_update_coarse_parent(m.c->dim, m.c->stream, m.T, list_args(m.c, lvl - 1, 0), m.V, 0);
+    launch_uc_child(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), m.V, 0, 0);
+    launch_uc_parent(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl - 1), m.V, 0);
 }
 
 // init_phi_rhs (m_aX_mg.f90:520-543)
@@ -447,11 +463,15 @@ void init_phi_rhs(Mg &m)
     }
 }
 
-// correct_children over the parents of lvl-1 (m_aX_mg.f90:368-395)
-void correct_children(Mg &m, int lvl)
+// correct_children over the parents of lvl-1 (m_aX_mg.f90:368-395) followed
+// by relaxation and the ghost fill of lvl: the correction is formed on the
+// fly from the parents' phi and tmp, and the parents' tmp = phi - tmp is
+// written with the final ghost fill of lvl (nothing reads it in between).
+void correct_relax(Mg &m, int lvl, int n_cycle)
 {
-    launch_diff_phi_tmp(m.c->dim, m.c->stream, m.T, list_args(m.c, lvl - 1, 0), m.V);
-    launch_correct(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), m.V);
+    launch_correct(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), m.V, 1);
+    passes(m, lvl, n_cycle, 1);
+    launch_fill_diff(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), list_args(m.c, lvl - 1, 0), m.V);
 }
 
 void residual_level(Mg &m, int lvl)
@@ -459,54 +479,46 @@ void residual_level(Mg &m, int lvl)
     launch_residual(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), m.V);
 }
 
-BottomArgs bottom_args(Mg &m, int top, int mode)
+BottomArgs bottom_args(Mg &m, int top)
 {
     afg_ctx *c = m.c;
     BottomArgs B;
-    B.levels = c->d_bottom;
+    B.levels = c->d_bottom[m.V.op_auto ? 1 : 0];
     B.hl = top - c->min_lvl;
     B.n_down = m.o->n_cycle_down;
     B.n_up = m.o->n_cycle_up;
     B.n_base = m.o->n_cycle_base;
-    B.mode = mode;
     return B;
 }
 
-// mg$D_fas_vcycle (m_aX_mg.f90:208-255).  Levels up to c->bottom_lvl run in
-// the one-workgroup bottom kernel; the levels above, stage by stage.
+// highest level run by the bottom kernel for these options
+int bottom_top(const Mg &m) { return m.c->bottom_lvl[m.V.op_auto ? 1 : 0]; }
+
+// mg$D_fas_vcycle (m_aX_mg.f90:208-255).  Levels up to bottom_top run in the
+// one-workgroup LDS bottom kernel; the levels above, stage by stage.
 void vcycle_body(Mg &m, int highest_lvl, int set_residual)
 {
     afg_ctx *c = m.c;
-    const int lb = std::min(c->bottom_lvl, highest_lvl);
+    const int lb = std::min(bottom_top(m), highest_lvl);
+    const int low = lb >= c->min_lvl ? lb : c->min_lvl;
+    for (int lvl = highest_lvl; lvl > low; lvl--) {
+        passes(m, lvl, m.o->n_cycle_down, 0);
+        update_coarse(m, lvl, m.o->n_cycle_down > 0);
+    }
     if (lb >= c->min_lvl) {
-        for (int lvl = highest_lvl; lvl > lb; lvl--) {
-            gsrb_level(m, lvl, m.o->n_cycle_down);
-            update_coarse(m, lvl);
-        }
-        launch_bottom(c->dim, c->stream, m.T, m.V, bottom_args(m, lb, 0));
-        for (int lvl = lb + 1; lvl <= highest_lvl; lvl++) {
-            correct_children(m, lvl);
-            fill_gc(m, m.V.phi, lvl);
-            gsrb_level(m, lvl, m.o->n_cycle_up);
-        }
-        if (set_residual) {
-            launch_bottom(c->dim, c->stream, m.T, m.V, bottom_args(m, lb, 1));
+        launch_bottom(c->dim, c->stream, m.T, m.V, bottom_args(m, lb), c->bottom_lds[m.V.op_auto ? 1 : 0]);
+    } else {
+        gsrb_level(m, c->min_lvl, m.o->n_cycle_base);
+    }
+    for (int lvl = low + 1; lvl <= highest_lvl; lvl++) correct_relax(m, lvl, m.o->n_cycle_up);
+    if (set_residual) {
+        if (lb >= c->min_lvl) {
+            launch_bottom_residual(c->dim, c->stream, m.T, m.V, bottom_args(m, lb));
             for (int lvl = lb + 1; lvl <= highest_lvl; lvl++) residual_level(m, lvl);
+        } else {
+            for (int lvl = c->min_lvl; lvl <= highest_lvl; lvl++) residual_level(m, lvl);
         }
-        return;
     }
-    for (int lvl = highest_lvl; lvl >= c->min_lvl + 1; lvl--) {
-        gsrb_level(m, lvl, m.o->n_cycle_down);
-        update_coarse(m, lvl);
-    }
-    gsrb_level(m, c->min_lvl, m.o->n_cycle_base);
-    for (int lvl = c->min_lvl + 1; lvl <= highest_lvl; lvl++) {
-        correct_children(m, lvl);
-        fill_gc(m, m.V.phi, lvl);
-        gsrb_level(m, lvl, m.o->n_cycle_up);
-    }
-    if (set_residual)
-        for (int lvl = c->min_lvl; lvl <= highest_lvl; lvl++) residual_level(m, lvl);
 }
 
 // Run a V-cycle, from a cached HIP graph when enabled.  In timing mode the
@@ -569,22 +581,52 @@ int check_level(afg_ctx *c, int lvl, bool need_parent)
 }
 
 // The bottom kernel takes the lowest levels while each holds at most
-// bottom_cells cells; their tables go to device memory.
+// bottom_cells cells and their phi, rhs, tmp (and eps, for the auto
+// operators) fit in the LDS of one workgroup.  Two tables (3 and 4 variables)
+// go to device memory.
 int choose_bottom(afg_ctx *c)
 {
-    c->bottom_lvl = -1000;
-    BottomLevels h;
-    std::memset(&h, 0, sizeof h);
-    const int nl = (int)c->levels.size();
-    for (int li = 0; li < nl && li < MAX_BOTTOM_LEVELS; li++) {
-        const Level &L = c->levels[li];
-        const int64_t cells = (int64_t)L.n * (c->dim == 2 ? L.nc * L.nc : L.nc * L.nc * L.nc);
-        if (cells > c->bottom_cells) break;
-        c->bottom_lvl = L.lvl;
-        h.lv[li] = level_args(c, L.lvl);
-        h.par[li] = list_args(c, L.lvl, 0);
+    for (int k = 0; k < 2; k++) {
+        const int nv = 3 + k;
+        c->bottom_lvl[k] = -1000;
+        BottomLevels h;
+        std::memset(&h, 0, sizeof h);
+        const int nl = (int)c->levels.size();
+        int64_t per_var = 0;   // doubles of one variable over the levels taken
+        int top = -1;
+        for (int li = 0; li < nl && li < MAX_BOTTOM_LEVELS; li++) {
+            const Level &L = c->levels[li];
+            const int64_t cells = (int64_t)L.n * (c->dim == 2 ? L.nc * L.nc : L.nc * L.nc * L.nc);
+            const int64_t len = box_len(c->dim, L.nc);
+            if (cells > c->bottom_cells) break;
+            if ((per_var + L.n * len) * nv * (int64_t)sizeof(double) > c->lds_limit) break;
+            per_var += L.n * len;
+            top = li;
+        }
+        int64_t off = 0;
+        for (int li = 0; li <= top; li++) {
+            const Level &L = c->levels[li];
+            h.lv[li] = level_args(c, L.lvl);
+            h.par[li] = list_args(c, L.lvl, 0);
+            h.box_len[li] = box_len(c->dim, L.nc);
+        }
+        // variable-major layout over all levels taken
+        c->bottom_lds[k] = (size_t)(per_var * nv * sizeof(double));
+        for (int v = 0; v < 4; v++) {
+            off = v * per_var;
+            for (int li = 0; li <= top; li++) {
+                h.lds_off[v][li] = (int)off;
+                off += (int64_t)c->levels[li].n * h.box_len[li];
+            }
+        }
+        if (top >= 0) {
+            c->bottom_lvl[k] = c->levels[top].lvl;
+            if (bottom_prepare(c->dim, (size_t)(per_var * nv * sizeof(double))) != 0)
+                return c->fail(AFG_ERR_HIP, "cannot allow %lld bytes of LDS for the bottom kernel",
+                               (long long)(per_var * nv * sizeof(double)));
+        }
+        HIP_TRY(c, hipMemcpy(c->d_bottom[k], &h, sizeof h, hipMemcpyHostToDevice));
     }
-    HIP_TRY(c, hipMemcpy(c->d_bottom, &h, sizeof h, hipMemcpyHostToDevice));
     for (auto &g : c->graphs) destroy_graph(g.second);
     c->graphs.clear();
     return 0;
@@ -621,6 +663,12 @@ int afg_ctx_create(int dim, int n_cell, int n_var, afg_ctx **out)
         return AFG_ERR_HIP;
     }
     afg_ctx *c = new afg_ctx;
+    {
+        int a = 0, b = 0;
+        hipDeviceGetAttribute(&a, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+        hipDeviceGetAttribute(&b, hipDeviceAttributeSharedMemPerBlockOptin, dev);
+        c->lds_limit = std::max(a, b) > 0 ? std::max(a, b) : 65536;
+    }
     c->dim = dim;
     c->nc = n_cell;
     c->n_var = n_var;
@@ -854,7 +902,17 @@ int afg_set_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base)
         maxp += extreme_blocks(dim, a);
     }
     c->partial_cap = std::max(maxp, 1);
-    HIP_TRY(c, hipMalloc(&c->d_bottom, sizeof(BottomLevels)));
+    HIP_TRY(c, hipMalloc(&c->d_bottom[0], sizeof(BottomLevels)));
+    HIP_TRY(c, hipMalloc(&c->d_bottom[1], sizeof(BottomLevels)));
+    {
+        std::vector<int8_t> isp(sn, 0);
+        for (int s2 = 0; s2 < slot; s2++) {
+            const int32_t *q = recs + (size_t)rec_of[c->slot2id[s2]] * rl;
+            isp[s2] = q[3 + dim] != 0;   // a$D_has_children
+        }
+        HIP_TRY(c, hipMalloc(&c->d_is_parent, sn));
+        HIP_TRY(c, hipMemcpy(c->d_is_parent, isp.data(), sn, hipMemcpyHostToDevice));
+    }
     c->have_tree = true;
     if (int e = choose_bottom(c)) return e;
     HIP_TRY(c, hipMalloc(&c->d_partial, c->partial_cap * sizeof(double)));
@@ -1054,8 +1112,8 @@ int afg_fas_fmg(afg_ctx *c, const afg_mg_opts *o, int set_residual, int have_gue
         const LevelArgs L = level_args(c, lvl);
         launch_copy_boxes(c->dim, c->stream, m.T, L.slot0, L.n, L.nc, m.V.phi, m.V.tmp);
         if (lvl > c->min_lvl) {
-            correct_children(m, lvl);
-            fill_gc(m, m.V.phi, lvl);
+            // correct_children + fill_gc_phi (m_aX_mg.f90:193-196)
+            correct_relax(m, lvl, 0);
         }
         if (int e = run_vcycle(m, lvl, (set_residual && lvl == c->highest_lvl) ? 1 : 0)) return e;
     }
@@ -1159,7 +1217,10 @@ int afg_correct_children(afg_ctx *c, const afg_mg_opts *o, int lvl)
     Mg m;
     if (int e = make_mg(c, o, &m)) return e;
     set_tags(m);
-    correct_children(m, lvl);
+    // correct_children (m_aX_mg.f90:368-395): tmp = phi - tmp on the parents,
+    // then the prolongation of tmp into the children
+    launch_diff(c->dim, c->stream, m.T, list_args(c, lvl - 1, 0), m.V);
+    launch_correct(c->dim, c->stream, m.T, level_args(c, lvl), m.V, 0);
     return 0;
 }
 
@@ -1170,7 +1231,7 @@ int afg_update_coarse(afg_ctx *c, const afg_mg_opts *o, int lvl)
     Mg m;
     if (int e = make_mg(c, o, &m)) return e;
     set_tags(m);
-    update_coarse(m, lvl);
+    update_coarse(m, lvl, 0);
     return 0;
 }
 

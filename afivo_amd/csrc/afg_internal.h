@@ -34,6 +34,7 @@ struct DevTree {
     const int32_t *bc_voff;   // offset of the boundary values, -1 for b = 0
     const double *bc_vals;
     int8_t *tag;              // per slot (auto operators)
+    const int8_t *is_parent;  // per slot: the box has children
 };
 
 // One level: slots [slot0, slot0 + n), nc cells per dimension, cell size dr;
@@ -66,33 +67,37 @@ struct MgVars {
 constexpr int MAX_BOTTOM_LEVELS = 12;
 constexpr int BOTTOM_BLOCK = 1024;
 // Level tables of the bottom kernel, kept in device memory (index: level
-// relative to the lowest one).
+// relative to the lowest one), with the LDS layout of its copies: variable v
+// (phi, rhs, tmp, eps) of level li starts at lds_off[v][li] doubles.
 struct BottomLevels {
     LevelArgs lv[MAX_BOTTOM_LEVELS];
     ListArgs par[MAX_BOTTOM_LEVELS];   // parents of each level
+    int box_len[MAX_BOTTOM_LEVELS];    // (nc+2)^dim
+    int lds_off[4][MAX_BOTTOM_LEVELS];
 };
 struct BottomArgs {
     const BottomLevels *levels;   // device pointer
     int hl;                       // highest level of this call (relative index)
     int n_down, n_up, n_base;
-    int mode;                     // 0: V-cycle part below and at hl; 1: residual on levels 0..hl
 };
 
 // ---- launchers (afg_kernels.hip) ----
-void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVars &V, const BottomArgs &B);
 void launch_fill_gc(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, double *var);
+// ghost fill of L and tmp = phi - tmp on the parents P of the level below
+void launch_fill_diff(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
+                      const ListArgs &P, const MgVars &V);
+void launch_diff(int dim, hipStream_t s, const DevTree &T, const ListArgs &P, const MgVars &V);
 void launch_gsrb_pass(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                      const MgVars &V, int rb, int stored_ghosts);
+                      const MgVars &V, int rb, int fly);
 void launch_residual(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
                      const MgVars &V);
-void launch_update_coarse_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                                const MgVars &V, int keep_child_tmp);
-void launch_update_coarse_parent(int dim, hipStream_t s, const DevTree &T, const ListArgs &P,
-                                 const MgVars &V, int copy_tmp);
-void launch_diff_phi_tmp(int dim, hipStream_t s, const DevTree &T, const ListArgs &P,
-                         const MgVars &V);
+void launch_uc_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
+                     const MgVars &V, int keep_child_tmp, int fly);
+// L: the parent level (all its boxes get their ghosts filled)
+void launch_uc_parent(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
+                      const MgVars &V, int copy_tmp);
 void launch_correct(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                    const MgVars &V);
+                    const MgVars &V, int fly_diff);
 void launch_restrict(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
                      const double *from, double *to);
 void launch_prolong2(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
@@ -111,5 +116,11 @@ void launch_extreme_list(int dim, hipStream_t s, const DevTree &T, const ListArg
 // out[0] = extreme of partial[0..n).
 void launch_reduce_partials(hipStream_t s, const double *partial, int n, int want_max,
                             double *out);
+// bottom kernel: allow lds_bytes of dynamic LDS (0 on success), then launch
+int bottom_prepare(int dim, size_t lds_bytes);
+void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVars &V, const BottomArgs &B,
+                   size_t lds_bytes);
+void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVars &V,
+                            const BottomArgs &B);
 
 }  // namespace afg

@@ -200,6 +200,7 @@ def test_timing_mode_is_exact():
     mg = M.default_mg(n_base=p.n_base)
     om = O.default_opts(2, 2, p.n_base)
     dt = M.DeviceTree.from_state(p.state)
+    dt.set_bottom_cells(0)      # every level through the timed per-level passes
     dt.set_timing(True)
     dt, ot, rd, ro = solve_both(p.state, 3, mg, om, 1, 3, dt=dt)
     n, ms, b = dt.timing()
