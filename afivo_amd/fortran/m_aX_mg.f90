@@ -1,0 +1,623 @@
+! Drop-in replacement of afivo's multigrid module m_a$D_mg, running the FAS
+! multigrid cycles on an MI355X through libafivo_amd (include/afivo_amd.h).
+!
+! The public names, the mg$D_t options type and the procedure interfaces are
+! those of the reference module (src/m_aX_mg.f90:6-125), so user programs
+! compile unchanged; this module is compiled against the user's afivo tree
+! modules (m_a$D_t, m_a$D_gc) and linked in place of the reference m_a$D_mg.
+!
+! What runs where:
+!  * mg$D_fas_fmg / mg$D_fas_vcycle upload the tree structure (when it
+!    changed), the cell variables phi, rhs, tmp (and eps) and the boundary
+!    values captured from sides_bc, run the whole cycle on the GPU and copy
+!    phi, rhs and tmp back into box%cc (unless mg%device_resident is set).
+!  * The operator procedures (mg$D_box_lpl, mg$D_auto_op, ...) identify the
+!    operator family by procedure identity; they are not meant to be called
+!    directly and stop if they are.  Operator families the device does not
+!    implement (level set, cylindrical) and user-written operators stop in
+!    mg$D_fas_fmg / mg$D_fas_vcycle: there is no CPU fallback.
+!  * A user sides_bc is called on the host once per physical boundary face
+!    and solve, on a copy of the box, to capture the boundary values it
+!    writes (they may depend on the geometry, not on phi).
+!
+! Dimension template: "$D" is replaced by 2 or 3, then cpp runs (the same
+! transform afivo applies to its own m_aX_*.f90 sources).
+module m_a$D_mg
+  use iso_c_binding
+  use m_a$D_t
+  use m_afg_capi
+
+  implicit none
+  private
+
+  integer, parameter, public :: mg_normal_box = 1 !< Normal box
+  integer, parameter, public :: mg_lsf_box = 2    !< Box with an internal boundary
+  integer, parameter, public :: mg_ceps_box = 3   !< Box with constant eps /= 1
+  integer, parameter, public :: mg_veps_box = 4   !< Box with varying eps (on face)
+
+  !> Multigrid options (components of the reference's mg$D_t, plus
+  !> device_resident)
+  type, public :: mg$D_t
+     integer :: i_phi        = -1
+     integer :: i_rhs        = -1
+     integer :: i_tmp        = -1
+     integer :: i_eps        = -1
+     integer :: i_lsf        = -1
+     integer :: i_bval       = -1
+     integer :: n_cycle_down = -1
+     integer :: n_cycle_up   = -1
+     integer :: n_cycle_base = -1
+     logical :: initialized  = .false.
+     !> Keep cell data on the device between calls; mg$D_to_host copies it back
+     logical :: device_resident = .false.
+     procedure(a$D_subr_bc), pointer, nopass   :: sides_bc => null()
+     procedure(a$D_subr_rb), pointer, nopass   :: sides_rb => null()
+     procedure(mg$D_box_op), pointer, nopass   :: box_op => null()
+     procedure(mg$D_box_gsrb), pointer, nopass :: box_gsrb => null()
+     procedure(mg$D_box_corr), pointer, nopass :: box_corr => null()
+     procedure(mg$D_box_rstr), pointer, nopass :: box_rstr => null()
+  end type mg$D_t
+
+  abstract interface
+     subroutine mg$D_box_op(box, i_out, mg)
+       import
+       type(box$D_t), intent(inout) :: box
+       type(mg$D_t), intent(in)     :: mg
+       integer, intent(in)          :: i_out
+     end subroutine mg$D_box_op
+
+     subroutine mg$D_box_gsrb(box, redblack_cntr, mg)
+       import
+       type(box$D_t), intent(inout) :: box
+       type(mg$D_t), intent(in)     :: mg
+       integer, intent(in)          :: redblack_cntr
+     end subroutine mg$D_box_gsrb
+
+     subroutine mg$D_box_corr(box_p, box_c, mg)
+       import
+       type(box$D_t), intent(inout) :: box_c
+       type(box$D_t), intent(in)    :: box_p
+       type(mg$D_t), intent(in)     :: mg
+     end subroutine mg$D_box_corr
+
+     subroutine mg$D_box_rstr(box_c, box_p, iv, mg)
+       import
+       type(box$D_t), intent(in)    :: box_c
+       type(box$D_t), intent(inout) :: box_p
+       integer, intent(in)          :: iv
+       type(mg$D_t), intent(in)     :: mg
+     end subroutine mg$D_box_rstr
+  end interface
+
+  public :: mg$D_init_mg
+  public :: mg$D_fas_fmg
+  public :: mg$D_fas_vcycle
+  public :: mg$D_box_op
+  public :: mg$D_box_gsrb
+  public :: mg$D_box_corr
+  public :: mg$D_box_rstr
+  public :: mg$D_set_box_tag
+  public :: mg$D_auto_op
+  public :: mg$D_auto_gsrb
+  public :: mg$D_auto_corr
+  public :: mg$D_auto_rstr
+  public :: mg$D_box_lpl
+  public :: mg$D_box_gsrb_lpl
+  public :: mg$D_box_corr_lpl
+  public :: mg$D_box_rstr_lpl
+  public :: mg$D_sides_rb
+  public :: mg$D_box_lpld
+  public :: mg$D_box_gsrb_lpld
+  public :: mg$D_box_corr_lpld
+  public :: mg$D_box_lpllsf
+  public :: mg$D_box_gsrb_lpllsf
+  public :: mg$D_box_corr_lpllsf
+  public :: mg$D_box_rstr_lpllsf
+#if $D == 2
+  public :: mg$D_box_clpl
+  public :: mg$D_box_gsrb_clpl
+  public :: mg$D_box_clpld
+  public :: mg$D_box_gsrb_clpld
+#endif
+  ! device helpers (not in the reference)
+  public :: mg$D_to_host
+  public :: mg$D_device_release
+
+  ! device context of this module and the structure it holds
+  type(c_ptr), save :: ctx = c_null_ptr
+  integer, save :: ctx_nc = -1, ctx_nvar = -1
+  integer(c_int32_t), allocatable, save :: recs_dev(:)
+  integer(c_int32_t), allocatable, save :: ids_dev(:)
+  integer(c_int64_t), save :: n_values = 0
+
+  integer, parameter :: rec_len = 3 + $D + 2**$D + 2*$D
+
+contains
+
+  !> mg$D_init_mg (m_aX_mg.f90:130-154): check the options and set defaults
+  subroutine mg$D_init_mg(mg)
+    type(mg$D_t), intent(inout) :: mg
+
+    if (mg%i_phi < 0)                  stop "mg$D_init_mg: i_phi not set"
+    if (mg%i_tmp < 0)                  stop "mg$D_init_mg: i_tmp not set"
+    if (mg%i_rhs < 0)                  stop "mg$D_init_mg: i_rhs not set"
+    if (mg%i_lsf * mg%i_bval < 0) &
+         stop "mg$D_init_mg: you have to set both i_lsf and i_bval"
+    if (.not. associated(mg%sides_bc)) stop "mg$D_init_mg: sides_bc not set"
+
+    if (mg%n_cycle_down < 0)           mg%n_cycle_down = 2
+    if (mg%n_cycle_up < 0)             mg%n_cycle_up = 2
+    if (mg%n_cycle_base < 0)           mg%n_cycle_base = 2
+
+    if (.not. associated(mg%sides_rb)) mg%sides_rb => mg$D_sides_rb
+    if (.not. associated(mg%box_op))   mg%box_op => mg$D_box_lpl
+    if (.not. associated(mg%box_gsrb)) mg%box_gsrb => mg$D_box_gsrb_lpl
+    if (.not. associated(mg%box_corr)) mg%box_corr => mg$D_box_corr_lpl
+    if (.not. associated(mg%box_rstr)) mg%box_rstr => mg$D_box_rstr_lpl
+
+    mg%initialized = .true.
+  end subroutine mg$D_init_mg
+
+  subroutine check_mg(mg)
+    type(mg$D_t), intent(in) :: mg
+    if (.not. mg%initialized) stop "check_mg: you haven't called mg$D_init_mg"
+  end subroutine check_mg
+
+  !> mg$D_fas_fmg (m_aX_mg.f90:164-203) on the device
+  subroutine mg$D_fas_fmg(tree, mg, set_residual, have_guess)
+    type(a$D_t), intent(inout) :: tree
+    type(mg$D_t), intent(in)   :: mg
+    logical, intent(in)        :: set_residual
+    logical, intent(in)        :: have_guess
+    type(afg_mg_opts)          :: o
+
+    call check_mg(mg)
+    call device_prepare(tree, mg, o)
+    call afg_check(afg_fas_fmg(ctx, o, merge(1, 0, set_residual), &
+         merge(1, 0, have_guess)), ctx, "mg$D_fas_fmg")
+    if (.not. mg%device_resident) call mg$D_to_host(tree, mg)
+  end subroutine mg$D_fas_fmg
+
+  !> mg$D_fas_vcycle (m_aX_mg.f90:208-255) on the device
+  subroutine mg$D_fas_vcycle(tree, mg, highest_lvl, set_residual)
+    type(a$D_t), intent(inout) :: tree
+    type(mg$D_t), intent(in)   :: mg
+    integer, intent(in)        :: highest_lvl
+    logical, intent(in)        :: set_residual
+    type(afg_mg_opts)          :: o
+
+    call check_mg(mg)
+    call device_prepare(tree, mg, o)
+    call afg_check(afg_fas_vcycle(ctx, o, highest_lvl, merge(1, 0, set_residual)), &
+         ctx, "mg$D_fas_vcycle")
+    if (.not. mg%device_resident) call mg$D_to_host(tree, mg)
+  end subroutine mg$D_fas_vcycle
+
+  !> Copy phi, rhs and tmp from the device into box%cc
+  subroutine mg$D_to_host(tree, mg)
+    type(a$D_t), intent(inout) :: tree
+    type(mg$D_t), intent(in)   :: mg
+    if (.not. c_associated(ctx)) return
+    call download_var(tree, mg%i_phi)
+    call download_var(tree, mg%i_rhs)
+    call download_var(tree, mg%i_tmp)
+  end subroutine mg$D_to_host
+
+  !> Free the device context of this module
+  subroutine mg$D_device_release()
+    integer(c_int) :: rc
+    if (c_associated(ctx)) rc = afg_ctx_destroy(ctx)
+    ctx = c_null_ptr
+    ctx_nc = -1
+    ctx_nvar = -1
+    if (allocated(recs_dev)) deallocate(recs_dev)
+    if (allocated(ids_dev)) deallocate(ids_dev)
+  end subroutine mg$D_device_release
+
+  !> Map the options to the device, (re)build the device tree when its
+  !> structure changed, and upload the cell data and boundary values.
+  subroutine device_prepare(tree, mg, o)
+    type(a$D_t), intent(inout)     :: tree
+    type(mg$D_t), intent(in)       :: mg
+    type(afg_mg_opts), intent(out) :: o
+    integer(c_int32_t), allocatable :: recs(:)
+    logical :: new_tree
+
+    o%i_phi = mg%i_phi
+    o%i_rhs = mg%i_rhs
+    o%i_tmp = mg%i_tmp
+    o%i_eps = -1
+    o%n_cycle_down = mg%n_cycle_down
+    o%n_cycle_up = mg%n_cycle_up
+    o%n_cycle_base = mg%n_cycle_base
+    o%op = operator_family(mg)
+    if (o%op == AFG_OP_AUTO) o%i_eps = mg%i_eps
+    call afg_check(afg_mg_init(o), c_null_ptr, "mg$D_init_mg")
+
+    if (.not. associated(mg%sides_rb, mg$D_sides_rb)) &
+         error stop "mg$D: only mg$D_sides_rb is implemented on the device for sides_rb"
+
+    if (c_associated(ctx) .and. (ctx_nc /= tree%n_cell .or. ctx_nvar /= tree%n_var_cell)) &
+         call mg$D_device_release()
+    if (.not. c_associated(ctx)) then
+       call afg_check(afg_ctx_create($D, tree%n_cell, tree%n_var_cell, ctx), &
+            c_null_ptr, "afg_ctx_create")
+       ctx_nc = tree%n_cell
+       ctx_nvar = tree%n_var_cell
+    end if
+
+    call build_records(tree, recs)
+    new_tree = .true.
+    if (allocated(recs_dev)) then
+       if (size(recs_dev) == size(recs)) new_tree = any(recs_dev /= recs)
+    end if
+    if (new_tree) then
+       call afg_check(afg_set_tree(ctx, size(recs) / rec_len, recs, tree%dr_base), &
+            ctx, "afg_set_tree")
+       recs_dev = recs
+       ids_dev = recs(1::rec_len)
+    end if
+
+    if (new_tree .or. .not. mg%device_resident) then
+       call upload_var(tree, mg%i_phi)
+       call upload_var(tree, mg%i_rhs)
+       call upload_var(tree, mg%i_tmp)
+       if (o%op == AFG_OP_AUTO) call upload_var(tree, mg%i_eps)
+    end if
+    call set_boundaries(tree, mg)
+  end subroutine device_prepare
+
+  !> Device operator family from the procedure pointers of mg
+  integer(c_int) function operator_family(mg) result(op)
+    type(mg$D_t), intent(in) :: mg
+
+    if (associated(mg%box_op, mg$D_box_lpl) .and. &
+         associated(mg%box_gsrb, mg$D_box_gsrb_lpl) .and. &
+         associated(mg%box_corr, mg$D_box_corr_lpl) .and. &
+         associated(mg%box_rstr, mg$D_box_rstr_lpl)) then
+       op = AFG_OP_LPL
+    else if (associated(mg%box_op, mg$D_auto_op) .and. &
+         associated(mg%box_gsrb, mg$D_auto_gsrb) .and. &
+         associated(mg%box_corr, mg$D_auto_corr) .and. &
+         (associated(mg%box_rstr, mg$D_box_rstr_lpl) .or. &
+         associated(mg%box_rstr, mg$D_auto_rstr))) then
+       if (mg%i_eps < 0) error stop "mg$D: the auto operators need i_eps"
+       if (mg%i_lsf /= -1) &
+            error stop "mg$D: level-set boxes are not implemented on the device"
+       op = AFG_OP_AUTO
+    else
+       error stop "mg$D: operator set not implemented on the device " // &
+            "(supported: the lpl set, and auto_op/auto_gsrb/auto_corr with eps)"
+    end if
+  end function operator_family
+
+  !> One record [id, lvl, ix, parent, children, neighbors] per box in use,
+  !> in level order (the layout afg_set_tree takes)
+  subroutine build_records(tree, recs)
+    type(a$D_t), intent(in) :: tree
+    integer(c_int32_t), allocatable, intent(out) :: recs(:)
+    integer :: lvl, i, id, n, pos
+
+    n = 0
+    do lvl = lbound(tree%lvls, 1), tree%highest_lvl
+       n = n + size(tree%lvls(lvl)%ids)
+    end do
+    allocate(recs(n * rec_len))
+    pos = 0
+    do lvl = lbound(tree%lvls, 1), tree%highest_lvl
+       do i = 1, size(tree%lvls(lvl)%ids)
+          id = tree%lvls(lvl)%ids(i)
+          recs(pos+1) = id
+          recs(pos+2) = tree%boxes(id)%lvl
+          recs(pos+3:pos+2+$D) = tree%boxes(id)%ix
+          recs(pos+3+$D) = tree%boxes(id)%parent
+          recs(pos+4+$D:pos+3+$D+2**$D) = tree%boxes(id)%children
+          recs(pos+4+$D+2**$D:pos+rec_len) = tree%boxes(id)%neighbors
+          pos = pos + rec_len
+       end do
+    end do
+  end subroutine build_records
+
+  integer function box_len(box)
+    type(box$D_t), intent(in) :: box
+    box_len = (box%n_cell + 2)**$D
+  end function box_len
+
+  subroutine upload_var(tree, iv)
+    type(a$D_t), intent(in) :: tree
+    integer, intent(in)     :: iv
+    real(c_double), allocatable :: buf(:)
+    integer :: k, id, pos, len
+
+    allocate(buf(total_values(tree)))
+    pos = 0
+    do k = 1, size(ids_dev)
+       id = ids_dev(k)
+       len = box_len(tree%boxes(id))
+#if $D == 2
+       buf(pos+1:pos+len) = reshape(tree%boxes(id)%cc(:, :, iv), [len])
+#elif $D == 3
+       buf(pos+1:pos+len) = reshape(tree%boxes(id)%cc(:, :, :, iv), [len])
+#endif
+       pos = pos + len
+    end do
+    call afg_check(afg_upload_cc(ctx, iv, size(ids_dev), ids_dev, buf), ctx, "afg_upload_cc")
+  end subroutine upload_var
+
+  subroutine download_var(tree, iv)
+    type(a$D_t), intent(inout) :: tree
+    integer, intent(in)        :: iv
+    real(c_double), allocatable :: buf(:)
+    integer :: k, id, pos, len, n
+
+    allocate(buf(total_values(tree)))
+    call afg_check(afg_download_cc(ctx, iv, size(ids_dev), ids_dev, buf), ctx, "afg_download_cc")
+    pos = 0
+    do k = 1, size(ids_dev)
+       id = ids_dev(k)
+       n = tree%boxes(id)%n_cell
+       len = box_len(tree%boxes(id))
+#if $D == 2
+       tree%boxes(id)%cc(:, :, iv) = reshape(buf(pos+1:pos+len), [n+2, n+2])
+#elif $D == 3
+       tree%boxes(id)%cc(:, :, :, iv) = reshape(buf(pos+1:pos+len), [n+2, n+2, n+2])
+#endif
+       pos = pos + len
+    end do
+  end subroutine download_var
+
+  integer function total_values(tree)
+    type(a$D_t), intent(in) :: tree
+    integer :: k
+    total_values = 0
+    do k = 1, size(ids_dev)
+       total_values = total_values + box_len(tree%boxes(ids_dev(k)))
+    end do
+  end function total_values
+
+  !> Boundary table: for every physical face, the type and values the user's
+  !> sides_bc writes (built-in zero conditions need no call)
+  subroutine set_boundaries(tree, mg)
+    use m_a$D_gc, only: a$D_bc_dirichlet_zero, a$D_bc_neumann_zero
+    type(a$D_t), intent(in)  :: tree
+    type(mg$D_t), intent(in) :: mg
+    type(box$D_t)            :: scratch
+    integer                  :: k, id, nb, bc_type, n
+    real(c_double), allocatable, target :: b(:)
+
+    do k = 1, size(ids_dev)
+       id = ids_dev(k)
+       do nb = 1, a$D_num_neighbors
+          if (tree%boxes(id)%neighbors(nb) >= a5_no_box) cycle
+          if (associated(mg%sides_bc, a$D_bc_dirichlet_zero)) then
+             call afg_check(afg_set_bc(ctx, id, nb, AFG_BC_DIRICHLET, c_null_ptr), ctx, "afg_set_bc")
+          else if (associated(mg%sides_bc, a$D_bc_neumann_zero)) then
+             call afg_check(afg_set_bc(ctx, id, nb, AFG_BC_NEUMANN, c_null_ptr), ctx, "afg_set_bc")
+          else
+             scratch = tree%boxes(id)
+             call mg%sides_bc(scratch, nb, mg%i_phi, bc_type)
+             n = scratch%n_cell
+             call face_values(scratch, nb, mg%i_phi, b)
+             call afg_check(afg_set_bc(ctx, id, nb, bc_type, c_loc(b)), ctx, "afg_set_bc")
+          end if
+       end do
+    end do
+  end subroutine set_boundaries
+
+  !> Ghost values of face nb, first tangential index fastest
+  subroutine face_values(box, nb, iv, b)
+    type(box$D_t), intent(in) :: box
+    integer, intent(in) :: nb, iv
+    real(c_double), allocatable, intent(out) :: b(:)
+    integer :: nc, g
+    nc = box%n_cell
+    g = a$D_neighb_high_01(nb) * (nc + 1)
+#if $D == 2
+    select case (a$D_neighb_dim(nb))
+    case (1)
+       b = box%cc(g, 1:nc, iv)
+    case (2)
+       b = box%cc(1:nc, g, iv)
+    end select
+#elif $D == 3
+    select case (a$D_neighb_dim(nb))
+    case (1)
+       b = reshape(box%cc(g, 1:nc, 1:nc, iv), [nc*nc])
+    case (2)
+       b = reshape(box%cc(1:nc, g, 1:nc, iv), [nc*nc])
+    case (3)
+       b = reshape(box%cc(1:nc, 1:nc, g, iv), [nc*nc])
+    end select
+#endif
+  end subroutine face_values
+
+  !> mg$D_set_box_tag (m_aX_mg.f90:667-706): tag from min/max of eps over the
+  !> whole array (ghost cells included) and the sign change of the level set
+  subroutine mg$D_set_box_tag(box, mg)
+    type(box$D_t), intent(inout) :: box
+    type(mg$D_t), intent(in)     :: mg
+    real(dp) :: a, b
+    logical :: is_lsf, is_deps, is_eps
+
+    is_lsf = .false.
+    is_eps = .false.
+    is_deps = .false.
+#if $D == 2
+    if (mg%i_lsf /= -1) is_lsf = minval(box%cc(:, :, mg%i_lsf)) * &
+         maxval(box%cc(:, :, mg%i_lsf)) < 0
+    if (mg%i_eps /= -1) then
+       a = minval(box%cc(:, :, mg%i_eps))
+       b = maxval(box%cc(:, :, mg%i_eps))
+#elif $D == 3
+    if (mg%i_lsf /= -1) is_lsf = minval(box%cc(:, :, :, mg%i_lsf)) * &
+         maxval(box%cc(:, :, :, mg%i_lsf)) < 0
+    if (mg%i_eps /= -1) then
+       a = minval(box%cc(:, :, :, mg%i_eps))
+       b = maxval(box%cc(:, :, :, mg%i_eps))
+#endif
+       is_deps = (b > a)
+       if (.not. is_deps) is_eps = (a < 1 .or. a > 1)
+    end if
+    if (count([is_lsf, is_eps, is_deps]) > 1) &
+         stop "mg$D_set_box_tag: Cannot set lsf and eps tag for same box"
+    box%tag = mg_normal_box
+    if (is_lsf) box%tag = mg_lsf_box
+    if (is_eps) box%tag = mg_ceps_box
+    if (is_deps) box%tag = mg_veps_box
+  end subroutine mg$D_set_box_tag
+
+  ! ------------------------------------------------------------------
+  ! Operator procedures.  They identify the operator family for the device
+  ! path (by procedure identity) and do no work on the host.
+
+  subroutine host_call(name)
+    character(len=*), intent(in) :: name
+    write(*, "(A,A)") name, ": runs on the device only, through mg$D_fas_fmg / mg$D_fas_vcycle"
+    error stop "afivo_amd: box operator called on the host"
+  end subroutine host_call
+
+  subroutine mg$D_sides_rb(boxes, id, nb, iv)
+    type(box$D_t), intent(inout) :: boxes(:)
+    integer, intent(in) :: id, nb, iv
+    call host_call("mg$D_sides_rb")
+  end subroutine mg$D_sides_rb
+
+  subroutine mg$D_box_lpl(box, i_out, mg)
+    type(box$D_t), intent(inout) :: box
+    integer, intent(in)          :: i_out
+    type(mg$D_t), intent(in)     :: mg
+    call host_call("mg$D_box_lpl")
+  end subroutine mg$D_box_lpl
+
+  subroutine mg$D_box_gsrb_lpl(box, redblack_cntr, mg)
+    type(box$D_t), intent(inout) :: box
+    integer, intent(in)          :: redblack_cntr
+    type(mg$D_t), intent(in)     :: mg
+    call host_call("mg$D_box_gsrb_lpl")
+  end subroutine mg$D_box_gsrb_lpl
+
+  subroutine mg$D_box_corr_lpl(box_p, box_c, mg)
+    type(box$D_t), intent(inout) :: box_c
+    type(box$D_t), intent(in)    :: box_p
+    type(mg$D_t), intent(in)     :: mg
+    call host_call("mg$D_box_corr_lpl")
+  end subroutine mg$D_box_corr_lpl
+
+  subroutine mg$D_box_rstr_lpl(box_c, box_p, iv, mg)
+    type(box$D_t), intent(in)    :: box_c
+    type(box$D_t), intent(inout) :: box_p
+    integer, intent(in)          :: iv
+    type(mg$D_t), intent(in)     :: mg
+    call host_call("mg$D_box_rstr_lpl")
+  end subroutine mg$D_box_rstr_lpl
+
+  subroutine mg$D_auto_op(box, i_out, mg)
+    type(box$D_t), intent(inout) :: box
+    integer, intent(in)          :: i_out
+    type(mg$D_t), intent(in)     :: mg
+    call host_call("mg$D_auto_op")
+  end subroutine mg$D_auto_op
+
+  subroutine mg$D_auto_gsrb(box, redblack_cntr, mg)
+    type(box$D_t), intent(inout) :: box
+    integer, intent(in)          :: redblack_cntr
+    type(mg$D_t), intent(in)     :: mg
+    call host_call("mg$D_auto_gsrb")
+  end subroutine mg$D_auto_gsrb
+
+  subroutine mg$D_auto_corr(box_p, box_c, mg)
+    type(box$D_t), intent(inout) :: box_c
+    type(box$D_t), intent(in)    :: box_p
+    type(mg$D_t), intent(in)     :: mg
+    call host_call("mg$D_auto_corr")
+  end subroutine mg$D_auto_corr
+
+  subroutine mg$D_auto_rstr(box_c, box_p, iv, mg)
+    type(box$D_t), intent(in)    :: box_c
+    type(box$D_t), intent(inout) :: box_p
+    integer, intent(in)          :: iv
+    type(mg$D_t), intent(in)     :: mg
+    call host_call("mg$D_auto_rstr")
+  end subroutine mg$D_auto_rstr
+
+  subroutine mg$D_box_lpld(box, i_out, mg)
+    type(box$D_t), intent(inout) :: box
+    integer, intent(in)          :: i_out
+    type(mg$D_t), intent(in)     :: mg
+    call host_call("mg$D_box_lpld")
+  end subroutine mg$D_box_lpld
+
+  subroutine mg$D_box_gsrb_lpld(box, redblack_cntr, mg)
+    type(box$D_t), intent(inout) :: box
+    integer, intent(in)          :: redblack_cntr
+    type(mg$D_t), intent(in)     :: mg
+    call host_call("mg$D_box_gsrb_lpld")
+  end subroutine mg$D_box_gsrb_lpld
+
+  subroutine mg$D_box_corr_lpld(box_p, box_c, mg)
+    type(box$D_t), intent(inout) :: box_c
+    type(box$D_t), intent(in)    :: box_p
+    type(mg$D_t), intent(in)     :: mg
+    call host_call("mg$D_box_corr_lpld")
+  end subroutine mg$D_box_corr_lpld
+
+  subroutine mg$D_box_lpllsf(box, i_out, mg)
+    type(box$D_t), intent(inout) :: box
+    integer, intent(in)          :: i_out
+    type(mg$D_t), intent(in)     :: mg
+    call host_call("mg$D_box_lpllsf")
+  end subroutine mg$D_box_lpllsf
+
+  subroutine mg$D_box_gsrb_lpllsf(box, redblack_cntr, mg)
+    type(box$D_t), intent(inout) :: box
+    integer, intent(in)          :: redblack_cntr
+    type(mg$D_t), intent(in)     :: mg
+    call host_call("mg$D_box_gsrb_lpllsf")
+  end subroutine mg$D_box_gsrb_lpllsf
+
+  subroutine mg$D_box_corr_lpllsf(box_p, box_c, mg)
+    type(box$D_t), intent(inout) :: box_c
+    type(box$D_t), intent(in)    :: box_p
+    type(mg$D_t), intent(in)     :: mg
+    call host_call("mg$D_box_corr_lpllsf")
+  end subroutine mg$D_box_corr_lpllsf
+
+  subroutine mg$D_box_rstr_lpllsf(box_c, box_p, iv, mg)
+    type(box$D_t), intent(in)    :: box_c
+    type(box$D_t), intent(inout) :: box_p
+    integer, intent(in)          :: iv
+    type(mg$D_t), intent(in)     :: mg
+    call host_call("mg$D_box_rstr_lpllsf")
+  end subroutine mg$D_box_rstr_lpllsf
+
+#if $D == 2
+  subroutine mg$D_box_clpl(box, i_out, mg)
+    type(box$D_t), intent(inout) :: box
+    integer, intent(in)          :: i_out
+    type(mg$D_t), intent(in)     :: mg
+    call host_call("mg$D_box_clpl")
+  end subroutine mg$D_box_clpl
+
+  subroutine mg$D_box_gsrb_clpl(box, redblack_cntr, mg)
+    type(box$D_t), intent(inout) :: box
+    integer, intent(in)          :: redblack_cntr
+    type(mg$D_t), intent(in)     :: mg
+    call host_call("mg$D_box_gsrb_clpl")
+  end subroutine mg$D_box_gsrb_clpl
+
+  subroutine mg$D_box_clpld(box, i_out, mg)
+    type(box$D_t), intent(inout) :: box
+    integer, intent(in)          :: i_out
+    type(mg$D_t), intent(in)     :: mg
+    call host_call("mg$D_box_clpld")
+  end subroutine mg$D_box_clpld
+
+  subroutine mg$D_box_gsrb_clpld(box, redblack_cntr, mg)
+    type(box$D_t), intent(inout) :: box
+    integer, intent(in)          :: redblack_cntr
+    type(mg$D_t), intent(in)     :: mg
+    call host_call("mg$D_box_gsrb_clpld")
+  end subroutine mg$D_box_gsrb_clpld
+#endif
+
+end module m_a$D_mg

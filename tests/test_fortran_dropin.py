@@ -1,0 +1,96 @@
+"""The Fortran drop-in (afivo_amd/fortran/m_aX_mg.f90).
+
+The reference's own harness program (oracle/ref_harness_X.f90, which uses
+afivo's public multigrid API) is compiled unchanged against the drop-in
+m_a2_mg / m_a3_mg and linked with libafivo_amd (tests/fortran/Makefile).
+Run with the same configuration as the reference build, it must write the
+same tree, the same cell data bit for bit, and the same residual history.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from afivo_amd import _lib
+from afivo_amd.treefile import read_tree_file
+from helpers import assert_bitexact
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "fortran", "_build")
+G2 = dict(n_gauss=2, ampl=[1.0, 1.0], sigma=[0.04, 0.04], r0=[0.25, 0.25, 0.75, 0.75])
+G3 = dict(n_gauss=2, ampl=[1.0, -1.0], sigma=[0.1, 0.1], r0=[0.3, 0.3, 0.4, 0.7, 0.6, 0.5])
+
+
+def exe(dim):
+    return os.path.join(BUILD, f"gpu_harness_{dim}d")
+
+
+def have_gpu():
+    h = ctypes.c_void_p()
+    L = _lib.load()
+    if L.afg_ctx_create(2, 8, 1, ctypes.byref(h)) == 0:
+        L.afg_ctx_destroy(h)
+        return True
+    return False
+
+
+def run_dropin(dim, cfg, workdir):
+    """Run the drop-in harness; returns (returncode, output, prefix)."""
+    prefix = os.path.join(workdir, "gpu")
+    cfg = dict(cfg, out_file=prefix)
+    nml = os.path.join(workdir, "gpu.nml")
+    with open(nml, "w") as f:
+        f.write("&cfg\n" + ",\n".join(f" {k}={O._fmt(v)}" for k, v in cfg.items()) + "\n/\n")
+    p = subprocess.run([exe(dim), nml], capture_output=True, text=True, timeout=600,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+    return p.returncode, p.stdout + p.stderr, prefix
+
+
+needs_build = pytest.mark.skipif(not (os.path.exists(exe(2)) and O.ref_available(2)),
+                                 reason="drop-in harness or reference build absent")
+
+
+@needs_build
+def test_dropin_fails_loudly_without_gpu(tmp_path):
+    """No CPU fallback: without a device the drop-in stops with the
+    library's error message."""
+    if have_gpu():
+        pytest.skip("a GPU is present")
+    rc, out, _ = run_dropin(2, dict(nc=8, ref_mode="gauss", l_uniform=2, l_max=3, thr=1e-3,
+                                    n_base=2, n_vcycle=0, **G2), str(tmp_path))
+    assert rc != 0
+    assert "afivo_amd error" in out
+
+
+CASES = {
+    "2d_c1": (2, dict(nc=8, ref_mode="gauss", l_uniform=5, l_max=7, thr=1e-4, n_base=50,
+                      n_fmg=1, n_vcycle=4, **G2)),
+    "2d_guess": (2, dict(nc=4, ref_mode="gauss", l_uniform=3, l_max=6, thr=2e-3, n_base=6,
+                         n_fmg=3, n_vcycle=1, n_down=1, n_up=3, **G2)),
+    "3d": (3, dict(nc=8, ref_mode="gauss", l_uniform=2, l_max=4, thr=1e-3, n_base=4,
+                   n_fmg=1, n_vcycle=2, **G3)),
+    "3d_eps": (3, dict(nc=4, ref_mode="gauss", l_uniform=2, l_max=4, thr=1e-3, n_base=4,
+                       n_fmg=2, n_vcycle=1, use_eps=True, eps_hi=6.0, **G3)),
+}
+
+
+@pytest.mark.gpu
+@needs_build
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_dropin_matches_reference(tmp_path, case):
+    dim, cfg = CASES[case]
+    ref_dir = tmp_path / "ref"
+    gpu_dir = tmp_path / "gpu"
+    ref_dir.mkdir()
+    gpu_dir.mkdir()
+    prefix_ref, _ = O.run_ref(dim, dict(cfg, out_file="ref"), workdir=str(ref_dir))
+    rc, out, prefix_gpu = run_dropin(dim, cfg, str(gpu_dir))
+    assert rc == 0, out
+    a, b = read_tree_file(prefix_gpu + ".out"), read_tree_file(prefix_ref + ".out")
+    assert np.array_equal(a.recs, b.recs)
+    for iv in range(1, b.n_var + 1):
+        assert_bitexact(a.data[iv - 1], b.data[iv - 1], f"{case} var {iv}")
+    assert np.array_equal(O.read_res(prefix_gpu), O.read_res(prefix_ref))
