@@ -1,19 +1,10 @@
-// afg_kernels.hip -- gfx950 kernels of the afivo_amd FAS multigrid path.
-//
-// Every stage restates one afivo routine over all boxes of a level (the
-// reference runs these as OpenMP loops over boxes).  Floating-point
-// expressions keep the reference's evaluation order and the library is built
-// with -ffp-contract=off, so results are bit-identical to the reference.
-//
-// A stage is a per-thread body over one level.  The bodies address boxes
-// through LevelView, so the same code runs on the device pools in HBM (one
-// kernel launch per stage for the large levels) and on copies of the small
-// levels held in LDS (the bottom kernel, one launch for the whole lower part
-// of a V-cycle).  Box arrays hold cc(0:nc+1, 0:nc+1[, 0:nc+1]) with x fastest;
-// s = nc + 2 is the row pitch.  NC is the compile-time cell count per
-// dimension (0: taken from the level arguments).
+// afg_kernels.hip -- gfx950 kernels and launchers of the afivo_amd FAS
+// multigrid path.  The per-thread work of every stage is in afg_bodies.h; the
+// kernels here map threads to it (one launch per stage and level over the
+// HBM pools, or one workgroup for the whole bottom of a V-cycle on LDS).
 
 #include "afg_internal.h"
+#include "afg_bodies.h"
 
 namespace afg {
 
@@ -21,26 +12,10 @@ namespace {
 
 constexpr int BLOCK = 256;
 
-__host__ __device__ constexpr int ipow(int n, int d) { return d == 2 ? n * n : n * n * n; }
-
 inline int n_blocks(long long threads, int block = BLOCK)
 {
     return (int)((threads + block - 1) / block);
 }
-
-// Where the boxes of one level live: box `slot` of variable X starts at
-// X + (slot - slot0) * stride.  The HBM pools use slot0 = 0 and the pool
-// stride for every level; an LDS copy of a level uses its own slot0 and the
-// level's box size.
-struct LevelView {
-    double *phi, *rhs, *tmp;
-    const double *eps;
-    int slot0, stride;
-    __device__ __forceinline__ size_t off(int slot) const
-    {
-        return (size_t)(slot - slot0) * (size_t)stride;
-    }
-};
 
 LevelView pool_view(const DevTree &T, const MgVars &V)
 {
@@ -50,482 +25,6 @@ LevelView pool_view(const DevTree &T, const MgVars &V)
 LevelView var_view(const DevTree &T, double *var)
 {
     return LevelView{var, nullptr, nullptr, nullptr, 0, T.stride};
-}
-
-// ---------------------------------------------------------------------------
-// ghost value of face f next to interior cell (i, j, k) of box `slot`
-//
-// The one expression used by the ghost fills and, on the fly, by the stages
-// that read face neighbours.  Restates a$D_gc_box (m_aX_gc.f90:50-69):
-//   neighbour at the same level  -> sides_from_nb  (m_aX_gc.f90:381-415)
-//   refinement boundary          -> mg$D_sides_rb  (m_aX_mg.f90:262-350) after
-//                                   a$D_gc_prolong0 (m_aX_gc.f90:135-151,
-//                                   m_aX_prolong.f90:39-108)
-//   physical boundary            -> captured sides_bc value b, then bc_to_gc
-//                                   (m_aX_gc.f90:71-132): c2*b + c1*interior
-// `center` is phi at (i, j, k); `own` holds the level, `par` its parent level.
-template <int DIM>
-__device__ __forceinline__ double ghost_value(const DevTree &T, const LevelView &own,
-                                              const LevelView &par, int pnc, double dr, int slot,
-                                              int f, int nc, int i, int j, int k, double center)
-{
-    const int s = nc + 2;
-    const int nbv = T.nb[slot * (2 * DIM) + f];
-    const int dn = f >> 1, high = f & 1;
-    if (nbv >= 0) {
-        const int m = high ? 1 : nc;
-        const int a0 = dn == 0 ? m : i, a1 = dn == 1 ? m : j, a2 = dn == 2 ? m : k;
-        return own.phi[own.off(nbv) + a0 + s * (a1 + s * a2)];
-    }
-    if (nbv == NB_REFINE) {
-        const int p = T.parent[slot];
-        const int co = T.choff[slot];
-        const int gv = high ? nc + 1 : 0;
-        const int g0 = dn == 0 ? gv : i, g1 = dn == 1 ? gv : j, g2 = dn == 2 ? gv : k;
-        const int ps = pnc + 2;
-        const int pi = (co & 1023) + ((g0 + 1) >> 1);
-        const int pj = ((co >> 10) & 1023) + ((g1 + 1) >> 1);
-        const int pk = DIM == 3 ? ((co >> 20) & 1023) + ((g2 + 1) >> 1) : 0;
-        const double P = par.phi[par.off(p) + pi + ps * (pj + ps * pk)];
-        const double *box = own.phi + own.off(slot);
-        const int inward = high ? -1 : 1;
-        if (DIM == 2) {
-            const int di = dn == 0 ? inward : -1 + 2 * (i & 1);
-            const int dj = dn == 1 ? inward : -1 + 2 * (j & 1);
-            const double a = box[(i + di) + s * j];
-            const double b = box[i + s * (j + dj)];
-            return 0.5 * P + center - 0.25 * (a + b);
-        } else {
-            const int d0 = dn == 0 ? inward : -1 + 2 * (i & 1);
-            const int d1 = dn == 1 ? inward : -1 + 2 * (j & 1);
-            const int d2 = dn == 2 ? inward : -1 + 2 * (k & 1);
-            const double d = box[(i + d0) + s * ((j + d1) + s * (k + d2))];
-            return 0.5 * P + 0.75 * center - 0.25 * d;
-        }
-    }
-    const int rec = NB_BC0 - nbv;
-    const int voff = T.bc_voff[rec];
-    double b = 0.0;
-    if (voff >= 0) {
-        const int t1 = dn == 0 ? j : i;
-        const int t2 = dn == 2 ? j : k;
-        b = T.bc_vals[voff + (t1 - 1) + (DIM == 3 ? nc * (t2 - 1) : 0)];
-    }
-    double c1, c2;
-    if (T.bc_type[rec] == -1) {   // a5_bc_dirichlet
-        c1 = -1;
-        c2 = 2;
-    } else {                      // a5_bc_neumann
-        c1 = 1;
-        c2 = dr * (high ? 1.0 : -1.0);
-    }
-    return c2 * b + c1 * center;
-}
-
-// The 2*DIM face neighbours of interior cell q = (i, j, k), in the order
-// i-1, i+1, j-1, j+1, k-1, k+1.  A neighbour in the ghost layer is read from
-// memory when `fly` is 0 and computed with ghost_value when `fly` is 1.
-template <int DIM>
-__device__ __forceinline__ void stencil(const DevTree &T, const LevelArgs &L, const LevelView &own,
-                                        const LevelView &par, int slot, const double *box, int nc,
-                                        int i, int j, int k, int q, int fly, double center,
-                                        double u[6])
-{
-    const int s = nc + 2;
-    u[0] = (i > 1 || !fly) ? box[q - 1] : ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, 0, nc, i, j, k, center);
-    u[1] = (i < nc || !fly) ? box[q + 1] : ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, 1, nc, i, j, k, center);
-    u[2] = (j > 1 || !fly) ? box[q - s] : ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, 2, nc, i, j, k, center);
-    u[3] = (j < nc || !fly) ? box[q + s] : ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, 3, nc, i, j, k, center);
-    if (DIM == 3) {
-        u[4] = (k > 1 || !fly) ? box[q - s * s] : ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, 4, nc, i, j, k, center);
-        u[5] = (k < nc || !fly) ? box[q + s * s] : ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, 5, nc, i, j, k, center);
-    } else {
-        u[4] = u[5] = 0;
-    }
-}
-
-// A phi at interior cell q from its neighbours u: mg$D_box_lpl
-// (m_aX_mg.f90:763-797) or, for tagged boxes, mg$D_box_lpld (:866-916).
-template <int DIM>
-__device__ __forceinline__ double box_op(const double u[6], double u0, const double *e, int q,
-                                         int s, double inv_dr_sq, bool lpld)
-{
-    if (lpld) {
-        const int dq[6] = {-1, 1, -s, s, -s * s, s * s};
-        const double a0 = e[q];
-        double sum = 0;
-#pragma unroll
-        for (int m = 0; m < 2 * DIM; m++) {
-            const double a = e[q + dq[m]];
-            const double t = a0 * a / (a0 + a) * (u[m] - u0);
-            sum = m == 0 ? t : sum + t;
-        }
-        return inv_dr_sq * 2 * sum;
-    }
-    if (DIM == 2) return inv_dr_sq * (u[0] + u[1] + u[2] + u[3] - 4 * u0);
-    return inv_dr_sq * (u[0] + u[1] + u[2] + u[3] + u[4] + u[5] - 6 * u0);
-}
-
-__device__ __forceinline__ bool lpld_box(const DevTree &T, int op_auto, int slot)
-{
-    return op_auto && T.tag[slot] != TAG_NORMAL;
-}
-
-// ---------------------------------------------------------------------------
-// ghost fill of a level: fill_gc_phi (m_aX_mg.f90:352-365)
-// one thread per face ghost cell: n * 2*DIM * nc^(DIM-1)
-
-template <int DIM>
-__host__ __device__ inline int fill_threads(int n, int nc)
-{
-    return n * 2 * DIM * (DIM == 2 ? nc : nc * nc);
-}
-
-template <int DIM, int NC>
-__device__ __forceinline__ void fill_body(const DevTree &T, const LevelArgs &L, const LevelView &own,
-                                          const LevelView &par, int gid)
-{
-    const int nc = NC ? NC : L.nc;
-    const int per_face = DIM == 2 ? nc : nc * nc;
-    const int per_box = 2 * DIM * per_face;
-    const int b = gid / per_box;
-    const int r = gid - b * per_box;
-    const int f = r / per_face;
-    const int t = r - f * per_face;
-    const int t1 = t % nc + 1, t2 = DIM == 3 ? t / nc + 1 : 0;
-    const int dn = f >> 1, high = f & 1;
-    const int in = high ? nc : 1;
-    const int i = dn == 0 ? in : t1;
-    const int j = dn == 1 ? in : (dn == 0 ? t1 : (DIM == 3 ? t2 : 0));
-    const int k = DIM == 3 ? (dn == 2 ? in : t2) : 0;
-    const int slot = L.slot0 + b;
-    const int s = nc + 2;
-    double *box = own.phi + own.off(slot);
-    const double center = box[i + s * (j + s * k)];
-    const double g = ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, f, nc, i, j, k, center);
-    const int gi = dn == 0 ? (high ? nc + 1 : 0) : i;
-    const int gj = dn == 1 ? (high ? nc + 1 : 0) : j;
-    const int gk = dn == 2 ? (high ? nc + 1 : 0) : k;
-    box[gi + s * (gj + s * gk)] = g;
-}
-
-// correct_children, parent part (m_aX_mg.f90:380-381): tmp = phi - tmp over
-// the whole array of each parent.
-template <int DIM, int NC>
-__device__ __forceinline__ void diff_body(const DevTree &T, const ListArgs &P, const LevelView &v,
-                                          int gid)
-{
-    const int nc = NC ? NC : P.nc;
-    const int per_box = ipow(nc + 2, DIM);
-    const int b = gid / per_box;
-    const int q = gid - b * per_box;
-    const size_t o = v.off(P.slots[b]) + q;
-    v.tmp[o] = v.phi[o] - v.tmp[o];
-}
-
-// ---------------------------------------------------------------------------
-// one colour pass of GSRB over a level: the body of gsrb_boxes
-// (m_aX_mg.f90:404-411) with mg$D_box_gsrb_lpl (m_aX_mg.f90:718-760) or, for
-// tagged boxes under the auto operators, mg$D_box_gsrb_lpld (:810-863).
-//
-// With fly = 0 the face neighbours come from the ghost layer in memory, as in
-// the reference.  With fly = 1 they are computed with ghost_value from the
-// state at the start of the pass, which equals what the reference's ghost
-// fill after the previous pass stores: a face ghost feeds only the interior
-// cell next to it, and everything it depends on (that cell, the neighbour's
-// mirrored cell, the parent level) is untouched by the pass, nc being even.
-// One thread per cell of colour rb.
-
-template <int DIM>
-__host__ __device__ inline int gsrb_threads(int n, int nc)
-{
-    return n * ipow(nc, DIM) / 2;
-}
-
-template <int DIM, int NC>
-__device__ __forceinline__ void gsrb_body(const DevTree &T, const LevelArgs &L, const LevelView &own,
-                                          const LevelView &par, int op_auto, int rb, int fly,
-                                          int gid)
-{
-    const int nc = NC ? NC : L.nc;
-    const int hn = nc >> 1;
-    const int half = DIM == 2 ? nc * hn : nc * nc * hn;
-    const int b = gid / half;
-    const int c = gid - b * half;
-    const int r = c / hn;
-    const int p = c - r * hn;
-    const int j = DIM == 2 ? r + 1 : r % nc + 1;
-    const int k = DIM == 2 ? 0 : r / nc + 1;
-    const int i = 2 - ((rb ^ (DIM == 2 ? j : k + j)) & 1) + 2 * p;
-    const int slot = L.slot0 + b;
-    const int s = nc + 2;
-    const size_t base = own.off(slot);
-    double *box = own.phi + base;
-    const int q = i + s * (j + s * k);
-    const double center = box[q];
-    double u[6];
-    stencil<DIM>(T, L, own, par, slot, box, nc, i, j, k, q, fly, center, u);
-    const double rhs = own.rhs[base + q];
-    if (lpld_box(T, op_auto, slot)) {
-        // mg$D_box_gsrb_lpld: c = 2 a0 a / (a0 + a); phi = (sum(c u) - dx2 rhs) / sum(c)
-        const double *e = own.eps + base;
-        const double a0 = e[q];
-        const int dq[6] = {-1, 1, -s, s, -s * s, s * s};
-        double su = 0, sc = 0;
-#pragma unroll
-        for (int m = 0; m < 2 * DIM; m++) {
-            const double a = e[q + dq[m]];
-            const double cm = 2 * a0 * a / (a0 + a);
-            su = m == 0 ? cm * u[m] : su + cm * u[m];
-            sc = m == 0 ? cm : sc + cm;
-        }
-        box[q] = (su - L.dx2 * rhs) / sc;
-    } else if (DIM == 2) {
-        // order of m_aX_mg.f90:740-743: i+1, i-1, j+1, j-1
-        box[q] = 0.25 * (u[1] + u[0] + u[3] + u[2] - L.dx2 * rhs);
-    } else {
-        constexpr double sixth = 1 / 6.0;
-        box[q] = sixth * (u[1] + u[0] + u[3] + u[2] + u[5] + u[4] - L.dx2 * rhs);
-    }
-}
-
-// residual_box (m_aX_mg.f90:545-559) over a level: tmp = rhs - A phi
-template <int DIM, int NC>
-__device__ __forceinline__ void residual_body(const DevTree &T, const LevelArgs &L,
-                                              const LevelView &own, const LevelView &par,
-                                              int op_auto, int gid)
-{
-    const int nc = NC ? NC : L.nc;
-    const int per_box = ipow(nc, DIM);
-    const int b = gid / per_box;
-    const int c = gid - b * per_box;
-    const int i = c % nc + 1;
-    const int j = (c / nc) % nc + 1;
-    const int k = DIM == 3 ? c / (nc * nc) + 1 : 0;
-    const int slot = L.slot0 + b;
-    const int s = nc + 2;
-    const size_t base = own.off(slot);
-    const double *box = own.phi + base;
-    const int q = i + s * (j + s * k);
-    const bool lpld = lpld_box(T, op_auto, slot);
-    double u[6];
-    stencil<DIM>(T, L, own, par, slot, box, nc, i, j, k, q, 0, box[q], u);
-    const double a = box_op<DIM>(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld);
-    own.tmp[base + q] = own.rhs[base + q] - a;
-}
-
-// update_coarse, child part (m_aX_mg.f90:444-465), fused: per coarse cell of a
-// child's octant, the residual of its 2^DIM fine cells restricted into the
-// parent's tmp and phi restricted into the parent's phi (a$D_restrict_box,
-// m_aX_restrict.f90:67-132).  The reference stores the residual in the
-// child's tmp and restores tmp afterwards; with keep_child_tmp = 0
-// (set_coarse_phi_rhs, m_aX_mg.f90:495-504) the residual stays in the
-// child's tmp.  With fly = 1 the residual's face neighbours are computed as
-// the ghost fill after the preceding relaxation would store them.
-template <int DIM, int NC>
-__device__ __forceinline__ void uc_child_body(const DevTree &T, const LevelArgs &L,
-                                              const LevelView &own, const LevelView &par,
-                                              int op_auto, int keep_child_tmp, int fly, int gid)
-{
-    const int nc = NC ? NC : L.nc;
-    const int hn = nc >> 1;
-    const int per_box = ipow(hn, DIM);
-    const int b = gid / per_box;
-    const int c = gid - b * per_box;
-    const int ii = c % hn + 1;
-    const int jj = (c / hn) % hn + 1;
-    const int kk = DIM == 3 ? c / (hn * hn) + 1 : 0;
-    const int slot = L.slot0 + b;
-    const int s = nc + 2;
-    const size_t base = own.off(slot);
-    const bool lpld = lpld_box(T, op_auto, slot);
-    const double *ph = own.phi + base;
-    const double *e = lpld ? own.eps + base : nullptr;
-    const double *rh = own.rhs + base;
-    double sr = 0, sp = 0;
-#pragma unroll
-    for (int m = 0; m < (1 << DIM); m++) {
-        const int fi = 2 * ii - 1 + (m & 1);
-        const int fj = 2 * jj - 1 + ((m >> 1) & 1);
-        const int fk = DIM == 3 ? 2 * kk - 1 + ((m >> 2) & 1) : 0;
-        const int q = fi + s * (fj + s * fk);
-        double u[6];
-        stencil<DIM>(T, L, own, par, slot, ph, nc, fi, fj, fk, q, fly, ph[q], u);
-        const double res = rh[q] - box_op<DIM>(u, ph[q], e, q, s, L.inv_dr_sq, lpld);
-        if (!keep_child_tmp) own.tmp[base + q] = res;
-        sr = m == 0 ? res : sr + res;
-        sp = m == 0 ? ph[q] : sp + ph[q];
-    }
-    const int p = T.parent[slot];
-    const int co = T.choff[slot];
-    const int ps = L.pnc + 2;
-    const int pq = ((co & 1023) + ii) +
-                   ps * ((((co >> 10) & 1023) + jj) + ps * (DIM == 3 ? ((co >> 20) & 1023) + kk : 0));
-    const size_t pb = par.off(p);
-    constexpr double w = DIM == 2 ? 0.25 : 0.125;
-    par.tmp[pb + pq] = w * sr;
-    par.phi[pb + pq] = w * sp;
-}
-
-// update_coarse, parent part (m_aX_mg.f90:467-479), fused with the ghost fill
-// of the parent level that precedes it: one thread per cell of the whole
-// (nc+2)^DIM array of every box of the level.  Face ghost cells get the
-// ghost value (fill_gc_phi); on parents, rhs = A phi + tmp on the interior,
-// rhs = rhs + tmp elsewhere (a$D_box_add_cc over the whole array), then
-// tmp = phi (a$D_box_copy_cc) if copy_tmp.  A phi uses the filled ghost
-// values, computed on the fly.  L is the parent level, own its view, par the
-// view of the level below it.
-template <int DIM, int NC>
-__device__ __forceinline__ void uc_parent_body(const DevTree &T, const LevelArgs &L,
-                                               const LevelView &own, const LevelView &par,
-                                               int op_auto, int copy_tmp, int gid)
-{
-    const int nc = NC ? NC : L.nc;
-    const int s = nc + 2;
-    const int per_box = ipow(s, DIM);
-    const int b = gid / per_box;
-    const int q = gid - b * per_box;
-    const int i = q % s;
-    const int j = (q / s) % s;
-    const int k = DIM == 3 ? q / (s * s) : 1;
-    const int slot = L.slot0 + b;
-    const size_t base = own.off(slot);
-    double *box = own.phi + base;
-    const int oi = (i < 1 || i > nc), oj = (j < 1 || j > nc), ok = DIM == 3 ? (k < 1 || k > nc) : 0;
-    const int n_out = oi + oj + ok;
-    const bool is_par = T.is_parent[slot] != 0;
-    if (n_out == 0) {
-        if (!is_par) return;
-        const bool lpld = lpld_box(T, op_auto, slot);
-        double u[6];
-        stencil<DIM>(T, L, own, par, slot, box, nc, i, j, k, q, 1, box[q], u);
-        const double a = box_op<DIM>(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld);
-        own.rhs[base + q] = a + own.tmp[base + q];
-        if (copy_tmp) own.tmp[base + q] = box[q];
-        return;
-    }
-    double v = box[q];
-    if (n_out == 1) {   // face ghost: fill it
-        const int f = oi ? (i > nc) : oj ? 2 + (j > nc) : 4 + (k > nc);
-        const int ci = oi ? (i > nc ? nc : 1) : i;
-        const int cj = oj ? (j > nc ? nc : 1) : j;
-        const int ck = DIM == 3 ? (ok ? (k > nc ? nc : 1) : k) : 0;
-        const double center = box[ci + s * (cj + s * ck)];
-        v = ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, f, nc, ci, cj, ck, center);
-        box[q] = v;
-    }
-    if (!is_par) return;
-    own.rhs[base + q] = own.rhs[base + q] + own.tmp[base + q];
-    if (copy_tmp) own.tmp[base + q] = v;
-}
-
-// Quadratic prolongation stencil of a$D_prolong2 (m_aX_prolong.f90:281-343)
-// on coarse values c(q) supplied by `get`; the value of child octant m is
-// (((f0 -/+ fx) -/+ fy) -/+ fz) + f2.
-template <int DIM>
-struct Prolong2 {
-    double f0, fx, fy, fz, f2;
-    template <typename G>
-    __device__ __forceinline__ Prolong2(G get, int pq, int ps)
-    {
-        f0 = get(pq);
-        const double xm = get(pq - 1), xp = get(pq + 1);
-        const double ym = get(pq - ps), yp = get(pq + ps);
-        fx = 0.125 * (xp - xm);
-        fy = 0.125 * (yp - ym);
-        const double fxx = 0.03125 * (xm - 2 * f0 + xp);
-        const double fyy = 0.03125 * (ym - 2 * f0 + yp);
-        if (DIM == 3) {
-            const double zm = get(pq - ps * ps), zp = get(pq + ps * ps);
-            fz = 0.125 * (zp - zm);
-            const double fzz = 0.03125 * (zm - 2 * f0 + zp);
-            f2 = fxx + fyy + fzz;
-        } else {
-            fz = 0;
-            f2 = fxx + fyy;
-        }
-    }
-    __device__ __forceinline__ double at(int m) const
-    {
-        double v = (m & 1) ? f0 + fx : f0 - fx;
-        v = ((m >> 1) & 1) ? v + fy : v - fy;
-        if (DIM == 3) v = ((m >> 2) & 1) ? v + fz : v - fz;
-        return v + f2;
-    }
-};
-
-// correct_children, child part: mg%box_corr for each child.  Untagged boxes:
-// mg$D_box_corr_lpl = a$D_prolong2(add) of the parent's correction into phi
-// (m_aX_mg.f90:708-715); tagged boxes under the auto operators:
-// mg$D_box_corr_lpld (m_aX_mg.f90:918-985).  The correction is the parent's
-// tmp after tmp = phi - tmp; with fly_diff = 1 that difference is formed
-// here from the parent's phi and tmp (the parent's tmp is then updated by a
-// later stage).  One thread per coarse cell of the child's octant.
-template <int DIM, int NC>
-__device__ __forceinline__ void correct_body(const DevTree &T, const LevelArgs &L,
-                                             const LevelView &own, const LevelView &par,
-                                             int op_auto, int fly_diff, int gid)
-{
-    const int nc = NC ? NC : L.nc;
-    const int hn = nc >> 1;
-    const int per_box = ipow(hn, DIM);
-    const int b = gid / per_box;
-    const int c = gid - b * per_box;
-    const int ii = c % hn + 1;
-    const int jj = (c / hn) % hn + 1;
-    const int kk = DIM == 3 ? c / (hn * hn) + 1 : 0;
-    const int slot = L.slot0 + b;
-    const int s = nc + 2;
-    const int p = T.parent[slot];
-    const int co = T.choff[slot];
-    const int ps = L.pnc + 2;
-    const int ic = (co & 1023) + ii, jc = ((co >> 10) & 1023) + jj;
-    const int kc = DIM == 3 ? ((co >> 20) & 1023) + kk : 0;
-    const int pq = ic + ps * (jc + ps * kc);
-    const size_t pb = par.off(p);
-    const double *pt = par.tmp + pb;
-    const double *pp = par.phi + pb;
-    auto corr = [&](int q) { return fly_diff ? pp[q] - pt[q] : pt[q]; };
-    double *cph = own.phi + own.off(slot);
-    if (lpld_box(T, op_auto, slot)) {
-        const double *pe = par.eps + pb;
-        constexpr double third = 1 / 3.0;
-        const double u0 = corr(pq), a0 = pe[pq];
-#pragma unroll
-        for (int m = 0; m < (1 << DIM); m++) {
-            const int sx = m & 1, sy = (m >> 1) & 1, sz = (m >> 2) & 1;
-            const int fi = 2 * ii - 1 + sx, fj = 2 * jj - 1 + sy;
-            const int fk = DIM == 3 ? 2 * kk - 1 + sz : 0;
-            // one-but-closest coarse cell: fine index odd -> -1, even -> +1
-            // (m_aX_mg.f90:944-958)
-            const int qn[3] = {pq + (sx ? 1 : -1), pq + (sy ? ps : -ps),
-                               pq + (sz ? ps * ps : -ps * ps)};
-            double *t = &cph[fi + s * (fj + s * fk)];
-            if (DIM == 2) {
-                double sum = (a0 * u0 + pe[qn[0]] * corr(qn[0])) / (a0 + pe[qn[0]]);
-                sum += (a0 * u0 + pe[qn[1]] * corr(qn[1])) / (a0 + pe[qn[1]]);
-                *t = *t + 0.5 * sum;
-            } else {
-                double sum = 0;
-#pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    const double v = (a0 * u0 + pe[qn[d]] * (1.5 * corr(qn[d]) - 0.5 * u0)) /
-                                     (a0 + pe[qn[d]]);
-                    sum = d == 0 ? v : sum + v;
-                }
-                *t = *t + third * sum;
-            }
-        }
-    } else {
-        const Prolong2<DIM> P2(corr, pq, ps);
-#pragma unroll
-        for (int m = 0; m < (1 << DIM); m++) {
-            const int fi = 2 * ii - 1 + (m & 1), fj = 2 * jj - 1 + ((m >> 1) & 1);
-            const int fk = DIM == 3 ? 2 * kk - 1 + ((m >> 2) & 1) : 0;
-            double *t = &cph[fi + s * (fj + s * fk)];
-            *t = P2.at(m) + *t;
-        }
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -604,77 +103,22 @@ __global__ __launch_bounds__(BLOCK) void k_correct(DevTree T, LevelArgs L, Level
         correct_body<DIM, NC>(T, L, v, v, op_auto, fly_diff, gid);
 }
 
-// a$D_restrict_box (m_aX_restrict.f90:67-132) of every box of a level into
-// its parent: one thread per coarse cell.
+// a$D_restrict_box and a$D_prolong2 over the boxes of a level
 template <int DIM, int NC>
 __global__ __launch_bounds__(BLOCK) void k_restrict(DevTree T, LevelArgs L, const double *from,
                                                     double *to)
 {
-    const int nc = NC ? NC : L.nc;
-    const int hn = nc >> 1;
-    const int per_box = ipow(hn, DIM);
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= L.n * per_box) return;
-    const int b = gid / per_box;
-    const int c = gid - b * per_box;
-    const int ii = c % hn + 1;
-    const int jj = (c / hn) % hn + 1;
-    const int kk = DIM == 3 ? c / (hn * hn) + 1 : 0;
-    const int slot = L.slot0 + b;
-    const int s = nc + 2;
-    const double *cb = from + (size_t)slot * T.stride;
-    double sum = 0;
-#pragma unroll
-    for (int m = 0; m < (1 << DIM); m++) {
-        const int fi = 2 * ii - 1 + (m & 1), fj = 2 * jj - 1 + ((m >> 1) & 1);
-        const int fk = DIM == 3 ? 2 * kk - 1 + ((m >> 2) & 1) : 0;
-        const double v = cb[fi + s * (fj + s * fk)];
-        sum = m == 0 ? v : sum + v;
-    }
-    const int p = T.parent[slot];
-    const int co = T.choff[slot];
-    const int ps = L.pnc + 2;
-    const int pq = ((co & 1023) + ii) +
-                   ps * ((((co >> 10) & 1023) + jj) + ps * (DIM == 3 ? ((co >> 20) & 1023) + kk : 0));
-    constexpr double w = DIM == 2 ? 0.25 : 0.125;
-    to[(size_t)p * T.stride + pq] = w * sum;
+    if (gid < L.n * ipow((NC ? NC : L.nc) / 2, DIM)) restrict_body<DIM, NC>(T, L, from, to, gid);
 }
 
-// a$D_prolong2 (m_aX_prolong.f90:243-348) from the parent into every box of
-// a level; add = 0 overwrites the interior (the reference zeroes it first and
-// adds, so the stored value is expr + 0).
 template <int DIM, int NC>
 __global__ __launch_bounds__(BLOCK) void k_prolong2(DevTree T, LevelArgs L, const double *from,
                                                     double *to, int add)
 {
-    const int nc = NC ? NC : L.nc;
-    const int hn = nc >> 1;
-    const int per_box = ipow(hn, DIM);
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= L.n * per_box) return;
-    const int b = gid / per_box;
-    const int c = gid - b * per_box;
-    const int ii = c % hn + 1;
-    const int jj = (c / hn) % hn + 1;
-    const int kk = DIM == 3 ? c / (hn * hn) + 1 : 0;
-    const int slot = L.slot0 + b;
-    const int s = nc + 2;
-    const int p = T.parent[slot];
-    const int co = T.choff[slot];
-    const int ps = L.pnc + 2;
-    const int pq = ((co & 1023) + ii) +
-                   ps * ((((co >> 10) & 1023) + jj) + ps * (DIM == 3 ? ((co >> 20) & 1023) + kk : 0));
-    const double *pb = from + (size_t)p * T.stride;
-    const Prolong2<DIM> P2([&](int q) { return pb[q]; }, pq, ps);
-    double *cb = to + (size_t)slot * T.stride;
-#pragma unroll
-    for (int m = 0; m < (1 << DIM); m++) {
-        const int fi = 2 * ii - 1 + (m & 1), fj = 2 * jj - 1 + ((m >> 1) & 1);
-        const int fk = DIM == 3 ? 2 * kk - 1 + ((m >> 2) & 1) : 0;
-        double *t = &cb[fi + s * (fj + s * fk)];
-        const double old = add ? *t : 0.0;
-        *t = P2.at(m) + old;
-    }
+    if (gid < L.n * ipow((NC ? NC : L.nc) / 2, DIM))
+        prolong2_body<DIM, NC>(T, L, from, to, add, gid);
 }
 
 // whole-array fill and copy over a contiguous slot range
@@ -783,165 +227,19 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_partials(const double *partial
 
 // ---------------------------------------------------------------------------
 // bottom of the V-cycle in one workgroup, on LDS copies of the small levels
-//
-// Loads phi, rhs, tmp (and eps) of levels 0..hl (relative to the lowest)
-// into LDS, runs the stage sequence of mg$D_fas_vcycle below and at hl
-// (gsrb_boxes, update_coarse, the base relaxation, correct_children,
-// fill_gc_phi; m_aX_mg.f90:218-241) with the same thread bodies separated by
-// workgroup barriers, and writes phi, rhs and tmp back.
-
-template <int N>
-struct IC {
-    static constexpr int value = N;
-};
-
-template <typename F>
-__device__ __forceinline__ void nc_dispatch(int nc, F &&f)
-{
-    switch (nc) {
-    case 2: f(IC<2>()); break;
-    case 4: f(IC<4>()); break;
-    case 8: f(IC<8>()); break;
-    case 16: f(IC<16>()); break;
-    default: f(IC<0>()); break;
-    }
-}
+// (bottom_run), and the residual of those levels (bottom_residual_run)
 
 template <int DIM>
 __global__ __launch_bounds__(BOTTOM_BLOCK) void k_bottom(DevTree T, MgVars V, BottomArgs A)
 {
     extern __shared__ double lds[];
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const BottomLevels &B = *A.levels;
-    auto copy_in = [&](const double *src, int off, int li) {
-        const int len = B.box_len[li], n = B.lv[li].n, slot0 = B.lv[li].slot0;
-        for (int e = tid; e < n * len; e += nt) {
-            const int b = e / len;
-            lds[off + e] = src[(size_t)(slot0 + b) * T.stride + (e - b * len)];
-        }
-    };
-    auto copy_out = [&](double *dst, int off, int li) {
-        const int len = B.box_len[li], n = B.lv[li].n, slot0 = B.lv[li].slot0;
-        for (int e = tid; e < n * len; e += nt) {
-            const int b = e / len;
-            dst[(size_t)(slot0 + b) * T.stride + (e - b * len)] = lds[off + e];
-        }
-    };
-    for (int li = 0; li <= A.hl; li++) {
-        copy_in(V.phi, B.lds_off[0][li], li);
-        copy_in(V.rhs, B.lds_off[1][li], li);
-        copy_in(V.tmp, B.lds_off[2][li], li);
-        if (V.op_auto) copy_in(V.eps, B.lds_off[3][li], li);
-    }
-    __syncthreads();
-
-    auto view = [&](int li) {
-        LevelView w;
-        w.phi = lds + B.lds_off[0][li];
-        w.rhs = lds + B.lds_off[1][li];
-        w.tmp = lds + B.lds_off[2][li];
-        w.eps = V.op_auto ? lds + B.lds_off[3][li] : nullptr;
-        w.slot0 = B.lv[li].slot0;
-        w.stride = B.box_len[li];
-        return w;
-    };
-    auto fill = [&](int li) {
-        const LevelArgs &L = B.lv[li];
-        const LevelView own = view(li), par = view(li > 0 ? li - 1 : 0);
-        nc_dispatch(L.nc, [&](auto ncv) {
-            constexpr int NC = decltype(ncv)::value;
-            const int tot = fill_threads<DIM>(L.n, L.nc);
-            for (int g = tid; g < tot; g += nt) fill_body<DIM, NC>(T, L, own, par, g);
-        });
-        __syncthreads();
-    };
-    auto gsrb = [&](int li, int n_cycle) {
-        const LevelArgs &L = B.lv[li];
-        const LevelView own = view(li), par = view(li > 0 ? li - 1 : 0);
-        for (int rb = 1; rb <= 2 * n_cycle; rb++) {
-            nc_dispatch(L.nc, [&](auto ncv) {
-                constexpr int NC = decltype(ncv)::value;
-                const int tot = gsrb_threads<DIM>(L.n, L.nc);
-                for (int g = tid; g < tot; g += nt)
-                    gsrb_body<DIM, NC>(T, L, own, par, V.op_auto, rb, rb == 1 ? 0 : 1, g);
-            });
-            __syncthreads();
-        }
-        if (n_cycle > 0) fill(li);
-    };
-    auto update_coarse = [&](int li) {
-        const LevelArgs &L = B.lv[li];
-        const LevelView own = view(li), par = view(li - 1);
-        nc_dispatch(L.nc, [&](auto ncv) {
-            constexpr int NC = decltype(ncv)::value;
-            const int tot = L.n * ipow(L.nc / 2, DIM);
-            for (int g = tid; g < tot; g += nt)
-                uc_child_body<DIM, NC>(T, L, own, par, V.op_auto, 1, 0, g);
-        });
-        __syncthreads();
-        const LevelArgs &Lp = B.lv[li - 1];
-        const LevelView ppar = view(li > 1 ? li - 2 : 0);
-        nc_dispatch(Lp.nc, [&](auto ncv) {
-            constexpr int NC = decltype(ncv)::value;
-            const int tot = Lp.n * ipow(Lp.nc + 2, DIM);
-            for (int g = tid; g < tot; g += nt)
-                uc_parent_body<DIM, NC>(T, Lp, par, ppar, V.op_auto, 1, g);
-        });
-        __syncthreads();
-    };
-    auto correct_children = [&](int li) {
-        const LevelArgs &L = B.lv[li];
-        const LevelView own = view(li), par = view(li - 1);
-        nc_dispatch(L.nc, [&](auto ncv) {
-            constexpr int NC = decltype(ncv)::value;
-            const int tot = L.n * ipow(L.nc / 2, DIM);
-            for (int g = tid; g < tot; g += nt)
-                correct_body<DIM, NC>(T, L, own, par, V.op_auto, 1, g);
-        });
-        __syncthreads();
-        const ListArgs &P = B.par[li - 1];
-        nc_dispatch(P.nc, [&](auto ncv) {
-            constexpr int NC = decltype(ncv)::value;
-            const int tot = P.n * ipow(P.nc + 2, DIM);
-            for (int g = tid; g < tot; g += nt) diff_body<DIM, NC>(T, P, par, g);
-        });
-        __syncthreads();
-    };
-
-    for (int li = A.hl; li >= 1; li--) {
-        gsrb(li, A.n_down);
-        update_coarse(li);
-    }
-    gsrb(0, A.n_base);
-    for (int li = 1; li <= A.hl; li++) {
-        correct_children(li);
-        fill(li);
-        gsrb(li, A.n_up);
-    }
-
-    for (int li = 0; li <= A.hl; li++) {
-        copy_out(V.phi, B.lds_off[0][li], li);
-        copy_out(V.rhs, B.lds_off[1][li], li);
-        copy_out(V.tmp, B.lds_off[2][li], li);
-    }
+    bottom_run<DIM>(T, V, A, lds, threadIdx.x, blockDim.x, [] { __syncthreads(); });
 }
 
-// residual on the small levels 0..hl in one launch (set_residual part of
-// mg$D_fas_vcycle, m_aX_mg.f90:243-254), on the HBM pools
 template <int DIM>
 __global__ __launch_bounds__(BOTTOM_BLOCK) void k_bottom_residual(DevTree T, MgVars V, BottomArgs A)
 {
-    const BottomLevels &B = *A.levels;
-    const LevelView v = LevelView{V.phi, V.rhs, V.tmp, V.eps, 0, T.stride};
-    for (int li = 0; li <= A.hl; li++) {
-        const LevelArgs &L = B.lv[li];
-        nc_dispatch(L.nc, [&](auto ncv) {
-            constexpr int NC = decltype(ncv)::value;
-            const int tot = L.n * ipow(L.nc, DIM);
-            for (int g = threadIdx.x; g < tot; g += blockDim.x)
-                residual_body<DIM, NC>(T, L, v, v, V.op_auto, g);
-        });
-    }
+    bottom_residual_run<DIM>(T, V, A, threadIdx.x, blockDim.x);
 }
 
 }  // namespace

@@ -1,0 +1,83 @@
+// afg_types.h -- plain data types shared by the host orchestration and the
+// kernel bodies of the afivo_amd multigrid library (no HIP dependency).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+namespace afg {
+
+// Encoding of the per-slot neighbour table (the device form of
+// box%neighbors, m_aX_t.f90:163):
+//   >= 0        slot of the same-level neighbour
+//   NB_REFINE   no neighbour at this level: refinement boundary (a5_no_box)
+//   <= NB_BC0   physical boundary, boundary record index NB_BC0 - value
+constexpr int32_t NB_REFINE = -1;
+constexpr int32_t NB_BC0 = -2;
+
+// Tags of mg$D_set_box_tag (m_aX_mg.f90:13-17, 667-706).
+constexpr int8_t TAG_UNSET = 0;
+constexpr int8_t TAG_NORMAL = 1;
+constexpr int8_t TAG_CEPS = 3;
+constexpr int8_t TAG_VEPS = 4;
+
+// Device topology of one tree.  Slots of a level are contiguous and in Morton
+// order; every box array of a variable occupies `stride` doubles at
+// var + slot * stride and holds cc(0:nc_l+1, ...) in Fortran order.
+struct DevTree {
+    int stride;
+    const int32_t *nb;        // [slot * 2*dim + face]
+    const int32_t *parent;    // parent slot, -1 if none
+    const int32_t *choff;     // child offset in the parent: x | y << 10 | z << 20
+    const int32_t *bc_type;   // per boundary record
+    const int32_t *bc_voff;   // offset of the boundary values, -1 for b = 0
+    const double *bc_vals;
+    int8_t *tag;              // per slot (auto operators)
+    const int8_t *is_parent;  // per slot: the box has children
+};
+
+// One level: slots [slot0, slot0 + n), nc cells per dimension, cell size dr;
+// pnc is the cell count of the parent level.
+struct LevelArgs {
+    int slot0, n, nc, pnc;
+    double dr;          // box%dr of the level
+    double dx2;         // dr * dr  (dr**2, m_aX_mg.f90:730)
+    double inv_dr_sq;   // 1 / (dr * dr)  (m_aX_mg.f90:772)
+};
+
+// A list of slots (parents or leaves of a level), all with nc cells.
+struct ListArgs {
+    const int32_t *slots;
+    int n, nc;
+    double dr, dx2, inv_dr_sq;
+};
+
+// Variables of the multigrid options, as device pointers.
+struct MgVars {
+    double *phi, *rhs, *tmp;
+    const double *eps;   // nullptr unless the auto operators are used
+    int op_auto;
+};
+
+// The lower part of a V-cycle -- every level from `hl` down to the lowest
+// and back up -- run by one workgroup in one launch (levels with few cells,
+// where per-stage launches would cost more than the work).  Levels are
+// indexed relative to the lowest one.
+constexpr int MAX_BOTTOM_LEVELS = 12;
+constexpr int BOTTOM_BLOCK = 1024;
+// Level tables of the bottom kernel, kept in device memory (index: level
+// relative to the lowest one), with the LDS layout of its copies: variable v
+// (phi, rhs, tmp, eps) of level li starts at lds_off[v][li] doubles.
+struct BottomLevels {
+    LevelArgs lv[MAX_BOTTOM_LEVELS];
+    ListArgs par[MAX_BOTTOM_LEVELS];   // parents of each level
+    int box_len[MAX_BOTTOM_LEVELS];    // (nc+2)^dim
+    int lds_off[4][MAX_BOTTOM_LEVELS];
+};
+struct BottomArgs {
+    const BottomLevels *levels;   // device pointer
+    int hl;                       // highest level of this call (relative index)
+    int n_down, n_up, n_base;
+};
+
+}  // namespace afg

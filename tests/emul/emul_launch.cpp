@@ -1,0 +1,279 @@
+// CPU emulation of the afivo_amd kernel launchers.  TEST INFRASTRUCTURE ONLY.
+//
+// Each launcher runs the threads of its stage one after the other through
+// the same per-thread bodies the HIP kernels use (afivo_amd/csrc/afg_bodies.h,
+// compiled here for the host).  A stage is race-free, so any thread order
+// gives the kernel's result; the bottom kernel runs as one thread whose
+// workgroup barriers are no-ops.  Built with -ffp-contract=off, the host
+// arithmetic rounds like the device's, so results compare bit for bit.
+#define AFG_DEV inline
+#define AFG_HD
+#include "afg_internal.h"
+#include "afg_bodies.h"
+#include "emul.h"
+
+#include <algorithm>
+#include <cfloat>
+
+namespace afg {
+
+namespace {
+
+constexpr int BLOCK = 256;
+
+LevelView pool_view(const DevTree &T, const MgVars &V)
+{
+    return LevelView{V.phi, V.rhs, V.tmp, V.eps, 0, T.stride};
+}
+
+LevelView var_view(const DevTree &T, double *var)
+{
+    return LevelView{var, nullptr, nullptr, nullptr, 0, T.stride};
+}
+
+// Run body<DIM, NC>(gid) for gid in [0, n(DIM)) with NC dispatched like the
+// device launchers do.
+template <typename Body, typename Count>
+void run_stage(int dim, int nc, Count count, Body body)
+{
+    if (dim == 2) {
+        nc_dispatch(nc, [&](auto ncv) {
+            const long long n = count(std::integral_constant<int, 2>());
+            for (long long g = 0; g < n; g++) body(std::integral_constant<int, 2>(), ncv, (int)g);
+        });
+    } else {
+        nc_dispatch(nc, [&](auto ncv) {
+            const long long n = count(std::integral_constant<int, 3>());
+            for (long long g = 0; g < n; g++) body(std::integral_constant<int, 3>(), ncv, (int)g);
+        });
+    }
+}
+
+#define DIMV decltype(d)::value
+#define NCV decltype(c)::value
+
+}  // namespace
+
+void launch_fill_gc(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, double *var)
+{
+    if (L.n <= 0) return;
+    emul_submit(s, [=] {
+        const LevelView v = var_view(T, var);
+        run_stage(dim, L.nc, [&](auto d) { return fill_threads<DIMV>(L.n, L.nc); },
+                  [&](auto d, auto c, int g) { fill_body<DIMV, NCV>(T, L, v, v, g); });
+    });
+}
+
+void launch_fill_diff(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
+                      const ListArgs &P, const MgVars &V)
+{
+    if (L.n <= 0) return;
+    emul_submit(s, [=] {
+        const LevelView v = pool_view(T, V);
+        run_stage(dim, L.nc, [&](auto d) { return fill_threads<DIMV>(L.n, L.nc); },
+                  [&](auto d, auto c, int g) { fill_body<DIMV, NCV>(T, L, v, v, g); });
+        run_stage(dim, P.nc, [&](auto d) { return (long long)P.n * ipow(P.nc + 2, DIMV); },
+                  [&](auto d, auto c, int g) { diff_body<DIMV, 0>(T, P, v, g); });
+    });
+}
+
+void launch_diff(int dim, hipStream_t s, const DevTree &T, const ListArgs &P, const MgVars &V)
+{
+    if (P.n <= 0) return;
+    emul_submit(s, [=] {
+        const LevelView v = pool_view(T, V);
+        run_stage(dim, P.nc, [&](auto d) { return (long long)P.n * ipow(P.nc + 2, DIMV); },
+                  [&](auto d, auto c, int g) { diff_body<DIMV, NCV>(T, P, v, g); });
+    });
+}
+
+void launch_gsrb_pass(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
+                      const MgVars &V, int rb, int fly)
+{
+    if (L.n <= 0) return;
+    emul_submit(s, [=] {
+        const LevelView v = pool_view(T, V);
+        run_stage(dim, L.nc, [&](auto d) { return gsrb_threads<DIMV>(L.n, L.nc); },
+                  [&](auto d, auto c, int g) { gsrb_body<DIMV, NCV>(T, L, v, v, V.op_auto, rb, fly, g); });
+    });
+}
+
+void launch_residual(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVars &V)
+{
+    if (L.n <= 0) return;
+    emul_submit(s, [=] {
+        const LevelView v = pool_view(T, V);
+        run_stage(dim, L.nc, [&](auto d) { return (long long)L.n * ipow(L.nc, DIMV); },
+                  [&](auto d, auto c, int g) { residual_body<DIMV, NCV>(T, L, v, v, V.op_auto, g); });
+    });
+}
+
+void launch_uc_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
+                     const MgVars &V, int keep_child_tmp, int fly)
+{
+    if (L.n <= 0) return;
+    emul_submit(s, [=] {
+        const LevelView v = pool_view(T, V);
+        run_stage(dim, L.nc, [&](auto d) { return (long long)L.n * ipow(L.nc / 2, DIMV); },
+                  [&](auto d, auto c, int g) {
+                      uc_child_body<DIMV, NCV>(T, L, v, v, V.op_auto, keep_child_tmp, fly, g);
+                  });
+    });
+}
+
+void launch_uc_parent(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
+                      const MgVars &V, int copy_tmp)
+{
+    if (L.n <= 0) return;
+    emul_submit(s, [=] {
+        const LevelView v = pool_view(T, V);
+        run_stage(dim, L.nc, [&](auto d) { return (long long)L.n * ipow(L.nc + 2, DIMV); },
+                  [&](auto d, auto c, int g) { uc_parent_body<DIMV, NCV>(T, L, v, v, V.op_auto, copy_tmp, g); });
+    });
+}
+
+void launch_correct(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
+                    const MgVars &V, int fly_diff)
+{
+    if (L.n <= 0) return;
+    emul_submit(s, [=] {
+        const LevelView v = pool_view(T, V);
+        run_stage(dim, L.nc, [&](auto d) { return (long long)L.n * ipow(L.nc / 2, DIMV); },
+                  [&](auto d, auto c, int g) { correct_body<DIMV, NCV>(T, L, v, v, V.op_auto, fly_diff, g); });
+    });
+}
+
+void launch_restrict(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
+                     const double *from, double *to)
+{
+    if (L.n <= 0) return;
+    emul_submit(s, [=] {
+        run_stage(dim, L.nc, [&](auto d) { return (long long)L.n * ipow(L.nc / 2, DIMV); },
+                  [&](auto d, auto c, int g) { restrict_body<DIMV, NCV>(T, L, from, to, g); });
+    });
+}
+
+void launch_prolong2(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
+                     const double *from, double *to, int add)
+{
+    if (L.n <= 0) return;
+    emul_submit(s, [=] {
+        run_stage(dim, L.nc, [&](auto d) { return (long long)L.n * ipow(L.nc / 2, DIMV); },
+                  [&](auto d, auto c, int g) { prolong2_body<DIMV, NCV>(T, L, from, to, add, g); });
+    });
+}
+
+void launch_fill_value(int dim, hipStream_t s, const DevTree &T, int slot0, int n_slots, int nc,
+                       double *var, double value)
+{
+    if (n_slots <= 0) return;
+    emul_submit(s, [=] {
+        const int len = ipow(nc + 2, dim);
+        for (int b = 0; b < n_slots; b++)
+            for (int q = 0; q < len; q++) var[(size_t)(slot0 + b) * T.stride + q] = value;
+    });
+}
+
+void launch_copy_boxes(int dim, hipStream_t s, const DevTree &T, int slot0, int n_slots, int nc,
+                       const double *from, double *to)
+{
+    if (n_slots <= 0) return;
+    emul_submit(s, [=] {
+        const int len = ipow(nc + 2, dim);
+        for (int b = 0; b < n_slots; b++)
+            for (int q = 0; q < len; q++) {
+                const size_t o = (size_t)(slot0 + b) * T.stride + q;
+                to[o] = from[o];
+            }
+    });
+}
+
+void launch_set_tags(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const double *eps)
+{
+    if (L.n <= 0) return;
+    emul_submit(s, [=] {
+        const int len = ipow(L.nc + 2, dim);
+        for (int b = 0; b < L.n; b++) {
+            const int slot = L.slot0 + b;
+            if (T.tag[slot] != TAG_UNSET) continue;
+            const double *e = eps + (size_t)slot * T.stride;
+            double lo = e[0], hi = e[0];
+            for (int q = 0; q < len; q++) {
+                lo = e[q] < lo ? e[q] : lo;
+                hi = e[q] > hi ? e[q] : hi;
+            }
+            int8_t tag = TAG_NORMAL;
+            if (hi > lo)
+                tag = TAG_VEPS;
+            else if (lo < 1 || lo > 1)
+                tag = TAG_CEPS;
+            T.tag[slot] = tag;
+        }
+    });
+}
+
+int extreme_blocks(int dim, const ListArgs &P)
+{
+    if (P.n <= 0) return 0;
+    const long long th = (long long)P.n * ipow(P.nc, dim);
+    const int nb = (int)((th + BLOCK - 1) / BLOCK);
+    return nb < 1024 ? nb : 1024;
+}
+
+void launch_extreme_list(int dim, hipStream_t s, const DevTree &T, const ListArgs &P,
+                         const double *var, int want_max, double *partial)
+{
+    const int nb = extreme_blocks(dim, P);
+    if (nb == 0) return;
+    emul_submit(s, [=] {
+        double v = want_max ? -DBL_MAX : DBL_MAX;
+        const int nc = P.nc, sp = nc + 2;
+        for (int b = 0; b < P.n; b++)
+            for (int k = dim == 3 ? 1 : 0; k <= (dim == 3 ? nc : 0); k++)
+                for (int j = 1; j <= nc; j++)
+                    for (int i = 1; i <= nc; i++) {
+                        const double x = var[(size_t)P.slots[b] * T.stride + i + sp * (j + sp * k)];
+                        v = want_max ? (x > v ? x : v) : (x < v ? x : v);
+                    }
+        partial[0] = v;
+        for (int k = 1; k < nb; k++) partial[k] = want_max ? -DBL_MAX : DBL_MAX;
+    });
+}
+
+void launch_reduce_partials(hipStream_t s, const double *partial, int n, int want_max, double *out)
+{
+    emul_submit(s, [=] {
+        double v = want_max ? -DBL_MAX : DBL_MAX;
+        for (int k = 0; k < n; k++) v = want_max ? (partial[k] > v ? partial[k] : v)
+                                                 : (partial[k] < v ? partial[k] : v);
+        out[0] = v;
+    });
+}
+
+int bottom_prepare(int, size_t lds_bytes) { return lds_bytes <= 163840 ? 0 : -1; }
+
+void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVars &V, const BottomArgs &B,
+                   size_t lds_bytes)
+{
+    emul_submit(s, [=] {
+        double *lds = (double *)emul_alloc(lds_bytes ? lds_bytes : 8);
+        if (dim == 2)
+            bottom_run<2>(T, V, B, lds, 0, 1, [] {});
+        else
+            bottom_run<3>(T, V, B, lds, 0, 1, [] {});
+        emul_free(lds);
+    });
+}
+
+void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVars &V,
+                            const BottomArgs &B)
+{
+    emul_submit(s, [=] {
+        if (dim == 2)
+            bottom_residual_run<2>(T, V, B, 0, 1);
+        else
+            bottom_residual_run<3>(T, V, B, 0, 1);
+    });
+}
+
+}  // namespace afg

@@ -1,0 +1,39 @@
+"""The GPU parity suite run through the CPU emulation build (tests/emul).
+
+The emulation compiles the library's host code and the kernels' per-thread
+bodies with g++ and runs every stage's threads one after the other (a stage
+is race-free, so the order does not matter), with guard pages after every
+device allocation.  It checks the stage logic, indexing and orchestration on
+the CPU, bit for bit against the oracle; the same tests run on the GPU under
+the gpu marker.  (Test infrastructure: the product never loads it.)
+"""
+import os
+
+import pytest
+
+import conftest
+from afivo_amd import _lib
+
+pytestmark = pytest.mark.skipif(not os.path.exists(conftest.EMUL_LIB),
+                                reason="emulation library not built (make -C tests/emul)")
+
+
+@pytest.fixture(autouse=True)
+def emulated_library():
+    previous = conftest.load_emulation()
+    yield
+    _lib._lib = previous
+
+
+from test_gpu_parity import (  # noqa: E402,F401
+    test_boundary_values_and_neumann,
+    test_bottom_kernel_limit_is_exact,
+    test_c1_full_config_bitexact,
+    test_c2_full_config_bitexact,
+    test_golden_solve_bitexact,
+    test_graphs_and_direct_launches_agree,
+    test_periodic_base,
+    test_solve_bitexact,
+    test_stage_bitexact,
+    test_timing_mode_is_exact,
+)
