@@ -616,28 +616,60 @@ AFG_DEV void nc_dispatch(int nc, F &&f)
     }
 }
 
-// The whole bottom part for thread tid of nt; `lds` holds the level copies
-// and `sync()` is the workgroup barrier.
+// The whole bottom part for thread tid of nt; `lds` is the workgroup's
+// dynamic LDS and `sync()` the workgroup barrier.
 template <int DIM, typename Sync>
-AFG_DEV void bottom_run(const DevTree &T, const MgVars &V, const BottomArgs &A, double *lds,
+AFG_DEV void bottom_run(const DevTree &Tg, const MgVars &V, const BottomArgs &A, double *lds,
                         int tid, int nt, Sync sync)
 {
     const BottomLevels &B = *A.levels;
+    constexpr int NNB = 2 * DIM;
+    // topology tables of the bottom slots, in LDS
+    char *tb = (char *)lds;
+    int32_t *t_nb = (int32_t *)tb;
+    int32_t *t_par = t_nb + B.n_slots * NNB;
+    int32_t *t_co = t_par + B.n_slots;
+    int32_t *t_bct = t_co + B.n_slots;
+    int32_t *t_bcv = t_bct + B.n_bc;
+    int8_t *t_tag = (int8_t *)(t_bcv + B.n_bc);
+    int8_t *t_isp = t_tag + B.n_slots;
+    for (int e = tid; e < B.n_slots * NNB; e += nt) t_nb[e] = Tg.nb[e];
+    for (int e = tid; e < B.n_slots; e += nt) {
+        t_par[e] = Tg.parent[e];
+        t_co[e] = Tg.choff[e];
+        t_tag[e] = Tg.tag[e];
+        t_isp[e] = Tg.is_parent[e];
+    }
+    for (int e = tid; e < B.n_bc; e += nt) {
+        t_bct[e] = Tg.bc_type[e];
+        t_bcv[e] = Tg.bc_voff[e];
+    }
+    DevTree T = Tg;
+    T.nb = t_nb;
+    T.parent = t_par;
+    T.choff = t_co;
+    T.bc_type = t_bct;
+    T.bc_voff = t_bcv;
+    T.tag = t_tag;
+    T.is_parent = t_isp;
+    double *data = (double *)(tb + B.table_bytes);
+
     auto copy_in = [&](const double *src, int off, int li) {
         const int len = B.box_len[li], n = B.lv[li].n, slot0 = B.lv[li].slot0;
         for (int e = tid; e < n * len; e += nt) {
             const int b = e / len;
-            lds[off + e] = src[(size_t)(slot0 + b) * T.stride + (e - b * len)];
+            data[off + e] = src[(size_t)(slot0 + b) * T.stride + (e - b * len)];
         }
     };
     auto copy_out = [&](double *dst, int off, int li) {
         const int len = B.box_len[li], n = B.lv[li].n, slot0 = B.lv[li].slot0;
         for (int e = tid; e < n * len; e += nt) {
             const int b = e / len;
-            dst[(size_t)(slot0 + b) * T.stride + (e - b * len)] = lds[off + e];
+            dst[(size_t)(slot0 + b) * T.stride + (e - b * len)] = data[off + e];
         }
     };
     for (int li = 0; li <= A.hl; li++) {
+        if (!B.in_lds[li]) continue;
         copy_in(V.phi, B.lds_off[0][li], li);
         copy_in(V.rhs, B.lds_off[1][li], li);
         copy_in(V.tmp, B.lds_off[2][li], li);
@@ -647,12 +679,16 @@ AFG_DEV void bottom_run(const DevTree &T, const MgVars &V, const BottomArgs &A, 
 
     auto view = [&](int li) {
         LevelView w;
-        w.phi = lds + B.lds_off[0][li];
-        w.rhs = lds + B.lds_off[1][li];
-        w.tmp = lds + B.lds_off[2][li];
-        w.eps = V.op_auto ? lds + B.lds_off[3][li] : nullptr;
-        w.slot0 = B.lv[li].slot0;
-        w.stride = B.box_len[li];
+        if (B.in_lds[li]) {
+            w.phi = data + B.lds_off[0][li];
+            w.rhs = data + B.lds_off[1][li];
+            w.tmp = data + B.lds_off[2][li];
+            w.eps = V.op_auto ? data + B.lds_off[3][li] : nullptr;
+            w.slot0 = B.lv[li].slot0;
+            w.stride = B.box_len[li];
+        } else {
+            w = LevelView{V.phi, V.rhs, V.tmp, V.eps, 0, T.stride};
+        }
         return w;
     };
     auto fill = [&](int li) {
@@ -730,6 +766,7 @@ AFG_DEV void bottom_run(const DevTree &T, const MgVars &V, const BottomArgs &A, 
     }
 
     for (int li = 0; li <= A.hl; li++) {
+        if (!B.in_lds[li]) continue;
         copy_out(V.phi, B.lds_off[0][li], li);
         copy_out(V.rhs, B.lds_off[1][li], li);
         copy_out(V.tmp, B.lds_off[2][li], li);

@@ -118,6 +118,7 @@ struct afg_ctx {
     // execution controls
     int bottom_lvl[2] = {-1000, -1000};  // highest level run by the bottom kernel (3 / 4 variables)
     size_t bottom_lds[2] = {0, 0};       // its dynamic LDS bytes
+    int bottom_threads = BOTTOM_BLOCK;   // its workgroup size
     int64_t lds_limit = 65536;           // LDS bytes one workgroup may use
     int64_t bottom_cells = 8192;        // cell limit per level for the bottom kernel
     bool use_graphs = true;
@@ -506,14 +507,15 @@ void vcycle_body(Mg &m, int highest_lvl, int set_residual)
         update_coarse(m, lvl, m.o->n_cycle_down > 0);
     }
     if (lb >= c->min_lvl) {
-        launch_bottom(c->dim, c->stream, m.T, m.V, bottom_args(m, lb), c->bottom_lds[m.V.op_auto ? 1 : 0]);
+        launch_bottom(c->dim, c->stream, m.T, m.V, bottom_args(m, lb), c->bottom_lds[m.V.op_auto ? 1 : 0],
+                      c->bottom_threads);
     } else {
         gsrb_level(m, c->min_lvl, m.o->n_cycle_base);
     }
     for (int lvl = low + 1; lvl <= highest_lvl; lvl++) correct_relax(m, lvl, m.o->n_cycle_up);
     if (set_residual) {
         if (lb >= c->min_lvl) {
-            launch_bottom_residual(c->dim, c->stream, m.T, m.V, bottom_args(m, lb));
+            launch_bottom_residual(c->dim, c->stream, m.T, m.V, bottom_args(m, lb), c->bottom_threads);
             for (int lvl = lb + 1; lvl <= highest_lvl; lvl++) residual_level(m, lvl);
         } else {
             for (int lvl = c->min_lvl; lvl <= highest_lvl; lvl++) residual_level(m, lvl);
@@ -581,50 +583,62 @@ int check_level(afg_ctx *c, int lvl, bool need_parent)
 }
 
 // The bottom kernel takes the lowest levels while each holds at most
-// bottom_cells cells and their phi, rhs, tmp (and eps, for the auto
-// operators) fit in the LDS of one workgroup.  Two tables (3 and 4 variables)
-// go to device memory.
+// bottom_cells cells.  Their topology tables always go to its LDS; the
+// lowest of them whose phi, rhs, tmp (and eps, for the auto operators) also
+// fit there are copied in as well, the others are used in place in the HBM
+// pools.  Two tables (3 and 4 variables) go to device memory.
 int choose_bottom(afg_ctx *c)
 {
+    const int nl = (int)c->levels.size();
+    int top = -1;
+    int64_t max_threads = 64;
+    for (int li = 0; li < nl && li < MAX_BOTTOM_LEVELS; li++) {
+        const Level &L = c->levels[li];
+        const int64_t cells = (int64_t)L.n * (c->dim == 2 ? L.nc * L.nc : L.nc * L.nc * L.nc);
+        if (cells > c->bottom_cells) break;
+        top = li;
+        max_threads = std::max<int64_t>(max_threads, (int64_t)L.n * box_len(c->dim, L.nc) / 2);
+    }
+    c->bottom_threads = (int)std::min<int64_t>(BOTTOM_BLOCK, (max_threads + 63) / 64 * 64);
+    int n_slots = 0, n_bc = 0;
+    if (top >= 0) {
+        n_slots = c->levels[top].slot0 + c->levels[top].n;
+        for (const BcRec &r : c->bc) n_bc += r.slot < n_slots;
+    }
+    const int table_bytes =
+        (int)((((int64_t)n_slots * (c->nnb + 2) + 2 * n_bc) * 4 + 2 * n_slots + 7) / 8 * 8);
     for (int k = 0; k < 2; k++) {
         const int nv = 3 + k;
-        c->bottom_lvl[k] = -1000;
         BottomLevels h;
         std::memset(&h, 0, sizeof h);
-        const int nl = (int)c->levels.size();
-        int64_t per_var = 0;   // doubles of one variable over the levels taken
-        int top = -1;
-        for (int li = 0; li < nl && li < MAX_BOTTOM_LEVELS; li++) {
-            const Level &L = c->levels[li];
-            const int64_t cells = (int64_t)L.n * (c->dim == 2 ? L.nc * L.nc : L.nc * L.nc * L.nc);
-            const int64_t len = box_len(c->dim, L.nc);
-            if (cells > c->bottom_cells) break;
-            if ((per_var + L.n * len) * nv * (int64_t)sizeof(double) > c->lds_limit) break;
-            per_var += L.n * len;
-            top = li;
-        }
-        int64_t off = 0;
+        h.n_slots = n_slots;
+        h.n_bc = n_bc;
+        h.table_bytes = table_bytes;
+        int64_t per_var = 0;   // doubles of one variable over the levels in LDS
+        bool fits = true;
         for (int li = 0; li <= top; li++) {
             const Level &L = c->levels[li];
             h.lv[li] = level_args(c, L.lvl);
             h.par[li] = list_args(c, L.lvl, 0);
             h.box_len[li] = box_len(c->dim, L.nc);
+            const int64_t add = (int64_t)L.n * h.box_len[li];
+            fits = fits && table_bytes + (per_var + add) * nv * (int64_t)sizeof(double) <= c->lds_limit;
+            h.in_lds[li] = fits ? 1 : 0;
+            if (fits) per_var += add;
         }
-        // variable-major layout over all levels taken
-        c->bottom_lds[k] = (size_t)(per_var * nv * sizeof(double));
-        for (int v = 0; v < 4; v++) {
-            off = v * per_var;
+        for (int v = 0; v < 4; v++) {   // variable-major layout over the levels in LDS
+            int64_t off = v * per_var;
             for (int li = 0; li <= top; li++) {
+                if (!h.in_lds[li]) continue;
                 h.lds_off[v][li] = (int)off;
                 off += (int64_t)c->levels[li].n * h.box_len[li];
             }
         }
-        if (top >= 0) {
-            c->bottom_lvl[k] = c->levels[top].lvl;
-            if (bottom_prepare(c->dim, (size_t)(per_var * nv * sizeof(double))) != 0)
-                return c->fail(AFG_ERR_HIP, "cannot allow %lld bytes of LDS for the bottom kernel",
-                               (long long)(per_var * nv * sizeof(double)));
-        }
+        c->bottom_lvl[k] = top >= 0 ? c->levels[top].lvl : -1000;
+        c->bottom_lds[k] = (size_t)table_bytes + (size_t)(per_var * nv * sizeof(double));
+        if (top >= 0 && bottom_prepare(c->dim, c->bottom_lds[k]) != 0)
+            return c->fail(AFG_ERR_HIP, "cannot allow %zu bytes of LDS for the bottom kernel",
+                           c->bottom_lds[k]);
         HIP_TRY(c, hipMemcpy(c->d_bottom[k], &h, sizeof h, hipMemcpyHostToDevice));
     }
     for (auto &g : c->graphs) destroy_graph(g.second);

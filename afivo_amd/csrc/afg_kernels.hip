@@ -408,21 +408,21 @@ int bottom_prepare(int dim, size_t lds_bytes)
 }
 
 void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVars &V, const BottomArgs &B,
-                   size_t lds_bytes)
+                   size_t lds_bytes, int threads)
 {
     if (dim == 2)
-        k_bottom<2><<<1, BOTTOM_BLOCK, lds_bytes, s>>>(T, V, B);
+        k_bottom<2><<<1, threads, lds_bytes, s>>>(T, V, B);
     else
-        k_bottom<3><<<1, BOTTOM_BLOCK, lds_bytes, s>>>(T, V, B);
+        k_bottom<3><<<1, threads, lds_bytes, s>>>(T, V, B);
 }
 
 void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVars &V,
-                            const BottomArgs &B)
+                            const BottomArgs &B, int threads)
 {
     if (dim == 2)
-        k_bottom_residual<2><<<1, BOTTOM_BLOCK, 0, s>>>(T, V, B);
+        k_bottom_residual<2><<<1, threads, 0, s>>>(T, V, B);
     else
-        k_bottom_residual<3><<<1, BOTTOM_BLOCK, 0, s>>>(T, V, B);
+        k_bottom_residual<3><<<1, threads, 0, s>>>(T, V, B);
 }
 
 }  // namespace afg

@@ -66,13 +66,18 @@ struct MgVars {
 constexpr int MAX_BOTTOM_LEVELS = 12;
 constexpr int BOTTOM_BLOCK = 1024;
 // Level tables of the bottom kernel, kept in device memory (index: level
-// relative to the lowest one), with the LDS layout of its copies: variable v
-// (phi, rhs, tmp, eps) of level li starts at lds_off[v][li] doubles.
+// relative to the lowest one).  The bottom levels own slots [0, n_slots) and
+// boundary records [0, n_bc); the kernel copies their topology tables into
+// the first table_bytes of its LDS.  Levels with in_lds set are copied to LDS
+// too, after the tables: variable v (phi, rhs, tmp, eps) of level li starts
+// at lds_off[v][li] doubles there; the others stay in the HBM pools.
 struct BottomLevels {
     LevelArgs lv[MAX_BOTTOM_LEVELS];
     ListArgs par[MAX_BOTTOM_LEVELS];   // parents of each level
     int box_len[MAX_BOTTOM_LEVELS];    // (nc+2)^dim
+    int in_lds[MAX_BOTTOM_LEVELS];
     int lds_off[4][MAX_BOTTOM_LEVELS];
+    int n_slots, n_bc, table_bytes;
 };
 struct BottomArgs {
     const BottomLevels *levels;   // device pointer

@@ -253,7 +253,7 @@ void launch_reduce_partials(hipStream_t s, const double *partial, int n, int wan
 int bottom_prepare(int, size_t lds_bytes) { return lds_bytes <= 163840 ? 0 : -1; }
 
 void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVars &V, const BottomArgs &B,
-                   size_t lds_bytes)
+                   size_t lds_bytes, int)
 {
     emul_submit(s, [=] {
         double *lds = (double *)emul_alloc(lds_bytes ? lds_bytes : 8);
@@ -266,7 +266,7 @@ void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVars &V, co
 }
 
 void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVars &V,
-                            const BottomArgs &B)
+                            const BottomArgs &B, int)
 {
     emul_submit(s, [=] {
         if (dim == 2)
