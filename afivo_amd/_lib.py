@@ -61,7 +61,7 @@ SIGNATURES = {
     "afg_prolong2_level": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "afg_correct_children": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int]),
     "afg_update_coarse": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int]),
-    "afg_set_bottom_cells": (ctypes.c_int, [P, I64]),
+    "afg_set_bottom_cells": (ctypes.c_int, [P, I64, I64]),
     "afg_set_graphs": (ctypes.c_int, [P, ctypes.c_int]),
     "afg_set_timing": (ctypes.c_int, [P, ctypes.c_int]),
     "afg_timing": (ctypes.c_int, [P, I64P, F64P, F64P]),

@@ -235,33 +235,15 @@ AFG_HD inline int gsrb_threads(int n, int nc)
     return n * ipow(nc, DIM) / 2;
 }
 
-template <int DIM, int NC>
-AFG_DEV void gsrb_body(const DevTree &T, const LevelArgs &L, const LevelView &own,
-                                          const LevelView &par, int op_auto, int rb, int fly,
-                                          int gid)
+// New phi at interior cell q of a box from its face neighbours u (order of
+// stencil) and rhs: mg$D_box_gsrb_lpl (m_aX_mg.f90:718-760) or, when lpld,
+// mg$D_box_gsrb_lpld (:810-863) with eps e of the box.
+template <int DIM>
+AFG_DEV double gsrb_value(const double u[6], double rhs, const double *e, int q, int s,
+                          double dx2, bool lpld)
 {
-    const int nc = NC ? NC : L.nc;
-    const int hn = nc >> 1;
-    const int half = DIM == 2 ? nc * hn : nc * nc * hn;
-    const int b = gid / half;
-    const int c = gid - b * half;
-    const int r = c / hn;
-    const int p = c - r * hn;
-    const int j = DIM == 2 ? r + 1 : r % nc + 1;
-    const int k = DIM == 2 ? 0 : r / nc + 1;
-    const int i = 2 - ((rb ^ (DIM == 2 ? j : k + j)) & 1) + 2 * p;
-    const int slot = L.slot0 + b;
-    const int s = nc + 2;
-    const size_t base = own.off(slot);
-    double *box = own.phi + base;
-    const int q = i + s * (j + s * k);
-    const double center = box[q];
-    double u[6];
-    stencil<DIM>(T, L, own, par, slot, box, nc, i, j, k, q, fly, center, u);
-    const double rhs = own.rhs[base + q];
-    if (lpld_box(T, op_auto, slot)) {
-        // mg$D_box_gsrb_lpld: c = 2 a0 a / (a0 + a); phi = (sum(c u) - dx2 rhs) / sum(c)
-        const double *e = own.eps + base;
+    if (lpld) {
+        // c = 2 a0 a / (a0 + a); phi = (sum(c u) - dx2 rhs) / sum(c)
         const double a0 = e[q];
         const int dq[6] = {-1, 1, -s, s, -s * s, s * s};
         double su = 0, sc = 0;
@@ -272,13 +254,169 @@ AFG_DEV void gsrb_body(const DevTree &T, const LevelArgs &L, const LevelView &ow
             su = m == 0 ? cm * u[m] : su + cm * u[m];
             sc = m == 0 ? cm : sc + cm;
         }
-        box[q] = (su - L.dx2 * rhs) / sc;
-    } else if (DIM == 2) {
+        return (su - dx2 * rhs) / sc;
+    }
+    if (DIM == 2) {
         // order of m_aX_mg.f90:740-743: i+1, i-1, j+1, j-1
-        box[q] = 0.25 * (u[1] + u[0] + u[3] + u[2] - L.dx2 * rhs);
-    } else {
-        constexpr double sixth = 1 / 6.0;
-        box[q] = sixth * (u[1] + u[0] + u[3] + u[2] + u[5] + u[4] - L.dx2 * rhs);
+        return 0.25 * (u[1] + u[0] + u[3] + u[2] - dx2 * rhs);
+    }
+    constexpr double sixth = 1 / 6.0;
+    return sixth * (u[1] + u[0] + u[3] + u[2] + u[5] + u[4] - dx2 * rhs);
+}
+
+// interior cell (i, j, k) of colour rb for index c in [0, nc^DIM / 2): the
+// cells with i + j (+ k) = rb (mod 2), i fastest
+template <int DIM>
+AFG_HD inline void colour_cell(int c, int nc, int rb, int &i, int &j, int &k)
+{
+    const int hn = nc >> 1;
+    const int r = c / hn;
+    const int p = c - r * hn;
+    j = DIM == 2 ? r + 1 : r % nc + 1;
+    k = DIM == 2 ? 0 : r / nc + 1;
+    i = 2 - ((rb ^ (DIM == 2 ? j : k + j)) & 1) + 2 * p;
+}
+
+template <int DIM, int NC>
+AFG_DEV void gsrb_body(const DevTree &T, const LevelArgs &L, const LevelView &own,
+                       const LevelView &par, int op_auto, int rb, int fly, int gid)
+{
+    const int nc = NC ? NC : L.nc;
+    const int half = ipow(nc, DIM) / 2;
+    const int b = gid / half;
+    int i, j, k;
+    colour_cell<DIM>(gid - b * half, nc, rb, i, j, k);
+    const int slot = L.slot0 + b;
+    const int s = nc + 2;
+    const size_t base = own.off(slot);
+    double *box = own.phi + base;
+    const int q = i + s * (j + s * k);
+    const double center = box[q];
+    double u[6];
+    stencil<DIM>(T, L, own, par, slot, box, nc, i, j, k, q, fly, center, u);
+    const bool lpld = lpld_box(T, op_auto, slot);
+    box[q] = gsrb_value<DIM>(u, own.rhs[base + q], lpld ? own.eps + base : nullptr, q, s, L.dx2, lpld);
+}
+
+// ---------------------------------------------------------------------------
+// one colour pass of GSRB over a level by tiles: a workgroup takes BOXES
+// boxes.  Stage A copies their interiors to LDS, puts in their ghost layer the
+// face ghosts that cells of the colour read -- computed with ghost_value
+// (FLY) or read from memory -- and fetches rhs of the thread's cells of the
+// colour; after a workgroup barrier, stage B updates those cells from LDS.
+// Every global read is issued in stage A.  Results equal gsrb_body's: the
+// ghosts a pass computes depend only on cells of the other colour and on the
+// parent level, which the pass does not change.
+template <int DIM, int NC>
+struct GsrbTile {
+    static constexpr int THREADS = 256;
+    static constexpr int S = NC + 2;
+    static constexpr int LEN = ipow(S, DIM);            // LDS doubles per box
+    static constexpr int CELLS = ipow(NC, DIM);
+    static constexpr int HALF = CELLS / 2;              // cells of one colour
+    static constexpr int BOXES = HALF >= THREADS ? 1 : THREADS / HALF;
+    static constexpr int FACE = DIM == 2 ? NC : NC * NC;
+    static constexpr int GHOSTS = DIM * FACE;           // face ghosts of one colour
+    static constexpr int PER = (BOXES * HALF + THREADS - 1) / THREADS;   // cells per thread
+};
+
+// Workgroup index -> tile index such that each of the 8 XCDs, which take
+// workgroups round-robin, runs one contiguous range of tiles (neighbouring
+// boxes share that XCD's L2).
+AFG_HD inline int xcd_tile(int blk, int n_blk)
+{
+    const int q = n_blk >> 3, r = n_blk & 7;
+    const int x = blk & 7, y = blk >> 3;
+    return x * q + (x < r ? x : r) + y;
+}
+
+template <int DIM, int NC, int FLY>
+AFG_DEV void tile_stage_a(const DevTree &T, const LevelArgs &L, const LevelView &v, double *lds,
+                          int box0, int rb, int tid, double *rhs)
+{
+    using G = GsrbTile<DIM, NC>;
+    constexpr int S = G::S;
+    // rhs of this thread's cells of the colour
+#pragma unroll
+    for (int r = 0; r < G::PER; r++) {
+        const int e = tid + r * G::THREADS;
+        const int b = e / G::HALF;
+        if (e < G::BOXES * G::HALF && box0 + b < L.n) {
+            int i, j, k;
+            colour_cell<DIM>(e - b * G::HALF, NC, rb, i, j, k);
+            rhs[r] = v.rhs[v.off(L.slot0 + box0 + b) + i + S * (j + S * k)];
+        }
+    }
+    // interiors
+    for (int e = tid; e < G::BOXES * G::CELLS; e += G::THREADS) {
+        const int b = e / G::CELLS;
+        if (box0 + b >= L.n) break;
+        const int c = e - b * G::CELLS;
+        const int i = c % NC + 1, j = (c / NC) % NC + 1, k = DIM == 3 ? c / (NC * NC) + 1 : 0;
+        const int q = i + S * (j + S * k);
+        lds[b * G::LEN + q] = v.phi[v.off(L.slot0 + box0 + b) + q];
+    }
+    // face ghosts next to cells of the colour
+    constexpr int FH = G::FACE / 2;
+    for (int e = tid; e < G::BOXES * G::GHOSTS; e += G::THREADS) {
+        const int b = e / G::GHOSTS;
+        if (box0 + b >= L.n) break;
+        const int r = e - b * G::GHOSTS;
+        const int f = r / FH, t = r - f * FH;
+        const int dn = f >> 1, high = f & 1;
+        const int m = high ? NC : 1;
+        int t1, t2;
+        if (DIM == 2) {
+            t1 = 2 * t + 1 + ((rb + m + 1) & 1);
+            t2 = 0;
+        } else {
+            t2 = t / (NC / 2) + 1;
+            t1 = 2 * (t % (NC / 2)) + 1 + ((rb + m + t2 + 1) & 1);
+        }
+        const int i = dn == 0 ? m : t1;
+        const int j = dn == 1 ? m : (dn == 0 ? t1 : t2);
+        const int k = DIM == 3 ? (dn == 2 ? m : t2) : 0;
+        const int gi = dn == 0 ? (high ? NC + 1 : 0) : i;
+        const int gj = dn == 1 ? (high ? NC + 1 : 0) : j;
+        const int gk = dn == 2 ? (high ? NC + 1 : 0) : k;
+        const int gq = gi + S * (gj + S * gk);
+        const int slot = L.slot0 + box0 + b;
+        const size_t base = v.off(slot);
+        double g;
+        if (FLY)
+            g = ghost_value<DIM>(T, v, v, L.pnc, L.dr, slot, f, NC, i, j, k, v.phi[base + i + S * (j + S * k)]);
+        else
+            g = v.phi[base + gq];
+        lds[b * G::LEN + gq] = g;
+    }
+}
+
+template <int DIM, int NC>
+AFG_DEV void tile_stage_b(const DevTree &T, const LevelArgs &L, const LevelView &v,
+                          const double *lds, int box0, int op_auto, int rb, int tid, const double *rhs)
+{
+    using G = GsrbTile<DIM, NC>;
+    constexpr int S = G::S;
+#pragma unroll
+    for (int r = 0; r < G::PER; r++) {
+        const int e = tid + r * G::THREADS;
+        const int b = e / G::HALF;
+        if (e >= G::BOXES * G::HALF || box0 + b >= L.n) break;
+        int i, j, k;
+        colour_cell<DIM>(e - b * G::HALF, NC, rb, i, j, k);
+        const int q = i + S * (j + S * k);
+        const double *x = lds + b * G::LEN + q;
+        double u[6];
+        u[0] = x[-1];
+        u[1] = x[1];
+        u[2] = x[-S];
+        u[3] = x[S];
+        u[4] = DIM == 3 ? x[-S * S] : 0.0;
+        u[5] = DIM == 3 ? x[S * S] : 0.0;
+        const int slot = L.slot0 + box0 + b;
+        const size_t base = v.off(slot);
+        const bool lpld = lpld_box(T, op_auto, slot);
+        v.phi[base + q] = gsrb_value<DIM>(u, rhs[r], lpld ? v.eps + base : nullptr, q, S, L.dx2, lpld);
     }
 }
 
@@ -616,13 +754,83 @@ AFG_DEV void nc_dispatch(int nc, F &&f)
     }
 }
 
-// The whole bottom part for thread tid of nt; `lds` is the workgroup's
-// dynamic LDS and `sync()` the workgroup barrier.
-template <int DIM, typename Sync>
-AFG_DEV void bottom_run(const DevTree &Tg, const MgVars &V, const BottomArgs &A, double *lds,
-                        int tid, int nt, Sync sync)
+// Stages of the bottom part, in order (mg$D_fas_vcycle, m_aX_mg.f90:218-241):
+// per level li from hl down to 1: a relaxation block of n_down sweeps, then
+// update_coarse (child part, then parent part with the parent's ghost fill);
+// at level 0: a block of n_base sweeps; per level li from 1 up to hl:
+// correct_children (correction, then tmp = phi - tmp on the parents), the
+// ghost fill and a block of n_up sweeps.  A block of n sweeps is the first
+// colour pass, then (ghost fill, colour pass) for passes 2..2n, then the
+// ghost fill: 4n steps.  Filling the ghosts before a pass gives the face
+// neighbours that the reference's fill after the previous pass stores, so
+// every pass reads them from memory; the ghost layer after the block is the
+// reference's, the last fill covering every face.  At level 0 the block is
+// one step for all 2n passes (from the cell list, computing the face
+// neighbours) and the ghost fill.
+enum BottomOp { BOP_GSRB, BOP_FILL, BOP_UC_CHILD, BOP_UC_PARENT, BOP_CORRECT, BOP_DIFF, BOP_BASE };
+
+struct BottomStep {
+    int op, li, rb;
+};
+
+// step r of a relaxation block of 4n steps
+AFG_HD inline BottomStep relax_step(int li, int r, int n)
 {
-    const BottomLevels &B = *A.levels;
+    if (r == 0) return {BOP_GSRB, li, 1};
+    if (r == 4 * n - 1 || (r & 1)) return {BOP_FILL, li, 0};
+    return {BOP_GSRB, li, r / 2 + 1};
+}
+
+// Step k of the sequence above; every block of steps has a fixed length.
+AFG_HD inline BottomStep bottom_step(const BottomArgs &A, int k)
+{
+    const int nd = 4 * A.n_down + 2;
+    const int nb = A.n_base > 0 ? 2 : 0;
+    const int nu = 3 + 4 * A.n_up;
+    if (k < A.hl * nd) {
+        const int li = A.hl - k / nd, r = k % nd;
+        if (r < 4 * A.n_down) return relax_step(li, r, A.n_down);
+        return {r == 4 * A.n_down ? BOP_UC_CHILD : BOP_UC_PARENT, li, 0};
+    }
+    k -= A.hl * nd;
+    if (k < nb) return {k == 0 ? BOP_BASE : BOP_FILL, 0, 0};
+    k -= nb;
+    const int li = 1 + k / nu, r = k % nu;
+    if (r == 0) return {BOP_CORRECT, li, 0};
+    if (r == 1) return {BOP_DIFF, li, 0};
+    if (r == 2) return {BOP_FILL, li, 0};
+    return relax_step(li, r - 3, A.n_up);
+}
+
+AFG_HD inline int bottom_steps(const BottomArgs &A)
+{
+    return A.hl * (4 * A.n_down + 2) + (A.n_base > 0 ? 2 : 0) + A.hl * (3 + 4 * A.n_up);
+}
+
+// cell counts with their own code in the bottom kernel (others: generic)
+template <typename F>
+AFG_DEV void bottom_nc_dispatch(int nc, F &&f)
+{
+    switch (nc) {
+    case 2: f(IC<2>()); break;
+    case 4: f(IC<4>()); break;
+    case 8: f(IC<8>()); break;
+    default: f(IC<0>()); break;
+    }
+}
+
+// The whole bottom part for thread tid of nt; `lds` is the workgroup's
+// dynamic LDS and `sync()` the workgroup barrier; AUTO is the operator
+// choice of the options (mg$D_auto_op with eps, or mg$D_box_lpl).  Every thread walks the
+// same step sequence.  Steps on levels 0..wave_top run in threads
+// [0, wave) alone, one wavefront, and are ordered by `wsync()` among
+// themselves; the other threads pass them by and wait at the next workgroup
+// barrier.
+template <int DIM, int AUTO, typename Sync, typename WaveSync>
+AFG_DEV void bottom_run(const DevTree &Tg, const MgVars &V, const BottomLevels &B,
+                        const BottomArgs &A, double *lds, int tid, int nt, int wave, Sync sync,
+                        WaveSync wsync)
+{
     constexpr int NNB = 2 * DIM;
     // topology tables of the bottom slots, in LDS
     char *tb = (char *)lds;
@@ -631,7 +839,8 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVars &V, const BottomArgs &A,
     int32_t *t_co = t_par + B.n_slots;
     int32_t *t_bct = t_co + B.n_slots;
     int32_t *t_bcv = t_bct + B.n_bc;
-    int8_t *t_tag = (int8_t *)(t_bcv + B.n_bc);
+    int32_t *t_list = t_bcv + B.n_bc;
+    int8_t *t_tag = (int8_t *)(t_list + B.n_par);
     int8_t *t_isp = t_tag + B.n_slots;
     for (int e = tid; e < B.n_slots * NNB; e += nt) t_nb[e] = Tg.nb[e];
     for (int e = tid; e < B.n_slots; e += nt) {
@@ -644,6 +853,8 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVars &V, const BottomArgs &A,
         t_bct[e] = Tg.bc_type[e];
         t_bcv[e] = Tg.bc_voff[e];
     }
+    for (int li = 0; li <= A.hl; li++)
+        for (int e = tid; e < B.par[li].n; e += nt) t_list[B.par_off[li] + e] = B.par[li].slots[e];
     DevTree T = Tg;
     T.nb = t_nb;
     T.parent = t_par;
@@ -673,7 +884,7 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVars &V, const BottomArgs &A,
         copy_in(V.phi, B.lds_off[0][li], li);
         copy_in(V.rhs, B.lds_off[1][li], li);
         copy_in(V.tmp, B.lds_off[2][li], li);
-        if (V.op_auto) copy_in(V.eps, B.lds_off[3][li], li);
+        if (AUTO) copy_in(V.eps, B.lds_off[3][li], li);
     }
     sync();
 
@@ -683,7 +894,7 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVars &V, const BottomArgs &A,
             w.phi = data + B.lds_off[0][li];
             w.rhs = data + B.lds_off[1][li];
             w.tmp = data + B.lds_off[2][li];
-            w.eps = V.op_auto ? data + B.lds_off[3][li] : nullptr;
+            w.eps = AUTO ? data + B.lds_off[3][li] : nullptr;
             w.slot0 = B.lv[li].slot0;
             w.stride = B.box_len[li];
         } else {
@@ -691,78 +902,117 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVars &V, const BottomArgs &A,
         }
         return w;
     };
-    auto fill = [&](int li) {
-        const LevelArgs &L = B.lv[li];
+    // The steps of one cell count are compiled together (dispatch on the
+    // count first, then on the operation): a level's steps then run from one
+    // compact range of code, which stays in the instruction cache.
+    // One step on threads t of n; `in_wave` picks the barrier that orders the
+    // passes of BOP_BASE.
+    auto exec = [&](const BottomStep st, int t, int n, bool in_wave) {
+        const int li = st.li;
         const LevelView own = view(li), par = view(li > 0 ? li - 1 : 0);
-        nc_dispatch(L.nc, [&](auto ncv) {
+        const bool on_parent = st.op == BOP_UC_PARENT || st.op == BOP_DIFF;
+        bottom_nc_dispatch(B.lv[on_parent ? li - 1 : li].nc, [&](auto ncv) {
             constexpr int NC = decltype(ncv)::value;
-            const int tot = fill_threads<DIM>(L.n, L.nc);
-            for (int g = tid; g < tot; g += nt) fill_body<DIM, NC>(T, L, own, par, g);
+            switch (st.op) {
+            case BOP_BASE: {
+                // 2 n_base colour passes of gsrb_boxes on level 0: the first
+                // reads the stored ghosts, the others the neighbours the ghost
+                // fill after the previous pass would store (cells of the other
+                // colour, or K + c1 phi at physical boundaries)
+                const LevelArgs L = B.lv[0];
+                const int s = (NC ? NC : L.nc) + 2;
+                const int dq[6] = {-1, 1, -s, s, -s * s, s * s};
+                auto relax = [&](const BaseCell &d, int rb) {
+                    const double center = own.phi[d.pos];
+                    double u[6];
+#pragma unroll
+                    for (int m = 0; m < 2 * DIM; m++) {
+                        const int f = d.fly[m];
+                        if (rb == 1 || f >= 0)
+                            u[m] = own.phi[rb == 1 ? d.pos + dq[m] : f];
+                        else
+                            u[m] = d.K[m] + (f == -1 ? -center : center);
+                    }
+                    if (DIM == 2) u[4] = u[5] = 0;
+                    const bool lpld = lpld_box(T, AUTO, d.slot);
+                    own.phi[d.pos] = gsrb_value<DIM>(u, own.rhs[d.pos], lpld ? own.eps + d.pos : nullptr,
+                                                     0, s, L.dx2, lpld);
+                };
+                auto sy = [&] {
+                    if (in_wave)
+                        wsync();
+                    else
+                        sync();
+                };
+                const int n1 = B.base_n[0], n2 = B.base_n[1];
+                if (n1 <= n && n2 <= n) {   // one cell of each colour per thread, kept in registers
+                    BaseCell d1, d2;
+                    if (t < n1) d1 = B.base[t];
+                    if (t < n2) d2 = B.base[n1 + t];
+                    for (int rb = 1; rb <= 2 * A.n_base; rb++) {
+                        if (rb & 1) {
+                            if (t < n1) relax(d1, rb);
+                        } else {
+                            if (t < n2) relax(d2, rb);
+                        }
+                        if (rb < 2 * A.n_base) sy();
+                    }
+                } else {
+                    for (int rb = 1; rb <= 2 * A.n_base; rb++) {
+                        const int c0 = rb & 1 ? 0 : n1, nn = rb & 1 ? n1 : n2;
+                        for (int c = t; c < nn; c += n) relax(B.base[c0 + c], rb);
+                        if (rb < 2 * A.n_base) sy();
+                    }
+                }
+            } break;
+            case BOP_GSRB: {
+                const LevelArgs L = B.lv[li];
+                const int tot = gsrb_threads<DIM>(L.n, NC ? NC : L.nc);
+                for (int g = t; g < tot; g += n) gsrb_body<DIM, NC>(T, L, own, par, AUTO, st.rb, 0, g);
+            } break;
+            case BOP_FILL: {
+                const LevelArgs L = B.lv[li];
+                const int tot = fill_threads<DIM>(L.n, NC ? NC : L.nc);
+                for (int g = t; g < tot; g += n) fill_body<DIM, NC>(T, L, own, par, g);
+            } break;
+            case BOP_UC_CHILD: {
+                const LevelArgs L = B.lv[li];
+                const int tot = L.n * ipow((NC ? NC : L.nc) / 2, DIM);
+                for (int g = t; g < tot; g += n) uc_child_body<DIM, NC>(T, L, own, par, AUTO, 1, 0, g);
+            } break;
+            case BOP_UC_PARENT: {   // parent part on level li - 1
+                const LevelArgs Lp = B.lv[li - 1];
+                const LevelView ppar = view(li > 1 ? li - 2 : 0);
+                const int tot = Lp.n * ipow((NC ? NC : Lp.nc) + 2, DIM);
+                for (int g = t; g < tot; g += n) uc_parent_body<DIM, NC>(T, Lp, par, ppar, AUTO, 1, g);
+            } break;
+            case BOP_CORRECT: {
+                const LevelArgs L = B.lv[li];
+                const int tot = L.n * ipow((NC ? NC : L.nc) / 2, DIM);
+                for (int g = t; g < tot; g += n) correct_body<DIM, NC>(T, L, own, par, AUTO, 1, g);
+            } break;
+            default: {   // BOP_DIFF: parents of level li, on level li - 1
+                ListArgs P = B.par[li - 1];
+                P.slots = t_list + B.par_off[li - 1];
+                const int tot = P.n * ipow((NC ? NC : P.nc) + 2, DIM);
+                for (int g = t; g < tot; g += n) diff_body<DIM, NC>(T, P, par, g);
+            } break;
+            }
         });
-        sync();
-    };
-    auto gsrb = [&](int li, int n_cycle) {
-        const LevelArgs &L = B.lv[li];
-        const LevelView own = view(li), par = view(li > 0 ? li - 1 : 0);
-        for (int rb = 1; rb <= 2 * n_cycle; rb++) {
-            nc_dispatch(L.nc, [&](auto ncv) {
-                constexpr int NC = decltype(ncv)::value;
-                const int tot = gsrb_threads<DIM>(L.n, L.nc);
-                for (int g = tid; g < tot; g += nt)
-                    gsrb_body<DIM, NC>(T, L, own, par, V.op_auto, rb, rb == 1 ? 0 : 1, g);
-            });
-            sync();
-        }
-        if (n_cycle > 0) fill(li);
-    };
-    auto update_coarse = [&](int li) {
-        const LevelArgs &L = B.lv[li];
-        const LevelView own = view(li), par = view(li - 1);
-        nc_dispatch(L.nc, [&](auto ncv) {
-            constexpr int NC = decltype(ncv)::value;
-            const int tot = L.n * ipow(L.nc / 2, DIM);
-            for (int g = tid; g < tot; g += nt)
-                uc_child_body<DIM, NC>(T, L, own, par, V.op_auto, 1, 0, g);
-        });
-        sync();
-        const LevelArgs &Lp = B.lv[li - 1];
-        const LevelView ppar = view(li > 1 ? li - 2 : 0);
-        nc_dispatch(Lp.nc, [&](auto ncv) {
-            constexpr int NC = decltype(ncv)::value;
-            const int tot = Lp.n * ipow(Lp.nc + 2, DIM);
-            for (int g = tid; g < tot; g += nt)
-                uc_parent_body<DIM, NC>(T, Lp, par, ppar, V.op_auto, 1, g);
-        });
-        sync();
-    };
-    auto correct_children = [&](int li) {
-        const LevelArgs &L = B.lv[li];
-        const LevelView own = view(li), par = view(li - 1);
-        nc_dispatch(L.nc, [&](auto ncv) {
-            constexpr int NC = decltype(ncv)::value;
-            const int tot = L.n * ipow(L.nc / 2, DIM);
-            for (int g = tid; g < tot; g += nt)
-                correct_body<DIM, NC>(T, L, own, par, V.op_auto, 1, g);
-        });
-        sync();
-        const ListArgs &P = B.par[li - 1];
-        nc_dispatch(P.nc, [&](auto ncv) {
-            constexpr int NC = decltype(ncv)::value;
-            const int tot = P.n * ipow(P.nc + 2, DIM);
-            for (int g = tid; g < tot; g += nt) diff_body<DIM, NC>(T, P, par, g);
-        });
-        sync();
     };
 
-    for (int li = A.hl; li >= 1; li--) {
-        gsrb(li, A.n_down);
-        update_coarse(li);
-    }
-    gsrb(0, A.n_base);
-    for (int li = 1; li <= A.hl; li++) {
-        correct_children(li);
-        fill(li);
-        gsrb(li, A.n_up);
+    const int W = A.hl < B.wave_top ? A.hl : B.wave_top;
+    const int n_steps = bottom_steps(A);
+    BottomStep st = bottom_step(A, 0);
+    for (int k = 0; k < n_steps; k++) {
+        const bool in_wave = st.li <= W;
+        if (!in_wave || tid < wave) exec(st, tid, in_wave ? wave : nt, in_wave);
+        const BottomStep nx = k + 1 < n_steps ? bottom_step(A, k + 1) : BottomStep{0, A.hl + 1, 0};
+        if (in_wave && nx.li <= W)
+            wsync();
+        else
+            sync();
+        st = nx;
     }
 
     for (int li = 0; li <= A.hl; li++) {
@@ -776,10 +1026,9 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVars &V, const BottomArgs &A,
 // residual on the small levels 0..hl in one launch (set_residual part of
 // mg$D_fas_vcycle, m_aX_mg.f90:243-254), on the HBM pools
 template <int DIM>
-AFG_DEV void bottom_residual_run(const DevTree &T, const MgVars &V, const BottomArgs &A, int tid,
-                                 int nt)
+AFG_DEV void bottom_residual_run(const DevTree &T, const MgVars &V, const BottomLevels &B,
+                                 const BottomArgs &A, int tid, int nt)
 {
-    const BottomLevels &B = *A.levels;
     const LevelView v = LevelView{V.phi, V.rhs, V.tmp, V.eps, 0, T.stride};
     for (int li = 0; li <= A.hl; li++) {
         const LevelArgs &L = B.lv[li];

@@ -5,7 +5,7 @@
 // mg$D_fas_vcycle :208-255 and the stages they call) stage by stage; every
 // stage is one or a few kernel launches over all boxes of a level.
 
-#include "../../include/afivo_amd.h"
+#include "afivo_amd.h"
 #include "afg_internal.h"
 
 #include <algorithm>
@@ -106,7 +106,10 @@ struct afg_ctx {
     double *d_bc_vals = nullptr;
     size_t bc_vals_cap = 0, bc_cap = 0;
     int8_t *d_tag = nullptr;
-    BottomLevels *d_bottom[2] = {nullptr, nullptr};   // 3 and 4 variables in LDS
+    BottomLevels *d_bottom[2] = {nullptr, nullptr};   // bottom kernel tables: 3 / 4 variables in LDS
+    BottomLevels h_bottom[2];                         // their host copies
+    BaseCell *d_base[2] = {nullptr, nullptr};         // lowest-level cell lists of the two
+    int base_cap = 0;
     int8_t *d_is_parent = nullptr;
     std::vector<double *> d_var;
     double *d_partial = nullptr;
@@ -121,6 +124,7 @@ struct afg_ctx {
     int bottom_threads = BOTTOM_BLOCK;   // its workgroup size
     int64_t lds_limit = 65536;           // LDS bytes one workgroup may use
     int64_t bottom_cells = 8192;        // cell limit per level for the bottom kernel
+    int64_t bottom_wave_cells = 512;    // ... and for its one-wavefront levels
     bool use_graphs = true;
     std::map<std::tuple<int, int, int, int, int, int, int, int, int, int>, GraphRec> graphs;
     bool capturing = false;
@@ -211,11 +215,15 @@ void free_tree(afg_ctx *c)
     hipFree(c->d_bc_voff);
     hipFree(c->d_bc_vals);
     hipFree(c->d_tag);
+    hipFree(c->d_is_parent);
+    c->d_is_parent = nullptr;
     hipFree(c->d_bottom[0]);
     hipFree(c->d_bottom[1]);
     c->d_bottom[0] = c->d_bottom[1] = nullptr;
-    hipFree(c->d_is_parent);
-    c->d_is_parent = nullptr;
+    hipFree(c->d_base[0]);
+    hipFree(c->d_base[1]);
+    c->d_base[0] = c->d_base[1] = nullptr;
+    c->base_cap = 0;
     hipFree(c->d_partial);
     for (double *p : c->d_var) hipFree(p);
     c->d_nb = c->d_parent = c->d_choff = c->d_lists = c->d_bc_type = c->d_bc_voff = nullptr;
@@ -262,6 +270,8 @@ int ensure_var(afg_ctx *c, int iv, double **out)
     return 0;
 }
 
+int build_base(afg_ctx *c);
+
 int upload_bc(afg_ctx *c)
 {
     if (!c->bc_dirty) return 0;
@@ -297,7 +307,7 @@ int upload_bc(afg_ctx *c)
                                   hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->bc_dirty = false;
-    return 0;
+    return c->have_tree ? build_base(c) : 0;   // the boundary values enter the base cell lists
 }
 
 int ensure_stage(afg_ctx *c, size_t n_doubles)
@@ -484,7 +494,6 @@ BottomArgs bottom_args(Mg &m, int top)
 {
     afg_ctx *c = m.c;
     BottomArgs B;
-    B.levels = c->d_bottom[m.V.op_auto ? 1 : 0];
     B.hl = top - c->min_lvl;
     B.n_down = m.o->n_cycle_down;
     B.n_up = m.o->n_cycle_up;
@@ -507,15 +516,16 @@ void vcycle_body(Mg &m, int highest_lvl, int set_residual)
         update_coarse(m, lvl, m.o->n_cycle_down > 0);
     }
     if (lb >= c->min_lvl) {
-        launch_bottom(c->dim, c->stream, m.T, m.V, bottom_args(m, lb), c->bottom_lds[m.V.op_auto ? 1 : 0],
-                      c->bottom_threads);
+        launch_bottom(c->dim, c->stream, m.T, m.V, c->d_bottom[m.V.op_auto ? 1 : 0], bottom_args(m, lb),
+                      c->bottom_lds[m.V.op_auto ? 1 : 0], c->bottom_threads);
     } else {
         gsrb_level(m, c->min_lvl, m.o->n_cycle_base);
     }
     for (int lvl = low + 1; lvl <= highest_lvl; lvl++) correct_relax(m, lvl, m.o->n_cycle_up);
     if (set_residual) {
         if (lb >= c->min_lvl) {
-            launch_bottom_residual(c->dim, c->stream, m.T, m.V, bottom_args(m, lb), c->bottom_threads);
+            launch_bottom_residual(c->dim, c->stream, m.T, m.V, c->d_bottom[m.V.op_auto ? 1 : 0],
+                                   bottom_args(m, lb), c->bottom_threads);
             for (int lvl = lb + 1; lvl <= highest_lvl; lvl++) residual_level(m, lvl);
         } else {
             for (int lvl = c->min_lvl; lvl <= highest_lvl; lvl++) residual_level(m, lvl);
@@ -586,7 +596,8 @@ int check_level(afg_ctx *c, int lvl, bool need_parent)
 // bottom_cells cells.  Their topology tables always go to its LDS; the
 // lowest of them whose phi, rhs, tmp (and eps, for the auto operators) also
 // fit there are copied in as well, the others are used in place in the HBM
-// pools.  Two tables (3 and 4 variables) go to device memory.
+// pools.  Levels of at most bottom_wave_cells cells each, from the lowest
+// up, run in one wavefront.  Two tables are kept: 3 and 4 variables in LDS.
 int choose_bottom(afg_ctx *c)
 {
     const int nl = (int)c->levels.size();
@@ -600,26 +611,38 @@ int choose_bottom(afg_ctx *c)
         max_threads = std::max<int64_t>(max_threads, (int64_t)L.n * box_len(c->dim, L.nc) / 2);
     }
     c->bottom_threads = (int)std::min<int64_t>(BOTTOM_BLOCK, (max_threads + 63) / 64 * 64);
-    int n_slots = 0, n_bc = 0;
+    int n_slots = 0, n_bc = 0, n_par = 0, wave_top = -1;
     if (top >= 0) {
         n_slots = c->levels[top].slot0 + c->levels[top].n;
         for (const BcRec &r : c->bc) n_bc += r.slot < n_slots;
     }
+    for (int li = 0; li <= top; li++) {
+        const Level &L = c->levels[li];
+        n_par += list_args(c, L.lvl, 0).n;
+        if ((int64_t)L.n * (c->dim == 2 ? L.nc * L.nc : L.nc * L.nc * L.nc) <= c->bottom_wave_cells &&
+            wave_top == li - 1)
+            wave_top = li;
+    }
     const int table_bytes =
-        (int)((((int64_t)n_slots * (c->nnb + 2) + 2 * n_bc) * 4 + 2 * n_slots + 7) / 8 * 8);
+        (int)((((int64_t)n_slots * (c->nnb + 2) + 2 * n_bc + n_par) * 4 + 2 * n_slots + 7) / 8 * 8);
     for (int k = 0; k < 2; k++) {
         const int nv = 3 + k;
         BottomLevels h;
         std::memset(&h, 0, sizeof h);
         h.n_slots = n_slots;
         h.n_bc = n_bc;
+        h.n_par = n_par;
         h.table_bytes = table_bytes;
+        h.wave_top = wave_top;
         int64_t per_var = 0;   // doubles of one variable over the levels in LDS
+        int par_off = 0;
         bool fits = true;
         for (int li = 0; li <= top; li++) {
             const Level &L = c->levels[li];
             h.lv[li] = level_args(c, L.lvl);
             h.par[li] = list_args(c, L.lvl, 0);
+            h.par_off[li] = par_off;
+            par_off += h.par[li].n;
             h.box_len[li] = box_len(c->dim, L.nc);
             const int64_t add = (int64_t)L.n * h.box_len[li];
             fits = fits && table_bytes + (per_var + add) * nv * (int64_t)sizeof(double) <= c->lds_limit;
@@ -636,13 +659,97 @@ int choose_bottom(afg_ctx *c)
         }
         c->bottom_lvl[k] = top >= 0 ? c->levels[top].lvl : -1000;
         c->bottom_lds[k] = (size_t)table_bytes + (size_t)(per_var * nv * sizeof(double));
-        if (top >= 0 && bottom_prepare(c->dim, c->bottom_lds[k]) != 0)
-            return c->fail(AFG_ERR_HIP, "cannot allow %zu bytes of LDS for the bottom kernel",
-                           c->bottom_lds[k]);
-        HIP_TRY(c, hipMemcpy(c->d_bottom[k], &h, sizeof h, hipMemcpyHostToDevice));
+        c->h_bottom[k] = h;
     }
+    const size_t lds_max = std::max(c->bottom_lds[0], c->bottom_lds[1]);
+    if (top >= 0 && bottom_prepare(c->dim, lds_max) != 0)
+        return c->fail(AFG_ERR_HIP, "cannot allow %zu bytes of LDS for the bottom kernel", lds_max);
     for (auto &g : c->graphs) destroy_graph(g.second);
     c->graphs.clear();
+    return build_base(c);
+}
+
+// The cell lists of the lowest level for the bottom kernel (BaseCell), for
+// both tables, and the tables' upload.  Positions follow where the table
+// keeps level 0: its LDS copy or the HBM pool.
+int build_base(afg_ctx *c)
+{
+    const int dim = c->dim, nnb = c->nnb;
+    const Level &L0 = c->levels[0];
+    const int nc = L0.nc, s = nc + 2;
+    const int ncells = L0.n * (dim == 2 ? nc * nc : nc * nc * nc);
+    if (ncells > c->base_cap) {
+        for (int k = 0; k < 2; k++) {
+            hipFree(c->d_base[k]);
+            c->d_base[k] = nullptr;
+            HIP_TRY(c, hipMalloc(&c->d_base[k], (size_t)ncells * sizeof(BaseCell)));
+        }
+        c->base_cap = ncells;
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 2; k++) {
+        BottomLevels &h = c->h_bottom[k];
+        if (c->bottom_lvl[k] < c->min_lvl) {
+            h.base = nullptr;
+            h.base_n[0] = h.base_n[1] = 0;
+            HIP_TRY(c, hipMemcpy(c->d_bottom[k], &h, sizeof h, hipMemcpyHostToDevice));
+            continue;
+        }
+        const bool lds = h.in_lds[0] != 0;
+        const int vstride = lds ? h.box_len[0] : c->stride, vslot0 = lds ? L0.slot0 : 0;
+        std::vector<BaseCell> cells[2];
+        for (int slot = L0.slot0; slot < L0.slot0 + L0.n; slot++) {
+            for (int kk = dim == 3 ? 1 : 0; kk <= (dim == 3 ? nc : 0); kk++)
+                for (int j = 1; j <= nc; j++)
+                    for (int i = 1; i <= nc; i++) {
+                        const int x[3] = {i, j, kk};
+                        BaseCell d;
+                        std::memset(&d, 0, sizeof d);
+                        d.slot = slot;
+                        d.pos = (slot - vslot0) * vstride + i + s * (j + s * kk);
+                        const int dq[6] = {-1, 1, -s, s, -s * s, s * s};
+                        for (int m = 0; m < nnb; m++) {
+                            const int dn = m >> 1, high = m & 1;
+                            if (x[dn] != (high ? nc : 1)) {   // neighbour inside the box
+                                d.fly[m] = d.pos + dq[m];
+                                continue;
+                            }
+                            const int nid = c->h_nb[(size_t)slot * nnb + m];
+                            if (nid > 0) {   // sides_from_nb: the neighbour's mirrored cell
+                                int y[3] = {i, j, kk};
+                                y[dn] = high ? 1 : nc;
+                                const int ns = c->id2slot[nid];
+                                d.fly[m] = (ns - vslot0) * vstride + y[0] + s * (y[1] + s * y[2]);
+                            } else if (nid < 0) {   // bc_to_gc with the captured value b
+                                const BcRec &r = c->bc[c->bc_of[(size_t)slot * nnb + m]];
+                                double b = 0.0;
+                                if (r.voff >= 0) {
+                                    const int t1 = dn == 0 ? j : i, t2 = dn == 2 ? j : kk;
+                                    b = c->bc_vals[r.voff + (t1 - 1) + (dim == 3 ? nc * (t2 - 1) : 0)];
+                                }
+                                if (r.type == AFG_BC_DIRICHLET) {
+                                    d.fly[m] = -1;
+                                    d.K[m] = 2.0 * b;
+                                } else {
+                                    const double c2 = L0.dr * (high ? 1.0 : -1.0);
+                                    d.fly[m] = -2;
+                                    d.K[m] = c2 * b;
+                                }
+                            } else {
+                                return c->fail(AFG_ERR_ARG, "refinement boundary on the lowest level");
+                            }
+                        }
+                        cells[(i + j + kk) & 1 ? 0 : 1].push_back(d);
+                    }
+        }
+        std::vector<BaseCell> all(cells[0]);
+        all.insert(all.end(), cells[1].begin(), cells[1].end());
+        HIP_TRY(c, hipMemcpy(c->d_base[k], all.data(), all.size() * sizeof(BaseCell), hipMemcpyHostToDevice));
+        h.base = c->d_base[k];
+        h.base_n[0] = (int)cells[0].size();
+        h.base_n[1] = (int)cells[1].size();
+        HIP_TRY(c, hipMemcpy(c->d_bottom[k], &h, sizeof h, hipMemcpyHostToDevice));
+    }
     return 0;
 }
 
@@ -916,8 +1023,6 @@ int afg_set_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base)
         maxp += extreme_blocks(dim, a);
     }
     c->partial_cap = std::max(maxp, 1);
-    HIP_TRY(c, hipMalloc(&c->d_bottom[0], sizeof(BottomLevels)));
-    HIP_TRY(c, hipMalloc(&c->d_bottom[1], sizeof(BottomLevels)));
     {
         std::vector<int8_t> isp(sn, 0);
         for (int s2 = 0; s2 < slot; s2++) {
@@ -928,6 +1033,8 @@ int afg_set_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base)
         HIP_TRY(c, hipMemcpy(c->d_is_parent, isp.data(), sn, hipMemcpyHostToDevice));
     }
     c->have_tree = true;
+    HIP_TRY(c, hipMalloc(&c->d_bottom[0], sizeof(BottomLevels)));
+    HIP_TRY(c, hipMalloc(&c->d_bottom[1], sizeof(BottomLevels)));
     if (int e = choose_bottom(c)) return e;
     HIP_TRY(c, hipMalloc(&c->d_partial, c->partial_cap * sizeof(double)));
     c->bc_dirty = true;
@@ -1249,12 +1356,13 @@ int afg_update_coarse(afg_ctx *c, const afg_mg_opts *o, int lvl)
     return 0;
 }
 
-int afg_set_bottom_cells(afg_ctx *c, int64_t cells)
+int afg_set_bottom_cells(afg_ctx *c, int64_t cells, int64_t wave_cells)
 {
     if (int e = check_ctx(c)) return e;
-    if (cells < 0) return c->fail(AFG_ERR_ARG, "bottom cell limit must be >= 0");
+    if (cells < 0 || wave_cells < 0) return c->fail(AFG_ERR_ARG, "bottom cell limits must be >= 0");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->bottom_cells = cells;
+    c->bottom_wave_cells = wave_cells;
     return choose_bottom(c);
 }
 

@@ -45,9 +45,9 @@ void launch_reduce_partials(hipStream_t s, const double *partial, int n, int wan
                             double *out);
 // bottom kernel: allow lds_bytes of dynamic LDS (0 on success), then launch
 int bottom_prepare(int dim, size_t lds_bytes);
-void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVars &V, const BottomArgs &B,
-                   size_t lds_bytes, int threads);
+void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVars &V,
+                   const BottomLevels *B, const BottomArgs &A, size_t lds_bytes, int threads);
 void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVars &V,
-                            const BottomArgs &B, int threads);
+                            const BottomLevels *B, const BottomArgs &A, int threads);
 
 }  // namespace afg

@@ -69,6 +69,21 @@ __global__ __launch_bounds__(BLOCK) void k_gsrb_pass(DevTree T, LevelArgs L, Lev
         gsrb_body<DIM, NC>(T, L, v, v, op_auto, rb, fly, gid);
 }
 
+// one colour pass by tiles of boxes in LDS (NC in 2, 4, 8, 16)
+template <int DIM, int NC, int FLY>
+__global__ __launch_bounds__(BLOCK) void k_gsrb_tile(DevTree T, LevelArgs L, LevelView v,
+                                                     int op_auto, int rb)
+{
+    using G = GsrbTile<DIM, NC>;
+    static_assert(G::THREADS == BLOCK, "tile size");
+    __shared__ double lds[G::BOXES * G::LEN];
+    double rhs[G::PER];
+    const int box0 = xcd_tile(blockIdx.x, gridDim.x) * G::BOXES;
+    tile_stage_a<DIM, NC, FLY>(T, L, v, lds, box0, rb, threadIdx.x, rhs);
+    __syncthreads();
+    tile_stage_b<DIM, NC>(T, L, v, lds, box0, op_auto, rb, threadIdx.x, rhs);
+}
+
 template <int DIM, int NC>
 __global__ __launch_bounds__(BLOCK) void k_residual(DevTree T, LevelArgs L, LevelView v, int op_auto)
 {
@@ -229,17 +244,29 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_partials(const double *partial
 // bottom of the V-cycle in one workgroup, on LDS copies of the small levels
 // (bottom_run), and the residual of those levels (bottom_residual_run)
 
-template <int DIM>
-__global__ __launch_bounds__(BOTTOM_BLOCK) void k_bottom(DevTree T, MgVars V, BottomArgs A)
+template <int DIM, int AUTO>
+__global__ __launch_bounds__(BOTTOM_BLOCK) void k_bottom(DevTree T, MgVars V,
+                                                          const BottomLevels *__restrict__ B,
+                                                          BottomArgs A)
 {
     extern __shared__ double lds[];
-    bottom_run<DIM>(T, V, A, lds, threadIdx.x, blockDim.x, [] { __syncthreads(); });
+    // lanes of one wavefront see each other's LDS and global accesses in
+    // program order; the fences only keep the compiler from moving them
+    const auto wave_sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    bottom_run<DIM, AUTO>(T, V, *B, A, lds, threadIdx.x, blockDim.x, warpSize, [] { __syncthreads(); },
+                          wave_sync);
 }
 
 template <int DIM>
-__global__ __launch_bounds__(BOTTOM_BLOCK) void k_bottom_residual(DevTree T, MgVars V, BottomArgs A)
+__global__ __launch_bounds__(BOTTOM_BLOCK) void k_bottom_residual(DevTree T, MgVars V,
+                                                                   const BottomLevels *__restrict__ B,
+                                                                   BottomArgs A)
 {
-    bottom_residual_run<DIM>(T, V, A, threadIdx.x, blockDim.x);
+    bottom_residual_run<DIM>(T, V, *B, A, threadIdx.x, blockDim.x);
 }
 
 }  // namespace
@@ -291,12 +318,37 @@ void launch_diff(int dim, hipStream_t s, const DevTree &T, const ListArgs &P, co
     AFG_DISPATCH(dim, P.nc, k_diff, n_blocks(th), T, P, pool_view(T, V));
 }
 
+template <int DIM, int NC>
+void gsrb_tiles(hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVars &V, int rb, int fly)
+{
+    const int grid = (L.n + GsrbTile<DIM, NC>::BOXES - 1) / GsrbTile<DIM, NC>::BOXES;
+    if (fly)
+        k_gsrb_tile<DIM, NC, 1><<<grid, BLOCK, 0, s>>>(T, L, pool_view(T, V), V.op_auto, rb);
+    else
+        k_gsrb_tile<DIM, NC, 0><<<grid, BLOCK, 0, s>>>(T, L, pool_view(T, V), V.op_auto, rb);
+}
+
 void launch_gsrb_pass(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
                       const MgVars &V, int rb, int fly)
 {
     if (L.n <= 0) return;
+    switch (dim * 100 + L.nc) {
+    case 202: gsrb_tiles<2, 2>(s, T, L, V, rb, fly); return;
+    case 204: gsrb_tiles<2, 4>(s, T, L, V, rb, fly); return;
+    case 208: gsrb_tiles<2, 8>(s, T, L, V, rb, fly); return;
+    case 216: gsrb_tiles<2, 16>(s, T, L, V, rb, fly); return;
+    case 302: gsrb_tiles<3, 2>(s, T, L, V, rb, fly); return;
+    case 304: gsrb_tiles<3, 4>(s, T, L, V, rb, fly); return;
+    case 308: gsrb_tiles<3, 8>(s, T, L, V, rb, fly); return;
+    case 316: gsrb_tiles<3, 16>(s, T, L, V, rb, fly); return;
+    default: break;
+    }
+    // other cell counts: one thread per cell of the colour
     const long long th = (long long)L.n * ipow(L.nc, dim) / 2;
-    AFG_DISPATCH(dim, L.nc, k_gsrb_pass, n_blocks(th), T, L, pool_view(T, V), V.op_auto, rb, fly);
+    if (dim == 2)
+        k_gsrb_pass<2, 0><<<n_blocks(th), BLOCK, 0, s>>>(T, L, pool_view(T, V), V.op_auto, rb, fly);
+    else
+        k_gsrb_pass<3, 0><<<n_blocks(th), BLOCK, 0, s>>>(T, L, pool_view(T, V), V.op_auto, rb, fly);
 }
 
 void launch_residual(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
@@ -400,29 +452,37 @@ void launch_reduce_partials(hipStream_t s, const double *partial, int n, int wan
 
 int bottom_prepare(int dim, size_t lds_bytes)
 {
-    const void *fn = dim == 2 ? (const void *)k_bottom<2> : (const void *)k_bottom<3>;
-    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) ==
-                   hipSuccess
-               ? 0
-               : -1;
+    const void *fn[2] = {dim == 2 ? (const void *)k_bottom<2, 0> : (const void *)k_bottom<3, 0>,
+                         dim == 2 ? (const void *)k_bottom<2, 1> : (const void *)k_bottom<3, 1>};
+    for (const void *f : fn)
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) != hipSuccess)
+            return -1;
+    return 0;
 }
 
-void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVars &V, const BottomArgs &B,
-                   size_t lds_bytes, int threads)
+void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVars &V,
+                   const BottomLevels *B, const BottomArgs &A, size_t lds_bytes, int threads)
 {
-    if (dim == 2)
-        k_bottom<2><<<1, threads, lds_bytes, s>>>(T, V, B);
-    else
-        k_bottom<3><<<1, threads, lds_bytes, s>>>(T, V, B);
+    if (dim == 2) {
+        if (V.op_auto)
+            k_bottom<2, 1><<<1, threads, lds_bytes, s>>>(T, V, B, A);
+        else
+            k_bottom<2, 0><<<1, threads, lds_bytes, s>>>(T, V, B, A);
+    } else {
+        if (V.op_auto)
+            k_bottom<3, 1><<<1, threads, lds_bytes, s>>>(T, V, B, A);
+        else
+            k_bottom<3, 0><<<1, threads, lds_bytes, s>>>(T, V, B, A);
+    }
 }
 
 void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVars &V,
-                            const BottomArgs &B, int threads)
+                            const BottomLevels *B, const BottomArgs &A, int threads)
 {
     if (dim == 2)
-        k_bottom_residual<2><<<1, threads, 0, s>>>(T, V, B);
+        k_bottom_residual<2><<<1, threads, 0, s>>>(T, V, B, A);
     else
-        k_bottom_residual<3><<<1, threads, 0, s>>>(T, V, B);
+        k_bottom_residual<3><<<1, threads, 0, s>>>(T, V, B, A);
 }
 
 }  // namespace afg

@@ -64,23 +64,45 @@ struct MgVars {
 // where per-stage launches would cost more than the work).  Levels are
 // indexed relative to the lowest one.
 constexpr int MAX_BOTTOM_LEVELS = 12;
-constexpr int BOTTOM_BLOCK = 1024;
-// Level tables of the bottom kernel, kept in device memory (index: level
-// relative to the lowest one).  The bottom levels own slots [0, n_slots) and
-// boundary records [0, n_bc); the kernel copies their topology tables into
-// the first table_bytes of its LDS.  Levels with in_lds set are copied to LDS
-// too, after the tables: variable v (phi, rhs, tmp, eps) of level li starts
-// at lds_off[v][li] doubles there; the others stay in the HBM pools.
+constexpr int BOTTOM_BLOCK = 512;
+// One cell of the lowest level, for the relaxation there (no refinement
+// boundaries): its position in the level's array, its box slot and, per face
+// in stencil order, where a colour pass after the first reads the
+// neighbour: fly >= 0 is a position in the level's array (a cell of this or
+// a neighbouring box); fly = -1 (Dirichlet) or -2 (Neumann) is a physical
+// boundary, whose ghost is K - phi or K + phi with K = c2 * b (bc_to_gc,
+// m_aX_gc.f90:71-132).
+struct BaseCell {
+    int pos, slot;
+    int fly[6];
+    double K[6];
+};
+
+// Level tables of the bottom kernel (index: level relative to the lowest
+// one), in device memory that the kernel only reads (its own argument,
+// declared __restrict__, so that the reads are scalar loads).  The bottom
+// levels own slots [0, n_slots), boundary records [0, n_bc) and n_par
+// parent-list entries (list of level li at par_off[li]); the kernel copies
+// these topology tables
+// to the first table_bytes of its LDS.  Levels with in_lds set are copied to
+// LDS too, after the tables: variable v (phi, rhs, tmp, eps) of level li
+// starts at lds_off[v][li] doubles there; the others stay in the HBM pools.
+// Levels 0..wave_top run in the first wavefront alone, with wavefront-level
+// ordering in place of workgroup barriers (-1: none).  The relaxation of
+// level 0 runs from the cell list `base`.
 struct BottomLevels {
     LevelArgs lv[MAX_BOTTOM_LEVELS];
     ListArgs par[MAX_BOTTOM_LEVELS];   // parents of each level
     int box_len[MAX_BOTTOM_LEVELS];    // (nc+2)^dim
     int in_lds[MAX_BOTTOM_LEVELS];
     int lds_off[4][MAX_BOTTOM_LEVELS];
-    int n_slots, n_bc, table_bytes;
+    int par_off[MAX_BOTTOM_LEVELS];
+    int n_slots, n_bc, n_par, table_bytes;
+    int wave_top;
+    const BaseCell *base;   // cells of the lowest level: base_n[0] of colour 1, then colour 2
+    int base_n[2];
 };
 struct BottomArgs {
-    const BottomLevels *levels;   // device pointer
     int hl;                       // highest level of this call (relative index)
     int n_down, n_up, n_base;
 };

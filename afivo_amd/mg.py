@@ -124,8 +124,8 @@ class DeviceTree:
         self._c(_lib.load().afg_update_coarse(self.h, ctypes.byref(mg), lvl))
 
     # controls and measurements
-    def set_bottom_cells(self, cells):
-        self._c(_lib.load().afg_set_bottom_cells(self.h, int(cells)))
+    def set_bottom_cells(self, cells, wave_cells=512):
+        self._c(_lib.load().afg_set_bottom_cells(self.h, int(cells), int(wave_cells)))
 
     def set_graphs(self, enabled):
         self._c(_lib.load().afg_set_graphs(self.h, int(enabled)))

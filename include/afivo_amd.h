@@ -152,8 +152,10 @@ int afg_update_coarse(afg_ctx *ctx, const afg_mg_opts *opts, int lvl);
 
 /* The lowest levels of the tree, while each holds at most `cells` cells
  * (default 8192), run their part of every V-cycle in one single-workgroup
- * launch; 0 turns this off.  Results are identical either way. */
-int afg_set_bottom_cells(afg_ctx *ctx, int64_t cells);
+ * launch; 0 turns this off.  Within it, the lowest levels holding at most
+ * `wave_cells` cells each (default 512) run in a single wavefront.  Results
+ * are identical either way. */
+int afg_set_bottom_cells(afg_ctx *ctx, int64_t cells, int64_t wave_cells);
 /* Replay whole V-cycles from HIP graphs (default on). */
 int afg_set_graphs(afg_ctx *ctx, int enabled);
 /* Per-stage device timing: while enabled, each launch of the GSRB colour-pass

@@ -13,6 +13,7 @@
 #include "emul.h"
 
 #include <algorithm>
+#include <vector>
 #include <cfloat>
 
 namespace afg {
@@ -87,14 +88,49 @@ void launch_diff(int dim, hipStream_t s, const DevTree &T, const ListArgs &P, co
     });
 }
 
+// the tiled pass of the device: workgroups one after the other, each stage
+// over all threads of the workgroup before the next
+template <int DIM, int NC>
+void gsrb_tiles(const DevTree &T, const LevelArgs &L, const MgVars &V, int rb, int fly)
+{
+    using G = GsrbTile<DIM, NC>;
+    const LevelView v = pool_view(T, V);
+    const int grid = (L.n + G::BOXES - 1) / G::BOXES;
+    double *lds = (double *)emul_alloc(sizeof(double) * G::BOXES * G::LEN);
+    std::vector<double> rhs((size_t)G::THREADS * G::PER);
+    for (int blk = 0; blk < grid; blk++) {
+        const int box0 = xcd_tile(blk, grid) * G::BOXES;
+        for (int t = 0; t < G::THREADS; t++) {
+            if (fly)
+                tile_stage_a<DIM, NC, 1>(T, L, v, lds, box0, rb, t, &rhs[(size_t)t * G::PER]);
+            else
+                tile_stage_a<DIM, NC, 0>(T, L, v, lds, box0, rb, t, &rhs[(size_t)t * G::PER]);
+        }
+        for (int t = 0; t < G::THREADS; t++)
+            tile_stage_b<DIM, NC>(T, L, v, lds, box0, V.op_auto, rb, t, &rhs[(size_t)t * G::PER]);
+    }
+    emul_free(lds);
+}
+
 void launch_gsrb_pass(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
                       const MgVars &V, int rb, int fly)
 {
     if (L.n <= 0) return;
     emul_submit(s, [=] {
+        switch (dim * 100 + L.nc) {
+        case 202: gsrb_tiles<2, 2>(T, L, V, rb, fly); return;
+        case 204: gsrb_tiles<2, 4>(T, L, V, rb, fly); return;
+        case 208: gsrb_tiles<2, 8>(T, L, V, rb, fly); return;
+        case 216: gsrb_tiles<2, 16>(T, L, V, rb, fly); return;
+        case 302: gsrb_tiles<3, 2>(T, L, V, rb, fly); return;
+        case 304: gsrb_tiles<3, 4>(T, L, V, rb, fly); return;
+        case 308: gsrb_tiles<3, 8>(T, L, V, rb, fly); return;
+        case 316: gsrb_tiles<3, 16>(T, L, V, rb, fly); return;
+        default: break;
+        }
         const LevelView v = pool_view(T, V);
-        run_stage(dim, L.nc, [&](auto d) { return gsrb_threads<DIMV>(L.n, L.nc); },
-                  [&](auto d, auto c, int g) { gsrb_body<DIMV, NCV>(T, L, v, v, V.op_auto, rb, fly, g); });
+        run_stage(dim, 0, [&](auto d) { return gsrb_threads<DIMV>(L.n, L.nc); },
+                  [&](auto d, auto c, int g) { gsrb_body<DIMV, 0>(T, L, v, v, V.op_auto, rb, fly, g); });
     });
 }
 
@@ -252,27 +288,34 @@ void launch_reduce_partials(hipStream_t s, const double *partial, int n, int wan
 
 int bottom_prepare(int, size_t lds_bytes) { return lds_bytes <= 163840 ? 0 : -1; }
 
-void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVars &V, const BottomArgs &B,
-                   size_t lds_bytes, int)
+void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVars &V,
+                   const BottomLevels *B, const BottomArgs &A, size_t lds_bytes, int)
 {
     emul_submit(s, [=] {
         double *lds = (double *)emul_alloc(lds_bytes ? lds_bytes : 8);
-        if (dim == 2)
-            bottom_run<2>(T, V, B, lds, 0, 1, [] {});
-        else
-            bottom_run<3>(T, V, B, lds, 0, 1, [] {});
+        if (dim == 2) {
+            if (V.op_auto)
+                bottom_run<2, 1>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {});
+            else
+                bottom_run<2, 0>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {});
+        } else {
+            if (V.op_auto)
+                bottom_run<3, 1>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {});
+            else
+                bottom_run<3, 0>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {});
+        }
         emul_free(lds);
     });
 }
 
 void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVars &V,
-                            const BottomArgs &B, int)
+                            const BottomLevels *B, const BottomArgs &A, int)
 {
     emul_submit(s, [=] {
         if (dim == 2)
-            bottom_residual_run<2>(T, V, B, 0, 1);
+            bottom_residual_run<2>(T, V, *B, A, 0, 1);
         else
-            bottom_residual_run<3>(T, V, B, 0, 1);
+            bottom_residual_run<3>(T, V, *B, A, 0, 1);
     });
 }
 

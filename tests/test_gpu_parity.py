@@ -48,10 +48,14 @@ def problem(kind):
         return P.small_3d(nc=4, l_uniform=2, l_max=4, eps_hi=4.0)
     if kind == "2d_nc8_eps":
         return P.make_problem("e2", 2, 8, P.C1_GAUSS, 3, 5, 1e-3, n_base=4, n_vcycle=2, eps_hi=7.0)
+    if kind == "2d_nc32":          # a cell count without a specialised kernel
+        return P.small_2d(nc=32, l_uniform=2, l_max=4)
+    if kind == "3d_nc16_eps":
+        return P.small_3d(nc=16, l_uniform=1, l_max=3, eps_hi=3.0)
     raise KeyError(kind)
 
 
-KINDS = ["2d_nc8", "2d_nc4", "2d_nc16", "3d_nc8", "3d_nc4_eps", "2d_nc8_eps"]
+KINDS = ["2d_nc8", "2d_nc4", "2d_nc16", "3d_nc8", "3d_nc4_eps", "2d_nc8_eps", "2d_nc32", "3d_nc16_eps"]
 STAGES = ["fill_gc", "gsrb", "residual", "restrict", "prolong2", "prolong2_add", "correct", "update_coarse"]
 
 
@@ -250,15 +254,23 @@ def test_periodic_base():
     compare_all(dt, ot, 3, "periodic")
 
 
-@pytest.mark.parametrize("cells", [0, 512, 1 << 20])
-def test_bottom_kernel_limit_is_exact(cells):
-    """The one-workgroup bottom kernel and per-level launches give the same
-    bits, whichever levels the bottom kernel takes."""
-    p = P.small_3d()
-    mg = M.default_mg(n_base=p.n_base)
-    om = O.default_opts(2, 2, p.n_base)
+@pytest.mark.parametrize("kind", ["3d", "3d_eps", "2d"])
+@pytest.mark.parametrize("cells,wave_cells", [(0, 0), (512, 0), (512, 512), (1 << 20, 0),
+                                              (1 << 20, 64), (1 << 20, 1 << 20)])
+def test_bottom_kernel_limit_is_exact(kind, cells, wave_cells):
+    """The one-workgroup bottom kernel (with or without its one-wavefront
+    part) and per-level launches give the same bits, whichever levels each
+    takes."""
+    if kind == "2d":
+        p = P.small_2d()
+    else:
+        p = P.small_3d(eps_hi=4.0 if kind == "3d_eps" else None)
+    eps = p.eps_hi is not None
+    mg = M.default_mg(eps=eps, n_base=p.n_base)
+    om = O.default_opts(2, 2, p.n_base, eps=eps)
+    n_var = len(p.state.data)
     dt = M.DeviceTree.from_state(p.state)
-    dt.set_bottom_cells(cells)
-    dt, ot, rd, ro = solve_both(p.state, 3, mg, om, 1, 2, dt=dt)
-    compare_all(dt, ot, 3, f"bottom {cells}")
+    dt.set_bottom_cells(cells, wave_cells)
+    dt, ot, rd, ro = solve_both(p.state, n_var, mg, om, 1, 2, dt=dt)
+    compare_all(dt, ot, n_var, f"{kind} bottom {cells}/{wave_cells}")
     assert np.array_equal(rd, ro)
