@@ -63,7 +63,7 @@ SIGNATURES = {
     "afg_update_coarse": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int]),
     "afg_set_bottom_cells": (ctypes.c_int, [P, I64, I64]),
     "afg_set_graphs": (ctypes.c_int, [P, ctypes.c_int]),
-    "afg_set_timing": (ctypes.c_int, [P, ctypes.c_int]),
+    "afg_set_timing": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
     "afg_timing": (ctypes.c_int, [P, I64P, F64P, F64P]),
     "afg_vcycle_cell_updates": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int, F64P]),
     "afg_device_bytes": (ctypes.c_int, [P, I64P]),

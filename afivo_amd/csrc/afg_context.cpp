@@ -39,10 +39,17 @@ struct BcRec {
 
 // An instantiated V-cycle graph; in timing mode it also owns the event pairs
 // recorded around its colour-pass launches and their algorithmic bytes.
+// A timed block of launches: their number and summed algorithmic bytes.
+struct TimedBlock {
+    int n;
+    double bytes;
+};
+
 struct GraphRec {
     hipGraphExec_t exec = nullptr;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
-    std::vector<double> bytes;
+    std::vector<TimedBlock> blocks;
+    bool pending = false;   // launched since its events were last read
 };
 
 void destroy_graph(GraphRec &g)
@@ -54,7 +61,7 @@ void destroy_graph(GraphRec &g)
     }
     g.exec = nullptr;
     g.ev.clear();
-    g.bytes.clear();
+    g.blocks.clear();
 }
 
 uint64_t spread2(uint64_t x)   // 2D interleave (bit_space_1, m_morton.f90)
@@ -129,11 +136,13 @@ struct afg_ctx {
     std::map<std::tuple<int, int, int, int, int, int, int, int, int, int>, GraphRec> graphs;
     bool capturing = false;
     bool timing = false;
+    int timing_lvl = -1000;             // level whose passes are timed (below min_lvl: all)
+    unsigned timing_ring = 0;           // replay counter: graph copy used in timing mode
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_used, ev_free;
-    std::vector<double> ev_bytes;
+    std::vector<TimedBlock> ev_blocks;
     // events recorded into the graph being captured (timing mode)
     std::vector<std::pair<hipEvent_t, hipEvent_t>> cap_ev;
-    std::vector<double> cap_bytes;
+    std::vector<TimedBlock> cap_blocks;
     int64_t t_launches = 0;
     double t_ms = 0, t_bytes = 0;
 
@@ -334,48 +343,71 @@ double gsrb_pass_bytes(const afg_ctx *c, const LevelArgs &L, const MgVars &V)
     return 0.5 * cells * per_cell;
 }
 
-void timed_gsrb(afg_ctx *c, const DevTree &T, const LevelArgs &L, const MgVars &V, int rb, int fly)
+// Consecutive colour passes of one level, bracketed by one pair of HIP
+// events on the context stream when that level is timed (a timed block holds
+// n launches and their summed algorithmic bytes).
+void timed_passes(afg_ctx *c, const DevTree &T, const LevelArgs &L, int lvl, const MgVars &V,
+                  int n_pass, int first_fly)
 {
-    if (!c->timing) {
-        launch_gsrb_pass(c->dim, c->stream, T, L, V, rb, fly);
-        return;
-    }
+    const bool on = c->timing && n_pass > 0 && (c->timing_lvl < c->min_lvl || lvl == c->timing_lvl);
     std::pair<hipEvent_t, hipEvent_t> ev;
-    if (!c->capturing && !c->ev_free.empty()) {
-        ev = c->ev_free.back();
-        c->ev_free.pop_back();
-    } else {
-        hipEventCreate(&ev.first);
-        hipEventCreate(&ev.second);
-    }
     // inside a capture the events must become event-record nodes of the graph
     const unsigned flags = c->capturing ? hipEventRecordExternal : 0;
-    hipEventRecordWithFlags(ev.first, c->stream, flags);
-    launch_gsrb_pass(c->dim, c->stream, T, L, V, rb, fly);
+    if (on) {
+        if (!c->capturing && !c->ev_free.empty()) {
+            ev = c->ev_free.back();
+            c->ev_free.pop_back();
+        } else {
+            hipEventCreate(&ev.first);
+            hipEventCreate(&ev.second);
+        }
+        hipEventRecordWithFlags(ev.first, c->stream, flags);
+    }
+    for (int rb = 1; rb <= n_pass; rb++) launch_gsrb_pass(c->dim, c->stream, T, L, V, rb, rb == 1 ? first_fly : 1);
+    if (!on) return;
     hipEventRecordWithFlags(ev.second, c->stream, flags);
+    const TimedBlock tb{n_pass, n_pass * gsrb_pass_bytes(c, L, V)};
     if (c->capturing) {
         c->cap_ev.push_back(ev);
-        c->cap_bytes.push_back(gsrb_pass_bytes(c, L, V));
+        c->cap_blocks.push_back(tb);
     } else {
         c->ev_used.push_back(ev);
-        c->ev_bytes.push_back(gsrb_pass_bytes(c, L, V));
+        c->ev_blocks.push_back(tb);
     }
+}
+
+// Add the event times of a graph's last replay to the totals.
+int harvest_graph(afg_ctx *c, GraphRec &g)
+{
+    if (!g.pending) return 0;
+    HIP_TRY(c, hipEventSynchronize(g.ev.back().second));
+    for (size_t k = 0; k < g.ev.size(); k++) {
+        float ms = 0;
+        HIP_TRY(c, hipEventElapsedTime(&ms, g.ev[k].first, g.ev[k].second));
+        c->t_ms += ms;
+        c->t_bytes += g.blocks[k].bytes;
+        c->t_launches += g.blocks[k].n;
+    }
+    g.pending = false;
+    return 0;
 }
 
 int collect_timing(afg_ctx *c)
 {
+    for (auto &g : c->graphs)
+        if (int e = harvest_graph(c, g.second)) return e;
     if (c->ev_used.empty()) return 0;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     for (size_t k = 0; k < c->ev_used.size(); k++) {
         float ms = 0;
         hipEventElapsedTime(&ms, c->ev_used[k].first, c->ev_used[k].second);
         c->t_ms += ms;
-        c->t_bytes += c->ev_bytes[k];
-        c->t_launches++;
+        c->t_bytes += c->ev_blocks[k].bytes;
+        c->t_launches += c->ev_blocks[k].n;
         c->ev_free.push_back(c->ev_used[k]);
     }
     c->ev_used.clear();
-    c->ev_bytes.clear();
+    c->ev_blocks.clear();
     return 0;
 }
 
@@ -435,7 +467,7 @@ void fill_gc(Mg &m, double *var, int lvl)
 void passes(Mg &m, int lvl, int n_cycle, int first_fly)
 {
     const LevelArgs L = level_args(m.c, lvl);
-    for (int rb = 1; rb <= 2 * n_cycle; rb++) timed_gsrb(m.c, m.T, L, m.V, rb, rb == 1 ? first_fly : 1);
+    timed_passes(m.c, m.T, L, lvl, m.V, 2 * n_cycle, first_fly);
 }
 
 // gsrb_boxes as the reference has it: passes, then the last ghost fill.
@@ -544,14 +576,18 @@ int run_vcycle(Mg &m, int highest_lvl, int set_residual)
         return 0;
     }
     const afg_mg_opts *o = m.o;
+    // In timing mode each key has TIMING_RING graph copies, each with its own
+    // events, used in turn: a copy's events are read before it is replayed
+    // again, which by then has long finished, so the stream never stalls.
+    constexpr unsigned TIMING_RING = 8;
     auto key = std::make_tuple(highest_lvl, set_residual, o->i_phi, o->i_rhs, o->i_tmp, o->i_eps,
                                o->op, o->n_cycle_down * 1000000 + o->n_cycle_up * 1000, o->n_cycle_base,
-                               c->timing ? 1 : 0);
+                               c->timing ? 1 + (int)(c->timing_ring++ % TIMING_RING) : 0);
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
         hipGraph_t g = nullptr;
         c->cap_ev.clear();
-        c->cap_bytes.clear();
+        c->cap_blocks.clear();
         HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         c->capturing = true;
         vcycle_body(m, highest_lvl, set_residual);
@@ -561,25 +597,16 @@ int run_vcycle(Mg &m, int highest_lvl, int set_residual)
         hipError_t e = hipGraphInstantiate(&rec.exec, g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
         rec.ev.swap(c->cap_ev);
-        rec.bytes.swap(c->cap_bytes);
+        rec.blocks.swap(c->cap_blocks);
         if (e != hipSuccess) {
             destroy_graph(rec);
             return c->fail(AFG_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
         }
         it = c->graphs.emplace(key, std::move(rec)).first;
     }
+    if (int e = harvest_graph(c, it->second)) return e;
     HIP_TRY(c, hipGraphLaunch(it->second.exec, c->stream));
-    if (c->timing && !it->second.ev.empty()) {
-        HIP_TRY(c, hipStreamSynchronize(c->stream));
-        const GraphRec &g = it->second;
-        for (size_t k = 0; k < g.ev.size(); k++) {
-            float ms = 0;
-            hipEventElapsedTime(&ms, g.ev[k].first, g.ev[k].second);
-            c->t_ms += ms;
-            c->t_bytes += g.bytes[k];
-            c->t_launches++;
-        }
-    }
+    it->second.pending = !it->second.ev.empty();
     return 0;
 }
 
@@ -1373,11 +1400,12 @@ int afg_set_graphs(afg_ctx *c, int enabled)
     return 0;
 }
 
-int afg_set_timing(afg_ctx *c, int enabled)
+int afg_set_timing(afg_ctx *c, int enabled, int lvl)
 {
     if (!c) return AFG_ERR_ARG;
     if (int e = collect_timing(c)) return e;
     c->timing = enabled != 0;
+    c->timing_lvl = lvl;
     if (c->timing) {
         c->t_launches = 0;
         c->t_ms = 0;

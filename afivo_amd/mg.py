@@ -130,8 +130,9 @@ class DeviceTree:
     def set_graphs(self, enabled):
         self._c(_lib.load().afg_set_graphs(self.h, int(enabled)))
 
-    def set_timing(self, enabled):
-        self._c(_lib.load().afg_set_timing(self.h, int(enabled)))
+    def set_timing(self, enabled, lvl=-1000):
+        """Time the GSRB passes of level `lvl` (all levels by default)."""
+        self._c(_lib.load().afg_set_timing(self.h, int(enabled), int(lvl)))
 
     def timing(self):
         n, ms, b = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()

@@ -11,8 +11,9 @@ residual stored after every cycle (as the reference examples do).
 `value` is cell-updates/s over the timed steps (a cell-update is one cell
 relaxed by one full red+black sweep; FMG's cycles count), with the tree and
 its data resident in device memory before timing starts.  The roofline object
-is for the dominant kernel, the GSRB colour pass, timed with HIP events on
-the context's stream during the timed steps.  The CPU baseline runs the
+is for the dominant HBM kernel, the GSRB colour pass on the finest level,
+timed with HIP events on the context's stream during the timed steps; its
+traffic comes from the committed rocprofv3 counter summary.  The CPU baseline runs the
 reference's own solver (built from its sources into oracle/_ref) on the same
 tree on this host's cores.
 
@@ -44,7 +45,26 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all cores up to 16")
     ap.add_argument("--no-timing", action="store_true",
                     help="skip the per-kernel event timing (roofline is then null)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy rhs to tmp over every box once, from cold caches (for the "
+                         "HBM byte counters' scale factor), print its byte count and exit")
     return ap.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_gsrb_tile_c2_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the finest-level GSRB pass, from the committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, scaled by the
+    factors a plain copy of known size gives); None when absent."""
+    import re
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
 
 
 def cpu_baseline(p, cell_updates_per_solve, threads):
@@ -84,6 +104,24 @@ def main():
     build_s = time.time() - t0
     dt = M.DeviceTree.from_state(st, 3)
     mg = M.default_mg(n_base=p.n_base)
+    if a.copy_calibration:
+        import ctypes
+        hip = ctypes.CDLL("libamdhip64.so")
+        buf = ctypes.c_void_p()
+        flush_bytes = ctypes.c_size_t(1 << 31)            # 2 GiB written: evicts L2 and MALL
+        dt.sync()
+        if hip.hipMalloc(ctypes.byref(buf), flush_bytes) != 0:
+            raise RuntimeError("hipMalloc of the cache-flush buffer failed")
+        if hip.hipMemset(buf, 1, flush_bytes) != 0 or hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("cache flush failed")
+        hip.hipFree(buf)
+        dt.copy_cc(2, 3)
+        dt.sync()
+        nbytes = int(((st.ncell().astype(np.int64) + 2) ** 3).sum()) * 8
+        print(json.dumps({"copy_calibration": {"kernel": "k_copy_boxes", "bytes_read": nbytes,
+                                               "bytes_written": nbytes}}))
+        dt.close()
+        return 0
 
     # cell-updates of one solve: FMG runs a V-cycle up to every level
     cu_solve = sum(dt.vcycle_cell_updates(mg, lvl) for lvl in range(dt.min_lvl, dt.highest_lvl + 1))
@@ -100,7 +138,7 @@ def main():
     rel_res = M.max_abs_cc(dt, 3) / max(abs(M.tree_min_cc(dt, 2)), abs(M.tree_max_cc(dt, 2)))
 
     if not a.no_timing:
-        dt.set_timing(True)
+        dt.set_timing(True, dt.highest_lvl)   # the finest level's colour passes
     dt.sync()
     t1 = time.perf_counter()
     for _ in range(a.steps):
@@ -142,10 +180,12 @@ def main():
     }
     if n_launch and k_ms > 0:
         achieved = k_bytes / (k_ms * 1e-3) / 1e9
+        n_top = int((st.lvl == st.highest_lvl).sum())
         out["roofline"] = {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "k_gsrb_pass (GSRB colour pass, all levels)",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+            "kernel": "k_gsrb_tile<3,8> (GSRB colour pass) on the finest level (%d, %d boxes)"
+                      % (st.highest_lvl, n_top),
             "launches": n_launch, "avg_launch_us": 1e3 * k_ms / n_launch,
             "alg_bytes_per_launch": k_bytes / n_launch,
             "share_of_step_time": (k_ms * 1e-3) / elapsed,

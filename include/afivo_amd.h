@@ -159,10 +159,11 @@ int afg_set_bottom_cells(afg_ctx *ctx, int64_t cells, int64_t wave_cells);
 /* Replay whole V-cycles from HIP graphs (default on). */
 int afg_set_graphs(afg_ctx *ctx, int enabled);
 /* Per-stage device timing: while enabled, each launch of the GSRB colour-pass
- * kernel is bracketed with HIP events on the context stream; afg_timing reads
- * the number of launches, their summed duration (ms) and their summed
- * algorithmic bytes. */
-int afg_set_timing(afg_ctx *ctx, int enabled);
+ * kernel on level `lvl` (every level when lvl is below the lowest) is
+ * bracketed with HIP events on the context stream; afg_timing reads the
+ * number of launches, their summed duration (ms) and their summed
+ * algorithmic bytes.  Results are identical with timing on or off. */
+int afg_set_timing(afg_ctx *ctx, int enabled, int lvl);
 int afg_timing(afg_ctx *ctx, int64_t *n_launches, double *ms, double *bytes);
 /* Number of cell-updates (cells relaxed by one full red+black sweep) one
  * V-cycle with these options performs, summed over levels. */

@@ -198,18 +198,29 @@ def test_graphs_and_direct_launches_agree():
         assert_bitexact(a, b, "graph vs direct")
 
 
-def test_timing_mode_is_exact():
-    """Event timing (graphs with event records) leaves results unchanged."""
+@pytest.mark.parametrize("one_level", [False, True])
+def test_timing_mode_is_exact(one_level):
+    """Event timing (graphs with event records, used in turn from a ring of
+    copies) leaves results unchanged and counts every timed launch."""
     p = P.small_2d()
     mg = M.default_mg(n_base=p.n_base)
     om = O.default_opts(2, 2, p.n_base)
     dt = M.DeviceTree.from_state(p.state)
     dt.set_bottom_cells(0)      # every level through the timed per-level passes
-    dt.set_timing(True)
-    dt, ot, rd, ro = solve_both(p.state, 3, mg, om, 1, 3, dt=dt)
+    top = p.state.highest_lvl
+    dt.set_timing(True, top if one_level else -1000)
+    n_cyc = 11                  # more V-cycles than graph copies in the ring
+    dt, ot, rd, ro = solve_both(p.state, 3, mg, om, 1, n_cyc, dt=dt)
     n, ms, b = dt.timing()
-    assert n > 0 and ms > 0 and b > 0
+    assert ms > 0 and b > 0
+    passes = 2 * (mg.n_cycle_down + mg.n_cycle_up)
+    if one_level:
+        # FMG's last V-cycle, then the V-cycles
+        assert n == (n_cyc + 1) * passes
+    else:
+        assert n > (n_cyc + 1) * passes
     compare_all(dt, ot, 3, "timing")
+    assert np.array_equal(rd, ro)
 
 
 def test_boundary_values_and_neumann():
