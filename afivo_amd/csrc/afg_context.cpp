@@ -30,6 +30,10 @@ struct Level {
     int parents_off = 0, n_parents = 0;
     int leaves_off = 0, n_leaves = 0;
     int all_off = 0;
+    // In a forest the boxes of a level are ordered by their tree's highest
+    // level, deepest trees first; n_act[b - min_lvl] boxes (parents, leaves)
+    // belong to trees whose highest level is >= b (b = min_lvl..highest+1).
+    std::vector<int> n_act, n_par_act, n_leaf_act;
 };
 
 struct BcRec {
@@ -94,6 +98,15 @@ struct afg_ctx {
     hipStream_t stream = nullptr;
     std::string err;
 
+    // forest: trees solved together (one tree unless afg_set_forest says otherwise)
+    int n_trees = 1;
+    std::vector<int> tree_hi;            // highest level of each tree
+    int min_tree_hi = 0;                 // the smallest of them
+    int bound = 0;                       // stages take trees with highest level >= bound
+    int res_upper = 1 << 20;             // ... and store residuals of trees below res_upper
+    int32_t *d_tree_leaves = nullptr, *d_tree_off = nullptr;   // leaf slots grouped by tree
+    double *d_tree_out = nullptr, *h_tree_out = nullptr;
+
     // tree
     bool have_tree = false;
     int min_lvl = 0, highest_lvl = -1, n_slots = 0, max_id = 0;
@@ -133,7 +146,7 @@ struct afg_ctx {
     int64_t bottom_cells = 8192;        // cell limit per level for the bottom kernel
     int64_t bottom_wave_cells = 512;    // ... and for its one-wavefront levels
     bool use_graphs = true;
-    std::map<std::tuple<int, int, int, int, int, int, int, int, int, int>, GraphRec> graphs;
+    std::map<std::tuple<int, int, int, int, int, int, int, int, int, int, int>, GraphRec> graphs;
     bool capturing = false;
     bool timing = false;
     int timing_lvl = -1000;             // level whose passes are timed (below min_lvl: all)
@@ -187,7 +200,7 @@ LevelArgs level_args(const afg_ctx *c, int lvl)
     const Level &L = c->levels[lvl - c->min_lvl];
     LevelArgs a;
     a.slot0 = L.slot0;
-    a.n = L.n;
+    a.n = L.n_act.empty() ? L.n : L.n_act[c->bound - c->min_lvl];
     a.nc = L.nc;
     a.pnc = lvl > c->min_lvl ? c->levels[lvl - 1 - c->min_lvl].nc : 0;
     a.dr = L.dr;
@@ -202,7 +215,11 @@ ListArgs list_args(const afg_ctx *c, int lvl, int which)   // 0 parents, 1 leave
     ListArgs a;
     const int off = which == 0 ? L.parents_off : which == 1 ? L.leaves_off : L.all_off;
     a.slots = c->d_lists + off;
-    a.n = which == 0 ? L.n_parents : which == 1 ? L.n_leaves : L.n;
+    const int b = c->bound - c->min_lvl;
+    if (L.n_act.empty())
+        a.n = which == 0 ? L.n_parents : which == 1 ? L.n_leaves : L.n;
+    else
+        a.n = which == 0 ? L.n_par_act[b] : which == 1 ? L.n_leaf_act[b] : L.n_act[b];
     a.nc = L.nc;
     a.dr = L.dr;
     a.dx2 = L.dx2;
@@ -233,6 +250,12 @@ void free_tree(afg_ctx *c)
     hipFree(c->d_base[1]);
     c->d_base[0] = c->d_base[1] = nullptr;
     c->base_cap = 0;
+    hipFree(c->d_tree_leaves);
+    hipFree(c->d_tree_off);
+    hipFree(c->d_tree_out);
+    if (c->h_tree_out) hipHostFree(c->h_tree_out);
+    c->d_tree_leaves = c->d_tree_off = nullptr;
+    c->d_tree_out = c->h_tree_out = nullptr;
     hipFree(c->d_partial);
     for (double *p : c->d_var) hipFree(p);
     c->d_nb = c->d_parent = c->d_choff = c->d_lists = c->d_bc_type = c->d_bc_voff = nullptr;
@@ -517,9 +540,20 @@ void correct_relax(Mg &m, int lvl, int n_cycle)
     launch_fill_diff(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), list_args(m.c, lvl - 1, 0), m.V);
 }
 
+// residual_box over level lvl, for the trees whose residual is stored: in a
+// forest those with highest level below res_upper (a tail of the level's
+// active boxes, deepest trees coming first)
 void residual_level(Mg &m, int lvl)
 {
-    launch_residual(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), m.V);
+    afg_ctx *c = m.c;
+    LevelArgs a = level_args(c, lvl);
+    const Level &L = c->levels[lvl - c->min_lvl];
+    if (!L.n_act.empty() && c->res_upper <= c->highest_lvl) {
+        const int skip = L.n_act[c->res_upper - c->min_lvl];
+        a.slot0 += skip;
+        a.n -= skip;
+    }
+    launch_residual(c->dim, c->stream, m.T, a, m.V);
 }
 
 BottomArgs bottom_args(Mg &m, int top)
@@ -541,7 +575,8 @@ int bottom_top(const Mg &m) { return m.c->bottom_lvl[m.V.op_auto ? 1 : 0]; }
 void vcycle_body(Mg &m, int highest_lvl, int set_residual)
 {
     afg_ctx *c = m.c;
-    const int lb = std::min(bottom_top(m), highest_lvl);
+    // the bottom kernel's tables cover every tree: used when all take part
+    const int lb = c->bound <= c->min_tree_hi ? std::min(bottom_top(m), highest_lvl) : -1000;
     const int low = lb >= c->min_lvl ? lb : c->min_lvl;
     for (int lvl = highest_lvl; lvl > low; lvl--) {
         passes(m, lvl, m.o->n_cycle_down, 0);
@@ -555,7 +590,7 @@ void vcycle_body(Mg &m, int highest_lvl, int set_residual)
     }
     for (int lvl = low + 1; lvl <= highest_lvl; lvl++) correct_relax(m, lvl, m.o->n_cycle_up);
     if (set_residual) {
-        if (lb >= c->min_lvl) {
+        if (lb >= c->min_lvl && c->res_upper > c->highest_lvl) {
             launch_bottom_residual(c->dim, c->stream, m.T, m.V, c->d_bottom[m.V.op_auto ? 1 : 0],
                                    bottom_args(m, lb), c->bottom_threads);
             for (int lvl = lb + 1; lvl <= highest_lvl; lvl++) residual_level(m, lvl);
@@ -568,9 +603,22 @@ void vcycle_body(Mg &m, int highest_lvl, int set_residual)
 // Run a V-cycle, from a cached HIP graph when enabled.  In timing mode the
 // graph carries event records around its colour passes; the stream is then
 // synchronised after each launch to read them before the next replay.
-int run_vcycle(Mg &m, int highest_lvl, int set_residual)
+// `bound` and `res_upper` select the trees of a forest: the V-cycle runs on
+// trees whose highest level is >= bound and stores the residual of those
+// below res_upper (one tree: bound = min_lvl, res_upper above the highest).
+int run_vcycle(Mg &m, int highest_lvl, int set_residual, int bound, int res_upper)
 {
     afg_ctx *c = m.c;
+    struct Scope {   // the selection holds for this V-cycle only
+        afg_ctx *c;
+        ~Scope()
+        {
+            c->bound = c->min_lvl;
+            c->res_upper = c->highest_lvl + 1;
+        }
+    } scope{c};
+    c->bound = bound;
+    c->res_upper = std::min(res_upper, c->highest_lvl + 1);
     if (!c->use_graphs) {
         vcycle_body(m, highest_lvl, set_residual);
         return 0;
@@ -582,7 +630,8 @@ int run_vcycle(Mg &m, int highest_lvl, int set_residual)
     constexpr unsigned TIMING_RING = 8;
     auto key = std::make_tuple(highest_lvl, set_residual, o->i_phi, o->i_rhs, o->i_tmp, o->i_eps,
                                o->op, o->n_cycle_down * 1000000 + o->n_cycle_up * 1000, o->n_cycle_base,
-                               c->timing ? 1 + (int)(c->timing_ring++ % TIMING_RING) : 0);
+                               c->timing ? 1 + (int)(c->timing_ring++ % TIMING_RING) : 0,
+                               c->bound * 1000 + c->res_upper);
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
         hipGraph_t g = nullptr;
@@ -875,6 +924,11 @@ int afg_sync(afg_ctx *c)
 
 int afg_set_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base)
 {
+    return afg_set_forest(c, n_boxes, recs, dr_base, nullptr);
+}
+
+int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base, const int32_t *tree_of)
+{
     if (!c) return AFG_ERR_ARG;
     if (n_boxes < 1 || !recs) return c->fail(AFG_ERR_ARG, "empty tree");
     if (!(dr_base > 0)) return c->fail(AFG_ERR_ARG, "dr_base must be > 0");
@@ -898,10 +952,31 @@ int afg_set_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base)
         if (rec_of[id] >= 0) return c->fail(AFG_ERR_ARG, "box id %d appears twice", id);
         rec_of[id] = r;
     }
-    // group by level, Morton order within a level (the key of a$D_tidy_up,
-    // m_aX_core.f90:272-279: morton_from_ix$D(ix - 1))
+    // trees of a forest: each tree's highest level
+    int n_trees = 1;
+    if (tree_of) {
+        for (int r = 0; r < n_boxes; r++) {
+            if (tree_of[r] < 0 || tree_of[r] >= n_boxes)
+                return c->fail(AFG_ERR_ARG, "record %d: tree index %d out of range", r, tree_of[r]);
+            n_trees = std::max(n_trees, tree_of[r] + 1);
+        }
+    }
+    auto tree = [&](int r) { return tree_of ? tree_of[r] : 0; };
+    std::vector<int> tree_hi(n_trees, -(1 << 30));
+    for (int r = 0; r < n_boxes; r++) tree_hi[tree(r)] = std::max(tree_hi[tree(r)], recs[(size_t)r * rl + 1]);
+    for (int t = 0; t < n_trees; t++)
+        if (tree_hi[t] < min_lvl) return c->fail(AFG_ERR_ARG, "tree %d has no boxes", t);
+    // group by level, then by tree (deepest trees first, so that the trees
+    // reaching a level are a prefix of every level), Morton order within a
+    // tree's level (the key of a$D_tidy_up, m_aX_core.f90:272-279:
+    // morton_from_ix$D(ix - 1))
     const int nl = hi_lvl - min_lvl + 1;
-    std::vector<std::vector<std::pair<uint64_t, int>>> per(nl);
+    struct Key {
+        int neg_hi, tree;
+        uint64_t morton;
+        int rec;
+    };
+    std::vector<std::vector<Key>> per(nl);
     for (int r = 0; r < n_boxes; r++) {
         const int32_t *q = recs + (size_t)r * rl;
         uint64_t key;
@@ -909,7 +984,7 @@ int afg_set_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base)
             key = spread2(q[2] - 1) | (spread2(q[3] - 1) << 1);
         else
             key = spread3(q[2] - 1) | (spread3(q[3] - 1) << 1) | (spread3(q[4] - 1) << 2);
-        per[q[1] - min_lvl].push_back({key, r});
+        per[q[1] - min_lvl].push_back({-tree_hi[tree(r)], tree(r), key, r});
     }
     c->levels.assign(nl, Level());
     c->id2slot.assign(max_id + 1, -1);
@@ -920,8 +995,11 @@ int afg_set_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base)
     for (int li = 0; li < nl; li++) {
         const int lvl = min_lvl + li;
         if (per[li].empty()) return c->fail(AFG_ERR_ARG, "level %d has no boxes", lvl);
-        std::stable_sort(per[li].begin(), per[li].end(),
-                         [](const std::pair<uint64_t, int> &a, const std::pair<uint64_t, int> &b) { return a.first < b.first; });
+        std::stable_sort(per[li].begin(), per[li].end(), [](const Key &a, const Key &b) {
+            if (a.neg_hi != b.neg_hi) return a.neg_hi < b.neg_hi;
+            if (a.tree != b.tree) return a.tree < b.tree;
+            return a.morton < b.morton;
+        });
         Level &L = c->levels[li];
         L.lvl = lvl;
         // a$D_n_cell (m_aX_utils.f90:676-686)
@@ -937,7 +1015,7 @@ int afg_set_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base)
         L.slot0 = slot;
         L.n = (int)per[li].size();
         for (auto &kr : per[li]) {
-            const int id = recs[(size_t)kr.second * rl];
+            const int id = recs[(size_t)kr.rec * rl];
             c->id2slot[id] = slot;
             c->slot2id[slot] = id;
             c->slot_lvl[slot] = lvl;
@@ -948,6 +1026,13 @@ int afg_set_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base)
     c->n_slots = slot;
     c->min_lvl = min_lvl;
     c->highest_lvl = hi_lvl;
+    c->n_trees = n_trees;
+    c->tree_hi = tree_hi;
+    c->min_tree_hi = *std::min_element(tree_hi.begin(), tree_hi.end());
+    c->bound = min_lvl;
+    c->res_upper = hi_lvl + 1;
+    std::vector<int> slot_tree(slot);
+    for (int r = 0; r < n_boxes; r++) slot_tree[c->id2slot[recs[(size_t)r * rl]]] = tree(r);
     c->max_id = max_id;
     c->dr_base = dr_base;
 
@@ -967,6 +1052,7 @@ int afg_set_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base)
                 return c->fail(AFG_ERR_ARG, "box %d: parent %d missing", q[0], pid);
             const int ps = c->id2slot[pid];
             if (c->slot_lvl[ps] != lvl - 1) return c->fail(AFG_ERR_ARG, "box %d: parent %d not on level %d", q[0], pid, lvl - 1);
+            if (slot_tree[ps] != slot_tree[s]) return c->fail(AFG_ERR_ARG, "box %d: parent %d in another tree", q[0], pid);
             par[s] = ps;
             // a$D_get_child_offset (m_aX_t.f90:362-374)
             int co = 0;
@@ -981,6 +1067,7 @@ int afg_set_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base)
                 if (nid > max_id || c->id2slot[nid] < 0) return c->fail(AFG_ERR_ARG, "box %d: neighbour %d missing", q[0], nid);
                 const int ns = c->id2slot[nid];
                 if (c->slot_lvl[ns] != lvl) return c->fail(AFG_ERR_ARG, "box %d: neighbour %d on another level", q[0], nid);
+                if (slot_tree[ns] != slot_tree[s]) return c->fail(AFG_ERR_ARG, "box %d: neighbour %d in another tree", q[0], nid);
                 nbv[(size_t)s * nnb + f] = ns;
             } else if (nid == 0) {
                 if (lvl == min_lvl) return c->fail(AFG_ERR_ARG, "box %d: refinement boundary on the lowest level", q[0]);
@@ -1009,6 +1096,21 @@ int afg_set_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base)
         L.n_leaves = (int)lists.size() - L.leaves_off;
         L.all_off = (int)lists.size();
         for (int s = L.slot0; s < L.slot0 + L.n; s++) lists.push_back(s);
+    }
+    // boxes, parents and leaves of each level in trees reaching bound b
+    for (int li = 0; li < nl; li++) {
+        Level &L = c->levels[li];
+        L.n_act.assign(nl + 1, 0);
+        L.n_par_act.assign(nl + 1, 0);
+        L.n_leaf_act.assign(nl + 1, 0);
+        for (int b = 0; b <= nl; b++) {
+            const int lb = min_lvl + b;
+            for (int s2 = L.slot0; s2 < L.slot0 + L.n; s2++) L.n_act[b] += tree_hi[slot_tree[s2]] >= lb;
+            for (int k = 0; k < L.n_parents; k++)
+                L.n_par_act[b] += tree_hi[slot_tree[lists[L.parents_off + k]]] >= lb;
+            for (int k = 0; k < L.n_leaves; k++)
+                L.n_leaf_act[b] += tree_hi[slot_tree[lists[L.leaves_off + k]]] >= lb;
+        }
     }
     // every box of a level above the lowest must be the child of a parent of
     // the level below (all children of a refined box exist)
@@ -1058,6 +1160,31 @@ int afg_set_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base)
         }
         HIP_TRY(c, hipMalloc(&c->d_is_parent, sn));
         HIP_TRY(c, hipMemcpy(c->d_is_parent, isp.data(), sn, hipMemcpyHostToDevice));
+    }
+    {   // leaf slots grouped by tree, for the per-tree norms
+        std::vector<int32_t> tl, toff(n_trees + 1, 0);
+        std::vector<std::vector<int32_t>> by(n_trees);
+        for (int li = 0; li < nl; li++) {
+            const Level &L = c->levels[li];
+            for (int k = 0; k < L.n_leaves; k++) {
+                const int s2 = lists[L.leaves_off + k];
+                if (c->slot_nc[s2] != c->nc)
+                    return c->fail(AFG_ERR_ARG, "leaf box %d below level 1", c->slot2id[s2]);
+                by[slot_tree[s2]].push_back(s2);
+            }
+        }
+        for (int t = 0; t < n_trees; t++) {
+            toff[t] = (int)tl.size();
+            tl.insert(tl.end(), by[t].begin(), by[t].end());
+        }
+        toff[n_trees] = (int)tl.size();
+        HIP_TRY(c, hipMalloc(&c->d_tree_leaves, std::max<size_t>(tl.size(), 1) * sizeof(int32_t)));
+        HIP_TRY(c, hipMalloc(&c->d_tree_off, toff.size() * sizeof(int32_t)));
+        HIP_TRY(c, hipMalloc(&c->d_tree_out, n_trees * sizeof(double)));
+        HIP_TRY(c, hipHostMalloc((void **)&c->h_tree_out, n_trees * sizeof(double), hipHostMallocDefault));
+        if (!tl.empty())
+            HIP_TRY(c, hipMemcpy(c->d_tree_leaves, tl.data(), tl.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(c->d_tree_off, toff.data(), toff.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     }
     c->have_tree = true;
     HIP_TRY(c, hipMalloc(&c->d_bottom[0], sizeof(BottomLevels)));
@@ -1241,7 +1368,7 @@ int afg_fas_vcycle(afg_ctx *c, const afg_mg_opts *o, int highest_lvl, int set_re
     Mg m;
     if (int e = make_mg(c, o, &m)) return e;
     set_tags(m);
-    return run_vcycle(m, highest_lvl, set_residual ? 1 : 0);
+    return run_vcycle(m, highest_lvl, set_residual ? 1 : 0, c->min_lvl, c->highest_lvl + 1);
 }
 
 int afg_fas_fmg(afg_ctx *c, const afg_mg_opts *o, int set_residual, int have_guess)
@@ -1263,8 +1390,35 @@ int afg_fas_fmg(afg_ctx *c, const afg_mg_opts *o, int set_residual, int have_gue
             // correct_children + fill_gc_phi (m_aX_mg.f90:193-196)
             correct_relax(m, lvl, 0);
         }
-        if (int e = run_vcycle(m, lvl, (set_residual && lvl == c->highest_lvl) ? 1 : 0)) return e;
+        // the V-cycle of FMG step lvl runs on the trees reaching lvl; a tree
+        // stores its residual in its last step, lvl == its highest level
+        if (int e = run_vcycle(m, lvl, set_residual ? 1 : 0, lvl, lvl + 1)) return e;
     }
+    return 0;
+}
+
+int afg_forest_max_abs(afg_ctx *c, int iv, int n_trees, double *out)
+{
+    if (int e = check_ctx(c)) return e;
+    if (!out) return c->fail(AFG_ERR_ARG, "null output");
+    if (n_trees != c->n_trees) return c->fail(AFG_ERR_ARG, "the forest has %d trees, not %d", c->n_trees, n_trees);
+    double *v = nullptr;
+    if (int e = ensure_var(c, iv, &v)) return e;
+    launch_tree_maxabs(c->dim, c->stream, c->stride, c->nc, c->d_tree_leaves, c->d_tree_off, c->n_trees, v,
+                       c->d_tree_out);
+    HIP_TRY(c, hipMemcpyAsync(c->h_tree_out, c->d_tree_out, c->n_trees * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    std::memcpy(out, c->h_tree_out, c->n_trees * sizeof(double));
+    return 0;
+}
+
+int afg_forest_trees(const afg_ctx *c, int *n_trees, int32_t *highest)
+{
+    if (!c || !c->have_tree || !n_trees) return AFG_ERR_ARG;
+    *n_trees = c->n_trees;
+    if (highest)
+        for (int t = 0; t < c->n_trees; t++) highest[t] = c->tree_hi[t];
     return 0;
 }
 

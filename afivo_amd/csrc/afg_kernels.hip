@@ -240,6 +240,31 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_partials(const double *partial
     if (threadIdx.x == 0) out[0] = v;
 }
 
+// max |var| over the leaf interiors of each tree of a forest (a$D_tree_max_cc
+// and a$D_tree_min_cc of one tree, max(|min|, |max|), m_aX_utils.f90:468-571):
+// one workgroup per tree over its leaves (all with nc cells per dimension)
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_tree_maxabs(int stride, int nc, const int32_t *leaves,
+                                                       const int32_t *off, const double *var,
+                                                       double *out)
+{
+    const int t = blockIdx.x;
+    const int per_box = ipow(nc, DIM);
+    const long long total = (long long)(off[t + 1] - off[t]) * per_box;
+    double v = 0;
+    for (long long g = threadIdx.x; g < total; g += blockDim.x) {
+        const int b = (int)(g / per_box);
+        const int c = (int)(g - (long long)b * per_box);
+        const int i = c % nc + 1;
+        const int j = (c / nc) % nc + 1;
+        const int k = DIM == 3 ? c / (nc * nc) + 1 : 0;
+        const int s = nc + 2;
+        v = pick(v, fabs(var[(size_t)leaves[off[t] + b] * stride + i + s * (j + s * k)]), 1);
+    }
+    v = block_extreme(v, 1);
+    if (threadIdx.x == 0) out[t] = v;
+}
+
 // ---------------------------------------------------------------------------
 // bottom of the V-cycle in one workgroup, on LDS copies of the small levels
 // (bottom_run), and the residual of those levels (bottom_residual_run)
@@ -442,6 +467,16 @@ void launch_extreme_list(int dim, hipStream_t s, const DevTree &T, const ListArg
         k_extreme_list<2><<<nb, BLOCK, 0, s>>>(T, P, var, want_max, partial);
     else
         k_extreme_list<3><<<nb, BLOCK, 0, s>>>(T, P, var, want_max, partial);
+}
+
+void launch_tree_maxabs(int dim, hipStream_t s, int stride, int nc, const int32_t *leaves,
+                        const int32_t *off, int n_trees, const double *var, double *out)
+{
+    if (n_trees <= 0) return;
+    if (dim == 2)
+        k_tree_maxabs<2><<<n_trees, BLOCK, 0, s>>>(stride, nc, leaves, off, var, out);
+    else
+        k_tree_maxabs<3><<<n_trees, BLOCK, 0, s>>>(stride, nc, leaves, off, var, out);
 }
 
 void launch_reduce_partials(hipStream_t s, const double *partial, int n, int want_max,

@@ -32,7 +32,7 @@ def _f64(a):
 class DeviceTree:
     """A tree whose cell data lives in device memory (one afg_ctx)."""
 
-    def __init__(self, dim, n_cell, n_var, records, dr_base):
+    def __init__(self, dim, n_cell, n_var, records, dr_base, tree_of=None):
         L = _lib.load()
         h = ctypes.c_void_p()
         check(L.afg_ctx_create(dim, n_cell, n_var, ctypes.byref(h)))
@@ -40,7 +40,12 @@ class DeviceTree:
         self.dim, self.nc, self.n_var = dim, n_cell, n_var
         self.records = np.ascontiguousarray(records, dtype=np.int32)
         self.ids = np.ascontiguousarray(self.records[:, 0])
-        check(L.afg_set_tree(self.h, self.records.shape[0], _i32(self.records), float(dr_base)), self.h)
+        if tree_of is None:
+            check(L.afg_set_tree(self.h, self.records.shape[0], _i32(self.records), float(dr_base)), self.h)
+        else:
+            self.tree_of = np.ascontiguousarray(tree_of, dtype=np.int32)
+            check(L.afg_set_forest(self.h, self.records.shape[0], _i32(self.records), float(dr_base),
+                                   _i32(self.tree_of)), self.h)
         lo, hi = ctypes.c_int(), ctypes.c_int()
         check(L.afg_levels(self.h, ctypes.byref(lo), ctypes.byref(hi)), self.h)
         self.min_lvl, self.highest_lvl = lo.value, hi.value
@@ -49,11 +54,33 @@ class DeviceTree:
         self.n_values = n.value
 
     @classmethod
-    def from_state(cls, st: TreeState, n_var=None):
-        t = cls(st.dim, st.nc, n_var or max(st.n_var, len(st.data)), st.recs, st.dr_base)
+    def from_state(cls, st: TreeState, n_var=None, tree_of=None):
+        t = cls(st.dim, st.nc, n_var or max(st.n_var, len(st.data)), st.recs, st.dr_base, tree_of)
         for iv, a in enumerate(st.data, start=1):
             t.upload(iv, a)
         return t
+
+    @classmethod
+    def from_forest(cls, states, n_var=None):
+        """One context holding the trees `states` (afg_set_forest)."""
+        from . import forest
+        st, tree_of = forest.combine(states)
+        return cls.from_state(st, n_var, tree_of)
+
+    # forests
+    def forest_trees(self):
+        n = ctypes.c_int()
+        self._c(_lib.load().afg_forest_trees(self.h, ctypes.byref(n), None))
+        hi = np.zeros(n.value, dtype=np.int32)
+        self._c(_lib.load().afg_forest_trees(self.h, ctypes.byref(n), _i32(hi)))
+        return hi
+
+    def forest_max_abs(self, iv):
+        """Per tree: max |variable iv| over its leaf interiors."""
+        n = self.forest_trees().size
+        out = np.zeros(n)
+        self._c(_lib.load().afg_forest_max_abs(self.h, iv, n, _f64(out)))
+        return out
 
     def close(self):
         if getattr(self, "h", None):

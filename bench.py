@@ -45,6 +45,10 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all cores up to 16")
     ap.add_argument("--no-timing", action="store_true",
                     help="skip the per-kernel event timing (roofline is then null)")
+    ap.add_argument("--workload", choices=["c2", "c3"], default="c2",
+                    help="c2: the 3D FP64 tree (default); c3: batched 2D FP64 forest of --trees trees")
+    ap.add_argument("--trees", type=int, default=256, help="c3: number of trees")
+    ap.add_argument("--tol", type=float, default=1e-10, help="c3: relative residual to reach")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy rhs to tmp over every box once, from cold caches (for the "
                          "HBM byte counters' scale factor), print its byte count and exit")
@@ -88,12 +92,117 @@ def cpu_baseline(p, cell_updates_per_solve, threads):
             "s_per_solve": med}
 
 
+def solve_to_tol(dt, mg, tol, rhs_norm, max_cycles=30):
+    """FMG, then V-cycles until every tree's max|residual| <= tol * max|rhs|
+    (per tree for a forest).  Returns the number of V-cycles."""
+    from afivo_amd import mg as M
+    M.mg_fas_fmg(dt, mg, True, False)
+    for k in range(max_cycles + 1):
+        if np.all(dt.forest_max_abs(3) <= tol * rhs_norm):
+            return k
+        M.mg_fas_vcycle(dt, mg, dt.highest_lvl, True)
+    raise RuntimeError("no convergence")
+
+
+def bench_c3(a):
+    """Batched 2D FP64: --trees independent trees solved to --tol relative
+    residual, as forests (one per n_cell) through afg_set_forest, against the
+    same trees solved one at a time, each in its own context."""
+    from afivo_amd import forest as F
+    from afivo_amd import mg as M
+
+    t0 = time.time()
+    trees = [F.c3_tree(a.seed, t, l_uniform=3, p_ref=C3_P_REF) for t in range(a.trees)]
+    build_s = time.time() - t0
+    mg = M.default_mg(n_base=50)
+    leaf_cells = np.array([int((st.children[:, 0] == 0).sum()) * spec.nc ** 2 for st, spec in trees])
+
+    def fresh(st):   # phi = tmp = 0, rhs from the tree
+        return [np.zeros_like(st.data[0]), st.data[1], np.zeros_like(st.data[2])]
+
+    # batched: one forest per n_cell
+    groups = {}
+    for t, (st, spec) in enumerate(trees):
+        groups.setdefault(spec.nc, []).append(t)
+    forests = []
+    for nc, idx in sorted(groups.items()):
+        sts = [trees[t][0] for t in idx]
+        fdt = M.DeviceTree.from_forest(sts)
+        fst, _ = F.combine(sts)
+        rhs_norm = np.array([np.abs(st.data[1][st.leaf_mask()]).max() for st in sts])
+        forests.append((fdt, fst, rhs_norm))
+    cycles_batch = []
+    for rep in range(2):   # the first solve captures the graphs
+        for fdt, fst, _ in forests:
+            for iv, v in enumerate(fresh(fst), start=1):
+                fdt.upload(iv, v)
+            fdt.sync()
+        tb0 = time.perf_counter()
+        cycles_batch = [solve_to_tol(fdt, mg, a.tol, rn) for fdt, _, rn in forests]
+        for fdt, _, _ in forests:
+            fdt.sync()
+        t_batch = time.perf_counter() - tb0
+    for fdt, _, _ in forests:
+        fdt.close()
+
+    # one tree at a time, each tree in its own context, warm (second solve)
+    t_loop, cycles_loop = 0.0, []
+    for st, spec in trees:
+        dt = M.DeviceTree.from_state(st)
+        rn = np.abs(st.data[1][st.leaf_mask()]).max()
+        for rep in range(2):
+            for iv, v in enumerate(fresh(st), start=1):
+                dt.upload(iv, v)
+            dt.sync()
+            s0 = time.perf_counter()
+            k = solve_to_tol(dt, mg, a.tol, rn)
+            dt.sync()
+            if rep == 1:
+                t_loop += time.perf_counter() - s0
+                cycles_loop.append(k)
+        dt.close()
+
+    out = {
+        "metric": "solves/s (batched, relative residual <= %g)" % a.tol,
+        "value": a.trees / t_batch,
+        "unit": "solves/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": 1e3 * t_batch,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {
+            "workload": "batched 2D FP64 Poisson: %d trees from seed %d, n_cell 8 or 16, uniform to "
+                        "level 3, then boxes below L ~ U{4..8} refined where a hash of (seed, tree, "
+                        "level, index) < %g; rhs 1-3 Gaussians (sigma U[0.02,0.08]); Dirichlet 0; "
+                        "FMG + V-cycles (2+2, 50 base) until max|res| <= %g max|rhs| per tree"
+                        % (a.trees, a.seed, C3_P_REF, a.tol),
+            "leaf_cells_total": int(leaf_cells.sum()),
+            "leaf_cells_per_tree": [int(leaf_cells.min()), int(np.median(leaf_cells)), int(leaf_cells.max())],
+            "forests": {str(nc): len(idx) for nc, idx in sorted(groups.items())},
+        },
+        "one_at_a_time_solves_per_s": a.trees / t_loop,
+        "speedup_vs_one_at_a_time": t_loop / t_batch,
+        "vcycles_batch": cycles_batch,
+        "vcycles_one_at_a_time": [int(min(cycles_loop)), int(max(cycles_loop))],
+        "host_tree_build_s": build_s,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+C3_P_REF = 0.25
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported "
               "(the multi-GPU design is described in DESIGN.md, not built)", file=sys.stderr)
         return 2
+    if a.workload == "c3":
+        return bench_c3(a)
 
     from afivo_amd import mg as M
     from afivo_amd import problems as P

@@ -91,6 +91,23 @@ int afg_sync(afg_ctx *ctx);
  * every physical boundary to Dirichlet zero.  Existing device data is lost. */
 int afg_set_tree(afg_ctx *ctx, int n_boxes, const int32_t *records, double dr_base);
 
+/* Batched solves: a forest of independent trees (same dim, n_cell and
+ * dr_base) in one context.  As afg_set_tree, plus tree_of[r] in 0..n_trees-1
+ * for record r; a box's parent, children and neighbours must be in its own
+ * tree.  Every stage then runs over the boxes of a level in all trees in one
+ * launch.  afg_fas_fmg and afg_fas_vcycle give every tree exactly the result
+ * of solving it alone (each tree's FMG stops at its own highest level and
+ * stores its residual there; a V-cycle from highest_lvl runs on each tree
+ * from min(highest_lvl, its highest level)).  No counterpart in the
+ * reference: afivo solves one tree per call. */
+int afg_set_forest(afg_ctx *ctx, int n_boxes, const int32_t *records, double dr_base,
+                   const int32_t *tree_of);
+/* Number of trees and each tree's highest level (highest may be NULL). */
+int afg_forest_trees(const afg_ctx *ctx, int *n_trees, int32_t *highest);
+/* Per tree: max |variable iv| over its leaf interiors, i.e. max(|min|, |max|)
+ * of a$D_tree_max_cc / a$D_tree_min_cc (m_aX_utils.f90:468-571) per tree. */
+int afg_forest_max_abs(afg_ctx *ctx, int iv, int n_trees, double *out);
+
 /* Boundary condition of box `id` in direction nb (1-based, a$D_neighb_*):
  * bc_type AFG_BC_DIRICHLET or AFG_BC_NEUMANN and the boundary values b written
  * by the user's sides_bc (nc_l^(dim-1) values, first tangential index

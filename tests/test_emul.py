@@ -37,3 +37,8 @@ from test_gpu_parity import (  # noqa: E402,F401
     test_stage_bitexact,
     test_timing_mode_is_exact,
 )
+from test_forest_gpu import (  # noqa: E402,F401
+    test_forest_equals_single_solves,
+    test_forest_partial_vcycle_and_oracle,
+    test_forest_rejects_links_across_trees,
+)

@@ -1,0 +1,77 @@
+"""Batched solves on the device: a forest of trees in one context gives every
+tree exactly the result of solving it alone (and, through that, of the
+reference).  Also run through the CPU emulation (test_emul.py)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from afivo_amd import forest as F
+from afivo_amd import mg as M
+from afivo_amd._lib import AfgError
+from helpers import assert_bitexact
+from test_forest import small_forest
+
+pytestmark = pytest.mark.gpu
+
+
+def solve_sequence(dt, mg, per_tree_norms, n_vcycle=3, partial_lvl=None):
+    """FMG with residual, then V-cycles; residual norms after each step."""
+    norms = []
+    M.mg_fas_fmg(dt, mg, True, False)
+    norms.append(per_tree_norms(dt))
+    for _ in range(n_vcycle):
+        M.mg_fas_vcycle(dt, mg, dt.highest_lvl if partial_lvl is None else partial_lvl, True)
+        norms.append(per_tree_norms(dt))
+    return norms
+
+
+@pytest.mark.parametrize("dim,n_trees", [(2, 6), (3, 3)])
+@pytest.mark.parametrize("bottom", [8192, 0])
+def test_forest_equals_single_solves(dim, n_trees, bottom):
+    sts = small_forest(dim, n_trees)
+    his = [s.highest_lvl for s in sts]
+    assert len(set(his)) > 1, "the forest should mix tree depths"
+    mg = M.default_mg(n_base=4)
+    fdt = M.DeviceTree.from_forest(sts)
+    fdt.set_bottom_cells(bottom)
+    assert list(fdt.forest_trees()) == his
+    fnorms = solve_sequence(fdt, mg, lambda d: d.forest_max_abs(3))
+    fdata = [F.split(sts, fdt.download(iv)) for iv in (1, 2, 3)]
+    for t, st in enumerate(sts):
+        dt = M.DeviceTree.from_state(st)
+        dt.set_bottom_cells(bottom)
+        norms = solve_sequence(dt, mg, lambda d: np.array([M.max_abs_cc(d, 3)]))
+        for iv in (1, 2, 3):
+            assert_bitexact(fdata[iv - 1][t], dt.download(iv), f"tree {t} var {iv}")
+        assert np.array_equal([n[t] for n in fnorms], [n[0] for n in norms]), f"tree {t} residuals"
+        dt.close()
+    fdt.close()
+
+
+def test_forest_partial_vcycle_and_oracle():
+    """A V-cycle from a level below some trees' highest level runs each tree
+    from min(level, its highest); one tree checked against the oracle."""
+    sts = small_forest(2, 4, seed=11)
+    mg = M.default_mg(n_base=4)
+    om = O.default_opts(2, 2, 4)
+    fdt = M.DeviceTree.from_forest(sts)
+    lvl = sorted(s.highest_lvl for s in sts)[1]
+    solve_sequence(fdt, mg, lambda d: d.forest_max_abs(3), n_vcycle=2, partial_lvl=lvl)
+    fdata = [F.split(sts, fdt.download(iv)) for iv in (1, 2, 3)]
+    for t, st in enumerate(sts):
+        ot = O.OracleTree(st)
+        ot.fas_fmg(om, True, False)
+        for _ in range(2):
+            ot.fas_vcycle(om, min(lvl, st.highest_lvl))
+        for iv in (1, 2, 3):
+            assert_bitexact(fdata[iv - 1][t], ot.get(iv), f"tree {t} var {iv}")
+    fdt.close()
+
+
+def test_forest_rejects_links_across_trees():
+    sts = small_forest(2, 2)
+    st, tree_of = F.combine(sts)
+    bad = tree_of.copy()
+    bad[1] = 1 - bad[1]                # one box moved to the other tree
+    with pytest.raises(AfgError):
+        M.DeviceTree(st.dim, st.nc, 3, st.recs, st.dr_base, bad)
