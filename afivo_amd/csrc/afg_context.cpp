@@ -104,8 +104,12 @@ struct afg_ctx {
     int min_tree_hi = 0;                 // the smallest of them
     int bound = 0;                       // stages take trees with highest level >= bound
     int res_upper = 1 << 20;             // ... and store residuals of trees below res_upper
-    int32_t *d_tree_leaves = nullptr, *d_tree_off = nullptr;   // leaf slots grouped by tree
-    double *d_tree_out = nullptr, *h_tree_out = nullptr;
+    // leaf slots grouped by tree, cut into chunks of at most ~4096 cells (each
+    // within one tree): chunk g = leaves [d_chunk[g], d_chunk[g+1]), tree t =
+    // chunks [d_tree_chunk[t], d_tree_chunk[t+1])
+    int32_t *d_tree_leaves = nullptr, *d_chunk = nullptr, *d_tree_chunk = nullptr;
+    int n_chunks = 0;
+    double *d_chunk_max = nullptr, *d_tree_out = nullptr, *h_tree_out = nullptr;
 
     // tree
     bool have_tree = false;
@@ -176,6 +180,15 @@ struct afg_ctx {
         hipError_t e_ = (expr);                                                       \
         if (e_ != hipSuccess)                                                         \
             return (ctx)->fail(AFG_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+// Host-to-device copy on the context's own (non-blocking) stream, complete on
+// return.  The legacy stream is never used: another thread may be capturing
+// a graph on its own stream meanwhile.
+#define HIP_H2D(ctx, dst, src, bytes)                                                       \
+    do {                                                                                    \
+        HIP_TRY(ctx, hipMemcpyAsync((dst), (src), (bytes), hipMemcpyHostToDevice, (ctx)->stream)); \
+        HIP_TRY(ctx, hipStreamSynchronize((ctx)->stream));                                  \
     } while (0)
 
 namespace {
@@ -251,11 +264,14 @@ void free_tree(afg_ctx *c)
     c->d_base[0] = c->d_base[1] = nullptr;
     c->base_cap = 0;
     hipFree(c->d_tree_leaves);
-    hipFree(c->d_tree_off);
+    hipFree(c->d_chunk);
+    hipFree(c->d_tree_chunk);
+    hipFree(c->d_chunk_max);
     hipFree(c->d_tree_out);
     if (c->h_tree_out) hipHostFree(c->h_tree_out);
-    c->d_tree_leaves = c->d_tree_off = nullptr;
-    c->d_tree_out = c->h_tree_out = nullptr;
+    c->d_tree_leaves = c->d_chunk = c->d_tree_chunk = nullptr;
+    c->d_chunk_max = c->d_tree_out = c->h_tree_out = nullptr;
+    c->n_chunks = 0;
     hipFree(c->d_partial);
     for (double *p : c->d_var) hipFree(p);
     c->d_nb = c->d_parent = c->d_choff = c->d_lists = c->d_bc_type = c->d_bc_voff = nullptr;
@@ -637,7 +653,7 @@ int run_vcycle(Mg &m, int highest_lvl, int set_residual, int bound, int res_uppe
         hipGraph_t g = nullptr;
         c->cap_ev.clear();
         c->cap_blocks.clear();
-        HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
         c->capturing = true;
         vcycle_body(m, highest_lvl, set_residual);
         c->capturing = false;
@@ -768,7 +784,7 @@ int build_base(afg_ctx *c)
         if (c->bottom_lvl[k] < c->min_lvl) {
             h.base = nullptr;
             h.base_n[0] = h.base_n[1] = 0;
-            HIP_TRY(c, hipMemcpy(c->d_bottom[k], &h, sizeof h, hipMemcpyHostToDevice));
+            HIP_H2D(c, c->d_bottom[k], &h, sizeof h);
             continue;
         }
         const bool lds = h.in_lds[0] != 0;
@@ -820,11 +836,11 @@ int build_base(afg_ctx *c)
         }
         std::vector<BaseCell> all(cells[0]);
         all.insert(all.end(), cells[1].begin(), cells[1].end());
-        HIP_TRY(c, hipMemcpy(c->d_base[k], all.data(), all.size() * sizeof(BaseCell), hipMemcpyHostToDevice));
+        HIP_H2D(c, c->d_base[k], all.data(), all.size() * sizeof(BaseCell));
         h.base = c->d_base[k];
         h.base_n[0] = (int)cells[0].size();
         h.base_n[1] = (int)cells[1].size();
-        HIP_TRY(c, hipMemcpy(c->d_bottom[k], &h, sizeof h, hipMemcpyHostToDevice));
+        HIP_H2D(c, c->d_bottom[k], &h, sizeof h);
     }
     return 0;
 }
@@ -1138,11 +1154,12 @@ int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base,
     HIP_TRY(c, hipMalloc(&c->d_choff, sn * sizeof(int32_t)));
     HIP_TRY(c, hipMalloc(&c->d_lists, std::max<size_t>(lists.size(), 1) * sizeof(int32_t)));
     HIP_TRY(c, hipMalloc(&c->d_tag, sn * sizeof(int8_t)));
-    HIP_TRY(c, hipMemcpy(c->d_nb, nbv.data(), sn * nnb * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemcpy(c->d_parent, par.data(), sn * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemcpy(c->d_choff, choff.data(), sn * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemcpy(c->d_lists, lists.data(), lists.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemset(c->d_tag, 0, sn * sizeof(int8_t)));
+    HIP_H2D(c, c->d_nb, nbv.data(), sn * nnb * sizeof(int32_t));
+    HIP_H2D(c, c->d_parent, par.data(), sn * sizeof(int32_t));
+    HIP_H2D(c, c->d_choff, choff.data(), sn * sizeof(int32_t));
+    HIP_H2D(c, c->d_lists, lists.data(), lists.size() * sizeof(int32_t));
+    HIP_TRY(c, hipMemsetAsync(c->d_tag, 0, sn * sizeof(int8_t), c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
     int maxp = 0;
     for (int li = 0; li < nl; li++) {
         const Level &L = c->levels[li];
@@ -1159,7 +1176,7 @@ int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base,
             isp[s2] = q[3 + dim] != 0;   // a$D_has_children
         }
         HIP_TRY(c, hipMalloc(&c->d_is_parent, sn));
-        HIP_TRY(c, hipMemcpy(c->d_is_parent, isp.data(), sn, hipMemcpyHostToDevice));
+        HIP_H2D(c, c->d_is_parent, isp.data(), sn);
     }
     {   // leaf slots grouped by tree, for the per-tree norms
         std::vector<int32_t> tl, toff(n_trees + 1, 0);
@@ -1173,18 +1190,27 @@ int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base,
                 by[slot_tree[s2]].push_back(s2);
             }
         }
+        const int per_box = dim == 2 ? c->nc * c->nc : c->nc * c->nc * c->nc;
+        const int per_chunk = std::max(1, 4096 / per_box);
+        std::vector<int32_t> chunk;
         for (int t = 0; t < n_trees; t++) {
-            toff[t] = (int)tl.size();
+            toff[t] = (int)chunk.size();
+            for (size_t k = 0; k < by[t].size(); k += per_chunk) chunk.push_back((int)(tl.size() + k));
             tl.insert(tl.end(), by[t].begin(), by[t].end());
         }
-        toff[n_trees] = (int)tl.size();
+        toff[n_trees] = (int)chunk.size();
+        c->n_chunks = (int)chunk.size();
+        chunk.push_back((int)tl.size());
         HIP_TRY(c, hipMalloc(&c->d_tree_leaves, std::max<size_t>(tl.size(), 1) * sizeof(int32_t)));
-        HIP_TRY(c, hipMalloc(&c->d_tree_off, toff.size() * sizeof(int32_t)));
+        HIP_TRY(c, hipMalloc(&c->d_chunk, chunk.size() * sizeof(int32_t)));
+        HIP_TRY(c, hipMalloc(&c->d_tree_chunk, toff.size() * sizeof(int32_t)));
+        HIP_TRY(c, hipMalloc(&c->d_chunk_max, std::max(c->n_chunks, 1) * sizeof(double)));
         HIP_TRY(c, hipMalloc(&c->d_tree_out, n_trees * sizeof(double)));
         HIP_TRY(c, hipHostMalloc((void **)&c->h_tree_out, n_trees * sizeof(double), hipHostMallocDefault));
         if (!tl.empty())
-            HIP_TRY(c, hipMemcpy(c->d_tree_leaves, tl.data(), tl.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-        HIP_TRY(c, hipMemcpy(c->d_tree_off, toff.data(), toff.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+            HIP_H2D(c, c->d_tree_leaves, tl.data(), tl.size() * sizeof(int32_t));
+        HIP_H2D(c, c->d_chunk, chunk.data(), chunk.size() * sizeof(int32_t));
+        HIP_H2D(c, c->d_tree_chunk, toff.data(), toff.size() * sizeof(int32_t));
     }
     c->have_tree = true;
     HIP_TRY(c, hipMalloc(&c->d_bottom[0], sizeof(BottomLevels)));
@@ -1404,8 +1430,8 @@ int afg_forest_max_abs(afg_ctx *c, int iv, int n_trees, double *out)
     if (n_trees != c->n_trees) return c->fail(AFG_ERR_ARG, "the forest has %d trees, not %d", c->n_trees, n_trees);
     double *v = nullptr;
     if (int e = ensure_var(c, iv, &v)) return e;
-    launch_tree_maxabs(c->dim, c->stream, c->stride, c->nc, c->d_tree_leaves, c->d_tree_off, c->n_trees, v,
-                       c->d_tree_out);
+    launch_tree_maxabs(c->dim, c->stream, c->stride, c->nc, c->d_tree_leaves, c->d_chunk, c->n_chunks,
+                       c->d_tree_chunk, c->n_trees, v, c->d_chunk_max, c->d_tree_out);
     HIP_TRY(c, hipMemcpyAsync(c->h_tree_out, c->d_tree_out, c->n_trees * sizeof(double), hipMemcpyDeviceToHost,
                               c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));

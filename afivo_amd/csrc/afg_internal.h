@@ -42,7 +42,8 @@ void launch_extreme_list(int dim, hipStream_t s, const DevTree &T, const ListArg
                          const double *var, int want_max, double *partial);
 // out[0] = extreme of partial[0..n).
 void launch_tree_maxabs(int dim, hipStream_t s, int stride, int nc, const int32_t *leaves,
-                        const int32_t *off, int n_trees, const double *var, double *out);
+                        const int32_t *chunk, int n_chunks, const int32_t *tree_chunk, int n_trees,
+                        const double *var, double *partial, double *out);
 void launch_reduce_partials(hipStream_t s, const double *partial, int n, int want_max,
                             double *out);
 // bottom kernel: allow lds_bytes of dynamic LDS (0 on success), then launch

@@ -3,6 +3,8 @@
 // kernels here map threads to it (one launch per stage and level over the
 // HBM pools, or one workgroup for the whole bottom of a V-cycle on LDS).
 
+#include <mutex>
+
 #include "afg_internal.h"
 #include "afg_bodies.h"
 
@@ -241,26 +243,40 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_partials(const double *partial
 }
 
 // max |var| over the leaf interiors of each tree of a forest (a$D_tree_max_cc
-// and a$D_tree_min_cc of one tree, max(|min|, |max|), m_aX_utils.f90:468-571):
-// one workgroup per tree over its leaves (all with nc cells per dimension)
+// and a$D_tree_min_cc of one tree, max(|min|, |max|), m_aX_utils.f90:468-571).
+// Workgroup g takes leaves [chunk[g], chunk[g+1]) of the grouped leaf list
+// (all of one tree, nc cells per dimension) and writes its maximum to
+// partial[g]; then one workgroup per tree reduces the partials of its chunks
+// [tree_chunk[t], tree_chunk[t+1]).
 template <int DIM>
 __global__ __launch_bounds__(BLOCK) void k_tree_maxabs(int stride, int nc, const int32_t *leaves,
-                                                       const int32_t *off, const double *var,
-                                                       double *out)
+                                                       const int32_t *chunk, const double *var,
+                                                       double *partial)
 {
-    const int t = blockIdx.x;
+    const int g = blockIdx.x;
     const int per_box = ipow(nc, DIM);
-    const long long total = (long long)(off[t + 1] - off[t]) * per_box;
+    const int first = chunk[g];
+    const int total = (chunk[g + 1] - first) * per_box;
     double v = 0;
-    for (long long g = threadIdx.x; g < total; g += blockDim.x) {
-        const int b = (int)(g / per_box);
-        const int c = (int)(g - (long long)b * per_box);
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        const int b = e / per_box;
+        const int c = e - b * per_box;
         const int i = c % nc + 1;
         const int j = (c / nc) % nc + 1;
         const int k = DIM == 3 ? c / (nc * nc) + 1 : 0;
         const int s = nc + 2;
-        v = pick(v, fabs(var[(size_t)leaves[off[t] + b] * stride + i + s * (j + s * k)]), 1);
+        v = pick(v, fabs(var[(size_t)leaves[first + b] * stride + i + s * (j + s * k)]), 1);
     }
+    v = block_extreme(v, 1);
+    if (threadIdx.x == 0) partial[g] = v;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_tree_reduce(const int32_t *tree_chunk, const double *partial,
+                                                       double *out)
+{
+    const int t = blockIdx.x;
+    double v = 0;
+    for (int q = tree_chunk[t] + threadIdx.x; q < tree_chunk[t + 1]; q += blockDim.x) v = pick(v, partial[q], 1);
     v = block_extreme(v, 1);
     if (threadIdx.x == 0) out[t] = v;
 }
@@ -470,13 +486,17 @@ void launch_extreme_list(int dim, hipStream_t s, const DevTree &T, const ListArg
 }
 
 void launch_tree_maxabs(int dim, hipStream_t s, int stride, int nc, const int32_t *leaves,
-                        const int32_t *off, int n_trees, const double *var, double *out)
+                        const int32_t *chunk, int n_chunks, const int32_t *tree_chunk, int n_trees,
+                        const double *var, double *partial, double *out)
 {
     if (n_trees <= 0) return;
-    if (dim == 2)
-        k_tree_maxabs<2><<<n_trees, BLOCK, 0, s>>>(stride, nc, leaves, off, var, out);
-    else
-        k_tree_maxabs<3><<<n_trees, BLOCK, 0, s>>>(stride, nc, leaves, off, var, out);
+    if (n_chunks > 0) {
+        if (dim == 2)
+            k_tree_maxabs<2><<<n_chunks, BLOCK, 0, s>>>(stride, nc, leaves, chunk, var, partial);
+        else
+            k_tree_maxabs<3><<<n_chunks, BLOCK, 0, s>>>(stride, nc, leaves, chunk, var, partial);
+    }
+    k_tree_reduce<<<n_trees, BLOCK, 0, s>>>(tree_chunk, partial, out);
 }
 
 void launch_reduce_partials(hipStream_t s, const double *partial, int n, int want_max,
@@ -485,13 +505,22 @@ void launch_reduce_partials(hipStream_t s, const double *partial, int n, int wan
     k_reduce_partials<<<1, BLOCK, 0, s>>>(partial, n, want_max, out);
 }
 
+// The LDS limit is an attribute of the kernel, shared by every context in
+// the process (contexts may be set up from several threads): it only grows,
+// to the largest amount any context asked for.
 int bottom_prepare(int dim, size_t lds_bytes)
 {
+    static std::mutex mu;
+    static size_t granted[2] = {0, 0};
+    std::lock_guard<std::mutex> lock(mu);
+    size_t &g = granted[dim == 2 ? 0 : 1];
+    if (lds_bytes <= g) return 0;
     const void *fn[2] = {dim == 2 ? (const void *)k_bottom<2, 0> : (const void *)k_bottom<3, 0>,
                          dim == 2 ? (const void *)k_bottom<2, 1> : (const void *)k_bottom<3, 1>};
     for (const void *f : fn)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) != hipSuccess)
             return -1;
+    g = lds_bytes;
     return 0;
 }
 

@@ -45,8 +45,11 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all cores up to 16")
     ap.add_argument("--no-timing", action="store_true",
                     help="skip the per-kernel event timing (roofline is then null)")
-    ap.add_argument("--workload", choices=["c2", "c3"], default="c2",
-                    help="c2: the 3D FP64 tree (default); c3: batched 2D FP64 forest of --trees trees")
+    ap.add_argument("--workload", choices=["c2", "c3", "c5"], default="c2",
+                    help="c2: the 3D FP64 tree (default); c3: batched 2D FP64 forest of --trees "
+                         "trees; c5: served request trace")
+    ap.add_argument("--requests", type=int, default=64, help="c5: number of requests")
+    ap.add_argument("--rate", type=float, default=100.0, help="c5: arrival rate (requests/s)")
     ap.add_argument("--trees", type=int, default=256, help="c3: number of trees")
     ap.add_argument("--tol", type=float, default=1e-10, help="c3: relative residual to reach")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -195,6 +198,47 @@ def bench_c3(a):
 C3_P_REF = 0.25
 
 
+def bench_c5(a):
+    """Serving: a Poisson-arrival request trace through the continuously
+    batching server, and the same trace through the plain solver one request
+    at a time."""
+    from afivo_amd import serving as S
+    t0 = time.time()
+    trace = S.make_trace(a.seed, a.requests, a.rate)
+    build_s = time.time() - t0
+    S.serve(trace)
+    served = S.summary(trace)
+    cells = np.array([r.leaf_cells for r in trace])
+    for r in trace:
+        r.start = r.finish = -1.0
+        r.phi = None
+    S.one_at_a_time(trace)
+    solo = S.summary(trace)
+    out = {
+        "metric": "solves/s and p50/p99 latency (served, FP64)",
+        "value": served["solves_per_s"], "unit": "solves/s",
+        "n_gpus": 1, "steps": a.requests, "warmup": 0,
+        "ms_per_step": 1e3 / served["solves_per_s"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {
+            "workload": "serving: %d requests, Poisson arrivals at %g/s (seed %d); 70%% 2D / 30%% 3D "
+                        "trees, n_cell 8, leaf cells log-uniform in [1e4, 1e7] (uniform to the level "
+                        "below the target, then two levels around 1-4 Gaussians); tolerance 1e-8 or "
+                        "1e-10 relative residual; FP64; continuous batching into forests, up to 4 "
+                        "lanes, 64 trees per lane, lanes set up by 3 host threads" % (a.requests, a.rate, a.seed),
+            "leaf_cells": [int(cells.min()), int(np.median(cells)), int(cells.max())],
+        },
+        "served": served,
+        "one_at_a_time": solo,
+        "host_tree_build_s": build_s,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -203,6 +247,8 @@ def main():
         return 2
     if a.workload == "c3":
         return bench_c3(a)
+    if a.workload == "c5":
+        return bench_c5(a)
 
     from afivo_amd import mg as M
     from afivo_amd import problems as P

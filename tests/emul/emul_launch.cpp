@@ -278,20 +278,26 @@ void launch_extreme_list(int dim, hipStream_t s, const DevTree &T, const ListArg
 }
 
 void launch_tree_maxabs(int dim, hipStream_t s, int stride, int nc, const int32_t *leaves,
-                        const int32_t *off, int n_trees, const double *var, double *out)
+                        const int32_t *chunk, int n_chunks, const int32_t *tree_chunk, int n_trees,
+                        const double *var, double *partial, double *out)
 {
     if (n_trees <= 0) return;
     emul_submit(s, [=] {
         const int sp = nc + 2;
-        for (int t = 0; t < n_trees; t++) {
+        for (int g = 0; g < n_chunks; g++) {
             double v = 0;
-            for (int e = off[t]; e < off[t + 1]; e++)
+            for (int e = chunk[g]; e < chunk[g + 1]; e++)
                 for (int k = dim == 3 ? 1 : 0; k <= (dim == 3 ? nc : 0); k++)
                     for (int j = 1; j <= nc; j++)
                         for (int i = 1; i <= nc; i++) {
                             const double x = std::fabs(var[(size_t)leaves[e] * stride + i + sp * (j + sp * k)]);
                             v = x > v ? x : v;
                         }
+            partial[g] = v;
+        }
+        for (int t = 0; t < n_trees; t++) {
+            double v = 0;
+            for (int q = tree_chunk[t]; q < tree_chunk[t + 1]; q++) v = partial[q] > v ? partial[q] : v;
             out[t] = v;
         }
     });

@@ -42,3 +42,7 @@ from test_forest_gpu import (  # noqa: E402,F401
     test_forest_partial_vcycle_and_oracle,
     test_forest_rejects_links_across_trees,
 )
+from test_serving import (  # noqa: E402,F401
+    test_serving_admits_by_memory_budget,
+    test_serving_meets_tolerances_and_matches_solo_runs,
+)
