@@ -35,15 +35,19 @@ AFG_HD constexpr int ipow(int n, int d) { return d == 2 ? n * n : n * n * n; }
 // X + (slot - slot0) * stride.  The HBM pools use slot0 = 0 and the pool
 // stride for every level; an LDS copy of a level uses its own slot0 and the
 // level's box size.
-struct LevelView {
-    double *phi, *rhs, *tmp;
-    const double *eps;
+// R is the value type: double for the reference arithmetic, float for the
+// FP32 stages of the mixed-precision solve.
+template <typename R>
+struct LevelViewT {
+    R *phi, *rhs, *tmp;
+    const R *eps;
     int slot0, stride;
     AFG_DEV size_t off(int slot) const
     {
         return (size_t)(slot - slot0) * (size_t)stride;
     }
 };
+using LevelView = LevelViewT<double>;
 
 // ---------------------------------------------------------------------------
 // ghost value of face f next to interior cell (i, j, k) of box `slot`
@@ -57,10 +61,9 @@ struct LevelView {
 //   physical boundary            -> captured sides_bc value b, then bc_to_gc
 //                                   (m_aX_gc.f90:71-132): c2*b + c1*interior
 // `center` is phi at (i, j, k); `own` holds the level, `par` its parent level.
-template <int DIM>
-AFG_DEV double ghost_value(const DevTree &T, const LevelView &own,
-                                              const LevelView &par, int pnc, double dr, int slot,
-                                              int f, int nc, int i, int j, int k, double center)
+template <int DIM, typename R>
+AFG_DEV R ghost_value(const DevTree &T, const LevelViewT<R> &own, const LevelViewT<R> &par, int pnc,
+                      double dr, int slot, int f, int nc, int i, int j, int k, R center)
 {
     const int s = nc + 2;
     const int nbv = T.nb[slot * (2 * DIM) + f];
@@ -79,38 +82,38 @@ AFG_DEV double ghost_value(const DevTree &T, const LevelView &own,
         const int pi = (co & 1023) + ((g0 + 1) >> 1);
         const int pj = ((co >> 10) & 1023) + ((g1 + 1) >> 1);
         const int pk = DIM == 3 ? ((co >> 20) & 1023) + ((g2 + 1) >> 1) : 0;
-        const double P = par.phi[par.off(p) + pi + ps * (pj + ps * pk)];
-        const double *box = own.phi + own.off(slot);
+        const R P = par.phi[par.off(p) + pi + ps * (pj + ps * pk)];
+        const R *box = own.phi + own.off(slot);
         const int inward = high ? -1 : 1;
         if (DIM == 2) {
             const int di = dn == 0 ? inward : -1 + 2 * (i & 1);
             const int dj = dn == 1 ? inward : -1 + 2 * (j & 1);
-            const double a = box[(i + di) + s * j];
-            const double b = box[i + s * (j + dj)];
-            return 0.5 * P + center - 0.25 * (a + b);
+            const R a = box[(i + di) + s * j];
+            const R b = box[i + s * (j + dj)];
+            return R(0.5) * P + center - R(0.25) * (a + b);
         } else {
             const int d0 = dn == 0 ? inward : -1 + 2 * (i & 1);
             const int d1 = dn == 1 ? inward : -1 + 2 * (j & 1);
             const int d2 = dn == 2 ? inward : -1 + 2 * (k & 1);
-            const double d = box[(i + d0) + s * ((j + d1) + s * (k + d2))];
-            return 0.5 * P + 0.75 * center - 0.25 * d;
+            const R d = box[(i + d0) + s * ((j + d1) + s * (k + d2))];
+            return R(0.5) * P + R(0.75) * center - R(0.25) * d;
         }
     }
     const int rec = NB_BC0 - nbv;
     const int voff = T.bc_voff[rec];
-    double b = 0.0;
+    R b = 0;
     if (voff >= 0) {
         const int t1 = dn == 0 ? j : i;
         const int t2 = dn == 2 ? j : k;
-        b = T.bc_vals[voff + (t1 - 1) + (DIM == 3 ? nc * (t2 - 1) : 0)];
+        b = R(T.bc_vals[voff + (t1 - 1) + (DIM == 3 ? nc * (t2 - 1) : 0)]);
     }
-    double c1, c2;
+    R c1, c2;
     if (T.bc_type[rec] == -1) {   // a5_bc_dirichlet
         c1 = -1;
         c2 = 2;
     } else {                      // a5_bc_neumann
         c1 = 1;
-        c2 = dr * (high ? 1.0 : -1.0);
+        c2 = R(dr * (high ? 1.0 : -1.0));
     }
     return c2 * b + c1 * center;
 }
@@ -118,11 +121,10 @@ AFG_DEV double ghost_value(const DevTree &T, const LevelView &own,
 // The 2*DIM face neighbours of interior cell q = (i, j, k), in the order
 // i-1, i+1, j-1, j+1, k-1, k+1.  A neighbour in the ghost layer is read from
 // memory when `fly` is 0 and computed with ghost_value when `fly` is 1.
-template <int DIM>
-AFG_DEV void stencil(const DevTree &T, const LevelArgs &L, const LevelView &own,
-                                        const LevelView &par, int slot, const double *box, int nc,
-                                        int i, int j, int k, int q, int fly, double center,
-                                        double u[6])
+template <int DIM, typename R>
+AFG_DEV void stencil(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &own,
+                     const LevelViewT<R> &par, int slot, const R *box, int nc, int i, int j, int k,
+                     int q, int fly, R center, R u[6])
 {
     const int s = nc + 2;
     u[0] = (i > 1 || !fly) ? box[q - 1] : ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, 0, nc, i, j, k, center);
@@ -139,24 +141,23 @@ AFG_DEV void stencil(const DevTree &T, const LevelArgs &L, const LevelView &own,
 
 // A phi at interior cell q from its neighbours u: mg$D_box_lpl
 // (m_aX_mg.f90:763-797) or, for tagged boxes, mg$D_box_lpld (:866-916).
-template <int DIM>
-AFG_DEV double box_op(const double u[6], double u0, const double *e, int q,
-                                         int s, double inv_dr_sq, bool lpld)
+template <int DIM, typename R>
+AFG_DEV R box_op(const R u[6], R u0, const R *e, int q, int s, double inv_dr_sq, bool lpld)
 {
     if (lpld) {
         const int dq[6] = {-1, 1, -s, s, -s * s, s * s};
-        const double a0 = e[q];
-        double sum = 0;
+        const R a0 = e[q];
+        R sum = 0;
 #pragma unroll
         for (int m = 0; m < 2 * DIM; m++) {
-            const double a = e[q + dq[m]];
-            const double t = a0 * a / (a0 + a) * (u[m] - u0);
+            const R a = e[q + dq[m]];
+            const R t = a0 * a / (a0 + a) * (u[m] - u0);
             sum = m == 0 ? t : sum + t;
         }
-        return inv_dr_sq * 2 * sum;
+        return R(inv_dr_sq) * 2 * sum;
     }
-    if (DIM == 2) return inv_dr_sq * (u[0] + u[1] + u[2] + u[3] - 4 * u0);
-    return inv_dr_sq * (u[0] + u[1] + u[2] + u[3] + u[4] + u[5] - 6 * u0);
+    if (DIM == 2) return R(inv_dr_sq) * (u[0] + u[1] + u[2] + u[3] - 4 * u0);
+    return R(inv_dr_sq) * (u[0] + u[1] + u[2] + u[3] + u[4] + u[5] - 6 * u0);
 }
 
 AFG_DEV bool lpld_box(const DevTree &T, int op_auto, int slot)
@@ -174,9 +175,9 @@ AFG_HD inline int fill_threads(int n, int nc)
     return n * 2 * DIM * (DIM == 2 ? nc : nc * nc);
 }
 
-template <int DIM, int NC>
-AFG_DEV void fill_body(const DevTree &T, const LevelArgs &L, const LevelView &own,
-                                          const LevelView &par, int gid)
+template <int DIM, int NC, typename R>
+AFG_DEV void fill_body(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &own,
+                       const LevelViewT<R> &par, int gid)
 {
     const int nc = NC ? NC : L.nc;
     const int per_face = DIM == 2 ? nc : nc * nc;
@@ -193,9 +194,9 @@ AFG_DEV void fill_body(const DevTree &T, const LevelArgs &L, const LevelView &ow
     const int k = DIM == 3 ? (dn == 2 ? in : t2) : 0;
     const int slot = L.slot0 + b;
     const int s = nc + 2;
-    double *box = own.phi + own.off(slot);
-    const double center = box[i + s * (j + s * k)];
-    const double g = ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, f, nc, i, j, k, center);
+    R *box = own.phi + own.off(slot);
+    const R center = box[i + s * (j + s * k)];
+    const R g = ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, f, nc, i, j, k, center);
     const int gi = dn == 0 ? (high ? nc + 1 : 0) : i;
     const int gj = dn == 1 ? (high ? nc + 1 : 0) : j;
     const int gk = dn == 2 ? (high ? nc + 1 : 0) : k;
@@ -204,9 +205,8 @@ AFG_DEV void fill_body(const DevTree &T, const LevelArgs &L, const LevelView &ow
 
 // correct_children, parent part (m_aX_mg.f90:380-381): tmp = phi - tmp over
 // the whole array of each parent.
-template <int DIM, int NC>
-AFG_DEV void diff_body(const DevTree &T, const ListArgs &P, const LevelView &v,
-                                          int gid)
+template <int DIM, int NC, typename R>
+AFG_DEV void diff_body(const DevTree &T, const ListArgs &P, const LevelViewT<R> &v, int gid)
 {
     const int nc = NC ? NC : P.nc;
     const int per_box = ipow(nc + 2, DIM);
@@ -238,30 +238,29 @@ AFG_HD inline int gsrb_threads(int n, int nc)
 // New phi at interior cell q of a box from its face neighbours u (order of
 // stencil) and rhs: mg$D_box_gsrb_lpl (m_aX_mg.f90:718-760) or, when lpld,
 // mg$D_box_gsrb_lpld (:810-863) with eps e of the box.
-template <int DIM>
-AFG_DEV double gsrb_value(const double u[6], double rhs, const double *e, int q, int s,
-                          double dx2, bool lpld)
+template <int DIM, typename R>
+AFG_DEV R gsrb_value(const R u[6], R rhs, const R *e, int q, int s, double dx2, bool lpld)
 {
     if (lpld) {
         // c = 2 a0 a / (a0 + a); phi = (sum(c u) - dx2 rhs) / sum(c)
-        const double a0 = e[q];
+        const R a0 = e[q];
         const int dq[6] = {-1, 1, -s, s, -s * s, s * s};
-        double su = 0, sc = 0;
+        R su = 0, sc = 0;
 #pragma unroll
         for (int m = 0; m < 2 * DIM; m++) {
-            const double a = e[q + dq[m]];
-            const double cm = 2 * a0 * a / (a0 + a);
+            const R a = e[q + dq[m]];
+            const R cm = 2 * a0 * a / (a0 + a);
             su = m == 0 ? cm * u[m] : su + cm * u[m];
             sc = m == 0 ? cm : sc + cm;
         }
-        return (su - dx2 * rhs) / sc;
+        return (su - R(dx2) * rhs) / sc;
     }
     if (DIM == 2) {
         // order of m_aX_mg.f90:740-743: i+1, i-1, j+1, j-1
-        return 0.25 * (u[1] + u[0] + u[3] + u[2] - dx2 * rhs);
+        return R(0.25) * (u[1] + u[0] + u[3] + u[2] - R(dx2) * rhs);
     }
     constexpr double sixth = 1 / 6.0;
-    return sixth * (u[1] + u[0] + u[3] + u[2] + u[5] + u[4] - dx2 * rhs);
+    return R(sixth) * (u[1] + u[0] + u[3] + u[2] + u[5] + u[4] - R(dx2) * rhs);
 }
 
 // interior cell (i, j, k) of colour rb for index c in [0, nc^DIM / 2): the
@@ -277,9 +276,9 @@ AFG_HD inline void colour_cell(int c, int nc, int rb, int &i, int &j, int &k)
     i = 2 - ((rb ^ (DIM == 2 ? j : k + j)) & 1) + 2 * p;
 }
 
-template <int DIM, int NC>
-AFG_DEV void gsrb_body(const DevTree &T, const LevelArgs &L, const LevelView &own,
-                       const LevelView &par, int op_auto, int rb, int fly, int gid)
+template <int DIM, int NC, typename R>
+AFG_DEV void gsrb_body(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &own,
+                       const LevelViewT<R> &par, int op_auto, int rb, int fly, int gid)
 {
     const int nc = NC ? NC : L.nc;
     const int half = ipow(nc, DIM) / 2;
@@ -289,10 +288,10 @@ AFG_DEV void gsrb_body(const DevTree &T, const LevelArgs &L, const LevelView &ow
     const int slot = L.slot0 + b;
     const int s = nc + 2;
     const size_t base = own.off(slot);
-    double *box = own.phi + base;
+    R *box = own.phi + base;
     const int q = i + s * (j + s * k);
-    const double center = box[q];
-    double u[6];
+    const R center = box[q];
+    R u[6];
     stencil<DIM>(T, L, own, par, slot, box, nc, i, j, k, q, fly, center, u);
     const bool lpld = lpld_box(T, op_auto, slot);
     box[q] = gsrb_value<DIM>(u, own.rhs[base + q], lpld ? own.eps + base : nullptr, q, s, L.dx2, lpld);
@@ -330,9 +329,9 @@ AFG_HD inline int xcd_tile(int blk, int n_blk)
     return x * q + (x < r ? x : r) + y;
 }
 
-template <int DIM, int NC, int FLY>
-AFG_DEV void tile_stage_a(const DevTree &T, const LevelArgs &L, const LevelView &v, double *lds,
-                          int box0, int rb, int tid, double *rhs)
+template <int DIM, int NC, int FLY, typename R>
+AFG_DEV void tile_stage_a(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &v, R *lds,
+                          int box0, int rb, int tid, R *rhs)
 {
     using G = GsrbTile<DIM, NC>;
     constexpr int S = G::S;
@@ -382,7 +381,7 @@ AFG_DEV void tile_stage_a(const DevTree &T, const LevelArgs &L, const LevelView 
         const int gq = gi + S * (gj + S * gk);
         const int slot = L.slot0 + box0 + b;
         const size_t base = v.off(slot);
-        double g;
+        R g;
         if (FLY)
             g = ghost_value<DIM>(T, v, v, L.pnc, L.dr, slot, f, NC, i, j, k, v.phi[base + i + S * (j + S * k)]);
         else
@@ -391,9 +390,9 @@ AFG_DEV void tile_stage_a(const DevTree &T, const LevelArgs &L, const LevelView 
     }
 }
 
-template <int DIM, int NC>
-AFG_DEV void tile_stage_b(const DevTree &T, const LevelArgs &L, const LevelView &v,
-                          const double *lds, int box0, int op_auto, int rb, int tid, const double *rhs)
+template <int DIM, int NC, typename R>
+AFG_DEV void tile_stage_b(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &v,
+                          const R *lds, int box0, int op_auto, int rb, int tid, const R *rhs)
 {
     using G = GsrbTile<DIM, NC>;
     constexpr int S = G::S;
@@ -405,14 +404,14 @@ AFG_DEV void tile_stage_b(const DevTree &T, const LevelArgs &L, const LevelView 
         int i, j, k;
         colour_cell<DIM>(e - b * G::HALF, NC, rb, i, j, k);
         const int q = i + S * (j + S * k);
-        const double *x = lds + b * G::LEN + q;
-        double u[6];
+        const R *x = lds + b * G::LEN + q;
+        R u[6];
         u[0] = x[-1];
         u[1] = x[1];
         u[2] = x[-S];
         u[3] = x[S];
-        u[4] = DIM == 3 ? x[-S * S] : 0.0;
-        u[5] = DIM == 3 ? x[S * S] : 0.0;
+        u[4] = DIM == 3 ? x[-S * S] : R(0);
+        u[5] = DIM == 3 ? x[S * S] : R(0);
         const int slot = L.slot0 + box0 + b;
         const size_t base = v.off(slot);
         const bool lpld = lpld_box(T, op_auto, slot);
@@ -421,10 +420,9 @@ AFG_DEV void tile_stage_b(const DevTree &T, const LevelArgs &L, const LevelView 
 }
 
 // residual_box (m_aX_mg.f90:545-559) over a level: tmp = rhs - A phi
-template <int DIM, int NC>
-AFG_DEV void residual_body(const DevTree &T, const LevelArgs &L,
-                                              const LevelView &own, const LevelView &par,
-                                              int op_auto, int gid)
+template <int DIM, int NC, typename R>
+AFG_DEV void residual_body(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &own,
+                           const LevelViewT<R> &par, int op_auto, int gid)
 {
     const int nc = NC ? NC : L.nc;
     const int per_box = ipow(nc, DIM);
@@ -436,12 +434,12 @@ AFG_DEV void residual_body(const DevTree &T, const LevelArgs &L,
     const int slot = L.slot0 + b;
     const int s = nc + 2;
     const size_t base = own.off(slot);
-    const double *box = own.phi + base;
+    const R *box = own.phi + base;
     const int q = i + s * (j + s * k);
     const bool lpld = lpld_box(T, op_auto, slot);
-    double u[6];
+    R u[6];
     stencil<DIM>(T, L, own, par, slot, box, nc, i, j, k, q, 0, box[q], u);
-    const double a = box_op<DIM>(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld);
+    const R a = box_op<DIM>(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld);
     own.tmp[base + q] = own.rhs[base + q] - a;
 }
 
@@ -453,10 +451,9 @@ AFG_DEV void residual_body(const DevTree &T, const LevelArgs &L,
 // (set_coarse_phi_rhs, m_aX_mg.f90:495-504) the residual stays in the
 // child's tmp.  With fly = 1 the residual's face neighbours are computed as
 // the ghost fill after the preceding relaxation would store them.
-template <int DIM, int NC>
-AFG_DEV void uc_child_body(const DevTree &T, const LevelArgs &L,
-                                              const LevelView &own, const LevelView &par,
-                                              int op_auto, int keep_child_tmp, int fly, int gid)
+template <int DIM, int NC, typename R>
+AFG_DEV void uc_child_body(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &own,
+                           const LevelViewT<R> &par, int op_auto, int keep_child_tmp, int fly, int gid)
 {
     const int nc = NC ? NC : L.nc;
     const int hn = nc >> 1;
@@ -470,19 +467,19 @@ AFG_DEV void uc_child_body(const DevTree &T, const LevelArgs &L,
     const int s = nc + 2;
     const size_t base = own.off(slot);
     const bool lpld = lpld_box(T, op_auto, slot);
-    const double *ph = own.phi + base;
-    const double *e = lpld ? own.eps + base : nullptr;
-    const double *rh = own.rhs + base;
-    double sr = 0, sp = 0;
+    const R *ph = own.phi + base;
+    const R *e = lpld ? own.eps + base : nullptr;
+    const R *rh = own.rhs + base;
+    R sr = 0, sp = 0;
 #pragma unroll
     for (int m = 0; m < (1 << DIM); m++) {
         const int fi = 2 * ii - 1 + (m & 1);
         const int fj = 2 * jj - 1 + ((m >> 1) & 1);
         const int fk = DIM == 3 ? 2 * kk - 1 + ((m >> 2) & 1) : 0;
         const int q = fi + s * (fj + s * fk);
-        double u[6];
+        R u[6];
         stencil<DIM>(T, L, own, par, slot, ph, nc, fi, fj, fk, q, fly, ph[q], u);
-        const double res = rh[q] - box_op<DIM>(u, ph[q], e, q, s, L.inv_dr_sq, lpld);
+        const R res = rh[q] - box_op<DIM>(u, ph[q], e, q, s, L.inv_dr_sq, lpld);
         if (!keep_child_tmp) own.tmp[base + q] = res;
         sr = m == 0 ? res : sr + res;
         sp = m == 0 ? ph[q] : sp + ph[q];
@@ -494,8 +491,8 @@ AFG_DEV void uc_child_body(const DevTree &T, const LevelArgs &L,
                    ps * ((((co >> 10) & 1023) + jj) + ps * (DIM == 3 ? ((co >> 20) & 1023) + kk : 0));
     const size_t pb = par.off(p);
     constexpr double w = DIM == 2 ? 0.25 : 0.125;
-    par.tmp[pb + pq] = w * sr;
-    par.phi[pb + pq] = w * sp;
+    par.tmp[pb + pq] = R(w) * sr;
+    par.phi[pb + pq] = R(w) * sp;
 }
 
 // update_coarse, parent part (m_aX_mg.f90:467-479), fused with the ghost fill
@@ -506,10 +503,9 @@ AFG_DEV void uc_child_body(const DevTree &T, const LevelArgs &L,
 // tmp = phi (a$D_box_copy_cc) if copy_tmp.  A phi uses the filled ghost
 // values, computed on the fly.  L is the parent level, own its view, par the
 // view of the level below it.
-template <int DIM, int NC>
-AFG_DEV void uc_parent_body(const DevTree &T, const LevelArgs &L,
-                                               const LevelView &own, const LevelView &par,
-                                               int op_auto, int copy_tmp, int gid)
+template <int DIM, int NC, typename R>
+AFG_DEV void uc_parent_body(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &own,
+                            const LevelViewT<R> &par, int op_auto, int copy_tmp, int gid)
 {
     const int nc = NC ? NC : L.nc;
     const int s = nc + 2;
@@ -521,27 +517,27 @@ AFG_DEV void uc_parent_body(const DevTree &T, const LevelArgs &L,
     const int k = DIM == 3 ? q / (s * s) : 0;
     const int slot = L.slot0 + b;
     const size_t base = own.off(slot);
-    double *box = own.phi + base;
+    R *box = own.phi + base;
     const int oi = (i < 1 || i > nc), oj = (j < 1 || j > nc), ok = DIM == 3 ? (k < 1 || k > nc) : 0;
     const int n_out = oi + oj + ok;
     const bool is_par = T.is_parent[slot] != 0;
     if (n_out == 0) {
         if (!is_par) return;
         const bool lpld = lpld_box(T, op_auto, slot);
-        double u[6];
+        R u[6];
         stencil<DIM>(T, L, own, par, slot, box, nc, i, j, k, q, 1, box[q], u);
-        const double a = box_op<DIM>(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld);
+        const R a = box_op<DIM>(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld);
         own.rhs[base + q] = a + own.tmp[base + q];
         if (copy_tmp) own.tmp[base + q] = box[q];
         return;
     }
-    double v = box[q];
+    R v = box[q];
     if (n_out == 1) {   // face ghost: fill it
         const int f = oi ? (i > nc) : oj ? 2 + (j > nc) : 4 + (k > nc);
         const int ci = oi ? (i > nc ? nc : 1) : i;
         const int cj = oj ? (j > nc ? nc : 1) : j;
         const int ck = DIM == 3 ? (ok ? (k > nc ? nc : 1) : k) : 0;
-        const double center = box[ci + s * (cj + s * ck)];
+        const R center = box[ci + s * (cj + s * ck)];
         v = ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, f, nc, ci, cj, ck, center);
         box[q] = v;
     }
@@ -553,32 +549,32 @@ AFG_DEV void uc_parent_body(const DevTree &T, const LevelArgs &L,
 // Quadratic prolongation stencil of a$D_prolong2 (m_aX_prolong.f90:281-343)
 // on coarse values c(q) supplied by `get`; the value of child octant m is
 // (((f0 -/+ fx) -/+ fy) -/+ fz) + f2.
-template <int DIM>
+template <int DIM, typename R>
 struct Prolong2 {
-    double f0, fx, fy, fz, f2;
+    R f0, fx, fy, fz, f2;
     template <typename G>
     AFG_DEV Prolong2(G get, int pq, int ps)
     {
         f0 = get(pq);
-        const double xm = get(pq - 1), xp = get(pq + 1);
-        const double ym = get(pq - ps), yp = get(pq + ps);
-        fx = 0.125 * (xp - xm);
-        fy = 0.125 * (yp - ym);
-        const double fxx = 0.03125 * (xm - 2 * f0 + xp);
-        const double fyy = 0.03125 * (ym - 2 * f0 + yp);
+        const R xm = get(pq - 1), xp = get(pq + 1);
+        const R ym = get(pq - ps), yp = get(pq + ps);
+        fx = R(0.125) * (xp - xm);
+        fy = R(0.125) * (yp - ym);
+        const R fxx = R(0.03125) * (xm - 2 * f0 + xp);
+        const R fyy = R(0.03125) * (ym - 2 * f0 + yp);
         if (DIM == 3) {
-            const double zm = get(pq - ps * ps), zp = get(pq + ps * ps);
-            fz = 0.125 * (zp - zm);
-            const double fzz = 0.03125 * (zm - 2 * f0 + zp);
+            const R zm = get(pq - ps * ps), zp = get(pq + ps * ps);
+            fz = R(0.125) * (zp - zm);
+            const R fzz = R(0.03125) * (zm - 2 * f0 + zp);
             f2 = fxx + fyy + fzz;
         } else {
             fz = 0;
             f2 = fxx + fyy;
         }
     }
-    AFG_DEV double at(int m) const
+    AFG_DEV R at(int m) const
     {
-        double v = (m & 1) ? f0 + fx : f0 - fx;
+        R v = (m & 1) ? f0 + fx : f0 - fx;
         v = ((m >> 1) & 1) ? v + fy : v - fy;
         if (DIM == 3) v = ((m >> 2) & 1) ? v + fz : v - fz;
         return v + f2;
@@ -592,10 +588,9 @@ struct Prolong2 {
 // tmp after tmp = phi - tmp; with fly_diff = 1 that difference is formed
 // here from the parent's phi and tmp (the parent's tmp is then updated by a
 // later stage).  One thread per coarse cell of the child's octant.
-template <int DIM, int NC>
-AFG_DEV void correct_body(const DevTree &T, const LevelArgs &L,
-                                             const LevelView &own, const LevelView &par,
-                                             int op_auto, int fly_diff, int gid)
+template <int DIM, int NC, typename R>
+AFG_DEV void correct_body(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &own,
+                          const LevelViewT<R> &par, int op_auto, int fly_diff, int gid)
 {
     const int nc = NC ? NC : L.nc;
     const int hn = nc >> 1;
@@ -614,14 +609,14 @@ AFG_DEV void correct_body(const DevTree &T, const LevelArgs &L,
     const int kc = DIM == 3 ? ((co >> 20) & 1023) + kk : 0;
     const int pq = ic + ps * (jc + ps * kc);
     const size_t pb = par.off(p);
-    const double *pt = par.tmp + pb;
-    const double *pp = par.phi + pb;
-    auto corr = [&](int q) { return fly_diff ? pp[q] - pt[q] : pt[q]; };
-    double *cph = own.phi + own.off(slot);
+    const R *pt = par.tmp + pb;
+    const R *pp = par.phi + pb;
+    auto corr = [&](int q) -> R { return fly_diff ? pp[q] - pt[q] : pt[q]; };
+    R *cph = own.phi + own.off(slot);
     if (lpld_box(T, op_auto, slot)) {
-        const double *pe = par.eps + pb;
+        const R *pe = par.eps + pb;
         constexpr double third = 1 / 3.0;
-        const double u0 = corr(pq), a0 = pe[pq];
+        const R u0 = corr(pq), a0 = pe[pq];
 #pragma unroll
         for (int m = 0; m < (1 << DIM); m++) {
             const int sx = m & 1, sy = (m >> 1) & 1, sz = (m >> 2) & 1;
@@ -631,29 +626,29 @@ AFG_DEV void correct_body(const DevTree &T, const LevelArgs &L,
             // (m_aX_mg.f90:944-958)
             const int qn[3] = {pq + (sx ? 1 : -1), pq + (sy ? ps : -ps),
                                pq + (sz ? ps * ps : -ps * ps)};
-            double *t = &cph[fi + s * (fj + s * fk)];
+            R *t = &cph[fi + s * (fj + s * fk)];
             if (DIM == 2) {
-                double sum = (a0 * u0 + pe[qn[0]] * corr(qn[0])) / (a0 + pe[qn[0]]);
+                R sum = (a0 * u0 + pe[qn[0]] * corr(qn[0])) / (a0 + pe[qn[0]]);
                 sum += (a0 * u0 + pe[qn[1]] * corr(qn[1])) / (a0 + pe[qn[1]]);
-                *t = *t + 0.5 * sum;
+                *t = *t + R(0.5) * sum;
             } else {
-                double sum = 0;
+                R sum = 0;
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
-                    const double v = (a0 * u0 + pe[qn[d]] * (1.5 * corr(qn[d]) - 0.5 * u0)) /
-                                     (a0 + pe[qn[d]]);
+                    const R v = (a0 * u0 + pe[qn[d]] * (R(1.5) * corr(qn[d]) - R(0.5) * u0)) /
+                                (a0 + pe[qn[d]]);
                     sum = d == 0 ? v : sum + v;
                 }
-                *t = *t + third * sum;
+                *t = *t + R(third) * sum;
             }
         }
     } else {
-        const Prolong2<DIM> P2(corr, pq, ps);
+        const Prolong2<DIM, R> P2(corr, pq, ps);
 #pragma unroll
         for (int m = 0; m < (1 << DIM); m++) {
             const int fi = 2 * ii - 1 + (m & 1), fj = 2 * jj - 1 + ((m >> 1) & 1);
             const int fk = DIM == 3 ? 2 * kk - 1 + ((m >> 2) & 1) : 0;
-            double *t = &cph[fi + s * (fj + s * fk)];
+            R *t = &cph[fi + s * (fj + s * fk)];
             *t = P2.at(m) + *t;
         }
     }
@@ -661,9 +656,8 @@ AFG_DEV void correct_body(const DevTree &T, const LevelArgs &L,
 
 // a$D_restrict_box (m_aX_restrict.f90:67-132) of every box of a level into
 // its parent: one thread per coarse cell.
-template <int DIM, int NC>
-AFG_DEV void restrict_body(const DevTree &T, const LevelArgs &L, const double *from, double *to,
-                           int gid)
+template <int DIM, int NC, typename R>
+AFG_DEV void restrict_body(const DevTree &T, const LevelArgs &L, const R *from, R *to, int gid)
 {
     const int nc = NC ? NC : L.nc;
     const int hn = nc >> 1;
@@ -675,13 +669,13 @@ AFG_DEV void restrict_body(const DevTree &T, const LevelArgs &L, const double *f
     const int kk = DIM == 3 ? c / (hn * hn) + 1 : 0;
     const int slot = L.slot0 + b;
     const int s = nc + 2;
-    const double *cb = from + (size_t)slot * T.stride;
-    double sum = 0;
+    const R *cb = from + (size_t)slot * T.stride;
+    R sum = 0;
 #pragma unroll
     for (int m = 0; m < (1 << DIM); m++) {
         const int fi = 2 * ii - 1 + (m & 1), fj = 2 * jj - 1 + ((m >> 1) & 1);
         const int fk = DIM == 3 ? 2 * kk - 1 + ((m >> 2) & 1) : 0;
-        const double v = cb[fi + s * (fj + s * fk)];
+        const R v = cb[fi + s * (fj + s * fk)];
         sum = m == 0 ? v : sum + v;
     }
     const int p = T.parent[slot];
@@ -690,15 +684,14 @@ AFG_DEV void restrict_body(const DevTree &T, const LevelArgs &L, const double *f
     const int pq = ((co & 1023) + ii) +
                    ps * ((((co >> 10) & 1023) + jj) + ps * (DIM == 3 ? ((co >> 20) & 1023) + kk : 0));
     constexpr double w = DIM == 2 ? 0.25 : 0.125;
-    to[(size_t)p * T.stride + pq] = w * sum;
+    to[(size_t)p * T.stride + pq] = R(w) * sum;
 }
 
 // a$D_prolong2 (m_aX_prolong.f90:243-348) from the parent into every box of
 // a level; add = 0 overwrites the interior (the reference zeroes it first and
 // adds, so the stored value is expr + 0).
-template <int DIM, int NC>
-AFG_DEV void prolong2_body(const DevTree &T, const LevelArgs &L, const double *from, double *to,
-                           int add, int gid)
+template <int DIM, int NC, typename R>
+AFG_DEV void prolong2_body(const DevTree &T, const LevelArgs &L, const R *from, R *to, int add, int gid)
 {
     const int nc = NC ? NC : L.nc;
     const int hn = nc >> 1;
@@ -715,15 +708,15 @@ AFG_DEV void prolong2_body(const DevTree &T, const LevelArgs &L, const double *f
     const int ps = L.pnc + 2;
     const int pq = ((co & 1023) + ii) +
                    ps * ((((co >> 10) & 1023) + jj) + ps * (DIM == 3 ? ((co >> 20) & 1023) + kk : 0));
-    const double *pb = from + (size_t)p * T.stride;
-    const Prolong2<DIM> P2([&](int q) { return pb[q]; }, pq, ps);
-    double *cb = to + (size_t)slot * T.stride;
+    const R *pb = from + (size_t)p * T.stride;
+    const Prolong2<DIM, R> P2([&](int q) -> R { return pb[q]; }, pq, ps);
+    R *cb = to + (size_t)slot * T.stride;
 #pragma unroll
     for (int m = 0; m < (1 << DIM); m++) {
         const int fi = 2 * ii - 1 + (m & 1), fj = 2 * jj - 1 + ((m >> 1) & 1);
         const int fk = DIM == 3 ? 2 * kk - 1 + ((m >> 2) & 1) : 0;
-        double *t = &cb[fi + s * (fj + s * fk)];
-        const double old = add ? *t : 0.0;
+        R *t = &cb[fi + s * (fj + s * fk)];
+        const R old = add ? *t : R(0);
         *t = P2.at(m) + old;
     }
 }
@@ -826,9 +819,9 @@ AFG_DEV void bottom_nc_dispatch(int nc, F &&f)
 // [0, wave) alone, one wavefront, and are ordered by `wsync()` among
 // themselves; the other threads pass them by and wait at the next workgroup
 // barrier.
-template <int DIM, int AUTO, typename Sync, typename WaveSync>
-AFG_DEV void bottom_run(const DevTree &Tg, const MgVars &V, const BottomLevels &B,
-                        const BottomArgs &A, double *lds, int tid, int nt, int wave, Sync sync,
+template <int DIM, int AUTO, typename R, typename Sync, typename WaveSync>
+AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLevels &B,
+                        const BottomArgs &A, void *lds, int tid, int nt, int wave, Sync sync,
                         WaveSync wsync)
 {
     constexpr int NNB = 2 * DIM;
@@ -863,16 +856,16 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVars &V, const BottomLevels &
     T.bc_voff = t_bcv;
     T.tag = t_tag;
     T.is_parent = t_isp;
-    double *data = (double *)(tb + B.table_bytes);
+    R *data = (R *)(tb + B.table_bytes);
 
-    auto copy_in = [&](const double *src, int off, int li) {
+    auto copy_in = [&](const R *src, int off, int li) {
         const int len = B.box_len[li], n = B.lv[li].n, slot0 = B.lv[li].slot0;
         for (int e = tid; e < n * len; e += nt) {
             const int b = e / len;
             data[off + e] = src[(size_t)(slot0 + b) * T.stride + (e - b * len)];
         }
     };
-    auto copy_out = [&](double *dst, int off, int li) {
+    auto copy_out = [&](R *dst, int off, int li) {
         const int len = B.box_len[li], n = B.lv[li].n, slot0 = B.lv[li].slot0;
         for (int e = tid; e < n * len; e += nt) {
             const int b = e / len;
@@ -889,7 +882,7 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVars &V, const BottomLevels &
     sync();
 
     auto view = [&](int li) {
-        LevelView w;
+        LevelViewT<R> w;
         if (B.in_lds[li]) {
             w.phi = data + B.lds_off[0][li];
             w.rhs = data + B.lds_off[1][li];
@@ -898,7 +891,7 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVars &V, const BottomLevels &
             w.slot0 = B.lv[li].slot0;
             w.stride = B.box_len[li];
         } else {
-            w = LevelView{V.phi, V.rhs, V.tmp, V.eps, 0, T.stride};
+            w = LevelViewT<R>{V.phi, V.rhs, V.tmp, V.eps, 0, T.stride};
         }
         return w;
     };
@@ -909,7 +902,7 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVars &V, const BottomLevels &
     // passes of BOP_BASE.
     auto exec = [&](const BottomStep st, int t, int n, bool in_wave) {
         const int li = st.li;
-        const LevelView own = view(li), par = view(li > 0 ? li - 1 : 0);
+        const LevelViewT<R> own = view(li), par = view(li > 0 ? li - 1 : 0);
         const bool on_parent = st.op == BOP_UC_PARENT || st.op == BOP_DIFF;
         bottom_nc_dispatch(B.lv[on_parent ? li - 1 : li].nc, [&](auto ncv) {
             constexpr int NC = decltype(ncv)::value;
@@ -923,15 +916,15 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVars &V, const BottomLevels &
                 const int s = (NC ? NC : L.nc) + 2;
                 const int dq[6] = {-1, 1, -s, s, -s * s, s * s};
                 auto relax = [&](const BaseCell &d, int rb) {
-                    const double center = own.phi[d.pos];
-                    double u[6];
+                    const R center = own.phi[d.pos];
+                    R u[6];
 #pragma unroll
                     for (int m = 0; m < 2 * DIM; m++) {
                         const int f = d.fly[m];
                         if (rb == 1 || f >= 0)
                             u[m] = own.phi[rb == 1 ? d.pos + dq[m] : f];
                         else
-                            u[m] = d.K[m] + (f == -1 ? -center : center);
+                            u[m] = R(d.K[m]) + (f == -1 ? -center : center);
                     }
                     if (DIM == 2) u[4] = u[5] = 0;
                     const bool lpld = lpld_box(T, AUTO, d.slot);
@@ -982,7 +975,7 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVars &V, const BottomLevels &
             } break;
             case BOP_UC_PARENT: {   // parent part on level li - 1
                 const LevelArgs Lp = B.lv[li - 1];
-                const LevelView ppar = view(li > 1 ? li - 2 : 0);
+                const LevelViewT<R> ppar = view(li > 1 ? li - 2 : 0);
                 const int tot = Lp.n * ipow((NC ? NC : Lp.nc) + 2, DIM);
                 for (int g = t; g < tot; g += n) uc_parent_body<DIM, NC>(T, Lp, par, ppar, AUTO, 1, g);
             } break;
@@ -1025,11 +1018,11 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVars &V, const BottomLevels &
 
 // residual on the small levels 0..hl in one launch (set_residual part of
 // mg$D_fas_vcycle, m_aX_mg.f90:243-254), on the HBM pools
-template <int DIM>
-AFG_DEV void bottom_residual_run(const DevTree &T, const MgVars &V, const BottomLevels &B,
+template <int DIM, typename R>
+AFG_DEV void bottom_residual_run(const DevTree &T, const MgVarsT<R> &V, const BottomLevels &B,
                                  const BottomArgs &A, int tid, int nt)
 {
-    const LevelView v = LevelView{V.phi, V.rhs, V.tmp, V.eps, 0, T.stride};
+    const LevelViewT<R> v = LevelViewT<R>{V.phi, V.rhs, V.tmp, V.eps, 0, T.stride};
     for (int li = 0; li <= A.hl; li++) {
         const LevelArgs &L = B.lv[li];
         nc_dispatch(L.nc, [&](auto ncv) {

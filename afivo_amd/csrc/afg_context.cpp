@@ -130,12 +130,15 @@ struct afg_ctx {
     double *d_bc_vals = nullptr;
     size_t bc_vals_cap = 0, bc_cap = 0;
     int8_t *d_tag = nullptr;
-    BottomLevels *d_bottom[2] = {nullptr, nullptr};   // bottom kernel tables: 3 / 4 variables in LDS
-    BottomLevels h_bottom[2];                         // their host copies
-    BaseCell *d_base[2] = {nullptr, nullptr};         // lowest-level cell lists of the two
+    // bottom kernel tables, index 2 * fp32 + (eps in LDS): FP64 / FP32 values,
+    // 3 / 4 variables in LDS
+    BottomLevels *d_bottom[4] = {nullptr, nullptr, nullptr, nullptr};
+    BottomLevels h_bottom[4];                         // their host copies
+    BaseCell *d_base[4] = {nullptr, nullptr, nullptr, nullptr};   // lowest-level cell lists
     int base_cap = 0;
     int8_t *d_is_parent = nullptr;
     std::vector<double *> d_var;
+    std::vector<float *> d_var32;        // FP32 copies (mixed-precision solve), allocated on use
     double *d_partial = nullptr;
     int partial_cap = 0;
     double *d_out = nullptr, *h_out = nullptr;
@@ -143,8 +146,8 @@ struct afg_ctx {
     size_t stage_cap = 0;
 
     // execution controls
-    int bottom_lvl[2] = {-1000, -1000};  // highest level run by the bottom kernel (3 / 4 variables)
-    size_t bottom_lds[2] = {0, 0};       // its dynamic LDS bytes
+    int bottom_lvl[4] = {-1000, -1000, -1000, -1000};  // highest level run by the bottom kernel
+    size_t bottom_lds[4] = {0, 0, 0, 0};               // its dynamic LDS bytes
     int bottom_threads = BOTTOM_BLOCK;   // its workgroup size
     int64_t lds_limit = 65536;           // LDS bytes one workgroup may use
     int64_t bottom_cells = 8192;        // cell limit per level for the bottom kernel
@@ -256,12 +259,12 @@ void free_tree(afg_ctx *c)
     hipFree(c->d_tag);
     hipFree(c->d_is_parent);
     c->d_is_parent = nullptr;
-    hipFree(c->d_bottom[0]);
-    hipFree(c->d_bottom[1]);
-    c->d_bottom[0] = c->d_bottom[1] = nullptr;
-    hipFree(c->d_base[0]);
-    hipFree(c->d_base[1]);
-    c->d_base[0] = c->d_base[1] = nullptr;
+    for (int k = 0; k < 4; k++) {
+        hipFree(c->d_bottom[k]);
+        hipFree(c->d_base[k]);
+        c->d_bottom[k] = nullptr;
+        c->d_base[k] = nullptr;
+    }
     c->base_cap = 0;
     hipFree(c->d_tree_leaves);
     hipFree(c->d_chunk);
@@ -274,6 +277,7 @@ void free_tree(afg_ctx *c)
     c->n_chunks = 0;
     hipFree(c->d_partial);
     for (double *p : c->d_var) hipFree(p);
+    for (float *p : c->d_var32) hipFree(p);
     c->d_nb = c->d_parent = c->d_choff = c->d_lists = c->d_bc_type = c->d_bc_voff = nullptr;
     c->d_bc_vals = nullptr;
     c->d_tag = nullptr;
@@ -281,6 +285,7 @@ void free_tree(afg_ctx *c)
     c->partial_cap = 0;
     c->bc_vals_cap = c->bc_cap = 0;
     std::fill(c->d_var.begin(), c->d_var.end(), nullptr);
+    std::fill(c->d_var32.begin(), c->d_var32.end(), nullptr);
     c->have_tree = false;
 }
 
@@ -299,13 +304,16 @@ int check_var(afg_ctx *c, int iv)
 }
 
 // Device pool of variable iv, allocated (and zeroed) on first use.
-int ensure_var(afg_ctx *c, int iv, double **out)
+// The device pool of variable iv (FP64, or its FP32 copy), allocated and
+// zeroed on first use.
+template <typename R>
+int ensure_pool(afg_ctx *c, int iv, std::vector<R *> &pools, R **out)
 {
     if (int e = check_var(c, iv)) return e;
-    double *&p = c->d_var[iv - 1];
+    R *&p = pools[iv - 1];
     if (!p) {
         if (c->capturing) return c->fail(AFG_ERR_STATE, "variable %d first used inside a graph capture", iv);
-        const size_t bytes = (size_t)c->n_slots * c->stride * sizeof(double);
+        const size_t bytes = (size_t)c->n_slots * c->stride * sizeof(R);
         hipError_t e = hipMalloc(&p, bytes);
         if (e != hipSuccess) {
             p = nullptr;
@@ -317,6 +325,9 @@ int ensure_var(afg_ctx *c, int iv, double **out)
     if (out) *out = p;
     return 0;
 }
+
+int ensure_var(afg_ctx *c, int iv, double **out) { return ensure_pool(c, iv, c->d_var, out); }
+int ensure_var(afg_ctx *c, int iv, float **out) { return ensure_pool(c, iv, c->d_var32, out); }
 
 int build_base(afg_ctx *c);
 
@@ -373,19 +384,21 @@ int ensure_stage(afg_ctx *c, size_t n_doubles)
 // Algorithmic bytes of one colour pass over a level: half of a full sweep's
 // (3 + 2*dim/nc) * 8 per cell (read phi with its face ghosts, read rhs, write
 // phi), plus (1 + 2*dim/nc) * 8 per cell of eps reads for the auto operators.
-double gsrb_pass_bytes(const afg_ctx *c, const LevelArgs &L, const MgVars &V)
+template <typename R>
+double gsrb_pass_bytes(const afg_ctx *c, const LevelArgs &L, const MgVarsT<R> &V)
 {
     const double cells = (double)L.n * (c->dim == 2 ? L.nc * L.nc : L.nc * L.nc * L.nc);
     const double g = 2.0 * c->dim / L.nc;
-    double per_cell = (3 + g) * 8.0;
-    if (V.op_auto) per_cell += (1 + g) * 8.0;
+    double per_cell = (3 + g) * sizeof(R);
+    if (V.op_auto) per_cell += (1 + g) * sizeof(R);
     return 0.5 * cells * per_cell;
 }
 
 // Consecutive colour passes of one level, bracketed by one pair of HIP
 // events on the context stream when that level is timed (a timed block holds
 // n launches and their summed algorithmic bytes).
-void timed_passes(afg_ctx *c, const DevTree &T, const LevelArgs &L, int lvl, const MgVars &V,
+template <typename R>
+void timed_passes(afg_ctx *c, const DevTree &T, const LevelArgs &L, int lvl, const MgVarsT<R> &V,
                   int n_pass, int first_fly)
 {
     const bool on = c->timing && n_pass > 0 && (c->timing_lvl < c->min_lvl || lvl == c->timing_lvl);
@@ -452,14 +465,19 @@ int collect_timing(afg_ctx *c)
 
 // ---- stages (m_aX_mg.f90) ----
 
-struct Mg {
+// A solve's state: the context, its device tables, the variables (R = double:
+// the cell variables; R = float: their FP32 copies) and the options.
+template <typename R>
+struct MgT {
     afg_ctx *c;
     DevTree T;
-    MgVars V;
+    MgVarsT<R> V;
     const afg_mg_opts *o;
 };
+using Mg = MgT<double>;
 
-int make_mg(afg_ctx *c, const afg_mg_opts *o, Mg *m)
+template <typename R>
+int make_mg(afg_ctx *c, const afg_mg_opts *o, MgT<R> *m)
 {
     if (!o) return c->fail(AFG_ERR_ARG, "null options");
     if (!o->initialized) return c->fail(AFG_ERR_STATE, "check_mg: you haven't called mg%d_init_mg", c->dim);
@@ -472,7 +490,7 @@ int make_mg(afg_ctx *c, const afg_mg_opts *o, Mg *m)
     m->V.eps = nullptr;
     m->V.op_auto = o->op == AFG_OP_AUTO;
     if (m->V.op_auto) {
-        double *e = nullptr;
+        R *e = nullptr;
         if (int r = ensure_var(c, o->i_eps, &e)) return r;
         m->V.eps = e;
     }
@@ -491,7 +509,8 @@ void set_tags(Mg &m)
         launch_set_tags(m.c->dim, m.c->stream, m.T, level_args(m.c, l), m.V.eps);
 }
 
-void fill_gc(Mg &m, double *var, int lvl)
+template <typename R>
+void fill_gc(MgT<R> &m, R *var, int lvl)
 {
     launch_fill_gc(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), var);
 }
@@ -503,14 +522,16 @@ void fill_gc(Mg &m, double *var, int lvl)
 // remains, which the caller launches or fuses into the next stage.  The
 // first pass reads the ghosts in memory (first_fly = 0), as the reference
 // does, or computes them when a ghost fill precedes it (first_fly = 1).
-void passes(Mg &m, int lvl, int n_cycle, int first_fly)
+template <typename R>
+void passes(MgT<R> &m, int lvl, int n_cycle, int first_fly)
 {
     const LevelArgs L = level_args(m.c, lvl);
     timed_passes(m.c, m.T, L, lvl, m.V, 2 * n_cycle, first_fly);
 }
 
 // gsrb_boxes as the reference has it: passes, then the last ghost fill.
-void gsrb_level(Mg &m, int lvl, int n_cycle)
+template <typename R>
+void gsrb_level(MgT<R> &m, int lvl, int n_cycle)
 {
     passes(m, lvl, n_cycle, 0);
     if (n_cycle > 0) fill_gc(m, m.V.phi, lvl);
@@ -521,14 +542,16 @@ void gsrb_level(Mg &m, int lvl, int n_cycle)
 // stores them (that fill is not launched: nothing else reads those ghosts
 // before the upward sweep fills lvl again), and the ghost fill of lvl-1 runs
 // inside the parent stage.
-void update_coarse(Mg &m, int lvl, int relaxed)
+template <typename R>
+void update_coarse(MgT<R> &m, int lvl, int relaxed)
 {
     launch_uc_child(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), m.V, 1, relaxed ? 1 : 0);
     launch_uc_parent(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl - 1), m.V, 1);
 }
 
 // set_coarse_phi_rhs (m_aX_mg.f90:484-517)
-void set_coarse_phi_rhs(Mg &m, int lvl)
+template <typename R>
+void set_coarse_phi_rhs(MgT<R> &m, int lvl)
 {
     if (lvl == m.c->highest_lvl) fill_gc(m, m.V.phi, lvl);
     launch_uc_child(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), m.V, 0, 0);
@@ -536,7 +559,8 @@ void set_coarse_phi_rhs(Mg &m, int lvl)
 }
 
 // init_phi_rhs (m_aX_mg.f90:520-543)
-void init_phi_rhs(Mg &m)
+template <typename R>
+void init_phi_rhs(MgT<R> &m)
 {
     for (int lvl = m.c->highest_lvl; lvl >= m.c->min_lvl + 1; lvl--) {
         const LevelArgs L = level_args(m.c, lvl);
@@ -549,7 +573,8 @@ void init_phi_rhs(Mg &m)
 // by relaxation and the ghost fill of lvl: the correction is formed on the
 // fly from the parents' phi and tmp, and the parents' tmp = phi - tmp is
 // written with the final ghost fill of lvl (nothing reads it in between).
-void correct_relax(Mg &m, int lvl, int n_cycle)
+template <typename R>
+void correct_relax(MgT<R> &m, int lvl, int n_cycle)
 {
     launch_correct(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), m.V, 1);
     passes(m, lvl, n_cycle, 1);
@@ -559,7 +584,8 @@ void correct_relax(Mg &m, int lvl, int n_cycle)
 // residual_box over level lvl, for the trees whose residual is stored: in a
 // forest those with highest level below res_upper (a tail of the level's
 // active boxes, deepest trees coming first)
-void residual_level(Mg &m, int lvl)
+template <typename R>
+void residual_level(MgT<R> &m, int lvl)
 {
     afg_ctx *c = m.c;
     LevelArgs a = level_args(c, lvl);
@@ -572,7 +598,8 @@ void residual_level(Mg &m, int lvl)
     launch_residual(c->dim, c->stream, m.T, a, m.V);
 }
 
-BottomArgs bottom_args(Mg &m, int top)
+template <typename R>
+BottomArgs bottom_args(MgT<R> &m, int top)
 {
     afg_ctx *c = m.c;
     BottomArgs B;
@@ -584,11 +611,17 @@ BottomArgs bottom_args(Mg &m, int top)
 }
 
 // highest level run by the bottom kernel for these options
-int bottom_top(const Mg &m) { return m.c->bottom_lvl[m.V.op_auto ? 1 : 0]; }
+template <typename R>
+int bottom_top(const MgT<R> &m) { return m.c->bottom_lvl[m.V.op_auto ? 1 : 0]; }
+
+// index of the bottom kernel table for these values and options
+template <typename R>
+int bottom_table(const MgT<R> &m) { return (sizeof(R) == sizeof(float) ? 2 : 0) + (m.V.op_auto ? 1 : 0); }
 
 // mg$D_fas_vcycle (m_aX_mg.f90:208-255).  Levels up to bottom_top run in the
 // one-workgroup LDS bottom kernel; the levels above, stage by stage.
-void vcycle_body(Mg &m, int highest_lvl, int set_residual)
+template <typename R>
+void vcycle_body(MgT<R> &m, int highest_lvl, int set_residual)
 {
     afg_ctx *c = m.c;
     // the bottom kernel's tables cover every tree: used when all take part
@@ -599,15 +632,15 @@ void vcycle_body(Mg &m, int highest_lvl, int set_residual)
         update_coarse(m, lvl, m.o->n_cycle_down > 0);
     }
     if (lb >= c->min_lvl) {
-        launch_bottom(c->dim, c->stream, m.T, m.V, c->d_bottom[m.V.op_auto ? 1 : 0], bottom_args(m, lb),
-                      c->bottom_lds[m.V.op_auto ? 1 : 0], c->bottom_threads);
+        launch_bottom(c->dim, c->stream, m.T, m.V, c->d_bottom[bottom_table(m)], bottom_args(m, lb),
+                      c->bottom_lds[bottom_table(m)], c->bottom_threads);
     } else {
         gsrb_level(m, c->min_lvl, m.o->n_cycle_base);
     }
     for (int lvl = low + 1; lvl <= highest_lvl; lvl++) correct_relax(m, lvl, m.o->n_cycle_up);
     if (set_residual) {
         if (lb >= c->min_lvl && c->res_upper > c->highest_lvl) {
-            launch_bottom_residual(c->dim, c->stream, m.T, m.V, c->d_bottom[m.V.op_auto ? 1 : 0],
+            launch_bottom_residual(c->dim, c->stream, m.T, m.V, c->d_bottom[bottom_table(m)],
                                    bottom_args(m, lb), c->bottom_threads);
             for (int lvl = lb + 1; lvl <= highest_lvl; lvl++) residual_level(m, lvl);
         } else {
@@ -622,7 +655,8 @@ void vcycle_body(Mg &m, int highest_lvl, int set_residual)
 // `bound` and `res_upper` select the trees of a forest: the V-cycle runs on
 // trees whose highest level is >= bound and stores the residual of those
 // below res_upper (one tree: bound = min_lvl, res_upper above the highest).
-int run_vcycle(Mg &m, int highest_lvl, int set_residual, int bound, int res_upper)
+template <typename R>
+int run_vcycle(MgT<R> &m, int highest_lvl, int set_residual, int bound, int res_upper)
 {
     afg_ctx *c = m.c;
     struct Scope {   // the selection holds for this V-cycle only
@@ -645,7 +679,8 @@ int run_vcycle(Mg &m, int highest_lvl, int set_residual, int bound, int res_uppe
     // again, which by then has long finished, so the stream never stalls.
     constexpr unsigned TIMING_RING = 8;
     auto key = std::make_tuple(highest_lvl, set_residual, o->i_phi, o->i_rhs, o->i_tmp, o->i_eps,
-                               o->op, o->n_cycle_down * 1000000 + o->n_cycle_up * 1000, o->n_cycle_base,
+                               o->op + 10 * (int)sizeof(R), o->n_cycle_down * 1000000 + o->n_cycle_up * 1000,
+                               o->n_cycle_base,
                                c->timing ? 1 + (int)(c->timing_ring++ % TIMING_RING) : 0,
                                c->bound * 1000 + c->res_upper);
     auto it = c->graphs.find(key);
@@ -675,6 +710,30 @@ int run_vcycle(Mg &m, int highest_lvl, int set_residual, int bound, int res_uppe
     return 0;
 }
 
+// mg$D_fas_fmg (m_aX_mg.f90:164-203)
+template <typename R>
+int fas_fmg(MgT<R> &m, int set_residual, int have_guess)
+{
+    afg_ctx *c = m.c;
+    if (have_guess) {
+        for (int lvl = c->highest_lvl; lvl >= c->min_lvl + 1; lvl--) set_coarse_phi_rhs(m, lvl);
+    } else {
+        init_phi_rhs(m);
+    }
+    for (int lvl = c->min_lvl; lvl <= c->highest_lvl; lvl++) {
+        const LevelArgs L = level_args(c, lvl);
+        launch_copy_boxes(c->dim, c->stream, m.T, L.slot0, L.n, L.nc, m.V.phi, m.V.tmp);
+        if (lvl > c->min_lvl) {
+            // correct_children + fill_gc_phi (m_aX_mg.f90:193-196)
+            correct_relax(m, lvl, 0);
+        }
+        // the V-cycle of FMG step lvl runs on the trees reaching lvl; a tree
+        // stores its residual in its last step, lvl == its highest level
+        if (int e = run_vcycle(m, lvl, set_residual ? 1 : 0, lvl, lvl + 1)) return e;
+    }
+    return 0;
+}
+
 int check_level(afg_ctx *c, int lvl, bool need_parent)
 {
     if (lvl < c->min_lvl || lvl > c->highest_lvl)
@@ -689,7 +748,8 @@ int check_level(afg_ctx *c, int lvl, bool need_parent)
 // lowest of them whose phi, rhs, tmp (and eps, for the auto operators) also
 // fit there are copied in as well, the others are used in place in the HBM
 // pools.  Levels of at most bottom_wave_cells cells each, from the lowest
-// up, run in one wavefront.  Two tables are kept: 3 and 4 variables in LDS.
+// up, run in one wavefront.  Four tables are kept: FP64 or FP32 values, 3 or
+// 4 variables in LDS.
 int choose_bottom(afg_ctx *c)
 {
     const int nl = (int)c->levels.size();
@@ -717,8 +777,9 @@ int choose_bottom(afg_ctx *c)
     }
     const int table_bytes =
         (int)((((int64_t)n_slots * (c->nnb + 2) + 2 * n_bc + n_par) * 4 + 2 * n_slots + 7) / 8 * 8);
-    for (int k = 0; k < 2; k++) {
-        const int nv = 3 + k;
+    for (int k = 0; k < 4; k++) {
+        const int nv = 3 + (k & 1);
+        const int64_t elem = k >= 2 ? (int64_t)sizeof(float) : (int64_t)sizeof(double);
         BottomLevels h;
         std::memset(&h, 0, sizeof h);
         h.n_slots = n_slots;
@@ -737,7 +798,7 @@ int choose_bottom(afg_ctx *c)
             par_off += h.par[li].n;
             h.box_len[li] = box_len(c->dim, L.nc);
             const int64_t add = (int64_t)L.n * h.box_len[li];
-            fits = fits && table_bytes + (per_var + add) * nv * (int64_t)sizeof(double) <= c->lds_limit;
+            fits = fits && table_bytes + (per_var + add) * nv * elem <= c->lds_limit;
             h.in_lds[li] = fits ? 1 : 0;
             if (fits) per_var += add;
         }
@@ -750,10 +811,10 @@ int choose_bottom(afg_ctx *c)
             }
         }
         c->bottom_lvl[k] = top >= 0 ? c->levels[top].lvl : -1000;
-        c->bottom_lds[k] = (size_t)table_bytes + (size_t)(per_var * nv * sizeof(double));
+        c->bottom_lds[k] = (size_t)table_bytes + (size_t)(per_var * nv * elem);
         c->h_bottom[k] = h;
     }
-    const size_t lds_max = std::max(c->bottom_lds[0], c->bottom_lds[1]);
+    const size_t lds_max = *std::max_element(c->bottom_lds, c->bottom_lds + 4);
     if (top >= 0 && bottom_prepare(c->dim, lds_max) != 0)
         return c->fail(AFG_ERR_HIP, "cannot allow %zu bytes of LDS for the bottom kernel", lds_max);
     for (auto &g : c->graphs) destroy_graph(g.second);
@@ -771,7 +832,7 @@ int build_base(afg_ctx *c)
     const int nc = L0.nc, s = nc + 2;
     const int ncells = L0.n * (dim == 2 ? nc * nc : nc * nc * nc);
     if (ncells > c->base_cap) {
-        for (int k = 0; k < 2; k++) {
+        for (int k = 0; k < 4; k++) {
             hipFree(c->d_base[k]);
             c->d_base[k] = nullptr;
             HIP_TRY(c, hipMalloc(&c->d_base[k], (size_t)ncells * sizeof(BaseCell)));
@@ -779,7 +840,7 @@ int build_base(afg_ctx *c)
         c->base_cap = ncells;
     }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    for (int k = 0; k < 2; k++) {
+    for (int k = 0; k < 4; k++) {
         BottomLevels &h = c->h_bottom[k];
         if (c->bottom_lvl[k] < c->min_lvl) {
             h.base = nullptr;
@@ -891,6 +952,7 @@ int afg_ctx_create(int dim, int n_cell, int n_var, afg_ctx **out)
     // each box array padded to a multiple of 16 doubles (128 B)
     c->stride = (box_len(dim, n_cell) + 15) / 16 * 16;
     c->d_var.assign(n_var, nullptr);
+    c->d_var32.assign(n_var, nullptr);
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&c->d_out, 8 * sizeof(double));
     if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_out, 8 * sizeof(double), hipHostMallocDefault);
@@ -1213,8 +1275,7 @@ int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base,
         HIP_H2D(c, c->d_tree_chunk, toff.data(), toff.size() * sizeof(int32_t));
     }
     c->have_tree = true;
-    HIP_TRY(c, hipMalloc(&c->d_bottom[0], sizeof(BottomLevels)));
-    HIP_TRY(c, hipMalloc(&c->d_bottom[1], sizeof(BottomLevels)));
+    for (int k = 0; k < 4; k++) HIP_TRY(c, hipMalloc(&c->d_bottom[k], sizeof(BottomLevels)));
     if (int e = choose_bottom(c)) return e;
     HIP_TRY(c, hipMalloc(&c->d_partial, c->partial_cap * sizeof(double)));
     c->bc_dirty = true;
@@ -1403,23 +1464,123 @@ int afg_fas_fmg(afg_ctx *c, const afg_mg_opts *o, int set_residual, int have_gue
     Mg m;
     if (int e = make_mg(c, o, &m)) return e;
     set_tags(m);
-    // mg$D_fas_fmg (m_aX_mg.f90:164-203)
-    if (have_guess) {
-        for (int lvl = c->highest_lvl; lvl >= c->min_lvl + 1; lvl--) set_coarse_phi_rhs(m, lvl);
-    } else {
-        init_phi_rhs(m);
-    }
-    for (int lvl = c->min_lvl; lvl <= c->highest_lvl; lvl++) {
-        const LevelArgs L = level_args(c, lvl);
-        launch_copy_boxes(c->dim, c->stream, m.T, L.slot0, L.n, L.nc, m.V.phi, m.V.tmp);
-        if (lvl > c->min_lvl) {
-            // correct_children + fill_gc_phi (m_aX_mg.f90:193-196)
-            correct_relax(m, lvl, 0);
+    return fas_fmg(m, set_residual, have_guess);
+}
+
+// ---- FP32 stages and the mixed-precision solve ----
+
+int afg_fp32_from_cc(afg_ctx *c, int iv, int iv32)
+{
+    if (int e = check_ctx(c)) return e;
+    double *a = nullptr;
+    float *b = nullptr;
+    if (int e = ensure_var(c, iv, &a)) return e;
+    if (int e = ensure_var(c, iv32, &b)) return e;
+    launch_to_fp32(c->stream, (long long)c->n_slots * c->stride, a, b);
+    return 0;
+}
+
+int afg_fp32_to_cc(afg_ctx *c, int iv32, int iv, int add)
+{
+    if (int e = check_ctx(c)) return e;
+    double *a = nullptr;
+    float *b = nullptr;
+    if (int e = ensure_var(c, iv, &a)) return e;
+    if (int e = ensure_var(c, iv32, &b)) return e;
+    launch_from_fp32(c->stream, (long long)c->n_slots * c->stride, b, a, add ? 1 : 0);
+    return 0;
+}
+
+// FP32 solves need the tags of the auto operators, set from the FP64 eps
+static int fp32_mg(afg_ctx *c, const afg_mg_opts *o, MgT<float> *m32)
+{
+    Mg m64;
+    if (int e = make_mg(c, o, &m64)) return e;
+    set_tags(m64);
+    return make_mg(c, o, m32);
+}
+
+int afg_fas_fmg_fp32(afg_ctx *c, const afg_mg_opts *o, int set_residual, int have_guess)
+{
+    if (int e = check_ctx(c)) return e;
+    MgT<float> m;
+    if (int e = fp32_mg(c, o, &m)) return e;
+    return fas_fmg(m, set_residual, have_guess);
+}
+
+int afg_fas_vcycle_fp32(afg_ctx *c, const afg_mg_opts *o, int highest_lvl, int set_residual)
+{
+    if (int e = check_ctx(c)) return e;
+    if (int e = check_level(c, highest_lvl, false)) return e;
+    MgT<float> m;
+    if (int e = fp32_mg(c, o, &m)) return e;
+    return run_vcycle(m, highest_lvl, set_residual ? 1 : 0, c->min_lvl, c->highest_lvl + 1);
+}
+
+static int tree_extreme(afg_ctx *c, int iv, int include_parents, int want_max, double *out);
+
+// max |variable iv| over leaf interiors
+static int leaf_max_abs(afg_ctx *c, int iv, double *out)
+{
+    double hi = 0, lo = 0;
+    if (int e = tree_extreme(c, iv, 0, 1, &hi)) return e;
+    if (int e = tree_extreme(c, iv, 0, 0, &lo)) return e;
+    *out = std::max(std::fabs(hi), std::fabs(lo));
+    return 0;
+}
+
+int afg_mixed_solve(afg_ctx *c, const afg_mg_opts *o, int have_guess, double tol, int max_iter,
+                    int *iterations, double *rel_residual)
+{
+    if (int e = check_ctx(c)) return e;
+    if (c->n_trees != 1) return c->fail(AFG_ERR_UNSUPPORTED, "mixed-precision solve of a forest");
+    if (!(tol > 0) || max_iter < 0) return c->fail(AFG_ERR_ARG, "tol must be > 0 and max_iter >= 0");
+    Mg m;
+    if (int e = make_mg(c, o, &m)) return e;
+    set_tags(m);
+    MgT<float> f;
+    if (int e = make_mg(c, o, &f)) return e;
+    const long long n = (long long)c->n_slots * c->stride;
+    if (m.V.op_auto) launch_to_fp32(c->stream, n, m.V.eps, const_cast<float *>(f.V.eps));
+    if (!have_guess)
+        for (int lvl = c->min_lvl; lvl <= c->highest_lvl; lvl++) {
+            const LevelArgs L = level_args(c, lvl);
+            launch_fill_value(c->dim, c->stream, m.T, L.slot0, L.n, L.nc, m.V.phi, 0.0);
         }
-        // the V-cycle of FMG step lvl runs on the trees reaching lvl; a tree
-        // stores its residual in its last step, lvl == its highest level
-        if (int e = run_vcycle(m, lvl, set_residual ? 1 : 0, lvl, lvl + 1)) return e;
+    double rhs_norm = 0;
+    if (int e = leaf_max_abs(c, o->i_rhs, &rhs_norm)) return e;
+    // The FP64 residual: phi of every parent set to the restriction of its
+    // children (where the FAS iteration converges to, its corrections
+    // vanishing), then the ghost fill from the lowest level up
+    // (refinement-boundary ghosts read the parent's) and rhs - A phi.
+    auto residual = [&] {
+        for (int lvl = c->highest_lvl; lvl > c->min_lvl; lvl--)
+            launch_restrict(c->dim, c->stream, m.T, level_args(c, lvl), m.V.phi, m.V.phi);
+        for (int lvl = c->min_lvl; lvl <= c->highest_lvl; lvl++) {
+            fill_gc(m, m.V.phi, lvl);
+            residual_level(m, lvl);
+        }
+    };
+    residual();
+    int it = 0;
+    double res = 0;
+    for (;; it++) {
+        if (int e = leaf_max_abs(c, o->i_tmp, &res)) return e;
+        if (res <= tol * rhs_norm || it >= max_iter) break;
+        // correction e from A e = r in FP32 (a V-cycle from e = 0; FMG first)
+        launch_to_fp32(c->stream, n, m.V.tmp, f.V.rhs);
+        HIP_TRY(c, hipMemsetAsync(f.V.phi, 0, n * sizeof(float), c->stream));
+        HIP_TRY(c, hipMemsetAsync(f.V.tmp, 0, n * sizeof(float), c->stream));
+        if (it == 0) {
+            if (int e = fas_fmg(f, 0, 0)) return e;
+        } else {
+            if (int e = run_vcycle(f, c->highest_lvl, 0, c->min_lvl, c->highest_lvl + 1)) return e;
+        }
+        launch_from_fp32(c->stream, n, f.V.phi, m.V.phi, 1);
+        residual();
     }
+    if (iterations) *iterations = it;
+    if (rel_residual) *rel_residual = rhs_norm > 0 ? res / rhs_norm : res;
     return 0;
 }
 
@@ -1628,6 +1789,8 @@ int afg_device_bytes(const afg_ctx *cc, int64_t *out)
     int64_t b = 0;
     for (double *p : c->d_var)
         if (p) b += (int64_t)c->n_slots * c->stride * 8;
+    for (float *p : c->d_var32)
+        if (p) b += (int64_t)c->n_slots * c->stride * 4;
     b += (int64_t)c->n_slots * (c->nnb + 3) * 4 + c->n_slots;
     *out = b;
     return 0;

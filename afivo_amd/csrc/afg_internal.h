@@ -9,30 +9,47 @@
 namespace afg {
 
 // ---- launchers (afg_kernels.hip) ----
-void launch_fill_gc(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, double *var);
+// R, the value type: double, or float for the FP32 stages of the
+// mixed-precision solve (both are instantiated).
+template <typename R>
+void launch_fill_gc(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, R *var);
 // ghost fill of L and tmp = phi - tmp on the parents P of the level below
+template <typename R>
 void launch_fill_diff(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                      const ListArgs &P, const MgVars &V);
-void launch_diff(int dim, hipStream_t s, const DevTree &T, const ListArgs &P, const MgVars &V);
+                      const ListArgs &P, const MgVarsT<R> &V);
+template <typename R>
+void launch_diff(int dim, hipStream_t s, const DevTree &T, const ListArgs &P, const MgVarsT<R> &V);
+template <typename R>
 void launch_gsrb_pass(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                      const MgVars &V, int rb, int fly);
+                      const MgVarsT<R> &V, int rb, int fly);
+template <typename R>
 void launch_residual(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                     const MgVars &V);
+                     const MgVarsT<R> &V);
+template <typename R>
 void launch_uc_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                     const MgVars &V, int keep_child_tmp, int fly);
+                     const MgVarsT<R> &V, int keep_child_tmp, int fly);
 // L: the parent level (all its boxes get their ghosts filled)
+template <typename R>
 void launch_uc_parent(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                      const MgVars &V, int copy_tmp);
+                      const MgVarsT<R> &V, int copy_tmp);
+template <typename R>
 void launch_correct(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                    const MgVars &V, int fly_diff);
+                    const MgVarsT<R> &V, int fly_diff);
+template <typename R>
 void launch_restrict(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                     const double *from, double *to);
+                     const R *from, R *to);
+template <typename R>
 void launch_prolong2(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                     const double *from, double *to, int add);
+                     const R *from, R *to, int add);
+template <typename R>
 void launch_fill_value(int dim, hipStream_t s, const DevTree &T, int slot0, int n_slots,
-                       int nc, double *var, double value);
+                       int nc, R *var, double value);
+template <typename R>
 void launch_copy_boxes(int dim, hipStream_t s, const DevTree &T, int slot0, int n_slots, int nc,
-                       const double *from, double *to);
+                       const R *from, R *to);
+// n values of a pool to FP32, and back (optionally added to the FP64 values)
+void launch_to_fp32(hipStream_t s, long long n, const double *from, float *to);
+void launch_from_fp32(hipStream_t s, long long n, const float *from, double *to, int add);
 void launch_set_tags(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
                      const double *eps);
 // Extreme (max or min) of the interiors of the boxes of a slot list: one
@@ -48,9 +65,11 @@ void launch_reduce_partials(hipStream_t s, const double *partial, int n, int wan
                             double *out);
 // bottom kernel: allow lds_bytes of dynamic LDS (0 on success), then launch
 int bottom_prepare(int dim, size_t lds_bytes);
-void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVars &V,
+template <typename R>
+void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVarsT<R> &V,
                    const BottomLevels *B, const BottomArgs &A, size_t lds_bytes, int threads);
-void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVars &V,
+template <typename R>
+void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVarsT<R> &V,
                             const BottomLevels *B, const BottomArgs &A, int threads);
 
 }  // namespace afg

@@ -19,21 +19,23 @@ inline int n_blocks(long long threads, int block = BLOCK)
     return (int)((threads + block - 1) / block);
 }
 
-LevelView pool_view(const DevTree &T, const MgVars &V)
+template <typename R>
+LevelViewT<R> pool_view(const DevTree &T, const MgVarsT<R> &V)
 {
-    return LevelView{V.phi, V.rhs, V.tmp, V.eps, 0, T.stride};
+    return LevelViewT<R>{V.phi, V.rhs, V.tmp, V.eps, 0, T.stride};
 }
 
-LevelView var_view(const DevTree &T, double *var)
+template <typename R>
+LevelViewT<R> var_view(const DevTree &T, R *var)
 {
-    return LevelView{var, nullptr, nullptr, nullptr, 0, T.stride};
+    return LevelViewT<R>{var, nullptr, nullptr, nullptr, 0, T.stride};
 }
 
 // ---------------------------------------------------------------------------
 // kernels over the HBM pools: one launch per stage and level
 
-template <int DIM, int NC>
-__global__ __launch_bounds__(BLOCK) void k_fill_gc(DevTree T, LevelArgs L, LevelView v)
+template <int DIM, int NC, typename R>
+__global__ __launch_bounds__(BLOCK) void k_fill_gc(DevTree T, LevelArgs L, LevelViewT<R> v)
 {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid < fill_threads<DIM>(L.n, NC ? NC : L.nc)) fill_body<DIM, NC>(T, L, v, v, gid);
@@ -41,9 +43,9 @@ __global__ __launch_bounds__(BLOCK) void k_fill_gc(DevTree T, LevelArgs L, Level
 
 // ghost fill of level L and, in the same launch, tmp = phi - tmp on the
 // parents P of the level below (independent work: the fill reads phi only)
-template <int DIM, int NC>
+template <int DIM, int NC, typename R>
 __global__ __launch_bounds__(BLOCK) void k_fill_diff(DevTree T, LevelArgs L, ListArgs P,
-                                                     LevelView v)
+                                                     LevelViewT<R> v)
 {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int nf = fill_threads<DIM>(L.n, NC ? NC : L.nc);
@@ -55,15 +57,15 @@ __global__ __launch_bounds__(BLOCK) void k_fill_diff(DevTree T, LevelArgs L, Lis
     }
 }
 
-template <int DIM, int NC>
-__global__ __launch_bounds__(BLOCK) void k_diff(DevTree T, ListArgs P, LevelView v)
+template <int DIM, int NC, typename R>
+__global__ __launch_bounds__(BLOCK) void k_diff(DevTree T, ListArgs P, LevelViewT<R> v)
 {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid < P.n * ipow((NC ? NC : P.nc) + 2, DIM)) diff_body<DIM, NC>(T, P, v, gid);
 }
 
-template <int DIM, int NC>
-__global__ __launch_bounds__(BLOCK) void k_gsrb_pass(DevTree T, LevelArgs L, LevelView v,
+template <int DIM, int NC, typename R>
+__global__ __launch_bounds__(BLOCK) void k_gsrb_pass(DevTree T, LevelArgs L, LevelViewT<R> v,
                                                      int op_auto, int rb, int fly)
 {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -72,29 +74,29 @@ __global__ __launch_bounds__(BLOCK) void k_gsrb_pass(DevTree T, LevelArgs L, Lev
 }
 
 // one colour pass by tiles of boxes in LDS (NC in 2, 4, 8, 16)
-template <int DIM, int NC, int FLY>
-__global__ __launch_bounds__(BLOCK) void k_gsrb_tile(DevTree T, LevelArgs L, LevelView v,
+template <int DIM, int NC, int FLY, typename R>
+__global__ __launch_bounds__(BLOCK) void k_gsrb_tile(DevTree T, LevelArgs L, LevelViewT<R> v,
                                                      int op_auto, int rb)
 {
     using G = GsrbTile<DIM, NC>;
     static_assert(G::THREADS == BLOCK, "tile size");
-    __shared__ double lds[G::BOXES * G::LEN];
-    double rhs[G::PER];
+    __shared__ R lds[G::BOXES * G::LEN];
+    R rhs[G::PER];
     const int box0 = xcd_tile(blockIdx.x, gridDim.x) * G::BOXES;
     tile_stage_a<DIM, NC, FLY>(T, L, v, lds, box0, rb, threadIdx.x, rhs);
     __syncthreads();
     tile_stage_b<DIM, NC>(T, L, v, lds, box0, op_auto, rb, threadIdx.x, rhs);
 }
 
-template <int DIM, int NC>
-__global__ __launch_bounds__(BLOCK) void k_residual(DevTree T, LevelArgs L, LevelView v, int op_auto)
+template <int DIM, int NC, typename R>
+__global__ __launch_bounds__(BLOCK) void k_residual(DevTree T, LevelArgs L, LevelViewT<R> v, int op_auto)
 {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid < L.n * ipow(NC ? NC : L.nc, DIM)) residual_body<DIM, NC>(T, L, v, v, op_auto, gid);
 }
 
-template <int DIM, int NC>
-__global__ __launch_bounds__(BLOCK) void k_uc_child(DevTree T, LevelArgs L, LevelView v,
+template <int DIM, int NC, typename R>
+__global__ __launch_bounds__(BLOCK) void k_uc_child(DevTree T, LevelArgs L, LevelViewT<R> v,
                                                     int op_auto, int keep_child_tmp, int fly)
 {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -102,8 +104,8 @@ __global__ __launch_bounds__(BLOCK) void k_uc_child(DevTree T, LevelArgs L, Leve
         uc_child_body<DIM, NC>(T, L, v, v, op_auto, keep_child_tmp, fly, gid);
 }
 
-template <int DIM, int NC>
-__global__ __launch_bounds__(BLOCK) void k_uc_parent(DevTree T, LevelArgs L, LevelView v,
+template <int DIM, int NC, typename R>
+__global__ __launch_bounds__(BLOCK) void k_uc_parent(DevTree T, LevelArgs L, LevelViewT<R> v,
                                                      int op_auto, int copy_tmp)
 {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -111,8 +113,8 @@ __global__ __launch_bounds__(BLOCK) void k_uc_parent(DevTree T, LevelArgs L, Lev
         uc_parent_body<DIM, NC>(T, L, v, v, op_auto, copy_tmp, gid);
 }
 
-template <int DIM, int NC>
-__global__ __launch_bounds__(BLOCK) void k_correct(DevTree T, LevelArgs L, LevelView v,
+template <int DIM, int NC, typename R>
+__global__ __launch_bounds__(BLOCK) void k_correct(DevTree T, LevelArgs L, LevelViewT<R> v,
                                                    int op_auto, int fly_diff)
 {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -121,17 +123,16 @@ __global__ __launch_bounds__(BLOCK) void k_correct(DevTree T, LevelArgs L, Level
 }
 
 // a$D_restrict_box and a$D_prolong2 over the boxes of a level
-template <int DIM, int NC>
-__global__ __launch_bounds__(BLOCK) void k_restrict(DevTree T, LevelArgs L, const double *from,
-                                                    double *to)
+template <int DIM, int NC, typename R>
+__global__ __launch_bounds__(BLOCK) void k_restrict(DevTree T, LevelArgs L, const R *from, R *to)
 {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid < L.n * ipow((NC ? NC : L.nc) / 2, DIM)) restrict_body<DIM, NC>(T, L, from, to, gid);
 }
 
-template <int DIM, int NC>
-__global__ __launch_bounds__(BLOCK) void k_prolong2(DevTree T, LevelArgs L, const double *from,
-                                                    double *to, int add)
+template <int DIM, int NC, typename R>
+__global__ __launch_bounds__(BLOCK) void k_prolong2(DevTree T, LevelArgs L, const R *from, R *to,
+                                                    int add)
 {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid < L.n * ipow((NC ? NC : L.nc) / 2, DIM))
@@ -140,8 +141,9 @@ __global__ __launch_bounds__(BLOCK) void k_prolong2(DevTree T, LevelArgs L, cons
 
 // whole-array fill and copy over a contiguous slot range
 // (a$D_box_clear_cc, m_aX_utils.f90:262-270; a$D_box_copy_cc, :331-339)
-__global__ __launch_bounds__(BLOCK) void k_fill_value(int stride, int slot0, int n, int len,
-                                                      double *var, double value)
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_fill_value(int stride, int slot0, int n, int len, R *var,
+                                                      R value)
 {
     const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= (long long)n * len) return;
@@ -150,8 +152,9 @@ __global__ __launch_bounds__(BLOCK) void k_fill_value(int stride, int slot0, int
     var[(size_t)(slot0 + b) * stride + q] = value;
 }
 
+template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_copy_boxes(int stride, int slot0, int n, int len,
-                                                      const double *from, double *to)
+                                                      const R *from, R *to)
 {
     const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= (long long)n * len) return;
@@ -159,6 +162,20 @@ __global__ __launch_bounds__(BLOCK) void k_copy_boxes(int stride, int slot0, int
     const int q = (int)(gid - (long long)b * len);
     const size_t o = (size_t)(slot0 + b) * stride + q;
     to[o] = from[o];
+}
+
+// FP32 copies of a variable pool and back (mixed-precision solve):
+// to[i] = float(from[i]), or to[i] = (add ? to[i] : 0) + double(from[i])
+__global__ __launch_bounds__(BLOCK) void k_to_fp32(long long n, const double *from, float *to)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) to[i] = (float)from[i];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_from_fp32(long long n, const float *from, double *to, int add)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) to[i] = add ? to[i] + (double)from[i] : (double)from[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -285,8 +302,8 @@ __global__ __launch_bounds__(BLOCK) void k_tree_reduce(const int32_t *tree_chunk
 // bottom of the V-cycle in one workgroup, on LDS copies of the small levels
 // (bottom_run), and the residual of those levels (bottom_residual_run)
 
-template <int DIM, int AUTO>
-__global__ __launch_bounds__(BOTTOM_BLOCK) void k_bottom(DevTree T, MgVars V,
+template <int DIM, int AUTO, typename R>
+__global__ __launch_bounds__(BOTTOM_BLOCK) void k_bottom(DevTree T, MgVarsT<R> V,
                                                           const BottomLevels *__restrict__ B,
                                                           BottomArgs A)
 {
@@ -298,16 +315,16 @@ __global__ __launch_bounds__(BOTTOM_BLOCK) void k_bottom(DevTree T, MgVars V,
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     };
-    bottom_run<DIM, AUTO>(T, V, *B, A, lds, threadIdx.x, blockDim.x, warpSize, [] { __syncthreads(); },
-                          wave_sync);
+    bottom_run<DIM, AUTO, R>(T, V, *B, A, lds, threadIdx.x, blockDim.x, warpSize,
+                             [] { __syncthreads(); }, wave_sync);
 }
 
-template <int DIM>
-__global__ __launch_bounds__(BOTTOM_BLOCK) void k_bottom_residual(DevTree T, MgVars V,
+template <int DIM, typename R>
+__global__ __launch_bounds__(BOTTOM_BLOCK) void k_bottom_residual(DevTree T, MgVarsT<R> V,
                                                                    const BottomLevels *__restrict__ B,
                                                                    BottomArgs A)
 {
-    bottom_residual_run<DIM>(T, V, *B, A, threadIdx.x, blockDim.x);
+    bottom_residual_run<DIM, R>(T, V, *B, A, threadIdx.x, blockDim.x);
 }
 
 }  // namespace
@@ -319,32 +336,34 @@ __global__ __launch_bounds__(BOTTOM_BLOCK) void k_bottom_residual(DevTree T, MgV
     do {                                                                           \
         if ((dim) == 2) {                                                          \
             switch (nc) {                                                          \
-            case 2: KERNEL<2, 2><<<GRID, BLOCK, 0, s>>>(__VA_ARGS__); break;       \
-            case 4: KERNEL<2, 4><<<GRID, BLOCK, 0, s>>>(__VA_ARGS__); break;       \
-            case 8: KERNEL<2, 8><<<GRID, BLOCK, 0, s>>>(__VA_ARGS__); break;       \
-            case 16: KERNEL<2, 16><<<GRID, BLOCK, 0, s>>>(__VA_ARGS__); break;     \
-            default: KERNEL<2, 0><<<GRID, BLOCK, 0, s>>>(__VA_ARGS__); break;      \
+            case 2: KERNEL<2, 2, R><<<GRID, BLOCK, 0, s>>>(__VA_ARGS__); break;       \
+            case 4: KERNEL<2, 4, R><<<GRID, BLOCK, 0, s>>>(__VA_ARGS__); break;       \
+            case 8: KERNEL<2, 8, R><<<GRID, BLOCK, 0, s>>>(__VA_ARGS__); break;       \
+            case 16: KERNEL<2, 16, R><<<GRID, BLOCK, 0, s>>>(__VA_ARGS__); break;     \
+            default: KERNEL<2, 0, R><<<GRID, BLOCK, 0, s>>>(__VA_ARGS__); break;      \
             }                                                                      \
         } else {                                                                   \
             switch (nc) {                                                          \
-            case 2: KERNEL<3, 2><<<GRID, BLOCK, 0, s>>>(__VA_ARGS__); break;       \
-            case 4: KERNEL<3, 4><<<GRID, BLOCK, 0, s>>>(__VA_ARGS__); break;       \
-            case 8: KERNEL<3, 8><<<GRID, BLOCK, 0, s>>>(__VA_ARGS__); break;       \
-            case 16: KERNEL<3, 16><<<GRID, BLOCK, 0, s>>>(__VA_ARGS__); break;     \
-            default: KERNEL<3, 0><<<GRID, BLOCK, 0, s>>>(__VA_ARGS__); break;      \
+            case 2: KERNEL<3, 2, R><<<GRID, BLOCK, 0, s>>>(__VA_ARGS__); break;       \
+            case 4: KERNEL<3, 4, R><<<GRID, BLOCK, 0, s>>>(__VA_ARGS__); break;       \
+            case 8: KERNEL<3, 8, R><<<GRID, BLOCK, 0, s>>>(__VA_ARGS__); break;       \
+            case 16: KERNEL<3, 16, R><<<GRID, BLOCK, 0, s>>>(__VA_ARGS__); break;     \
+            default: KERNEL<3, 0, R><<<GRID, BLOCK, 0, s>>>(__VA_ARGS__); break;      \
             }                                                                      \
         }                                                                          \
     } while (0)
 
-void launch_fill_gc(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, double *var)
+template <typename R>
+void launch_fill_gc(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, R *var)
 {
     if (L.n <= 0) return;
     const long long th = dim == 2 ? fill_threads<2>(L.n, L.nc) : fill_threads<3>(L.n, L.nc);
     AFG_DISPATCH(dim, L.nc, k_fill_gc, n_blocks(th), T, L, var_view(T, var));
 }
 
+template <typename R>
 void launch_fill_diff(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                      const ListArgs &P, const MgVars &V)
+                      const ListArgs &P, const MgVarsT<R> &V)
 {
     if (L.n <= 0) return;
     const long long th = (dim == 2 ? fill_threads<2>(L.n, L.nc) : fill_threads<3>(L.n, L.nc)) +
@@ -352,56 +371,60 @@ void launch_fill_diff(int dim, hipStream_t s, const DevTree &T, const LevelArgs 
     AFG_DISPATCH(dim, L.nc, k_fill_diff, n_blocks(th), T, L, P, pool_view(T, V));
 }
 
-void launch_diff(int dim, hipStream_t s, const DevTree &T, const ListArgs &P, const MgVars &V)
+template <typename R>
+void launch_diff(int dim, hipStream_t s, const DevTree &T, const ListArgs &P, const MgVarsT<R> &V)
 {
     if (P.n <= 0) return;
     const long long th = (long long)P.n * ipow(P.nc + 2, dim);
     AFG_DISPATCH(dim, P.nc, k_diff, n_blocks(th), T, P, pool_view(T, V));
 }
 
-template <int DIM, int NC>
-void gsrb_tiles(hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVars &V, int rb, int fly)
+template <int DIM, int NC, typename R>
+void gsrb_tiles(hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int rb, int fly)
 {
     const int grid = (L.n + GsrbTile<DIM, NC>::BOXES - 1) / GsrbTile<DIM, NC>::BOXES;
     if (fly)
-        k_gsrb_tile<DIM, NC, 1><<<grid, BLOCK, 0, s>>>(T, L, pool_view(T, V), V.op_auto, rb);
+        k_gsrb_tile<DIM, NC, 1, R><<<grid, BLOCK, 0, s>>>(T, L, pool_view(T, V), V.op_auto, rb);
     else
-        k_gsrb_tile<DIM, NC, 0><<<grid, BLOCK, 0, s>>>(T, L, pool_view(T, V), V.op_auto, rb);
+        k_gsrb_tile<DIM, NC, 0, R><<<grid, BLOCK, 0, s>>>(T, L, pool_view(T, V), V.op_auto, rb);
 }
 
+template <typename R>
 void launch_gsrb_pass(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                      const MgVars &V, int rb, int fly)
+                      const MgVarsT<R> &V, int rb, int fly)
 {
     if (L.n <= 0) return;
     switch (dim * 100 + L.nc) {
-    case 202: gsrb_tiles<2, 2>(s, T, L, V, rb, fly); return;
-    case 204: gsrb_tiles<2, 4>(s, T, L, V, rb, fly); return;
-    case 208: gsrb_tiles<2, 8>(s, T, L, V, rb, fly); return;
-    case 216: gsrb_tiles<2, 16>(s, T, L, V, rb, fly); return;
-    case 302: gsrb_tiles<3, 2>(s, T, L, V, rb, fly); return;
-    case 304: gsrb_tiles<3, 4>(s, T, L, V, rb, fly); return;
-    case 308: gsrb_tiles<3, 8>(s, T, L, V, rb, fly); return;
-    case 316: gsrb_tiles<3, 16>(s, T, L, V, rb, fly); return;
+    case 202: gsrb_tiles<2, 2, R>(s, T, L, V, rb, fly); return;
+    case 204: gsrb_tiles<2, 4, R>(s, T, L, V, rb, fly); return;
+    case 208: gsrb_tiles<2, 8, R>(s, T, L, V, rb, fly); return;
+    case 216: gsrb_tiles<2, 16, R>(s, T, L, V, rb, fly); return;
+    case 302: gsrb_tiles<3, 2, R>(s, T, L, V, rb, fly); return;
+    case 304: gsrb_tiles<3, 4, R>(s, T, L, V, rb, fly); return;
+    case 308: gsrb_tiles<3, 8, R>(s, T, L, V, rb, fly); return;
+    case 316: gsrb_tiles<3, 16, R>(s, T, L, V, rb, fly); return;
     default: break;
     }
     // other cell counts: one thread per cell of the colour
     const long long th = (long long)L.n * ipow(L.nc, dim) / 2;
     if (dim == 2)
-        k_gsrb_pass<2, 0><<<n_blocks(th), BLOCK, 0, s>>>(T, L, pool_view(T, V), V.op_auto, rb, fly);
+        k_gsrb_pass<2, 0, R><<<n_blocks(th), BLOCK, 0, s>>>(T, L, pool_view(T, V), V.op_auto, rb, fly);
     else
-        k_gsrb_pass<3, 0><<<n_blocks(th), BLOCK, 0, s>>>(T, L, pool_view(T, V), V.op_auto, rb, fly);
+        k_gsrb_pass<3, 0, R><<<n_blocks(th), BLOCK, 0, s>>>(T, L, pool_view(T, V), V.op_auto, rb, fly);
 }
 
+template <typename R>
 void launch_residual(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                     const MgVars &V)
+                     const MgVarsT<R> &V)
 {
     if (L.n <= 0) return;
     const long long th = (long long)L.n * ipow(L.nc, dim);
     AFG_DISPATCH(dim, L.nc, k_residual, n_blocks(th), T, L, pool_view(T, V), V.op_auto);
 }
 
+template <typename R>
 void launch_uc_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                     const MgVars &V, int keep_child_tmp, int fly)
+                     const MgVarsT<R> &V, int keep_child_tmp, int fly)
 {
     if (L.n <= 0) return;
     const long long th = (long long)L.n * ipow(L.nc / 2, dim);
@@ -409,54 +432,70 @@ void launch_uc_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &
                  keep_child_tmp, fly);
 }
 
+template <typename R>
 void launch_uc_parent(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                      const MgVars &V, int copy_tmp)
+                      const MgVarsT<R> &V, int copy_tmp)
 {
     if (L.n <= 0) return;
     const long long th = (long long)L.n * ipow(L.nc + 2, dim);
     AFG_DISPATCH(dim, L.nc, k_uc_parent, n_blocks(th), T, L, pool_view(T, V), V.op_auto, copy_tmp);
 }
 
+template <typename R>
 void launch_correct(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                    const MgVars &V, int fly_diff)
+                    const MgVarsT<R> &V, int fly_diff)
 {
     if (L.n <= 0) return;
     const long long th = (long long)L.n * ipow(L.nc / 2, dim);
     AFG_DISPATCH(dim, L.nc, k_correct, n_blocks(th), T, L, pool_view(T, V), V.op_auto, fly_diff);
 }
 
+template <typename R>
 void launch_restrict(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                     const double *from, double *to)
+                     const R *from, R *to)
 {
     if (L.n <= 0) return;
     const long long th = (long long)L.n * ipow(L.nc / 2, dim);
     AFG_DISPATCH(dim, L.nc, k_restrict, n_blocks(th), T, L, from, to);
 }
 
+template <typename R>
 void launch_prolong2(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                     const double *from, double *to, int add)
+                     const R *from, R *to, int add)
 {
     if (L.n <= 0) return;
     const long long th = (long long)L.n * ipow(L.nc / 2, dim);
     AFG_DISPATCH(dim, L.nc, k_prolong2, n_blocks(th), T, L, from, to, add);
 }
 
+template <typename R>
 void launch_fill_value(int dim, hipStream_t s, const DevTree &T, int slot0, int n_slots, int nc,
-                       double *var, double value)
+                       R *var, double value)
 {
     if (n_slots <= 0) return;
     const int len = ipow(nc + 2, dim);
-    k_fill_value<<<n_blocks((long long)n_slots * len), BLOCK, 0, s>>>(T.stride, slot0, n_slots,
-                                                                      len, var, value);
+    k_fill_value<R><<<n_blocks((long long)n_slots * len), BLOCK, 0, s>>>(T.stride, slot0, n_slots,
+                                                                         len, var, R(value));
 }
 
+template <typename R>
 void launch_copy_boxes(int dim, hipStream_t s, const DevTree &T, int slot0, int n_slots, int nc,
-                       const double *from, double *to)
+                       const R *from, R *to)
 {
     if (n_slots <= 0) return;
     const int len = ipow(nc + 2, dim);
-    k_copy_boxes<<<n_blocks((long long)n_slots * len), BLOCK, 0, s>>>(T.stride, slot0, n_slots,
-                                                                      len, from, to);
+    k_copy_boxes<R><<<n_blocks((long long)n_slots * len), BLOCK, 0, s>>>(T.stride, slot0, n_slots,
+                                                                         len, from, to);
+}
+
+void launch_to_fp32(hipStream_t s, long long n, const double *from, float *to)
+{
+    if (n > 0) k_to_fp32<<<n_blocks(n), BLOCK, 0, s>>>(n, from, to);
+}
+
+void launch_from_fp32(hipStream_t s, long long n, const float *from, double *to, int add)
+{
+    if (n > 0) k_from_fp32<<<n_blocks(n), BLOCK, 0, s>>>(n, from, to, add);
 }
 
 void launch_set_tags(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
@@ -515,8 +554,10 @@ int bottom_prepare(int dim, size_t lds_bytes)
     std::lock_guard<std::mutex> lock(mu);
     size_t &g = granted[dim == 2 ? 0 : 1];
     if (lds_bytes <= g) return 0;
-    const void *fn[2] = {dim == 2 ? (const void *)k_bottom<2, 0> : (const void *)k_bottom<3, 0>,
-                         dim == 2 ? (const void *)k_bottom<2, 1> : (const void *)k_bottom<3, 1>};
+    const void *fn[4] = {dim == 2 ? (const void *)k_bottom<2, 0, double> : (const void *)k_bottom<3, 0, double>,
+                         dim == 2 ? (const void *)k_bottom<2, 1, double> : (const void *)k_bottom<3, 1, double>,
+                         dim == 2 ? (const void *)k_bottom<2, 0, float> : (const void *)k_bottom<3, 0, float>,
+                         dim == 2 ? (const void *)k_bottom<2, 1, float> : (const void *)k_bottom<3, 1, float>};
     for (const void *f : fn)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) != hipSuccess)
             return -1;
@@ -524,29 +565,60 @@ int bottom_prepare(int dim, size_t lds_bytes)
     return 0;
 }
 
-void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVars &V,
+template <typename R>
+void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVarsT<R> &V,
                    const BottomLevels *B, const BottomArgs &A, size_t lds_bytes, int threads)
 {
     if (dim == 2) {
         if (V.op_auto)
-            k_bottom<2, 1><<<1, threads, lds_bytes, s>>>(T, V, B, A);
+            k_bottom<2, 1, R><<<1, threads, lds_bytes, s>>>(T, V, B, A);
         else
-            k_bottom<2, 0><<<1, threads, lds_bytes, s>>>(T, V, B, A);
+            k_bottom<2, 0, R><<<1, threads, lds_bytes, s>>>(T, V, B, A);
     } else {
         if (V.op_auto)
-            k_bottom<3, 1><<<1, threads, lds_bytes, s>>>(T, V, B, A);
+            k_bottom<3, 1, R><<<1, threads, lds_bytes, s>>>(T, V, B, A);
         else
-            k_bottom<3, 0><<<1, threads, lds_bytes, s>>>(T, V, B, A);
+            k_bottom<3, 0, R><<<1, threads, lds_bytes, s>>>(T, V, B, A);
     }
 }
 
-void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVars &V,
+template <typename R>
+void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVarsT<R> &V,
                             const BottomLevels *B, const BottomArgs &A, int threads)
 {
     if (dim == 2)
-        k_bottom_residual<2><<<1, threads, 0, s>>>(T, V, B, A);
+        k_bottom_residual<2, R><<<1, threads, 0, s>>>(T, V, B, A);
     else
-        k_bottom_residual<3><<<1, threads, 0, s>>>(T, V, B, A);
+        k_bottom_residual<3, R><<<1, threads, 0, s>>>(T, V, B, A);
 }
+
+// the two value types of the launchers
+#define AFG_INSTANTIATE(R)                                                                               \
+    template void launch_fill_gc<R>(int, hipStream_t, const DevTree &, const LevelArgs &, R *);         \
+    template void launch_fill_diff<R>(int, hipStream_t, const DevTree &, const LevelArgs &,            \
+                                      const ListArgs &, const MgVarsT<R> &);                           \
+    template void launch_diff<R>(int, hipStream_t, const DevTree &, const ListArgs &,                  \
+                                 const MgVarsT<R> &);                                                  \
+    template void launch_gsrb_pass<R>(int, hipStream_t, const DevTree &, const LevelArgs &,            \
+                                      const MgVarsT<R> &, int, int);                                   \
+    template void launch_residual<R>(int, hipStream_t, const DevTree &, const LevelArgs &,             \
+                                     const MgVarsT<R> &);                                              \
+    template void launch_uc_child<R>(int, hipStream_t, const DevTree &, const LevelArgs &,             \
+                                     const MgVarsT<R> &, int, int);                                    \
+    template void launch_uc_parent<R>(int, hipStream_t, const DevTree &, const LevelArgs &,            \
+                                      const MgVarsT<R> &, int);                                        \
+    template void launch_correct<R>(int, hipStream_t, const DevTree &, const LevelArgs &,              \
+                                    const MgVarsT<R> &, int);                                          \
+    template void launch_restrict<R>(int, hipStream_t, const DevTree &, const LevelArgs &, const R *, R *); \
+    template void launch_prolong2<R>(int, hipStream_t, const DevTree &, const LevelArgs &, const R *, R *, \
+                                     int);                                                             \
+    template void launch_fill_value<R>(int, hipStream_t, const DevTree &, int, int, int, R *, double);  \
+    template void launch_copy_boxes<R>(int, hipStream_t, const DevTree &, int, int, int, const R *, R *); \
+    template void launch_bottom<R>(int, hipStream_t, const DevTree &, const MgVarsT<R> &,              \
+                                   const BottomLevels *, const BottomArgs &, size_t, int);             \
+    template void launch_bottom_residual<R>(int, hipStream_t, const DevTree &, const MgVarsT<R> &,     \
+                                            const BottomLevels *, const BottomArgs &, int);
+AFG_INSTANTIATE(double)
+AFG_INSTANTIATE(float)
 
 }  // namespace afg

@@ -52,12 +52,15 @@ struct ListArgs {
     double dr, dx2, inv_dr_sq;
 };
 
-// Variables of the multigrid options, as device pointers.
-struct MgVars {
-    double *phi, *rhs, *tmp;
-    const double *eps;   // nullptr unless the auto operators are used
+// Variables of the multigrid options, as device pointers (R: double, or
+// float for the FP32 stages of the mixed-precision solve).
+template <typename R>
+struct MgVarsT {
+    R *phi, *rhs, *tmp;
+    const R *eps;   // nullptr unless the auto operators are used
     int op_auto;
 };
+using MgVars = MgVarsT<double>;
 
 // The lower part of a V-cycle -- every level from `hl` down to the lowest
 // and back up -- run by one workgroup in one launch (levels with few cells,

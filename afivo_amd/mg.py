@@ -112,6 +112,12 @@ class DeviceTree:
         self._c(_lib.load().afg_packed_size(self.h, ids.size, _i32(ids), ctypes.byref(n)))
         return n.value
 
+    def fp32_from_cc(self, iv, iv32):
+        self._c(_lib.load().afg_fp32_from_cc(self.h, iv, iv32))
+
+    def fp32_to_cc(self, iv32, iv, add=False):
+        self._c(_lib.load().afg_fp32_to_cc(self.h, iv32, iv, int(add)))
+
     def copy_cc(self, iv_from, iv_to):
         self._c(_lib.load().afg_copy_cc(self.h, iv_from, iv_to))
 
@@ -206,6 +212,25 @@ def tree_min_cc(tree: DeviceTree, iv: int, include_parents: bool = False) -> flo
     out = ctypes.c_double()
     tree._c(_lib.load().afg_tree_min_cc(tree.h, iv, int(include_parents), ctypes.byref(out)))
     return out.value
+
+
+def mg_fas_fmg_fp32(tree: DeviceTree, mg: MgOpts, set_residual: bool, have_guess: bool):
+    """mg$D_fas_fmg on the FP32 copies of the variables."""
+    tree._c(_lib.load().afg_fas_fmg_fp32(tree.h, ctypes.byref(mg), int(set_residual), int(have_guess)))
+
+
+def mg_fas_vcycle_fp32(tree: DeviceTree, mg: MgOpts, highest_lvl: int, set_residual: bool):
+    """mg$D_fas_vcycle on the FP32 copies of the variables."""
+    tree._c(_lib.load().afg_fas_vcycle_fp32(tree.h, ctypes.byref(mg), highest_lvl, int(set_residual)))
+
+
+def mixed_solve(tree: DeviceTree, mg: MgOpts, tol: float, max_iter: int = 50, have_guess=False):
+    """FP64 defect correction with FP32 multigrid corrections to relative
+    residual `tol`; returns (corrections, relative residual)."""
+    it, rr = ctypes.c_int(), ctypes.c_double()
+    tree._c(_lib.load().afg_mixed_solve(tree.h, ctypes.byref(mg), int(have_guess), float(tol),
+                                        int(max_iter), ctypes.byref(it), ctypes.byref(rr)))
+    return it.value, rr.value
 
 
 def max_abs_cc(tree: DeviceTree, iv: int) -> float:

@@ -197,6 +197,33 @@ def c2_gaussians(seed=1):
     return Gaussians(np.array([1.0, -1.0, 1.0, -1.0]), np.full(4, 0.03), r0)
 
 
+def c4_gaussians(seed=1):
+    rng = np.random.default_rng(seed)
+    r0 = rng.uniform(0.2, 0.8, size=(6, 3))
+    ampl = np.where(rng.uniform(size=6) < 0.5, 1.0, -1.0)
+    # numpy's exp: this configuration is compared with FP64 solves of the
+    # same rhs, not bit for bit with the reference
+    return Gaussians(ampl, np.full(6, 0.04), r0, exact=False)
+
+
+def c4_eps(seed=1):
+    """eps above z = 0.5, drawn from U[2, 10]."""
+    return float(np.random.default_rng(seed + 1000).uniform(2.0, 10.0))
+
+
+# Threshold of the 3D variable-eps configuration (n_cell 16, uniform to level
+# 3, adaptive to level 8), tuned once for about 1e7 leaf cells and frozen.
+THR_C4 = 4.0e-3
+
+
+def c4(seed=1, thr=THR_C4, l_max=8):
+    """3D variable-coefficient Poisson: n_cell 16, uniform to level 3,
+    adaptive to level `l_max` around 6 Gaussians, eps = 1 below z = 0.5 and
+    U[2, 10] above (interior and ghost cells), auto operators."""
+    return make_problem("c4", 3, 16, c4_gaussians(seed), 3, l_max, thr, n_base=50, n_vcycle=8,
+                        eps_hi=c4_eps(seed))
+
+
 def c1():
     return make_problem("c1", 2, 8, C1_GAUSS, 5, 7, 1e-4, n_base=50, n_vcycle=10)
 

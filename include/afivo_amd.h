@@ -165,6 +165,31 @@ int afg_correct_children(afg_ctx *ctx, const afg_mg_opts *opts, int lvl);
 int afg_update_coarse(afg_ctx *ctx, const afg_mg_opts *opts, int lvl);
 
 /* ------------------------------------------------------------------ */
+/* mixed precision                                                     */
+
+/* FP32 copies of the cell variables, for the FP32 stages: variable iv32 of the
+ * FP32 pool = float(variable iv), and back (added to variable iv if add). */
+int afg_fp32_from_cc(afg_ctx *ctx, int iv, int iv32);
+int afg_fp32_to_cc(afg_ctx *ctx, int iv32, int iv, int add);
+/* mg$D_fas_fmg / mg$D_fas_vcycle on the FP32 pool: the variables of opts are
+ * FP32 variables; the same stages in FP32 arithmetic (tags of the auto
+ * operators come from the FP64 eps). */
+int afg_fas_fmg_fp32(afg_ctx *ctx, const afg_mg_opts *opts, int set_residual, int have_guess);
+int afg_fas_vcycle_fp32(afg_ctx *ctx, const afg_mg_opts *opts, int highest_lvl, int set_residual);
+/* Solve A phi = rhs (the options' operator, phi, rhs, tmp, eps in FP64) to
+ * max|rhs - A phi| <= tol * max|rhs| over leaf interiors, by FP64 defect
+ * correction: parents' phi set to the restriction of their children's (the
+ * FAS fixed point), the residual r = rhs - A phi in FP64 on every level, a
+ * correction e from A e = r by FP32 FAS multigrid (FMG the first time, then
+ * one V-cycle from e = 0; in the FP32 copies of the same variables), phi += e,
+ * until the tolerance or max_iter corrections.  phi starts from zero unless
+ * have_guess; on return tmp holds the FP64 residual.  Physical boundaries
+ * must be homogeneous (b = 0: the correction solves with the same ghost
+ * rules).  One tree only.  No counterpart in the reference. */
+int afg_mixed_solve(afg_ctx *ctx, const afg_mg_opts *opts, int have_guess, double tol, int max_iter,
+                    int *iterations, double *rel_residual);
+
+/* ------------------------------------------------------------------ */
 /* execution controls                                                  */
 
 /* The lowest levels of the tree, while each holds at most `cells` cells

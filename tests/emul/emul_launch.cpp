@@ -23,14 +23,16 @@ namespace {
 
 constexpr int BLOCK = 256;
 
-LevelView pool_view(const DevTree &T, const MgVars &V)
+template <typename R>
+LevelViewT<R> pool_view(const DevTree &T, const MgVarsT<R> &V)
 {
-    return LevelView{V.phi, V.rhs, V.tmp, V.eps, 0, T.stride};
+    return LevelViewT<R>{V.phi, V.rhs, V.tmp, V.eps, 0, T.stride};
 }
 
-LevelView var_view(const DevTree &T, double *var)
+template <typename R>
+LevelViewT<R> var_view(const DevTree &T, R *var)
 {
-    return LevelView{var, nullptr, nullptr, nullptr, 0, T.stride};
+    return LevelViewT<R>{var, nullptr, nullptr, nullptr, 0, T.stride};
 }
 
 // Run body<DIM, NC>(gid) for gid in [0, n(DIM)) with NC dispatched like the
@@ -56,22 +58,24 @@ void run_stage(int dim, int nc, Count count, Body body)
 
 }  // namespace
 
-void launch_fill_gc(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, double *var)
+template <typename R>
+void launch_fill_gc(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, R *var)
 {
     if (L.n <= 0) return;
     emul_submit(s, [=] {
-        const LevelView v = var_view(T, var);
+        const LevelViewT<R> v = var_view(T, var);
         run_stage(dim, L.nc, [&](auto d) { return fill_threads<DIMV>(L.n, L.nc); },
                   [&](auto d, auto c, int g) { fill_body<DIMV, NCV>(T, L, v, v, g); });
     });
 }
 
+template <typename R>
 void launch_fill_diff(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                      const ListArgs &P, const MgVars &V)
+                      const ListArgs &P, const MgVarsT<R> &V)
 {
     if (L.n <= 0) return;
     emul_submit(s, [=] {
-        const LevelView v = pool_view(T, V);
+        const LevelViewT<R> v = pool_view(T, V);
         run_stage(dim, L.nc, [&](auto d) { return fill_threads<DIMV>(L.n, L.nc); },
                   [&](auto d, auto c, int g) { fill_body<DIMV, NCV>(T, L, v, v, g); });
         run_stage(dim, P.nc, [&](auto d) { return (long long)P.n * ipow(P.nc + 2, DIMV); },
@@ -79,11 +83,12 @@ void launch_fill_diff(int dim, hipStream_t s, const DevTree &T, const LevelArgs 
     });
 }
 
-void launch_diff(int dim, hipStream_t s, const DevTree &T, const ListArgs &P, const MgVars &V)
+template <typename R>
+void launch_diff(int dim, hipStream_t s, const DevTree &T, const ListArgs &P, const MgVarsT<R> &V)
 {
     if (P.n <= 0) return;
     emul_submit(s, [=] {
-        const LevelView v = pool_view(T, V);
+        const LevelViewT<R> v = pool_view(T, V);
         run_stage(dim, P.nc, [&](auto d) { return (long long)P.n * ipow(P.nc + 2, DIMV); },
                   [&](auto d, auto c, int g) { diff_body<DIMV, NCV>(T, P, v, g); });
     });
@@ -91,14 +96,14 @@ void launch_diff(int dim, hipStream_t s, const DevTree &T, const ListArgs &P, co
 
 // the tiled pass of the device: workgroups one after the other, each stage
 // over all threads of the workgroup before the next
-template <int DIM, int NC>
-void gsrb_tiles(const DevTree &T, const LevelArgs &L, const MgVars &V, int rb, int fly)
+template <int DIM, int NC, typename R>
+void gsrb_tiles(const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int rb, int fly)
 {
     using G = GsrbTile<DIM, NC>;
-    const LevelView v = pool_view(T, V);
+    const LevelViewT<R> v = pool_view(T, V);
     const int grid = (L.n + G::BOXES - 1) / G::BOXES;
-    double *lds = (double *)emul_alloc(sizeof(double) * G::BOXES * G::LEN);
-    std::vector<double> rhs((size_t)G::THREADS * G::PER);
+    R *lds = (R *)emul_alloc(sizeof(R) * G::BOXES * G::LEN);
+    std::vector<R> rhs((size_t)G::THREADS * G::PER);
     for (int blk = 0; blk < grid; blk++) {
         const int box0 = xcd_tile(blk, grid) * G::BOXES;
         for (int t = 0; t < G::THREADS; t++) {
@@ -113,44 +118,47 @@ void gsrb_tiles(const DevTree &T, const LevelArgs &L, const MgVars &V, int rb, i
     emul_free(lds);
 }
 
+template <typename R>
 void launch_gsrb_pass(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                      const MgVars &V, int rb, int fly)
+                      const MgVarsT<R> &V, int rb, int fly)
 {
     if (L.n <= 0) return;
     emul_submit(s, [=] {
         switch (dim * 100 + L.nc) {
-        case 202: gsrb_tiles<2, 2>(T, L, V, rb, fly); return;
-        case 204: gsrb_tiles<2, 4>(T, L, V, rb, fly); return;
-        case 208: gsrb_tiles<2, 8>(T, L, V, rb, fly); return;
-        case 216: gsrb_tiles<2, 16>(T, L, V, rb, fly); return;
-        case 302: gsrb_tiles<3, 2>(T, L, V, rb, fly); return;
-        case 304: gsrb_tiles<3, 4>(T, L, V, rb, fly); return;
-        case 308: gsrb_tiles<3, 8>(T, L, V, rb, fly); return;
-        case 316: gsrb_tiles<3, 16>(T, L, V, rb, fly); return;
+        case 202: gsrb_tiles<2, 2, R>(T, L, V, rb, fly); return;
+        case 204: gsrb_tiles<2, 4, R>(T, L, V, rb, fly); return;
+        case 208: gsrb_tiles<2, 8, R>(T, L, V, rb, fly); return;
+        case 216: gsrb_tiles<2, 16, R>(T, L, V, rb, fly); return;
+        case 302: gsrb_tiles<3, 2, R>(T, L, V, rb, fly); return;
+        case 304: gsrb_tiles<3, 4, R>(T, L, V, rb, fly); return;
+        case 308: gsrb_tiles<3, 8, R>(T, L, V, rb, fly); return;
+        case 316: gsrb_tiles<3, 16, R>(T, L, V, rb, fly); return;
         default: break;
         }
-        const LevelView v = pool_view(T, V);
+        const LevelViewT<R> v = pool_view(T, V);
         run_stage(dim, 0, [&](auto d) { return gsrb_threads<DIMV>(L.n, L.nc); },
                   [&](auto d, auto c, int g) { gsrb_body<DIMV, 0>(T, L, v, v, V.op_auto, rb, fly, g); });
     });
 }
 
-void launch_residual(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVars &V)
+template <typename R>
+void launch_residual(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V)
 {
     if (L.n <= 0) return;
     emul_submit(s, [=] {
-        const LevelView v = pool_view(T, V);
+        const LevelViewT<R> v = pool_view(T, V);
         run_stage(dim, L.nc, [&](auto d) { return (long long)L.n * ipow(L.nc, DIMV); },
                   [&](auto d, auto c, int g) { residual_body<DIMV, NCV>(T, L, v, v, V.op_auto, g); });
     });
 }
 
+template <typename R>
 void launch_uc_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                     const MgVars &V, int keep_child_tmp, int fly)
+                     const MgVarsT<R> &V, int keep_child_tmp, int fly)
 {
     if (L.n <= 0) return;
     emul_submit(s, [=] {
-        const LevelView v = pool_view(T, V);
+        const LevelViewT<R> v = pool_view(T, V);
         run_stage(dim, L.nc, [&](auto d) { return (long long)L.n * ipow(L.nc / 2, DIMV); },
                   [&](auto d, auto c, int g) {
                       uc_child_body<DIMV, NCV>(T, L, v, v, V.op_auto, keep_child_tmp, fly, g);
@@ -158,30 +166,33 @@ void launch_uc_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &
     });
 }
 
+template <typename R>
 void launch_uc_parent(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                      const MgVars &V, int copy_tmp)
+                      const MgVarsT<R> &V, int copy_tmp)
 {
     if (L.n <= 0) return;
     emul_submit(s, [=] {
-        const LevelView v = pool_view(T, V);
+        const LevelViewT<R> v = pool_view(T, V);
         run_stage(dim, L.nc, [&](auto d) { return (long long)L.n * ipow(L.nc + 2, DIMV); },
                   [&](auto d, auto c, int g) { uc_parent_body<DIMV, NCV>(T, L, v, v, V.op_auto, copy_tmp, g); });
     });
 }
 
+template <typename R>
 void launch_correct(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                    const MgVars &V, int fly_diff)
+                    const MgVarsT<R> &V, int fly_diff)
 {
     if (L.n <= 0) return;
     emul_submit(s, [=] {
-        const LevelView v = pool_view(T, V);
+        const LevelViewT<R> v = pool_view(T, V);
         run_stage(dim, L.nc, [&](auto d) { return (long long)L.n * ipow(L.nc / 2, DIMV); },
                   [&](auto d, auto c, int g) { correct_body<DIMV, NCV>(T, L, v, v, V.op_auto, fly_diff, g); });
     });
 }
 
+template <typename R>
 void launch_restrict(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                     const double *from, double *to)
+                     const R *from, R *to)
 {
     if (L.n <= 0) return;
     emul_submit(s, [=] {
@@ -190,8 +201,9 @@ void launch_restrict(int dim, hipStream_t s, const DevTree &T, const LevelArgs &
     });
 }
 
+template <typename R>
 void launch_prolong2(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                     const double *from, double *to, int add)
+                     const R *from, R *to, int add)
 {
     if (L.n <= 0) return;
     emul_submit(s, [=] {
@@ -200,19 +212,21 @@ void launch_prolong2(int dim, hipStream_t s, const DevTree &T, const LevelArgs &
     });
 }
 
+template <typename R>
 void launch_fill_value(int dim, hipStream_t s, const DevTree &T, int slot0, int n_slots, int nc,
-                       double *var, double value)
+                       R *var, double value)
 {
     if (n_slots <= 0) return;
     emul_submit(s, [=] {
         const int len = ipow(nc + 2, dim);
         for (int b = 0; b < n_slots; b++)
-            for (int q = 0; q < len; q++) var[(size_t)(slot0 + b) * T.stride + q] = value;
+            for (int q = 0; q < len; q++) var[(size_t)(slot0 + b) * T.stride + q] = R(value);
     });
 }
 
+template <typename R>
 void launch_copy_boxes(int dim, hipStream_t s, const DevTree &T, int slot0, int n_slots, int nc,
-                       const double *from, double *to)
+                       const R *from, R *to)
 {
     if (n_slots <= 0) return;
     emul_submit(s, [=] {
@@ -223,6 +237,19 @@ void launch_copy_boxes(int dim, hipStream_t s, const DevTree &T, int slot0, int 
                 to[o] = from[o];
             }
     });
+}
+
+void launch_to_fp32(hipStream_t s, long long n, const double *from, float *to)
+{
+    if (n > 0) emul_submit(s, [=] { for (long long i = 0; i < n; i++) to[i] = (float)from[i]; });
+}
+
+void launch_from_fp32(hipStream_t s, long long n, const float *from, double *to, int add)
+{
+    if (n > 0)
+        emul_submit(s, [=] {
+            for (long long i = 0; i < n; i++) to[i] = add ? to[i] + (double)from[i] : (double)from[i];
+        });
 }
 
 void launch_set_tags(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const double *eps)
@@ -315,35 +342,65 @@ void launch_reduce_partials(hipStream_t s, const double *partial, int n, int wan
 
 int bottom_prepare(int, size_t lds_bytes) { return lds_bytes <= 163840 ? 0 : -1; }
 
-void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVars &V,
+template <typename R>
+void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVarsT<R> &V,
                    const BottomLevels *B, const BottomArgs &A, size_t lds_bytes, int)
 {
     emul_submit(s, [=] {
         double *lds = (double *)emul_alloc(lds_bytes ? lds_bytes : 8);
         if (dim == 2) {
             if (V.op_auto)
-                bottom_run<2, 1>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {});
+                bottom_run<2, 1, R>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {});
             else
-                bottom_run<2, 0>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {});
+                bottom_run<2, 0, R>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {});
         } else {
             if (V.op_auto)
-                bottom_run<3, 1>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {});
+                bottom_run<3, 1, R>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {});
             else
-                bottom_run<3, 0>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {});
+                bottom_run<3, 0, R>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {});
         }
         emul_free(lds);
     });
 }
 
-void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVars &V,
+template <typename R>
+void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVarsT<R> &V,
                             const BottomLevels *B, const BottomArgs &A, int)
 {
     emul_submit(s, [=] {
         if (dim == 2)
-            bottom_residual_run<2>(T, V, *B, A, 0, 1);
+            bottom_residual_run<2, R>(T, V, *B, A, 0, 1);
         else
-            bottom_residual_run<3>(T, V, *B, A, 0, 1);
+            bottom_residual_run<3, R>(T, V, *B, A, 0, 1);
     });
 }
+
+#define AFG_INSTANTIATE(R)                                                                               \
+    template void launch_fill_gc<R>(int, hipStream_t, const DevTree &, const LevelArgs &, R *);         \
+    template void launch_fill_diff<R>(int, hipStream_t, const DevTree &, const LevelArgs &,            \
+                                      const ListArgs &, const MgVarsT<R> &);                           \
+    template void launch_diff<R>(int, hipStream_t, const DevTree &, const ListArgs &,                  \
+                                 const MgVarsT<R> &);                                                  \
+    template void launch_gsrb_pass<R>(int, hipStream_t, const DevTree &, const LevelArgs &,            \
+                                      const MgVarsT<R> &, int, int);                                   \
+    template void launch_residual<R>(int, hipStream_t, const DevTree &, const LevelArgs &,             \
+                                     const MgVarsT<R> &);                                              \
+    template void launch_uc_child<R>(int, hipStream_t, const DevTree &, const LevelArgs &,             \
+                                     const MgVarsT<R> &, int, int);                                    \
+    template void launch_uc_parent<R>(int, hipStream_t, const DevTree &, const LevelArgs &,            \
+                                      const MgVarsT<R> &, int);                                        \
+    template void launch_correct<R>(int, hipStream_t, const DevTree &, const LevelArgs &,              \
+                                    const MgVarsT<R> &, int);                                          \
+    template void launch_restrict<R>(int, hipStream_t, const DevTree &, const LevelArgs &, const R *, R *); \
+    template void launch_prolong2<R>(int, hipStream_t, const DevTree &, const LevelArgs &, const R *, R *, \
+                                     int);                                                             \
+    template void launch_fill_value<R>(int, hipStream_t, const DevTree &, int, int, int, R *, double);  \
+    template void launch_copy_boxes<R>(int, hipStream_t, const DevTree &, int, int, int, const R *, R *); \
+    template void launch_bottom<R>(int, hipStream_t, const DevTree &, const MgVarsT<R> &,              \
+                                   const BottomLevels *, const BottomArgs &, size_t, int);             \
+    template void launch_bottom_residual<R>(int, hipStream_t, const DevTree &, const MgVarsT<R> &,     \
+                                            const BottomLevels *, const BottomArgs &, int);
+AFG_INSTANTIATE(double)
+AFG_INSTANTIATE(float)
 
 }  // namespace afg

@@ -46,3 +46,7 @@ from test_serving import (  # noqa: E402,F401
     test_serving_admits_by_memory_budget,
     test_serving_meets_tolerances_and_matches_solo_runs,
 )
+from test_mixed import (  # noqa: E402,F401
+    test_fp32_multigrid_tracks_fp64,
+    test_mixed_solve_matches_fp64_solution,
+)

@@ -1,0 +1,87 @@
+"""Mixed precision: the FP32 multigrid stages track the FP64 reference
+arithmetic to FP32 accuracy, and FP64 defect correction with FP32
+corrections reaches the tolerance with the FP64 solution.  Device tests;
+also run through the CPU emulation (test_emul.py)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from afivo_amd import mg as M
+from afivo_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_problem(kind):
+    if kind == "2d":
+        return P.small_2d()
+    if kind == "3d":
+        return P.small_3d()
+    return P.small_3d(nc=8, l_uniform=2, l_max=4, eps_hi=5.0)
+
+
+@pytest.mark.parametrize("kind", ["2d", "3d", "3d_eps"])
+def test_fp32_multigrid_tracks_fp64(kind):
+    """FMG + 2 V-cycles on the FP32 copies agree with the oracle's FP64
+    result to FP32 accuracy (and differ from it: the arithmetic is FP32)."""
+    p = mixed_problem(kind)
+    st = p.state
+    nv = len(st.data)
+    eps = p.eps_hi is not None
+    dt = M.DeviceTree.from_state(st, nv + 1)
+    mg = M.default_mg(eps=eps, n_base=p.n_base)
+    for iv in range(1, nv + 1):
+        dt.fp32_from_cc(iv, iv)
+    M.mg_fas_fmg_fp32(dt, mg, False, False)
+    for _ in range(2):
+        M.mg_fas_vcycle_fp32(dt, mg, dt.highest_lvl, False)
+    dt.fp32_to_cc(1, nv + 1)
+    phi32 = dt.download(nv + 1)
+    ot = O.OracleTree(st, n_var=nv)
+    om = O.default_opts(2, 2, p.n_base, eps=eps)
+    ot.fas_fmg(om, False, False)
+    for _ in range(2):
+        ot.fas_vcycle(om, set_residual=False)
+    phi64 = ot.get(1)
+    mask = st.interior_mask()
+    scale = np.abs(phi64[mask]).max()
+    err = np.abs(phi32[mask] - phi64[mask]).max() / scale
+    assert 0 < err < 1e-5, err
+    dt.close()
+
+
+@pytest.mark.parametrize("kind", ["2d", "3d_eps"])
+def test_mixed_solve_matches_fp64_solution(kind):
+    """Defect correction reaches 1e-10 and its phi agrees with FP64 solves
+    to the same tolerance (the device FP64 solver and the oracle) within
+    1e-9 of max|phi|."""
+    p = mixed_problem(kind)
+    st = p.state
+    eps = p.eps_hi is not None
+    mg = M.default_mg(eps=eps, n_base=p.n_base)
+    tol = 1e-10
+    dt = M.DeviceTree.from_state(st)
+    it, rr = M.mixed_solve(dt, mg, tol, max_iter=30)
+    assert rr <= tol and it >= 2
+    phi_mixed = dt.download(1)
+    dt.close()
+    # FP64 on the device: FMG, then V-cycles to the tolerance
+    d64 = M.DeviceTree.from_state(st)
+    rhs_norm = M.max_abs_cc(d64, 2)
+    M.mg_fas_fmg(d64, mg, True, False)
+    k = 0
+    while M.max_abs_cc(d64, 3) > tol * rhs_norm and k < 30:
+        M.mg_fas_vcycle(d64, mg, d64.highest_lvl, True)
+        k += 1
+    phi64 = d64.download(1)
+    d64.close()
+    # the oracle, with as many V-cycles
+    ot = O.OracleTree(st)
+    om = O.default_opts(2, 2, p.n_base, eps=eps)
+    ot.fas_fmg(om, True, False)
+    for _ in range(k):
+        ot.fas_vcycle(om)
+    mask = st.interior_mask()
+    scale = np.abs(phi64[mask]).max()
+    assert np.abs(phi_mixed[mask] - phi64[mask]).max() <= 1e-9 * scale
+    assert np.abs(phi_mixed[mask] - ot.get(1)[mask]).max() <= 1e-9 * scale
