@@ -69,7 +69,7 @@ SIGNATURES = {
     "afg_fas_fmg_fp32": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int, ctypes.c_int]),
     "afg_fas_vcycle_fp32": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int, ctypes.c_int]),
     "afg_mixed_solve": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int, F64, ctypes.c_int,
-                                       ctypes.POINTER(ctypes.c_int), F64P]),
+                                       ctypes.c_int, ctypes.POINTER(ctypes.c_int), F64P]),
     "afg_set_bottom_cells": (ctypes.c_int, [P, I64, I64]),
     "afg_set_graphs": (ctypes.c_int, [P, ctypes.c_int]),
     "afg_set_timing": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),

@@ -1530,11 +1530,12 @@ static int leaf_max_abs(afg_ctx *c, int iv, double *out)
 }
 
 int afg_mixed_solve(afg_ctx *c, const afg_mg_opts *o, int have_guess, double tol, int max_iter,
-                    int *iterations, double *rel_residual)
+                    int cycles, int *iterations, double *rel_residual)
 {
     if (int e = check_ctx(c)) return e;
     if (c->n_trees != 1) return c->fail(AFG_ERR_UNSUPPORTED, "mixed-precision solve of a forest");
     if (!(tol > 0) || max_iter < 0) return c->fail(AFG_ERR_ARG, "tol must be > 0 and max_iter >= 0");
+    if (cycles < 1) return c->fail(AFG_ERR_ARG, "cycles per correction must be >= 1");
     Mg m;
     if (int e = make_mg(c, o, &m)) return e;
     set_tags(m);
@@ -1567,14 +1568,17 @@ int afg_mixed_solve(afg_ctx *c, const afg_mg_opts *o, int have_guess, double tol
     for (;; it++) {
         if (int e = leaf_max_abs(c, o->i_tmp, &res)) return e;
         if (res <= tol * rhs_norm || it >= max_iter) break;
-        // correction e from A e = r in FP32 (a V-cycle from e = 0; FMG first)
+        // correction e from A e = r in FP32: `cycles` V-cycles from e = 0
+        // (the first of the first correction an FMG)
         launch_to_fp32(c->stream, n, m.V.tmp, f.V.rhs);
         HIP_TRY(c, hipMemsetAsync(f.V.phi, 0, n * sizeof(float), c->stream));
         HIP_TRY(c, hipMemsetAsync(f.V.tmp, 0, n * sizeof(float), c->stream));
-        if (it == 0) {
-            if (int e = fas_fmg(f, 0, 0)) return e;
-        } else {
-            if (int e = run_vcycle(f, c->highest_lvl, 0, c->min_lvl, c->highest_lvl + 1)) return e;
+        for (int k = 0; k < cycles; k++) {
+            if (it == 0 && k == 0) {
+                if (int e = fas_fmg(f, 0, 0)) return e;
+            } else {
+                if (int e = run_vcycle(f, c->highest_lvl, 0, c->min_lvl, c->highest_lvl + 1)) return e;
+            }
         }
         launch_from_fp32(c->stream, n, f.V.phi, m.V.phi, 1);
         residual();

@@ -224,12 +224,14 @@ def mg_fas_vcycle_fp32(tree: DeviceTree, mg: MgOpts, highest_lvl: int, set_resid
     tree._c(_lib.load().afg_fas_vcycle_fp32(tree.h, ctypes.byref(mg), highest_lvl, int(set_residual)))
 
 
-def mixed_solve(tree: DeviceTree, mg: MgOpts, tol: float, max_iter: int = 50, have_guess=False):
-    """FP64 defect correction with FP32 multigrid corrections to relative
-    residual `tol`; returns (corrections, relative residual)."""
+def mixed_solve(tree: DeviceTree, mg: MgOpts, tol: float, max_iter: int = 50, have_guess=False,
+                cycles=2):
+    """FP64 defect correction with FP32 multigrid corrections (`cycles`
+    V-cycles each) to relative residual `tol`; returns (corrections, relative
+    residual)."""
     it, rr = ctypes.c_int(), ctypes.c_double()
     tree._c(_lib.load().afg_mixed_solve(tree.h, ctypes.byref(mg), int(have_guess), float(tol),
-                                        int(max_iter), ctypes.byref(it), ctypes.byref(rr)))
+                                        int(max_iter), int(cycles), ctypes.byref(it), ctypes.byref(rr)))
     return it.value, rr.value
 
 

@@ -180,14 +180,14 @@ int afg_fas_vcycle_fp32(afg_ctx *ctx, const afg_mg_opts *opts, int highest_lvl, 
  * max|rhs - A phi| <= tol * max|rhs| over leaf interiors, by FP64 defect
  * correction: parents' phi set to the restriction of their children's (the
  * FAS fixed point), the residual r = rhs - A phi in FP64 on every level, a
- * correction e from A e = r by FP32 FAS multigrid (FMG the first time, then
- * one V-cycle from e = 0; in the FP32 copies of the same variables), phi += e,
- * until the tolerance or max_iter corrections.  phi starts from zero unless
+ * correction e from A e = r by FP32 FAS multigrid (`cycles` V-cycles from
+ * e = 0, the first one of the first correction an FMG; in the FP32 copies of
+ * the same variables), phi += e, until the tolerance or max_iter corrections.  phi starts from zero unless
  * have_guess; on return tmp holds the FP64 residual.  Physical boundaries
  * must be homogeneous (b = 0: the correction solves with the same ghost
  * rules).  One tree only.  No counterpart in the reference. */
 int afg_mixed_solve(afg_ctx *ctx, const afg_mg_opts *opts, int have_guess, double tol, int max_iter,
-                    int *iterations, double *rel_residual);
+                    int cycles, int *iterations, double *rel_residual);
 
 /* ------------------------------------------------------------------ */
 /* execution controls                                                  */
