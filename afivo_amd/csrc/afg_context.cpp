@@ -110,6 +110,8 @@ struct afg_ctx {
     int32_t *d_tree_leaves = nullptr, *d_chunk = nullptr, *d_tree_chunk = nullptr;
     int n_chunks = 0;
     double *d_chunk_max = nullptr, *d_tree_out = nullptr, *h_tree_out = nullptr;
+    // mixed-precision solve in steps: begun, and the next correction is the first (an FMG)
+    bool mixed_ready = false, mixed_fmg = false;
 
     // tree
     bool have_tree = false;
@@ -1013,6 +1015,7 @@ int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base,
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     free_tree(c);
+    c->mixed_ready = c->mixed_fmg = false;
     const int dim = c->dim, nch = c->nch, nnb = c->nnb;
     const int rl = 3 + dim + nch + nnb;
     int max_id = 0, min_lvl = 1 << 30, hi_lvl = -(1 << 30);
@@ -1529,70 +1532,9 @@ static int leaf_max_abs(afg_ctx *c, int iv, double *out)
     return 0;
 }
 
-int afg_mixed_solve(afg_ctx *c, const afg_mg_opts *o, int have_guess, double tol, int max_iter,
-                    int cycles, int *iterations, double *rel_residual)
+// Per tree: max |variable iv| over its leaf interiors, into out[0..n_trees).
+static int forest_norms(afg_ctx *c, int iv, double *out)
 {
-    if (int e = check_ctx(c)) return e;
-    if (c->n_trees != 1) return c->fail(AFG_ERR_UNSUPPORTED, "mixed-precision solve of a forest");
-    if (!(tol > 0) || max_iter < 0) return c->fail(AFG_ERR_ARG, "tol must be > 0 and max_iter >= 0");
-    if (cycles < 1) return c->fail(AFG_ERR_ARG, "cycles per correction must be >= 1");
-    Mg m;
-    if (int e = make_mg(c, o, &m)) return e;
-    set_tags(m);
-    MgT<float> f;
-    if (int e = make_mg(c, o, &f)) return e;
-    const long long n = (long long)c->n_slots * c->stride;
-    if (m.V.op_auto) launch_to_fp32(c->stream, n, m.V.eps, const_cast<float *>(f.V.eps));
-    if (!have_guess)
-        for (int lvl = c->min_lvl; lvl <= c->highest_lvl; lvl++) {
-            const LevelArgs L = level_args(c, lvl);
-            launch_fill_value(c->dim, c->stream, m.T, L.slot0, L.n, L.nc, m.V.phi, 0.0);
-        }
-    double rhs_norm = 0;
-    if (int e = leaf_max_abs(c, o->i_rhs, &rhs_norm)) return e;
-    // The FP64 residual: phi of every parent set to the restriction of its
-    // children (where the FAS iteration converges to, its corrections
-    // vanishing), then the ghost fill from the lowest level up
-    // (refinement-boundary ghosts read the parent's) and rhs - A phi.
-    auto residual = [&] {
-        for (int lvl = c->highest_lvl; lvl > c->min_lvl; lvl--)
-            launch_restrict(c->dim, c->stream, m.T, level_args(c, lvl), m.V.phi, m.V.phi);
-        for (int lvl = c->min_lvl; lvl <= c->highest_lvl; lvl++) {
-            fill_gc(m, m.V.phi, lvl);
-            residual_level(m, lvl);
-        }
-    };
-    residual();
-    int it = 0;
-    double res = 0;
-    for (;; it++) {
-        if (int e = leaf_max_abs(c, o->i_tmp, &res)) return e;
-        if (res <= tol * rhs_norm || it >= max_iter) break;
-        // correction e from A e = r in FP32: `cycles` V-cycles from e = 0
-        // (the first of the first correction an FMG)
-        launch_to_fp32(c->stream, n, m.V.tmp, f.V.rhs);
-        HIP_TRY(c, hipMemsetAsync(f.V.phi, 0, n * sizeof(float), c->stream));
-        HIP_TRY(c, hipMemsetAsync(f.V.tmp, 0, n * sizeof(float), c->stream));
-        for (int k = 0; k < cycles; k++) {
-            if (it == 0 && k == 0) {
-                if (int e = fas_fmg(f, 0, 0)) return e;
-            } else {
-                if (int e = run_vcycle(f, c->highest_lvl, 0, c->min_lvl, c->highest_lvl + 1)) return e;
-            }
-        }
-        launch_from_fp32(c->stream, n, f.V.phi, m.V.phi, 1);
-        residual();
-    }
-    if (iterations) *iterations = it;
-    if (rel_residual) *rel_residual = rhs_norm > 0 ? res / rhs_norm : res;
-    return 0;
-}
-
-int afg_forest_max_abs(afg_ctx *c, int iv, int n_trees, double *out)
-{
-    if (int e = check_ctx(c)) return e;
-    if (!out) return c->fail(AFG_ERR_ARG, "null output");
-    if (n_trees != c->n_trees) return c->fail(AFG_ERR_ARG, "the forest has %d trees, not %d", c->n_trees, n_trees);
     double *v = nullptr;
     if (int e = ensure_var(c, iv, &v)) return e;
     launch_tree_maxabs(c->dim, c->stream, c->stride, c->nc, c->d_tree_leaves, c->d_chunk, c->n_chunks,
@@ -1602,6 +1544,107 @@ int afg_forest_max_abs(afg_ctx *c, int iv, int n_trees, double *out)
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     std::memcpy(out, c->h_tree_out, c->n_trees * sizeof(double));
     return 0;
+}
+
+// The FP64 residual of the mixed-precision solve: phi of every parent set to
+// the restriction of its children (where the FAS iteration converges to, its
+// corrections vanishing), then the ghost fill from the lowest level up
+// (refinement-boundary ghosts read the parent's) and tmp = rhs - A phi.
+static void mixed_residual(Mg &m)
+{
+    afg_ctx *c = m.c;
+    for (int lvl = c->highest_lvl; lvl > c->min_lvl; lvl--)
+        launch_restrict(c->dim, c->stream, m.T, level_args(c, lvl), m.V.phi, m.V.phi);
+    for (int lvl = c->min_lvl; lvl <= c->highest_lvl; lvl++) {
+        fill_gc(m, m.V.phi, lvl);
+        residual_level(m, lvl);
+    }
+}
+
+int afg_mixed_begin(afg_ctx *c, const afg_mg_opts *o, int have_guess)
+{
+    if (int e = check_ctx(c)) return e;
+    Mg m;
+    if (int e = make_mg(c, o, &m)) return e;
+    set_tags(m);
+    MgT<float> f;
+    if (int e = make_mg(c, o, &f)) return e;
+    if (m.V.op_auto)
+        launch_to_fp32(c->stream, (long long)c->n_slots * c->stride, m.V.eps, const_cast<float *>(f.V.eps));
+    if (!have_guess)
+        for (int lvl = c->min_lvl; lvl <= c->highest_lvl; lvl++) {
+            const LevelArgs L = level_args(c, lvl);
+            launch_fill_value(c->dim, c->stream, m.T, L.slot0, L.n, L.nc, m.V.phi, 0.0);
+        }
+    mixed_residual(m);
+    c->mixed_fmg = true;
+    c->mixed_ready = true;
+    return 0;
+}
+
+int afg_mixed_correct(afg_ctx *c, const afg_mg_opts *o, int cycles)
+{
+    if (int e = check_ctx(c)) return e;
+    if (!c->mixed_ready) return c->fail(AFG_ERR_STATE, "afg_mixed_begin has not been called");
+    if (cycles < 1) return c->fail(AFG_ERR_ARG, "cycles per correction must be >= 1");
+    Mg m;
+    if (int e = make_mg(c, o, &m)) return e;
+    MgT<float> f;
+    if (int e = make_mg(c, o, &f)) return e;
+    const long long n = (long long)c->n_slots * c->stride;
+    // correction e from A e = r in FP32: `cycles` V-cycles from e = 0 (the
+    // first of the first correction an FMG), then phi += e
+    launch_to_fp32(c->stream, n, m.V.tmp, f.V.rhs);
+    HIP_TRY(c, hipMemsetAsync(f.V.phi, 0, n * sizeof(float), c->stream));
+    HIP_TRY(c, hipMemsetAsync(f.V.tmp, 0, n * sizeof(float), c->stream));
+    for (int k = 0; k < cycles; k++) {
+        if (c->mixed_fmg) {
+            if (int e = fas_fmg(f, 0, 0)) return e;
+            c->mixed_fmg = false;
+        } else {
+            if (int e = run_vcycle(f, c->highest_lvl, 0, c->min_lvl, c->highest_lvl + 1)) return e;
+        }
+    }
+    launch_from_fp32(c->stream, n, f.V.phi, m.V.phi, 1);
+    mixed_residual(m);
+    return 0;
+}
+
+int afg_mixed_solve(afg_ctx *c, const afg_mg_opts *o, int have_guess, double tol, int max_iter,
+                    int cycles, int *iterations, double *rel_residual)
+{
+    if (int e = check_ctx(c)) return e;
+    if (!(tol > 0) || max_iter < 0) return c->fail(AFG_ERR_ARG, "tol must be > 0 and max_iter >= 0");
+    if (cycles < 1) return c->fail(AFG_ERR_ARG, "cycles per correction must be >= 1");
+    if (o && (o->i_rhs < 1 || o->i_rhs > c->n_var)) return c->fail(AFG_ERR_ARG, "bad i_rhs");
+    if (int e = afg_mixed_begin(c, o, have_guess)) return e;
+    const int nt = c->n_trees;
+    std::vector<double> rhs_norm(nt), res(nt);
+    if (int e = forest_norms(c, o->i_rhs, rhs_norm.data())) return e;
+    int it = 0;
+    double worst = 0;
+    for (;; it++) {
+        if (int e = forest_norms(c, o->i_tmp, res.data())) return e;
+        bool done = true;
+        worst = 0;
+        for (int t = 0; t < nt; t++) {
+            done = done && res[t] <= tol * rhs_norm[t];
+            worst = std::max(worst, rhs_norm[t] > 0 ? res[t] / rhs_norm[t] : res[t]);
+        }
+        if (done || it >= max_iter) break;
+        if (int e = afg_mixed_correct(c, o, cycles)) return e;
+    }
+    if (iterations) *iterations = it;
+    if (rel_residual) *rel_residual = worst;
+    return 0;
+}
+
+int afg_forest_max_abs(afg_ctx *c, int iv, int n_trees, double *out)
+{
+    if (int e = check_ctx(c)) return e;
+    if (!out) return c->fail(AFG_ERR_ARG, "null output");
+    if (n_trees != c->n_trees) return c->fail(AFG_ERR_ARG, "the forest has %d trees, not %d", c->n_trees, n_trees);
+    return forest_norms(c, iv, out);
 }
 
 int afg_forest_trees(const afg_ctx *c, int *n_trees, int32_t *highest)

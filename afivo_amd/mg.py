@@ -224,11 +224,24 @@ def mg_fas_vcycle_fp32(tree: DeviceTree, mg: MgOpts, highest_lvl: int, set_resid
     tree._c(_lib.load().afg_fas_vcycle_fp32(tree.h, ctypes.byref(mg), highest_lvl, int(set_residual)))
 
 
+def mixed_begin(tree: DeviceTree, mg: MgOpts, have_guess=False):
+    """Start FP64 defect correction: phi = 0 (unless have_guess), FP64
+    residual in tmp."""
+    tree._c(_lib.load().afg_mixed_begin(tree.h, ctypes.byref(mg), int(have_guess)))
+
+
+def mixed_correct(tree: DeviceTree, mg: MgOpts, cycles=2):
+    """One FP32 multigrid correction (`cycles` V-cycles; the first after
+    mixed_begin starts with an FMG), phi += e, new FP64 residual in tmp.
+    Asynchronous: check the residual with forest_max_abs."""
+    tree._c(_lib.load().afg_mixed_correct(tree.h, ctypes.byref(mg), int(cycles)))
+
+
 def mixed_solve(tree: DeviceTree, mg: MgOpts, tol: float, max_iter: int = 50, have_guess=False,
                 cycles=2):
     """FP64 defect correction with FP32 multigrid corrections (`cycles`
-    V-cycles each) to relative residual `tol`; returns (corrections, relative
-    residual)."""
+    V-cycles each) until every tree's relative residual is <= `tol`; returns
+    (corrections, largest relative residual over the trees)."""
     it, rr = ctypes.c_int(), ctypes.c_double()
     tree._c(_lib.load().afg_mixed_solve(tree.h, ctypes.byref(mg), int(have_guess), float(tol),
                                         int(max_iter), int(cycles), ctypes.byref(it), ctypes.byref(rr)))

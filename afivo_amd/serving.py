@@ -14,13 +14,19 @@ as a new lane.  Lanes advance in turn, one V-cycle each, so the work of
 several lanes overlaps on the device; a lane retires when each of its trees
 meets its tolerance, and its results are copied back.
 
+Lanes solve in mixed precision by default: FP64 defect correction with FP32
+FAS multigrid corrections (afg_mixed_begin / afg_mixed_correct), one
+correction of `cycles` FP32 V-cycles per step; with precision="fp64" a step is
+one FP64 V-cycle after an FP64 FMG.
+
 A request's result does not depend on the mix it was solved in beyond its
 tolerance: a tree in a forest goes through exactly the stages it would go
-through alone, and only the number of V-cycles (the lane stops when its last
+through alone, and only the number of steps (the lane stops when its last
 tree converges) can differ.
 
-`one_at_a_time` replays the same trace through the plain solver: one request
-after the other, each in its own context.
+`one_at_a_time` replays the same trace one request after the other, each in
+its own context: through the plain FP64 solver, or through the same
+mixed-precision steps as a lane of one tree.
 """
 from __future__ import annotations
 
@@ -93,28 +99,40 @@ def make_trace(seed, n_requests, rate, frac_2d=0.7, cells=(1e4, 1e7), tols=(1e-8
     return out
 
 
+MAX_STEPS = 40
+
+
 class Lane:
     """One forest of admitted requests in flight."""
 
-    def __init__(self, reqs, mg, now):
+    def __init__(self, reqs, mg, now, precision="mixed", cycles=2):
         self.reqs = reqs
         self.states = [r.state for r in reqs]
         self.dt = M.DeviceTree.from_forest(self.states)
         self.mg = mg
+        self.mixed = precision == "mixed"
+        self.n_cyc = cycles
         self.tol = np.array([r.tol * r.rhs_norm for r in reqs])
         self.cycles = 0
         for r in reqs:
             r.start = now
-        M.mg_fas_fmg(self.dt, mg, True, False)
+        if self.mixed:
+            M.mixed_begin(self.dt, mg)
+        else:
+            M.mg_fas_fmg(self.dt, mg, True, False)
 
     def step(self):
-        """Check convergence of the last cycle; launch the next one if needed.
-        Returns True when every tree meets its tolerance."""
+        """Check convergence of the last step; launch the next one if needed
+        (a mixed-precision correction, or an FP64 V-cycle).  Returns True when
+        every tree meets its tolerance."""
         res = self.dt.forest_max_abs(3)          # waits for this lane's work
-        if np.all(res <= self.tol) or self.cycles >= 40:
+        if np.all(res <= self.tol) or self.cycles >= MAX_STEPS:
             self.res = res
             return True
-        M.mg_fas_vcycle(self.dt, self.mg, self.dt.highest_lvl, True)
+        if self.mixed:
+            M.mixed_correct(self.dt, self.mg, self.n_cyc)
+        else:
+            M.mg_fas_vcycle(self.dt, self.mg, self.dt.highest_lvl, True)
         self.cycles += 1
         return False
 
@@ -127,26 +145,29 @@ class Lane:
         return self.dt.device_bytes()
 
 
-def forest_bytes(reqs, n_var=3):
-    """Device bytes a forest of these requests takes (pools and tables)."""
+def forest_bytes(reqs, n_var=3, precision="mixed"):
+    """Device bytes a forest of these requests takes (pools and tables; the
+    FP32 copies of phi, rhs and tmp in mixed precision)."""
     boxes = sum(r.state.n_boxes for r in reqs)
     st = reqs[0].state
     stride = -(-(st.nc + 2) ** st.dim // 16) * 16
-    return boxes * (n_var * stride * 8 + 64)
+    per_box = n_var * stride * 8 + (3 * stride * 4 if precision == "mixed" else 0)
+    return boxes * (per_box + 64)
 
 
 def serve(trace, hbm_budget=64 << 30, max_lanes=4, max_trees=64, n_base=50, lane_log=None,
-          builders=3):
+          builders=3, precision="mixed", cycles=2):
     """Run the trace through the continuously batching server; returns the
     requests with start/finish times (seconds after the trace start).
 
-    Lanes are set up (forest tables, uploads, the FMG launch) by `builders`
-    host threads while the main thread drives the running lanes, so host
-    work overlaps device work.  If given, `lane_log` receives one (admitted,
-    setup_s, retired, n_trees, cycles) tuple per lane."""
+    Lanes are set up (forest tables, uploads, the first launches) by
+    `builders` host threads while the main thread drives the running lanes,
+    so host work overlaps device work.  If given, `lane_log` receives one
+    (admitted, setup_s, retired, n_trees, steps) tuple per lane."""
     import concurrent.futures as cf
 
     mg = M.default_mg(n_base=n_base)
+    fbytes = lambda g: forest_bytes(g, precision=precision)   # noqa: E731
     pending = collections.deque(sorted(trace, key=lambda r: r.arrival))
     waiting, lanes, building = [], [], []
     # Retired lanes release their device memory only while no lane runs (or
@@ -162,7 +183,7 @@ def serve(trace, hbm_budget=64 << 30, max_lanes=4, max_trees=64, n_base=50, lane
     t0 = time.perf_counter()
 
     def build(group, t_adm):
-        lane = Lane(group, mg, t_adm)
+        lane = Lane(group, mg, t_adm, precision, cycles)
         lane.t_adm, lane.setup = t_adm, time.perf_counter() - t0 - t_adm
         return lane
 
@@ -178,15 +199,15 @@ def serve(trace, hbm_budget=64 << 30, max_lanes=4, max_trees=64, n_base=50, lane
                 cls = (waiting[0].state.dim, waiting[0].state.nc)
                 group = [r for r in waiting if (r.state.dim, r.state.nc) == cls][:max_trees]
                 used = (sum(l.device_bytes() for l in lanes + retired) +
-                        sum(forest_bytes(g) for g, _ in building))
-                if retired and used + forest_bytes(group[:1]) > hbm_budget:
+                        sum(fbytes(g) for g, _ in building))
+                if retired and used + fbytes(group[:1]) > hbm_budget:
                     if building:
                         break                           # release once the builders are done
                     release()
                     used = sum(l.device_bytes() for l in lanes)
-                while len(group) > 1 and used + forest_bytes(group) > hbm_budget:
+                while len(group) > 1 and used + fbytes(group) > hbm_budget:
                     group = group[:-1]
-                if used + forest_bytes(group) > hbm_budget and (lanes or building):
+                if used + fbytes(group) > hbm_budget and (lanes or building):
                     break                               # wait for memory to free up
                 for r in group:
                     waiting.remove(r)
@@ -213,9 +234,10 @@ def serve(trace, hbm_budget=64 << 30, max_lanes=4, max_trees=64, n_base=50, lane
     return trace
 
 
-def one_at_a_time(trace, n_base=50):
-    """The same trace through the plain solver, one request after the other in
-    arrival order, each in its own context."""
+def one_at_a_time(trace, n_base=50, precision="fp64", cycles=2):
+    """The same trace one request after the other in arrival order, each in
+    its own context: the plain FP64 solver (FMG, then V-cycles), or the
+    mixed-precision steps of a lane."""
     mg = M.default_mg(n_base=n_base)
     t0 = time.perf_counter()
     for r in sorted(trace, key=lambda q: q.arrival):
@@ -224,13 +246,19 @@ def one_at_a_time(trace, n_base=50):
             time.sleep(r.arrival - now)
         r.start = time.perf_counter() - t0
         dt = M.DeviceTree.from_state(r.state)
-        M.mg_fas_fmg(dt, mg, True, False)
+        if precision == "mixed":
+            M.mixed_begin(dt, mg)
+        else:
+            M.mg_fas_fmg(dt, mg, True, False)
         k = 0
         while True:
-            res = M.max_abs_cc(dt, 3)
-            if res <= r.tol * r.rhs_norm or k >= 40:
+            res = float(dt.forest_max_abs(3)[0])
+            if res <= r.tol * r.rhs_norm or k >= MAX_STEPS:
                 break
-            M.mg_fas_vcycle(dt, mg, dt.highest_lvl, True)
+            if precision == "mixed":
+                M.mixed_correct(dt, mg, cycles)
+            else:
+                M.mg_fas_vcycle(dt, mg, dt.highest_lvl, True)
             k += 1
         r.cycles, r.residual, r.phi = k, res / r.rhs_norm, dt.download(1)
         dt.close()

@@ -51,8 +51,10 @@ def parse():
                          "c5: served request trace")
     ap.add_argument("--requests", type=int, default=64, help="c5: number of requests")
     ap.add_argument("--rate", type=float, default=100.0, help="c5: arrival rate (requests/s)")
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
+                    help="c5: arithmetic of the served lanes")
     ap.add_argument("--trees", type=int, default=256, help="c3: number of trees")
-    ap.add_argument("--tol", type=float, default=1e-10, help="c3: relative residual to reach")
+    ap.add_argument("--tol", type=float, default=1e-10, help="c3, c4: relative residual to reach")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy rhs to tmp over every box once, from cold caches (for the "
                          "HBM byte counters' scale factor), print its byte count and exit")
@@ -225,7 +227,7 @@ def bench_c4(a):
     def fp64_solve():
         M.mg_fas_fmg(dt, mg, True, False)
         k = 0
-        while M.max_abs_cc(dt, 3) > a.tol * rhs_norm and k < 40:
+        while dt.forest_max_abs(3)[0] > a.tol * rhs_norm and k < 40:
             M.mg_fas_vcycle(dt, mg, dt.highest_lvl, True)
             k += 1
         return k
@@ -297,8 +299,9 @@ def bench_c5(a):
     t0 = time.time()
     trace = S.make_trace(a.seed, a.requests, a.rate)
     build_s = time.time() - t0
-    S.serve(trace)
+    S.serve(trace, precision=a.precision)
     served = S.summary(trace)
+    steps = np.array([r.cycles for r in trace])
     cells = np.array([r.leaf_cells for r in trace])
     for r in trace:
         r.start = r.finish = -1.0
@@ -306,21 +309,26 @@ def bench_c5(a):
     S.one_at_a_time(trace)
     solo = S.summary(trace)
     out = {
-        "metric": "solves/s and p50/p99 latency (served, FP64)",
+        "metric": "solves/s and p50/p99 latency (served, %s)" % ("mixed precision" if a.precision == "mixed"
+                                                                 else "FP64"),
         "value": served["solves_per_s"], "unit": "solves/s",
         "n_gpus": 1, "steps": a.requests, "warmup": 0,
         "ms_per_step": 1e3 / served["solves_per_s"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
+        "dtype": "f32 in f64 defect correction" if a.precision == "mixed" else "f64", "data": "synthetic",
         "config": {
             "workload": "serving: %d requests, Poisson arrivals at %g/s (seed %d); 70%% 2D / 30%% 3D "
                         "trees, n_cell 8, leaf cells log-uniform in [1e4, 1e7] (uniform to the level "
                         "below the target, then two levels around 1-4 Gaussians); tolerance 1e-8 or "
-                        "1e-10 relative residual; FP64; continuous batching into forests, up to 4 "
-                        "lanes, 64 trees per lane, lanes set up by 3 host threads" % (a.requests, a.rate, a.seed),
+                        "1e-10 relative residual; %s; continuous batching into forests, up to 4 "
+                        "lanes, 64 trees per lane, lanes set up by 3 host threads; baseline: the plain "
+                        "FP64 solver (FMG + V-cycles), one request at a time"
+                        % (a.requests, a.rate, a.seed,
+                           "mixed precision (FP64 defect correction, FP32 FAS corrections of 2 V-cycles)"
+                           if a.precision == "mixed" else "FP64 (FMG + V-cycles)"),
             "leaf_cells": [int(cells.min()), int(np.median(cells)), int(cells.max())],
         },
-        "served": served,
+        "served": dict(served, steps_per_request=[int(steps.min()), int(steps.max())]),
         "one_at_a_time": solo,
         "host_tree_build_s": build_s,
         "roofline": None,
@@ -334,7 +342,7 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported "
-              "(the multi-GPU design is described in DESIGN.md, not built)", file=sys.stderr)
+              "(sharding over several GPUs is designed, not built)", file=sys.stderr)
         return 2
     if a.workload == "c3":
         return bench_c3(a)

@@ -176,16 +176,24 @@ int afg_fp32_to_cc(afg_ctx *ctx, int iv32, int iv, int add);
  * operators come from the FP64 eps). */
 int afg_fas_fmg_fp32(afg_ctx *ctx, const afg_mg_opts *opts, int set_residual, int have_guess);
 int afg_fas_vcycle_fp32(afg_ctx *ctx, const afg_mg_opts *opts, int highest_lvl, int set_residual);
-/* Solve A phi = rhs (the options' operator, phi, rhs, tmp, eps in FP64) to
- * max|rhs - A phi| <= tol * max|rhs| over leaf interiors, by FP64 defect
- * correction: parents' phi set to the restriction of their children's (the
- * FAS fixed point), the residual r = rhs - A phi in FP64 on every level, a
- * correction e from A e = r by FP32 FAS multigrid (`cycles` V-cycles from
- * e = 0, the first one of the first correction an FMG; in the FP32 copies of
- * the same variables), phi += e, until the tolerance or max_iter corrections.  phi starts from zero unless
- * have_guess; on return tmp holds the FP64 residual.  Physical boundaries
- * must be homogeneous (b = 0: the correction solves with the same ghost
- * rules).  One tree only.  No counterpart in the reference. */
+/* Solve A phi = rhs (the options' operator, phi, rhs, tmp, eps in FP64) by
+ * FP64 defect correction, in steps (no counterpart in the reference):
+ * afg_mixed_begin sets phi = 0 unless have_guess and stores the FP64
+ * residual r = rhs - A phi in tmp on every level, phi of every parent first
+ * set to the restriction of its children's (the FAS fixed point).  Each
+ * afg_mixed_correct computes a correction e from A e = r by FP32 FAS
+ * multigrid in the FP32 copies of the same variables (`cycles` V-cycles from
+ * e = 0; the first correction after afg_mixed_begin starts with an FMG),
+ * adds it to phi and stores the new FP64 residual.  The caller checks the
+ * residual (afg_forest_max_abs of tmp).  Physical boundaries must be
+ * homogeneous (b = 0: the correction solves with the same ghost rules).
+ * Forests: every tree is corrected together, each with its own stages. */
+int afg_mixed_begin(afg_ctx *ctx, const afg_mg_opts *opts, int have_guess);
+int afg_mixed_correct(afg_ctx *ctx, const afg_mg_opts *opts, int cycles);
+/* The whole solve: afg_mixed_begin, then corrections until every tree has
+ * max|rhs - A phi| <= tol * max|rhs| over its leaf interiors, or max_iter
+ * corrections.  Returns the number of corrections and the largest relative
+ * residual over the trees; tmp holds the FP64 residual. */
 int afg_mixed_solve(afg_ctx *ctx, const afg_mg_opts *opts, int have_guess, double tol, int max_iter,
                     int cycles, int *iterations, double *rel_residual);
 

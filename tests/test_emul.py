@@ -39,10 +39,13 @@ from test_gpu_parity import (  # noqa: E402,F401
 )
 from test_forest_gpu import (  # noqa: E402,F401
     test_forest_equals_single_solves,
+    test_forest_mixed_precision_equals_single_solves,
     test_forest_partial_vcycle_and_oracle,
     test_forest_rejects_links_across_trees,
+    test_mixed_correct_needs_begin,
 )
 from test_serving import (  # noqa: E402,F401
+    test_mixed_serving_matches_fp64_solver,
     test_serving_admits_by_memory_budget,
     test_serving_meets_tolerances_and_matches_solo_runs,
 )

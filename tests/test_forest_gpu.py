@@ -75,3 +75,45 @@ def test_forest_rejects_links_across_trees():
     bad[1] = 1 - bad[1]                # one box moved to the other tree
     with pytest.raises(AfgError):
         M.DeviceTree(st.dim, st.nc, 3, st.recs, st.dr_base, bad)
+
+
+@pytest.mark.parametrize("dim,n_trees", [(2, 5), (3, 3)])
+def test_forest_mixed_precision_equals_single_solves(dim, n_trees):
+    """Mixed-precision corrections on a forest: every tree gets exactly the
+    phi and residual of the same number of corrections alone, and the forest
+    solve meets the tolerance on every tree."""
+    sts = small_forest(dim, n_trees)
+    mg = M.default_mg(n_base=4)
+    fdt = M.DeviceTree.from_forest(sts)
+    M.mixed_begin(fdt, mg)
+    fnorms = [fdt.forest_max_abs(3)]
+    for _ in range(3):
+        M.mixed_correct(fdt, mg, 2)
+        fnorms.append(fdt.forest_max_abs(3))
+    fdata = [F.split(sts, fdt.download(iv)) for iv in (1, 3)]
+    for t, st in enumerate(sts):
+        dt = M.DeviceTree.from_state(st)
+        M.mixed_begin(dt, mg)
+        norms = [dt.forest_max_abs(3)]
+        for _ in range(3):
+            M.mixed_correct(dt, mg, 2)
+            norms.append(dt.forest_max_abs(3))
+        assert_bitexact(fdata[0][t], dt.download(1), f"tree {t} phi")
+        assert_bitexact(fdata[1][t], dt.download(3), f"tree {t} residual")
+        assert np.array_equal([n[t] for n in fnorms], [n[0] for n in norms]), f"tree {t} residuals"
+        dt.close()
+    # the whole solve: every tree below the tolerance
+    it, rr = M.mixed_solve(fdt, mg, 1e-10, max_iter=30)
+    rhs = fdt.forest_max_abs(2)
+    res = fdt.forest_max_abs(3)
+    assert rr <= 1e-10 and np.all(res <= 1e-10 * rhs) and it >= 2
+    assert rr == pytest.approx(float(np.max(res / rhs)), rel=0, abs=0)
+    fdt.close()
+
+
+def test_mixed_correct_needs_begin():
+    sts = small_forest(2, 2)
+    fdt = M.DeviceTree.from_forest(sts)
+    with pytest.raises(AfgError):
+        M.mixed_correct(fdt, M.default_mg(n_base=4), 2)
+    fdt.close()

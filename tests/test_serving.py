@@ -12,13 +12,10 @@ def small_trace(n=8, rate=2000.0, seed=4):
     return S.make_trace(seed, n, rate, cells=(1e3, 2e4), tols=(1e-8, 1e-10))
 
 
-def test_serving_meets_tolerances_and_matches_solo_runs():
-    """Every request meets its tolerance in the mix; a request that ran as
-    many V-cycles in the mix as alone has bit-identical phi, the others agree
-    within their tolerance."""
-    mix = S.serve(small_trace(), max_lanes=2, max_trees=3, n_base=20)
-    solo = S.one_at_a_time(small_trace(), n_base=20)
-    assert len({(r.state.dim) for r in mix}) == 2, "the trace should mix 2D and 3D"
+def check_mix_against_solo(mix, solo):
+    """Every request meets its tolerance in the mix and alone; a request that
+    took as many steps in the mix as alone has bit-identical phi, the others
+    agree within their tolerance.  Returns the number of identical ones."""
     same = 0
     for a, b in zip(sorted(mix, key=lambda r: r.rid), sorted(solo, key=lambda r: r.rid)):
         assert a.finish >= a.start >= a.arrival >= 0
@@ -29,7 +26,29 @@ def test_serving_meets_tolerances_and_matches_solo_runs():
         else:
             scale = np.abs(b.phi).max()
             assert np.abs(a.phi - b.phi).max() <= 100 * a.tol * scale, f"request {a.rid}"
-    assert same > 0
+    return same
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_serving_meets_tolerances_and_matches_solo_runs(precision):
+    """The same requests served in forests of up to 3 trees and solved one at
+    a time through the same steps (mixed-precision corrections, or FP64
+    V-cycles) give the same phi."""
+    mix = S.serve(small_trace(), max_lanes=2, max_trees=3, n_base=20, precision=precision)
+    solo = S.one_at_a_time(small_trace(), n_base=20, precision=precision)
+    assert len({(r.state.dim) for r in mix}) == 2, "the trace should mix 2D and 3D"
+    assert check_mix_against_solo(mix, solo) > 0
+
+
+def test_mixed_serving_matches_fp64_solver():
+    """Mixed-precision lanes reach each tolerance with the FP64 solver's
+    solution (within 100 x tolerance of max|phi|)."""
+    mix = S.serve(small_trace(), max_lanes=2, max_trees=3, n_base=20)
+    ref = S.one_at_a_time(small_trace(), n_base=20, precision="fp64")
+    for a, b in zip(sorted(mix, key=lambda r: r.rid), sorted(ref, key=lambda r: r.rid)):
+        assert a.residual <= a.tol
+        scale = np.abs(b.phi).max()
+        assert np.abs(a.phi - b.phi).max() <= 100 * a.tol * scale, f"request {a.rid}"
 
 
 def test_serving_admits_by_memory_budget():
