@@ -20,7 +20,6 @@ F64P = ctypes.POINTER(ctypes.c_double)
 
 AFG_OP_LPL, AFG_OP_AUTO = 0, 1
 AFG_BC_DIRICHLET, AFG_BC_NEUMANN = -1, -2
-AFG_PREC_FP64, AFG_PREC_MIXED = 0, 1
 
 
 class MgOpts(ctypes.Structure):

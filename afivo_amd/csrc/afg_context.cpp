@@ -112,6 +112,7 @@ struct afg_ctx {
     double *d_chunk_max = nullptr, *d_tree_out = nullptr, *h_tree_out = nullptr;
     // mixed-precision solve in steps: begun, and the next correction is the first (an FMG)
     bool mixed_ready = false, mixed_fmg = false;
+    bool tags_set = false;               // every box has its eps tag (they are set once)
 
     // tree
     bool have_tree = false;
@@ -504,11 +505,15 @@ int make_mg(afg_ctx *c, const afg_mg_opts *o, MgT<R> *m)
 // Lazy tags of the auto operators (m_aX_mg.f90:567,602,655): boxes whose tag
 // is unset get it now; eps is not changed by the solver, so this equals the
 // reference's first-use evaluation.
+// The lazy tags of mg$D_set_box_tag (m_aX_mg.f90:667-706): set once per box,
+// from eps at the first call of an auto operator, and never re-evaluated.
+// The first solve sets every box's; later calls have nothing to do.
 void set_tags(Mg &m)
 {
-    if (!m.V.op_auto) return;
+    if (!m.V.op_auto || m.c->tags_set) return;
     for (int l = m.c->min_lvl; l <= m.c->highest_lvl; l++)
         launch_set_tags(m.c->dim, m.c->stream, m.T, level_args(m.c, l), m.V.eps);
+    m.c->tags_set = true;
 }
 
 template <typename R>
@@ -1016,6 +1021,7 @@ int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base,
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     free_tree(c);
     c->mixed_ready = c->mixed_fmg = false;
+    c->tags_set = false;
     const int dim = c->dim, nch = c->nch, nnb = c->nnb;
     const int rl = 3 + dim + nch + nnb;
     int max_id = 0, min_lvl = 1 << 30, hi_lvl = -(1 << 30);

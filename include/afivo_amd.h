@@ -51,11 +51,6 @@ enum afg_op {
 enum { AFG_BC_DIRICHLET = -1, AFG_BC_NEUMANN = -2 };
 
 /* Arithmetic of the relaxation, transfer and correction stages. */
-enum afg_precision {
-    AFG_PREC_FP64 = 0,   /* every stage in FP64 (reference arithmetic) */
-    AFG_PREC_MIXED = 1   /* FP32 GSRB/restrict/prolong inside FP64 defect correction */
-};
-
 /* The multigrid options of mg$D_t (m_aX_mg.f90:20-52).  Unset values are -1,
  * filled in by afg_mg_init exactly as mg$D_init_mg does (m_aX_mg.f90:130-154). */
 typedef struct afg_mg_opts {
