@@ -67,6 +67,9 @@ SIGNATURES = {
     "afg_fp32_to_cc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "afg_fas_fmg_fp32": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int, ctypes.c_int]),
     "afg_fas_vcycle_fp32": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int, ctypes.c_int]),
+    "afg_set_tile": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "afg_get_tile": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "afg_autotune": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int, ctypes.c_int, F64P]),
     "afg_mixed_begin": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int]),
     "afg_mixed_correct": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int]),
     "afg_mixed_solve": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int, F64, ctypes.c_int,

@@ -306,18 +306,30 @@ AFG_DEV void gsrb_body(const DevTree &T, const LevelArgs &L, const LevelViewT<R>
 // Every global read is issued in stage A.  Results equal gsrb_body's: the
 // ghosts a pass computes depend only on cells of the other colour and on the
 // parent level, which the pass does not change.
-template <int DIM, int NC>
+template <int DIM, int NC, int V = 0>
 struct GsrbTile {
-    static constexpr int THREADS = 256;
     static constexpr int S = NC + 2;
-    static constexpr int LEN = ipow(S, DIM);            // LDS doubles per box
+    static constexpr int LEN = ipow(S, DIM);            // LDS values per box
     static constexpr int CELLS = ipow(NC, DIM);
     static constexpr int HALF = CELLS / 2;              // cells of one colour
-    static constexpr int BOXES = HALF >= THREADS ? 1 : THREADS / HALF;
+    // shape V: workgroup size and boxes per workgroup.  0: 256 threads and
+    // as many boxes as give one cell of the colour per thread (at least
+    // one); 1: twice the boxes, or (one box of 1024+ cells of a colour) 512
+    // threads; 2: 512 threads on 2 boxes, or 128 threads on half the boxes,
+    // or (one box of 1024+ cells) 1024 threads.
+    static constexpr int B0 = HALF >= 256 ? 1 : 256 / HALF;
+    static constexpr int THREADS = V == 0 ? 256
+                                   : V == 1 ? (HALF >= 1024 ? 512 : 256)
+                                   : (HALF >= 1024 ? 1024 : HALF >= 256 ? 512 : 128);
+    static constexpr int BOXES = V == 0 ? B0
+                                 : V == 1 ? (HALF >= 1024 ? 1 : 2 * B0)
+                                 : (HALF >= 1024 ? 1 : HALF >= 256 ? 2 : (B0 > 1 ? B0 / 2 : 1));
     static constexpr int FACE = DIM == 2 ? NC : NC * NC;
     static constexpr int GHOSTS = DIM * FACE;           // face ghosts of one colour
     static constexpr int PER = (BOXES * HALF + THREADS - 1) / THREADS;   // cells per thread
 };
+
+
 
 // Workgroup index -> tile index such that each of the 8 XCDs, which take
 // workgroups round-robin, runs one contiguous range of tiles (neighbouring
@@ -329,11 +341,11 @@ AFG_HD inline int xcd_tile(int blk, int n_blk)
     return x * q + (x < r ? x : r) + y;
 }
 
-template <int DIM, int NC, int FLY, typename R>
+template <int DIM, int NC, int FLY, int V, typename R>
 AFG_DEV void tile_stage_a(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &v, R *lds,
                           int box0, int rb, int tid, R *rhs)
 {
-    using G = GsrbTile<DIM, NC>;
+    using G = GsrbTile<DIM, NC, V>;
     constexpr int S = G::S;
     // rhs of this thread's cells of the colour
 #pragma unroll
@@ -390,11 +402,11 @@ AFG_DEV void tile_stage_a(const DevTree &T, const LevelArgs &L, const LevelViewT
     }
 }
 
-template <int DIM, int NC, typename R>
+template <int DIM, int NC, int V, typename R>
 AFG_DEV void tile_stage_b(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &v,
                           const R *lds, int box0, int op_auto, int rb, int tid, const R *rhs)
 {
-    using G = GsrbTile<DIM, NC>;
+    using G = GsrbTile<DIM, NC, V>;
     constexpr int S = G::S;
 #pragma unroll
     for (int r = 0; r < G::PER; r++) {

@@ -34,6 +34,7 @@ struct Level {
     // level, deepest trees first; n_act[b - min_lvl] boxes (parents, leaves)
     // belong to trees whose highest level is >= b (b = min_lvl..highest+1).
     std::vector<int> n_act, n_par_act, n_leaf_act;
+    int tile[2] = {0, 0};   // tile shape of the colour pass: FP64, FP32 stages
 };
 
 struct BcRec {
@@ -221,6 +222,7 @@ LevelArgs level_args(const afg_ctx *c, int lvl)
     a.slot0 = L.slot0;
     a.n = L.n_act.empty() ? L.n : L.n_act[c->bound - c->min_lvl];
     a.nc = L.nc;
+    a.tile = 0;
     a.pnc = lvl > c->min_lvl ? c->levels[lvl - 1 - c->min_lvl].nc : 0;
     a.dr = L.dr;
     a.dx2 = L.dx2;
@@ -418,7 +420,9 @@ void timed_passes(afg_ctx *c, const DevTree &T, const LevelArgs &L, int lvl, con
         }
         hipEventRecordWithFlags(ev.first, c->stream, flags);
     }
-    for (int rb = 1; rb <= n_pass; rb++) launch_gsrb_pass(c->dim, c->stream, T, L, V, rb, rb == 1 ? first_fly : 1);
+    LevelArgs Lt = L;
+    Lt.tile = c->levels[lvl - c->min_lvl].tile[sizeof(R) == 4];
+    for (int rb = 1; rb <= n_pass; rb++) launch_gsrb_pass(c->dim, c->stream, T, Lt, V, rb, rb == 1 ? first_fly : 1);
     if (!on) return;
     hipEventRecordWithFlags(ev.second, c->stream, flags);
     const TimedBlock tb{n_pass, n_pass * gsrb_pass_bytes(c, L, V)};
@@ -918,6 +922,72 @@ int build_base(afg_ctx *c)
 // ===========================================================================
 // C ABI
 
+// The autotuner of the tiled colour pass: on every level with more than one
+// tile shape, each shape runs `reps` passes (colours alternating, face
+// ghosts computed) on a copy of phi, three times; the shape with the least
+// time per pass is kept for the level.  Every shape computes the same
+// values, so the choice changes speed only.
+template <typename R>
+static int tune_tiles(afg_ctx *c, const afg_mg_opts *o, int reps, double *ms, const double *phi64)
+{
+    MgT<R> m;
+    if (int e = make_mg(c, o, &m)) return e;
+    const size_t n = (size_t)c->n_slots * c->stride;
+    R *scratch = nullptr;
+    HIP_TRY(c, hipMalloc(&scratch, n * sizeof(R)));
+    MgVarsT<R> V = m.V;
+    V.phi = scratch;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int err = 0;
+    const int bound = c->bound;
+    c->bound = c->min_lvl;      // every tree's boxes
+    auto run = [&]() -> int {
+        if constexpr (sizeof(R) == 4)
+            launch_to_fp32(c->stream, (long long)n, phi64, scratch);
+        else
+            HIP_TRY(c, hipMemcpyAsync(scratch, phi64, n * sizeof(R), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipEventCreate(&e0));
+        HIP_TRY(c, hipEventCreate(&e1));
+        for (int lvl = c->min_lvl; lvl <= c->highest_lvl; lvl++) {
+            Level &Lv = c->levels[lvl - c->min_lvl];
+            const int nv = tile_shapes(Lv.nc);
+            LevelArgs L = level_args(c, lvl);
+            if (nv < 2 || L.n == 0) continue;
+            double best = 0;
+            for (int v = 0; v < nv; v++) {
+                L.tile = v;
+                for (int r = 0; r < 2; r++) launch_gsrb_pass(c->dim, c->stream, m.T, L, V, 1 + r, 1);
+                float t_min = 0;
+                for (int trial = 0; trial < 3; trial++) {
+                    HIP_TRY(c, hipEventRecord(e0, c->stream));
+                    for (int r = 0; r < reps; r++) launch_gsrb_pass(c->dim, c->stream, m.T, L, V, 1 + (r & 1), 1);
+                    HIP_TRY(c, hipEventRecord(e1, c->stream));
+                    HIP_TRY(c, hipEventSynchronize(e1));
+                    float t = 0;
+                    HIP_TRY(c, hipEventElapsedTime(&t, e0, e1));
+                    if (trial == 0 || t < t_min) t_min = t;
+                }
+                const double per = (double)t_min / reps;
+                if (ms) ms[(lvl - c->min_lvl) * 3 + v] = per;
+                if (v == 0 || per < best) {
+                    best = per;
+                    Lv.tile[sizeof(R) == 4] = v;
+                }
+            }
+        }
+        return 0;
+    };
+    err = run();
+    c->bound = bound;
+    hipStreamSynchronize(c->stream);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipFree(scratch);
+    for (auto &g : c->graphs) destroy_graph(g.second);
+    c->graphs.clear();
+    return err;
+}
+
 extern "C" {
 
 int afg_ctx_create(int dim, int n_cell, int n_var, afg_ctx **out)
@@ -1095,6 +1165,9 @@ int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base,
             return c->fail(AFG_ERR_ARG, "level %d: n_cell %d cannot be coarsened to it", lvl, c->nc);
         if (L.nc < 2 || (L.nc & 1))
             return c->fail(AFG_ERR_UNSUPPORTED, "level %d: n_cell %d not supported (coarse levels need an even n_cell >= 2)", lvl, L.nc);
+        // default tile shapes (afg_autotune refines them): 3D boxes of 16^3
+        // cells take 1024 threads, two cells of the colour per thread
+        L.tile[0] = L.tile[1] = dim == 3 && L.nc == 16 ? 2 : 0;
         // box%dr = dr_base * 0.5**(lvl-1) (m_aX_core.f90:196, halved per child :764)
         L.dr = dr_base * std::pow(0.5, lvl - 1);
         L.dx2 = L.dr * L.dr;
@@ -1785,6 +1858,41 @@ int afg_set_bottom_cells(afg_ctx *c, int64_t cells, int64_t wave_cells)
     c->bottom_cells = cells;
     c->bottom_wave_cells = wave_cells;
     return choose_bottom(c);
+}
+
+int afg_set_tile(afg_ctx *c, int lvl, int fp32, int shape)
+{
+    if (int e = check_ctx(c)) return e;
+    if (int e = check_level(c, lvl, false)) return e;
+    Level &L = c->levels[lvl - c->min_lvl];
+    if (shape < 0 || shape >= tile_shapes(L.nc))
+        return c->fail(AFG_ERR_ARG, "level %d (nc %d) has tile shapes 0..%d", lvl, L.nc, tile_shapes(L.nc) - 1);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    L.tile[fp32 != 0] = shape;
+    for (auto &g : c->graphs) destroy_graph(g.second);
+    c->graphs.clear();
+    return 0;
+}
+
+int afg_get_tile(const afg_ctx *c, int lvl, int fp32, int *shape)
+{
+    if (!c || !c->have_tree || !shape || lvl < c->min_lvl || lvl > c->highest_lvl) return AFG_ERR_ARG;
+    *shape = c->levels[lvl - c->min_lvl].tile[fp32 != 0];
+    return 0;
+}
+
+int afg_autotune(afg_ctx *c, const afg_mg_opts *o, int fp32, int reps, double *ms)
+{
+    if (int e = check_ctx(c)) return e;
+    if (reps < 1) return c->fail(AFG_ERR_ARG, "reps must be >= 1");
+    if (c->capturing) return c->fail(AFG_ERR_STATE, "autotune inside a graph capture");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (ms)
+        for (int i = 0; i < 3 * (int)c->levels.size(); i++) ms[i] = -1;
+    Mg m;
+    if (int e = make_mg(c, o, &m)) return e;
+    set_tags(m);
+    return fp32 ? tune_tiles<float>(c, o, reps, ms, m.V.phi) : tune_tiles<double>(c, o, reps, ms, m.V.phi);
 }
 
 int afg_set_graphs(afg_ctx *c, int enabled)

@@ -73,19 +73,19 @@ __global__ __launch_bounds__(BLOCK) void k_gsrb_pass(DevTree T, LevelArgs L, Lev
         gsrb_body<DIM, NC>(T, L, v, v, op_auto, rb, fly, gid);
 }
 
-// one colour pass by tiles of boxes in LDS (NC in 2, 4, 8, 16)
-template <int DIM, int NC, int FLY, typename R>
-__global__ __launch_bounds__(BLOCK) void k_gsrb_tile(DevTree T, LevelArgs L, LevelViewT<R> v,
-                                                     int op_auto, int rb)
+// one colour pass by tiles of boxes in LDS (NC in 2, 4, 8, 16; tile shape V)
+template <int DIM, int NC, int FLY, typename R, int V>
+__global__ __launch_bounds__((GsrbTile<DIM, NC, V>::THREADS)) void k_gsrb_tile(DevTree T, LevelArgs L,
+                                                                              LevelViewT<R> v, int op_auto,
+                                                                              int rb)
 {
-    using G = GsrbTile<DIM, NC>;
-    static_assert(G::THREADS == BLOCK, "tile size");
+    using G = GsrbTile<DIM, NC, V>;
     __shared__ R lds[G::BOXES * G::LEN];
     R rhs[G::PER];
     const int box0 = xcd_tile(blockIdx.x, gridDim.x) * G::BOXES;
-    tile_stage_a<DIM, NC, FLY>(T, L, v, lds, box0, rb, threadIdx.x, rhs);
+    tile_stage_a<DIM, NC, FLY, V>(T, L, v, lds, box0, rb, threadIdx.x, rhs);
     __syncthreads();
-    tile_stage_b<DIM, NC>(T, L, v, lds, box0, op_auto, rb, threadIdx.x, rhs);
+    tile_stage_b<DIM, NC, V>(T, L, v, lds, box0, op_auto, rb, threadIdx.x, rhs);
 }
 
 template <int DIM, int NC, typename R>
@@ -379,14 +379,25 @@ void launch_diff(int dim, hipStream_t s, const DevTree &T, const ListArgs &P, co
     AFG_DISPATCH(dim, P.nc, k_diff, n_blocks(th), T, P, pool_view(T, V));
 }
 
+template <int DIM, int NC, typename R, int V>
+void gsrb_tiles_v(hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &Vs, int rb, int fly)
+{
+    using G = GsrbTile<DIM, NC, V>;
+    const int grid = (L.n + G::BOXES - 1) / G::BOXES;
+    if (fly)
+        k_gsrb_tile<DIM, NC, 1, R, V><<<grid, G::THREADS, 0, s>>>(T, L, pool_view(T, Vs), Vs.op_auto, rb);
+    else
+        k_gsrb_tile<DIM, NC, 0, R, V><<<grid, G::THREADS, 0, s>>>(T, L, pool_view(T, Vs), Vs.op_auto, rb);
+}
+
 template <int DIM, int NC, typename R>
 void gsrb_tiles(hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int rb, int fly)
 {
-    const int grid = (L.n + GsrbTile<DIM, NC>::BOXES - 1) / GsrbTile<DIM, NC>::BOXES;
-    if (fly)
-        k_gsrb_tile<DIM, NC, 1, R><<<grid, BLOCK, 0, s>>>(T, L, pool_view(T, V), V.op_auto, rb);
-    else
-        k_gsrb_tile<DIM, NC, 0, R><<<grid, BLOCK, 0, s>>>(T, L, pool_view(T, V), V.op_auto, rb);
+    if constexpr (tile_shapes(NC) > 1) {
+        if (L.tile == 1) return gsrb_tiles_v<DIM, NC, R, 1>(s, T, L, V, rb, fly);
+        if (L.tile == 2) return gsrb_tiles_v<DIM, NC, R, 2>(s, T, L, V, rb, fly);
+    }
+    gsrb_tiles_v<DIM, NC, R, 0>(s, T, L, V, rb, fly);
 }
 
 template <typename R>

@@ -38,8 +38,13 @@ struct DevTree {
 
 // One level: slots [slot0, slot0 + n), nc cells per dimension, cell size dr;
 // pnc is the cell count of the parent level.
+// Number of tile shapes of the tiled colour pass for a cell count: nc 8 and
+// 16 have three, the others one.
+constexpr int tile_shapes(int nc) { return nc == 8 || nc == 16 ? 3 : 1; }
+
 struct LevelArgs {
     int slot0, n, nc, pnc;
+    int tile;           // shape of the tiled colour pass (TileShape), per level and precision
     double dr;          // box%dr of the level
     double dx2;         // dr * dr  (dr**2, m_aX_mg.f90:730)
     double inv_dr_sq;   // 1 / (dr * dr)  (m_aX_mg.f90:772)

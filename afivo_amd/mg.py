@@ -82,6 +82,23 @@ class DeviceTree:
         self._c(_lib.load().afg_forest_max_abs(self.h, iv, n, _f64(out)))
         return out
 
+    def set_tile(self, lvl, shape, fp32=False):
+        """Tile shape (0, 1, 2) of the colour pass on level lvl."""
+        self._c(_lib.load().afg_set_tile(self.h, int(lvl), int(fp32), int(shape)))
+
+    def tile(self, lvl, fp32=False):
+        v = ctypes.c_int()
+        self._c(_lib.load().afg_get_tile(self.h, int(lvl), int(fp32), ctypes.byref(v)))
+        return v.value
+
+    def autotune(self, mg, fp32=False, reps=8):
+        """Pick the fastest tile shape per level; returns ms per pass as an
+        array [level - min_lvl, shape] (-1 where not timed)."""
+        n = self.highest_lvl - self.min_lvl + 1
+        ms = np.zeros(3 * n)
+        self._c(_lib.load().afg_autotune(self.h, ctypes.byref(mg), int(fp32), int(reps), _f64(ms)))
+        return ms.reshape(n, 3)
+
     def close(self):
         if getattr(self, "h", None):
             _lib.load().afg_ctx_destroy(self.h)

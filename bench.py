@@ -221,6 +221,12 @@ def bench_c4(a):
     rhs_norm = float(np.abs(st.data[1][mask]).max())
 
     dt = M.DeviceTree.from_state(st)
+    # the autotuner picks the tile shapes per level for both precisions
+    # (on a copy of phi, before timing)
+    dt.autotune(mg)
+    dt.autotune(mg, fp32=True)
+    tiles = {"fp64": [dt.tile(lvl) for lvl in range(dt.min_lvl, dt.highest_lvl + 1)],
+             "fp32": [dt.tile(lvl, True) for lvl in range(dt.min_lvl, dt.highest_lvl + 1)]}
 
     def reset():
         dt.upload(1, st.data[0])
@@ -287,6 +293,7 @@ def bench_c4(a):
                         % (a.seed, P.THR_C4, p.eps_hi),
             "boxes": int(st.n_boxes), "leaf_cells": leaf_cells,
             "levels": [int(st.min_lvl), int(st.highest_lvl)],
+            "tile_shapes": tiles,
         },
         "mixed": {"ms": 1e3 * times["mixed"][0], "corrections": it, "rel_residual": rr,
                   "vcycles_per_correction": int(best[5:])},
@@ -392,6 +399,10 @@ def main():
         dt.close()
         return 0
 
+    # the autotuner picks the tile shape of the colour pass per level (on a
+    # copy of phi, before timing)
+    dt.autotune(mg)
+    tiles = [dt.tile(lvl) for lvl in range(dt.min_lvl, dt.highest_lvl + 1)]
     # cell-updates of one solve: FMG runs a V-cycle up to every level
     cu_solve = sum(dt.vcycle_cell_updates(mg, lvl) for lvl in range(dt.min_lvl, dt.highest_lvl + 1))
     cu_solve += p.n_vcycle * dt.vcycle_cell_updates(mg)
@@ -442,6 +453,7 @@ def main():
             "cells_all_levels": int((st.ncell().astype(np.int64) ** 3).sum()),
             "levels": [int(st.min_lvl), int(st.highest_lvl)],
             "cell_updates_per_step": cu_solve,
+            "tile_shapes": tiles,
         },
         "solves_per_s": a.steps / elapsed,
         "rel_residual_after_step": rel_res,

@@ -201,6 +201,17 @@ int afg_mixed_solve(afg_ctx *ctx, const afg_mg_opts *opts, int have_guess, doubl
  * `wave_cells` cells each (default 512) run in a single wavefront.  Results
  * are identical either way. */
 int afg_set_bottom_cells(afg_ctx *ctx, int64_t cells, int64_t wave_cells);
+/* Shape of the tiled colour pass on level lvl, for the FP64 (fp32 = 0) or
+ * FP32 stages: 0 (default), 1 or 2 for nc 8 and 16 (workgroup size and boxes
+ * per workgroup), 0 otherwise.  Results are identical for every shape. */
+int afg_set_tile(afg_ctx *ctx, int lvl, int fp32, int shape);
+int afg_get_tile(const afg_ctx *ctx, int lvl, int fp32, int *shape);
+/* Autotuner: on every level with several tile shapes, time `reps` colour
+ * passes of each shape (on a copy of phi; the tree's data is not changed)
+ * and keep the fastest.  If ms is given it receives, per level from the
+ * lowest and per shape, the milliseconds per pass (-1: not timed; 3 values
+ * per level).  No counterpart in the reference. */
+int afg_autotune(afg_ctx *ctx, const afg_mg_opts *opts, int fp32, int reps, double *ms);
 /* Replay whole V-cycles from HIP graphs (default on). */
 int afg_set_graphs(afg_ctx *ctx, int enabled);
 /* Per-stage device timing: while enabled, each launch of the GSRB colour-pass

@@ -96,10 +96,10 @@ void launch_diff(int dim, hipStream_t s, const DevTree &T, const ListArgs &P, co
 
 // the tiled pass of the device: workgroups one after the other, each stage
 // over all threads of the workgroup before the next
-template <int DIM, int NC, typename R>
-void gsrb_tiles(const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int rb, int fly)
+template <int DIM, int NC, typename R, int SH>
+void gsrb_tiles_v(const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int rb, int fly)
 {
-    using G = GsrbTile<DIM, NC>;
+    using G = GsrbTile<DIM, NC, SH>;
     const LevelViewT<R> v = pool_view(T, V);
     const int grid = (L.n + G::BOXES - 1) / G::BOXES;
     R *lds = (R *)emul_alloc(sizeof(R) * G::BOXES * G::LEN);
@@ -108,14 +108,24 @@ void gsrb_tiles(const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int r
         const int box0 = xcd_tile(blk, grid) * G::BOXES;
         for (int t = 0; t < G::THREADS; t++) {
             if (fly)
-                tile_stage_a<DIM, NC, 1>(T, L, v, lds, box0, rb, t, &rhs[(size_t)t * G::PER]);
+                tile_stage_a<DIM, NC, 1, SH>(T, L, v, lds, box0, rb, t, &rhs[(size_t)t * G::PER]);
             else
-                tile_stage_a<DIM, NC, 0>(T, L, v, lds, box0, rb, t, &rhs[(size_t)t * G::PER]);
+                tile_stage_a<DIM, NC, 0, SH>(T, L, v, lds, box0, rb, t, &rhs[(size_t)t * G::PER]);
         }
         for (int t = 0; t < G::THREADS; t++)
-            tile_stage_b<DIM, NC>(T, L, v, lds, box0, V.op_auto, rb, t, &rhs[(size_t)t * G::PER]);
+            tile_stage_b<DIM, NC, SH>(T, L, v, lds, box0, V.op_auto, rb, t, &rhs[(size_t)t * G::PER]);
     }
     emul_free(lds);
+}
+
+template <int DIM, int NC, typename R>
+void gsrb_tiles(const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int rb, int fly)
+{
+    if constexpr (tile_shapes(NC) > 1) {
+        if (L.tile == 1) return gsrb_tiles_v<DIM, NC, R, 1>(T, L, V, rb, fly);
+        if (L.tile == 2) return gsrb_tiles_v<DIM, NC, R, 2>(T, L, V, rb, fly);
+    }
+    gsrb_tiles_v<DIM, NC, R, 0>(T, L, V, rb, fly);
 }
 
 template <typename R>

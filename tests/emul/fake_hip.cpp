@@ -147,6 +147,7 @@ hipError_t hipEventRecordWithFlags(hipEvent_t e, hipStream_t s, unsigned)
     emul_submit(s, [e] { e->stamp = ++g_clock; });
     return hipSuccess;
 }
+hipError_t hipEventRecord(hipEvent_t e, hipStream_t s) { return hipEventRecordWithFlags(e, s, 0); }
 hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b)
 {

@@ -46,6 +46,7 @@ hipError_t hipMemsetAsync(void *p, int v, size_t bytes, hipStream_t s);
 hipError_t hipEventCreate(hipEvent_t *e);
 hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipEventRecordWithFlags(hipEvent_t e, hipStream_t s, unsigned flags);
+hipError_t hipEventRecord(hipEvent_t e, hipStream_t s);
 hipError_t hipEventSynchronize(hipEvent_t e);
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b);
 hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode m);

@@ -35,6 +35,8 @@ from test_gpu_parity import (  # noqa: E402,F401
     test_periodic_base,
     test_solve_bitexact,
     test_stage_bitexact,
+    test_tile_shapes_are_exact,
+    test_autotune_keeps_state,
     test_timing_mode_is_exact,
 )
 from test_forest_gpu import (  # noqa: E402,F401

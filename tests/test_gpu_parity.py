@@ -285,3 +285,68 @@ def test_bottom_kernel_limit_is_exact(kind, cells, wave_cells):
     dt, ot, rd, ro = solve_both(p.state, n_var, mg, om, 1, 2, dt=dt)
     compare_all(dt, ot, n_var, f"{kind} bottom {cells}/{wave_cells}")
     assert np.array_equal(rd, ro)
+
+
+@pytest.mark.parametrize("kind", ["2d_nc8", "2d_nc16", "3d_nc8", "3d_nc16_eps"])
+@pytest.mark.parametrize("shape", [1, 2])
+def test_tile_shapes_are_exact(kind, shape):
+    """Every tile shape of the colour pass (workgroup size, boxes per
+    workgroup) on every level gives the oracle's bits, in FP64; FP32 stages
+    give the same bits with every shape as with the default one."""
+    p = problem(kind)
+    eps = p.eps_hi is not None
+    mg = M.default_mg(eps=eps, n_base=p.n_base)
+    om = O.default_opts(2, 2, p.n_base, eps=eps)
+    n_var = len(p.state.data)
+    dt = M.DeviceTree.from_state(p.state)
+    dt.set_bottom_cells(0)            # every level through the tiled pass
+    for lvl in range(dt.min_lvl, dt.highest_lvl + 1):
+        if T.level_ncell(p.state.nc, lvl) in (8, 16):
+            dt.set_tile(lvl, shape)
+            assert dt.tile(lvl) == shape
+    dt, ot, rd, ro = solve_both(p.state, n_var, mg, om, 1, 2, dt=dt)
+    compare_all(dt, ot, n_var, f"{kind} shape {shape}")
+    assert np.array_equal(rd, ro)
+    # FP32 stages: shape 0 against this shape
+    out = []
+    for sh in (0, shape):
+        d = M.DeviceTree.from_state(p.state, n_var + 1)
+        d.set_bottom_cells(0)
+        for lvl in range(d.min_lvl, d.highest_lvl + 1):
+            if T.level_ncell(p.state.nc, lvl) in (8, 16):
+                d.set_tile(lvl, sh, fp32=True)
+        for iv in range(1, n_var + 1):
+            d.fp32_from_cc(iv, iv)
+        M.mg_fas_fmg_fp32(d, mg, False, False)
+        M.mg_fas_vcycle_fp32(d, mg, d.highest_lvl, False)
+        d.fp32_to_cc(1, n_var + 1)
+        out.append(d.download(n_var + 1))
+        d.close()
+    assert_bitexact(out[1], out[0], f"{kind} FP32 shape {shape}")
+    dt.close()
+
+
+@pytest.mark.parametrize("kind", ["2d_nc8", "3d_nc16_eps"])
+def test_autotune_keeps_state(kind):
+    """The autotuner times every shape on a copy of phi: the tree's data is
+    unchanged, each tuned level gets a valid shape and solves stay exact."""
+    p = problem(kind)
+    eps = p.eps_hi is not None
+    mg = M.default_mg(eps=eps, n_base=p.n_base)
+    om = O.default_opts(2, 2, p.n_base, eps=eps)
+    n_var = len(p.state.data)
+    dt = M.DeviceTree.from_state(p.state)
+    before = [dt.download(iv) for iv in range(1, n_var + 1)]
+    for fp32 in (False, True):
+        ms = dt.autotune(mg, fp32=fp32, reps=2)
+        for lvl in range(dt.min_lvl, dt.highest_lvl + 1):
+            row = ms[lvl - dt.min_lvl]
+            if T.level_ncell(p.state.nc, lvl) in (8, 16):
+                assert np.all(row >= 0) and dt.tile(lvl, fp32) == int(np.argmin(row))
+            else:
+                assert np.all(row == -1) and dt.tile(lvl, fp32) == 0
+    for iv in range(1, n_var + 1):
+        assert_bitexact(dt.download(iv), before[iv - 1], f"{kind} var {iv} after autotune")
+    dt, ot, rd, ro = solve_both(p.state, n_var, mg, om, 1, 2, dt=dt)
+    compare_all(dt, ot, n_var, f"{kind} tuned")
+    assert np.array_equal(rd, ro)
