@@ -507,6 +507,155 @@ AFG_DEV void uc_child_body(const DevTree &T, const LevelArgs &L, const LevelView
     par.phi[pb + pq] = R(w) * sp;
 }
 
+// update_coarse, child part, by tiles in LDS (NC 8 and 16): the same values
+// as uc_child_body.  A workgroup of THREADS threads takes BOXES boxes:
+//   A  the interiors and every face ghost into LDS (a ghost computed with the
+//      ghost fill's expression from the final interior when FLY, else read);
+//   B  each thread computes the residual of its PER fine cells from LDS
+//      (written to the child's tmp unless keep_child_tmp), and one thread per
+//      coarse cell restricts phi into the parent;
+//   D  the residuals replace phi in LDS;
+//   E  one thread per coarse cell restricts them into the parent's tmp.
+// Sums run over the 2^DIM fine cells in the order of uc_child_body.
+template <int DIM, int NC>
+struct UcTile {
+    static constexpr int S = NC + 2;
+    static constexpr int LEN = ipow(S, DIM);
+    static constexpr int CELLS = ipow(NC, DIM);
+    static constexpr int HN = NC / 2;
+    static constexpr int COARSE = ipow(HN, DIM);
+    static constexpr int THREADS = CELLS >= 4096 ? 1024 : 256;
+    static constexpr int BOXES = CELLS >= THREADS ? 1 : THREADS / CELLS;
+    static constexpr int PER = BOXES * CELLS / THREADS;     // fine cells per thread
+    static constexpr int FACE = DIM == 2 ? NC : NC * NC;
+    static constexpr int GHOSTS = 2 * DIM * FACE;
+    static_assert(BOXES * CELLS % THREADS == 0 && BOXES * COARSE <= THREADS, "tile shape");
+};
+
+template <int DIM, int NC, int FLY, typename R>
+AFG_DEV void uc_tile_a(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &v, R *lds, int box0,
+                       int tid)
+{
+    using G = UcTile<DIM, NC>;
+    constexpr int S = G::S;
+    for (int e = tid; e < G::BOXES * G::CELLS; e += G::THREADS) {
+        const int b = e / G::CELLS;
+        if (box0 + b >= L.n) break;
+        const int c = e - b * G::CELLS;
+        const int i = c % NC + 1, j = (c / NC) % NC + 1, k = DIM == 3 ? c / (NC * NC) + 1 : 0;
+        const int q = i + S * (j + S * k);
+        lds[b * G::LEN + q] = v.phi[v.off(L.slot0 + box0 + b) + q];
+    }
+    for (int e = tid; e < G::BOXES * G::GHOSTS; e += G::THREADS) {
+        const int b = e / G::GHOSTS;
+        if (box0 + b >= L.n) break;
+        const int r = e - b * G::GHOSTS;
+        const int f = r / G::FACE, t = r - f * G::FACE;
+        const int dn = f >> 1, high = f & 1;
+        const int m = high ? NC : 1;
+        const int t1 = DIM == 2 ? t + 1 : t % NC + 1;
+        const int t2 = DIM == 2 ? 0 : t / NC + 1;
+        const int i = dn == 0 ? m : t1;
+        const int j = dn == 1 ? m : (dn == 0 ? t1 : t2);
+        const int k = DIM == 3 ? (dn == 2 ? m : t2) : 0;
+        const int gi = dn == 0 ? (high ? NC + 1 : 0) : i;
+        const int gj = dn == 1 ? (high ? NC + 1 : 0) : j;
+        const int gk = dn == 2 ? (high ? NC + 1 : 0) : k;
+        const int gq = gi + S * (gj + S * gk);
+        const int slot = L.slot0 + box0 + b;
+        const size_t base = v.off(slot);
+        R g;
+        if (FLY)
+            g = ghost_value<DIM>(T, v, v, L.pnc, L.dr, slot, f, NC, i, j, k, v.phi[base + i + S * (j + S * k)]);
+        else
+            g = v.phi[base + gq];
+        lds[b * G::LEN + gq] = g;
+    }
+}
+
+// fine cell of the tile -> (box, LDS index)
+template <int DIM, int NC>
+AFG_DEV int uc_tile_cell(int e, int &b)
+{
+    using G = UcTile<DIM, NC>;
+    b = e / G::CELLS;
+    const int c = e - b * G::CELLS;
+    const int i = c % NC + 1, j = (c / NC) % NC + 1, k = DIM == 3 ? c / (NC * NC) + 1 : 0;
+    return i + G::S * (j + G::S * k);
+}
+
+// restriction of LDS values of coarse cell t of the tile into `to` of the parent
+template <int DIM, int NC, typename R>
+AFG_DEV void uc_tile_restrict(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &v, const R *lds,
+                              int box0, int t, R *to)
+{
+    using G = UcTile<DIM, NC>;
+    constexpr int S = G::S, HN = G::HN;
+    if (t >= G::BOXES * G::COARSE) return;
+    const int b = t / G::COARSE;
+    if (box0 + b >= L.n) return;
+    const int c = t - b * G::COARSE;
+    const int ii = c % HN + 1, jj = (c / HN) % HN + 1, kk = DIM == 3 ? c / (HN * HN) + 1 : 0;
+    const R *x = lds + b * G::LEN;
+    R sum = 0;
+#pragma unroll
+    for (int m = 0; m < (1 << DIM); m++) {
+        const int fi = 2 * ii - 1 + (m & 1), fj = 2 * jj - 1 + ((m >> 1) & 1);
+        const int fk = DIM == 3 ? 2 * kk - 1 + ((m >> 2) & 1) : 0;
+        const R val = x[fi + S * (fj + S * fk)];
+        sum = m == 0 ? val : sum + val;
+    }
+    const int slot = L.slot0 + box0 + b;
+    const int p = T.parent[slot];
+    const int co = T.choff[slot];
+    const int ps = L.pnc + 2;
+    const int pq = ((co & 1023) + ii) +
+                   ps * ((((co >> 10) & 1023) + jj) + ps * (DIM == 3 ? ((co >> 20) & 1023) + kk : 0));
+    constexpr double w = DIM == 2 ? 0.25 : 0.125;
+    to[v.off(p) + pq] = R(w) * sum;
+}
+
+template <int DIM, int NC, typename R>
+AFG_DEV void uc_tile_b(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &v, const R *lds, int box0,
+                       int op_auto, int keep_child_tmp, int tid, R *res)
+{
+    using G = UcTile<DIM, NC>;
+    constexpr int S = G::S;
+#pragma unroll
+    for (int r = 0; r < G::PER; r++) {
+        int b;
+        const int q = uc_tile_cell<DIM, NC>(tid + r * G::THREADS, b);
+        if (box0 + b >= L.n) break;
+        const int slot = L.slot0 + box0 + b;
+        const size_t base = v.off(slot);
+        const R *x = lds + b * G::LEN + q;
+        R u[6];
+        u[0] = x[-1];
+        u[1] = x[1];
+        u[2] = x[-S];
+        u[3] = x[S];
+        u[4] = DIM == 3 ? x[-S * S] : R(0);
+        u[5] = DIM == 3 ? x[S * S] : R(0);
+        const bool lpld = lpld_box(T, op_auto, slot);
+        res[r] = v.rhs[base + q] - box_op<DIM>(u, x[0], lpld ? v.eps + base : nullptr, q, S, L.inv_dr_sq, lpld);
+        if (!keep_child_tmp) v.tmp[base + q] = res[r];
+    }
+    uc_tile_restrict<DIM, NC>(T, L, v, lds, box0, tid, v.phi);
+}
+
+template <int DIM, int NC, typename R>
+AFG_DEV void uc_tile_d(const LevelArgs &L, R *lds, int box0, int tid, const R *res)
+{
+    using G = UcTile<DIM, NC>;
+#pragma unroll
+    for (int r = 0; r < G::PER; r++) {
+        int b;
+        const int q = uc_tile_cell<DIM, NC>(tid + r * G::THREADS, b);
+        if (box0 + b >= L.n) break;
+        lds[b * G::LEN + q] = res[r];
+    }
+}
+
 // update_coarse, parent part (m_aX_mg.f90:467-479), fused with the ghost fill
 // of the parent level that precedes it: one thread per cell of the whole
 // (nc+2)^DIM array of every box of the level.  Face ghost cells get the

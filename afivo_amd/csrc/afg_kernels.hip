@@ -104,6 +104,25 @@ __global__ __launch_bounds__(BLOCK) void k_uc_child(DevTree T, LevelArgs L, Leve
         uc_child_body<DIM, NC>(T, L, v, v, op_auto, keep_child_tmp, fly, gid);
 }
 
+// update_coarse, child part, by tiles in LDS (NC 8 and 16)
+template <int DIM, int NC, int FLY, typename R>
+__global__ __launch_bounds__((UcTile<DIM, NC>::THREADS)) void k_uc_child_tile(DevTree T, LevelArgs L,
+                                                                             LevelViewT<R> v, int op_auto,
+                                                                             int keep_child_tmp)
+{
+    using G = UcTile<DIM, NC>;
+    __shared__ R lds[G::BOXES * G::LEN];
+    R res[G::PER];
+    const int box0 = xcd_tile(blockIdx.x, gridDim.x) * G::BOXES;
+    uc_tile_a<DIM, NC, FLY>(T, L, v, lds, box0, threadIdx.x);
+    __syncthreads();
+    uc_tile_b<DIM, NC>(T, L, v, lds, box0, op_auto, keep_child_tmp, threadIdx.x, res);
+    __syncthreads();
+    uc_tile_d<DIM, NC>(L, lds, box0, threadIdx.x, res);
+    __syncthreads();
+    uc_tile_restrict<DIM, NC>(T, L, v, lds, box0, threadIdx.x, v.tmp);
+}
+
 template <int DIM, int NC, typename R>
 __global__ __launch_bounds__(BLOCK) void k_uc_parent(DevTree T, LevelArgs L, LevelViewT<R> v,
                                                      int op_auto, int copy_tmp)
@@ -438,6 +457,24 @@ void launch_uc_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &
                      const MgVarsT<R> &V, int keep_child_tmp, int fly)
 {
     if (L.n <= 0) return;
+    if (L.nc == 8 || L.nc == 16) {
+        auto go = [&](auto dv, auto nv) {
+            constexpr int D = decltype(dv)::value, N = decltype(nv)::value;
+            using G = UcTile<D, N>;
+            const int grid = (L.n + G::BOXES - 1) / G::BOXES;
+            if (fly)
+                k_uc_child_tile<D, N, 1, R><<<grid, G::THREADS, 0, s>>>(T, L, pool_view(T, V), V.op_auto,
+                                                                        keep_child_tmp);
+            else
+                k_uc_child_tile<D, N, 0, R><<<grid, G::THREADS, 0, s>>>(T, L, pool_view(T, V), V.op_auto,
+                                                                        keep_child_tmp);
+        };
+        if (dim == 2)
+            L.nc == 8 ? go(IC<2>(), IC<8>()) : go(IC<2>(), IC<16>());
+        else
+            L.nc == 8 ? go(IC<3>(), IC<8>()) : go(IC<3>(), IC<16>());
+        return;
+    }
     const long long th = (long long)L.n * ipow(L.nc / 2, dim);
     AFG_DISPATCH(dim, L.nc, k_uc_child, n_blocks(th), T, L, pool_view(T, V), V.op_auto,
                  keep_child_tmp, fly);

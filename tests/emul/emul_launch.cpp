@@ -128,6 +128,32 @@ void gsrb_tiles(const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int r
     gsrb_tiles_v<DIM, NC, R, 0>(T, L, V, rb, fly);
 }
 
+// the tiled child stage of the device: workgroups one after the other, each
+// phase over all threads of the workgroup before the next
+template <int DIM, int NC, typename R>
+void uc_tiles(const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int keep_child_tmp, int fly)
+{
+    using G = UcTile<DIM, NC>;
+    const LevelViewT<R> v = pool_view(T, V);
+    const int grid = (L.n + G::BOXES - 1) / G::BOXES;
+    R *lds = (R *)emul_alloc(sizeof(R) * G::BOXES * G::LEN);
+    std::vector<R> res((size_t)G::THREADS * G::PER);
+    for (int blk = 0; blk < grid; blk++) {
+        const int box0 = xcd_tile(blk, grid) * G::BOXES;
+        for (int t = 0; t < G::THREADS; t++) {
+            if (fly)
+                uc_tile_a<DIM, NC, 1>(T, L, v, lds, box0, t);
+            else
+                uc_tile_a<DIM, NC, 0>(T, L, v, lds, box0, t);
+        }
+        for (int t = 0; t < G::THREADS; t++)
+            uc_tile_b<DIM, NC>(T, L, v, lds, box0, V.op_auto, keep_child_tmp, t, &res[(size_t)t * G::PER]);
+        for (int t = 0; t < G::THREADS; t++) uc_tile_d<DIM, NC>(L, lds, box0, t, &res[(size_t)t * G::PER]);
+        for (int t = 0; t < G::THREADS; t++) uc_tile_restrict<DIM, NC>(T, L, v, lds, box0, t, v.tmp);
+    }
+    emul_free(lds);
+}
+
 template <typename R>
 void launch_gsrb_pass(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
                       const MgVarsT<R> &V, int rb, int fly)
@@ -167,6 +193,15 @@ void launch_uc_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &
                      const MgVarsT<R> &V, int keep_child_tmp, int fly)
 {
     if (L.n <= 0) return;
+    if (L.nc == 8 || L.nc == 16) {
+        emul_submit(s, [=] {
+            if (dim == 2)
+                L.nc == 8 ? uc_tiles<2, 8, R>(T, L, V, keep_child_tmp, fly) : uc_tiles<2, 16, R>(T, L, V, keep_child_tmp, fly);
+            else
+                L.nc == 8 ? uc_tiles<3, 8, R>(T, L, V, keep_child_tmp, fly) : uc_tiles<3, 16, R>(T, L, V, keep_child_tmp, fly);
+        });
+        return;
+    }
     emul_submit(s, [=] {
         const LevelViewT<R> v = pool_view(T, V);
         run_stage(dim, L.nc, [&](auto d) { return (long long)L.n * ipow(L.nc / 2, DIMV); },
