@@ -973,6 +973,89 @@ AFG_DEV void bottom_nc_dispatch(int nc, F &&f)
     }
 }
 
+// The base relaxation when every box of the lowest level has 2^DIM cells and
+// physical boundaries on all faces: the 2 n_base colour passes of
+// bottom_run's BOP_BASE with the same expressions, one cell per lane of a
+// wavefront (64 / 2^DIM boxes at a time), the cell values in registers and
+// the neighbour cell of the same box read from its lane (xshfl(v, mask): the
+// value of v in lane ^ mask).  The first pass reads the stored ghosts, the
+// others K + c1 phi at the boundary.  The eps weights of mg$D_box_gsrb_lpld
+// are formed once, with its expressions.  Cell c of a box has coordinates
+// i = (c & 1) + 1, j = ((c >> 1) & 1) + 1, k = (c >> 2) + 1; its
+// descriptors are in the cell list of its colour at the rank it has among
+// the box's cells of that colour.
+template <int DIM, int AUTO, typename R, typename Xshfl>
+AFG_DEV void base_wave_run(const DevTree &T, const LevelViewT<R> &own, const BottomLevels &B, const BottomArgs &A,
+                           double dx2, int n_box, int lane, Xshfl xshfl)
+{
+    constexpr int NCL = 1 << DIM, H = NCL / 2, S = 4, PER_WAVE = 64 / NCL;
+    const int dq[6] = {-1, 1, -S, S, -S * S, S * S};
+    auto odd_of = [](int c) { return ((c & 1) + ((c >> 1) & 1) + ((c >> 2) & 1) + DIM) & 1; };
+    const int c = lane % NCL;
+    const int odd = odd_of(c);
+    int rank = 0;
+    for (int c2 = 0; c2 < c; c2++) rank += odd_of(c2) == odd;
+    for (int g0 = 0; g0 < n_box; g0 += PER_WAVE) {
+        const int b = g0 + lane / NCL;
+        const bool on = b < n_box;
+        R ph = 0, rh = 0, K[3] = {0, 0, 0}, cm[6] = {0, 0, 0, 0, 0, 0}, sc = 1;
+        int pos = 0, neg = 0;
+        bool lpld = false;
+        if (on) {
+            const BaseCell &d = B.base[(odd ? 0 : B.base_n[0]) + b * H + rank];
+            pos = d.pos;
+            ph = own.phi[pos];
+            rh = own.rhs[pos];
+#pragma unroll
+            for (int dn = 0; dn < DIM; dn++) {
+                const int m = 2 * dn + ((c >> dn) & 1);   // the face of cell c on the box boundary
+                K[dn] = R(d.K[m]);
+                neg |= (d.fly[m] == -1 ? 1 : 0) << dn;
+            }
+            lpld = lpld_box(T, AUTO, d.slot);
+            if (lpld) {
+                const R *e = own.eps + pos;
+                const R a0 = e[0];
+#pragma unroll
+                for (int m = 0; m < 2 * DIM; m++) {
+                    const R a = e[dq[m]];
+                    cm[m] = 2 * a0 * a / (a0 + a);
+                    sc = m == 0 ? cm[m] : sc + cm[m];
+                }
+            }
+        }
+        for (int rb = 1; rb <= 2 * A.n_base; rb++) {
+            R u[6];
+#pragma unroll
+            for (int dn = 0; dn < DIM; dn++) {
+                const R other = xshfl(ph, 1 << dn);   // every lane takes part
+                const R ghost = K[dn] + (((neg >> dn) & 1) ? -ph : ph);
+                const int bit = (c >> dn) & 1;
+#pragma unroll
+                for (int high = 0; high < 2; high++) {
+                    const int m = 2 * dn + high;
+                    if (rb == 1)
+                        u[m] = on ? own.phi[pos + dq[m]] : R(0);
+                    else
+                        u[m] = bit != high ? other : ghost;
+                }
+            }
+            if (DIM == 2) u[4] = u[5] = 0;
+            R nv;
+            if (lpld) {
+                R su = 0;
+#pragma unroll
+                for (int m = 0; m < 2 * DIM; m++) su = m == 0 ? cm[m] * u[m] : su + cm[m] * u[m];
+                nv = (su - R(dx2) * rh) / sc;
+            } else {
+                nv = gsrb_value<DIM>(u, rh, (const R *)nullptr, 0, S, dx2, false);
+            }
+            if (odd == (rb & 1)) ph = nv;
+        }
+        if (on) own.phi[pos] = ph;
+    }
+}
+
 // The whole bottom part for thread tid of nt; `lds` is the workgroup's
 // dynamic LDS and `sync()` the workgroup barrier; AUTO is the operator
 // choice of the options (mg$D_auto_op with eps, or mg$D_box_lpl).  Every thread walks the
@@ -980,10 +1063,10 @@ AFG_DEV void bottom_nc_dispatch(int nc, F &&f)
 // [0, wave) alone, one wavefront, and are ordered by `wsync()` among
 // themselves; the other threads pass them by and wait at the next workgroup
 // barrier.
-template <int DIM, int AUTO, typename R, typename Sync, typename WaveSync>
+template <int DIM, int AUTO, typename R, typename Sync, typename WaveSync, typename Xshfl>
 AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLevels &B,
                         const BottomArgs &A, void *lds, int tid, int nt, int wave, Sync sync,
-                        WaveSync wsync)
+                        WaveSync wsync, Xshfl xshfl)
 {
     constexpr int NNB = 2 * DIM;
     // topology tables of the bottom slots, in LDS
@@ -1034,7 +1117,6 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLeve
         }
     };
     for (int li = 0; li <= A.hl; li++) {
-        if (!B.in_lds[li]) continue;
         copy_in(V.phi, B.lds_off[0][li], li);
         copy_in(V.rhs, B.lds_off[1][li], li);
         copy_in(V.tmp, B.lds_off[2][li], li);
@@ -1042,18 +1124,16 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLeve
     }
     sync();
 
+    // every level of the kernel is in LDS (the host takes only levels that
+    // fit), so the views hold LDS addresses only
     auto view = [&](int li) {
         LevelViewT<R> w;
-        if (B.in_lds[li]) {
-            w.phi = data + B.lds_off[0][li];
-            w.rhs = data + B.lds_off[1][li];
-            w.tmp = data + B.lds_off[2][li];
-            w.eps = AUTO ? data + B.lds_off[3][li] : nullptr;
-            w.slot0 = B.lv[li].slot0;
-            w.stride = B.box_len[li];
-        } else {
-            w = LevelViewT<R>{V.phi, V.rhs, V.tmp, V.eps, 0, T.stride};
-        }
+        w.phi = data + B.lds_off[0][li];
+        w.rhs = data + B.lds_off[1][li];
+        w.tmp = data + B.lds_off[2][li];
+        w.eps = AUTO ? data + B.lds_off[3][li] : nullptr;
+        w.slot0 = B.lv[li].slot0;
+        w.stride = B.box_len[li];
         return w;
     };
     // The steps of one cell count are compiled together (dispatch on the
@@ -1074,6 +1154,12 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLeve
                 // fill after the previous pass would store (cells of the other
                 // colour, or K + c1 phi at physical boundaries)
                 const LevelArgs L = B.lv[0];
+                if constexpr (NC == 2) {
+                    if (B.base_fast && wave == 64) {   // a full wavefront: the lanes of the first
+                        if (t < 64) base_wave_run<DIM, AUTO>(T, own, B, A, L.dx2, L.n, t, xshfl);
+                        break;
+                    }
+                }
                 const int s = (NC ? NC : L.nc) + 2;
                 const int dq[6] = {-1, 1, -s, s, -s * s, s * s};
                 auto relax = [&](const BaseCell &d, int rb) {
@@ -1170,7 +1256,6 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLeve
     }
 
     for (int li = 0; li <= A.hl; li++) {
-        if (!B.in_lds[li]) continue;
         copy_out(V.phi, B.lds_off[0][li], li);
         copy_out(V.rhs, B.lds_off[1][li], li);
         copy_out(V.tmp, B.lds_off[2][li], li);

@@ -155,7 +155,7 @@ struct afg_ctx {
     int bottom_threads = BOTTOM_BLOCK;   // its workgroup size
     int64_t lds_limit = 65536;           // LDS bytes one workgroup may use
     int64_t bottom_cells = 8192;        // cell limit per level for the bottom kernel
-    int64_t bottom_wave_cells = 512;    // ... and for its one-wavefront levels
+    int64_t bottom_wave_cells = 0;      // ... and for its one-wavefront levels
     bool use_graphs = true;
     std::map<std::tuple<int, int, int, int, int, int, int, int, int, int, int>, GraphRec> graphs;
     bool capturing = false;
@@ -621,13 +621,14 @@ BottomArgs bottom_args(MgT<R> &m, int top)
     return B;
 }
 
-// highest level run by the bottom kernel for these options
-template <typename R>
-int bottom_top(const MgT<R> &m) { return m.c->bottom_lvl[m.V.op_auto ? 1 : 0]; }
-
 // index of the bottom kernel table for these values and options
 template <typename R>
 int bottom_table(const MgT<R> &m) { return (sizeof(R) == sizeof(float) ? 2 : 0) + (m.V.op_auto ? 1 : 0); }
+
+// highest level run by the bottom kernel for these options
+template <typename R>
+int bottom_top(const MgT<R> &m) { return m.c->bottom_lvl[bottom_table(m)]; }
+
 
 // mg$D_fas_vcycle (m_aX_mg.f90:208-255).  Levels up to bottom_top run in the
 // one-workgroup LDS bottom kernel; the levels above, stage by stage.
@@ -755,78 +756,80 @@ int check_level(afg_ctx *c, int lvl, bool need_parent)
 }
 
 // The bottom kernel takes the lowest levels while each holds at most
-// bottom_cells cells.  Their topology tables always go to its LDS; the
-// lowest of them whose phi, rhs, tmp (and eps, for the auto operators) also
-// fit there are copied in as well, the others are used in place in the HBM
-// pools.  Levels of at most bottom_wave_cells cells each, from the lowest
-// up, run in one wavefront.  Four tables are kept: FP64 or FP32 values, 3 or
-// 4 variables in LDS.
+// bottom_cells cells and all of them, with their topology tables, fit in the
+// LDS of one workgroup: phi, rhs, tmp (and eps, for the auto operators) of
+// those levels are copied in, so every access of the kernel is an LDS access.
+// Levels of at most bottom_wave_cells cells each, from the lowest up, run in
+// one wavefront.  Four tables are kept: FP64 or FP32 values, 3 or 4
+// variables in LDS; each has its own highest level.
 int choose_bottom(afg_ctx *c)
 {
     const int nl = (int)c->levels.size();
     int top = -1;
-    int64_t max_threads = 64;
     for (int li = 0; li < nl && li < MAX_BOTTOM_LEVELS; li++) {
         const Level &L = c->levels[li];
         const int64_t cells = (int64_t)L.n * (c->dim == 2 ? L.nc * L.nc : L.nc * L.nc * L.nc);
         if (cells > c->bottom_cells) break;
         top = li;
-        max_threads = std::max<int64_t>(max_threads, (int64_t)L.n * box_len(c->dim, L.nc) / 2);
     }
-    c->bottom_threads = (int)std::min<int64_t>(BOTTOM_BLOCK, (max_threads + 63) / 64 * 64);
-    int n_slots = 0, n_bc = 0, n_par = 0, wave_top = -1;
-    if (top >= 0) {
-        n_slots = c->levels[top].slot0 + c->levels[top].n;
-        for (const BcRec &r : c->bc) n_bc += r.slot < n_slots;
-    }
-    for (int li = 0; li <= top; li++) {
-        const Level &L = c->levels[li];
-        n_par += list_args(c, L.lvl, 0).n;
-        if ((int64_t)L.n * (c->dim == 2 ? L.nc * L.nc : L.nc * L.nc * L.nc) <= c->bottom_wave_cells &&
-            wave_top == li - 1)
-            wave_top = li;
-    }
-    const int table_bytes =
-        (int)((((int64_t)n_slots * (c->nnb + 2) + 2 * n_bc + n_par) * 4 + 2 * n_slots + 7) / 8 * 8);
+    int64_t max_threads = 64;
     for (int k = 0; k < 4; k++) {
         const int nv = 3 + (k & 1);
         const int64_t elem = k >= 2 ? (int64_t)sizeof(float) : (int64_t)sizeof(double);
+        // the highest level tk <= top whose tables and data (levels 0..tk) fit
+        int tk = -1, n_slots = 0, n_bc = 0, n_par = 0, table_bytes = 0;
+        int64_t per_var = 0;   // values of one variable over the levels
+        for (int li = 0; li <= top; li++) {
+            const Level &L = c->levels[li];
+            const int ns = L.slot0 + L.n;
+            int nb = 0;
+            for (const BcRec &r : c->bc) nb += r.slot < ns;
+            const int np = n_par + list_args(c, L.lvl, 0).n;
+            const int tb = (int)((((int64_t)ns * (c->nnb + 2) + 2 * nb + np) * 4 + 2 * ns + 7) / 8 * 8);
+            const int64_t pv = per_var + (int64_t)L.n * box_len(c->dim, L.nc);
+            if (tb + pv * nv * elem > c->lds_limit) break;
+            tk = li;
+            n_slots = ns;
+            n_bc = nb;
+            n_par = np;
+            table_bytes = tb;
+            per_var = pv;
+        }
         BottomLevels h;
         std::memset(&h, 0, sizeof h);
         h.n_slots = n_slots;
         h.n_bc = n_bc;
         h.n_par = n_par;
         h.table_bytes = table_bytes;
-        h.wave_top = wave_top;
-        int64_t per_var = 0;   // doubles of one variable over the levels in LDS
+        h.wave_top = -1;
         int par_off = 0;
-        bool fits = true;
-        for (int li = 0; li <= top; li++) {
+        for (int li = 0; li <= tk; li++) {
             const Level &L = c->levels[li];
             h.lv[li] = level_args(c, L.lvl);
             h.par[li] = list_args(c, L.lvl, 0);
             h.par_off[li] = par_off;
             par_off += h.par[li].n;
             h.box_len[li] = box_len(c->dim, L.nc);
-            const int64_t add = (int64_t)L.n * h.box_len[li];
-            fits = fits && table_bytes + (per_var + add) * nv * elem <= c->lds_limit;
-            h.in_lds[li] = fits ? 1 : 0;
-            if (fits) per_var += add;
+            h.in_lds[li] = 1;
+            if ((int64_t)L.n * (c->dim == 2 ? L.nc * L.nc : L.nc * L.nc * L.nc) <= c->bottom_wave_cells &&
+                h.wave_top == li - 1)
+                h.wave_top = li;
+            max_threads = std::max<int64_t>(max_threads, (int64_t)L.n * h.box_len[li] / 2);
         }
-        for (int v = 0; v < 4; v++) {   // variable-major layout over the levels in LDS
+        for (int v = 0; v < 4; v++) {   // variable-major layout over the levels
             int64_t off = v * per_var;
-            for (int li = 0; li <= top; li++) {
-                if (!h.in_lds[li]) continue;
+            for (int li = 0; li <= tk; li++) {
                 h.lds_off[v][li] = (int)off;
                 off += (int64_t)c->levels[li].n * h.box_len[li];
             }
         }
-        c->bottom_lvl[k] = top >= 0 ? c->levels[top].lvl : -1000;
-        c->bottom_lds[k] = (size_t)table_bytes + (size_t)(per_var * nv * elem);
+        c->bottom_lvl[k] = tk >= 0 ? c->levels[tk].lvl : -1000;
+        c->bottom_lds[k] = tk >= 0 ? (size_t)table_bytes + (size_t)(per_var * nv * elem) : 0;
         c->h_bottom[k] = h;
     }
+    c->bottom_threads = (int)std::min<int64_t>(BOTTOM_BLOCK, (max_threads + 63) / 64 * 64);
     const size_t lds_max = *std::max_element(c->bottom_lds, c->bottom_lds + 4);
-    if (top >= 0 && bottom_prepare(c->dim, lds_max) != 0)
+    if (lds_max > 0 && bottom_prepare(c->dim, lds_max) != 0)
         return c->fail(AFG_ERR_HIP, "cannot allow %zu bytes of LDS for the bottom kernel", lds_max);
     for (auto &g : c->graphs) destroy_graph(g.second);
     c->graphs.clear();
@@ -909,6 +912,10 @@ int build_base(afg_ctx *c)
         std::vector<BaseCell> all(cells[0]);
         all.insert(all.end(), cells[1].begin(), cells[1].end());
         HIP_H2D(c, c->d_base[k], all.data(), all.size() * sizeof(BaseCell));
+        bool fast = nc == 2;
+        for (int slot = L0.slot0; fast && slot < L0.slot0 + L0.n; slot++)
+            for (int m = 0; m < nnb; m++) fast = fast && c->h_nb[(size_t)slot * nnb + m] < 0;
+        h.base_fast = fast ? 1 : 0;
         h.base = c->d_base[k];
         h.base_n[0] = (int)cells[0].size();
         h.base_n[1] = (int)cells[1].size();

@@ -109,6 +109,9 @@ struct BottomLevels {
     int wave_top;
     const BaseCell *base;   // cells of the lowest level: base_n[0] of colour 1, then colour 2
     int base_n[2];
+    // 1: every box of the lowest level has 2^dim cells and physical
+    // boundaries on all faces (its cells then run in registers, a box per thread)
+    int base_fast;
 };
 struct BottomArgs {
     int hl;                       // highest level of this call (relative index)
