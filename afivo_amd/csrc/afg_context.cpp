@@ -154,7 +154,7 @@ struct afg_ctx {
     size_t bottom_lds[4] = {0, 0, 0, 0};               // its dynamic LDS bytes
     int bottom_threads = BOTTOM_BLOCK;   // its workgroup size
     int64_t lds_limit = 65536;           // LDS bytes one workgroup may use
-    int64_t bottom_cells = 8192;        // cell limit per level for the bottom kernel
+    int64_t bottom_cells[2] = {2048, 2048};   // cell limit per level for the bottom kernel: FP64, FP32
     int64_t bottom_wave_cells = 0;      // ... and for its one-wavefront levels
     bool use_graphs = true;
     std::map<std::tuple<int, int, int, int, int, int, int, int, int, int, int>, GraphRec> graphs;
@@ -756,7 +756,7 @@ int check_level(afg_ctx *c, int lvl, bool need_parent)
 }
 
 // The bottom kernel takes the lowest levels while each holds at most
-// bottom_cells cells and all of them, with their topology tables, fit in the
+// bottom_cells cells (per precision) and all of them, with their topology tables, fit in the
 // LDS of one workgroup: phi, rhs, tmp (and eps, for the auto operators) of
 // those levels are copied in, so every access of the kernel is an LDS access.
 // Levels of at most bottom_wave_cells cells each, from the lowest up, run in
@@ -765,15 +765,15 @@ int check_level(afg_ctx *c, int lvl, bool need_parent)
 int choose_bottom(afg_ctx *c)
 {
     const int nl = (int)c->levels.size();
-    int top = -1;
-    for (int li = 0; li < nl && li < MAX_BOTTOM_LEVELS; li++) {
-        const Level &L = c->levels[li];
-        const int64_t cells = (int64_t)L.n * (c->dim == 2 ? L.nc * L.nc : L.nc * L.nc * L.nc);
-        if (cells > c->bottom_cells) break;
-        top = li;
-    }
     int64_t max_threads = 64;
     for (int k = 0; k < 4; k++) {
+        int top = -1;
+        for (int li = 0; li < nl && li < MAX_BOTTOM_LEVELS; li++) {
+            const Level &L = c->levels[li];
+            const int64_t cells = (int64_t)L.n * (c->dim == 2 ? L.nc * L.nc : L.nc * L.nc * L.nc);
+            if (cells > c->bottom_cells[k >= 2]) break;
+            top = li;
+        }
         const int nv = 3 + (k & 1);
         const int64_t elem = k >= 2 ? (int64_t)sizeof(float) : (int64_t)sizeof(double);
         // the highest level tk <= top whose tables and data (levels 0..tk) fit
@@ -932,7 +932,8 @@ int build_base(afg_ctx *c)
 // The autotuner of the tiled colour pass: on every level with more than one
 // tile shape, each shape runs `reps` passes (colours alternating, face
 // ghosts computed) on a copy of phi, three times; the shape with the least
-// time per pass is kept for the level.  Every shape computes the same
+// time per pass is kept for the level (the current one unless another is
+// more than 3% faster).  Every shape computes the same
 // values, so the choice changes speed only.
 template <typename R>
 static int tune_tiles(afg_ctx *c, const afg_mg_opts *o, int reps, double *ms, const double *phi64)
@@ -960,7 +961,8 @@ static int tune_tiles(afg_ctx *c, const afg_mg_opts *o, int reps, double *ms, co
             const int nv = tile_shapes(Lv.nc);
             LevelArgs L = level_args(c, lvl);
             if (nv < 2 || L.n == 0) continue;
-            double best = 0;
+            const int dflt = Lv.tile[sizeof(R) == 4];
+            std::vector<double> per_v(nv, 0.0);
             for (int v = 0; v < nv; v++) {
                 L.tile = v;
                 for (int r = 0; r < 2; r++) launch_gsrb_pass(c->dim, c->stream, m.T, L, V, 1 + r, 1);
@@ -974,13 +976,15 @@ static int tune_tiles(afg_ctx *c, const afg_mg_opts *o, int reps, double *ms, co
                     HIP_TRY(c, hipEventElapsedTime(&t, e0, e1));
                     if (trial == 0 || t < t_min) t_min = t;
                 }
-                const double per = (double)t_min / reps;
-                if (ms) ms[(lvl - c->min_lvl) * 3 + v] = per;
-                if (v == 0 || per < best) {
-                    best = per;
-                    Lv.tile[sizeof(R) == 4] = v;
-                }
+                per_v[v] = (double)t_min / reps;
+                if (ms) ms[(lvl - c->min_lvl) * 3 + v] = per_v[v];
             }
+            // the fastest shape, unless the current one is within 3% of it
+            // (timing noise would otherwise flip shapes between runs)
+            int best = 0;
+            for (int v = 1; v < nv; v++)
+                if (per_v[v] < per_v[best]) best = v;
+            Lv.tile[sizeof(R) == 4] = per_v[dflt] <= 1.03 * per_v[best] ? dflt : best;
         }
         return 0;
     };
@@ -993,6 +997,61 @@ static int tune_tiles(afg_ctx *c, const afg_mg_opts *o, int reps, double *ms, co
     for (auto &g : c->graphs) destroy_graph(g.second);
     c->graphs.clear();
     return err;
+}
+
+// The autotuner of the bottom kernel's cell limit (per precision): a few
+// V-cycles with each candidate limit, timed with HIP events; phi, rhs and tmp
+// are saved before and restored after, so the tree's data is unchanged.
+template <typename R>
+static int tune_bottom(afg_ctx *c, const afg_mg_opts *o, int reps, double *ms)
+{
+    MgT<R> m;
+    if (int e = make_mg(c, o, &m)) return e;
+    const size_t n = (size_t)c->n_slots * c->stride;
+    R *saved = nullptr;
+    HIP_TRY(c, hipMalloc(&saved, 3 * n * sizeof(R)));
+    R *vars[3] = {m.V.phi, m.V.rhs, m.V.tmp};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const int p = sizeof(R) == 4;
+    const int64_t keep = c->bottom_cells[p];
+    const int64_t cand[4] = {512, 2048, 8192, 32768};
+    double t_best = 0;
+    int64_t best = keep;
+    auto run = [&]() -> int {
+        for (int v = 0; v < 3; v++)
+            HIP_TRY(c, hipMemcpyAsync(saved + v * n, vars[v], n * sizeof(R), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipEventCreate(&e0));
+        HIP_TRY(c, hipEventCreate(&e1));
+        for (int i = 0; i < 4; i++) {
+            c->bottom_cells[p] = cand[i];
+            if (int e = choose_bottom(c)) return e;
+            for (int r = 0; r < 2; r++)
+                if (int e = run_vcycle(m, c->highest_lvl, 0, c->min_lvl, c->highest_lvl + 1)) return e;
+            HIP_TRY(c, hipEventRecord(e0, c->stream));
+            for (int r = 0; r < reps; r++)
+                if (int e = run_vcycle(m, c->highest_lvl, 0, c->min_lvl, c->highest_lvl + 1)) return e;
+            HIP_TRY(c, hipEventRecord(e1, c->stream));
+            HIP_TRY(c, hipEventSynchronize(e1));
+            float t = 0;
+            HIP_TRY(c, hipEventElapsedTime(&t, e0, e1));
+            if (ms) ms[i] = (double)t / reps;
+            if (i == 0 || t < t_best) {
+                t_best = t;
+                best = cand[i];
+            }
+        }
+        for (int v = 0; v < 3; v++)
+            HIP_TRY(c, hipMemcpyAsync(vars[v], saved + v * n, n * sizeof(R), hipMemcpyDeviceToDevice, c->stream));
+        return 0;
+    };
+    const int err = run();
+    c->bottom_cells[p] = err ? keep : best;
+    const int e2 = choose_bottom(c);
+    hipStreamSynchronize(c->stream);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipFree(saved);
+    return err ? err : e2;
 }
 
 extern "C" {
@@ -1862,7 +1921,7 @@ int afg_set_bottom_cells(afg_ctx *c, int64_t cells, int64_t wave_cells)
     if (int e = check_ctx(c)) return e;
     if (cells < 0 || wave_cells < 0) return c->fail(AFG_ERR_ARG, "bottom cell limits must be >= 0");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    c->bottom_cells = cells;
+    c->bottom_cells[0] = c->bottom_cells[1] = cells;
     c->bottom_wave_cells = wave_cells;
     return choose_bottom(c);
 }
@@ -1895,11 +1954,21 @@ int afg_autotune(afg_ctx *c, const afg_mg_opts *o, int fp32, int reps, double *m
     if (c->capturing) return c->fail(AFG_ERR_STATE, "autotune inside a graph capture");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (ms)
-        for (int i = 0; i < 3 * (int)c->levels.size(); i++) ms[i] = -1;
+        for (int i = 0; i < 3 * (int)c->levels.size() + 4; i++) ms[i] = -1;
     Mg m;
     if (int e = make_mg(c, o, &m)) return e;
     set_tags(m);
-    return fp32 ? tune_tiles<float>(c, o, reps, ms, m.V.phi) : tune_tiles<double>(c, o, reps, ms, m.V.phi);
+    if (int e = fp32 ? tune_tiles<float>(c, o, reps, ms, m.V.phi) : tune_tiles<double>(c, o, reps, ms, m.V.phi))
+        return e;
+    double *bms = ms ? ms + 3 * (int)c->levels.size() : nullptr;
+    return fp32 ? tune_bottom<float>(c, o, reps, bms) : tune_bottom<double>(c, o, reps, bms);
+}
+
+int afg_get_bottom_cells(const afg_ctx *c, int fp32, int64_t *cells)
+{
+    if (!c || !cells) return AFG_ERR_ARG;
+    *cells = c->bottom_cells[fp32 != 0];
+    return 0;
 }
 
 int afg_set_graphs(afg_ctx *c, int enabled)

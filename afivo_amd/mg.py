@@ -92,12 +92,19 @@ class DeviceTree:
         return v.value
 
     def autotune(self, mg, fp32=False, reps=8):
-        """Pick the fastest tile shape per level; returns ms per pass as an
-        array [level - min_lvl, shape] (-1 where not timed)."""
+        """Pick the fastest tile shape per level and the fastest bottom cell
+        limit; returns (ms per pass as an array [level - min_lvl, shape],
+        ms per V-cycle for the limits 512, 2048, 8192, 32768); -1 where not
+        timed."""
         n = self.highest_lvl - self.min_lvl + 1
-        ms = np.zeros(3 * n)
+        ms = np.zeros(3 * n + 4)
         self._c(_lib.load().afg_autotune(self.h, ctypes.byref(mg), int(fp32), int(reps), _f64(ms)))
-        return ms.reshape(n, 3)
+        return ms[:3 * n].reshape(n, 3), ms[3 * n:]
+
+    def bottom_cells(self, fp32=False):
+        v = ctypes.c_int64()
+        self._c(_lib.load().afg_get_bottom_cells(self.h, int(fp32), ctypes.byref(v)))
+        return v.value
 
     def close(self):
         if getattr(self, "h", None):
@@ -174,7 +181,7 @@ class DeviceTree:
         self._c(_lib.load().afg_update_coarse(self.h, ctypes.byref(mg), lvl))
 
     # controls and measurements
-    def set_bottom_cells(self, cells, wave_cells=512):
+    def set_bottom_cells(self, cells, wave_cells=0):
         self._c(_lib.load().afg_set_bottom_cells(self.h, int(cells), int(wave_cells)))
 
     def set_graphs(self, enabled):

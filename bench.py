@@ -226,7 +226,8 @@ def bench_c4(a):
     dt.autotune(mg)
     dt.autotune(mg, fp32=True)
     tiles = {"fp64": [dt.tile(lvl) for lvl in range(dt.min_lvl, dt.highest_lvl + 1)],
-             "fp32": [dt.tile(lvl, True) for lvl in range(dt.min_lvl, dt.highest_lvl + 1)]}
+             "fp32": [dt.tile(lvl, True) for lvl in range(dt.min_lvl, dt.highest_lvl + 1)],
+             "bottom_cells": [dt.bottom_cells(), dt.bottom_cells(True)]}
 
     def reset():
         dt.upload(1, st.data[0])
@@ -403,6 +404,7 @@ def main():
     # copy of phi, before timing)
     dt.autotune(mg)
     tiles = [dt.tile(lvl) for lvl in range(dt.min_lvl, dt.highest_lvl + 1)]
+    bottom = dt.bottom_cells()
     # cell-updates of one solve: FMG runs a V-cycle up to every level
     cu_solve = sum(dt.vcycle_cell_updates(mg, lvl) for lvl in range(dt.min_lvl, dt.highest_lvl + 1))
     cu_solve += p.n_vcycle * dt.vcycle_cell_updates(mg)
@@ -454,6 +456,7 @@ def main():
             "levels": [int(st.min_lvl), int(st.highest_lvl)],
             "cell_updates_per_step": cu_solve,
             "tile_shapes": tiles,
+            "bottom_cells": bottom,
         },
         "solves_per_s": a.steps / elapsed,
         "rel_residual_after_step": rel_res,

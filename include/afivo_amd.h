@@ -196,21 +196,28 @@ int afg_mixed_solve(afg_ctx *ctx, const afg_mg_opts *opts, int have_guess, doubl
 /* execution controls                                                  */
 
 /* The lowest levels of the tree, while each holds at most `cells` cells
- * (default 8192), run their part of every V-cycle in one single-workgroup
- * launch; 0 turns this off.  Within it, the lowest levels holding at most
- * `wave_cells` cells each (default 512) run in a single wavefront.  Results
+ * (default 2048) and all of them fit in one workgroup's LDS, run their part
+ * of every V-cycle in one single-workgroup launch; 0 turns this off.  Within
+ * it, the lowest levels holding at most `wave_cells` cells each (default 0)
+ * run in a single wavefront.  Sets the limit of both precisions.  Results
  * are identical either way. */
 int afg_set_bottom_cells(afg_ctx *ctx, int64_t cells, int64_t wave_cells);
+/* The cell limit in use for the FP64 (fp32 = 0) or FP32 stages. */
+int afg_get_bottom_cells(const afg_ctx *ctx, int fp32, int64_t *cells);
 /* Shape of the tiled colour pass on level lvl, for the FP64 (fp32 = 0) or
  * FP32 stages: 0 (default), 1 or 2 for nc 8 and 16 (workgroup size and boxes
  * per workgroup), 0 otherwise.  Results are identical for every shape. */
 int afg_set_tile(afg_ctx *ctx, int lvl, int fp32, int shape);
 int afg_get_tile(const afg_ctx *ctx, int lvl, int fp32, int *shape);
-/* Autotuner: on every level with several tile shapes, time `reps` colour
- * passes of each shape (on a copy of phi; the tree's data is not changed)
- * and keep the fastest.  If ms is given it receives, per level from the
- * lowest and per shape, the milliseconds per pass (-1: not timed; 3 values
- * per level).  No counterpart in the reference. */
+/* Autotuner, for the FP64 (fp32 = 0) or FP32 stages: on every level with
+ * several tile shapes, time `reps` colour passes of each shape (on a copy of
+ * phi) and keep the fastest (the current shape unless another is more than
+ * 3% faster); then time `reps` V-cycles with each bottom cell limit of
+ * 512, 2048, 8192 and 32768 and keep the fastest.  The tree's data is
+ * unchanged (saved and restored).  If ms is given it receives, per level
+ * from the lowest and per shape, the milliseconds per pass (3 values per
+ * level), then the milliseconds per V-cycle of the 4 limits (-1: not
+ * timed).  No counterpart in the reference. */
 int afg_autotune(afg_ctx *ctx, const afg_mg_opts *opts, int fp32, int reps, double *ms);
 /* Replay whole V-cycles from HIP graphs (default on). */
 int afg_set_graphs(afg_ctx *ctx, int enabled);

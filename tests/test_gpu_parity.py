@@ -338,11 +338,12 @@ def test_autotune_keeps_state(kind):
     dt = M.DeviceTree.from_state(p.state)
     before = [dt.download(iv) for iv in range(1, n_var + 1)]
     for fp32 in (False, True):
-        ms = dt.autotune(mg, fp32=fp32, reps=2)
+        ms, bms = dt.autotune(mg, fp32=fp32, reps=2)
+        assert np.all(bms > 0) and dt.bottom_cells(fp32) in (512, 2048, 8192, 32768)
         for lvl in range(dt.min_lvl, dt.highest_lvl + 1):
             row = ms[lvl - dt.min_lvl]
             if T.level_ncell(p.state.nc, lvl) in (8, 16):
-                assert np.all(row >= 0) and dt.tile(lvl, fp32) == int(np.argmin(row))
+                assert np.all(row >= 0) and row[dt.tile(lvl, fp32)] <= 1.03 * row.min()
             else:
                 assert np.all(row == -1) and dt.tile(lvl, fp32) == 0
     for iv in range(1, n_var + 1):
