@@ -314,16 +314,18 @@ struct GsrbTile {
     static constexpr int HALF = CELLS / 2;              // cells of one colour
     // shape V: workgroup size and boxes per workgroup.  0: 256 threads and
     // as many boxes as give one cell of the colour per thread (at least
-    // one); 1: twice the boxes, or (one box of 1024+ cells of a colour) 512
-    // threads; 2: 512 threads on 2 boxes, or 128 threads on half the boxes,
-    // or (one box of 1024+ cells) 1024 threads.
+    // one).  Boxes of 1024+ cells of a colour: 1: 512 threads, 2: 1024
+    // threads.  256..1023 cells: 1: 128 threads on one box, 2: 512 threads on
+    // 2 boxes.  Fewer: 1: twice the boxes, 2: 128 threads on half the boxes.
     static constexpr int B0 = HALF >= 256 ? 1 : 256 / HALF;
     static constexpr int THREADS = V == 0 ? 256
-                                   : V == 1 ? (HALF >= 1024 ? 512 : 256)
-                                   : (HALF >= 1024 ? 1024 : HALF >= 256 ? 512 : 128);
+                                   : HALF >= 1024 ? (V == 1 ? 512 : 1024)
+                                   : HALF >= 256 ? (V == 1 ? 128 : 512)
+                                   : (V == 1 ? 256 : 128);
     static constexpr int BOXES = V == 0 ? B0
-                                 : V == 1 ? (HALF >= 1024 ? 1 : 2 * B0)
-                                 : (HALF >= 1024 ? 1 : HALF >= 256 ? 2 : (B0 > 1 ? B0 / 2 : 1));
+                                 : HALF >= 1024 ? 1
+                                 : HALF >= 256 ? (V == 1 ? 1 : 2)
+                                 : (V == 1 ? 2 * B0 : (B0 > 1 ? B0 / 2 : 1));
     static constexpr int FACE = DIM == 2 ? NC : NC * NC;
     static constexpr int GHOSTS = DIM * FACE;           // face ghosts of one colour
     static constexpr int PER = (BOXES * HALF + THREADS - 1) / THREADS;   // cells per thread
