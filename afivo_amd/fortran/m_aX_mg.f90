@@ -119,9 +119,11 @@ module m_a$D_mg
   public :: mg$D_box_clpld
   public :: mg$D_box_gsrb_clpld
 #endif
-  ! device helpers (not in the reference)
+  ! device helpers and extensions (not in the reference)
   public :: mg$D_to_host
   public :: mg$D_device_release
+  public :: mg$D_mixed_solve
+  public :: mg$D_autotune
 
   ! device context of this module and the structure it holds
   type(c_ptr), save :: ctx = c_null_ptr
@@ -192,6 +194,50 @@ contains
          ctx, "mg$D_fas_vcycle")
     if (.not. mg%device_resident) call mg$D_to_host(tree, mg)
   end subroutine mg$D_fas_vcycle
+
+  !> Solve to max|rhs - A phi| <= tol * max|rhs| over the leaves by FP64 defect
+  !> correction with FP32 FAS multigrid corrections on the device
+  !> (afg_mixed_solve: each correction `cycles` FP32 V-cycles, the first an
+  !> FMG); phi starts from zero unless have_guess.  No counterpart in the
+  !> reference.  Returns the corrections and the relative residual reached.
+  subroutine mg$D_mixed_solve(tree, mg, tol, max_iter, have_guess, n_iter, rel_res, cycles)
+    type(a$D_t), intent(inout)    :: tree
+    type(mg$D_t), intent(in)      :: mg
+    real(dp), intent(in)          :: tol
+    integer, intent(in)           :: max_iter
+    logical, intent(in)           :: have_guess
+    integer, intent(out)          :: n_iter
+    real(dp), intent(out)         :: rel_res
+    integer, intent(in), optional :: cycles
+    type(afg_mg_opts)             :: o
+    integer(c_int)                :: it, cyc
+    real(c_double)                :: rr
+
+    call check_mg(mg)
+    cyc = 3
+    if (present(cycles)) cyc = cycles
+    call device_prepare(tree, mg, o)
+    call afg_check(afg_mixed_solve(ctx, o, merge(1, 0, have_guess), real(tol, c_double), &
+         int(max_iter, c_int), cyc, it, rr), ctx, "mg$D_mixed_solve")
+    n_iter = it
+    rel_res = rr
+    if (.not. mg%device_resident) call mg$D_to_host(tree, mg)
+  end subroutine mg$D_mixed_solve
+
+  !> Let the device library pick its tile shapes and bottom-kernel limit for
+  !> this tree from timed trial runs, for the FP64 and the FP32 stages
+  !> (afg_autotune; the tree's data is not changed).  No counterpart in the
+  !> reference.
+  subroutine mg$D_autotune(tree, mg)
+    type(a$D_t), intent(inout) :: tree
+    type(mg$D_t), intent(in)   :: mg
+    type(afg_mg_opts)          :: o
+
+    call check_mg(mg)
+    call device_prepare(tree, mg, o)
+    call afg_check(afg_autotune(ctx, o, 0, 8, c_null_ptr), ctx, "mg$D_autotune")
+    call afg_check(afg_autotune(ctx, o, 1, 8, c_null_ptr), ctx, "mg$D_autotune")
+  end subroutine mg$D_autotune
 
   !> Copy phi, rhs and tmp from the device into box%cc
   subroutine mg$D_to_host(tree, mg)

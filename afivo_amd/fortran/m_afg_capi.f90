@@ -84,6 +84,25 @@ module m_afg_capi
        integer(c_int), value :: highest_lvl, set_residual
      end function afg_fas_vcycle
 
+     integer(c_int) function afg_mixed_solve(ctx, opts, have_guess, tol, max_iter, cycles, iterations, &
+          rel_residual) bind(c, name="afg_mixed_solve")
+       import :: c_int, c_ptr, c_double, afg_mg_opts
+       type(c_ptr), value :: ctx
+       type(afg_mg_opts), intent(in) :: opts
+       integer(c_int), value :: have_guess, max_iter, cycles
+       real(c_double), value :: tol
+       integer(c_int), intent(out) :: iterations
+       real(c_double), intent(out) :: rel_residual
+     end function afg_mixed_solve
+
+     integer(c_int) function afg_autotune(ctx, opts, fp32, reps, ms) bind(c, name="afg_autotune")
+       import :: c_int, c_ptr, afg_mg_opts
+       type(c_ptr), value :: ctx
+       type(afg_mg_opts), intent(in) :: opts
+       integer(c_int), value :: fp32, reps
+       type(c_ptr), value :: ms
+     end function afg_autotune
+
      integer(c_int) function afg_tree_max_cc(ctx, iv, include_parents, out) bind(c, name="afg_tree_max_cc")
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx

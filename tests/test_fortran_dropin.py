@@ -94,3 +94,23 @@ def test_dropin_matches_reference(tmp_path, case):
     for iv in range(1, b.n_var + 1):
         assert_bitexact(a.data[iv - 1], b.data[iv - 1], f"{case} var {iv}")
     assert np.array_equal(O.read_res(prefix_gpu), O.read_res(prefix_ref))
+
+
+MIXED = os.path.join(BUILD, "mixed_2d")
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(MIXED), reason="drop-in test program absent")
+def test_dropin_extensions_autotune_and_mixed_precision():
+    """The drop-in's extensions from Fortran (tests/fortran/mixed_2d.f90):
+    mg2_autotune, then mg2_mixed_solve to 1e-10, against the module's plain
+    FP64 cycles on a copy of the tree."""
+    p = subprocess.run([MIXED], capture_output=True, text=True, timeout=600,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+    assert p.returncode == 0, p.stdout + p.stderr
+    w = p.stdout.split()
+    vals = dict(zip(w[0::2], w[1::2]))
+    assert int(vals["corrections"]) >= 2
+    assert float(vals["rel_res"]) <= 1e-10
+    assert int(vals["fp64_vcycles"]) >= 2
+    assert float(vals["phi_diff"]) <= 1e-9
