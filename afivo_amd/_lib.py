@@ -67,6 +67,8 @@ SIGNATURES = {
     "afg_fp32_to_cc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "afg_fas_fmg_fp32": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int, ctypes.c_int]),
     "afg_fas_vcycle_fp32": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int, ctypes.c_int]),
+    "afg_set_pass_pairs": (ctypes.c_int, [P, ctypes.c_int]),
+    "afg_get_pass_pairs": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "afg_set_tile": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "afg_get_tile": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "afg_get_bottom_cells": (ctypes.c_int, [P, ctypes.c_int, I64P]),

@@ -60,10 +60,13 @@ using LevelView = LevelViewT<double>;
 //                                   m_aX_prolong.f90:39-108)
 //   physical boundary            -> captured sides_bc value b, then bc_to_gc
 //                                   (m_aX_gc.f90:71-132): c2*b + c1*interior
-// `center` is phi at (i, j, k); `own` holds the level, `par` its parent level.
+// `center` is phi at (i, j, k); `own` holds the level, `par` its parent level;
+// `xbox`, when given, holds the box's own cells in place of own's (same
+// indexing), for a box whose current values are in LDS.
 template <int DIM, typename R>
 AFG_DEV R ghost_value(const DevTree &T, const LevelViewT<R> &own, const LevelViewT<R> &par, int pnc,
-                      double dr, int slot, int f, int nc, int i, int j, int k, R center)
+                      double dr, int slot, int f, int nc, int i, int j, int k, R center,
+                      const R *xbox = nullptr)
 {
     const int s = nc + 2;
     const int nbv = T.nb[slot * (2 * DIM) + f];
@@ -83,7 +86,7 @@ AFG_DEV R ghost_value(const DevTree &T, const LevelViewT<R> &own, const LevelVie
         const int pj = ((co >> 10) & 1023) + ((g1 + 1) >> 1);
         const int pk = DIM == 3 ? ((co >> 20) & 1023) + ((g2 + 1) >> 1) : 0;
         const R P = par.phi[par.off(p) + pi + ps * (pj + ps * pk)];
-        const R *box = own.phi + own.off(slot);
+        const R *box = xbox ? xbox : own.phi + own.off(slot);
         const int inward = high ? -1 : 1;
         if (DIM == 2) {
             const int di = dn == 0 ? inward : -1 + 2 * (i & 1);
@@ -343,9 +346,11 @@ AFG_HD inline int xcd_tile(int blk, int n_blk)
     return x * q + (x < r ? x : r) + y;
 }
 
+// `v` holds the level (its phi the pool read by this pass), `vp` the parent
+// level (for the refinement-boundary ghosts).
 template <int DIM, int NC, int FLY, int V, typename R>
-AFG_DEV void tile_stage_a(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &v, R *lds,
-                          int box0, int rb, int tid, R *rhs)
+AFG_DEV void tile_stage_a(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &v, const LevelViewT<R> &vp,
+                          R *lds, int box0, int rb, int tid, R *rhs)
 {
     using G = GsrbTile<DIM, NC, V>;
     constexpr int S = G::S;
@@ -397,7 +402,7 @@ AFG_DEV void tile_stage_a(const DevTree &T, const LevelArgs &L, const LevelViewT
         const size_t base = v.off(slot);
         R g;
         if (FLY)
-            g = ghost_value<DIM>(T, v, v, L.pnc, L.dr, slot, f, NC, i, j, k, v.phi[base + i + S * (j + S * k)]);
+            g = ghost_value<DIM>(T, v, vp, L.pnc, L.dr, slot, f, NC, i, j, k, v.phi[base + i + S * (j + S * k)]);
         else
             g = v.phi[base + gq];
         lds[b * G::LEN + gq] = g;
@@ -430,6 +435,201 @@ AFG_DEV void tile_stage_b(const DevTree &T, const LevelArgs &L, const LevelViewT
         const size_t base = v.off(slot);
         const bool lpld = lpld_box(T, op_auto, slot);
         v.phi[base + q] = gsrb_value<DIM>(u, rhs[r], lpld ? v.eps + base : nullptr, q, S, L.dx2, lpld);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Two colour passes in one launch: pass rb (colour A) and pass rb + 1
+// (colour B) of a relaxation block, reading the level's phi from one pool
+// and writing it to another (the two pools swap roles every pair), so that
+// every global read sees the state before the pair.  The views: `v` with the
+// pool read for the level, `vp` with the pool of the other levels (the
+// parent's phi of the refinement-boundary ghosts).  A workgroup takes the
+// boxes of a tile as k_gsrb_tile does:
+//   stage A   (tile_stage_a) interiors, colour-A face ghosts and rhs;
+//   phase 1   pass A on the tile's colour-A cells, in LDS; the colour-B face
+//             ghosts at same-level neighbours: the neighbour's mirrored cell
+//             after pass A, computed here from the pool exactly as the
+//             neighbour's own workgroup computes it; those at physical
+//             boundaries: the boundary formula;
+//   phase 2   the colour-B ghosts at refinement boundaries (they read the
+//             box's own cells after pass A, from LDS);
+//   phase 3   pass B on the colour-B cells; every interior cell is written
+//             to the output pool.
+// The values are those of the two passes run one after the other: each
+// cell and ghost is formed by the same expression from the same inputs.
+
+// rhs of the tile's cells of colour rb, as tile_stage_a loads it
+template <int DIM, int NC, int V, typename R>
+AFG_DEV void pair_load_rhs(const LevelArgs &L, const LevelViewT<R> &v, int box0, int rb, int tid, R *rhs)
+{
+    using G = GsrbTile<DIM, NC, V>;
+    constexpr int S = G::S;
+#pragma unroll
+    for (int r = 0; r < G::PER; r++) {
+        const int e = tid + r * G::THREADS;
+        const int b = e / G::HALF;
+        if (e < G::BOXES * G::HALF && box0 + b < L.n) {
+            int i, j, k;
+            colour_cell<DIM>(e - b * G::HALF, NC, rb, i, j, k);
+            rhs[r] = v.rhs[v.off(L.slot0 + box0 + b) + i + S * (j + S * k)];
+        }
+    }
+}
+
+// The value cell (i, j, k) of box y has after a colour pass on y's level
+// (the pass of that cell's colour; the pool holds the state before it):
+// what tile_stage_a / tile_stage_b compute for it, ghosts computed when FLY
+// and read from memory otherwise.
+template <int DIM, int NC, int FLY, typename R>
+AFG_DEV R cell_after_pass(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &v, const LevelViewT<R> &vp,
+                          int op_auto, int y, int i, int j, int k)
+{
+    constexpr int S = NC + 2;
+    const size_t base = v.off(y);
+    const R *box = v.phi + base;
+    const int q = i + S * (j + S * k);
+    const int x[3] = {i, j, k};
+    const int dq[6] = {-1, 1, -S, S, -S * S, S * S};
+    R u[6];
+#pragma unroll
+    for (int m = 0; m < 2 * DIM; m++) {
+        const int dn = m >> 1, high = m & 1;
+        const bool inside = high ? x[dn] < NC : x[dn] > 1;
+        if (inside || !FLY)
+            u[m] = box[q + dq[m]];
+        else
+            u[m] = ghost_value<DIM>(T, v, vp, L.pnc, L.dr, y, m, NC, i, j, k, box[q]);
+    }
+    if (DIM == 2) u[4] = u[5] = R(0);
+    const bool lpld = lpld_box(T, op_auto, y);
+    return gsrb_value<DIM>(u, v.rhs[base + q], lpld ? v.eps + base : nullptr, q, S, L.dx2, lpld);
+}
+
+// the colour-B face ghost e of the tile: (box b, face f, adjacent interior
+// cell (i, j, k), ghost index gq), in the enumeration of tile_stage_a
+template <int DIM, int NC, int V>
+AFG_DEV void pair_ghost(int e, int rb, int &b, int &f, int &i, int &j, int &k, int &gq)
+{
+    using G = GsrbTile<DIM, NC, V>;
+    constexpr int S = G::S, FH = G::FACE / 2;
+    b = e / G::GHOSTS;
+    const int r = e - b * G::GHOSTS;
+    f = r / FH;
+    const int t = r - f * FH;
+    const int dn = f >> 1, high = f & 1;
+    const int m = high ? NC : 1;
+    int t1, t2;
+    if (DIM == 2) {
+        t1 = 2 * t + 1 + ((rb + m + 1) & 1);
+        t2 = 0;
+    } else {
+        t2 = t / (NC / 2) + 1;
+        t1 = 2 * (t % (NC / 2)) + 1 + ((rb + m + t2 + 1) & 1);
+    }
+    i = dn == 0 ? m : t1;
+    j = dn == 1 ? m : (dn == 0 ? t1 : t2);
+    k = DIM == 3 ? (dn == 2 ? m : t2) : 0;
+    const int gi = dn == 0 ? (high ? NC + 1 : 0) : i;
+    const int gj = dn == 1 ? (high ? NC + 1 : 0) : j;
+    const int gk = dn == 2 ? (high ? NC + 1 : 0) : k;
+    gq = gi + S * (gj + S * gk);
+}
+
+template <int DIM, int NC, int FLYA, int V, typename R>
+AFG_DEV void pair_phase1(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &v, const LevelViewT<R> &vp,
+                         R *lds, int box0, int op_auto, int rbA, int tid, const R *rhsA)
+{
+    using G = GsrbTile<DIM, NC, V>;
+    constexpr int S = G::S;
+#pragma unroll
+    for (int r = 0; r < G::PER; r++) {
+        const int e = tid + r * G::THREADS;
+        const int b = e / G::HALF;
+        if (e >= G::BOXES * G::HALF || box0 + b >= L.n) break;
+        int i, j, k;
+        colour_cell<DIM>(e - b * G::HALF, NC, rbA, i, j, k);
+        const int q = i + S * (j + S * k);
+        R *x = lds + b * G::LEN + q;
+        R u[6];
+        u[0] = x[-1];
+        u[1] = x[1];
+        u[2] = x[-S];
+        u[3] = x[S];
+        u[4] = DIM == 3 ? x[-S * S] : R(0);
+        u[5] = DIM == 3 ? x[S * S] : R(0);
+        const int slot = L.slot0 + box0 + b;
+        const bool lpld = lpld_box(T, op_auto, slot);
+        x[0] = gsrb_value<DIM>(u, rhsA[r], lpld ? v.eps + v.off(slot) : nullptr, q, S, L.dx2, lpld);
+    }
+    const int rbB = rbA + 1;
+    for (int e = tid; e < G::BOXES * G::GHOSTS; e += G::THREADS) {
+        int b, f, i, j, k, gq;
+        pair_ghost<DIM, NC, V>(e, rbB, b, f, i, j, k, gq);
+        if (box0 + b >= L.n) break;
+        const int slot = L.slot0 + box0 + b;
+        const int nbv = T.nb[slot * (2 * DIM) + f];
+        if (nbv == NB_REFINE) continue;   // phase 2
+        R g;
+        if (nbv >= 0) {
+            const int dn = f >> 1, high = f & 1;
+            const int mm = high ? 1 : NC;   // the neighbour's mirrored cell
+            g = cell_after_pass<DIM, NC, FLYA>(T, L, v, vp, op_auto, nbv, dn == 0 ? mm : i, dn == 1 ? mm : j,
+                                              dn == 2 ? mm : k);
+        } else {
+            g = ghost_value<DIM>(T, v, vp, L.pnc, L.dr, slot, f, NC, i, j, k, lds[b * G::LEN + i + S * (j + S * k)]);
+        }
+        lds[b * G::LEN + gq] = g;
+    }
+}
+
+template <int DIM, int NC, int V, typename R>
+AFG_DEV void pair_phase2(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &v, const LevelViewT<R> &vp,
+                         R *lds, int box0, int rbA, int tid)
+{
+    using G = GsrbTile<DIM, NC, V>;
+    constexpr int S = G::S;
+    for (int e = tid; e < G::BOXES * G::GHOSTS; e += G::THREADS) {
+        int b, f, i, j, k, gq;
+        pair_ghost<DIM, NC, V>(e, rbA + 1, b, f, i, j, k, gq);
+        if (box0 + b >= L.n) break;
+        const int slot = L.slot0 + box0 + b;
+        if (T.nb[slot * (2 * DIM) + f] != NB_REFINE) continue;
+        R *xb = lds + b * G::LEN;
+        xb[gq] = ghost_value<DIM>(T, v, vp, L.pnc, L.dr, slot, f, NC, i, j, k, xb[i + S * (j + S * k)], xb);
+    }
+}
+
+template <int DIM, int NC, int V, typename R>
+AFG_DEV void pair_phase3(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &v, const R *lds, int box0,
+                         int op_auto, int rbA, int tid, const R *rhsB, R *out)
+{
+    using G = GsrbTile<DIM, NC, V>;
+    constexpr int S = G::S;
+#pragma unroll
+    for (int r = 0; r < G::PER; r++) {
+        const int e = tid + r * G::THREADS;
+        const int b = e / G::HALF;
+        if (e >= G::BOXES * G::HALF || box0 + b >= L.n) break;
+        const int slot = L.slot0 + box0 + b;
+        const size_t base = v.off(slot);
+        int i, j, k;
+        colour_cell<DIM>(e - b * G::HALF, NC, rbA + 1, i, j, k);
+        int q = i + S * (j + S * k);
+        const R *x = lds + b * G::LEN + q;
+        R u[6];
+        u[0] = x[-1];
+        u[1] = x[1];
+        u[2] = x[-S];
+        u[3] = x[S];
+        u[4] = DIM == 3 ? x[-S * S] : R(0);
+        u[5] = DIM == 3 ? x[S * S] : R(0);
+        const bool lpld = lpld_box(T, op_auto, slot);
+        out[base + q] = gsrb_value<DIM>(u, rhsB[r], lpld ? v.eps + base : nullptr, q, S, L.dx2, lpld);
+        // the colour-A cell of the same index, final since phase 1
+        colour_cell<DIM>(e - b * G::HALF, NC, rbA, i, j, k);
+        q = i + S * (j + S * k);
+        out[base + q] = lds[b * G::LEN + q];
     }
 }
 

@@ -35,6 +35,7 @@ struct Level {
     // belong to trees whose highest level is >= b (b = min_lvl..highest+1).
     std::vector<int> n_act, n_par_act, n_leaf_act;
     int tile[2] = {0, 0};   // tile shape of the colour pass: FP64, FP32 stages
+    bool pairs[2] = {false, false};   // colour passes as fused pairs: FP64, FP32 stages
 };
 
 struct BcRec {
@@ -143,6 +144,10 @@ struct afg_ctx {
     int8_t *d_is_parent = nullptr;
     std::vector<double *> d_var;
     std::vector<float *> d_var32;        // FP32 copies (mixed-precision solve), allocated on use
+    // second phi pool of the fused pass pairs (FP64, FP32): a level's phi goes
+    // there and back within a relaxation block
+    double *d_phi2 = nullptr;
+    float *d_phi2_32 = nullptr;
     double *d_partial = nullptr;
     int partial_cap = 0;
     double *d_out = nullptr, *h_out = nullptr;
@@ -283,6 +288,10 @@ void free_tree(afg_ctx *c)
     hipFree(c->d_partial);
     for (double *p : c->d_var) hipFree(p);
     for (float *p : c->d_var32) hipFree(p);
+    hipFree(c->d_phi2);
+    hipFree(c->d_phi2_32);
+    c->d_phi2 = nullptr;
+    c->d_phi2_32 = nullptr;
     c->d_nb = c->d_parent = c->d_choff = c->d_lists = c->d_bc_type = c->d_bc_voff = nullptr;
     c->d_bc_vals = nullptr;
     c->d_tag = nullptr;
@@ -308,7 +317,6 @@ int check_var(afg_ctx *c, int iv)
     return 0;
 }
 
-// Device pool of variable iv, allocated (and zeroed) on first use.
 // The device pool of variable iv (FP64, or its FP32 copy), allocated and
 // zeroed on first use.
 template <typename R>
@@ -403,7 +411,7 @@ double gsrb_pass_bytes(const afg_ctx *c, const LevelArgs &L, const MgVarsT<R> &V
 // events on the context stream when that level is timed (a timed block holds
 // n launches and their summed algorithmic bytes).
 template <typename R>
-void timed_passes(afg_ctx *c, const DevTree &T, const LevelArgs &L, int lvl, const MgVarsT<R> &V,
+void timed_passes(afg_ctx *c, const DevTree &T, const LevelArgs &L, int lvl, const MgVarsT<R> &V, R *phi2,
                   int n_pass, int first_fly)
 {
     const bool on = c->timing && n_pass > 0 && (c->timing_lvl < c->min_lvl || lvl == c->timing_lvl);
@@ -421,11 +429,23 @@ void timed_passes(afg_ctx *c, const DevTree &T, const LevelArgs &L, int lvl, con
         hipEventRecordWithFlags(ev.first, c->stream, flags);
     }
     LevelArgs Lt = L;
-    Lt.tile = c->levels[lvl - c->min_lvl].tile[sizeof(R) == 4];
-    for (int rb = 1; rb <= n_pass; rb++) launch_gsrb_pass(c->dim, c->stream, T, Lt, V, rb, rb == 1 ? first_fly : 1);
+    const Level &Lv = c->levels[lvl - c->min_lvl];
+    Lt.tile = Lv.tile[sizeof(R) == 4];
+    // Passes go by fused pairs, phi to the second pool and back, while an
+    // even number of pairs is left (on levels set to pairs); the rest one by
+    // one.
+    int rb = 1, launches = 0;
+    if (phi2 && Lv.pairs[sizeof(R) == 4])
+        for (; rb + 3 <= n_pass; rb += 4) {
+            if (!launch_gsrb_pair(c->dim, c->stream, T, Lt, V, V.phi, phi2, rb, rb == 1 ? first_fly : 1)) break;
+            launch_gsrb_pair(c->dim, c->stream, T, Lt, V, phi2, V.phi, rb + 2, 1);
+            launches += 2;
+        }
+    for (; rb <= n_pass; rb++, launches++)
+        launch_gsrb_pass(c->dim, c->stream, T, Lt, V, rb, rb == 1 ? first_fly : 1);
     if (!on) return;
     hipEventRecordWithFlags(ev.second, c->stream, flags);
-    const TimedBlock tb{n_pass, n_pass * gsrb_pass_bytes(c, L, V)};
+    const TimedBlock tb{launches, n_pass * gsrb_pass_bytes(c, L, V)};
     if (c->capturing) {
         c->cap_ev.push_back(ev);
         c->cap_blocks.push_back(tb);
@@ -480,8 +500,12 @@ struct MgT {
     DevTree T;
     MgVarsT<R> V;
     const afg_mg_opts *o;
+    R *phi2 = nullptr;   // second phi pool of the pass pairs (null: single passes)
 };
 using Mg = MgT<double>;
+
+double *&phi2_pool(afg_ctx *c, double *) { return c->d_phi2; }
+float *&phi2_pool(afg_ctx *c, float *) { return c->d_phi2_32; }
 
 template <typename R>
 int make_mg(afg_ctx *c, const afg_mg_opts *o, MgT<R> *m)
@@ -501,14 +525,24 @@ int make_mg(afg_ctx *c, const afg_mg_opts *o, MgT<R> *m)
         if (int r = ensure_var(c, o->i_eps, &e)) return r;
         m->V.eps = e;
     }
+    {
+        R *&p2 = phi2_pool(c, (R *)nullptr);
+        if (!p2) {
+            if (c->capturing) return c->fail(AFG_ERR_STATE, "second phi pool first used inside a graph capture");
+            const size_t bytes = (size_t)c->n_slots * c->stride * sizeof(R);
+            if (hipMalloc(&p2, bytes) != hipSuccess) {
+                p2 = nullptr;
+                return c->fail(AFG_ERR_MEMORY, "hipMalloc(%zu) for the second phi pool", bytes);
+            }
+            HIP_TRY(c, hipMemsetAsync(p2, 0, bytes, c->stream));
+        }
+        m->phi2 = p2;
+    }
     if (int e = upload_bc(c)) return e;
     m->T = dev_tree(c);
     return 0;
 }
 
-// Lazy tags of the auto operators (m_aX_mg.f90:567,602,655): boxes whose tag
-// is unset get it now; eps is not changed by the solver, so this equals the
-// reference's first-use evaluation.
 // The lazy tags of mg$D_set_box_tag (m_aX_mg.f90:667-706): set once per box,
 // from eps at the first call of an auto operator, and never re-evaluated.
 // The first solve sets every box's; later calls have nothing to do.
@@ -537,7 +571,7 @@ template <typename R>
 void passes(MgT<R> &m, int lvl, int n_cycle, int first_fly)
 {
     const LevelArgs L = level_args(m.c, lvl);
-    timed_passes(m.c, m.T, L, lvl, m.V, 2 * n_cycle, first_fly);
+    timed_passes(m.c, m.T, L, lvl, m.V, m.phi2, 2 * n_cycle, first_fly);
 }
 
 // gsrb_boxes as the reference has it: passes, then the last ghost fill.
@@ -930,11 +964,11 @@ int build_base(afg_ctx *c)
 // C ABI
 
 // The autotuner of the tiled colour pass: on every level with more than one
-// tile shape, each shape runs `reps` passes (colours alternating, face
-// ghosts computed) on a copy of phi, three times; the shape with the least
-// time per pass is kept for the level (the current one unless another is
-// more than 3% faster).  Every shape computes the same
-// values, so the choice changes speed only.
+// tile shape, each shape runs `reps` times two passes (two single passes,
+// and one fused pair; face ghosts computed) on a copy of phi, three times;
+// the shape and mode with the least time are kept for the level (the
+// current ones unless another is more than 3% faster).  Every shape and
+// mode computes the same values, so the choice changes speed only.
 template <typename R>
 static int tune_tiles(afg_ctx *c, const afg_mg_opts *o, int reps, double *ms, const double *phi64)
 {
@@ -961,30 +995,44 @@ static int tune_tiles(afg_ctx *c, const afg_mg_opts *o, int reps, double *ms, co
             const int nv = tile_shapes(Lv.nc);
             LevelArgs L = level_args(c, lvl);
             if (nv < 2 || L.n == 0) continue;
-            const int dflt = Lv.tile[sizeof(R) == 4];
-            std::vector<double> per_v(nv, 0.0);
-            for (int v = 0; v < nv; v++) {
-                L.tile = v;
-                for (int r = 0; r < 2; r++) launch_gsrb_pass(c->dim, c->stream, m.T, L, V, 1 + r, 1);
-                float t_min = 0;
-                for (int trial = 0; trial < 3; trial++) {
-                    HIP_TRY(c, hipEventRecord(e0, c->stream));
-                    for (int r = 0; r < reps; r++) launch_gsrb_pass(c->dim, c->stream, m.T, L, V, 1 + (r & 1), 1);
-                    HIP_TRY(c, hipEventRecord(e1, c->stream));
-                    HIP_TRY(c, hipEventSynchronize(e1));
-                    float t = 0;
-                    HIP_TRY(c, hipEventElapsedTime(&t, e0, e1));
-                    if (trial == 0 || t < t_min) t_min = t;
+            const int p = sizeof(R) == 4;
+            const int dflt = Lv.tile[p] + (Lv.pairs[p] ? 3 : 0);
+            // time of two passes: two single passes (mode 0) or one fused
+            // pair (mode 1: the copy of phi and the second pool in turn, the
+            // other levels from the tree's phi), per shape
+            double per[6];
+            for (int mode = 0; mode < 2; mode++)
+                for (int v = 0; v < nv; v++) {
+                    L.tile = v;
+                    auto step = [&](int r) {
+                        if (mode)
+                            launch_gsrb_pair(c->dim, c->stream, m.T, L, m.V, (r & 1) ? m.phi2 : scratch,
+                                             (r & 1) ? scratch : m.phi2, 1, 1);
+                        else
+                            for (int rb = 1; rb <= 2; rb++) launch_gsrb_pass(c->dim, c->stream, m.T, L, V, rb, 1);
+                    };
+                    for (int r = 0; r < 2; r++) step(r);
+                    float t_min = 0;
+                    for (int trial = 0; trial < 3; trial++) {
+                        HIP_TRY(c, hipEventRecord(e0, c->stream));
+                        for (int r = 0; r < reps; r++) step(r);
+                        HIP_TRY(c, hipEventRecord(e1, c->stream));
+                        HIP_TRY(c, hipEventSynchronize(e1));
+                        float t = 0;
+                        HIP_TRY(c, hipEventElapsedTime(&t, e0, e1));
+                        if (trial == 0 || t < t_min) t_min = t;
+                    }
+                    per[3 * mode + v] = (double)t_min / reps;
+                    if (ms) ms[(lvl - c->min_lvl) * 6 + 3 * mode + v] = per[3 * mode + v];
                 }
-                per_v[v] = (double)t_min / reps;
-                if (ms) ms[(lvl - c->min_lvl) * 3 + v] = per_v[v];
-            }
-            // the fastest shape, unless the current one is within 3% of it
-            // (timing noise would otherwise flip shapes between runs)
+            // the fastest (mode, shape), unless the current one is within 3%
+            // of it (timing noise would otherwise flip choices between runs)
             int best = 0;
-            for (int v = 1; v < nv; v++)
-                if (per_v[v] < per_v[best]) best = v;
-            Lv.tile[sizeof(R) == 4] = per_v[dflt] <= 1.03 * per_v[best] ? dflt : best;
+            for (int k = 1; k < 6; k++)
+                if (k % 3 < nv && per[k] < per[best]) best = k;
+            const int pick = per[dflt] <= 1.03 * per[best] ? dflt : best;
+            Lv.tile[p] = pick % 3;
+            Lv.pairs[p] = pick >= 3;
         }
         return 0;
     };
@@ -1234,6 +1282,9 @@ int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base,
         // default tile shapes (afg_autotune refines them): 3D boxes of 16^3
         // cells take 1024 threads, two cells of the colour per thread
         L.tile[0] = L.tile[1] = dim == 3 && L.nc == 16 ? 2 : 0;
+        // fused pass pairs by default in 2D (faster there; in 3D the
+        // neighbour work they repeat costs more than the launch they save)
+        L.pairs[0] = L.pairs[1] = dim == 2;
         // box%dr = dr_base * 0.5**(lvl-1) (m_aX_core.f90:196, halved per child :764)
         L.dr = dr_base * std::pow(0.5, lvl - 1);
         L.dx2 = L.dr * L.dr;
@@ -1954,13 +2005,13 @@ int afg_autotune(afg_ctx *c, const afg_mg_opts *o, int fp32, int reps, double *m
     if (c->capturing) return c->fail(AFG_ERR_STATE, "autotune inside a graph capture");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (ms)
-        for (int i = 0; i < 3 * (int)c->levels.size() + 4; i++) ms[i] = -1;
+        for (int i = 0; i < 6 * (int)c->levels.size() + 4; i++) ms[i] = -1;
     Mg m;
     if (int e = make_mg(c, o, &m)) return e;
     set_tags(m);
     if (int e = fp32 ? tune_tiles<float>(c, o, reps, ms, m.V.phi) : tune_tiles<double>(c, o, reps, ms, m.V.phi))
         return e;
-    double *bms = ms ? ms + 3 * (int)c->levels.size() : nullptr;
+    double *bms = ms ? ms + 6 * (int)c->levels.size() : nullptr;
     return fp32 ? tune_bottom<float>(c, o, reps, bms) : tune_bottom<double>(c, o, reps, bms);
 }
 
@@ -1968,6 +2019,23 @@ int afg_get_bottom_cells(const afg_ctx *c, int fp32, int64_t *cells)
 {
     if (!c || !cells) return AFG_ERR_ARG;
     *cells = c->bottom_cells[fp32 != 0];
+    return 0;
+}
+
+int afg_set_pass_pairs(afg_ctx *c, int enabled)
+{
+    if (int e = check_ctx(c)) return e;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (Level &L : c->levels) L.pairs[0] = L.pairs[1] = enabled != 0;
+    for (auto &g : c->graphs) destroy_graph(g.second);
+    c->graphs.clear();
+    return 0;
+}
+
+int afg_get_pass_pairs(const afg_ctx *c, int lvl, int fp32, int *enabled)
+{
+    if (!c || !c->have_tree || !enabled || lvl < c->min_lvl || lvl > c->highest_lvl) return AFG_ERR_ARG;
+    *enabled = c->levels[lvl - c->min_lvl].pairs[fp32 != 0] ? 1 : 0;
     return 0;
 }
 

@@ -22,6 +22,12 @@ void launch_diff(int dim, hipStream_t s, const DevTree &T, const ListArgs &P, co
 template <typename R>
 void launch_gsrb_pass(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
                       const MgVarsT<R> &V, int rb, int fly);
+// passes rbA and rbA + 1 in one launch (nc 8 and 16): the level's phi read
+// from `in` and written to `out` (V.phi holds the other levels); returns 0
+// (nothing launched) for other cell counts
+template <typename R>
+int launch_gsrb_pair(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V,
+                     const R *in, R *out, int rbA, int fly);
 template <typename R>
 void launch_residual(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
                      const MgVarsT<R> &V);

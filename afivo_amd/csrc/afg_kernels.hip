@@ -83,9 +83,29 @@ __global__ __launch_bounds__((GsrbTile<DIM, NC, V>::THREADS)) void k_gsrb_tile(D
     __shared__ R lds[G::BOXES * G::LEN];
     R rhs[G::PER];
     const int box0 = xcd_tile(blockIdx.x, gridDim.x) * G::BOXES;
-    tile_stage_a<DIM, NC, FLY, V>(T, L, v, lds, box0, rb, threadIdx.x, rhs);
+    tile_stage_a<DIM, NC, FLY, V>(T, L, v, v, lds, box0, rb, threadIdx.x, rhs);
     __syncthreads();
     tile_stage_b<DIM, NC, V>(T, L, v, lds, box0, op_auto, rb, threadIdx.x, rhs);
+}
+
+// two colour passes (rbA, rbA + 1) by tiles, phi read from v and written to out
+template <int DIM, int NC, int FLYA, typename R, int V>
+__global__ __launch_bounds__((GsrbTile<DIM, NC, V>::THREADS)) void k_gsrb_pair(DevTree T, LevelArgs L,
+                                                                              LevelViewT<R> v, LevelViewT<R> vp,
+                                                                              R *out, int op_auto, int rbA)
+{
+    using G = GsrbTile<DIM, NC, V>;
+    __shared__ R lds[G::BOXES * G::LEN];
+    R rhsA[G::PER], rhsB[G::PER];
+    const int box0 = xcd_tile(blockIdx.x, gridDim.x) * G::BOXES;
+    tile_stage_a<DIM, NC, FLYA, V>(T, L, v, vp, lds, box0, rbA, threadIdx.x, rhsA);
+    pair_load_rhs<DIM, NC, V>(L, v, box0, rbA + 1, threadIdx.x, rhsB);
+    __syncthreads();
+    pair_phase1<DIM, NC, FLYA, V>(T, L, v, vp, lds, box0, op_auto, rbA, threadIdx.x, rhsA);
+    __syncthreads();
+    pair_phase2<DIM, NC, V>(T, L, v, vp, lds, box0, rbA, threadIdx.x);
+    __syncthreads();
+    pair_phase3<DIM, NC, V>(T, L, v, lds, box0, op_auto, rbA, threadIdx.x, rhsB, out);
 }
 
 template <int DIM, int NC, typename R>
@@ -420,6 +440,44 @@ void gsrb_tiles(hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVar
     gsrb_tiles_v<DIM, NC, R, 0>(s, T, L, V, rb, fly);
 }
 
+template <int DIM, int NC, typename R, int V>
+void gsrb_pair_v(hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &Vs, const R *in, R *out,
+                 int rbA, int fly)
+{
+    using G = GsrbTile<DIM, NC, V>;
+    const int grid = (L.n + G::BOXES - 1) / G::BOXES;
+    const LevelViewT<R> vp = pool_view(T, Vs);
+    LevelViewT<R> v = vp;
+    v.phi = const_cast<R *>(in);
+    if (fly)
+        k_gsrb_pair<DIM, NC, 1, R, V><<<grid, G::THREADS, 0, s>>>(T, L, v, vp, out, Vs.op_auto, rbA);
+    else
+        k_gsrb_pair<DIM, NC, 0, R, V><<<grid, G::THREADS, 0, s>>>(T, L, v, vp, out, Vs.op_auto, rbA);
+}
+
+template <int DIM, int NC, typename R>
+void gsrb_pair(hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, const R *in, R *out,
+               int rbA, int fly)
+{
+    if (L.tile == 1) return gsrb_pair_v<DIM, NC, R, 1>(s, T, L, V, in, out, rbA, fly);
+    if (L.tile == 2) return gsrb_pair_v<DIM, NC, R, 2>(s, T, L, V, in, out, rbA, fly);
+    gsrb_pair_v<DIM, NC, R, 0>(s, T, L, V, in, out, rbA, fly);
+}
+
+template <typename R>
+int launch_gsrb_pair(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V,
+                     const R *in, R *out, int rbA, int fly)
+{
+    if (L.n <= 0) return 1;
+    switch (dim * 100 + L.nc) {
+    case 208: gsrb_pair<2, 8, R>(s, T, L, V, in, out, rbA, fly); return 1;
+    case 216: gsrb_pair<2, 16, R>(s, T, L, V, in, out, rbA, fly); return 1;
+    case 308: gsrb_pair<3, 8, R>(s, T, L, V, in, out, rbA, fly); return 1;
+    case 316: gsrb_pair<3, 16, R>(s, T, L, V, in, out, rbA, fly); return 1;
+    default: return 0;
+    }
+}
+
 template <typename R>
 void launch_gsrb_pass(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
                       const MgVarsT<R> &V, int rb, int fly)
@@ -644,6 +702,8 @@ void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVa
 // the two value types of the launchers
 #define AFG_INSTANTIATE(R)                                                                               \
     template void launch_fill_gc<R>(int, hipStream_t, const DevTree &, const LevelArgs &, R *);         \
+    template int launch_gsrb_pair<R>(int, hipStream_t, const DevTree &, const LevelArgs &,             \
+                                     const MgVarsT<R> &, const R *, R *, int, int);                    \
     template void launch_fill_diff<R>(int, hipStream_t, const DevTree &, const LevelArgs &,            \
                                       const ListArgs &, const MgVarsT<R> &);                           \
     template void launch_diff<R>(int, hipStream_t, const DevTree &, const ListArgs &,                  \

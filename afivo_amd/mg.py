@@ -92,14 +92,20 @@ class DeviceTree:
         return v.value
 
     def autotune(self, mg, fp32=False, reps=8):
-        """Pick the fastest tile shape per level and the fastest bottom cell
-        limit; returns (ms per pass as an array [level - min_lvl, shape],
+        """Pick the fastest tile shape and pass mode per level and the
+        fastest bottom cell limit; returns (ms per two passes as an array
+        [level - min_lvl, mode (0 single passes, 1 fused pair), shape],
         ms per V-cycle for the limits 512, 2048, 8192, 32768); -1 where not
         timed."""
         n = self.highest_lvl - self.min_lvl + 1
-        ms = np.zeros(3 * n + 4)
+        ms = np.zeros(6 * n + 4)
         self._c(_lib.load().afg_autotune(self.h, ctypes.byref(mg), int(fp32), int(reps), _f64(ms)))
-        return ms[:3 * n].reshape(n, 3), ms[3 * n:]
+        return ms[:6 * n].reshape(n, 2, 3), ms[6 * n:]
+
+    def pass_pairs(self, lvl, fp32=False):
+        v = ctypes.c_int()
+        self._c(_lib.load().afg_get_pass_pairs(self.h, int(lvl), int(fp32), ctypes.byref(v)))
+        return bool(v.value)
 
     def bottom_cells(self, fp32=False):
         v = ctypes.c_int64()
@@ -183,6 +189,10 @@ class DeviceTree:
     # controls and measurements
     def set_bottom_cells(self, cells, wave_cells=0):
         self._c(_lib.load().afg_set_bottom_cells(self.h, int(cells), int(wave_cells)))
+
+    def set_pass_pairs(self, enabled):
+        """Fused pairs of colour passes (default) or one pass per launch."""
+        self._c(_lib.load().afg_set_pass_pairs(self.h, int(enabled)))
 
     def set_graphs(self, enabled):
         self._c(_lib.load().afg_set_graphs(self.h, int(enabled)))

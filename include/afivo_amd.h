@@ -210,15 +210,23 @@ int afg_get_bottom_cells(const afg_ctx *ctx, int fp32, int64_t *cells);
 int afg_set_tile(afg_ctx *ctx, int lvl, int fp32, int shape);
 int afg_get_tile(const afg_ctx *ctx, int lvl, int fp32, int *shape);
 /* Autotuner, for the FP64 (fp32 = 0) or FP32 stages: on every level with
- * several tile shapes, time `reps` colour passes of each shape (on a copy of
- * phi) and keep the fastest (the current shape unless another is more than
- * 3% faster); then time `reps` V-cycles with each bottom cell limit of
- * 512, 2048, 8192 and 32768 and keep the fastest.  The tree's data is
- * unchanged (saved and restored).  If ms is given it receives, per level
- * from the lowest and per shape, the milliseconds per pass (3 values per
- * level), then the milliseconds per V-cycle of the 4 limits (-1: not
- * timed).  No counterpart in the reference. */
+ * several tile shapes, time `reps` times two colour passes with each shape,
+ * as two single passes and as one fused pair (on a copy of phi), and keep
+ * the fastest shape and mode (the current ones unless another is more than
+ * 3% faster); then time `reps` V-cycles with each bottom cell limit of 512,
+ * 2048, 8192 and 32768 and keep the fastest.  The tree's data is unchanged
+ * (saved and restored).  If ms is given it receives, per level from the
+ * lowest, the milliseconds per two passes of shapes 0..2 as single passes,
+ * then as pairs (6 values per level), then the milliseconds per V-cycle of
+ * the 4 limits (-1: not timed).  No counterpart in the reference. */
 int afg_autotune(afg_ctx *ctx, const afg_mg_opts *opts, int fp32, int reps, double *ms);
+/* Run the colour passes of the relaxation blocks (nc 8 and 16) of every
+ * level as fused pairs, two passes per launch with phi going to a second
+ * pool and back, or one pass per launch (default: pairs in 2D, single
+ * passes in 3D; afg_autotune chooses per level).  Results are identical
+ * either way. */
+int afg_set_pass_pairs(afg_ctx *ctx, int enabled);
+int afg_get_pass_pairs(const afg_ctx *ctx, int lvl, int fp32, int *enabled);
 /* Replay whole V-cycles from HIP graphs (default on). */
 int afg_set_graphs(afg_ctx *ctx, int enabled);
 /* Per-stage device timing: while enabled, each launch of the GSRB colour-pass

@@ -108,9 +108,9 @@ void gsrb_tiles_v(const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int
         const int box0 = xcd_tile(blk, grid) * G::BOXES;
         for (int t = 0; t < G::THREADS; t++) {
             if (fly)
-                tile_stage_a<DIM, NC, 1, SH>(T, L, v, lds, box0, rb, t, &rhs[(size_t)t * G::PER]);
+                tile_stage_a<DIM, NC, 1, SH>(T, L, v, v, lds, box0, rb, t, &rhs[(size_t)t * G::PER]);
             else
-                tile_stage_a<DIM, NC, 0, SH>(T, L, v, lds, box0, rb, t, &rhs[(size_t)t * G::PER]);
+                tile_stage_a<DIM, NC, 0, SH>(T, L, v, v, lds, box0, rb, t, &rhs[(size_t)t * G::PER]);
         }
         for (int t = 0; t < G::THREADS; t++)
             tile_stage_b<DIM, NC, SH>(T, L, v, lds, box0, V.op_auto, rb, t, &rhs[(size_t)t * G::PER]);
@@ -152,6 +152,66 @@ void uc_tiles(const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int kee
         for (int t = 0; t < G::THREADS; t++) uc_tile_restrict<DIM, NC>(T, L, v, lds, box0, t, v.tmp);
     }
     emul_free(lds);
+}
+
+// the fused pass pair of the device: workgroups one after the other, each
+// phase over all threads of the workgroup before the next
+template <int DIM, int NC, typename R, int SH>
+void gsrb_pair_v(const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, const R *in, R *out, int rbA, int fly)
+{
+    using G = GsrbTile<DIM, NC, SH>;
+    const LevelViewT<R> vp = pool_view(T, V);
+    LevelViewT<R> v = vp;
+    v.phi = const_cast<R *>(in);
+    const int grid = (L.n + G::BOXES - 1) / G::BOXES;
+    R *lds = (R *)emul_alloc(sizeof(R) * G::BOXES * G::LEN);
+    std::vector<R> rA((size_t)G::THREADS * G::PER), rB((size_t)G::THREADS * G::PER);
+    for (int blk = 0; blk < grid; blk++) {
+        const int box0 = xcd_tile(blk, grid) * G::BOXES;
+        for (int t = 0; t < G::THREADS; t++) {
+            if (fly)
+                tile_stage_a<DIM, NC, 1, SH>(T, L, v, vp, lds, box0, rbA, t, &rA[(size_t)t * G::PER]);
+            else
+                tile_stage_a<DIM, NC, 0, SH>(T, L, v, vp, lds, box0, rbA, t, &rA[(size_t)t * G::PER]);
+            pair_load_rhs<DIM, NC, SH>(L, v, box0, rbA + 1, t, &rB[(size_t)t * G::PER]);
+        }
+        for (int t = 0; t < G::THREADS; t++) {
+            if (fly)
+                pair_phase1<DIM, NC, 1, SH>(T, L, v, vp, lds, box0, V.op_auto, rbA, t, &rA[(size_t)t * G::PER]);
+            else
+                pair_phase1<DIM, NC, 0, SH>(T, L, v, vp, lds, box0, V.op_auto, rbA, t, &rA[(size_t)t * G::PER]);
+        }
+        for (int t = 0; t < G::THREADS; t++) pair_phase2<DIM, NC, SH>(T, L, v, vp, lds, box0, rbA, t);
+        for (int t = 0; t < G::THREADS; t++)
+            pair_phase3<DIM, NC, SH>(T, L, v, lds, box0, V.op_auto, rbA, t, &rB[(size_t)t * G::PER], out);
+    }
+    emul_free(lds);
+}
+
+template <int DIM, int NC, typename R>
+void gsrb_pair(const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, const R *in, R *out, int rbA, int fly)
+{
+    if (L.tile == 1) return gsrb_pair_v<DIM, NC, R, 1>(T, L, V, in, out, rbA, fly);
+    if (L.tile == 2) return gsrb_pair_v<DIM, NC, R, 2>(T, L, V, in, out, rbA, fly);
+    gsrb_pair_v<DIM, NC, R, 0>(T, L, V, in, out, rbA, fly);
+}
+
+template <typename R>
+int launch_gsrb_pair(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V,
+                     const R *in, R *out, int rbA, int fly)
+{
+    if (L.n <= 0) return 1;
+    const int key = dim * 100 + L.nc;
+    if (key != 208 && key != 216 && key != 308 && key != 316) return 0;
+    emul_submit(s, [=] {
+        switch (key) {
+        case 208: gsrb_pair<2, 8, R>(T, L, V, in, out, rbA, fly); break;
+        case 216: gsrb_pair<2, 16, R>(T, L, V, in, out, rbA, fly); break;
+        case 308: gsrb_pair<3, 8, R>(T, L, V, in, out, rbA, fly); break;
+        default: gsrb_pair<3, 16, R>(T, L, V, in, out, rbA, fly); break;
+        }
+    });
+    return 1;
 }
 
 template <typename R>
@@ -421,6 +481,8 @@ void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVa
 }
 
 #define AFG_INSTANTIATE(R)                                                                               \
+    template int launch_gsrb_pair<R>(int, hipStream_t, const DevTree &, const LevelArgs &,             \
+                                     const MgVarsT<R> &, const R *, R *, int, int);                    \
     template void launch_fill_gc<R>(int, hipStream_t, const DevTree &, const LevelArgs &, R *);         \
     template void launch_fill_diff<R>(int, hipStream_t, const DevTree &, const LevelArgs &,            \
                                       const ListArgs &, const MgVarsT<R> &);                           \

@@ -36,6 +36,7 @@ from test_gpu_parity import (  # noqa: E402,F401
     test_solve_bitexact,
     test_stage_bitexact,
     test_tile_shapes_are_exact,
+    test_pass_pairs_are_exact,
     test_autotune_keeps_state,
     test_timing_mode_is_exact,
 )

@@ -198,8 +198,16 @@ def test_graphs_and_direct_launches_agree():
         assert_bitexact(a, b, "graph vs direct")
 
 
+def launches_per_block(n_cycle, pairs):
+    """Launches of a relaxation block of 2 n_cycle passes: fused pairs while
+    an even number of pairs is left, then single passes."""
+    n_pass = 2 * n_cycle
+    return 2 * (n_pass // 4) + n_pass % 4 if pairs else n_pass
+
+
 @pytest.mark.parametrize("one_level", [False, True])
-def test_timing_mode_is_exact(one_level):
+@pytest.mark.parametrize("pairs", [True, False])
+def test_timing_mode_is_exact(one_level, pairs):
     """Event timing (graphs with event records, used in turn from a ring of
     copies) leaves results unchanged and counts every timed launch."""
     p = P.small_2d()
@@ -207,18 +215,19 @@ def test_timing_mode_is_exact(one_level):
     om = O.default_opts(2, 2, p.n_base)
     dt = M.DeviceTree.from_state(p.state)
     dt.set_bottom_cells(0)      # every level through the timed per-level passes
+    dt.set_pass_pairs(pairs)
     top = p.state.highest_lvl
     dt.set_timing(True, top if one_level else -1000)
     n_cyc = 11                  # more V-cycles than graph copies in the ring
     dt, ot, rd, ro = solve_both(p.state, 3, mg, om, 1, n_cyc, dt=dt)
     n, ms, b = dt.timing()
     assert ms > 0 and b > 0
-    passes = 2 * (mg.n_cycle_down + mg.n_cycle_up)
+    per_cycle = launches_per_block(mg.n_cycle_down, pairs) + launches_per_block(mg.n_cycle_up, pairs)
     if one_level:
         # FMG's last V-cycle, then the V-cycles
-        assert n == (n_cyc + 1) * passes
+        assert n == (n_cyc + 1) * per_cycle
     else:
-        assert n > (n_cyc + 1) * passes
+        assert n > (n_cyc + 1) * per_cycle
     compare_all(dt, ot, 3, "timing")
     assert np.array_equal(rd, ro)
 
@@ -343,7 +352,8 @@ def test_autotune_keeps_state(kind):
         for lvl in range(dt.min_lvl, dt.highest_lvl + 1):
             row = ms[lvl - dt.min_lvl]
             if T.level_ncell(p.state.nc, lvl) in (8, 16):
-                assert np.all(row >= 0) and row[dt.tile(lvl, fp32)] <= 1.03 * row.min()
+                chosen = row[int(dt.pass_pairs(lvl, fp32)), dt.tile(lvl, fp32)]
+                assert np.all(row >= 0) and chosen <= 1.03 * row.min()
             else:
                 assert np.all(row == -1) and dt.tile(lvl, fp32) == 0
     for iv in range(1, n_var + 1):
@@ -351,3 +361,36 @@ def test_autotune_keeps_state(kind):
     dt, ot, rd, ro = solve_both(p.state, n_var, mg, om, 1, 2, dt=dt)
     compare_all(dt, ot, n_var, f"{kind} tuned")
     assert np.array_equal(rd, ro)
+
+
+@pytest.mark.parametrize("kind", ["2d_nc8", "2d_nc16", "3d_nc8", "3d_nc16_eps", "2d_nc8_eps"])
+@pytest.mark.parametrize("n_down,n_up", [(2, 2), (1, 3), (4, 2)])
+def test_pass_pairs_are_exact(kind, n_down, n_up):
+    """Fused pass pairs (phi to the second pool and back) and single passes
+    give the oracle's bits, for blocks of 2, 4, 6 and 8 passes (pairs and
+    single passes mixed); FP32 stages give the same bits either way."""
+    p = problem(kind)
+    eps = p.eps_hi is not None
+    n_var = len(p.state.data)
+    mg = M.default_mg(eps=eps, n_down=n_down, n_up=n_up, n_base=p.n_base)
+    om = O.default_opts(n_down, n_up, p.n_base, eps=eps)
+    res = []
+    for pairs in (True, False):
+        dt = M.DeviceTree.from_state(p.state)
+        dt.set_bottom_cells(0)
+        dt.set_pass_pairs(pairs)
+        dt, ot, rd, ro = solve_both(p.state, n_var, mg, om, 1, 2, dt=dt)
+        compare_all(dt, ot, n_var, f"{kind} pairs {pairs}")
+        assert np.array_equal(rd, ro)
+        d = M.DeviceTree.from_state(p.state, n_var + 1)
+        d.set_bottom_cells(0)
+        d.set_pass_pairs(pairs)
+        for iv in range(1, n_var + 1):
+            d.fp32_from_cc(iv, iv)
+        M.mg_fas_fmg_fp32(d, mg, False, False)
+        M.mg_fas_vcycle_fp32(d, mg, d.highest_lvl, False)
+        d.fp32_to_cc(1, n_var + 1)
+        res.append(d.download(n_var + 1))
+        d.close()
+        dt.close()
+    assert_bitexact(res[0], res[1], f"{kind} FP32 pairs")
