@@ -81,9 +81,10 @@ def measured_traffic():
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(p, cell_updates_per_solve, threads):
+def cpu_baseline(p, cell_updates_per_solve, threads, n_time=5):
     """The reference solver, built from the reference sources (oracle/_ref),
-    on the same tree: median solve time of 5 solves after one warm-up solve."""
+    on the same tree: median solve time of n_time solves after one warm-up
+    solve."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from afivo_amd import problems as P
@@ -91,7 +92,7 @@ def cpu_baseline(p, cell_updates_per_solve, threads):
         return None
     import tempfile
     work = tempfile.mkdtemp(prefix="afg_cpu_")
-    cfg = P.harness_cfg(p, n_time=5, dump_input=False, dump_output=False)
+    cfg = P.harness_cfg(p, n_time=n_time, dump_input=False, dump_output=False)
     prefix, _ = O.run_ref(3, cfg, workdir=work, threads=threads, timeout=900)
     n_thr, t = O.read_time(prefix)
     med = float(np.median(t))
@@ -504,6 +505,10 @@ def main():
     if not a.no_cpu_baseline:
         thr = a.cpu_threads or min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(p, cu_solve, thr)
+        one = cpu_baseline(p, cu_solve, 1, n_time=2) if out["cpu_baseline"] and thr > 1 else None
+        if one:   # the same on one core, for the per-core figure
+            out["cpu_baseline"]["one_thread"] = {"value": one["value"], "s_per_solve": one["s_per_solve"],
+                                                 "sample": one["sample"]}
     else:
         out["cpu_baseline"] = None
     dt.close()
