@@ -42,6 +42,7 @@ SIGNATURES = {
     "afg_sync": (ctypes.c_int, [P]),
     "afg_set_tree": (ctypes.c_int, [P, ctypes.c_int, I32P, F64]),
     "afg_set_forest": (ctypes.c_int, [P, ctypes.c_int, I32P, F64, I32P]),
+    "afg_update_tree": (ctypes.c_int, [P, ctypes.c_int, I32P, F64, I32P, ctypes.c_int, ctypes.c_int, I32P]),
     "afg_forest_trees": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), I32P]),
     "afg_forest_max_abs": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, F64P]),
     "afg_set_bc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, F64P]),

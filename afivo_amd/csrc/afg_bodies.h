@@ -1085,6 +1085,85 @@ AFG_DEV void prolong2_body(const DevTree &T, const LevelArgs &L, const R *from, 
 }
 
 // ---------------------------------------------------------------------------
+// tree update (the data side of a$D_adjust_refinement, m_aX_core.f90:452-541)
+
+// Whole box arrays of a new pool: slot b takes the array of old slot src[b]
+// (the same box before the update), or zeros for a box the update added.
+// Two values per thread: a box array holds (nc+2)^dim values with nc even,
+// so every array starts on a 16-byte boundary.
+struct alignas(16) Pair64 {
+    double x, y;
+};
+
+AFG_DEV void gather_body(int stride, const int32_t *src, const double *from, double *to, long long gid)
+{
+    const int half = stride >> 1;
+    const int b = (int)(gid / half);
+    const int q = (int)(gid - (long long)b * half);
+    const int o = src[b];
+    Pair64 v = {0.0, 0.0};
+    if (o >= 0) v = reinterpret_cast<const Pair64 *>(from + (size_t)o * stride)[q];
+    reinterpret_cast<Pair64 *>(to + (size_t)b * stride)[q] = v;
+}
+
+// a$D_prolong0 / a$D_prolong1 / a$D_prolong2 (m_aX_prolong.f90:39-108,
+// 130-219, 243-348) of a variable from each listed box's parent into the
+// box's interior.  Orders 1 and 2 zero the interior and add, so the stored
+// value is expr + 0; order 0 stores the parent's value.  Order 1 sums
+// f0 + f_x + f_y (+ f_z) left to right, f0 = 0.5 (2D) or 0.25 (3D) times the
+// parent's cell and f_x.. = 0.25 times the neighbour towards the child cell.
+// One thread per coarse cell of the child's octant.
+template <int DIM, int ORDER, typename R>
+AFG_DEV void prolong_list_body(const DevTree &T, int nc, int pnc, const int32_t *slots, R *var, int gid)
+{
+    const int hn = nc >> 1;
+    const int per_box = ipow(hn, DIM);
+    const int b = gid / per_box;
+    const int c = gid - b * per_box;
+    const int ii = c % hn + 1;
+    const int jj = (c / hn) % hn + 1;
+    const int kk = DIM == 3 ? c / (hn * hn) + 1 : 0;
+    const int slot = slots[b];
+    const int s = nc + 2;
+    const int co = T.choff[slot];
+    const int ps = pnc + 2;
+    const int pq = ((co & 1023) + ii) +
+                   ps * ((((co >> 10) & 1023) + jj) + ps * (DIM == 3 ? ((co >> 20) & 1023) + kk : 0));
+    const R *pb = var + (size_t)T.parent[slot] * T.stride;
+    R *cb = var + (size_t)slot * T.stride;
+    auto at = [&](int m) -> R & {
+        const int fi = 2 * ii - 1 + (m & 1), fj = 2 * jj - 1 + ((m >> 1) & 1);
+        const int fk = DIM == 3 ? 2 * kk - 1 + ((m >> 2) & 1) : 0;
+        return cb[fi + s * (fj + s * fk)];
+    };
+    if constexpr (ORDER == 0) {
+        const R v = pb[pq];
+#pragma unroll
+        for (int m = 0; m < (1 << DIM); m++) at(m) = v;
+    } else if constexpr (ORDER == 1) {
+        const int step[3] = {1, ps, ps * ps};
+        const R f0 = R(DIM == 2 ? 0.5 : 0.25) * pb[pq];
+        R lo[3], hi[3];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            lo[d] = R(0.25) * pb[pq - step[d]];
+            hi[d] = R(0.25) * pb[pq + step[d]];
+        }
+#pragma unroll
+        for (int m = 0; m < (1 << DIM); m++) {
+            R v = f0;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) v = v + (((m >> d) & 1) ? hi[d] : lo[d]);
+            at(m) = v + R(0);
+        }
+    } else {
+        const Prolong2<DIM, R> P2([&](int q) -> R { return pb[q]; }, pq, ps);
+#pragma unroll
+        for (int m = 0; m < (1 << DIM); m++) at(m) = P2.at(m) + R(0);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // bottom of the V-cycle in one workgroup, on LDS copies of the small levels
 //
 // Loads phi, rhs, tmp (and eps) of levels 0..hl (relative to the lowest)

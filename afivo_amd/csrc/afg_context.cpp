@@ -16,6 +16,7 @@
 #include <map>
 #include <string>
 #include <tuple>
+#include <unordered_map>
 #include <vector>
 
 using namespace afg;
@@ -123,6 +124,7 @@ struct afg_ctx {
     std::vector<Level> levels;           // index lvl - min_lvl
     std::vector<int32_t> id2slot;        // -1 if absent
     std::vector<int32_t> slot2id, slot_lvl, slot_nc;
+    std::vector<int32_t> slot_ix, slot_tree;   // box%ix (3 per slot) and tree index per slot
     std::vector<int32_t> h_nb;           // slot * nnb + face (afivo ids)
     std::vector<BcRec> bc;
     std::vector<int32_t> bc_of;          // slot * nnb + face -> bc record or -1
@@ -1102,6 +1104,32 @@ static int tune_bottom(afg_ctx *c, const afg_mg_opts *o, int reps, double *ms)
     return err ? err : e2;
 }
 
+// A box as the tree update matches it between the old and the new tree: its
+// tree, level and index (ids may differ: a$D_tidy_up renumbers them,
+// m_aX_core.f90:238-314, and freed ids are handed out again).
+struct BoxKey {
+    int tree, lvl, ix[3];
+    bool operator==(const BoxKey &o) const
+    {
+        return tree == o.tree && lvl == o.lvl && ix[0] == o.ix[0] && ix[1] == o.ix[1] && ix[2] == o.ix[2];
+    }
+};
+
+struct BoxKeyHash {
+    size_t operator()(const BoxKey &k) const
+    {
+        uint64_t h = 0xcbf29ce484222325ull;
+        for (int v : {k.tree, k.lvl, k.ix[0], k.ix[1], k.ix[2]}) h = (h ^ (uint32_t)v) * 0x100000001b3ull;
+        return (size_t)(h ^ (h >> 31));
+    }
+};
+
+static BoxKey box_key(const afg_ctx *c, int s)
+{
+    const int32_t *ix = &c->slot_ix[(size_t)s * 3];
+    return BoxKey{c->slot_tree[s], c->slot_lvl[s], {ix[0], ix[1], ix[2]}};
+}
+
 extern "C" {
 
 int afg_ctx_create(int dim, int n_cell, int n_var, afg_ctx **out)
@@ -1262,6 +1290,7 @@ int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base,
     c->slot2id.assign(n_boxes, 0);
     c->slot_lvl.assign(n_boxes, 0);
     c->slot_nc.assign(n_boxes, 0);
+    c->slot_ix.assign((size_t)n_boxes * 3, 0);
     int slot = 0;
     for (int li = 0; li < nl; li++) {
         const int lvl = min_lvl + li;
@@ -1297,6 +1326,7 @@ int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base,
             c->slot2id[slot] = id;
             c->slot_lvl[slot] = lvl;
             c->slot_nc[slot] = L.nc;
+            for (int d = 0; d < dim; d++) c->slot_ix[(size_t)slot * 3 + d] = recs[(size_t)kr.rec * rl + 2 + d];
             slot++;
         }
     }
@@ -1310,6 +1340,7 @@ int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base,
     c->res_upper = hi_lvl + 1;
     std::vector<int> slot_tree(slot);
     for (int r = 0; r < n_boxes; r++) slot_tree[c->id2slot[recs[(size_t)r * rl]]] = tree(r);
+    c->slot_tree = slot_tree;
     c->max_id = max_id;
     c->dr_base = dr_base;
 
@@ -1479,6 +1510,123 @@ int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base,
     HIP_TRY(c, hipMalloc(&c->d_partial, c->partial_cap * sizeof(double)));
     c->bc_dirty = true;
     if (int e = upload_bc(c)) return e;
+    return 0;
+}
+
+int afg_update_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base, const int32_t *tree_of,
+                    int prolong_order, int n_vars, const int32_t *vars)
+{
+    if (int e = check_ctx(c)) return e;
+    if (c->capturing) return c->fail(AFG_ERR_STATE, "tree update inside a graph capture");
+    if (prolong_order < -1 || prolong_order > 2)
+        return c->fail(AFG_ERR_ARG, "prolong_order %d not in -1..2", prolong_order);
+    if (n_vars < 0 || (n_vars > 0 && !vars)) return c->fail(AFG_ERR_ARG, "invalid variable list");
+    for (int k = 0; k < n_vars; k++)
+        if (int e = check_var(c, vars[k])) return e;
+    if (c->stride & 1) return c->fail(AFG_ERR_UNSUPPORTED, "odd box length %d", c->stride);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+
+    // the old tree: its boxes, boundary conditions, level settings and pools
+    const int nnb = c->nnb, dim = c->dim;
+    std::unordered_map<BoxKey, int, BoxKeyHash> old_slot;
+    old_slot.reserve((size_t)c->n_slots * 2);
+    for (int s = 0; s < c->n_slots; s++) old_slot.emplace(box_key(c, s), s);
+    const std::vector<BcRec> old_bc = c->bc;
+    const std::vector<int32_t> old_bc_of = c->bc_of;
+    const std::vector<double> old_bc_vals = c->bc_vals;
+    std::map<int, Level> old_level;
+    for (const Level &L : c->levels) old_level[L.lvl] = L;
+    std::vector<double *> old_pool = c->d_var;
+    std::fill(c->d_var.begin(), c->d_var.end(), nullptr);
+    int32_t *d_src = nullptr;
+    auto release = [&](int code) {
+        for (double *p : old_pool) hipFree(p);
+        hipFree(d_src);
+        return code;
+    };
+    if (int e = afg_set_forest(c, n_boxes, recs, dr_base, tree_of)) return release(e);
+
+    // src[slot]: the box's old slot, -1 for a box the update added; the
+    // added boxes level by level (lowest first)
+    const int ns = c->n_slots, nl = (int)c->levels.size();
+    std::vector<int32_t> src(ns, -1), added;
+    std::vector<int> lv_off(nl + 1, 0);
+    for (int li = 0; li < nl; li++) {
+        const Level &L = c->levels[li];
+        lv_off[li] = (int)added.size();
+        for (int s = L.slot0; s < L.slot0 + L.n; s++) {
+            auto it = old_slot.find(box_key(c, s));
+            if (it != old_slot.end()) src[s] = it->second;
+            else added.push_back(s);
+        }
+    }
+    lv_off[nl] = (int)added.size();
+    if (hipMalloc(&d_src, ((size_t)ns + added.size()) * sizeof(int32_t)) != hipSuccess) {
+        d_src = nullptr;
+        return release(c->fail(AFG_ERR_MEMORY, "tree update: slot map of %d slots", ns));
+    }
+    int32_t *d_added = d_src + ns;
+    if (hipMemcpyAsync(d_src, src.data(), (size_t)ns * sizeof(int32_t), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        (!added.empty() && hipMemcpyAsync(d_added, added.data(), added.size() * sizeof(int32_t),
+                                          hipMemcpyHostToDevice, c->stream) != hipSuccess))
+        return release(c->fail(AFG_ERR_HIP, "tree update: copying the slot map"));
+
+    // every variable the old tree had: gathered into a new pool (zeros on
+    // the added boxes)
+    const size_t pool_bytes = (size_t)ns * c->stride * sizeof(double);
+    for (int iv = 1; iv <= c->n_var; iv++) {
+        if (!old_pool[iv - 1]) continue;
+        double *p = nullptr;
+        if (hipMalloc(&p, pool_bytes) != hipSuccess)
+            return release(c->fail(AFG_ERR_MEMORY, "hipMalloc(%zu) for variable %d", pool_bytes, iv));
+        launch_gather(c->stream, c->stride, ns, d_src, old_pool[iv - 1], p);
+        c->d_var[iv - 1] = p;
+    }
+    // the listed variables of the added boxes from their parents, level by
+    // level (a box's parent may itself be new); added boxes of the lowest
+    // level have no parent and keep zeros
+    if (prolong_order >= 0) {
+        const DevTree T = dev_tree(c);
+        for (int k = 0; k < n_vars; k++) {
+            double *v = nullptr;
+            if (int e = ensure_var(c, vars[k], &v)) return release(e);
+            for (int li = 1; li < nl; li++)
+                launch_prolong_list(dim, c->stream, T, c->levels[li].nc, c->levels[li - 1].nc,
+                                    d_added + lv_off[li], lv_off[li + 1] - lv_off[li], prolong_order, v);
+        }
+    }
+    const hipError_t se = hipStreamSynchronize(c->stream);
+    release(0);
+    if (se != hipSuccess) return c->fail(AFG_ERR_HIP, "tree update: %s", hipGetErrorString(se));
+
+    // boundary conditions of the faces that were physical boundaries before
+    for (int s = 0; s < ns; s++) {
+        if (src[s] < 0) continue;
+        for (int f = 0; f < nnb; f++) {
+            const int rec = c->bc_of[(size_t)s * nnb + f];
+            const int orec = old_bc_of[(size_t)src[s] * nnb + f];
+            if (rec < 0 || orec < 0) continue;
+            BcRec &r = c->bc[rec];
+            r.type = old_bc[orec].type;
+            if (old_bc[orec].voff >= 0) {
+                const int m = dim == 2 ? c->slot_nc[s] : c->slot_nc[s] * c->slot_nc[s];
+                r.voff = (int)c->bc_vals.size();
+                c->bc_vals.insert(c->bc_vals.end(), old_bc_vals.begin() + old_bc[orec].voff,
+                                  old_bc_vals.begin() + old_bc[orec].voff + m);
+            }
+        }
+    }
+    c->bc_dirty = true;
+    if (int e = upload_bc(c)) return e;
+    // tile shapes and pass fusion of the levels that remain
+    for (Level &L : c->levels) {
+        auto it = old_level.find(L.lvl);
+        if (it == old_level.end() || it->second.nc != L.nc) continue;
+        for (int k = 0; k < 2; k++) {
+            L.tile[k] = it->second.tile[k];
+            L.pairs[k] = it->second.pairs[k];
+        }
+    }
     return 0;
 }
 

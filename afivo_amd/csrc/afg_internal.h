@@ -53,6 +53,14 @@ void launch_fill_value(int dim, hipStream_t s, const DevTree &T, int slot0, int 
 template <typename R>
 void launch_copy_boxes(int dim, hipStream_t s, const DevTree &T, int slot0, int n_slots, int nc,
                        const R *from, R *to);
+// tree update: to[slot] = from[src[slot]] (whole box arrays of `stride`
+// values), zeros where src[slot] < 0
+void launch_gather(hipStream_t s, int stride, int n_slots, const int32_t *src, const double *from,
+                   double *to);
+// a$D_prolong0/1/2 (order 0/1/2) of var from the parents of the n listed
+// slots (all with nc cells, their parents with pnc) into their interiors
+void launch_prolong_list(int dim, hipStream_t s, const DevTree &T, int nc, int pnc, const int32_t *slots,
+                         int n, int order, double *var);
 // n values of a pool to FP32, and back (optionally added to the FP64 values)
 void launch_to_fp32(hipStream_t s, long long n, const double *from, float *to);
 void launch_from_fp32(hipStream_t s, long long n, const float *from, double *to, int add);

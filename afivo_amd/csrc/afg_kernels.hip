@@ -178,6 +178,23 @@ __global__ __launch_bounds__(BLOCK) void k_prolong2(DevTree T, LevelArgs L, cons
         prolong2_body<DIM, NC>(T, L, from, to, add, gid);
 }
 
+// tree update: box arrays gathered into a new pool, and the prolongation of
+// the boxes the update added
+__global__ __launch_bounds__(BLOCK) void k_gather(int stride, int n_slots, const int32_t *src,
+                                                  const double *from, double *to)
+{
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < (long long)n_slots * (stride >> 1)) gather_body(stride, src, from, to, gid);
+}
+
+template <int DIM, int ORDER>
+__global__ __launch_bounds__(BLOCK) void k_prolong_list(DevTree T, int nc, int pnc, const int32_t *slots,
+                                                        int n, double *var)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < n * ipow(nc / 2, DIM)) prolong_list_body<DIM, ORDER>(T, nc, pnc, slots, var, gid);
+}
+
 // whole-array fill and copy over a contiguous slot range
 // (a$D_box_clear_cc, m_aX_utils.f90:262-270; a$D_box_copy_cc, :331-339)
 template <typename R>
@@ -593,6 +610,29 @@ void launch_copy_boxes(int dim, hipStream_t s, const DevTree &T, int slot0, int 
     const int len = ipow(nc + 2, dim);
     k_copy_boxes<R><<<n_blocks((long long)n_slots * len), BLOCK, 0, s>>>(T.stride, slot0, n_slots,
                                                                          len, from, to);
+}
+
+void launch_gather(hipStream_t s, int stride, int n_slots, const int32_t *src, const double *from,
+                   double *to)
+{
+    const long long th = (long long)n_slots * (stride >> 1);
+    if (th > 0) k_gather<<<n_blocks(th), BLOCK, 0, s>>>(stride, n_slots, src, from, to);
+}
+
+void launch_prolong_list(int dim, hipStream_t s, const DevTree &T, int nc, int pnc, const int32_t *slots,
+                         int n, int order, double *var)
+{
+    if (n <= 0) return;
+    const int g = n_blocks((long long)n * ipow(nc / 2, dim));
+    if (dim == 2) {
+        if (order == 0) k_prolong_list<2, 0><<<g, BLOCK, 0, s>>>(T, nc, pnc, slots, n, var);
+        else if (order == 1) k_prolong_list<2, 1><<<g, BLOCK, 0, s>>>(T, nc, pnc, slots, n, var);
+        else k_prolong_list<2, 2><<<g, BLOCK, 0, s>>>(T, nc, pnc, slots, n, var);
+    } else {
+        if (order == 0) k_prolong_list<3, 0><<<g, BLOCK, 0, s>>>(T, nc, pnc, slots, n, var);
+        else if (order == 1) k_prolong_list<3, 1><<<g, BLOCK, 0, s>>>(T, nc, pnc, slots, n, var);
+        else k_prolong_list<3, 2><<<g, BLOCK, 0, s>>>(T, nc, pnc, slots, n, var);
+    }
 }
 
 void launch_to_fp32(hipStream_t s, long long n, const double *from, float *to)

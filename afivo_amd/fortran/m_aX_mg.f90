@@ -124,6 +124,8 @@ module m_a$D_mg
   public :: mg$D_device_release
   public :: mg$D_mixed_solve
   public :: mg$D_autotune
+  public :: mg$D_device_update
+  public :: mg$D_to_device
 
   ! device context of this module and the structure it holds
   type(c_ptr), save :: ctx = c_null_ptr
@@ -238,6 +240,43 @@ contains
     call afg_check(afg_autotune(ctx, o, 0, 8, c_null_ptr), ctx, "mg$D_autotune")
     call afg_check(afg_autotune(ctx, o, 1, 8, c_null_ptr), ctx, "mg$D_autotune")
   end subroutine mg$D_autotune
+
+  !> After a$D_adjust_refinement of the tree (m_aX_core.f90:452-541): move
+  !> the device copy of the tree to the new structure without a round trip
+  !> through box%cc (afg_update_tree).  Boxes the update kept keep their
+  !> device values; the boxes it added get the variables ivs set from their
+  !> parents by a$D_prolong<order> (order 0, 1 or 2; m_aX_prolong.f90) and
+  !> zeros elsewhere.  For device-resident solves, where the device holds
+  !> the newest phi; set the new boxes' rhs on the host and send it with
+  !> mg$D_to_device.  No counterpart in the reference.
+  subroutine mg$D_device_update(tree, order, ivs)
+    type(a$D_t), intent(in) :: tree
+    integer, intent(in)     :: order
+    integer, intent(in)     :: ivs(:)
+    integer(c_int32_t), allocatable :: recs(:)
+
+    if (.not. c_associated(ctx)) return   ! nothing on the device: the next solve uploads
+    call build_records(tree, recs)
+    call afg_check(afg_update_tree(ctx, size(recs) / rec_len, recs, tree%dr_base, c_null_ptr, &
+         int(order, c_int), int(size(ivs), c_int), int(ivs, c_int32_t)), ctx, "mg$D_device_update")
+    recs_dev = recs
+    ids_dev = recs(1::rec_len)
+  end subroutine mg$D_device_update
+
+  !> Copy variable iv of every box from box%cc to the device (for instance
+  !> the rhs of boxes added by a refinement).  No counterpart in the
+  !> reference.
+  subroutine mg$D_to_device(tree, iv)
+    type(a$D_t), intent(in) :: tree
+    integer, intent(in)     :: iv
+    integer(c_int32_t), allocatable :: recs(:)
+
+    if (.not. c_associated(ctx)) return   ! nothing on the device: the next solve uploads
+    call build_records(tree, recs)
+    if (size(recs) /= size(recs_dev)) error stop "mg$D_to_device: tree changed (call mg$D_device_update first)"
+    if (any(recs /= recs_dev)) error stop "mg$D_to_device: tree changed (call mg$D_device_update first)"
+    call upload_var(tree, iv)
+  end subroutine mg$D_to_device
 
   !> Copy phi, rhs and tmp from the device into box%cc
   subroutine mg$D_to_host(tree, mg)

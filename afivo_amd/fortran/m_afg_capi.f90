@@ -42,6 +42,18 @@ module m_afg_capi
        real(c_double), value :: dr_base
      end function afg_set_tree
 
+     integer(c_int) function afg_update_tree(ctx, n_boxes, records, dr_base, tree_of, prolong_order, &
+          n_vars, vars) bind(c, name="afg_update_tree")
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: n_boxes
+       integer(c_int32_t), intent(in) :: records(*)
+       real(c_double), value :: dr_base
+       type(c_ptr), value :: tree_of
+       integer(c_int), value :: prolong_order, n_vars
+       integer(c_int32_t), intent(in) :: vars(*)
+     end function afg_update_tree
+
      integer(c_int) function afg_set_bc(ctx, id, nb, bc_type, b) bind(c, name="afg_set_bc")
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

@@ -46,12 +46,37 @@ class DeviceTree:
             self.tree_of = np.ascontiguousarray(tree_of, dtype=np.int32)
             check(L.afg_set_forest(self.h, self.records.shape[0], _i32(self.records), float(dr_base),
                                    _i32(self.tree_of)), self.h)
+        self.dr_base = float(dr_base)
+        self._tree_info()
+
+    def _tree_info(self):
+        L = _lib.load()
         lo, hi = ctypes.c_int(), ctypes.c_int()
         check(L.afg_levels(self.h, ctypes.byref(lo), ctypes.byref(hi)), self.h)
         self.min_lvl, self.highest_lvl = lo.value, hi.value
         n = ctypes.c_int64()
         check(L.afg_packed_size(self.h, self.ids.size, _i32(self.ids), ctypes.byref(n)), self.h)
         self.n_values = n.value
+
+    def update_tree(self, records, prolong_order=2, vars=(1,), tree_of=None, dr_base=None):
+        """Switch to the refined / derefined tree `records` keeping the data
+        on the device (afg_update_tree): kept boxes keep all their values,
+        new boxes get the variables `vars` prolonged from their parents
+        (a$D_prolong0/1/2 for prolong_order 0/1/2, -1 for none)."""
+        recs = np.ascontiguousarray(records, dtype=np.int32)
+        vs = np.ascontiguousarray(list(vars), dtype=np.int32)
+        t_of = None if tree_of is None else np.ascontiguousarray(tree_of, dtype=np.int32)
+        dr = self.dr_base if dr_base is None else float(dr_base)
+        rc = _lib.load().afg_update_tree(self.h, recs.shape[0], _i32(recs), dr,
+                                          None if t_of is None else _i32(t_of), int(prolong_order),
+                                          vs.size, _i32(vs))
+        self._c(rc)
+        self.records = recs
+        self.ids = np.ascontiguousarray(recs[:, 0])
+        if t_of is not None:
+            self.tree_of = t_of
+        self.dr_base = dr
+        self._tree_info()
 
     @classmethod
     def from_state(cls, st: TreeState, n_var=None, tree_of=None):

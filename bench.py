@@ -45,10 +45,11 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all cores up to 16")
     ap.add_argument("--no-timing", action="store_true",
                     help="skip the per-kernel event timing (roofline is then null)")
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "update"], default="c2",
                     help="c2: the 3D FP64 tree (default); c3: batched 2D FP64 forest of --trees "
                          "trees; c4: 3D variable-eps tree, mixed precision against FP64; "
-                         "c5: served request trace")
+                         "c5: served request trace; update: refinement steps of the c2 tree "
+                         "on the device")
     ap.add_argument("--requests", type=int, default=64, help="c5: number of requests")
     ap.add_argument("--rate", type=float, default=100.0, help="c5: arrival rate (requests/s)")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
@@ -385,6 +386,98 @@ def bench_c5(a):
     return 0
 
 
+def bench_update(a):
+    """Tree updates: the 3D benchmark tree (phi, rhs, tmp on the device)
+    refined around one point and derefined elsewhere, and back, alternately
+    (afg_update_tree with a$D_prolong2 of phi into the added boxes), against
+    the round trip a program without it makes: download phi, rhs and tmp,
+    set the new tree, upload the three variables (the host-side refinement
+    and prolongation not counted)."""
+    from afivo_amd import _lib
+    from afivo_amd import mg as M
+    from afivo_amd import problems as P
+    from afivo_amd import tree as T
+    t0 = time.time()
+    p = P.c2(seed=a.seed)
+    st = p.state
+    recs_a = st.recs.copy()
+    c0 = np.array([0.31, 0.62, 0.45])
+
+    def ref(view, cur):
+        out = np.array(cur, dtype=np.int64, copy=True) if np.ndim(cur) else np.full(view.ids.size, cur)
+        mid = view.r_min + 0.5 * view.n_cell[:, None] * view.dr[:, None]
+        d = np.sqrt(((mid - c0) ** 2).sum(axis=1))
+        out[(d < 0.12) & (view.lvl < 8)] = T.REFINE
+        out[(d > 0.3) & (view.lvl > 4)] = T.DEREFINE
+        return out
+
+    n_add = p.tree.adjust_refinement(ref)
+    recs_b = p.tree.records()
+    build_s = time.time() - t0
+    ka = {tuple(r[1:5]) for r in recs_a}
+    kb = {tuple(r[1:5]) for r in recs_b}
+    dt = M.DeviceTree.from_state(st)
+    stride = (st.nc + 2) ** 3
+    seq = [recs_b, recs_a] * (a.warmup + a.steps)
+    times = []
+    for k, recs in enumerate(seq):
+        dt.sync()
+        s0 = time.perf_counter()
+        dt.update_tree(recs, prolong_order=2, vars=(1,))
+        dt.sync()
+        if k >= 2 * a.warmup:
+            times.append(time.perf_counter() - s0)
+    ms_update = 1e3 * float(np.median(times))
+    # the round trip without a device-side update
+    L = _lib.load()
+    trip = []
+    for k, recs in enumerate(seq[:2 * (1 + min(a.steps, 5))]):
+        dt.sync()
+        s0 = time.perf_counter()
+        data = [dt.download(iv) for iv in (1, 2, 3)]
+        recs_c = np.ascontiguousarray(recs, dtype=np.int32)
+        M.check(L.afg_set_tree(dt.h, recs_c.shape[0], M._i32(recs_c), float(st.dr_base)), dt.h)
+        dt.records, dt.ids = recs_c, np.ascontiguousarray(recs_c[:, 0])
+        dt._tree_info()
+        new_vals = dt.n_values
+        for iv in (1, 2, 3):
+            # the host would upload the new tree's arrays; same byte count
+            dt.upload(iv, np.resize(data[iv - 1], new_vals))
+        dt.sync()
+        if k >= 2:
+            trip.append(time.perf_counter() - s0)
+    ms_trip = 1e3 * float(np.median(trip))
+    dt.close()
+    n_b, n_a = len(recs_b), len(recs_a)
+    kept = len(ka & kb)
+    # algorithmic bytes of one update A -> B: every variable's pool gathered
+    # (kept boxes read, all boxes written), plus phi of the added boxes
+    gather_bytes = 3 * 8 * stride * (kept + n_b)
+    out = {
+        "metric": "tree updates/s (device-side refinement step)",
+        "value": 1e3 / ms_update, "unit": "updates/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms_update,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {
+            "workload": "tree update of the 3D benchmark tree (8^3 boxes, %d boxes): leaves within 0.12 "
+                        "of (0.31, 0.62, 0.45) refined (below level 8), leaves beyond 0.3 above level 4 "
+                        "derefined, %d boxes after (%d added, %d removed), and back; phi, rhs, tmp kept, "
+                        "phi prolonged (a$D_prolong2) into the added boxes" % (n_a, n_b, n_add, n_a - kept),
+            "boxes": [n_a, n_b], "kept": kept,
+        },
+        "update_ms": ms_update,
+        "round_trip_ms": ms_trip,
+        "speedup_vs_round_trip": ms_trip / ms_update,
+        "gather_bytes_per_update": gather_bytes,
+        "host_tree_build_s": build_s,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -397,6 +490,8 @@ def main():
         return bench_c5(a)
     if a.workload == "c4":
         return bench_c4(a)
+    if a.workload == "update":
+        return bench_update(a)
 
     from afivo_amd import mg as M
     from afivo_amd import problems as P

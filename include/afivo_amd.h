@@ -97,6 +97,25 @@ int afg_set_tree(afg_ctx *ctx, int n_boxes, const int32_t *records, double dr_ba
  * reference: afivo solves one tree per call. */
 int afg_set_forest(afg_ctx *ctx, int n_boxes, const int32_t *records, double dr_base,
                    const int32_t *tree_of);
+/* Replace the tree with its refined / derefined successor, keeping the data on
+ * the device: the data side of a$D_adjust_refinement (m_aX_core.f90:452-541)
+ * followed by the prolongation of the new boxes that afivo's programs do
+ * (e.g. examples/random_refinement_2d.f90:121-144).  records / dr_base /
+ * tree_of describe the new tree as for afg_set_forest (tree_of may be NULL
+ * for one tree; tree indices must denote the same trees as before).  A box
+ * of the new tree that the old tree also had (same tree, level and ix; its
+ * id may differ) keeps every value of every variable, ghost cells included.
+ * A box the update added gets the n_vars variables `vars` set from its
+ * parent, lowest level first, with a$D_prolong0 / a$D_prolong1 /
+ * a$D_prolong2 (prolong_order 0 / 1 / 2, m_aX_prolong.f90:39-348; -1: none)
+ * on its interior; its other values are zero, so call afg_fill_gc before
+ * using its ghost cells.  Boundary conditions set with afg_set_bc carry over
+ * to the faces of kept boxes; new boundary faces get Dirichlet zero.  Tile
+ * shapes and pass fusion chosen for a level carry over; FP32 copies, graphs
+ * and mixed-precision state are reset.  On failure the context has no tree
+ * (as after a failed afg_set_forest). */
+int afg_update_tree(afg_ctx *ctx, int n_boxes, const int32_t *records, double dr_base,
+                    const int32_t *tree_of, int prolong_order, int n_vars, const int32_t *vars);
 /* Number of trees and each tree's highest level (highest may be NULL). */
 int afg_forest_trees(const afg_ctx *ctx, int *n_trees, int32_t *highest);
 /* Per tree: max |variable iv| over its leaf interiors, i.e. max(|min|, |max|)

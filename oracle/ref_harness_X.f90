@@ -51,10 +51,23 @@ program ref_harness_$Dd
   logical            :: dump_fmg = .false.
   logical            :: dump_output = .true.
   integer            :: n_time = 0
+  ! refinement step after the solve (0: none): boxes refined below
+  ! refine2_lmax where dr^2 max|rhs| > refine2_thr, flagged for removal where
+  ! it is below refine2_rm_thr; then the variables 1..refine2_nvar of each
+  ! new box set from its parent with a$D_prolong1 (refine2 = 1),
+  ! a$D_prolong2 (2) or a$D_prolong0 (3), level by level (as
+  ! examples/random_refinement_2d.f90:121-144 does), and the tree written to
+  ! out_file.ref2
+  integer            :: refine2 = 0
+  integer            :: refine2_lmax = 0
+  real(dp)           :: refine2_thr = 1.0_dp
+  real(dp)           :: refine2_rm_thr = -1.0_dp
+  integer            :: refine2_nvar = 1
 
   namelist /cfg/ nc, ref_mode, l_uniform, l_max, thr, n_gauss, ampl, sigma, &
        r0, list_file, n_down, n_up, n_base, n_fmg, n_vcycle, use_eps, eps_hi, &
-       eps_cut, out_file, dump_input, dump_fmg, dump_output, n_time
+       eps_cut, out_file, dump_input, dump_fmg, dump_output, n_time, refine2, &
+       refine2_lmax, refine2_thr, refine2_rm_thr, refine2_nvar
 
   type(a$D_t)         :: tree, tree_init
   type(ref_info_t)    :: ref_info
@@ -118,6 +131,12 @@ program ref_harness_$Dd
   call solve(.true.)
 
   if (dump_output) call write_dump(trim(out_file) // '.out')
+
+  if (refine2 > 0) then
+     call a$D_adjust_refinement(tree, ref_second, ref_info)
+     call prolong_new_boxes()
+     call write_dump(trim(out_file) // '.ref2')
+  end if
 
   open(newunit=u, file=trim(out_file) // '.res', status='replace', &
        action='write')
@@ -211,6 +230,70 @@ contains
 #endif
     if (dr2 * rmax > thr) ref_flag = a5_do_ref
   end subroutine ref_gauss
+
+  !> The refinement rule of the step after the solve (see refine2_*)
+  subroutine ref_second(boxes, id, ref_flag)
+    type(box$D_t), intent(in) :: boxes(:)
+    integer, intent(in)       :: id
+    integer, intent(inout)    :: ref_flag
+    integer                   :: i, j, nc_b
+#if $D == 3
+    integer                   :: k
+#endif
+    real(dp)                  :: dr2, rmax
+
+    if (boxes(id)%lvl < l_uniform) then
+       ref_flag = a5_do_ref
+       return
+    end if
+    nc_b = boxes(id)%n_cell
+    dr2 = boxes(id)%dr**2
+    rmax = 0
+#if $D == 2
+    do j = 1, nc_b
+       do i = 1, nc_b
+          rmax = max(rmax, abs(gauss_lpl(gs, a$D_r_cc(boxes(id), [i, j]))))
+       end do
+    end do
+#elif $D == 3
+    do k = 1, nc_b
+       do j = 1, nc_b
+          do i = 1, nc_b
+             rmax = max(rmax, abs(gauss_lpl(gs, &
+                  a$D_r_cc(boxes(id), [i, j, k]))))
+          end do
+       end do
+    end do
+#endif
+    if (boxes(id)%lvl < refine2_lmax .and. dr2 * rmax > refine2_thr) then
+       ref_flag = a5_do_ref
+    else if (dr2 * rmax < refine2_rm_thr) then
+       ref_flag = a5_rm_ref
+    end if
+  end subroutine ref_second
+
+  !> Variables 1..refine2_nvar of every box added by the step after the
+  !> solve, from its parent, lowest level first
+  subroutine prolong_new_boxes()
+    use m_a$D_prolong
+    integer :: lvl, i, id, iv
+
+    do lvl = lbound(ref_info%lvls, 1), ubound(ref_info%lvls, 1)
+       if (.not. allocated(ref_info%lvls(lvl)%add)) cycle
+       do i = 1, size(ref_info%lvls(lvl)%add)
+          id = ref_info%lvls(lvl)%add(i)
+          do iv = 1, refine2_nvar
+             if (refine2 == 1) then
+                call a$D_prolong1(tree%boxes(tree%boxes(id)%parent), tree%boxes(id), iv)
+             else if (refine2 == 2) then
+                call a$D_prolong2(tree%boxes(tree%boxes(id)%parent), tree%boxes(id), iv)
+             else
+                call a$D_prolong0(tree%boxes(tree%boxes(id)%parent), tree%boxes(id), iv)
+             end if
+          end do
+       end do
+    end do
+  end subroutine prolong_new_boxes
 
   !> Refine exactly the boxes listed in list_file (sorted by lvl, ix($D) ..
   !> ix(1), lexicographically).

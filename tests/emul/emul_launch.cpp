@@ -344,6 +344,34 @@ void launch_copy_boxes(int dim, hipStream_t s, const DevTree &T, int slot0, int 
     });
 }
 
+void launch_gather(hipStream_t s, int stride, int n_slots, const int32_t *src, const double *from,
+                   double *to)
+{
+    const long long th = (long long)n_slots * (stride >> 1);
+    if (th > 0)
+        emul_submit(s, [=] { for (long long g = 0; g < th; g++) gather_body(stride, src, from, to, g); });
+}
+
+void launch_prolong_list(int dim, hipStream_t s, const DevTree &T, int nc, int pnc, const int32_t *slots,
+                         int n, int order, double *var)
+{
+    if (n <= 0) return;
+    emul_submit(s, [=] {
+        const int th = n * ipow(nc / 2, dim);
+        for (int g = 0; g < th; g++) {
+            if (dim == 2) {
+                if (order == 0) prolong_list_body<2, 0>(T, nc, pnc, slots, var, g);
+                else if (order == 1) prolong_list_body<2, 1>(T, nc, pnc, slots, var, g);
+                else prolong_list_body<2, 2>(T, nc, pnc, slots, var, g);
+            } else {
+                if (order == 0) prolong_list_body<3, 0>(T, nc, pnc, slots, var, g);
+                else if (order == 1) prolong_list_body<3, 1>(T, nc, pnc, slots, var, g);
+                else prolong_list_body<3, 2>(T, nc, pnc, slots, var, g);
+            }
+        }
+    });
+}
+
 void launch_to_fp32(hipStream_t s, long long n, const double *from, float *to)
 {
     if (n > 0) emul_submit(s, [=] { for (long long i = 0; i < n; i++) to[i] = (float)from[i]; });

@@ -5,7 +5,16 @@ reference sources by `make -C oracle ref`) on small problems and stores, per
 case, the reference's own input dump (tree + initial cell data), output dump
 (after FMG and V-cycles) and residual history.  Fixtures are data only.
 
-Usage: python tests/golden/make_golden.py
+Tree updates (the UPDATES cases): the solved tree of a golden case is
+refined / derefined once more by the harness's refine2 step
+(a$D_adjust_refinement, m_aX_core.f90:452-541, then the variables 1..2 of
+every added box prolonged from its parent with a$D_prolong0/1/2).  Stored per
+case (`<name>.npz`, no pickles): the new tree's records, the record indices
+of the added boxes and the interiors of their prolonged variables.  The
+script checks that the reference left every kept box's values unchanged and
+that the solved tree equals the committed fixture of the base case.
+
+Usage: python tests/golden/make_golden.py [case ...]   (default: all)
 """
 import gzip
 import json
@@ -14,9 +23,12 @@ import shutil
 import sys
 import tempfile
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, ROOT)
 
 import oracle  # noqa: E402
 
@@ -37,8 +49,68 @@ CASES = {
 }
 
 
-def main():
+# name: (base case, refine2 namelist values); box counts as the reference makes them
+UPDATES = {
+    # 86 -> 118 boxes: 32 added, a$D_prolong1
+    "update_g2d_nc4_p1": ("g2d_nc4", dict(refine2=1, refine2_lmax=6, refine2_thr=0.5)),
+    # 86 -> 54 boxes: 32 removed
+    "update_g2d_nc4_rm": ("g2d_nc4", dict(refine2=1, refine2_lmax=6, refine2_thr=2.0)),
+    # 55 -> 87 boxes: 32 added, a$D_prolong0
+    "update_g2d_nc8_guess_p0": ("g2d_nc8_guess", dict(refine2=3, refine2_lmax=5, refine2_thr=0.5)),
+    # 74 -> 66 boxes: 16 removed, 8 added, a$D_prolong2
+    "update_g3d_nc4_p2": ("g3d_nc4", dict(refine2=2, refine2_lmax=4, refine2_thr=2.0)),
+    # 74 -> 130 boxes: 56 added, a$D_prolong1
+    "update_g3d_nc4_p1": ("g3d_nc4", dict(refine2=1, refine2_lmax=4, refine2_thr=0.5)),
+}
+UPDATE_NVAR = 2
+PROLONG_ORDER = {1: 1, 2: 2, 3: 0}   # refine2 value -> order of a$D_prolong<order>
+
+
+def box_keys(st):
+    return {(int(st.lvl[r]),) + tuple(int(x) for x in st.ix[r]): r for r in range(st.n_boxes)}
+
+
+def make_update(name, base, extra):
+    from afivo_amd.treefile import read_tree_file
+    g = json.load(open(os.path.join(HERE, base + ".json")))
+    dim = g["dim"]
+    cfg = dict(g["cfg"], refine2_rm_thr=1e9, refine2_nvar=UPDATE_NVAR, **extra)
+    work = tempfile.mkdtemp(prefix="afg_golden_")
+    prefix, _ = oracle.run_ref(dim, cfg, workdir=work)
+    old, new = read_tree_file(prefix + ".out"), read_tree_file(prefix + ".ref2")
+    with gzip.open(os.path.join(HERE, base + ".out.gz")) as f:
+        assert f.read() == open(prefix + ".out", "rb").read(), "solved tree differs from the base fixture"
+    ko, kn = box_keys(old), box_keys(new)
+    oo, on = old.offsets(), new.offsets()
+    for key, r in kn.items():
+        if key in ko:
+            ro = ko[key]
+            for iv in range(old.n_var):
+                a = old.data[iv][oo[ro]:oo[ro + 1]]
+                b = new.data[iv][on[r]:on[r + 1]]
+                assert a.tobytes() == b.tobytes(), f"kept box {key} changed (var {iv + 1})"
+    added = np.array(sorted(r for key, r in kn.items() if key not in ko), dtype=np.int32)
+    inner = (slice(1, -1),) * dim
+    vals = np.zeros((UPDATE_NVAR, len(added), new.nc ** dim))
+    for iv in range(1, UPDATE_NVAR + 1):
+        for k, r in enumerate(added):
+            vals[iv - 1, k] = new.box_array(iv, int(r))[inner].ravel()
+    meta = dict(base=base, dim=dim, cfg=cfg, prolong_order=PROLONG_ORDER[cfg["refine2"]],
+                n_vars=UPDATE_NVAR, n_old=old.n_boxes, n_new=new.n_boxes,
+                n_removed=len(set(ko) - set(kn)), n_added=len(added))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), recs=new.recs, dr_base=np.float64(new.dr_base),
+                        added=added, vals=vals, meta=np.array(json.dumps(meta, sort_keys=True)))
+    shutil.rmtree(work)
+    print("wrote", name, {k: meta[k] for k in ("n_old", "n_new", "n_removed", "n_added")})
+
+
+def main(names):
+    for name, (base, extra) in UPDATES.items():
+        if not names or name in names:
+            make_update(name, base, extra)
     for name, (dim, cfg) in CASES.items():
+        if names and name not in names:
+            continue
         work = tempfile.mkdtemp(prefix="afg_golden_")
         cfg = dict(cfg, ref_mode="gauss", out_file=name)
         prefix, _ = oracle.run_ref(dim, cfg, workdir=work)
@@ -53,4 +125,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

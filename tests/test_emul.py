@@ -56,3 +56,8 @@ from test_mixed import (  # noqa: E402,F401
     test_fp32_multigrid_tracks_fp64,
     test_mixed_solve_matches_fp64_solution,
 )
+from test_tree_update import (  # noqa: E402,F401
+    test_update_full_size_properties,
+    test_update_matches_boxes_not_ids,
+    test_update_matches_reference,
+)

@@ -114,3 +114,24 @@ def test_dropin_extensions_autotune_and_mixed_precision():
     assert float(vals["rel_res"]) <= 1e-10
     assert int(vals["fp64_vcycles"]) >= 2
     assert float(vals["phi_diff"]) <= 1e-9
+
+
+UPDATE = os.path.join(BUILD, "update_2d")
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(UPDATE), reason="drop-in test program absent")
+def test_dropin_device_update_matches_host_refinement():
+    """mg2_device_update from Fortran (tests/fortran/update_2d.f90): one
+    refinement step of a device-resident solve gives, bit for bit, what
+    afivo's own step on the host gives (kept boxes unchanged, added boxes
+    a$D_prolong1 of phi), and the solve continues on the new tree."""
+    p = subprocess.run([UPDATE], capture_output=True, text=True, timeout=600,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+    assert p.returncode == 0, p.stdout + p.stderr
+    w = p.stdout.split()
+    vals = dict(zip(w[0::2], w[1::2]))
+    assert int(vals["added"]) > 0 and int(vals["after"]) != int(vals["before"])
+    assert int(vals["kept_bad"]) == 0
+    assert int(vals["new_bad"]) == 0
+    assert float(vals["rel_res"]) < 1e-3

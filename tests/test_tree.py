@@ -113,3 +113,35 @@ def test_morton_ids_and_periodic_base():
     top = st.lvl == st.highest_lvl
     assert (st.neighbors[top][:, :2] > 0).all()
     assert (st.neighbors[top][:, 2:] != 0).all()
+
+
+UPDATES = sorted(f[:-4] for f in os.listdir(os.path.join(os.path.dirname(__file__), "golden"))
+                 if f.startswith("update_") and f.endswith(".npz"))
+
+
+@pytest.mark.parametrize("name", UPDATES)
+def test_adjust_refinement_reproduces_reference_update(name):
+    """A solved golden tree refined and derefined once more (the reference's
+    own a$D_adjust_refinement, fixture update_*.npz): the builder's pass with
+    the same rule (refine below refine2_lmax where dr^2 max|lpl| >
+    refine2_thr, else derefine, below l_uniform always refine) gives the
+    same boxes and links."""
+    import json
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", name + ".npz"))
+    meta = json.loads(str(z["meta"]))
+    cfg, _, st_out, _ = load_golden(meta["base"])
+    c = meta["cfg"]
+    gs = gauss_of(cfg)
+    p = P.make_problem(name, cfg["dim"], c["nc"], gs, c["l_uniform"], c["l_max"], c["thr"],
+                       c["n_base"], 0)
+    assert keyed(p.state) == keyed(st_out)
+    grow = P.gauss_refinement(gs, c["l_uniform"], c["refine2_lmax"], c["refine2_thr"])
+
+    def ref(view, cur):
+        out = grow(view, T.DEREFINE)
+        out[view.lvl < c["l_uniform"]] = T.REFINE
+        return out
+
+    p.tree.adjust_refinement(ref)
+    st_new = TreeState(cfg["dim"], c["nc"], 0, 0, 0, 1.0, z["recs"], [])
+    assert keyed(p.tree.state()) == keyed(st_new)
