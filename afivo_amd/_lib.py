@@ -42,6 +42,14 @@ SIGNATURES = {
     "afg_sync": (ctypes.c_int, [P]),
     "afg_set_tree": (ctypes.c_int, [P, ctypes.c_int, I32P, F64]),
     "afg_set_forest": (ctypes.c_int, [P, ctypes.c_int, I32P, F64, I32P]),
+    "afg_gc_tree": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "afg_gc_level": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "afg_restrict_tree": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
+    "afg_prolong_level": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "afg_add_cc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
+    "afg_sub_cc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
+    "afg_times_cc": (ctypes.c_int, [P, F64, ctypes.c_int]),
+    "afg_lincomb_cc": (ctypes.c_int, [P, F64, ctypes.c_int, F64, ctypes.c_int]),
     "afg_update_tree": (ctypes.c_int, [P, ctypes.c_int, I32P, F64, I32P, ctypes.c_int, ctypes.c_int, I32P]),
     "afg_forest_trees": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), I32P]),
     "afg_forest_max_abs": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, F64P]),

@@ -1107,30 +1107,27 @@ AFG_DEV void gather_body(int stride, const int32_t *src, const double *from, dou
 }
 
 // a$D_prolong0 / a$D_prolong1 / a$D_prolong2 (m_aX_prolong.f90:39-108,
-// 130-219, 243-348) of a variable from each listed box's parent into the
-// box's interior.  Orders 1 and 2 zero the interior and add, so the stored
-// value is expr + 0; order 0 stores the parent's value.  Order 1 sums
-// f0 + f_x + f_y (+ f_z) left to right, f0 = 0.5 (2D) or 0.25 (3D) times the
-// parent's cell and f_x.. = 0.25 times the neighbour towards the child cell.
-// One thread per coarse cell of the child's octant.
+// 130-219, 243-348) from the parent of box `slot` (values `from`) into the
+// box's interior (values `to`), for coarse cell c of the box's octant.
+// add = 0: orders 1 and 2 zero the interior and add, so the stored value is
+// expr + 0, and order 0 stores the parent's value; add = 1: old + value
+// (order 0: old + parent).  Order 1 sums f0 + f_x + f_y (+ f_z) + old left to
+// right, f0 = 0.5 (2D) or 0.25 (3D) times the parent's cell and f_x.. = 0.25
+// times the neighbour towards the child cell.
 template <int DIM, int ORDER, typename R>
-AFG_DEV void prolong_list_body(const DevTree &T, int nc, int pnc, const int32_t *slots, R *var, int gid)
+AFG_DEV void prolong_cell(const DevTree &T, int nc, int pnc, int slot, const R *from, R *to, int add, int c)
 {
     const int hn = nc >> 1;
-    const int per_box = ipow(hn, DIM);
-    const int b = gid / per_box;
-    const int c = gid - b * per_box;
     const int ii = c % hn + 1;
     const int jj = (c / hn) % hn + 1;
     const int kk = DIM == 3 ? c / (hn * hn) + 1 : 0;
-    const int slot = slots[b];
     const int s = nc + 2;
     const int co = T.choff[slot];
     const int ps = pnc + 2;
     const int pq = ((co & 1023) + ii) +
                    ps * ((((co >> 10) & 1023) + jj) + ps * (DIM == 3 ? ((co >> 20) & 1023) + kk : 0));
-    const R *pb = var + (size_t)T.parent[slot] * T.stride;
-    R *cb = var + (size_t)slot * T.stride;
+    const R *pb = from + (size_t)T.parent[slot] * T.stride;
+    R *cb = to + (size_t)slot * T.stride;
     auto at = [&](int m) -> R & {
         const int fi = 2 * ii - 1 + (m & 1), fj = 2 * jj - 1 + ((m >> 1) & 1);
         const int fk = DIM == 3 ? 2 * kk - 1 + ((m >> 2) & 1) : 0;
@@ -1139,7 +1136,7 @@ AFG_DEV void prolong_list_body(const DevTree &T, int nc, int pnc, const int32_t 
     if constexpr (ORDER == 0) {
         const R v = pb[pq];
 #pragma unroll
-        for (int m = 0; m < (1 << DIM); m++) at(m) = v;
+        for (int m = 0; m < (1 << DIM); m++) at(m) = add ? at(m) + v : v;
     } else if constexpr (ORDER == 1) {
         const int step[3] = {1, ps, ps * ps};
         const R f0 = R(DIM == 2 ? 0.5 : 0.25) * pb[pq];
@@ -1154,13 +1151,130 @@ AFG_DEV void prolong_list_body(const DevTree &T, int nc, int pnc, const int32_t 
             R v = f0;
 #pragma unroll
             for (int d = 0; d < DIM; d++) v = v + (((m >> d) & 1) ? hi[d] : lo[d]);
-            at(m) = v + R(0);
+            at(m) = v + (add ? at(m) : R(0));
         }
     } else {
         const Prolong2<DIM, R> P2([&](int q) -> R { return pb[q]; }, pq, ps);
 #pragma unroll
-        for (int m = 0; m < (1 << DIM); m++) at(m) = P2.at(m) + R(0);
+        for (int m = 0; m < (1 << DIM); m++) at(m) = P2.at(m) + (add ? at(m) : R(0));
     }
+}
+
+// The boxes a tree update added, listed in `slots` (all with nc cells, their
+// parents with pnc): variable `var` prolonged from the parents, add = 0.
+template <int DIM, int ORDER, typename R>
+AFG_DEV void prolong_list_body(const DevTree &T, int nc, int pnc, const int32_t *slots, R *var, int gid)
+{
+    const int per_box = ipow(nc >> 1, DIM);
+    const int b = gid / per_box;
+    prolong_cell<DIM, ORDER>(T, nc, pnc, slots[b], (const R *)var, var, 0, gid - b * per_box);
+}
+
+// ---------------------------------------------------------------------------
+// ghost cells of any variable (a$D_gc_tree, m_aX_gc.f90:28-69) and whole-pool
+// arithmetic (a$D_box_add_cc .. a$D_box_lincomb_cc, m_aX_utils.f90:273-329)
+
+// One face ghost cell of a level's box, as fill_body, with the
+// refinement-boundary method rb and the physical boundary condition bc (0:
+// the boundary table; -1 / -2: a$D_bc_dirichlet_zero / a$D_bc_neumann_zero,
+// m_aX_gc.f90:361-379, i.e. that condition with b = 0 on every face).
+//   gc_interp: ghost = 0.5 c1 + c2/6 + f/3 (2D), c1/3 + c2/6 + c3/6 + f/3
+//   (3D), summed left to right, c1..c3 cells of the parent's neighbour next
+//   to the face (c1 the one over the ghost, c2 / c3 the tangential
+//   neighbours towards the ghost's sibling: -1 for odd, +1 for even cell
+//   indices), f the box's cell next to the ghost; the limited form caps the
+//   ghost at 2 c1.  Face z of 3D takes c2 along y and c3 along x, the other
+//   faces c2 along their first tangential axis.
+template <int DIM>
+AFG_DEV void gc_body(const DevTree &T, const LevelArgs &L, double *var, int rb, int bc, int gid)
+{
+    constexpr double sixth = 1 / 6.0, third = 1 / 3.0;
+    const int nc = L.nc;
+    const int per_face = DIM == 2 ? nc : nc * nc;
+    const int per_box = 2 * DIM * per_face;
+    const int b = gid / per_box;
+    const int r = gid - b * per_box;
+    const int f = r / per_face;
+    const int t = r - f * per_face;
+    const int t1 = t % nc + 1, t2 = DIM == 3 ? t / nc + 1 : 0;
+    const int dn = f >> 1, high = f & 1;
+    const int in = high ? nc : 1;
+    const int i = dn == 0 ? in : t1;
+    const int j = dn == 1 ? in : (dn == 0 ? t1 : (DIM == 3 ? t2 : 0));
+    const int k = DIM == 3 ? (dn == 2 ? in : t2) : 0;
+    const int slot = L.slot0 + b;
+    const int s = nc + 2;
+    double *box = var + (size_t)slot * T.stride;
+    const double center = box[i + s * (j + s * k)];
+    const int nbv = T.nb[slot * (2 * DIM) + f];
+    const int gi = dn == 0 ? (high ? nc + 1 : 0) : i;
+    const int gj = dn == 1 ? (high ? nc + 1 : 0) : j;
+    const int gk = dn == 2 ? (high ? nc + 1 : 0) : k;
+    double g;
+    if (nbv >= 0 || (nbv == NB_REFINE && rb == RB_MG) || (nbv <= NB_BC0 && bc == 0)) {
+        const LevelViewT<double> v{var, nullptr, nullptr, nullptr, 0, T.stride};
+        g = ghost_value<DIM>(T, v, v, L.pnc, L.dr, slot, f, nc, i, j, k, center);
+    } else if (nbv <= NB_BC0) {
+        double c1, c2;
+        if (bc == -1) {
+            c1 = -1;
+            c2 = 2;
+        } else {
+            c1 = 1;
+            c2 = L.dr * (high ? 1.0 : -1.0);
+        }
+        g = c2 * 0.0 + c1 * center;
+    } else {
+        const int co = T.choff[slot];
+        const int off[3] = {co & 1023, (co >> 10) & 1023, (co >> 20) & 1023};
+        const int ps = L.pnc + 2;
+        const int p = T.parent[slot];
+        if (rb == RB_PROLONG0) {
+            const int pi = off[0] + ((gi + 1) >> 1);
+            const int pj = off[1] + ((gj + 1) >> 1);
+            const int pk = DIM == 3 ? off[2] + ((gk + 1) >> 1) : 0;
+            g = var[(size_t)p * T.stride + pi + ps * (pj + ps * pk)];
+        } else {
+            const double *q = var + (size_t)T.nb[p * (2 * DIM) + f] * T.stride;
+            const int ixc = high ? 1 : L.pnc;
+            // tangential axes: a (first), bb (second); their cell indices
+            const int ta = dn == 0 ? 1 : 0, tb = dn == 2 ? 1 : 2;
+            const int ca = dn == 0 ? j : i, cb = dn == 2 ? j : k;
+            const int a1 = off[ta] + ((ca + 1) >> 1), a2 = a1 + 1 - 2 * (ca & 1);
+            auto cell = [&](int va, int vb) {
+                int x[3];
+                x[dn] = ixc;
+                x[ta] = va;
+                x[tb] = vb;
+                return q[x[0] + ps * (x[1] + ps * (DIM == 3 ? x[2] : 0))];
+            };
+            double c1;
+            if (DIM == 2) {
+                c1 = cell(a1, 0);
+                const double c2 = cell(a2, 0);
+                g = 0.5 * c1 + sixth * c2 + third * center;
+            } else {
+                const int b1 = off[tb] + ((cb + 1) >> 1), b2 = b1 + 1 - 2 * (cb & 1);
+                c1 = cell(a1, b1);
+                const double ca2 = cell(a2, b1), cb2 = cell(a1, b2);
+                // c2 along the first tangential axis and c3 along the second,
+                // except on z faces (c2 along y, c3 along x)
+                const double c2 = dn == 2 ? cb2 : ca2, c3 = dn == 2 ? ca2 : cb2;
+                g = third * c1 + sixth * c2 + sixth * c3 + third * center;
+            }
+            if (rb == RB_INTERP_LIM && g > 2 * c1) g = 2 * c1;
+        }
+    }
+    box[gi + s * (gj + s * gk)] = g;
+}
+
+// y = y + x, y = y - x, y = a y, y = a x + b y over whole pools (ARITH_*)
+AFG_DEV void arith_body(int op, double a, double b, const double *x, double *y, long long i)
+{
+    if (op == ARITH_ADD) y[i] = y[i] + x[i];
+    else if (op == ARITH_SUB) y[i] = y[i] - x[i];
+    else if (op == ARITH_TIMES) y[i] = a * y[i];
+    else y[i] = a * x[i] + b * y[i];
 }
 
 // ---------------------------------------------------------------------------

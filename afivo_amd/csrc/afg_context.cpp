@@ -125,6 +125,7 @@ struct afg_ctx {
     std::vector<int32_t> id2slot;        // -1 if absent
     std::vector<int32_t> slot2id, slot_lvl, slot_nc;
     std::vector<int32_t> slot_ix, slot_tree;   // box%ix (3 per slot) and tree index per slot
+    bool rb_parent_nb = true;            // every refinement-boundary face has a neighbour of the parent there
     std::vector<int32_t> h_nb;           // slot * nnb + face (afivo ids)
     std::vector<BcRec> bc;
     std::vector<int32_t> bc_of;          // slot * nnb + face -> bc record or -1
@@ -1388,6 +1389,12 @@ int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base,
             }
         }
     }
+    // a$D_gc_interp reads the parent's neighbour across a refinement boundary
+    c->rb_parent_nb = true;
+    for (int s = 0; s < slot; s++)
+        for (int f = 0; f < nnb; f++)
+            if (nbv[(size_t)s * nnb + f] == NB_REFINE && (par[s] < 0 || nbv[(size_t)par[s] * nnb + f] < 0))
+                c->rb_parent_nb = false;
     for (int li = 0; li < nl; li++) {
         Level &L = c->levels[li];
         L.parents_off = (int)lists.size();
@@ -2113,6 +2120,83 @@ int afg_update_coarse(afg_ctx *c, const afg_mg_opts *o, int lvl)
     set_tags(m);
     update_coarse(m, lvl, 0);
     return 0;
+}
+
+static int check_gc_args(afg_ctx *c, int rb, int bc)
+{
+    if (rb < RB_MG || rb > RB_INTERP_LIM) return c->fail(AFG_ERR_ARG, "unknown refinement-boundary method %d", rb);
+    if (bc != 0 && bc != AFG_BC_DIRICHLET && bc != AFG_BC_NEUMANN)
+        return c->fail(AFG_ERR_ARG, "unknown boundary condition %d", bc);
+    if ((rb == RB_INTERP || rb == RB_INTERP_LIM) && !c->rb_parent_nb)
+        return c->fail(AFG_ERR_ARG, "gc_interp: a refinement boundary without a neighbour of the parent");
+    return 0;
+}
+
+int afg_gc_level(afg_ctx *c, int iv, int lvl, int rb, int bc)
+{
+    if (int e = check_ctx(c)) return e;
+    if (int e = check_level(c, lvl, false)) return e;
+    if (int e = check_gc_args(c, rb, bc)) return e;
+    double *v = nullptr;
+    if (int e = ensure_var(c, iv, &v)) return e;
+    if (int e = upload_bc(c)) return e;
+    launch_gc(c->dim, c->stream, dev_tree(c), level_args(c, lvl), v, rb, bc);
+    return 0;
+}
+
+int afg_gc_tree(afg_ctx *c, int iv, int rb, int bc)
+{
+    if (int e = check_ctx(c)) return e;
+    if (int e = check_gc_args(c, rb, bc)) return e;
+    double *v = nullptr;
+    if (int e = ensure_var(c, iv, &v)) return e;
+    if (int e = upload_bc(c)) return e;
+    const DevTree T = dev_tree(c);
+    for (int lvl = c->min_lvl; lvl <= c->highest_lvl; lvl++)
+        launch_gc(c->dim, c->stream, T, level_args(c, lvl), v, rb, bc);
+    return 0;
+}
+
+int afg_restrict_tree(afg_ctx *c, int iv, int iv_to)
+{
+    if (int e = check_ctx(c)) return e;
+    double *a = nullptr, *b = nullptr;
+    if (int e = ensure_var(c, iv, &a)) return e;
+    if (int e = ensure_var(c, iv_to, &b)) return e;
+    const DevTree T = dev_tree(c);
+    for (int lvl = c->highest_lvl; lvl > c->min_lvl; lvl--)
+        launch_restrict(c->dim, c->stream, T, level_args(c, lvl), (const double *)a, b);
+    return 0;
+}
+
+int afg_prolong_level(afg_ctx *c, int iv, int iv_to, int lvl, int order, int add)
+{
+    if (int e = check_ctx(c)) return e;
+    if (int e = check_level(c, lvl, true)) return e;
+    if (order < 0 || order > 2) return c->fail(AFG_ERR_ARG, "prolongation order %d not in 0..2", order);
+    double *a = nullptr, *b = nullptr;
+    if (int e = ensure_var(c, iv, &a)) return e;
+    if (int e = ensure_var(c, iv_to, &b)) return e;
+    launch_prolong_level(c->dim, c->stream, dev_tree(c), level_args(c, lvl), a, b, order, add ? 1 : 0);
+    return 0;
+}
+
+static int pool_arith(afg_ctx *c, int op, double a, int iv_x, double b, int iv_y)
+{
+    if (int e = check_ctx(c)) return e;
+    double *x = nullptr, *y = nullptr;
+    if (int e = ensure_var(c, iv_x, &x)) return e;
+    if (int e = ensure_var(c, iv_y, &y)) return e;
+    launch_arith(c->stream, (long long)c->n_slots * c->stride, op, a, b, x, y);
+    return 0;
+}
+
+int afg_add_cc(afg_ctx *c, int iv_from, int iv_to) { return pool_arith(c, ARITH_ADD, 0, iv_from, 0, iv_to); }
+int afg_sub_cc(afg_ctx *c, int iv_from, int iv_to) { return pool_arith(c, ARITH_SUB, 0, iv_from, 0, iv_to); }
+int afg_times_cc(afg_ctx *c, double a, int iv) { return pool_arith(c, ARITH_TIMES, a, iv, 0, iv); }
+int afg_lincomb_cc(afg_ctx *c, double a, int iv_a, double b, int iv_b)
+{
+    return pool_arith(c, ARITH_LINCOMB, a, iv_a, b, iv_b);
 }
 
 int afg_set_bottom_cells(afg_ctx *c, int64_t cells, int64_t wave_cells)

@@ -61,6 +61,16 @@ void launch_gather(hipStream_t s, int stride, int n_slots, const int32_t *src, c
 // slots (all with nc cells, their parents with pnc) into their interiors
 void launch_prolong_list(int dim, hipStream_t s, const DevTree &T, int nc, int pnc, const int32_t *slots,
                          int n, int order, double *var);
+// a$D_prolong0/1/2 (order 0/1/2) of `from` into every box of level L
+// (`to`), added to the old values when add
+void launch_prolong_level(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const double *from,
+                          double *to, int order, int add);
+// face ghosts of every box of level L with refinement-boundary method rb
+// (RB_*) and boundary condition bc (0: the table; -1 / -2: Dirichlet /
+// Neumann zero)
+void launch_gc(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, double *var, int rb, int bc);
+// y = y + x, y - x, a y or a x + b y (ARITH_*) over n values
+void launch_arith(hipStream_t s, long long n, int op, double a, double b, const double *x, double *y);
 // n values of a pool to FP32, and back (optionally added to the FP64 values)
 void launch_to_fp32(hipStream_t s, long long n, const double *from, float *to);
 void launch_from_fp32(hipStream_t s, long long n, const float *from, double *to, int add);

@@ -195,6 +195,32 @@ __global__ __launch_bounds__(BLOCK) void k_prolong_list(DevTree T, int nc, int p
     if (gid < n * ipow(nc / 2, DIM)) prolong_list_body<DIM, ORDER>(T, nc, pnc, slots, var, gid);
 }
 
+template <int DIM, int ORDER>
+__global__ __launch_bounds__(BLOCK) void k_prolong_level(DevTree T, LevelArgs L, const double *from, double *to,
+                                                         int add)
+{
+    const int per_box = ipow(L.nc / 2, DIM);
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= L.n * per_box) return;
+    const int b = gid / per_box;
+    prolong_cell<DIM, ORDER>(T, L.nc, L.pnc, L.slot0 + b, from, to, add, gid - b * per_box);
+}
+
+// ghost cells of any variable with a choice of methods, and pool arithmetic
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_gc(DevTree T, LevelArgs L, double *var, int rb, int bc)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < fill_threads<DIM>(L.n, L.nc)) gc_body<DIM>(T, L, var, rb, bc, gid);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_arith(long long n, int op, double a, double b, const double *x,
+                                                 double *y)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) arith_body(op, a, b, x, y, i);
+}
+
 // whole-array fill and copy over a contiguous slot range
 // (a$D_box_clear_cc, m_aX_utils.f90:262-270; a$D_box_copy_cc, :331-339)
 template <typename R>
@@ -633,6 +659,34 @@ void launch_prolong_list(int dim, hipStream_t s, const DevTree &T, int nc, int p
         else if (order == 1) k_prolong_list<3, 1><<<g, BLOCK, 0, s>>>(T, nc, pnc, slots, n, var);
         else k_prolong_list<3, 2><<<g, BLOCK, 0, s>>>(T, nc, pnc, slots, n, var);
     }
+}
+
+void launch_prolong_level(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const double *from,
+                          double *to, int order, int add)
+{
+    if (L.n <= 0) return;
+    const int g = n_blocks((long long)L.n * ipow(L.nc / 2, dim));
+    if (dim == 2) {
+        if (order == 0) k_prolong_level<2, 0><<<g, BLOCK, 0, s>>>(T, L, from, to, add);
+        else if (order == 1) k_prolong_level<2, 1><<<g, BLOCK, 0, s>>>(T, L, from, to, add);
+        else k_prolong_level<2, 2><<<g, BLOCK, 0, s>>>(T, L, from, to, add);
+    } else {
+        if (order == 0) k_prolong_level<3, 0><<<g, BLOCK, 0, s>>>(T, L, from, to, add);
+        else if (order == 1) k_prolong_level<3, 1><<<g, BLOCK, 0, s>>>(T, L, from, to, add);
+        else k_prolong_level<3, 2><<<g, BLOCK, 0, s>>>(T, L, from, to, add);
+    }
+}
+
+void launch_gc(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, double *var, int rb, int bc)
+{
+    if (L.n <= 0) return;
+    if (dim == 2) k_gc<2><<<n_blocks(fill_threads<2>(L.n, L.nc)), BLOCK, 0, s>>>(T, L, var, rb, bc);
+    else k_gc<3><<<n_blocks(fill_threads<3>(L.n, L.nc)), BLOCK, 0, s>>>(T, L, var, rb, bc);
+}
+
+void launch_arith(hipStream_t s, long long n, int op, double a, double b, const double *x, double *y)
+{
+    if (n > 0) k_arith<<<n_blocks(n), BLOCK, 0, s>>>(n, op, a, b, x, y);
 }
 
 void launch_to_fp32(hipStream_t s, long long n, const double *from, float *to)

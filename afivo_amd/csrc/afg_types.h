@@ -21,6 +21,17 @@ constexpr int8_t TAG_NORMAL = 1;
 constexpr int8_t TAG_CEPS = 3;
 constexpr int8_t TAG_VEPS = 4;
 
+// Refinement-boundary methods of the general ghost fill (subr_rb of
+// a$D_gc_tree, m_aX_gc.f90:28-69).
+constexpr int RB_MG = 0;           // mg$D_sides_rb (m_aX_mg.f90:262-350)
+constexpr int RB_PROLONG0 = 1;     // a$D_gc_prolong0 (m_aX_gc.f90:135-151)
+constexpr int RB_INTERP = 2;       // a$D_gc_interp (m_aX_gc.f90:155-251)
+constexpr int RB_INTERP_LIM = 3;   // a$D_gc_interp_lim (m_aX_gc.f90:254-358)
+
+// Whole-pool arithmetic: y = y + x, y - x, a y, a x + b y
+// (a$D_box_add_cc, _sub_cc, _times_cc, _lincomb_cc, m_aX_utils.f90:273-318).
+constexpr int ARITH_ADD = 0, ARITH_SUB = 1, ARITH_TIMES = 2, ARITH_LINCOMB = 3;
+
 // Device topology of one tree.  Slots of a level are contiguous and in Morton
 // order; every box array of a variable occupies `stride` doubles at
 // var + slot * stride and holds cc(0:nc_l+1, ...) in Fortran order.

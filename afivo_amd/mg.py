@@ -58,6 +58,34 @@ class DeviceTree:
         check(L.afg_packed_size(self.h, self.ids.size, _i32(self.ids), ctypes.byref(n)), self.h)
         self.n_values = n.value
 
+    # ghost cells and transfers of any variable
+    def gc_tree(self, iv, rb=0, bc=0):
+        """a$D_gc_tree: rb 0 mg sides_rb, 1 gc_prolong0, 2 gc_interp, 3
+        gc_interp_lim; bc 0 the boundary table, AFG_BC_DIRICHLET /
+        AFG_BC_NEUMANN that condition with b = 0."""
+        self._c(_lib.load().afg_gc_tree(self.h, int(iv), int(rb), int(bc)))
+
+    def gc_level(self, iv, lvl, rb=0, bc=0):
+        self._c(_lib.load().afg_gc_level(self.h, int(iv), int(lvl), int(rb), int(bc)))
+
+    def restrict_tree(self, iv, iv_to=None):
+        self._c(_lib.load().afg_restrict_tree(self.h, int(iv), int(iv if iv_to is None else iv_to)))
+
+    def prolong_level(self, iv, iv_to, lvl, order, add=False):
+        self._c(_lib.load().afg_prolong_level(self.h, int(iv), int(iv_to), int(lvl), int(order), int(add)))
+
+    def add_cc(self, iv_from, iv_to):
+        self._c(_lib.load().afg_add_cc(self.h, int(iv_from), int(iv_to)))
+
+    def sub_cc(self, iv_from, iv_to):
+        self._c(_lib.load().afg_sub_cc(self.h, int(iv_from), int(iv_to)))
+
+    def times_cc(self, a, iv):
+        self._c(_lib.load().afg_times_cc(self.h, float(a), int(iv)))
+
+    def lincomb_cc(self, a, iv_a, b, iv_b):
+        self._c(_lib.load().afg_lincomb_cc(self.h, float(a), int(iv_a), float(b), int(iv_b)))
+
     def update_tree(self, records, prolong_order=2, vars=(1,), tree_of=None, dr_base=None):
         """Switch to the refined / derefined tree `records` keeping the data
         on the device (afg_update_tree): kept boxes keep all their values,

@@ -50,6 +50,15 @@ enum afg_op {
 /* Physical boundary types (a5_bc_dirichlet, a5_bc_neumann; m_aX_t.f90:45-48). */
 enum { AFG_BC_DIRICHLET = -1, AFG_BC_NEUMANN = -2 };
 
+/* Refinement-boundary methods of afg_gc_tree / afg_gc_level (the subr_rb
+ * argument of a$D_gc_tree, m_aX_gc.f90:28-69). */
+enum {
+    AFG_RB_MG = 0,          /* mg$D_sides_rb (m_aX_mg.f90:262-350), as afg_fill_gc */
+    AFG_RB_PROLONG0 = 1,    /* a$D_gc_prolong0 (m_aX_gc.f90:135-151) */
+    AFG_RB_INTERP = 2,      /* a$D_gc_interp (m_aX_gc.f90:155-251) */
+    AFG_RB_INTERP_LIM = 3   /* a$D_gc_interp_lim (m_aX_gc.f90:254-358) */
+};
+
 /* Arithmetic of the relaxation, transfer and correction stages. */
 /* The multigrid options of mg$D_t (m_aX_mg.f90:20-52).  Unset values are -1,
  * filled in by afg_mg_init exactly as mg$D_init_mg does (m_aX_mg.f90:130-154). */
@@ -121,6 +130,35 @@ int afg_forest_trees(const afg_ctx *ctx, int *n_trees, int32_t *highest);
 /* Per tree: max |variable iv| over its leaf interiors, i.e. max(|min|, |max|)
  * of a$D_tree_max_cc / a$D_tree_min_cc (m_aX_utils.f90:468-571) per tree. */
 int afg_forest_max_abs(afg_ctx *ctx, int iv, int n_trees, double *out);
+
+/* ------------------------------------------------------------------ */
+/* ghost cells and transfers of any variable                           */
+
+/* a$D_gc_tree (m_aX_gc.f90:28-69) for variable iv: the face ghosts of every
+ * box, lowest level first; a same-level neighbour gives sides_from_nb
+ * (:381-415), a refinement boundary the method rb (AFG_RB_*), a physical
+ * boundary bc_to_gc (:71-132) with the boundary table of afg_set_bc (bc = 0)
+ * or with a$D_bc_dirichlet_zero / a$D_bc_neumann_zero on every face
+ * (bc = AFG_BC_DIRICHLET / AFG_BC_NEUMANN, :361-379).  afg_gc_level does
+ * one level of it (the loop body of a$D_gc_tree, :38-43). */
+int afg_gc_tree(afg_ctx *ctx, int iv, int rb, int bc);
+int afg_gc_level(afg_ctx *ctx, int iv, int lvl, int rb, int bc);
+/* a$D_restrict_tree (m_aX_restrict.f90:47-57): children of every parent,
+ * highest level first, variable iv restricted into iv_to of the parents. */
+int afg_restrict_tree(afg_ctx *ctx, int iv, int iv_to);
+/* a$D_prolong0 / a$D_prolong1 / a$D_prolong2 (order 0 / 1 / 2,
+ * m_aX_prolong.f90:39-348) of variable iv from the parents of level lvl into
+ * iv_to of every box of level lvl, i.e. a$D_prolong<order>_from for every
+ * parent of level lvl - 1; add = 1 adds to the old values. */
+int afg_prolong_level(afg_ctx *ctx, int iv, int iv_to, int lvl, int order, int add);
+/* Whole-array arithmetic over every box (ghost cells included):
+ * a$D_box_add_cc (iv_to += iv_from), a$D_box_sub_cc (iv_to -= iv_from),
+ * a$D_box_times_cc (iv *= a), a$D_box_lincomb_cc (iv_b = a*iv_a + b*iv_b),
+ * m_aX_utils.f90:273-318, applied to the whole tree. */
+int afg_add_cc(afg_ctx *ctx, int iv_from, int iv_to);
+int afg_sub_cc(afg_ctx *ctx, int iv_from, int iv_to);
+int afg_times_cc(afg_ctx *ctx, double a, int iv);
+int afg_lincomb_cc(afg_ctx *ctx, double a, int iv_a, double b, int iv_b);
 
 /* Boundary condition of box `id` in direction nb (1-based, a$D_neighb_*):
  * bc_type AFG_BC_DIRICHLET or AFG_BC_NEUMANN and the boundary values b written

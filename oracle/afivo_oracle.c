@@ -339,6 +339,133 @@ static void gc_box(orc_tree *t, int id, int iv)
     }
 }
 
+/* a$D_gc_interp and a$D_gc_interp_lim (lim = 1), m_aX_gc.f90:155-358:
+ * ghost = interpolation between the parent's neighbour (coarse, c1..c3) and
+ * the box's own first interior layer (f); the limited form caps it at
+ * 2*c1.  The parent's neighbour is indexed with a$D_get_child_offset(box, nb)
+ * (m_aX_t.f90:362-374), whose tangential components are the box's offset in
+ * its parent; its normal index is nc (low face) or 1 (high face). */
+static void gc_interp(orc_tree *t, int id, int f, int iv, int lim)
+{
+    const double sixth = 1 / 6.0, third = 1 / 3.0;
+    int n = t->ncell[id], dn = f / 2, high = f % 2;
+    int p = t->parent[id], pnb = t->nbr[p * 6 + f];
+    int off[3];
+    child_offset(t, id, off);
+    double *c = cc(t, id, iv);
+    const double *q = cc(t, pnb, iv);
+    int ix = high ? n + 1 : 0, ix_f = high ? n : 1, ix_c = high ? 1 : n;
+    if (t->dim == 2) {
+        for (int a = 1; a <= n; a++) {
+            int o = dn == 0 ? off[1] : off[0];
+            int a1 = o + ((a + 1) >> 1);
+            int a2 = a1 + 1 - 2 * (a & 1);
+            double c1, c2, fv;
+            int g;
+            if (dn == 0) {              /* case (1) */
+                c1 = q[ix3(n, ix_c, a1, 0)];
+                c2 = q[ix3(n, ix_c, a2, 0)];
+                fv = c[ix3(n, ix_f, a, 0)];
+                g = ix3(n, ix, a, 0);
+            } else {                    /* case (2) */
+                c1 = q[ix3(n, a1, ix_c, 0)];
+                c2 = q[ix3(n, a2, ix_c, 0)];
+                fv = c[ix3(n, a, ix_f, 0)];
+                g = ix3(n, a, ix, 0);
+            }
+            c[g] = 0.5 * c1 + sixth * c2 + third * fv;
+            if (lim && c[g] > 2 * c1) c[g] = 2 * c1;
+        }
+        return;
+    }
+    for (int b = 1; b <= n; b++)
+        for (int a = 1; a <= n; a++) {
+            double c1, c2, c3, fv;
+            int g;
+            if (dn == 0) {              /* case (1): a = j, b = k */
+                int j1 = off[1] + ((a + 1) >> 1), j2 = j1 + 1 - 2 * (a & 1);
+                int k1 = off[2] + ((b + 1) >> 1), k2 = k1 + 1 - 2 * (b & 1);
+                c1 = q[ix3(n, ix_c, j1, k1)];
+                c2 = q[ix3(n, ix_c, j2, k1)];
+                c3 = q[ix3(n, ix_c, j1, k2)];
+                fv = c[ix3(n, ix_f, a, b)];
+                g = ix3(n, ix, a, b);
+            } else if (dn == 1) {       /* case (2): a = i, b = k */
+                int i1 = off[0] + ((a + 1) >> 1), i2 = i1 + 1 - 2 * (a & 1);
+                int k1 = off[2] + ((b + 1) >> 1), k2 = k1 + 1 - 2 * (b & 1);
+                c1 = q[ix3(n, i1, ix_c, k1)];
+                c2 = q[ix3(n, i2, ix_c, k1)];
+                c3 = q[ix3(n, i1, ix_c, k2)];
+                fv = c[ix3(n, a, ix_f, b)];
+                g = ix3(n, a, ix, b);
+            } else {                    /* case (3): a = i, b = j */
+                int i1 = off[0] + ((a + 1) >> 1), i2 = i1 + 1 - 2 * (a & 1);
+                int j1 = off[1] + ((b + 1) >> 1), j2 = j1 + 1 - 2 * (b & 1);
+                c1 = q[ix3(n, i1, j1, ix_c)];
+                c2 = q[ix3(n, i1, j2, ix_c)];
+                c3 = q[ix3(n, i2, j1, ix_c)];
+                fv = c[ix3(n, a, b, ix_f)];
+                g = ix3(n, a, b, ix);
+            }
+            c[g] = third * c1 + sixth * c2 + sixth * c3 + third * fv;
+            if (lim && c[g] > 2 * c1) c[g] = 2 * c1;
+        }
+}
+
+/* a$D_bc_dirichlet_zero / a$D_bc_neumann_zero (m_aX_gc.f90:361-379) followed
+ * by bc_to_gc: the boundary value is 0 on every face. */
+static void sides_bc_zero(orc_tree *t, int id, int f, int iv, int bc_type)
+{
+    int n = t->ncell[id], dn = f / 2, high = f % 2;
+    double c1, c2;
+    if (bc_type == ORC_BC_DIRICHLET) {
+        c1 = -1;
+        c2 = 2;
+    } else {
+        c1 = 1;
+        c2 = t->dr[id] * (high ? 1 : -1);
+    }
+    double *c = cc(t, id, iv);
+    int g = high ? n + 1 : 0, in = high ? n : 1;
+    int n2 = t->dim == 3 ? n : 1;
+    for (int bb = 1; bb <= n2; bb++)
+        for (int a = 1; a <= n; a++) {
+            int z = t->dim == 3 ? bb : 0;
+            int gi = face_ix(n, dn, g, a, z);
+            c[gi] = 0.0;
+            c[gi] = c2 * c[gi] + c1 * c[face_ix(n, dn, in, a, z)];
+        }
+}
+
+/* a$D_gc_tree, m_aX_gc.f90:28-69: every box of every level, lowest level
+ * first, with subr_rb = mg$D_sides_rb / a$D_gc_prolong0 / a$D_gc_interp /
+ * a$D_gc_interp_lim (rb = ORC_RB_*) and subr_bc = the boundary table
+ * (bc = 0) or a$D_bc_dirichlet_zero / a$D_bc_neumann_zero (bc =
+ * ORC_BC_DIRICHLET / ORC_BC_NEUMANN). */
+int orc_gc_tree(orc_tree *t, int iv, int rb, int bc)
+{
+    if (rb < ORC_RB_MG || rb > ORC_RB_INTERP_LIM) return 1;
+    if (bc != 0 && bc != ORC_BC_DIRICHLET && bc != ORC_BC_NEUMANN) return 1;
+    for (int lvl = t->min_lvl; lvl <= t->highest_lvl; lvl++)
+        for (int i = 0; i < n_ids(t, lvl); i++) {
+            int id = level_id(t, lvl, i);
+            for (int f = 0; f < t->nnb; f++) {
+                int nb_id = t->nbr[id * 6 + f];
+                if (nb_id > 0)
+                    sides_from_nb(t, id, nb_id, f, iv);
+                else if (nb_id == 0) {
+                    if (rb == ORC_RB_MG) sides_rb(t, id, f, iv);
+                    else if (rb == ORC_RB_PROLONG0) gc_prolong0(t, id, f, iv);
+                    else gc_interp(t, id, f, iv, rb == ORC_RB_INTERP_LIM);
+                } else if (bc == 0)
+                    sides_bc(t, id, f, iv);
+                else
+                    sides_bc_zero(t, id, f, iv, bc);
+            }
+        }
+    return 0;
+}
+
 /* fill_gc_phi, m_aX_mg.f90:352-365 (over all boxes of a level) */
 int orc_fill_gc_level(orc_tree *t, int iv, int lvl)
 {
@@ -703,6 +830,110 @@ int orc_prolong2_level(orc_tree *t, int iv, int iv_to, int lvl, int add)
         int id = level_id(t, lvl, i);
         prolong2(t, t->parent[id], id, iv, iv_to, add);
     }
+    return 0;
+}
+
+/* a$D_prolong0 (m_aX_prolong.f90:39-108) over the child's interior */
+static void prolong0(orc_tree *t, int p_id, int c_id, int iv, int ivc, int add)
+{
+    int n = t->ncell[c_id], np = t->ncell[p_id];
+    int off[3];
+    child_offset(t, c_id, off);
+    const double *p = cc(t, p_id, iv);
+    double *c = cc(t, c_id, ivc);
+    int k0 = t->dim == 3 ? 1 : 0, k1 = t->dim == 3 ? n : 0;
+    for (int k = k0; k <= k1; k++) {
+        int kc = t->dim == 3 ? off[2] + ((k + 1) >> 1) : 0;
+        for (int j = 1; j <= n; j++) {
+            int jc = off[1] + ((j + 1) >> 1);
+            for (int i = 1; i <= n; i++) {
+                int ic = off[0] + ((i + 1) >> 1);
+                double *q = &c[ix3(n, i, j, k)];
+                if (add) *q = *q + p[ix3(np, ic, jc, kc)];
+                else *q = p[ix3(np, ic, jc, kc)];
+            }
+        }
+    }
+}
+
+/* a$D_prolong1 (m_aX_prolong.f90:130-219): 2-1-1 (2D) and 1-1-1-1 (3D)
+ * linear interpolation, added to the old values or to a zeroed interior */
+static void prolong1(orc_tree *t, int p_id, int c_id, int iv, int ivc, int add)
+{
+    int n = t->ncell[c_id], np = t->ncell[p_id], hnc = n >> 1;
+    int off[3];
+    child_offset(t, c_id, off);
+    const double *p = cc(t, p_id, iv);
+    double *c = cc(t, c_id, ivc);
+    if (!add) {
+        int kk = t->dim == 3 ? n : 0;
+        for (int k = t->dim == 3 ? 1 : 0; k <= kk; k++)
+            for (int j = 1; j <= n; j++)
+                for (int i = 1; i <= n; i++) c[ix3(n, i, j, k)] = 0;
+    }
+    if (t->dim == 2) {
+        for (int j = 1; j <= hnc; j++) {
+            int jc = j + off[1], jf = 2 * j - 1;
+            for (int i = 1; i <= hnc; i++) {
+                int ic = i + off[0], i_f = 2 * i - 1;
+                double f0 = 0.5 * p[ix3(np, ic, jc, 0)];
+                double flx = 0.25 * p[ix3(np, ic - 1, jc, 0)];
+                double fhx = 0.25 * p[ix3(np, ic + 1, jc, 0)];
+                double fly = 0.25 * p[ix3(np, ic, jc - 1, 0)];
+                double fhy = 0.25 * p[ix3(np, ic, jc + 1, 0)];
+                double *q;
+                q = &c[ix3(n, i_f, jf, 0)];         *q = f0 + flx + fly + *q;
+                q = &c[ix3(n, i_f + 1, jf, 0)];     *q = f0 + fhx + fly + *q;
+                q = &c[ix3(n, i_f, jf + 1, 0)];     *q = f0 + flx + fhy + *q;
+                q = &c[ix3(n, i_f + 1, jf + 1, 0)]; *q = f0 + fhx + fhy + *q;
+            }
+        }
+        return;
+    }
+    for (int k = 1; k <= hnc; k++) {
+        int kc = k + off[2], kf = 2 * k - 1;
+        for (int j = 1; j <= hnc; j++) {
+            int jc = j + off[1], jf = 2 * j - 1;
+            for (int i = 1; i <= hnc; i++) {
+                int ic = i + off[0], i_f = 2 * i - 1;
+                double f0 = 0.25 * p[ix3(np, ic, jc, kc)];
+                double fl[3] = {0.25 * p[ix3(np, ic - 1, jc, kc)], 0.25 * p[ix3(np, ic, jc - 1, kc)],
+                                0.25 * p[ix3(np, ic, jc, kc - 1)]};
+                double fh[3] = {0.25 * p[ix3(np, ic + 1, jc, kc)], 0.25 * p[ix3(np, ic, jc + 1, kc)],
+                                0.25 * p[ix3(np, ic, jc, kc + 1)]};
+                for (int s = 0; s < 8; s++) {
+                    int sx = s & 1, sy = (s >> 1) & 1, sz = (s >> 2) & 1;
+                    double *q = &c[ix3(n, i_f + sx, jf + sy, kf + sz)];
+                    *q = f0 + (sx ? fh[0] : fl[0]) + (sy ? fh[1] : fl[1]) + (sz ? fh[2] : fl[2]) + *q;
+                }
+            }
+        }
+    }
+}
+
+/* a$D_prolong0/1/2_from (m_aX_prolong.f90:23-36, 112-125, 224-237) for every
+ * parent of level lvl - 1, i.e. into every box of level lvl */
+int orc_prolong_level(orc_tree *t, int iv, int iv_to, int lvl, int order, int add)
+{
+    if (order < 0 || order > 2 || lvl <= t->min_lvl) return 1;
+    for (int i = 0; i < n_ids(t, lvl); i++) {
+        int id = level_id(t, lvl, i);
+        if (order == 0) prolong0(t, t->parent[id], id, iv, iv_to, add);
+        else if (order == 1) prolong1(t, t->parent[id], id, iv, iv_to, add);
+        else prolong2(t, t->parent[id], id, iv, iv_to, add);
+    }
+    return 0;
+}
+
+/* a$D_restrict_tree, m_aX_restrict.f90:47-57: the children of every parent,
+ * highest level first */
+int orc_restrict_tree(orc_tree *t, int iv, int iv_to)
+{
+    for (int lvl = t->highest_lvl; lvl > t->min_lvl; lvl--)
+        for (int i = 0; i < n_ids(t, lvl); i++) {
+            int id = level_id(t, lvl, i);
+            restrict_box(t, id, t->parent[id], iv, iv_to);
+        }
     return 0;
 }
 

@@ -32,6 +32,8 @@ typedef struct orc_mg_opts {
 } orc_mg_opts;
 
 enum { ORC_BC_DIRICHLET = -1, ORC_BC_NEUMANN = -2 };
+/* refinement-boundary ghost methods of orc_gc_tree */
+enum { ORC_RB_MG = 0, ORC_RB_PROLONG0 = 1, ORC_RB_INTERP = 2, ORC_RB_INTERP_LIM = 3 };
 
 orc_tree *orc_create(int dim, int nc, int n_var, double dr_base, int n_boxes,
                      const int32_t *recs);
@@ -55,6 +57,14 @@ int orc_residual_level(orc_tree *t, const orc_mg_opts *o, int lvl);
 int orc_restrict_level(orc_tree *t, int iv, int lvl);          /* lvl -> lvl-1 */
 int orc_prolong2_level(orc_tree *t, int iv, int iv_to, int lvl, int add); /* lvl-1 -> lvl */
 int orc_correct_children_level(orc_tree *t, const orc_mg_opts *o, int lvl);
+
+/* Ghost cells and transfers outside the multigrid: a$D_gc_tree with a choice
+ * of refinement-boundary method and boundary condition (bc 0: the boundary
+ * table; ORC_BC_*: that condition with b = 0 everywhere), a$D_restrict_tree,
+ * and a$D_prolong0/1/2 into every box of level lvl (order 0/1/2). */
+int orc_gc_tree(orc_tree *t, int iv, int rb, int bc);
+int orc_restrict_tree(orc_tree *t, int iv, int iv_to);
+int orc_prolong_level(orc_tree *t, int iv, int iv_to, int lvl, int order, int add);
 int orc_update_coarse_level(orc_tree *t, const orc_mg_opts *o, int lvl);
 
 #ifdef __cplusplus

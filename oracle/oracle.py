@@ -55,6 +55,10 @@ def lib():
         L.orc_restrict_level.argtypes = [P, ctypes.c_int, ctypes.c_int]
         L.orc_prolong2_level.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_correct_children_level.argtypes = [P, ctypes.POINTER(MgOpts), ctypes.c_int]
+        L.orc_gc_tree.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_restrict_tree.argtypes = [P, ctypes.c_int, ctypes.c_int]
+        L.orc_prolong_level.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int]
         L.orc_update_coarse_level.argtypes = [P, ctypes.POINTER(MgOpts), ctypes.c_int]
         _lib = L
     return _lib
@@ -113,6 +117,20 @@ class OracleTree:
         if lib().orc_fas_vcycle(self.h, ctypes.byref(opts), hl, int(set_residual)):
             raise ValueError("orc_fas_vcycle failed")
 
+    def gc_tree(self, iv, rb, bc=0):
+        """a$D_gc_tree: rb 0 mg sides_rb, 1 gc_prolong0, 2 gc_interp, 3
+        gc_interp_lim; bc 0 the boundary table, BC_DIRICHLET / BC_NEUMANN
+        that condition with b = 0."""
+        if lib().orc_gc_tree(self.h, iv, rb, bc):
+            raise ValueError("orc_gc_tree failed")
+
+    def restrict_tree(self, iv, iv_to=None):
+        lib().orc_restrict_tree(self.h, iv, iv if iv_to is None else iv_to)
+
+    def prolong_level(self, iv, iv_to, lvl, order, add=False):
+        if lib().orc_prolong_level(self.h, iv, iv_to, lvl, order, int(add)):
+            raise ValueError("orc_prolong_level failed")
+
     def max_cc(self, iv, include_parents=False):
         return lib().orc_tree_max_cc(self.h, iv, int(include_parents))
 
@@ -121,6 +139,10 @@ class OracleTree:
 
     def max_abs(self, iv):
         return max(abs(self.min_cc(iv)), abs(self.max_cc(iv)))
+
+
+BC_DIRICHLET, BC_NEUMANN = -1, -2             # ORC_BC_* (afivo's a5_bc_dirichlet / a5_bc_neumann)
+RB_MG, RB_PROLONG0, RB_INTERP, RB_INTERP_LIM = 0, 1, 2, 3
 
 
 def default_opts(n_down=2, n_up=2, n_base=2, eps=False):

@@ -63,11 +63,15 @@ program ref_harness_$Dd
   real(dp)           :: refine2_thr = 1.0_dp
   real(dp)           :: refine2_rm_thr = -1.0_dp
   integer            :: refine2_nvar = 1
+  ! transfer dumps after the solve (before any refine2 step): variable 4 set
+  ! from phi or rhs and then changed by one ghost-cell or transfer routine of
+  ! afivo, the tree written after each (see transfer_dumps)
+  logical            :: xfer = .false.
 
   namelist /cfg/ nc, ref_mode, l_uniform, l_max, thr, n_gauss, ampl, sigma, &
        r0, list_file, n_down, n_up, n_base, n_fmg, n_vcycle, use_eps, eps_hi, &
        eps_cut, out_file, dump_input, dump_fmg, dump_output, n_time, refine2, &
-       refine2_lmax, refine2_thr, refine2_rm_thr, refine2_nvar
+       refine2_lmax, refine2_thr, refine2_rm_thr, refine2_nvar, xfer
 
   type(a$D_t)         :: tree, tree_init
   type(ref_info_t)    :: ref_info
@@ -131,6 +135,8 @@ program ref_harness_$Dd
   call solve(.true.)
 
   if (dump_output) call write_dump(trim(out_file) // '.out')
+
+  if (xfer) call transfer_dumps()
 
   if (refine2 > 0) then
      call a$D_adjust_refinement(tree, ref_second, ref_info)
@@ -294,6 +300,56 @@ contains
        end do
     end do
   end subroutine prolong_new_boxes
+
+  !> Ghost cells and transfers of afivo on variable 4 (a$D_tree_copy_cc from
+  !> phi or rhs first), one dump each:
+  !>   .gci  a$D_gc_tree with a$D_gc_interp and a$D_bc_dirichlet_zero
+  !>   .gcl  a$D_gc_tree with a$D_gc_interp_lim and a$D_bc_neumann_zero
+  !>   .gc0  a$D_gc_tree with a$D_gc_prolong0 and a$D_bc_neumann_zero
+  !>   .rst  a$D_restrict_tree of a copy of rhs
+  !>   .pr1  a$D_prolong1_from(add) of phi into a copy of rhs, every parent
+  !>   .pr0  a$D_prolong0_from of phi into a copy of rhs, every parent
+  !>   .pr2  a$D_prolong2_from of phi into a copy of rhs, every parent
+  subroutine transfer_dumps()
+    use m_a$D_restrict
+    use m_a$D_prolong
+    integer, parameter :: i_x = 4
+    integer :: lvl, i
+
+    call a$D_tree_copy_cc(tree, i_phi, i_x)
+    call a$D_gc_tree(tree, i_x, a$D_gc_interp, a$D_bc_dirichlet_zero)
+    call write_dump(trim(out_file) // '.gci')
+    call a$D_tree_copy_cc(tree, i_phi, i_x)
+    call a$D_gc_tree(tree, i_x, a$D_gc_interp_lim, a$D_bc_neumann_zero)
+    call write_dump(trim(out_file) // '.gcl')
+    call a$D_tree_copy_cc(tree, i_phi, i_x)
+    call a$D_gc_tree(tree, i_x, a$D_gc_prolong0, a$D_bc_neumann_zero)
+    call write_dump(trim(out_file) // '.gc0')
+    call a$D_tree_copy_cc(tree, i_rhs, i_x)
+    call a$D_restrict_tree(tree, i_x)
+    call write_dump(trim(out_file) // '.rst')
+    call a$D_tree_copy_cc(tree, i_rhs, i_x)
+    do lvl = lbound(tree%lvls, 1), tree%highest_lvl - 1
+       do i = 1, size(tree%lvls(lvl)%parents)
+          call a$D_prolong1_from(tree%boxes, tree%lvls(lvl)%parents(i), i_phi, i_x, .true.)
+       end do
+    end do
+    call write_dump(trim(out_file) // '.pr1')
+    call a$D_tree_copy_cc(tree, i_rhs, i_x)
+    do lvl = lbound(tree%lvls, 1), tree%highest_lvl - 1
+       do i = 1, size(tree%lvls(lvl)%parents)
+          call a$D_prolong0_from(tree%boxes, tree%lvls(lvl)%parents(i), i_phi, i_x)
+       end do
+    end do
+    call write_dump(trim(out_file) // '.pr0')
+    call a$D_tree_copy_cc(tree, i_rhs, i_x)
+    do lvl = lbound(tree%lvls, 1), tree%highest_lvl - 1
+       do i = 1, size(tree%lvls(lvl)%parents)
+          call a$D_prolong2_from(tree%boxes, tree%lvls(lvl)%parents(i), i_phi, i_x)
+       end do
+    end do
+    call write_dump(trim(out_file) // '.pr2')
+  end subroutine transfer_dumps
 
   !> Refine exactly the boxes listed in list_file (sorted by lvl, ix($D) ..
   !> ix(1), lexicographically).

@@ -372,6 +372,43 @@ void launch_prolong_list(int dim, hipStream_t s, const DevTree &T, int nc, int p
     });
 }
 
+void launch_prolong_level(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const double *from,
+                          double *to, int order, int add)
+{
+    if (L.n <= 0) return;
+    emul_submit(s, [=] {
+        const int per_box = ipow(L.nc / 2, dim);
+        for (int g = 0; g < L.n * per_box; g++) {
+            const int b = g / per_box, c = g - b * per_box, slot = L.slot0 + b;
+            if (dim == 2) {
+                if (order == 0) prolong_cell<2, 0>(T, L.nc, L.pnc, slot, from, to, add, c);
+                else if (order == 1) prolong_cell<2, 1>(T, L.nc, L.pnc, slot, from, to, add, c);
+                else prolong_cell<2, 2>(T, L.nc, L.pnc, slot, from, to, add, c);
+            } else {
+                if (order == 0) prolong_cell<3, 0>(T, L.nc, L.pnc, slot, from, to, add, c);
+                else if (order == 1) prolong_cell<3, 1>(T, L.nc, L.pnc, slot, from, to, add, c);
+                else prolong_cell<3, 2>(T, L.nc, L.pnc, slot, from, to, add, c);
+            }
+        }
+    });
+}
+
+void launch_gc(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, double *var, int rb, int bc)
+{
+    if (L.n <= 0) return;
+    emul_submit(s, [=] {
+        if (dim == 2)
+            for (int g = 0; g < fill_threads<2>(L.n, L.nc); g++) gc_body<2>(T, L, var, rb, bc, g);
+        else
+            for (int g = 0; g < fill_threads<3>(L.n, L.nc); g++) gc_body<3>(T, L, var, rb, bc, g);
+    });
+}
+
+void launch_arith(hipStream_t s, long long n, int op, double a, double b, const double *x, double *y)
+{
+    if (n > 0) emul_submit(s, [=] { for (long long i = 0; i < n; i++) arith_body(op, a, b, x, y, i); });
+}
+
 void launch_to_fp32(hipStream_t s, long long n, const double *from, float *to)
 {
     if (n > 0) emul_submit(s, [=] { for (long long i = 0; i < n; i++) to[i] = (float)from[i]; });

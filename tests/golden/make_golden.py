@@ -14,6 +14,12 @@ of the added boxes and the interiors of their prolonged variables.  The
 script checks that the reference left every kept box's values unchanged and
 that the solved tree equals the committed fixture of the base case.
 
+Ghost cells and transfers (the XFERS cases): variable 4 of a solved golden
+tree set from phi or rhs and changed by a$D_gc_tree (with a$D_gc_interp,
+a$D_gc_interp_lim, a$D_gc_prolong0), a$D_restrict_tree or
+a$D_prolong0/1/2_from; stored per routine (`<name>.npz`): the packed
+positions it changed and their values.
+
 Usage: python tests/golden/make_golden.py [case ...]   (default: all)
 """
 import gzip
@@ -46,6 +52,8 @@ CASES = {
                             use_eps=True, eps_hi=5.0, **G3)),
     # 3D, 4^3 boxes, constant coefficient
     "g3d_nc4": (3, dict(nc=4, l_uniform=1, l_max=3, thr=5e-3, n_base=4, n_fmg=1, n_vcycle=1, **G3)),
+    # 3D, 4^3 boxes, adaptive (42 boxes, 32 faces at refinement boundaries)
+    "g3d_nc4_ad": (3, dict(nc=4, l_uniform=2, l_max=3, thr=0.5, n_base=4, n_fmg=1, n_vcycle=1, **G3)),
 }
 
 
@@ -104,10 +112,42 @@ def make_update(name, base, extra):
     print("wrote", name, {k: meta[k] for k in ("n_old", "n_new", "n_removed", "n_added")})
 
 
+# Ghost cells and transfers (the XFERS cases): the solved tree of a golden
+# case with variable 4 set from phi or rhs and changed by one routine of
+# afivo (the harness's transfer_dumps).  Stored per routine: the packed
+# positions of variable 4 that the routine changed and their values.
+XFERS = {"xfer_g2d_nc4": "g2d_nc4", "xfer_g2d_nc8_guess": "g2d_nc8_guess", "xfer_g3d_nc4_ad": "g3d_nc4_ad"}
+XFER_OPS = {"gci": 1, "gcl": 1, "gc0": 1, "rst": 2, "pr1": 2, "pr0": 2, "pr2": 2}   # op: source variable
+
+
+def make_xfer(name, base):
+    from afivo_amd.treefile import read_tree_file
+    g = json.load(open(os.path.join(HERE, base + ".json")))
+    cfg = dict(g["cfg"], xfer=True)
+    work = tempfile.mkdtemp(prefix="afg_golden_")
+    prefix, _ = oracle.run_ref(g["dim"], cfg, workdir=work)
+    with gzip.open(os.path.join(HERE, base + ".out.gz")) as f:
+        assert f.read() == open(prefix + ".out", "rb").read(), "solved tree differs from the base fixture"
+    out = read_tree_file(prefix + ".out")
+    arrays = {}
+    for op, src in XFER_OPS.items():
+        st = read_tree_file(prefix + "." + op)
+        assert np.array_equal(st.recs, out.recs)
+        for iv in range(3):
+            assert st.data[iv].tobytes() == out.data[iv].tobytes(), f"{op} changed variable {iv + 1}"
+        before = out.data[src - 1].view(np.int64)
+        after = st.data[3]
+        pos = np.nonzero(after.view(np.int64) != before)[0].astype(np.int32)
+        arrays[op + "_pos"] = pos
+        arrays[op + "_val"] = after[pos]
+        print(name, op, "changed", pos.size, "of", after.size)
+    meta = dict(base=base, dim=g["dim"], cfg=cfg, ops=XFER_OPS)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), meta=np.array(json.dumps(meta, sort_keys=True)),
+                        **arrays)
+    shutil.rmtree(work)
+
+
 def main(names):
-    for name, (base, extra) in UPDATES.items():
-        if not names or name in names:
-            make_update(name, base, extra)
     for name, (dim, cfg) in CASES.items():
         if names and name not in names:
             continue
@@ -122,6 +162,12 @@ def main(names):
             json.dump({"dim": dim, "cfg": cfg}, f, indent=1, sort_keys=True)
         shutil.rmtree(work)
         print("wrote", name)
+    for name, (base, extra) in UPDATES.items():
+        if not names or name in names:
+            make_update(name, base, extra)
+    for name, base in XFERS.items():
+        if not names or name in names:
+            make_xfer(name, base)
 
 
 if __name__ == "__main__":

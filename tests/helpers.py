@@ -23,3 +23,34 @@ def random_state(st, n_var, seed):
     """Random values in every variable, ghosts and corners included."""
     rng = np.random.default_rng(seed)
     return [rng.standard_normal(st.n_values()) for _ in range(n_var)]
+
+
+# The routines of the transfer fixtures (tests/golden/xfer_*.npz, written by
+# make_golden.py from the reference's own a$D_gc_tree, a$D_restrict_tree and
+# a$D_prolong*_from): op -> (source variable, what it runs on variable 4).
+# `t` is an OracleTree or a DeviceTree (same method names).
+RB_MG, RB_PROLONG0, RB_INTERP, RB_INTERP_LIM = 0, 1, 2, 3
+BC_DIRICHLET, BC_NEUMANN = -1, -2
+
+
+def run_xfer(t, op, min_lvl, highest_lvl):
+    if op == "gci":
+        t.gc_tree(4, RB_INTERP, BC_DIRICHLET)
+    elif op == "gcl":
+        t.gc_tree(4, RB_INTERP_LIM, BC_NEUMANN)
+    elif op == "gc0":
+        t.gc_tree(4, RB_PROLONG0, BC_NEUMANN)
+    elif op == "rst":
+        t.restrict_tree(4)
+    else:
+        order = {"pr0": 0, "pr1": 1, "pr2": 2}[op]
+        for lvl in range(min_lvl + 1, highest_lvl + 1):
+            t.prolong_level(1, 4, lvl, order, add=(op == "pr1"))
+
+
+def xfer_expected(source, pos, val):
+    """Variable 4 after a fixture's routine: the source variable with the
+    routine's changes."""
+    out = np.array(source, dtype=np.float64, copy=True)
+    out[pos] = val
+    return out

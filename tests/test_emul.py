@@ -61,3 +61,9 @@ from test_tree_update import (  # noqa: E402,F401
     test_update_matches_boxes_not_ids,
     test_update_matches_reference,
 )
+from test_transfers_gpu import (  # noqa: E402,F401
+    test_gc_rejects_bad_arguments,
+    test_transfers_match_oracle_full_size,
+    test_transfers_match_reference,
+    test_tree_arithmetic,
+)

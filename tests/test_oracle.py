@@ -20,7 +20,7 @@ import oracle as O
 from afivo_amd import problems as P
 from afivo_amd import tree as T
 from afivo_amd.treefile import read_tree_file
-from helpers import assert_bitexact
+from helpers import assert_bitexact, run_xfer, xfer_expected
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 CASES = sorted(f[:-5] for f in os.listdir(GOLDEN) if f.endswith(".json"))
@@ -40,6 +40,15 @@ def load_golden(name):
             os.remove(path)
     res = np.loadtxt(os.path.join(GOLDEN, name + ".res"), ndmin=1)
     return cfg, st["in"], st["out"], res
+
+
+XFERS = sorted(f[:-4] for f in os.listdir(GOLDEN) if f.startswith("xfer_") and f.endswith(".npz"))
+
+
+def load_xfer(name):
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    meta = json.loads(str(z["meta"]))
+    return meta, {op: (z[op + "_pos"], z[op + "_val"]) for op in meta["ops"]}
 
 
 def opts_of(cfg):
@@ -273,3 +282,21 @@ def test_planted_mistake_is_caught(what):
     for _ in range(cfg["cfg"]["n_vcycle"]):
         ot.fas_vcycle(opts)
     assert not np.array_equal(ot.get(1), st_out.data[0])
+
+
+@pytest.mark.parametrize("name", XFERS)
+def test_oracle_transfers_match_reference(name):
+    """a$D_gc_tree (gc_interp, gc_interp_lim, gc_prolong0 with Dirichlet or
+    Neumann zero), a$D_restrict_tree and a$D_prolong0/1/2_from: the oracle
+    reproduces the reference's variable 4 bit for bit (fixture written by the
+    reference build)."""
+    meta, ops = load_xfer(name)
+    _, _, st, _ = load_golden(meta["base"])
+    for op, (pos, val) in ops.items():
+        src = meta["ops"][op]
+        ot = O.OracleTree(st, n_var=4)
+        ot.set(4, st.data[src - 1])
+        run_xfer(ot, op, st.min_lvl, st.highest_lvl)
+        assert_bitexact(ot.get(4), xfer_expected(st.data[src - 1], pos, val), f"{name} {op}")
+        for iv in (1, 2, 3):
+            assert_bitexact(ot.get(iv), st.data[iv - 1], f"{name} {op}: variable {iv}")
