@@ -50,6 +50,9 @@ SIGNATURES = {
     "afg_sub_cc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
     "afg_times_cc": (ctypes.c_int, [P, F64, ctypes.c_int]),
     "afg_lincomb_cc": (ctypes.c_int, [P, F64, ctypes.c_int, F64, ctypes.c_int]),
+    "afg_upload_fc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, I32P, F64P]),
+    "afg_download_fc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, I32P, F64P]),
+    "afg_consistent_fluxes": (ctypes.c_int, [P, ctypes.c_int, I32P]),
     "afg_update_tree": (ctypes.c_int, [P, ctypes.c_int, I32P, F64, I32P, ctypes.c_int, ctypes.c_int, I32P]),
     "afg_forest_trees": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), I32P]),
     "afg_forest_max_abs": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, F64P]),

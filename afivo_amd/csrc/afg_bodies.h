@@ -1268,6 +1268,45 @@ AFG_DEV void gc_body(const DevTree &T, const LevelArgs &L, double *var, int rb, 
     box[gi + s * (gj + s * gk)] = g;
 }
 
+// flux_from_children (m_aX_core.f90:838-943, Cartesian) for one face cell
+// of a leaf neighbour: the mean of the 2^(dim-1) fine fluxes of the parent's
+// child on that face, 0.5 (a + b) in 2D and 0.25 (((a + b) + c) + d) in 3D
+// (first tangential index fastest).  Face arrays of a slot: fx, fy (, fz)
+// at fc + slot * fstride, each (nc+1) nc^(dim-1) values with extent nc+1
+// along its own direction, Fortran order.
+template <int DIM>
+AFG_DEV void flux_body(const FluxRec *recs, int nc, int fstride, double *fc, int gid)
+{
+    const int per = DIM == 2 ? nc : nc * nc;
+    const int r = gid / per, t = gid - r * per;
+    const FluxRec R = recs[r];
+    const int d = R.f >> 1, high = R.f & 1;
+    const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+    const int nch = nc >> 1;
+    const int A = t % nc + 1, B = DIM == 3 ? t / nc + 1 : 1;
+    const int ba = (A - 1) / nch, bb = DIM == 3 ? (B - 1) / nch : 0;
+    const int m = A - ba * nch, m2 = B - bb * nch;
+    const int cs = R.child[ba | (bb << 1)];
+    const int i = high ? nc + 1 : 1, i_nb = high ? 1 : nc + 1;
+    const int len = (nc + 1) * (DIM == 3 ? nc * nc : nc);
+    const int e0 = d == 0 ? nc + 1 : nc, e1 = d == 1 ? nc + 1 : nc;
+    auto at = [&](int slot, int xn, int xa, int xb) -> double & {
+        int x[3];
+        x[d] = xn;
+        x[ta] = xa;
+        x[tb] = xb;
+        return fc[(size_t)slot * fstride + d * len + (x[0] - 1) + e0 * ((x[1] - 1) + e1 * (DIM == 3 ? x[2] - 1 : 0))];
+    };
+    double v;
+    if (DIM == 2) {
+        v = 0.5 * (at(cs, i, 2 * m - 1, 1) + at(cs, i, 2 * m, 1));
+    } else {
+        v = 0.25 * (at(cs, i, 2 * m - 1, 2 * m2 - 1) + at(cs, i, 2 * m, 2 * m2 - 1) +
+                    at(cs, i, 2 * m - 1, 2 * m2) + at(cs, i, 2 * m, 2 * m2));
+    }
+    at(R.nb, i_nb, A, B) = v;
+}
+
 // y = y + x, y = y - x, y = a y, y = a x + b y over whole pools (ARITH_*)
 AFG_DEV void arith_body(int op, double a, double b, const double *x, double *y, long long i)
 {

@@ -125,6 +125,7 @@ struct afg_ctx {
     std::vector<int32_t> id2slot;        // -1 if absent
     std::vector<int32_t> slot2id, slot_lvl, slot_nc;
     std::vector<int32_t> slot_ix, slot_tree;   // box%ix (3 per slot) and tree index per slot
+    std::vector<int32_t> slot_par;             // parent slot, -1 if none
     bool rb_parent_nb = true;            // every refinement-boundary face has a neighbour of the parent there
     std::vector<int32_t> h_nb;           // slot * nnb + face (afivo ids)
     std::vector<BcRec> bc;
@@ -147,6 +148,13 @@ struct afg_ctx {
     int8_t *d_is_parent = nullptr;
     std::vector<double *> d_var;
     std::vector<float *> d_var32;        // FP32 copies (mixed-precision solve), allocated on use
+    // face variables (box%fx, fy, fz): fstride doubles per slot, allocated on use
+    std::vector<double *> d_fc;
+    int fstride = 0;
+    // a$D_consistent_fluxes work records, by level (built on first use)
+    bool flux_ready = false;
+    std::vector<int> flux_off;
+    FluxRec *d_flux = nullptr;
     // second phi pool of the fused pass pairs (FP64, FP32): a level's phi goes
     // there and back within a relaxation block
     double *d_phi2 = nullptr;
@@ -291,6 +299,11 @@ void free_tree(afg_ctx *c)
     hipFree(c->d_partial);
     for (double *p : c->d_var) hipFree(p);
     for (float *p : c->d_var32) hipFree(p);
+    for (double *p : c->d_fc) hipFree(p);
+    std::fill(c->d_fc.begin(), c->d_fc.end(), nullptr);
+    hipFree(c->d_flux);
+    c->d_flux = nullptr;
+    c->flux_ready = false;
     hipFree(c->d_phi2);
     hipFree(c->d_phi2_32);
     c->d_phi2 = nullptr;
@@ -344,6 +357,25 @@ int ensure_pool(afg_ctx *c, int iv, std::vector<R *> &pools, R **out)
 
 int ensure_var(afg_ctx *c, int iv, double **out) { return ensure_pool(c, iv, c->d_var, out); }
 int ensure_var(afg_ctx *c, int iv, float **out) { return ensure_pool(c, iv, c->d_var32, out); }
+
+// The device pool of face variable ifc (1..AFG_MAX_FC), allocated and zeroed
+// on first use.
+int ensure_fc(afg_ctx *c, int ifc, double **out)
+{
+    if (ifc < 1 || ifc > AFG_MAX_FC) return c->fail(AFG_ERR_ARG, "face variable %d out of range 1..%d", ifc, AFG_MAX_FC);
+    double *&p = c->d_fc[ifc - 1];
+    if (!p) {
+        if (c->capturing) return c->fail(AFG_ERR_STATE, "face variable %d first used inside a graph capture", ifc);
+        const size_t bytes = (size_t)c->n_slots * c->fstride * sizeof(double);
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            p = nullptr;
+            return c->fail(AFG_ERR_MEMORY, "hipMalloc(%zu) for face variable %d", bytes, ifc);
+        }
+        HIP_TRY(c, hipMemsetAsync(p, 0, bytes, c->stream));
+    }
+    *out = p;
+    return 0;
+}
 
 int build_base(afg_ctx *c);
 
@@ -1171,6 +1203,8 @@ int afg_ctx_create(int dim, int n_cell, int n_var, afg_ctx **out)
     c->device = dev;
     // each box array padded to a multiple of 16 doubles (128 B)
     c->stride = (box_len(dim, n_cell) + 15) / 16 * 16;
+    c->fstride = (dim * (n_cell + 1) * (dim == 3 ? n_cell * n_cell : n_cell) + 15) / 16 * 16;
+    c->d_fc.assign(AFG_MAX_FC, nullptr);
     c->d_var.assign(n_var, nullptr);
     c->d_var32.assign(n_var, nullptr);
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -1455,6 +1489,7 @@ int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base,
     HIP_TRY(c, hipMalloc(&c->d_tag, sn * sizeof(int8_t)));
     HIP_H2D(c, c->d_nb, nbv.data(), sn * nnb * sizeof(int32_t));
     HIP_H2D(c, c->d_parent, par.data(), sn * sizeof(int32_t));
+    c->slot_par = par;
     HIP_H2D(c, c->d_choff, choff.data(), sn * sizeof(int32_t));
     HIP_H2D(c, c->d_lists, lists.data(), lists.size() * sizeof(int32_t));
     HIP_TRY(c, hipMemsetAsync(c->d_tag, 0, sn * sizeof(int8_t), c->stream));
@@ -1543,11 +1578,13 @@ int afg_update_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base
     const std::vector<double> old_bc_vals = c->bc_vals;
     std::map<int, Level> old_level;
     for (const Level &L : c->levels) old_level[L.lvl] = L;
-    std::vector<double *> old_pool = c->d_var;
+    std::vector<double *> old_pool = c->d_var, old_fc = c->d_fc;
     std::fill(c->d_var.begin(), c->d_var.end(), nullptr);
+    std::fill(c->d_fc.begin(), c->d_fc.end(), nullptr);
     int32_t *d_src = nullptr;
     auto release = [&](int code) {
         for (double *p : old_pool) hipFree(p);
+        for (double *p : old_fc) hipFree(p);
         hipFree(d_src);
         return code;
     };
@@ -1588,6 +1625,15 @@ int afg_update_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base
             return release(c->fail(AFG_ERR_MEMORY, "hipMalloc(%zu) for variable %d", pool_bytes, iv));
         launch_gather(c->stream, c->stride, ns, d_src, old_pool[iv - 1], p);
         c->d_var[iv - 1] = p;
+    }
+    for (int k = 0; k < AFG_MAX_FC; k++) {
+        if (!old_fc[k]) continue;
+        double *p = nullptr;
+        const size_t fbytes = (size_t)ns * c->fstride * sizeof(double);
+        if (hipMalloc(&p, fbytes) != hipSuccess)
+            return release(c->fail(AFG_ERR_MEMORY, "hipMalloc(%zu) for face variable %d", fbytes, k + 1));
+        launch_gather(c->stream, c->fstride, ns, d_src, old_fc[k], p);
+        c->d_fc[k] = p;
     }
     // the listed variables of the added boxes from their parents, level by
     // level (a box's parent may itself be new); added boxes of the lowest
@@ -1685,7 +1731,7 @@ int afg_packed_size(const afg_ctx *cc, int n_ids, const int32_t *ids, int64_t *n
 }
 
 // Copy the contiguous runs of marked slots between device pool and staging.
-static int copy_runs(afg_ctx *c, double *dev, const std::vector<char> &mark, bool to_device)
+static int copy_runs(afg_ctx *c, double *dev, const std::vector<char> &mark, bool to_device, int stride)
 {
     int s = 0;
     while (s < c->n_slots) {
@@ -1695,7 +1741,7 @@ static int copy_runs(afg_ctx *c, double *dev, const std::vector<char> &mark, boo
         }
         int e = s;
         while (e < c->n_slots && mark[e]) e++;
-        const size_t off = (size_t)s * c->stride, len = (size_t)(e - s) * c->stride * sizeof(double);
+        const size_t off = (size_t)s * stride, len = (size_t)(e - s) * stride * sizeof(double);
         if (to_device)
             HIP_TRY(c, hipMemcpyAsync(dev + off, c->h_stage + off, len, hipMemcpyHostToDevice, c->stream));
         else
@@ -1724,7 +1770,7 @@ int afg_upload_cc(afg_ctx *c, int iv, int n_ids, const int32_t *ids, const doubl
         pos += len;
         mark[s] = 1;
     }
-    if (int e = copy_runs(c, dev, mark, true)) return e;
+    if (int e = copy_runs(c, dev, mark, true, c->stride)) return e;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -1741,7 +1787,7 @@ int afg_download_cc(afg_ctx *c, int iv, int n_ids, const int32_t *ids, double *p
         if (id < 1 || id > c->max_id || c->id2slot[id] < 0) return c->fail(AFG_ERR_ARG, "no box with id %d", id);
         mark[c->id2slot[id]] = 1;
     }
-    if (int e = copy_runs(c, dev, mark, false)) return e;
+    if (int e = copy_runs(c, dev, mark, false, c->stride)) return e;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     size_t pos = 0;
     for (int k = 0; k < n_ids; k++) {
@@ -1749,6 +1795,129 @@ int afg_download_cc(afg_ctx *c, int iv, int n_ids, const int32_t *ids, double *p
         const int len = box_len(c->dim, c->slot_nc[s]);
         std::memcpy(packed + pos, c->h_stage + (size_t)s * c->stride, len * sizeof(double));
         pos += len;
+    }
+    return 0;
+}
+
+static int face_len(int dim, int n) { return dim * (n + 1) * (dim == 3 ? n * n : n); }
+
+int afg_upload_fc(afg_ctx *c, int ifc, int n_ids, const int32_t *ids, const double *packed)
+{
+    if (int e = check_ctx(c)) return e;
+    double *dev = nullptr;
+    if (int e = ensure_fc(c, ifc, &dev)) return e;
+    if (int e = ensure_stage(c, (size_t)c->n_slots * c->fstride)) return e;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));   // staging may be in use
+    std::vector<char> mark(c->n_slots, 0);
+    size_t pos = 0;
+    for (int k = 0; k < n_ids; k++) {
+        const int id = ids[k];
+        if (id < 1 || id > c->max_id || c->id2slot[id] < 0) return c->fail(AFG_ERR_ARG, "no box with id %d", id);
+        const int s = c->id2slot[id];
+        const int len = face_len(c->dim, c->slot_nc[s]);
+        std::memcpy(c->h_stage + (size_t)s * c->fstride, packed + pos, len * sizeof(double));
+        if (len < c->fstride) std::memset(c->h_stage + (size_t)s * c->fstride + len, 0, (c->fstride - len) * sizeof(double));
+        pos += len;
+        mark[s] = 1;
+    }
+    if (int e = copy_runs(c, dev, mark, true, c->fstride)) return e;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int afg_download_fc(afg_ctx *c, int ifc, int n_ids, const int32_t *ids, double *packed)
+{
+    if (int e = check_ctx(c)) return e;
+    double *dev = nullptr;
+    if (int e = ensure_fc(c, ifc, &dev)) return e;
+    if (int e = ensure_stage(c, (size_t)c->n_slots * c->fstride)) return e;
+    std::vector<char> mark(c->n_slots, 0);
+    for (int k = 0; k < n_ids; k++) {
+        const int id = ids[k];
+        if (id < 1 || id > c->max_id || c->id2slot[id] < 0) return c->fail(AFG_ERR_ARG, "no box with id %d", id);
+        mark[c->id2slot[id]] = 1;
+    }
+    if (int e = copy_runs(c, dev, mark, false, c->fstride)) return e;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    size_t pos = 0;
+    for (int k = 0; k < n_ids; k++) {
+        const int s = c->id2slot[ids[k]];
+        const int len = face_len(c->dim, c->slot_nc[s]);
+        std::memcpy(packed + pos, c->h_stage + (size_t)s * c->fstride, len * sizeof(double));
+        pos += len;
+    }
+    return 0;
+}
+
+// The work records of a$D_consistent_fluxes: for every parent of a level
+// >= 1 (the boxes below level 1 always have children) and face whose
+// neighbour exists and has no children, the neighbour, the face and the
+// children on that face.
+static int build_flux_records(afg_ctx *c)
+{
+    if (c->flux_ready) return 0;
+    const int dim = c->dim, nnb = c->nnb, nch = c->nch;
+    std::vector<FluxRec> recs;
+    const int nl = (int)c->levels.size();
+    c->flux_off.assign(nl + 1, 0);
+    std::vector<int32_t> children;
+    for (int li = 0; li < nl; li++) {
+        const Level &L = c->levels[li];
+        c->flux_off[li] = (int)recs.size();
+        if (L.lvl < 1 || li + 1 >= nl) continue;
+        // children slots of this level's boxes: the next level's parent slots
+        const Level &C = c->levels[li + 1];
+        children.assign((size_t)L.n * nch, -1);
+        for (int k = 0; k < C.n; k++) {
+            const int ps = c->slot_par[C.slot0 + k] - L.slot0;
+            const int cs = C.slot0 + k;
+            int ci = 0;
+            for (int d = 0; d < dim; d++) ci |= ((c->slot_ix[(size_t)cs * 3 + d] - 1) & 1) << d;   // a$D_ix_to_ichild
+            children[(size_t)ps * nch + ci] = cs;
+        }
+        for (int b = 0; b < L.n; b++) {
+            if (children[(size_t)b * nch] < 0) continue;   // a leaf
+            const int s = L.slot0 + b;
+            for (int f = 0; f < nnb; f++) {
+                const int nid = c->h_nb[(size_t)s * nnb + f];
+                if (nid <= 0) continue;
+                const int ns = c->id2slot[nid];
+                if (children[(size_t)(ns - L.slot0) * nch] >= 0) continue;   // has children
+                FluxRec r;
+                r.nb = ns;
+                r.f = f;
+                const int d = f >> 1, high = f & 1;
+                const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+                for (int q = 0; q < 4; q++) r.child[q] = -1;
+                for (int q = 0; q < (1 << (dim - 1)); q++) {
+                    const int ci = (high << d) | ((q & 1) << ta) | (dim == 3 ? ((q >> 1) & 1) << tb : 0);
+                    r.child[q] = children[(size_t)b * nch + ci];
+                }
+                recs.push_back(r);
+            }
+        }
+    }
+    c->flux_off[nl] = (int)recs.size();
+    if (!recs.empty()) {
+        HIP_TRY(c, hipMalloc(&c->d_flux, recs.size() * sizeof(FluxRec)));
+        HIP_H2D(c, c->d_flux, recs.data(), recs.size() * sizeof(FluxRec));
+    }
+    c->flux_ready = true;
+    return 0;
+}
+
+int afg_consistent_fluxes(afg_ctx *c, int n_f, const int32_t *f_ixs)
+{
+    if (int e = check_ctx(c)) return e;
+    if (n_f < 0 || (n_f > 0 && !f_ixs)) return c->fail(AFG_ERR_ARG, "invalid face-variable list");
+    std::vector<double *> pools(n_f);
+    for (int k = 0; k < n_f; k++)
+        if (int e = ensure_fc(c, f_ixs[k], &pools[k])) return e;
+    if (int e = build_flux_records(c)) return e;
+    for (size_t li = 0; li + 1 < c->flux_off.size(); li++) {
+        const int n = c->flux_off[li + 1] - c->flux_off[li];
+        for (double *p : pools)
+            launch_flux(c->dim, c->stream, c->d_flux + c->flux_off[li], n, c->levels[li].nc, c->fstride, p);
     }
     return 0;
 }
@@ -2328,6 +2497,8 @@ int afg_device_bytes(const afg_ctx *cc, int64_t *out)
         if (p) b += (int64_t)c->n_slots * c->stride * 8;
     for (float *p : c->d_var32)
         if (p) b += (int64_t)c->n_slots * c->stride * 4;
+    for (double *p : c->d_fc)
+        if (p) b += (int64_t)c->n_slots * c->fstride * 8;
     b += (int64_t)c->n_slots * (c->nnb + 3) * 4 + c->n_slots;
     *out = b;
     return 0;

@@ -214,6 +214,13 @@ __global__ __launch_bounds__(BLOCK) void k_gc(DevTree T, LevelArgs L, double *va
     if (gid < fill_threads<DIM>(L.n, L.nc)) gc_body<DIM>(T, L, var, rb, bc, gid);
 }
 
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_flux(const FluxRec *recs, int n, int nc, int fstride, double *fc)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < n * (DIM == 2 ? nc : nc * nc)) flux_body<DIM>(recs, nc, fstride, fc, gid);
+}
+
 __global__ __launch_bounds__(BLOCK) void k_arith(long long n, int op, double a, double b, const double *x,
                                                  double *y)
 {
@@ -682,6 +689,14 @@ void launch_gc(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, dou
     if (L.n <= 0) return;
     if (dim == 2) k_gc<2><<<n_blocks(fill_threads<2>(L.n, L.nc)), BLOCK, 0, s>>>(T, L, var, rb, bc);
     else k_gc<3><<<n_blocks(fill_threads<3>(L.n, L.nc)), BLOCK, 0, s>>>(T, L, var, rb, bc);
+}
+
+void launch_flux(int dim, hipStream_t s, const FluxRec *recs, int n, int nc, int fstride, double *fc)
+{
+    if (n <= 0) return;
+    const int g = n_blocks((long long)n * (dim == 2 ? nc : nc * nc));
+    if (dim == 2) k_flux<2><<<g, BLOCK, 0, s>>>(recs, n, nc, fstride, fc);
+    else k_flux<3><<<g, BLOCK, 0, s>>>(recs, n, nc, fstride, fc);
 }
 
 void launch_arith(hipStream_t s, long long n, int op, double a, double b, const double *x, double *y)

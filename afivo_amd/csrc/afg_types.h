@@ -32,6 +32,15 @@ constexpr int RB_INTERP_LIM = 3;   // a$D_gc_interp_lim (m_aX_gc.f90:254-358)
 // (a$D_box_add_cc, _sub_cc, _times_cc, _lincomb_cc, m_aX_utils.f90:273-318).
 constexpr int ARITH_ADD = 0, ARITH_SUB = 1, ARITH_TIMES = 2, ARITH_LINCOMB = 3;
 
+// One work record of a$D_consistent_fluxes (m_aX_core.f90:811-943): the
+// leaf neighbour `nb` of a parent across the parent's face f, and the
+// parent's children on that face, indexed by their tangential offset bits
+// (first tangential axis | second << 1).
+struct FluxRec {
+    int32_t nb, f;
+    int32_t child[4];
+};
+
 // Device topology of one tree.  Slots of a level are contiguous and in Morton
 // order; every box array of a variable occupies `stride` doubles at
 // var + slot * stride and holds cc(0:nc_l+1, ...) in Fortran order.

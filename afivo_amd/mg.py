@@ -74,6 +74,27 @@ class DeviceTree:
     def prolong_level(self, iv, iv_to, lvl, order, add=False):
         self._c(_lib.load().afg_prolong_level(self.h, int(iv), int(iv_to), int(lvl), int(order), int(add)))
 
+    def n_face_values(self):
+        """Values of one face variable over all boxes (fx, fy (, fz) per box)."""
+        from .treefile import level_ncell
+        n = level_ncell(self.nc, self.records[:, 1])
+        return int((self.dim * (n + 1) * n ** (self.dim - 1)).sum())
+
+    def upload_fc(self, ifc, packed):
+        """Face variable ifc of every box (record order), fx, fy (, fz) per box."""
+        a = np.ascontiguousarray(packed, dtype=np.float64)
+        assert a.size == self.n_face_values()
+        self._c(_lib.load().afg_upload_fc(self.h, int(ifc), self.ids.size, _i32(self.ids), _f64(a)))
+
+    def download_fc(self, ifc):
+        out = np.empty(self.n_face_values(), dtype=np.float64)
+        self._c(_lib.load().afg_download_fc(self.h, int(ifc), self.ids.size, _i32(self.ids), _f64(out)))
+        return out
+
+    def consistent_fluxes(self, f_ixs):
+        f = np.ascontiguousarray(list(f_ixs), dtype=np.int32)
+        self._c(_lib.load().afg_consistent_fluxes(self.h, f.size, _i32(f)))
+
     def add_cc(self, iv_from, iv_to):
         self._c(_lib.load().afg_add_cc(self.h, int(iv_from), int(iv_to)))
 

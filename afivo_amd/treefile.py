@@ -89,6 +89,18 @@ class TreeState:
         np.cumsum(s, out=off[1:])
         return off
 
+    def face_sizes(self):
+        """Values of one face variable per box: fx, fy (, fz) with
+        (nc_l + 1) * nc_l^(dim-1) values each (box%fx.., m_aX_core.f90:359-369)."""
+        n = self.ncell()
+        return self.dim * (n + 1) * n ** (self.dim - 1)
+
+    def face_offsets(self):
+        s = self.face_sizes()
+        off = np.zeros(self.n_boxes + 1, dtype=np.int64)
+        np.cumsum(s, out=off[1:])
+        return off
+
     def n_values(self) -> int:
         return int(self.box_sizes().sum())
 

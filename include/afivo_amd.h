@@ -160,6 +160,19 @@ int afg_sub_cc(afg_ctx *ctx, int iv_from, int iv_to);
 int afg_times_cc(afg_ctx *ctx, double a, int iv);
 int afg_lincomb_cc(afg_ctx *ctx, double a, int iv_a, double b, int iv_b);
 
+/* Face variables ifc = 1..AFG_MAX_FC (box%fx, fy (, fz), m_aX_core.f90:
+ * 359-369), allocated and zeroed on first use.  Packed data of a box: fx, fy
+ * (, fz), each (nc_l+1) * nc_l^(dim-1) values with extent nc_l+1 along its
+ * own direction, Fortran order; boxes in the order of `ids`. */
+#define AFG_MAX_FC 16
+int afg_upload_fc(afg_ctx *ctx, int ifc, int n_ids, const int32_t *ids, const double *packed);
+int afg_download_fc(afg_ctx *ctx, int ifc, int n_ids, const int32_t *ids, double *packed);
+/* a$D_consistent_fluxes (m_aX_core.f90:811-943, Cartesian) over the n_f
+ * face variables f_ixs: on every face a parent shares with a neighbour that
+ * has no children, the neighbour's fluxes become the mean of the parent's
+ * children's fluxes on that face; lowest level first. */
+int afg_consistent_fluxes(afg_ctx *ctx, int n_f, const int32_t *f_ixs);
+
 /* Boundary condition of box `id` in direction nb (1-based, a$D_neighb_*):
  * bc_type AFG_BC_DIRICHLET or AFG_BC_NEUMANN and the boundary values b written
  * by the user's sides_bc (nc_l^(dim-1) values, first tangential index

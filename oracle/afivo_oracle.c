@@ -35,6 +35,11 @@ struct orc_tree {
     int *lvl_start;        /* record-order ids of level l: [lvl_start[l-min], lvl_start[l-min+1]) */
     int *bc_type;          /* per (id, nb) */
     double **bc_val;       /* per (id, nb), NULL means b = 0 */
+    /* face variables: fx, fy (, fz) of each box, packed in record order at
+     * foff[id]; allocated on first use */
+    size_t *foff;
+    size_t n_face;
+    double *fc[ORC_MAX_FC];
 };
 
 #define TAG_UNSET 0
@@ -120,6 +125,14 @@ orc_tree *orc_create(int dim, int nc, int n_var, double dr_base, int n_boxes,
         off += (size_t)box_len(t, t->ncell[id]);
     }
     t->n_values = off;
+    t->foff = calloc(n, sizeof(size_t));
+    size_t fo = 0;
+    for (int r2 = 0; r2 < n_boxes; r2++) {
+        int id = t->rec_ids[r2], m = t->ncell[id];
+        t->foff[id] = fo;
+        fo += (size_t)dim * (m + 1) * (dim == 3 ? m * m : m);
+    }
+    t->n_face = fo;
     t->data = calloc((size_t)n_var, sizeof(double *));
     for (int v = 0; v < n_var; v++) t->data[v] = calloc(off, sizeof(double));
     int nl = hi_lvl - min_lvl + 1;
@@ -149,6 +162,8 @@ void orc_destroy(orc_tree *t)
     free(t->lvl); free(t->ix); free(t->parent); free(t->children); free(t->nbr);
     free(t->ncell); free(t->tag); free(t->dr); free(t->off); free(t->rec_ids);
     free(t->lvl_start);
+    free(t->foff);
+    for (int v = 0; v < ORC_MAX_FC; v++) free(t->fc[v]);
     free(t);
 }
 
@@ -167,6 +182,7 @@ int orc_get_cc(const orc_tree *t, int iv, double *packed)
     memcpy(packed, t->data[iv - 1], t->n_values * sizeof(double));
     return 0;
 }
+
 
 int orc_set_bc(orc_tree *t, int id, int nb, int bc_type, const double *b)
 {
@@ -206,6 +222,99 @@ static void child_offset(const orc_tree *t, int id, int off[3])
     if (t->lvl[id] > 1)
         for (int d = 0; d < t->dim; d++)
             off[d] = ((t->ix[id * 3 + d] - 1) & 1) * (t->ncell[id] >> 1);
+}
+
+/* --------------------------------------------------------------------- */
+/* face variables                                                         */
+
+int orc_n_face_values(const orc_tree *t) { return (int)t->n_face; }
+
+int orc_set_fc(orc_tree *t, int ifc, const double *packed)
+{
+    if (ifc < 1 || ifc > ORC_MAX_FC) return -1;
+    if (!t->fc[ifc - 1]) t->fc[ifc - 1] = calloc(t->n_face, sizeof(double));
+    memcpy(t->fc[ifc - 1], packed, t->n_face * sizeof(double));
+    return 0;
+}
+
+int orc_get_fc(const orc_tree *t, int ifc, double *packed)
+{
+    if (ifc < 1 || ifc > ORC_MAX_FC || !t->fc[ifc - 1]) return -1;
+    memcpy(packed, t->fc[ifc - 1], t->n_face * sizeof(double));
+    return 0;
+}
+
+/* Face value of direction d at 1-based cell index (x0, x1, x2) of box id:
+ * box%fx / fy / fz (m_aX_core.f90:359-369), extent n+1 along d, Fortran
+ * order. */
+static double *face_at(const orc_tree *t, int ifc, int id, int d, int x0, int x1, int x2)
+{
+    int n = t->ncell[id];
+    size_t len = (size_t)(n + 1) * (t->dim == 3 ? n * n : n);
+    int e0 = d == 0 ? n + 1 : n, e1 = d == 1 ? n + 1 : n;
+    size_t q = (size_t)(x0 - 1) + (size_t)e0 * ((x1 - 1) + (size_t)e1 * (t->dim == 3 ? x2 - 1 : 0));
+    return t->fc[ifc - 1] + t->foff[id] + (size_t)d * len + q;
+}
+
+/* flux_from_children, m_aX_core.f90:838-943 (Cartesian): the fluxes of the
+ * leaf neighbour nb_id on the face it shares with parent id, set to the mean
+ * of the fluxes of id's children on that face (0.5 (a + b) in 2D, 0.25
+ * (((a + b) + c) + d) in 3D, first tangential index fastest). */
+static void flux_from_children(orc_tree *t, int id, int f, int ifc)
+{
+    int nc = t->ncell[id], nch = nc >> 1, d = f / 2, high = f % 2;
+    int nb_id = t->nbr[id * 6 + f];
+    int i = high ? nc + 1 : 1, i_nb = high ? 1 : nc + 1;
+    int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+    for (int c = 0; c < t->nch; c++) {
+        if (((c >> d) & 1) != high) continue;            /* a$D_child_adj_nb */
+        int c_id = t->children[id * 8 + c];
+        int oa = nch * ((c >> ta) & 1), ob = t->dim == 3 ? nch * ((c >> tb) & 1) : 0;
+        int nb2 = t->dim == 3 ? nch : 1;
+        for (int m2 = 1; m2 <= nb2; m2++)
+            for (int m = 1; m <= nch; m++) {
+                int x[3], y[3];
+                x[d] = i_nb; x[ta] = oa + m; x[tb] = t->dim == 3 ? ob + m2 : 1;
+                double v;
+                if (t->dim == 2) {
+                    y[d] = i; y[ta] = 2 * m - 1; y[tb] = 1;
+                    double a = *face_at(t, ifc, c_id, d, y[0], y[1], y[2]);
+                    y[ta] = 2 * m;
+                    double b = *face_at(t, ifc, c_id, d, y[0], y[1], y[2]);
+                    v = 0.5 * (a + b);
+                } else {
+                    double w[4];
+                    for (int k = 0; k < 4; k++) {
+                        y[d] = i;
+                        y[ta] = 2 * m - 1 + (k & 1);
+                        y[tb] = 2 * m2 - 1 + (k >> 1);
+                        w[k] = *face_at(t, ifc, c_id, d, y[0], y[1], y[2]);
+                    }
+                    v = 0.25 * (w[0] + w[1] + w[2] + w[3]);
+                }
+                *face_at(t, ifc, nb_id, d, x[0], x[1], x[2]) = v;
+            }
+    }
+}
+
+/* a$D_consistent_fluxes, m_aX_core.f90:811-836: every parent, lowest level
+ * first, sets the fluxes of each neighbour that exists and has no
+ * children. */
+int orc_consistent_fluxes(orc_tree *t, int n_f, const int *f_ixs)
+{
+    for (int k = 0; k < n_f; k++)
+        if (f_ixs[k] < 1 || f_ixs[k] > ORC_MAX_FC || !t->fc[f_ixs[k] - 1]) return -1;
+    for (int lvl = t->min_lvl; lvl < t->highest_lvl; lvl++)
+        for (int i = 0; i < n_ids(t, lvl); i++) {
+            int id = level_id(t, lvl, i);
+            if (!has_children(t, id)) continue;
+            for (int f = 0; f < t->nnb; f++) {
+                int nb_id = t->nbr[id * 6 + f];
+                if (nb_id > 0 && !has_children(t, nb_id))
+                    for (int k = 0; k < n_f; k++) flux_from_children(t, id, f, f_ixs[k]);
+            }
+        }
+    return 0;
 }
 
 /* --------------------------------------------------------------------- */

@@ -32,6 +32,7 @@ typedef struct orc_mg_opts {
 } orc_mg_opts;
 
 enum { ORC_BC_DIRICHLET = -1, ORC_BC_NEUMANN = -2 };
+#define ORC_MAX_FC 16   /* face variables */
 /* refinement-boundary ghost methods of orc_gc_tree */
 enum { ORC_RB_MG = 0, ORC_RB_PROLONG0 = 1, ORC_RB_INTERP = 2, ORC_RB_INTERP_LIM = 3 };
 
@@ -65,6 +66,13 @@ int orc_correct_children_level(orc_tree *t, const orc_mg_opts *o, int lvl);
 int orc_gc_tree(orc_tree *t, int iv, int rb, int bc);
 int orc_restrict_tree(orc_tree *t, int iv, int iv_to);
 int orc_prolong_level(orc_tree *t, int iv, int iv_to, int lvl, int order, int add);
+/* Face variables 1..ORC_MAX_FC: per box fx, fy (, fz), each (nc_l+1) *
+ * nc_l^(dim-1) values in Fortran order, boxes in record order; and
+ * a$D_consistent_fluxes over the face variables f_ixs. */
+int orc_n_face_values(const orc_tree *t);
+int orc_set_fc(orc_tree *t, int ifc, const double *packed);
+int orc_get_fc(const orc_tree *t, int ifc, double *packed);
+int orc_consistent_fluxes(orc_tree *t, int n_f, const int *f_ixs);
 int orc_update_coarse_level(orc_tree *t, const orc_mg_opts *o, int lvl);
 
 #ifdef __cplusplus

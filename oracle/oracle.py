@@ -56,6 +56,10 @@ def lib():
         L.orc_prolong2_level.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_correct_children_level.argtypes = [P, ctypes.POINTER(MgOpts), ctypes.c_int]
         L.orc_gc_tree.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_n_face_values.argtypes = [P]
+        L.orc_set_fc.argtypes = [P, ctypes.c_int, f64p]
+        L.orc_get_fc.argtypes = [P, ctypes.c_int, f64p]
+        L.orc_consistent_fluxes.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.orc_restrict_tree.argtypes = [P, ctypes.c_int, ctypes.c_int]
         L.orc_prolong_level.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int]
@@ -130,6 +134,23 @@ class OracleTree:
     def prolong_level(self, iv, iv_to, lvl, order, add=False):
         if lib().orc_prolong_level(self.h, iv, iv_to, lvl, order, int(add)):
             raise ValueError("orc_prolong_level failed")
+
+    def set_fc(self, ifc, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.size == lib().orc_n_face_values(self.h)
+        if lib().orc_set_fc(self.h, ifc, _f64(a)):
+            raise ValueError("orc_set_fc failed")
+
+    def get_fc(self, ifc):
+        a = np.empty(lib().orc_n_face_values(self.h), dtype=np.float64)
+        if lib().orc_get_fc(self.h, ifc, _f64(a)):
+            raise ValueError("orc_get_fc failed")
+        return a
+
+    def consistent_fluxes(self, f_ixs):
+        f = np.ascontiguousarray(list(f_ixs), dtype=np.intc)
+        if lib().orc_consistent_fluxes(self.h, f.size, f.ctypes.data_as(ctypes.POINTER(ctypes.c_int))):
+            raise ValueError("orc_consistent_fluxes failed")
 
     def max_cc(self, iv, include_parents=False):
         return lib().orc_tree_max_cc(self.h, iv, int(include_parents))

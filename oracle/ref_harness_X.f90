@@ -93,7 +93,7 @@ program ref_harness_$Dd
 
   if (ref_mode == 'list') call read_list()
 
-  call a$D_init(tree, nc, n_var, 0, 1.0_dp / nc, coarsen_to=2, &
+  call a$D_init(tree, nc, n_var, merge(1, 0, xfer), 1.0_dp / nc, coarsen_to=2, &
        mem_limit_gb=64.0_dp)
   ix_list(:, 1) = 1
   nb_list(:, 1) = -1
@@ -310,6 +310,7 @@ contains
   !>   .pr1  a$D_prolong1_from(add) of phi into a copy of rhs, every parent
   !>   .pr0  a$D_prolong0_from of phi into a copy of rhs, every parent
   !>   .pr2  a$D_prolong2_from of phi into a copy of rhs, every parent
+  !>   .flx  face variable 1 (the gradient of phi) after a$D_consistent_fluxes
   subroutine transfer_dumps()
     use m_a$D_restrict
     use m_a$D_prolong
@@ -349,7 +350,51 @@ contains
        end do
     end do
     call write_dump(trim(out_file) // '.pr2')
+    ! face variable 1 = the gradient of phi, (phi(i) - phi(i-1)) / dr on
+    ! every face, then a$D_consistent_fluxes
+    call a$D_loop_box(tree, set_gradient)
+    call a$D_consistent_fluxes(tree, [1])
+    call write_faces(trim(out_file) // '.flx')
   end subroutine transfer_dumps
+
+  subroutine set_gradient(box)
+    type(box$D_t), intent(inout) :: box
+    integer :: nc_b
+    nc_b = box%n_cell
+#if $D == 2
+    box%fx(:, :, 1) = (box%cc(1:nc_b+1, 1:nc_b, i_phi) - box%cc(0:nc_b, 1:nc_b, i_phi)) / box%dr
+    box%fy(:, :, 1) = (box%cc(1:nc_b, 1:nc_b+1, i_phi) - box%cc(1:nc_b, 0:nc_b, i_phi)) / box%dr
+#elif $D == 3
+    box%fx(:, :, :, 1) = (box%cc(1:nc_b+1, 1:nc_b, 1:nc_b, i_phi) - &
+         box%cc(0:nc_b, 1:nc_b, 1:nc_b, i_phi)) / box%dr
+    box%fy(:, :, :, 1) = (box%cc(1:nc_b, 1:nc_b+1, 1:nc_b, i_phi) - &
+         box%cc(1:nc_b, 0:nc_b, 1:nc_b, i_phi)) / box%dr
+    box%fz(:, :, :, 1) = (box%cc(1:nc_b, 1:nc_b, 1:nc_b+1, i_phi) - &
+         box%cc(1:nc_b, 1:nc_b, 0:nc_b, i_phi)) / box%dr
+#endif
+  end subroutine set_gradient
+
+  !> Face variable 1 of every box, boxes in the order of write_dump: fx, fy
+  !> (, fz), each in Fortran order
+  subroutine write_faces(fname)
+    character(len=*), intent(in) :: fname
+    integer :: lu, lvl, n, id
+
+    open(newunit=lu, file=fname, access='stream', form='unformatted', &
+         status='replace', action='write')
+    do lvl = lbound(tree%lvls, 1), tree%highest_lvl
+       do n = 1, size(tree%lvls(lvl)%ids)
+          id = tree%lvls(lvl)%ids(n)
+#if $D == 2
+          write(lu) tree%boxes(id)%fx(:, :, 1), tree%boxes(id)%fy(:, :, 1)
+#elif $D == 3
+          write(lu) tree%boxes(id)%fx(:, :, :, 1), tree%boxes(id)%fy(:, :, :, 1), &
+               tree%boxes(id)%fz(:, :, :, 1)
+#endif
+       end do
+    end do
+    close(lu)
+  end subroutine write_faces
 
   !> Refine exactly the boxes listed in list_file (sorted by lvl, ix($D) ..
   !> ix(1), lexicographically).

@@ -404,6 +404,18 @@ void launch_gc(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, dou
     });
 }
 
+void launch_flux(int dim, hipStream_t s, const FluxRec *recs, int n, int nc, int fstride, double *fc)
+{
+    if (n <= 0) return;
+    emul_submit(s, [=] {
+        const int th = n * (dim == 2 ? nc : nc * nc);
+        for (int g = 0; g < th; g++) {
+            if (dim == 2) flux_body<2>(recs, nc, fstride, fc, g);
+            else flux_body<3>(recs, nc, fstride, fc, g);
+        }
+    });
+}
+
 void launch_arith(hipStream_t s, long long n, int op, double a, double b, const double *x, double *y)
 {
     if (n > 0) emul_submit(s, [=] { for (long long i = 0; i < n; i++) arith_body(op, a, b, x, y, i); });

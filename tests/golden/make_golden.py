@@ -17,7 +17,8 @@ that the solved tree equals the committed fixture of the base case.
 Ghost cells and transfers (the XFERS cases): variable 4 of a solved golden
 tree set from phi or rhs and changed by a$D_gc_tree (with a$D_gc_interp,
 a$D_gc_interp_lim, a$D_gc_prolong0), a$D_restrict_tree or
-a$D_prolong0/1/2_from; stored per routine (`<name>.npz`): the packed
+a$D_prolong0/1/2_from, and face variable 1 (the gradient of phi) changed
+by a$D_consistent_fluxes; stored per routine (`<name>.npz`): the packed
 positions it changed and their values.
 
 Usage: python tests/golden/make_golden.py [case ...]   (default: all)
@@ -141,6 +142,16 @@ def make_xfer(name, base):
         arrays[op + "_pos"] = pos
         arrays[op + "_val"] = after[pos]
         print(name, op, "changed", pos.size, "of", after.size)
+    # face variable 1: the gradient of phi, then a$D_consistent_fluxes
+    sys.path.insert(0, os.path.dirname(HERE))
+    from helpers import face_gradient
+    before = face_gradient(out, out.data[0])
+    after = np.fromfile(prefix + ".flx", dtype="<f8")
+    assert after.size == before.size
+    pos = np.nonzero(after.view(np.int64) != before.view(np.int64))[0].astype(np.int32)
+    arrays["flx_pos"] = pos
+    arrays["flx_val"] = after[pos]
+    print(name, "flx changed", pos.size, "of", after.size)
     meta = dict(base=base, dim=g["dim"], cfg=cfg, ops=XFER_OPS)
     np.savez_compressed(os.path.join(HERE, name + ".npz"), meta=np.array(json.dumps(meta, sort_keys=True)),
                         **arrays)

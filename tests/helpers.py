@@ -54,3 +54,26 @@ def xfer_expected(source, pos, val):
     out = np.array(source, dtype=np.float64, copy=True)
     out[pos] = val
     return out
+
+
+def face_gradient(st, phi):
+    """Face variable holding (phi(i) - phi(i-1)) / dr on every face of every
+    box (fx, fy (, fz) per box, Fortran order), packed as the face layout of
+    afg_upload_fc: the flux the transfer fixtures start from."""
+    out = np.zeros(int(st.face_offsets()[-1]))
+    off, foff = st.offsets(), st.face_offsets()
+    for r in range(st.n_boxes):
+        n = int(st.ncell()[r])
+        m = n + 2
+        dr = st.dr_base * 0.5 ** (int(st.lvl[r]) - 1)
+        a = phi[off[r]:off[r + 1]].reshape((m,) * st.dim)    # C order: (z,) y, x
+        parts = []
+        for d in range(st.dim):
+            ax = st.dim - 1 - d                               # the array axis of dimension d
+            hi = [slice(1, n + 1)] * st.dim
+            lo = [slice(1, n + 1)] * st.dim
+            hi[ax] = slice(1, n + 2)
+            lo[ax] = slice(0, n + 1)
+            parts.append(((a[tuple(hi)] - a[tuple(lo)]) / dr).ravel())
+        out[foff[r]:foff[r + 1]] = np.concatenate(parts)
+    return out

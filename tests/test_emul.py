@@ -67,3 +67,7 @@ from test_transfers_gpu import (  # noqa: E402,F401
     test_transfers_match_reference,
     test_tree_arithmetic,
 )
+from test_transfers_gpu import (  # noqa: E402,F401
+    test_consistent_fluxes_match_oracle_full_size,
+    test_consistent_fluxes_match_reference,
+)

@@ -300,3 +300,20 @@ def test_oracle_transfers_match_reference(name):
         assert_bitexact(ot.get(4), xfer_expected(st.data[src - 1], pos, val), f"{name} {op}")
         for iv in (1, 2, 3):
             assert_bitexact(ot.get(iv), st.data[iv - 1], f"{name} {op}: variable {iv}")
+
+
+@pytest.mark.parametrize("name", XFERS)
+def test_oracle_consistent_fluxes_match_reference(name):
+    """a$D_consistent_fluxes on face variable 1 = the gradient of phi: the
+    oracle reproduces the reference's faces bit for bit."""
+    from helpers import face_gradient
+    meta, _ = load_xfer(name)
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    _, _, st, _ = load_golden(meta["base"])
+    before = face_gradient(st, st.data[0])
+    ot = O.OracleTree(st)
+    ot.set_fc(1, before)
+    ot.set_fc(2, before)
+    ot.consistent_fluxes([1])
+    assert_bitexact(ot.get_fc(1), xfer_expected(before, z["flx_pos"], z["flx_val"]), name)
+    assert_bitexact(ot.get_fc(2), before, "face variable 2 untouched")

@@ -1,8 +1,9 @@
 """Ghost cells and transfers of any variable on the device: afg_gc_tree /
 afg_gc_level (a$D_gc_tree with mg$D_sides_rb, a$D_gc_prolong0,
 a$D_gc_interp, a$D_gc_interp_lim and the boundary table or Dirichlet /
-Neumann zero), afg_restrict_tree, afg_prolong_level (a$D_prolong0/1/2) and
-the whole-tree arithmetic (a$D_box_add_cc .. a$D_box_lincomb_cc).
+Neumann zero), afg_restrict_tree, afg_prolong_level (a$D_prolong0/1/2),
+afg_consistent_fluxes on face variables and the whole-tree arithmetic
+(a$D_box_add_cc .. a$D_box_lincomb_cc).
 
 * against the reference's own output (tests/golden/xfer_*.npz): bit for bit;
 * against the oracle on the 2D and 3D benchmark trees with random values in
@@ -11,6 +12,8 @@ the whole-tree arithmetic (a$D_box_add_cc .. a$D_box_lincomb_cc).
 * the arithmetic against numpy's IEEE operations.
 
 Device tests; also run through the CPU emulation (test_emul.py)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -18,9 +21,9 @@ import oracle as O
 from afivo_amd import mg as M
 from afivo_amd import problems as P
 from afivo_amd._lib import AfgError
-from helpers import (BC_DIRICHLET, BC_NEUMANN, assert_bitexact, random_state, run_xfer,
+from helpers import (BC_DIRICHLET, BC_NEUMANN, assert_bitexact, face_gradient, random_state, run_xfer,
                      xfer_expected)
-from test_oracle import XFERS, load_golden, load_xfer
+from test_oracle import GOLDEN, XFERS, load_golden, load_xfer
 
 pytestmark = pytest.mark.gpu
 
@@ -133,4 +136,42 @@ def test_gc_rejects_bad_arguments():
         dt.prolong_level(1, 3, dt.min_lvl, 1)
     with pytest.raises(AfgError):
         dt.prolong_level(1, 3, dt.highest_lvl, 3)
+    dt.close()
+
+
+@pytest.mark.parametrize("name", XFERS)
+def test_consistent_fluxes_match_reference(name):
+    """a$D_consistent_fluxes on face variable 1 = the gradient of phi: the
+    reference's faces bit for bit; another face variable is untouched."""
+    meta, _ = load_xfer(name)
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    _, _, st, _ = load_golden(meta["base"])
+    before = face_gradient(st, st.data[0])
+    dt = M.DeviceTree.from_state(st)
+    dt.upload_fc(1, before)
+    dt.upload_fc(3, before)
+    dt.consistent_fluxes([1])
+    assert_bitexact(dt.download_fc(1), xfer_expected(before, z["flx_pos"], z["flx_val"]), name)
+    assert_bitexact(dt.download_fc(3), before, "face variable 3 untouched")
+    dt.close()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_consistent_fluxes_match_oracle_full_size(dim):
+    st, _ = random_tree(dim, seed=60 + dim)
+    rng = np.random.default_rng(7)
+    n = int(st.face_offsets()[-1])
+    f1, f2 = rng.standard_normal(n), rng.standard_normal(n)
+    dt = M.DeviceTree.from_state(st)
+    ot = O.OracleTree(st)
+    dt.upload_fc(1, f1)
+    dt.upload_fc(2, f2)
+    ot.set_fc(1, f1)
+    ot.set_fc(2, f2)
+    dt.consistent_fluxes([1, 2])
+    ot.consistent_fluxes([1, 2])
+    for k in (1, 2):
+        got, want = dt.download_fc(k), ot.get_fc(k)
+        assert_bitexact(got, want, f"face variable {k}")
+        assert np.count_nonzero(want != (f1 if k == 1 else f2)) > 0
     dt.close()

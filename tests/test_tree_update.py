@@ -100,10 +100,17 @@ def test_update_matches_boxes_not_ids():
     for col in [0, 2 + st.dim] + list(range(3 + st.dim, recs.shape[1])):
         v = recs[:, col]
         recs[:, col] = np.where(v > 0, new_id[np.maximum(v, 0)], v)
-    recs = recs[rng.permutation(st.n_boxes)]
+    faces = rng.standard_normal(int(st.face_offsets()[-1]))
+    dt.upload_fc(2, faces)
+    order = rng.permutation(st.n_boxes)
+    recs = recs[order]
     dt.update_tree(recs, prolong_order=2, vars=(1,))
     new = downloaded_state(dt, st, recs, st.dr_base)
     assert check_update(st, new, (1,)) == 0
+    # face variables follow their boxes too
+    fo = st.face_offsets()
+    want = np.concatenate([faces[fo[r]:fo[r + 1]] for r in order])
+    assert_bitexact(dt.download_fc(2), want, "face variable 2")
     dt.close()
 
 
