@@ -45,11 +45,12 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all cores up to 16")
     ap.add_argument("--no-timing", action="store_true",
                     help="skip the per-kernel event timing (roofline is then null)")
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "update"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "update", "transfers"], default="c2",
                     help="c2: the 3D FP64 tree (default); c3: batched 2D FP64 forest of --trees "
                          "trees; c4: 3D variable-eps tree, mixed precision against FP64; "
                          "c5: served request trace; update: refinement steps of the c2 tree "
-                         "on the device")
+                         "on the device; transfers: ghost cells, restriction, prolongation, "
+                         "fluxes and arithmetic on the c2 tree")
     ap.add_argument("--requests", type=int, default=64, help="c5: number of requests")
     ap.add_argument("--rate", type=float, default=100.0, help="c5: arrival rate (requests/s)")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
@@ -478,6 +479,69 @@ def bench_update(a):
     return 0
 
 
+def bench_transfers(a):
+    """Ghost cells and transfers of any variable on the 3D benchmark tree
+    (random values): the time of one whole-tree call of each routine, and
+    the HBM rate of its algorithmic bytes (one read and one write of each
+    value the routine produces, plus the values it reads once more)."""
+    from afivo_amd import mg as M
+    from afivo_amd import problems as P
+    p = P.c2(seed=a.seed)
+    st = p.state
+    rng = np.random.default_rng(3)
+    st.data = [rng.standard_normal(st.n_values()) for _ in range(3)]
+    dt = M.DeviceTree.from_state(st)
+    dt.upload_fc(1, rng.standard_normal(int(st.face_offsets()[-1])))
+    nc, dim = st.nc, 3
+    n = st.ncell()
+    ghost = int((2 * dim * n ** (dim - 1)).sum())
+    fine = int((n ** dim)[st.lvl > st.min_lvl].sum())
+    vals = st.n_values()
+    # algorithmic bytes: ghost fill reads the source cell and writes the
+    # ghost (16 B per ghost); restriction reads every fine cell, writes
+    # 1/2^dim; prolongation reads 1/2^dim, writes every fine cell;
+    # lincomb reads two values and writes one
+    ops = {
+        "gc_tree_mg_sides_rb": (lambda: dt.gc_tree(1, 0, 0), 16.0 * ghost),
+        "gc_tree_interp": (lambda: dt.gc_tree(1, 2, 0), 16.0 * ghost),
+        "gc_tree_interp_lim_neumann": (lambda: dt.gc_tree(1, 3, -2), 16.0 * ghost),
+        "restrict_tree": (lambda: dt.restrict_tree(2, 3), 8.0 * fine * (1 + 2.0 ** -dim)),
+        "prolong2_all_levels": (lambda: [dt.prolong_level(1, 3, l, 2) for l in range(st.min_lvl + 1, st.highest_lvl + 1)],
+                                8.0 * fine * (1 + 2.0 ** -dim)),
+        "consistent_fluxes": (lambda: dt.consistent_fluxes([1]), None),
+        "lincomb_cc": (lambda: dt.lincomb_cc(0.5, 1, 2.0, 3), 24.0 * vals),
+    }
+    out = {}
+    for name, (fn, nbytes) in ops.items():
+        for _ in range(max(a.warmup, 1)):
+            fn()
+        dt.sync()
+        reps = max(a.steps, 5)
+        s0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        dt.sync()
+        ms = 1e3 * (time.perf_counter() - s0) / reps
+        out[name] = {"ms": ms, "GB_per_s": None if nbytes is None else nbytes / ms / 1e6}
+    dt.close()
+    head = out["gc_tree_interp"]
+    res = {
+        "metric": "whole-tree ghost fills/s (a$D_gc_tree with a$D_gc_interp)",
+        "value": 1e3 / head["ms"], "unit": "fills/s",
+        "n_gpus": 1, "steps": max(a.steps, 5), "warmup": max(a.warmup, 1), "ms_per_step": head["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "ghost cells and transfers on the 3D benchmark tree (%d boxes of 8^3, levels "
+                               "%d..%d), random values; host-timed whole-tree calls (every level's launch "
+                               "included)" % (st.n_boxes, st.min_lvl, st.highest_lvl),
+                   "ghost_cells": ghost, "fine_cells": fine},
+        "routines": out,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(res))
+    return 0
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -492,6 +556,8 @@ def main():
         return bench_c4(a)
     if a.workload == "update":
         return bench_update(a)
+    if a.workload == "transfers":
+        return bench_transfers(a)
 
     from afivo_amd import mg as M
     from afivo_amd import problems as P
