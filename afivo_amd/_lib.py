@@ -53,6 +53,7 @@ SIGNATURES = {
     "afg_upload_fc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, I32P, F64P]),
     "afg_download_fc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, I32P, F64P]),
     "afg_consistent_fluxes": (ctypes.c_int, [P, ctypes.c_int, I32P]),
+    "afg_set_coord": (ctypes.c_int, [P, ctypes.c_int, F64]),
     "afg_update_tree": (ctypes.c_int, [P, ctypes.c_int, I32P, F64, I32P, ctypes.c_int, ctypes.c_int, I32P]),
     "afg_forest_trees": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), I32P]),
     "afg_forest_max_abs": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, F64P]),

@@ -168,6 +168,83 @@ AFG_DEV bool lpld_box(const DevTree &T, int op_auto, int slot)
     return op_auto && T.tag[slot] != TAG_NORMAL;
 }
 
+// ---- cylindrical coordinates (2D, x the radius): the auto operators of a
+// cylindrical tree, mg2_box_gsrb_clpl / mg2_box_clpl for untagged boxes and
+// mg2_box_gsrb_clpld / mg2_box_clpld for eps boxes (m_aX_mg.f90:1323-1443),
+// with the radial factors rfac = [ir - 1, ir] / (ir - 0.5) of global radial
+// cell index ir = i + (ix(1) - 1) nc (1 for the y neighbours), and the
+// radius-weighted restriction (m_aX_restrict.f90:90-103).
+AFG_DEV bool cyl_op(const DevTree &T, int op_auto) { return T.cyl_ioff != nullptr && op_auto; }
+
+template <typename R>
+AFG_DEV void cyl_rfac(int ir, R rf[4])
+{
+    const double d = double(ir) - 0.5;
+    rf[0] = R(double(ir - 1) / d);
+    rf[1] = R(double(ir) / d);
+    rf[2] = rf[3] = R(1);
+}
+
+// u in the order i-1, i+1, j-1, j+1
+template <typename R>
+AFG_DEV R gsrb_value_cyl(const R u[6], R rhs, const R *e, int q, int s, double dx2, bool lpld, int ir)
+{
+    R rf[4];
+    cyl_rfac(ir, rf);
+    if (lpld) {
+        // c = 2 a0 a / (a0 + a); phi = (sum(c rfac u) - dx2 rhs) / sum(c rfac)
+        const R a0 = e[q];
+        const int dq[4] = {-1, 1, -s, s};
+        R su = 0, sc = 0;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const R a = e[q + dq[m]];
+            const R cm = 2 * a0 * a / (a0 + a);
+            const R cr = m < 2 ? cm * rf[m] : cm;
+            su = m == 0 ? cr * u[m] : su + cr * u[m];
+            sc = m == 0 ? cr : sc + cr;
+        }
+        return (su - R(dx2) * rhs) / sc;
+    }
+    // order of m_aX_mg.f90:1345-1349: i-1, i+1, j+1, j-1
+    return R(0.25) * (rf[0] * u[0] + rf[1] * u[1] + u[3] + u[2] - R(dx2) * rhs);
+}
+
+template <typename R>
+AFG_DEV R box_op_cyl(const R u[6], R u0, const R *e, int q, int s, double inv_dr_sq, bool lpld, int ir)
+{
+    R rf[4];
+    cyl_rfac(ir, rf);
+    if (lpld) {
+        const int dq[4] = {-1, 1, -s, s};
+        const R a0 = e[q];
+        R sum = 0;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const R a = e[q + dq[m]];
+            const R t = (m < 2 ? rf[m] * a0 : a0) * a / (a0 + a) * (u[m] - u0);
+            sum = m == 0 ? t : sum + t;
+        }
+        return R(inv_dr_sq) * 2 * sum;
+    }
+    return (rf[0] * u[0] + rf[1] * u[1] + u[2] + u[3] - 4 * u0) * R(inv_dr_sq);
+}
+
+// The weighted restriction of the children values v (in the order of the
+// reference's sum: (i_f, j_f), (i_f+1, j_f), (i_f, j_f+1), (i_f+1, j_f+1)):
+// 0.25 (w1 (v0 + v2) + w2 (v1 + v3)), w = 1 -/+ 0.25 dr_p / r with r the
+// radius r_min(1) + (ii - 0.5) dr_p of the parent's cell ii -- the octant
+// column, as a2_cyl_child_weights(box_p, i, ...) is called with the loop
+// index (m_aX_restrict.f90:98, m_aX_t.f90:507-528).
+template <typename R>
+AFG_DEV R cyl_restrict(const R v[4], double rmin_p, double dr_p, int ii)
+{
+    const double r = rmin_p + (ii - 0.5) * dr_p;
+    const double t = 0.25 * dr_p / r;
+    const R w1 = R(1 - t), w2 = R(1 + t);
+    return R(0.25) * (w1 * (v[0] + v[2]) + w2 * (v[1] + v[3]));
+}
+
 // ---------------------------------------------------------------------------
 // ghost fill of a level: fill_gc_phi (m_aX_mg.f90:352-365)
 // one thread per face ghost cell: n * 2*DIM * nc^(DIM-1)
@@ -297,7 +374,11 @@ AFG_DEV void gsrb_body(const DevTree &T, const LevelArgs &L, const LevelViewT<R>
     R u[6];
     stencil<DIM>(T, L, own, par, slot, box, nc, i, j, k, q, fly, center, u);
     const bool lpld = lpld_box(T, op_auto, slot);
-    box[q] = gsrb_value<DIM>(u, own.rhs[base + q], lpld ? own.eps + base : nullptr, q, s, L.dx2, lpld);
+    if (DIM == 2 && cyl_op(T, op_auto))
+        box[q] = gsrb_value_cyl(u, own.rhs[base + q], lpld ? own.eps + base : nullptr, q, s, L.dx2, lpld,
+                                T.cyl_ioff[slot] + i);
+    else
+        box[q] = gsrb_value<DIM>(u, own.rhs[base + q], lpld ? own.eps + base : nullptr, q, s, L.dx2, lpld);
 }
 
 // ---------------------------------------------------------------------------
@@ -653,7 +734,9 @@ AFG_DEV void residual_body(const DevTree &T, const LevelArgs &L, const LevelView
     const bool lpld = lpld_box(T, op_auto, slot);
     R u[6];
     stencil<DIM>(T, L, own, par, slot, box, nc, i, j, k, q, 0, box[q], u);
-    const R a = box_op<DIM>(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld);
+    const R a = DIM == 2 && cyl_op(T, op_auto)
+                    ? box_op_cyl(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld, T.cyl_ioff[slot] + i)
+                    : box_op<DIM>(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld);
     own.tmp[base + q] = own.rhs[base + q] - a;
 }
 
@@ -684,7 +767,8 @@ AFG_DEV void uc_child_body(const DevTree &T, const LevelArgs &L, const LevelView
     const R *ph = own.phi + base;
     const R *e = lpld ? own.eps + base : nullptr;
     const R *rh = own.rhs + base;
-    R sr = 0, sp = 0;
+    R sr = 0, sp = 0, vr[4], vp[4];
+    const bool cyl = DIM == 2 && cyl_op(T, op_auto);
 #pragma unroll
     for (int m = 0; m < (1 << DIM); m++) {
         const int fi = 2 * ii - 1 + (m & 1);
@@ -693,10 +777,15 @@ AFG_DEV void uc_child_body(const DevTree &T, const LevelArgs &L, const LevelView
         const int q = fi + s * (fj + s * fk);
         R u[6];
         stencil<DIM>(T, L, own, par, slot, ph, nc, fi, fj, fk, q, fly, ph[q], u);
-        const R res = rh[q] - box_op<DIM>(u, ph[q], e, q, s, L.inv_dr_sq, lpld);
+        const R res = rh[q] - (cyl ? box_op_cyl(u, ph[q], e, q, s, L.inv_dr_sq, lpld, T.cyl_ioff[slot] + fi)
+                                   : box_op<DIM>(u, ph[q], e, q, s, L.inv_dr_sq, lpld));
         if (!keep_child_tmp) own.tmp[base + q] = res;
         sr = m == 0 ? res : sr + res;
         sp = m == 0 ? ph[q] : sp + ph[q];
+        if (DIM == 2) {
+            vr[m & 3] = res;
+            vp[m & 3] = ph[q];
+        }
     }
     const int p = T.parent[slot];
     const int co = T.choff[slot];
@@ -705,6 +794,11 @@ AFG_DEV void uc_child_body(const DevTree &T, const LevelArgs &L, const LevelView
                    ps * ((((co >> 10) & 1023) + jj) + ps * (DIM == 3 ? ((co >> 20) & 1023) + kk : 0));
     const size_t pb = par.off(p);
     constexpr double w = DIM == 2 ? 0.25 : 0.125;
+    if (DIM == 2 && T.cyl_rmin) {
+        par.tmp[pb + pq] = cyl_restrict(vr, T.cyl_rmin[p], 2 * L.dr, ii);
+        par.phi[pb + pq] = cyl_restrict(vp, T.cyl_rmin[p], 2 * L.dr, ii);
+        return;
+    }
     par.tmp[pb + pq] = R(w) * sr;
     par.phi[pb + pq] = R(w) * sp;
 }
@@ -889,7 +983,10 @@ AFG_DEV void uc_parent_body(const DevTree &T, const LevelArgs &L, const LevelVie
         const bool lpld = lpld_box(T, op_auto, slot);
         R u[6];
         stencil<DIM>(T, L, own, par, slot, box, nc, i, j, k, q, 1, box[q], u);
-        const R a = box_op<DIM>(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld);
+        const R a = DIM == 2 && cyl_op(T, op_auto)
+                        ? box_op_cyl(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld,
+                                     T.cyl_ioff[slot] + i)
+                        : box_op<DIM>(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld);
         own.rhs[base + q] = a + own.tmp[base + q];
         if (copy_tmp) own.tmp[base + q] = box[q];
         return;
@@ -1033,13 +1130,14 @@ AFG_DEV void restrict_body(const DevTree &T, const LevelArgs &L, const R *from, 
     const int slot = L.slot0 + b;
     const int s = nc + 2;
     const R *cb = from + (size_t)slot * T.stride;
-    R sum = 0;
+    R sum = 0, v4[4];
 #pragma unroll
     for (int m = 0; m < (1 << DIM); m++) {
         const int fi = 2 * ii - 1 + (m & 1), fj = 2 * jj - 1 + ((m >> 1) & 1);
         const int fk = DIM == 3 ? 2 * kk - 1 + ((m >> 2) & 1) : 0;
         const R v = cb[fi + s * (fj + s * fk)];
         sum = m == 0 ? v : sum + v;
+        if (DIM == 2) v4[m & 3] = v;
     }
     const int p = T.parent[slot];
     const int co = T.choff[slot];
@@ -1047,7 +1145,10 @@ AFG_DEV void restrict_body(const DevTree &T, const LevelArgs &L, const R *from, 
     const int pq = ((co & 1023) + ii) +
                    ps * ((((co >> 10) & 1023) + jj) + ps * (DIM == 3 ? ((co >> 20) & 1023) + kk : 0));
     constexpr double w = DIM == 2 ? 0.25 : 0.125;
-    to[(size_t)p * T.stride + pq] = R(w) * sum;
+    if (DIM == 2 && T.cyl_rmin)
+        to[(size_t)p * T.stride + pq] = cyl_restrict(v4, T.cyl_rmin[p], 2 * L.dr, ii);
+    else
+        to[(size_t)p * T.stride + pq] = R(w) * sum;
 }
 
 // a$D_prolong2 (m_aX_prolong.f90:243-348) from the parent into every box of

@@ -127,6 +127,11 @@ struct afg_ctx {
     std::vector<int32_t> slot_ix, slot_tree;   // box%ix (3 per slot) and tree index per slot
     std::vector<int32_t> slot_par;             // parent slot, -1 if none
     bool rb_parent_nb = true;            // every refinement-boundary face has a neighbour of the parent there
+    // cylindrical coordinates (2D): x is the radius, r_base the radius at ix = 1
+    bool cyl = false;
+    double r_base = 0;
+    int32_t *d_cyl_ioff = nullptr;
+    double *d_cyl_rmin = nullptr;
     std::vector<int32_t> h_nb;           // slot * nnb + face (afivo ids)
     std::vector<BcRec> bc;
     std::vector<int32_t> bc_of;          // slot * nnb + face -> bc record or -1
@@ -228,6 +233,8 @@ DevTree dev_tree(const afg_ctx *c)
     T.bc_vals = c->d_bc_vals;
     T.tag = c->d_tag;
     T.is_parent = c->d_is_parent;
+    T.cyl_ioff = c->d_cyl_ioff;
+    T.cyl_rmin = c->d_cyl_rmin;
     return T;
 }
 
@@ -303,6 +310,10 @@ void free_tree(afg_ctx *c)
     std::fill(c->d_fc.begin(), c->d_fc.end(), nullptr);
     hipFree(c->d_flux);
     c->d_flux = nullptr;
+    hipFree(c->d_cyl_ioff);
+    hipFree(c->d_cyl_rmin);
+    c->d_cyl_ioff = nullptr;
+    c->d_cyl_rmin = nullptr;
     c->flux_ready = false;
     hipFree(c->d_phi2);
     hipFree(c->d_phi2_32);
@@ -548,14 +559,17 @@ int make_mg(afg_ctx *c, const afg_mg_opts *o, MgT<R> *m)
     if (!o) return c->fail(AFG_ERR_ARG, "null options");
     if (!o->initialized) return c->fail(AFG_ERR_STATE, "check_mg: you haven't called mg%d_init_mg", c->dim);
     if (o->op != AFG_OP_LPL && o->op != AFG_OP_AUTO) return c->fail(AFG_ERR_UNSUPPORTED, "operator %d not supported on the device", o->op);
+    if (c->cyl && sizeof(R) == 4) return c->fail(AFG_ERR_UNSUPPORTED, "FP32 stages on a cylindrical tree");
     m->c = c;
     m->o = o;
     if (int e = ensure_var(c, o->i_phi, &m->V.phi)) return e;
     if (int e = ensure_var(c, o->i_rhs, &m->V.rhs)) return e;
     if (int e = ensure_var(c, o->i_tmp, &m->V.tmp)) return e;
     m->V.eps = nullptr;
-    m->V.op_auto = o->op == AFG_OP_AUTO;
-    if (m->V.op_auto) {
+    // the auto operators without eps leave every box untagged: the plain
+    // operators, except on a cylindrical tree (mg2_box_clpl, examples/poisson_cyl.f90)
+    m->V.op_auto = o->op == AFG_OP_AUTO && (o->i_eps != -1 || c->cyl);
+    if (m->V.op_auto && o->i_eps != -1) {
         R *e = nullptr;
         if (int r = ensure_var(c, o->i_eps, &e)) return r;
         m->V.eps = e;
@@ -837,7 +851,8 @@ int choose_bottom(afg_ctx *c)
     int64_t max_threads = 64;
     for (int k = 0; k < 4; k++) {
         int top = -1;
-        for (int li = 0; li < nl && li < MAX_BOTTOM_LEVELS; li++) {
+        // cylindrical trees run every level with per-level launches
+        for (int li = 0; li < nl && li < MAX_BOTTOM_LEVELS && !c->cyl; li++) {
             const Level &L = c->levels[li];
             const int64_t cells = (int64_t)L.n * (c->dim == 2 ? L.nc * L.nc : L.nc * L.nc * L.nc);
             if (cells > c->bottom_cells[k >= 2]) break;
@@ -1259,6 +1274,8 @@ int afg_set_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base)
     return afg_set_forest(c, n_boxes, recs, dr_base, nullptr);
 }
 
+static int build_cyl(afg_ctx *c);
+
 int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base, const int32_t *tree_of)
 {
     if (!c) return AFG_ERR_ARG;
@@ -1552,7 +1569,25 @@ int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base,
     HIP_TRY(c, hipMalloc(&c->d_partial, c->partial_cap * sizeof(double)));
     c->bc_dirty = true;
     if (int e = upload_bc(c)) return e;
+    if (int e = build_cyl(c)) return e;
     return 0;
+}
+
+int afg_set_coord(afg_ctx *c, int coord, double r_base)
+{
+    if (!c) return AFG_ERR_ARG;
+    if (coord != AFG_COORD_XYZ && coord != AFG_COORD_CYL) return c->fail(AFG_ERR_ARG, "unknown coordinate system %d", coord);
+    if (coord == AFG_COORD_CYL && c->dim != 2) return c->fail(AFG_ERR_ARG, "a$D_init: cannot have 3d cyl coords");
+    if (c->capturing) return c->fail(AFG_ERR_STATE, "coordinate change inside a graph capture");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->cyl = coord == AFG_COORD_CYL;
+    c->r_base = r_base;
+    for (auto &g : c->graphs) destroy_graph(g.second);
+    c->graphs.clear();
+    if (!c->have_tree) return 0;
+    if (int e = choose_bottom(c)) return e;
+    return build_cyl(c);
 }
 
 int afg_update_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base, const int32_t *tree_of,
@@ -1951,8 +1986,8 @@ int afg_mg_init(afg_mg_opts *o)
         g_create_error = o->i_phi < 1 ? "mg_init_mg: i_phi not set" : o->i_tmp < 1 ? "mg_init_mg: i_tmp not set" : "mg_init_mg: i_rhs not set";
         return AFG_ERR_ARG;
     }
-    if (o->op == AFG_OP_AUTO && o->i_eps < 1) {
-        g_create_error = "mg_init_mg: auto operators need i_eps";
+    if (o->op == AFG_OP_AUTO && o->i_eps != -1 && o->i_eps < 1) {
+        g_create_error = "mg_init_mg: invalid i_eps";
         return AFG_ERR_ARG;
     }
     if (o->n_cycle_down < 0) o->n_cycle_down = 2;
@@ -2288,6 +2323,39 @@ int afg_update_coarse(afg_ctx *c, const afg_mg_opts *o, int lvl)
     if (int e = make_mg(c, o, &m)) return e;
     set_tags(m);
     update_coarse(m, lvl, 0);
+    return 0;
+}
+
+// The per-slot tables of a cylindrical tree: (ix(1) - 1) * nc_l and
+// box%r_min(1), computed as the reference does: r_base + (ix - 1) dr_base
+// n_cell on levels <= 1 (m_aX_core.f90:195-196), the parent's plus
+// 0.5 dr_p dix n_cell_p for a child (:774-775); slots are in level order, so
+// parents come first.
+static int build_cyl(afg_ctx *c)
+{
+    hipFree(c->d_cyl_ioff);
+    hipFree(c->d_cyl_rmin);
+    c->d_cyl_ioff = nullptr;
+    c->d_cyl_rmin = nullptr;
+    if (!c->cyl || !c->have_tree) return 0;
+    const int ns = c->n_slots;
+    std::vector<int32_t> ioff(ns);
+    std::vector<double> rmin(ns);
+    for (int s = 0; s < ns; s++) {
+        const int ix = c->slot_ix[(size_t)s * 3], lvl = c->slot_lvl[s];
+        ioff[s] = (ix - 1) * c->slot_nc[s];
+        if (lvl <= 1) {
+            rmin[s] = c->r_base + (ix - 1) * c->dr_base * c->nc;
+        } else {
+            const int p = c->slot_par[s];
+            const double dr_p = c->levels[lvl - 1 - c->min_lvl].dr;
+            rmin[s] = rmin[p] + 0.5 * dr_p * ((ix - 1) & 1) * c->slot_nc[p];
+        }
+    }
+    HIP_TRY(c, hipMalloc(&c->d_cyl_ioff, ns * sizeof(int32_t)));
+    HIP_TRY(c, hipMalloc(&c->d_cyl_rmin, ns * sizeof(double)));
+    HIP_H2D(c, c->d_cyl_ioff, ioff.data(), ns * sizeof(int32_t));
+    HIP_H2D(c, c->d_cyl_rmin, rmin.data(), ns * sizeof(double));
     return 0;
 }
 

@@ -296,6 +296,10 @@ __global__ __launch_bounds__(BLOCK) void k_set_tags(DevTree T, int slot0, int le
 {
     const int slot = slot0 + blockIdx.x;
     if (T.tag[slot] != TAG_UNSET) return;
+    if (!eps) {   // auto operators without eps: every box untagged (mg%i_eps unset)
+        if (threadIdx.x == 0) T.tag[slot] = TAG_NORMAL;
+        return;
+    }
     const double *e = eps + (size_t)slot * T.stride;
     double lo = e[0], hi = e[0];
     for (int q = threadIdx.x; q < len; q += blockDim.x) {
@@ -518,6 +522,7 @@ template <typename R>
 int launch_gsrb_pair(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V,
                      const R *in, R *out, int rbA, int fly)
 {
+    if (T.cyl_rmin) return 0;
     if (L.n <= 0) return 1;
     switch (dim * 100 + L.nc) {
     case 208: gsrb_pair<2, 8, R>(s, T, L, V, in, out, rbA, fly); return 1;
@@ -533,7 +538,8 @@ void launch_gsrb_pass(int dim, hipStream_t s, const DevTree &T, const LevelArgs 
                       const MgVarsT<R> &V, int rb, int fly)
 {
     if (L.n <= 0) return;
-    switch (dim * 100 + L.nc) {
+    // cylindrical trees take the per-cell pass (the tiles hold Cartesian operators only)
+    switch (T.cyl_rmin ? 0 : dim * 100 + L.nc) {
     case 202: gsrb_tiles<2, 2, R>(s, T, L, V, rb, fly); return;
     case 204: gsrb_tiles<2, 4, R>(s, T, L, V, rb, fly); return;
     case 208: gsrb_tiles<2, 8, R>(s, T, L, V, rb, fly); return;
@@ -566,7 +572,7 @@ void launch_uc_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &
                      const MgVarsT<R> &V, int keep_child_tmp, int fly)
 {
     if (L.n <= 0) return;
-    if (L.nc == 8 || L.nc == 16) {
+    if ((L.nc == 8 || L.nc == 16) && !T.cyl_rmin) {
         auto go = [&](auto dv, auto nv) {
             constexpr int D = decltype(dv)::value, N = decltype(nv)::value;
             using G = UcTile<D, N>;

@@ -54,6 +54,10 @@ struct DevTree {
     const double *bc_vals;
     int8_t *tag;              // per slot (auto operators)
     const int8_t *is_parent;  // per slot: the box has children
+    // cylindrical coordinates (2D, x the radius; null when Cartesian): per
+    // slot (ix(1) - 1) * nc_l and box%r_min(1)
+    const int32_t *cyl_ioff;
+    const double *cyl_rmin;
 };
 
 // One level: slots [slot0, slot0 + n), nc cells per dimension, cell size dr;

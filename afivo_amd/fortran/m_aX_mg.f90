@@ -13,9 +13,11 @@
 !    phi, rhs and tmp back into box%cc (unless mg%device_resident is set).
 !  * The operator procedures (mg$D_box_lpl, mg$D_auto_op, ...) identify the
 !    operator family by procedure identity; they are not meant to be called
-!    directly and stop if they are.  Operator families the device does not
-!    implement (level set, cylindrical) and user-written operators stop in
-!    mg$D_fas_fmg / mg$D_fas_vcycle: there is no CPU fallback.
+!    directly and stop if they are.  A cylindrical tree (coord = a5_cyl)
+!    with the auto operators runs mg2_box_clpl / clpld on the device.
+!    Operator families the device does not implement (level set) and
+!    user-written operators stop in mg$D_fas_fmg / mg$D_fas_vcycle: there is
+!    no CPU fallback.
 !  * A user sides_bc is called on the host once per physical boundary face
 !    and solve, on a copy of the box, to capture the boundary values it
 !    writes (they may depend on the geometry, not on phi).
@@ -129,7 +131,7 @@ module m_a$D_mg
 
   ! device context of this module and the structure it holds
   type(c_ptr), save :: ctx = c_null_ptr
-  integer, save :: ctx_nc = -1, ctx_nvar = -1
+  integer, save :: ctx_nc = -1, ctx_nvar = -1, ctx_coord = -1
   integer(c_int32_t), allocatable, save :: recs_dev(:)
   integer(c_int32_t), allocatable, save :: ids_dev(:)
   integer(c_int64_t), save :: n_values = 0
@@ -295,6 +297,7 @@ contains
     ctx = c_null_ptr
     ctx_nc = -1
     ctx_nvar = -1
+    ctx_coord = -1
     if (allocated(recs_dev)) deallocate(recs_dev)
     if (allocated(ids_dev)) deallocate(ids_dev)
   end subroutine mg$D_device_release
@@ -342,12 +345,17 @@ contains
        recs_dev = recs
        ids_dev = recs(1::rec_len)
     end if
+    if (tree%coord_t /= ctx_coord) then
+       call afg_check(afg_set_coord(ctx, merge(AFG_COORD_CYL, AFG_COORD_XYZ, tree%coord_t == a5_cyl), &
+            real(tree%r_base(1), c_double)), ctx, "afg_set_coord")
+       ctx_coord = tree%coord_t
+    end if
 
     if (new_tree .or. .not. mg%device_resident) then
        call upload_var(tree, mg%i_phi)
        call upload_var(tree, mg%i_rhs)
        call upload_var(tree, mg%i_tmp)
-       if (o%op == AFG_OP_AUTO) call upload_var(tree, mg%i_eps)
+       if (o%op == AFG_OP_AUTO .and. mg%i_eps > 0) call upload_var(tree, mg%i_eps)
     end if
     call set_boundaries(tree, mg)
   end subroutine device_prepare
@@ -366,13 +374,12 @@ contains
          associated(mg%box_corr, mg$D_auto_corr) .and. &
          (associated(mg%box_rstr, mg$D_box_rstr_lpl) .or. &
          associated(mg%box_rstr, mg$D_auto_rstr))) then
-       if (mg%i_eps < 0) error stop "mg$D: the auto operators need i_eps"
        if (mg%i_lsf /= -1) &
             error stop "mg$D: level-set boxes are not implemented on the device"
        op = AFG_OP_AUTO
     else
        error stop "mg$D: operator set not implemented on the device " // &
-            "(supported: the lpl set, and auto_op/auto_gsrb/auto_corr with eps)"
+            "(supported: the lpl set, and auto_op/auto_gsrb/auto_corr)"
     end if
   end function operator_family
 

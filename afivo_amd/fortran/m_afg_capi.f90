@@ -8,6 +8,7 @@ module m_afg_capi
 
   integer(c_int), parameter :: AFG_OP_LPL = 0, AFG_OP_AUTO = 1
   integer(c_int), parameter :: AFG_BC_DIRICHLET = -1, AFG_BC_NEUMANN = -2
+  integer(c_int), parameter :: AFG_COORD_XYZ = 0, AFG_COORD_CYL = 1
 
   !> afg_mg_opts of include/afivo_amd.h
   type, bind(c) :: afg_mg_opts
@@ -41,6 +42,13 @@ module m_afg_capi
        integer(c_int32_t), intent(in) :: records(*)
        real(c_double), value :: dr_base
      end function afg_set_tree
+
+     integer(c_int) function afg_set_coord(ctx, coord, r_base) bind(c, name="afg_set_coord")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: coord
+       real(c_double), value :: r_base
+     end function afg_set_coord
 
      integer(c_int) function afg_update_tree(ctx, n_boxes, records, dr_base, tree_of, prolong_order, &
           n_vars, vars) bind(c, name="afg_update_tree")

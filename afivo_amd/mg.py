@@ -58,6 +58,11 @@ class DeviceTree:
         check(L.afg_packed_size(self.h, self.ids.size, _i32(self.ids), ctypes.byref(n)), self.h)
         self.n_values = n.value
 
+    def set_coord(self, cyl, r_base=0.0):
+        """Cylindrical coordinates (2D; x the radius; the auto operators become
+        mg2_box_clpl / clpld) or Cartesian (afg_set_coord)."""
+        self._c(_lib.load().afg_set_coord(self.h, 1 if cyl else 0, float(r_base)))
+
     # ghost cells and transfers of any variable
     def gc_tree(self, iv, rb=0, bc=0):
         """a$D_gc_tree: rb 0 mg sides_rb, 1 gc_prolong0, 2 gc_interp, 3
@@ -361,9 +366,11 @@ def max_abs_cc(tree: DeviceTree, iv: int) -> float:
     return max(abs(tree_min_cc(tree, iv)), abs(tree_max_cc(tree, iv)))
 
 
-def default_mg(eps: bool = False, n_down=2, n_up=2, n_base=2):
+def default_mg(eps: bool = False, n_down=2, n_up=2, n_base=2, auto: bool = False):
     """Options with phi, rhs, tmp (and eps) in variables 1, 2, 3 (4), as the
-    problems of afivo_amd.problems store them."""
+    problems of afivo_amd.problems store them; the auto operators when eps
+    or auto (auto without eps: every box untagged, as for a cylindrical tree
+    in examples/poisson_cyl.f90)."""
     mg = MgOpts(i_phi=1, i_rhs=2, i_tmp=3, i_eps=4 if eps else -1, n_cycle_down=n_down,
-                n_cycle_up=n_up, n_cycle_base=n_base, op=AFG_OP_AUTO if eps else AFG_OP_LPL)
+                n_cycle_up=n_up, n_cycle_base=n_base, op=AFG_OP_AUTO if (eps or auto) else AFG_OP_LPL)
     return mg_init_mg(mg)

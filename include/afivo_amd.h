@@ -50,6 +50,9 @@ enum afg_op {
 /* Physical boundary types (a5_bc_dirichlet, a5_bc_neumann; m_aX_t.f90:45-48). */
 enum { AFG_BC_DIRICHLET = -1, AFG_BC_NEUMANN = -2 };
 
+/* Coordinate systems (a5_xyz, a5_cyl; m_aX_t.f90:50-53). */
+enum { AFG_COORD_XYZ = 0, AFG_COORD_CYL = 1 };
+
 /* Refinement-boundary methods of afg_gc_tree / afg_gc_level (the subr_rb
  * argument of a$D_gc_tree, m_aX_gc.f90:28-69). */
 enum {
@@ -125,6 +128,16 @@ int afg_set_forest(afg_ctx *ctx, int n_boxes, const int32_t *records, double dr_
  * (as after a failed afg_set_forest). */
 int afg_update_tree(afg_ctx *ctx, int n_boxes, const int32_t *records, double dr_base,
                     const int32_t *tree_of, int prolong_order, int n_vars, const int32_t *vars);
+/* The coordinate system of the trees of the context (the coord argument of
+ * a$D_init, m_aX_core.f90:24-131; default AFG_COORD_XYZ).  AFG_COORD_CYL (2D
+ * only; x is the radius, r_base the radius at the low side of ix = 1):
+ * the auto operators (AFG_OP_AUTO) become mg2_box_clpl / mg2_box_gsrb_clpl
+ * for boxes without an eps tag and mg2_box_clpld / mg2_box_gsrb_clpld for eps
+ * boxes (m_aX_mg.f90:1323-1443), and every restriction is the
+ * radius-weighted one of a$D_restrict_box (m_aX_restrict.f90:90-103).
+ * Cylindrical trees run the per-cell kernels (no tiles, fused pass pairs or
+ * bottom kernel) and FP64 only.  The setting survives afg_set_tree. */
+int afg_set_coord(afg_ctx *ctx, int coord, double r_base);
 /* Number of trees and each tree's highest level (highest may be NULL). */
 int afg_forest_trees(const afg_ctx *ctx, int *n_trees, int32_t *highest);
 /* Per tree: max |variable iv| over its leaf interiors, i.e. max(|min|, |max|)

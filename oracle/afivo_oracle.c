@@ -40,6 +40,9 @@ struct orc_tree {
     size_t *foff;
     size_t n_face;
     double *fc[ORC_MAX_FC];
+    /* cylindrical coordinates (2D, x the radius): box%r_min(1) per id */
+    int cyl;
+    double *rmin;
 };
 
 #define TAG_UNSET 0
@@ -163,6 +166,7 @@ void orc_destroy(orc_tree *t)
     free(t->ncell); free(t->tag); free(t->dr); free(t->off); free(t->rec_ids);
     free(t->lvl_start);
     free(t->foff);
+    free(t->rmin);
     for (int v = 0; v < ORC_MAX_FC; v++) free(t->fc[v]);
     free(t);
 }
@@ -593,7 +597,20 @@ static void restrict_box(orc_tree *t, int c_id, int p_id, int iv, int iv_to)
     child_offset(t, c_id, off);
     const double *c = cc(t, c_id, iv);
     double *p = cc(t, p_id, iv_to);
-    if (t->dim == 2) {
+    if (t->dim == 2 && t->cyl) {
+        /* cylindrical branch, m_aX_restrict.f90:90-103; the weights are those
+         * of the parent's cell i (not i_c), a2_cyl_child_weights
+         * (m_aX_t.f90:519-528) */
+        for (int j = 1; j <= hnc; j++)
+            for (int i = 1; i <= hnc; i++) {
+                int f = 2 * i - 1, g = 2 * j - 1;
+                double r = t->rmin[p_id] + (i - 0.5) * t->dr[p_id];
+                double tmp = 0.25 * t->dr[p_id] / r, w1 = 1 - tmp, w2 = 1 + tmp;
+                p[ix3(np, off[0] + i, off[1] + j, 0)] =
+                    0.25 * (w1 * (c[ix3(n, f, g, 0)] + c[ix3(n, f, g + 1, 0)]) +
+                            w2 * (c[ix3(n, f + 1, g, 0)] + c[ix3(n, f + 1, g + 1, 0)]));
+            }
+    } else if (t->dim == 2) {
         for (int j = 1; j <= hnc; j++)
             for (int i = 1; i <= hnc; i++) {
                 int f = 2 * i - 1, g = 2 * j - 1;
@@ -688,6 +705,10 @@ static void prolong2(orc_tree *t, int p_id, int c_id, int iv, int ivc, int add)
 /* mg$D_set_box_tag, m_aX_mg.f90:667-706 (eps part; no level set) */
 static void set_box_tag(orc_tree *t, int id, const orc_mg_opts *o)
 {
+    if (o->i_eps < 1) {   /* mg%i_eps unset: no eps tag (m_aX_mg.f90:686) */
+        t->tag[id] = TAG_NORMAL;
+        return;
+    }
     const double *e = cc(t, id, o->i_eps);
     int len = box_len(t, t->ncell[id]);
     double a = e[0], b = e[0];
@@ -860,9 +881,133 @@ static void corr_lpld(orc_tree *t, int p_id, int c_id, const orc_mg_opts *o)
     }
 }
 
+/* --------------------------------------------------------------------- */
+/* cylindrical coordinates (2D, x the radius)                             */
+
+/* box%r_min(1) of every box as the reference computes it: r_base +
+ * (ix-1) dr_base n_cell on levels <= 1 (m_aX_core.f90:195-196), the parent's
+ * plus 0.5 dr_p dix n_cell for a child (:774-775). */
+int orc_set_coord(orc_tree *t, int cyl, double r_base)
+{
+    if (cyl && t->dim != 2) return -1;
+    t->cyl = cyl;
+    free(t->rmin);
+    t->rmin = NULL;
+    if (!cyl) return 0;
+    t->rmin = calloc((size_t)t->max_id + 1, sizeof(double));
+    for (int r = 0; r < t->n_boxes; r++) {   /* level order: parents first */
+        int id = t->rec_ids[r], ix = t->ix[id * 3];
+        if (t->lvl[id] <= 1)
+            t->rmin[id] = r_base + (ix - 1) * t->dr_base * t->nc;
+        else {
+            int p = t->parent[id];
+            t->rmin[id] = t->rmin[p] + 0.5 * t->dr[p] * ((ix - 1) & 1) * t->ncell[p];
+        }
+    }
+    return 0;
+}
+
+/* rfac = [i+ioff-1, i+ioff] / (i+ioff-0.5), ioff = (ix(1)-1) n_cell
+ * (m_aX_mg.f90:1338,1345) */
+static void cyl_rfac(const orc_tree *t, int id, int i, double rf[2])
+{
+    int ir = i + (t->ix[id * 3] - 1) * t->ncell[id];
+    double d = ir - 0.5;
+    rf[0] = (ir - 1) / d;
+    rf[1] = ir / d;
+}
+
+/* mg2_box_gsrb_clpl, m_aX_mg.f90:1325-1351 */
+static void gsrb_clpl(orc_tree *t, int id, int rb, const orc_mg_opts *o)
+{
+    int n = t->ncell[id], s = n + 2;
+    double dx2 = t->dr[id] * t->dr[id];
+    double *p = cc(t, id, o->i_phi);
+    const double *r = cc(t, id, o->i_rhs);
+    for (int j = 1; j <= n; j++) {
+        int i0 = 2 - ((rb ^ j) & 1);
+        for (int i = i0; i <= n; i += 2) {
+            int q = ix3(n, i, j, 0);
+            double rf[2];
+            cyl_rfac(t, id, i, rf);
+            p[q] = 0.25 * (rf[0] * p[q - 1] + rf[1] * p[q + 1] + p[q + s] + p[q - s] - dx2 * r[q]);
+        }
+    }
+}
+
+/* mg2_box_gsrb_clpld, m_aX_mg.f90:1411-1443 */
+static void gsrb_clpld(orc_tree *t, int id, int rb, const orc_mg_opts *o)
+{
+    int n = t->ncell[id], s = n + 2;
+    double dx2 = t->dr[id] * t->dr[id];
+    double *p = cc(t, id, o->i_phi);
+    const double *r = cc(t, id, o->i_rhs);
+    const double *e = cc(t, id, o->i_eps);
+    int dq[4] = {-1, 1, -s, s};
+    for (int j = 1; j <= n; j++) {
+        int i0 = 2 - ((rb ^ j) & 1);
+        for (int i = i0; i <= n; i += 2) {
+            int q = ix3(n, i, j, 0);
+            double rf[4] = {0, 0, 1, 1}, c[4], su, sc;
+            cyl_rfac(t, id, i, rf);
+            double a0 = e[q];
+            for (int m = 0; m < 4; m++) c[m] = 2 * a0 * e[q + dq[m]] / (a0 + e[q + dq[m]]);
+            su = c[0] * rf[0] * p[q + dq[0]];
+            for (int m = 1; m < 4; m++) su += c[m] * rf[m] * p[q + dq[m]];
+            sc = c[0] * rf[0];
+            for (int m = 1; m < 4; m++) sc += c[m] * rf[m];
+            p[q] = (su - dx2 * r[q]) / sc;
+        }
+    }
+}
+
+/* mg2_box_clpl, m_aX_mg.f90:1353-1376 */
+static void box_clpl(orc_tree *t, int id, int i_out, const orc_mg_opts *o)
+{
+    int n = t->ncell[id], s = n + 2;
+    double inv_dr_sq = 1 / (t->dr[id] * t->dr[id]);
+    const double *p = cc(t, id, o->i_phi);
+    double *out = cc(t, id, i_out);
+    for (int j = 1; j <= n; j++)
+        for (int i = 1; i <= n; i++) {
+            int q = ix3(n, i, j, 0);
+            double rf[2];
+            cyl_rfac(t, id, i, rf);
+            out[q] = (rf[0] * p[q - 1] + rf[1] * p[q + 1] + p[q - s] + p[q + s] - 4 * p[q]) * inv_dr_sq;
+        }
+}
+
+/* mg2_box_clpld, m_aX_mg.f90:1378-1408 */
+static void box_clpld(orc_tree *t, int id, int i_out, const orc_mg_opts *o)
+{
+    int n = t->ncell[id], s = n + 2;
+    double inv_dr_sq = 1 / (t->dr[id] * t->dr[id]);
+    const double *p = cc(t, id, o->i_phi);
+    const double *e = cc(t, id, o->i_eps);
+    double *out = cc(t, id, i_out);
+    int dq[4] = {-1, 1, -s, s};
+    for (int j = 1; j <= n; j++)
+        for (int i = 1; i <= n; i++) {
+            int q = ix3(n, i, j, 0);
+            double rf[4] = {0, 0, 1, 1}, u0 = p[q], a0 = e[q], sum = 0;
+            cyl_rfac(t, id, i, rf);
+            for (int m = 0; m < 4; m++) {
+                double a = e[q + dq[m]];
+                double term = rf[m] * a0 * a / (a0 + a) * (p[q + dq[m]] - u0);
+                sum = m == 0 ? term : sum + term;
+            }
+            out[q] = inv_dr_sq * 2 * sum;
+        }
+}
+
 /* mg%box_gsrb: mg$D_box_gsrb_lpl or mg$D_auto_gsrb (m_aX_mg.f90:562-590) */
 static void box_gsrb(orc_tree *t, int id, int rb, const orc_mg_opts *o)
 {
+    if (t->cyl && o->op == 1) {
+        if (use_lpld(t, id, o)) gsrb_clpld(t, id, rb, o);
+        else gsrb_clpl(t, id, rb, o);
+        return;
+    }
     if (use_lpld(t, id, o))
         gsrb_lpld(t, id, rb, o);
     else
@@ -872,6 +1017,11 @@ static void box_gsrb(orc_tree *t, int id, int rb, const orc_mg_opts *o)
 /* mg%box_op: mg$D_box_lpl or mg$D_auto_op (m_aX_mg.f90:593-627) */
 static void box_op(orc_tree *t, int id, int i_out, const orc_mg_opts *o)
 {
+    if (t->cyl && o->op == 1) {
+        if (use_lpld(t, id, o)) box_clpld(t, id, i_out, o);
+        else box_clpl(t, id, i_out, o);
+        return;
+    }
     if (use_lpld(t, id, o))
         box_lpld(t, id, i_out, o);
     else
@@ -1143,7 +1293,7 @@ static int check_opts(const orc_tree *t, const orc_mg_opts *o)
     if (o->i_phi < 1 || o->i_phi > t->n_var || o->i_rhs < 1 || o->i_rhs > t->n_var ||
         o->i_tmp < 1 || o->i_tmp > t->n_var)
         return -1;
-    if (o->op == 1 && (o->i_eps < 1 || o->i_eps > t->n_var)) return -1;
+    if (o->op == 1 && o->i_eps != -1 && (o->i_eps < 1 || o->i_eps > t->n_var)) return -1;
     if (o->n_cycle_down < 0 || o->n_cycle_up < 0 || o->n_cycle_base < 0) return -1;
     return 0;
 }

@@ -46,6 +46,11 @@ int orc_get_cc(const orc_tree *t, int iv, double *packed);
  * boundary values b (nc_l^(dim-1), x fastest), or NULL for b = 0. */
 int orc_set_bc(orc_tree *t, int id, int nb, int bc_type, const double *b);
 
+/* Cylindrical coordinates (2D only; x is the radius, a5_cyl): the auto
+ * operators become mg2_box_clpl / clpld and the restriction is weighted by
+ * radius; r_base is the tree's r_min(1) at ix = 1. */
+int orc_set_coord(orc_tree *t, int cyl, double r_base);
+
 int orc_fas_fmg(orc_tree *t, const orc_mg_opts *o, int set_residual, int have_guess);
 int orc_fas_vcycle(orc_tree *t, const orc_mg_opts *o, int highest_lvl, int set_residual);
 double orc_tree_max_cc(const orc_tree *t, int iv, int include_parents);

@@ -56,6 +56,7 @@ def lib():
         L.orc_prolong2_level.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_correct_children_level.argtypes = [P, ctypes.POINTER(MgOpts), ctypes.c_int]
         L.orc_gc_tree.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_set_coord.argtypes = [P, ctypes.c_int, ctypes.c_double]
         L.orc_n_face_values.argtypes = [P]
         L.orc_set_fc.argtypes = [P, ctypes.c_int, f64p]
         L.orc_get_fc.argtypes = [P, ctypes.c_int, f64p]
@@ -121,6 +122,11 @@ class OracleTree:
         if lib().orc_fas_vcycle(self.h, ctypes.byref(opts), hl, int(set_residual)):
             raise ValueError("orc_fas_vcycle failed")
 
+    def set_coord(self, cyl, r_base=0.0):
+        """Cylindrical coordinates (2D; x the radius) or Cartesian."""
+        if lib().orc_set_coord(self.h, int(cyl), float(r_base)):
+            raise ValueError("orc_set_coord failed")
+
     def gc_tree(self, iv, rb, bc=0):
         """a$D_gc_tree: rb 0 mg sides_rb, 1 gc_prolong0, 2 gc_interp, 3
         gc_interp_lim; bc 0 the boundary table, BC_DIRICHLET / BC_NEUMANN
@@ -166,8 +172,10 @@ BC_DIRICHLET, BC_NEUMANN = -1, -2             # ORC_BC_* (afivo's a5_bc_dirichle
 RB_MG, RB_PROLONG0, RB_INTERP, RB_INTERP_LIM = 0, 1, 2, 3
 
 
-def default_opts(n_down=2, n_up=2, n_base=2, eps=False):
-    return MgOpts(1, 2, 3, 4 if eps else -1, n_down, n_up, n_base, 1 if eps else 0)
+def default_opts(n_down=2, n_up=2, n_base=2, eps=False, auto=False):
+    """Options of the lpl operators, or of the auto operators when eps (i_eps
+    4) or auto (no eps: every box untagged, as examples/poisson_cyl.f90)."""
+    return MgOpts(1, 2, 3, 4 if eps else -1, n_down, n_up, n_base, 1 if (eps or auto) else 0)
 
 
 # ---------------------------------------------------------------------------

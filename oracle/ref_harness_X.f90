@@ -67,11 +67,14 @@ program ref_harness_$Dd
   ! from phi or rhs and then changed by one ghost-cell or transfer routine of
   ! afivo, the tree written after each (see transfer_dumps)
   logical            :: xfer = .false.
+  ! cylindrical coordinates (2D: x is the radius, a5_cyl), solved with the
+  ! auto operators as examples/poisson_cyl.f90:41-69 does
+  logical            :: cyl = .false.
 
   namelist /cfg/ nc, ref_mode, l_uniform, l_max, thr, n_gauss, ampl, sigma, &
        r0, list_file, n_down, n_up, n_base, n_fmg, n_vcycle, use_eps, eps_hi, &
        eps_cut, out_file, dump_input, dump_fmg, dump_output, n_time, refine2, &
-       refine2_lmax, refine2_thr, refine2_rm_thr, refine2_nvar, xfer
+       refine2_lmax, refine2_thr, refine2_rm_thr, refine2_nvar, xfer, cyl
 
   type(a$D_t)         :: tree, tree_init
   type(ref_info_t)    :: ref_info
@@ -93,7 +96,12 @@ program ref_harness_$Dd
 
   if (ref_mode == 'list') call read_list()
 
+#if $D == 2
   call a$D_init(tree, nc, n_var, merge(1, 0, xfer), 1.0_dp / nc, coarsen_to=2, &
+       coord=merge(a5_cyl, a5_xyz, cyl), &
+#else
+  call a$D_init(tree, nc, n_var, merge(1, 0, xfer), 1.0_dp / nc, coarsen_to=2, &
+#endif
        mem_limit_gb=64.0_dp)
   ix_list(:, 1) = 1
   nb_list(:, 1) = -1
@@ -117,8 +125,8 @@ program ref_harness_$Dd
   mg%n_cycle_down = n_down
   mg%n_cycle_up   = n_up
   mg%n_cycle_base = n_base
-  if (use_eps) then
-     mg%i_eps    = i_eps
+  if (use_eps .or. cyl) then
+     if (use_eps) mg%i_eps = i_eps
      mg%box_op   => mg$D_auto_op
      mg%box_gsrb => mg$D_auto_gsrb
      mg%box_corr => mg$D_auto_corr

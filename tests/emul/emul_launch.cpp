@@ -200,6 +200,7 @@ template <typename R>
 int launch_gsrb_pair(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V,
                      const R *in, R *out, int rbA, int fly)
 {
+    if (T.cyl_rmin) return 0;
     if (L.n <= 0) return 1;
     const int key = dim * 100 + L.nc;
     if (key != 208 && key != 216 && key != 308 && key != 316) return 0;
@@ -220,7 +221,7 @@ void launch_gsrb_pass(int dim, hipStream_t s, const DevTree &T, const LevelArgs 
 {
     if (L.n <= 0) return;
     emul_submit(s, [=] {
-        switch (dim * 100 + L.nc) {
+        switch (T.cyl_rmin ? 0 : dim * 100 + L.nc) {
         case 202: gsrb_tiles<2, 2, R>(T, L, V, rb, fly); return;
         case 204: gsrb_tiles<2, 4, R>(T, L, V, rb, fly); return;
         case 208: gsrb_tiles<2, 8, R>(T, L, V, rb, fly); return;
@@ -253,7 +254,7 @@ void launch_uc_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &
                      const MgVarsT<R> &V, int keep_child_tmp, int fly)
 {
     if (L.n <= 0) return;
-    if (L.nc == 8 || L.nc == 16) {
+    if ((L.nc == 8 || L.nc == 16) && !T.cyl_rmin) {
         emul_submit(s, [=] {
             if (dim == 2)
                 L.nc == 8 ? uc_tiles<2, 8, R>(T, L, V, keep_child_tmp, fly) : uc_tiles<2, 16, R>(T, L, V, keep_child_tmp, fly);
@@ -442,6 +443,10 @@ void launch_set_tags(int dim, hipStream_t s, const DevTree &T, const LevelArgs &
         for (int b = 0; b < L.n; b++) {
             const int slot = L.slot0 + b;
             if (T.tag[slot] != TAG_UNSET) continue;
+            if (!eps) {
+                T.tag[slot] = TAG_NORMAL;
+                continue;
+            }
             const double *e = eps + (size_t)slot * T.stride;
             double lo = e[0], hi = e[0];
             for (int q = 0; q < len; q++) {

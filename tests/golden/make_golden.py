@@ -51,6 +51,12 @@ CASES = {
     # 3D, 4^3 boxes, auto operators with a jump in eps at z = 0.5
     "g3d_nc4_eps": (3, dict(nc=4, l_uniform=2, l_max=3, thr=2e-3, n_base=4, n_fmg=1, n_vcycle=1,
                             use_eps=True, eps_hi=5.0, **G3)),
+    # 2D, 4x4 boxes, cylindrical coordinates (x the radius), auto operators:
+    # mg2_box_clpl / gsrb_clpl, cylindrical restriction
+    "g2d_nc4_cyl": (2, dict(nc=4, l_uniform=2, l_max=5, thr=2e-3, n_base=4, n_fmg=1, n_vcycle=2, cyl=True, **G2)),
+    # as above with eps = 1 below y = 0.5 and 5 above: mg2_box_clpld / gsrb_clpld
+    "g2d_nc4_cyl_eps": (2, dict(nc=4, l_uniform=2, l_max=5, thr=2e-3, n_base=4, n_fmg=1, n_vcycle=2, cyl=True,
+                                use_eps=True, eps_hi=5.0, **G2)),
     # 3D, 4^3 boxes, constant coefficient
     "g3d_nc4": (3, dict(nc=4, l_uniform=1, l_max=3, thr=5e-3, n_base=4, n_fmg=1, n_vcycle=1, **G3)),
     # 3D, 4^3 boxes, adaptive (42 boxes, 32 faces at refinement boundaries)

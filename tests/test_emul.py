@@ -30,6 +30,7 @@ from test_gpu_parity import (  # noqa: E402,F401
     test_bottom_kernel_limit_is_exact,
     test_c1_full_config_bitexact,
     test_c2_full_config_bitexact,
+    test_cylindrical_solve_bitexact,
     test_golden_solve_bitexact,
     test_graphs_and_direct_launches_agree,
     test_periodic_base,

@@ -74,6 +74,11 @@ CASES = {
                    n_fmg=1, n_vcycle=2, **G3)),
     "3d_eps": (3, dict(nc=4, ref_mode="gauss", l_uniform=2, l_max=4, thr=1e-3, n_base=4,
                        n_fmg=2, n_vcycle=1, use_eps=True, eps_hi=6.0, **G3)),
+    # cylindrical trees with the auto operators (mg2_box_clpl, and clpld on eps boxes)
+    "2d_cyl": (2, dict(nc=8, ref_mode="gauss", l_uniform=4, l_max=7, thr=1e-4, n_base=50,
+                       n_fmg=1, n_vcycle=3, cyl=True, **G2)),
+    "2d_cyl_eps": (2, dict(nc=8, ref_mode="gauss", l_uniform=4, l_max=6, thr=1e-4, n_base=50,
+                           n_fmg=1, n_vcycle=3, cyl=True, use_eps=True, eps_hi=4.0, **G2)),
 }
 
 

@@ -15,6 +15,7 @@ import oracle as O
 from afivo_amd import mg as M
 from afivo_amd import problems as P
 from afivo_amd import tree as T
+from afivo_amd._lib import AfgError
 from helpers import assert_bitexact, random_state
 from test_oracle import CASES, load_golden, opts_of
 
@@ -134,8 +135,11 @@ def test_golden_solve_bitexact(name):
     cfg, st_in, st_out, res_ref = load_golden(name)
     c = cfg["cfg"]
     eps = c.get("use_eps", False)
-    mg = M.default_mg(eps=eps, n_down=c.get("n_down", 2), n_up=c.get("n_up", 2), n_base=c["n_base"])
+    mg = M.default_mg(eps=eps, n_down=c.get("n_down", 2), n_up=c.get("n_up", 2), n_base=c["n_base"],
+                      auto=c.get("cyl", False))
     dt = M.DeviceTree.from_state(st_in)
+    if c.get("cyl"):
+        dt.set_coord(True)
     res = []
     for it in range(c.get("n_fmg", 1)):
         M.mg_fas_fmg(dt, mg, True, it > 0)
@@ -394,3 +398,30 @@ def test_pass_pairs_are_exact(kind, n_down, n_up):
         d.close()
         dt.close()
     assert_bitexact(res[0], res[1], f"{kind} FP32 pairs")
+
+
+@pytest.mark.parametrize("eps", [False, True])
+def test_cylindrical_solve_bitexact(eps):
+    """The 2D benchmark tree (951 boxes of 8^2, levels -1..7) in cylindrical
+    coordinates with the auto operators (mg2_box_clpl; mg2_box_clpld on the
+    boxes of an eps jump at y = 0.5), FMG (twice, the second from the guess)
+    and 3 V-cycles: every variable and the residual history bit for bit
+    against the oracle.  FP32 stages are refused on such a tree."""
+    p = P.c1()
+    st = p.state
+    if eps:
+        st.data.append(P.set_eps(st, p.tree, 3.0))
+    nv = len(st.data)
+    dt = M.DeviceTree.from_state(st, nv)
+    ot = O.OracleTree(st, n_var=nv)
+    dt.set_coord(True)
+    ot.set_coord(True)
+    mg = M.default_mg(eps=eps, n_base=50, auto=True)
+    om = O.default_opts(2, 2, 50, eps=eps, auto=True)
+    dt, ot, rd, ro = solve_both(st, nv, mg, om, 2, 3, dt, ot)
+    compare_all(dt, ot, nv, f"cylindrical eps={eps}")
+    assert np.array_equal(rd, ro)
+    assert rd[-1] < 1e-3 * rd[0]
+    with pytest.raises(AfgError):
+        M.mixed_solve(dt, mg, 1e-8, max_iter=5)
+    dt.close()

@@ -53,7 +53,8 @@ def load_xfer(name):
 
 def opts_of(cfg):
     c = cfg["cfg"]
-    return O.default_opts(c.get("n_down", 2), c.get("n_up", 2), c["n_base"], eps=c.get("use_eps", False))
+    return O.default_opts(c.get("n_down", 2), c.get("n_up", 2), c["n_base"], eps=c.get("use_eps", False),
+                          auto=c.get("cyl", False))
 
 
 def run_solve(solver, cfg, res_fn):
@@ -73,6 +74,8 @@ def run_solve(solver, cfg, res_fn):
 def test_oracle_matches_reference_golden(name):
     cfg, st_in, st_out, res_ref = load_golden(name)
     ot = O.OracleTree(st_in)
+    if cfg["cfg"].get("cyl"):
+        ot.set_coord(True)
     res = run_solve(ot, cfg, lambda: ot.max_abs(3))
     for iv in range(1, st_out.n_var + 1):
         assert_bitexact(ot.get(iv), st_out.data[iv - 1], f"{name} var {iv}")
