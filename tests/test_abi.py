@@ -44,8 +44,13 @@ def test_mg_init_defaults_and_errors():
     bad = _lib.MgOpts(i_phi=1, i_rhs=2)
     assert L.afg_mg_init(ctypes.byref(bad)) != 0
     assert b"i_tmp not set" in L.afg_last_error(None)
-    bad = _lib.MgOpts(i_phi=1, i_rhs=2, i_tmp=3, op=_lib.AFG_OP_AUTO)
+    # the auto operators without eps are allowed (every box untagged, as
+    # examples/poisson_cyl.f90 uses them); an invalid eps index is not
+    ok = _lib.MgOpts(i_phi=1, i_rhs=2, i_tmp=3, i_eps=-1, op=_lib.AFG_OP_AUTO)
+    assert L.afg_mg_init(ctypes.byref(ok)) == 0
+    bad = _lib.MgOpts(i_phi=1, i_rhs=2, i_tmp=3, i_eps=0, op=_lib.AFG_OP_AUTO)
     assert L.afg_mg_init(ctypes.byref(bad)) != 0
+    assert b"i_eps" in L.afg_last_error(None)
 
 
 def test_ctx_create_argument_checks():
