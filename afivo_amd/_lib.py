@@ -18,7 +18,8 @@ I32P = ctypes.POINTER(ctypes.c_int32)
 I64P = ctypes.POINTER(ctypes.c_int64)
 F64P = ctypes.POINTER(ctypes.c_double)
 
-AFG_OP_LPL, AFG_OP_AUTO = 0, 1
+AFG_OP_LPL, AFG_OP_AUTO, AFG_OP_LSF = 0, 1, 2
+AFG_RSTR_LPL, AFG_RSTR_AUTO, AFG_RSTR_LSF = 0, 1, 2
 AFG_BC_DIRICHLET, AFG_BC_NEUMANN = -1, -2
 
 
@@ -26,12 +27,13 @@ class MgOpts(ctypes.Structure):
     """afg_mg_opts: the fields of mg$D_t (reference src/m_aX_mg.f90:20-52)."""
     _fields_ = [("i_phi", I32), ("i_rhs", I32), ("i_tmp", I32), ("i_eps", I32),
                 ("n_cycle_down", I32), ("n_cycle_up", I32), ("n_cycle_base", I32),
-                ("op", I32), ("initialized", I32), ("reserved", I32 * 7)]
+                ("op", I32), ("initialized", I32), ("i_lsf", I32), ("i_bval", I32), ("rstr", I32),
+                ("reserved", I32 * 4)]
 
     def __init__(self, i_phi=-1, i_rhs=-1, i_tmp=-1, i_eps=-1, n_cycle_down=-1,
-                 n_cycle_up=-1, n_cycle_base=-1, op=AFG_OP_LPL):
+                 n_cycle_up=-1, n_cycle_base=-1, op=AFG_OP_LPL, i_lsf=-1, i_bval=-1, rstr=AFG_RSTR_LPL):
         super().__init__(i_phi, i_rhs, i_tmp, i_eps, n_cycle_down, n_cycle_up, n_cycle_base,
-                         op, 0)
+                         op, 0, i_lsf, i_bval, rstr)
 
 
 # name -> (restype, argtypes); every symbol of include/afivo_amd.h

@@ -42,6 +42,9 @@ struct LevelViewT {
     R *phi, *rhs, *tmp;
     const R *eps;
     int slot0, stride;
+    // level set and boundary value of the lpllsf operators, with the modes of MgVarsT
+    const R *lsf = nullptr, *bval = nullptr;
+    int lsf_mode = 0, rstr_mode = 0;
     AFG_DEV size_t off(int slot) const
     {
         return (size_t)(slot - slot0) * (size_t)stride;
@@ -165,7 +168,113 @@ AFG_DEV R box_op(const R u[6], R u0, const R *e, int q, int s, double inv_dr_sq,
 
 AFG_DEV bool lpld_box(const DevTree &T, int op_auto, int slot)
 {
-    return op_auto && T.tag[slot] != TAG_NORMAL;
+    return op_auto && (T.tag[slot] == TAG_CEPS || T.tag[slot] == TAG_VEPS);
+}
+
+// ---- internal boundaries given by a level set (m_aX_mg.f90:987-1321) ----
+
+// the lpllsf operators for this box: every box (the lpllsf set), or the
+// boxes tagged mg_lsf_box (the auto set)
+template <typename R>
+AFG_DEV bool lsf_box(const DevTree &T, const LevelViewT<R> &v, int slot)
+{
+    return v.lsf_mode == 2 || (v.lsf_mode == 1 && T.tag[slot] == TAG_LSF);
+}
+
+// mg$D_box_rstr_lpllsf for this child: every box, or by its tag (mg$D_auto_rstr)
+template <typename R>
+AFG_DEV bool lsf_rstr_box(const DevTree &T, const LevelViewT<R> &v, int slot)
+{
+    return v.rstr_mode == 2 || (v.rstr_mode == 1 && T.tag[slot] == TAG_LSF);
+}
+
+// lsf_dist_val (m_aX_mg.f90:990-1017): distance in (0, 1] from a to the
+// neighbour b and the value there; where the level set changes sign between
+// them, the distance to the zero crossing and the boundary value
+// interpolated to it
+template <typename R>
+AFG_DEV void lsf_dist_val(R la, R ba, R lb, R vb, R bb, R &dist, R &val)
+{
+    if (la * lb < 0) {
+        const R d = la / (la - lb);
+        dist = d;
+        val = ba * (1 - d) + bb * d;
+    } else {
+        dist = 1;
+        val = vb;
+    }
+}
+
+// distances dd[1..2*DIM] and values val[1..] of the face neighbours u of
+// cell q (stencil order i-1, i+1, j-1, j+1, k-1, k+1); l, b: the box's level
+// set and boundary value
+template <int DIM, typename R>
+AFG_DEV void lsf_neighbours(const R u[6], const R *l, const R *b, int q, int s, R dd[7], R val[7])
+{
+    const int dq[6] = {-1, 1, -s, s, -s * s, s * s};
+#pragma unroll
+    for (int m = 0; m < 2 * DIM; m++)
+        lsf_dist_val(l[q], b[q], l[q + dq[m]], u[m], b[q + dq[m]], dd[m + 1], val[m + 1]);
+}
+
+// mg$D_box_gsrb_lpllsf (m_aX_mg.f90:1105-1179): new phi at cell q
+template <int DIM, typename R>
+AFG_DEV R gsrb_value_lsf(const R u[6], R rhs, const R *l, const R *b, int q, int s, double dx2)
+{
+    R d[7], v[7];
+    lsf_neighbours<DIM>(u, l, b, q, s, d, v);
+    if (DIM == 2) {
+        const R prod = d[1] * d[2] * d[3] * d[4];
+        return R(1) / (d[1] * d[2] + d[3] * d[4]) *
+               ((d[2] * v[1] + d[1] * v[2]) * d[3] * d[4] / (d[1] + d[2]) +
+                (d[4] * v[3] + d[3] * v[4]) * d[1] * d[2] / (d[3] + d[4]) - R(0.5) * prod * R(dx2) * rhs);
+    }
+    return R(1) / (R(1) / (d[1] * d[2]) + R(1) / (d[3] * d[4]) + R(1) / (d[5] * d[6])) *
+           ((d[2] * v[1] + d[1] * v[2]) / ((d[1] + d[2]) * d[1] * d[2]) +
+            (d[4] * v[3] + d[3] * v[4]) / ((d[3] + d[4]) * d[3] * d[4]) +
+            (d[6] * v[5] + d[5] * v[6]) / ((d[5] + d[6]) * d[5] * d[6]) - R(0.5) * R(dx2) * rhs);
+}
+
+// mg$D_box_lpllsf (m_aX_mg.f90:1182-1251): the operator at cell q with value f0
+template <int DIM, typename R>
+AFG_DEV R box_op_lsf(const R u[6], R f0, const R *l, const R *b, int q, int s, double inv_dr_sq)
+{
+    R d[7], v[7];
+    lsf_neighbours<DIM>(u, l, b, q, s, d, v);
+    R sum = (d[2] * v[1] + d[1] * v[2] - (d[1] + d[2]) * f0) / ((d[1] + d[2]) * d[1] * d[2]) +
+            (d[4] * v[3] + d[3] * v[4] - (d[3] + d[4]) * f0) / ((d[3] + d[4]) * d[3] * d[4]);
+    if (DIM == 3)
+        sum = sum + (d[6] * v[5] + d[5] * v[6] - (d[5] + d[6]) * f0) / ((d[5] + d[6]) * d[5] * d[6]);
+    return R(2) * R(inv_dr_sq) * sum;
+}
+
+// mg$D_box_rstr_lpllsf (m_aX_mg.f90:1254-1320) of one coarse cell: values v
+// and level set lc of its 2^DIM children in Fortran order, level set lp of
+// the parent cell.  Children on the parent cell's side of the interface
+// only, where some but not all are; 1 / n_ch is integer division there, so
+// 2..2^DIM-1 such children give 0 * sum.
+template <int DIM, typename R>
+AFG_DEV R restrict_lsf(const R v[8], const R lc[8], R lp)
+{
+    constexpr int NCH = 1 << DIM;
+    bool mk[8];
+    int n_ch = 0;
+#pragma unroll
+    for (int m = 0; m < NCH; m++) {
+        mk[m] = lp * lc[m] > 0;
+        n_ch += mk[m];
+    }
+    if (n_ch < NCH && n_ch > 0) {
+        R sm = 0;
+#pragma unroll
+        for (int m = 0; m < NCH; m++)
+            if (mk[m]) sm += v[m];
+        return R(1 / n_ch) * sm;
+    }
+    R sm = v[0];
+#pragma unroll
+    for (int m = 1; m < NCH; m++) sm += v[m];
+    return R(DIM == 2 ? 0.25 : 0.125) * sm;
 }
 
 // ---- cylindrical coordinates (2D, x the radius): the auto operators of a
@@ -373,6 +482,10 @@ AFG_DEV void gsrb_body(const DevTree &T, const LevelArgs &L, const LevelViewT<R>
     const R center = box[q];
     R u[6];
     stencil<DIM>(T, L, own, par, slot, box, nc, i, j, k, q, fly, center, u);
+    if (lsf_box(T, own, slot)) {
+        box[q] = gsrb_value_lsf<DIM>(u, own.rhs[base + q], own.lsf + base, own.bval + base, q, s, L.dx2);
+        return;
+    }
     const bool lpld = lpld_box(T, op_auto, slot);
     if (DIM == 2 && cyl_op(T, op_auto))
         box[q] = gsrb_value_cyl(u, own.rhs[base + q], lpld ? own.eps + base : nullptr, q, s, L.dx2, lpld,
@@ -734,7 +847,9 @@ AFG_DEV void residual_body(const DevTree &T, const LevelArgs &L, const LevelView
     const bool lpld = lpld_box(T, op_auto, slot);
     R u[6];
     stencil<DIM>(T, L, own, par, slot, box, nc, i, j, k, q, 0, box[q], u);
-    const R a = DIM == 2 && cyl_op(T, op_auto)
+    const R a = lsf_box(T, own, slot)
+                    ? box_op_lsf<DIM>(u, box[q], own.lsf + base, own.bval + base, q, s, L.inv_dr_sq)
+                : DIM == 2 && cyl_op(T, op_auto)
                     ? box_op_cyl(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld, T.cyl_ioff[slot] + i)
                     : box_op<DIM>(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld);
     own.tmp[base + q] = own.rhs[base + q] - a;
@@ -767,8 +882,9 @@ AFG_DEV void uc_child_body(const DevTree &T, const LevelArgs &L, const LevelView
     const R *ph = own.phi + base;
     const R *e = lpld ? own.eps + base : nullptr;
     const R *rh = own.rhs + base;
-    R sr = 0, sp = 0, vr[4], vp[4];
+    R sr = 0, sp = 0, vr[8], vp[8], lc[8];
     const bool cyl = DIM == 2 && cyl_op(T, op_auto);
+    const bool lsf = lsf_box(T, own, slot), lsf_rstr = lsf_rstr_box(T, own, slot);
 #pragma unroll
     for (int m = 0; m < (1 << DIM); m++) {
         const int fi = 2 * ii - 1 + (m & 1);
@@ -777,15 +893,15 @@ AFG_DEV void uc_child_body(const DevTree &T, const LevelArgs &L, const LevelView
         const int q = fi + s * (fj + s * fk);
         R u[6];
         stencil<DIM>(T, L, own, par, slot, ph, nc, fi, fj, fk, q, fly, ph[q], u);
-        const R res = rh[q] - (cyl ? box_op_cyl(u, ph[q], e, q, s, L.inv_dr_sq, lpld, T.cyl_ioff[slot] + fi)
-                                   : box_op<DIM>(u, ph[q], e, q, s, L.inv_dr_sq, lpld));
+        const R res = rh[q] - (lsf ? box_op_lsf<DIM>(u, ph[q], own.lsf + base, own.bval + base, q, s, L.inv_dr_sq)
+                               : cyl ? box_op_cyl(u, ph[q], e, q, s, L.inv_dr_sq, lpld, T.cyl_ioff[slot] + fi)
+                                     : box_op<DIM>(u, ph[q], e, q, s, L.inv_dr_sq, lpld));
         if (!keep_child_tmp) own.tmp[base + q] = res;
         sr = m == 0 ? res : sr + res;
         sp = m == 0 ? ph[q] : sp + ph[q];
-        if (DIM == 2) {
-            vr[m & 3] = res;
-            vp[m & 3] = ph[q];
-        }
+        vr[m] = res;
+        vp[m] = ph[q];
+        if (lsf_rstr) lc[m] = own.lsf[base + q];
     }
     const int p = T.parent[slot];
     const int co = T.choff[slot];
@@ -794,6 +910,12 @@ AFG_DEV void uc_child_body(const DevTree &T, const LevelArgs &L, const LevelView
                    ps * ((((co >> 10) & 1023) + jj) + ps * (DIM == 3 ? ((co >> 20) & 1023) + kk : 0));
     const size_t pb = par.off(p);
     constexpr double w = DIM == 2 ? 0.25 : 0.125;
+    if (lsf_rstr) {
+        const R lp = par.lsf[pb + pq];
+        par.tmp[pb + pq] = restrict_lsf<DIM>(vr, lc, lp);
+        par.phi[pb + pq] = restrict_lsf<DIM>(vp, lc, lp);
+        return;
+    }
     if (DIM == 2 && T.cyl_rmin) {
         par.tmp[pb + pq] = cyl_restrict(vr, T.cyl_rmin[p], 2 * L.dr, ii);
         par.phi[pb + pq] = cyl_restrict(vp, T.cyl_rmin[p], 2 * L.dr, ii);
@@ -983,7 +1105,9 @@ AFG_DEV void uc_parent_body(const DevTree &T, const LevelArgs &L, const LevelVie
         const bool lpld = lpld_box(T, op_auto, slot);
         R u[6];
         stencil<DIM>(T, L, own, par, slot, box, nc, i, j, k, q, 1, box[q], u);
-        const R a = DIM == 2 && cyl_op(T, op_auto)
+        const R a = lsf_box(T, own, slot)
+                        ? box_op_lsf<DIM>(u, box[q], own.lsf + base, own.bval + base, q, s, L.inv_dr_sq)
+                    : DIM == 2 && cyl_op(T, op_auto)
                         ? box_op_cyl(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld,
                                      T.cyl_ioff[slot] + i)
                         : box_op<DIM>(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld);
@@ -1073,7 +1197,42 @@ AFG_DEV void correct_body(const DevTree &T, const LevelArgs &L, const LevelViewT
     const R *pp = par.phi + pb;
     auto corr = [&](int q) -> R { return fly_diff ? pp[q] - pt[q] : pt[q]; };
     R *cph = own.phi + own.off(slot);
-    if (lpld_box(T, op_auto, slot)) {
+    if (lsf_box(T, own, slot)) {
+        // mg$D_box_corr_lpllsf (m_aX_mg.f90:1019-1102): the correction from the
+        // closest coarse cell and the one-but-closest in each direction, with
+        // boundary value 0 where the level set changes sign
+        const R *pl = par.lsf + pb;
+        const R *cl = own.lsf + own.off(slot);
+#pragma unroll
+        for (int m = 0; m < (1 << DIM); m++) {
+            const int sx = m & 1, sy = (m >> 1) & 1, sz = (m >> 2) & 1;
+            const int fi = 2 * ii - 1 + sx, fj = 2 * jj - 1 + sy;
+            const int fk = DIM == 3 ? 2 * kk - 1 + sz : 0;
+            const int fq = fi + s * (fj + s * fk);
+            const int qb[4] = {pq, pq + (sx ? 1 : -1), pq + (sy ? ps : -ps), pq + (sz ? ps * ps : -ps * ps)};
+            R d[5], v[5];
+#pragma unroll
+            for (int t = 0; t <= DIM; t++) lsf_dist_val(cl[fq], R(0), pl[qb[t]], corr(qb[t]), R(0), d[t + 1], v[t + 1]);
+            R c[5];
+            if (DIM == 2) {
+                c[1] = 2 * d[2] * d[3];
+                c[2] = d[1] * d[3];
+                c[3] = d[1] * d[2];
+            } else {
+                c[1] = d[2] * d[3] * d[4];
+                c[2] = d[1] * d[3] * d[4];
+                c[3] = d[1] * d[2] * d[4];
+                c[4] = d[1] * d[2] * d[3];
+            }
+            R sv = 0, sc = 0;   // sum(c * val) / sum(c), each sum from zero
+#pragma unroll
+            for (int t = 1; t <= DIM + 1; t++) {
+                sv += c[t] * v[t];
+                sc += c[t];
+            }
+            cph[fq] = cph[fq] + sv / sc;
+        }
+    } else if (lpld_box(T, op_auto, slot)) {
         const R *pe = par.eps + pb;
         constexpr double third = 1 / 3.0;
         const R u0 = corr(pq), a0 = pe[pq];
@@ -1116,8 +1275,11 @@ AFG_DEV void correct_body(const DevTree &T, const LevelArgs &L, const LevelViewT
 
 // a$D_restrict_box (m_aX_restrict.f90:67-132) of every box of a level into
 // its parent: one thread per coarse cell.
+// With lsf set, children selected by rstr_mode (mg%box_rstr, as in
+// lsf_rstr_box) take mg$D_box_rstr_lpllsf.
 template <int DIM, int NC, typename R>
-AFG_DEV void restrict_body(const DevTree &T, const LevelArgs &L, const R *from, R *to, int gid)
+AFG_DEV void restrict_body(const DevTree &T, const LevelArgs &L, const R *from, R *to, int gid,
+                           const R *lsf = nullptr, int rstr_mode = 0)
 {
     const int nc = NC ? NC : L.nc;
     const int hn = nc >> 1;
@@ -1130,14 +1292,16 @@ AFG_DEV void restrict_body(const DevTree &T, const LevelArgs &L, const R *from, 
     const int slot = L.slot0 + b;
     const int s = nc + 2;
     const R *cb = from + (size_t)slot * T.stride;
-    R sum = 0, v4[4];
+    const bool lsf_rstr = lsf && (rstr_mode == 2 || (rstr_mode == 1 && T.tag[slot] == TAG_LSF));
+    R sum = 0, v8[8], lc[8];
 #pragma unroll
     for (int m = 0; m < (1 << DIM); m++) {
         const int fi = 2 * ii - 1 + (m & 1), fj = 2 * jj - 1 + ((m >> 1) & 1);
         const int fk = DIM == 3 ? 2 * kk - 1 + ((m >> 2) & 1) : 0;
         const R v = cb[fi + s * (fj + s * fk)];
         sum = m == 0 ? v : sum + v;
-        if (DIM == 2) v4[m & 3] = v;
+        v8[m] = v;
+        if (lsf_rstr) lc[m] = lsf[(size_t)slot * T.stride + fi + s * (fj + s * fk)];
     }
     const int p = T.parent[slot];
     const int co = T.choff[slot];
@@ -1145,8 +1309,10 @@ AFG_DEV void restrict_body(const DevTree &T, const LevelArgs &L, const R *from, 
     const int pq = ((co & 1023) + ii) +
                    ps * ((((co >> 10) & 1023) + jj) + ps * (DIM == 3 ? ((co >> 20) & 1023) + kk : 0));
     constexpr double w = DIM == 2 ? 0.25 : 0.125;
-    if (DIM == 2 && T.cyl_rmin)
-        to[(size_t)p * T.stride + pq] = cyl_restrict(v4, T.cyl_rmin[p], 2 * L.dr, ii);
+    if (lsf_rstr)
+        to[(size_t)p * T.stride + pq] = restrict_lsf<DIM>(v8, lc, lsf[(size_t)p * T.stride + pq]);
+    else if (DIM == 2 && T.cyl_rmin)
+        to[(size_t)p * T.stride + pq] = cyl_restrict(v8, T.cyl_rmin[p], 2 * L.dr, ii);
     else
         to[(size_t)p * T.stride + pq] = R(w) * sum;
 }

@@ -558,8 +558,17 @@ int make_mg(afg_ctx *c, const afg_mg_opts *o, MgT<R> *m)
 {
     if (!o) return c->fail(AFG_ERR_ARG, "null options");
     if (!o->initialized) return c->fail(AFG_ERR_STATE, "check_mg: you haven't called mg%d_init_mg", c->dim);
-    if (o->op != AFG_OP_LPL && o->op != AFG_OP_AUTO) return c->fail(AFG_ERR_UNSUPPORTED, "operator %d not supported on the device", o->op);
+    if (o->op != AFG_OP_LPL && o->op != AFG_OP_AUTO && o->op != AFG_OP_LSF)
+        return c->fail(AFG_ERR_UNSUPPORTED, "operator %d not supported on the device", o->op);
+    if (o->rstr != AFG_RSTR_LPL && o->rstr != AFG_RSTR_AUTO && o->rstr != AFG_RSTR_LSF)
+        return c->fail(AFG_ERR_UNSUPPORTED, "restriction %d not supported on the device", o->rstr);
     if (c->cyl && sizeof(R) == 4) return c->fail(AFG_ERR_UNSUPPORTED, "FP32 stages on a cylindrical tree");
+    // the level set takes part with the lpllsf set, the auto set (its tags)
+    // and the level-set restrictions
+    const bool use_lsf = o->i_lsf > 0 && (o->op != AFG_OP_LPL || o->rstr != AFG_RSTR_LPL);
+    if ((o->op == AFG_OP_LSF || o->rstr == AFG_RSTR_LSF) && o->i_lsf <= 0)
+        return c->fail(AFG_ERR_ARG, "mg%d: the lpllsf operators need i_lsf and i_bval", c->dim);
+    if (use_lsf && sizeof(R) == 4) return c->fail(AFG_ERR_UNSUPPORTED, "FP32 stages with the level-set operators");
     m->c = c;
     m->o = o;
     if (int e = ensure_var(c, o->i_phi, &m->V.phi)) return e;
@@ -568,11 +577,22 @@ int make_mg(afg_ctx *c, const afg_mg_opts *o, MgT<R> *m)
     m->V.eps = nullptr;
     // the auto operators without eps leave every box untagged: the plain
     // operators, except on a cylindrical tree (mg2_box_clpl, examples/poisson_cyl.f90)
-    m->V.op_auto = o->op == AFG_OP_AUTO && (o->i_eps != -1 || c->cyl);
+    m->V.op_auto = o->op == AFG_OP_AUTO && (o->i_eps != -1 || c->cyl || use_lsf);
     if (m->V.op_auto && o->i_eps != -1) {
         R *e = nullptr;
         if (int r = ensure_var(c, o->i_eps, &e)) return r;
         m->V.eps = e;
+    }
+    m->V.lsf = m->V.bval = nullptr;
+    m->V.lsf_mode = m->V.rstr_mode = 0;
+    if (use_lsf) {
+        R *l = nullptr, *b = nullptr;
+        if (int r = ensure_var(c, o->i_lsf, &l)) return r;
+        if (int r = ensure_var(c, o->i_bval, &b)) return r;
+        m->V.lsf = l;
+        m->V.bval = b;
+        m->V.lsf_mode = o->op == AFG_OP_LSF ? 2 : o->op == AFG_OP_AUTO ? 1 : 0;
+        m->V.rstr_mode = o->rstr == AFG_RSTR_LSF ? 2 : o->rstr == AFG_RSTR_AUTO ? 1 : 0;
     }
     {
         R *&p2 = phi2_pool(c, (R *)nullptr);
@@ -593,14 +613,27 @@ int make_mg(afg_ctx *c, const afg_mg_opts *o, MgT<R> *m)
 }
 
 // The lazy tags of mg$D_set_box_tag (m_aX_mg.f90:667-706): set once per box,
-// from eps at the first call of an auto operator, and never re-evaluated.
-// The first solve sets every box's; later calls have nothing to do.
-void set_tags(Mg &m)
+// from the level set and eps at the first call of an auto operator (or of
+// mg$D_auto_rstr, which needs them set), and never re-evaluated.  The first
+// solve sets every box's; later calls have nothing to do.  With both a level
+// set and eps the tags are read back once to report a box that has both
+// (where the reference stops).
+int set_tags(Mg &m)
 {
-    if (!m.V.op_auto || m.c->tags_set) return;
-    for (int l = m.c->min_lvl; l <= m.c->highest_lvl; l++)
-        launch_set_tags(m.c->dim, m.c->stream, m.T, level_args(m.c, l), m.V.eps);
-    m.c->tags_set = true;
+    afg_ctx *c = m.c;
+    if (!(m.V.op_auto || m.V.rstr_mode == 1) || c->tags_set) return 0;
+    for (int l = c->min_lvl; l <= c->highest_lvl; l++)
+        launch_set_tags(c->dim, c->stream, m.T, level_args(c, l), m.V.eps, m.V.lsf);
+    c->tags_set = true;
+    if (m.V.eps && m.V.lsf) {
+        std::vector<int8_t> h(c->n_slots);
+        HIP_TRY(c, hipMemcpyAsync(h.data(), m.T.tag, h.size(), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        for (int8_t t : h)
+            if (t == TAG_CONFLICT)
+                return c->fail(AFG_ERR_ARG, "mg%d_set_box_tag: Cannot set lsf and eps tag for same box", c->dim);
+    }
+    return 0;
 }
 
 template <typename R>
@@ -659,7 +692,7 @@ void init_phi_rhs(MgT<R> &m)
     for (int lvl = m.c->highest_lvl; lvl >= m.c->min_lvl + 1; lvl--) {
         const LevelArgs L = level_args(m.c, lvl);
         launch_fill_value(m.c->dim, m.c->stream, m.T, L.slot0, L.n, L.nc, m.V.phi, 0.0);
-        launch_restrict(m.c->dim, m.c->stream, m.T, L, m.V.rhs, m.V.rhs);
+        launch_restrict(m.c->dim, m.c->stream, m.T, L, m.V.rhs, m.V.rhs, (const R *)m.V.lsf, m.V.rstr_mode);
     }
 }
 
@@ -710,7 +743,7 @@ int bottom_table(const MgT<R> &m) { return (sizeof(R) == sizeof(float) ? 2 : 0) 
 
 // highest level run by the bottom kernel for these options
 template <typename R>
-int bottom_top(const MgT<R> &m) { return m.c->bottom_lvl[bottom_table(m)]; }
+int bottom_top(const MgT<R> &m) { return m.V.lsf ? -1000 : m.c->bottom_lvl[bottom_table(m)]; }
 
 
 // mg$D_fas_vcycle (m_aX_mg.f90:208-255).  Levels up to bottom_top run in the
@@ -773,8 +806,9 @@ int run_vcycle(MgT<R> &m, int highest_lvl, int set_residual, int bound, int res_
     // events, used in turn: a copy's events are read before it is replayed
     // again, which by then has long finished, so the stream never stalls.
     constexpr unsigned TIMING_RING = 8;
-    auto key = std::make_tuple(highest_lvl, set_residual, o->i_phi, o->i_rhs, o->i_tmp, o->i_eps,
-                               o->op + 10 * (int)sizeof(R), o->n_cycle_down * 1000000 + o->n_cycle_up * 1000,
+    auto key = std::make_tuple(highest_lvl, set_residual, o->i_phi, o->i_rhs, o->i_tmp,
+                               o->i_eps + 1000 * o->i_lsf + 1000000 * o->i_bval,
+                               o->op + 10 * (int)sizeof(R) + 1000 * o->rstr, o->n_cycle_down * 1000000 + o->n_cycle_up * 1000,
                                o->n_cycle_base,
                                c->timing ? 1 + (int)(c->timing_ring++ % TIMING_RING) : 0,
                                c->bound * 1000 + c->res_upper);
@@ -1990,6 +2024,13 @@ int afg_mg_init(afg_mg_opts *o)
         g_create_error = "mg_init_mg: invalid i_eps";
         return AFG_ERR_ARG;
     }
+    // level set and boundary value: 0 or -1 is unset (zero-filled reserved words of older callers)
+    if (o->i_lsf == 0) o->i_lsf = -1;
+    if (o->i_bval == 0) o->i_bval = -1;
+    if (o->i_lsf * o->i_bval < 0 || o->i_lsf < -1 || o->i_bval < -1) {
+        g_create_error = "mg_init_mg: you have to set both i_lsf and i_bval";
+        return AFG_ERR_ARG;
+    }
     if (o->n_cycle_down < 0) o->n_cycle_down = 2;
     if (o->n_cycle_up < 0) o->n_cycle_up = 2;
     if (o->n_cycle_base < 0) o->n_cycle_base = 2;
@@ -2012,7 +2053,7 @@ int afg_fas_vcycle(afg_ctx *c, const afg_mg_opts *o, int highest_lvl, int set_re
     if (int e = check_level(c, highest_lvl, false)) return e;
     Mg m;
     if (int e = make_mg(c, o, &m)) return e;
-    set_tags(m);
+    if (int e = set_tags(m)) return e;
     return run_vcycle(m, highest_lvl, set_residual ? 1 : 0, c->min_lvl, c->highest_lvl + 1);
 }
 
@@ -2021,7 +2062,7 @@ int afg_fas_fmg(afg_ctx *c, const afg_mg_opts *o, int set_residual, int have_gue
     if (int e = check_ctx(c)) return e;
     Mg m;
     if (int e = make_mg(c, o, &m)) return e;
-    set_tags(m);
+    if (int e = set_tags(m)) return e;
     return fas_fmg(m, set_residual, have_guess);
 }
 
@@ -2121,7 +2162,7 @@ int afg_mixed_begin(afg_ctx *c, const afg_mg_opts *o, int have_guess)
     if (int e = check_ctx(c)) return e;
     Mg m;
     if (int e = make_mg(c, o, &m)) return e;
-    set_tags(m);
+    if (int e = set_tags(m)) return e;
     MgT<float> f;
     if (int e = make_mg(c, o, &f)) return e;
     if (m.V.op_auto)
@@ -2263,7 +2304,7 @@ int afg_gsrb_level(afg_ctx *c, const afg_mg_opts *o, int lvl, int n_cycle)
     if (n_cycle < 0) return c->fail(AFG_ERR_ARG, "n_cycle < 0");
     Mg m;
     if (int e = make_mg(c, o, &m)) return e;
-    set_tags(m);
+    if (int e = set_tags(m)) return e;
     gsrb_level(m, lvl, n_cycle);
     return 0;
 }
@@ -2274,7 +2315,7 @@ int afg_residual_level(afg_ctx *c, const afg_mg_opts *o, int lvl)
     if (int e = check_level(c, lvl, false)) return e;
     Mg m;
     if (int e = make_mg(c, o, &m)) return e;
-    set_tags(m);
+    if (int e = set_tags(m)) return e;
     residual_level(m, lvl);
     return 0;
 }
@@ -2307,7 +2348,7 @@ int afg_correct_children(afg_ctx *c, const afg_mg_opts *o, int lvl)
     if (int e = check_level(c, lvl, true)) return e;
     Mg m;
     if (int e = make_mg(c, o, &m)) return e;
-    set_tags(m);
+    if (int e = set_tags(m)) return e;
     // correct_children (m_aX_mg.f90:368-395): tmp = phi - tmp on the parents,
     // then the prolongation of tmp into the children
     launch_diff(c->dim, c->stream, m.T, list_args(c, lvl - 1, 0), m.V);
@@ -2321,7 +2362,7 @@ int afg_update_coarse(afg_ctx *c, const afg_mg_opts *o, int lvl)
     if (int e = check_level(c, lvl, true)) return e;
     Mg m;
     if (int e = make_mg(c, o, &m)) return e;
-    set_tags(m);
+    if (int e = set_tags(m)) return e;
     update_coarse(m, lvl, 0);
     return 0;
 }
@@ -2477,7 +2518,7 @@ int afg_autotune(afg_ctx *c, const afg_mg_opts *o, int fp32, int reps, double *m
         for (int i = 0; i < 6 * (int)c->levels.size() + 4; i++) ms[i] = -1;
     Mg m;
     if (int e = make_mg(c, o, &m)) return e;
-    set_tags(m);
+    if (int e = set_tags(m)) return e;
     if (int e = fp32 ? tune_tiles<float>(c, o, reps, ms, m.V.phi) : tune_tiles<double>(c, o, reps, ms, m.V.phi))
         return e;
     double *bms = ms ? ms + 6 * (int)c->levels.size() : nullptr;

@@ -43,7 +43,7 @@ void launch_correct(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L
                     const MgVarsT<R> &V, int fly_diff);
 template <typename R>
 void launch_restrict(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                     const R *from, R *to);
+                     const R *from, R *to, const R *lsf = nullptr, int rstr_mode = 0);
 template <typename R>
 void launch_prolong2(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
                      const R *from, R *to, int add);
@@ -77,7 +77,7 @@ void launch_arith(hipStream_t s, long long n, int op, double a, double b, const 
 void launch_to_fp32(hipStream_t s, long long n, const double *from, float *to);
 void launch_from_fp32(hipStream_t s, long long n, const float *from, double *to, int add);
 void launch_set_tags(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                     const double *eps);
+                     const double *eps, const double *lsf);
 // Extreme (max or min) of the interiors of the boxes of a slot list: one
 // partial per block into `partial`; returns the number of partials written.
 int extreme_blocks(int dim, const ListArgs &P);

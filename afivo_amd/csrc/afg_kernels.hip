@@ -22,7 +22,7 @@ inline int n_blocks(long long threads, int block = BLOCK)
 template <typename R>
 LevelViewT<R> pool_view(const DevTree &T, const MgVarsT<R> &V)
 {
-    return LevelViewT<R>{V.phi, V.rhs, V.tmp, V.eps, 0, T.stride};
+    return LevelViewT<R>{V.phi, V.rhs, V.tmp, V.eps, 0, T.stride, V.lsf, V.bval, V.lsf_mode, V.rstr_mode};
 }
 
 template <typename R>
@@ -163,10 +163,11 @@ __global__ __launch_bounds__(BLOCK) void k_correct(DevTree T, LevelArgs L, Level
 
 // a$D_restrict_box and a$D_prolong2 over the boxes of a level
 template <int DIM, int NC, typename R>
-__global__ __launch_bounds__(BLOCK) void k_restrict(DevTree T, LevelArgs L, const R *from, R *to)
+__global__ __launch_bounds__(BLOCK) void k_restrict(DevTree T, LevelArgs L, const R *from, R *to,
+                                                    const R *lsf, int rstr_mode)
 {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid < L.n * ipow((NC ? NC : L.nc) / 2, DIM)) restrict_body<DIM, NC>(T, L, from, to, gid);
+    if (gid < L.n * ipow((NC ? NC : L.nc) / 2, DIM)) restrict_body<DIM, NC>(T, L, from, to, gid, lsf, rstr_mode);
 }
 
 template <int DIM, int NC, typename R>
@@ -289,19 +290,11 @@ __device__ __forceinline__ double block_extreme(double v, int want_max)
     return v;
 }
 
-// mg$D_set_box_tag (m_aX_mg.f90:667-706), eps part: min and max of eps over
-// the whole array, one block per box, only for boxes whose tag is unset.
-__global__ __launch_bounds__(BLOCK) void k_set_tags(DevTree T, int slot0, int len,
-                                                    const double *eps)
+// min and max of a box's whole array (ghosts included), on thread 0
+__device__ void box_min_max(const double *e, int len, double &lo, double &hi)
 {
-    const int slot = slot0 + blockIdx.x;
-    if (T.tag[slot] != TAG_UNSET) return;
-    if (!eps) {   // auto operators without eps: every box untagged (mg%i_eps unset)
-        if (threadIdx.x == 0) T.tag[slot] = TAG_NORMAL;
-        return;
-    }
-    const double *e = eps + (size_t)slot * T.stride;
-    double lo = e[0], hi = e[0];
+    lo = e[0];
+    hi = e[0];
     for (int q = threadIdx.x; q < len; q += blockDim.x) {
         lo = e[q] < lo ? e[q] : lo;
         hi = e[q] > hi ? e[q] : hi;
@@ -309,12 +302,35 @@ __global__ __launch_bounds__(BLOCK) void k_set_tags(DevTree T, int slot0, int le
     lo = block_extreme(lo, 0);
     __syncthreads();
     hi = block_extreme(hi, 1);
+    __syncthreads();
+}
+
+// mg$D_set_box_tag (m_aX_mg.f90:667-706): from min and max of the level set
+// (when set) and of eps (when set) over the whole array, one block per box,
+// only for boxes whose tag is unset.  A box that gets both a level-set and an
+// eps tag (where the reference stops, :700-701) is marked TAG_CONFLICT.
+__global__ __launch_bounds__(BLOCK) void k_set_tags(DevTree T, int slot0, int len,
+                                                    const double *eps, const double *lsf)
+{
+    const int slot = slot0 + blockIdx.x;
+    if (T.tag[slot] != TAG_UNSET) return;
+    bool is_lsf = false, is_eps = false, is_deps = false;
+    double lo, hi;
+    if (lsf) {
+        box_min_max(lsf + (size_t)slot * T.stride, len, lo, hi);
+        is_lsf = lo * hi < 0;
+    }
+    if (eps) {
+        box_min_max(eps + (size_t)slot * T.stride, len, lo, hi);
+        is_deps = hi > lo;
+        if (!is_deps) is_eps = lo < 1 || lo > 1;
+    }
     if (threadIdx.x == 0) {
         int8_t tag = TAG_NORMAL;
-        if (hi > lo)
-            tag = TAG_VEPS;
-        else if (lo < 1 || lo > 1)
-            tag = TAG_CEPS;
+        if (is_lsf) tag = TAG_LSF;
+        if (is_eps) tag = TAG_CEPS;
+        if (is_deps) tag = TAG_VEPS;
+        if ((int)is_lsf + (int)is_eps + (int)is_deps > 1) tag = TAG_CONFLICT;
         T.tag[slot] = tag;
     }
 }
@@ -522,7 +538,7 @@ template <typename R>
 int launch_gsrb_pair(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V,
                      const R *in, R *out, int rbA, int fly)
 {
-    if (T.cyl_rmin) return 0;
+    if (per_cell_only(T, V)) return 0;
     if (L.n <= 0) return 1;
     switch (dim * 100 + L.nc) {
     case 208: gsrb_pair<2, 8, R>(s, T, L, V, in, out, rbA, fly); return 1;
@@ -539,7 +555,7 @@ void launch_gsrb_pass(int dim, hipStream_t s, const DevTree &T, const LevelArgs 
 {
     if (L.n <= 0) return;
     // cylindrical trees take the per-cell pass (the tiles hold Cartesian operators only)
-    switch (T.cyl_rmin ? 0 : dim * 100 + L.nc) {
+    switch (per_cell_only(T, V) ? 0 : dim * 100 + L.nc) {
     case 202: gsrb_tiles<2, 2, R>(s, T, L, V, rb, fly); return;
     case 204: gsrb_tiles<2, 4, R>(s, T, L, V, rb, fly); return;
     case 208: gsrb_tiles<2, 8, R>(s, T, L, V, rb, fly); return;
@@ -572,7 +588,7 @@ void launch_uc_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &
                      const MgVarsT<R> &V, int keep_child_tmp, int fly)
 {
     if (L.n <= 0) return;
-    if ((L.nc == 8 || L.nc == 16) && !T.cyl_rmin) {
+    if ((L.nc == 8 || L.nc == 16) && !per_cell_only(T, V)) {
         auto go = [&](auto dv, auto nv) {
             constexpr int D = decltype(dv)::value, N = decltype(nv)::value;
             using G = UcTile<D, N>;
@@ -615,11 +631,11 @@ void launch_correct(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L
 
 template <typename R>
 void launch_restrict(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                     const R *from, R *to)
+                     const R *from, R *to, const R *lsf, int rstr_mode)
 {
     if (L.n <= 0) return;
     const long long th = (long long)L.n * ipow(L.nc / 2, dim);
-    AFG_DISPATCH(dim, L.nc, k_restrict, n_blocks(th), T, L, from, to);
+    AFG_DISPATCH(dim, L.nc, k_restrict, n_blocks(th), T, L, from, to, lsf, rstr_mode);
 }
 
 template <typename R>
@@ -721,10 +737,10 @@ void launch_from_fp32(hipStream_t s, long long n, const float *from, double *to,
 }
 
 void launch_set_tags(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                     const double *eps)
+                     const double *eps, const double *lsf)
 {
     if (L.n <= 0) return;
-    k_set_tags<<<L.n, BLOCK, 0, s>>>(T, L.slot0, ipow(L.nc + 2, dim), eps);
+    k_set_tags<<<L.n, BLOCK, 0, s>>>(T, L.slot0, ipow(L.nc + 2, dim), eps, lsf);
 }
 
 int extreme_blocks(int dim, const ListArgs &P)
@@ -833,7 +849,7 @@ void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVa
                                       const MgVarsT<R> &, int);                                        \
     template void launch_correct<R>(int, hipStream_t, const DevTree &, const LevelArgs &,              \
                                     const MgVarsT<R> &, int);                                          \
-    template void launch_restrict<R>(int, hipStream_t, const DevTree &, const LevelArgs &, const R *, R *); \
+    template void launch_restrict<R>(int, hipStream_t, const DevTree &, const LevelArgs &, const R *, R *, const R *, int); \
     template void launch_prolong2<R>(int, hipStream_t, const DevTree &, const LevelArgs &, const R *, R *, \
                                      int);                                                             \
     template void launch_fill_value<R>(int, hipStream_t, const DevTree &, int, int, int, R *, double);  \

@@ -18,8 +18,10 @@ constexpr int32_t NB_BC0 = -2;
 // Tags of mg$D_set_box_tag (m_aX_mg.f90:13-17, 667-706).
 constexpr int8_t TAG_UNSET = 0;
 constexpr int8_t TAG_NORMAL = 1;
+constexpr int8_t TAG_LSF = 2;
 constexpr int8_t TAG_CEPS = 3;
 constexpr int8_t TAG_VEPS = 4;
+constexpr int8_t TAG_CONFLICT = -1;   // both a level-set and an eps tag: an error
 
 // Refinement-boundary methods of the general ghost fill (subr_rb of
 // a$D_gc_tree, m_aX_gc.f90:28-69).
@@ -88,8 +90,22 @@ struct MgVarsT {
     R *phi, *rhs, *tmp;
     const R *eps;   // nullptr unless the auto operators are used
     int op_auto;
+    // level set and boundary value (nullptr unless used); lsf_mode: 0 none,
+    // 1 boxes tagged TAG_LSF, 2 every box; rstr_mode: 0 rstr_lpl, 1 by the
+    // child's tag, 2 rstr_lpllsf for every box
+    const R *lsf = nullptr, *bval = nullptr;
+    int lsf_mode = 0, rstr_mode = 0;
 };
 using MgVars = MgVarsT<double>;
+
+// Stages that take the per-cell kernels only: cylindrical trees and the
+// level-set operators (the tiled, paired, tiled-restriction and LDS bottom
+// kernels hold the Cartesian lpl and lpld operators).
+template <typename R>
+inline bool per_cell_only(const DevTree &T, const MgVarsT<R> &V)
+{
+    return T.cyl_rmin != nullptr || V.lsf != nullptr;
+}
 
 // The lower part of a V-cycle -- every level from `hl` down to the lowest
 // and back up -- run by one workgroup in one launch (levels with few cells,

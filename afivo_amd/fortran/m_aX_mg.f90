@@ -179,8 +179,11 @@ contains
 
     call check_mg(mg)
     call device_prepare(tree, mg, o)
+    ! init_phi_rhs restricts rhs before any other operator has tagged a box
+    call check_rstr_tags(tree, o, .not. have_guess)
     call afg_check(afg_fas_fmg(ctx, o, merge(1, 0, set_residual), &
          merge(1, 0, have_guess)), ctx, "mg$D_fas_fmg")
+    call mirror_tags(tree, mg, o)
     if (.not. mg%device_resident) call mg$D_to_host(tree, mg)
   end subroutine mg$D_fas_fmg
 
@@ -194,10 +197,50 @@ contains
 
     call check_mg(mg)
     call device_prepare(tree, mg, o)
+    call check_rstr_tags(tree, o, .false.)
     call afg_check(afg_fas_vcycle(ctx, o, highest_lvl, merge(1, 0, set_residual)), &
          ctx, "mg$D_fas_vcycle")
+    call mirror_tags(tree, mg, o)
     if (.not. mg%device_resident) call mg$D_to_host(tree, mg)
   end subroutine mg$D_fas_vcycle
+
+  !> mg$D_auto_rstr stops on a child whose tag is unset (m_aX_mg.f90:639).
+  !> The auto operators tag a box before it is restricted, except in
+  !> init_phi_rhs (fmg_init); other operators never tag.  Boxes on levels
+  !> above the lowest are checked where the reference could meet an unset tag.
+  subroutine check_rstr_tags(tree, o, fmg_init)
+    type(a$D_t), intent(in)       :: tree
+    type(afg_mg_opts), intent(in) :: o
+    logical, intent(in)           :: fmg_init
+    integer                       :: lvl, i
+
+    if (o%rstr /= AFG_RSTR_AUTO) return
+    if (o%op == AFG_OP_AUTO .and. .not. fmg_init) return
+    do lvl = lbound(tree%lvls, 1) + 1, tree%highest_lvl
+       do i = 1, size(tree%lvls(lvl)%ids)
+          if (tree%boxes(tree%lvls(lvl)%ids(i))%tag == a5_init_tag) &
+               stop "mg$D_auto_rstr: box_c tag not set"
+       end do
+    end do
+  end subroutine check_rstr_tags
+
+  !> The auto operators set each box's tag at their first call on it
+  !> (m_aX_mg.f90:567,602,655); after a device solve the host tree's tags
+  !> are set as the reference's solve leaves them.
+  subroutine mirror_tags(tree, mg, o)
+    type(a$D_t), intent(inout)    :: tree
+    type(mg$D_t), intent(in)      :: mg
+    type(afg_mg_opts), intent(in) :: o
+    integer                       :: lvl, i, id
+
+    if (o%op /= AFG_OP_AUTO) return
+    do lvl = lbound(tree%lvls, 1), tree%highest_lvl
+       do i = 1, size(tree%lvls(lvl)%ids)
+          id = tree%lvls(lvl)%ids(i)
+          if (tree%boxes(id)%tag == a5_init_tag) call mg$D_set_box_tag(tree%boxes(id), mg)
+       end do
+    end do
+  end subroutine mirror_tags
 
   !> Solve to max|rhs - A phi| <= tol * max|rhs| over the leaves by FP64 defect
   !> correction with FP32 FAS multigrid corrections on the device
@@ -319,7 +362,10 @@ contains
     o%n_cycle_up = mg%n_cycle_up
     o%n_cycle_base = mg%n_cycle_base
     o%op = operator_family(mg)
+    o%rstr = restriction_kind(mg)
     if (o%op == AFG_OP_AUTO) o%i_eps = mg%i_eps
+    o%i_lsf = mg%i_lsf
+    o%i_bval = mg%i_bval
     call afg_check(afg_mg_init(o), c_null_ptr, "mg$D_init_mg")
 
     if (.not. associated(mg%sides_rb, mg$D_sides_rb)) &
@@ -356,6 +402,10 @@ contains
        call upload_var(tree, mg%i_rhs)
        call upload_var(tree, mg%i_tmp)
        if (o%op == AFG_OP_AUTO .and. mg%i_eps > 0) call upload_var(tree, mg%i_eps)
+       if (mg%i_lsf > 0 .and. (o%op /= AFG_OP_LPL .or. o%rstr /= AFG_RSTR_LPL)) then
+          call upload_var(tree, mg%i_lsf)
+          call upload_var(tree, mg%i_bval)
+       end if
     end if
     call set_boundaries(tree, mg)
   end subroutine device_prepare
@@ -366,22 +416,37 @@ contains
 
     if (associated(mg%box_op, mg$D_box_lpl) .and. &
          associated(mg%box_gsrb, mg$D_box_gsrb_lpl) .and. &
-         associated(mg%box_corr, mg$D_box_corr_lpl) .and. &
-         associated(mg%box_rstr, mg$D_box_rstr_lpl)) then
+         associated(mg%box_corr, mg$D_box_corr_lpl)) then
        op = AFG_OP_LPL
     else if (associated(mg%box_op, mg$D_auto_op) .and. &
          associated(mg%box_gsrb, mg$D_auto_gsrb) .and. &
-         associated(mg%box_corr, mg$D_auto_corr) .and. &
-         (associated(mg%box_rstr, mg$D_box_rstr_lpl) .or. &
-         associated(mg%box_rstr, mg$D_auto_rstr))) then
-       if (mg%i_lsf /= -1) &
-            error stop "mg$D: level-set boxes are not implemented on the device"
+         associated(mg%box_corr, mg$D_auto_corr)) then
        op = AFG_OP_AUTO
+    else if (associated(mg%box_op, mg$D_box_lpllsf) .and. &
+         associated(mg%box_gsrb, mg$D_box_gsrb_lpllsf) .and. &
+         associated(mg%box_corr, mg$D_box_corr_lpllsf)) then
+       op = AFG_OP_LSF
     else
        error stop "mg$D: operator set not implemented on the device " // &
-            "(supported: the lpl set, and auto_op/auto_gsrb/auto_corr)"
+            "(supported: the lpl set, auto_op/auto_gsrb/auto_corr and the lpllsf set)"
     end if
   end function operator_family
+
+  !> Device restriction from mg%box_rstr
+  integer(c_int) function restriction_kind(mg) result(rstr)
+    type(mg$D_t), intent(in) :: mg
+
+    if (associated(mg%box_rstr, mg$D_box_rstr_lpl)) then
+       rstr = AFG_RSTR_LPL
+    else if (associated(mg%box_rstr, mg$D_auto_rstr)) then
+       rstr = AFG_RSTR_AUTO
+    else if (associated(mg%box_rstr, mg$D_box_rstr_lpllsf)) then
+       rstr = AFG_RSTR_LSF
+    else
+       error stop "mg$D: box_rstr not implemented on the device " // &
+            "(supported: mg$D_box_rstr_lpl, mg$D_auto_rstr, mg$D_box_rstr_lpllsf)"
+    end if
+  end function restriction_kind
 
   !> One record [id, lvl, ix, parent, children, neighbors] per box in use,
   !> in level order (the layout afg_set_tree takes)

@@ -6,7 +6,8 @@ module m_afg_capi
   implicit none
   public
 
-  integer(c_int), parameter :: AFG_OP_LPL = 0, AFG_OP_AUTO = 1
+  integer(c_int), parameter :: AFG_OP_LPL = 0, AFG_OP_AUTO = 1, AFG_OP_LSF = 2
+  integer(c_int), parameter :: AFG_RSTR_LPL = 0, AFG_RSTR_AUTO = 1, AFG_RSTR_LSF = 2
   integer(c_int), parameter :: AFG_BC_DIRICHLET = -1, AFG_BC_NEUMANN = -2
   integer(c_int), parameter :: AFG_COORD_XYZ = 0, AFG_COORD_CYL = 1
 
@@ -15,7 +16,8 @@ module m_afg_capi
      integer(c_int32_t) :: i_phi = -1, i_rhs = -1, i_tmp = -1, i_eps = -1
      integer(c_int32_t) :: n_cycle_down = -1, n_cycle_up = -1, n_cycle_base = -1
      integer(c_int32_t) :: op = 0, initialized = 0
-     integer(c_int32_t) :: reserved(7) = 0
+     integer(c_int32_t) :: i_lsf = -1, i_bval = -1, rstr = 0
+     integer(c_int32_t) :: reserved(4) = 0
   end type afg_mg_opts
 
   interface

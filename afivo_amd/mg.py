@@ -13,7 +13,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import AFG_BC_DIRICHLET, AFG_BC_NEUMANN, AFG_OP_AUTO, AFG_OP_LPL, MgOpts, check
+from ._lib import (AFG_BC_DIRICHLET, AFG_BC_NEUMANN, AFG_OP_AUTO, AFG_OP_LPL, AFG_OP_LSF, AFG_RSTR_AUTO,  # noqa: F401
+                   AFG_RSTR_LPL, AFG_RSTR_LSF, MgOpts, check)
 from .treefile import TreeState
 
 __all__ = ["DeviceTree", "MgOpts", "mg_init_mg", "mg_fas_fmg", "mg_fas_vcycle",
@@ -366,11 +367,17 @@ def max_abs_cc(tree: DeviceTree, iv: int) -> float:
     return max(abs(tree_min_cc(tree, iv)), abs(tree_max_cc(tree, iv)))
 
 
-def default_mg(eps: bool = False, n_down=2, n_up=2, n_base=2, auto: bool = False):
+def default_mg(eps: bool = False, n_down=2, n_up=2, n_base=2, auto: bool = False, lsf: bool = False,
+               lsf_set: bool = False, rstr: int = AFG_RSTR_LPL):
     """Options with phi, rhs, tmp (and eps) in variables 1, 2, 3 (4), as the
-    problems of afivo_amd.problems store them; the auto operators when eps
-    or auto (auto without eps: every box untagged, as for a cylindrical tree
-    in examples/poisson_cyl.f90)."""
+    problems of afivo_amd.problems store them; the auto operators when eps,
+    lsf or auto (auto without eps or level set: every box untagged, as for a
+    cylindrical tree in examples/poisson_cyl.f90).  lsf: level set and
+    boundary value in variables 5 and 6; lsf_set: the lpllsf operators for
+    every box (implies lsf); rstr: the restriction (AFG_RSTR_*)."""
+    lsf = lsf or lsf_set
+    op = AFG_OP_LSF if lsf_set else AFG_OP_AUTO if (eps or auto or lsf) else AFG_OP_LPL
     mg = MgOpts(i_phi=1, i_rhs=2, i_tmp=3, i_eps=4 if eps else -1, n_cycle_down=n_down,
-                n_cycle_up=n_up, n_cycle_base=n_base, op=AFG_OP_AUTO if (eps or auto) else AFG_OP_LPL)
+                n_cycle_up=n_up, n_cycle_base=n_base, op=op, i_lsf=5 if lsf else -1,
+                i_bval=6 if lsf else -1, rstr=rstr)
     return mg_init_mg(mg)

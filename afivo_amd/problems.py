@@ -150,6 +150,26 @@ def set_eps(st: TreeState, t: treemod.Tree, eps_hi: float, cut: float = 0.5) -> 
     return out
 
 
+def set_level_set(st: TreeState, t: treemod.Tree, centre, radius: float):
+    """Packed level set |r - centre| - radius and boundary value 1 + x (the
+    value on the zero contour) on interior and ghost cells, as a program sets
+    mg%i_lsf and mg%i_bval for the lpllsf operators."""
+    view = t.box_view(st.ids)
+    lsf = np.zeros(st.n_values())
+    bval = np.zeros(st.n_values())
+    off = st.offsets()
+    dim = t.dim
+    for r, n in enumerate(st.ncell()):
+        n = int(n)
+        # coordinate d of the cell centres varies along numpy axis dim-1-d (x fastest)
+        ax = [view.r_min[r, d] + (np.arange(0, n + 2) - 0.5) * view.dr[r] for d in range(dim)]
+        grid = np.meshgrid(*ax[::-1], indexing="ij")[::-1]
+        dist = np.sqrt(sum((grid[d] - centre[d]) ** 2 for d in range(dim)))
+        lsf[off[r]:off[r + 1]] = (dist - radius).ravel()
+        bval[off[r]:off[r + 1]] = (1.0 + grid[0]).ravel()
+    return lsf, bval
+
+
 @dataclasses.dataclass
 class Problem:
     name: str

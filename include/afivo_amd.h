@@ -44,7 +44,19 @@ enum afg_status {
 /* Box operator families of m_aX_mg (mg$D_t procedure pointers). */
 enum afg_op {
     AFG_OP_LPL = 0,   /* mg$D_box_lpl / gsrb_lpl / corr_lpl / rstr_lpl   (m_aX_mg.f90:708-808) */
-    AFG_OP_AUTO = 1   /* mg$D_auto_op / auto_gsrb / auto_corr with rstr_lpl (m_aX_mg.f90:562-665) */
+    AFG_OP_AUTO = 1,  /* mg$D_auto_op / auto_gsrb / auto_corr (m_aX_mg.f90:562-665): per box by its tag,
+                         lpl, lpld (eps) or lpllsf (level set) */
+    AFG_OP_LSF = 2    /* mg$D_box_lpllsf / gsrb_lpllsf / corr_lpllsf for every box (m_aX_mg.f90:1019-1251) */
+};
+
+/* Restriction of the multigrid (mg%box_rstr). */
+enum afg_rstr {
+    AFG_RSTR_LPL = 0,   /* mg$D_box_rstr_lpl (m_aX_mg.f90:800-808) */
+    AFG_RSTR_AUTO = 1,  /* mg$D_auto_rstr (m_aX_mg.f90:629-644): rstr_lpllsf for children tagged
+                           mg_lsf_box, else rstr_lpl.  The reference stops on a child whose tag is
+                           unset (:639); the device sets every box's tag before the first stage,
+                           which is the state of a program that called mg$D_set_box_tag on each box */
+    AFG_RSTR_LSF = 2    /* mg$D_box_rstr_lpllsf for every box (m_aX_mg.f90:1254-1320) */
 };
 
 /* Physical boundary types (a5_bc_dirichlet, a5_bc_neumann; m_aX_t.f90:45-48). */
@@ -71,7 +83,13 @@ typedef struct afg_mg_opts {
     int32_t n_cycle_down, n_cycle_up, n_cycle_base;
     int32_t op;                      /* enum afg_op */
     int32_t initialized;             /* set by afg_mg_init */
-    int32_t reserved[7];
+    /* level set and the boundary value on its zero contour (mg%i_lsf,
+     * mg%i_bval; both or neither; -1 or 0: unset), for AFG_OP_AUTO tags,
+     * AFG_OP_LSF and the level-set restrictions.  Their ghost cells are the
+     * caller's, as for eps.  The level-set operators run in FP64 only. */
+    int32_t i_lsf, i_bval;
+    int32_t rstr;                    /* enum afg_rstr */
+    int32_t reserved[4];
 } afg_mg_opts;
 
 /* ------------------------------------------------------------------ */

@@ -43,10 +43,13 @@ struct orc_tree {
     /* cylindrical coordinates (2D, x the radius): box%r_min(1) per id */
     int cyl;
     double *rmin;
+    /* set when mg$D_set_box_tag finds both a level-set and an eps tag */
+    int tag_err;
 };
 
 #define TAG_UNSET 0
 #define TAG_NORMAL 1   /* mg_normal_box, m_aX_mg.f90:14 */
+#define TAG_LSF 2      /* mg_lsf_box, m_aX_mg.f90:15 */
 #define TAG_CEPS 3     /* mg_ceps_box, m_aX_mg.f90:16 */
 #define TAG_VEPS 4     /* mg_veps_box, m_aX_mg.f90:17 */
 
@@ -702,32 +705,62 @@ static void prolong2(orc_tree *t, int p_id, int c_id, int iv, int ivc, int add)
 /* --------------------------------------------------------------------- */
 /* operators                                                              */
 
-/* mg$D_set_box_tag, m_aX_mg.f90:667-706 (eps part; no level set) */
-static void set_box_tag(orc_tree *t, int id, const orc_mg_opts *o)
+/* minval and maxval of variable iv over the whole box array, ghosts included */
+static void box_min_max(const orc_tree *t, int id, int iv, double *lo, double *hi)
 {
-    if (o->i_eps < 1) {   /* mg%i_eps unset: no eps tag (m_aX_mg.f90:686) */
-        t->tag[id] = TAG_NORMAL;
-        return;
-    }
-    const double *e = cc(t, id, o->i_eps);
+    const double *e = cc(t, id, iv);
     int len = box_len(t, t->ncell[id]);
     double a = e[0], b = e[0];
     for (int q = 1; q < len; q++) {
         if (e[q] < a) a = e[q];
         if (e[q] > b) b = e[q];
     }
-    int is_deps = b > a, is_eps = 0;
-    if (!is_deps) is_eps = (a < 1 || a > 1);
+    *lo = a;
+    *hi = b;
+}
+
+/* mg$D_set_box_tag, m_aX_mg.f90:667-706 */
+static void set_box_tag(orc_tree *t, int id, const orc_mg_opts *o)
+{
+    int is_lsf = 0, is_eps = 0, is_deps = 0;
+    double a, b;
+    if (o->i_lsf != -1) {   /* :680-688 */
+        box_min_max(t, id, o->i_lsf, &a, &b);
+        is_lsf = a * b < 0;
+    }
+    if (o->i_eps != -1) {   /* :690-698 */
+        box_min_max(t, id, o->i_eps, &a, &b);
+        is_deps = b > a;
+        if (!is_deps) is_eps = (a < 1 || a > 1);
+    }
+    if (is_lsf + is_eps + is_deps > 1) t->tag_err = 1;   /* :700-701 stops */
     t->tag[id] = TAG_NORMAL;
+    if (is_lsf) t->tag[id] = TAG_LSF;
     if (is_eps) t->tag[id] = TAG_CEPS;
     if (is_deps) t->tag[id] = TAG_VEPS;
+}
+
+/* the box's tag under the auto operators, set lazily (m_aX_mg.f90:567,602,655) */
+static int auto_tag(orc_tree *t, int id, const orc_mg_opts *o)
+{
+    if (t->tag[id] == TAG_UNSET) set_box_tag(t, id, o);
+    return t->tag[id];
 }
 
 static int use_lpld(orc_tree *t, int id, const orc_mg_opts *o)
 {
     if (o->op != 1) return 0;
-    if (t->tag[id] == TAG_UNSET) set_box_tag(t, id, o);   /* lazy, m_aX_mg.f90:567,602,655 */
-    return t->tag[id] != TAG_NORMAL;
+    int tg = auto_tag(t, id, o);
+    return tg == TAG_CEPS || tg == TAG_VEPS;
+}
+
+/* the lpllsf operators for this box: every box with the lpllsf set, boxes
+ * tagged mg_lsf_box with the auto set */
+static int use_lsf(orc_tree *t, int id, const orc_mg_opts *o)
+{
+    if (o->op == 2) return 1;
+    if (o->op != 1) return 0;
+    return auto_tag(t, id, o) == TAG_LSF;
 }
 
 /* mg$D_box_gsrb_lpl, m_aX_mg.f90:718-760 */
@@ -1000,9 +1033,214 @@ static void box_clpld(orc_tree *t, int id, int i_out, const orc_mg_opts *o)
         }
 }
 
+/* --------------------------------------------------------------------- */
+/* internal boundaries given by a level set (m_aX_mg.f90:987-1321)         */
+
+/* lsf_dist_val, m_aX_mg.f90:990-1017: distance (0..1] to the neighbour b and
+ * the value there, the interface's boundary value where the level set
+ * changes sign between a and b */
+static void lsf_dist_val(double la, double ba, double lb, double vb, double bb, double *dist,
+                         double *val)
+{
+    if (la * lb < 0) {
+        double d = la / (la - lb);
+        *dist = d;
+        *val = ba * (1 - d) + bb * d;
+    } else {
+        *dist = 1;
+        *val = vb;
+    }
+}
+
+/* distances and values of the 2*dim face neighbours of cell q, in the order
+ * i-1, i+1, j-1, j+1 (, k-1, k+1) */
+static void lsf_neighbours(const orc_tree *t, int id, const orc_mg_opts *o, int q, double dd[7],
+                           double val[7])
+{
+    int n = t->ncell[id], s = n + 2, s2 = s * s;
+    const double *p = cc(t, id, o->i_phi), *l = cc(t, id, o->i_lsf), *b = cc(t, id, o->i_bval);
+    int dq[6] = {-1, 1, -s, s, -s2, s2};
+    for (int m = 0; m < 2 * t->dim; m++) {
+        int qb = q + dq[m];
+        lsf_dist_val(l[q], b[q], l[qb], p[qb], b[qb], &dd[m + 1], &val[m + 1]);
+    }
+}
+
+/* mg$D_box_gsrb_lpllsf, m_aX_mg.f90:1105-1179 */
+static void gsrb_lpllsf(orc_tree *t, int id, int rb, const orc_mg_opts *o)
+{
+    int n = t->ncell[id];
+    double dx2 = t->dr[id] * t->dr[id];
+    double *p = cc(t, id, o->i_phi);
+    const double *r = cc(t, id, o->i_rhs);
+    double d[7], v[7];
+    int kk = t->dim == 3 ? n : 0;
+    for (int k = t->dim == 3 ? 1 : 0; k <= kk; k++)
+        for (int j = 1; j <= n; j++) {
+            int i0 = 2 - ((rb ^ (t->dim == 3 ? k + j : j)) & 1);
+            for (int i = i0; i <= n; i += 2) {
+                int q = ix3(n, i, j, k);
+                lsf_neighbours(t, id, o, q, d, v);
+                if (t->dim == 2) {
+                    double prod = d[1] * d[2] * d[3] * d[4];
+                    p[q] = 1 / (d[1] * d[2] + d[3] * d[4]) *
+                           ((d[2] * v[1] + d[1] * v[2]) * d[3] * d[4] / (d[1] + d[2]) +
+                            (d[4] * v[3] + d[3] * v[4]) * d[1] * d[2] / (d[3] + d[4]) -
+                            0.5 * prod * dx2 * r[q]);
+                } else {
+                    p[q] = 1 / (1 / (d[1] * d[2]) + 1 / (d[3] * d[4]) + 1 / (d[5] * d[6])) *
+                           ((d[2] * v[1] + d[1] * v[2]) / ((d[1] + d[2]) * d[1] * d[2]) +
+                            (d[4] * v[3] + d[3] * v[4]) / ((d[3] + d[4]) * d[3] * d[4]) +
+                            (d[6] * v[5] + d[5] * v[6]) / ((d[5] + d[6]) * d[5] * d[6]) -
+                            0.5 * dx2 * r[q]);
+                }
+            }
+        }
+}
+
+/* mg$D_box_lpllsf, m_aX_mg.f90:1182-1251 */
+static void box_lpllsf(orc_tree *t, int id, int i_out, const orc_mg_opts *o)
+{
+    int n = t->ncell[id];
+    double inv_dr_sq = 1 / (t->dr[id] * t->dr[id]);
+    const double *p = cc(t, id, o->i_phi);
+    double *out = cc(t, id, i_out);
+    double d[7], v[7];
+    int kk = t->dim == 3 ? n : 0;
+    for (int k = t->dim == 3 ? 1 : 0; k <= kk; k++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                int q = ix3(n, i, j, k);
+                lsf_neighbours(t, id, o, q, d, v);
+                double f0 = p[q];
+                double sum = (d[2] * v[1] + d[1] * v[2] - (d[1] + d[2]) * f0) /
+                                 ((d[1] + d[2]) * d[1] * d[2]) +
+                             (d[4] * v[3] + d[3] * v[4] - (d[3] + d[4]) * f0) /
+                                 ((d[3] + d[4]) * d[3] * d[4]);
+                if (t->dim == 3)
+                    sum = sum + (d[6] * v[5] + d[5] * v[6] - (d[5] + d[6]) * f0) /
+                                    ((d[5] + d[6]) * d[5] * d[6]);
+                out[q] = 2 * inv_dr_sq * sum;
+            }
+}
+
+/* mg$D_box_corr_lpllsf, m_aX_mg.f90:1019-1102: the correction (the parent's
+ * tmp) interpolated from the closest coarse cell and the one-but-closest in
+ * each direction, with boundary value 0 where the level set changes sign */
+static void corr_lpllsf(orc_tree *t, int p_id, int c_id, const orc_mg_opts *o)
+{
+    int n = t->ncell[c_id], np = t->ncell[p_id];
+    int off[3];
+    child_offset(t, c_id, off);
+    double *ph = cc(t, c_id, o->i_phi);
+    const double *lc = cc(t, c_id, o->i_lsf);
+    const double *lp = cc(t, p_id, o->i_lsf), *cp = cc(t, p_id, o->i_tmp);
+    int kk = t->dim == 3 ? n : 0;
+    for (int k = t->dim == 3 ? 1 : 0; k <= kk; k++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                int ic1 = off[0] + ((i + 1) >> 1), ic2 = ic1 + 1 - 2 * (i & 1);
+                int jc1 = off[1] + ((j + 1) >> 1), jc2 = jc1 + 1 - 2 * (j & 1);
+                int kc1 = 0, kc2 = 0;
+                if (t->dim == 3) {
+                    kc1 = off[2] + ((k + 1) >> 1);
+                    kc2 = kc1 + 1 - 2 * (k & 1);
+                }
+                int q = ix3(n, i, j, k);
+                int qb[4] = {ix3(np, ic1, jc1, kc1), ix3(np, ic2, jc1, kc1), ix3(np, ic1, jc2, kc1),
+                             ix3(np, ic1, jc1, kc2)};
+                double d[5] = {0}, v[5] = {0}, c[5] = {0};
+                for (int m = 0; m <= t->dim; m++)
+                    lsf_dist_val(lc[q], 0.0, lp[qb[m]], cp[qb[m]], 0.0, &d[m + 1], &v[m + 1]);
+                double sv, sc;
+                if (t->dim == 2) {
+                    c[1] = 2 * d[2] * d[3];
+                    c[2] = d[1] * d[3];
+                    c[3] = d[1] * d[2];
+                } else {
+                    c[1] = d[2] * d[3] * d[4];
+                    c[2] = d[1] * d[3] * d[4];
+                    c[3] = d[1] * d[2] * d[4];
+                    c[4] = d[1] * d[2] * d[3];
+                }
+                /* sum(c * val) / sum(c): Fortran SUM starts from zero */
+                sv = 0.0;
+                sc = 0.0;
+                for (int m = 1; m <= t->dim + 1; m++) {
+                    sv += c[m] * v[m];
+                    sc += c[m];
+                }
+                ph[q] = ph[q] + sv / sc;
+            }
+}
+
+/* mg$D_box_rstr_lpllsf, m_aX_mg.f90:1254-1320: children on the parent cell's
+ * side of the interface only, where some but not all are.  The factor
+ * 1 / n_ch is integer division in the reference, so a mean over 2..2^dim-1
+ * children comes out as 0 * sum. */
+static void rstr_lpllsf(orc_tree *t, int c_id, int p_id, int iv, const orc_mg_opts *o)
+{
+    int n = t->ncell[c_id], np = t->ncell[p_id], hnc = n >> 1;
+    int off[3];
+    child_offset(t, c_id, off);
+    const double *c = cc(t, c_id, iv), *lc = cc(t, c_id, o->i_lsf);
+    double *p = cc(t, p_id, iv);
+    const double *lp = cc(t, p_id, o->i_lsf);
+    int kk = t->dim == 3 ? hnc : 0;
+    for (int k = t->dim == 3 ? 1 : 0; k <= kk; k++)
+        for (int j = 1; j <= hnc; j++)
+            for (int i = 1; i <= hnc; i++) {
+                int pq = ix3(np, off[0] + i, off[1] + j, t->dim == 3 ? off[2] + k : 0);
+                int f = 2 * i - 1, g = 2 * j - 1, h = t->dim == 3 ? 2 * k - 1 : 0;
+                int qs[8] = {0}, mask[8] = {0}, n_ch = 0;
+                for (int m = 0; m < t->nch; m++) {
+                    qs[m] = ix3(n, f + (m & 1), g + ((m >> 1) & 1), h + ((m >> 2) & 1));
+                    mask[m] = lp[pq] * lc[qs[m]] > 0;
+                    n_ch += mask[m];
+                }
+                if (n_ch < t->nch && n_ch > 0) {
+                    double sm = 0.0;
+                    for (int m = 0; m < t->nch; m++)
+                        if (mask[m]) sm += c[qs[m]];
+                    p[pq] = (double)(1 / n_ch) * sm;
+                } else {
+                    double sm = c[qs[0]];
+                    for (int m = 1; m < t->nch; m++) sm += c[qs[m]];
+                    p[pq] = (t->dim == 2 ? 0.25 : 0.125) * sm;
+                }
+            }
+}
+
+/* mg%box_rstr: mg$D_box_rstr_lpl, mg$D_auto_rstr (m_aX_mg.f90:629-644; the
+ * child's tag must be set) or mg$D_box_rstr_lpllsf */
+static void box_rstr(orc_tree *t, int c_id, int p_id, int iv, const orc_mg_opts *o)
+{
+    int lsf = o->rstr == 2;
+    if (o->rstr == 1) {
+        if (t->tag[c_id] == TAG_UNSET) t->tag_err = 1;   /* :639 stops */
+        lsf = t->tag[c_id] == TAG_LSF;
+    }
+    if (lsf)
+        rstr_lpllsf(t, c_id, p_id, iv, o);
+    else
+        restrict_box(t, c_id, p_id, iv, iv);
+}
+
+/* every box's tag, as a program calling mg$D_set_box_tag on each box does */
+int orc_set_tags(orc_tree *t, const orc_mg_opts *o)
+{
+    for (int id = 1; id <= t->max_id; id++)
+        if (t->ncell[id] > 0) set_box_tag(t, id, o);
+    return t->tag_err ? -1 : 0;
+}
+
 /* mg%box_gsrb: mg$D_box_gsrb_lpl or mg$D_auto_gsrb (m_aX_mg.f90:562-590) */
 static void box_gsrb(orc_tree *t, int id, int rb, const orc_mg_opts *o)
 {
+    if (use_lsf(t, id, o)) {
+        gsrb_lpllsf(t, id, rb, o);
+        return;
+    }
     if (t->cyl && o->op == 1) {
         if (use_lpld(t, id, o)) gsrb_clpld(t, id, rb, o);
         else gsrb_clpl(t, id, rb, o);
@@ -1017,6 +1255,10 @@ static void box_gsrb(orc_tree *t, int id, int rb, const orc_mg_opts *o)
 /* mg%box_op: mg$D_box_lpl or mg$D_auto_op (m_aX_mg.f90:593-627) */
 static void box_op(orc_tree *t, int id, int i_out, const orc_mg_opts *o)
 {
+    if (use_lsf(t, id, o)) {
+        box_lpllsf(t, id, i_out, o);
+        return;
+    }
     if (t->cyl && o->op == 1) {
         if (use_lpld(t, id, o)) box_clpld(t, id, i_out, o);
         else box_clpl(t, id, i_out, o);
@@ -1032,7 +1274,9 @@ static void box_op(orc_tree *t, int id, int i_out, const orc_mg_opts *o)
  * (m_aX_mg.f90:646-665); the tag is that of the child. */
 static void box_corr(orc_tree *t, int p_id, int c_id, const orc_mg_opts *o)
 {
-    if (use_lpld(t, c_id, o))
+    if (use_lsf(t, c_id, o))
+        corr_lpllsf(t, p_id, c_id, o);
+    else if (use_lpld(t, c_id, o))
         corr_lpld(t, p_id, c_id, o);
     else
         prolong2(t, p_id, c_id, o->i_tmp, o->i_phi, 1);
@@ -1232,8 +1476,8 @@ int orc_update_coarse_level(orc_tree *t, const orc_mg_opts *o, int lvl)
         double *save = malloc((size_t)len * sizeof(double));
         memcpy(save, tm, (size_t)len * sizeof(double));   /* interior is what matters */
         residual_box(t, id, o);
-        restrict_box(t, id, p_id, o->i_tmp, o->i_tmp);      /* box_rstr = lpl */
-        restrict_box(t, id, p_id, o->i_phi, o->i_phi);
+        box_rstr(t, id, p_id, o->i_tmp, o);
+        box_rstr(t, id, p_id, o->i_phi, o);
         int kk = t->dim == 3 ? n : 0;
         for (int k = t->dim == 3 ? 1 : 0; k <= kk; k++)
             for (int j = 1; j <= n; j++)
@@ -1263,8 +1507,8 @@ static void set_coarse_phi_rhs(orc_tree *t, int lvl, const orc_mg_opts *o)
     for (int i = 0; i < n_ids(t, lvl); i++) {
         int id = level_id(t, lvl, i), p_id = t->parent[id];
         residual_box(t, id, o);
-        restrict_box(t, id, p_id, o->i_tmp, o->i_tmp);
-        restrict_box(t, id, p_id, o->i_phi, o->i_phi);
+        box_rstr(t, id, p_id, o->i_tmp, o);
+        box_rstr(t, id, p_id, o->i_phi, o);
     }
     orc_fill_gc_level(t, o->i_phi, lvl - 1);
     for (int i = 0; i < n_ids(t, lvl - 1); i++) {
@@ -1284,7 +1528,7 @@ static void init_phi_rhs(orc_tree *t, const orc_mg_opts *o)
         for (int i = 0; i < n_ids(t, lvl); i++) {
             int id = level_id(t, lvl, i);
             memset(cc(t, id, o->i_phi), 0, (size_t)box_len(t, t->ncell[id]) * sizeof(double));
-            restrict_box(t, id, t->parent[id], o->i_rhs, o->i_rhs);
+            box_rstr(t, id, t->parent[id], o->i_rhs, o);
         }
 }
 
@@ -1294,6 +1538,11 @@ static int check_opts(const orc_tree *t, const orc_mg_opts *o)
         o->i_tmp < 1 || o->i_tmp > t->n_var)
         return -1;
     if (o->op == 1 && o->i_eps != -1 && (o->i_eps < 1 || o->i_eps > t->n_var)) return -1;
+    if (o->op < 0 || o->op > 2 || o->rstr < 0 || o->rstr > 2) return -1;
+    if ((o->i_lsf == -1) != (o->i_bval == -1)) return -1;   /* m_aX_mg.f90:136-137 */
+    if (o->i_lsf != -1 && (o->i_lsf < 1 || o->i_lsf > t->n_var || o->i_bval < 1 || o->i_bval > t->n_var))
+        return -1;
+    if ((o->op == 2 || o->rstr == 2) && o->i_lsf == -1) return -1;
     if (o->n_cycle_down < 0 || o->n_cycle_up < 0 || o->n_cycle_base < 0) return -1;
     return 0;
 }
@@ -1315,7 +1564,7 @@ int orc_fas_vcycle(orc_tree *t, const orc_mg_opts *o, int highest_lvl, int set_r
     }
     if (set_residual)
         for (int lvl = min_lvl; lvl <= highest_lvl; lvl++) orc_residual_level(t, o, lvl);
-    return 0;
+    return t->tag_err ? -1 : 0;
 }
 
 /* mg$D_fas_fmg, m_aX_mg.f90:164-203 */
@@ -1337,7 +1586,7 @@ int orc_fas_fmg(orc_tree *t, const orc_mg_opts *o, int set_residual, int have_gu
         }
         orc_fas_vcycle(t, o, lvl, set_residual && lvl == t->highest_lvl);
     }
-    return 0;
+    return t->tag_err ? -1 : 0;
 }
 
 /* a$D_tree_max_cc / a$D_tree_min_cc, m_aX_utils.f90:468-571 */

@@ -28,7 +28,11 @@ typedef struct orc_tree orc_tree;
 typedef struct orc_mg_opts {
     int32_t i_phi, i_rhs, i_tmp, i_eps;     /* 1-based variable indices */
     int32_t n_cycle_down, n_cycle_up, n_cycle_base;
-    int32_t op;                             /* 0: lpl set, 1: auto (lpld) set */
+    int32_t op;                             /* 0: lpl set, 1: auto set, 2: lpllsf set */
+    /* level set and boundary value (mg%i_lsf, mg%i_bval; -1 unset) and the
+     * restriction: 0 mg$D_box_rstr_lpl, 1 mg$D_auto_rstr (by the child's
+     * tag), 2 mg$D_box_rstr_lpllsf */
+    int32_t i_lsf, i_bval, rstr;
 } orc_mg_opts;
 
 enum { ORC_BC_DIRICHLET = -1, ORC_BC_NEUMANN = -2 };
@@ -52,6 +56,8 @@ int orc_set_bc(orc_tree *t, int id, int nb, int bc_type, const double *b);
 int orc_set_coord(orc_tree *t, int cyl, double r_base);
 
 int orc_fas_fmg(orc_tree *t, const orc_mg_opts *o, int set_residual, int have_guess);
+/* every box's tag (mg$D_set_box_tag on each box), as mg$D_auto_rstr needs */
+int orc_set_tags(orc_tree *t, const orc_mg_opts *o);
 int orc_fas_vcycle(orc_tree *t, const orc_mg_opts *o, int highest_lvl, int set_residual);
 double orc_tree_max_cc(const orc_tree *t, int iv, int include_parents);
 double orc_tree_min_cc(const orc_tree *t, int iv, int include_parents);

@@ -23,7 +23,7 @@ _lib = None
 class MgOpts(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
         "i_phi", "i_rhs", "i_tmp", "i_eps",
-        "n_cycle_down", "n_cycle_up", "n_cycle_base", "op")]
+        "n_cycle_down", "n_cycle_up", "n_cycle_base", "op", "i_lsf", "i_bval", "rstr")]
 
 
 def lib():
@@ -65,6 +65,7 @@ def lib():
         L.orc_prolong_level.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int]
         L.orc_update_coarse_level.argtypes = [P, ctypes.POINTER(MgOpts), ctypes.c_int]
+        L.orc_set_tags.argtypes = [P, ctypes.POINTER(MgOpts)]
         _lib = L
     return _lib
 
@@ -122,6 +123,11 @@ class OracleTree:
         if lib().orc_fas_vcycle(self.h, ctypes.byref(opts), hl, int(set_residual)):
             raise ValueError("orc_fas_vcycle failed")
 
+    def set_tags(self, opts):
+        """mg$D_set_box_tag on every box (what mg$D_auto_rstr requires)."""
+        if lib().orc_set_tags(self.h, ctypes.byref(opts)):
+            raise ValueError("orc_set_tags: a box has both a level-set and an eps tag")
+
     def set_coord(self, cyl, r_base=0.0):
         """Cylindrical coordinates (2D; x the radius) or Cartesian."""
         if lib().orc_set_coord(self.h, int(cyl), float(r_base)):
@@ -172,10 +178,20 @@ BC_DIRICHLET, BC_NEUMANN = -1, -2             # ORC_BC_* (afivo's a5_bc_dirichle
 RB_MG, RB_PROLONG0, RB_INTERP, RB_INTERP_LIM = 0, 1, 2, 3
 
 
-def default_opts(n_down=2, n_up=2, n_base=2, eps=False, auto=False):
+OP_LPL, OP_AUTO, OP_LSF = 0, 1, 2             # operator sets: lpl, auto, lpllsf
+RSTR_LPL, RSTR_AUTO, RSTR_LSF = 0, 1, 2       # box_rstr: rstr_lpl, auto_rstr, rstr_lpllsf
+
+
+def default_opts(n_down=2, n_up=2, n_base=2, eps=False, auto=False, lsf=False, lsf_set=False,
+                 rstr=RSTR_LPL):
     """Options of the lpl operators, or of the auto operators when eps (i_eps
-    4) or auto (no eps: every box untagged, as examples/poisson_cyl.f90)."""
-    return MgOpts(1, 2, 3, 4 if eps else -1, n_down, n_up, n_base, 1 if (eps or auto) else 0)
+    4), lsf (level set and boundary value in variables 5 and 6) or auto (no
+    eps: every box untagged, as examples/poisson_cyl.f90); lsf_set selects
+    the lpllsf operators for every box (and implies lsf)."""
+    lsf = lsf or lsf_set
+    op = OP_LSF if lsf_set else OP_AUTO if (eps or auto or lsf) else OP_LPL
+    return MgOpts(1, 2, 3, 4 if eps else -1, n_down, n_up, n_base, op,
+                  5 if lsf else -1, 6 if lsf else -1, rstr)
 
 
 # ---------------------------------------------------------------------------

@@ -26,7 +26,8 @@ program ref_harness_$Dd
   implicit none
 
   integer, parameter :: i_phi = 1, i_rhs = 2, i_tmp = 3, i_eps = 4
-  integer, parameter :: n_var = 4
+  integer, parameter :: i_lsf = 5, i_bval = 6
+  integer            :: n_var = 4
   integer, parameter :: max_gauss = 16
 
   ! Configuration (namelist /cfg/)
@@ -70,11 +71,21 @@ program ref_harness_$Dd
   ! cylindrical coordinates (2D: x is the radius, a5_cyl), solved with the
   ! auto operators as examples/poisson_cyl.f90:41-69 does
   logical            :: cyl = .false.
+  ! internal boundary given by a level set (variables 5 and 6 added): the
+  ! sphere |r - lsf_c| = lsf_rad, with boundary value 1 + x there.
+  ! lsf_mode 1: the auto operators (mg_lsf_box tags), with box_rstr =>
+  ! mg$D_auto_rstr when rstr_auto (every tag set by mg$D_set_box_tag first);
+  ! lsf_mode 2: the lpllsf operator set for every box
+  integer            :: lsf_mode = 0
+  logical            :: rstr_auto = .false.
+  real(dp)           :: lsf_c($D) = 0.5_dp
+  real(dp)           :: lsf_rad = 0.25_dp
 
   namelist /cfg/ nc, ref_mode, l_uniform, l_max, thr, n_gauss, ampl, sigma, &
        r0, list_file, n_down, n_up, n_base, n_fmg, n_vcycle, use_eps, eps_hi, &
        eps_cut, out_file, dump_input, dump_fmg, dump_output, n_time, refine2, &
-       refine2_lmax, refine2_thr, refine2_rm_thr, refine2_nvar, xfer, cyl
+       refine2_lmax, refine2_thr, refine2_rm_thr, refine2_nvar, xfer, cyl, &
+       lsf_mode, rstr_auto, lsf_c, lsf_rad
 
   type(a$D_t)         :: tree, tree_init
   type(ref_info_t)    :: ref_info
@@ -95,6 +106,7 @@ program ref_harness_$Dd
   call gauss_init(gs, ampl(1:n_gauss), sigma(1:n_gauss), r0(:, 1:n_gauss))
 
   if (ref_mode == 'list') call read_list()
+  if (lsf_mode > 0) n_var = 6
 
 #if $D == 2
   call a$D_init(tree, nc, n_var, merge(1, 0, xfer), 1.0_dp / nc, coarsen_to=2, &
@@ -125,7 +137,16 @@ program ref_harness_$Dd
   mg%n_cycle_down = n_down
   mg%n_cycle_up   = n_up
   mg%n_cycle_base = n_base
-  if (use_eps .or. cyl) then
+  if (lsf_mode > 0) then
+     mg%i_lsf  = i_lsf
+     mg%i_bval = i_bval
+  end if
+  if (lsf_mode == 2) then
+     mg%box_op   => mg$D_box_lpllsf
+     mg%box_gsrb => mg$D_box_gsrb_lpllsf
+     mg%box_corr => mg$D_box_corr_lpllsf
+     mg%box_rstr => mg$D_box_rstr_lpllsf
+  else if (use_eps .or. cyl .or. lsf_mode == 1) then
      if (use_eps) mg%i_eps = i_eps
      mg%box_op   => mg$D_auto_op
      mg%box_gsrb => mg$D_auto_gsrb
@@ -134,7 +155,9 @@ program ref_harness_$Dd
      ! examples/poisson_cyl_dielectric.f90:62-64 (auto_rstr would stop in
      ! init_phi_rhs, m_aX_mg.f90:639, because no tag is set yet)
   end if
+  if (rstr_auto) mg%box_rstr => mg$D_auto_rstr
   call mg$D_init_mg(mg)
+  if (rstr_auto) call a$D_loop_box(tree, set_tag)
 
   if (dump_input) call write_dump(trim(out_file) // '.in')
 
@@ -178,6 +201,12 @@ program ref_harness_$Dd
   end if
 
 contains
+
+  !> mg$D_set_box_tag of a box (what mg$D_auto_rstr requires)
+  subroutine set_tag(box)
+    type(box$D_t), intent(inout) :: box
+    call mg$D_set_box_tag(box, mg)
+  end subroutine set_tag
 
   !> FMG (n_fmg times, warm start after the first) then n_vcycle V-cycles,
   !> as in examples/poisson_basic_2d.f90:84-98.
@@ -477,6 +506,15 @@ contains
           box%cc(i, j, i_rhs) = gauss_lpl(gs, a$D_r_cc(box, [i, j]))
        end do
     end do
+    if (lsf_mode > 0) then
+       do j = 0, nc_b+1
+          do i = 0, nc_b+1
+             r = a$D_r_cc(box, [i, j])
+             box%cc(i, j, i_lsf) = norm2(r - lsf_c) - lsf_rad
+             box%cc(i, j, i_bval) = 1.0_dp + r(1)
+          end do
+       end do
+    end if
     if (use_eps) then
        do j = 0, nc_b+1
           do i = 0, nc_b+1
@@ -497,6 +535,17 @@ contains
           end do
        end do
     end do
+    if (lsf_mode > 0) then
+       do k = 0, nc_b+1
+          do j = 0, nc_b+1
+             do i = 0, nc_b+1
+                r = a$D_r_cc(box, [i, j, k])
+                box%cc(i, j, k, i_lsf) = norm2(r - lsf_c) - lsf_rad
+                box%cc(i, j, k, i_bval) = 1.0_dp + r(1)
+             end do
+          end do
+       end do
+    end if
     if (use_eps) then
        do k = 0, nc_b+1
           do j = 0, nc_b+1

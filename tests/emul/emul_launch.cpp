@@ -26,7 +26,7 @@ constexpr int BLOCK = 256;
 template <typename R>
 LevelViewT<R> pool_view(const DevTree &T, const MgVarsT<R> &V)
 {
-    return LevelViewT<R>{V.phi, V.rhs, V.tmp, V.eps, 0, T.stride};
+    return LevelViewT<R>{V.phi, V.rhs, V.tmp, V.eps, 0, T.stride, V.lsf, V.bval, V.lsf_mode, V.rstr_mode};
 }
 
 template <typename R>
@@ -200,7 +200,7 @@ template <typename R>
 int launch_gsrb_pair(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V,
                      const R *in, R *out, int rbA, int fly)
 {
-    if (T.cyl_rmin) return 0;
+    if (per_cell_only(T, V)) return 0;
     if (L.n <= 0) return 1;
     const int key = dim * 100 + L.nc;
     if (key != 208 && key != 216 && key != 308 && key != 316) return 0;
@@ -221,7 +221,7 @@ void launch_gsrb_pass(int dim, hipStream_t s, const DevTree &T, const LevelArgs 
 {
     if (L.n <= 0) return;
     emul_submit(s, [=] {
-        switch (T.cyl_rmin ? 0 : dim * 100 + L.nc) {
+        switch (per_cell_only(T, V) ? 0 : dim * 100 + L.nc) {
         case 202: gsrb_tiles<2, 2, R>(T, L, V, rb, fly); return;
         case 204: gsrb_tiles<2, 4, R>(T, L, V, rb, fly); return;
         case 208: gsrb_tiles<2, 8, R>(T, L, V, rb, fly); return;
@@ -254,7 +254,7 @@ void launch_uc_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &
                      const MgVarsT<R> &V, int keep_child_tmp, int fly)
 {
     if (L.n <= 0) return;
-    if ((L.nc == 8 || L.nc == 16) && !T.cyl_rmin) {
+    if ((L.nc == 8 || L.nc == 16) && !per_cell_only(T, V)) {
         emul_submit(s, [=] {
             if (dim == 2)
                 L.nc == 8 ? uc_tiles<2, 8, R>(T, L, V, keep_child_tmp, fly) : uc_tiles<2, 16, R>(T, L, V, keep_child_tmp, fly);
@@ -298,12 +298,12 @@ void launch_correct(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L
 
 template <typename R>
 void launch_restrict(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                     const R *from, R *to)
+                     const R *from, R *to, const R *lsf, int rstr_mode)
 {
     if (L.n <= 0) return;
     emul_submit(s, [=] {
         run_stage(dim, L.nc, [&](auto d) { return (long long)L.n * ipow(L.nc / 2, DIMV); },
-                  [&](auto d, auto c, int g) { restrict_body<DIMV, NCV>(T, L, from, to, g); });
+                  [&](auto d, auto c, int g) { restrict_body<DIMV, NCV>(T, L, from, to, g, lsf, rstr_mode); });
     });
 }
 
@@ -435,29 +435,38 @@ void launch_from_fp32(hipStream_t s, long long n, const float *from, double *to,
         });
 }
 
-void launch_set_tags(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const double *eps)
+void launch_set_tags(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const double *eps,
+                     const double *lsf)
 {
     if (L.n <= 0) return;
     emul_submit(s, [=] {
         const int len = ipow(L.nc + 2, dim);
-        for (int b = 0; b < L.n; b++) {
-            const int slot = L.slot0 + b;
-            if (T.tag[slot] != TAG_UNSET) continue;
-            if (!eps) {
-                T.tag[slot] = TAG_NORMAL;
-                continue;
-            }
-            const double *e = eps + (size_t)slot * T.stride;
-            double lo = e[0], hi = e[0];
+        auto min_max = [&](const double *e, double &lo, double &hi) {
+            lo = hi = e[0];
             for (int q = 0; q < len; q++) {
                 lo = e[q] < lo ? e[q] : lo;
                 hi = e[q] > hi ? e[q] : hi;
             }
+        };
+        for (int b = 0; b < L.n; b++) {
+            const int slot = L.slot0 + b;
+            if (T.tag[slot] != TAG_UNSET) continue;
+            bool is_lsf = false, is_eps = false, is_deps = false;
+            double lo, hi;
+            if (lsf) {
+                min_max(lsf + (size_t)slot * T.stride, lo, hi);
+                is_lsf = lo * hi < 0;
+            }
+            if (eps) {
+                min_max(eps + (size_t)slot * T.stride, lo, hi);
+                is_deps = hi > lo;
+                if (!is_deps) is_eps = lo < 1 || lo > 1;
+            }
             int8_t tag = TAG_NORMAL;
-            if (hi > lo)
-                tag = TAG_VEPS;
-            else if (lo < 1 || lo > 1)
-                tag = TAG_CEPS;
+            if (is_lsf) tag = TAG_LSF;
+            if (is_eps) tag = TAG_CEPS;
+            if (is_deps) tag = TAG_VEPS;
+            if ((int)is_lsf + (int)is_eps + (int)is_deps > 1) tag = TAG_CONFLICT;
             T.tag[slot] = tag;
         }
     });
@@ -580,7 +589,7 @@ void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVa
                                       const MgVarsT<R> &, int);                                        \
     template void launch_correct<R>(int, hipStream_t, const DevTree &, const LevelArgs &,              \
                                     const MgVarsT<R> &, int);                                          \
-    template void launch_restrict<R>(int, hipStream_t, const DevTree &, const LevelArgs &, const R *, R *); \
+    template void launch_restrict<R>(int, hipStream_t, const DevTree &, const LevelArgs &, const R *, R *, const R *, int); \
     template void launch_prolong2<R>(int, hipStream_t, const DevTree &, const LevelArgs &, const R *, R *, \
                                      int);                                                             \
     template void launch_fill_value<R>(int, hipStream_t, const DevTree &, int, int, int, R *, double);  \

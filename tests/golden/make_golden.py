@@ -57,6 +57,23 @@ CASES = {
     # as above with eps = 1 below y = 0.5 and 5 above: mg2_box_clpld / gsrb_clpld
     "g2d_nc4_cyl_eps": (2, dict(nc=4, l_uniform=2, l_max=5, thr=2e-3, n_base=4, n_fmg=1, n_vcycle=2, cyl=True,
                                 use_eps=True, eps_hi=5.0, **G2)),
+    # internal boundary on the circle |r - (0.45, 0.55)| = 0.23 (level set in
+    # variable 5, boundary value 1 + x in variable 6): auto operators
+    # (mg_lsf_box tags: gsrb/box/corr_lpllsf) with rstr_lpl ...
+    "g2d_nc4_lsf": (2, dict(nc=4, l_uniform=2, l_max=5, thr=2e-3, n_base=4, n_fmg=1, n_vcycle=2, lsf_mode=1,
+                            lsf_c=[0.45, 0.55], lsf_rad=0.23, **G2)),
+    # ... with mg2_auto_rstr (every tag set first) ...
+    "g2d_nc4_lsf_rstr": (2, dict(nc=4, l_uniform=2, l_max=5, thr=2e-3, n_base=4, n_fmg=1, n_vcycle=2, lsf_mode=1,
+                                 rstr_auto=True, lsf_c=[0.45, 0.55], lsf_rad=0.23, **G2)),
+    # ... and the lpllsf operator set for every box
+    "g2d_nc4_lsf_set": (2, dict(nc=4, l_uniform=2, l_max=5, thr=2e-3, n_base=4, n_fmg=1, n_vcycle=2, lsf_mode=2,
+                                lsf_c=[0.45, 0.55], lsf_rad=0.23, **G2)),
+    # 3D, the sphere |r - (0.45, 0.55, 0.5)| = 0.27: auto operators with
+    # mg3_auto_rstr, and the lpllsf set
+    "g3d_nc4_lsf_rstr": (3, dict(nc=4, l_uniform=2, l_max=3, thr=2e-3, n_base=4, n_fmg=1, n_vcycle=1, lsf_mode=1,
+                                 rstr_auto=True, lsf_c=[0.45, 0.55, 0.5], lsf_rad=0.27, **G3)),
+    "g3d_nc4_lsf_set": (3, dict(nc=4, l_uniform=2, l_max=3, thr=2e-3, n_base=4, n_fmg=1, n_vcycle=1, lsf_mode=2,
+                                lsf_c=[0.45, 0.55, 0.5], lsf_rad=0.27, **G3)),
     # 3D, 4^3 boxes, constant coefficient
     "g3d_nc4": (3, dict(nc=4, l_uniform=1, l_max=3, thr=5e-3, n_base=4, n_fmg=1, n_vcycle=1, **G3)),
     # 3D, 4^3 boxes, adaptive (42 boxes, 32 faces at refinement boundaries)

@@ -32,6 +32,8 @@ from test_gpu_parity import (  # noqa: E402,F401
     test_c2_full_config_bitexact,
     test_cylindrical_solve_bitexact,
     test_golden_solve_bitexact,
+    test_level_set_and_eps_tag_conflict,
+    test_level_set_solve_bitexact,
     test_graphs_and_direct_launches_agree,
     test_periodic_base,
     test_solve_bitexact,

@@ -79,6 +79,16 @@ CASES = {
                        n_fmg=1, n_vcycle=3, cyl=True, **G2)),
     "2d_cyl_eps": (2, dict(nc=8, ref_mode="gauss", l_uniform=4, l_max=6, thr=1e-4, n_base=50,
                            n_fmg=1, n_vcycle=3, cyl=True, use_eps=True, eps_hi=4.0, **G2)),
+    # internal boundary on a circle / sphere (level set and boundary value):
+    # the auto operators with mg2_auto_rstr (tags set by mg2_set_box_tag
+    # first) and with rstr_lpl, and the lpllsf set
+    "2d_lsf_rstr": (2, dict(nc=8, ref_mode="gauss", l_uniform=4, l_max=7, thr=1e-4, n_base=50,
+                            n_fmg=1, n_vcycle=3, lsf_mode=1, rstr_auto=True, lsf_c=[0.45, 0.55],
+                            lsf_rad=0.23, **G2)),
+    "2d_lsf": (2, dict(nc=8, ref_mode="gauss", l_uniform=4, l_max=6, thr=1e-4, n_base=50,
+                       n_fmg=2, n_vcycle=1, lsf_mode=1, lsf_c=[0.45, 0.55], lsf_rad=0.23, **G2)),
+    "3d_lsf_set": (3, dict(nc=8, ref_mode="gauss", l_uniform=2, l_max=4, thr=1e-3, n_base=4,
+                           n_fmg=1, n_vcycle=2, lsf_mode=2, lsf_c=[0.45, 0.55, 0.5], lsf_rad=0.27, **G3)),
 }
 
 

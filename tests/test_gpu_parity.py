@@ -17,7 +17,7 @@ from afivo_amd import problems as P
 from afivo_amd import tree as T
 from afivo_amd._lib import AfgError
 from helpers import assert_bitexact, random_state
-from test_oracle import CASES, load_golden, opts_of
+from test_oracle import CASES, load_golden, opts_of, rstr_of
 
 pytestmark = pytest.mark.gpu
 
@@ -135,8 +135,9 @@ def test_golden_solve_bitexact(name):
     cfg, st_in, st_out, res_ref = load_golden(name)
     c = cfg["cfg"]
     eps = c.get("use_eps", False)
+    lsf = c.get("lsf_mode", 0)
     mg = M.default_mg(eps=eps, n_down=c.get("n_down", 2), n_up=c.get("n_up", 2), n_base=c["n_base"],
-                      auto=c.get("cyl", False))
+                      auto=c.get("cyl", False), lsf=lsf == 1, lsf_set=lsf == 2, rstr=rstr_of(c))
     dt = M.DeviceTree.from_state(st_in)
     if c.get("cyl"):
         dt.set_coord(True)
@@ -424,4 +425,50 @@ def test_cylindrical_solve_bitexact(eps):
     assert rd[-1] < 1e-3 * rd[0]
     with pytest.raises(AfgError):
         M.mixed_solve(dt, mg, 1e-8, max_iter=5)
+    dt.close()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("mode", ["auto", "auto_rstr", "set"])
+def test_level_set_solve_bitexact(dim, mode):
+    """An internal boundary on a circle (2D benchmark tree, 951 boxes of 8^2)
+    or a sphere (3D, 8^3 boxes, levels -1..4): level set and boundary value
+    in variables 5 and 6.  Modes: the auto operators (boxes tagged
+    mg_lsf_box take gsrb/box/corr_lpllsf) with rstr_lpl or with
+    mg$D_auto_rstr, and the lpllsf set for every box.  FMG (twice, the second
+    from the guess) and 2 V-cycles: every variable and the residual history
+    bit for bit against the oracle.  FP32 stages are refused."""
+    p = P.c1() if dim == 2 else P.small_3d(nc=8, l_uniform=3, l_max=4)
+    st = p.state
+    while len(st.data) < 4:
+        st.data.append(np.zeros(st.n_values()))
+    lsf, bval = P.set_level_set(st, p.tree, [0.45, 0.55, 0.5][:dim], 0.23)
+    st.data += [lsf, bval]
+    rstr = {"auto": O.RSTR_LPL, "auto_rstr": O.RSTR_AUTO, "set": O.RSTR_LSF}[mode]
+    mg = M.default_mg(n_base=p.n_base, lsf=True, lsf_set=mode == "set", rstr=rstr)
+    om = O.default_opts(2, 2, p.n_base, lsf=True, lsf_set=mode == "set", rstr=rstr)
+    dt = M.DeviceTree.from_state(st, 6)
+    ot = O.OracleTree(st, n_var=6)
+    if mode == "auto_rstr":
+        ot.set_tags(om)
+    dt, ot, rd, ro = solve_both(st, 6, mg, om, 2, 2, dt, ot)
+    compare_all(dt, ot, 6, f"level set {dim}D {mode}")
+    assert np.array_equal(rd, ro)
+    assert rd[-1] < 1e-2 * rd[0]
+    with pytest.raises(AfgError):
+        M.mixed_solve(dt, mg, 1e-8, max_iter=5)
+    dt.close()
+
+
+def test_level_set_and_eps_tag_conflict():
+    """A box with both a level-set and an eps tag is an error, where
+    mg$D_set_box_tag stops (m_aX_mg.f90:700-701)."""
+    p = P.small_3d(nc=8, l_uniform=2, l_max=3, eps_hi=3.0)
+    st = p.state
+    lsf, bval = P.set_level_set(st, p.tree, [0.5, 0.5, 0.5], 0.3)
+    st.data += [lsf, bval]
+    dt = M.DeviceTree.from_state(st, 6)
+    mg = M.default_mg(eps=True, n_base=p.n_base, lsf=True)
+    with pytest.raises(AfgError, match="Cannot set lsf and eps tag"):
+        M.mg_fas_fmg(dt, mg, True, False)
     dt.close()

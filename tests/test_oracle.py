@@ -51,10 +51,17 @@ def load_xfer(name):
     return meta, {op: (z[op + "_pos"], z[op + "_val"]) for op in meta["ops"]}
 
 
+def rstr_of(c):
+    """The restriction of a harness configuration: rstr_lpllsf with the
+    lpllsf set (lsf_mode 2), mg$D_auto_rstr when rstr_auto, else rstr_lpl."""
+    return O.RSTR_LSF if c.get("lsf_mode", 0) == 2 else O.RSTR_AUTO if c.get("rstr_auto") else O.RSTR_LPL
+
+
 def opts_of(cfg):
     c = cfg["cfg"]
+    lsf = c.get("lsf_mode", 0)
     return O.default_opts(c.get("n_down", 2), c.get("n_up", 2), c["n_base"], eps=c.get("use_eps", False),
-                          auto=c.get("cyl", False))
+                          auto=c.get("cyl", False), lsf=lsf == 1, lsf_set=lsf == 2, rstr=rstr_of(c))
 
 
 def run_solve(solver, cfg, res_fn):
@@ -76,6 +83,8 @@ def test_oracle_matches_reference_golden(name):
     ot = O.OracleTree(st_in)
     if cfg["cfg"].get("cyl"):
         ot.set_coord(True)
+    if cfg["cfg"].get("rstr_auto"):   # the harness tags every box first (mg$D_set_box_tag)
+        ot.set_tags(opts_of(cfg))
     res = run_solve(ot, cfg, lambda: ot.max_abs(3))
     for iv in range(1, st_out.n_var + 1):
         assert_bitexact(ot.get(iv), st_out.data[iv - 1], f"{name} var {iv}")
