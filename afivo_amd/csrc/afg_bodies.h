@@ -45,6 +45,9 @@ struct LevelViewT {
     // level set and boundary value of the lpllsf operators, with the modes of MgVarsT
     const R *lsf = nullptr, *bval = nullptr;
     int lsf_mode = 0, rstr_mode = 0;
+    // rhs by colour (MgVarsT::rhs_rb), or nullptr
+    R *rhs_rb = nullptr;
+    int rb_stride = 0;
     AFG_DEV size_t off(int slot) const
     {
         return (size_t)(slot - slot0) * (size_t)stride;
@@ -465,6 +468,34 @@ AFG_HD inline void colour_cell(int c, int nc, int rb, int &i, int &j, int &k)
     i = 2 - ((rb ^ (DIM == 2 ? j : k + j)) & 1) + 2 * p;
 }
 
+// position of interior cell (i, j, k) in the colour-split rhs of its box:
+// colour rb - 1 block (rb = 1 where i + j + k is odd), index c of colour_cell
+template <int DIM>
+AFG_HD inline int colour_index(int i, int j, int k, int nc)
+{
+    const int hn = nc >> 1;
+    const int r = DIM == 2 ? j - 1 : (j - 1) + nc * (k - 1);
+    const int odd = (i + j + k) & 1;
+    return (odd ? 0 : ipow(nc, DIM) / 2) + r * hn + ((i - 1) >> 1);
+}
+
+// rhs_rb of one slot from rhs: one thread per interior cell of a level
+template <int DIM, typename R>
+AFG_DEV void split_rhs_body(const DevTree &T, const LevelArgs &L, const R *rhs, R *rhs_rb, int rb_stride, int gid)
+{
+    const int nc = L.nc;
+    const int per_box = ipow(nc, DIM);
+    const int b = gid / per_box;
+    const int c = gid - b * per_box;
+    const int i = c % nc + 1;
+    const int j = (c / nc) % nc + 1;
+    const int k = DIM == 3 ? c / (nc * nc) + 1 : 0;
+    const int s = nc + 2;
+    const int slot = L.slot0 + b;
+    rhs_rb[(size_t)slot * rb_stride + colour_index<DIM>(i, j, k, nc)] =
+        rhs[(size_t)slot * T.stride + i + s * (j + s * k)];
+}
+
 template <int DIM, int NC, typename R>
 AFG_DEV void gsrb_body(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &own,
                        const LevelViewT<R> &par, int op_auto, int rb, int fly, int gid)
@@ -554,9 +585,14 @@ AFG_DEV void tile_stage_a(const DevTree &T, const LevelArgs &L, const LevelViewT
         const int e = tid + r * G::THREADS;
         const int b = e / G::HALF;
         if (e < G::BOXES * G::HALF && box0 + b < L.n) {
-            int i, j, k;
-            colour_cell<DIM>(e - b * G::HALF, NC, rb, i, j, k);
-            rhs[r] = v.rhs[v.off(L.slot0 + box0 + b) + i + S * (j + S * k)];
+            const int c = e - b * G::HALF;
+            if (v.rhs_rb) {   // rb counts passes; odd passes take colour 1
+                rhs[r] = v.rhs_rb[(size_t)(L.slot0 + box0 + b) * v.rb_stride + ((rb + 1) & 1) * G::HALF + c];
+            } else {
+                int i, j, k;
+                colour_cell<DIM>(c, NC, rb, i, j, k);
+                rhs[r] = v.rhs[v.off(L.slot0 + box0 + b) + i + S * (j + S * k)];
+            }
         }
     }
     // interiors
@@ -664,9 +700,14 @@ AFG_DEV void pair_load_rhs(const LevelArgs &L, const LevelViewT<R> &v, int box0,
         const int e = tid + r * G::THREADS;
         const int b = e / G::HALF;
         if (e < G::BOXES * G::HALF && box0 + b < L.n) {
-            int i, j, k;
-            colour_cell<DIM>(e - b * G::HALF, NC, rb, i, j, k);
-            rhs[r] = v.rhs[v.off(L.slot0 + box0 + b) + i + S * (j + S * k)];
+            const int c = e - b * G::HALF;
+            if (v.rhs_rb) {   // rb counts passes; odd passes take colour 1
+                rhs[r] = v.rhs_rb[(size_t)(L.slot0 + box0 + b) * v.rb_stride + ((rb + 1) & 1) * G::HALF + c];
+            } else {
+                int i, j, k;
+                colour_cell<DIM>(c, NC, rb, i, j, k);
+                rhs[r] = v.rhs[v.off(L.slot0 + box0 + b) + i + S * (j + S * k)];
+            }
         }
     }
 }
@@ -1111,7 +1152,9 @@ AFG_DEV void uc_parent_body(const DevTree &T, const LevelArgs &L, const LevelVie
                         ? box_op_cyl(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld,
                                      T.cyl_ioff[slot] + i)
                         : box_op<DIM>(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld);
-        own.rhs[base + q] = a + own.tmp[base + q];
+        const R tau = a + own.tmp[base + q];
+        own.rhs[base + q] = tau;
+        if (own.rhs_rb) own.rhs_rb[(size_t)slot * own.rb_stride + colour_index<DIM>(i, j, k, nc)] = tau;
         if (copy_tmp) own.tmp[base + q] = box[q];
         return;
     }

@@ -164,6 +164,12 @@ struct afg_ctx {
     // there and back within a relaxation block
     double *d_phi2 = nullptr;
     float *d_phi2_32 = nullptr;
+    // rhs by colour without ghosts (FP64, FP32 stages), read by the tiled
+    // colour passes; valid while it equals the interiors of variable rb_var
+    double *d_rhs_rb = nullptr;
+    float *d_rhs_rb32 = nullptr;
+    bool rb_valid[2] = {false, false};
+    int rb_var[2] = {-1, -1};
     double *d_partial = nullptr;
     int partial_cap = 0;
     double *d_out = nullptr, *h_out = nullptr;
@@ -319,6 +325,11 @@ void free_tree(afg_ctx *c)
     hipFree(c->d_phi2_32);
     c->d_phi2 = nullptr;
     c->d_phi2_32 = nullptr;
+    hipFree(c->d_rhs_rb);
+    hipFree(c->d_rhs_rb32);
+    c->d_rhs_rb = nullptr;
+    c->d_rhs_rb32 = nullptr;
+    c->rb_valid[0] = c->rb_valid[1] = false;
     c->d_nb = c->d_parent = c->d_choff = c->d_lists = c->d_bc_type = c->d_bc_voff = nullptr;
     c->d_bc_vals = nullptr;
     c->d_tag = nullptr;
@@ -552,6 +563,32 @@ using Mg = MgT<double>;
 
 double *&phi2_pool(afg_ctx *c, double *) { return c->d_phi2; }
 float *&phi2_pool(afg_ctx *c, float *) { return c->d_phi2_32; }
+double *&rhs_rb_pool(afg_ctx *c, double *) { return c->d_rhs_rb; }
+float *&rhs_rb_pool(afg_ctx *c, float *) { return c->d_rhs_rb32; }
+
+// values per slot of the colour-split rhs: the interior of the largest boxes
+int rb_stride(const afg_ctx *c) { return c->dim == 2 ? c->nc * c->nc : c->nc * c->nc * c->nc; }
+
+// Cell data changed outside the multigrid stages: the colour-split rhs must
+// be formed again before the next colour pass reads it.
+void data_written(afg_ctx *c) { c->rb_valid[0] = c->rb_valid[1] = false; }
+
+// Form the colour-split rhs of every level from rhs (consecutive levels of
+// equal cell count in one launch) and mark it valid for variable iv.
+template <typename R>
+void refresh_rhs_rb(afg_ctx *c, const DevTree &T, const R *rhs, R *rb, int iv)
+{
+    const int nl = (int)c->levels.size();
+    for (int a = 0; a < nl;) {
+        LevelArgs L = level_args(c, c->levels[a].lvl);
+        int b = a + 1;
+        for (; b < nl && c->levels[b].nc == c->levels[a].nc; b++) L.n += c->levels[b].n;
+        launch_split_rhs(c->dim, c->stream, T, L, rhs, rb, rb_stride(c));
+        a = b;
+    }
+    c->rb_valid[sizeof(R) == 4] = true;
+    c->rb_var[sizeof(R) == 4] = iv;
+}
 
 template <typename R>
 int make_mg(afg_ctx *c, const afg_mg_opts *o, MgT<R> *m)
@@ -609,6 +646,24 @@ int make_mg(afg_ctx *c, const afg_mg_opts *o, MgT<R> *m)
     }
     if (int e = upload_bc(c)) return e;
     m->T = dev_tree(c);
+    // the colour-split rhs, for the tiled colour passes (not on the trees and
+    // operators that take the per-cell kernels only)
+    m->V.rhs_rb = nullptr;
+    m->V.rb_stride = rb_stride(c);
+    if (!per_cell_only(m->T, m->V) && !c->capturing) {
+        const int k = sizeof(R) == 4;
+        R *&rb = rhs_rb_pool(c, (R *)nullptr);
+        if (!rb) {
+            const size_t bytes = (size_t)c->n_slots * rb_stride(c) * sizeof(R);
+            if (hipMalloc(&rb, bytes) != hipSuccess) {
+                rb = nullptr;
+                return c->fail(AFG_ERR_MEMORY, "hipMalloc(%zu) for the colour-split rhs", bytes);
+            }
+            c->rb_valid[k] = false;
+        }
+        if (!c->rb_valid[k] || c->rb_var[k] != o->i_rhs) refresh_rhs_rb(c, m->T, m->V.rhs, rb, o->i_rhs);
+        m->V.rhs_rb = rb;
+    }
     return 0;
 }
 
@@ -650,10 +705,16 @@ void fill_gc(MgT<R> &m, R *var, int lvl)
 // first pass reads the ghosts in memory (first_fly = 0), as the reference
 // does, or computes them when a ghost fill precedes it (first_fly = 1).
 template <typename R>
+int bottom_top(const MgT<R> &m);
+
+template <typename R>
 void passes(MgT<R> &m, int lvl, int n_cycle, int first_fly)
 {
     const LevelArgs L = level_args(m.c, lvl);
-    timed_passes(m.c, m.T, L, lvl, m.V, m.phi2, 2 * n_cycle, first_fly);
+    MgVarsT<R> V = m.V;
+    // the bottom kernel writes the rhs of its levels without the colour-split copy
+    if (lvl <= bottom_top(m)) V.rhs_rb = nullptr;
+    timed_passes(m.c, m.T, L, lvl, V, m.phi2, 2 * n_cycle, first_fly);
 }
 
 // gsrb_boxes as the reference has it: passes, then the last ghost fill.
@@ -694,6 +755,7 @@ void init_phi_rhs(MgT<R> &m)
         launch_fill_value(m.c->dim, m.c->stream, m.T, L.slot0, L.n, L.nc, m.V.phi, 0.0);
         launch_restrict(m.c->dim, m.c->stream, m.T, L, m.V.rhs, m.V.rhs, (const R *)m.V.lsf, m.V.rstr_mode);
     }
+    if (m.V.rhs_rb) refresh_rhs_rb(m.c, m.T, (const R *)m.V.rhs, m.V.rhs_rb, m.o->i_rhs);
 }
 
 // correct_children over the parents of lvl-1 (m_aX_mg.f90:368-395) followed
@@ -881,6 +943,7 @@ int check_level(afg_ctx *c, int lvl, bool need_parent)
 // variables in LDS; each has its own highest level.
 int choose_bottom(afg_ctx *c)
 {
+    data_written(c);   // levels may move between the bottom kernel and the per-level passes
     const int nl = (int)c->levels.size();
     int64_t max_threads = 64;
     for (int k = 0; k < 4; k++) {
@@ -1823,6 +1886,7 @@ static int copy_runs(afg_ctx *c, double *dev, const std::vector<char> &mark, boo
 int afg_upload_cc(afg_ctx *c, int iv, int n_ids, const int32_t *ids, const double *packed)
 {
     if (int e = check_ctx(c)) return e;
+    data_written(c);
     double *dev = nullptr;
     if (int e = ensure_var(c, iv, &dev)) return e;
     if (int e = ensure_stage(c, (size_t)c->n_slots * c->stride)) return e;
@@ -1994,6 +2058,7 @@ int afg_consistent_fluxes(afg_ctx *c, int n_f, const int32_t *f_ixs)
 int afg_copy_cc(afg_ctx *c, int iv_from, int iv_to)
 {
     if (int e = check_ctx(c)) return e;
+    data_written(c);
     double *a = nullptr, *b = nullptr;
     if (int e = ensure_var(c, iv_from, &a)) return e;
     if (int e = ensure_var(c, iv_to, &b)) return e;
@@ -2005,6 +2070,7 @@ int afg_copy_cc(afg_ctx *c, int iv_from, int iv_to)
 int afg_set_cc(afg_ctx *c, int iv, double value)
 {
     if (int e = check_ctx(c)) return e;
+    data_written(c);
     double *a = nullptr;
     if (int e = ensure_var(c, iv, &a)) return e;
     const DevTree T = dev_tree(c);
@@ -2071,6 +2137,7 @@ int afg_fas_fmg(afg_ctx *c, const afg_mg_opts *o, int set_residual, int have_gue
 int afg_fp32_from_cc(afg_ctx *c, int iv, int iv32)
 {
     if (int e = check_ctx(c)) return e;
+    data_written(c);
     double *a = nullptr;
     float *b = nullptr;
     if (int e = ensure_var(c, iv, &a)) return e;
@@ -2082,6 +2149,7 @@ int afg_fp32_from_cc(afg_ctx *c, int iv, int iv32)
 int afg_fp32_to_cc(afg_ctx *c, int iv32, int iv, int add)
 {
     if (int e = check_ctx(c)) return e;
+    data_written(c);
     double *a = nullptr;
     float *b = nullptr;
     if (int e = ensure_var(c, iv, &a)) return e;
@@ -2191,6 +2259,7 @@ int afg_mixed_correct(afg_ctx *c, const afg_mg_opts *o, int cycles)
     // correction e from A e = r in FP32: `cycles` V-cycles from e = 0 (the
     // first of the first correction an FMG), then phi += e
     launch_to_fp32(c->stream, n, m.V.tmp, f.V.rhs);
+    if (f.V.rhs_rb) refresh_rhs_rb(c, f.T, (const float *)f.V.rhs, f.V.rhs_rb, o->i_rhs);
     HIP_TRY(c, hipMemsetAsync(f.V.phi, 0, n * sizeof(float), c->stream));
     HIP_TRY(c, hipMemsetAsync(f.V.tmp, 0, n * sizeof(float), c->stream));
     for (int k = 0; k < cycles; k++) {
@@ -2323,6 +2392,7 @@ int afg_residual_level(afg_ctx *c, const afg_mg_opts *o, int lvl)
 int afg_restrict_level(afg_ctx *c, int iv, int lvl)
 {
     if (int e = check_ctx(c)) return e;
+    data_written(c);
     if (int e = check_level(c, lvl, true)) return e;
     double *v = nullptr;
     if (int e = ensure_var(c, iv, &v)) return e;
@@ -2333,6 +2403,7 @@ int afg_restrict_level(afg_ctx *c, int iv, int lvl)
 int afg_prolong2_level(afg_ctx *c, int iv, int iv_to, int lvl, int add)
 {
     if (int e = check_ctx(c)) return e;
+    data_written(c);
     if (int e = check_level(c, lvl, true)) return e;
     double *a = nullptr, *b = nullptr;
     if (int e = ensure_var(c, iv, &a)) return e;
@@ -2438,6 +2509,7 @@ int afg_gc_tree(afg_ctx *c, int iv, int rb, int bc)
 int afg_restrict_tree(afg_ctx *c, int iv, int iv_to)
 {
     if (int e = check_ctx(c)) return e;
+    data_written(c);
     double *a = nullptr, *b = nullptr;
     if (int e = ensure_var(c, iv, &a)) return e;
     if (int e = ensure_var(c, iv_to, &b)) return e;
@@ -2450,6 +2522,7 @@ int afg_restrict_tree(afg_ctx *c, int iv, int iv_to)
 int afg_prolong_level(afg_ctx *c, int iv, int iv_to, int lvl, int order, int add)
 {
     if (int e = check_ctx(c)) return e;
+    data_written(c);
     if (int e = check_level(c, lvl, true)) return e;
     if (order < 0 || order > 2) return c->fail(AFG_ERR_ARG, "prolongation order %d not in 0..2", order);
     double *a = nullptr, *b = nullptr;
@@ -2462,6 +2535,7 @@ int afg_prolong_level(afg_ctx *c, int iv, int iv_to, int lvl, int order, int add
 static int pool_arith(afg_ctx *c, int op, double a, int iv_x, double b, int iv_y)
 {
     if (int e = check_ctx(c)) return e;
+    data_written(c);
     double *x = nullptr, *y = nullptr;
     if (int e = ensure_var(c, iv_x, &x)) return e;
     if (int e = ensure_var(c, iv_y, &y)) return e;

@@ -41,6 +41,10 @@ void launch_uc_parent(int dim, hipStream_t s, const DevTree &T, const LevelArgs 
 template <typename R>
 void launch_correct(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
                     const MgVarsT<R> &V, int fly_diff);
+// rhs by colour (MgVarsT::rhs_rb) of every box of a level, from rhs
+template <typename R>
+void launch_split_rhs(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const R *rhs, R *rhs_rb,
+                      int rb_stride);
 template <typename R>
 void launch_restrict(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
                      const R *from, R *to, const R *lsf = nullptr, int rstr_mode = 0);

@@ -22,7 +22,8 @@ inline int n_blocks(long long threads, int block = BLOCK)
 template <typename R>
 LevelViewT<R> pool_view(const DevTree &T, const MgVarsT<R> &V)
 {
-    return LevelViewT<R>{V.phi, V.rhs, V.tmp, V.eps, 0, T.stride, V.lsf, V.bval, V.lsf_mode, V.rstr_mode};
+    return LevelViewT<R>{V.phi, V.rhs, V.tmp, V.eps, 0, T.stride, V.lsf, V.bval, V.lsf_mode, V.rstr_mode,
+                         V.rhs_rb, V.rb_stride};
 }
 
 template <typename R>
@@ -159,6 +160,14 @@ __global__ __launch_bounds__(BLOCK) void k_correct(DevTree T, LevelArgs L, Level
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid < L.n * ipow((NC ? NC : L.nc) / 2, DIM))
         correct_body<DIM, NC>(T, L, v, v, op_auto, fly_diff, gid);
+}
+
+// rhs by colour from rhs over the boxes of a level
+template <int DIM, typename R>
+__global__ __launch_bounds__(BLOCK) void k_split_rhs(DevTree T, LevelArgs L, const R *rhs, R *rhs_rb, int rb_stride)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < L.n * ipow(L.nc, DIM)) split_rhs_body<DIM>(T, L, rhs, rhs_rb, rb_stride, gid);
 }
 
 // a$D_restrict_box and a$D_prolong2 over the boxes of a level
@@ -630,6 +639,18 @@ void launch_correct(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L
 }
 
 template <typename R>
+void launch_split_rhs(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const R *rhs, R *rhs_rb,
+                      int rb_stride)
+{
+    if (L.n <= 0) return;
+    const long long th = (long long)L.n * ipow(L.nc, dim);
+    if (dim == 2)
+        k_split_rhs<2, R><<<n_blocks(th), BLOCK, 0, s>>>(T, L, rhs, rhs_rb, rb_stride);
+    else
+        k_split_rhs<3, R><<<n_blocks(th), BLOCK, 0, s>>>(T, L, rhs, rhs_rb, rb_stride);
+}
+
+template <typename R>
 void launch_restrict(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
                      const R *from, R *to, const R *lsf, int rstr_mode)
 {
@@ -850,6 +871,7 @@ void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVa
     template void launch_correct<R>(int, hipStream_t, const DevTree &, const LevelArgs &,              \
                                     const MgVarsT<R> &, int);                                          \
     template void launch_restrict<R>(int, hipStream_t, const DevTree &, const LevelArgs &, const R *, R *, const R *, int); \
+    template void launch_split_rhs<R>(int, hipStream_t, const DevTree &, const LevelArgs &, const R *, R *, int); \
     template void launch_prolong2<R>(int, hipStream_t, const DevTree &, const LevelArgs &, const R *, R *, \
                                      int);                                                             \
     template void launch_fill_value<R>(int, hipStream_t, const DevTree &, int, int, int, R *, double);  \

@@ -26,7 +26,8 @@ constexpr int BLOCK = 256;
 template <typename R>
 LevelViewT<R> pool_view(const DevTree &T, const MgVarsT<R> &V)
 {
-    return LevelViewT<R>{V.phi, V.rhs, V.tmp, V.eps, 0, T.stride, V.lsf, V.bval, V.lsf_mode, V.rstr_mode};
+    return LevelViewT<R>{V.phi, V.rhs, V.tmp, V.eps, 0, T.stride, V.lsf, V.bval, V.lsf_mode, V.rstr_mode,
+                         V.rhs_rb, V.rb_stride};
 }
 
 template <typename R>
@@ -293,6 +294,22 @@ void launch_correct(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L
         const LevelViewT<R> v = pool_view(T, V);
         run_stage(dim, L.nc, [&](auto d) { return (long long)L.n * ipow(L.nc / 2, DIMV); },
                   [&](auto d, auto c, int g) { correct_body<DIMV, NCV>(T, L, v, v, V.op_auto, fly_diff, g); });
+    });
+}
+
+template <typename R>
+void launch_split_rhs(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const R *rhs, R *rhs_rb,
+                      int rb_stride)
+{
+    if (L.n <= 0) return;
+    emul_submit(s, [=] {
+        const long long n = (long long)L.n * ipow(L.nc, dim);
+        for (long long g = 0; g < n; g++) {
+            if (dim == 2)
+                split_rhs_body<2>(T, L, rhs, rhs_rb, rb_stride, (int)g);
+            else
+                split_rhs_body<3>(T, L, rhs, rhs_rb, rb_stride, (int)g);
+        }
     });
 }
 
@@ -590,6 +607,7 @@ void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVa
     template void launch_correct<R>(int, hipStream_t, const DevTree &, const LevelArgs &,              \
                                     const MgVarsT<R> &, int);                                          \
     template void launch_restrict<R>(int, hipStream_t, const DevTree &, const LevelArgs &, const R *, R *, const R *, int); \
+    template void launch_split_rhs<R>(int, hipStream_t, const DevTree &, const LevelArgs &, const R *, R *, int); \
     template void launch_prolong2<R>(int, hipStream_t, const DevTree &, const LevelArgs &, const R *, R *, \
                                      int);                                                             \
     template void launch_fill_value<R>(int, hipStream_t, const DevTree &, int, int, int, R *, double);  \

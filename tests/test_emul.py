@@ -36,6 +36,7 @@ from test_gpu_parity import (  # noqa: E402,F401
     test_level_set_solve_bitexact,
     test_graphs_and_direct_launches_agree,
     test_periodic_base,
+    test_rhs_changes_between_solves,
     test_solve_bitexact,
     test_stage_bitexact,
     test_tile_shapes_are_exact,
