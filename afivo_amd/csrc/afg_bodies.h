@@ -586,7 +586,7 @@ AFG_DEV void tile_stage_a(const DevTree &T, const LevelArgs &L, const LevelViewT
         const int b = e / G::HALF;
         if (e < G::BOXES * G::HALF && box0 + b < L.n) {
             const int c = e - b * G::HALF;
-            if (v.rhs_rb) {   // rb counts passes; odd passes take colour 1
+            if (DIM == 3 && v.rhs_rb) {   // rb counts passes; odd passes take colour 1
                 rhs[r] = v.rhs_rb[(size_t)(L.slot0 + box0 + b) * v.rb_stride + ((rb + 1) & 1) * G::HALF + c];
             } else {
                 int i, j, k;
@@ -701,7 +701,7 @@ AFG_DEV void pair_load_rhs(const LevelArgs &L, const LevelViewT<R> &v, int box0,
         const int b = e / G::HALF;
         if (e < G::BOXES * G::HALF && box0 + b < L.n) {
             const int c = e - b * G::HALF;
-            if (v.rhs_rb) {   // rb counts passes; odd passes take colour 1
+            if (DIM == 3 && v.rhs_rb) {   // rb counts passes; odd passes take colour 1
                 rhs[r] = v.rhs_rb[(size_t)(L.slot0 + box0 + b) * v.rb_stride + ((rb + 1) & 1) * G::HALF + c];
             } else {
                 int i, j, k;
@@ -738,7 +738,8 @@ AFG_DEV R cell_after_pass(const DevTree &T, const LevelArgs &L, const LevelViewT
     }
     if (DIM == 2) u[4] = u[5] = R(0);
     const bool lpld = lpld_box(T, op_auto, y);
-    return gsrb_value<DIM>(u, v.rhs[base + q], lpld ? v.eps + base : nullptr, q, S, L.dx2, lpld);
+    const R rhs = DIM == 3 && v.rhs_rb ? v.rhs_rb[(size_t)y * v.rb_stride + colour_index<DIM>(i, j, k, NC)] : v.rhs[base + q];
+    return gsrb_value<DIM>(u, rhs, lpld ? v.eps + base : nullptr, q, S, L.dx2, lpld);
 }
 
 // the colour-B face ghost e of the tile: (box b, face f, adjacent interior
@@ -893,7 +894,9 @@ AFG_DEV void residual_body(const DevTree &T, const LevelArgs &L, const LevelView
                 : DIM == 2 && cyl_op(T, op_auto)
                     ? box_op_cyl(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld, T.cyl_ioff[slot] + i)
                     : box_op<DIM>(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld);
-    own.tmp[base + q] = own.rhs[base + q] - a;
+    const R rh = DIM == 3 && own.rhs_rb ? own.rhs_rb[(size_t)slot * own.rb_stride + colour_index<DIM>(i, j, k, nc)]
+                            : own.rhs[base + q];
+    own.tmp[base + q] = rh - a;
 }
 
 // update_coarse, child part (m_aX_mg.f90:444-465), fused: per coarse cell of a
@@ -1096,7 +1099,10 @@ AFG_DEV void uc_tile_b(const DevTree &T, const LevelArgs &L, const LevelViewT<R>
         u[4] = DIM == 3 ? x[-S * S] : R(0);
         u[5] = DIM == 3 ? x[S * S] : R(0);
         const bool lpld = lpld_box(T, op_auto, slot);
-        res[r] = v.rhs[base + q] - box_op<DIM>(u, x[0], lpld ? v.eps + base : nullptr, q, S, L.inv_dr_sq, lpld);
+        const R rh = DIM == 3 && v.rhs_rb ? v.rhs_rb[(size_t)slot * v.rb_stride +
+                                         colour_index<DIM>(q % S, (q / S) % S, DIM == 3 ? q / (S * S) : 0, NC)]
+                              : v.rhs[base + q];
+        res[r] = rh - box_op<DIM>(u, x[0], lpld ? v.eps + base : nullptr, q, S, L.inv_dr_sq, lpld);
         if (!keep_child_tmp) v.tmp[base + q] = res[r];
     }
     uc_tile_restrict<DIM, NC>(T, L, v, lds, box0, tid, v.phi);
@@ -1154,7 +1160,7 @@ AFG_DEV void uc_parent_body(const DevTree &T, const LevelArgs &L, const LevelVie
                         : box_op<DIM>(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld);
         const R tau = a + own.tmp[base + q];
         own.rhs[base + q] = tau;
-        if (own.rhs_rb) own.rhs_rb[(size_t)slot * own.rb_stride + colour_index<DIM>(i, j, k, nc)] = tau;
+        if (DIM == 3 && own.rhs_rb) own.rhs_rb[(size_t)slot * own.rb_stride + colour_index<DIM>(i, j, k, nc)] = tau;
         if (copy_tmp) own.tmp[base + q] = box[q];
         return;
     }

@@ -646,11 +646,14 @@ int make_mg(afg_ctx *c, const afg_mg_opts *o, MgT<R> *m)
     }
     if (int e = upload_bc(c)) return e;
     m->T = dev_tree(c);
-    // the colour-split rhs, for the tiled colour passes (not on the trees and
-    // operators that take the per-cell kernels only)
+    // the colour-split rhs, for the tiled colour passes in 3D (not on the
+    // trees and operators that take the per-cell kernels only).  In 2D a box's
+    // rhs rows are short and the fused pass pairs read the neighbours' rows
+    // anyway, so the copy saves little there and the 2D kernels keep one rhs
+    // path (fewer registers).
     m->V.rhs_rb = nullptr;
     m->V.rb_stride = rb_stride(c);
-    if (!per_cell_only(m->T, m->V) && !c->capturing) {
+    if (c->dim == 3 && !per_cell_only(m->T, m->V) && !c->capturing) {
         const int k = sizeof(R) == 4;
         R *&rb = rhs_rb_pool(c, (R *)nullptr);
         if (!rb) {
@@ -707,14 +710,23 @@ void fill_gc(MgT<R> &m, R *var, int lvl)
 template <typename R>
 int bottom_top(const MgT<R> &m);
 
+// The variables for the stages of level lvl that read rhs: the colour-split
+// copy only above the bottom kernel's levels (that kernel writes the rhs of
+// its levels without the copy; the stages that write rhs keep it current on
+// every level).
+template <typename R>
+MgVarsT<R> level_vars(const MgT<R> &m, int lvl)
+{
+    MgVarsT<R> V = m.V;
+    if (lvl <= bottom_top(m)) V.rhs_rb = nullptr;
+    return V;
+}
+
 template <typename R>
 void passes(MgT<R> &m, int lvl, int n_cycle, int first_fly)
 {
     const LevelArgs L = level_args(m.c, lvl);
-    MgVarsT<R> V = m.V;
-    // the bottom kernel writes the rhs of its levels without the colour-split copy
-    if (lvl <= bottom_top(m)) V.rhs_rb = nullptr;
-    timed_passes(m.c, m.T, L, lvl, V, m.phi2, 2 * n_cycle, first_fly);
+    timed_passes(m.c, m.T, L, lvl, level_vars(m, lvl), m.phi2, 2 * n_cycle, first_fly);
 }
 
 // gsrb_boxes as the reference has it: passes, then the last ghost fill.
@@ -733,7 +745,7 @@ void gsrb_level(MgT<R> &m, int lvl, int n_cycle)
 template <typename R>
 void update_coarse(MgT<R> &m, int lvl, int relaxed)
 {
-    launch_uc_child(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), m.V, 1, relaxed ? 1 : 0);
+    launch_uc_child(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), level_vars(m, lvl), 1, relaxed ? 1 : 0);
     launch_uc_parent(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl - 1), m.V, 1);
 }
 
@@ -742,7 +754,7 @@ template <typename R>
 void set_coarse_phi_rhs(MgT<R> &m, int lvl)
 {
     if (lvl == m.c->highest_lvl) fill_gc(m, m.V.phi, lvl);
-    launch_uc_child(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), m.V, 0, 0);
+    launch_uc_child(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), level_vars(m, lvl), 0, 0);
     launch_uc_parent(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl - 1), m.V, 0);
 }
 
@@ -784,7 +796,7 @@ void residual_level(MgT<R> &m, int lvl)
         a.slot0 += skip;
         a.n -= skip;
     }
-    launch_residual(c->dim, c->stream, m.T, a, m.V);
+    launch_residual(c->dim, c->stream, m.T, a, level_vars(m, lvl));
 }
 
 template <typename R>

@@ -95,7 +95,7 @@ struct MgVarsT {
     // child's tag, 2 rstr_lpllsf for every box
     const R *lsf = nullptr, *bval = nullptr;
     int lsf_mode = 0, rstr_mode = 0;
-    // rhs by colour, without ghosts (nullptr: not kept): slot s holds at
+    // rhs by colour, without ghosts (nullptr: not kept; 3D only): slot s holds at
     // s * rb_stride the interior values of colour 1, then of colour 2, each in
     // the order of colour_cell; the tiled colour passes read it in place of
     // rhs (a pass then reads half of the interior instead of every cache
