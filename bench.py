@@ -140,6 +140,29 @@ def solve_to_tol_together(items, mg, tol, max_cycles=30):
     return cycles
 
 
+def mixed_to_tol_together(items, mg, tol, cycles, max_corrections=40):
+    """FP64 defect correction with FP32 FAS corrections (`cycles` V-cycles
+    each, the first an FMG) for several forests at once, each on its own
+    stream, until every tree of each meets `tol`; returns the corrections of
+    each."""
+    from afivo_amd import mg as M
+    for dt, _ in items:
+        M.mixed_begin(dt, mg)
+    corr = [0] * len(items)
+    running = list(range(len(items)))
+    while running:
+        for i in list(running):
+            dt, rn = items[i]
+            if np.all(dt.forest_max_abs(3) <= tol * rn):
+                running.remove(i)
+            elif corr[i] >= max_corrections:
+                raise RuntimeError("no convergence")
+            else:
+                M.mixed_correct(dt, mg, cycles)
+                corr[i] += 1
+    return corr
+
+
 def bench_c3(a):
     """Batched 2D FP64: --trees independent trees solved to --tol relative
     residual, as forests (one per n_cell) through afg_set_forest, the forests
@@ -179,8 +202,29 @@ def bench_c3(a):
         for fdt, _, _ in forests:
             fdt.sync()
         t_batch = time.perf_counter() - tb0
+    phi64 = [fdt.download(1) for fdt, _, _ in forests]
+    # the same forests by the mixed-precision batched solver: FP32 FAS
+    # corrections of 2 or 3 V-cycles in FP64 defect correction (the faster is
+    # reported), each solve after a warm-up solve
+    mixed = {}
+    for cyc in (2, 3):
+        for rep in range(2):
+            for fdt, fst, _ in forests:
+                for iv, v in enumerate(fresh(fst), start=1):
+                    fdt.upload(iv, v)
+                fdt.sync()
+            tm0 = time.perf_counter()
+            corr = mixed_to_tol_together([(fdt, rn) for fdt, _, rn in forests], mg, a.tol, cyc)
+            for fdt, _, _ in forests:
+                fdt.sync()
+            t_mixed = time.perf_counter() - tm0
+        dphi = max(float(np.abs(fdt.download(1) - p64).max() / np.abs(p64).max())
+                   for (fdt, _, _), p64 in zip(forests, phi64))
+        mixed[cyc] = {"ms": 1e3 * t_mixed, "solves_per_s": a.trees / t_mixed, "corrections": corr,
+                      "max_phi_difference_vs_fp64": dphi}
     for fdt, _, _ in forests:
         fdt.close()
+    best = min(mixed, key=lambda k: mixed[k]["ms"])
 
     # one tree at a time, each tree in its own context, warm (second solve)
     t_loop, cycles_loop = 0.0, []
@@ -220,6 +264,10 @@ def bench_c3(a):
         },
         "one_at_a_time_solves_per_s": a.trees / t_loop,
         "speedup_vs_one_at_a_time": t_loop / t_batch,
+        "mixed_batched": dict(mixed[best], vcycles_per_correction=best,
+                              vs_fp64_batched=t_batch / (mixed[best]["ms"] / 1e3),
+                              vs_fp64_one_at_a_time=t_loop / (mixed[best]["ms"] / 1e3),
+                              by_vcycles_per_correction={str(k): v for k, v in mixed.items()}),
         "vcycles_batch": cycles_batch,
         "vcycles_one_at_a_time": [int(min(cycles_loop)), int(max(cycles_loop))],
         "host_tree_build_s": build_s,
