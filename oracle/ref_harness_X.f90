@@ -78,6 +78,7 @@ program ref_harness_$Dd
   ! lsf_mode 2: the lpllsf operator set for every box
   integer            :: lsf_mode = 0
   logical            :: rstr_auto = .false.
+  logical            :: tag_boxes = .true.   ! with rstr_auto: mg$D_set_box_tag on every box first
   real(dp)           :: lsf_c($D) = 0.5_dp
   real(dp)           :: lsf_rad = 0.25_dp
 
@@ -85,7 +86,7 @@ program ref_harness_$Dd
        r0, list_file, n_down, n_up, n_base, n_fmg, n_vcycle, use_eps, eps_hi, &
        eps_cut, out_file, dump_input, dump_fmg, dump_output, n_time, refine2, &
        refine2_lmax, refine2_thr, refine2_rm_thr, refine2_nvar, xfer, cyl, &
-       lsf_mode, rstr_auto, lsf_c, lsf_rad
+       lsf_mode, rstr_auto, tag_boxes, lsf_c, lsf_rad
 
   type(a$D_t)         :: tree, tree_init
   type(ref_info_t)    :: ref_info
@@ -157,7 +158,7 @@ program ref_harness_$Dd
   end if
   if (rstr_auto) mg%box_rstr => mg$D_auto_rstr
   call mg$D_init_mg(mg)
-  if (rstr_auto) call a$D_loop_box(tree, set_tag)
+  if (rstr_auto .and. tag_boxes) call a$D_loop_box(tree, set_tag)
 
   if (dump_input) call write_dump(trim(out_file) // '.in')
 

@@ -150,3 +150,26 @@ def test_dropin_device_update_matches_host_refinement():
     assert int(vals["kept_bad"]) == 0
     assert int(vals["new_bad"]) == 0
     assert float(vals["rel_res"]) < 1e-3
+
+
+@pytest.mark.gpu
+@needs_build
+def test_dropin_auto_rstr_without_tags_stops_like_reference(tmp_path):
+    """mg2_auto_rstr with no tag set: the reference stops in the FMG's first
+    restriction (m_aX_mg.f90:639), and so does the drop-in, with the same
+    message; neither writes a solved tree."""
+    cfg = dict(nc=4, ref_mode="gauss", l_uniform=2, l_max=4, thr=2e-3, n_base=4, n_fmg=1, n_vcycle=1,
+               lsf_mode=1, rstr_auto=True, tag_boxes=False, lsf_c=[0.45, 0.55], lsf_rad=0.23, **G2)
+    msg = "mg2_auto_rstr: box_c tag not set"
+    nml = tmp_path / "ref.nml"
+    nml.write_text("&cfg\n" + ",\n".join(f" {k}={O._fmt(v)}" for k, v in
+                                         dict(cfg, out_file=str(tmp_path / "ref")).items()) + "\n/\n")
+    p = subprocess.run([os.path.join(O.REF_DIR, "ref_harness_2d"), str(nml)], capture_output=True,
+                       text=True, timeout=600, env=dict(os.environ, OMP_NUM_THREADS="1"))
+    assert msg in p.stdout + p.stderr
+    assert not (tmp_path / "ref.out").exists()
+    gpu_dir = tmp_path / "gpu"
+    gpu_dir.mkdir()
+    rc, out, prefix = run_dropin(2, cfg, str(gpu_dir))
+    assert msg in out
+    assert not os.path.exists(prefix + ".out")
