@@ -3,6 +3,7 @@ declares (no device compute: these run without a GPU)."""
 import ctypes
 import os
 import re
+import subprocess
 
 import pytest
 
@@ -74,3 +75,32 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(_lib.AfgError):
         _lib.load()
+
+
+def test_mg_opts_layout_matches_header(tmp_path):
+    """The ctypes mirror of afg_mg_opts has the header's field offsets and
+    size (the header compiled by gcc prints them)."""
+    fields = [f[0] for f in _lib.MgOpts._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "afivo_amd.h"\nint main(void) {\n' +
+                   "".join(f'    printf("%zu\\n", offsetof(afg_mg_opts, {n}));\n' for n in fields) +
+                   '    printf("%zu\\n", sizeof(afg_mg_opts));\n    return 0;\n}\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [getattr(_lib.MgOpts, n).offset for n in fields] + [ctypes.sizeof(_lib.MgOpts)]
+    assert got == want
+
+
+def test_mg_init_level_set_indices():
+    """i_lsf and i_bval go together (m_aX_mg.f90:136-137); 0 reads as unset
+    (the zero-filled spare words of callers that predate the fields)."""
+    L = _lib.load()
+    both = _lib.MgOpts(i_phi=1, i_rhs=2, i_tmp=3, op=_lib.AFG_OP_AUTO, i_lsf=5, i_bval=6)
+    assert L.afg_mg_init(ctypes.byref(both)) == 0
+    one = _lib.MgOpts(i_phi=1, i_rhs=2, i_tmp=3, op=_lib.AFG_OP_AUTO, i_lsf=5, i_bval=-1)
+    assert L.afg_mg_init(ctypes.byref(one)) != 0
+    assert b"both i_lsf and i_bval" in L.afg_last_error(None)
+    zero = _lib.MgOpts(i_phi=1, i_rhs=2, i_tmp=3, i_lsf=0, i_bval=0)
+    assert L.afg_mg_init(ctypes.byref(zero)) == 0
+    assert (zero.i_lsf, zero.i_bval, zero.rstr) == (-1, -1, _lib.AFG_RSTR_LPL)
