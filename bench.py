@@ -45,8 +45,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all cores up to 16")
     ap.add_argument("--no-timing", action="store_true",
                     help="skip the per-kernel event timing (roofline is then null)")
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "update", "transfers"], default="c2",
-                    help="c2: the 3D FP64 tree (default); c3: batched 2D FP64 forest of --trees "
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5", "update", "transfers"], default="c2",
+                    help="c1: the 2D FP64 tree (poisson_basic_2d form); c2: the 3D FP64 tree (default); c3: batched 2D FP64 forest of --trees "
                          "trees; c4: 3D variable-eps tree, mixed precision against FP64; "
                          "c5: served request trace; update: refinement steps of the c2 tree "
                          "on the device; transfers: ghost cells, restriction, prolongation, "
@@ -90,12 +90,13 @@ def cpu_baseline(p, cell_updates_per_solve, threads, n_time=5):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from afivo_amd import problems as P
-    if not O.ref_available(3):
+    dim = int(p.state.dim)
+    if not O.ref_available(dim):
         return None
     import tempfile
     work = tempfile.mkdtemp(prefix="afg_cpu_")
     cfg = P.harness_cfg(p, n_time=n_time, dump_input=False, dump_output=False)
-    prefix, _ = O.run_ref(3, cfg, workdir=work, threads=threads, timeout=900)
+    prefix, _ = O.run_ref(dim, cfg, workdir=work, threads=threads, timeout=900)
     n_thr, t = O.read_time(prefix)
     med = float(np.median(t))
     return {"value": cell_updates_per_solve / med, "unit": "cell-updates/s", "cores": int(n_thr),
@@ -611,8 +612,9 @@ def main():
     from afivo_amd import problems as P
 
     t0 = time.time()
-    p = P.c2(seed=a.seed)
+    p = P.c1() if a.workload == "c1" else P.c2(seed=a.seed)
     st = p.state
+    dim = int(st.dim)
     build_s = time.time() - t0
     dt = M.DeviceTree.from_state(st, 3)
     mg = M.default_mg(n_base=p.n_base)
@@ -667,6 +669,15 @@ def main():
     dt.set_timing(False)
 
     value = cu_solve * a.steps / elapsed
+    if a.workload == "c1":
+        workload = ("2D FP64 Poisson (poisson_basic_2d form), 8^2 boxes, uniform to level 5, refined to level 7 "
+                    "where dr^2 max|rhs| > %g, rhs = Laplacian of two Gaussians (sigma 0.04); step = FMG + %d "
+                    "V-cycles, 2+2 GSRB sweeps, %d base sweeps" % (p.thr, p.n_vcycle, p.n_base))
+    else:
+        workload = ("3D FP64 Poisson (poisson_basic_3d form), 8^3 boxes, uniform to level 4, "
+                    "adaptive to level 7 around 4 Gaussians (sigma 0.03, seed %d), "
+                    "threshold dr^2 max|rhs| > %g; step = FMG + %d V-cycles, 2+2 GSRB sweeps, "
+                    "%d base sweeps" % (a.seed, p.thr, p.n_vcycle, p.n_base))
     out = {
         "metric": "cell-updates/s",
         "value": value,
@@ -681,13 +692,10 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "3D FP64 Poisson (poisson_basic_3d form), 8^3 boxes, uniform to level 4, "
-                        "adaptive to level 7 around 4 Gaussians (sigma 0.03, seed %d), "
-                        "threshold dr^2 max|rhs| > %g; step = FMG + %d V-cycles, 2+2 GSRB sweeps, "
-                        "%d base sweeps" % (a.seed, p.thr, p.n_vcycle, p.n_base),
+            "workload": workload,
             "boxes": int(st.n_boxes),
             "leaf_cells": int(p.tree.n_leaf_cells()),
-            "cells_all_levels": int((st.ncell().astype(np.int64) ** 3).sum()),
+            "cells_all_levels": int((st.ncell().astype(np.int64) ** dim).sum()),
             "levels": [int(st.min_lvl), int(st.highest_lvl)],
             "cell_updates_per_step": cu_solve,
             "tile_shapes": tiles,
@@ -702,9 +710,11 @@ def main():
         n_top = int((st.lvl == st.highest_lvl).sum())
         out["roofline"] = {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-            "kernel": "k_gsrb_tile<3,8> (GSRB colour pass) on the finest level (%d, %d boxes)"
-                      % (st.highest_lvl, n_top),
+            "frac": achieved / HBM_PEAK_GBS,
+            # counter traffic is profiled for the 3D kernel only
+            "traffic": measured_traffic() if dim == 3 else None,
+            "kernel": "k_gsrb_tile<%d,8> or k_gsrb_pair<%d,8> (GSRB colour passes) on the finest level (%d, %d boxes)"
+                      % (dim, dim, st.highest_lvl, n_top),
             "launches": n_launch, "avg_launch_us": 1e3 * k_ms / n_launch,
             "alg_bytes_per_launch": k_bytes / n_launch,
             "share_of_step_time": (k_ms * 1e-3) / elapsed,
