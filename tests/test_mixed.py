@@ -85,3 +85,46 @@ def test_mixed_solve_matches_fp64_solution(kind):
     scale = np.abs(phi64[mask]).max()
     assert np.abs(phi_mixed[mask] - phi64[mask]).max() <= 1e-9 * scale
     assert np.abs(phi_mixed[mask] - ot.get(1)[mask]).max() <= 1e-9 * scale
+
+
+def test_c4_configuration_against_the_oracle():
+    """The 3D variable-eps configuration itself (nc 16, 2804 boxes, 1.00e7
+    leaf cells, levels -2..7): the FP64 GPU solve (FMG + 11 V-cycles, the
+    count that reaches 1e-10) equals the oracle's bit for bit, every variable
+    and every residual norm; the mixed-precision solve to 1e-10 (3 FP32
+    V-cycles per correction) is within 1e-9 (relative, max norm) of the
+    oracle's phi.  The oracle's solve takes about 20 s on one host core."""
+    p = P.c4(seed=1)
+    st = p.state
+    nv = len(st.data)
+    mg = M.default_mg(eps=True, n_base=p.n_base)
+    om = O.default_opts(2, 2, p.n_base, eps=True)
+    dt = M.DeviceTree.from_state(st, nv)
+    ot = O.OracleTree(st, n_var=nv)
+    rd, ro = [], []
+    M.mg_fas_fmg(dt, mg, True, False)
+    ot.fas_fmg(om, True, False)
+    rd.append(M.max_abs_cc(dt, 3))
+    ro.append(ot.max_abs(3))
+    for _ in range(11):
+        M.mg_fas_vcycle(dt, mg, dt.highest_lvl, True)
+        ot.fas_vcycle(om)
+        rd.append(M.max_abs_cc(dt, 3))
+        ro.append(ot.max_abs(3))
+    assert rd == ro
+    for iv in range(1, nv + 1):
+        a, b = dt.download(iv), ot.get(iv)
+        assert np.array_equal(a.view(np.int64), b.view(np.int64)), f"variable {iv} differs from the oracle"
+    mask = st.leaf_mask()
+    rhs_norm = float(np.abs(st.data[1][mask]).max())
+    assert rd[-1] <= 1e-10 * rhs_norm
+    phi_oracle = ot.get(1)
+    for iv, a in enumerate(st.data, start=1):   # start again from the problem's data
+        dt.upload(iv, a)
+    it, rel = M.mixed_solve(dt, mg, 1e-10, max_iter=10, cycles=3)
+    assert rel <= 1e-10
+    phi_mixed = dt.download(1)
+    err = float(np.abs(phi_mixed[mask] - phi_oracle[mask]).max() / np.abs(phi_oracle[mask]).max())
+    print(f"C4: mixed phi against the oracle's FP64 phi: {err:.2e} (relative), {it} corrections")
+    assert err <= 1e-9
+    dt.close()
