@@ -69,6 +69,7 @@ SIGNATURES = {
     "afg_fas_fmg": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int, ctypes.c_int]),
     "afg_fas_vcycle": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int, ctypes.c_int]),
     "afg_levels": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "afg_level_ids": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, I32P, ctypes.POINTER(ctypes.c_int)]),
     "afg_tree_max_cc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, F64P]),
     "afg_tree_min_cc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, F64P]),
     "afg_fill_gc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),

@@ -1584,14 +1584,19 @@ AFG_DEV void gc_body(const DevTree &T, const LevelArgs &L, double *var, int rb, 
     box[gi + s * (gj + s * gk)] = g;
 }
 
-// flux_from_children (m_aX_core.f90:838-943, Cartesian) for one face cell
-// of a leaf neighbour: the mean of the 2^(dim-1) fine fluxes of the parent's
-// child on that face, 0.5 (a + b) in 2D and 0.25 (((a + b) + c) + d) in 3D
-// (first tangential index fastest).  Face arrays of a slot: fx, fy (, fz)
-// at fc + slot * fstride, each (nc+1) nc^(dim-1) values with extent nc+1
-// along its own direction, Fortran order.
+// flux_from_children (m_aX_core.f90:838-943) for one face cell of a leaf
+// neighbour: the mean of the 2^(dim-1) fine fluxes of the parent's child on
+// that face, 0.5 (a + b) in 2D and 0.25 (((a + b) + c) + d) in 3D (first
+// tangential index fastest).  On a cylindrical tree (cyl_rmin: r_min(1) per
+// slot, dr the level's cell size) the y faces take the radius-weighted mean
+// 0.5 (w1 a + w2 b) of m_aX_core.f90:877-887, w = 1 -/+ 0.25 dr / r at the
+// neighbour's cell centre r = r_min(1) + (A - 0.5) dr (a2_cyl_child_weights,
+// m_aX_t.f90:518-528).  Face arrays of a slot: fx, fy (, fz) at
+// fc + slot * fstride, each (nc+1) nc^(dim-1) values with extent nc+1 along
+// its own direction, Fortran order.
 template <int DIM>
-AFG_DEV void flux_body(const FluxRec *recs, int nc, int fstride, double *fc, int gid)
+AFG_DEV void flux_body(const FluxRec *recs, int nc, int fstride, double *fc, int gid,
+                       const double *cyl_rmin = nullptr, double dr = 0)
 {
     const int per = DIM == 2 ? nc : nc * nc;
     const int r = gid / per, t = gid - r * per;
@@ -1615,7 +1620,14 @@ AFG_DEV void flux_body(const FluxRec *recs, int nc, int fstride, double *fc, int
     };
     double v;
     if (DIM == 2) {
-        v = 0.5 * (at(cs, i, 2 * m - 1, 1) + at(cs, i, 2 * m, 1));
+        if (cyl_rmin && d == 1) {
+            const double r = cyl_rmin[R.nb] + (A - 0.5) * dr;
+            const double tw = 0.25 * dr / r;
+            const double w1 = 1 - tw, w2 = 1 + tw;
+            v = 0.5 * (w1 * at(cs, i, 2 * m - 1, 1) + w2 * at(cs, i, 2 * m, 1));
+        } else {
+            v = 0.5 * (at(cs, i, 2 * m - 1, 1) + at(cs, i, 2 * m, 1));
+        }
     } else {
         v = 0.25 * (at(cs, i, 2 * m - 1, 2 * m2 - 1) + at(cs, i, 2 * m, 2 * m2 - 1) +
                     at(cs, i, 2 * m - 1, 2 * m2) + at(cs, i, 2 * m, 2 * m2));

@@ -196,6 +196,9 @@ struct afg_ctx {
     std::vector<TimedBlock> cap_blocks;
     int64_t t_launches = 0;
     double t_ms = 0, t_bytes = 0;
+    // every device allocation of the context and its size (afg_device_bytes)
+    std::unordered_map<void *, size_t> dev_alloc;
+    int64_t dev_bytes = 0;
 
     int fail(int code, const char *fmt, ...)
     {
@@ -208,6 +211,40 @@ struct afg_ctx {
         return code;
     }
 };
+
+// Every device allocation of a context goes through these two, so that
+// afg_device_bytes reports the context's whole footprint.
+template <typename T>
+hipError_t ctx_malloc(afg_ctx *c, T **p, size_t bytes)
+{
+    const hipError_t e = hipMalloc((void **)p, bytes);
+    if (e == hipSuccess) {
+        c->dev_alloc[(void *)*p] = bytes;
+        c->dev_bytes += (int64_t)bytes;
+    }
+    return e;
+}
+
+void ctx_free(afg_ctx *c, void *p)
+{
+    if (!p) return;
+    auto it = c->dev_alloc.find(p);
+    if (it != c->dev_alloc.end()) {
+        c->dev_bytes -= (int64_t)it->second;
+        c->dev_alloc.erase(it);
+    }
+    hipFree(p);
+}
+
+// A kernel launch that failed (bad configuration, no code object for the
+// device) leaves its error for hipGetLastError; the public entry points that
+// launch kernels end with this check.
+int launched(afg_ctx *c, const char *what)
+{
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return c->fail(AFG_ERR_HIP, "%s: kernel launch failed: %s", what, hipGetErrorString(e));
+    return 0;
+}
 
 #define HIP_TRY(ctx, expr)                                                            \
     do {                                                                              \
@@ -283,50 +320,50 @@ void free_tree(afg_ctx *c)
 {
     for (auto &g : c->graphs) destroy_graph(g.second);
     c->graphs.clear();
-    hipFree(c->d_nb);
-    hipFree(c->d_parent);
-    hipFree(c->d_choff);
-    hipFree(c->d_lists);
-    hipFree(c->d_bc_type);
-    hipFree(c->d_bc_voff);
-    hipFree(c->d_bc_vals);
-    hipFree(c->d_tag);
-    hipFree(c->d_is_parent);
+    ctx_free(c, c->d_nb);
+    ctx_free(c, c->d_parent);
+    ctx_free(c, c->d_choff);
+    ctx_free(c, c->d_lists);
+    ctx_free(c, c->d_bc_type);
+    ctx_free(c, c->d_bc_voff);
+    ctx_free(c, c->d_bc_vals);
+    ctx_free(c, c->d_tag);
+    ctx_free(c, c->d_is_parent);
     c->d_is_parent = nullptr;
     for (int k = 0; k < 4; k++) {
-        hipFree(c->d_bottom[k]);
-        hipFree(c->d_base[k]);
+        ctx_free(c, c->d_bottom[k]);
+        ctx_free(c, c->d_base[k]);
         c->d_bottom[k] = nullptr;
         c->d_base[k] = nullptr;
     }
     c->base_cap = 0;
-    hipFree(c->d_tree_leaves);
-    hipFree(c->d_chunk);
-    hipFree(c->d_tree_chunk);
-    hipFree(c->d_chunk_max);
-    hipFree(c->d_tree_out);
+    ctx_free(c, c->d_tree_leaves);
+    ctx_free(c, c->d_chunk);
+    ctx_free(c, c->d_tree_chunk);
+    ctx_free(c, c->d_chunk_max);
+    ctx_free(c, c->d_tree_out);
     if (c->h_tree_out) hipHostFree(c->h_tree_out);
     c->d_tree_leaves = c->d_chunk = c->d_tree_chunk = nullptr;
     c->d_chunk_max = c->d_tree_out = c->h_tree_out = nullptr;
     c->n_chunks = 0;
-    hipFree(c->d_partial);
-    for (double *p : c->d_var) hipFree(p);
-    for (float *p : c->d_var32) hipFree(p);
-    for (double *p : c->d_fc) hipFree(p);
+    ctx_free(c, c->d_partial);
+    for (double *p : c->d_var) ctx_free(c, p);
+    for (float *p : c->d_var32) ctx_free(c, p);
+    for (double *p : c->d_fc) ctx_free(c, p);
     std::fill(c->d_fc.begin(), c->d_fc.end(), nullptr);
-    hipFree(c->d_flux);
+    ctx_free(c, c->d_flux);
     c->d_flux = nullptr;
-    hipFree(c->d_cyl_ioff);
-    hipFree(c->d_cyl_rmin);
+    ctx_free(c, c->d_cyl_ioff);
+    ctx_free(c, c->d_cyl_rmin);
     c->d_cyl_ioff = nullptr;
     c->d_cyl_rmin = nullptr;
     c->flux_ready = false;
-    hipFree(c->d_phi2);
-    hipFree(c->d_phi2_32);
+    ctx_free(c, c->d_phi2);
+    ctx_free(c, c->d_phi2_32);
     c->d_phi2 = nullptr;
     c->d_phi2_32 = nullptr;
-    hipFree(c->d_rhs_rb);
-    hipFree(c->d_rhs_rb32);
+    ctx_free(c, c->d_rhs_rb);
+    ctx_free(c, c->d_rhs_rb32);
     c->d_rhs_rb = nullptr;
     c->d_rhs_rb32 = nullptr;
     c->rb_valid[0] = c->rb_valid[1] = false;
@@ -346,6 +383,7 @@ int check_ctx(afg_ctx *c)
     if (!c) return AFG_ERR_ARG;
     if (!c->have_tree) return c->fail(AFG_ERR_STATE, "no tree set (call afg_set_tree first)");
     HIP_TRY(c, hipSetDevice(c->device));
+    (void)hipGetLastError();   // a launch error reported later is one of this call's
     return 0;
 }
 
@@ -365,7 +403,7 @@ int ensure_pool(afg_ctx *c, int iv, std::vector<R *> &pools, R **out)
     if (!p) {
         if (c->capturing) return c->fail(AFG_ERR_STATE, "variable %d first used inside a graph capture", iv);
         const size_t bytes = (size_t)c->n_slots * c->stride * sizeof(R);
-        hipError_t e = hipMalloc(&p, bytes);
+        hipError_t e = ctx_malloc(c, &p, bytes);
         if (e != hipSuccess) {
             p = nullptr;
             return c->fail(AFG_ERR_MEMORY, "hipMalloc(%zu) for variable %d: %s", bytes, iv,
@@ -389,7 +427,7 @@ int ensure_fc(afg_ctx *c, int ifc, double **out)
     if (!p) {
         if (c->capturing) return c->fail(AFG_ERR_STATE, "face variable %d first used inside a graph capture", ifc);
         const size_t bytes = (size_t)c->n_slots * c->fstride * sizeof(double);
-        if (hipMalloc(&p, bytes) != hipSuccess) {
+        if (ctx_malloc(c, &p, bytes) != hipSuccess) {
             p = nullptr;
             return c->fail(AFG_ERR_MEMORY, "hipMalloc(%zu) for face variable %d", bytes, ifc);
         }
@@ -411,20 +449,20 @@ int upload_bc(afg_ctx *c)
         vo[r] = c->bc[r].voff;
     }
     if (nrec > c->bc_cap) {
-        hipFree(c->d_bc_type);
-        hipFree(c->d_bc_voff);
+        ctx_free(c, c->d_bc_type);
+        ctx_free(c, c->d_bc_voff);
         c->d_bc_type = c->d_bc_voff = nullptr;
-        HIP_TRY(c, hipMalloc(&c->d_bc_type, nrec * sizeof(int32_t)));
-        HIP_TRY(c, hipMalloc(&c->d_bc_voff, nrec * sizeof(int32_t)));
+        HIP_TRY(c, ctx_malloc(c, &c->d_bc_type, nrec * sizeof(int32_t)));
+        HIP_TRY(c, ctx_malloc(c, &c->d_bc_voff, nrec * sizeof(int32_t)));
         c->bc_cap = nrec;
         for (auto &g : c->graphs) destroy_graph(g.second);
         c->graphs.clear();
     }
     const size_t nv = std::max<size_t>(c->bc_vals.size(), 1);
     if (nv > c->bc_vals_cap) {
-        hipFree(c->d_bc_vals);
+        ctx_free(c, c->d_bc_vals);
         c->d_bc_vals = nullptr;
-        HIP_TRY(c, hipMalloc(&c->d_bc_vals, nv * sizeof(double)));
+        HIP_TRY(c, ctx_malloc(c, &c->d_bc_vals, nv * sizeof(double)));
         c->bc_vals_cap = nv;
         for (auto &g : c->graphs) destroy_graph(g.second);
         c->graphs.clear();
@@ -636,7 +674,7 @@ int make_mg(afg_ctx *c, const afg_mg_opts *o, MgT<R> *m)
         if (!p2) {
             if (c->capturing) return c->fail(AFG_ERR_STATE, "second phi pool first used inside a graph capture");
             const size_t bytes = (size_t)c->n_slots * c->stride * sizeof(R);
-            if (hipMalloc(&p2, bytes) != hipSuccess) {
+            if (ctx_malloc(c, &p2, bytes) != hipSuccess) {
                 p2 = nullptr;
                 return c->fail(AFG_ERR_MEMORY, "hipMalloc(%zu) for the second phi pool", bytes);
             }
@@ -658,7 +696,7 @@ int make_mg(afg_ctx *c, const afg_mg_opts *o, MgT<R> *m)
         R *&rb = rhs_rb_pool(c, (R *)nullptr);
         if (!rb) {
             const size_t bytes = (size_t)c->n_slots * rb_stride(c) * sizeof(R);
-            if (hipMalloc(&rb, bytes) != hipSuccess) {
+            if (ctx_malloc(c, &rb, bytes) != hipSuccess) {
                 rb = nullptr;
                 return c->fail(AFG_ERR_MEMORY, "hipMalloc(%zu) for the colour-split rhs", bytes);
             }
@@ -1040,9 +1078,9 @@ int build_base(afg_ctx *c)
     const int ncells = L0.n * (dim == 2 ? nc * nc : nc * nc * nc);
     if (ncells > c->base_cap) {
         for (int k = 0; k < 4; k++) {
-            hipFree(c->d_base[k]);
+            ctx_free(c, c->d_base[k]);
             c->d_base[k] = nullptr;
-            HIP_TRY(c, hipMalloc(&c->d_base[k], (size_t)ncells * sizeof(BaseCell)));
+            HIP_TRY(c, ctx_malloc(c, &c->d_base[k], (size_t)ncells * sizeof(BaseCell)));
         }
         c->base_cap = ncells;
     }
@@ -1135,7 +1173,7 @@ static int tune_tiles(afg_ctx *c, const afg_mg_opts *o, int reps, double *ms, co
     if (int e = make_mg(c, o, &m)) return e;
     const size_t n = (size_t)c->n_slots * c->stride;
     R *scratch = nullptr;
-    HIP_TRY(c, hipMalloc(&scratch, n * sizeof(R)));
+    HIP_TRY(c, ctx_malloc(c, &scratch, n * sizeof(R)));
     MgVarsT<R> V = m.V;
     V.phi = scratch;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1200,7 +1238,7 @@ static int tune_tiles(afg_ctx *c, const afg_mg_opts *o, int reps, double *ms, co
     hipStreamSynchronize(c->stream);
     if (e0) hipEventDestroy(e0);
     if (e1) hipEventDestroy(e1);
-    hipFree(scratch);
+    ctx_free(c, scratch);
     for (auto &g : c->graphs) destroy_graph(g.second);
     c->graphs.clear();
     return err;
@@ -1216,7 +1254,7 @@ static int tune_bottom(afg_ctx *c, const afg_mg_opts *o, int reps, double *ms)
     if (int e = make_mg(c, o, &m)) return e;
     const size_t n = (size_t)c->n_slots * c->stride;
     R *saved = nullptr;
-    HIP_TRY(c, hipMalloc(&saved, 3 * n * sizeof(R)));
+    HIP_TRY(c, ctx_malloc(c, &saved, 3 * n * sizeof(R)));
     R *vars[3] = {m.V.phi, m.V.rhs, m.V.tmp};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     const int p = sizeof(R) == 4;
@@ -1257,7 +1295,7 @@ static int tune_bottom(afg_ctx *c, const afg_mg_opts *o, int reps, double *ms)
     hipStreamSynchronize(c->stream);
     if (e0) hipEventDestroy(e0);
     if (e1) hipEventDestroy(e1);
-    hipFree(saved);
+    ctx_free(c, saved);
     return err ? err : e2;
 }
 
@@ -1332,7 +1370,7 @@ int afg_ctx_create(int dim, int n_cell, int n_var, afg_ctx **out)
     c->d_var.assign(n_var, nullptr);
     c->d_var32.assign(n_var, nullptr);
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc(&c->d_out, 8 * sizeof(double));
+    if (e == hipSuccess) e = ctx_malloc(c, &c->d_out, 8 * sizeof(double));
     if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_out, 8 * sizeof(double), hipHostMallocDefault);
     if (e != hipSuccess) {
         g_create_error = std::string("HIP set-up: ") + hipGetErrorString(e);
@@ -1357,7 +1395,7 @@ int afg_ctx_destroy(afg_ctx *c)
         hipEventDestroy(ev.first);
         hipEventDestroy(ev.second);
     }
-    hipFree(c->d_out);
+    ctx_free(c, c->d_out);
     if (c->h_out) hipHostFree(c->h_out);
     if (c->h_stage) hipHostFree(c->h_stage);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -1608,11 +1646,11 @@ int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base,
     }
 
     const size_t sn = (size_t)slot;
-    HIP_TRY(c, hipMalloc(&c->d_nb, sn * nnb * sizeof(int32_t)));
-    HIP_TRY(c, hipMalloc(&c->d_parent, sn * sizeof(int32_t)));
-    HIP_TRY(c, hipMalloc(&c->d_choff, sn * sizeof(int32_t)));
-    HIP_TRY(c, hipMalloc(&c->d_lists, std::max<size_t>(lists.size(), 1) * sizeof(int32_t)));
-    HIP_TRY(c, hipMalloc(&c->d_tag, sn * sizeof(int8_t)));
+    HIP_TRY(c, ctx_malloc(c, &c->d_nb, sn * nnb * sizeof(int32_t)));
+    HIP_TRY(c, ctx_malloc(c, &c->d_parent, sn * sizeof(int32_t)));
+    HIP_TRY(c, ctx_malloc(c, &c->d_choff, sn * sizeof(int32_t)));
+    HIP_TRY(c, ctx_malloc(c, &c->d_lists, std::max<size_t>(lists.size(), 1) * sizeof(int32_t)));
+    HIP_TRY(c, ctx_malloc(c, &c->d_tag, sn * sizeof(int8_t)));
     HIP_H2D(c, c->d_nb, nbv.data(), sn * nnb * sizeof(int32_t));
     HIP_H2D(c, c->d_parent, par.data(), sn * sizeof(int32_t));
     c->slot_par = par;
@@ -1635,7 +1673,7 @@ int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base,
             const int32_t *q = recs + (size_t)rec_of[c->slot2id[s2]] * rl;
             isp[s2] = q[3 + dim] != 0;   // a$D_has_children
         }
-        HIP_TRY(c, hipMalloc(&c->d_is_parent, sn));
+        HIP_TRY(c, ctx_malloc(c, &c->d_is_parent, sn));
         HIP_H2D(c, c->d_is_parent, isp.data(), sn);
     }
     {   // leaf slots grouped by tree, for the per-tree norms
@@ -1661,11 +1699,11 @@ int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base,
         toff[n_trees] = (int)chunk.size();
         c->n_chunks = (int)chunk.size();
         chunk.push_back((int)tl.size());
-        HIP_TRY(c, hipMalloc(&c->d_tree_leaves, std::max<size_t>(tl.size(), 1) * sizeof(int32_t)));
-        HIP_TRY(c, hipMalloc(&c->d_chunk, chunk.size() * sizeof(int32_t)));
-        HIP_TRY(c, hipMalloc(&c->d_tree_chunk, toff.size() * sizeof(int32_t)));
-        HIP_TRY(c, hipMalloc(&c->d_chunk_max, std::max(c->n_chunks, 1) * sizeof(double)));
-        HIP_TRY(c, hipMalloc(&c->d_tree_out, n_trees * sizeof(double)));
+        HIP_TRY(c, ctx_malloc(c, &c->d_tree_leaves, std::max<size_t>(tl.size(), 1) * sizeof(int32_t)));
+        HIP_TRY(c, ctx_malloc(c, &c->d_chunk, chunk.size() * sizeof(int32_t)));
+        HIP_TRY(c, ctx_malloc(c, &c->d_tree_chunk, toff.size() * sizeof(int32_t)));
+        HIP_TRY(c, ctx_malloc(c, &c->d_chunk_max, std::max(c->n_chunks, 1) * sizeof(double)));
+        HIP_TRY(c, ctx_malloc(c, &c->d_tree_out, n_trees * sizeof(double)));
         HIP_TRY(c, hipHostMalloc((void **)&c->h_tree_out, n_trees * sizeof(double), hipHostMallocDefault));
         if (!tl.empty())
             HIP_H2D(c, c->d_tree_leaves, tl.data(), tl.size() * sizeof(int32_t));
@@ -1673,9 +1711,9 @@ int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base,
         HIP_H2D(c, c->d_tree_chunk, toff.data(), toff.size() * sizeof(int32_t));
     }
     c->have_tree = true;
-    for (int k = 0; k < 4; k++) HIP_TRY(c, hipMalloc(&c->d_bottom[k], sizeof(BottomLevels)));
+    for (int k = 0; k < 4; k++) HIP_TRY(c, ctx_malloc(c, &c->d_bottom[k], sizeof(BottomLevels)));
     if (int e = choose_bottom(c)) return e;
-    HIP_TRY(c, hipMalloc(&c->d_partial, c->partial_cap * sizeof(double)));
+    HIP_TRY(c, ctx_malloc(c, &c->d_partial, c->partial_cap * sizeof(double)));
     c->bc_dirty = true;
     if (int e = upload_bc(c)) return e;
     if (int e = build_cyl(c)) return e;
@@ -1727,9 +1765,9 @@ int afg_update_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base
     std::fill(c->d_fc.begin(), c->d_fc.end(), nullptr);
     int32_t *d_src = nullptr;
     auto release = [&](int code) {
-        for (double *p : old_pool) hipFree(p);
-        for (double *p : old_fc) hipFree(p);
-        hipFree(d_src);
+        for (double *p : old_pool) ctx_free(c, p);
+        for (double *p : old_fc) ctx_free(c, p);
+        ctx_free(c, d_src);
         return code;
     };
     if (int e = afg_set_forest(c, n_boxes, recs, dr_base, tree_of)) return release(e);
@@ -1749,7 +1787,7 @@ int afg_update_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base
         }
     }
     lv_off[nl] = (int)added.size();
-    if (hipMalloc(&d_src, ((size_t)ns + added.size()) * sizeof(int32_t)) != hipSuccess) {
+    if (ctx_malloc(c, &d_src, ((size_t)ns + added.size()) * sizeof(int32_t)) != hipSuccess) {
         d_src = nullptr;
         return release(c->fail(AFG_ERR_MEMORY, "tree update: slot map of %d slots", ns));
     }
@@ -1765,7 +1803,7 @@ int afg_update_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base
     for (int iv = 1; iv <= c->n_var; iv++) {
         if (!old_pool[iv - 1]) continue;
         double *p = nullptr;
-        if (hipMalloc(&p, pool_bytes) != hipSuccess)
+        if (ctx_malloc(c, &p, pool_bytes) != hipSuccess)
             return release(c->fail(AFG_ERR_MEMORY, "hipMalloc(%zu) for variable %d", pool_bytes, iv));
         launch_gather(c->stream, c->stride, ns, d_src, old_pool[iv - 1], p);
         c->d_var[iv - 1] = p;
@@ -1774,7 +1812,7 @@ int afg_update_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base
         if (!old_fc[k]) continue;
         double *p = nullptr;
         const size_t fbytes = (size_t)ns * c->fstride * sizeof(double);
-        if (hipMalloc(&p, fbytes) != hipSuccess)
+        if (ctx_malloc(c, &p, fbytes) != hipSuccess)
             return release(c->fail(AFG_ERR_MEMORY, "hipMalloc(%zu) for face variable %d", fbytes, k + 1));
         launch_gather(c->stream, c->fstride, ns, d_src, old_fc[k], p);
         c->d_fc[k] = p;
@@ -1795,6 +1833,7 @@ int afg_update_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base
     const hipError_t se = hipStreamSynchronize(c->stream);
     release(0);
     if (se != hipSuccess) return c->fail(AFG_ERR_HIP, "tree update: %s", hipGetErrorString(se));
+    if (int e = launched(c, "afg_update_tree")) return e;
 
     // boundary conditions of the faces that were physical boundaries before
     for (int s = 0; s < ns; s++) {
@@ -2044,7 +2083,7 @@ static int build_flux_records(afg_ctx *c)
     }
     c->flux_off[nl] = (int)recs.size();
     if (!recs.empty()) {
-        HIP_TRY(c, hipMalloc(&c->d_flux, recs.size() * sizeof(FluxRec)));
+        HIP_TRY(c, ctx_malloc(c, &c->d_flux, recs.size() * sizeof(FluxRec)));
         HIP_H2D(c, c->d_flux, recs.data(), recs.size() * sizeof(FluxRec));
     }
     c->flux_ready = true;
@@ -2062,9 +2101,10 @@ int afg_consistent_fluxes(afg_ctx *c, int n_f, const int32_t *f_ixs)
     for (size_t li = 0; li + 1 < c->flux_off.size(); li++) {
         const int n = c->flux_off[li + 1] - c->flux_off[li];
         for (double *p : pools)
-            launch_flux(c->dim, c->stream, c->d_flux + c->flux_off[li], n, c->levels[li].nc, c->fstride, p);
+            launch_flux(c->dim, c->stream, c->d_flux + c->flux_off[li], n, c->levels[li].nc, c->fstride, p,
+                        c->cyl ? c->d_cyl_rmin : nullptr, c->levels[li].dr);
     }
-    return 0;
+    return launched(c, "afg_consistent_fluxes");
 }
 
 int afg_copy_cc(afg_ctx *c, int iv_from, int iv_to)
@@ -2076,7 +2116,7 @@ int afg_copy_cc(afg_ctx *c, int iv_from, int iv_to)
     if (int e = ensure_var(c, iv_to, &b)) return e;
     const DevTree T = dev_tree(c);
     for (const Level &L : c->levels) launch_copy_boxes(c->dim, c->stream, T, L.slot0, L.n, L.nc, a, b);
-    return 0;
+    return launched(c, "afg_copy_cc");
 }
 
 int afg_set_cc(afg_ctx *c, int iv, double value)
@@ -2087,7 +2127,7 @@ int afg_set_cc(afg_ctx *c, int iv, double value)
     if (int e = ensure_var(c, iv, &a)) return e;
     const DevTree T = dev_tree(c);
     for (const Level &L : c->levels) launch_fill_value(c->dim, c->stream, T, L.slot0, L.n, L.nc, a, value);
-    return 0;
+    return launched(c, "afg_set_cc");
 }
 
 int afg_mg_init(afg_mg_opts *o)
@@ -2125,6 +2165,19 @@ int afg_levels(const afg_ctx *cc, int *min_lvl, int *highest_lvl)
     return 0;
 }
 
+int afg_level_ids(const afg_ctx *cc, int lvl, int n, int32_t *ids, int *n_out)
+{
+    afg_ctx *c = const_cast<afg_ctx *>(cc);
+    if (int e = check_ctx(c)) return e;
+    if (int e = check_level(c, lvl, false)) return e;
+    const Level &L = c->levels[lvl - c->min_lvl];
+    if (n_out) *n_out = L.n;
+    if (!ids) return 0;
+    if (n < L.n) return c->fail(AFG_ERR_ARG, "level %d has %d boxes, room for %d", lvl, L.n, n);
+    for (int b = 0; b < L.n; b++) ids[b] = c->slot2id[L.slot0 + b];
+    return 0;
+}
+
 int afg_fas_vcycle(afg_ctx *c, const afg_mg_opts *o, int highest_lvl, int set_residual)
 {
     if (int e = check_ctx(c)) return e;
@@ -2132,7 +2185,8 @@ int afg_fas_vcycle(afg_ctx *c, const afg_mg_opts *o, int highest_lvl, int set_re
     Mg m;
     if (int e = make_mg(c, o, &m)) return e;
     if (int e = set_tags(m)) return e;
-    return run_vcycle(m, highest_lvl, set_residual ? 1 : 0, c->min_lvl, c->highest_lvl + 1);
+    if (int e = run_vcycle(m, highest_lvl, set_residual ? 1 : 0, c->min_lvl, c->highest_lvl + 1)) return e;
+    return launched(c, "afg_fas_vcycle");
 }
 
 int afg_fas_fmg(afg_ctx *c, const afg_mg_opts *o, int set_residual, int have_guess)
@@ -2141,7 +2195,8 @@ int afg_fas_fmg(afg_ctx *c, const afg_mg_opts *o, int set_residual, int have_gue
     Mg m;
     if (int e = make_mg(c, o, &m)) return e;
     if (int e = set_tags(m)) return e;
-    return fas_fmg(m, set_residual, have_guess);
+    if (int e = fas_fmg(m, set_residual, have_guess)) return e;
+    return launched(c, "afg_fas_fmg");
 }
 
 // ---- FP32 stages and the mixed-precision solve ----
@@ -2155,7 +2210,7 @@ int afg_fp32_from_cc(afg_ctx *c, int iv, int iv32)
     if (int e = ensure_var(c, iv, &a)) return e;
     if (int e = ensure_var(c, iv32, &b)) return e;
     launch_to_fp32(c->stream, (long long)c->n_slots * c->stride, a, b);
-    return 0;
+    return launched(c, "afg_fp32_from_cc");
 }
 
 int afg_fp32_to_cc(afg_ctx *c, int iv32, int iv, int add)
@@ -2167,7 +2222,7 @@ int afg_fp32_to_cc(afg_ctx *c, int iv32, int iv, int add)
     if (int e = ensure_var(c, iv, &a)) return e;
     if (int e = ensure_var(c, iv32, &b)) return e;
     launch_from_fp32(c->stream, (long long)c->n_slots * c->stride, b, a, add ? 1 : 0);
-    return 0;
+    return launched(c, "afg_fp32_to_cc");
 }
 
 // FP32 solves need the tags of the auto operators, set from the FP64 eps
@@ -2184,7 +2239,8 @@ int afg_fas_fmg_fp32(afg_ctx *c, const afg_mg_opts *o, int set_residual, int hav
     if (int e = check_ctx(c)) return e;
     MgT<float> m;
     if (int e = fp32_mg(c, o, &m)) return e;
-    return fas_fmg(m, set_residual, have_guess);
+    if (int e = fas_fmg(m, set_residual, have_guess)) return e;
+    return launched(c, "afg_fas_fmg_fp32");
 }
 
 int afg_fas_vcycle_fp32(afg_ctx *c, const afg_mg_opts *o, int highest_lvl, int set_residual)
@@ -2193,7 +2249,8 @@ int afg_fas_vcycle_fp32(afg_ctx *c, const afg_mg_opts *o, int highest_lvl, int s
     if (int e = check_level(c, highest_lvl, false)) return e;
     MgT<float> m;
     if (int e = fp32_mg(c, o, &m)) return e;
-    return run_vcycle(m, highest_lvl, set_residual ? 1 : 0, c->min_lvl, c->highest_lvl + 1);
+    if (int e = run_vcycle(m, highest_lvl, set_residual ? 1 : 0, c->min_lvl, c->highest_lvl + 1)) return e;
+    return launched(c, "afg_fas_vcycle_fp32");
 }
 
 static int tree_extreme(afg_ctx *c, int iv, int include_parents, int want_max, double *out);
@@ -2215,6 +2272,7 @@ static int forest_norms(afg_ctx *c, int iv, double *out)
     if (int e = ensure_var(c, iv, &v)) return e;
     launch_tree_maxabs(c->dim, c->stream, c->stride, c->nc, c->d_tree_leaves, c->d_chunk, c->n_chunks,
                        c->d_tree_chunk, c->n_trees, v, c->d_chunk_max, c->d_tree_out);
+    if (int e = launched(c, "forest norms")) return e;
     HIP_TRY(c, hipMemcpyAsync(c->h_tree_out, c->d_tree_out, c->n_trees * sizeof(double), hipMemcpyDeviceToHost,
                               c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -2255,7 +2313,7 @@ int afg_mixed_begin(afg_ctx *c, const afg_mg_opts *o, int have_guess)
     mixed_residual(m);
     c->mixed_fmg = true;
     c->mixed_ready = true;
-    return 0;
+    return launched(c, "afg_mixed_begin");
 }
 
 int afg_mixed_correct(afg_ctx *c, const afg_mg_opts *o, int cycles)
@@ -2284,7 +2342,7 @@ int afg_mixed_correct(afg_ctx *c, const afg_mg_opts *o, int cycles)
     }
     launch_from_fp32(c->stream, n, f.V.phi, m.V.phi, 1);
     mixed_residual(m);
-    return 0;
+    return launched(c, "afg_mixed_correct");
 }
 
 int afg_mixed_solve(afg_ctx *c, const afg_mg_opts *o, int have_guess, double tol, int max_iter,
@@ -2351,6 +2409,7 @@ static int tree_extreme(afg_ctx *c, int iv, int include_parents, int want_max, d
         return 0;
     }
     launch_reduce_partials(c->stream, c->d_partial, used, want_max, c->d_out);
+    if (int e = launched(c, "tree extreme")) return e;
     HIP_TRY(c, hipMemcpyAsync(c->h_out, c->d_out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     *out = c->h_out[0];
@@ -2375,7 +2434,7 @@ int afg_fill_gc(afg_ctx *c, int iv, int lvl)
     if (int e = ensure_var(c, iv, &v)) return e;
     if (int e = upload_bc(c)) return e;
     launch_fill_gc(c->dim, c->stream, dev_tree(c), level_args(c, lvl), v);
-    return 0;
+    return launched(c, "afg_fill_gc");
 }
 
 int afg_gsrb_level(afg_ctx *c, const afg_mg_opts *o, int lvl, int n_cycle)
@@ -2387,7 +2446,7 @@ int afg_gsrb_level(afg_ctx *c, const afg_mg_opts *o, int lvl, int n_cycle)
     if (int e = make_mg(c, o, &m)) return e;
     if (int e = set_tags(m)) return e;
     gsrb_level(m, lvl, n_cycle);
-    return 0;
+    return launched(c, "afg_gsrb_level");
 }
 
 int afg_residual_level(afg_ctx *c, const afg_mg_opts *o, int lvl)
@@ -2398,7 +2457,7 @@ int afg_residual_level(afg_ctx *c, const afg_mg_opts *o, int lvl)
     if (int e = make_mg(c, o, &m)) return e;
     if (int e = set_tags(m)) return e;
     residual_level(m, lvl);
-    return 0;
+    return launched(c, "afg_residual_level");
 }
 
 int afg_restrict_level(afg_ctx *c, int iv, int lvl)
@@ -2409,7 +2468,7 @@ int afg_restrict_level(afg_ctx *c, int iv, int lvl)
     double *v = nullptr;
     if (int e = ensure_var(c, iv, &v)) return e;
     launch_restrict(c->dim, c->stream, dev_tree(c), level_args(c, lvl), v, v);
-    return 0;
+    return launched(c, "afg_restrict_level");
 }
 
 int afg_prolong2_level(afg_ctx *c, int iv, int iv_to, int lvl, int add)
@@ -2422,7 +2481,7 @@ int afg_prolong2_level(afg_ctx *c, int iv, int iv_to, int lvl, int add)
     if (int e = ensure_var(c, iv_to, &b)) return e;
     if (a == b) return c->fail(AFG_ERR_ARG, "prolong2: source and destination must differ");
     launch_prolong2(c->dim, c->stream, dev_tree(c), level_args(c, lvl), a, b, add ? 1 : 0);
-    return 0;
+    return launched(c, "afg_prolong2_level");
 }
 
 int afg_correct_children(afg_ctx *c, const afg_mg_opts *o, int lvl)
@@ -2436,7 +2495,7 @@ int afg_correct_children(afg_ctx *c, const afg_mg_opts *o, int lvl)
     // then the prolongation of tmp into the children
     launch_diff(c->dim, c->stream, m.T, list_args(c, lvl - 1, 0), m.V);
     launch_correct(c->dim, c->stream, m.T, level_args(c, lvl), m.V, 0);
-    return 0;
+    return launched(c, "afg_correct_children");
 }
 
 int afg_update_coarse(afg_ctx *c, const afg_mg_opts *o, int lvl)
@@ -2447,7 +2506,7 @@ int afg_update_coarse(afg_ctx *c, const afg_mg_opts *o, int lvl)
     if (int e = make_mg(c, o, &m)) return e;
     if (int e = set_tags(m)) return e;
     update_coarse(m, lvl, 0);
-    return 0;
+    return launched(c, "afg_update_coarse");
 }
 
 // The per-slot tables of a cylindrical tree: (ix(1) - 1) * nc_l and
@@ -2457,8 +2516,8 @@ int afg_update_coarse(afg_ctx *c, const afg_mg_opts *o, int lvl)
 // parents come first.
 static int build_cyl(afg_ctx *c)
 {
-    hipFree(c->d_cyl_ioff);
-    hipFree(c->d_cyl_rmin);
+    ctx_free(c, c->d_cyl_ioff);
+    ctx_free(c, c->d_cyl_rmin);
     c->d_cyl_ioff = nullptr;
     c->d_cyl_rmin = nullptr;
     if (!c->cyl || !c->have_tree) return 0;
@@ -2476,8 +2535,8 @@ static int build_cyl(afg_ctx *c)
             rmin[s] = rmin[p] + 0.5 * dr_p * ((ix - 1) & 1) * c->slot_nc[p];
         }
     }
-    HIP_TRY(c, hipMalloc(&c->d_cyl_ioff, ns * sizeof(int32_t)));
-    HIP_TRY(c, hipMalloc(&c->d_cyl_rmin, ns * sizeof(double)));
+    HIP_TRY(c, ctx_malloc(c, &c->d_cyl_ioff, ns * sizeof(int32_t)));
+    HIP_TRY(c, ctx_malloc(c, &c->d_cyl_rmin, ns * sizeof(double)));
     HIP_H2D(c, c->d_cyl_ioff, ioff.data(), ns * sizeof(int32_t));
     HIP_H2D(c, c->d_cyl_rmin, rmin.data(), ns * sizeof(double));
     return 0;
@@ -2502,7 +2561,7 @@ int afg_gc_level(afg_ctx *c, int iv, int lvl, int rb, int bc)
     if (int e = ensure_var(c, iv, &v)) return e;
     if (int e = upload_bc(c)) return e;
     launch_gc(c->dim, c->stream, dev_tree(c), level_args(c, lvl), v, rb, bc);
-    return 0;
+    return launched(c, "afg_gc_level");
 }
 
 int afg_gc_tree(afg_ctx *c, int iv, int rb, int bc)
@@ -2515,7 +2574,7 @@ int afg_gc_tree(afg_ctx *c, int iv, int rb, int bc)
     const DevTree T = dev_tree(c);
     for (int lvl = c->min_lvl; lvl <= c->highest_lvl; lvl++)
         launch_gc(c->dim, c->stream, T, level_args(c, lvl), v, rb, bc);
-    return 0;
+    return launched(c, "afg_gc_tree");
 }
 
 int afg_restrict_tree(afg_ctx *c, int iv, int iv_to)
@@ -2528,7 +2587,7 @@ int afg_restrict_tree(afg_ctx *c, int iv, int iv_to)
     const DevTree T = dev_tree(c);
     for (int lvl = c->highest_lvl; lvl > c->min_lvl; lvl--)
         launch_restrict(c->dim, c->stream, T, level_args(c, lvl), (const double *)a, b);
-    return 0;
+    return launched(c, "afg_restrict_tree");
 }
 
 int afg_prolong_level(afg_ctx *c, int iv, int iv_to, int lvl, int order, int add)
@@ -2541,7 +2600,7 @@ int afg_prolong_level(afg_ctx *c, int iv, int iv_to, int lvl, int order, int add
     if (int e = ensure_var(c, iv, &a)) return e;
     if (int e = ensure_var(c, iv_to, &b)) return e;
     launch_prolong_level(c->dim, c->stream, dev_tree(c), level_args(c, lvl), a, b, order, add ? 1 : 0);
-    return 0;
+    return launched(c, "afg_prolong_level");
 }
 
 static int pool_arith(afg_ctx *c, int op, double a, int iv_x, double b, int iv_y)
@@ -2552,7 +2611,7 @@ static int pool_arith(afg_ctx *c, int op, double a, int iv_x, double b, int iv_y
     if (int e = ensure_var(c, iv_x, &x)) return e;
     if (int e = ensure_var(c, iv_y, &y)) return e;
     launch_arith(c->stream, (long long)c->n_slots * c->stride, op, a, b, x, y);
-    return 0;
+    return launched(c, "tree arithmetic");
 }
 
 int afg_add_cc(afg_ctx *c, int iv_from, int iv_to) { return pool_arith(c, ARITH_ADD, 0, iv_from, 0, iv_to); }
@@ -2608,7 +2667,8 @@ int afg_autotune(afg_ctx *c, const afg_mg_opts *o, int fp32, int reps, double *m
     if (int e = fp32 ? tune_tiles<float>(c, o, reps, ms, m.V.phi) : tune_tiles<double>(c, o, reps, ms, m.V.phi))
         return e;
     double *bms = ms ? ms + 6 * (int)c->levels.size() : nullptr;
-    return fp32 ? tune_bottom<float>(c, o, reps, bms) : tune_bottom<double>(c, o, reps, bms);
+    if (int e = fp32 ? tune_bottom<float>(c, o, reps, bms) : tune_bottom<double>(c, o, reps, bms)) return e;
+    return launched(c, "afg_autotune");
 }
 
 int afg_get_bottom_cells(const afg_ctx *c, int fp32, int64_t *cells)
@@ -2687,15 +2747,7 @@ int afg_device_bytes(const afg_ctx *cc, int64_t *out)
 {
     afg_ctx *c = const_cast<afg_ctx *>(cc);
     if (!c || !out) return AFG_ERR_ARG;
-    int64_t b = 0;
-    for (double *p : c->d_var)
-        if (p) b += (int64_t)c->n_slots * c->stride * 8;
-    for (float *p : c->d_var32)
-        if (p) b += (int64_t)c->n_slots * c->stride * 4;
-    for (double *p : c->d_fc)
-        if (p) b += (int64_t)c->n_slots * c->fstride * 8;
-    b += (int64_t)c->n_slots * (c->nnb + 3) * 4 + c->n_slots;
-    *out = b;
+    *out = c->dev_bytes;
     return 0;
 }
 

@@ -73,8 +73,11 @@ void launch_prolong_level(int dim, hipStream_t s, const DevTree &T, const LevelA
 // (RB_*) and boundary condition bc (0: the table; -1 / -2: Dirichlet /
 // Neumann zero)
 void launch_gc(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, double *var, int rb, int bc);
-// a$D_consistent_fluxes for the n records of one level (face pool fc)
-void launch_flux(int dim, hipStream_t s, const FluxRec *recs, int n, int nc, int fstride, double *fc);
+// a$D_consistent_fluxes for the n records of one level (face pool fc); on
+// a cylindrical tree cyl_rmin holds r_min(1) per slot (else nullptr) and dr
+// is the level's cell size
+void launch_flux(int dim, hipStream_t s, const FluxRec *recs, int n, int nc, int fstride, double *fc,
+                 const double *cyl_rmin, double dr);
 // y = y + x, y - x, a y or a x + b y (ARITH_*) over n values
 void launch_arith(hipStream_t s, long long n, int op, double a, double b, const double *x, double *y);
 // n values of a pool to FP32, and back (optionally added to the FP64 values)

@@ -225,10 +225,11 @@ __global__ __launch_bounds__(BLOCK) void k_gc(DevTree T, LevelArgs L, double *va
 }
 
 template <int DIM>
-__global__ __launch_bounds__(BLOCK) void k_flux(const FluxRec *recs, int n, int nc, int fstride, double *fc)
+__global__ __launch_bounds__(BLOCK) void k_flux(const FluxRec *recs, int n, int nc, int fstride, double *fc,
+                                                const double *cyl_rmin, double dr)
 {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid < n * (DIM == 2 ? nc : nc * nc)) flux_body<DIM>(recs, nc, fstride, fc, gid);
+    if (gid < n * (DIM == 2 ? nc : nc * nc)) flux_body<DIM>(recs, nc, fstride, fc, gid, cyl_rmin, dr);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_arith(long long n, int op, double a, double b, const double *x,
@@ -734,12 +735,13 @@ void launch_gc(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, dou
     else k_gc<3><<<n_blocks(fill_threads<3>(L.n, L.nc)), BLOCK, 0, s>>>(T, L, var, rb, bc);
 }
 
-void launch_flux(int dim, hipStream_t s, const FluxRec *recs, int n, int nc, int fstride, double *fc)
+void launch_flux(int dim, hipStream_t s, const FluxRec *recs, int n, int nc, int fstride, double *fc,
+                 const double *cyl_rmin, double dr)
 {
     if (n <= 0) return;
     const int g = n_blocks((long long)n * (dim == 2 ? nc : nc * nc));
-    if (dim == 2) k_flux<2><<<g, BLOCK, 0, s>>>(recs, n, nc, fstride, fc);
-    else k_flux<3><<<g, BLOCK, 0, s>>>(recs, n, nc, fstride, fc);
+    if (dim == 2) k_flux<2><<<g, BLOCK, 0, s>>>(recs, n, nc, fstride, fc, cyl_rmin, dr);
+    else k_flux<3><<<g, BLOCK, 0, s>>>(recs, n, nc, fstride, fc, nullptr, dr);
 }
 
 void launch_arith(hipStream_t s, long long n, int op, double a, double b, const double *x, double *y)

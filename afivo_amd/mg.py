@@ -147,6 +147,15 @@ class DeviceTree:
         st, tree_of = forest.combine(states)
         return cls.from_state(st, n_var, tree_of)
 
+    def level_ids(self, lvl):
+        """The box ids of level lvl in the device's slot order (afg_level_ids)."""
+        L = _lib.load()
+        n = ctypes.c_int()
+        self._c(L.afg_level_ids(self.h, int(lvl), 0, None, ctypes.byref(n)))
+        ids = np.zeros(n.value, dtype=np.int32)
+        self._c(L.afg_level_ids(self.h, int(lvl), ids.size, _i32(ids), ctypes.byref(n)))
+        return ids
+
     # forests
     def forest_trees(self):
         n = ctypes.c_int()

@@ -42,7 +42,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0: all cores up to 16")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0: every core this process may use, at most OMP_NUM_THREADS when that is set")
     ap.add_argument("--no-timing", action="store_true",
                     help="skip the per-kernel event timing (roofline is then null)")
     ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5", "update", "transfers"], default="c2",
@@ -83,10 +84,37 @@ def measured_traffic():
     return float(m.group(1)) if m else None
 
 
+def host_cpu():
+    """The host's CPU model, its logical CPU count and the CPUs this process
+    may run on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, usable
+
+
+def baseline_threads():
+    """All cores this process may use, within the host's thread budget
+    (OMP_NUM_THREADS when set: the CPU share of one GPU on a shared host)."""
+    _, _, usable = host_cpu()
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return min(usable, int(env)) if env.isdigit() and int(env) > 0 else usable
+
+
 def cpu_baseline(p, cell_updates_per_solve, threads, n_time=5):
     """The reference solver, built from the reference sources (oracle/_ref),
     on the same tree: median solve time of n_time solves after one warm-up
-    solve."""
+    solve, one OpenMP thread per core (OMP_PROC_BIND=close,
+    OMP_PLACES=cores)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from afivo_amd import problems as P
@@ -96,14 +124,16 @@ def cpu_baseline(p, cell_updates_per_solve, threads, n_time=5):
     import tempfile
     work = tempfile.mkdtemp(prefix="afg_cpu_")
     cfg = P.harness_cfg(p, n_time=n_time, dump_input=False, dump_output=False)
-    prefix, _ = O.run_ref(dim, cfg, workdir=work, threads=threads, timeout=900)
+    prefix, _ = O.run_ref(dim, cfg, workdir=work, threads=threads, timeout=900, bind=True)
     n_thr, t = O.read_time(prefix)
     med = float(np.median(t))
+    model, n_cpu, usable = host_cpu()
     return {"value": cell_updates_per_solve / med, "unit": "cell-updates/s", "cores": int(n_thr),
             "kind": "reference",
             "sample": f"same tree and options, whole solve (FMG + {p.n_vcycle} V-cycles), "
-                      f"median of {len(t)} solves after one warm-up, OpenMP threads={n_thr}",
-            "s_per_solve": med}
+                      f"median of {len(t)} solves after one warm-up, OpenMP threads={n_thr}, "
+                      f"OMP_PROC_BIND=close OMP_PLACES=cores",
+            "s_per_solve": med, "cpu_model": model, "host_cpus": n_cpu, "usable_cpus": usable}
 
 
 def solve_to_tol(dt, mg, tol, rhs_norm, max_cycles=30):
@@ -722,7 +752,7 @@ def main():
     else:
         out["roofline"] = None
     if not a.no_cpu_baseline:
-        thr = a.cpu_threads or min(os.cpu_count() or 1, 16)
+        thr = a.cpu_threads or baseline_threads()
         out["cpu_baseline"] = cpu_baseline(p, cu_solve, thr)
         one = cpu_baseline(p, cu_solve, 1, n_time=2) if out["cpu_baseline"] and thr > 1 else None
         if one:   # the same on one core, for the per-core figure

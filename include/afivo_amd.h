@@ -235,6 +235,13 @@ int afg_fas_vcycle(afg_ctx *ctx, const afg_mg_opts *opts, int highest_lvl, int s
 /* Highest level present in the tree (a$D_t%highest_lvl) and the lowest
  * (lbound(tree%lvls)). */
 int afg_levels(const afg_ctx *ctx, int *min_lvl, int *highest_lvl);
+/* The ids of the boxes of level lvl in the order the device holds them (the
+ * level's slot range), into ids[0..n) -- tree%lvls(lvl)%ids (m_aX_t.f90:
+ * 145-149) sorted as a$D_tidy_up sorts it (m_aX_core.f90:272-279, key
+ * morton_from_ix$D(ix - 1), m_morton.f90:33-59); in a forest grouped by
+ * tree first (trees of higher highest level first, then tree index).  n
+ * must be at least the level's box count; *n_out receives it. */
+int afg_level_ids(const afg_ctx *ctx, int lvl, int n, int32_t *ids, int *n_out);
 
 /* a$D_tree_max_cc / a$D_tree_min_cc (m_aX_utils.f90:468-571): extreme of
  * cc(..., iv) over leaf interiors (or all boxes if include_parents). */

@@ -263,10 +263,14 @@ static double *face_at(const orc_tree *t, int ifc, int id, int d, int x0, int x1
     return t->fc[ifc - 1] + t->foff[id] + (size_t)d * len + q;
 }
 
-/* flux_from_children, m_aX_core.f90:838-943 (Cartesian): the fluxes of the
- * leaf neighbour nb_id on the face it shares with parent id, set to the mean
- * of the fluxes of id's children on that face (0.5 (a + b) in 2D, 0.25
- * (((a + b) + c) + d) in 3D, first tangential index fastest). */
+/* flux_from_children, m_aX_core.f90:838-943: the fluxes of the leaf
+ * neighbour nb_id on the face it shares with parent id, set to the mean of
+ * the fluxes of id's children on that face (0.5 (a + b) in 2D, 0.25
+ * (((a + b) + c) + d) in 3D, first tangential index fastest).  On a
+ * cylindrical tree the y faces (d = 1) take the radius-weighted mean
+ * 0.5 (w1 a + w2 b) of m_aX_core.f90:877-887, with the weights of
+ * a2_cyl_child_weights (m_aX_t.f90:518-528) for the neighbour's cell
+ * ioff(1) + n: t = 0.25 dr / r, r = r_min(1) + (i - 0.5) dr, w = 1 -/+ t. */
 static void flux_from_children(orc_tree *t, int id, int f, int ifc)
 {
     int nc = t->ncell[id], nch = nc >> 1, d = f / 2, high = f % 2;
@@ -288,7 +292,14 @@ static void flux_from_children(orc_tree *t, int id, int f, int ifc)
                     double a = *face_at(t, ifc, c_id, d, y[0], y[1], y[2]);
                     y[ta] = 2 * m;
                     double b = *face_at(t, ifc, c_id, d, y[0], y[1], y[2]);
-                    v = 0.5 * (a + b);
+                    if (t->cyl && d == 1) {
+                        double r = t->rmin[nb_id] + (x[ta] - 0.5) * t->dr[nb_id];
+                        double tw = 0.25 * t->dr[nb_id] / r;
+                        double w1 = 1 - tw, w2 = 1 + tw;
+                        v = 0.5 * (w1 * a + w2 * b);
+                    } else {
+                        v = 0.5 * (a + b);
+                    }
                 } else {
                     double w[4];
                     for (int k = 0; k < 4; k++) {

@@ -215,8 +215,10 @@ def _fmt(v):
     return str(v)
 
 
-def run_ref(dim: int, cfg: dict, workdir=None, threads=1, timeout=600):
-    """Run the reference harness with namelist values `cfg`.
+def run_ref(dim: int, cfg: dict, workdir=None, threads=1, timeout=600, bind=False):
+    """Run the reference harness with namelist values `cfg` on `threads`
+    OpenMP threads (bound one per core, OMP_PROC_BIND=close and
+    OMP_PLACES=cores, when `bind`).
 
     Returns (prefix, stdout) where prefix + '.in' / '.out' / '.res' /
     '.time' are the files it wrote."""
@@ -233,6 +235,9 @@ def run_ref(dim: int, cfg: dict, workdir=None, threads=1, timeout=600):
         f.write("&cfg\n" + body + "\n/\n")
     env = dict(os.environ)
     env["OMP_NUM_THREADS"] = str(threads)
+    if bind:
+        env["OMP_PROC_BIND"] = "close"
+        env["OMP_PLACES"] = "cores"
     p = subprocess.run([exe, nml], capture_output=True, text=True, env=env,
                        timeout=timeout)
     if p.returncode != 0 or "STOP" in p.stdout or "STOP" in p.stderr:

@@ -422,13 +422,14 @@ void launch_gc(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, dou
     });
 }
 
-void launch_flux(int dim, hipStream_t s, const FluxRec *recs, int n, int nc, int fstride, double *fc)
+void launch_flux(int dim, hipStream_t s, const FluxRec *recs, int n, int nc, int fstride, double *fc,
+                 const double *cyl_rmin, double dr)
 {
     if (n <= 0) return;
     emul_submit(s, [=] {
         const int th = n * (dim == 2 ? nc : nc * nc);
         for (int g = 0; g < th; g++) {
-            if (dim == 2) flux_body<2>(recs, nc, fstride, fc, g);
+            if (dim == 2) flux_body<2>(recs, nc, fstride, fc, g, cyl_rmin, dr);
             else flux_body<3>(recs, nc, fstride, fc, g);
         }
     });

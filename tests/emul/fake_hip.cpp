@@ -59,8 +59,30 @@ void emul_free(void *p)
     g_maps.erase(it);
 }
 
+namespace {
+thread_local hipError_t g_last_error = hipSuccess;
+int g_fail_launches = 0;   // the next this many submissions fail as a launch would
+}  // namespace
+
+// Test hook: make the next n submitted kernels fail to launch (they do not
+// run, and hipGetLastError reports hipErrorLaunchFailure), as a launch with
+// a bad configuration does on the device.
+extern "C" void emul_fail_launches(int n) { g_fail_launches = n; }
+
+hipError_t hipGetLastError()
+{
+    const hipError_t e = g_last_error;
+    g_last_error = hipSuccess;
+    return e;
+}
+
 void emul_submit(hipStream_t s, std::function<void()> work)
 {
+    if (g_fail_launches > 0) {
+        g_fail_launches--;
+        g_last_error = hipErrorLaunchFailure;
+        return;
+    }
     if (s && s->capturing)
         s->graph->work.push_back(std::move(work));
     else
@@ -78,7 +100,10 @@ hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t a, int)
     *v = a == hipDeviceAttributeSharedMemPerBlockOptin ? 163840 : 65536;
     return hipSuccess;
 }
-const char *hipGetErrorString(hipError_t e) { return e == hipSuccess ? "no error" : "emulated error"; }
+const char *hipGetErrorString(hipError_t e)
+{
+    return e == hipSuccess ? "no error" : e == hipErrorLaunchFailure ? "unspecified launch failure" : "emulated error";
+}
 hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned)
 {
     *s = new EmulStream;

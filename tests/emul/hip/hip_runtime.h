@@ -9,7 +9,7 @@
 
 typedef int hipError_t;
 enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorMemoryAllocation = 2,
-       hipErrorNoDevice = 100, hipErrorNotSupported = 801 };
+       hipErrorNoDevice = 100, hipErrorLaunchFailure = 719, hipErrorNotSupported = 801 };
 typedef struct EmulStream *hipStream_t;
 typedef struct EmulEvent *hipEvent_t;
 typedef struct EmulGraph *hipGraph_t;
@@ -30,6 +30,7 @@ hipError_t hipGetDevice(int *dev);
 hipError_t hipSetDevice(int dev);
 hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t a, int dev);
 const char *hipGetErrorString(hipError_t e);
+hipError_t hipGetLastError();
 hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned flags);
 hipError_t hipStreamDestroy(hipStream_t s);
 hipError_t hipStreamSynchronize(hipStream_t s);

@@ -140,7 +140,9 @@ def make_update(name, base, extra):
 # case with variable 4 set from phi or rhs and changed by one routine of
 # afivo (the harness's transfer_dumps).  Stored per routine: the packed
 # positions of variable 4 that the routine changed and their values.
-XFERS = {"xfer_g2d_nc4": "g2d_nc4", "xfer_g2d_nc8_guess": "g2d_nc8_guess", "xfer_g3d_nc4_ad": "g3d_nc4_ad"}
+XFERS = {"xfer_g2d_nc4": "g2d_nc4", "xfer_g2d_nc8_guess": "g2d_nc8_guess", "xfer_g3d_nc4_ad": "g3d_nc4_ad",
+         # cylindrical: the radius-weighted restriction and y-face fluxes
+         "xfer_g2d_nc4_cyl": "g2d_nc4_cyl"}
 XFER_OPS = {"gci": 1, "gcl": 1, "gc0": 1, "rst": 2, "pr1": 2, "pr0": 2, "pr2": 2}   # op: source variable
 
 

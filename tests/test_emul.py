@@ -75,3 +75,87 @@ from test_transfers_gpu import (  # noqa: E402,F401
     test_consistent_fluxes_match_oracle_full_size,
     test_consistent_fluxes_match_reference,
 )
+
+
+def test_failed_launch_is_an_error():
+    """A kernel launch that fails (emulated: the next kernels do not run and
+    hipGetLastError reports the failure, as for a launch with a bad
+    configuration on the device) makes the entry point return AFG_ERR_HIP,
+    naming the entry point, instead of status 0 with stale data; the next
+    call works again."""
+    import ctypes
+
+    import numpy as np
+
+    from afivo_amd import mg as M
+    from afivo_amd import problems as P
+    from afivo_amd._lib import AfgError
+
+    hook = ctypes.CDLL(conftest.EMUL_LIB).emul_fail_launches
+    hook.argtypes = [ctypes.c_int]
+    p = P.small_2d()
+    dt = M.DeviceTree.from_state(p.state)
+    dt.set_graphs(False)
+    mg = M.default_mg(n_base=p.n_base)
+    M.mg_fas_fmg(dt, mg, True, False)
+    for call, what in ((lambda: M.mg_fas_vcycle(dt, mg, dt.highest_lvl, True), "afg_fas_vcycle"),
+                       (lambda: dt.restrict_tree(1), "afg_restrict_tree"),
+                       (lambda: M.max_abs_cc(dt, 3), "tree extreme")):
+        hook(1)
+        with pytest.raises(AfgError, match=what + ".*launch failed"):
+            call()
+    M.mg_fas_vcycle(dt, mg, dt.highest_lvl, True)
+    assert np.isfinite(M.max_abs_cc(dt, 3))
+    dt.close()
+
+
+def brute_morton(ix0, dim):
+    """Bit interleave of ix - 1 done bit by bit (bit b of dimension d goes to
+    bit b*dim + d): morton_from_ix2/3 (m_morton.f90:33-59) on ix - 1, the key
+    of a$D_tidy_up (m_aX_core.f90:272-279)."""
+    key = 0
+    for b in range(21):
+        for d in range(dim):
+            key |= ((int(ix0[d]) >> b) & 1) << (b * dim + d)
+    return key
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_slot_order_is_morton(dim):
+    """Every level's slots are in ascending Morton order of ix - 1 (brute-force
+    interleave), on the benchmark tree of the dimension; in a forest they are
+    grouped by tree (higher highest level first, then tree index) and in
+    Morton order within a tree.  The interleave itself is pinned on values
+    whose key is known: ix - 1 = (1, 0[, 0]) -> 1, (0, 1[, 0]) -> 2,
+    (0, 0, 1) -> 4, (3, 3) -> 15, all ones -> 2^(21 dim) - 1."""
+    import numpy as np
+
+    from afivo_amd import forest as F
+    from afivo_amd import mg as M
+    from afivo_amd import problems as P
+
+    assert brute_morton((1, 0, 0), dim) == 1 and brute_morton((0, 1, 0), dim) == 2
+    assert brute_morton((3, 3, 0), 2) == 15 and brute_morton((0, 0, 1), 3) == 4
+    assert brute_morton(((1 << 21) - 1,) * 3, dim) == (1 << (21 * dim)) - 1
+    st = (P.c1() if dim == 2 else P.c2()).state
+    dt = M.DeviceTree(st.dim, st.nc, 1, st.recs, st.dr_base)
+    row = {int(i): r for r, i in enumerate(st.ids)}
+    for lvl in range(dt.min_lvl, dt.highest_lvl + 1):
+        ids = dt.level_ids(lvl)
+        assert ids.size == int(np.sum(st.lvl == lvl))
+        keys = [brute_morton(st.ix[row[int(i)]] - 1, dim) for i in ids]
+        assert all(a < b for a, b in zip(keys, keys[1:])), f"level {lvl} not in Morton order"
+    dt.close()
+    if dim == 2:
+        states = [F.c3_tree(5, k)[0] for k in range(6)]
+        states = [s for s in states if s.nc == states[0].nc][:4]
+        comb, tree_of = F.combine(states)
+        dt = M.DeviceTree(comb.dim, comb.nc, 1, comb.recs, comb.dr_base, tree_of)
+        hi = [int(s.lvl.max()) for s in states]
+        row = {int(i): r for r, i in enumerate(comb.ids)}
+        for lvl in range(dt.min_lvl, dt.highest_lvl + 1):
+            ids = dt.level_ids(lvl)
+            order = [(-hi[tree_of[row[int(i)]]], int(tree_of[row[int(i)]]),
+                      brute_morton(comb.ix[row[int(i)]] - 1, 2)) for i in ids]
+            assert order == sorted(order) and len(set(order)) == len(order)
+        dt.close()

@@ -307,6 +307,8 @@ def test_oracle_transfers_match_reference(name):
     for op, (pos, val) in ops.items():
         src = meta["ops"][op]
         ot = O.OracleTree(st, n_var=4)
+        if meta["cfg"].get("cyl"):
+            ot.set_coord(True)
         ot.set(4, st.data[src - 1])
         run_xfer(ot, op, st.min_lvl, st.highest_lvl)
         assert_bitexact(ot.get(4), xfer_expected(st.data[src - 1], pos, val), f"{name} {op}")
@@ -324,8 +326,42 @@ def test_oracle_consistent_fluxes_match_reference(name):
     _, _, st, _ = load_golden(meta["base"])
     before = face_gradient(st, st.data[0])
     ot = O.OracleTree(st)
+    if meta["cfg"].get("cyl"):
+        ot.set_coord(True)
     ot.set_fc(1, before)
     ot.set_fc(2, before)
     ot.consistent_fluxes([1])
     assert_bitexact(ot.get_fc(1), xfer_expected(before, z["flx_pos"], z["flx_val"]), name)
     assert_bitexact(ot.get_fc(2), before, "face variable 2 untouched")
+
+
+def test_cylindrical_flux_fixture_needs_the_weights():
+    """The cylindrical flux fixture pins the radius weights: the Cartesian
+    mean (the same tree without cylindrical coordinates) differs from it on
+    y faces only, and a weight taken at the fine cell's radius instead of
+    the coarse one would differ too (checked by the weights' closed form)."""
+    from helpers import face_gradient
+    name = "xfer_g2d_nc4_cyl"
+    meta, _ = load_xfer(name)
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    _, _, st, _ = load_golden(meta["base"])
+    before = face_gradient(st, st.data[0])
+    want = xfer_expected(before, z["flx_pos"], z["flx_val"])
+    ot = O.OracleTree(st)
+    ot.set_fc(1, before)
+    ot.consistent_fluxes([1])
+    cart = ot.get_fc(1)
+    diff = np.nonzero(cart != want)[0]
+    assert diff.size > 0
+    # every differing value is a y face: offsets within a box past its fx block
+    foff, nc = st.face_offsets(), st.ncell()
+    box = np.searchsorted(foff, diff, side="right") - 1
+    local = diff - foff[box]
+    assert np.all(local >= (nc[box] + 1) * nc[box])
+    # weights w = 1 -/+ 0.25 dr / r: their mean is 1, so a flux constant
+    # along x restricts to itself, and the weighted mean exceeds the plain
+    # one where the outer flux is the larger
+    a, b, dr, r = 1.0, 3.0, 0.125, 0.3125
+    t = 0.25 * dr / r
+    assert 0.5 * ((1 - t) * a + (1 + t) * b) > 0.5 * (a + b)
+    assert 0.5 * ((1 - t) * 2.0 + (1 + t) * 2.0) == 2.0

@@ -33,6 +33,8 @@ def test_transfers_match_reference(name):
     meta, ops = load_xfer(name)
     _, _, st, _ = load_golden(meta["base"])
     dt = M.DeviceTree.from_state(st, 4)
+    if meta["cfg"].get("cyl"):
+        dt.set_coord(True)
     for op, (pos, val) in ops.items():
         src = meta["ops"][op]
         dt.upload(4, st.data[src - 1])
@@ -148,6 +150,8 @@ def test_consistent_fluxes_match_reference(name):
     _, _, st, _ = load_golden(meta["base"])
     before = face_gradient(st, st.data[0])
     dt = M.DeviceTree.from_state(st)
+    if meta["cfg"].get("cyl"):
+        dt.set_coord(True)
     dt.upload_fc(1, before)
     dt.upload_fc(3, before)
     dt.consistent_fluxes([1])
