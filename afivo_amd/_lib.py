@@ -57,6 +57,8 @@ SIGNATURES = {
     "afg_consistent_fluxes": (ctypes.c_int, [P, ctypes.c_int, I32P]),
     "afg_set_coord": (ctypes.c_int, [P, ctypes.c_int, F64]),
     "afg_update_tree": (ctypes.c_int, [P, ctypes.c_int, I32P, F64, I32P, ctypes.c_int, ctypes.c_int, I32P]),
+    "afg_update_forest": (ctypes.c_int, [P, ctypes.c_int, I32P, F64, I32P, I32P]),
+    "afg_mixed_first_fmg": (ctypes.c_int, [P, ctypes.c_int]),
     "afg_forest_trees": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), I32P]),
     "afg_forest_max_abs": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, F64P]),
     "afg_set_bc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, F64P]),

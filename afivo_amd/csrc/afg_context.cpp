@@ -115,6 +115,7 @@ struct afg_ctx {
     double *d_chunk_max = nullptr, *d_tree_out = nullptr, *h_tree_out = nullptr;
     // mixed-precision solve in steps: begun, and the next correction is the first (an FMG)
     bool mixed_ready = false, mixed_fmg = false;
+    bool mixed_first_fmg = true;         // the first correction after afg_mixed_begin is an FMG
     bool tags_set = false;               // every box has its eps tag (they are set once)
 
     // tree
@@ -1737,8 +1738,13 @@ int afg_set_coord(afg_ctx *c, int coord, double r_base)
     return build_cyl(c);
 }
 
-int afg_update_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base, const int32_t *tree_of,
-                    int prolong_order, int n_vars, const int32_t *vars)
+// The tree update behind afg_update_tree and afg_update_forest.  A box of the
+// new tree keeps its data when the old tree has a box of the same level and
+// ix in the same tree: tree t of the new forest is tree old_tree[t] of the
+// old one (identity when old_tree is null; -1: a tree without a predecessor,
+// whose boxes start from zeros).
+static int update_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base, const int32_t *tree_of,
+                       const int32_t *old_tree, int prolong_order, int n_vars, const int32_t *vars)
 {
     if (int e = check_ctx(c)) return e;
     if (c->capturing) return c->fail(AFG_ERR_STATE, "tree update inside a graph capture");
@@ -1770,7 +1776,13 @@ int afg_update_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base
         ctx_free(c, d_src);
         return code;
     };
+    const int old_n_trees = c->n_trees;
     if (int e = afg_set_forest(c, n_boxes, recs, dr_base, tree_of)) return release(e);
+    if (old_tree)
+        for (int t = 0; t < c->n_trees; t++)
+            if (old_tree[t] < -1 || old_tree[t] >= old_n_trees)
+                return release(c->fail(AFG_ERR_ARG, "tree %d: predecessor %d not in -1..%d", t, old_tree[t],
+                                       old_n_trees - 1));
 
     // src[slot]: the box's old slot, -1 for a box the update added; the
     // added boxes level by level (lowest first)
@@ -1781,7 +1793,9 @@ int afg_update_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base
         const Level &L = c->levels[li];
         lv_off[li] = (int)added.size();
         for (int s = L.slot0; s < L.slot0 + L.n; s++) {
-            auto it = old_slot.find(box_key(c, s));
+            BoxKey key = box_key(c, s);
+            if (old_tree) key.tree = old_tree[key.tree];
+            auto it = key.tree < 0 ? old_slot.end() : old_slot.find(key);
             if (it != old_slot.end()) src[s] = it->second;
             else added.push_back(s);
         }
@@ -1864,6 +1878,20 @@ int afg_update_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base
         }
     }
     return 0;
+}
+
+int afg_update_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base, const int32_t *tree_of,
+                    int prolong_order, int n_vars, const int32_t *vars)
+{
+    return update_tree(c, n_boxes, recs, dr_base, tree_of, nullptr, prolong_order, n_vars, vars);
+}
+
+int afg_update_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base, const int32_t *tree_of,
+                      const int32_t *old_tree)
+{
+    if (!c) return AFG_ERR_ARG;
+    if (!old_tree) return c->fail(AFG_ERR_ARG, "null predecessor list");
+    return update_tree(c, n_boxes, recs, dr_base, tree_of, old_tree, -1, 0, nullptr);
 }
 
 int afg_set_bc(afg_ctx *c, int id, int nb, int bc_type, const double *b)
@@ -2311,7 +2339,7 @@ int afg_mixed_begin(afg_ctx *c, const afg_mg_opts *o, int have_guess)
             launch_fill_value(c->dim, c->stream, m.T, L.slot0, L.n, L.nc, m.V.phi, 0.0);
         }
     mixed_residual(m);
-    c->mixed_fmg = true;
+    c->mixed_fmg = c->mixed_first_fmg;
     c->mixed_ready = true;
     return launched(c, "afg_mixed_begin");
 }
@@ -2343,6 +2371,13 @@ int afg_mixed_correct(afg_ctx *c, const afg_mg_opts *o, int cycles)
     launch_from_fp32(c->stream, n, f.V.phi, m.V.phi, 1);
     mixed_residual(m);
     return launched(c, "afg_mixed_correct");
+}
+
+int afg_mixed_first_fmg(afg_ctx *c, int enabled)
+{
+    if (!c) return AFG_ERR_ARG;
+    c->mixed_first_fmg = enabled != 0;
+    return 0;
 }
 
 int afg_mixed_solve(afg_ctx *c, const afg_mg_opts *o, int have_guess, double tol, int max_iter,

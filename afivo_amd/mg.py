@@ -133,6 +133,26 @@ class DeviceTree:
         self.dr_base = dr
         self._tree_info()
 
+    def update_forest(self, records, tree_of, old_tree, dr_base=None):
+        """Switch to another forest keeping the trees it shares with this one
+        (afg_update_forest): tree t of the new forest is tree old_tree[t] of
+        the current one (its boxes keep all values), or new (-1: zeros)."""
+        recs = np.ascontiguousarray(records, dtype=np.int32)
+        t_of = np.ascontiguousarray(tree_of, dtype=np.int32)
+        old = np.ascontiguousarray(old_tree, dtype=np.int32)
+        dr = self.dr_base if dr_base is None else float(dr_base)
+        self._c(_lib.load().afg_update_forest(self.h, recs.shape[0], _i32(recs), dr, _i32(t_of), _i32(old)))
+        self.records = recs
+        self.ids = np.ascontiguousarray(recs[:, 0])
+        self.tree_of = t_of
+        self.dr_base = dr
+        self._tree_info()
+
+    def mixed_first_fmg(self, enabled):
+        """Whether the first mixed-precision correction after mixed_begin
+        starts with an FMG (afg_mixed_first_fmg; default True)."""
+        self._c(_lib.load().afg_mixed_first_fmg(self.h, 1 if enabled else 0))
+
     @classmethod
     def from_state(cls, st: TreeState, n_var=None, tree_of=None):
         t = cls(st.dim, st.nc, n_var or max(st.n_var, len(st.data)), st.recs, st.dr_base, tree_of)

@@ -1,32 +1,45 @@
 """Serving: Poisson-solve requests arriving over time, solved on one GPU with
-continuous batching.
+continuous batching at request granularity.
 
 A request is one tree (2D or 3D) with its right-hand side and a tolerance on
-the relative residual max|rhs - A phi| / max|rhs| over leaf interiors.  The
-server keeps a set of *lanes*; a lane is one forest context (afg_set_forest:
-its own device pools, tables and HIP stream) whose trees run through the
-same FMG / V-cycle stages together, every stage one launch over the boxes of
-a level in all the lane's trees.  Requests that arrive while lanes are busy
-wait; whenever a lane is free and the device memory of the running lanes
-plus the new forest fits the HBM budget, the waiting requests of one
-(dimension, n_cell) class -- oldest first, up to `max_trees` -- are admitted
-as a new lane.  Lanes advance in turn, one V-cycle each, so the work of
-several lanes overlaps on the device; a lane retires when each of its trees
-meets its tolerance, and its results are copied back.
+the relative residual max|rhs - A phi| / max|rhs| over leaf interiors.
 
-Lanes solve in mixed precision by default: FP64 defect correction with FP32
-FAS multigrid corrections (afg_mixed_begin / afg_mixed_correct), one
-correction of `cycles` FP32 V-cycles per step; with precision="fp64" a step is
-one FP64 V-cycle after an FP64 FMG.
+The server keeps a set of *lanes*.  A lane is one forest context
+(afg_set_forest: its own device pools, tables and HIP stream) whose member
+trees, all of one request class (dimension, n_cell and, with `big_cells`,
+small or large), run through the same multigrid stages together -- every
+stage one launch over the boxes of a level in all member trees.
 
-A request's result does not depend on the mix it was solved in beyond its
-tolerance: a tree in a forest goes through exactly the stages it would go
-through alone, and only the number of steps (the lane stops when its last
-tree converges) can differ.
+* Admission: while fewer than `max_lanes` lanes run and the device bytes of
+  the lanes (the exact allocations of each context) plus the new forest fit
+  the HBM budget, the waiting requests of the oldest request's class are
+  admitted together as a new lane, set up on host threads (forest tables,
+  uploads, the first residual) while the main thread steps the running
+  lanes.
+* Steps: lanes advance in turn, one step each: one mixed-precision
+  correction (FP64 defect correction, `cycles` FP32 FAS V-cycles from
+  e = 0; afg_mixed_first_fmg(0), so that every correction a tree receives is
+  the same whenever it joined) or one FP64 V-cycle.  Launches are
+  asynchronous, so one lane's work runs while the host checks another's
+  per-tree residual norms (which waits for that lane's stream only).
+* Release: a member that meets its tolerance leaves at once -- its phi is
+  copied back and its latency ends there, whatever its lane-mates still need.
+* Compaction: when the members that left hold at least `compact_at` of a
+  lane's boxes, the forest is rebuilt in place with afg_update_forest: the
+  live members keep phi, rhs and tmp on the device, and waiting requests of
+  the class that fit join it (their rhs uploaded; in mixed precision
+  afg_mixed_begin with have_guess forms their residual and leaves the live
+  members' state bit for bit as it was).
 
-`one_at_a_time` replays the same trace one request after the other, each in
-its own context: through the plain FP64 solver, or through the same
-mixed-precision steps as a lane of one tree.
+A tree goes through exactly the operations it would go through alone and
+stops at the same residual test, so each request's phi is bit-identical to
+the same request solved alone (`solve_alone`): a tree in a forest gets
+exactly its solo results (links never cross trees, and each tree's stages
+stop at its own highest level).
+
+FP64 lanes run plain FAS V-cycles from phi = 0 (mg$D_fas_vcycle,
+m_aX_mg.f90:208-255): a tree can join a running forest at any step, where an
+FMG would have to run for every member at once.
 """
 from __future__ import annotations
 
@@ -99,93 +112,206 @@ def make_trace(seed, n_requests, rate, frac_2d=0.7, cells=(1e4, 1e7), tols=(1e-8
     return out
 
 
-MAX_STEPS = 40
+def retime(trace, rate, seed):
+    """The same requests with new Poisson arrival times at `rate` per second
+    (results cleared), for a load sweep over one set of trees."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=len(trace)))
+    for r, a in zip(sorted(trace, key=lambda q: q.rid), t):
+        r.arrival, r.start, r.finish, r.cycles, r.residual, r.phi = float(a), -1.0, -1.0, -1, float("nan"), None
+    return trace
+
+
+MAX_STEPS = 60
+
+
+def forest_bytes(states, precision="mixed", n_var=3):
+    """Device bytes a forest of these trees takes: every pool the solver
+    allocates per box (the variables, the second phi pool of the fused pass
+    pairs, in 3D the colour-split rhs; in mixed precision the FP32 copies of
+    phi, rhs and tmp and their second phi pool and colour-split rhs), the
+    topology tables, per-tree and fixed parts for norms and the bottom
+    kernel's tables, and the allocator's rounding of each of the context's
+    dozen large allocations to 2 MiB.  An upper bound of afg_device_bytes
+    and of the device memory the forest takes."""
+    st = states[0]
+    dim, nc = st.dim, st.nc
+    stride = -(-(nc + 2) ** dim // 16) * 16
+    interior = nc ** dim
+    per_box = (n_var + 1) * stride * 8 + (interior * 8 if dim == 3 else 0)
+    if precision == "mixed":
+        per_box += 4 * stride * 4 + (interior * 4 if dim == 3 else 0)
+    per_box += (2 * dim + 8) * 4 + 2          # neighbours, parent, child offset, lists, tags
+    boxes = sum(s.n_boxes for s in states)
+    return boxes * per_box + len(states) * 4096 + (128 << 10) + 12 * (2 << 20)
+
+
+def combine_records(states):
+    """The records of a forest of `states` (afg_set_forest), tree_of per
+    record and each tree's ids in the forest, in record order."""
+    D = states[0].dim
+    c_par, c_ch, c_nb = 2 + D, 3 + D, 3 + D + (1 << D)
+    recs, tree_of, ids, off = [], [], [], 0
+    for t, st in enumerate(states):
+        r = st.recs.astype(np.int64).copy()
+        r[:, 0] += off
+        for cols in (slice(c_par, c_par + 1), slice(c_ch, c_nb), slice(c_nb, c_nb + 2 * D)):
+            block = r[:, cols]
+            block[block > 0] += off
+            r[:, cols] = block
+        recs.append(r)
+        tree_of.append(np.full(st.n_boxes, t, dtype=np.int32))
+        ids.append(np.ascontiguousarray(r[:, 0], dtype=np.int32))
+        off += int(st.ids.max())
+    if off >= 2 ** 31:
+        raise ValueError("too many boxes for int32 ids")
+    return np.concatenate(recs).astype(np.int32), np.concatenate(tree_of), ids
 
 
 class Lane:
-    """One forest of admitted requests in flight."""
+    """One forest context of a request class: the member trees run every
+    step together; members leave as soon as they converge, and the forest
+    is compacted (and takes in waiting requests) when enough of it has
+    left."""
 
-    def __init__(self, reqs, mg, now, precision="mixed", cycles=2):
-        self.reqs = reqs
-        self.states = [r.state for r in reqs]
-        self.dt = M.DeviceTree.from_forest(self.states)
-        self.mg = mg
-        self.mixed = precision == "mixed"
-        self.n_cyc = cycles
-        self.tol = np.array([r.tol * r.rhs_norm for r in reqs])
-        self.cycles = 0
-        for r in reqs:
-            r.start = now
-        if self.mixed:
-            M.mixed_begin(self.dt, mg)
+    def __init__(self, key, mg, precision, cycles, use_graphs):
+        self.key, self.mg, self.mixed, self.n_cyc = key, mg, precision == "mixed", cycles
+        self.precision, self.use_graphs = precision, use_graphs
+        self.dt = None
+        self.members = []       # Requests, in tree order
+        self.live = []          # member still being solved
+        self.steps = []         # steps done per member
+        self.ids = []           # each member's ids in the forest
+        self.boxes = []         # each member's box count
+        self.rebuilds = 0
+
+    def device_bytes(self):
+        return self.dt.device_bytes() if self.dt is not None else 0
+
+    def live_members(self):
+        return [r for r, a in zip(self.members, self.live) if a]
+
+    def dead_fraction(self):
+        tot = sum(self.boxes)
+        return sum(b for b, a in zip(self.boxes, self.live) if not a) / tot if tot else 0.0
+
+    def rebuild(self, keep, add):
+        """The forest of members `keep` (live members, in their current
+        order: they keep their state on the device) and `add` (new: their
+        rhs is uploaded); with neither, the context is closed."""
+        old_index = {id(r): k for k, r in enumerate(self.members)}
+        old_steps = {id(r): s for r, s in zip(self.members, self.steps)}
+        members = keep + add
+        if not members:
+            if self.dt is not None:
+                self.dt.close()
+            self.dt, self.members, self.live, self.steps, self.ids, self.boxes = None, [], [], [], [], []
+            return
+        states = [r.state for r in members]
+        recs, tree_of, ids = combine_records(states)
+        st0 = states[0]
+        if self.dt is None:
+            self.dt = M.DeviceTree(st0.dim, st0.nc, 3, recs, st0.dr_base, tree_of)
+            self.dt.set_graphs(self.use_graphs)
+            self.dt.mixed_first_fmg(False)
         else:
-            M.mg_fas_fmg(self.dt, mg, True, False)
+            old = [old_index[id(r)] for r in keep] + [-1] * len(add)
+            self.dt.update_forest(recs, tree_of, old, st0.dr_base)
+        self.rebuilds += 1
+        for r, tids in zip(add, ids[len(keep):]):
+            self.dt.upload(2, r.state.data[1], tids)
+        if self.mixed:
+            M.mixed_begin(self.dt, self.mg, have_guess=True)
+        self.members, self.ids = members, ids
+        self.live = [True] * len(members)
+        self.steps = [old_steps.get(id(r), 0) for r in members]
+        self.boxes = [r.state.n_boxes for r in members]
+
+    def check(self):
+        """Per-member residual norms of the last step (waits for this lane's
+        stream); the live members whose residual is valid and meets their
+        tolerance, or that ran out of steps."""
+        if not any(self.live):
+            return []
+        res = self.dt.forest_max_abs(3)
+        done = []
+        for k, r in enumerate(self.members):
+            valid = self.mixed or self.steps[k] > 0
+            if self.live[k] and valid and (res[k] <= r.tol * r.rhs_norm or self.steps[k] >= MAX_STEPS):
+                r.residual = float(res[k]) / r.rhs_norm
+                done.append(k)
+        return done
+
+    def release(self, done, now):
+        """Members `done` leave now: phi is copied back and their latency
+        ends, whatever their lane-mates still need."""
+        for k in done:
+            r = self.members[k]
+            r.phi = self.dt.download(1, self.ids[k])
+            r.finish, r.cycles = now, self.steps[k]
+            self.live[k] = False
 
     def step(self):
-        """Check convergence of the last step; launch the next one if needed
-        (a mixed-precision correction, or an FP64 V-cycle).  Returns True when
-        every tree meets its tolerance."""
-        res = self.dt.forest_max_abs(3)          # waits for this lane's work
-        if np.all(res <= self.tol) or self.cycles >= MAX_STEPS:
-            self.res = res
-            return True
+        """Launch the next step of every member (asynchronous)."""
         if self.mixed:
             M.mixed_correct(self.dt, self.mg, self.n_cyc)
         else:
             M.mg_fas_vcycle(self.dt, self.mg, self.dt.highest_lvl, True)
-        self.cycles += 1
-        return False
-
-    def retire(self, now):
-        phis = F.split(self.states, self.dt.download(1))
-        for r, phi, res in zip(self.reqs, phis, self.res):
-            r.finish, r.cycles, r.residual, r.phi = now, self.cycles, float(res) / r.rhs_norm, phi
-
-    def device_bytes(self):
-        return self.dt.device_bytes()
+        self.steps = [s + 1 for s in self.steps]
 
 
-def forest_bytes(reqs, n_var=3, precision="mixed"):
-    """Device bytes a forest of these requests takes (pools and tables; the
-    FP32 copies of phi, rhs and tmp in mixed precision)."""
-    boxes = sum(r.state.n_boxes for r in reqs)
-    st = reqs[0].state
-    stride = -(-(st.nc + 2) ** st.dim // 16) * 16
-    per_box = n_var * stride * 8 + (3 * stride * 4 if precision == "mixed" else 0)
-    return boxes * (per_box + 64)
+def request_class(r, big_cells):
+    return (r.state.dim, r.state.nc, bool(big_cells) and r.leaf_cells >= big_cells)
 
 
-def serve(trace, hbm_budget=64 << 30, max_lanes=4, max_trees=64, n_base=50, lane_log=None,
-          builders=3, precision="mixed", cycles=2):
-    """Run the trace through the continuously batching server; returns the
-    requests with start/finish times (seconds after the trace start).
+def serve(trace, hbm_budget=64 << 30, max_lanes=6, max_trees=64, n_base=50, precision="mixed", cycles=2,
+          big_cells=0, use_graphs=True, compact_at=0.5, builders=3, log=None):
+    """Run the trace through the server; returns the requests with
+    start/finish times (seconds after the trace start), their step counts,
+    residuals and phi.
 
-    Lanes are set up (forest tables, uploads, the first launches) by
-    `builders` host threads while the main thread drives the running lanes,
-    so host work overlaps device work.  If given, `lane_log` receives one
-    (admitted, setup_s, retired, n_trees, steps) tuple per lane."""
+    Waiting requests of one class (the oldest request's) are admitted
+    together as a new lane while fewer than `max_lanes` run and the HBM
+    budget allows; `builders` host threads set lanes up (forest tables,
+    uploads, the first residual) while the main thread steps the running
+    lanes in turn.  A member leaves its lane as soon as it converges.  When
+    the members that left hold at least `compact_at` of a lane's boxes, the
+    lane is rebuilt in place (afg_update_forest) with its live members and
+    the waiting requests of its class that fit.  `log`, if given, receives
+    counts of lanes, rebuilds and steps."""
     import concurrent.futures as cf
 
     mg = M.default_mg(n_base=n_base)
-    fbytes = lambda g: forest_bytes(g, precision=precision)   # noqa: E731
     pending = collections.deque(sorted(trace, key=lambda r: r.arrival))
-    waiting, lanes, building = [], [], []
-    # Retired lanes release their device memory only while no lane runs (or
-    # when memory is needed): freeing device memory waits for all the
-    # device's work, which would stall the running lanes.
-    retired = []
-
-    def release():
-        for lane in retired:
-            lane.dt.close()
-        retired.clear()
-
+    waiting, lanes, building, retired = [], [], [], []
+    counts = collections.Counter()
     t0 = time.perf_counter()
 
-    def build(group, t_adm):
-        lane = Lane(group, mg, t_adm, precision, cycles)
-        lane.t_adm, lane.setup = t_adm, time.perf_counter() - t0 - t_adm
+    def used_bytes():
+        return (sum(l.device_bytes() for l in lanes + retired) +
+                sum(forest_bytes([r.state for r in g], precision) for g, _ in building))
+
+    def release_retired():
+        # freeing device memory waits for all the device's work: done only
+        # while nothing runs, or when the memory is needed
+        for lane in retired:
+            lane.rebuild([], [])
+        retired.clear()
+
+    def build(group):
+        lane = Lane(request_class(group[0], big_cells), mg, precision, cycles, use_graphs)
+        lane.rebuild([], group)
         return lane
+
+    def fit(cands, base_states, room):
+        out = []
+        for r in cands:
+            if len(base_states) + len(out) >= max_trees:
+                break
+            if forest_bytes(base_states + [q.state for q in out] + [r.state], precision) > room:
+                break
+            out.append(r)
+        return out
 
     with cf.ThreadPoolExecutor(max_workers=builders) as pool:
         while pending or waiting or lanes or building:
@@ -193,25 +319,26 @@ def serve(trace, hbm_budget=64 << 30, max_lanes=4, max_trees=64, n_base=50, lane
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
             if not lanes and not building:
-                release()
-            # admission: one class at a time, oldest waiting request first
+                release_retired()
+            # admission of new lanes, oldest waiting request first
             while waiting and len(lanes) + len(building) < max_lanes:
-                cls = (waiting[0].state.dim, waiting[0].state.nc)
-                group = [r for r in waiting if (r.state.dim, r.state.nc) == cls][:max_trees]
-                used = (sum(l.device_bytes() for l in lanes + retired) +
-                        sum(fbytes(g) for g, _ in building))
-                if retired and used + fbytes(group[:1]) > hbm_budget:
+                key = request_class(waiting[0], big_cells)
+                cands = [r for r in waiting if request_class(r, big_cells) == key]
+                group = fit(cands, [], hbm_budget - used_bytes())
+                if not group and retired:
                     if building:
-                        break                           # release once the builders are done
-                    release()
-                    used = sum(l.device_bytes() for l in lanes)
-                while len(group) > 1 and used + fbytes(group) > hbm_budget:
-                    group = group[:-1]
-                if used + fbytes(group) > hbm_budget and (lanes or building):
-                    break                               # wait for memory to free up
+                        break                       # release once the builders are done
+                    release_retired()
+                    group = fit(cands, [], hbm_budget - used_bytes())
+                if not group:
+                    if not lanes and not building:
+                        raise RuntimeError("request %d does not fit the HBM budget" % waiting[0].rid)
+                    break                           # wait for memory to free up
                 for r in group:
                     waiting.remove(r)
-                building.append((group, pool.submit(build, group, time.perf_counter() - t0)))
+                    r.start = time.perf_counter() - t0
+                building.append((group, pool.submit(build, group)))
+                counts["lanes"] += 1
             for item in list(building):
                 if item[1].done():
                     building.remove(item)
@@ -223,45 +350,86 @@ def serve(trace, hbm_budget=64 << 30, max_lanes=4, max_trees=64, n_base=50, lane
                     time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
                 continue
             for lane in list(lanes):
-                if lane.step():
-                    t_ret = time.perf_counter() - t0
-                    lane.retire(t_ret)
+                done = lane.check()
+                lane.release(done, time.perf_counter() - t0)
+                if not any(lane.live):
                     lanes.remove(lane)
                     retired.append(lane)
-                    if lane_log is not None:
-                        lane_log.append((lane.t_adm, lane.setup, t_ret, len(lane.reqs), lane.cycles))
-    release()
+                    continue
+                if done and lane.dead_fraction() >= compact_at:
+                    keep = lane.live_members()
+                    cands = [r for r in waiting if request_class(r, big_cells) == lane.key]
+                    room = hbm_budget - used_bytes()     # the old forest stays allocated meanwhile
+                    add = fit(cands, [r.state for r in keep], room)
+                    for r in add:
+                        waiting.remove(r)
+                        r.start = time.perf_counter() - t0
+                    lane.rebuild(keep, add)
+                    counts["compactions"] += 1
+                    counts["joined"] += len(add)
+                lane.step()
+                counts["steps"] += 1
+    release_retired()
+    if log is not None:
+        log.update(counts)
+        log["rebuilds"] = counts["lanes"] + counts["compactions"]
     return trace
 
 
-def one_at_a_time(trace, n_base=50, precision="fp64", cycles=2):
+def solve_alone(r, mg, precision="fp64", cycles=2, use_graphs=True):
+    """One request in its own context through the operations a pool gives
+    it: mixed-precision corrections (afg_mixed_begin, then V-cycle
+    corrections until the residual test passes) or FP64 V-cycles from
+    phi = 0.  Sets r.cycles, r.residual and r.phi."""
+    dt = M.DeviceTree.from_state(r.state)
+    dt.set_graphs(use_graphs)
+    mixed = precision == "mixed"
+    if mixed:
+        dt.mixed_first_fmg(False)
+        M.mixed_begin(dt, mg, have_guess=True)
+    k = 0
+    while True:
+        if mixed or k > 0:
+            res = float(dt.forest_max_abs(3)[0])
+            if res <= r.tol * r.rhs_norm or k >= MAX_STEPS:
+                break
+        if mixed:
+            M.mixed_correct(dt, mg, cycles)
+        else:
+            M.mg_fas_vcycle(dt, mg, dt.highest_lvl, True)
+        k += 1
+    r.cycles, r.residual, r.phi = k, res / r.rhs_norm, dt.download(1)
+    dt.close()
+    return r
+
+
+def one_at_a_time(trace, n_base=50, precision="fp64", cycles=2, fmg=None):
     """The same trace one request after the other in arrival order, each in
-    its own context: the plain FP64 solver (FMG, then V-cycles), or the
-    mixed-precision steps of a lane."""
+    its own context.  fmg=False (the default for the served precisions):
+    the operations a pool gives the request (`solve_alone`); fmg=True, FP64
+    only: the plain FP64 solver, FMG and then V-cycles to the tolerance."""
     mg = M.default_mg(n_base=n_base)
+    fmg = False if fmg is None else fmg
     t0 = time.perf_counter()
     for r in sorted(trace, key=lambda q: q.arrival):
         now = time.perf_counter() - t0
         if now < r.arrival:
             time.sleep(r.arrival - now)
         r.start = time.perf_counter() - t0
-        dt = M.DeviceTree.from_state(r.state)
-        if precision == "mixed":
-            M.mixed_begin(dt, mg)
-        else:
+        if fmg:
+            dt = M.DeviceTree.from_state(r.state)
             M.mg_fas_fmg(dt, mg, True, False)
-        k = 0
-        while True:
-            res = float(dt.forest_max_abs(3)[0])
-            if res <= r.tol * r.rhs_norm or k >= MAX_STEPS:
-                break
-            if precision == "mixed":
-                M.mixed_correct(dt, mg, cycles)
-            else:
+            k = 0
+            while True:
+                res = float(dt.forest_max_abs(3)[0])
+                if res <= r.tol * r.rhs_norm or k >= MAX_STEPS:
+                    break
                 M.mg_fas_vcycle(dt, mg, dt.highest_lvl, True)
-            k += 1
-        r.cycles, r.residual, r.phi = k, res / r.rhs_norm, dt.download(1)
-        dt.close()
+                k += 1
+            r.cycles, r.residual, r.phi = k, res / r.rhs_norm, dt.download(1)
+            dt.close()
+        else:
+            solve_alone(r, mg, precision, cycles)
         r.finish = time.perf_counter() - t0
     return trace
 

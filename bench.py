@@ -55,7 +55,9 @@ def parse():
     ap.add_argument("--requests", type=int, default=64, help="c5: number of requests")
     ap.add_argument("--rate", type=float, default=100.0, help="c5: arrival rate (requests/s)")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
-                    help="c5: arithmetic of the served lanes")
+                    help="c5: arithmetic of the served forests")
+    ap.add_argument("--big-cells", type=float, default=0,
+                    help="c5: trees of at least this many leaf cells get forests of their own (0: no split)")
     ap.add_argument("--c4-solver", choices=["both", "fp64", "mixed"], default="both",
                     help="c4: solvers to run (one alone for profiling)")
     ap.add_argument("--mixed-cycles", type=int, default=0,
@@ -420,21 +422,39 @@ def bench_c4(a):
 
 
 def bench_c5(a):
-    """Serving: a Poisson-arrival request trace through the continuously
-    batching server, and the same trace through the plain solver one request
-    at a time."""
+    """Serving: a Poisson-arrival request trace through the server
+    (continuous batching at request granularity), the same requests offered
+    at once (the server's capacity) and at a sweep of arrival rates (latency
+    against load), and the same trace through the plain FP64 solver one
+    request at a time."""
     from afivo_amd import serving as S
     t0 = time.time()
     trace = S.make_trace(a.seed, a.requests, a.rate)
     build_s = time.time() - t0
-    S.serve(trace, precision=a.precision)
-    served = S.summary(trace)
-    steps = np.array([r.cycles for r in trace])
     cells = np.array([r.leaf_cells for r in trace])
-    for r in trace:
-        r.start = r.finish = -1.0
-        r.phi = None
-    S.one_at_a_time(trace)
+
+    print("c5: %d requests built in %.1f s" % (len(trace), build_s), file=sys.stderr, flush=True)
+
+    def run(rate, seed):
+        print("c5: serving at %.4g requests/s" % rate, file=sys.stderr, flush=True)
+        S.retime(trace, rate, seed)
+        log = {}
+        S.serve(trace, precision=a.precision, big_cells=a.big_cells, log=log)
+        out = S.summary(trace)
+        out["steps_per_request"] = [int(min(r.cycles for r in trace)), int(max(r.cycles for r in trace))]
+        out["lanes"], out["compactions"], out["joined"] = (int(log.get(k, 0)) for k in ("lanes", "compactions", "joined"))
+        return out
+
+    S.serve(S.make_trace(a.seed + 100, 4, 1e4, cells=(1e4, 1e5)), precision=a.precision)   # warm-up
+    served = run(a.rate, a.seed)   # retime with the trace's seed: its own arrival times
+    saturated = run(1e6, a.seed)
+    sweep = {}
+    for f in (0.25, 0.5, 1.0, 2.0):
+        rate = f * saturated["solves_per_s"]
+        sweep["%.0f" % rate] = run(rate, a.seed + 1)
+    print("c5: one at a time", file=sys.stderr, flush=True)
+    S.retime(trace, a.rate, a.seed)
+    S.one_at_a_time(trace, fmg=True)
     solo = S.summary(trace)
     out = {
         "metric": "solves/s and p50/p99 latency (served, %s)" % ("mixed precision" if a.precision == "mixed"
@@ -448,15 +468,21 @@ def bench_c5(a):
             "workload": "serving: %d requests, Poisson arrivals at %g/s (seed %d); 70%% 2D / 30%% 3D "
                         "trees, n_cell 8, leaf cells log-uniform in [1e4, 1e7] (uniform to the level "
                         "below the target, then two levels around 1-4 Gaussians); tolerance 1e-8 or "
-                        "1e-10 relative residual; %s; continuous batching into forests, up to 4 "
-                        "lanes, 64 trees per lane, lanes set up by 3 host threads; baseline: the plain "
-                        "FP64 solver (FMG + V-cycles), one request at a time"
+                        "1e-10 relative residual; %s; continuous batching at request granularity: lanes "
+                        "(forests of one class: dimension%s) of up to 64 trees, at most 6 lanes, admission "
+                        "by HBM budget, set up by 3 host threads; a tree leaves its lane when it converges, "
+                        "and a lane half of whose boxes have left is rebuilt with its live trees and the "
+                        "waiting ones of its class; baseline: the plain FP64 solver (FMG + V-cycles), one "
+                        "request at a time"
                         % (a.requests, a.rate, a.seed,
                            "mixed precision (FP64 defect correction, FP32 FAS corrections of 2 V-cycles)"
-                           if a.precision == "mixed" else "FP64 (FMG + V-cycles)"),
+                           if a.precision == "mixed" else "FP64 V-cycles from zero",
+                           ", and trees above %g leaf cells apart" % a.big_cells if a.big_cells else ""),
             "leaf_cells": [int(cells.min()), int(np.median(cells)), int(cells.max())],
         },
-        "served": dict(served, steps_per_request=[int(steps.min()), int(steps.max())]),
+        "served": served,
+        "saturated": saturated,
+        "latency_vs_offered_rate": sweep,
         "one_at_a_time": solo,
         "host_tree_build_s": build_s,
         "roofline": None,

@@ -146,6 +146,17 @@ int afg_set_forest(afg_ctx *ctx, int n_boxes, const int32_t *records, double dr_
  * (as after a failed afg_set_forest). */
 int afg_update_tree(afg_ctx *ctx, int n_boxes, const int32_t *records, double dr_base,
                     const int32_t *tree_of, int prolong_order, int n_vars, const int32_t *vars);
+/* Replace the forest with another one that keeps some of its trees: tree t of
+ * the new forest (records and tree_of as for afg_set_forest) is tree
+ * old_tree[t] of the current one, or new (old_tree[t] = -1).  The boxes of a
+ * kept tree keep every cell and face variable (matched by level and ix
+ * within the tree, so a kept tree may also have been refined); the boxes of
+ * new trees start from zeros, ready for afg_upload_cc.  Boundary conditions
+ * of kept physical faces stay; the others are Dirichlet zero.  For serving
+ * loops whose forests take in and release trees between solver steps.  No
+ * counterpart in the reference. */
+int afg_update_forest(afg_ctx *ctx, int n_boxes, const int32_t *records, double dr_base,
+                      const int32_t *tree_of, const int32_t *old_tree);
 /* The coordinate system of the trees of the context (the coord argument of
  * a$D_init, m_aX_core.f90:24-131; default AFG_COORD_XYZ).  AFG_COORD_CYL (2D
  * only; x is the radius, r_base the radius at the low side of ix = 1):
@@ -293,6 +304,12 @@ int afg_fas_vcycle_fp32(afg_ctx *ctx, const afg_mg_opts *opts, int highest_lvl, 
  * Forests: every tree is corrected together, each with its own stages. */
 int afg_mixed_begin(afg_ctx *ctx, const afg_mg_opts *opts, int have_guess);
 int afg_mixed_correct(afg_ctx *ctx, const afg_mg_opts *opts, int cycles);
+/* Whether the first correction after afg_mixed_begin starts with an FMG
+ * (default 1).  With 0 every correction is `cycles` V-cycles from e = 0, so
+ * the corrections a tree receives do not depend on when afg_mixed_begin was
+ * last called: afg_mixed_begin with have_guess after afg_update_forest
+ * leaves the kept trees' phi and residual bit for bit as they were. */
+int afg_mixed_first_fmg(afg_ctx *ctx, int enabled);
 /* The whole solve: afg_mixed_begin, then corrections until every tree has
  * max|rhs - A phi| <= tol * max|rhs| over its leaf interiors, or max_iter
  * corrections.  Returns the number of corrections and the largest relative

@@ -52,9 +52,9 @@ from test_forest_gpu import (  # noqa: E402,F401
     test_mixed_correct_needs_begin,
 )
 from test_serving import (  # noqa: E402,F401
-    test_mixed_serving_matches_fp64_solver,
+    test_forest_bytes_bounds_the_allocations,
+    test_served_phi_equals_solo_runs,
     test_serving_admits_by_memory_budget,
-    test_serving_meets_tolerances_and_matches_solo_runs,
 )
 from test_mixed import (  # noqa: E402,F401
     test_fp32_multigrid_tracks_fp64,
