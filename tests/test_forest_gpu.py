@@ -48,10 +48,12 @@ def test_forest_equals_single_solves(dim, n_trees, bottom):
     fdt.close()
 
 
-def test_forest_partial_vcycle_and_oracle():
+@pytest.mark.parametrize("nc", [8, 16])
+def test_forest_partial_vcycle_and_oracle(nc):
     """A V-cycle from a level below some trees' highest level runs each tree
-    from min(level, its highest); one tree checked against the oracle."""
-    sts = small_forest(2, 4, seed=11)
+    from min(level, its highest); every tree checked against the oracle, for
+    both box sizes of the batched configuration (8^2 and 16^2)."""
+    sts = small_forest(2, 4, seed=11, nc=nc)
     mg = M.default_mg(n_base=4)
     om = O.default_opts(2, 2, 4)
     fdt = M.DeviceTree.from_forest(sts)
