@@ -74,6 +74,7 @@ SIGNATURES = {
     "afg_level_ids": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, I32P, ctypes.POINTER(ctypes.c_int)]),
     "afg_tree_max_cc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, F64P]),
     "afg_tree_min_cc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, F64P]),
+    "afg_tree_extreme_noalloc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "afg_fill_gc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
     "afg_gsrb_level": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int, ctypes.c_int]),
     "afg_residual_level": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int]),

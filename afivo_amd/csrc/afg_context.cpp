@@ -174,6 +174,7 @@ struct afg_ctx {
     double *d_partial = nullptr;
     int partial_cap = 0;
     double *d_out = nullptr, *h_out = nullptr;
+    double *scratch = nullptr;           // this device's scratch array (afg_tree_extreme_noalloc)
     double *h_stage = nullptr;
     size_t stage_cap = 0;
 
@@ -1371,6 +1372,9 @@ int afg_ctx_create(int dim, int n_cell, int n_var, afg_ctx **out)
     c->d_var.assign(n_var, nullptr);
     c->d_var32.assign(n_var, nullptr);
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    // the allocation-free entry point's scratch array on this device (the
+    // current one): its address, looked up once per device
+    if (e == hipSuccess) e = scratch_address(dev, &c->scratch);
     if (e == hipSuccess) e = ctx_malloc(c, &c->d_out, 8 * sizeof(double));
     if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_out, 8 * sizeof(double), hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -2407,6 +2411,34 @@ int afg_mixed_solve(afg_ctx *c, const afg_mg_opts *o, int have_guess, double tol
     if (iterations) *iterations = it;
     if (rel_residual) *rel_residual = worst;
     return 0;
+}
+
+int afg_tree_extreme_noalloc(afg_ctx *c, int iv, int include_parents, int want_max, double *d_out)
+{
+    if (int e = check_ctx(c)) return e;
+    if (int e = check_var(c, iv)) return e;
+    if (!d_out) return c->fail(AFG_ERR_ARG, "null output");
+    if (c->capturing) return c->fail(AFG_ERR_STATE, "afg_tree_extreme_noalloc inside the library's own capture");
+    const double *v = c->d_var[iv - 1];
+    if (!v) return c->fail(AFG_ERR_STATE, "variable %d has no device data (allocating it is not allowed here)", iv);
+    if (!c->scratch) return c->fail(AFG_ERR_STATE, "no scratch array on device %d", c->device);
+    int used = 0;
+    for (int lvl = c->min_lvl; lvl <= c->highest_lvl; lvl++)
+        used += extreme_blocks(c->dim, list_args(c, lvl, include_parents ? 2 : 1));
+    if (used > AFG_NOALLOC_MAX_PARTIALS)
+        return c->fail(AFG_ERR_UNSUPPORTED, "the tree needs %d partial results; the allocation-free reduction holds %d",
+                       used, AFG_NOALLOC_MAX_PARTIALS);
+    const DevTree T = dev_tree(c);
+    int off = 0;
+    for (int lvl = c->min_lvl; lvl <= c->highest_lvl; lvl++) {
+        const ListArgs P = list_args(c, lvl, include_parents ? 2 : 1);
+        launch_extreme_list(c->dim, c->stream, T, P, v, want_max ? 1 : 0, c->scratch + off);
+        off += extreme_blocks(c->dim, P);
+    }
+    // every partial is written before it is read: nothing to clear; with no
+    // partials the reduction writes the identity, as afg_tree_max_cc returns
+    launch_reduce_partials(c->stream, c->scratch, used, want_max ? 1 : 0, d_out);
+    return launched(c, "afg_tree_extreme_noalloc");
 }
 
 int afg_forest_max_abs(afg_ctx *c, int iv, int n_trees, double *out)

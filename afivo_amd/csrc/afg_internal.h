@@ -96,6 +96,10 @@ void launch_tree_maxabs(int dim, hipStream_t s, int stride, int nc, const int32_
                         const double *var, double *partial, double *out);
 void launch_reduce_partials(hipStream_t s, const double *partial, int n, int want_max,
                             double *out);
+// The device address of the allocation-free entry points' scratch array
+// (AFG_NOALLOC_MAX_PARTIALS doubles) on `device`, which must be the current
+// device: looked up once per device (hipGetSymbolAddress) and kept.
+hipError_t scratch_address(int device, double **ptr);
 // bottom kernel: allow lds_bytes of dynamic LDS (0 on success), then launch
 int bottom_prepare(int dim, size_t lds_bytes);
 template <typename R>

@@ -3,8 +3,10 @@
 // kernels here map threads to it (one launch per stage and level over the
 // HBM pools, or one workgroup for the whole bottom of a V-cycle on LDS).
 
+#include <map>
 #include <mutex>
 
+#include "afivo_amd.h"
 #include "afg_internal.h"
 #include "afg_bodies.h"
 
@@ -764,6 +766,34 @@ void launch_set_tags(int dim, hipStream_t s, const DevTree &T, const LevelArgs &
 {
     if (L.n <= 0) return;
     k_set_tags<<<L.n, BLOCK, 0, s>>>(T, L.slot0, ipow(L.nc + 2, dim), eps, lsf);
+}
+
+// Scratch memory of the allocation-free reduction (afg_tree_extreme_noalloc):
+// the per-block partial results.  One copy per device, held until the
+// process ends.  Host code never uses this name as an address: it is a
+// host-side stand-in there; scratch_address gives the device address.
+static __device__ __align__(256) double scratch[AFG_NOALLOC_MAX_PARTIALS];
+
+hipError_t scratch_address(int device, double **ptr)
+{
+    static std::mutex mu;
+    static std::map<int, double *> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(device);
+    if (it != addr.end()) {
+        *ptr = it->second;
+        return hipSuccess;
+    }
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice;
+    double *p = nullptr;
+    e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[device] = p;
+    *ptr = p;
+    return hipSuccess;
 }
 
 int extreme_blocks(int dim, const ListArgs &P)

@@ -259,6 +259,25 @@ int afg_level_ids(const afg_ctx *ctx, int lvl, int n, int32_t *ids, int *n_out);
 int afg_tree_max_cc(afg_ctx *ctx, int iv, int include_parents, double *out);
 int afg_tree_min_cc(afg_ctx *ctx, int iv, int include_parents, double *out);
 
+/* Largest number of per-block partial results the allocation-free reduction
+ * below holds (doubles of its scratch array, 128 KiB per device). */
+#define AFG_NOALLOC_MAX_PARTIALS 16384
+/* Allocation-free variant of afg_tree_max_cc (want_max = 1) and
+ * afg_tree_min_cc (want_max = 0): the same value, bit for bit, written to
+ * d_out[0] in device memory, stream-ordered on the context's stream (no
+ * synchronisation, no host copy).  It calls no hipMalloc or hipFree and
+ * nothing that allocates: the per-block partial results go to a scratch
+ * array of AFG_NOALLOC_MAX_PARTIALS doubles that the library holds on each
+ * device until the process ends.  Trees that need more partials are
+ * rejected (AFG_ERR_UNSUPPORTED), and so is a variable that has no device
+ * data yet (AFG_ERR_STATE: creating it would allocate).  Every call on a
+ * device shares that device's scratch array, which keeps the last call's
+ * partials: calls that overlap on one device (several contexts, streams or
+ * host threads at once) race; calls in one stream do not.  No counterpart
+ * in the reference: it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). */
+int afg_tree_extreme_noalloc(afg_ctx *ctx, int iv, int include_parents, int want_max, double *d_out);
+
 /* Single stages over one level (the building blocks of the cycles above,
  * exported for stage-level checks and for callers that drive their own
  * cycles). */

@@ -8,6 +8,7 @@
 // arithmetic rounds like the device's, so results compare bit for bit.
 #define AFG_DEV inline
 #define AFG_HD
+#include "afivo_amd.h"
 #include "afg_internal.h"
 #include "afg_bodies.h"
 #include "emul.h"
@@ -555,6 +556,14 @@ void launch_reduce_partials(hipStream_t s, const double *partial, int n, int wan
 }
 
 int bottom_prepare(int, size_t lds_bytes) { return lds_bytes <= 163840 ? 0 : -1; }
+
+// host stand-in of the device scratch array of the allocation-free entry point
+hipError_t scratch_address(int, double **ptr)
+{
+    static double *p = (double *)emul_alloc(AFG_NOALLOC_MAX_PARTIALS * sizeof(double));
+    *ptr = p;
+    return p ? hipSuccess : hipErrorMemoryAllocation;
+}
 
 template <typename R>
 void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVarsT<R> &V,
