@@ -88,6 +88,7 @@ SIGNATURES = {
     "afg_fas_vcycle_fp32": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int, ctypes.c_int]),
     "afg_set_pass_pairs": (ctypes.c_int, [P, ctypes.c_int]),
     "afg_get_pass_pairs": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "afg_set_level_pass_pairs": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "afg_set_tile": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "afg_get_tile": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "afg_get_bottom_cells": (ctypes.c_int, [P, ctypes.c_int, I64P]),

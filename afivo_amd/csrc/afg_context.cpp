@@ -2755,6 +2755,17 @@ int afg_set_pass_pairs(afg_ctx *c, int enabled)
     return 0;
 }
 
+int afg_set_level_pass_pairs(afg_ctx *c, int lvl, int fp32, int enabled)
+{
+    if (int e = check_ctx(c)) return e;
+    if (int e = check_level(c, lvl, false)) return e;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->levels[lvl - c->min_lvl].pairs[fp32 != 0] = enabled != 0;
+    for (auto &g : c->graphs) destroy_graph(g.second);
+    c->graphs.clear();
+    return 0;
+}
+
 int afg_get_pass_pairs(const afg_ctx *c, int lvl, int fp32, int *enabled)
 {
     if (!c || !c->have_tree || !enabled || lvl < c->min_lvl || lvl > c->highest_lvl) return AFG_ERR_ARG;

@@ -216,6 +216,10 @@ class DeviceTree:
         self._c(_lib.load().afg_get_pass_pairs(self.h, int(lvl), int(fp32), ctypes.byref(v)))
         return bool(v.value)
 
+    def set_level_pass_pairs(self, lvl, enabled, fp32=False):
+        """Fused pass pairs (True) or single passes on one level."""
+        self._c(_lib.load().afg_set_level_pass_pairs(self.h, int(lvl), int(fp32), 1 if enabled else 0))
+
     def bottom_cells(self, fp32=False):
         v = ctypes.c_int64()
         self._c(_lib.load().afg_get_bottom_cells(self.h, int(fp32), ctypes.byref(v)))

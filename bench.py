@@ -64,26 +64,56 @@ def parse():
                     help="c4: FP32 V-cycles per correction (0: try 1, 2 and 3)")
     ap.add_argument("--trees", type=int, default=256, help="c3: number of trees")
     ap.add_argument("--tol", type=float, default=1e-10, help="c3, c4: relative residual to reach")
+    ap.add_argument("--tiles", default="",
+                    help="c1, c2: tile shape per level from the lowest, comma-separated (with --pairs and "
+                         "--bottom-cells: this configuration instead of the autotuner's)")
+    ap.add_argument("--pairs", default="", help="c1, c2: fused pass pairs per level (0/1), comma-separated")
+    ap.add_argument("--bottom-cells", type=int, default=-1, help="c1, c2: the bottom kernel's cell limit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy rhs to tmp over every box once, from cold caches (for the "
                          "HBM byte counters' scale factor), print its byte count and exit")
     return ap.parse_args()
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_gsrb_tile_c2_traffic.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_gsrb_c2_level7_traffic.md")
 
 
-def measured_traffic():
-    """HBM bytes per launch of the finest-level GSRB pass, from the committed
-    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, scaled by the
-    factors a plain copy of known size gives); None when absent."""
+def measured_traffic(instances):
+    """HBM bytes per launch of the finest-level colour passes: the committed
+    rocprofv3 counter summary's figure (FETCH_SIZE and WRITE_SIZE, scaled by
+    the factors a plain copy of known size gives) for each kernel instance
+    that ran, `traffic[<instance>] = <bytes>`, averaged with the instances'
+    launch counts; None when an instance has no figure there."""
     import re
     try:
         text = open(TRAFFIC_FILE).read()
     except OSError:
         return None
-    m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", text)
-    return float(m.group(1)) if m else None
+    table = {m.group(1).strip(): float(m.group(2))
+             for m in re.finditer(r"traffic\[([^\]]+)\]\s*=\s*([0-9.eE+]+)", text)}
+    tot = n = 0
+    for name, count in instances.items():
+        if name not in table:
+            return None
+        tot += table[name] * count
+        n += count
+    return tot / n if n else None
+
+
+def pass_instances(dim, nc, shape, pairs, n_down, n_up):
+    """The colour-pass kernel instances of a level per V-cycle and how often
+    each runs: a down block whose first pass reads the stored ghosts, an up
+    block that computes them all (after the correction); fused pairs take
+    two passes per launch."""
+    kern = "k_gsrb_pair" if pairs else "k_gsrb_tile"
+    per = 2 if pairs else 1
+    first = 1
+    rest = (2 * n_down) // per - 1 + (2 * n_up) // per
+    out = {}
+    if first:
+        out["%s<%d, %d, 0, double, %d>" % (kern, dim, nc, shape)] = first
+    out["%s<%d, %d, 1, double, %d>" % (kern, dim, nc, shape)] = rest
+    return out
 
 
 def host_cpu():
@@ -693,10 +723,25 @@ def main():
         dt.close()
         return 0
 
-    # the autotuner picks the tile shape of the colour pass per level (on a
-    # copy of phi, before timing)
-    dt.autotune(mg)
-    tiles = [dt.tile(lvl) for lvl in range(dt.min_lvl, dt.highest_lvl + 1)]
+    # the autotuner picks the tile shape and pass mode of the colour passes
+    # per level and the bottom kernel's limit (on a copy of phi, before
+    # timing), unless they are given
+    lvls = list(range(dt.min_lvl, dt.highest_lvl + 1))
+    if a.tiles or a.pairs or a.bottom_cells >= 0:
+        if a.tiles:
+            for lvl, v in zip(lvls, a.tiles.split(",")):
+                dt.set_tile(lvl, int(v))
+        if a.pairs:
+            for lvl, v in zip(lvls, a.pairs.split(",")):
+                dt.set_level_pass_pairs(lvl, int(v) != 0)
+        if a.bottom_cells >= 0:
+            dt.set_bottom_cells(a.bottom_cells)
+        tuned = False
+    else:
+        dt.autotune(mg)
+        tuned = True
+    tiles = [dt.tile(lvl) for lvl in lvls]
+    pairs = [int(dt.pass_pairs(lvl)) for lvl in lvls]
     bottom = dt.bottom_cells()
     # cell-updates of one solve: FMG runs a V-cycle up to every level
     cu_solve = sum(dt.vcycle_cell_updates(mg, lvl) for lvl in range(dt.min_lvl, dt.highest_lvl + 1))
@@ -755,7 +800,9 @@ def main():
             "levels": [int(st.min_lvl), int(st.highest_lvl)],
             "cell_updates_per_step": cu_solve,
             "tile_shapes": tiles,
+            "pass_pairs": pairs,
             "bottom_cells": bottom,
+            "autotuned": tuned,
         },
         "solves_per_s": a.steps / elapsed,
         "rel_residual_after_step": rel_res,
@@ -764,13 +811,19 @@ def main():
     if n_launch and k_ms > 0:
         achieved = k_bytes / (k_ms * 1e-3) / 1e9
         n_top = int((st.lvl == st.highest_lvl).sum())
+        inst = pass_instances(dim, st.nc, tiles[-1], pairs[-1], mg.n_cycle_down, mg.n_cycle_up)
         out["roofline"] = {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            # counter traffic is profiled for the 3D kernel only
-            "traffic": measured_traffic() if dim == 3 else None,
-            "kernel": "k_gsrb_tile<%d,8> or k_gsrb_pair<%d,8> (GSRB colour passes) on the finest level (%d, %d boxes)"
-                      % (dim, dim, st.highest_lvl, n_top),
+            # counter traffic is profiled for the 3D tree's finest level only
+            "traffic": measured_traffic(inst) if a.workload == "c2" else None,
+            "kernel": "%s (GSRB colour passes) on the finest level (%d, %d boxes)"
+                      % (" and ".join(inst), st.highest_lvl, n_top),
+            "instances_per_vcycle": inst,
+            # the whole step at the pass's algorithmic bytes per cell-update,
+            # (3 + 2D/nc) w: every level's relaxation counted, nothing else
+            "whole_step_alg_GBs": cu_solve * (3 + 2 * dim / st.nc) * 8 * a.steps / elapsed / 1e9,
+            "whole_step_frac": cu_solve * (3 + 2 * dim / st.nc) * 8 * a.steps / elapsed / 1e9 / HBM_PEAK_GBS,
             "launches": n_launch, "avg_launch_us": 1e3 * k_ms / n_launch,
             "alg_bytes_per_launch": k_bytes / n_launch,
             "share_of_step_time": (k_ms * 1e-3) / elapsed,

@@ -371,6 +371,8 @@ int afg_autotune(afg_ctx *ctx, const afg_mg_opts *opts, int fp32, int reps, doub
  * either way. */
 int afg_set_pass_pairs(afg_ctx *ctx, int enabled);
 int afg_get_pass_pairs(const afg_ctx *ctx, int lvl, int fp32, int *enabled);
+/* The same for one level and precision (what afg_autotune chooses per level). */
+int afg_set_level_pass_pairs(afg_ctx *ctx, int lvl, int fp32, int enabled);
 /* Replay whole V-cycles from HIP graphs (default on). */
 int afg_set_graphs(afg_ctx *ctx, int enabled);
 /* Per-stage device timing: while enabled, each launch of the GSRB colour-pass
