@@ -1900,7 +1900,10 @@ int afg_update_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_ba
 
 int afg_set_bc(afg_ctx *c, int id, int nb, int bc_type, const double *b)
 {
-    if (int e = check_ctx(c)) return e;
+    // host tables only (no HIP call): a caller sets every physical face
+    // before each solve; the table is sent when it changed
+    if (!c) return AFG_ERR_ARG;
+    if (!c->have_tree) return c->fail(AFG_ERR_STATE, "no tree set (call afg_set_tree first)");
     if (id < 1 || id > c->max_id || c->id2slot[id] < 0) return c->fail(AFG_ERR_ARG, "no box with id %d", id);
     if (nb < 1 || nb > c->nnb) return c->fail(AFG_ERR_ARG, "neighbour direction %d out of range", nb);
     if (bc_type != AFG_BC_DIRICHLET && bc_type != AFG_BC_NEUMANN)
@@ -1909,25 +1912,26 @@ int afg_set_bc(afg_ctx *c, int id, int nb, int bc_type, const double *b)
     const int rec = c->bc_of[(size_t)s * c->nnb + nb - 1];
     if (rec < 0) return c->fail(AFG_ERR_ARG, "box %d direction %d is not a physical boundary", id, nb);
     BcRec &r = c->bc[rec];
+    bool changed = r.type != bc_type;
     r.type = bc_type;
-    if (b) {
-        const int n = c->slot_nc[s];
-        const int m = c->dim == 2 ? n : n * n;
-        bool zero = true;
+    const int n = c->slot_nc[s];
+    const int m = c->dim == 2 ? n : n * n;
+    bool zero = true;
+    if (b)
         for (int k = 0; k < m; k++) zero = zero && b[k] == 0.0 && !std::signbit(b[k]);
-        if (zero) {
-            r.voff = -1;
-        } else {
-            if (r.voff < 0) {
-                r.voff = (int)c->bc_vals.size();
-                c->bc_vals.resize(c->bc_vals.size() + m);
-            }
-            std::memcpy(&c->bc_vals[r.voff], b, m * sizeof(double));
-        }
-    } else {
+    if (zero) {
+        changed = changed || r.voff >= 0;
         r.voff = -1;
+    } else {
+        if (r.voff < 0) {
+            r.voff = (int)c->bc_vals.size();
+            c->bc_vals.resize(c->bc_vals.size() + m);
+            changed = true;
+        }
+        changed = changed || std::memcmp(&c->bc_vals[r.voff], b, m * sizeof(double)) != 0;
+        std::memcpy(&c->bc_vals[r.voff], b, m * sizeof(double));
     }
-    c->bc_dirty = true;
+    if (changed) c->bc_dirty = true;
     return 0;
 }
 

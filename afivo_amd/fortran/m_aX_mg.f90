@@ -135,6 +135,9 @@ module m_a$D_mg
   integer(c_int32_t), allocatable, save :: recs_dev(:)
   integer(c_int32_t), allocatable, save :: ids_dev(:)
   integer(c_int64_t), save :: n_values = 0
+  ! packed cell data of one variable on its way to or from the device, kept
+  ! between calls (a new buffer per call would fault in its pages each time)
+  real(c_double), allocatable, save :: stage(:)
 
   integer, parameter :: rec_len = 3 + $D + 2**$D + 2*$D
 
@@ -343,6 +346,7 @@ contains
     ctx_coord = -1
     if (allocated(recs_dev)) deallocate(recs_dev)
     if (allocated(ids_dev)) deallocate(ids_dev)
+    if (allocated(stage)) deallocate(stage)
   end subroutine mg$D_device_release
 
   !> Map the options to the device, (re)build the device tree when its
@@ -480,46 +484,74 @@ contains
     box_len = (box%n_cell + 2)**$D
   end function box_len
 
+  subroutine reserve_stage(n)
+    integer, intent(in) :: n
+    if (allocated(stage)) then
+       if (size(stage) >= n) return
+       deallocate(stage)
+    end if
+    allocate(stage(n))
+  end subroutine reserve_stage
+
+  !> box%cc(..., iv) of every box into the packed layout (one contiguous
+  !> column of nc+2 values at a time: no temporary arrays), then to the device
   subroutine upload_var(tree, iv)
     type(a$D_t), intent(in) :: tree
     integer, intent(in)     :: iv
-    real(c_double), allocatable :: buf(:)
-    integer :: k, id, pos, len
+    integer :: k, id, pos, m, j
+#if $D == 3
+    integer :: kk
+#endif
 
-    allocate(buf(total_values(tree)))
+    call reserve_stage(total_values(tree))
     pos = 0
     do k = 1, size(ids_dev)
        id = ids_dev(k)
-       len = box_len(tree%boxes(id))
+       m = tree%boxes(id)%n_cell + 2
 #if $D == 2
-       buf(pos+1:pos+len) = reshape(tree%boxes(id)%cc(:, :, iv), [len])
+       do j = 0, m - 1
+          stage(pos+1:pos+m) = tree%boxes(id)%cc(:, j, iv)
+          pos = pos + m
+       end do
 #elif $D == 3
-       buf(pos+1:pos+len) = reshape(tree%boxes(id)%cc(:, :, :, iv), [len])
+       do kk = 0, m - 1
+          do j = 0, m - 1
+             stage(pos+1:pos+m) = tree%boxes(id)%cc(:, j, kk, iv)
+             pos = pos + m
+          end do
+       end do
 #endif
-       pos = pos + len
     end do
-    call afg_check(afg_upload_cc(ctx, iv, size(ids_dev), ids_dev, buf), ctx, "afg_upload_cc")
+    call afg_check(afg_upload_cc(ctx, iv, size(ids_dev), ids_dev, stage), ctx, "afg_upload_cc")
   end subroutine upload_var
 
   subroutine download_var(tree, iv)
     type(a$D_t), intent(inout) :: tree
     integer, intent(in)        :: iv
-    real(c_double), allocatable :: buf(:)
-    integer :: k, id, pos, len, n
+    integer :: k, id, pos, m, j
+#if $D == 3
+    integer :: kk
+#endif
 
-    allocate(buf(total_values(tree)))
-    call afg_check(afg_download_cc(ctx, iv, size(ids_dev), ids_dev, buf), ctx, "afg_download_cc")
+    call reserve_stage(total_values(tree))
+    call afg_check(afg_download_cc(ctx, iv, size(ids_dev), ids_dev, stage), ctx, "afg_download_cc")
     pos = 0
     do k = 1, size(ids_dev)
        id = ids_dev(k)
-       n = tree%boxes(id)%n_cell
-       len = box_len(tree%boxes(id))
+       m = tree%boxes(id)%n_cell + 2
 #if $D == 2
-       tree%boxes(id)%cc(:, :, iv) = reshape(buf(pos+1:pos+len), [n+2, n+2])
+       do j = 0, m - 1
+          tree%boxes(id)%cc(:, j, iv) = stage(pos+1:pos+m)
+          pos = pos + m
+       end do
 #elif $D == 3
-       tree%boxes(id)%cc(:, :, :, iv) = reshape(buf(pos+1:pos+len), [n+2, n+2, n+2])
+       do kk = 0, m - 1
+          do j = 0, m - 1
+             tree%boxes(id)%cc(:, j, kk, iv) = stage(pos+1:pos+m)
+             pos = pos + m
+          end do
+       end do
 #endif
-       pos = pos + len
     end do
   end subroutine download_var
 
@@ -533,13 +565,17 @@ contains
   end function total_values
 
   !> Boundary table: for every physical face, the type and values the user's
-  !> sides_bc writes (built-in zero conditions need no call)
+  !> sides_bc writes (built-in zero conditions need no call).  sides_bc is
+  !> called on the tree's own box, as the reference's ghost fill calls it
+  !> (m_aX_gc.f90:58-67): it writes the boundary values into the face's
+  !> ghost cells of phi, which are read back here; the ghost fill on the
+  !> device turns them into ghost values, and the download of phi after the
+  !> solve overwrites them with those.
   subroutine set_boundaries(tree, mg)
     use m_a$D_gc, only: a$D_bc_dirichlet_zero, a$D_bc_neumann_zero
-    type(a$D_t), intent(in)  :: tree
-    type(mg$D_t), intent(in) :: mg
-    type(box$D_t)            :: scratch
-    integer                  :: k, id, nb, bc_type, n
+    type(a$D_t), intent(inout) :: tree
+    type(mg$D_t), intent(in)   :: mg
+    integer                    :: k, id, nb, bc_type
     real(c_double), allocatable, target :: b(:)
 
     do k = 1, size(ids_dev)
@@ -551,10 +587,8 @@ contains
           else if (associated(mg%sides_bc, a$D_bc_neumann_zero)) then
              call afg_check(afg_set_bc(ctx, id, nb, AFG_BC_NEUMANN, c_null_ptr), ctx, "afg_set_bc")
           else
-             scratch = tree%boxes(id)
-             call mg%sides_bc(scratch, nb, mg%i_phi, bc_type)
-             n = scratch%n_cell
-             call face_values(scratch, nb, mg%i_phi, b)
+             call mg%sides_bc(tree%boxes(id), nb, mg%i_phi, bc_type)
+             call face_values(tree%boxes(id), nb, mg%i_phi, b)
              call afg_check(afg_set_bc(ctx, id, nb, bc_type, c_loc(b)), ctx, "afg_set_bc")
           end if
        end do

@@ -81,12 +81,23 @@ program ref_harness_$Dd
   logical            :: tag_boxes = .true.   ! with rstr_auto: mg$D_set_box_tag on every box first
   real(dp)           :: lsf_c($D) = 0.5_dp
   real(dp)           :: lsf_rad = 0.25_dp
+  ! builds against the device drop-in only: keep the cell data on the device
+  ! between calls (mg%device_resident), copying it back after each solve
+  logical            :: resident = .false.
 
+#ifdef AFG_DROPIN
+  namelist /cfg/ nc, ref_mode, l_uniform, l_max, thr, n_gauss, ampl, sigma, &
+       r0, list_file, n_down, n_up, n_base, n_fmg, n_vcycle, use_eps, eps_hi, &
+       eps_cut, out_file, dump_input, dump_fmg, dump_output, n_time, refine2, &
+       refine2_lmax, refine2_thr, refine2_rm_thr, refine2_nvar, xfer, cyl, &
+       lsf_mode, rstr_auto, tag_boxes, lsf_c, lsf_rad, resident
+#else
   namelist /cfg/ nc, ref_mode, l_uniform, l_max, thr, n_gauss, ampl, sigma, &
        r0, list_file, n_down, n_up, n_base, n_fmg, n_vcycle, use_eps, eps_hi, &
        eps_cut, out_file, dump_input, dump_fmg, dump_output, n_time, refine2, &
        refine2_lmax, refine2_thr, refine2_rm_thr, refine2_nvar, xfer, cyl, &
        lsf_mode, rstr_auto, tag_boxes, lsf_c, lsf_rad
+#endif
 
   type(a$D_t)         :: tree, tree_init
   type(ref_info_t)    :: ref_info
@@ -158,6 +169,9 @@ program ref_harness_$Dd
   end if
   if (rstr_auto) mg%box_rstr => mg$D_auto_rstr
   call mg$D_init_mg(mg)
+#ifdef AFG_DROPIN
+  mg%device_resident = resident
+#endif
   if (rstr_auto .and. tag_boxes) call a$D_loop_box(tree, set_tag)
 
   if (dump_input) call write_dump(trim(out_file) // '.in')
@@ -165,6 +179,9 @@ program ref_harness_$Dd
   if (n_time > 0) tree_init = tree
 
   call solve(.true.)
+#ifdef AFG_DROPIN
+  if (resident) call mg$D_to_host(tree, mg)
+#endif
 
   if (dump_output) call write_dump(trim(out_file) // '.out')
 
@@ -189,6 +206,9 @@ program ref_harness_$Dd
         tree = tree_init
         t0 = omp_get_wtime()
         call solve(.false.)
+#ifdef AFG_DROPIN
+        if (resident) call mg$D_to_host(tree, mg)
+#endif
         t1 = omp_get_wtime()
         t_runs(i) = t1 - t0
      end do

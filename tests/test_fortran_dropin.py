@@ -173,3 +173,38 @@ def test_dropin_auto_rstr_without_tags_stops_like_reference(tmp_path):
     rc, out, prefix = run_dropin(2, cfg, str(gpu_dir))
     assert msg in out
     assert not os.path.exists(prefix + ".out")
+
+
+@pytest.mark.gpu
+@needs_build
+def test_dropin_end_to_end_on_the_3d_benchmark_tree(tmp_path):
+    """The reference's harness program through the drop-in on the 3D
+    benchmark tree (5875 boxes of 8^3, FMG + 8 V-cycles, 50 base sweeps),
+    in the default mode (every mg3_fas_fmg / mg3_fas_vcycle call uploads phi,
+    rhs and tmp from box%cc and downloads them) and device-resident
+    (mg%device_resident, one mg3_to_host after each solve): both write the
+    reference build's tree bit for bit, and the default mode its residual
+    history (the harness reads it from box%cc after each call, which the
+    resident mode leaves to mg3_to_host); the solve times
+    (5 timed solves each) give the share of the host copies."""
+    from afivo_amd import problems as P
+    cfg = P.harness_cfg(P.c2(), n_time=5, dump_input=False)
+    prefix_ref, _ = O.run_ref(3, dict(cfg, out_file="ref", n_time=0), workdir=str(tmp_path), threads=8)
+    ref = read_tree_file(prefix_ref + ".out")
+    times = {}
+    for resident in (False, True):
+        d = tmp_path / ("resident" if resident else "default")
+        d.mkdir()
+        rc, out, prefix = run_dropin(3, dict(cfg, resident=resident), str(d))
+        assert rc == 0, out
+        got = read_tree_file(prefix + ".out")
+        assert np.array_equal(got.recs, ref.recs)
+        for iv in range(1, ref.n_var + 1):
+            assert_bitexact(got.data[iv - 1], ref.data[iv - 1], f"resident={resident} var {iv}")
+        if not resident:   # the harness reads its residual history from box%cc
+            assert np.array_equal(O.read_res(prefix), O.read_res(prefix_ref))
+        times[resident] = float(np.median(O.read_time(prefix)[1]))
+    share = 1 - times[True] / times[False]
+    print(f"drop-in, 3D benchmark tree: {1e3 * times[False]:.2f} ms per solve with host copies on every call, "
+          f"{1e3 * times[True]:.2f} ms device-resident; host copies {100 * share:.0f}% of the default mode")
+    assert times[True] < times[False]
