@@ -1746,7 +1746,8 @@ AFG_DEV void bottom_nc_dispatch(int nc, F &&f)
 // i = (c & 1) + 1, j = ((c >> 1) & 1) + 1, k = (c >> 2) + 1; its
 // descriptors are in the cell list of its colour at the rank it has among
 // the box's cells of that colour.
-template <int DIM, int AUTO, typename R, typename Xshfl>
+// W is the arithmetic type: R, or double for an FP32 level relaxed in FP64.
+template <int DIM, int AUTO, typename R, typename W, typename Xshfl>
 AFG_DEV void base_wave_run(const DevTree &T, const LevelViewT<R> &own, const BottomLevels &B, const BottomArgs &A,
                            double dx2, int n_box, int lane, Xshfl xshfl)
 {
@@ -1760,61 +1761,61 @@ AFG_DEV void base_wave_run(const DevTree &T, const LevelViewT<R> &own, const Bot
     for (int g0 = 0; g0 < n_box; g0 += PER_WAVE) {
         const int b = g0 + lane / NCL;
         const bool on = b < n_box;
-        R ph = 0, rh = 0, K[3] = {0, 0, 0}, cm[6] = {0, 0, 0, 0, 0, 0}, sc = 1;
+        W ph = 0, rh = 0, K[3] = {0, 0, 0}, cm[6] = {0, 0, 0, 0, 0, 0}, sc = 1;
         int pos = 0, neg = 0;
         bool lpld = false;
         if (on) {
             const BaseCell &d = B.base[(odd ? 0 : B.base_n[0]) + b * H + rank];
             pos = d.pos;
-            ph = own.phi[pos];
-            rh = own.rhs[pos];
+            ph = W(own.phi[pos]);
+            rh = W(own.rhs[pos]);
 #pragma unroll
             for (int dn = 0; dn < DIM; dn++) {
                 const int m = 2 * dn + ((c >> dn) & 1);   // the face of cell c on the box boundary
-                K[dn] = R(d.K[m]);
+                K[dn] = W(d.K[m]);
                 neg |= (d.fly[m] == -1 ? 1 : 0) << dn;
             }
             lpld = lpld_box(T, AUTO, d.slot);
             if (lpld) {
                 const R *e = own.eps + pos;
-                const R a0 = e[0];
+                const W a0 = W(e[0]);
 #pragma unroll
                 for (int m = 0; m < 2 * DIM; m++) {
-                    const R a = e[dq[m]];
+                    const W a = W(e[dq[m]]);
                     cm[m] = 2 * a0 * a / (a0 + a);
                     sc = m == 0 ? cm[m] : sc + cm[m];
                 }
             }
         }
         for (int rb = 1; rb <= 2 * A.n_base; rb++) {
-            R u[6];
+            W u[6];
 #pragma unroll
             for (int dn = 0; dn < DIM; dn++) {
-                const R other = xshfl(ph, 1 << dn);   // every lane takes part
-                const R ghost = K[dn] + (((neg >> dn) & 1) ? -ph : ph);
+                const W other = xshfl(ph, 1 << dn);   // every lane takes part
+                const W ghost = K[dn] + (((neg >> dn) & 1) ? -ph : ph);
                 const int bit = (c >> dn) & 1;
 #pragma unroll
                 for (int high = 0; high < 2; high++) {
                     const int m = 2 * dn + high;
                     if (rb == 1)
-                        u[m] = on ? own.phi[pos + dq[m]] : R(0);
+                        u[m] = on ? W(own.phi[pos + dq[m]]) : W(0);
                     else
                         u[m] = bit != high ? other : ghost;
                 }
             }
             if (DIM == 2) u[4] = u[5] = 0;
-            R nv;
+            W nv;
             if (lpld) {
-                R su = 0;
+                W su = 0;
 #pragma unroll
                 for (int m = 0; m < 2 * DIM; m++) su = m == 0 ? cm[m] * u[m] : su + cm[m] * u[m];
-                nv = (su - R(dx2) * rh) / sc;
+                nv = (su - W(dx2) * rh) / sc;
             } else {
-                nv = gsrb_value<DIM>(u, rh, (const R *)nullptr, 0, S, dx2, false);
+                nv = gsrb_value<DIM>(u, rh, (const W *)nullptr, 0, S, dx2, false);
             }
             if (odd == (rb & 1)) ph = nv;
         }
-        if (on) own.phi[pos] = ph;
+        if (on) own.phi[pos] = R(ph);
     }
 }
 
@@ -1918,7 +1919,13 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLeve
                 const LevelArgs L = B.lv[0];
                 if constexpr (NC == 2) {
                     if (B.base_fast && wave == 64) {   // a full wavefront: the lanes of the first
-                        if (t < 64) base_wave_run<DIM, AUTO>(T, own, B, A, L.dx2, L.n, t, xshfl);
+                        if constexpr (sizeof(R) == 4) {
+                            if (A.base64) {
+                                if (t < 64) base_wave_run<DIM, AUTO, R, double>(T, own, B, A, L.dx2, L.n, t, xshfl);
+                                break;
+                            }
+                        }
+                        if (t < 64) base_wave_run<DIM, AUTO, R, R>(T, own, B, A, L.dx2, L.n, t, xshfl);
                         break;
                     }
                 }

@@ -116,6 +116,7 @@ struct afg_ctx {
     // mixed-precision solve in steps: begun, and the next correction is the first (an FMG)
     bool mixed_ready = false, mixed_fmg = false;
     bool mixed_first_fmg = true;         // the first correction after afg_mixed_begin is an FMG
+    bool mixed_base64 = false;           // FP32 stages: the coarsest-grid relaxation in FP64 arithmetic
     bool tags_set = false;               // every box has its eps tag (they are set once)
 
     // tree
@@ -848,6 +849,7 @@ BottomArgs bottom_args(MgT<R> &m, int top)
     B.n_down = m.o->n_cycle_down;
     B.n_up = m.o->n_cycle_up;
     B.n_base = m.o->n_cycle_base;
+    B.base64 = sizeof(R) == 4 && c->mixed_base64 ? 1 : 0;
     return B;
 }
 
@@ -2379,6 +2381,16 @@ int afg_mixed_correct(afg_ctx *c, const afg_mg_opts *o, int cycles)
     launch_from_fp32(c->stream, n, f.V.phi, m.V.phi, 1);
     mixed_residual(m);
     return launched(c, "afg_mixed_correct");
+}
+
+int afg_mixed_fp64_base(afg_ctx *c, int enabled)
+{
+    if (!c) return AFG_ERR_ARG;
+    if (c->stream) HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->mixed_base64 = enabled != 0;
+    for (auto &g : c->graphs) destroy_graph(g.second);   // the flag is a kernel argument
+    c->graphs.clear();
+    return 0;
 }
 
 int afg_mixed_first_fmg(afg_ctx *c, int enabled)

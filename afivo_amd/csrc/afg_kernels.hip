@@ -438,7 +438,7 @@ __global__ __launch_bounds__(BOTTOM_BLOCK) void k_bottom(DevTree T, MgVarsT<R> V
     };
     bottom_run<DIM, AUTO, R>(T, V, *B, A, lds, threadIdx.x, blockDim.x, warpSize,
                              [] { __syncthreads(); }, wave_sync,
-                             [](R v, int mask) { return __shfl_xor(v, mask); });
+                             [](auto v, int mask) { return __shfl_xor(v, mask); });
 }
 
 template <int DIM, typename R>

@@ -163,6 +163,9 @@ struct BottomLevels {
 struct BottomArgs {
     int hl;                       // highest level of this call (relative index)
     int n_down, n_up, n_base;
+    // FP32 stages: the relaxation of the lowest level (the coarsest-grid
+    // solve) in FP64 arithmetic on the FP32 values
+    int base64 = 0;
 };
 
 }  // namespace afg

@@ -148,6 +148,11 @@ class DeviceTree:
         self.dr_base = dr
         self._tree_info()
 
+    def mixed_fp64_base(self, enabled):
+        """The FP32 stages' coarsest-grid relaxation in FP64 arithmetic
+        (afg_mixed_fp64_base)."""
+        self._c(_lib.load().afg_mixed_fp64_base(self.h, 1 if enabled else 0))
+
     def mixed_first_fmg(self, enabled):
         """Whether the first mixed-precision correction after mixed_begin
         starts with an FMG (afg_mixed_first_fmg; default True)."""

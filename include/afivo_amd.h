@@ -329,6 +329,13 @@ int afg_mixed_correct(afg_ctx *ctx, const afg_mg_opts *opts, int cycles);
  * last called: afg_mixed_begin with have_guess after afg_update_forest
  * leaves the kept trees' phi and residual bit for bit as they were. */
 int afg_mixed_first_fmg(afg_ctx *ctx, int enabled);
+/* The FP32 stages' coarsest-grid solve (the n_cycle_base sweeps on the
+ * lowest level) in FP64 arithmetic on the FP32 values, kept in registers
+ * across the sweeps (default 0: FP32).  It applies where the lowest level's
+ * boxes have 2^dim cells and physical boundaries on every face (one or more
+ * base boxes of the coarsened tree, as afivo builds them with coarsen_to = 2);
+ * elsewhere the coarsest solve stays FP32. */
+int afg_mixed_fp64_base(afg_ctx *ctx, int enabled);
 /* The whole solve: afg_mixed_begin, then corrections until every tree has
  * max|rhs - A phi| <= tol * max|rhs| over its leaf interiors, or max_iter
  * corrections.  Returns the number of corrections and the largest relative

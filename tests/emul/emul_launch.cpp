@@ -573,14 +573,14 @@ void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVarsT<R> &V
         double *lds = (double *)emul_alloc(lds_bytes ? lds_bytes : 8);
         if (dim == 2) {
             if (V.op_auto)
-                bottom_run<2, 1, R>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {}, [](R v, int) { return v; });
+                bottom_run<2, 1, R>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {}, [](auto v, int) { return v; });
             else
-                bottom_run<2, 0, R>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {}, [](R v, int) { return v; });
+                bottom_run<2, 0, R>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {}, [](auto v, int) { return v; });
         } else {
             if (V.op_auto)
-                bottom_run<3, 1, R>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {}, [](R v, int) { return v; });
+                bottom_run<3, 1, R>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {}, [](auto v, int) { return v; });
             else
-                bottom_run<3, 0, R>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {}, [](R v, int) { return v; });
+                bottom_run<3, 0, R>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {}, [](auto v, int) { return v; });
         }
         emul_free(lds);
     });
