@@ -104,7 +104,19 @@ SIGNATURES = {
     "afg_timing": (ctypes.c_int, [P, I64P, F64P, F64P]),
     "afg_vcycle_cell_updates": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int, F64P]),
     "afg_device_bytes": (ctypes.c_int, [P, I64P]),
+    # per-box operators on the host
+    "afg_box_op": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, F64, ctypes.c_int, ctypes.c_int,
+                                  F64P, F64P, F64P, F64P, F64P]),
+    "afg_box_gsrb": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, F64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    F64P, F64P, F64P, F64P, F64P]),
+    "afg_box_corr": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, I32P,
+                                    F64P, F64P, F64P, F64P, F64P]),
+    "afg_box_rstr": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, I32P,
+                                    F64P, F64P, F64P, F64, F64, F64P]),
+    "afg_box_sides_rb": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, I32P,
+                                        F64P, F64P]),
 }
+AFG_BOX_LPL, AFG_BOX_LPLD, AFG_BOX_LSF, AFG_BOX_CLPL, AFG_BOX_CLPLD = 0, 1, 2, 3, 4
 
 _lib = None
 

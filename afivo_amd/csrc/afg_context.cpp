@@ -1329,6 +1329,11 @@ static BoxKey box_key(const afg_ctx *c, int s)
     return BoxKey{c->slot_tree[s], c->slot_lvl[s], {ix[0], ix[1], ix[2]}};
 }
 
+namespace afg {
+// the message of a failed call that has no context (afg_last_error(NULL))
+void set_call_error(const char *msg) { g_create_error = msg; }
+}  // namespace afg
+
 extern "C" {
 
 int afg_ctx_create(int dim, int n_cell, int n_var, afg_ctx **out)

@@ -12,15 +12,16 @@
 !    values captured from sides_bc, run the whole cycle on the GPU and copy
 !    phi, rhs and tmp back into box%cc (unless mg%device_resident is set).
 !  * The operator procedures (mg$D_box_lpl, mg$D_auto_op, ...) identify the
-!    operator family by procedure identity; they are not meant to be called
-!    directly and stop if they are.  A cylindrical tree (coord = a5_cyl)
-!    with the auto operators runs mg2_box_clpl / clpld on the device.
-!    Operator families the device does not implement (level set) and
-!    user-written operators stop in mg$D_fas_fmg / mg$D_fas_vcycle: there is
-!    no CPU fallback.
+!    operator family by procedure identity for the device cycles.  Called
+!    directly, they apply their operator to one box on the host (the
+!    library's afg_box_* functions), as the reference's do.  A cylindrical
+!    tree (coord = a5_cyl) with the auto operators runs mg2_box_clpl / clpld
+!    on the device.  Operator sets other than the lpl set, the auto
+!    operators and the lpllsf set (user-written operators) stop in
+!    mg$D_fas_fmg / mg$D_fas_vcycle: there is no CPU fallback for a cycle.
 !  * A user sides_bc is called on the host once per physical boundary face
-!    and solve, on a copy of the box, to capture the boundary values it
-!    writes (they may depend on the geometry, not on phi).
+!    and solve, on the tree's box, to capture the boundary values it writes
+!    (they may depend on the geometry, not on phi).
 !
 ! Dimension template: "$D" is replaced by 2 or 3, then cpp runs (the same
 ! transform afivo applies to its own m_aX_*.f90 sources).
@@ -658,156 +659,328 @@ contains
   end subroutine mg$D_set_box_tag
 
   ! ------------------------------------------------------------------
-  ! Operator procedures.  They identify the operator family for the device
-  ! path (by procedure identity) and do no work on the host.
+  ! Operator procedures.  mg$D_fas_fmg / mg$D_fas_vcycle identify the
+  ! operator family by procedure identity and run it on the device.  Called
+  ! directly, each applies its operator to the one box on the host, through
+  ! the library's per-box operators (afg_box_*: the per-cell code of the
+  ! device stages).  Arrays an operator does not read are passed as stand-ins.
 
-  subroutine host_call(name)
+#if $D == 2
+#define CC(b, iv) b%cc(:, :, iv)
+#elif $D == 3
+#define CC(b, iv) b%cc(:, :, :, iv)
+#endif
+
+  !> iv if it is set, else the stand-in variable
+  integer function var_or(iv, stand_in)
+    integer, intent(in) :: iv, stand_in
+    var_or = merge(iv, stand_in, iv > 0)
+  end function var_or
+
+  !> stop unless the variables of operator kind `kind` are set
+  subroutine need_vars(mg, kind, name)
+    type(mg$D_t), intent(in)     :: mg
+    integer, intent(in)          :: kind
     character(len=*), intent(in) :: name
-    write(*, "(A,A)") name, ": runs on the device only, through mg$D_fas_fmg / mg$D_fas_vcycle"
-    error stop "afivo_amd: box operator called on the host"
-  end subroutine host_call
+    logical                      :: ok
 
+    ok = .true.
+    if (kind == AFG_BOX_LPLD .or. kind == AFG_BOX_CLPLD) ok = mg%i_eps > 0
+    if (kind == AFG_BOX_LSF) ok = mg%i_lsf > 0 .and. mg%i_bval > 0
+    if (.not. ok) then
+       write(*, "(A,A)") name, ": needs mg%i_eps (lpld operators) or mg%i_lsf and mg%i_bval (lpllsf operators)"
+       error stop "afivo_amd: multigrid variable not set"
+    end if
+  end subroutine need_vars
+
+  !> whether the box's operators are the cylindrical ones (2D, a5_cyl)
+  logical function is_cyl(box)
+    type(box$D_t), intent(in) :: box
+#if $D == 2
+    is_cyl = box%coord_t == a5_cyl
+#else
+    is_cyl = .false.
+#endif
+  end function is_cyl
+
+  subroutine host_op(box, i_out, mg, kind, name)
+    type(box$D_t), intent(inout) :: box
+    integer, intent(in)          :: i_out, kind
+    type(mg$D_t), intent(in)     :: mg
+    character(len=*), intent(in) :: name
+    integer                      :: ie, il, ib
+
+    call need_vars(mg, kind, name)
+    ie = var_or(mg%i_eps, mg%i_phi)
+    il = var_or(mg%i_lsf, mg%i_phi)
+    ib = var_or(mg%i_bval, mg%i_phi)
+    call afg_check(afg_box_op($D, box%n_cell, box%dr, kind, (box%ix(1) - 1) * box%n_cell, &
+         CC(box, mg%i_phi), CC(box, ie), CC(box, il), CC(box, ib), CC(box, i_out)), c_null_ptr, name)
+  end subroutine host_op
+
+  subroutine host_gsrb(box, redblack_cntr, mg, kind, name)
+    type(box$D_t), intent(inout) :: box
+    integer, intent(in)          :: redblack_cntr, kind
+    type(mg$D_t), intent(in)     :: mg
+    character(len=*), intent(in) :: name
+    integer                      :: ie, il, ib
+
+    call need_vars(mg, kind, name)
+    ie = var_or(mg%i_eps, mg%i_rhs)
+    il = var_or(mg%i_lsf, mg%i_rhs)
+    ib = var_or(mg%i_bval, mg%i_rhs)
+    call afg_check(afg_box_gsrb($D, box%n_cell, box%dr, kind, (box%ix(1) - 1) * box%n_cell, redblack_cntr, &
+         CC(box, mg%i_phi), CC(box, mg%i_rhs), CC(box, ie), CC(box, il), CC(box, ib)), c_null_ptr, name)
+  end subroutine host_gsrb
+
+  !> the child's phi corrected with the parent's tmp (phi - tmp after
+  !> correct_children's subtraction)
+  subroutine host_corr(box_p, box_c, mg, kind, name)
+    type(box$D_t), intent(inout) :: box_c
+    type(box$D_t), intent(in)    :: box_p
+    type(mg$D_t), intent(in)     :: mg
+    integer, intent(in)          :: kind
+    character(len=*), intent(in) :: name
+    integer(c_int32_t)           :: off($D)
+    integer                      :: ie, il, ic
+
+    call need_vars(mg, kind, name)
+    off = a$D_get_child_offset(box_c)
+    ie = var_or(mg%i_eps, mg%i_tmp)
+    il = var_or(mg%i_lsf, mg%i_tmp)
+    ic = var_or(mg%i_lsf, mg%i_rhs)
+    call afg_check(afg_box_corr($D, box_c%n_cell, box_p%n_cell, kind, off, CC(box_p, mg%i_tmp), &
+         CC(box_p, ie), CC(box_p, il), CC(box_c, mg%i_phi), CC(box_c, ic)), c_null_ptr, name)
+  end subroutine host_corr
+
+  subroutine host_rstr(box_c, box_p, iv, mg, lsf, name)
+    type(box$D_t), intent(in)    :: box_c
+    type(box$D_t), intent(inout) :: box_p
+    integer, intent(in)          :: iv
+    type(mg$D_t), intent(in)     :: mg
+    logical, intent(in)          :: lsf
+    character(len=*), intent(in) :: name
+    integer(c_int32_t)           :: off($D)
+    real(c_double)               :: p_rmin
+
+    off = a$D_get_child_offset(box_c)
+    p_rmin = box_p%r_min(1)
+    if (lsf) then
+       call need_vars(mg, AFG_BOX_LSF, name)
+       call afg_check(afg_box_rstr($D, box_c%n_cell, box_p%n_cell, AFG_BOX_LSF, off, CC(box_c, iv), &
+            CC(box_c, mg%i_lsf), CC(box_p, mg%i_lsf), p_rmin, box_c%dr, CC(box_p, iv)), c_null_ptr, name)
+    else
+       ! a$D_restrict_box: the radial weights on a cylindrical parent
+       call afg_check(afg_box_rstr($D, box_c%n_cell, box_p%n_cell, &
+            merge(AFG_BOX_CLPL, AFG_BOX_LPL, is_cyl(box_p)), off, CC(box_c, iv), CC(box_c, iv), &
+            CC(box_c, iv), p_rmin, box_c%dr, CC(box_p, iv)), c_null_ptr, name)
+    end if
+  end subroutine host_rstr
+
+  !> operator family of a tagged box under the auto operators
+  !> (m_aX_mg.f90:562-627); -1 for a tag the auto operators do not handle
+  integer function auto_kind(box)
+    type(box$D_t), intent(in) :: box
+
+    select case (box%tag)
+    case (mg_normal_box)
+       auto_kind = merge(AFG_BOX_CLPL, AFG_BOX_LPL, is_cyl(box))
+    case (mg_lsf_box)
+       auto_kind = AFG_BOX_LSF
+    case (mg_veps_box, mg_ceps_box)
+       auto_kind = merge(AFG_BOX_CLPLD, AFG_BOX_LPLD, is_cyl(box))
+    case default
+       auto_kind = -1
+    end select
+  end function auto_kind
+
+  !> mg$D_sides_rb (m_aX_mg.f90:262-350): ghost cells of face nb at a
+  !> refinement boundary, from the parent (a$D_gc_prolong0) and the box's
+  !> own cells
   subroutine mg$D_sides_rb(boxes, id, nb, iv)
     type(box$D_t), intent(inout) :: boxes(:)
     integer, intent(in) :: id, nb, iv
-    call host_call("mg$D_sides_rb")
+    integer(c_int32_t)  :: off($D)
+    integer             :: p
+
+    p = boxes(id)%parent
+    if (p <= a5_no_box) error stop "mg$D_sides_rb: box has no parent"
+    off = a$D_get_child_offset(boxes(id))
+    call afg_check(afg_box_sides_rb($D, boxes(id)%n_cell, boxes(p)%n_cell, nb, off, CC(boxes(p), iv), &
+         CC(boxes(id), iv)), c_null_ptr, "mg$D_sides_rb")
   end subroutine mg$D_sides_rb
 
+  !> mg$D_box_lpl (m_aX_mg.f90:763-797)
   subroutine mg$D_box_lpl(box, i_out, mg)
     type(box$D_t), intent(inout) :: box
     integer, intent(in)          :: i_out
     type(mg$D_t), intent(in)     :: mg
-    call host_call("mg$D_box_lpl")
+    call host_op(box, i_out, mg, AFG_BOX_LPL, "mg$D_box_lpl")
   end subroutine mg$D_box_lpl
 
+  !> mg$D_box_gsrb_lpl (m_aX_mg.f90:718-760)
   subroutine mg$D_box_gsrb_lpl(box, redblack_cntr, mg)
     type(box$D_t), intent(inout) :: box
     integer, intent(in)          :: redblack_cntr
     type(mg$D_t), intent(in)     :: mg
-    call host_call("mg$D_box_gsrb_lpl")
+    call host_gsrb(box, redblack_cntr, mg, AFG_BOX_LPL, "mg$D_box_gsrb_lpl")
   end subroutine mg$D_box_gsrb_lpl
 
+  !> mg$D_box_corr_lpl (m_aX_mg.f90:708-715)
   subroutine mg$D_box_corr_lpl(box_p, box_c, mg)
     type(box$D_t), intent(inout) :: box_c
     type(box$D_t), intent(in)    :: box_p
     type(mg$D_t), intent(in)     :: mg
-    call host_call("mg$D_box_corr_lpl")
+    call host_corr(box_p, box_c, mg, AFG_BOX_LPL, "mg$D_box_corr_lpl")
   end subroutine mg$D_box_corr_lpl
 
+  !> mg$D_box_rstr_lpl (m_aX_mg.f90:800-808)
   subroutine mg$D_box_rstr_lpl(box_c, box_p, iv, mg)
     type(box$D_t), intent(in)    :: box_c
     type(box$D_t), intent(inout) :: box_p
     integer, intent(in)          :: iv
     type(mg$D_t), intent(in)     :: mg
-    call host_call("mg$D_box_rstr_lpl")
+    call host_rstr(box_c, box_p, iv, mg, .false., "mg$D_box_rstr_lpl")
   end subroutine mg$D_box_rstr_lpl
 
+  !> mg$D_auto_op (m_aX_mg.f90:597-629)
   subroutine mg$D_auto_op(box, i_out, mg)
     type(box$D_t), intent(inout) :: box
     integer, intent(in)          :: i_out
     type(mg$D_t), intent(in)     :: mg
-    call host_call("mg$D_auto_op")
+    if (box%tag == a5_init_tag) call mg$D_set_box_tag(box, mg)
+    if (auto_kind(box) >= 0) call host_op(box, i_out, mg, auto_kind(box), "mg$D_auto_op")
   end subroutine mg$D_auto_op
 
+  !> mg$D_auto_gsrb (m_aX_mg.f90:562-594)
   subroutine mg$D_auto_gsrb(box, redblack_cntr, mg)
     type(box$D_t), intent(inout) :: box
     integer, intent(in)          :: redblack_cntr
     type(mg$D_t), intent(in)     :: mg
-    call host_call("mg$D_auto_gsrb")
+    if (box%tag == a5_init_tag) call mg$D_set_box_tag(box, mg)
+    if (auto_kind(box) >= 0) call host_gsrb(box, redblack_cntr, mg, auto_kind(box), "mg$D_auto_gsrb")
   end subroutine mg$D_auto_gsrb
 
+  !> mg$D_auto_corr (m_aX_mg.f90:650-665): by the child's tag, corr_lpl on
+  !> cylindrical trees too
   subroutine mg$D_auto_corr(box_p, box_c, mg)
     type(box$D_t), intent(inout) :: box_c
     type(box$D_t), intent(in)    :: box_p
     type(mg$D_t), intent(in)     :: mg
-    call host_call("mg$D_auto_corr")
+    if (box_c%tag == a5_init_tag) call mg$D_set_box_tag(box_c, mg)
+    select case (box_c%tag)
+    case (mg_normal_box)
+       call host_corr(box_p, box_c, mg, AFG_BOX_LPL, "mg$D_auto_corr")
+    case (mg_lsf_box)
+       call host_corr(box_p, box_c, mg, AFG_BOX_LSF, "mg$D_auto_corr")
+    case (mg_veps_box, mg_ceps_box)
+       call host_corr(box_p, box_c, mg, AFG_BOX_LPLD, "mg$D_auto_corr")
+    end select
   end subroutine mg$D_auto_corr
 
+  !> mg$D_auto_rstr (m_aX_mg.f90:632-647)
   subroutine mg$D_auto_rstr(box_c, box_p, iv, mg)
     type(box$D_t), intent(in)    :: box_c
     type(box$D_t), intent(inout) :: box_p
     integer, intent(in)          :: iv
     type(mg$D_t), intent(in)     :: mg
-    call host_call("mg$D_auto_rstr")
+    if (box_c%tag == a5_init_tag) stop "mg$D_auto_rstr: box_c tag not set"
+    select case (box_c%tag)
+    case (mg_normal_box, mg_veps_box, mg_ceps_box)
+       call host_rstr(box_c, box_p, iv, mg, .false., "mg$D_auto_rstr")
+    case (mg_lsf_box)
+       call host_rstr(box_c, box_p, iv, mg, .true., "mg$D_auto_rstr")
+    end select
   end subroutine mg$D_auto_rstr
 
+  !> mg$D_box_lpld (m_aX_mg.f90:866-916)
   subroutine mg$D_box_lpld(box, i_out, mg)
     type(box$D_t), intent(inout) :: box
     integer, intent(in)          :: i_out
     type(mg$D_t), intent(in)     :: mg
-    call host_call("mg$D_box_lpld")
+    call host_op(box, i_out, mg, AFG_BOX_LPLD, "mg$D_box_lpld")
   end subroutine mg$D_box_lpld
 
+  !> mg$D_box_gsrb_lpld (m_aX_mg.f90:810-863)
   subroutine mg$D_box_gsrb_lpld(box, redblack_cntr, mg)
     type(box$D_t), intent(inout) :: box
     integer, intent(in)          :: redblack_cntr
     type(mg$D_t), intent(in)     :: mg
-    call host_call("mg$D_box_gsrb_lpld")
+    call host_gsrb(box, redblack_cntr, mg, AFG_BOX_LPLD, "mg$D_box_gsrb_lpld")
   end subroutine mg$D_box_gsrb_lpld
 
+  !> mg$D_box_corr_lpld (m_aX_mg.f90:918-985)
   subroutine mg$D_box_corr_lpld(box_p, box_c, mg)
     type(box$D_t), intent(inout) :: box_c
     type(box$D_t), intent(in)    :: box_p
     type(mg$D_t), intent(in)     :: mg
-    call host_call("mg$D_box_corr_lpld")
+    call host_corr(box_p, box_c, mg, AFG_BOX_LPLD, "mg$D_box_corr_lpld")
   end subroutine mg$D_box_corr_lpld
 
+  !> mg$D_box_lpllsf (m_aX_mg.f90:1183-1253)
   subroutine mg$D_box_lpllsf(box, i_out, mg)
     type(box$D_t), intent(inout) :: box
     integer, intent(in)          :: i_out
     type(mg$D_t), intent(in)     :: mg
-    call host_call("mg$D_box_lpllsf")
+    call host_op(box, i_out, mg, AFG_BOX_LSF, "mg$D_box_lpllsf")
   end subroutine mg$D_box_lpllsf
 
+  !> mg$D_box_gsrb_lpllsf (m_aX_mg.f90:1103-1180)
   subroutine mg$D_box_gsrb_lpllsf(box, redblack_cntr, mg)
     type(box$D_t), intent(inout) :: box
     integer, intent(in)          :: redblack_cntr
     type(mg$D_t), intent(in)     :: mg
-    call host_call("mg$D_box_gsrb_lpllsf")
+    call host_gsrb(box, redblack_cntr, mg, AFG_BOX_LSF, "mg$D_box_gsrb_lpllsf")
   end subroutine mg$D_box_gsrb_lpllsf
 
+  !> mg$D_box_corr_lpllsf (m_aX_mg.f90:1020-1100)
   subroutine mg$D_box_corr_lpllsf(box_p, box_c, mg)
     type(box$D_t), intent(inout) :: box_c
     type(box$D_t), intent(in)    :: box_p
     type(mg$D_t), intent(in)     :: mg
-    call host_call("mg$D_box_corr_lpllsf")
+    call host_corr(box_p, box_c, mg, AFG_BOX_LSF, "mg$D_box_corr_lpllsf")
   end subroutine mg$D_box_corr_lpllsf
 
+  !> mg$D_box_rstr_lpllsf (m_aX_mg.f90:1256-1321)
   subroutine mg$D_box_rstr_lpllsf(box_c, box_p, iv, mg)
     type(box$D_t), intent(in)    :: box_c
     type(box$D_t), intent(inout) :: box_p
     integer, intent(in)          :: iv
     type(mg$D_t), intent(in)     :: mg
-    call host_call("mg$D_box_rstr_lpllsf")
+    call host_rstr(box_c, box_p, iv, mg, .true., "mg$D_box_rstr_lpllsf")
   end subroutine mg$D_box_rstr_lpllsf
 
 #if $D == 2
+  !> mg2_box_clpl (m_aX_mg.f90:1354-1376)
   subroutine mg$D_box_clpl(box, i_out, mg)
     type(box$D_t), intent(inout) :: box
     integer, intent(in)          :: i_out
     type(mg$D_t), intent(in)     :: mg
-    call host_call("mg$D_box_clpl")
+    call host_op(box, i_out, mg, AFG_BOX_CLPL, "mg$D_box_clpl")
   end subroutine mg$D_box_clpl
 
+  !> mg2_box_gsrb_clpl (m_aX_mg.f90:1325-1351)
   subroutine mg$D_box_gsrb_clpl(box, redblack_cntr, mg)
     type(box$D_t), intent(inout) :: box
     integer, intent(in)          :: redblack_cntr
     type(mg$D_t), intent(in)     :: mg
-    call host_call("mg$D_box_gsrb_clpl")
+    call host_gsrb(box, redblack_cntr, mg, AFG_BOX_CLPL, "mg$D_box_gsrb_clpl")
   end subroutine mg$D_box_gsrb_clpl
 
+  !> mg2_box_clpld (m_aX_mg.f90:1379-1407)
   subroutine mg$D_box_clpld(box, i_out, mg)
     type(box$D_t), intent(inout) :: box
     integer, intent(in)          :: i_out
     type(mg$D_t), intent(in)     :: mg
-    call host_call("mg$D_box_clpld")
+    call host_op(box, i_out, mg, AFG_BOX_CLPLD, "mg$D_box_clpld")
   end subroutine mg$D_box_clpld
 
+  !> mg2_box_gsrb_clpld (m_aX_mg.f90:1411-1443)
   subroutine mg$D_box_gsrb_clpld(box, redblack_cntr, mg)
     type(box$D_t), intent(inout) :: box
     integer, intent(in)          :: redblack_cntr
     type(mg$D_t), intent(in)     :: mg
-    call host_call("mg$D_box_gsrb_clpld")
+    call host_gsrb(box, redblack_cntr, mg, AFG_BOX_CLPLD, "mg$D_box_gsrb_clpld")
   end subroutine mg$D_box_gsrb_clpld
 #endif
 

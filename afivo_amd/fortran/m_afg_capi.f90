@@ -10,6 +10,8 @@ module m_afg_capi
   integer(c_int), parameter :: AFG_RSTR_LPL = 0, AFG_RSTR_AUTO = 1, AFG_RSTR_LSF = 2
   integer(c_int), parameter :: AFG_BC_DIRICHLET = -1, AFG_BC_NEUMANN = -2
   integer(c_int), parameter :: AFG_COORD_XYZ = 0, AFG_COORD_CYL = 1
+  integer(c_int), parameter :: AFG_BOX_LPL = 0, AFG_BOX_LPLD = 1, AFG_BOX_LSF = 2, AFG_BOX_CLPL = 3, &
+       AFG_BOX_CLPLD = 4
 
   !> afg_mg_opts of include/afivo_amd.h
   type, bind(c) :: afg_mg_opts
@@ -70,6 +72,53 @@ module m_afg_capi
        integer(c_int), value :: id, nb, bc_type
        type(c_ptr), value :: b
      end function afg_set_bc
+
+     ! per-box operators on the host (box arrays cc(0:nc+1, ...) of one variable)
+     integer(c_int) function afg_box_op(dim, nc, dr, kind, ir0, phi, eps, lsf, bval, out) &
+          bind(c, name="afg_box_op")
+       import :: c_int, c_double
+       integer(c_int), value :: dim, nc, kind, ir0
+       real(c_double), value :: dr
+       real(c_double), intent(in) :: phi(*), eps(*), lsf(*), bval(*)
+       real(c_double), intent(inout) :: out(*)
+     end function afg_box_op
+
+     integer(c_int) function afg_box_gsrb(dim, nc, dr, kind, ir0, redblack, phi, rhs, eps, lsf, bval) &
+          bind(c, name="afg_box_gsrb")
+       import :: c_int, c_double
+       integer(c_int), value :: dim, nc, kind, ir0, redblack
+       real(c_double), value :: dr
+       real(c_double), intent(inout) :: phi(*)
+       real(c_double), intent(in) :: rhs(*), eps(*), lsf(*), bval(*)
+     end function afg_box_gsrb
+
+     integer(c_int) function afg_box_corr(dim, nc, pnc, kind, offset, p_corr, p_eps, p_lsf, c_phi, c_lsf) &
+          bind(c, name="afg_box_corr")
+       import :: c_int, c_int32_t, c_double
+       integer(c_int), value :: dim, nc, pnc, kind
+       integer(c_int32_t), intent(in) :: offset(*)
+       real(c_double), intent(in) :: p_corr(*), p_eps(*), p_lsf(*), c_lsf(*)
+       real(c_double), intent(inout) :: c_phi(*)
+     end function afg_box_corr
+
+     integer(c_int) function afg_box_rstr(dim, nc, pnc, kind, offset, c_var, c_lsf, p_lsf, p_rmin, c_dr, p_var) &
+          bind(c, name="afg_box_rstr")
+       import :: c_int, c_int32_t, c_double
+       integer(c_int), value :: dim, nc, pnc, kind
+       integer(c_int32_t), intent(in) :: offset(*)
+       real(c_double), intent(in) :: c_var(*), c_lsf(*), p_lsf(*)
+       real(c_double), value :: p_rmin, c_dr
+       real(c_double), intent(inout) :: p_var(*)
+     end function afg_box_rstr
+
+     integer(c_int) function afg_box_sides_rb(dim, nc, pnc, nb, offset, p_var, c_var) &
+          bind(c, name="afg_box_sides_rb")
+       import :: c_int, c_int32_t, c_double
+       integer(c_int), value :: dim, nc, pnc, nb
+       integer(c_int32_t), intent(in) :: offset(*)
+       real(c_double), intent(in) :: p_var(*)
+       real(c_double), intent(inout) :: c_var(*)
+     end function afg_box_sides_rb
 
      integer(c_int) function afg_upload_cc(ctx, iv, n_ids, ids, packed) bind(c, name="afg_upload_cc")
        import :: c_int, c_ptr, c_int32_t, c_double

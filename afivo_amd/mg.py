@@ -13,7 +13,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import (AFG_BC_DIRICHLET, AFG_BC_NEUMANN, AFG_OP_AUTO, AFG_OP_LPL, AFG_OP_LSF, AFG_RSTR_AUTO,  # noqa: F401
+from ._lib import (AFG_BC_DIRICHLET, AFG_BC_NEUMANN, AFG_BOX_CLPL, AFG_BOX_CLPLD, AFG_BOX_LPL,  # noqa: F401
+                   AFG_BOX_LPLD, AFG_BOX_LSF, AFG_OP_AUTO, AFG_OP_LPL, AFG_OP_LSF, AFG_RSTR_AUTO,
                    AFG_RSTR_LPL, AFG_RSTR_LSF, MgOpts, check)
 from .treefile import TreeState
 
@@ -419,3 +420,81 @@ def default_mg(eps: bool = False, n_down=2, n_up=2, n_base=2, auto: bool = False
                 n_cycle_up=n_up, n_cycle_base=n_base, op=op, i_lsf=5 if lsf else -1,
                 i_bval=6 if lsf else -1, rstr=rstr)
     return mg_init_mg(mg)
+
+
+# ---------------------------------------------------------------------------
+# Per-box operators on the host (afg_box_*): the public box procedures of
+# m_a$D_mg applied to one box held in host memory.  `box` arrays are the
+# whole cc(0:nc+1, ...) array of one variable (any shape with (nc+2)^dim
+# float64 values, x fastest); kind is AFG_BOX_LPL, _LPLD (eps), _LSF (lsf,
+# bval), _CLPL or _CLPLD (2D cylindrical, ir0 = (ix(1) - 1) * nc).
+
+def _box(a, n):
+    if a is None:
+        return None
+    a = np.asarray(a)
+    if a.dtype != np.float64 or not a.flags.c_contiguous or a.size != n:
+        raise ValueError(f"box array must be {n} contiguous float64 values")
+    return a
+
+
+def _opt(a):
+    return None if a is None else _f64(a)
+
+
+def box_op(dim, nc, dr, kind, phi, out, eps=None, lsf=None, bval=None, ir0=0):
+    """out(1:nc, ...) = A phi: mg$D_box_lpl and its families (m_aX_mg.f90:763-797,
+    866-916, 1183-1253, 1354-1407); out's ghost cells are left as they are."""
+    n = (nc + 2) ** dim
+    phi, out, eps, lsf, bval = (_box(a, n) for a in (phi, out, eps, lsf, bval))
+    check(_lib.load().afg_box_op(dim, nc, dr, kind, ir0, _f64(phi), _opt(eps), _opt(lsf), _opt(bval), _f64(out)))
+
+
+def box_gsrb(dim, nc, dr, kind, redblack_cntr, phi, rhs, eps=None, lsf=None, bval=None, ir0=0):
+    """One colour of Gauss-Seidel red-black on phi, in place: mg$D_box_gsrb_lpl
+    and its families (m_aX_mg.f90:718-760, 810-863, 1103-1180, 1325-1351, 1411-1443)."""
+    n = (nc + 2) ** dim
+    phi, rhs, eps, lsf, bval = (_box(a, n) for a in (phi, rhs, eps, lsf, bval))
+    check(_lib.load().afg_box_gsrb(dim, nc, dr, kind, ir0, redblack_cntr, _f64(phi), _f64(rhs), _opt(eps),
+                                   _opt(lsf), _opt(bval)))
+
+
+def _offset(offset):
+    o = np.zeros(3, dtype=np.int32)
+    o[:len(offset)] = offset
+    return o
+
+
+def box_corr(dim, nc, pnc, kind, offset, p_corr, c_phi, p_eps=None, p_lsf=None, c_lsf=None):
+    """Correction of a child's phi (in place) from its parent's correction
+    p_corr (the parent's tmp after tmp = phi - tmp): mg$D_box_corr_lpl,
+    _lpld, _lpllsf (m_aX_mg.f90:708-715, 918-985, 1020-1100).  offset:
+    a$D_get_child_offset of the child (m_aX_t.f90:362-374)."""
+    n, m = (nc + 2) ** dim, (pnc + 2) ** dim
+    c_phi, c_lsf = _box(c_phi, n), _box(c_lsf, n)
+    p_corr, p_eps, p_lsf = _box(p_corr, m), _box(p_eps, m), _box(p_lsf, m)
+    o = _offset(offset)
+    check(_lib.load().afg_box_corr(dim, nc, pnc, kind, _i32(o), _f64(p_corr), _opt(p_eps), _opt(p_lsf),
+                                   _f64(c_phi), _opt(c_lsf)))
+
+
+def box_rstr(dim, nc, pnc, kind, offset, c_var, p_var, c_lsf=None, p_lsf=None, p_rmin=0.0, c_dr=0.0):
+    """Restriction of a child's variable into its parent's (in place):
+    mg$D_box_rstr_lpl = a$D_restrict_box (m_aX_restrict.f90:67-132; kind
+    AFG_BOX_CLPL for a cylindrical parent, with p_rmin = box_p%r_min(1) and
+    c_dr = box_c%dr) and mg$D_box_rstr_lpllsf (m_aX_mg.f90:1256-1321)."""
+    n, m = (nc + 2) ** dim, (pnc + 2) ** dim
+    c_var, c_lsf = _box(c_var, n), _box(c_lsf, n)
+    p_var, p_lsf = _box(p_var, m), _box(p_lsf, m)
+    o = _offset(offset)
+    check(_lib.load().afg_box_rstr(dim, nc, pnc, kind, _i32(o), _f64(c_var), _opt(c_lsf), _opt(p_lsf),
+                                   p_rmin, c_dr, _f64(p_var)))
+
+
+def sides_rb(dim, nc, pnc, nb, offset, p_var, c_var):
+    """mg$D_sides_rb (m_aX_mg.f90:262-350): the ghost cells of face nb
+    (1-based) of a child at a refinement boundary, from its parent's array."""
+    n, m = (nc + 2) ** dim, (pnc + 2) ** dim
+    c_var, p_var = _box(c_var, n), _box(p_var, m)
+    o = _offset(offset)
+    check(_lib.load().afg_box_sides_rb(dim, nc, pnc, nb, _i32(o), _f64(p_var), _f64(c_var)))

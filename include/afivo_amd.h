@@ -46,17 +46,17 @@ enum afg_op {
     AFG_OP_LPL = 0,   /* mg$D_box_lpl / gsrb_lpl / corr_lpl / rstr_lpl   (m_aX_mg.f90:708-808) */
     AFG_OP_AUTO = 1,  /* mg$D_auto_op / auto_gsrb / auto_corr (m_aX_mg.f90:562-665): per box by its tag,
                          lpl, lpld (eps) or lpllsf (level set) */
-    AFG_OP_LSF = 2    /* mg$D_box_lpllsf / gsrb_lpllsf / corr_lpllsf for every box (m_aX_mg.f90:1019-1251) */
+    AFG_OP_LSF = 2    /* mg$D_box_lpllsf / gsrb_lpllsf / corr_lpllsf for every box (m_aX_mg.f90:1020-1253) */
 };
 
 /* Restriction of the multigrid (mg%box_rstr). */
 enum afg_rstr {
     AFG_RSTR_LPL = 0,   /* mg$D_box_rstr_lpl (m_aX_mg.f90:800-808) */
-    AFG_RSTR_AUTO = 1,  /* mg$D_auto_rstr (m_aX_mg.f90:629-644): rstr_lpllsf for children tagged
+    AFG_RSTR_AUTO = 1,  /* mg$D_auto_rstr (m_aX_mg.f90:632-647): rstr_lpllsf for children tagged
                            mg_lsf_box, else rstr_lpl.  The reference stops on a child whose tag is
                            unset (:639); the device sets every box's tag before the first stage,
                            which is the state of a program that called mg$D_set_box_tag on each box */
-    AFG_RSTR_LSF = 2    /* mg$D_box_rstr_lpllsf for every box (m_aX_mg.f90:1254-1320) */
+    AFG_RSTR_LSF = 2    /* mg$D_box_rstr_lpllsf for every box (m_aX_mg.f90:1256-1321) */
 };
 
 /* Physical boundary types (a5_bc_dirichlet, a5_bc_neumann; m_aX_t.f90:45-48). */
@@ -394,6 +394,47 @@ int afg_timing(afg_ctx *ctx, int64_t *n_launches, double *ms, double *bytes);
 int afg_vcycle_cell_updates(const afg_ctx *ctx, const afg_mg_opts *opts, int highest_lvl, double *out);
 /* Device bytes held by the context (pools and tables). */
 int afg_device_bytes(const afg_ctx *ctx, int64_t *out);
+
+/* ------------------------------------------------------------------ */
+/* Per-box operators on the host (CPU): the public box procedures of the
+ * reference module for one box in host memory, for callers that apply them
+ * to single boxes (the drop-in module's mg$D_box_* procedures).  They run the
+ * per-cell bodies of the device stages, so each gives the device stage's
+ * value for that box.  Box arrays hold cc(0:nc+1, 0:nc+1[, 0:nc+1]) of one
+ * variable with x fastest; dr is box%dr; ir0 = (box%ix(1) - 1) * nc (the
+ * radial cell offset, cylindrical kinds only).  No context, no device. */
+#define AFG_BOX_LPL 0   /* mg$D_box_lpl / gsrb_lpl / corr_lpl / rstr_lpl */
+#define AFG_BOX_LPLD 1  /* mg$D_box_lpld / gsrb_lpld / corr_lpld */
+#define AFG_BOX_LSF 2   /* mg$D_box_lpllsf / gsrb_lpllsf / corr_lpllsf / rstr_lpllsf */
+#define AFG_BOX_CLPL 3  /* mg2_box_clpl / gsrb_clpl; restriction with the radial weights */
+#define AFG_BOX_CLPLD 4 /* mg2_box_clpld / gsrb_clpld */
+/* out(1:nc, ...) = A phi: mg$D_box_lpl (m_aX_mg.f90:763-797), _lpld
+ * (:866-916), _lpllsf (:1183-1253), mg2_box_clpl / _clpld (:1354-1407).
+ * eps for the lpld kinds, lsf and bval for AFG_BOX_LSF (else may be NULL). */
+int afg_box_op(int dim, int nc, double dr, int kind, int ir0, const double *phi, const double *eps,
+               const double *lsf, const double *bval, double *out);
+/* One colour of Gauss-Seidel red-black on phi: mg$D_box_gsrb_lpl
+ * (m_aX_mg.f90:718-760), _lpld (:810-863), _lpllsf (:1103-1180),
+ * mg2_box_gsrb_clpl (:1325-1351) / _clpld (:1411-1443); the parity of redblack selects the
+ * cells, as redblack_cntr does. */
+int afg_box_gsrb(int dim, int nc, double dr, int kind, int ir0, int redblack, double *phi, const double *rhs,
+                 const double *eps, const double *lsf, const double *bval);
+/* Correction of a child from its parent's tmp (the parent's correction):
+ * mg$D_box_corr_lpl (m_aX_mg.f90:708-715), _lpld (:918-985), _lpllsf
+ * (:1020-1100), adding into the child's phi.  offset is
+ * a$D_get_child_offset(box_c) (m_aX_t.f90); pnc the parent's cell count;
+ * p_eps for AFG_BOX_LPLD, p_lsf and c_lsf for AFG_BOX_LSF. */
+int afg_box_corr(int dim, int nc, int pnc, int kind, const int32_t *offset, const double *p_corr,
+                 const double *p_eps, const double *p_lsf, double *c_phi, const double *c_lsf);
+/* Restriction of a child into its parent: mg$D_box_rstr_lpl =
+ * a$D_restrict_box (m_aX_restrict.f90:67-132; AFG_BOX_CLPL: the radial
+ * weights of a cylindrical parent, p_rmin = box_p%r_min(1), c_dr = box_c%dr)
+ * and mg$D_box_rstr_lpllsf (m_aX_mg.f90:1256-1321; c_lsf, p_lsf). */
+int afg_box_rstr(int dim, int nc, int pnc, int kind, const int32_t *offset, const double *c_var,
+                 const double *c_lsf, const double *p_lsf, double p_rmin, double c_dr, double *p_var);
+/* mg$D_sides_rb (m_aX_mg.f90:262-350) of face nb (1-based, afivo's
+ * neighbour order) of a child from its parent (ghost cells included). */
+int afg_box_sides_rb(int dim, int nc, int pnc, int nb, const int32_t *offset, const double *p_var, double *c_var);
 
 #ifdef __cplusplus
 }

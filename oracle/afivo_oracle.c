@@ -1630,3 +1630,49 @@ double orc_tree_min_cc(const orc_tree *t, int iv, int include_parents)
 {
     return tree_ext(t, iv, include_parents, 0);
 }
+
+/* One box procedure of the family `kind` (0 lpl, 1 lpld, 2 lpllsf, 3 clpl,
+ * 4 clpld) on one box, as a program calling it directly does:
+ *   what 0: box op into i_out = arg (mg$D_box_lpl, _lpld, _lpllsf,
+ *           mg2_box_clpl, _clpld)
+ *   what 1: box gsrb with redblack_cntr = arg
+ *   what 2: box corr of child id from its parent (kinds 0-2)
+ *   what 3: box rstr of variable arg from child id into its parent
+ *           (kind 0: a$D_restrict_box, by the tree's coordinates; 2: lpllsf)
+ *   what 4: mg$D_sides_rb of face arg (1-based) of box id, variable o->i_phi */
+int orc_box_call(orc_tree *t, const orc_mg_opts *o, int what, int kind, int id, int arg)
+{
+    if (id < 1 || id > t->max_id || t->ncell[id] == 0 || kind < 0 || kind > 4) return -1;
+    if (kind >= 3 && !(t->dim == 2 && t->cyl)) return -1;
+    int p = t->parent[id];
+    switch (what) {
+    case 0: {
+        void (*f[5])(orc_tree *, int, int, const orc_mg_opts *) = {box_lpl, box_lpld, box_lpllsf, box_clpl,
+                                                                     box_clpld};
+        f[kind](t, id, arg, o);
+        return 0;
+    }
+    case 1: {
+        void (*f[5])(orc_tree *, int, int, const orc_mg_opts *) = {gsrb_lpl, gsrb_lpld, gsrb_lpllsf, gsrb_clpl,
+                                                                     gsrb_clpld};
+        f[kind](t, id, arg, o);
+        return 0;
+    }
+    case 2:
+        if (p <= 0 || kind > 2) return -1;
+        if (kind == 0) prolong2(t, p, id, o->i_tmp, o->i_phi, 1);
+        else if (kind == 1) corr_lpld(t, p, id, o);
+        else corr_lpllsf(t, p, id, o);
+        return 0;
+    case 3:
+        if (p <= 0 || (kind != 0 && kind != 2)) return -1;
+        if (kind == 0) restrict_box(t, id, p, arg, arg);
+        else rstr_lpllsf(t, id, p, arg, o);
+        return 0;
+    case 4:
+        if (p <= 0 || arg < 1 || arg > 2 * t->dim) return -1;
+        sides_rb(t, id, arg - 1, o->i_phi);
+        return 0;
+    }
+    return -1;
+}

@@ -58,6 +58,11 @@ int orc_set_coord(orc_tree *t, int cyl, double r_base);
 int orc_fas_fmg(orc_tree *t, const orc_mg_opts *o, int set_residual, int have_guess);
 /* every box's tag (mg$D_set_box_tag on each box), as mg$D_auto_rstr needs */
 int orc_set_tags(orc_tree *t, const orc_mg_opts *o);
+/* One box procedure on one box (per-box checks): what 0 op into i_out = arg,
+ * 1 gsrb with redblack_cntr = arg, 2 corr of child id, 3 rstr of variable
+ * arg from child id, 4 sides_rb of face arg; kind 0 lpl, 1 lpld, 2 lpllsf,
+ * 3 clpl, 4 clpld (cylindrical 2D trees). */
+int orc_box_call(orc_tree *t, const orc_mg_opts *o, int what, int kind, int id, int arg);
 int orc_fas_vcycle(orc_tree *t, const orc_mg_opts *o, int highest_lvl, int set_residual);
 double orc_tree_max_cc(const orc_tree *t, int iv, int include_parents);
 double orc_tree_min_cc(const orc_tree *t, int iv, int include_parents);

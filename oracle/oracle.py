@@ -66,6 +66,8 @@ def lib():
                                         ctypes.c_int]
         L.orc_update_coarse_level.argtypes = [P, ctypes.POINTER(MgOpts), ctypes.c_int]
         L.orc_set_tags.argtypes = [P, ctypes.POINTER(MgOpts)]
+        L.orc_box_call.argtypes = [P, ctypes.POINTER(MgOpts), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int]
         _lib = L
     return _lib
 

@@ -208,3 +208,34 @@ def test_dropin_end_to_end_on_the_3d_benchmark_tree(tmp_path):
     print(f"drop-in, 3D benchmark tree: {1e3 * times[False]:.2f} ms per solve with host copies on every call, "
           f"{1e3 * times[True]:.2f} ms device-resident; host copies {100 * share:.0f}% of the default mode")
     assert times[True] < times[False]
+
+
+BOX_OPS = {d: (os.path.join(BUILD, f"box_ops_{d}d"), os.path.join(O.REF_DIR, f"ref_box_ops_{d}d")) for d in (2, 3)}
+
+
+@pytest.mark.skipif(not all(os.path.exists(p) for pair in BOX_OPS.values() for p in pair),
+                    reason="box procedure programs absent (make -C oracle ref; make -C tests/fortran)")
+@pytest.mark.parametrize("dim,coord", [(2, ""), (2, "cyl"), (3, "")])
+def test_dropin_box_procedures_match_reference(tmp_path, dim, coord):
+    """The box procedures called directly (oracle/box_ops_X.f90): every
+    mg$D_box_* family, mg$D_sides_rb and the auto operators on every box of
+    a tree with normal, constant-eps, varying-eps and level-set boxes (and
+    mg2_box_clpl / clpld on the cylindrical tree), written out after each
+    procedure.  The drop-in's build writes the same bytes as the build
+    against the reference's m_a$D_mg.  The procedures are host code (the
+    library's afg_box_* functions): no device is used."""
+    outs = []
+    for prog in BOX_OPS[dim]:
+        out = str(tmp_path / (os.path.basename(prog) + ".bin"))
+        p = subprocess.run([prog, out] + ([coord] if coord else []), capture_output=True, text=True,
+                           timeout=120, env=dict(os.environ, OMP_NUM_THREADS="1"))
+        assert p.returncode == 0, p.stdout + p.stderr
+        outs.append(open(out, "rb").read())
+    new, ref = outs
+    n = int(np.frombuffer(ref[:4], dtype=np.int32)[0])
+    lsf_tags = np.frombuffer(ref[-4 * n:], dtype=np.int32)
+    assert n > 10 and set(lsf_tags) == {1, 2}, "the tree should hold level-set and normal boxes"
+    assert len(new) == len(ref)
+    a, b = np.frombuffer(new[4:-4 * n], dtype=np.uint8), np.frombuffer(ref[4:-4 * n], dtype=np.uint8)
+    assert np.array_equal(a, b), f"first differing byte at {int(np.nonzero(a != b)[0][0]) + 4}"
+    assert new[-4 * n:] == ref[-4 * n:]
