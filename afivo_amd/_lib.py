@@ -104,6 +104,8 @@ SIGNATURES = {
     "afg_timing": (ctypes.c_int, [P, I64P, F64P, F64P]),
     "afg_vcycle_cell_updates": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int, F64P]),
     "afg_device_bytes": (ctypes.c_int, [P, I64P]),
+    "afg_upload_boxes": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, I32P, ctypes.POINTER(F64P)]),
+    "afg_download_boxes": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, I32P, ctypes.POINTER(F64P)]),
     # per-box operators on the host
     "afg_box_op": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, F64, ctypes.c_int, ctypes.c_int,
                                   F64P, F64P, F64P, F64P, F64P]),

@@ -13,8 +13,11 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <map>
+#include <sched.h>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <unordered_map>
 #include <vector>
@@ -178,6 +181,7 @@ struct afg_ctx {
     double *scratch = nullptr;           // this device's scratch array (afg_tree_extreme_noalloc)
     double *h_stage = nullptr;
     size_t stage_cap = 0;
+    hipEvent_t xfer_ev[8] = {};          // per piece of a download (afg_download_boxes)
 
     // execution controls
     int bottom_lvl[4] = {-1000, -1000, -1000, -1000};  // highest level run by the bottom kernel
@@ -1410,6 +1414,8 @@ int afg_ctx_destroy(afg_ctx *c)
     ctx_free(c, c->d_out);
     if (c->h_out) hipHostFree(c->h_out);
     if (c->h_stage) hipHostFree(c->h_stage);
+    for (hipEvent_t ev : c->xfer_ev)
+        if (ev) hipEventDestroy(ev);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -1977,53 +1983,210 @@ static int copy_runs(afg_ctx *c, double *dev, const std::vector<char> &mark, boo
     return 0;
 }
 
-int afg_upload_cc(afg_ctx *c, int iv, int n_ids, const int32_t *ids, const double *packed)
+}  // extern "C"
+
+// ---- transfers of cell data between host box arrays and the device pools ----
+//
+// A variable moves through the context's pinned staging buffer, which holds
+// it in the pool's slot layout.  Host copies (box array <-> staging) run on
+// several threads; the slots are cut into XFER_PIECES pieces of consecutive
+// slots, and the device copy of a piece is queued as soon as its boxes are
+// staged (upload) or the boxes of a piece are copied out as soon as its device
+// copy has arrived (download), so that host copies and PCIe transfers overlap.
+
+constexpr int XFER_PIECES = 8;
+
+// threads for the host copies: the CPUs this process may run on, at most 16
+static int host_threads()
 {
-    if (int e = check_ctx(c)) return e;
-    data_written(c);
-    double *dev = nullptr;
-    if (int e = ensure_var(c, iv, &dev)) return e;
-    if (int e = ensure_stage(c, (size_t)c->n_slots * c->stride)) return e;
-    HIP_TRY(c, hipStreamSynchronize(c->stream));   // staging may be in use
-    std::vector<char> mark(c->n_slots, 0);
+    static const int n = [] {
+        cpu_set_t set;
+        CPU_ZERO(&set);
+        int k = sched_getaffinity(0, sizeof(set), &set) == 0 ? CPU_COUNT(&set) : 0;
+        if (k < 1) k = (int)std::thread::hardware_concurrency();
+        return std::max(1, std::min(k, 16));
+    }();
+    return n;
+}
+
+// The boxes of ids by piece: slot_of[k] and, per piece, the indices k whose
+// slot is in it; every id must be a box of the tree, once.
+struct XferPlan {
+    std::vector<int> slot_of;
+    std::vector<int> piece[XFER_PIECES];
+    std::vector<char> mark;
+};
+
+static int xfer_plan(afg_ctx *c, int n_ids, const int32_t *ids, const void *const *boxes, XferPlan &P)
+{
+    if (n_ids < 0 || (n_ids > 0 && (!ids || !boxes))) return c->fail(AFG_ERR_ARG, "transfer: invalid box list");
+    P.slot_of.resize(n_ids);
+    P.mark.assign(c->n_slots, 0);
+    for (int k = 0; k < n_ids; k++) {
+        const int id = ids[k];
+        if (id < 1 || id > c->max_id || c->id2slot[id] < 0) return c->fail(AFG_ERR_ARG, "no box with id %d", id);
+        if (!boxes[k]) return c->fail(AFG_ERR_ARG, "transfer: no array for box %d", id);
+        const int s = c->id2slot[id];
+        if (P.mark[s]) return c->fail(AFG_ERR_ARG, "transfer: box %d listed twice", id);
+        P.mark[s] = 1;
+        P.slot_of[k] = s;
+        P.piece[(int)((long long)s * XFER_PIECES / c->n_slots)].push_back(k);
+    }
+    return 0;
+}
+
+// the slot range [lo, hi) of piece p
+static void piece_slots(const afg_ctx *c, int p, int &lo, int &hi)
+{
+    // smallest s with s * P / n_slots >= p
+    lo = (int)(((long long)p * c->n_slots + XFER_PIECES - 1) / XFER_PIECES);
+    hi = (int)(((long long)(p + 1) * c->n_slots + XFER_PIECES - 1) / XFER_PIECES);
+}
+
+// device copies of the contiguous runs of marked slots in [lo, hi)
+static hipError_t copy_slot_runs(afg_ctx *c, double *dev, const std::vector<char> &mark, bool to_device, int lo,
+                                 int hi)
+{
+    int s = lo;
+    while (s < hi) {
+        if (!mark[s]) {
+            s++;
+            continue;
+        }
+        int e = s;
+        while (e < hi && mark[e]) e++;
+        const size_t off = (size_t)s * c->stride, len = (size_t)(e - s) * c->stride * sizeof(double);
+        const hipError_t r = to_device
+                                 ? hipMemcpyAsync(dev + off, c->h_stage + off, len, hipMemcpyHostToDevice, c->stream)
+                                 : hipMemcpyAsync(c->h_stage + off, dev + off, len, hipMemcpyDeviceToHost, c->stream);
+        if (r != hipSuccess) return r;
+        s = e;
+    }
+    return hipSuccess;
+}
+
+// f(t, p, i) on T threads: thread t takes its share of the indices i of every
+// piece p in turn; after(p) runs once per piece, on the thread that finished
+// the piece last (upload), or before(p) on every thread before its share of p
+template <typename Before, typename F, typename After>
+static void xfer_threads(const XferPlan &P, Before before, F f, After after)
+{
+    int total = 0;
+    for (const auto &pc : P.piece) total += (int)pc.size();
+    const int T = std::max(1, std::min(host_threads(), total / 64));
+    std::atomic<int> left[XFER_PIECES];
+    for (int p = 0; p < XFER_PIECES; p++) left[p] = T;
+    auto work = [&](int t) {
+        for (int p = 0; p < XFER_PIECES; p++) {
+            const int n = (int)P.piece[p].size();
+            before(p);
+            for (int i = (int)((long long)n * t / T); i < (int)((long long)n * (t + 1) / T); i++) f(P.piece[p][i]);
+            if (left[p].fetch_sub(1) == 1) after(p);
+        }
+    };
+    if (T == 1) {
+        work(0);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 1; t < T; t++) th.emplace_back(work, t);
+    work(0);
+    for (auto &x : th) x.join();
+}
+
+// the packed layout: box k's array at the sum of the lengths of boxes 0..k-1
+static int packed_boxes(afg_ctx *c, int n_ids, const int32_t *ids, const double *packed, std::vector<const double *> &out)
+{
+    if (n_ids < 0 || (n_ids > 0 && (!ids || !packed))) return c->fail(AFG_ERR_ARG, "transfer: invalid box list");
+    out.resize(n_ids);
     size_t pos = 0;
     for (int k = 0; k < n_ids; k++) {
         const int id = ids[k];
         if (id < 1 || id > c->max_id || c->id2slot[id] < 0) return c->fail(AFG_ERR_ARG, "no box with id %d", id);
-        const int s = c->id2slot[id];
-        const int len = box_len(c->dim, c->slot_nc[s]);
-        std::memcpy(c->h_stage + (size_t)s * c->stride, packed + pos, len * sizeof(double));
-        if (len < c->stride) std::memset(c->h_stage + (size_t)s * c->stride + len, 0, (c->stride - len) * sizeof(double));
-        pos += len;
-        mark[s] = 1;
+        out[k] = packed + pos;
+        pos += box_len(c->dim, c->slot_nc[c->id2slot[id]]);
     }
-    if (int e = copy_runs(c, dev, mark, true, c->stride)) return e;
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
     return 0;
+}
+
+extern "C" {
+
+int afg_upload_boxes(afg_ctx *c, int iv, int n_ids, const int32_t *ids, const double *const *boxes)
+{
+    if (int e = check_ctx(c)) return e;
+    XferPlan P;
+    if (int e = xfer_plan(c, n_ids, ids, (const void *const *)boxes, P)) return e;
+    data_written(c);
+    double *dev = nullptr;
+    if (int e = ensure_var(c, iv, &dev)) return e;
+    if (int e = ensure_stage(c, (size_t)c->n_slots * c->stride)) return e;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));   // the staging buffer may still be read by a device copy
+    std::atomic<int> err{hipSuccess};
+    xfer_threads(
+        P, [](int) {},
+        [&](int k) {
+            const int s = P.slot_of[k];
+            const int len = box_len(c->dim, c->slot_nc[s]);
+            double *st = c->h_stage + (size_t)s * c->stride;
+            std::memcpy(st, boxes[k], len * sizeof(double));
+            if (len < c->stride) std::memset(st + len, 0, (c->stride - len) * sizeof(double));
+        },
+        [&](int p) {
+            int lo, hi;
+            piece_slots(c, p, lo, hi);
+            const hipError_t r = copy_slot_runs(c, dev, P.mark, true, lo, hi);
+            if (r != hipSuccess) err = r;
+        });
+    if (err != hipSuccess) return c->fail(AFG_ERR_HIP, "upload: %s", hipGetErrorString((hipError_t)err.load()));
+    return 0;
+}
+
+int afg_download_boxes(afg_ctx *c, int iv, int n_ids, const int32_t *ids, double *const *boxes)
+{
+    if (int e = check_ctx(c)) return e;
+    XferPlan P;
+    if (int e = xfer_plan(c, n_ids, ids, (const void *const *)boxes, P)) return e;
+    double *dev = nullptr;
+    if (int e = ensure_var(c, iv, &dev)) return e;
+    if (int e = ensure_stage(c, (size_t)c->n_slots * c->stride)) return e;
+    for (int p = 0; p < XFER_PIECES; p++) {
+        if (!c->xfer_ev[p]) HIP_TRY(c, hipEventCreateWithFlags(&c->xfer_ev[p], hipEventDisableTiming));
+        int lo, hi;
+        piece_slots(c, p, lo, hi);
+        HIP_TRY(c, copy_slot_runs(c, dev, P.mark, false, lo, hi));
+        HIP_TRY(c, hipEventRecord(c->xfer_ev[p], c->stream));
+    }
+    std::atomic<int> err{hipSuccess};
+    xfer_threads(
+        P,
+        [&](int p) {
+            const hipError_t r = hipEventSynchronize(c->xfer_ev[p]);
+            if (r != hipSuccess) err = r;
+        },
+        [&](int k) {
+            if (err != hipSuccess) return;
+            const int s = P.slot_of[k];
+            std::memcpy(boxes[k], c->h_stage + (size_t)s * c->stride, box_len(c->dim, c->slot_nc[s]) * sizeof(double));
+        },
+        [](int) {});
+    if (err != hipSuccess) return c->fail(AFG_ERR_HIP, "download: %s", hipGetErrorString((hipError_t)err.load()));
+    return 0;
+}
+
+int afg_upload_cc(afg_ctx *c, int iv, int n_ids, const int32_t *ids, const double *packed)
+{
+    if (int e = check_ctx(c)) return e;
+    std::vector<const double *> boxes;
+    if (int e = packed_boxes(c, n_ids, ids, packed, boxes)) return e;
+    return afg_upload_boxes(c, iv, n_ids, ids, boxes.data());
 }
 
 int afg_download_cc(afg_ctx *c, int iv, int n_ids, const int32_t *ids, double *packed)
 {
     if (int e = check_ctx(c)) return e;
-    double *dev = nullptr;
-    if (int e = ensure_var(c, iv, &dev)) return e;
-    if (int e = ensure_stage(c, (size_t)c->n_slots * c->stride)) return e;
-    std::vector<char> mark(c->n_slots, 0);
-    for (int k = 0; k < n_ids; k++) {
-        const int id = ids[k];
-        if (id < 1 || id > c->max_id || c->id2slot[id] < 0) return c->fail(AFG_ERR_ARG, "no box with id %d", id);
-        mark[c->id2slot[id]] = 1;
-    }
-    if (int e = copy_runs(c, dev, mark, false, c->stride)) return e;
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    size_t pos = 0;
-    for (int k = 0; k < n_ids; k++) {
-        const int s = c->id2slot[ids[k]];
-        const int len = box_len(c->dim, c->slot_nc[s]);
-        std::memcpy(packed + pos, c->h_stage + (size_t)s * c->stride, len * sizeof(double));
-        pos += len;
-    }
-    return 0;
+    std::vector<const double *> boxes;
+    if (int e = packed_boxes(c, n_ids, ids, packed, boxes)) return e;
+    return afg_download_boxes(c, iv, n_ids, ids, const_cast<double *const *>(boxes.data()));
 }
 
 static int face_len(int dim, int n) { return dim * (n + 1) * (dim == 3 ? n * n : n); }

@@ -136,9 +136,6 @@ module m_a$D_mg
   integer(c_int32_t), allocatable, save :: recs_dev(:)
   integer(c_int32_t), allocatable, save :: ids_dev(:)
   integer(c_int64_t), save :: n_values = 0
-  ! packed cell data of one variable on its way to or from the device, kept
-  ! between calls (a new buffer per call would fault in its pages each time)
-  real(c_double), allocatable, save :: stage(:)
 
   integer, parameter :: rec_len = 3 + $D + 2**$D + 2*$D
 
@@ -347,7 +344,6 @@ contains
     ctx_coord = -1
     if (allocated(recs_dev)) deallocate(recs_dev)
     if (allocated(ids_dev)) deallocate(ids_dev)
-    if (allocated(stage)) deallocate(stage)
   end subroutine mg$D_device_release
 
   !> Map the options to the device, (re)build the device tree when its
@@ -480,90 +476,44 @@ contains
     end do
   end subroutine build_records
 
-  integer function box_len(box)
-    type(box$D_t), intent(in) :: box
-    box_len = (box%n_cell + 2)**$D
-  end function box_len
-
-  subroutine reserve_stage(n)
-    integer, intent(in) :: n
-    if (allocated(stage)) then
-       if (size(stage) >= n) return
-       deallocate(stage)
-    end if
-    allocate(stage(n))
-  end subroutine reserve_stage
-
-  !> box%cc(..., iv) of every box into the packed layout (one contiguous
-  !> column of nc+2 values at a time: no temporary arrays), then to the device
+  !> box%cc(..., iv) of every box to the device: the library copies each box's
+  !> slab from where it lies (no packing here)
   subroutine upload_var(tree, iv)
-    type(a$D_t), intent(in) :: tree
-    integer, intent(in)     :: iv
-    integer :: k, id, pos, m, j
-#if $D == 3
-    integer :: kk
-#endif
+    type(a$D_t), intent(in), target :: tree
+    integer, intent(in)             :: iv
+    type(c_ptr), allocatable        :: ptrs(:)
 
-    call reserve_stage(total_values(tree))
-    pos = 0
-    do k = 1, size(ids_dev)
-       id = ids_dev(k)
-       m = tree%boxes(id)%n_cell + 2
-#if $D == 2
-       do j = 0, m - 1
-          stage(pos+1:pos+m) = tree%boxes(id)%cc(:, j, iv)
-          pos = pos + m
-       end do
-#elif $D == 3
-       do kk = 0, m - 1
-          do j = 0, m - 1
-             stage(pos+1:pos+m) = tree%boxes(id)%cc(:, j, kk, iv)
-             pos = pos + m
-          end do
-       end do
-#endif
-    end do
-    call afg_check(afg_upload_cc(ctx, iv, size(ids_dev), ids_dev, stage), ctx, "afg_upload_cc")
+    call box_pointers(tree, iv, ptrs)
+    call afg_check(afg_upload_boxes(ctx, iv, size(ids_dev), ids_dev, ptrs), ctx, "afg_upload_boxes")
   end subroutine upload_var
 
   subroutine download_var(tree, iv)
-    type(a$D_t), intent(inout) :: tree
-    integer, intent(in)        :: iv
-    integer :: k, id, pos, m, j
-#if $D == 3
-    integer :: kk
-#endif
+    type(a$D_t), intent(inout), target :: tree
+    integer, intent(in)                :: iv
+    type(c_ptr), allocatable           :: ptrs(:)
 
-    call reserve_stage(total_values(tree))
-    call afg_check(afg_download_cc(ctx, iv, size(ids_dev), ids_dev, stage), ctx, "afg_download_cc")
-    pos = 0
-    do k = 1, size(ids_dev)
-       id = ids_dev(k)
-       m = tree%boxes(id)%n_cell + 2
-#if $D == 2
-       do j = 0, m - 1
-          tree%boxes(id)%cc(:, j, iv) = stage(pos+1:pos+m)
-          pos = pos + m
-       end do
-#elif $D == 3
-       do kk = 0, m - 1
-          do j = 0, m - 1
-             tree%boxes(id)%cc(:, j, kk, iv) = stage(pos+1:pos+m)
-             pos = pos + m
-          end do
-       end do
-#endif
-    end do
+    call box_pointers(tree, iv, ptrs)
+    call afg_check(afg_download_boxes(ctx, iv, size(ids_dev), ids_dev, ptrs), ctx, "afg_download_boxes")
   end subroutine download_var
 
-  integer function total_values(tree)
-    type(a$D_t), intent(in) :: tree
-    integer :: k
-    total_values = 0
+  !> the address of box%cc(0, 0[, 0], iv) of each box of ids_dev: the start of
+  !> the box's contiguous array of variable iv
+  subroutine box_pointers(tree, iv, ptrs)
+    type(a$D_t), intent(in), target       :: tree
+    integer, intent(in)                   :: iv
+    type(c_ptr), allocatable, intent(out) :: ptrs(:)
+    integer                               :: k
+
+    allocate(ptrs(size(ids_dev)))
     do k = 1, size(ids_dev)
-       total_values = total_values + box_len(tree%boxes(ids_dev(k)))
+#if $D == 2
+       ptrs(k) = c_loc(tree%boxes(ids_dev(k))%cc(0, 0, iv))
+#elif $D == 3
+       ptrs(k) = c_loc(tree%boxes(ids_dev(k))%cc(0, 0, 0, iv))
+#endif
     end do
-  end function total_values
+  end subroutine box_pointers
+
 
   !> Boundary table: for every physical face, the type and values the user's
   !> sides_bc writes (built-in zero conditions need no call).  sides_bc is

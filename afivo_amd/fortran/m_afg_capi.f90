@@ -136,6 +136,22 @@ module m_afg_capi
        real(c_double), intent(out) :: packed(*)
      end function afg_download_cc
 
+     integer(c_int) function afg_upload_boxes(ctx, iv, n_ids, ids, boxes) bind(c, name="afg_upload_boxes")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: iv, n_ids
+       integer(c_int32_t), intent(in) :: ids(*)
+       type(c_ptr), intent(in) :: boxes(*)
+     end function afg_upload_boxes
+
+     integer(c_int) function afg_download_boxes(ctx, iv, n_ids, ids, boxes) bind(c, name="afg_download_boxes")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: iv, n_ids
+       integer(c_int32_t), intent(in) :: ids(*)
+       type(c_ptr), intent(in) :: boxes(*)
+     end function afg_download_boxes
+
      integer(c_int) function afg_mg_init(opts) bind(c, name="afg_mg_init")
        import :: c_int, afg_mg_opts
        type(afg_mg_opts), intent(inout) :: opts

@@ -226,9 +226,18 @@ int afg_set_bc(afg_ctx *ctx, int id, int nb, int bc_type, const double *b);
  * whole cc(0:nc_l+1, ..., iv) array, Fortran order). */
 int afg_packed_size(const afg_ctx *ctx, int n_ids, const int32_t *ids, int64_t *n_values);
 /* Copy variable iv of boxes ids[0..n_ids) from host (packed) to the device,
- * and back. */
+ * and back.  The host side of a copy runs on up to 16 threads and overlaps
+ * the PCIe transfer; an upload returns once the values are staged (the
+ * packed array may be reused), a download once they are in `packed`. */
 int afg_upload_cc(afg_ctx *ctx, int iv, int n_ids, const int32_t *ids, const double *packed);
 int afg_download_cc(afg_ctx *ctx, int iv, int n_ids, const int32_t *ids, double *packed);
+/* The same from and to separate host arrays, one per box: boxes[k] is box
+ * ids[k]'s whole cc(0:nc_l+1, ..., iv) array of the variable, x fastest -- the
+ * box%cc(..., iv) slab of an afivo box, so that a Fortran caller passes
+ * c_loc(box%cc(0, 0, iv)) and packs nothing (the drop-in's transfers of box%cc,
+ * m_aX_mg.f90 mg$D_fas_fmg / mg$D_fas_vcycle).  Each id at most once. */
+int afg_upload_boxes(afg_ctx *ctx, int iv, int n_ids, const int32_t *ids, const double *const *boxes);
+int afg_download_boxes(afg_ctx *ctx, int iv, int n_ids, const int32_t *ids, double *const *boxes);
 /* Whole-array device copy cc(..., iv_to) = cc(..., iv_from) on every box, and
  * cc(..., iv) = value on every box (a$D_box_copy_cc / a$D_box_clear_cc). */
 int afg_copy_cc(afg_ctx *ctx, int iv_from, int iv_to);

@@ -162,6 +162,7 @@ hipError_t hipEventCreate(hipEvent_t *e)
     *e = new EmulEvent;
     return hipSuccess;
 }
+hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { return hipEventCreate(e); }
 hipError_t hipEventDestroy(hipEvent_t e)
 {
     delete e;

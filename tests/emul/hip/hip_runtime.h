@@ -45,6 +45,8 @@ hipError_t hipMemcpyAsync(void *dst, const void *src, size_t bytes, hipMemcpyKin
 hipError_t hipMemset(void *p, int v, size_t bytes);
 hipError_t hipMemsetAsync(void *p, int v, size_t bytes, hipStream_t s);
 hipError_t hipEventCreate(hipEvent_t *e);
+#define hipEventDisableTiming 2u
+hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned flags);
 hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipEventRecordWithFlags(hipEvent_t e, hipStream_t s, unsigned flags);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s);

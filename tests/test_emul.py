@@ -72,6 +72,7 @@ from test_transfers_gpu import (  # noqa: E402,F401
     test_tree_arithmetic,
 )
 from test_transfers_gpu import (  # noqa: E402,F401
+    test_box_array_transfers,
     test_consistent_fluxes_match_oracle_full_size,
     test_consistent_fluxes_match_reference,
 )

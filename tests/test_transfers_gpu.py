@@ -179,3 +179,51 @@ def test_consistent_fluxes_match_oracle_full_size(dim):
         assert_bitexact(got, want, f"face variable {k}")
         assert np.count_nonzero(want != (f1 if k == 1 else f2)) > 0
     dt.close()
+
+
+def _box_ptrs(arrays):
+    from afivo_amd import _lib
+    return (_lib.F64P * len(arrays))(*[a.ctypes.data_as(_lib.F64P) for a in arrays])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_box_array_transfers(dim):
+    """afg_upload_boxes / afg_download_boxes (one host array per box, as the
+    Fortran drop-in passes box%cc slabs): on the benchmark tree, boxes given
+    in a shuffled order and a subset, against the packed transfers, bit for
+    bit; other boxes keep their values; a repeated id, a missing array or an
+    unknown id is an error."""
+    from afivo_amd import _lib
+    p = P.c1() if dim == 2 else P.c2()
+    st = p.state
+    rng = np.random.default_rng(7)
+    a = rng.standard_normal(st.n_values())
+    b = rng.standard_normal(st.n_values())
+    dt = M.DeviceTree.from_state(st, len(st.data))
+    dt.upload(1, a)
+    off = st.offsets()
+    order = rng.permutation(st.n_boxes)[: st.n_boxes * 2 // 3]
+    ids = np.ascontiguousarray(st.ids[order], dtype=np.int32)
+    boxes = [np.ascontiguousarray(b[off[r]:off[r + 1]]) for r in order]
+    L = _lib.load()
+    dt._c(L.afg_upload_boxes(dt.h, 1, ids.size, ids.ctypes.data_as(_lib.I32P), _box_ptrs(boxes)))
+    want = a.copy()
+    for r in order:
+        want[off[r]:off[r + 1]] = b[off[r]:off[r + 1]]
+    assert_bitexact(dt.download(1), want, "upload_boxes")
+    got = [np.full_like(x, np.nan) for x in boxes]
+    dt._c(L.afg_download_boxes(dt.h, 1, ids.size, ids.ctypes.data_as(_lib.I32P), _box_ptrs(got)))
+    for r, g in zip(order, got):
+        assert_bitexact(g, b[off[r]:off[r + 1]], f"download_boxes box {st.ids[r]}")
+    dup = np.ascontiguousarray(np.r_[ids[:3], ids[:1]], dtype=np.int32)
+    with pytest.raises(AfgError, match="twice"):
+        dt._c(L.afg_upload_boxes(dt.h, 1, 4, dup.ctypes.data_as(_lib.I32P), _box_ptrs(boxes[:3] + boxes[:1])))
+    ptrs = _box_ptrs(boxes[:2])
+    ptrs[1] = _lib.F64P()
+    with pytest.raises(AfgError, match="no array"):
+        dt._c(L.afg_download_boxes(dt.h, 1, 2, ids.ctypes.data_as(_lib.I32P), ptrs))
+    bad = np.array([st.ids.max() + 5], dtype=np.int32)
+    with pytest.raises(AfgError, match="no box"):
+        dt._c(L.afg_upload_boxes(dt.h, 1, 1, bad.ctypes.data_as(_lib.I32P), _box_ptrs(boxes[:1])))
+    assert_bitexact(dt.download(1), want, "after the rejected calls")
+    dt.close()
