@@ -21,6 +21,13 @@ a$D_prolong0/1/2_from, and face variable 1 (the gradient of phi) changed
 by a$D_consistent_fluxes; stored per routine (`<name>.npz`): the packed
 positions it changed and their values.
 
+VTK output (the VTKS cases): oracle/_ref/vtk_ref_{2,3}d (the reference's XML
+writer m_vtk, with the arrays assembled as a$D_write_vtk assembles them) on a
+small tree with three cell variables and one face variable; stored per case
+(`<name>.npz`): the tree and its cell data, the face variable, and the
+written .vtu file's length, SHA-256 and XML header (the text before the raw
+appended block).
+
 Usage: python tests/golden/make_golden.py [case ...]   (default: all)
 """
 import gzip
@@ -183,6 +190,35 @@ def make_xfer(name, base):
     shutil.rmtree(work)
 
 
+# name -> (dim, r0 of the domain [r0, r0 + 1]^dim, cylindrical)
+VTKS = {
+    "vtk_g2d": (2, 0.25, False),
+    "vtk_g2d_cyl": (2, 0.0, True),
+    "vtk_g3d": (3, -0.5, False),
+}
+
+
+def make_vtk(name, dim, r0, cyl):
+    import hashlib
+    import subprocess
+    from afivo_amd.treefile import read_tree_file
+    work = tempfile.mkdtemp(prefix="afg_golden_")
+    out = os.path.join(work, name)
+    exe = os.path.join(oracle.REF_DIR, f"vtk_ref_{dim}d")
+    subprocess.run([exe, out, repr(r0)] + (["cyl"] if cyl else []), check=True, cwd=work)
+    st = read_tree_file(out + ".tree")
+    fc = np.fromfile(out + ".fc", dtype="<f8")
+    vtu = open(out + ".vtu", "rb").read()
+    head = vtu[:vtu.index(b"<AppendedData")].decode()
+    meta = dict(dim=dim, nc=st.nc, min_lvl=st.min_lvl, highest_lvl=st.highest_lvl, dr_base=st.dr_base,
+                r0=r0, cyl=cyl, n_cycle=7, time=0.125, cc_names=["phi", "rhs", "tmp"], fc_names=["flux"],
+                ixs_fc=[1], vtu_len=len(vtu), vtu_sha256=hashlib.sha256(vtu).hexdigest(), vtu_head=head)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), meta=np.array(json.dumps(meta, sort_keys=True)),
+                        recs=st.recs, data=np.stack(st.data), fc=fc)
+    print(name, "vtu", len(vtu), "bytes")
+    shutil.rmtree(work)
+
+
 def main(names):
     for name, (dim, cfg) in CASES.items():
         if names and name not in names:
@@ -204,6 +240,9 @@ def main(names):
     for name, base in XFERS.items():
         if not names or name in names:
             make_xfer(name, base)
+    for name, args in VTKS.items():
+        if not names or name in names:
+            make_vtk(name, *args)
 
 
 if __name__ == "__main__":
