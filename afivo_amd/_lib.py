@@ -47,6 +47,7 @@ SIGNATURES = {
     "afg_gc_tree": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "afg_gc_level": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "afg_restrict_tree": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
+    "afg_gc2_boxes": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, I32P, F64P]),
     "afg_prolong_level": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "afg_add_cc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
     "afg_sub_cc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),

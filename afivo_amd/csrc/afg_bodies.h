@@ -1584,6 +1584,65 @@ AFG_DEV void gc_body(const DevTree &T, const LevelArgs &L, double *var, int rb, 
     box[gi + s * (gj + s * gk)] = g;
 }
 
+// The second ghost layer of a box: a$D_gc2_box (m_aX_gc.f90:420-456) with
+// a$D_gc2_prolong1 at refinement boundaries (:498-597) and
+// a$D_bc2_neumann_zero at physical boundaries (:600-634), one thread per
+// value of gc_data(nc[, nc], 2*dim) (first tangential index fastest, then the
+// face), written to out + slot * gstride.  A same-level neighbour gives its
+// cell nc-1 (low face) or 2 (high face); a physical boundary the box's own
+// cell 2 (low) or nc-1 (high); a refinement boundary the linear
+// interpolation from the parent's neighbour across the face (ghost cells of
+// that box included), with the child offset of a$D_get_child_offset(box, nb).
+template <int DIM>
+AFG_DEV void gc2_body(const DevTree &T, const LevelArgs &L, const double *var, double *out, int gstride, int gid)
+{
+    const int nc = L.nc;
+    const int per_face = DIM == 2 ? nc : nc * nc;
+    const int per_box = 2 * DIM * per_face;
+    const int b = gid / per_box;
+    const int r = gid - b * per_box;
+    const int f = r / per_face;
+    const int t = r - f * per_face;
+    const int dn = f >> 1, high = f & 1;
+    const int ta = dn == 0 ? 1 : 0, tb = dn == 2 ? 1 : 2;   // tangential axes, in dimension order
+    const int slot = L.slot0 + b;
+    const int nbv = T.nb[slot * (2 * DIM) + f];
+    int x[3] = {0, 0, 0};
+    x[ta] = t % nc + 1;
+    x[tb] = DIM == 3 ? t / nc + 1 : 0;
+    double g;
+    if (nbv >= 0 || nbv <= NB_BC0) {
+        const bool own = nbv <= NB_BC0;
+        x[dn] = (high != (int)own) ? 2 : nc - 1;
+        const int s = nc + 2;
+        const double *box = var + (size_t)(own ? slot : nbv) * T.stride;
+        g = box[x[0] + s * (x[1] + s * x[2])];
+    } else {
+        const int p = T.parent[slot];
+        const double *q = var + (size_t)T.nb[p * (2 * DIM) + f] * T.stride;
+        const int co = T.choff[slot];
+        int off[3] = {co & 1023, (co >> 10) & 1023, DIM == 3 ? (co >> 20) & 1023 : 0};
+        off[dn] += high ? -L.pnc : L.pnc;
+        x[dn] = high ? nc + 2 : -1;                         // the second ghost cell's index
+        int c1[3] = {0, 0, 0}, c2[3] = {0, 0, 0};
+        for (int d = 0; d < DIM; d++) {
+            c1[d] = off[d] + ((x[d] + 1) >> 1);
+            c2[d] = c1[d] + 1 - 2 * (x[d] & 1);
+        }
+        const int ps = L.pnc + 2;
+        auto at = [&](int d2) {   // the parent's neighbour at c1, dimension d2 (if >= 0) at c2
+            const int y0 = d2 == 0 ? c2[0] : c1[0], y1 = d2 == 1 ? c2[1] : c1[1];
+            const int y2 = DIM == 3 ? (d2 == 2 ? c2[2] : c1[2]) : 0;
+            return q[y0 + ps * (y1 + ps * y2)];
+        };
+        if (DIM == 2)
+            g = 0.5 * at(-1) + 0.25 * at(0) + 0.25 * at(1);
+        else
+            g = 0.25 * at(-1) + 0.25 * at(0) + 0.25 * at(1) + 0.25 * at(2);
+    }
+    out[(size_t)slot * gstride + r] = g;
+}
+
 // flux_from_children (m_aX_core.f90:838-943) for one face cell of a leaf
 // neighbour: the mean of the 2^(dim-1) fine fluxes of the parent's child on
 // that face, 0.5 (a + b) in 2D and 0.25 (((a + b) + c) + d) in 3D (first

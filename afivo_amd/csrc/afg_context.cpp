@@ -2828,6 +2828,43 @@ int afg_gc_tree(afg_ctx *c, int iv, int rb, int bc)
     return launched(c, "afg_gc_tree");
 }
 
+int afg_gc2_boxes(afg_ctx *c, int iv, int n_ids, const int32_t *ids, double *packed)
+{
+    if (int e = check_ctx(c)) return e;
+    if (n_ids < 0 || (n_ids > 0 && (!ids || !packed))) return c->fail(AFG_ERR_ARG, "gc2: invalid box list");
+    for (int k = 0; k < n_ids; k++)
+        if (ids[k] < 1 || ids[k] > c->max_id || c->id2slot[ids[k]] < 0)
+            return c->fail(AFG_ERR_ARG, "no box with id %d", ids[k]);
+    if (!c->rb_parent_nb)
+        return c->fail(AFG_ERR_ARG, "gc2_prolong1: a refinement boundary without a neighbour of the parent");
+    double *v = nullptr;
+    if (int e = ensure_var(c, iv, &v)) return e;
+    const int gstride = 2 * c->dim * (c->dim == 2 ? c->nc : c->nc * c->nc);
+    const size_t n = (size_t)c->n_slots * gstride;
+    double *d_out = nullptr;
+    HIP_TRY(c, ctx_malloc(c, &d_out, n * sizeof(double)));
+    const DevTree T = dev_tree(c);
+    for (int lvl = c->min_lvl; lvl <= c->highest_lvl; lvl++)
+        launch_gc2(c->dim, c->stream, T, level_args(c, lvl), v, d_out, gstride);
+    int e = launched(c, "afg_gc2_boxes");
+    std::vector<double> h(n);
+    if (!e) {
+        hipError_t r = hipMemcpyAsync(h.data(), d_out, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (r == hipSuccess) r = hipStreamSynchronize(c->stream);
+        if (r != hipSuccess) e = c->fail(AFG_ERR_HIP, "afg_gc2_boxes: %s", hipGetErrorString(r));
+    }
+    ctx_free(c, d_out);
+    if (e) return e;
+    size_t pos = 0;
+    for (int k = 0; k < n_ids; k++) {
+        const int sl = c->id2slot[ids[k]], m = c->slot_nc[sl];
+        const int len = 2 * c->dim * (c->dim == 2 ? m : m * m);
+        std::memcpy(packed + pos, h.data() + (size_t)sl * gstride, len * sizeof(double));
+        pos += len;
+    }
+    return 0;
+}
+
 int afg_restrict_tree(afg_ctx *c, int iv, int iv_to)
 {
     if (int e = check_ctx(c)) return e;

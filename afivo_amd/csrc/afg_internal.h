@@ -73,6 +73,9 @@ void launch_prolong_level(int dim, hipStream_t s, const DevTree &T, const LevelA
 // (RB_*) and boundary condition bc (0: the table; -1 / -2: Dirichlet /
 // Neumann zero)
 void launch_gc(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, double *var, int rb, int bc);
+// the second ghost layer of every box of L (gc2_body) into out, gstride values per slot
+void launch_gc2(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const double *var, double *out,
+                int gstride);
 // a$D_consistent_fluxes for the n records of one level (face pool fc); on
 // a cylindrical tree cyl_rmin holds r_min(1) per slot (else nullptr) and dr
 // is the level's cell size

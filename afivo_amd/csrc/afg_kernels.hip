@@ -227,6 +227,13 @@ __global__ __launch_bounds__(BLOCK) void k_gc(DevTree T, LevelArgs L, double *va
 }
 
 template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_gc2(DevTree T, LevelArgs L, const double *var, double *out, int gstride)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < fill_threads<DIM>(L.n, L.nc)) gc2_body<DIM>(T, L, var, out, gstride, gid);
+}
+
+template <int DIM>
 __global__ __launch_bounds__(BLOCK) void k_flux(const FluxRec *recs, int n, int nc, int fstride, double *fc,
                                                 const double *cyl_rmin, double dr)
 {
@@ -735,6 +742,14 @@ void launch_gc(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, dou
     if (L.n <= 0) return;
     if (dim == 2) k_gc<2><<<n_blocks(fill_threads<2>(L.n, L.nc)), BLOCK, 0, s>>>(T, L, var, rb, bc);
     else k_gc<3><<<n_blocks(fill_threads<3>(L.n, L.nc)), BLOCK, 0, s>>>(T, L, var, rb, bc);
+}
+
+void launch_gc2(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const double *var, double *out,
+                int gstride)
+{
+    if (L.n <= 0) return;
+    if (dim == 2) k_gc2<2><<<n_blocks(fill_threads<2>(L.n, L.nc)), BLOCK, 0, s>>>(T, L, var, out, gstride);
+    else k_gc2<3><<<n_blocks(fill_threads<3>(L.n, L.nc)), BLOCK, 0, s>>>(T, L, var, out, gstride);
 }
 
 void launch_flux(int dim, hipStream_t s, const FluxRec *recs, int n, int nc, int fstride, double *fc,

@@ -75,6 +75,17 @@ class DeviceTree:
     def gc_level(self, iv, lvl, rb=0, bc=0):
         self._c(_lib.load().afg_gc_level(self.h, int(iv), int(lvl), int(rb), int(bc)))
 
+    def gc2(self, iv, ids=None):
+        """a$D_gc2_box of variable iv for every box (or ids): per box
+        gc_data(nc_l[, nc_l], 2*dim) in Fortran order, concatenated."""
+        from .treefile import level_ncell
+        ids = self.ids if ids is None else np.ascontiguousarray(ids, dtype=np.int32)
+        lvl_of = dict(zip(self.records[:, 0].tolist(), self.records[:, 1].tolist()))
+        nc_l = level_ncell(self.nc, np.array([lvl_of.get(int(i), 1) for i in ids], dtype=np.int64))
+        out = np.empty(int((2 * self.dim * nc_l ** (self.dim - 1)).sum()), dtype=np.float64)
+        self._c(_lib.load().afg_gc2_boxes(self.h, int(iv), ids.size, _i32(ids), _f64(out)))
+        return out
+
     def restrict_tree(self, iv, iv_to=None):
         self._c(_lib.load().afg_restrict_tree(self.h, int(iv), int(iv if iv_to is None else iv_to)))
 

@@ -185,6 +185,14 @@ int afg_forest_max_abs(afg_ctx *ctx, int iv, int n_trees, double *out);
  * one level of it (the loop body of a$D_gc_tree, :38-43). */
 int afg_gc_tree(afg_ctx *ctx, int iv, int rb, int bc);
 int afg_gc_level(afg_ctx *ctx, int iv, int lvl, int rb, int bc);
+/* a$D_gc2_box (m_aX_gc.f90:420-456) of variable iv for the boxes ids: the
+ * second layer of ghost values beyond each face, with a$D_gc2_prolong1 at
+ * refinement boundaries (:498-597) and a$D_bc2_neumann_zero at physical
+ * boundaries (:600-634), packed per box as gc_data(nc_l[, nc_l], 2*dim) in
+ * Fortran order (first tangential index fastest, then the face).  It reads
+ * the neighbours' cells and, at refinement boundaries, the parent's
+ * neighbour with its ghost cells: fill those first (afg_gc_tree). */
+int afg_gc2_boxes(afg_ctx *ctx, int iv, int n_ids, const int32_t *ids, double *packed);
 /* a$D_restrict_tree (m_aX_restrict.f90:47-57): children of every parent,
  * highest level first, variable iv restricted into iv_to of the parents. */
 int afg_restrict_tree(afg_ctx *ctx, int iv, int iv_to);

@@ -1676,3 +1676,75 @@ int orc_box_call(orc_tree *t, const orc_mg_opts *o, int what, int kind, int id, 
     }
     return -1;
 }
+
+/* a$D_gc2_box (m_aX_gc.f90:420-456) of variable iv on box id into
+ * gc(nc[, nc], 2*dim) (Fortran order): sides2_from_nb (:459-495) from a
+ * same-level neighbour, a$D_gc2_prolong1 (:498-597) at a refinement boundary,
+ * a$D_bc2_neumann_zero (:600-634) at a physical boundary. */
+static void gc2_box(orc_tree *t, int id, int iv, double *gc)
+{
+    int n = t->ncell[id], per = t->dim == 2 ? n : n * n;
+    const double *own = cc(t, id, iv);
+    for (int f = 0; f < t->nnb; f++) {
+        int nb_id = t->nbr[id * 6 + f], dn = f / 2, high = f % 2;
+        double *side = gc + (size_t)f * per;
+        if (nb_id != 0) {
+            /* sides2_from_nb: the neighbour's cell nc-1 (low face) or 2 (high
+             * face); bc2_neumann_zero: the box's own cell 2 (low) or nc-1 (high) */
+            const double *src = nb_id > 0 ? cc(t, nb_id, iv) : own;
+            int m = (nb_id > 0) == (high != 0) ? 2 : n - 1;
+            for (int b = 1; b <= (t->dim == 3 ? n : 1); b++)
+                for (int a = 1; a <= n; a++) {
+                    int x[3] = {0, 0, 0};
+                    x[dn] = m;
+                    x[dn == 0 ? 1 : 0] = a;
+                    if (t->dim == 3) x[dn == 2 ? 1 : 2] = b;
+                    side[(a - 1) + n * (b - 1)] = src[ix3(n, x[0], x[1], x[2])];
+                }
+            continue;
+        }
+        /* a$D_gc2_prolong1: ix_offset = a$D_get_child_offset(box, nb), the
+         * offset minus dix(nb) * n_cell (lvl > 1) or n_cell / 2 (lvl <= 1) */
+        int p = t->parent[id], pnb = t->nbr[p * 6 + f], np = t->ncell[pnb];
+        int off[3];
+        child_offset(t, id, off);
+        off[dn] -= (high ? 1 : -1) * (t->lvl[id] > 1 ? n : n / 2);
+        const double *q = cc(t, pnb, iv);
+        int ix = high ? n + 2 : -1;
+        int c1n = off[dn] + ((ix + 1) >> 1), c2n = c1n + 1 - 2 * (ix & 1);
+        for (int b = 1; b <= (t->dim == 3 ? n : 1); b++)
+            for (int a = 1; a <= n; a++) {
+                int ta = dn == 0 ? 1 : 0, tb = dn == 2 ? 1 : 2;
+                int c1[3] = {0, 0, 0}, c2[3] = {0, 0, 0};
+                c1[dn] = c1n;
+                c2[dn] = c2n;
+                c1[ta] = off[ta] + ((a + 1) >> 1);
+                c2[ta] = c1[ta] + 1 - 2 * (a & 1);
+                if (t->dim == 3) {
+                    c1[tb] = off[tb] + ((b + 1) >> 1);
+                    c2[tb] = c1[tb] + 1 - 2 * (b & 1);
+                }
+                /* terms in the order i, j(, k): cc(c1), cc(c2 in x), cc(c2 in y)(, cc(c2 in z)) */
+                double v0 = q[ix3(np, c1[0], c1[1], c1[2])];
+                double vx = q[ix3(np, c2[0], c1[1], c1[2])];
+                double vy = q[ix3(np, c1[0], c2[1], c1[2])];
+                double g;
+                if (t->dim == 2)
+                    g = 0.5 * v0 + 0.25 * vx + 0.25 * vy;
+                else
+                    g = 0.25 * v0 + 0.25 * vx + 0.25 * vy + 0.25 * q[ix3(np, c1[0], c1[1], c2[2])];
+                side[(a - 1) + n * (b - 1)] = g;
+            }
+    }
+}
+
+int orc_gc2(orc_tree *t, int iv, double *out)
+{
+    size_t pos = 0;
+    for (int r = 0; r < t->n_boxes; r++) {
+        int id = t->rec_ids[r], n = t->ncell[id];
+        gc2_box(t, id, iv, out + pos);
+        pos += (size_t)2 * t->dim * (t->dim == 2 ? n : n * n);
+    }
+    return 0;
+}

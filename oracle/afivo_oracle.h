@@ -80,6 +80,10 @@ int orc_correct_children_level(orc_tree *t, const orc_mg_opts *o, int lvl);
  * table; ORC_BC_*: that condition with b = 0 everywhere), a$D_restrict_tree,
  * and a$D_prolong0/1/2 into every box of level lvl (order 0/1/2). */
 int orc_gc_tree(orc_tree *t, int iv, int rb, int bc);
+/* a$D_gc2_box with a$D_gc2_prolong1 and a$D_bc2_neumann_zero of variable iv
+ * on every box, record order: per box gc_data(nc_l[, nc_l], 2*dim), Fortran
+ * order (the caller fills the first ghost layer first). */
+int orc_gc2(orc_tree *t, int iv, double *out);
 int orc_restrict_tree(orc_tree *t, int iv, int iv_to);
 int orc_prolong_level(orc_tree *t, int iv, int iv_to, int lvl, int order, int add);
 /* Face variables 1..ORC_MAX_FC: per box fx, fy (, fz), each (nc_l+1) *

@@ -66,6 +66,7 @@ def lib():
                                         ctypes.c_int]
         L.orc_update_coarse_level.argtypes = [P, ctypes.POINTER(MgOpts), ctypes.c_int]
         L.orc_set_tags.argtypes = [P, ctypes.POINTER(MgOpts)]
+        L.orc_gc2.argtypes = [P, ctypes.c_int, f64p]
         L.orc_box_call.argtypes = [P, ctypes.POINTER(MgOpts), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_int]
         _lib = L
@@ -141,6 +142,15 @@ class OracleTree:
         that condition with b = 0."""
         if lib().orc_gc_tree(self.h, iv, rb, bc):
             raise ValueError("orc_gc_tree failed")
+
+    def gc2(self, iv):
+        """a$D_gc2_box (gc2_prolong1, bc2_neumann_zero) of every box, record
+        order, per box gc_data(nc_l[, nc_l], 2*dim) in Fortran order."""
+        st = self.st
+        n = st.ncell()
+        out = np.empty(int((2 * st.dim * n ** (st.dim - 1)).sum()), dtype=np.float64)
+        lib().orc_gc2(self.h, iv, _f64(out))
+        return out
 
     def restrict_tree(self, iv, iv_to=None):
         lib().orc_restrict_tree(self.h, iv, iv if iv_to is None else iv_to)

@@ -384,6 +384,7 @@ contains
     call a$D_tree_copy_cc(tree, i_phi, i_x)
     call a$D_gc_tree(tree, i_x, a$D_gc_prolong0, a$D_bc_neumann_zero)
     call write_dump(trim(out_file) // '.gc0')
+    call write_gc2(trim(out_file) // '.gc2', i_x)
     call a$D_tree_copy_cc(tree, i_rhs, i_x)
     call a$D_restrict_tree(tree, i_x)
     call write_dump(trim(out_file) // '.rst')
@@ -431,6 +432,38 @@ contains
          box%cc(1:nc_b, 1:nc_b, 0:nc_b, i_phi)) / box%dr
 #endif
   end subroutine set_gradient
+
+  !> a$D_gc2_box of variable iv (a$D_gc2_prolong1 at refinement boundaries,
+  !> a$D_bc2_neumann_zero at physical ones) for every box, boxes in the order
+  !> of write_dump: gc_data(nc[, nc], 2*$D) of each
+  subroutine write_gc2(fname, iv)
+    character(len=*), intent(in) :: fname
+    integer, intent(in) :: iv
+    integer :: lu, lvl, n, id, nc_b
+#if $D == 2
+    real(dp), allocatable :: gc(:, :)
+#elif $D == 3
+    real(dp), allocatable :: gc(:, :, :)
+#endif
+
+    open(newunit=lu, file=fname, access='stream', form='unformatted', &
+         status='replace', action='write')
+    do lvl = lbound(tree%lvls, 1), tree%highest_lvl
+       do n = 1, size(tree%lvls(lvl)%ids)
+          id = tree%lvls(lvl)%ids(n)
+          nc_b = tree%boxes(id)%n_cell
+#if $D == 2
+          allocate(gc(nc_b, 2*$D))
+#elif $D == 3
+          allocate(gc(nc_b, nc_b, 2*$D))
+#endif
+          call a$D_gc2_box(tree%boxes, id, iv, a$D_gc2_prolong1, a$D_bc2_neumann_zero, gc, nc_b)
+          write(lu) gc
+          deallocate(gc)
+       end do
+    end do
+    close(lu)
+  end subroutine write_gc2
 
   !> Face variable 1 of every box, boxes in the order of write_dump: fx, fy
   !> (, fz), each in Fortran order

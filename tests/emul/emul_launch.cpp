@@ -423,6 +423,18 @@ void launch_gc(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, dou
     });
 }
 
+void launch_gc2(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const double *var, double *out,
+                int gstride)
+{
+    if (L.n <= 0) return;
+    emul_submit(s, [=] {
+        if (dim == 2)
+            for (int g = 0; g < fill_threads<2>(L.n, L.nc); g++) gc2_body<2>(T, L, var, out, gstride, g);
+        else
+            for (int g = 0; g < fill_threads<3>(L.n, L.nc); g++) gc2_body<3>(T, L, var, out, gstride, g);
+    });
+}
+
 void launch_flux(int dim, hipStream_t s, const FluxRec *recs, int n, int nc, int fstride, double *fc,
                  const double *cyl_rmin, double dr)
 {

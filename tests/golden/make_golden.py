@@ -19,7 +19,9 @@ tree set from phi or rhs and changed by a$D_gc_tree (with a$D_gc_interp,
 a$D_gc_interp_lim, a$D_gc_prolong0), a$D_restrict_tree or
 a$D_prolong0/1/2_from, and face variable 1 (the gradient of phi) changed
 by a$D_consistent_fluxes; stored per routine (`<name>.npz`): the packed
-positions it changed and their values.
+positions it changed and their values; and the second ghost layer of
+a$D_gc2_box (a$D_gc2_prolong1, a$D_bc2_neumann_zero) of variable 4 after the
+gc0 step, every box (`gc2`).
 
 VTK output (the VTKS cases): oracle/_ref/vtk_ref_{2,3}d (the reference's XML
 writer m_vtk, with the arrays assembled as a$D_write_vtk assembles them) on a
@@ -184,6 +186,9 @@ def make_xfer(name, base):
     arrays["flx_pos"] = pos
     arrays["flx_val"] = after[pos]
     print(name, "flx changed", pos.size, "of", after.size)
+    # a$D_gc2_box of variable 4 after the gc0 step, every box (record order)
+    arrays["gc2"] = np.fromfile(prefix + ".gc2", dtype="<f8")
+    print(name, "gc2", arrays["gc2"].size, "values")
     meta = dict(base=base, dim=g["dim"], cfg=cfg, ops=XFER_OPS)
     np.savez_compressed(os.path.join(HERE, name + ".npz"), meta=np.array(json.dumps(meta, sort_keys=True)),
                         **arrays)

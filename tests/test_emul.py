@@ -73,6 +73,8 @@ from test_transfers_gpu import (  # noqa: E402,F401
 )
 from test_transfers_gpu import (  # noqa: E402,F401
     test_box_array_transfers,
+    test_second_ghost_layer_matches_oracle_full_size,
+    test_second_ghost_layer_matches_reference,
     test_consistent_fluxes_match_oracle_full_size,
     test_consistent_fluxes_match_reference,
 )

@@ -317,6 +317,25 @@ def test_oracle_transfers_match_reference(name):
 
 
 @pytest.mark.parametrize("name", XFERS)
+def test_oracle_second_ghost_layer_matches_reference(name):
+    """a$D_gc2_box with a$D_gc2_prolong1 and a$D_bc2_neumann_zero
+    (m_aX_gc.f90:420-634) of variable 4 after the gc0 step (phi with
+    a$D_gc_prolong0 and Neumann-zero ghosts), every box: the oracle equals
+    the reference's dump bit for bit.  The fixture trees hold refinement
+    boundaries, same-level neighbours and physical faces."""
+    meta, ops = load_xfer(name)
+    _, _, st, _ = load_golden(meta["base"])
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    ot = O.OracleTree(st, n_var=4)
+    if meta["cfg"].get("cyl"):
+        ot.set_coord(True)
+    pos, val = ops["gc0"]
+    ot.set(4, xfer_expected(st.data[0], pos, val))
+    assert (st.neighbors == 0).any() and (st.neighbors > 0).any() and (st.neighbors < 0).any()
+    assert_bitexact(ot.gc2(4), z["gc2"], name)
+
+
+@pytest.mark.parametrize("name", XFERS)
 def test_oracle_consistent_fluxes_match_reference(name):
     """a$D_consistent_fluxes on face variable 1 = the gradient of phi: the
     oracle reproduces the reference's faces bit for bit."""

@@ -45,6 +45,27 @@ def test_transfers_match_reference(name):
     dt.close()
 
 
+@pytest.mark.parametrize("name", XFERS)
+def test_second_ghost_layer_matches_reference(name):
+    """afg_gc2_boxes (a$D_gc2_box with a$D_gc2_prolong1 and
+    a$D_bc2_neumann_zero) of variable 4 after the gc0 step, every box: the
+    reference's dump bit for bit; the same for a subset of boxes given in
+    another order."""
+    meta, ops = load_xfer(name)
+    _, _, st, _ = load_golden(meta["base"])
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", name + ".npz"))
+    dt = M.DeviceTree.from_state(st, 4)
+    pos, val = ops["gc0"]
+    dt.upload(4, xfer_expected(st.data[0], pos, val))
+    assert_bitexact(dt.gc2(4), z["gc2"], name)
+    n = (2 * st.dim * st.ncell() ** (st.dim - 1)).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(n)])
+    pick = np.arange(st.n_boxes)[::-3]
+    want = np.concatenate([z["gc2"][off[r]:off[r + 1]] for r in pick])
+    assert_bitexact(dt.gc2(4, st.ids[pick]), want, f"{name} subset")
+    dt.close()
+
+
 def random_tree(dim, seed):
     """The benchmark tree of the dimension (2D: 951 boxes of 8^2; 3D: 5875
     boxes of 8^3) with random values in 3 variables and random Dirichlet /
@@ -104,6 +125,18 @@ def test_transfers_match_oracle_full_size(dim):
                     dt.prolong_level(1, iv_to, lvl, order, add)
                     ot.prolong_level(1, iv_to, lvl, order, add)
                 assert_bitexact(dt.download(iv_to), ot.get(iv_to), f"prolong order {order} add {add} to {iv_to}")
+    dt.close()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_second_ghost_layer_matches_oracle_full_size(dim):
+    """afg_gc2_boxes on the benchmark trees with random values in every cell
+    (ghosts included, as left by a random first layer): the oracle's
+    a$D_gc2_box of every box, bit for bit."""
+    st, _ = random_tree(dim, seed=60 + dim)
+    dt = M.DeviceTree.from_state(st)
+    ot = O.OracleTree(st)
+    assert_bitexact(dt.gc2(2), ot.gc2(2), f"{dim}D gc2")
     dt.close()
 
 
