@@ -31,7 +31,8 @@ def load_golden(name):
     st = {}
     for ext in ("in", "out"):
         raw = gzip.open(os.path.join(GOLDEN, f"{name}.{ext}.gz")).read()
-        path = os.path.join(GOLDEN, f"_tmp_{name}.{ext}")
+        # one file per process: tests of several workers load the same case
+        path = os.path.join(GOLDEN, f"_tmp_{name}.{os.getpid()}.{ext}")
         try:
             with open(path, "wb") as f:
                 f.write(raw)
