@@ -100,6 +100,9 @@ SIGNATURES = {
     "afg_mixed_solve": (ctypes.c_int, [P, ctypes.POINTER(MgOpts), ctypes.c_int, F64, ctypes.c_int,
                                        ctypes.c_int, ctypes.POINTER(ctypes.c_int), F64P]),
     "afg_set_bottom_cells": (ctypes.c_int, [P, I64, I64]),
+    "afg_set_persist": (ctypes.c_int, [P, ctypes.c_int, I64]),
+    "afg_persist_trace": (ctypes.c_int, [P, ctypes.c_int, I64P, ctypes.POINTER(ctypes.c_int)]),
+    "afg_get_persist": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), I64P, ctypes.POINTER(ctypes.c_int)]),
     "afg_set_graphs": (ctypes.c_int, [P, ctypes.c_int]),
     "afg_set_timing": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
     "afg_timing": (ctypes.c_int, [P, I64P, F64P, F64P]),

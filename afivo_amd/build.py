@@ -32,7 +32,7 @@ HIP_FLAGS = [
     "-I", os.path.join(ROOT, "include"),
 ]
 
-SOURCES = ["afg_kernels.hip", "afg_context.cpp", "afg_host.cpp"]
+SOURCES = ["afg_kernels.hip", "afg_persist.hip", "afg_context.cpp", "afg_host.cpp"]
 
 
 def _run(cmd):

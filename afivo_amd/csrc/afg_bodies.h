@@ -31,6 +31,27 @@ namespace afg {
 
 AFG_HD constexpr int ipow(int n, int d) { return d == 2 ? n * n : n * n * n; }
 
+// Loads of the multigrid variables (phi, rhs, tmp: the arrays that stages
+// write).  Where the workgroups of one launch exchange these arrays between
+// its steps (the persistent kernel, compiled with AFG_COHERENT_LOADS = 1),
+// they are read with device-scope loads: those miss in the compute unit's
+// vector L1, which is not kept coherent with the stores of other compute
+// units, and read the L2 that the workgroups of one XCD share.  Elsewhere
+// they are plain loads.  Tables, eps and the level set are never written
+// within a launch and are read directly.
+#ifndef AFG_COHERENT_LOADS
+#define AFG_COHERENT_LOADS 0
+#endif
+template <typename R>
+AFG_DEV R ldv(const R *p)
+{
+#if AFG_COHERENT_LOADS
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#else
+    return *p;
+#endif
+}
+
 // Where the boxes of one level live: box `slot` of variable X starts at
 // X + (slot - slot0) * stride.  The HBM pools use slot0 = 0 and the pool
 // stride for every level; an LDS copy of a level uses its own slot0 and the
@@ -80,7 +101,7 @@ AFG_DEV R ghost_value(const DevTree &T, const LevelViewT<R> &own, const LevelVie
     if (nbv >= 0) {
         const int m = high ? 1 : nc;
         const int a0 = dn == 0 ? m : i, a1 = dn == 1 ? m : j, a2 = dn == 2 ? m : k;
-        return own.phi[own.off(nbv) + a0 + s * (a1 + s * a2)];
+        return ldv(own.phi + own.off(nbv) + a0 + s * (a1 + s * a2));
     }
     if (nbv == NB_REFINE) {
         const int p = T.parent[slot];
@@ -91,20 +112,20 @@ AFG_DEV R ghost_value(const DevTree &T, const LevelViewT<R> &own, const LevelVie
         const int pi = (co & 1023) + ((g0 + 1) >> 1);
         const int pj = ((co >> 10) & 1023) + ((g1 + 1) >> 1);
         const int pk = DIM == 3 ? ((co >> 20) & 1023) + ((g2 + 1) >> 1) : 0;
-        const R P = par.phi[par.off(p) + pi + ps * (pj + ps * pk)];
+        const R P = ldv(par.phi + par.off(p) + pi + ps * (pj + ps * pk));
         const R *box = xbox ? xbox : own.phi + own.off(slot);
         const int inward = high ? -1 : 1;
         if (DIM == 2) {
             const int di = dn == 0 ? inward : -1 + 2 * (i & 1);
             const int dj = dn == 1 ? inward : -1 + 2 * (j & 1);
-            const R a = box[(i + di) + s * j];
-            const R b = box[i + s * (j + dj)];
+            const R a = ldv(box + (i + di) + s * j);
+            const R b = ldv(box + i + s * (j + dj));
             return R(0.5) * P + center - R(0.25) * (a + b);
         } else {
             const int d0 = dn == 0 ? inward : -1 + 2 * (i & 1);
             const int d1 = dn == 1 ? inward : -1 + 2 * (j & 1);
             const int d2 = dn == 2 ? inward : -1 + 2 * (k & 1);
-            const R d = box[(i + d0) + s * ((j + d1) + s * (k + d2))];
+            const R d = ldv(box + (i + d0) + s * ((j + d1) + s * (k + d2)));
             return R(0.5) * P + R(0.75) * center - R(0.25) * d;
         }
     }
@@ -136,13 +157,13 @@ AFG_DEV void stencil(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &
                      int q, int fly, R center, R u[6])
 {
     const int s = nc + 2;
-    u[0] = (i > 1 || !fly) ? box[q - 1] : ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, 0, nc, i, j, k, center);
-    u[1] = (i < nc || !fly) ? box[q + 1] : ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, 1, nc, i, j, k, center);
-    u[2] = (j > 1 || !fly) ? box[q - s] : ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, 2, nc, i, j, k, center);
-    u[3] = (j < nc || !fly) ? box[q + s] : ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, 3, nc, i, j, k, center);
+    u[0] = (i > 1 || !fly) ? ldv(box + q - 1) : ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, 0, nc, i, j, k, center);
+    u[1] = (i < nc || !fly) ? ldv(box + q + 1) : ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, 1, nc, i, j, k, center);
+    u[2] = (j > 1 || !fly) ? ldv(box + q - s) : ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, 2, nc, i, j, k, center);
+    u[3] = (j < nc || !fly) ? ldv(box + q + s) : ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, 3, nc, i, j, k, center);
     if (DIM == 3) {
-        u[4] = (k > 1 || !fly) ? box[q - s * s] : ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, 4, nc, i, j, k, center);
-        u[5] = (k < nc || !fly) ? box[q + s * s] : ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, 5, nc, i, j, k, center);
+        u[4] = (k > 1 || !fly) ? ldv(box + q - s * s) : ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, 4, nc, i, j, k, center);
+        u[5] = (k < nc || !fly) ? ldv(box + q + s * s) : ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, 5, nc, i, j, k, center);
     } else {
         u[4] = u[5] = 0;
     }
@@ -387,7 +408,7 @@ AFG_DEV void fill_body(const DevTree &T, const LevelArgs &L, const LevelViewT<R>
     const int slot = L.slot0 + b;
     const int s = nc + 2;
     R *box = own.phi + own.off(slot);
-    const R center = box[i + s * (j + s * k)];
+    const R center = ldv(box + i + s * (j + s * k));
     const R g = ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, f, nc, i, j, k, center);
     const int gi = dn == 0 ? (high ? nc + 1 : 0) : i;
     const int gj = dn == 1 ? (high ? nc + 1 : 0) : j;
@@ -405,7 +426,7 @@ AFG_DEV void diff_body(const DevTree &T, const ListArgs &P, const LevelViewT<R> 
     const int b = gid / per_box;
     const int q = gid - b * per_box;
     const size_t o = v.off(P.slots[b]) + q;
-    v.tmp[o] = v.phi[o] - v.tmp[o];
+    v.tmp[o] = ldv(v.phi + o) - ldv(v.tmp + o);
 }
 
 // ---------------------------------------------------------------------------
@@ -510,19 +531,19 @@ AFG_DEV void gsrb_body(const DevTree &T, const LevelArgs &L, const LevelViewT<R>
     const size_t base = own.off(slot);
     R *box = own.phi + base;
     const int q = i + s * (j + s * k);
-    const R center = box[q];
+    const R center = ldv(box + q);
     R u[6];
     stencil<DIM>(T, L, own, par, slot, box, nc, i, j, k, q, fly, center, u);
     if (lsf_box(T, own, slot)) {
-        box[q] = gsrb_value_lsf<DIM>(u, own.rhs[base + q], own.lsf + base, own.bval + base, q, s, L.dx2);
+        box[q] = gsrb_value_lsf<DIM>(u, ldv(own.rhs + base + q), own.lsf + base, own.bval + base, q, s, L.dx2);
         return;
     }
     const bool lpld = lpld_box(T, op_auto, slot);
     if (DIM == 2 && cyl_op(T, op_auto))
-        box[q] = gsrb_value_cyl(u, own.rhs[base + q], lpld ? own.eps + base : nullptr, q, s, L.dx2, lpld,
+        box[q] = gsrb_value_cyl(u, ldv(own.rhs + base + q), lpld ? own.eps + base : nullptr, q, s, L.dx2, lpld,
                                 T.cyl_ioff[slot] + i);
     else
-        box[q] = gsrb_value<DIM>(u, own.rhs[base + q], lpld ? own.eps + base : nullptr, q, s, L.dx2, lpld);
+        box[q] = gsrb_value<DIM>(u, ldv(own.rhs + base + q), lpld ? own.eps + base : nullptr, q, s, L.dx2, lpld);
 }
 
 // ---------------------------------------------------------------------------
@@ -888,14 +909,14 @@ AFG_DEV void residual_body(const DevTree &T, const LevelArgs &L, const LevelView
     const int q = i + s * (j + s * k);
     const bool lpld = lpld_box(T, op_auto, slot);
     R u[6];
-    stencil<DIM>(T, L, own, par, slot, box, nc, i, j, k, q, 0, box[q], u);
+    stencil<DIM>(T, L, own, par, slot, box, nc, i, j, k, q, 0, ldv(box + q), u);
     const R a = lsf_box(T, own, slot)
-                    ? box_op_lsf<DIM>(u, box[q], own.lsf + base, own.bval + base, q, s, L.inv_dr_sq)
+                    ? box_op_lsf<DIM>(u, ldv(box + q), own.lsf + base, own.bval + base, q, s, L.inv_dr_sq)
                 : DIM == 2 && cyl_op(T, op_auto)
-                    ? box_op_cyl(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld, T.cyl_ioff[slot] + i)
-                    : box_op<DIM>(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld);
+                    ? box_op_cyl(u, ldv(box + q), lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld, T.cyl_ioff[slot] + i)
+                    : box_op<DIM>(u, ldv(box + q), lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld);
     const R rh = DIM == 3 && own.rhs_rb ? own.rhs_rb[(size_t)slot * own.rb_stride + colour_index<DIM>(i, j, k, nc)]
-                            : own.rhs[base + q];
+                            : ldv(own.rhs + base + q);
     own.tmp[base + q] = rh - a;
 }
 
@@ -936,15 +957,15 @@ AFG_DEV void uc_child_body(const DevTree &T, const LevelArgs &L, const LevelView
         const int fk = DIM == 3 ? 2 * kk - 1 + ((m >> 2) & 1) : 0;
         const int q = fi + s * (fj + s * fk);
         R u[6];
-        stencil<DIM>(T, L, own, par, slot, ph, nc, fi, fj, fk, q, fly, ph[q], u);
-        const R res = rh[q] - (lsf ? box_op_lsf<DIM>(u, ph[q], own.lsf + base, own.bval + base, q, s, L.inv_dr_sq)
-                               : cyl ? box_op_cyl(u, ph[q], e, q, s, L.inv_dr_sq, lpld, T.cyl_ioff[slot] + fi)
-                                     : box_op<DIM>(u, ph[q], e, q, s, L.inv_dr_sq, lpld));
+        stencil<DIM>(T, L, own, par, slot, ph, nc, fi, fj, fk, q, fly, ldv(ph + q), u);
+        const R res = ldv(rh + q) - (lsf ? box_op_lsf<DIM>(u, ldv(ph + q), own.lsf + base, own.bval + base, q, s, L.inv_dr_sq)
+                               : cyl ? box_op_cyl(u, ldv(ph + q), e, q, s, L.inv_dr_sq, lpld, T.cyl_ioff[slot] + fi)
+                                     : box_op<DIM>(u, ldv(ph + q), e, q, s, L.inv_dr_sq, lpld));
         if (!keep_child_tmp) own.tmp[base + q] = res;
         sr = m == 0 ? res : sr + res;
-        sp = m == 0 ? ph[q] : sp + ph[q];
+        sp = m == 0 ? ldv(ph + q) : sp + ldv(ph + q);
         vr[m] = res;
-        vp[m] = ph[q];
+        vp[m] = ldv(ph + q);
         if (lsf_rstr) lc[m] = own.lsf[base + q];
     }
     const int p = T.parent[slot];
@@ -1151,31 +1172,31 @@ AFG_DEV void uc_parent_body(const DevTree &T, const LevelArgs &L, const LevelVie
         if (!is_par) return;
         const bool lpld = lpld_box(T, op_auto, slot);
         R u[6];
-        stencil<DIM>(T, L, own, par, slot, box, nc, i, j, k, q, 1, box[q], u);
+        stencil<DIM>(T, L, own, par, slot, box, nc, i, j, k, q, 1, ldv(box + q), u);
         const R a = lsf_box(T, own, slot)
-                        ? box_op_lsf<DIM>(u, box[q], own.lsf + base, own.bval + base, q, s, L.inv_dr_sq)
+                        ? box_op_lsf<DIM>(u, ldv(box + q), own.lsf + base, own.bval + base, q, s, L.inv_dr_sq)
                     : DIM == 2 && cyl_op(T, op_auto)
-                        ? box_op_cyl(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld,
+                        ? box_op_cyl(u, ldv(box + q), lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld,
                                      T.cyl_ioff[slot] + i)
-                        : box_op<DIM>(u, box[q], lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld);
-        const R tau = a + own.tmp[base + q];
+                        : box_op<DIM>(u, ldv(box + q), lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld);
+        const R tau = a + ldv(own.tmp + base + q);
         own.rhs[base + q] = tau;
         if (DIM == 3 && own.rhs_rb) own.rhs_rb[(size_t)slot * own.rb_stride + colour_index<DIM>(i, j, k, nc)] = tau;
-        if (copy_tmp) own.tmp[base + q] = box[q];
+        if (copy_tmp) own.tmp[base + q] = ldv(box + q);
         return;
     }
-    R v = box[q];
+    R v = ldv(box + q);
     if (n_out == 1) {   // face ghost: fill it
         const int f = oi ? (i > nc) : oj ? 2 + (j > nc) : 4 + (k > nc);
         const int ci = oi ? (i > nc ? nc : 1) : i;
         const int cj = oj ? (j > nc ? nc : 1) : j;
         const int ck = DIM == 3 ? (ok ? (k > nc ? nc : 1) : k) : 0;
-        const R center = box[ci + s * (cj + s * ck)];
+        const R center = ldv(box + ci + s * (cj + s * ck));
         v = ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, f, nc, ci, cj, ck, center);
         box[q] = v;
     }
     if (!is_par) return;
-    own.rhs[base + q] = own.rhs[base + q] + own.tmp[base + q];
+    own.rhs[base + q] = ldv(own.rhs + base + q) + ldv(own.tmp + base + q);
     if (copy_tmp) own.tmp[base + q] = v;
 }
 
@@ -1244,7 +1265,7 @@ AFG_DEV void correct_body(const DevTree &T, const LevelArgs &L, const LevelViewT
     const size_t pb = par.off(p);
     const R *pt = par.tmp + pb;
     const R *pp = par.phi + pb;
-    auto corr = [&](int q) -> R { return fly_diff ? pp[q] - pt[q] : pt[q]; };
+    auto corr = [&](int q) -> R { return fly_diff ? ldv(pp + q) - ldv(pt + q) : ldv(pt + q); };
     R *cph = own.phi + own.off(slot);
     if (lsf_box(T, own, slot)) {
         // mg$D_box_corr_lpllsf (m_aX_mg.f90:1019-1102): the correction from the
@@ -1279,7 +1300,7 @@ AFG_DEV void correct_body(const DevTree &T, const LevelArgs &L, const LevelViewT
                 sv += c[t] * v[t];
                 sc += c[t];
             }
-            cph[fq] = cph[fq] + sv / sc;
+            cph[fq] = ldv(cph + fq) + sv / sc;
         }
     } else if (lpld_box(T, op_auto, slot)) {
         const R *pe = par.eps + pb;
@@ -1298,7 +1319,7 @@ AFG_DEV void correct_body(const DevTree &T, const LevelArgs &L, const LevelViewT
             if (DIM == 2) {
                 R sum = (a0 * u0 + pe[qn[0]] * corr(qn[0])) / (a0 + pe[qn[0]]);
                 sum += (a0 * u0 + pe[qn[1]] * corr(qn[1])) / (a0 + pe[qn[1]]);
-                *t = *t + R(0.5) * sum;
+                *t = ldv(t) + R(0.5) * sum;
             } else {
                 R sum = 0;
 #pragma unroll
@@ -1307,7 +1328,7 @@ AFG_DEV void correct_body(const DevTree &T, const LevelArgs &L, const LevelViewT
                                 (a0 + pe[qn[d]]);
                     sum = d == 0 ? v : sum + v;
                 }
-                *t = *t + R(third) * sum;
+                *t = ldv(t) + R(third) * sum;
             }
         }
     } else {
@@ -1317,7 +1338,7 @@ AFG_DEV void correct_body(const DevTree &T, const LevelArgs &L, const LevelViewT
             const int fi = 2 * ii - 1 + (m & 1), fj = 2 * jj - 1 + ((m >> 1) & 1);
             const int fk = DIM == 3 ? 2 * kk - 1 + ((m >> 2) & 1) : 0;
             R *t = &cph[fi + s * (fj + s * fk)];
-            *t = P2.at(m) + *t;
+            *t = P2.at(m) + ldv(t);
         }
     }
 }
@@ -1804,11 +1825,14 @@ AFG_DEV void bottom_nc_dispatch(int nc, F &&f)
 // are formed once, with its expressions.  Cell c of a box has coordinates
 // i = (c & 1) + 1, j = ((c >> 1) & 1) + 1, k = (c >> 2) + 1; its
 // descriptors are in the cell list of its colour at the rank it has among
-// the box's cells of that colour.
+// the box's cells of that colour.  Boxes are independent of each other
+// (physical boundaries on all faces), so several wavefronts can share them:
+// this one takes the groups of 64 / 2^DIM boxes wave_first, wave_first +
+// wave_step, ...
 // W is the arithmetic type: R, or double for an FP32 level relaxed in FP64.
 template <int DIM, int AUTO, typename R, typename W, typename Xshfl>
 AFG_DEV void base_wave_run(const DevTree &T, const LevelViewT<R> &own, const BottomLevels &B, const BottomArgs &A,
-                           double dx2, int n_box, int lane, Xshfl xshfl)
+                           double dx2, int n_box, int lane, Xshfl xshfl, int wave_first = 0, int wave_step = 1)
 {
     constexpr int NCL = 1 << DIM, H = NCL / 2, S = 4, PER_WAVE = 64 / NCL;
     const int dq[6] = {-1, 1, -S, S, -S * S, S * S};
@@ -1817,7 +1841,7 @@ AFG_DEV void base_wave_run(const DevTree &T, const LevelViewT<R> &own, const Bot
     const int odd = odd_of(c);
     int rank = 0;
     for (int c2 = 0; c2 < c; c2++) rank += odd_of(c2) == odd;
-    for (int g0 = 0; g0 < n_box; g0 += PER_WAVE) {
+    for (int g0 = wave_first * PER_WAVE; g0 < n_box; g0 += wave_step * PER_WAVE) {
         const int b = g0 + lane / NCL;
         const bool on = b < n_box;
         W ph = 0, rh = 0, K[3] = {0, 0, 0}, cm[6] = {0, 0, 0, 0, 0, 0}, sc = 1;
@@ -1826,8 +1850,8 @@ AFG_DEV void base_wave_run(const DevTree &T, const LevelViewT<R> &own, const Bot
         if (on) {
             const BaseCell &d = B.base[(odd ? 0 : B.base_n[0]) + b * H + rank];
             pos = d.pos;
-            ph = W(own.phi[pos]);
-            rh = W(own.rhs[pos]);
+            ph = W(ldv(own.phi + pos));
+            rh = W(ldv(own.rhs + pos));
 #pragma unroll
             for (int dn = 0; dn < DIM; dn++) {
                 const int m = 2 * dn + ((c >> dn) & 1);   // the face of cell c on the box boundary
@@ -1857,7 +1881,7 @@ AFG_DEV void base_wave_run(const DevTree &T, const LevelViewT<R> &own, const Bot
                 for (int high = 0; high < 2; high++) {
                     const int m = 2 * dn + high;
                     if (rb == 1)
-                        u[m] = on ? W(own.phi[pos + dq[m]]) : W(0);
+                        u[m] = on ? W(ldv(own.phi + pos + dq[m])) : W(0);
                     else
                         u[m] = bit != high ? other : ghost;
                 }
@@ -1875,6 +1899,56 @@ AFG_DEV void base_wave_run(const DevTree &T, const LevelViewT<R> &own, const Bot
             if (odd == (rb & 1)) ph = nv;
         }
         if (on) own.phi[pos] = R(ph);
+    }
+}
+
+// The base relaxation from the cell list B.base (BOP_BASE when the fast
+// path does not apply): the 2 n_base colour passes of gsrb_boxes on the
+// lowest level L, the first reading the stored ghosts, the others the
+// neighbours the ghost fill after the previous pass would store (cells of
+// the other colour, or K + c1 phi at physical boundaries).  Threads t of n;
+// sy() orders the passes among them.
+template <int DIM, int AUTO, typename R, typename Sy>
+AFG_DEV void base_list_run(const DevTree &T, const LevelViewT<R> &own, const BottomLevels &B, const BottomArgs &A,
+                           const LevelArgs &L, int t, int n, Sy sy)
+{
+    const int s = L.nc + 2;
+    const int dq[6] = {-1, 1, -s, s, -s * s, s * s};
+    auto relax = [&](const BaseCell &d, int rb) {
+        const R center = ldv(own.phi + d.pos);
+        R u[6];
+#pragma unroll
+        for (int m = 0; m < 2 * DIM; m++) {
+            const int f = d.fly[m];
+            if (rb == 1 || f >= 0)
+                u[m] = ldv(own.phi + (rb == 1 ? d.pos + dq[m] : f));
+            else
+                u[m] = R(d.K[m]) + (f == -1 ? -center : center);
+        }
+        if (DIM == 2) u[4] = u[5] = 0;
+        const bool lpld = lpld_box(T, AUTO, d.slot);
+        own.phi[d.pos] = gsrb_value<DIM>(u, ldv(own.rhs + d.pos), lpld ? own.eps + d.pos : nullptr, 0, s, L.dx2,
+                                         lpld);
+    };
+    const int n1 = B.base_n[0], n2 = B.base_n[1];
+    if (n1 <= n && n2 <= n) {   // one cell of each colour per thread, kept in registers
+        BaseCell d1{}, d2{};
+        if (t < n1) d1 = B.base[t];
+        if (t < n2) d2 = B.base[n1 + t];
+        for (int rb = 1; rb <= 2 * A.n_base; rb++) {
+            if (rb & 1) {
+                if (t < n1) relax(d1, rb);
+            } else {
+                if (t < n2) relax(d2, rb);
+            }
+            if (rb < 2 * A.n_base) sy();
+        }
+    } else {
+        for (int rb = 1; rb <= 2 * A.n_base; rb++) {
+            const int c0 = rb & 1 ? 0 : n1, nn = rb & 1 ? n1 : n2;
+            for (int c = t; c < nn; c += n) relax(B.base[c0 + c], rb);
+            if (rb < 2 * A.n_base) sy();
+        }
     }
 }
 
@@ -1988,50 +2062,12 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLeve
                         break;
                     }
                 }
-                const int s = (NC ? NC : L.nc) + 2;
-                const int dq[6] = {-1, 1, -s, s, -s * s, s * s};
-                auto relax = [&](const BaseCell &d, int rb) {
-                    const R center = own.phi[d.pos];
-                    R u[6];
-#pragma unroll
-                    for (int m = 0; m < 2 * DIM; m++) {
-                        const int f = d.fly[m];
-                        if (rb == 1 || f >= 0)
-                            u[m] = own.phi[rb == 1 ? d.pos + dq[m] : f];
-                        else
-                            u[m] = R(d.K[m]) + (f == -1 ? -center : center);
-                    }
-                    if (DIM == 2) u[4] = u[5] = 0;
-                    const bool lpld = lpld_box(T, AUTO, d.slot);
-                    own.phi[d.pos] = gsrb_value<DIM>(u, own.rhs[d.pos], lpld ? own.eps + d.pos : nullptr,
-                                                     0, s, L.dx2, lpld);
-                };
-                auto sy = [&] {
+                base_list_run<DIM, AUTO, R>(T, own, B, A, L, t, n, [&] {
                     if (in_wave)
                         wsync();
                     else
                         sync();
-                };
-                const int n1 = B.base_n[0], n2 = B.base_n[1];
-                if (n1 <= n && n2 <= n) {   // one cell of each colour per thread, kept in registers
-                    BaseCell d1, d2;
-                    if (t < n1) d1 = B.base[t];
-                    if (t < n2) d2 = B.base[n1 + t];
-                    for (int rb = 1; rb <= 2 * A.n_base; rb++) {
-                        if (rb & 1) {
-                            if (t < n1) relax(d1, rb);
-                        } else {
-                            if (t < n2) relax(d2, rb);
-                        }
-                        if (rb < 2 * A.n_base) sy();
-                    }
-                } else {
-                    for (int rb = 1; rb <= 2 * A.n_base; rb++) {
-                        const int c0 = rb & 1 ? 0 : n1, nn = rb & 1 ? n1 : n2;
-                        for (int c = t; c < nn; c += n) relax(B.base[c0 + c], rb);
-                        if (rb < 2 * A.n_base) sy();
-                    }
-                }
+                });
             } break;
             case BOP_GSRB: {
                 const LevelArgs L = B.lv[li];
@@ -2105,6 +2141,122 @@ AFG_DEV void bottom_residual_run(const DevTree &T, const MgVarsT<R> &V, const Bo
             for (int g = tid; g < tot; g += nt) residual_body<DIM, NC>(T, L, v, v, V.op_auto, g);
         });
     }
+}
+
+
+// residual_box (m_aX_mg.f90:545-559) on every cell of levels 0..hl: thread t of
+// n takes cells t, t + n, ... of the levels in order (each level a
+// contiguous slot range; the operators of the options)
+template <int DIM, typename R>
+AFG_DEV void residual_levels_run(const DevTree &T, const MgVarsT<R> &V, const BottomLevels &B, int hl, int t,
+                                 int n)
+{
+    const LevelViewT<R> v{V.phi, V.rhs, V.tmp, V.eps, 0, T.stride};
+    for (int li = 0; li <= hl; li++) {
+        const LevelArgs &L = B.lv[li];
+        nc_dispatch(L.nc, [&](auto ncv) {
+            constexpr int NC = decltype(ncv)::value;
+            const int tot = L.n * ipow(L.nc, DIM);
+            for (int g = t; g < tot; g += n) residual_body<DIM, NC>(T, L, v, v, V.op_auto, g);
+        });
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The lower part of a V-cycle in one launch of several workgroups (the
+// persistent kernel): levels 0..hl (relative to the lowest) with their data
+// in the HBM pools, every stage of mg$D_fas_vcycle below and at hl
+// (m_aX_mg.f90:218-241) in the order and with the fusions of the per-level
+// launches of the host orchestration -- colour passes with face ghosts
+// computed from the second pass on, update_coarse as its child and parent
+// parts, the base relaxation and its ghost fill, correct_children with the
+// correction formed on the fly, the up-sweep's passes and the fill that also
+// forms the parents' tmp = phi - tmp -- so the values are those of the
+// per-level path bit for bit.  Thread tid of nt in workgroup `rank` of
+// n_rank; gsync() orders the stages across all workgroups, wgsync() the
+// threads of one workgroup.  With set_residual the residual of levels 0..hl
+// is the last stage (only when hl is the V-cycle's highest level: above it,
+// the per-level up-sweep still writes tmp on the parents of level hl + 1).
+template <int DIM, int AUTO, typename R, typename GSync, typename WSync, typename Xshfl>
+AFG_DEV void persist_run(const DevTree &T, const MgVarsT<R> &V, const BottomLevels &B, const BottomArgs &A,
+                         int set_residual, int rank, int n_rank, int tid, int nt, int wave, GSync gsync,
+                         WSync wgsync, Xshfl xshfl)
+{
+    const int t = rank * nt + tid, n = n_rank * nt;
+    const LevelViewT<R> v{V.phi, V.rhs, V.tmp, V.eps, 0, T.stride};
+    // down: relaxation, then update_coarse (child part; parent part with the
+    // parent level's ghost fill)
+    for (int li = A.hl; li >= 1; li--) {
+        const LevelArgs L = B.lv[li];
+        bottom_nc_dispatch(L.nc, [&](auto ncv) {
+            constexpr int NC = decltype(ncv)::value;
+            const int nc = NC ? NC : L.nc;
+            const int tot = gsrb_threads<DIM>(L.n, nc);
+            for (int rb = 1; rb <= 2 * A.n_down; rb++) {
+                for (int g = t; g < tot; g += n) gsrb_body<DIM, NC>(T, L, v, v, AUTO, rb, rb == 1 ? 0 : 1, g);
+                gsync();
+            }
+            const int tc = L.n * ipow(nc / 2, DIM);
+            for (int g = t; g < tc; g += n) uc_child_body<DIM, NC>(T, L, v, v, AUTO, 1, A.n_down > 0 ? 1 : 0, g);
+        });
+        gsync();
+        const LevelArgs Lp = B.lv[li - 1];
+        bottom_nc_dispatch(Lp.nc, [&](auto ncv) {
+            constexpr int NC = decltype(ncv)::value;
+            const int tot = Lp.n * ipow((NC ? NC : Lp.nc) + 2, DIM);
+            for (int g = t; g < tot; g += n) uc_parent_body<DIM, NC>(T, Lp, v, v, AUTO, 1, g);
+        });
+        gsync();
+    }
+    // the lowest level: n_base cycles of gsrb_boxes, then its ghost fill
+    if (A.n_base > 0) {
+        const LevelArgs L = B.lv[0];
+        if (B.base_fast && L.nc == 2 && wave == 64) {
+            // boxes are independent: every wavefront of every workgroup takes groups of them
+            const int lane = tid & 63, w = t >> 6, nw = n >> 6;
+            if (sizeof(R) == 4 && A.base64)
+                base_wave_run<DIM, AUTO, R, double>(T, v, B, A, L.dx2, L.n, lane, xshfl, w, nw);
+            else
+                base_wave_run<DIM, AUTO, R, R>(T, v, B, A, L.dx2, L.n, lane, xshfl, w, nw);
+        } else if (rank == 0) {
+            base_list_run<DIM, AUTO, R>(T, v, B, A, L, tid, nt, wgsync);
+        }
+        gsync();
+        bottom_nc_dispatch(L.nc, [&](auto ncv) {
+            constexpr int NC = decltype(ncv)::value;
+            const int tot = fill_threads<DIM>(L.n, NC ? NC : L.nc);
+            for (int g = t; g < tot; g += n) fill_body<DIM, NC>(T, L, v, v, g);
+        });
+        gsync();
+    }
+    // up: correction, relaxation, then the ghost fill with tmp = phi - tmp
+    // on the parents of the level below
+    for (int li = 1; li <= A.hl; li++) {
+        const LevelArgs L = B.lv[li];
+        const ListArgs P = B.par[li - 1];
+        bottom_nc_dispatch(L.nc, [&](auto ncv) {
+            constexpr int NC = decltype(ncv)::value;
+            const int nc = NC ? NC : L.nc;
+            const int tc = L.n * ipow(nc / 2, DIM);
+            for (int g = t; g < tc; g += n) correct_body<DIM, NC>(T, L, v, v, AUTO, 1, g);
+            gsync();
+            const int tot = gsrb_threads<DIM>(L.n, nc);
+            for (int rb = 1; rb <= 2 * A.n_up; rb++) {
+                for (int g = t; g < tot; g += n) gsrb_body<DIM, NC>(T, L, v, v, AUTO, rb, 1, g);
+                gsync();
+            }
+            const int nf = fill_threads<DIM>(L.n, nc);
+            const int tf = nf + P.n * ipow(P.nc + 2, DIM);
+            for (int g = t; g < tf; g += n) {
+                if (g < nf)
+                    fill_body<DIM, NC>(T, L, v, v, g);
+                else
+                    diff_body<DIM, 0>(T, P, v, g - nf);
+            }
+        });
+        if (li < A.hl || set_residual) gsync();
+    }
+    if (set_residual) residual_levels_run<DIM, R>(T, V, B, A.hl, t, n);
 }
 
 

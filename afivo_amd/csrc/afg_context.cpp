@@ -155,6 +155,17 @@ struct afg_ctx {
     BottomLevels h_bottom[4];                         // their host copies
     BaseCell *d_base[4] = {nullptr, nullptr, nullptr, nullptr};   // lowest-level cell lists
     int base_cap = 0;
+    // persistent kernel tables (the same four), its lowest-level cell list
+    // (positions in the HBM pools) and its barrier counters
+    BottomLevels *d_persist[4] = {nullptr, nullptr, nullptr, nullptr};
+    BottomLevels h_persist[4];
+    BaseCell *d_pbase = nullptr;
+    int pbase_cap = 0;
+    unsigned *d_psync = nullptr;
+    unsigned *h_psync = nullptr;   // pinned: where a synchronisation reads them back
+    unsigned long long *d_ptrace = nullptr;   // stage trace of the persistent kernel (afg_persist_trace)
+    int ptrace_cap = 0;
+    bool persist_ran = false;      // launched since the last read-back
     int8_t *d_is_parent = nullptr;
     std::vector<double *> d_var;
     std::vector<float *> d_var32;        // FP32 copies (mixed-precision solve), allocated on use
@@ -190,6 +201,14 @@ struct afg_ctx {
     int64_t lds_limit = 65536;           // LDS bytes one workgroup may use
     int64_t bottom_cells[2] = {2048, 2048};   // cell limit per level for the bottom kernel: FP64, FP32
     int64_t bottom_wave_cells = 0;      // ... and for its one-wavefront levels
+    // The persistent kernel (in place of the bottom kernel when persist_wgs >
+    // 0): its workgroups, the XCD they run on, its cell limit per level (FP64,
+    // FP32) and the highest level it takes per table (before the timing cap)
+    int persist_wgs = 0;
+    int persist_xcd = 0;
+    int64_t persist_cells[2] = {32768, 32768};
+    int persist_lvl[4] = {-1000, -1000, -1000, -1000};
+    int64_t persist_max_cells[4] = {0, 0, 0, 0};      // most cells of one of its levels
     bool use_graphs = true;
     std::map<std::tuple<int, int, int, int, int, int, int, int, int, int, int>, GraphRec> graphs;
     bool capturing = false;
@@ -269,7 +288,35 @@ int launched(afg_ctx *c, const char *what)
         HIP_TRY(ctx, hipStreamSynchronize((ctx)->stream));                                  \
     } while (0)
 
+
 namespace {
+
+int choose_bottom(afg_ctx *c);
+
+// Synchronise the context's stream.  When the persistent kernel ran since
+// the last synchronisation, its barrier counters are read back with it: a
+// set error flag means a barrier wait timed out (its workgroups were not all
+// resident at once), so the results of that work are invalid.  The counters
+// are then cleared, the persistent kernel is turned off for the context and
+// the call fails.
+int stream_sync(afg_ctx *c)
+{
+    const bool check = c->persist_ran && !c->capturing;
+    if (check)
+        HIP_TRY(c, hipMemcpyAsync(c->h_psync, c->d_psync, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (!check) return 0;
+    c->persist_ran = false;
+    if (c->h_psync[2] == 0) return 0;
+    HIP_TRY(c, hipMemsetAsync(c->d_psync, 0, 4 * sizeof(unsigned), c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const unsigned code = c->h_psync[2];
+    c->persist_wgs = 0;
+    choose_bottom(c);
+    return c->fail(AFG_ERR_HIP, "persistent kernel: %s; the results since the last synchronisation are invalid and "
+                   "the persistent kernel is now off",
+                   code == 2 ? "its workgroups did not all run on one XCD" : "a barrier wait timed out");
+}
 
 DevTree dev_tree(const afg_ctx *c)
 {
@@ -340,10 +387,15 @@ void free_tree(afg_ctx *c)
     for (int k = 0; k < 4; k++) {
         ctx_free(c, c->d_bottom[k]);
         ctx_free(c, c->d_base[k]);
+        ctx_free(c, c->d_persist[k]);
         c->d_bottom[k] = nullptr;
         c->d_base[k] = nullptr;
+        c->d_persist[k] = nullptr;
     }
     c->base_cap = 0;
+    ctx_free(c, c->d_pbase);
+    c->d_pbase = nullptr;
+    c->pbase_cap = 0;
     ctx_free(c, c->d_tree_leaves);
     ctx_free(c, c->d_chunk);
     ctx_free(c, c->d_tree_chunk);
@@ -445,6 +497,7 @@ int ensure_fc(afg_ctx *c, int ifc, double **out)
 }
 
 int build_base(afg_ctx *c);
+int choose_persist(afg_ctx *c);
 
 int upload_bc(afg_ctx *c)
 {
@@ -479,7 +532,7 @@ int upload_bc(afg_ctx *c)
     if (!c->bc_vals.empty())
         HIP_TRY(c, hipMemcpyAsync(c->d_bc_vals, c->bc_vals.data(), c->bc_vals.size() * sizeof(double),
                                   hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (int e_ = stream_sync(c)) return e_;
     c->bc_dirty = false;
     return c->have_tree ? build_base(c) : 0;   // the boundary values enter the base cell lists
 }
@@ -578,7 +631,7 @@ int collect_timing(afg_ctx *c)
     for (auto &g : c->graphs)
         if (int e = harvest_graph(c, g.second)) return e;
     if (c->ev_used.empty()) return 0;
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (int e_ = stream_sync(c)) return e_;
     for (size_t k = 0; k < c->ev_used.size(); k++) {
         float ms = 0;
         hipEventElapsedTime(&ms, c->ev_used[k].first, c->ev_used[k].second);
@@ -731,7 +784,7 @@ int set_tags(Mg &m)
     if (m.V.eps && m.V.lsf) {
         std::vector<int8_t> h(c->n_slots);
         HIP_TRY(c, hipMemcpyAsync(h.data(), m.T.tag, h.size(), hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        if (int e_ = stream_sync(c)) return e_;
         for (int8_t t : h)
             if (t == TAG_CONFLICT)
                 return c->fail(AFG_ERR_ARG, "mg%d_set_box_tag: Cannot set lsf and eps tag for same box", c->dim);
@@ -861,35 +914,71 @@ BottomArgs bottom_args(MgT<R> &m, int top)
 template <typename R>
 int bottom_table(const MgT<R> &m) { return (sizeof(R) == sizeof(float) ? 2 : 0) + (m.V.op_auto ? 1 : 0); }
 
-// highest level run by the bottom kernel for these options
+// highest level run by the bottom kernel -- or, when it is on, by the
+// persistent kernel -- for these options.  In timing mode the timed level's
+// passes run by level (their launches carry the events).
 template <typename R>
-int bottom_top(const MgT<R> &m) { return m.V.lsf ? -1000 : m.c->bottom_lvl[bottom_table(m)]; }
+int bottom_top(const MgT<R> &m)
+{
+    const afg_ctx *c = m.c;
+    if (m.V.lsf) return -1000;
+    if (c->persist_wgs <= 0) return c->bottom_lvl[bottom_table(m)];
+    int top = c->persist_lvl[bottom_table(m)];
+    if (c->timing && c->timing_lvl >= c->min_lvl && top >= c->timing_lvl) top = c->timing_lvl - 1;
+    return top >= c->min_lvl ? top : -1000;
+}
 
 
 // mg$D_fas_vcycle (m_aX_mg.f90:208-255).  Levels up to bottom_top run in the
-// one-workgroup LDS bottom kernel; the levels above, stage by stage.
+// one-workgroup LDS bottom kernel, or in the persistent kernel; the levels
+// above, stage by stage.
 template <typename R>
 void vcycle_body(MgT<R> &m, int highest_lvl, int set_residual)
 {
     afg_ctx *c = m.c;
-    // the bottom kernel's tables cover every tree: used when all take part
+    // the bottom and persistent kernels' tables cover every tree: used when all take part
     const int lb = c->bound <= c->min_tree_hi ? std::min(bottom_top(m), highest_lvl) : -1000;
     const int low = lb >= c->min_lvl ? lb : c->min_lvl;
+    const bool persist = c->persist_wgs > 0;
+    const int k = bottom_table(m);
+    // the residual of the lower levels inside the persistent kernel: only when
+    // it holds the whole V-cycle (above it, the up-sweep still writes tmp on
+    // the parents of level lb + 1)
+    const bool res_inside = persist && set_residual && lb == highest_lvl && c->res_upper > c->highest_lvl;
     for (int lvl = highest_lvl; lvl > low; lvl--) {
         passes(m, lvl, m.o->n_cycle_down, 0);
         update_coarse(m, lvl, m.o->n_cycle_down > 0);
     }
-    if (lb >= c->min_lvl) {
-        launch_bottom(c->dim, c->stream, m.T, m.V, c->d_bottom[bottom_table(m)], bottom_args(m, lb),
-                      c->bottom_lds[bottom_table(m)], c->bottom_threads);
+    if (lb >= c->min_lvl && persist) {
+        PersistArgs P;
+        P.A = bottom_args(m, lb);
+        P.set_residual = res_inside ? 1 : 0;
+        P.n_rank = c->persist_wgs;
+        P.xcd = c->persist_xcd;
+        P.sync = c->d_psync;
+        P.trace = c->d_ptrace;
+        P.trace_cap = c->ptrace_cap;
+        launch_persist(c->dim, c->stream, m.T, m.V, c->d_persist[k], P);
+    } else if (lb >= c->min_lvl) {
+        launch_bottom(c->dim, c->stream, m.T, m.V, c->d_bottom[k], bottom_args(m, lb), c->bottom_lds[k],
+                      c->bottom_threads);
     } else {
         gsrb_level(m, c->min_lvl, m.o->n_cycle_base);
     }
     for (int lvl = low + 1; lvl <= highest_lvl; lvl++) correct_relax(m, lvl, m.o->n_cycle_up);
     if (set_residual) {
         if (lb >= c->min_lvl && c->res_upper > c->highest_lvl) {
-            launch_bottom_residual(c->dim, c->stream, m.T, m.V, c->d_bottom[bottom_table(m)],
-                                   bottom_args(m, lb), c->bottom_threads);
+            if (persist && !res_inside) {
+                int64_t most = 0;
+                for (int li = 0; li <= lb - c->min_lvl; li++) {
+                    const LevelArgs &L = c->h_persist[k].lv[li];
+                    most = std::max<int64_t>(most, (int64_t)L.n * (c->dim == 2 ? L.nc * L.nc : L.nc * L.nc * L.nc));
+                }
+                launch_residual_levels(c->dim, c->stream, m.T, m.V, c->d_persist[k], lb - c->min_lvl, most);
+            } else if (!persist) {
+                launch_bottom_residual(c->dim, c->stream, m.T, m.V, c->d_bottom[k], bottom_args(m, lb),
+                                       c->bottom_threads);
+            }
             for (int lvl = lb + 1; lvl <= highest_lvl; lvl++) residual_level(m, lvl);
         } else {
             for (int lvl = c->min_lvl; lvl <= highest_lvl; lvl++) residual_level(m, lvl);
@@ -917,6 +1006,7 @@ int run_vcycle(MgT<R> &m, int highest_lvl, int set_residual, int bound, int res_
     } scope{c};
     c->bound = bound;
     c->res_upper = std::min(res_upper, c->highest_lvl + 1);
+    if (c->persist_wgs > 0) c->persist_ran = true;
     if (!c->use_graphs) {
         vcycle_body(m, highest_lvl, set_residual);
         return 0;
@@ -1075,15 +1165,78 @@ int choose_bottom(afg_ctx *c)
     return build_base(c);
 }
 
+// The cells of the lowest level as the base relaxation reads them
+// (BaseCell): colour 1, then colour 2 (n[0], n[1] of them); positions in its
+// LDS copy (lds) or in the HBM pools.  fast: every box has 2^dim cells and
+// physical boundaries on all faces.
+int base_cell_list(afg_ctx *c, bool lds, std::vector<BaseCell> &all, int n[2], bool &fast)
+{
+    const int dim = c->dim, nnb = c->nnb;
+    const Level &L0 = c->levels[0];
+    const int nc = L0.nc, s = nc + 2;
+    const int vstride = lds ? box_len(dim, nc) : c->stride, vslot0 = lds ? L0.slot0 : 0;
+    std::vector<BaseCell> cells[2];
+    for (int slot = L0.slot0; slot < L0.slot0 + L0.n; slot++) {
+        for (int kk = dim == 3 ? 1 : 0; kk <= (dim == 3 ? nc : 0); kk++)
+            for (int j = 1; j <= nc; j++)
+                for (int i = 1; i <= nc; i++) {
+                    const int x[3] = {i, j, kk};
+                    BaseCell d;
+                    std::memset(&d, 0, sizeof d);
+                    d.slot = slot;
+                    d.pos = (slot - vslot0) * vstride + i + s * (j + s * kk);
+                    const int dq[6] = {-1, 1, -s, s, -s * s, s * s};
+                    for (int m = 0; m < nnb; m++) {
+                        const int dn = m >> 1, high = m & 1;
+                        if (x[dn] != (high ? nc : 1)) {   // neighbour inside the box
+                            d.fly[m] = d.pos + dq[m];
+                            continue;
+                        }
+                        const int nid = c->h_nb[(size_t)slot * nnb + m];
+                        if (nid > 0) {   // sides_from_nb: the neighbour's mirrored cell
+                            int y[3] = {i, j, kk};
+                            y[dn] = high ? 1 : nc;
+                            const int ns = c->id2slot[nid];
+                            d.fly[m] = (ns - vslot0) * vstride + y[0] + s * (y[1] + s * y[2]);
+                        } else if (nid < 0) {   // bc_to_gc with the captured value b
+                            const BcRec &r = c->bc[c->bc_of[(size_t)slot * nnb + m]];
+                            double b = 0.0;
+                            if (r.voff >= 0) {
+                                const int t1 = dn == 0 ? j : i, t2 = dn == 2 ? j : kk;
+                                b = c->bc_vals[r.voff + (t1 - 1) + (dim == 3 ? nc * (t2 - 1) : 0)];
+                            }
+                            if (r.type == AFG_BC_DIRICHLET) {
+                                d.fly[m] = -1;
+                                d.K[m] = 2.0 * b;
+                            } else {
+                                const double c2 = L0.dr * (high ? 1.0 : -1.0);
+                                d.fly[m] = -2;
+                                d.K[m] = c2 * b;
+                            }
+                        } else {
+                            return c->fail(AFG_ERR_ARG, "refinement boundary on the lowest level");
+                        }
+                    }
+                    cells[(i + j + kk) & 1 ? 0 : 1].push_back(d);
+                }
+    }
+    all = cells[0];
+    all.insert(all.end(), cells[1].begin(), cells[1].end());
+    n[0] = (int)cells[0].size();
+    n[1] = (int)cells[1].size();
+    fast = nc == 2;
+    for (int slot = L0.slot0; fast && slot < L0.slot0 + L0.n; slot++)
+        for (int m = 0; m < nnb; m++) fast = fast && c->h_nb[(size_t)slot * nnb + m] < 0;
+    return 0;
+}
+
 // The cell lists of the lowest level for the bottom kernel (BaseCell), for
 // both tables, and the tables' upload.  Positions follow where the table
 // keeps level 0: its LDS copy or the HBM pool.
 int build_base(afg_ctx *c)
 {
-    const int dim = c->dim, nnb = c->nnb;
     const Level &L0 = c->levels[0];
-    const int nc = L0.nc, s = nc + 2;
-    const int ncells = L0.n * (dim == 2 ? nc * nc : nc * nc * nc);
+    const int ncells = L0.n * (c->dim == 2 ? L0.nc * L0.nc : L0.nc * L0.nc * L0.nc);
     if (ncells > c->base_cap) {
         for (int k = 0; k < 4; k++) {
             ctx_free(c, c->d_base[k]);
@@ -1092,7 +1245,7 @@ int build_base(afg_ctx *c)
         }
         c->base_cap = ncells;
     }
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (int e_ = stream_sync(c)) return e_;
     for (int k = 0; k < 4; k++) {
         BottomLevels &h = c->h_bottom[k];
         if (c->bottom_lvl[k] < c->min_lvl) {
@@ -1101,65 +1254,73 @@ int build_base(afg_ctx *c)
             HIP_H2D(c, c->d_bottom[k], &h, sizeof h);
             continue;
         }
-        const bool lds = h.in_lds[0] != 0;
-        const int vstride = lds ? h.box_len[0] : c->stride, vslot0 = lds ? L0.slot0 : 0;
-        std::vector<BaseCell> cells[2];
-        for (int slot = L0.slot0; slot < L0.slot0 + L0.n; slot++) {
-            for (int kk = dim == 3 ? 1 : 0; kk <= (dim == 3 ? nc : 0); kk++)
-                for (int j = 1; j <= nc; j++)
-                    for (int i = 1; i <= nc; i++) {
-                        const int x[3] = {i, j, kk};
-                        BaseCell d;
-                        std::memset(&d, 0, sizeof d);
-                        d.slot = slot;
-                        d.pos = (slot - vslot0) * vstride + i + s * (j + s * kk);
-                        const int dq[6] = {-1, 1, -s, s, -s * s, s * s};
-                        for (int m = 0; m < nnb; m++) {
-                            const int dn = m >> 1, high = m & 1;
-                            if (x[dn] != (high ? nc : 1)) {   // neighbour inside the box
-                                d.fly[m] = d.pos + dq[m];
-                                continue;
-                            }
-                            const int nid = c->h_nb[(size_t)slot * nnb + m];
-                            if (nid > 0) {   // sides_from_nb: the neighbour's mirrored cell
-                                int y[3] = {i, j, kk};
-                                y[dn] = high ? 1 : nc;
-                                const int ns = c->id2slot[nid];
-                                d.fly[m] = (ns - vslot0) * vstride + y[0] + s * (y[1] + s * y[2]);
-                            } else if (nid < 0) {   // bc_to_gc with the captured value b
-                                const BcRec &r = c->bc[c->bc_of[(size_t)slot * nnb + m]];
-                                double b = 0.0;
-                                if (r.voff >= 0) {
-                                    const int t1 = dn == 0 ? j : i, t2 = dn == 2 ? j : kk;
-                                    b = c->bc_vals[r.voff + (t1 - 1) + (dim == 3 ? nc * (t2 - 1) : 0)];
-                                }
-                                if (r.type == AFG_BC_DIRICHLET) {
-                                    d.fly[m] = -1;
-                                    d.K[m] = 2.0 * b;
-                                } else {
-                                    const double c2 = L0.dr * (high ? 1.0 : -1.0);
-                                    d.fly[m] = -2;
-                                    d.K[m] = c2 * b;
-                                }
-                            } else {
-                                return c->fail(AFG_ERR_ARG, "refinement boundary on the lowest level");
-                            }
-                        }
-                        cells[(i + j + kk) & 1 ? 0 : 1].push_back(d);
-                    }
-        }
-        std::vector<BaseCell> all(cells[0]);
-        all.insert(all.end(), cells[1].begin(), cells[1].end());
+        std::vector<BaseCell> all;
+        bool fast = false;
+        if (int e = base_cell_list(c, h.in_lds[0] != 0, all, h.base_n, fast)) return e;
         HIP_H2D(c, c->d_base[k], all.data(), all.size() * sizeof(BaseCell));
-        bool fast = nc == 2;
-        for (int slot = L0.slot0; fast && slot < L0.slot0 + L0.n; slot++)
-            for (int m = 0; m < nnb; m++) fast = fast && c->h_nb[(size_t)slot * nnb + m] < 0;
         h.base_fast = fast ? 1 : 0;
         h.base = c->d_base[k];
-        h.base_n[0] = (int)cells[0].size();
-        h.base_n[1] = (int)cells[1].size();
         HIP_H2D(c, c->d_bottom[k], &h, sizeof h);
     }
+    return choose_persist(c);
+}
+
+// The persistent kernel's tables: it takes the lowest levels while each holds
+// at most persist_cells cells (per precision; no LDS bound, its data stay in
+// the HBM pools); the tables list those levels, the parent list of each and
+// the lowest level's cells at their positions in the pools (the same for all
+// four tables).
+int choose_persist(afg_ctx *c)
+{
+    const int nl = (int)c->levels.size();
+    bool any = false;
+    for (int k = 0; k < 4; k++) {
+        int top = -1;
+        int64_t most = 0;
+        // cylindrical trees run every level with per-level launches
+        for (int li = 0; li < nl && li < MAX_BOTTOM_LEVELS && !c->cyl; li++) {
+            const Level &L = c->levels[li];
+            const int64_t cells = (int64_t)L.n * (c->dim == 2 ? L.nc * L.nc : L.nc * L.nc * L.nc);
+            if (cells > c->persist_cells[k >= 2]) break;
+            top = li;
+            most = std::max(most, cells);
+        }
+        BottomLevels h;
+        std::memset(&h, 0, sizeof h);
+        h.wave_top = -1;
+        for (int li = 0; li <= top; li++) {
+            const Level &L = c->levels[li];
+            h.lv[li] = level_args(c, L.lvl);
+            h.par[li] = list_args(c, L.lvl, 0);
+            h.box_len[li] = box_len(c->dim, L.nc);
+        }
+        c->persist_lvl[k] = top >= 0 ? c->levels[top].lvl : -1000;
+        c->persist_max_cells[k] = most;
+        c->h_persist[k] = h;
+        any = any || top >= 0;
+    }
+    if (any) {
+        std::vector<BaseCell> all;
+        int n[2] = {0, 0};
+        bool fast = false;
+        if (int e = base_cell_list(c, false, all, n, fast)) return e;
+        if ((int)all.size() > c->pbase_cap) {
+            ctx_free(c, c->d_pbase);
+            c->d_pbase = nullptr;
+            c->pbase_cap = 0;
+            HIP_TRY(c, ctx_malloc(c, &c->d_pbase, all.size() * sizeof(BaseCell)));
+            c->pbase_cap = (int)all.size();
+        }
+        HIP_H2D(c, c->d_pbase, all.data(), all.size() * sizeof(BaseCell));
+        for (int k = 0; k < 4; k++) {
+            BottomLevels &h = c->h_persist[k];
+            h.base = c->d_pbase;
+            h.base_n[0] = n[0];
+            h.base_n[1] = n[1];
+            h.base_fast = fast ? 1 : 0;
+        }
+    }
+    for (int k = 0; k < 4; k++) HIP_H2D(c, c->d_persist[k], &c->h_persist[k], sizeof(BottomLevels));
     return 0;
 }
 
@@ -1388,6 +1549,16 @@ int afg_ctx_create(int dim, int n_cell, int n_var, afg_ctx **out)
     if (e == hipSuccess) e = scratch_address(dev, &c->scratch);
     if (e == hipSuccess) e = ctx_malloc(c, &c->d_out, 8 * sizeof(double));
     if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_out, 8 * sizeof(double), hipHostMallocDefault);
+    // the persistent kernel's barrier counters start at zero (it leaves them so)
+    if (e == hipSuccess) e = ctx_malloc(c, &c->d_psync, (4 + PERSIST_MAX_RANKS) * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemset(c->d_psync, 0, (4 + PERSIST_MAX_RANKS) * sizeof(unsigned));
+    if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_psync, 4 * sizeof(unsigned), hipHostMallocDefault);
+    {
+        // contexts take the XCDs in turn, so that the persistent kernels of
+        // contexts that run at once (on their own streams) use different L2s
+        static std::atomic<unsigned> n_created{0};
+        c->persist_xcd = (int)(n_created++ % 8);
+    }
     if (e != hipSuccess) {
         g_create_error = std::string("HIP set-up: ") + hipGetErrorString(e);
         afg_ctx_destroy(c);
@@ -1412,7 +1583,10 @@ int afg_ctx_destroy(afg_ctx *c)
         hipEventDestroy(ev.second);
     }
     ctx_free(c, c->d_out);
+    ctx_free(c, c->d_psync);
+    ctx_free(c, c->d_ptrace);
     if (c->h_out) hipHostFree(c->h_out);
+    if (c->h_psync) hipHostFree(c->h_psync);
     if (c->h_stage) hipHostFree(c->h_stage);
     for (hipEvent_t ev : c->xfer_ev)
         if (ev) hipEventDestroy(ev);
@@ -1430,7 +1604,7 @@ int afg_sync(afg_ctx *c)
 {
     if (!c) return AFG_ERR_ARG;
     HIP_TRY(c, hipSetDevice(c->device));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (int e_ = stream_sync(c)) return e_;
     return 0;
 }
 
@@ -1447,7 +1621,7 @@ int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base,
     if (n_boxes < 1 || !recs) return c->fail(AFG_ERR_ARG, "empty tree");
     if (!(dr_base > 0)) return c->fail(AFG_ERR_ARG, "dr_base must be > 0");
     HIP_TRY(c, hipSetDevice(c->device));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (int e_ = stream_sync(c)) return e_;
     free_tree(c);
     c->mixed_ready = c->mixed_fmg = false;
     c->tags_set = false;
@@ -1675,7 +1849,7 @@ int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base,
     HIP_H2D(c, c->d_choff, choff.data(), sn * sizeof(int32_t));
     HIP_H2D(c, c->d_lists, lists.data(), lists.size() * sizeof(int32_t));
     HIP_TRY(c, hipMemsetAsync(c->d_tag, 0, sn * sizeof(int8_t), c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (int e_ = stream_sync(c)) return e_;
     int maxp = 0;
     for (int li = 0; li < nl; li++) {
         const Level &L = c->levels[li];
@@ -1730,6 +1904,7 @@ int afg_set_forest(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_base,
     }
     c->have_tree = true;
     for (int k = 0; k < 4; k++) HIP_TRY(c, ctx_malloc(c, &c->d_bottom[k], sizeof(BottomLevels)));
+    for (int k = 0; k < 4; k++) HIP_TRY(c, ctx_malloc(c, &c->d_persist[k], sizeof(BottomLevels)));
     if (int e = choose_bottom(c)) return e;
     HIP_TRY(c, ctx_malloc(c, &c->d_partial, c->partial_cap * sizeof(double)));
     c->bc_dirty = true;
@@ -1745,7 +1920,7 @@ int afg_set_coord(afg_ctx *c, int coord, double r_base)
     if (coord == AFG_COORD_CYL && c->dim != 2) return c->fail(AFG_ERR_ARG, "a$D_init: cannot have 3d cyl coords");
     if (c->capturing) return c->fail(AFG_ERR_STATE, "coordinate change inside a graph capture");
     HIP_TRY(c, hipSetDevice(c->device));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (int e_ = stream_sync(c)) return e_;
     c->cyl = coord == AFG_COORD_CYL;
     c->r_base = r_base;
     for (auto &g : c->graphs) destroy_graph(g.second);
@@ -1771,7 +1946,7 @@ static int update_tree(afg_ctx *c, int n_boxes, const int32_t *recs, double dr_b
     for (int k = 0; k < n_vars; k++)
         if (int e = check_var(c, vars[k])) return e;
     if (c->stride & 1) return c->fail(AFG_ERR_UNSUPPORTED, "odd box length %d", c->stride);
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (int e_ = stream_sync(c)) return e_;
 
     // the old tree: its boxes, boundary conditions, level settings and pools
     const int nnb = c->nnb, dim = c->dim;
@@ -2120,7 +2295,7 @@ int afg_upload_boxes(afg_ctx *c, int iv, int n_ids, const int32_t *ids, const do
     double *dev = nullptr;
     if (int e = ensure_var(c, iv, &dev)) return e;
     if (int e = ensure_stage(c, (size_t)c->n_slots * c->stride)) return e;
-    HIP_TRY(c, hipStreamSynchronize(c->stream));   // the staging buffer may still be read by a device copy
+    if (int e_ = stream_sync(c)) return e_;   // the staging buffer may still be read by a device copy
     std::atomic<int> err{hipSuccess};
     xfer_threads(
         P, [](int) {},
@@ -2197,7 +2372,7 @@ int afg_upload_fc(afg_ctx *c, int ifc, int n_ids, const int32_t *ids, const doub
     double *dev = nullptr;
     if (int e = ensure_fc(c, ifc, &dev)) return e;
     if (int e = ensure_stage(c, (size_t)c->n_slots * c->fstride)) return e;
-    HIP_TRY(c, hipStreamSynchronize(c->stream));   // staging may be in use
+    if (int e_ = stream_sync(c)) return e_;   // staging may be in use
     std::vector<char> mark(c->n_slots, 0);
     size_t pos = 0;
     for (int k = 0; k < n_ids; k++) {
@@ -2211,7 +2386,7 @@ int afg_upload_fc(afg_ctx *c, int ifc, int n_ids, const int32_t *ids, const doub
         mark[s] = 1;
     }
     if (int e = copy_runs(c, dev, mark, true, c->fstride)) return e;
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (int e_ = stream_sync(c)) return e_;
     return 0;
 }
 
@@ -2228,7 +2403,7 @@ int afg_download_fc(afg_ctx *c, int ifc, int n_ids, const int32_t *ids, double *
         mark[c->id2slot[id]] = 1;
     }
     if (int e = copy_runs(c, dev, mark, false, c->fstride)) return e;
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (int e_ = stream_sync(c)) return e_;
     size_t pos = 0;
     for (int k = 0; k < n_ids; k++) {
         const int s = c->id2slot[ids[k]];
@@ -2481,7 +2656,7 @@ static int forest_norms(afg_ctx *c, int iv, double *out)
     if (int e = launched(c, "forest norms")) return e;
     HIP_TRY(c, hipMemcpyAsync(c->h_tree_out, c->d_tree_out, c->n_trees * sizeof(double), hipMemcpyDeviceToHost,
                               c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (int e_ = stream_sync(c)) return e_;
     std::memcpy(out, c->h_tree_out, c->n_trees * sizeof(double));
     return 0;
 }
@@ -2554,7 +2729,7 @@ int afg_mixed_correct(afg_ctx *c, const afg_mg_opts *o, int cycles)
 int afg_mixed_fp64_base(afg_ctx *c, int enabled)
 {
     if (!c) return AFG_ERR_ARG;
-    if (c->stream) HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->stream) if (int e_ = stream_sync(c)) return e_;
     c->mixed_base64 = enabled != 0;
     for (auto &g : c->graphs) destroy_graph(g.second);   // the flag is a kernel argument
     c->graphs.clear();
@@ -2662,7 +2837,7 @@ static int tree_extreme(afg_ctx *c, int iv, int include_parents, int want_max, d
     launch_reduce_partials(c->stream, c->d_partial, used, want_max, c->d_out);
     if (int e = launched(c, "tree extreme")) return e;
     HIP_TRY(c, hipMemcpyAsync(c->h_out, c->d_out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (int e_ = stream_sync(c)) return e_;
     *out = c->h_out[0];
     return 0;
 }
@@ -2914,10 +3089,61 @@ int afg_set_bottom_cells(afg_ctx *c, int64_t cells, int64_t wave_cells)
 {
     if (int e = check_ctx(c)) return e;
     if (cells < 0 || wave_cells < 0) return c->fail(AFG_ERR_ARG, "bottom cell limits must be >= 0");
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (int e_ = stream_sync(c)) return e_;
     c->bottom_cells[0] = c->bottom_cells[1] = cells;
     c->bottom_wave_cells = wave_cells;
     return choose_bottom(c);
+}
+
+int afg_set_persist(afg_ctx *c, int wgs, int64_t cells)
+{
+    if (int e = check_ctx(c)) return e;
+    if (wgs < 0 || wgs > PERSIST_MAX_RANKS)
+        return c->fail(AFG_ERR_ARG, "persistent kernel workgroups %d outside 0..%d", wgs, PERSIST_MAX_RANKS);
+    if (cells < 0) return c->fail(AFG_ERR_ARG, "persistent kernel cell limit must be >= 0");
+    if (int e_ = stream_sync(c)) return e_;
+    c->persist_wgs = wgs;
+    c->persist_cells[0] = c->persist_cells[1] = cells;
+    return choose_bottom(c);
+}
+
+int afg_persist_trace(afg_ctx *c, int cap, int64_t *ticks, int *n)
+{
+    if (int e = check_ctx(c)) return e;
+    if (cap < 0) return c->fail(AFG_ERR_ARG, "trace capacity must be >= 0");
+    if (int e_ = stream_sync(c)) return e_;
+    if (ticks && n && c->d_ptrace) {
+        std::vector<unsigned long long> h(c->ptrace_cap);
+        HIP_TRY(c, hipMemcpy(h.data(), c->d_ptrace, h.size() * sizeof(h[0]), hipMemcpyDeviceToHost));
+        int k = 0;
+        while (k < (int)h.size() && h[k] != 0) {
+            ticks[k] = (int64_t)h[k];
+            k++;
+        }
+        *n = k;
+    }
+    if (cap != c->ptrace_cap) {
+        ctx_free(c, c->d_ptrace);
+        c->d_ptrace = nullptr;
+        c->ptrace_cap = 0;
+        if (cap > 0) {
+            HIP_TRY(c, ctx_malloc(c, &c->d_ptrace, (size_t)cap * sizeof(unsigned long long)));
+            c->ptrace_cap = cap;
+        }
+        for (auto &g : c->graphs) destroy_graph(g.second);
+        c->graphs.clear();
+    }
+    if (c->d_ptrace) HIP_TRY(c, hipMemset(c->d_ptrace, 0, (size_t)c->ptrace_cap * sizeof(unsigned long long)));
+    return 0;
+}
+
+int afg_get_persist(const afg_ctx *c, int *wgs, int64_t *cells, int *top_lvl)
+{
+    if (!c) return AFG_ERR_ARG;
+    if (wgs) *wgs = c->persist_wgs;
+    if (cells) *cells = c->persist_cells[0];
+    if (top_lvl) *top_lvl = c->persist_wgs > 0 ? c->persist_lvl[0] : -1000;
+    return 0;
 }
 
 int afg_set_tile(afg_ctx *c, int lvl, int fp32, int shape)
@@ -2927,7 +3153,7 @@ int afg_set_tile(afg_ctx *c, int lvl, int fp32, int shape)
     Level &L = c->levels[lvl - c->min_lvl];
     if (shape < 0 || shape >= tile_shapes(L.nc))
         return c->fail(AFG_ERR_ARG, "level %d (nc %d) has tile shapes 0..%d", lvl, L.nc, tile_shapes(L.nc) - 1);
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (int e_ = stream_sync(c)) return e_;
     L.tile[fp32 != 0] = shape;
     for (auto &g : c->graphs) destroy_graph(g.second);
     c->graphs.clear();
@@ -2946,7 +3172,7 @@ int afg_autotune(afg_ctx *c, const afg_mg_opts *o, int fp32, int reps, double *m
     if (int e = check_ctx(c)) return e;
     if (reps < 1) return c->fail(AFG_ERR_ARG, "reps must be >= 1");
     if (c->capturing) return c->fail(AFG_ERR_STATE, "autotune inside a graph capture");
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (int e_ = stream_sync(c)) return e_;
     if (ms)
         for (int i = 0; i < 6 * (int)c->levels.size() + 4; i++) ms[i] = -1;
     Mg m;
@@ -2969,7 +3195,7 @@ int afg_get_bottom_cells(const afg_ctx *c, int fp32, int64_t *cells)
 int afg_set_pass_pairs(afg_ctx *c, int enabled)
 {
     if (int e = check_ctx(c)) return e;
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (int e_ = stream_sync(c)) return e_;
     for (Level &L : c->levels) L.pairs[0] = L.pairs[1] = enabled != 0;
     for (auto &g : c->graphs) destroy_graph(g.second);
     c->graphs.clear();
@@ -2980,7 +3206,7 @@ int afg_set_level_pass_pairs(afg_ctx *c, int lvl, int fp32, int enabled)
 {
     if (int e = check_ctx(c)) return e;
     if (int e = check_level(c, lvl, false)) return e;
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (int e_ = stream_sync(c)) return e_;
     c->levels[lvl - c->min_lvl].pairs[fp32 != 0] = enabled != 0;
     for (auto &g : c->graphs) destroy_graph(g.second);
     c->graphs.clear();
@@ -3005,6 +3231,10 @@ int afg_set_timing(afg_ctx *c, int enabled, int lvl)
 {
     if (!c) return AFG_ERR_ARG;
     if (int e = collect_timing(c)) return e;
+    // with the persistent kernel on, the timed level runs by level: levels may
+    // move between the two paths, and only the per-level stages keep the
+    // colour-split rhs current
+    if (c->persist_wgs > 0 && (c->timing != (enabled != 0) || c->timing_lvl != lvl)) data_written(c);
     c->timing = enabled != 0;
     c->timing_lvl = lvl;
     if (c->timing) {

@@ -111,5 +111,14 @@ void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVarsT<R> &V
 template <typename R>
 void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVarsT<R> &V,
                             const BottomLevels *B, const BottomArgs &A, int threads);
+// the residual of levels 0..hl of table B over the HBM pools in one launch
+// (max_cells: the most cells any of those levels holds)
+template <typename R>
+void launch_residual_levels(int dim, hipStream_t s, const DevTree &T, const MgVarsT<R> &V,
+                            const BottomLevels *B, int hl, int64_t max_cells);
+// ---- the persistent kernel (afg_persist.hip) ----
+template <typename R>
+void launch_persist(int dim, hipStream_t s, const DevTree &T, const MgVarsT<R> &V, const BottomLevels *B,
+                    const PersistArgs &P);
 
 }  // namespace afg

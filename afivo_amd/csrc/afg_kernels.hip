@@ -456,6 +456,13 @@ __global__ __launch_bounds__(BOTTOM_BLOCK) void k_bottom_residual(DevTree T, MgV
     bottom_residual_run<DIM, R>(T, V, *B, A, threadIdx.x, blockDim.x);
 }
 
+template <int DIM, typename R>
+__global__ __launch_bounds__(BLOCK) void k_residual_levels(DevTree T, MgVarsT<R> V,
+                                                           const BottomLevels *__restrict__ B, int hl)
+{
+    residual_levels_run<DIM, R>(T, V, *B, hl, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -889,6 +896,17 @@ void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVarsT<R> &V
 }
 
 template <typename R>
+void launch_residual_levels(int dim, hipStream_t s, const DevTree &T, const MgVarsT<R> &V,
+                            const BottomLevels *B, int hl, int64_t max_cells)
+{
+    if (max_cells <= 0) return;
+    if (dim == 2)
+        k_residual_levels<2, R><<<n_blocks(max_cells), BLOCK, 0, s>>>(T, V, B, hl);
+    else
+        k_residual_levels<3, R><<<n_blocks(max_cells), BLOCK, 0, s>>>(T, V, B, hl);
+}
+
+template <typename R>
 void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVarsT<R> &V,
                             const BottomLevels *B, const BottomArgs &A, int threads)
 {
@@ -926,7 +944,9 @@ void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVa
     template void launch_bottom<R>(int, hipStream_t, const DevTree &, const MgVarsT<R> &,              \
                                    const BottomLevels *, const BottomArgs &, size_t, int);             \
     template void launch_bottom_residual<R>(int, hipStream_t, const DevTree &, const MgVarsT<R> &,     \
-                                            const BottomLevels *, const BottomArgs &, int);
+                                            const BottomLevels *, const BottomArgs &, int);            \
+    template void launch_residual_levels<R>(int, hipStream_t, const DevTree &, const MgVarsT<R> &,     \
+                                            const BottomLevels *, int, int64_t);
 AFG_INSTANTIATE(double)
 AFG_INSTANTIATE(float)
 

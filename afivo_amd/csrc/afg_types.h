@@ -168,4 +168,24 @@ struct BottomArgs {
     int base64 = 0;
 };
 
+// The persistent kernel: the lower part of a V-cycle (levels 0..A.hl, data in
+// the HBM pools) in one launch of n_rank workgroups of PERSIST_BLOCK threads
+// that all run on one XCD and share its L2, ordered by barriers on the
+// counters `sync` ([0] arrivals, [1] departures, [2] error flag: 1 a wait
+// timed out, 2 the workgroups were not on one XCD; zero between launches;
+// [4 + rank] the XCD of each workgroup, plus one).  The grid has 8 * n_rank workgroups; those with
+// blockIdx % 8 == xcd take part (workgroups go to the XCDs round-robin).
+constexpr int PERSIST_BLOCK = 256;
+constexpr int PERSIST_MAX_RANKS = 32;
+struct PersistArgs {
+    BottomArgs A;
+    int set_residual;   // the residual of levels 0..A.hl as the last stage
+    int n_rank, xcd;
+    unsigned *sync;
+    // when not null: s_memrealtime at the start and after every barrier (the
+    // first workgroup's view), at most trace_cap values
+    unsigned long long *trace = nullptr;
+    int trace_cap = 0;
+};
+
 }  // namespace afg

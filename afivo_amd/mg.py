@@ -39,6 +39,7 @@ class DeviceTree:
         h = ctypes.c_void_p()
         check(L.afg_ctx_create(dim, n_cell, n_var, ctypes.byref(h)))
         self.h = h
+        self._ptrace_cap = 0
         self.dim, self.nc, self.n_var = dim, n_cell, n_var
         self.records = np.ascontiguousarray(records, dtype=np.int32)
         self.ids = np.ascontiguousarray(self.records[:, 0])
@@ -319,6 +320,27 @@ class DeviceTree:
     # controls and measurements
     def set_bottom_cells(self, cells, wave_cells=0):
         self._c(_lib.load().afg_set_bottom_cells(self.h, int(cells), int(wave_cells)))
+
+    def set_persist(self, wgs, cells=32768):
+        """The persistent kernel for the levels of at most `cells` cells each
+        (wgs workgroups on one XCD; 0 turns it off)."""
+        self._c(_lib.load().afg_set_persist(self.h, int(wgs), int(cells)))
+
+    def persist_trace(self, cap):
+        """Ticks (10 ns) of the persistent kernel's stage trace since the last
+        call; record up to `cap` per launch from now on (0: stop)."""
+        ticks = np.zeros(max(self._ptrace_cap, 1), dtype=np.int64)
+        n = ctypes.c_int(0)
+        self._c(_lib.load().afg_persist_trace(self.h, int(cap), ticks.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                              ctypes.byref(n)))
+        self._ptrace_cap = int(cap)
+        return ticks[:n.value].copy()
+
+    def persist(self):
+        """(workgroups, cell limit, highest level taken) of the persistent kernel."""
+        w, n, top = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int()
+        self._c(_lib.load().afg_get_persist(self.h, ctypes.byref(w), ctypes.byref(n), ctypes.byref(top)))
+        return w.value, n.value, top.value
 
     def set_pass_pairs(self, enabled):
         """Fused pairs of colour passes (default) or one pass per launch."""

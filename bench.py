@@ -69,6 +69,8 @@ def parse():
                          "--bottom-cells: this configuration instead of the autotuner's)")
     ap.add_argument("--pairs", default="", help="c1, c2: fused pass pairs per level (0/1), comma-separated")
     ap.add_argument("--bottom-cells", type=int, default=-1, help="c1, c2: the bottom kernel's cell limit")
+    ap.add_argument("--persist", default="",
+                    help="c1, c2, c3: the persistent kernel as WORKGROUPS,CELLS (default: off)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy rhs to tmp over every box once, from cold caches (for the "
                          "HBM byte counters' scale factor), print its byte count and exit")
@@ -226,6 +228,12 @@ def mixed_to_tol_together(items, mg, tol, cycles, max_corrections=40):
     return corr
 
 
+def persist_arg(text):
+    """WORKGROUPS,CELLS of --persist."""
+    wgs, cells = (int(v) for v in text.split(","))
+    return wgs, cells
+
+
 def bench_c3(a):
     """Batched 2D FP64: --trees independent trees solved to --tol relative
     residual, as forests (one per n_cell) through afg_set_forest, the forests
@@ -251,6 +259,8 @@ def bench_c3(a):
     for nc, idx in sorted(groups.items()):
         sts = [trees[t][0] for t in idx]
         fdt = M.DeviceTree.from_forest(sts)
+        if a.persist:
+            fdt.set_persist(*persist_arg(a.persist))
         fst, _ = F.combine(sts)
         rhs_norm = np.array([np.abs(st.data[1][st.leaf_mask()]).max() for st in sts])
         forests.append((fdt, fst, rhs_norm))
@@ -324,6 +334,7 @@ def bench_c3(a):
             "leaf_cells_total": int(leaf_cells.sum()),
             "leaf_cells_per_tree": [int(leaf_cells.min()), int(np.median(leaf_cells)), int(leaf_cells.max())],
             "forests": {str(nc): len(idx) for nc, idx in sorted(groups.items())},
+            "persist": list(persist_arg(a.persist)) if a.persist else [0, 0],
         },
         "one_at_a_time_solves_per_s": a.trees / t_loop,
         "speedup_vs_one_at_a_time": t_loop / t_batch,
@@ -762,6 +773,8 @@ def main():
     else:
         dt.autotune(mg)
         tuned = True
+    if a.persist:
+        dt.set_persist(*persist_arg(a.persist))
     tiles = [dt.tile(lvl) for lvl in lvls]
     pairs = [int(dt.pass_pairs(lvl)) for lvl in lvls]
     bottom = dt.bottom_cells()
@@ -824,6 +837,7 @@ def main():
             "tile_shapes": tiles,
             "pass_pairs": pairs,
             "bottom_cells": bottom,
+            "persist": list(dt.persist()),
             "autotuned": tuned,
         },
         "solves_per_s": a.steps / elapsed,

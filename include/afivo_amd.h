@@ -372,6 +372,27 @@ int afg_mixed_solve(afg_ctx *ctx, const afg_mg_opts *opts, int have_guess, doubl
 int afg_set_bottom_cells(afg_ctx *ctx, int64_t cells, int64_t wave_cells);
 /* The cell limit in use for the FP64 (fp32 = 0) or FP32 stages. */
 int afg_get_bottom_cells(const afg_ctx *ctx, int fp32, int64_t *cells);
+/* The persistent kernel: with wgs in 1..32, the lowest levels of the tree,
+ * while each holds at most `cells` cells, run their part of every V-cycle
+ * (mg$D_fas_vcycle below and at the highest such level, m_aX_mg.f90:218-241,
+ * with the residual when the V-cycle's highest level is among them) in one
+ * launch of wgs workgroups that run on one XCD and meet at barriers between
+ * the stages, in place of the bottom kernel and the per-level launches of
+ * those levels; wgs = 0 (default) turns it off.  Results are identical
+ * either way.  A barrier wait is bounded: should one time out, the next
+ * synchronising call fails with AFG_ERR_HIP and the kernel is turned off.
+ * No counterpart in the reference (an execution control of the path). */
+int afg_set_persist(afg_ctx *ctx, int wgs, int64_t cells);
+/* The workgroups (0: off), the cell limit, and the highest level the
+ * persistent kernel takes for the FP64 stages (-1000: none). */
+int afg_get_persist(const afg_ctx *ctx, int *wgs, int64_t *cells, int *top_lvl);
+/* Stage trace of the persistent kernel (a diagnostic): returns in ticks[0..n)
+ * the s_memrealtime values (100 MHz) that its first workgroup took at the
+ * start of the last launch and after each barrier since the previous call,
+ * when ticks and n are given (ticks holds the capacity set by the previous
+ * call); then records at most `cap` values per launch from now on (cap = 0:
+ * none, the default). */
+int afg_persist_trace(afg_ctx *ctx, int cap, int64_t *ticks, int *n);
 /* Shape of the tiled colour pass on level lvl, for the FP64 (fp32 = 0) or
  * FP32 stages: 0 (default), 1 or 2 for nc 8 and 16 (workgroup size and boxes
  * per workgroup), 0 otherwise.  Results are identical for every shape. */

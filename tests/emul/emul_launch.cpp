@@ -610,6 +610,41 @@ void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVa
     });
 }
 
+// The persistent kernel as one workgroup of one thread: the same stage
+// sequence (persist_run) with every barrier a no-op, stages in order.
+template <typename R>
+void launch_persist(int dim, hipStream_t s, const DevTree &T, const MgVarsT<R> &V, const BottomLevels *B,
+                    const PersistArgs &P)
+{
+    emul_submit(s, [=] {
+        auto none = [] {};
+        auto same = [](auto v, int) { return v; };
+        if (dim == 2) {
+            if (V.op_auto)
+                persist_run<2, 1, R>(T, V, *B, P.A, P.set_residual, 0, 1, 0, 1, 1, none, none, same);
+            else
+                persist_run<2, 0, R>(T, V, *B, P.A, P.set_residual, 0, 1, 0, 1, 1, none, none, same);
+        } else {
+            if (V.op_auto)
+                persist_run<3, 1, R>(T, V, *B, P.A, P.set_residual, 0, 1, 0, 1, 1, none, none, same);
+            else
+                persist_run<3, 0, R>(T, V, *B, P.A, P.set_residual, 0, 1, 0, 1, 1, none, none, same);
+        }
+    });
+}
+
+template <typename R>
+void launch_residual_levels(int dim, hipStream_t s, const DevTree &T, const MgVarsT<R> &V,
+                            const BottomLevels *B, int hl, int64_t)
+{
+    emul_submit(s, [=] {
+        if (dim == 2)
+            residual_levels_run<2, R>(T, V, *B, hl, 0, 1);
+        else
+            residual_levels_run<3, R>(T, V, *B, hl, 0, 1);
+    });
+}
+
 #define AFG_INSTANTIATE(R)                                                                               \
     template int launch_gsrb_pair<R>(int, hipStream_t, const DevTree &, const LevelArgs &,             \
                                      const MgVarsT<R> &, const R *, R *, int, int);                    \
@@ -637,7 +672,11 @@ void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVa
     template void launch_bottom<R>(int, hipStream_t, const DevTree &, const MgVarsT<R> &,              \
                                    const BottomLevels *, const BottomArgs &, size_t, int);             \
     template void launch_bottom_residual<R>(int, hipStream_t, const DevTree &, const MgVarsT<R> &,     \
-                                            const BottomLevels *, const BottomArgs &, int);
+                                            const BottomLevels *, const BottomArgs &, int);            \
+    template void launch_residual_levels<R>(int, hipStream_t, const DevTree &, const MgVarsT<R> &,     \
+                                            const BottomLevels *, int, int64_t);                       \
+    template void launch_persist<R>(int, hipStream_t, const DevTree &, const MgVarsT<R> &,             \
+                                    const BottomLevels *, const PersistArgs &);
 AFG_INSTANTIATE(double)
 AFG_INSTANTIATE(float)
 

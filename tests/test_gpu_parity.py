@@ -301,6 +301,46 @@ def test_bottom_kernel_limit_is_exact(kind, cells, wave_cells):
     assert np.array_equal(rd, ro)
 
 
+@pytest.mark.parametrize("kind", ["2d_nc8", "2d_nc16", "3d_nc8", "3d_nc4_eps", "3d_nc16_eps"])
+@pytest.mark.parametrize("wgs,cells", [(1, 1 << 30), (4, 512), (16, 4096), (32, 1 << 30)])
+def test_persistent_kernel_is_exact(kind, wgs, cells):
+    """The persistent kernel (the lower levels' stages in one launch of wgs
+    workgroups meeting at barriers) gives the oracle's bits, whichever levels
+    it takes: a few low ones (per-level launches above it, the residual of its
+    levels in a launch of its own) or the whole tree (the residual inside it);
+    FP32 stages give the same bits with it as without it."""
+    p = problem(kind)
+    eps = p.eps_hi is not None
+    mg = M.default_mg(eps=eps, n_base=p.n_base)
+    om = O.default_opts(2, 2, p.n_base, eps=eps)
+    n_var = len(p.state.data)
+    dt = M.DeviceTree.from_state(p.state)
+    dt.set_persist(wgs, cells)
+    w, c, top = dt.persist()
+    assert (w, c) == (wgs, cells) and top >= dt.min_lvl
+    assert top == dt.highest_lvl or cells < 1 << 30
+    dt, ot, rd, ro = solve_both(p.state, n_var, mg, om, 1, 2, dt=dt)
+    compare_all(dt, ot, n_var, f"{kind} persist {wgs}/{cells}")
+    assert np.array_equal(rd, ro)
+    dt.close()
+    # FP32 stages: without and with the persistent kernel
+    out = []
+    for on in (False, True):
+        d = M.DeviceTree.from_state(p.state, n_var + 1)
+        d.set_persist(wgs if on else 0, cells)
+        for iv in range(1, n_var + 1):
+            d.fp32_from_cc(iv, iv)
+        M.mg_fas_fmg_fp32(d, mg, True, False)
+        M.mg_fas_vcycle_fp32(d, mg, d.highest_lvl, True)
+        d.fp32_to_cc(1, n_var + 1)
+        out.append(d.download(n_var + 1))
+        d.fp32_to_cc(3, n_var + 1)
+        out.append(d.download(n_var + 1))
+        d.close()
+    assert_bitexact(out[2], out[0], f"{kind} FP32 persist phi")
+    assert_bitexact(out[3], out[1], f"{kind} FP32 persist residual")
+
+
 @pytest.mark.parametrize("kind", ["2d_nc8", "2d_nc16", "3d_nc8", "3d_nc16_eps"])
 @pytest.mark.parametrize("shape", [1, 2])
 def test_tile_shapes_are_exact(kind, shape):
