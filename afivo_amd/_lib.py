@@ -101,6 +101,7 @@ SIGNATURES = {
                                        ctypes.c_int, ctypes.POINTER(ctypes.c_int), F64P]),
     "afg_set_bottom_cells": (ctypes.c_int, [P, I64, I64]),
     "afg_set_persist": (ctypes.c_int, [P, ctypes.c_int, I64]),
+    "afg_set_split_passes": (ctypes.c_int, [P, ctypes.c_int]),
     "afg_persist_trace": (ctypes.c_int, [P, ctypes.c_int, I64P, ctypes.POINTER(ctypes.c_int)]),
     "afg_get_persist": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), I64P, ctypes.POINTER(ctypes.c_int)]),
     "afg_set_graphs": (ctypes.c_int, [P, ctypes.c_int]),

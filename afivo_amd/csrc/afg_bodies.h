@@ -690,6 +690,186 @@ AFG_DEV void tile_stage_b(const DevTree &T, const LevelArgs &L, const LevelViewT
 }
 
 // ---------------------------------------------------------------------------
+// Colour passes of a relaxation block on a colour-split copy of phi (3D
+// tiles).  A pass needs the other colour's cells and writes its own colour;
+// in the box arrays those are every other value of every interior row, so a
+// pass reads and writes whole rows.  Within a block of n >= 2 passes on one
+// level (consecutive launches that nothing else on the level comes between)
+// the level's phi is also kept per slot without ghosts, colour 1 then colour
+// 2 in colour_cell order (`phi_rb`, the layout of rhs_rb):
+//   first pass  reads the box arrays (as k_gsrb_tile) and writes both colours
+//               to phi_rb (its colour updated, the other as read);
+//   middle      reads both colours of its boxes from phi_rb and writes its
+//               colour there;
+//   last        reads phi_rb and writes every interior cell to the box arrays.
+// Face ghosts of the middle and last passes are computed as ghost_value
+// computes them, reading the cells of this level from phi_rb.  The box arrays
+// hold the state after the block, the ghost layer as before it, as after the
+// per-pass kernels; phi_rb is only read within the block that wrote it.
+
+// ghost_value on a level whose cells are read from the colour-split copy
+// `rbp` (rb_stride values per slot); the parent level from `par` as ever.
+// Expressions and their order are ghost_value's.
+template <int DIM, typename R>
+AFG_DEV R ghost_value_split(const DevTree &T, const R *rbp, int rb_stride, const LevelViewT<R> &par, int pnc,
+                            double dr, int slot, int f, int nc, int i, int j, int k, R center)
+{
+    const int nbv = T.nb[slot * (2 * DIM) + f];
+    const int dn = f >> 1, high = f & 1;
+    if (nbv >= 0) {
+        const int m = high ? 1 : nc;
+        const int a0 = dn == 0 ? m : i, a1 = dn == 1 ? m : j, a2 = dn == 2 ? m : k;
+        return ldv(rbp + (size_t)nbv * rb_stride + colour_index<DIM>(a0, a1, a2, nc));
+    }
+    if (nbv == NB_REFINE) {
+        const int p = T.parent[slot];
+        const int co = T.choff[slot];
+        const int gv = high ? nc + 1 : 0;
+        const int g0 = dn == 0 ? gv : i, g1 = dn == 1 ? gv : j, g2 = dn == 2 ? gv : k;
+        const int ps = pnc + 2;
+        const int pi = (co & 1023) + ((g0 + 1) >> 1);
+        const int pj = ((co >> 10) & 1023) + ((g1 + 1) >> 1);
+        const int pk = DIM == 3 ? ((co >> 20) & 1023) + ((g2 + 1) >> 1) : 0;
+        const R P = ldv(par.phi + par.off(p) + pi + ps * (pj + ps * pk));
+        const R *own = rbp + (size_t)slot * rb_stride;
+        const int inward = high ? -1 : 1;
+        if (DIM == 2) {
+            const int di = dn == 0 ? inward : -1 + 2 * (i & 1);
+            const int dj = dn == 1 ? inward : -1 + 2 * (j & 1);
+            const R a = ldv(own + colour_index<DIM>(i + di, j, k, nc));
+            const R b = ldv(own + colour_index<DIM>(i, j + dj, k, nc));
+            return R(0.5) * P + center - R(0.25) * (a + b);
+        } else {
+            const int d0 = dn == 0 ? inward : -1 + 2 * (i & 1);
+            const int d1 = dn == 1 ? inward : -1 + 2 * (j & 1);
+            const int d2 = dn == 2 ? inward : -1 + 2 * (k & 1);
+            const R d = ldv(own + colour_index<DIM>(i + d0, j + d1, k + d2, nc));
+            return R(0.5) * P + R(0.75) * center - R(0.25) * d;
+        }
+    }
+    const int rec = NB_BC0 - nbv;
+    const int voff = T.bc_voff[rec];
+    R b = 0;
+    if (voff >= 0) {
+        const int t1 = dn == 0 ? j : i;
+        const int t2 = dn == 2 ? j : k;
+        b = R(T.bc_vals[voff + (t1 - 1) + (DIM == 3 ? nc * (t2 - 1) : 0)]);
+    }
+    R c1, c2;
+    if (T.bc_type[rec] == -1) {   // a5_bc_dirichlet
+        c1 = -1;
+        c2 = 2;
+    } else {                      // a5_bc_neumann
+        c1 = 1;
+        c2 = R(dr * (high ? 1.0 : -1.0));
+    }
+    return c2 * b + c1 * center;
+}
+
+// Stage A of the middle and last passes: rhs of the thread's cells, both
+// colours of the tile's boxes from phi_rb into LDS, and the face ghosts next
+// to cells of colour rb, computed from phi_rb.
+template <int DIM, int NC, int V, typename R>
+AFG_DEV void split_stage_a(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &v, const R *rbp, R *lds,
+                           int box0, int rb, int tid, R *rhs)
+{
+    using G = GsrbTile<DIM, NC, V>;
+    constexpr int S = G::S;
+#pragma unroll
+    for (int r = 0; r < G::PER; r++) {
+        const int e = tid + r * G::THREADS;
+        const int b = e / G::HALF;
+        if (e < G::BOXES * G::HALF && box0 + b < L.n)
+            rhs[r] = v.rhs_rb[(size_t)(L.slot0 + box0 + b) * v.rb_stride + ((rb + 1) & 1) * G::HALF + (e - b * G::HALF)];
+    }
+    // interiors: split index c holds colour 1 (i + j + k odd) for c < HALF
+    for (int e = tid; e < G::BOXES * G::CELLS; e += G::THREADS) {
+        const int b = e / G::CELLS;
+        if (box0 + b >= L.n) break;
+        const int c = e - b * G::CELLS;
+        const int hi = c >= G::HALF;
+        int i, j, k;
+        colour_cell<DIM>(c - hi * G::HALF, NC, hi ? 2 : 1, i, j, k);
+        lds[b * G::LEN + i + S * (j + S * k)] = ldv(rbp + (size_t)(L.slot0 + box0 + b) * v.rb_stride + c);
+    }
+    // face ghosts next to cells of the colour (the enumeration of tile_stage_a)
+    constexpr int FH = G::FACE / 2;
+    for (int e = tid; e < G::BOXES * G::GHOSTS; e += G::THREADS) {
+        const int b = e / G::GHOSTS;
+        if (box0 + b >= L.n) break;
+        const int r = e - b * G::GHOSTS;
+        const int f = r / FH, t = r - f * FH;
+        const int dn = f >> 1, high = f & 1;
+        const int m = high ? NC : 1;
+        const int t2 = t / (NC / 2) + 1;
+        const int t1 = 2 * (t % (NC / 2)) + 1 + ((rb + m + t2 + 1) & 1);
+        const int i = dn == 0 ? m : t1;
+        const int j = dn == 1 ? m : (dn == 0 ? t1 : t2);
+        const int k = dn == 2 ? m : t2;
+        const int gi = dn == 0 ? (high ? NC + 1 : 0) : i;
+        const int gj = dn == 1 ? (high ? NC + 1 : 0) : j;
+        const int gk = dn == 2 ? (high ? NC + 1 : 0) : k;
+        const int slot = L.slot0 + box0 + b;
+        const R center = ldv(rbp + (size_t)slot * v.rb_stride + colour_index<DIM>(i, j, k, NC));
+        lds[b * G::LEN + gi + S * (gj + S * gk)] =
+            ghost_value_split<DIM>(T, rbp, v.rb_stride, v, L.pnc, L.dr, slot, f, NC, i, j, k, center);
+    }
+}
+
+// Stage B of the split passes: colour rb updated from LDS as tile_stage_b
+// does; MODE 0 (first) and 1 (middle) write it to phi_rb, the first pass
+// also the other colour as read; MODE 2 (last) writes both colours to the box
+// arrays.
+template <int DIM, int NC, int V, int MODE, typename R>
+AFG_DEV void split_stage_b(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &v, R *rbp, const R *lds,
+                           int box0, int op_auto, int rb, int tid, const R *rhs)
+{
+    using G = GsrbTile<DIM, NC, V>;
+    constexpr int S = G::S;
+#pragma unroll
+    for (int r = 0; r < G::PER; r++) {
+        const int e = tid + r * G::THREADS;
+        const int b = e / G::HALF;
+        if (e >= G::BOXES * G::HALF || box0 + b >= L.n) break;
+        const int c = e - b * G::HALF;
+        int i, j, k;
+        colour_cell<DIM>(c, NC, rb, i, j, k);
+        const int q = i + S * (j + S * k);
+        const R *x = lds + b * G::LEN + q;
+        R u[6];
+        u[0] = x[-1];
+        u[1] = x[1];
+        u[2] = x[-S];
+        u[3] = x[S];
+        u[4] = DIM == 3 ? x[-S * S] : R(0);
+        u[5] = DIM == 3 ? x[S * S] : R(0);
+        const int slot = L.slot0 + box0 + b;
+        const size_t base = v.off(slot);
+        const bool lpld = lpld_box(T, op_auto, slot);
+        const R nv = gsrb_value<DIM>(u, rhs[r], lpld ? v.eps + base : nullptr, q, S, L.dx2, lpld);
+        if (MODE == 2)
+            v.phi[base + q] = nv;
+        else
+            rbp[(size_t)slot * v.rb_stride + ((rb + 1) & 1) * G::HALF + c] = nv;
+    }
+    if (MODE == 1) return;
+    // the other colour, unchanged by this pass
+    for (int e = tid; e < G::BOXES * G::HALF; e += G::THREADS) {
+        const int b = e / G::HALF;
+        if (box0 + b >= L.n) break;
+        const int c = e - b * G::HALF;
+        int i, j, k;
+        colour_cell<DIM>(c, NC, rb + 1, i, j, k);
+        const int q = i + S * (j + S * k);
+        const int slot = L.slot0 + box0 + b;
+        if (MODE == 2)
+            v.phi[v.off(slot) + q] = lds[b * G::LEN + q];
+        else
+            rbp[(size_t)slot * v.rb_stride + (rb & 1) * G::HALF + c] = lds[b * G::LEN + q];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Two colour passes in one launch: pass rb (colour A) and pass rb + 1
 // (colour B) of a relaxation block, reading the level's phi from one pool
 // and writing it to another (the two pools swap roles every pair), so that

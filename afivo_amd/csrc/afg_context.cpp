@@ -184,6 +184,11 @@ struct afg_ctx {
     // colour passes; valid while it equals the interiors of variable rb_var
     double *d_rhs_rb = nullptr;
     float *d_rhs_rb32 = nullptr;
+    // phi by colour (FP64, FP32 stages): the working copy of the colour passes
+    // within a relaxation block, valid only inside the block
+    double *d_phi_rb = nullptr;
+    float *d_phi_rb32 = nullptr;
+    bool split_passes = true;            // relaxation blocks on the colour-split copy (3D tiles)
     bool rb_valid[2] = {false, false};
     int rb_var[2] = {-1, -1};
     double *d_partial = nullptr;
@@ -425,6 +430,10 @@ void free_tree(afg_ctx *c)
     ctx_free(c, c->d_rhs_rb32);
     c->d_rhs_rb = nullptr;
     c->d_rhs_rb32 = nullptr;
+    ctx_free(c, c->d_phi_rb);
+    ctx_free(c, c->d_phi_rb32);
+    c->d_phi_rb = nullptr;
+    c->d_phi_rb32 = nullptr;
     c->rb_valid[0] = c->rb_valid[1] = false;
     c->d_nb = c->d_parent = c->d_choff = c->d_lists = c->d_bc_type = c->d_bc_voff = nullptr;
     c->d_bc_vals = nullptr;
@@ -588,7 +597,8 @@ void timed_passes(afg_ctx *c, const DevTree &T, const LevelArgs &L, int lvl, con
     Lt.tile = Lv.tile[sizeof(R) == 4];
     // Passes go by fused pairs, phi to the second pool and back, while an
     // even number of pairs is left (on levels set to pairs); the rest one by
-    // one.
+    // one: on the colour-split copy of phi when two or more are left (first,
+    // middle and last passes), else on the box arrays.
     int rb = 1, launches = 0;
     if (phi2 && Lv.pairs[sizeof(R) == 4])
         for (; rb + 3 <= n_pass; rb += 4) {
@@ -596,6 +606,13 @@ void timed_passes(afg_ctx *c, const DevTree &T, const LevelArgs &L, int lvl, con
             launch_gsrb_pair(c->dim, c->stream, T, Lt, V, phi2, V.phi, rb + 2, 1);
             launches += 2;
         }
+    const int first = rb;
+    const bool split = n_pass - first >= 1 && launch_gsrb_split(c->dim, c->stream, T, Lt, V, rb,
+                                                                 0, rb == 1 ? first_fly : 1);
+    if (split) {
+        for (rb++, launches++; rb <= n_pass; rb++, launches++)
+            launch_gsrb_split(c->dim, c->stream, T, Lt, V, rb, rb == n_pass ? 2 : 1, 1);
+    }
     for (; rb <= n_pass; rb++, launches++)
         launch_gsrb_pass(c->dim, c->stream, T, Lt, V, rb, rb == 1 ? first_fly : 1);
     if (!on) return;
@@ -663,6 +680,8 @@ double *&phi2_pool(afg_ctx *c, double *) { return c->d_phi2; }
 float *&phi2_pool(afg_ctx *c, float *) { return c->d_phi2_32; }
 double *&rhs_rb_pool(afg_ctx *c, double *) { return c->d_rhs_rb; }
 float *&rhs_rb_pool(afg_ctx *c, float *) { return c->d_rhs_rb32; }
+double *&phi_rb_pool(afg_ctx *c, double *) { return c->d_phi_rb; }
+float *&phi_rb_pool(afg_ctx *c, float *) { return c->d_phi_rb32; }
 
 // values per slot of the colour-split rhs: the interior of the largest boxes
 int rb_stride(const afg_ctx *c) { return c->dim == 2 ? c->nc * c->nc : c->nc * c->nc * c->nc; }
@@ -764,6 +783,16 @@ int make_mg(afg_ctx *c, const afg_mg_opts *o, MgT<R> *m)
         }
         if (!c->rb_valid[k] || c->rb_var[k] != o->i_rhs) refresh_rhs_rb(c, m->T, m->V.rhs, rb, o->i_rhs);
         m->V.rhs_rb = rb;
+        // the working copy of phi for the colour passes of a relaxation block
+        R *&pr = phi_rb_pool(c, (R *)nullptr);
+        if (!pr) {
+            const size_t bytes = (size_t)c->n_slots * rb_stride(c) * sizeof(R);
+            if (ctx_malloc(c, &pr, bytes) != hipSuccess) {
+                pr = nullptr;
+                return c->fail(AFG_ERR_MEMORY, "hipMalloc(%zu) for the colour-split phi", bytes);
+            }
+        }
+        m->V.phi_rb = c->split_passes ? pr : nullptr;
     }
     return 0;
 }
@@ -907,6 +936,8 @@ BottomArgs bottom_args(MgT<R> &m, int top)
     B.n_up = m.o->n_cycle_up;
     B.n_base = m.o->n_cycle_base;
     B.base64 = sizeof(R) == 4 && c->mixed_base64 ? 1 : 0;
+    B.trace = c->d_ptrace;
+    B.trace_cap = c->ptrace_cap;
     return B;
 }
 
@@ -1363,9 +1394,11 @@ static int tune_tiles(afg_ctx *c, const afg_mg_opts *o, int reps, double *ms, co
             if (nv < 2 || L.n == 0) continue;
             const int p = sizeof(R) == 4;
             const int dflt = Lv.tile[p] + (Lv.pairs[p] ? 3 : 0);
-            // time of two passes: two single passes (mode 0) or one fused
-            // pair (mode 1: the copy of phi and the second pool in turn, the
-            // other levels from the tree's phi), per shape
+            // time of two passes: two single passes (mode 0; on the
+            // colour-split copy of phi, a first and a last pass, where the
+            // relaxation blocks use it) or one fused pair (mode 1: the copy
+            // of phi and the second pool in turn, the other levels from the
+            // tree's phi), per shape
             double per[6];
             for (int mode = 0; mode < 2; mode++)
                 for (int v = 0; v < nv; v++) {
@@ -1374,6 +1407,8 @@ static int tune_tiles(afg_ctx *c, const afg_mg_opts *o, int reps, double *ms, co
                         if (mode)
                             launch_gsrb_pair(c->dim, c->stream, m.T, L, m.V, (r & 1) ? m.phi2 : scratch,
                                              (r & 1) ? scratch : m.phi2, 1, 1);
+                        else if (launch_gsrb_split(c->dim, c->stream, m.T, L, V, 1, 0, 1))
+                            launch_gsrb_split(c->dim, c->stream, m.T, L, V, 2, 2, 1);
                         else
                             for (int rb = 1; rb <= 2; rb++) launch_gsrb_pass(c->dim, c->stream, m.T, L, V, rb, 1);
                     };
@@ -1551,7 +1586,10 @@ int afg_ctx_create(int dim, int n_cell, int n_var, afg_ctx **out)
     if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_out, 8 * sizeof(double), hipHostMallocDefault);
     // the persistent kernel's barrier counters start at zero (it leaves them so)
     if (e == hipSuccess) e = ctx_malloc(c, &c->d_psync, (4 + PERSIST_MAX_RANKS) * sizeof(unsigned));
-    if (e == hipSuccess) e = hipMemset(c->d_psync, 0, (4 + PERSIST_MAX_RANKS) * sizeof(unsigned));
+    // (on the context's own stream: a NULL-stream call here would disturb a
+    // graph capture that another thread's context may be running)
+    if (e == hipSuccess) e = hipMemsetAsync(c->d_psync, 0, (4 + PERSIST_MAX_RANKS) * sizeof(unsigned), c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_psync, 4 * sizeof(unsigned), hipHostMallocDefault);
     {
         // contexts take the XCDs in turn, so that the persistent kernels of
@@ -3095,6 +3133,16 @@ int afg_set_bottom_cells(afg_ctx *c, int64_t cells, int64_t wave_cells)
     return choose_bottom(c);
 }
 
+int afg_set_split_passes(afg_ctx *c, int enabled)
+{
+    if (int e = check_ctx(c)) return e;
+    if (int e_ = stream_sync(c)) return e_;
+    c->split_passes = enabled != 0;
+    for (auto &g : c->graphs) destroy_graph(g.second);
+    c->graphs.clear();
+    return 0;
+}
+
 int afg_set_persist(afg_ctx *c, int wgs, int64_t cells)
 {
     if (int e = check_ctx(c)) return e;
@@ -3114,7 +3162,8 @@ int afg_persist_trace(afg_ctx *c, int cap, int64_t *ticks, int *n)
     if (int e_ = stream_sync(c)) return e_;
     if (ticks && n && c->d_ptrace) {
         std::vector<unsigned long long> h(c->ptrace_cap);
-        HIP_TRY(c, hipMemcpy(h.data(), c->d_ptrace, h.size() * sizeof(h[0]), hipMemcpyDeviceToHost));
+        HIP_TRY(c, hipMemcpyAsync(h.data(), c->d_ptrace, h.size() * sizeof(h[0]), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
         int k = 0;
         while (k < (int)h.size() && h[k] != 0) {
             ticks[k] = (int64_t)h[k];
@@ -3133,7 +3182,10 @@ int afg_persist_trace(afg_ctx *c, int cap, int64_t *ticks, int *n)
         for (auto &g : c->graphs) destroy_graph(g.second);
         c->graphs.clear();
     }
-    if (c->d_ptrace) HIP_TRY(c, hipMemset(c->d_ptrace, 0, (size_t)c->ptrace_cap * sizeof(unsigned long long)));
+    if (c->d_ptrace) {
+        HIP_TRY(c, hipMemsetAsync(c->d_ptrace, 0, (size_t)c->ptrace_cap * sizeof(unsigned long long), c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
     return 0;
 }
 

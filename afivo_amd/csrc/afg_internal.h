@@ -28,6 +28,12 @@ void launch_gsrb_pass(int dim, hipStream_t s, const DevTree &T, const LevelArgs 
 template <typename R>
 int launch_gsrb_pair(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V,
                      const R *in, R *out, int rbA, int fly);
+// one colour pass of a relaxation block on the colour-split copy V.phi_rb
+// (3D, nc 8 and 16, tiles): mode 0 the first pass of the block, 1 a middle
+// one, 2 the last; returns 0 (nothing launched) where it does not apply
+template <typename R>
+int launch_gsrb_split(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int rb,
+                      int mode, int fly);
 template <typename R>
 void launch_residual(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
                      const MgVarsT<R> &V);

@@ -91,6 +91,26 @@ __global__ __launch_bounds__((GsrbTile<DIM, NC, V>::THREADS)) void k_gsrb_tile(D
     tile_stage_b<DIM, NC, V>(T, L, v, lds, box0, op_auto, rb, threadIdx.x, rhs);
 }
 
+// one colour pass of a relaxation block on the colour-split copy of phi (3D
+// tiles): MODE 0 the first pass (box arrays in, FLY as k_gsrb_tile), 1 a
+// middle one, 2 the last (phi_rb in, box arrays out)
+template <int DIM, int NC, int MODE, int FLY, typename R, int V>
+__global__ __launch_bounds__((GsrbTile<DIM, NC, V>::THREADS)) void k_gsrb_split(DevTree T, LevelArgs L,
+                                                                               LevelViewT<R> v, R *rbp,
+                                                                               int op_auto, int rb)
+{
+    using G = GsrbTile<DIM, NC, V>;
+    __shared__ R lds[G::BOXES * G::LEN];
+    R rhs[G::PER];
+    const int box0 = xcd_tile(blockIdx.x, gridDim.x) * G::BOXES;
+    if (MODE == 0)
+        tile_stage_a<DIM, NC, FLY, V>(T, L, v, v, lds, box0, rb, threadIdx.x, rhs);
+    else
+        split_stage_a<DIM, NC, V>(T, L, v, rbp, lds, box0, rb, threadIdx.x, rhs);
+    __syncthreads();
+    split_stage_b<DIM, NC, V, MODE>(T, L, v, rbp, lds, box0, op_auto, rb, threadIdx.x, rhs);
+}
+
 // two colour passes (rbA, rbA + 1) by tiles, phi read from v and written to out
 template <int DIM, int NC, int FLYA, typename R, int V>
 __global__ __launch_bounds__((GsrbTile<DIM, NC, V>::THREADS)) void k_gsrb_pair(DevTree T, LevelArgs L,
@@ -443,9 +463,18 @@ __global__ __launch_bounds__(BOTTOM_BLOCK) void k_bottom(DevTree T, MgVarsT<R> V
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     };
+    int n_trace = 0;
+    auto stamp = [&] {
+        if (A.trace && threadIdx.x == 0 && n_trace < A.trace_cap) A.trace[n_trace++] = __builtin_amdgcn_s_memrealtime();
+    };
+    stamp();
     bottom_run<DIM, AUTO, R>(T, V, *B, A, lds, threadIdx.x, blockDim.x, warpSize,
-                             [] { __syncthreads(); }, wave_sync,
-                             [](auto v, int mask) { return __shfl_xor(v, mask); });
+                             [&] {
+                                 __syncthreads();
+                                 stamp();
+                             },
+                             wave_sync, [](auto v, int mask) { return __shfl_xor(v, mask); });
+    stamp();
 }
 
 template <int DIM, typename R>
@@ -571,6 +600,44 @@ int launch_gsrb_pair(int dim, hipStream_t s, const DevTree &T, const LevelArgs &
     case 216: gsrb_pair<2, 16, R>(s, T, L, V, in, out, rbA, fly); return 1;
     case 308: gsrb_pair<3, 8, R>(s, T, L, V, in, out, rbA, fly); return 1;
     case 316: gsrb_pair<3, 16, R>(s, T, L, V, in, out, rbA, fly); return 1;
+    default: return 0;
+    }
+}
+
+template <int NC, typename R, int V>
+void gsrb_split_v(hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &Vs, int rb, int mode,
+                  int fly)
+{
+    using G = GsrbTile<3, NC, V>;
+    const int grid = (L.n + G::BOXES - 1) / G::BOXES;
+    const LevelViewT<R> v = pool_view(T, Vs);
+    if (mode == 0 && fly)
+        k_gsrb_split<3, NC, 0, 1, R, V><<<grid, G::THREADS, 0, s>>>(T, L, v, Vs.phi_rb, Vs.op_auto, rb);
+    else if (mode == 0)
+        k_gsrb_split<3, NC, 0, 0, R, V><<<grid, G::THREADS, 0, s>>>(T, L, v, Vs.phi_rb, Vs.op_auto, rb);
+    else if (mode == 1)
+        k_gsrb_split<3, NC, 1, 1, R, V><<<grid, G::THREADS, 0, s>>>(T, L, v, Vs.phi_rb, Vs.op_auto, rb);
+    else
+        k_gsrb_split<3, NC, 2, 1, R, V><<<grid, G::THREADS, 0, s>>>(T, L, v, Vs.phi_rb, Vs.op_auto, rb);
+}
+
+template <int NC, typename R>
+void gsrb_split(hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int rb, int mode, int fly)
+{
+    if (L.tile == 1) return gsrb_split_v<NC, R, 1>(s, T, L, V, rb, mode, fly);
+    if (L.tile == 2) return gsrb_split_v<NC, R, 2>(s, T, L, V, rb, mode, fly);
+    gsrb_split_v<NC, R, 0>(s, T, L, V, rb, mode, fly);
+}
+
+template <typename R>
+int launch_gsrb_split(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int rb,
+                      int mode, int fly)
+{
+    if (dim != 3 || !V.phi_rb || !V.rhs_rb || per_cell_only(T, V)) return 0;
+    if (L.n <= 0) return 1;
+    switch (L.nc) {
+    case 8: gsrb_split<8, R>(s, T, L, V, rb, mode, fly); return 1;
+    case 16: gsrb_split<16, R>(s, T, L, V, rb, mode, fly); return 1;
     default: return 0;
     }
 }
@@ -919,6 +986,8 @@ void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVa
 // the two value types of the launchers
 #define AFG_INSTANTIATE(R)                                                                               \
     template void launch_fill_gc<R>(int, hipStream_t, const DevTree &, const LevelArgs &, R *);         \
+    template int launch_gsrb_split<R>(int, hipStream_t, const DevTree &, const LevelArgs &,            \
+                                      const MgVarsT<R> &, int, int, int);                              \
     template int launch_gsrb_pair<R>(int, hipStream_t, const DevTree &, const LevelArgs &,             \
                                      const MgVarsT<R> &, const R *, R *, int, int);                    \
     template void launch_fill_diff<R>(int, hipStream_t, const DevTree &, const LevelArgs &,            \

@@ -102,6 +102,9 @@ struct MgVarsT {
     // line of the box), and the stages that write rhs keep it current
     R *rhs_rb = nullptr;
     int rb_stride = 0;
+    // phi by colour in the same layout, the working copy of the colour passes
+    // of a relaxation block (3D tiles; nullptr: not used)
+    R *phi_rb = nullptr;
 };
 using MgVars = MgVarsT<double>;
 
@@ -166,6 +169,10 @@ struct BottomArgs {
     // FP32 stages: the relaxation of the lowest level (the coarsest-grid
     // solve) in FP64 arithmetic on the FP32 values
     int base64 = 0;
+    // when not null: s_memrealtime at the start and after every step (the
+    // bottom kernel's thread 0), at most trace_cap values
+    unsigned long long *trace = nullptr;
+    int trace_cap = 0;
 };
 
 // The persistent kernel: the lower part of a V-cycle (levels 0..A.hl, data in

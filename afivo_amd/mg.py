@@ -321,6 +321,10 @@ class DeviceTree:
     def set_bottom_cells(self, cells, wave_cells=0):
         self._c(_lib.load().afg_set_bottom_cells(self.h, int(cells), int(wave_cells)))
 
+    def set_split_passes(self, enabled):
+        """Relaxation blocks on a colour-split working copy of phi (3D tiles)."""
+        self._c(_lib.load().afg_set_split_passes(self.h, int(bool(enabled))))
+
     def set_persist(self, wgs, cells=32768):
         """The persistent kernel for the levels of at most `cells` cells each
         (wgs workgroups on one XCD; 0 turns it off)."""

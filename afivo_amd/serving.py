@@ -128,22 +128,23 @@ MAX_STEPS = 60
 def forest_bytes(states, precision="mixed", n_var=3):
     """Device bytes a forest of these trees takes: every pool the solver
     allocates per box (the variables, the second phi pool of the fused pass
-    pairs, in 3D the colour-split rhs; in mixed precision the FP32 copies of
-    phi, rhs and tmp and their second phi pool and colour-split rhs), the
-    topology tables, per-tree and fixed parts for norms and the bottom
-    kernel's tables, and the allocator's rounding of each of the context's
-    dozen large allocations to 2 MiB.  An upper bound of afg_device_bytes
-    and of the device memory the forest takes."""
+    pairs, in 3D the colour-split rhs and the colour-split working copy of
+    phi; in mixed precision the FP32 copies of phi, rhs and tmp and their
+    second phi pool and colour-split rhs and phi), the topology tables,
+    per-tree and fixed parts for norms and the bottom kernel's tables, and
+    the allocator's rounding of each of the context's fourteen large
+    allocations to 2 MiB.  An upper bound of afg_device_bytes and of the
+    device memory the forest takes."""
     st = states[0]
     dim, nc = st.dim, st.nc
     stride = -(-(nc + 2) ** dim // 16) * 16
     interior = nc ** dim
-    per_box = (n_var + 1) * stride * 8 + (interior * 8 if dim == 3 else 0)
+    per_box = (n_var + 1) * stride * 8 + (2 * interior * 8 if dim == 3 else 0)
     if precision == "mixed":
-        per_box += 4 * stride * 4 + (interior * 4 if dim == 3 else 0)
+        per_box += 4 * stride * 4 + (2 * interior * 4 if dim == 3 else 0)
     per_box += (2 * dim + 8) * 4 + 2          # neighbours, parent, child offset, lists, tags
     boxes = sum(s.n_boxes for s in states)
-    return boxes * per_box + len(states) * 4096 + (128 << 10) + 12 * (2 << 20)
+    return boxes * per_box + len(states) * 4096 + (128 << 10) + 14 * (2 << 20)
 
 
 def combine_records(states):

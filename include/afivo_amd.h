@@ -372,6 +372,13 @@ int afg_mixed_solve(afg_ctx *ctx, const afg_mg_opts *opts, int have_guess, doubl
 int afg_set_bottom_cells(afg_ctx *ctx, int64_t cells, int64_t wave_cells);
 /* The cell limit in use for the FP64 (fp32 = 0) or FP32 stages. */
 int afg_get_bottom_cells(const afg_ctx *ctx, int fp32, int64_t *cells);
+/* Relaxation blocks of two or more single colour passes on 3D levels with
+ * 8^3 or 16^3 boxes keep the level's phi by colour in a working copy between
+ * their passes (the first pass reads the box arrays, the last writes them);
+ * enabled = 0 runs every pass on the box arrays.  Default on.  Results are
+ * identical either way.  No counterpart in the reference (an execution
+ * control of gsrb_boxes, m_aX_mg.f90:397-420). */
+int afg_set_split_passes(afg_ctx *ctx, int enabled);
 /* The persistent kernel: with wgs in 1..32, the lowest levels of the tree,
  * while each holds at most `cells` cells, run their part of every V-cycle
  * (mg$D_fas_vcycle below and at the highest such level, m_aX_mg.f90:218-241,
@@ -386,9 +393,10 @@ int afg_set_persist(afg_ctx *ctx, int wgs, int64_t cells);
 /* The workgroups (0: off), the cell limit, and the highest level the
  * persistent kernel takes for the FP64 stages (-1000: none). */
 int afg_get_persist(const afg_ctx *ctx, int *wgs, int64_t *cells, int *top_lvl);
-/* Stage trace of the persistent kernel (a diagnostic): returns in ticks[0..n)
- * the s_memrealtime values (100 MHz) that its first workgroup took at the
- * start of the last launch and after each barrier since the previous call,
+/* Stage trace of the persistent kernel and of the bottom kernel (a
+ * diagnostic): returns in ticks[0..n) the s_memrealtime values (100 MHz) that
+ * the first workgroup of the last such launch took at its start, after each
+ * barrier between its steps and at its end (the bottom kernel) since the previous call,
  * when ticks and n are given (ticks holds the capacity set by the previous
  * call); then records at most `cap` values per launch from now on (cap = 0:
  * none, the default). */

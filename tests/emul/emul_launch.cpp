@@ -130,6 +130,65 @@ void gsrb_tiles(const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int r
     gsrb_tiles_v<DIM, NC, R, 0>(T, L, V, rb, fly);
 }
 
+// the colour passes on the colour-split copy of phi: workgroups one after
+// the other, each stage over all threads of the workgroup before the next
+template <int NC, typename R, int SH>
+void gsrb_split_v(const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int rb, int mode, int fly)
+{
+    using G = GsrbTile<3, NC, SH>;
+    const LevelViewT<R> v = pool_view(T, V);
+    const int grid = (L.n + G::BOXES - 1) / G::BOXES;
+    R *lds = (R *)emul_alloc(sizeof(R) * G::BOXES * G::LEN);
+    std::vector<R> rhs((size_t)G::THREADS * G::PER);
+    for (int blk = 0; blk < grid; blk++) {
+        const int box0 = xcd_tile(blk, grid) * G::BOXES;
+        for (int t = 0; t < G::THREADS; t++) {
+            R *r = &rhs[(size_t)t * G::PER];
+            if (mode == 0 && fly)
+                tile_stage_a<3, NC, 1, SH>(T, L, v, v, lds, box0, rb, t, r);
+            else if (mode == 0)
+                tile_stage_a<3, NC, 0, SH>(T, L, v, v, lds, box0, rb, t, r);
+            else
+                split_stage_a<3, NC, SH>(T, L, v, V.phi_rb, lds, box0, rb, t, r);
+        }
+        for (int t = 0; t < G::THREADS; t++) {
+            const R *r = &rhs[(size_t)t * G::PER];
+            if (mode == 0)
+                split_stage_b<3, NC, SH, 0>(T, L, v, V.phi_rb, lds, box0, V.op_auto, rb, t, r);
+            else if (mode == 1)
+                split_stage_b<3, NC, SH, 1>(T, L, v, V.phi_rb, lds, box0, V.op_auto, rb, t, r);
+            else
+                split_stage_b<3, NC, SH, 2>(T, L, v, V.phi_rb, lds, box0, V.op_auto, rb, t, r);
+        }
+    }
+    emul_free(lds);
+}
+
+template <typename R>
+int launch_gsrb_split(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int rb,
+                      int mode, int fly)
+{
+    if (dim != 3 || !V.phi_rb || !V.rhs_rb || per_cell_only(T, V)) return 0;
+    if (L.n <= 0) return 1;
+    if (L.nc != 8 && L.nc != 16) return 0;
+    emul_submit(s, [=] {
+        auto go = [&](auto ncv) {
+            constexpr int NC = decltype(ncv)::value;
+            if (L.tile == 1)
+                gsrb_split_v<NC, R, 1>(T, L, V, rb, mode, fly);
+            else if (L.tile == 2)
+                gsrb_split_v<NC, R, 2>(T, L, V, rb, mode, fly);
+            else
+                gsrb_split_v<NC, R, 0>(T, L, V, rb, mode, fly);
+        };
+        if (L.nc == 8)
+            go(IC<8>());
+        else
+            go(IC<16>());
+    });
+    return 1;
+}
+
 // the tiled child stage of the device: workgroups one after the other, each
 // phase over all threads of the workgroup before the next
 template <int DIM, int NC, typename R>
@@ -646,6 +705,8 @@ void launch_residual_levels(int dim, hipStream_t s, const DevTree &T, const MgVa
 }
 
 #define AFG_INSTANTIATE(R)                                                                               \
+    template int launch_gsrb_split<R>(int, hipStream_t, const DevTree &, const LevelArgs &,            \
+                                      const MgVarsT<R> &, int, int, int);                              \
     template int launch_gsrb_pair<R>(int, hipStream_t, const DevTree &, const LevelArgs &,             \
                                      const MgVarsT<R> &, const R *, R *, int, int);                    \
     template void launch_fill_gc<R>(int, hipStream_t, const DevTree &, const LevelArgs &, R *);         \

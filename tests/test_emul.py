@@ -29,6 +29,7 @@ from test_gpu_parity import (  # noqa: E402,F401
     test_boundary_values_and_neumann,
     test_bottom_kernel_limit_is_exact,
     test_persistent_kernel_is_exact,
+    test_split_passes_are_exact,
     test_c1_full_config_bitexact,
     test_c2_full_config_bitexact,
     test_cylindrical_solve_bitexact,

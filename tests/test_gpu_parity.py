@@ -341,6 +341,47 @@ def test_persistent_kernel_is_exact(kind, wgs, cells):
     assert_bitexact(out[3], out[1], f"{kind} FP32 persist residual")
 
 
+@pytest.mark.parametrize("kind", ["3d_nc8", "3d_nc16_eps"])
+@pytest.mark.parametrize("shape", [0, 1, 2])
+@pytest.mark.parametrize("n_down,n_up", [(2, 2), (1, 3), (3, 1)])
+def test_split_passes_are_exact(kind, shape, n_down, n_up):
+    """Relaxation blocks on the colour-split working copy of phi (first,
+    middle and last passes) give the oracle's bits with every tile shape and
+    block length, in FP64; FP32 stages give the same bits as without it."""
+    p = problem(kind)
+    eps = p.eps_hi is not None
+    mg = M.default_mg(eps=eps, n_down=n_down, n_up=n_up, n_base=p.n_base)
+    om = O.default_opts(n_down, n_up, p.n_base, eps=eps)
+    n_var = len(p.state.data)
+    dt = M.DeviceTree.from_state(p.state)
+    dt.set_bottom_cells(0)            # every level through the tiled pass
+    for lvl in range(dt.min_lvl, dt.highest_lvl + 1):
+        if T.level_ncell(p.state.nc, lvl) in (8, 16):
+            dt.set_tile(lvl, shape)
+            dt.set_level_pass_pairs(lvl, False)
+    dt, ot, rd, ro = solve_both(p.state, n_var, mg, om, 1, 2, dt=dt)
+    compare_all(dt, ot, n_var, f"{kind} split shape {shape} {n_down}/{n_up}")
+    assert np.array_equal(rd, ro)
+    dt.close()
+    out = []
+    for split in (False, True):
+        d = M.DeviceTree.from_state(p.state, n_var + 1)
+        d.set_split_passes(split)
+        d.set_bottom_cells(0)
+        for lvl in range(d.min_lvl, d.highest_lvl + 1):
+            if T.level_ncell(p.state.nc, lvl) in (8, 16):
+                d.set_tile(lvl, shape, fp32=True)
+                d.set_level_pass_pairs(lvl, False, fp32=True)
+        for iv in range(1, n_var + 1):
+            d.fp32_from_cc(iv, iv)
+        M.mg_fas_fmg_fp32(d, mg, True, False)
+        M.mg_fas_vcycle_fp32(d, mg, d.highest_lvl, True)
+        d.fp32_to_cc(1, n_var + 1)
+        out.append(d.download(n_var + 1))
+        d.close()
+    assert_bitexact(out[1], out[0], f"{kind} FP32 split shape {shape}")
+
+
 @pytest.mark.parametrize("kind", ["2d_nc8", "2d_nc16", "3d_nc8", "3d_nc16_eps"])
 @pytest.mark.parametrize("shape", [1, 2])
 def test_tile_shapes_are_exact(kind, shape):
