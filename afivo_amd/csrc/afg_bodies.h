@@ -2178,11 +2178,42 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLeve
     T.is_parent = t_isp;
     R *data = (R *)(tb + B.table_bytes);
 
-    auto copy_in = [&](const R *src, int off, int li) {
-        const int len = B.box_len[li], n = B.lv[li].n, slot0 = B.lv[li].slot0;
-        for (int e = tid; e < n * len; e += nt) {
-            const int b = e / len;
-            data[off + e] = src[(size_t)(slot0 + b) * T.stride + (e - b * len)];
+    // the levels' values: one pass over all levels, each element's loads of
+    // every variable (and those of a second element) issued before any is
+    // stored, so that few memory round trips stand before the first step
+    auto copy_in_all = [&] {
+        int total = 0;
+        for (int li = 0; li <= A.hl; li++) total += B.lv[li].n * B.box_len[li];
+        auto locate = [&](int e, int &li, int &r) {
+            li = 0;
+            r = e;
+            while (r >= B.lv[li].n * B.box_len[li]) {
+                r -= B.lv[li].n * B.box_len[li];
+                li++;
+            }
+        };
+        for (int e = tid; e < total; e += 2 * nt) {
+            const int e2 = e + nt;
+            int li, r, li2 = 0, r2 = 0;
+            locate(e, li, r);
+            const bool two = e2 < total;
+            if (two) locate(e2, li2, r2);
+            const int b = r / B.box_len[li], b2 = two ? r2 / B.box_len[li2] : 0;
+            const size_t g = (size_t)(B.lv[li].slot0 + b) * T.stride + (r - b * B.box_len[li]);
+            const size_t g2 = two ? (size_t)(B.lv[li2].slot0 + b2) * T.stride + (r2 - b2 * B.box_len[li2]) : g;
+            const R p0 = V.phi[g], p1 = V.rhs[g], p2 = V.tmp[g];
+            const R q0 = V.phi[g2], q1 = V.rhs[g2], q2 = V.tmp[g2];
+            const R p3 = AUTO ? V.eps[g] : R(0), q3 = AUTO ? V.eps[g2] : R(0);
+            data[B.lds_off[0][li] + r] = p0;
+            data[B.lds_off[1][li] + r] = p1;
+            data[B.lds_off[2][li] + r] = p2;
+            if (AUTO) data[B.lds_off[3][li] + r] = p3;
+            if (two) {
+                data[B.lds_off[0][li2] + r2] = q0;
+                data[B.lds_off[1][li2] + r2] = q1;
+                data[B.lds_off[2][li2] + r2] = q2;
+                if (AUTO) data[B.lds_off[3][li2] + r2] = q3;
+            }
         }
     };
     auto copy_out = [&](R *dst, int off, int li) {
@@ -2192,12 +2223,7 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLeve
             dst[(size_t)(slot0 + b) * T.stride + (e - b * len)] = data[off + e];
         }
     };
-    for (int li = 0; li <= A.hl; li++) {
-        copy_in(V.phi, B.lds_off[0][li], li);
-        copy_in(V.rhs, B.lds_off[1][li], li);
-        copy_in(V.tmp, B.lds_off[2][li], li);
-        if (AUTO) copy_in(V.eps, B.lds_off[3][li], li);
-    }
+    copy_in_all();
     sync();
 
     // every level of the kernel is in LDS (the host takes only levels that
@@ -2231,14 +2257,17 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLeve
                 // colour, or K + c1 phi at physical boundaries)
                 const LevelArgs L = B.lv[0];
                 if constexpr (NC == 2) {
-                    if (B.base_fast && wave == 64) {   // a full wavefront: the lanes of the first
+                    if (B.base_fast && wave == 64) {
+                        // cells of a box on the lanes of a wavefront, the boxes' groups
+                        // over every wavefront of the threads of this step
+                        const int lane = t & 63, w = t >> 6, nw = n >> 6;
                         if constexpr (sizeof(R) == 4) {
                             if (A.base64) {
-                                if (t < 64) base_wave_run<DIM, AUTO, R, double>(T, own, B, A, L.dx2, L.n, t, xshfl);
+                                base_wave_run<DIM, AUTO, R, double>(T, own, B, A, L.dx2, L.n, lane, xshfl, w, nw);
                                 break;
                             }
                         }
-                        if (t < 64) base_wave_run<DIM, AUTO, R, R>(T, own, B, A, L.dx2, L.n, t, xshfl);
+                        base_wave_run<DIM, AUTO, R, R>(T, own, B, A, L.dx2, L.n, lane, xshfl, w, nw);
                         break;
                     }
                 }
