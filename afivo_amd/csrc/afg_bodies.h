@@ -766,9 +766,11 @@ AFG_DEV R ghost_value_split(const DevTree &T, const R *rbp, int rb_stride, const
     return c2 * b + c1 * center;
 }
 
-// Stage A of the middle and last passes: rhs of the thread's cells, both
-// colours of the tile's boxes from phi_rb into LDS, and the face ghosts next
-// to cells of colour rb, computed from phi_rb.
+// Stage A of the middle and last passes: rhs of the thread's cells, the other
+// colour of the tile's boxes from phi_rb into LDS (the only interior cells
+// the updates read), and the face ghosts next to cells of colour rb, computed
+// from phi_rb (the cell next to the ghost is read only where ghost_value
+// uses it: at refinement and physical boundaries).
 template <int DIM, int NC, int V, typename R>
 AFG_DEV void split_stage_a(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &v, const R *rbp, R *lds,
                            int box0, int rb, int tid, R *rhs)
@@ -782,15 +784,16 @@ AFG_DEV void split_stage_a(const DevTree &T, const LevelArgs &L, const LevelView
         if (e < G::BOXES * G::HALF && box0 + b < L.n)
             rhs[r] = v.rhs_rb[(size_t)(L.slot0 + box0 + b) * v.rb_stride + ((rb + 1) & 1) * G::HALF + (e - b * G::HALF)];
     }
-    // interiors: split index c holds colour 1 (i + j + k odd) for c < HALF
-    for (int e = tid; e < G::BOXES * G::CELLS; e += G::THREADS) {
-        const int b = e / G::CELLS;
+    // interior cells of the other colour: split index c holds colour 1
+    // (i + j + k odd) for c < HALF, colour 2 above
+    for (int e = tid; e < G::BOXES * G::HALF; e += G::THREADS) {
+        const int b = e / G::HALF;
         if (box0 + b >= L.n) break;
-        const int c = e - b * G::CELLS;
-        const int hi = c >= G::HALF;
+        const int c = e - b * G::HALF;
         int i, j, k;
-        colour_cell<DIM>(c - hi * G::HALF, NC, hi ? 2 : 1, i, j, k);
-        lds[b * G::LEN + i + S * (j + S * k)] = ldv(rbp + (size_t)(L.slot0 + box0 + b) * v.rb_stride + c);
+        colour_cell<DIM>(c, NC, rb + 1, i, j, k);
+        lds[b * G::LEN + i + S * (j + S * k)] =
+            ldv(rbp + (size_t)(L.slot0 + box0 + b) * v.rb_stride + (rb & 1) * G::HALF + c);
     }
     // face ghosts next to cells of the colour (the enumeration of tile_stage_a)
     constexpr int FH = G::FACE / 2;
@@ -810,7 +813,9 @@ AFG_DEV void split_stage_a(const DevTree &T, const LevelArgs &L, const LevelView
         const int gj = dn == 1 ? (high ? NC + 1 : 0) : j;
         const int gk = dn == 2 ? (high ? NC + 1 : 0) : k;
         const int slot = L.slot0 + box0 + b;
-        const R center = ldv(rbp + (size_t)slot * v.rb_stride + colour_index<DIM>(i, j, k, NC));
+        const R center = T.nb[slot * (2 * DIM) + f] >= 0
+                             ? R(0)
+                             : ldv(rbp + (size_t)slot * v.rb_stride + colour_index<DIM>(i, j, k, NC));
         lds[b * G::LEN + gi + S * (gj + S * gk)] =
             ghost_value_split<DIM>(T, rbp, v.rb_stride, v, L.pnc, L.dr, slot, f, NC, i, j, k, center);
     }
@@ -1935,52 +1940,73 @@ AFG_DEV void nc_dispatch(int nc, F &&f)
 // update_coarse (child part, then parent part with the parent's ghost fill);
 // at level 0: a block of n_base sweeps; per level li from 1 up to hl:
 // correct_children (correction, then tmp = phi - tmp on the parents), the
-// ghost fill and a block of n_up sweeps.  A block of n sweeps is the first
-// colour pass, then (ghost fill, colour pass) for passes 2..2n, then the
-// ghost fill: 4n steps.  Filling the ghosts before a pass gives the face
-// neighbours that the reference's fill after the previous pass stores, so
-// every pass reads them from memory; the ghost layer after the block is the
-// reference's, the last fill covering every face.  At level 0 the block is
-// one step for all 2n passes (from the cell list, computing the face
-// neighbours) and the ghost fill.
+// ghost fill and a block of n_up sweeps.  Two forms of a block of n sweeps:
+// - stored ghosts (A.fly = 0): the first colour pass, then (ghost fill,
+//   colour pass) for passes 2..2n, then the ghost fill: 4n steps.  Filling
+//   the ghosts before a pass gives the face neighbours that the reference's
+//   fill after the previous pass stores, so every pass reads them from
+//   memory; the ghost layer after the block is the reference's, the last
+//   fill covering every face.
+// - computed ghosts (A.fly = 1): the 2n colour passes, the first of a
+//   down-sweep block reading the stored ghosts and every other pass computing
+//   the face ghosts it reads (the values the reference's fill after the
+//   previous pass stores), then the ghost fill: 2n + 1 steps.  The down-sweep
+//   block has no fill (update_coarse's residual computes the face ghosts it
+//   would store) and the up-sweep block none before its first pass.
+// At level 0 the block is one step for all 2n passes (from the cell list,
+// computing the face neighbours) and the ghost fill.
 enum BottomOp { BOP_GSRB, BOP_FILL, BOP_UC_CHILD, BOP_UC_PARENT, BOP_CORRECT, BOP_DIFF, BOP_BASE };
 
 struct BottomStep {
-    int op, li, rb;
+    int op, li, rb, fly;
 };
 
-// step r of a relaxation block of 4n steps
-AFG_HD inline BottomStep relax_step(int li, int r, int n)
+// number of steps of a relaxation block of n sweeps
+AFG_HD inline int relax_steps(const BottomArgs &A, int n)
 {
-    if (r == 0) return {BOP_GSRB, li, 1};
-    if (r == 4 * n - 1 || (r & 1)) return {BOP_FILL, li, 0};
-    return {BOP_GSRB, li, r / 2 + 1};
+    return A.fly ? 2 * n + 1 : 4 * n;
+}
+
+// step r of a relaxation block of n sweeps; `first_fly` tells whether its
+// first pass computes its face ghosts
+AFG_HD inline BottomStep relax_step(const BottomArgs &A, int li, int r, int n, int first_fly)
+{
+    if (A.fly) {
+        if (r == 2 * n) return {BOP_FILL, li, 0, 0};
+        return {BOP_GSRB, li, r + 1, r > 0 ? 1 : first_fly};
+    }
+    if (r == 0) return {BOP_GSRB, li, 1, 0};
+    if (r == 4 * n - 1 || (r & 1)) return {BOP_FILL, li, 0, 0};
+    return {BOP_GSRB, li, r / 2 + 1, 0};
 }
 
 // Step k of the sequence above; every block of steps has a fixed length.
 AFG_HD inline BottomStep bottom_step(const BottomArgs &A, int k)
 {
-    const int nd = 4 * A.n_down + 2;
+    const int rd = A.fly ? 2 * A.n_down : 4 * A.n_down;   // the down block has no fill when A.fly
+    const int nd = rd + 2;
     const int nb = A.n_base > 0 ? 2 : 0;
-    const int nu = 3 + 4 * A.n_up;
+    const int nu = (A.fly ? 2 : 3) + relax_steps(A, A.n_up);
     if (k < A.hl * nd) {
         const int li = A.hl - k / nd, r = k % nd;
-        if (r < 4 * A.n_down) return relax_step(li, r, A.n_down);
-        return {r == 4 * A.n_down ? BOP_UC_CHILD : BOP_UC_PARENT, li, 0};
+        if (r < rd) return relax_step(A, li, r, A.n_down, 0);
+        return {r == rd ? BOP_UC_CHILD : BOP_UC_PARENT, li, 0, 0};
     }
     k -= A.hl * nd;
-    if (k < nb) return {k == 0 ? BOP_BASE : BOP_FILL, 0, 0};
+    if (k < nb) return {k == 0 ? BOP_BASE : BOP_FILL, 0, 0, 0};
     k -= nb;
     const int li = 1 + k / nu, r = k % nu;
-    if (r == 0) return {BOP_CORRECT, li, 0};
-    if (r == 1) return {BOP_DIFF, li, 0};
-    if (r == 2) return {BOP_FILL, li, 0};
-    return relax_step(li, r - 3, A.n_up);
+    if (r == 0) return {BOP_CORRECT, li, 0, 0};
+    if (r == 1) return {BOP_DIFF, li, 0, 0};
+    if (A.fly) return relax_step(A, li, r - 2, A.n_up, 1);
+    if (r == 2) return {BOP_FILL, li, 0, 0};
+    return relax_step(A, li, r - 3, A.n_up, 0);
 }
 
 AFG_HD inline int bottom_steps(const BottomArgs &A)
 {
-    return A.hl * (4 * A.n_down + 2) + (A.n_base > 0 ? 2 : 0) + A.hl * (3 + 4 * A.n_up);
+    const int rd = A.fly ? 2 * A.n_down : 4 * A.n_down;
+    return A.hl * (rd + 2) + (A.n_base > 0 ? 2 : 0) + A.hl * ((A.fly ? 2 : 3) + relax_steps(A, A.n_up));
 }
 
 // cell counts with their own code in the bottom kernel (others: generic)
@@ -2281,7 +2307,7 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLeve
             case BOP_GSRB: {
                 const LevelArgs L = B.lv[li];
                 const int tot = gsrb_threads<DIM>(L.n, NC ? NC : L.nc);
-                for (int g = t; g < tot; g += n) gsrb_body<DIM, NC>(T, L, own, par, AUTO, st.rb, 0, g);
+                for (int g = t; g < tot; g += n) gsrb_body<DIM, NC>(T, L, own, par, AUTO, st.rb, st.fly, g);
             } break;
             case BOP_FILL: {
                 const LevelArgs L = B.lv[li];
@@ -2291,7 +2317,8 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLeve
             case BOP_UC_CHILD: {
                 const LevelArgs L = B.lv[li];
                 const int tot = L.n * ipow((NC ? NC : L.nc) / 2, DIM);
-                for (int g = t; g < tot; g += n) uc_child_body<DIM, NC>(T, L, own, par, AUTO, 1, 0, g);
+                for (int g = t; g < tot; g += n)
+                    uc_child_body<DIM, NC>(T, L, own, par, AUTO, 1, A.fly && A.n_down > 0 ? 1 : 0, g);
             } break;
             case BOP_UC_PARENT: {   // parent part on level li - 1
                 const LevelArgs Lp = B.lv[li - 1];
@@ -2320,7 +2347,7 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLeve
     for (int k = 0; k < n_steps; k++) {
         const bool in_wave = st.li <= W;
         if (!in_wave || tid < wave) exec(st, tid, in_wave ? wave : nt, in_wave);
-        const BottomStep nx = k + 1 < n_steps ? bottom_step(A, k + 1) : BottomStep{0, A.hl + 1, 0};
+        const BottomStep nx = k + 1 < n_steps ? bottom_step(A, k + 1) : BottomStep{0, A.hl + 1, 0, 0};
         if (in_wave && nx.li <= W)
             wsync();
         else

@@ -936,6 +936,10 @@ BottomArgs bottom_args(MgT<R> &m, int top)
     B.n_up = m.o->n_cycle_up;
     B.n_base = m.o->n_cycle_base;
     B.base64 = sizeof(R) == 4 && c->mixed_base64 ? 1 : 0;
+    // measured: computing the face ghosts in the passes is faster in 2D
+    // (2D benchmark tree 3.90 -> 3.76 ms per solve), a fill before every pass
+    // in 3D (a pass with computed ghosts costs as much as a fill and a pass)
+    B.fly = c->dim == 2 ? 1 : 0;
     B.trace = c->d_ptrace;
     B.trace_cap = c->ptrace_cap;
     return B;
@@ -1394,23 +1398,26 @@ static int tune_tiles(afg_ctx *c, const afg_mg_opts *o, int reps, double *ms, co
             if (nv < 2 || L.n == 0) continue;
             const int p = sizeof(R) == 4;
             const int dflt = Lv.tile[p] + (Lv.pairs[p] ? 3 : 0);
-            // time of two passes: two single passes (mode 0; on the
-            // colour-split copy of phi, a first and a last pass, where the
-            // relaxation blocks use it) or one fused pair (mode 1: the copy
-            // of phi and the second pool in turn, the other levels from the
-            // tree's phi), per shape
+            // time of two passes, timed as blocks of four: single passes
+            // (mode 0; on the colour-split copy of phi a first, two middle
+            // and a last pass, where the relaxation blocks use it) or two
+            // fused pairs (mode 1: the copy of phi to the second pool and
+            // back, the other levels from the tree's phi), per shape
             double per[6];
             for (int mode = 0; mode < 2; mode++)
                 for (int v = 0; v < nv; v++) {
                     L.tile = v;
-                    auto step = [&](int r) {
-                        if (mode)
-                            launch_gsrb_pair(c->dim, c->stream, m.T, L, m.V, (r & 1) ? m.phi2 : scratch,
-                                             (r & 1) ? scratch : m.phi2, 1, 1);
-                        else if (launch_gsrb_split(c->dim, c->stream, m.T, L, V, 1, 0, 1))
-                            launch_gsrb_split(c->dim, c->stream, m.T, L, V, 2, 2, 1);
-                        else
-                            for (int rb = 1; rb <= 2; rb++) launch_gsrb_pass(c->dim, c->stream, m.T, L, V, rb, 1);
+                    auto step = [&](int) {
+                        if (mode) {
+                            launch_gsrb_pair(c->dim, c->stream, m.T, L, m.V, scratch, m.phi2, 1, 1);
+                            launch_gsrb_pair(c->dim, c->stream, m.T, L, m.V, m.phi2, scratch, 3, 1);
+                        } else if (launch_gsrb_split(c->dim, c->stream, m.T, L, V, 1, 0, 1)) {
+                            launch_gsrb_split(c->dim, c->stream, m.T, L, V, 2, 1, 1);
+                            launch_gsrb_split(c->dim, c->stream, m.T, L, V, 3, 1, 1);
+                            launch_gsrb_split(c->dim, c->stream, m.T, L, V, 4, 2, 1);
+                        } else {
+                            for (int rb = 1; rb <= 4; rb++) launch_gsrb_pass(c->dim, c->stream, m.T, L, V, rb, 1);
+                        }
                     };
                     for (int r = 0; r < 2; r++) step(r);
                     float t_min = 0;
@@ -1423,7 +1430,7 @@ static int tune_tiles(afg_ctx *c, const afg_mg_opts *o, int reps, double *ms, co
                         HIP_TRY(c, hipEventElapsedTime(&t, e0, e1));
                         if (trial == 0 || t < t_min) t_min = t;
                     }
-                    per[3 * mode + v] = (double)t_min / reps;
+                    per[3 * mode + v] = (double)t_min / reps / 2;
                     if (ms) ms[(lvl - c->min_lvl) * 6 + 3 * mode + v] = per[3 * mode + v];
                 }
             // the fastest (mode, shape), unless the current one is within 3%

@@ -169,6 +169,9 @@ struct BottomArgs {
     // FP32 stages: the relaxation of the lowest level (the coarsest-grid
     // solve) in FP64 arithmetic on the FP32 values
     int base64 = 0;
+    // relaxation blocks whose passes compute their face ghosts (1) or read
+    // them from a fill before every pass (0)
+    int fly = 0;
     // when not null: s_memrealtime at the start and after every step (the
     // bottom kernel's thread 0), at most trace_cap values
     unsigned long long *trace = nullptr;

@@ -102,19 +102,31 @@ def measured_traffic(instances):
     return tot / n if n else None
 
 
-def pass_instances(dim, nc, shape, pairs, n_down, n_up):
+def pass_instances(dim, nc, shape, pairs, n_down, n_up, split=True):
     """The colour-pass kernel instances of a level per V-cycle and how often
     each runs: a down block whose first pass reads the stored ghosts, an up
-    block that computes them all (after the correction); fused pairs take
-    two passes per launch."""
-    kern = "k_gsrb_pair" if pairs else "k_gsrb_tile"
-    per = 2 if pairs else 1
-    first = 1
-    rest = (2 * n_down) // per - 1 + (2 * n_up) // per
+    block that computes them all (after the correction).  Fused pairs take
+    two passes per launch; on 3D levels of 8^3 or 16^3 boxes a block of two
+    or more single passes runs on the colour-split copy of phi (k_gsrb_split:
+    mode 0 the first pass, 1 the middle ones, 2 the last)."""
     out = {}
-    if first:
-        out["%s<%d, %d, 0, double, %d>" % (kern, dim, nc, shape)] = first
-    out["%s<%d, %d, 1, double, %d>" % (kern, dim, nc, shape)] = rest
+
+    def add(name, k):
+        if k:
+            out[name] = out.get(name, 0) + k
+
+    if pairs:
+        add("k_gsrb_pair<%d, %d, 0, double, %d>" % (dim, nc, shape), 1)
+        add("k_gsrb_pair<%d, %d, 1, double, %d>" % (dim, nc, shape), (2 * n_down) // 2 - 1 + (2 * n_up) // 2)
+        return out
+    for n, fly in ((2 * n_down, 0), (2 * n_up, 1)):
+        if split and dim == 3 and nc in (8, 16) and n >= 2:
+            add("k_gsrb_split<%d, %d, 0, %d, double, %d>" % (dim, nc, fly, shape), 1)
+            add("k_gsrb_split<%d, %d, 1, 1, double, %d>" % (dim, nc, shape), n - 2)
+            add("k_gsrb_split<%d, %d, 2, 1, double, %d>" % (dim, nc, shape), 1)
+        else:
+            add("k_gsrb_tile<%d, %d, %d, double, %d>" % (dim, nc, fly, shape), 1)
+            add("k_gsrb_tile<%d, %d, 1, double, %d>" % (dim, nc, shape), n - 1)
     return out
 
 
