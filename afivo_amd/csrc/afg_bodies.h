@@ -710,11 +710,13 @@ AFG_DEV void tile_stage_b(const DevTree &T, const LevelArgs &L, const LevelViewT
 // ghost_value on a level whose cells are read from the colour-split copy
 // `rbp` (rb_stride values per slot); the parent level from `par` as ever.
 // Expressions and their order are ghost_value's.
+// nbv = T.nb of the face, p = T.parent and co = T.choff of the slot, loaded
+// by the caller.
 template <int DIM, typename R>
-AFG_DEV R ghost_value_split(const DevTree &T, const R *rbp, int rb_stride, const LevelViewT<R> &par, int pnc,
-                            double dr, int slot, int f, int nc, int i, int j, int k, R center)
+AFG_DEV R ghost_value_split(const DevTree &T, int nbv, int p, int co, const R *rbp, int rb_stride,
+                            const LevelViewT<R> &par, int pnc, double dr, int slot, int f, int nc, int i, int j,
+                            int k, R center)
 {
-    const int nbv = T.nb[slot * (2 * DIM) + f];
     const int dn = f >> 1, high = f & 1;
     if (nbv >= 0) {
         const int m = high ? 1 : nc;
@@ -722,8 +724,6 @@ AFG_DEV R ghost_value_split(const DevTree &T, const R *rbp, int rb_stride, const
         return ldv(rbp + (size_t)nbv * rb_stride + colour_index<DIM>(a0, a1, a2, nc));
     }
     if (nbv == NB_REFINE) {
-        const int p = T.parent[slot];
-        const int co = T.choff[slot];
         const int gv = high ? nc + 1 : 0;
         const int g0 = dn == 0 ? gv : i, g1 = dn == 1 ? gv : j, g2 = dn == 2 ? gv : k;
         const int ps = pnc + 2;
@@ -770,13 +770,38 @@ AFG_DEV R ghost_value_split(const DevTree &T, const R *rbp, int rb_stride, const
 // colour of the tile's boxes from phi_rb into LDS (the only interior cells
 // the updates read), and the face ghosts next to cells of colour rb, computed
 // from phi_rb (the cell next to the ghost is read only where ghost_value
-// uses it: at refinement and physical boundaries).
+// uses it: at refinement and physical boundaries).  The loads go out in the
+// order of their dependences: the topology of the thread's ghosts, then rhs
+// and the interior cells, then the ghosts' values (which need the topology);
+// the LDS stores come last, so a thread waits for one chain of two loads.
 template <int DIM, int NC, int V, typename R>
 AFG_DEV void split_stage_a(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &v, const R *rbp, R *lds,
                            int box0, int rb, int tid, R *rhs)
 {
     using G = GsrbTile<DIM, NC, V>;
     constexpr int S = G::S;
+    constexpr int FH = G::FACE / 2;
+    constexpr int NG = (G::BOXES * G::GHOSTS + G::THREADS - 1) / G::THREADS;   // ghosts per thread
+    constexpr int NI = (G::BOXES * G::HALF + G::THREADS - 1) / G::THREADS;     // interior cells per thread
+    // the face of ghost e of the tile and its box, false past the level's end
+    auto ghost_at = [&](int e, int &b, int &f, int &t) {
+        b = e / G::GHOSTS;
+        const int r = e - b * G::GHOSTS;
+        f = r / FH;
+        t = r - f * FH;
+        return e < G::BOXES * G::GHOSTS && box0 + b < L.n;
+    };
+    int nbv[NG] = {}, par[NG] = {}, co[NG] = {};
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+        int b, f, t;
+        if (ghost_at(tid + g * G::THREADS, b, f, t)) {
+            const int slot = L.slot0 + box0 + b;
+            nbv[g] = T.nb[slot * (2 * DIM) + f];
+            par[g] = T.parent[slot];
+            co[g] = T.choff[slot];
+        }
+    }
 #pragma unroll
     for (int r = 0; r < G::PER; r++) {
         const int e = tid + r * G::THREADS;
@@ -786,22 +811,22 @@ AFG_DEV void split_stage_a(const DevTree &T, const LevelArgs &L, const LevelView
     }
     // interior cells of the other colour: split index c holds colour 1
     // (i + j + k odd) for c < HALF, colour 2 above
-    for (int e = tid; e < G::BOXES * G::HALF; e += G::THREADS) {
+    R xin[NI] = {};
+#pragma unroll
+    for (int r = 0; r < NI; r++) {
+        const int e = tid + r * G::THREADS;
         const int b = e / G::HALF;
-        if (box0 + b >= L.n) break;
-        const int c = e - b * G::HALF;
-        int i, j, k;
-        colour_cell<DIM>(c, NC, rb + 1, i, j, k);
-        lds[b * G::LEN + i + S * (j + S * k)] =
-            ldv(rbp + (size_t)(L.slot0 + box0 + b) * v.rb_stride + (rb & 1) * G::HALF + c);
+        if (e < G::BOXES * G::HALF && box0 + b < L.n)
+            xin[r] = ldv(rbp + (size_t)(L.slot0 + box0 + b) * v.rb_stride + (rb & 1) * G::HALF + (e - b * G::HALF));
     }
     // face ghosts next to cells of the colour (the enumeration of tile_stage_a)
-    constexpr int FH = G::FACE / 2;
-    for (int e = tid; e < G::BOXES * G::GHOSTS; e += G::THREADS) {
-        const int b = e / G::GHOSTS;
-        if (box0 + b >= L.n) break;
-        const int r = e - b * G::GHOSTS;
-        const int f = r / FH, t = r - f * FH;
+    R gv[NG];
+    int gpos[NG];
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+        int b, f, t;
+        gpos[g] = -1;
+        if (!ghost_at(tid + g * G::THREADS, b, f, t)) continue;
         const int dn = f >> 1, high = f & 1;
         const int m = high ? NC : 1;
         const int t2 = t / (NC / 2) + 1;
@@ -813,12 +838,24 @@ AFG_DEV void split_stage_a(const DevTree &T, const LevelArgs &L, const LevelView
         const int gj = dn == 1 ? (high ? NC + 1 : 0) : j;
         const int gk = dn == 2 ? (high ? NC + 1 : 0) : k;
         const int slot = L.slot0 + box0 + b;
-        const R center = T.nb[slot * (2 * DIM) + f] >= 0
-                             ? R(0)
-                             : ldv(rbp + (size_t)slot * v.rb_stride + colour_index<DIM>(i, j, k, NC));
-        lds[b * G::LEN + gi + S * (gj + S * gk)] =
-            ghost_value_split<DIM>(T, rbp, v.rb_stride, v, L.pnc, L.dr, slot, f, NC, i, j, k, center);
+        const R center = nbv[g] >= 0 ? R(0) : ldv(rbp + (size_t)slot * v.rb_stride + colour_index<DIM>(i, j, k, NC));
+        gv[g] = ghost_value_split<DIM>(T, nbv[g], par[g], co[g], rbp, v.rb_stride, v, L.pnc, L.dr, slot, f, NC, i, j,
+                                       k, center);
+        gpos[g] = b * G::LEN + gi + S * (gj + S * gk);
     }
+#pragma unroll
+    for (int r = 0; r < NI; r++) {
+        const int e = tid + r * G::THREADS;
+        const int b = e / G::HALF;
+        if (e < G::BOXES * G::HALF && box0 + b < L.n) {
+            int i, j, k;
+            colour_cell<DIM>(e - b * G::HALF, NC, rb + 1, i, j, k);
+            lds[b * G::LEN + i + S * (j + S * k)] = xin[r];
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < NG; g++)
+        if (gpos[g] >= 0) lds[gpos[g]] = gv[g];
 }
 
 // Stage B of the split passes: colour rb updated from LDS as tile_stage_b

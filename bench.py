@@ -74,6 +74,10 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy rhs to tmp over every box once, from cold caches (for the "
                          "HBM byte counters' scale factor), print its byte count and exit")
+    ap.add_argument("--copy-ceiling", action="store_true",
+                    help="only time plain copies at the sizes of the colour passes (for the kernel trace): "
+                         "rhs to tmp over every box and flat copies of the finest level's byte count, "
+                         "each from cold caches and then three times warm; print the byte counts and exit")
     return ap.parse_args()
 
 
@@ -100,6 +104,50 @@ def measured_traffic(instances):
         tot += table[name] * count
         n += count
     return tot / n if n else None
+
+
+def copy_ceiling(dt, st):
+    """Plain copies of the colour passes' sizes, each first from cold caches
+    (after writing a 2 GiB buffer) and then three times warm: the library's
+    box copy (rhs to tmp over every box, one launch per level) and the HIP
+    runtime's device-to-device copy of flat buffers holding the finest
+    level's box bytes and half of them.  The kernel trace of this run gives
+    each copy's duration."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    top = st.lvl == st.lvl.max()
+    fin = int(st.box_sizes()[top].sum()) * 8
+    flush_bytes = ctypes.c_size_t(1 << 31)
+    bufs = [ctypes.c_void_p() for _ in range(3)]
+    sizes = [flush_bytes, ctypes.c_size_t(fin), ctypes.c_size_t(fin)]
+    dt.sync()
+    for b, n in zip(bufs, sizes):
+        if hip.hipMalloc(ctypes.byref(b), n) != 0:
+            raise RuntimeError("hipMalloc failed")
+    flush, x, y = bufs
+
+    def check(rc, what):
+        if rc != 0:
+            raise RuntimeError(what + " failed")
+
+    check(hip.hipMemset(x, 1, ctypes.c_size_t(fin)), "hipMemset")
+    for n in (fin, fin // 2):
+        check(hip.hipMemset(flush, 1, flush_bytes), "cache flush")
+        check(hip.hipDeviceSynchronize(), "sync")
+        for _ in range(4):
+            check(hip.hipMemcpy(y, x, ctypes.c_size_t(n), 3), "hipMemcpy")   # 3: device to device
+        check(hip.hipDeviceSynchronize(), "sync")
+    check(hip.hipMemset(flush, 2, flush_bytes), "cache flush")
+    check(hip.hipDeviceSynchronize(), "sync")
+    for _ in range(4):
+        dt.copy_cc(2, 3)
+    dt.sync()
+    for b in bufs:
+        hip.hipFree(b)
+    print(json.dumps({"copy_ceiling": {"flat_copy_bytes": [fin, fin // 2],
+                                       "box_copy": "rhs to tmp over every box, one launch per level"}}))
+    dt.close()
+    return 0
 
 
 def pass_instances(dim, nc, shape, pairs, n_down, n_up, split=True):
@@ -767,6 +815,8 @@ def main():
                                                "bytes_written": nbytes}}))
         dt.close()
         return 0
+    if a.copy_ceiling:
+        return copy_ceiling(dt, st)
 
     # the autotuner picks the tile shape and pass mode of the colour passes
     # per level and the bottom kernel's limit (on a copy of phi, before
