@@ -61,6 +61,7 @@ SIGNATURES = {
     "afg_update_forest": (ctypes.c_int, [P, ctypes.c_int, I32P, F64, I32P, I32P]),
     "afg_mixed_first_fmg": (ctypes.c_int, [P, ctypes.c_int]),
     "afg_mixed_fp64_base": (ctypes.c_int, [P, ctypes.c_int]),
+    "afg_mixed_fp64_tau": (ctypes.c_int, [P, ctypes.c_int]),
     "afg_forest_trees": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), I32P]),
     "afg_forest_max_abs": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, F64P]),
     "afg_set_bc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, F64P]),

@@ -18,6 +18,8 @@
 // bodies (used to check the stage logic on the CPU) defines them first.
 #pragma once
 
+#include <type_traits>
+
 #include "afg_types.h"
 
 #ifndef AFG_DEV
@@ -171,8 +173,9 @@ AFG_DEV void stencil(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &
 
 // A phi at interior cell q from its neighbours u: mg$D_box_lpl
 // (m_aX_mg.f90:763-797) or, for tagged boxes, mg$D_box_lpld (:866-916).
-template <int DIM, typename R>
-AFG_DEV R box_op(const R u[6], R u0, const R *e, int q, int s, double inv_dr_sq, bool lpld)
+// E: the type eps is stored in (R, or float for FP64 arithmetic on FP32 values).
+template <int DIM, typename R, typename E = R>
+AFG_DEV R box_op(const R u[6], R u0, const E *e, int q, int s, double inv_dr_sq, bool lpld)
 {
     if (lpld) {
         const int dq[6] = {-1, 1, -s, s, -s * s, s * s};
@@ -1150,10 +1153,11 @@ AFG_DEV void residual_body(const DevTree &T, const LevelArgs &L, const LevelView
 // (set_coarse_phi_rhs, m_aX_mg.f90:495-504) the residual stays in the
 // child's tmp.  With fly = 1 the residual's face neighbours are computed as
 // the ghost fill after the preceding relaxation would store them.
-template <int DIM, int NC, typename R>
+template <int DIM, int NC, typename R, typename W = R>
 AFG_DEV void uc_child_body(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &own,
                            const LevelViewT<R> &par, int op_auto, int keep_child_tmp, int fly, int gid)
 {
+    constexpr bool same = std::is_same<R, W>::value;
     const int nc = NC ? NC : L.nc;
     const int hn = nc >> 1;
     const int per_box = ipow(hn, DIM);
@@ -1169,7 +1173,8 @@ AFG_DEV void uc_child_body(const DevTree &T, const LevelArgs &L, const LevelView
     const R *ph = own.phi + base;
     const R *e = lpld ? own.eps + base : nullptr;
     const R *rh = own.rhs + base;
-    R sr = 0, sp = 0, vr[8], vp[8], lc[8];
+    W sr = 0, sp = 0, vr[8], vp[8];
+    R lc[8];
     const bool cyl = DIM == 2 && cyl_op(T, op_auto);
     const bool lsf = lsf_box(T, own, slot), lsf_rstr = lsf_rstr_box(T, own, slot);
 #pragma unroll
@@ -1180,12 +1185,20 @@ AFG_DEV void uc_child_body(const DevTree &T, const LevelArgs &L, const LevelView
         const int q = fi + s * (fj + s * fk);
         R u[6];
         stencil<DIM>(T, L, own, par, slot, ph, nc, fi, fj, fk, q, fly, ldv(ph + q), u);
-        const R res = ldv(rh + q) - (lsf ? box_op_lsf<DIM>(u, ldv(ph + q), own.lsf + base, own.bval + base, q, s, L.inv_dr_sq)
-                               : cyl ? box_op_cyl(u, ldv(ph + q), e, q, s, L.inv_dr_sq, lpld, T.cyl_ioff[slot] + fi)
-                                     : box_op<DIM>(u, ldv(ph + q), e, q, s, L.inv_dr_sq, lpld));
-        if (!keep_child_tmp) own.tmp[base + q] = res;
+        W res;
+        if constexpr (same) {
+            res = ldv(rh + q) - (lsf ? box_op_lsf<DIM>(u, ldv(ph + q), own.lsf + base, own.bval + base, q, s, L.inv_dr_sq)
+                           : cyl ? box_op_cyl(u, ldv(ph + q), e, q, s, L.inv_dr_sq, lpld, T.cyl_ioff[slot] + fi)
+                                 : box_op<DIM>(u, ldv(ph + q), e, q, s, L.inv_dr_sq, lpld));
+        } else {   // the Cartesian operators only (FP32 stages)
+            W uw[6];
+#pragma unroll
+            for (int d = 0; d < 6; d++) uw[d] = u[d];
+            res = W(ldv(rh + q)) - box_op<DIM, W, R>(uw, W(ldv(ph + q)), e, q, s, L.inv_dr_sq, lpld);
+        }
+        if (!keep_child_tmp) own.tmp[base + q] = R(res);
         sr = m == 0 ? res : sr + res;
-        sp = m == 0 ? ldv(ph + q) : sp + ldv(ph + q);
+        sp = m == 0 ? W(ldv(ph + q)) : sp + W(ldv(ph + q));
         vr[m] = res;
         vp[m] = ldv(ph + q);
         if (lsf_rstr) lc[m] = own.lsf[base + q];
@@ -1197,19 +1210,21 @@ AFG_DEV void uc_child_body(const DevTree &T, const LevelArgs &L, const LevelView
                    ps * ((((co >> 10) & 1023) + jj) + ps * (DIM == 3 ? ((co >> 20) & 1023) + kk : 0));
     const size_t pb = par.off(p);
     constexpr double w = DIM == 2 ? 0.25 : 0.125;
-    if (lsf_rstr) {
-        const R lp = par.lsf[pb + pq];
-        par.tmp[pb + pq] = restrict_lsf<DIM>(vr, lc, lp);
-        par.phi[pb + pq] = restrict_lsf<DIM>(vp, lc, lp);
-        return;
+    if constexpr (same) {
+        if (lsf_rstr) {
+            const R lp = par.lsf[pb + pq];
+            par.tmp[pb + pq] = restrict_lsf<DIM>(vr, lc, lp);
+            par.phi[pb + pq] = restrict_lsf<DIM>(vp, lc, lp);
+            return;
+        }
+        if (DIM == 2 && T.cyl_rmin) {
+            par.tmp[pb + pq] = cyl_restrict(vr, T.cyl_rmin[p], 2 * L.dr, ii);
+            par.phi[pb + pq] = cyl_restrict(vp, T.cyl_rmin[p], 2 * L.dr, ii);
+            return;
+        }
     }
-    if (DIM == 2 && T.cyl_rmin) {
-        par.tmp[pb + pq] = cyl_restrict(vr, T.cyl_rmin[p], 2 * L.dr, ii);
-        par.phi[pb + pq] = cyl_restrict(vp, T.cyl_rmin[p], 2 * L.dr, ii);
-        return;
-    }
-    par.tmp[pb + pq] = R(w) * sr;
-    par.phi[pb + pq] = R(w) * sp;
+    par.tmp[pb + pq] = R(W(w) * sr);
+    par.phi[pb + pq] = R(W(w) * sp);
 }
 
 // update_coarse, child part, by tiles in LDS (NC 8 and 16): the same values
@@ -1372,7 +1387,7 @@ AFG_DEV void uc_tile_d(const LevelArgs &L, R *lds, int box0, int tid, const R *r
 // tmp = phi (a$D_box_copy_cc) if copy_tmp.  A phi uses the filled ghost
 // values, computed on the fly.  L is the parent level, own its view, par the
 // view of the level below it.
-template <int DIM, int NC, typename R>
+template <int DIM, int NC, typename R, typename W = R>
 AFG_DEV void uc_parent_body(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &own,
                             const LevelViewT<R> &par, int op_auto, int copy_tmp, int gid)
 {
@@ -1395,13 +1410,23 @@ AFG_DEV void uc_parent_body(const DevTree &T, const LevelArgs &L, const LevelVie
         const bool lpld = lpld_box(T, op_auto, slot);
         R u[6];
         stencil<DIM>(T, L, own, par, slot, box, nc, i, j, k, q, 1, ldv(box + q), u);
-        const R a = lsf_box(T, own, slot)
-                        ? box_op_lsf<DIM>(u, ldv(box + q), own.lsf + base, own.bval + base, q, s, L.inv_dr_sq)
-                    : DIM == 2 && cyl_op(T, op_auto)
-                        ? box_op_cyl(u, ldv(box + q), lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld,
-                                     T.cyl_ioff[slot] + i)
-                        : box_op<DIM>(u, ldv(box + q), lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld);
-        const R tau = a + ldv(own.tmp + base + q);
+        R tau;
+        if constexpr (std::is_same<R, W>::value) {
+            const R a = lsf_box(T, own, slot)
+                            ? box_op_lsf<DIM>(u, ldv(box + q), own.lsf + base, own.bval + base, q, s, L.inv_dr_sq)
+                        : DIM == 2 && cyl_op(T, op_auto)
+                            ? box_op_cyl(u, ldv(box + q), lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld,
+                                         T.cyl_ioff[slot] + i)
+                            : box_op<DIM>(u, ldv(box + q), lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld);
+            tau = a + ldv(own.tmp + base + q);
+        } else {   // the Cartesian operators only (FP32 stages)
+            W uw[6];
+#pragma unroll
+            for (int d = 0; d < 6; d++) uw[d] = u[d];
+            const W a = box_op<DIM, W, R>(uw, W(ldv(box + q)), lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq,
+                                          lpld);
+            tau = R(a + W(ldv(own.tmp + base + q)));
+        }
         own.rhs[base + q] = tau;
         if (DIM == 3 && own.rhs_rb) own.rhs_rb[(size_t)slot * own.rb_stride + colour_index<DIM>(i, j, k, nc)] = tau;
         if (copy_tmp) own.tmp[base + q] = ldv(box + q);
@@ -2354,13 +2379,26 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLeve
             case BOP_UC_CHILD: {
                 const LevelArgs L = B.lv[li];
                 const int tot = L.n * ipow((NC ? NC : L.nc) / 2, DIM);
-                for (int g = t; g < tot; g += n)
-                    uc_child_body<DIM, NC>(T, L, own, par, AUTO, 1, A.fly && A.n_down > 0 ? 1 : 0, g);
+                const int fly = A.fly && A.n_down > 0 ? 1 : 0;
+                if constexpr (sizeof(R) == 4) {
+                    if (A.tau64) {
+                        for (int g = t; g < tot; g += n)
+                            uc_child_body<DIM, NC, R, double>(T, L, own, par, AUTO, 1, fly, g);
+                        break;
+                    }
+                }
+                for (int g = t; g < tot; g += n) uc_child_body<DIM, NC>(T, L, own, par, AUTO, 1, fly, g);
             } break;
             case BOP_UC_PARENT: {   // parent part on level li - 1
                 const LevelArgs Lp = B.lv[li - 1];
                 const LevelViewT<R> ppar = view(li > 1 ? li - 2 : 0);
                 const int tot = Lp.n * ipow((NC ? NC : Lp.nc) + 2, DIM);
+                if constexpr (sizeof(R) == 4) {
+                    if (A.tau64) {
+                        for (int g = t; g < tot; g += n) uc_parent_body<DIM, NC, R, double>(T, Lp, par, ppar, AUTO, 1, g);
+                        break;
+                    }
+                }
                 for (int g = t; g < tot; g += n) uc_parent_body<DIM, NC>(T, Lp, par, ppar, AUTO, 1, g);
             } break;
             case BOP_CORRECT: {
@@ -2470,13 +2508,26 @@ AFG_DEV void persist_run(const DevTree &T, const MgVarsT<R> &V, const BottomLeve
                 gsync();
             }
             const int tc = L.n * ipow(nc / 2, DIM);
-            for (int g = t; g < tc; g += n) uc_child_body<DIM, NC>(T, L, v, v, AUTO, 1, A.n_down > 0 ? 1 : 0, g);
+            const int fly = A.n_down > 0 ? 1 : 0;
+            if constexpr (sizeof(R) == 4) {
+                if (A.tau64) {
+                    for (int g = t; g < tc; g += n) uc_child_body<DIM, NC, R, double>(T, L, v, v, AUTO, 1, fly, g);
+                    return;
+                }
+            }
+            for (int g = t; g < tc; g += n) uc_child_body<DIM, NC>(T, L, v, v, AUTO, 1, fly, g);
         });
         gsync();
         const LevelArgs Lp = B.lv[li - 1];
         bottom_nc_dispatch(Lp.nc, [&](auto ncv) {
             constexpr int NC = decltype(ncv)::value;
             const int tot = Lp.n * ipow((NC ? NC : Lp.nc) + 2, DIM);
+            if constexpr (sizeof(R) == 4) {
+                if (A.tau64) {
+                    for (int g = t; g < tot; g += n) uc_parent_body<DIM, NC, R, double>(T, Lp, v, v, AUTO, 1, g);
+                    return;
+                }
+            }
             for (int g = t; g < tot; g += n) uc_parent_body<DIM, NC>(T, Lp, v, v, AUTO, 1, g);
         });
         gsync();

@@ -120,6 +120,7 @@ struct afg_ctx {
     bool mixed_ready = false, mixed_fmg = false;
     bool mixed_first_fmg = true;         // the first correction after afg_mixed_begin is an FMG
     bool mixed_base64 = false;           // FP32 stages: the coarsest-grid relaxation in FP64 arithmetic
+    bool mixed_tau64 = false;            // FP32 stages: update_coarse (the FAS tau) in FP64 arithmetic
     bool tags_set = false;               // every box has its eps tag (they are set once)
 
     // tree
@@ -732,6 +733,7 @@ int make_mg(afg_ctx *c, const afg_mg_opts *o, MgT<R> *m)
     // the auto operators without eps leave every box untagged: the plain
     // operators, except on a cylindrical tree (mg2_box_clpl, examples/poisson_cyl.f90)
     m->V.op_auto = o->op == AFG_OP_AUTO && (o->i_eps != -1 || c->cyl || use_lsf);
+    m->V.tau64 = sizeof(R) == 4 && c->mixed_tau64 ? 1 : 0;
     if (m->V.op_auto && o->i_eps != -1) {
         R *e = nullptr;
         if (int r = ensure_var(c, o->i_eps, &e)) return r;
@@ -936,6 +938,7 @@ BottomArgs bottom_args(MgT<R> &m, int top)
     B.n_up = m.o->n_cycle_up;
     B.n_base = m.o->n_cycle_base;
     B.base64 = sizeof(R) == 4 && c->mixed_base64 ? 1 : 0;
+    B.tau64 = sizeof(R) == 4 && c->mixed_tau64 ? 1 : 0;
     // measured: computing the face ghosts in the passes is faster in 2D
     // (2D benchmark tree 3.90 -> 3.76 ms per solve), a fill before every pass
     // in 3D (a pass with computed ghosts costs as much as a fill and a pass)
@@ -2777,6 +2780,16 @@ int afg_mixed_fp64_base(afg_ctx *c, int enabled)
     if (c->stream) if (int e_ = stream_sync(c)) return e_;
     c->mixed_base64 = enabled != 0;
     for (auto &g : c->graphs) destroy_graph(g.second);   // the flag is a kernel argument
+    c->graphs.clear();
+    return 0;
+}
+
+int afg_mixed_fp64_tau(afg_ctx *c, int enabled)
+{
+    if (!c) return AFG_ERR_ARG;
+    if (c->stream) if (int e_ = stream_sync(c)) return e_;
+    c->mixed_tau64 = enabled != 0;
+    for (auto &g : c->graphs) destroy_graph(g.second);   // the flag selects kernels
     c->graphs.clear();
     return 0;
 }

@@ -147,6 +147,26 @@ __global__ __launch_bounds__(BLOCK) void k_uc_child(DevTree T, LevelArgs L, Leve
         uc_child_body<DIM, NC>(T, L, v, v, op_auto, keep_child_tmp, fly, gid);
 }
 
+// update_coarse on FP32 values in FP64 arithmetic (MgVarsT::tau64): child
+// part per coarse cell, parent part per cell
+template <int DIM, int NC, typename R>
+__global__ __launch_bounds__(BLOCK) void k_uc_child64(DevTree T, LevelArgs L, LevelViewT<R> v,
+                                                      int op_auto, int keep_child_tmp, int fly)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < L.n * ipow((NC ? NC : L.nc) / 2, DIM))
+        uc_child_body<DIM, NC, R, double>(T, L, v, v, op_auto, keep_child_tmp, fly, gid);
+}
+
+template <int DIM, int NC, typename R>
+__global__ __launch_bounds__(BLOCK) void k_uc_parent64(DevTree T, LevelArgs L, LevelViewT<R> v,
+                                                       int op_auto, int copy_tmp)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < L.n * ipow((NC ? NC : L.nc) + 2, DIM))
+        uc_parent_body<DIM, NC, R, double>(T, L, v, v, op_auto, copy_tmp, gid);
+}
+
 // update_coarse, child part, by tiles in LDS (NC 8 and 16)
 template <int DIM, int NC, int FLY, typename R>
 __global__ __launch_bounds__((UcTile<DIM, NC>::THREADS)) void k_uc_child_tile(DevTree T, LevelArgs L,
@@ -681,6 +701,14 @@ void launch_uc_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &
                      const MgVarsT<R> &V, int keep_child_tmp, int fly)
 {
     if (L.n <= 0) return;
+    if constexpr (sizeof(R) == 4) {
+        if (V.tau64) {
+            const long long th = (long long)L.n * ipow(L.nc / 2, dim);
+            AFG_DISPATCH(dim, L.nc, k_uc_child64, n_blocks(th), T, L, pool_view(T, V), V.op_auto, keep_child_tmp,
+                         fly);
+            return;
+        }
+    }
     if ((L.nc == 8 || L.nc == 16) && !per_cell_only(T, V)) {
         auto go = [&](auto dv, auto nv) {
             constexpr int D = decltype(dv)::value, N = decltype(nv)::value;
@@ -710,6 +738,12 @@ void launch_uc_parent(int dim, hipStream_t s, const DevTree &T, const LevelArgs 
 {
     if (L.n <= 0) return;
     const long long th = (long long)L.n * ipow(L.nc + 2, dim);
+    if constexpr (sizeof(R) == 4) {
+        if (V.tau64) {
+            AFG_DISPATCH(dim, L.nc, k_uc_parent64, n_blocks(th), T, L, pool_view(T, V), V.op_auto, copy_tmp);
+            return;
+        }
+    }
     AFG_DISPATCH(dim, L.nc, k_uc_parent, n_blocks(th), T, L, pool_view(T, V), V.op_auto, copy_tmp);
 }
 

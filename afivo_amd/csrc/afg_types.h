@@ -105,6 +105,10 @@ struct MgVarsT {
     // phi by colour in the same layout, the working copy of the colour passes
     // of a relaxation block (3D tiles; nullptr: not used)
     R *phi_rb = nullptr;
+    // FP32 stages: update_coarse's arithmetic (the fine residual, the
+    // restriction sums and A phi + tmp on the parents, which form the FAS
+    // tau) in FP64 on the FP32 values, each result rounded once when stored
+    int tau64 = 0;
 };
 using MgVars = MgVarsT<double>;
 
@@ -169,6 +173,8 @@ struct BottomArgs {
     // FP32 stages: the relaxation of the lowest level (the coarsest-grid
     // solve) in FP64 arithmetic on the FP32 values
     int base64 = 0;
+    // FP32 stages: update_coarse in FP64 arithmetic (MgVarsT::tau64)
+    int tau64 = 0;
     // relaxation blocks whose passes compute their face ghosts (1) or read
     // them from a fill before every pass (0)
     int fly = 0;

@@ -166,6 +166,11 @@ class DeviceTree:
         (afg_mixed_fp64_base)."""
         self._c(_lib.load().afg_mixed_fp64_base(self.h, 1 if enabled else 0))
 
+    def mixed_fp64_tau(self, enabled):
+        """The FP32 stages' update_coarse (the FAS tau) in FP64 arithmetic
+        (afg_mixed_fp64_tau)."""
+        self._c(_lib.load().afg_mixed_fp64_tau(self.h, 1 if enabled else 0))
+
     def mixed_first_fmg(self, enabled):
         """Whether the first mixed-precision correction after mixed_begin
         starts with an FMG (afg_mixed_first_fmg; default True)."""

@@ -74,6 +74,9 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy rhs to tmp over every box once, from cold caches (for the "
                          "HBM byte counters' scale factor), print its byte count and exit")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="c4: launch the V-cycles' kernels directly instead of replaying HIP graphs "
+                         "(for kernel traces; the kernels are the same)")
     ap.add_argument("--copy-ceiling", action="store_true",
                     help="only time plain copies at the sizes of the colour passes (for the kernel trace): "
                          "rhs to tmp over every box and flat copies of the finest level's byte count, "
@@ -431,6 +434,8 @@ def bench_c4(a):
     rhs_norm = float(np.abs(st.data[1][mask]).max())
 
     dt = M.DeviceTree.from_state(st)
+    if a.no_graphs:
+        dt.set_graphs(False)
     # the autotuner picks the tile shapes per level for both precisions
     # (on a copy of phi, before timing)
     dt.autotune(mg)
@@ -478,27 +483,30 @@ def bench_c4(a):
             times[name] += (float(np.abs(phi_mixed[mask] - phi64[mask]).max()) / scale,)
         else:
             times[name] += (None,)
-    # the north star's precision split for the coarsest grid: the FP32
-    # corrections' base relaxation in FP64 arithmetic, at the fastest
-    # cycles-per-correction count
-    base64 = None
+    # the north star's precision split: the FP32 corrections' coarsest-grid
+    # solve, their update_coarse (the FAS tau), and both, in FP64 arithmetic,
+    # at the fastest cycles-per-correction count
+    splits = {}
     mixed_names = [k for k in times if k.startswith("mixed")]
     if mixed_names:
         kbest = int(min(mixed_names, key=lambda k: times[k][0])[5:])
-        dt.mixed_fp64_base(True)
-        best_t, info = None, None
-        for rep in range(3):
-            reset()
-            s0 = time.perf_counter()
-            info = M.mixed_solve(dt, mg, a.tol, 40, cycles=kbest)
-            dt.sync()
-            if rep > 0:
-                t = time.perf_counter() - s0
-                best_t = t if best_t is None else min(best_t, t)
-        diff = (float(np.abs(dt.download(1)[mask] - phi64[mask]).max()) / scale) if phi64 is not None else None
+        for name, base, tau in (("coarsest_solve", True, False), ("tau", False, True), ("tau_and_coarsest_solve", True, True)):
+            dt.mixed_fp64_base(base)
+            dt.mixed_fp64_tau(tau)
+            best_t, info = None, None
+            for rep in range(3):
+                reset()
+                s0 = time.perf_counter()
+                info = M.mixed_solve(dt, mg, a.tol, 40, cycles=kbest)
+                dt.sync()
+                if rep > 0:
+                    t = time.perf_counter() - s0
+                    best_t = t if best_t is None else min(best_t, t)
+            diff = (float(np.abs(dt.download(1)[mask] - phi64[mask]).max()) / scale) if phi64 is not None else None
+            splits[name] = {"ms": 1e3 * best_t, "corrections": info[0], "rel_residual": info[1],
+                            "vcycles_per_correction": kbest, "phi_difference": diff}
         dt.mixed_fp64_base(False)
-        base64 = {"ms": 1e3 * best_t, "corrections": info[0], "rel_residual": info[1],
-                  "vcycles_per_correction": kbest, "phi_difference": diff}
+        dt.mixed_fp64_tau(False)
     dt.close()
     if a.c4_solver == "fp64":     # profiling run of the FP64 solver alone
         print(json.dumps({"c4_solver": "fp64", "ms": 1e3 * times["fp64"][0],
@@ -535,7 +543,9 @@ def bench_c4(a):
         "fp64": {"ms": 1e3 * times["fp64"][0], "vcycles_after_fmg": times["fp64"][1],
                  "solves_per_s": 1.0 / times["fp64"][0]} if "fp64" in times else None,
         "speedup_vs_fp64": times["fp64"][0] / times["mixed"][0] if "fp64" in times else None,
-        "mixed_fp64_coarsest_solve": base64,
+        "mixed_fp64_coarsest_solve": splits.get("coarsest_solve"),
+        "mixed_fp64_tau": splits.get("tau"),
+        "mixed_fp64_tau_and_coarsest_solve": splits.get("tau_and_coarsest_solve"),
         "max_phi_difference_vs_fp64_relative": err,
         "host_tree_build_s": build_s,
         "roofline": None, "cpu_baseline": None,

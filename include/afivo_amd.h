@@ -353,6 +353,12 @@ int afg_mixed_first_fmg(afg_ctx *ctx, int enabled);
  * base boxes of the coarsened tree, as afivo builds them with coarsen_to = 2);
  * elsewhere the coarsest solve stays FP32. */
 int afg_mixed_fp64_base(afg_ctx *ctx, int enabled);
+/* The FP32 stages' update_coarse (m_aX_mg.f90:424-480: the fine residual,
+ * its restriction and that of phi, and rhs = A phi + tmp on the parents,
+ * which together form the FAS tau) in FP64 arithmetic on the FP32 values,
+ * each stored result rounded once to FP32 (default 0: FP32 arithmetic).
+ * Cartesian operators only, as the FP32 stages. */
+int afg_mixed_fp64_tau(afg_ctx *ctx, int enabled);
 /* The whole solve: afg_mixed_begin, then corrections until every tree has
  * max|rhs - A phi| <= tol * max|rhs| over its leaf interiors, or max_iter
  * corrections.  Returns the number of corrections and the largest relative

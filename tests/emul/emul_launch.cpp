@@ -315,6 +315,18 @@ void launch_uc_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &
                      const MgVarsT<R> &V, int keep_child_tmp, int fly)
 {
     if (L.n <= 0) return;
+    if constexpr (sizeof(R) == 4) {
+        if (V.tau64) {
+            emul_submit(s, [=] {
+                const LevelViewT<R> v = pool_view(T, V);
+                run_stage(dim, L.nc, [&](auto d) { return (long long)L.n * ipow(L.nc / 2, DIMV); },
+                          [&](auto d, auto c, int g) {
+                              uc_child_body<DIMV, NCV, R, double>(T, L, v, v, V.op_auto, keep_child_tmp, fly, g);
+                          });
+            });
+            return;
+        }
+    }
     if ((L.nc == 8 || L.nc == 16) && !per_cell_only(T, V)) {
         emul_submit(s, [=] {
             if (dim == 2)
@@ -341,7 +353,15 @@ void launch_uc_parent(int dim, hipStream_t s, const DevTree &T, const LevelArgs 
     emul_submit(s, [=] {
         const LevelViewT<R> v = pool_view(T, V);
         run_stage(dim, L.nc, [&](auto d) { return (long long)L.n * ipow(L.nc + 2, DIMV); },
-                  [&](auto d, auto c, int g) { uc_parent_body<DIMV, NCV>(T, L, v, v, V.op_auto, copy_tmp, g); });
+                  [&](auto d, auto c, int g) {
+                      if constexpr (sizeof(R) == 4) {
+                          if (V.tau64) {
+                              uc_parent_body<DIMV, NCV, R, double>(T, L, v, v, V.op_auto, copy_tmp, g);
+                              return;
+                          }
+                      }
+                      uc_parent_body<DIMV, NCV>(T, L, v, v, V.op_auto, copy_tmp, g);
+                  });
     });
 }
 

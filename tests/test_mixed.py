@@ -117,6 +117,38 @@ def test_mixed_solve_with_fp64_coarsest_solve(kind):
     assert np.abs(phi_base64[mask] - phi_default[mask]).max() <= 1e-9 * scale
 
 
+@pytest.mark.parametrize("kind", ["2d", "3d_eps"])
+def test_mixed_solve_with_fp64_tau(kind):
+    """The FP32 corrections' update_coarse (the FAS tau) in FP64 arithmetic
+    (afg_mixed_fp64_tau): the defect correction still reaches 1e-10, its phi
+    agrees with the default (FP32 tau) within 1e-9 of max|phi| but not bit for
+    bit (the option changes the arithmetic), and switching the option back
+    gives the default's phi bit for bit."""
+    p = mixed_problem(kind)
+    st = p.state
+    mg = M.default_mg(eps=p.eps_hi is not None, n_base=p.n_base)
+    tol = 1e-10
+    dt = M.DeviceTree.from_state(st)
+    M.mixed_solve(dt, mg, tol, max_iter=30)
+    phi_default = dt.download(1)
+    for iv, a in enumerate(st.data, start=1):
+        dt.upload(iv, a)
+    dt.mixed_fp64_tau(True)
+    it, rr = M.mixed_solve(dt, mg, tol, max_iter=30)
+    assert rr <= tol and it >= 2
+    phi_tau64 = dt.download(1)
+    for iv, a in enumerate(st.data, start=1):
+        dt.upload(iv, a)
+    dt.mixed_fp64_tau(False)
+    M.mixed_solve(dt, mg, tol, max_iter=30)
+    assert np.array_equal(dt.download(1).view(np.int64), phi_default.view(np.int64))
+    dt.close()
+    mask = st.interior_mask()
+    scale = np.abs(phi_default[mask]).max()
+    assert np.abs(phi_tau64[mask] - phi_default[mask]).max() <= 1e-9 * scale
+    assert not np.array_equal(phi_tau64.view(np.int64), phi_default.view(np.int64))
+
+
 def test_c4_configuration_against_the_oracle():
     """The 3D variable-eps configuration itself (nc 16, 2804 boxes, 1.00e7
     leaf cells, levels -2..7): the FP64 GPU solve (FMG + 11 V-cycles, the
