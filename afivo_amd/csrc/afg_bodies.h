@@ -411,7 +411,8 @@ AFG_DEV void fill_body(const DevTree &T, const LevelArgs &L, const LevelViewT<R>
     const int slot = L.slot0 + b;
     const int s = nc + 2;
     R *box = own.phi + own.off(slot);
-    const R center = ldv(box + i + s * (j + s * k));
+    // the cell next to the ghost enters only at refinement and physical boundaries
+    const R center = T.nb[slot * (2 * DIM) + f] >= 0 ? R(0) : ldv(box + i + s * (j + s * k));
     const R g = ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, f, nc, i, j, k, center);
     const int gi = dn == 0 ? (high ? nc + 1 : 0) : i;
     const int gj = dn == 1 ? (high ? nc + 1 : 0) : j;
@@ -656,7 +657,8 @@ AFG_DEV void tile_stage_a(const DevTree &T, const LevelArgs &L, const LevelViewT
         const size_t base = v.off(slot);
         R g;
         if (FLY)
-            g = ghost_value<DIM>(T, v, vp, L.pnc, L.dr, slot, f, NC, i, j, k, v.phi[base + i + S * (j + S * k)]);
+            g = ghost_value<DIM>(T, v, vp, L.pnc, L.dr, slot, f, NC, i, j, k,
+                                 T.nb[slot * (2 * DIM) + f] >= 0 ? R(0) : v.phi[base + i + S * (j + S * k)]);
         else
             g = v.phi[base + gq];
         lds[b * G::LEN + gq] = g;
@@ -1286,7 +1288,8 @@ AFG_DEV void uc_tile_a(const DevTree &T, const LevelArgs &L, const LevelViewT<R>
         const size_t base = v.off(slot);
         R g;
         if (FLY)
-            g = ghost_value<DIM>(T, v, v, L.pnc, L.dr, slot, f, NC, i, j, k, v.phi[base + i + S * (j + S * k)]);
+            g = ghost_value<DIM>(T, v, v, L.pnc, L.dr, slot, f, NC, i, j, k,
+                                 T.nb[slot * (2 * DIM) + f] >= 0 ? R(0) : v.phi[base + i + S * (j + S * k)]);
         else
             g = v.phi[base + gq];
         lds[b * G::LEN + gq] = g;
@@ -1438,7 +1441,7 @@ AFG_DEV void uc_parent_body(const DevTree &T, const LevelArgs &L, const LevelVie
         const int ci = oi ? (i > nc ? nc : 1) : i;
         const int cj = oj ? (j > nc ? nc : 1) : j;
         const int ck = DIM == 3 ? (ok ? (k > nc ? nc : 1) : k) : 0;
-        const R center = ldv(box + ci + s * (cj + s * ck));
+        const R center = T.nb[slot * (2 * DIM) + f] >= 0 ? R(0) : ldv(box + ci + s * (cj + s * ck));
         v = ghost_value<DIM>(T, own, par, L.pnc, L.dr, slot, f, nc, ci, cj, ck, center);
         box[q] = v;
     }
