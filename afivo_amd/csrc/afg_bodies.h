@@ -2006,13 +2006,13 @@ AFG_DEV void nc_dispatch(int nc, F &&f)
 // at level 0: a block of n_base sweeps; per level li from 1 up to hl:
 // correct_children (correction, then tmp = phi - tmp on the parents), the
 // ghost fill and a block of n_up sweeps.  Two forms of a block of n sweeps:
-// - stored ghosts (A.fly = 0): the first colour pass, then (ghost fill,
+// - stored ghosts (FLY = 0, 3D): the first colour pass, then (ghost fill,
 //   colour pass) for passes 2..2n, then the ghost fill: 4n steps.  Filling
 //   the ghosts before a pass gives the face neighbours that the reference's
 //   fill after the previous pass stores, so every pass reads them from
 //   memory; the ghost layer after the block is the reference's, the last
 //   fill covering every face.
-// - computed ghosts (A.fly = 1): the 2n colour passes, the first of a
+// - computed ghosts (FLY = 1, 2D): the 2n colour passes, the first of a
 //   down-sweep block reading the stored ghosts and every other pass computing
 //   the face ghosts it reads (the values the reference's fill after the
 //   previous pass stores), then the ghost fill: 2n + 1 steps.  The down-sweep
@@ -2026,17 +2026,23 @@ struct BottomStep {
     int op, li, rb, fly;
 };
 
+// The form is fixed per dimension at compile time (measured: computing the
+// face ghosts in the passes is faster in 2D; in 3D a pass with computed
+// ghosts costs as much as a fill and a pass, and the fill form keeps the
+// ghost code out of the passes' instruction stream).
 // number of steps of a relaxation block of n sweeps
-AFG_HD inline int relax_steps(const BottomArgs &A, int n)
+template <int FLY>
+AFG_HD inline int relax_steps(int n)
 {
-    return A.fly ? 2 * n + 1 : 4 * n;
+    return FLY ? 2 * n + 1 : 4 * n;
 }
 
 // step r of a relaxation block of n sweeps; `first_fly` tells whether its
 // first pass computes its face ghosts
-AFG_HD inline BottomStep relax_step(const BottomArgs &A, int li, int r, int n, int first_fly)
+template <int FLY>
+AFG_HD inline BottomStep relax_step(int li, int r, int n, int first_fly)
 {
-    if (A.fly) {
+    if (FLY) {
         if (r == 2 * n) return {BOP_FILL, li, 0, 0};
         return {BOP_GSRB, li, r + 1, r > 0 ? 1 : first_fly};
     }
@@ -2046,15 +2052,16 @@ AFG_HD inline BottomStep relax_step(const BottomArgs &A, int li, int r, int n, i
 }
 
 // Step k of the sequence above; every block of steps has a fixed length.
+template <int FLY>
 AFG_HD inline BottomStep bottom_step(const BottomArgs &A, int k)
 {
-    const int rd = A.fly ? 2 * A.n_down : 4 * A.n_down;   // the down block has no fill when A.fly
+    const int rd = FLY ? 2 * A.n_down : 4 * A.n_down;   // the down block has no fill when FLY
     const int nd = rd + 2;
     const int nb = A.n_base > 0 ? 2 : 0;
-    const int nu = (A.fly ? 2 : 3) + relax_steps(A, A.n_up);
+    const int nu = (FLY ? 2 : 3) + relax_steps<FLY>(A.n_up);
     if (k < A.hl * nd) {
         const int li = A.hl - k / nd, r = k % nd;
-        if (r < rd) return relax_step(A, li, r, A.n_down, 0);
+        if (r < rd) return relax_step<FLY>(li, r, A.n_down, 0);
         return {r == rd ? BOP_UC_CHILD : BOP_UC_PARENT, li, 0, 0};
     }
     k -= A.hl * nd;
@@ -2063,15 +2070,16 @@ AFG_HD inline BottomStep bottom_step(const BottomArgs &A, int k)
     const int li = 1 + k / nu, r = k % nu;
     if (r == 0) return {BOP_CORRECT, li, 0, 0};
     if (r == 1) return {BOP_DIFF, li, 0, 0};
-    if (A.fly) return relax_step(A, li, r - 2, A.n_up, 1);
+    if (FLY) return relax_step<FLY>(li, r - 2, A.n_up, 1);
     if (r == 2) return {BOP_FILL, li, 0, 0};
-    return relax_step(A, li, r - 3, A.n_up, 0);
+    return relax_step<FLY>(li, r - 3, A.n_up, 0);
 }
 
+template <int FLY>
 AFG_HD inline int bottom_steps(const BottomArgs &A)
 {
-    const int rd = A.fly ? 2 * A.n_down : 4 * A.n_down;
-    return A.hl * (rd + 2) + (A.n_base > 0 ? 2 : 0) + A.hl * ((A.fly ? 2 : 3) + relax_steps(A, A.n_up));
+    const int rd = FLY ? 2 * A.n_down : 4 * A.n_down;
+    return A.hl * (rd + 2) + (A.n_base > 0 ? 2 : 0) + A.hl * ((FLY ? 2 : 3) + relax_steps<FLY>(A.n_up));
 }
 
 // cell counts with their own code in the bottom kernel (others: generic)
@@ -2230,12 +2238,15 @@ AFG_DEV void base_list_run(const DevTree &T, const LevelViewT<R> &own, const Bot
 // [0, wave) alone, one wavefront, and are ordered by `wsync()` among
 // themselves; the other threads pass them by and wait at the next workgroup
 // barrier.
-template <int DIM, int AUTO, typename R, typename Sync, typename WaveSync, typename Xshfl>
+// TAU: the FP32 stages' update_coarse in FP64 arithmetic (MgVarsT::tau64), a
+// compile-time choice so that the other instances hold none of its code.
+template <int DIM, int AUTO, typename R, int TAU = 0, typename Sync, typename WaveSync, typename Xshfl>
 AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLevels &B,
                         const BottomArgs &A, void *lds, int tid, int nt, int wave, Sync sync,
                         WaveSync wsync, Xshfl xshfl)
 {
     constexpr int NNB = 2 * DIM;
+    constexpr int BFLY = DIM == 2 ? 1 : 0;   // the relaxation blocks' form (bottom_step)
     // topology tables of the bottom slots, in LDS
     char *tb = (char *)lds;
     int32_t *t_nb = (int32_t *)tb;
@@ -2372,7 +2383,7 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLeve
             case BOP_GSRB: {
                 const LevelArgs L = B.lv[li];
                 const int tot = gsrb_threads<DIM>(L.n, NC ? NC : L.nc);
-                for (int g = t; g < tot; g += n) gsrb_body<DIM, NC>(T, L, own, par, AUTO, st.rb, st.fly, g);
+                for (int g = t; g < tot; g += n) gsrb_body<DIM, NC>(T, L, own, par, AUTO, st.rb, BFLY ? st.fly : 0, g);
             } break;
             case BOP_FILL: {
                 const LevelArgs L = B.lv[li];
@@ -2382,13 +2393,10 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLeve
             case BOP_UC_CHILD: {
                 const LevelArgs L = B.lv[li];
                 const int tot = L.n * ipow((NC ? NC : L.nc) / 2, DIM);
-                const int fly = A.fly && A.n_down > 0 ? 1 : 0;
-                if constexpr (sizeof(R) == 4) {
-                    if (A.tau64) {
-                        for (int g = t; g < tot; g += n)
-                            uc_child_body<DIM, NC, R, double>(T, L, own, par, AUTO, 1, fly, g);
-                        break;
-                    }
+                const int fly = BFLY && A.n_down > 0 ? 1 : 0;
+                if constexpr (sizeof(R) == 4 && TAU) {
+                    for (int g = t; g < tot; g += n) uc_child_body<DIM, NC, R, double>(T, L, own, par, AUTO, 1, fly, g);
+                    break;
                 }
                 for (int g = t; g < tot; g += n) uc_child_body<DIM, NC>(T, L, own, par, AUTO, 1, fly, g);
             } break;
@@ -2396,11 +2404,9 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLeve
                 const LevelArgs Lp = B.lv[li - 1];
                 const LevelViewT<R> ppar = view(li > 1 ? li - 2 : 0);
                 const int tot = Lp.n * ipow((NC ? NC : Lp.nc) + 2, DIM);
-                if constexpr (sizeof(R) == 4) {
-                    if (A.tau64) {
-                        for (int g = t; g < tot; g += n) uc_parent_body<DIM, NC, R, double>(T, Lp, par, ppar, AUTO, 1, g);
-                        break;
-                    }
+                if constexpr (sizeof(R) == 4 && TAU) {
+                    for (int g = t; g < tot; g += n) uc_parent_body<DIM, NC, R, double>(T, Lp, par, ppar, AUTO, 1, g);
+                    break;
                 }
                 for (int g = t; g < tot; g += n) uc_parent_body<DIM, NC>(T, Lp, par, ppar, AUTO, 1, g);
             } break;
@@ -2420,12 +2426,12 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLeve
     };
 
     const int W = A.hl < B.wave_top ? A.hl : B.wave_top;
-    const int n_steps = bottom_steps(A);
-    BottomStep st = bottom_step(A, 0);
+    const int n_steps = bottom_steps<BFLY>(A);
+    BottomStep st = bottom_step<BFLY>(A, 0);
     for (int k = 0; k < n_steps; k++) {
         const bool in_wave = st.li <= W;
         if (!in_wave || tid < wave) exec(st, tid, in_wave ? wave : nt, in_wave);
-        const BottomStep nx = k + 1 < n_steps ? bottom_step(A, k + 1) : BottomStep{0, A.hl + 1, 0, 0};
+        const BottomStep nx = k + 1 < n_steps ? bottom_step<BFLY>(A, k + 1) : BottomStep{0, A.hl + 1, 0, 0};
         if (in_wave && nx.li <= W)
             wsync();
         else

@@ -939,10 +939,6 @@ BottomArgs bottom_args(MgT<R> &m, int top)
     B.n_base = m.o->n_cycle_base;
     B.base64 = sizeof(R) == 4 && c->mixed_base64 ? 1 : 0;
     B.tau64 = sizeof(R) == 4 && c->mixed_tau64 ? 1 : 0;
-    // measured: computing the face ghosts in the passes is faster in 2D
-    // (2D benchmark tree 3.90 -> 3.76 ms per solve), a fill before every pass
-    // in 3D (a pass with computed ghosts costs as much as a fill and a pass)
-    B.fly = c->dim == 2 ? 1 : 0;
     B.trace = c->d_ptrace;
     B.trace_cap = c->ptrace_cap;
     return B;

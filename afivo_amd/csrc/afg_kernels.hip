@@ -470,7 +470,7 @@ __global__ __launch_bounds__(BLOCK) void k_tree_reduce(const int32_t *tree_chunk
 // bottom of the V-cycle in one workgroup, on LDS copies of the small levels
 // (bottom_run), and the residual of those levels (bottom_residual_run)
 
-template <int DIM, int AUTO, typename R>
+template <int DIM, int AUTO, typename R, int TAU = 0>
 __global__ __launch_bounds__(BOTTOM_BLOCK) void k_bottom(DevTree T, MgVarsT<R> V,
                                                           const BottomLevels *__restrict__ B,
                                                           BottomArgs A)
@@ -488,7 +488,7 @@ __global__ __launch_bounds__(BOTTOM_BLOCK) void k_bottom(DevTree T, MgVarsT<R> V
         if (A.trace && threadIdx.x == 0 && n_trace < A.trace_cap) A.trace[n_trace++] = __builtin_amdgcn_s_memrealtime();
     };
     stamp();
-    bottom_run<DIM, AUTO, R>(T, V, *B, A, lds, threadIdx.x, blockDim.x, warpSize,
+    bottom_run<DIM, AUTO, R, TAU>(T, V, *B, A, lds, threadIdx.x, blockDim.x, warpSize,
                              [&] {
                                  __syncthreads();
                                  stamp();
@@ -968,10 +968,12 @@ int bottom_prepare(int dim, size_t lds_bytes)
     std::lock_guard<std::mutex> lock(mu);
     size_t &g = granted[dim == 2 ? 0 : 1];
     if (lds_bytes <= g) return 0;
-    const void *fn[4] = {dim == 2 ? (const void *)k_bottom<2, 0, double> : (const void *)k_bottom<3, 0, double>,
+    const void *fn[6] = {dim == 2 ? (const void *)k_bottom<2, 0, double> : (const void *)k_bottom<3, 0, double>,
                          dim == 2 ? (const void *)k_bottom<2, 1, double> : (const void *)k_bottom<3, 1, double>,
                          dim == 2 ? (const void *)k_bottom<2, 0, float> : (const void *)k_bottom<3, 0, float>,
-                         dim == 2 ? (const void *)k_bottom<2, 1, float> : (const void *)k_bottom<3, 1, float>};
+                         dim == 2 ? (const void *)k_bottom<2, 1, float> : (const void *)k_bottom<3, 1, float>,
+                         dim == 2 ? (const void *)k_bottom<2, 0, float, 1> : (const void *)k_bottom<3, 0, float, 1>,
+                         dim == 2 ? (const void *)k_bottom<2, 1, float, 1> : (const void *)k_bottom<3, 1, float, 1>};
     for (const void *f : fn)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) != hipSuccess)
             return -1;
@@ -983,6 +985,22 @@ template <typename R>
 void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVarsT<R> &V,
                    const BottomLevels *B, const BottomArgs &A, size_t lds_bytes, int threads)
 {
+    if constexpr (sizeof(R) == 4) {
+        if (A.tau64) {
+            if (dim == 2) {
+                if (V.op_auto)
+                    k_bottom<2, 1, R, 1><<<1, threads, lds_bytes, s>>>(T, V, B, A);
+                else
+                    k_bottom<2, 0, R, 1><<<1, threads, lds_bytes, s>>>(T, V, B, A);
+            } else {
+                if (V.op_auto)
+                    k_bottom<3, 1, R, 1><<<1, threads, lds_bytes, s>>>(T, V, B, A);
+                else
+                    k_bottom<3, 0, R, 1><<<1, threads, lds_bytes, s>>>(T, V, B, A);
+            }
+            return;
+        }
+    }
     if (dim == 2) {
         if (V.op_auto)
             k_bottom<2, 1, R><<<1, threads, lds_bytes, s>>>(T, V, B, A);

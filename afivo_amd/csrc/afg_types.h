@@ -175,9 +175,6 @@ struct BottomArgs {
     int base64 = 0;
     // FP32 stages: update_coarse in FP64 arithmetic (MgVarsT::tau64)
     int tau64 = 0;
-    // relaxation blocks whose passes compute their face ghosts (1) or read
-    // them from a fill before every pass (0)
-    int fly = 0;
     // when not null: s_memrealtime at the start and after every step (the
     // bottom kernel's thread 0), at most trace_cap values
     unsigned long long *trace = nullptr;

@@ -662,17 +662,24 @@ void launch_bottom(int dim, hipStream_t s, const DevTree &T, const MgVarsT<R> &V
 {
     emul_submit(s, [=] {
         double *lds = (double *)emul_alloc(lds_bytes ? lds_bytes : 8);
-        if (dim == 2) {
-            if (V.op_auto)
-                bottom_run<2, 1, R>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {}, [](auto v, int) { return v; });
-            else
-                bottom_run<2, 0, R>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {}, [](auto v, int) { return v; });
-        } else {
-            if (V.op_auto)
-                bottom_run<3, 1, R>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {}, [](auto v, int) { return v; });
-            else
-                bottom_run<3, 0, R>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {}, [](auto v, int) { return v; });
-        }
+        auto run = [&](auto tau) {
+            constexpr int TAU = decltype(tau)::value;
+            if (dim == 2) {
+                if (V.op_auto)
+                    bottom_run<2, 1, R, TAU>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {}, [](auto v, int) { return v; });
+                else
+                    bottom_run<2, 0, R, TAU>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {}, [](auto v, int) { return v; });
+            } else {
+                if (V.op_auto)
+                    bottom_run<3, 1, R, TAU>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {}, [](auto v, int) { return v; });
+                else
+                    bottom_run<3, 0, R, TAU>(T, V, *B, A, lds, 0, 1, 1, [] {}, [] {}, [](auto v, int) { return v; });
+            }
+        };
+        if (sizeof(R) == 4 && A.tau64)
+            run(std::integral_constant<int, 1>());
+        else
+            run(std::integral_constant<int, 0>());
         emul_free(lds);
     });
 }
