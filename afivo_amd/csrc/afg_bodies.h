@@ -1119,8 +1119,8 @@ AFG_DEV void pair_phase3(const DevTree &T, const LevelArgs &L, const LevelViewT<
 
 // residual_box (m_aX_mg.f90:545-559) over a level: tmp = rhs - A phi
 template <int DIM, int NC, typename R>
-AFG_DEV void residual_body(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &own,
-                           const LevelViewT<R> &par, int op_auto, int gid)
+AFG_DEV R residual_body(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &own,
+                        const LevelViewT<R> &par, int op_auto, int gid)
 {
     const int nc = NC ? NC : L.nc;
     const int per_box = ipow(nc, DIM);
@@ -1144,7 +1144,26 @@ AFG_DEV void residual_body(const DevTree &T, const LevelArgs &L, const LevelView
                     : box_op<DIM>(u, ldv(box + q), lpld ? own.eps + base : nullptr, q, s, L.inv_dr_sq, lpld);
     const R rh = DIM == 3 && own.rhs_rb ? own.rhs_rb[(size_t)slot * own.rb_stride + colour_index<DIM>(i, j, k, nc)]
                             : ldv(own.rhs + base + q);
-    own.tmp[base + q] = rh - a;
+    const R r = rh - a;
+    own.tmp[base + q] = r;
+    return r;
+}
+
+// residual_body, and (float) of the residual into r32 (an FP32 pool of the
+// same slot layout) and r32_rb (its colour-split copy, 3D; null: none)
+template <int DIM, int NC, typename R>
+AFG_DEV void residual32_body(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &v, int op_auto, float *r32,
+                             float *r32_rb, int rb_stride, int gid)
+{
+    const R r = residual_body<DIM, NC>(T, L, v, v, op_auto, gid);
+    const int nc = NC ? NC : L.nc;
+    const int per_box = ipow(nc, DIM);
+    const int b = gid / per_box, c = gid - b * per_box;
+    const int i = c % nc + 1, j = (c / nc) % nc + 1, k = DIM == 3 ? c / (nc * nc) + 1 : 0;
+    const int s = nc + 2, slot = L.slot0 + b;
+    const float f = (float)r;
+    r32[v.off(slot) + i + s * (j + s * k)] = f;
+    if (DIM == 3 && r32_rb) r32_rb[(size_t)slot * rb_stride + colour_index<DIM>(i, j, k, nc)] = f;
 }
 
 // update_coarse, child part (m_aX_mg.f90:444-465), fused: per coarse cell of a

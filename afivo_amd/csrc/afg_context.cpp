@@ -121,6 +121,8 @@ struct afg_ctx {
     bool mixed_first_fmg = true;         // the first correction after afg_mixed_begin is an FMG
     bool mixed_base64 = false;           // FP32 stages: the coarsest-grid relaxation in FP64 arithmetic
     bool mixed_tau64 = false;            // FP32 stages: update_coarse (the FAS tau) in FP64 arithmetic
+    bool mixed_r32 = false;              // the FP32 rhs holds (float) of the FP64 residual in tmp
+    const void *mixed_r32_of[2] = {nullptr, nullptr};   // that tmp pool and that FP32 rhs pool
     bool tags_set = false;               // every box has its eps tag (they are set once)
 
     // tree
@@ -436,6 +438,7 @@ void free_tree(afg_ctx *c)
     c->d_phi_rb = nullptr;
     c->d_phi_rb32 = nullptr;
     c->rb_valid[0] = c->rb_valid[1] = false;
+    c->mixed_r32 = false;
     c->d_nb = c->d_parent = c->d_choff = c->d_lists = c->d_bc_type = c->d_bc_voff = nullptr;
     c->d_bc_vals = nullptr;
     c->d_tag = nullptr;
@@ -689,7 +692,11 @@ int rb_stride(const afg_ctx *c) { return c->dim == 2 ? c->nc * c->nc : c->nc * c
 
 // Cell data changed outside the multigrid stages: the colour-split rhs must
 // be formed again before the next colour pass reads it.
-void data_written(afg_ctx *c) { c->rb_valid[0] = c->rb_valid[1] = false; }
+void data_written(afg_ctx *c)
+{
+    c->rb_valid[0] = c->rb_valid[1] = false;
+    c->mixed_r32 = false;
+}
 
 // Form the colour-split rhs of every level from rhs (consecutive levels of
 // equal cell count in one launch) and mark it valid for variable iv.
@@ -2709,14 +2716,32 @@ static int forest_norms(afg_ctx *c, int iv, double *out)
 // the restriction of its children (where the FAS iteration converges to, its
 // corrections vanishing), then the ghost fill from the lowest level up
 // (refinement-boundary ghosts read the parent's) and tmp = rhs - A phi.
-static void mixed_residual(Mg &m)
+// The FP64 residual on every level (parents' phi restricted from their
+// children first); with f, its FP32 image goes straight into f's rhs and
+// colour-split rhs, which the next correction then takes without converting
+// tmp (mixed_r32: cleared by any write of cell data in between).
+static void mixed_residual(Mg &m, MgT<float> *f = nullptr)
 {
     afg_ctx *c = m.c;
     for (int lvl = c->highest_lvl; lvl > c->min_lvl; lvl--)
         launch_restrict(c->dim, c->stream, m.T, level_args(c, lvl), m.V.phi, m.V.phi);
+    const bool to32 = f && c->res_upper > c->highest_lvl;
     for (int lvl = c->min_lvl; lvl <= c->highest_lvl; lvl++) {
         fill_gc(m, m.V.phi, lvl);
-        residual_level(m, lvl);
+        if (to32)
+            launch_residual32(c->dim, c->stream, m.T, level_args(c, lvl), level_vars(m, lvl), f->V.rhs, f->V.rhs_rb,
+                              f->V.rb_stride);
+        else
+            residual_level(m, lvl);
+    }
+    if (to32) {
+        c->mixed_r32 = true;
+        c->mixed_r32_of[0] = m.V.tmp;
+        c->mixed_r32_of[1] = f->V.rhs;
+        if (f->V.rhs_rb) {
+            c->rb_valid[1] = true;
+            c->rb_var[1] = f->o->i_rhs;
+        }
     }
 }
 
@@ -2735,7 +2760,7 @@ int afg_mixed_begin(afg_ctx *c, const afg_mg_opts *o, int have_guess)
             const LevelArgs L = level_args(c, lvl);
             launch_fill_value(c->dim, c->stream, m.T, L.slot0, L.n, L.nc, m.V.phi, 0.0);
         }
-    mixed_residual(m);
+    mixed_residual(m, &f);
     c->mixed_fmg = c->mixed_first_fmg;
     c->mixed_ready = true;
     return launched(c, "afg_mixed_begin");
@@ -2753,8 +2778,11 @@ int afg_mixed_correct(afg_ctx *c, const afg_mg_opts *o, int cycles)
     const long long n = (long long)c->n_slots * c->stride;
     // correction e from A e = r in FP32: `cycles` V-cycles from e = 0 (the
     // first of the first correction an FMG), then phi += e
-    launch_to_fp32(c->stream, n, m.V.tmp, f.V.rhs);
-    if (f.V.rhs_rb) refresh_rhs_rb(c, f.T, (const float *)f.V.rhs, f.V.rhs_rb, o->i_rhs);
+    if (!c->mixed_r32 || c->mixed_r32_of[0] != m.V.tmp || c->mixed_r32_of[1] != f.V.rhs) {
+        launch_to_fp32(c->stream, n, m.V.tmp, f.V.rhs);
+        if (f.V.rhs_rb) refresh_rhs_rb(c, f.T, (const float *)f.V.rhs, f.V.rhs_rb, o->i_rhs);
+    }
+    c->mixed_r32 = false;   // the V-cycles write rhs (tau on the parents)
     HIP_TRY(c, hipMemsetAsync(f.V.phi, 0, n * sizeof(float), c->stream));
     HIP_TRY(c, hipMemsetAsync(f.V.tmp, 0, n * sizeof(float), c->stream));
     for (int k = 0; k < cycles; k++) {
@@ -2766,7 +2794,7 @@ int afg_mixed_correct(afg_ctx *c, const afg_mg_opts *o, int cycles)
         }
     }
     launch_from_fp32(c->stream, n, f.V.phi, m.V.phi, 1);
-    mixed_residual(m);
+    mixed_residual(m, &f);
     return launched(c, "afg_mixed_correct");
 }
 

@@ -37,6 +37,10 @@ int launch_gsrb_split(int dim, hipStream_t s, const DevTree &T, const LevelArgs 
 template <typename R>
 void launch_residual(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
                      const MgVarsT<R> &V);
+// k_residual over a level (FP64) that also writes (float) of the residual
+// into r32 (the FP32 rhs pool) and r32_rb (its colour-split copy, or null)
+void launch_residual32(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<double> &V,
+                       float *r32, float *r32_rb, int rb_stride);
 template <typename R>
 void launch_uc_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
                      const MgVarsT<R> &V, int keep_child_tmp, int fly);

@@ -138,6 +138,18 @@ __global__ __launch_bounds__(BLOCK) void k_residual(DevTree T, LevelArgs L, Leve
     if (gid < L.n * ipow(NC ? NC : L.nc, DIM)) residual_body<DIM, NC>(T, L, v, v, op_auto, gid);
 }
 
+// The mixed solver's FP64 residual with its FP32 image: tmp = rhs - A phi as
+// k_residual, and (float) of it into the FP32 rhs pool (same slot layout) and
+// its colour-split copy (3D; null: none), which the next correction takes as
+// its right-hand side.
+template <int DIM, int NC, typename R>
+__global__ __launch_bounds__(BLOCK) void k_residual32(DevTree T, LevelArgs L, LevelViewT<R> v, int op_auto,
+                                                      float *r32, float *r32_rb, int rb_stride)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < L.n * ipow(NC ? NC : L.nc, DIM)) residual32_body<DIM, NC>(T, L, v, op_auto, r32, r32_rb, rb_stride, gid);
+}
+
 template <int DIM, int NC, typename R>
 __global__ __launch_bounds__(BLOCK) void k_uc_child(DevTree T, LevelArgs L, LevelViewT<R> v,
                                                     int op_auto, int keep_child_tmp, int fly)
@@ -694,6 +706,15 @@ void launch_residual(int dim, hipStream_t s, const DevTree &T, const LevelArgs &
     if (L.n <= 0) return;
     const long long th = (long long)L.n * ipow(L.nc, dim);
     AFG_DISPATCH(dim, L.nc, k_residual, n_blocks(th), T, L, pool_view(T, V), V.op_auto);
+}
+
+void launch_residual32(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<double> &V,
+                       float *r32, float *r32_rb, int rb_stride)
+{
+    using R = double;
+    if (L.n <= 0) return;
+    const long long th = (long long)L.n * ipow(L.nc, dim);
+    AFG_DISPATCH(dim, L.nc, k_residual32, n_blocks(th), T, L, pool_view(T, V), V.op_auto, r32, r32_rb, rb_stride);
 }
 
 template <typename R>

@@ -61,7 +61,7 @@ def parse():
     ap.add_argument("--c4-solver", choices=["both", "fp64", "mixed"], default="both",
                     help="c4: solvers to run (one alone for profiling)")
     ap.add_argument("--mixed-cycles", type=int, default=0,
-                    help="c4: FP32 V-cycles per correction (0: try 1, 2 and 3)")
+                    help="c4: FP32 V-cycles per correction (0: try 1 to 5)")
     ap.add_argument("--trees", type=int, default=256, help="c3: number of trees")
     ap.add_argument("--tol", type=float, default=1e-10, help="c3, c4: relative residual to reach")
     ap.add_argument("--tiles", default="",
@@ -343,7 +343,7 @@ def bench_c3(a):
     # corrections of 2 or 3 V-cycles in FP64 defect correction (the faster is
     # reported), each solve after a warm-up solve
     mixed = {}
-    for cyc in (2, 3):
+    for cyc in (2, 3, 4):
         for rep in range(2):
             for fdt, fst, _ in forests:
                 for iv, v in enumerate(fresh(fst), start=1):
@@ -459,7 +459,7 @@ def bench_c4(a):
         return k
 
     times = {}
-    cyc = [a.mixed_cycles] if a.mixed_cycles else [1, 2, 3]
+    cyc = [a.mixed_cycles] if a.mixed_cycles else [1, 2, 3, 4, 5]
     runs = ([("fp64", fp64_solve)] if a.c4_solver in ("both", "fp64") else []) + \
         ([("mixed%d" % k, lambda k=k: M.mixed_solve(dt, mg, a.tol, 40, cycles=k)) for k in cyc]
          if a.c4_solver in ("both", "mixed") else [])
@@ -528,7 +528,7 @@ def bench_c4(a):
             "workload": "3D variable-coefficient Poisson div(eps grad phi) = rhs: 16^3 boxes, uniform "
                         "to level 3, adaptive to level 8 around 6 Gaussians (seed %d, threshold %g); "
                         "eps = 1 below z = 0.5 and %.3f above (harmonic face means, mg3_auto_*); "
-                        "Dirichlet 0; mixed: FP32 FAS corrections (1-3 V-cycles each, the first an FMG; "
+                        "Dirichlet 0; mixed: FP32 FAS corrections (1-5 V-cycles each, the first an FMG; "
                         "the fastest reported) in FP64 defect correction; fp64: FMG + V-cycles (2+2, 50 base)"
                         % (a.seed, P.THR_C4, p.eps_hi),
             "boxes": int(st.n_boxes), "leaf_cells": leaf_cells,

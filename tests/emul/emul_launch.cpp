@@ -310,6 +310,17 @@ void launch_residual(int dim, hipStream_t s, const DevTree &T, const LevelArgs &
     });
 }
 
+void launch_residual32(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<double> &V,
+                       float *r32, float *r32_rb, int rb_stride)
+{
+    if (L.n <= 0) return;
+    emul_submit(s, [=] {
+        const LevelViewT<double> v = pool_view(T, V);
+        run_stage(dim, L.nc, [&](auto d) { return (long long)L.n * ipow(L.nc, DIMV); },
+                  [&](auto d, auto c, int g) { residual32_body<DIMV, NCV>(T, L, v, V.op_auto, r32, r32_rb, rb_stride, g); });
+    });
+}
+
 template <typename R>
 void launch_uc_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
                      const MgVarsT<R> &V, int keep_child_tmp, int fly)

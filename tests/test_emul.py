@@ -62,6 +62,7 @@ from test_mixed import (  # noqa: E402,F401
     test_fp32_multigrid_tracks_fp64,
     test_mixed_solve_matches_fp64_solution,
     test_mixed_solve_with_fp64_tau,
+    test_mixed_correction_rhs_paths_agree,
 )
 from test_tree_update import (  # noqa: E402,F401
     test_update_full_size_properties,

@@ -149,6 +149,30 @@ def test_mixed_solve_with_fp64_tau(kind):
     assert not np.array_equal(phi_tau64.view(np.int64), phi_default.view(np.int64))
 
 
+@pytest.mark.parametrize("kind", ["2d", "3d_eps"])
+def test_mixed_correction_rhs_paths_agree(kind):
+    """A correction takes its FP32 right-hand side straight from the FP64
+    residual kernel that ended the step before, or, after any write of cell
+    data in between, by converting tmp: both give the same phi bit for bit."""
+    p = mixed_problem(kind)
+    st = p.state
+    mg = M.default_mg(eps=p.eps_hi is not None, n_base=p.n_base)
+    dt = M.DeviceTree.from_state(st)
+    M.mixed_begin(dt, mg)
+    for _ in range(3):
+        M.mixed_correct(dt, mg, 2)
+    phi_direct = dt.download(1)
+    for iv, a in enumerate(st.data, start=1):
+        dt.upload(iv, a)
+    M.mixed_begin(dt, mg)
+    for _ in range(3):
+        dt.upload(2, st.data[1])   # the same rhs again: cell data written
+        M.mixed_correct(dt, mg, 2)
+    phi_converted = dt.download(1)
+    dt.close()
+    assert np.array_equal(phi_converted.view(np.int64), phi_direct.view(np.int64))
+
+
 def test_c4_configuration_against_the_oracle():
     """The 3D variable-eps configuration itself (nc 16, 2804 boxes, 1.00e7
     leaf cells, levels -2..7): the FP64 GPU solve (FMG + 11 V-cycles, the
