@@ -866,9 +866,10 @@ AFG_DEV void split_stage_a(const DevTree &T, const LevelArgs &L, const LevelView
 // Stage B of the split passes: colour rb updated from LDS as tile_stage_b
 // does; MODE 0 (first) and 1 (middle) write it to phi_rb, the first pass
 // also the other colour as read; MODE 2 (last) writes both colours to the box
-// arrays.
+// arrays; MODE 3 (last, with the ghost fill) also puts colour rb into LDS for
+// split_fill_push.
 template <int DIM, int NC, int V, int MODE, typename R>
-AFG_DEV void split_stage_b(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &v, R *rbp, const R *lds,
+AFG_DEV void split_stage_b(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &v, R *rbp, R *lds,
                            int box0, int op_auto, int rb, int tid, const R *rhs)
 {
     using G = GsrbTile<DIM, NC, V>;
@@ -894,10 +895,11 @@ AFG_DEV void split_stage_b(const DevTree &T, const LevelArgs &L, const LevelView
         const size_t base = v.off(slot);
         const bool lpld = lpld_box(T, op_auto, slot);
         const R nv = gsrb_value<DIM>(u, rhs[r], lpld ? v.eps + base : nullptr, q, S, L.dx2, lpld);
-        if (MODE == 2)
+        if (MODE >= 2)
             v.phi[base + q] = nv;
         else
             rbp[(size_t)slot * v.rb_stride + ((rb + 1) & 1) * G::HALF + c] = nv;
+        if (MODE == 3) lds[b * G::LEN + q] = nv;
     }
     if (MODE == 1) return;
     // the other colour, unchanged by this pass
@@ -909,10 +911,54 @@ AFG_DEV void split_stage_b(const DevTree &T, const LevelArgs &L, const LevelView
         colour_cell<DIM>(c, NC, rb + 1, i, j, k);
         const int q = i + S * (j + S * k);
         const int slot = L.slot0 + box0 + b;
-        if (MODE == 2)
+        if (MODE >= 2)
             v.phi[v.off(slot) + q] = lds[b * G::LEN + q];
         else
             rbp[(size_t)slot * v.rb_stride + (rb & 1) * G::HALF + c] = lds[b * G::LEN + q];
+    }
+}
+
+// The ghost fill of the level after its last pass (MODE 3), from the tile's
+// final cells in LDS (both colours, after a workgroup barrier): every face
+// cell goes to the ghost layer of the same-level neighbour on that face (the
+// value the neighbour's fill would read there: sides_from_nb), and at
+// refinement and physical boundaries the box's own ghost is formed with
+// ghost_value's expressions from its LDS cells and the parent level.  Every
+// face of every box of the level is then filled, as by fill_body: a face with
+// a same-level neighbour by that neighbour's workgroup.  Nothing in the launch
+// reads the box arrays' ghost layer (the pass reads phi_rb).
+template <int DIM, int NC, int V, typename R>
+AFG_DEV void split_fill_push(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &v, const R *lds, int box0,
+                             int tid)
+{
+    using G = GsrbTile<DIM, NC, V>;
+    constexpr int S = G::S;
+    constexpr int ALL = 2 * DIM * G::FACE;   // face cells of a box
+    for (int e = tid; e < G::BOXES * ALL; e += G::THREADS) {
+        const int b = e / ALL;
+        if (box0 + b >= L.n) break;
+        const int r = e - b * ALL;
+        const int f = r / G::FACE, t = r - f * G::FACE;
+        const int t1 = t % NC + 1, t2 = t / NC + 1;
+        const int dn = f >> 1, high = f & 1;
+        const int m = high ? NC : 1;
+        const int i = dn == 0 ? m : t1;
+        const int j = dn == 1 ? m : (dn == 0 ? t1 : t2);
+        const int k = dn == 2 ? m : t2;
+        const int slot = L.slot0 + box0 + b;
+        const R *xb = lds + b * G::LEN;
+        const R x = xb[i + S * (j + S * k)];
+        const int nbv = T.nb[slot * (2 * DIM) + f];
+        if (nbv >= 0) {   // the neighbour's ghost on its opposite face
+            const int gm = high ? 0 : NC + 1;
+            const int gi = dn == 0 ? gm : i, gj = dn == 1 ? gm : j, gk = dn == 2 ? gm : k;
+            v.phi[v.off(nbv) + gi + S * (gj + S * gk)] = x;
+        } else {
+            const int gm = high ? NC + 1 : 0;
+            const int gi = dn == 0 ? gm : i, gj = dn == 1 ? gm : j, gk = dn == 2 ? gm : k;
+            v.phi[v.off(slot) + gi + S * (gj + S * gk)] =
+                ghost_value<DIM>(T, v, v, L.pnc, L.dr, slot, f, NC, i, j, k, x, xb);
+        }
     }
 }
 

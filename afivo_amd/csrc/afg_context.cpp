@@ -575,12 +575,23 @@ double gsrb_pass_bytes(const afg_ctx *c, const LevelArgs &L, const MgVarsT<R> &V
     return 0.5 * cells * per_cell;
 }
 
+// Algorithmic bytes of the ghost fill of a level: a value read and a ghost
+// written per face ghost cell (2D/nc ghost cells per cell).
+template <typename R>
+double fill_bytes(const afg_ctx *c, const LevelArgs &L)
+{
+    const double ghosts = (double)L.n * 2 * c->dim * (c->dim == 2 ? L.nc : L.nc * L.nc);
+    return ghosts * 2 * sizeof(R);
+}
+
 // Consecutive colour passes of one level, bracketed by one pair of HIP
 // events on the context stream when that level is timed (a timed block holds
-// n launches and their summed algorithmic bytes).
+// n launches and their summed algorithmic bytes).  With `fill`, the level's
+// ghost fill after the last pass is done by that pass where the passes run on
+// the colour-split copy of phi; returns whether it was.
 template <typename R>
-void timed_passes(afg_ctx *c, const DevTree &T, const LevelArgs &L, int lvl, const MgVarsT<R> &V, R *phi2,
-                  int n_pass, int first_fly)
+bool timed_passes(afg_ctx *c, const DevTree &T, const LevelArgs &L, int lvl, const MgVarsT<R> &V, R *phi2,
+                  int n_pass, int first_fly, int fill = 0)
 {
     const bool on = c->timing && n_pass > 0 && (c->timing_lvl < c->min_lvl || lvl == c->timing_lvl);
     std::pair<hipEvent_t, hipEvent_t> ev;
@@ -615,13 +626,14 @@ void timed_passes(afg_ctx *c, const DevTree &T, const LevelArgs &L, int lvl, con
                                                                  0, rb == 1 ? first_fly : 1);
     if (split) {
         for (rb++, launches++; rb <= n_pass; rb++, launches++)
-            launch_gsrb_split(c->dim, c->stream, T, Lt, V, rb, rb == n_pass ? 2 : 1, 1);
+            launch_gsrb_split(c->dim, c->stream, T, Lt, V, rb, rb < n_pass ? 1 : fill ? 3 : 2, 1);
     }
     for (; rb <= n_pass; rb++, launches++)
         launch_gsrb_pass(c->dim, c->stream, T, Lt, V, rb, rb == 1 ? first_fly : 1);
-    if (!on) return;
+    const bool filled = split && fill;
+    if (!on) return filled;
     hipEventRecordWithFlags(ev.second, c->stream, flags);
-    const TimedBlock tb{launches, n_pass * gsrb_pass_bytes(c, L, V)};
+    const TimedBlock tb{launches, n_pass * gsrb_pass_bytes(c, L, V) + (filled ? fill_bytes<R>(c, L) : 0.0)};
     if (c->capturing) {
         c->cap_ev.push_back(ev);
         c->cap_blocks.push_back(tb);
@@ -629,6 +641,7 @@ void timed_passes(afg_ctx *c, const DevTree &T, const LevelArgs &L, int lvl, con
         c->ev_used.push_back(ev);
         c->ev_blocks.push_back(tb);
     }
+    return filled;
 }
 
 // Add the event times of a graph's last replay to the totals.
@@ -859,10 +872,10 @@ MgVarsT<R> level_vars(const MgT<R> &m, int lvl)
 }
 
 template <typename R>
-void passes(MgT<R> &m, int lvl, int n_cycle, int first_fly)
+bool passes(MgT<R> &m, int lvl, int n_cycle, int first_fly, int fill = 0)
 {
     const LevelArgs L = level_args(m.c, lvl);
-    timed_passes(m.c, m.T, L, lvl, level_vars(m, lvl), m.phi2, 2 * n_cycle, first_fly);
+    return timed_passes(m.c, m.T, L, lvl, level_vars(m, lvl), m.phi2, 2 * n_cycle, first_fly, fill);
 }
 
 // gsrb_boxes as the reference has it: passes, then the last ghost fill.
@@ -914,8 +927,10 @@ template <typename R>
 void correct_relax(MgT<R> &m, int lvl, int n_cycle)
 {
     launch_correct(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), m.V, 1);
-    passes(m, lvl, n_cycle, 1);
-    launch_fill_diff(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), list_args(m.c, lvl - 1, 0), m.V);
+    if (passes(m, lvl, n_cycle, 1, 1))   // the last pass filled the ghosts
+        launch_diff(m.c->dim, m.c->stream, m.T, list_args(m.c, lvl - 1, 0), m.V);
+    else
+        launch_fill_diff(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), list_args(m.c, lvl - 1, 0), m.V);
 }
 
 // residual_box over level lvl, for the trees whose residual is stored: in a

@@ -93,7 +93,8 @@ __global__ __launch_bounds__((GsrbTile<DIM, NC, V>::THREADS)) void k_gsrb_tile(D
 
 // one colour pass of a relaxation block on the colour-split copy of phi (3D
 // tiles): MODE 0 the first pass (box arrays in, FLY as k_gsrb_tile), 1 a
-// middle one, 2 the last (phi_rb in, box arrays out)
+// middle one, 2 the last (phi_rb in, box arrays out), 3 the last followed by
+// the level's ghost fill (split_fill_push)
 template <int DIM, int NC, int MODE, int FLY, typename R, int V>
 __global__ __launch_bounds__((GsrbTile<DIM, NC, V>::THREADS)) void k_gsrb_split(DevTree T, LevelArgs L,
                                                                                LevelViewT<R> v, R *rbp,
@@ -109,6 +110,10 @@ __global__ __launch_bounds__((GsrbTile<DIM, NC, V>::THREADS)) void k_gsrb_split(
         split_stage_a<DIM, NC, V>(T, L, v, rbp, lds, box0, rb, threadIdx.x, rhs);
     __syncthreads();
     split_stage_b<DIM, NC, V, MODE>(T, L, v, rbp, lds, box0, op_auto, rb, threadIdx.x, rhs);
+    if constexpr (MODE == 3) {
+        __syncthreads();
+        split_fill_push<DIM, NC, V>(T, L, v, lds, box0, threadIdx.x);
+    }
 }
 
 // two colour passes (rbA, rbA + 1) by tiles, phi read from v and written to out
@@ -649,8 +654,10 @@ void gsrb_split_v(hipStream_t s, const DevTree &T, const LevelArgs &L, const MgV
         k_gsrb_split<3, NC, 0, 0, R, V><<<grid, G::THREADS, 0, s>>>(T, L, v, Vs.phi_rb, Vs.op_auto, rb);
     else if (mode == 1)
         k_gsrb_split<3, NC, 1, 1, R, V><<<grid, G::THREADS, 0, s>>>(T, L, v, Vs.phi_rb, Vs.op_auto, rb);
-    else
+    else if (mode == 2)
         k_gsrb_split<3, NC, 2, 1, R, V><<<grid, G::THREADS, 0, s>>>(T, L, v, Vs.phi_rb, Vs.op_auto, rb);
+    else
+        k_gsrb_split<3, NC, 3, 1, R, V><<<grid, G::THREADS, 0, s>>>(T, L, v, Vs.phi_rb, Vs.op_auto, rb);
 }
 
 template <int NC, typename R>

@@ -159,7 +159,8 @@ def pass_instances(dim, nc, shape, pairs, n_down, n_up, split=True):
     block that computes them all (after the correction).  Fused pairs take
     two passes per launch; on 3D levels of 8^3 or 16^3 boxes a block of two
     or more single passes runs on the colour-split copy of phi (k_gsrb_split:
-    mode 0 the first pass, 1 the middle ones, 2 the last)."""
+    mode 0 the first pass, 1 the middle ones, 2 the last; 3 the last of an
+    up-sweep block, which also fills the level's ghosts)."""
     out = {}
 
     def add(name, k):
@@ -174,7 +175,7 @@ def pass_instances(dim, nc, shape, pairs, n_down, n_up, split=True):
         if split and dim == 3 and nc in (8, 16) and n >= 2:
             add("k_gsrb_split<%d, %d, 0, %d, double, %d>" % (dim, nc, fly, shape), 1)
             add("k_gsrb_split<%d, %d, 1, 1, double, %d>" % (dim, nc, shape), n - 2)
-            add("k_gsrb_split<%d, %d, 2, 1, double, %d>" % (dim, nc, shape), 1)
+            add("k_gsrb_split<%d, %d, %d, 1, double, %d>" % (dim, nc, 3 if fly else 2, shape), 1)
         else:
             add("k_gsrb_tile<%d, %d, %d, double, %d>" % (dim, nc, fly, shape), 1)
             add("k_gsrb_tile<%d, %d, 1, double, %d>" % (dim, nc, shape), n - 1)

@@ -157,9 +157,13 @@ void gsrb_split_v(const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int
                 split_stage_b<3, NC, SH, 0>(T, L, v, V.phi_rb, lds, box0, V.op_auto, rb, t, r);
             else if (mode == 1)
                 split_stage_b<3, NC, SH, 1>(T, L, v, V.phi_rb, lds, box0, V.op_auto, rb, t, r);
-            else
+            else if (mode == 2)
                 split_stage_b<3, NC, SH, 2>(T, L, v, V.phi_rb, lds, box0, V.op_auto, rb, t, r);
+            else
+                split_stage_b<3, NC, SH, 3>(T, L, v, V.phi_rb, lds, box0, V.op_auto, rb, t, r);
         }
+        if (mode == 3)
+            for (int t = 0; t < G::THREADS; t++) split_fill_push<3, NC, SH>(T, L, v, lds, box0, t);
     }
     emul_free(lds);
 }
