@@ -591,7 +591,7 @@ double fill_bytes(const afg_ctx *c, const LevelArgs &L)
 // the colour-split copy of phi; returns whether it was.
 template <typename R>
 bool timed_passes(afg_ctx *c, const DevTree &T, const LevelArgs &L, int lvl, const MgVarsT<R> &V, R *phi2,
-                  int n_pass, int first_fly, int fill = 0)
+                  int n_pass, int first_fly, int fill = 0, const ListArgs *diff = nullptr)
 {
     const bool on = c->timing && n_pass > 0 && (c->timing_lvl < c->min_lvl || lvl == c->timing_lvl);
     std::pair<hipEvent_t, hipEvent_t> ev;
@@ -626,14 +626,17 @@ bool timed_passes(afg_ctx *c, const DevTree &T, const LevelArgs &L, int lvl, con
                                                                  0, rb == 1 ? first_fly : 1);
     if (split) {
         for (rb++, launches++; rb <= n_pass; rb++, launches++)
-            launch_gsrb_split(c->dim, c->stream, T, Lt, V, rb, rb < n_pass ? 1 : fill ? 3 : 2, 1);
+            launch_gsrb_split(c->dim, c->stream, T, Lt, V, rb, rb < n_pass ? 1 : fill ? 3 : 2, 1,
+                              rb < n_pass ? nullptr : diff);
     }
     for (; rb <= n_pass; rb++, launches++)
         launch_gsrb_pass(c->dim, c->stream, T, Lt, V, rb, rb == 1 ? first_fly : 1);
     const bool filled = split && fill;
     if (!on) return filled;
     hipEventRecordWithFlags(ev.second, c->stream, flags);
-    const TimedBlock tb{launches, n_pass * gsrb_pass_bytes(c, L, V) + (filled ? fill_bytes<R>(c, L) : 0.0)};
+    const double pb = diff ? (double)(diff->nc + 2) * (diff->nc + 2) * (c->dim == 3 ? diff->nc + 2 : 1) : 0.0;
+    const double diff_bytes = filled && diff ? 3.0 * sizeof(R) * diff->n * pb : 0.0;
+    const TimedBlock tb{launches, n_pass * gsrb_pass_bytes(c, L, V) + (filled ? fill_bytes<R>(c, L) : 0.0) + diff_bytes};
     if (c->capturing) {
         c->cap_ev.push_back(ev);
         c->cap_blocks.push_back(tb);
@@ -872,10 +875,10 @@ MgVarsT<R> level_vars(const MgT<R> &m, int lvl)
 }
 
 template <typename R>
-bool passes(MgT<R> &m, int lvl, int n_cycle, int first_fly, int fill = 0)
+bool passes(MgT<R> &m, int lvl, int n_cycle, int first_fly, int fill = 0, const ListArgs *diff = nullptr)
 {
     const LevelArgs L = level_args(m.c, lvl);
-    return timed_passes(m.c, m.T, L, lvl, level_vars(m, lvl), m.phi2, 2 * n_cycle, first_fly, fill);
+    return timed_passes(m.c, m.T, L, lvl, level_vars(m, lvl), m.phi2, 2 * n_cycle, first_fly, fill, diff);
 }
 
 // gsrb_boxes as the reference has it: passes, then the last ghost fill.
@@ -927,10 +930,10 @@ template <typename R>
 void correct_relax(MgT<R> &m, int lvl, int n_cycle)
 {
     launch_correct(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), m.V, 1);
-    if (passes(m, lvl, n_cycle, 1, 1))   // the last pass filled the ghosts
-        launch_diff(m.c->dim, m.c->stream, m.T, list_args(m.c, lvl - 1, 0), m.V);
-    else
-        launch_fill_diff(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), list_args(m.c, lvl - 1, 0), m.V);
+    const ListArgs P = list_args(m.c, lvl - 1, 0);
+    // the last pass on the colour-split copy fills the ghosts and forms the parents' tmp
+    if (!passes(m, lvl, n_cycle, 1, 1, &P))
+        launch_fill_diff(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), P, m.V);
 }
 
 // residual_box over level lvl, for the trees whose residual is stored: in a

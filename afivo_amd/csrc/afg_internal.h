@@ -30,10 +30,12 @@ int launch_gsrb_pair(int dim, hipStream_t s, const DevTree &T, const LevelArgs &
                      const R *in, R *out, int rbA, int fly);
 // one colour pass of a relaxation block on the colour-split copy V.phi_rb
 // (3D, nc 8 and 16, tiles): mode 0 the first pass of the block, 1 a middle
-// one, 2 the last; returns 0 (nothing launched) where it does not apply
+// one, 2 the last, 3 the last followed by the level's ghost fill and, with P,
+// tmp = phi - tmp over the parents P; returns 0 (nothing launched) where it
+// does not apply
 template <typename R>
 int launch_gsrb_split(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int rb,
-                      int mode, int fly);
+                      int mode, int fly, const ListArgs *P = nullptr);
 template <typename R>
 void launch_residual(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
                      const MgVarsT<R> &V);

@@ -95,15 +95,26 @@ __global__ __launch_bounds__((GsrbTile<DIM, NC, V>::THREADS)) void k_gsrb_tile(D
 // tiles): MODE 0 the first pass (box arrays in, FLY as k_gsrb_tile), 1 a
 // middle one, 2 the last (phi_rb in, box arrays out), 3 the last followed by
 // the level's ghost fill (split_fill_push)
+// MODE 3 launches may carry, after their n_tiles tile workgroups, workgroups
+// that form tmp = phi - tmp over the parents P of the level below
+// (correct_children's parent part, which nothing reads before this point).
 template <int DIM, int NC, int MODE, int FLY, typename R, int V>
 __global__ __launch_bounds__((GsrbTile<DIM, NC, V>::THREADS)) void k_gsrb_split(DevTree T, LevelArgs L,
                                                                                LevelViewT<R> v, R *rbp,
-                                                                               int op_auto, int rb)
+                                                                               int op_auto, int rb, ListArgs P,
+                                                                               int n_tiles)
 {
     using G = GsrbTile<DIM, NC, V>;
     __shared__ R lds[G::BOXES * G::LEN];
     R rhs[G::PER];
-    const int box0 = xcd_tile(blockIdx.x, gridDim.x) * G::BOXES;
+    if constexpr (MODE == 3) {
+        if ((int)blockIdx.x >= n_tiles) {   // the whole workgroup
+            const long long g = (long long)(blockIdx.x - n_tiles) * G::THREADS + threadIdx.x;
+            if (g < (long long)P.n * ipow(P.nc + 2, DIM)) diff_body<DIM, 0>(T, P, v, (int)g);
+            return;
+        }
+    }
+    const int box0 = xcd_tile(blockIdx.x, n_tiles) * G::BOXES;
     if (MODE == 0)
         tile_stage_a<DIM, NC, FLY, V>(T, L, v, v, lds, box0, rb, threadIdx.x, rhs);
     else
@@ -643,40 +654,47 @@ int launch_gsrb_pair(int dim, hipStream_t s, const DevTree &T, const LevelArgs &
 
 template <int NC, typename R, int V>
 void gsrb_split_v(hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &Vs, int rb, int mode,
-                  int fly)
+                  int fly, const ListArgs *P)
 {
     using G = GsrbTile<3, NC, V>;
-    const int grid = (L.n + G::BOXES - 1) / G::BOXES;
+    const int n_tiles = (L.n + G::BOXES - 1) / G::BOXES;
+    ListArgs Pd{};
+    int grid = n_tiles;
+    if (mode == 3 && P && P->n > 0) {
+        Pd = *P;
+        grid += (int)(((long long)Pd.n * ipow(Pd.nc + 2, 3) + G::THREADS - 1) / G::THREADS);
+    }
     const LevelViewT<R> v = pool_view(T, Vs);
     if (mode == 0 && fly)
-        k_gsrb_split<3, NC, 0, 1, R, V><<<grid, G::THREADS, 0, s>>>(T, L, v, Vs.phi_rb, Vs.op_auto, rb);
+        k_gsrb_split<3, NC, 0, 1, R, V><<<grid, G::THREADS, 0, s>>>(T, L, v, Vs.phi_rb, Vs.op_auto, rb, Pd, n_tiles);
     else if (mode == 0)
-        k_gsrb_split<3, NC, 0, 0, R, V><<<grid, G::THREADS, 0, s>>>(T, L, v, Vs.phi_rb, Vs.op_auto, rb);
+        k_gsrb_split<3, NC, 0, 0, R, V><<<grid, G::THREADS, 0, s>>>(T, L, v, Vs.phi_rb, Vs.op_auto, rb, Pd, n_tiles);
     else if (mode == 1)
-        k_gsrb_split<3, NC, 1, 1, R, V><<<grid, G::THREADS, 0, s>>>(T, L, v, Vs.phi_rb, Vs.op_auto, rb);
+        k_gsrb_split<3, NC, 1, 1, R, V><<<grid, G::THREADS, 0, s>>>(T, L, v, Vs.phi_rb, Vs.op_auto, rb, Pd, n_tiles);
     else if (mode == 2)
-        k_gsrb_split<3, NC, 2, 1, R, V><<<grid, G::THREADS, 0, s>>>(T, L, v, Vs.phi_rb, Vs.op_auto, rb);
+        k_gsrb_split<3, NC, 2, 1, R, V><<<grid, G::THREADS, 0, s>>>(T, L, v, Vs.phi_rb, Vs.op_auto, rb, Pd, n_tiles);
     else
-        k_gsrb_split<3, NC, 3, 1, R, V><<<grid, G::THREADS, 0, s>>>(T, L, v, Vs.phi_rb, Vs.op_auto, rb);
+        k_gsrb_split<3, NC, 3, 1, R, V><<<grid, G::THREADS, 0, s>>>(T, L, v, Vs.phi_rb, Vs.op_auto, rb, Pd, n_tiles);
 }
 
 template <int NC, typename R>
-void gsrb_split(hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int rb, int mode, int fly)
+void gsrb_split(hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int rb, int mode, int fly,
+                const ListArgs *P)
 {
-    if (L.tile == 1) return gsrb_split_v<NC, R, 1>(s, T, L, V, rb, mode, fly);
-    if (L.tile == 2) return gsrb_split_v<NC, R, 2>(s, T, L, V, rb, mode, fly);
-    gsrb_split_v<NC, R, 0>(s, T, L, V, rb, mode, fly);
+    if (L.tile == 1) return gsrb_split_v<NC, R, 1>(s, T, L, V, rb, mode, fly, P);
+    if (L.tile == 2) return gsrb_split_v<NC, R, 2>(s, T, L, V, rb, mode, fly, P);
+    gsrb_split_v<NC, R, 0>(s, T, L, V, rb, mode, fly, P);
 }
 
 template <typename R>
 int launch_gsrb_split(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int rb,
-                      int mode, int fly)
+                      int mode, int fly, const ListArgs *P)
 {
     if (dim != 3 || !V.phi_rb || !V.rhs_rb || per_cell_only(T, V)) return 0;
     if (L.n <= 0) return 1;
     switch (L.nc) {
-    case 8: gsrb_split<8, R>(s, T, L, V, rb, mode, fly); return 1;
-    case 16: gsrb_split<16, R>(s, T, L, V, rb, mode, fly); return 1;
+    case 8: gsrb_split<8, R>(s, T, L, V, rb, mode, fly, P); return 1;
+    case 16: gsrb_split<16, R>(s, T, L, V, rb, mode, fly, P); return 1;
     default: return 0;
     }
 }
@@ -1067,7 +1085,7 @@ void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVa
 #define AFG_INSTANTIATE(R)                                                                               \
     template void launch_fill_gc<R>(int, hipStream_t, const DevTree &, const LevelArgs &, R *);         \
     template int launch_gsrb_split<R>(int, hipStream_t, const DevTree &, const LevelArgs &,            \
-                                      const MgVarsT<R> &, int, int, int);                              \
+                                      const MgVarsT<R> &, int, int, int, const ListArgs *);            \
     template int launch_gsrb_pair<R>(int, hipStream_t, const DevTree &, const LevelArgs &,             \
                                      const MgVarsT<R> &, const R *, R *, int, int);                    \
     template void launch_fill_diff<R>(int, hipStream_t, const DevTree &, const LevelArgs &,            \

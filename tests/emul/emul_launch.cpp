@@ -133,7 +133,8 @@ void gsrb_tiles(const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int r
 // the colour passes on the colour-split copy of phi: workgroups one after
 // the other, each stage over all threads of the workgroup before the next
 template <int NC, typename R, int SH>
-void gsrb_split_v(const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int rb, int mode, int fly)
+void gsrb_split_v(const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int rb, int mode, int fly,
+                  const ListArgs *P)
 {
     using G = GsrbTile<3, NC, SH>;
     const LevelViewT<R> v = pool_view(T, V);
@@ -165,25 +166,30 @@ void gsrb_split_v(const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int
         if (mode == 3)
             for (int t = 0; t < G::THREADS; t++) split_fill_push<3, NC, SH>(T, L, v, lds, box0, t);
     }
+    if (mode == 3 && P)
+        for (long long g = 0; g < (long long)P->n * ipow(P->nc + 2, 3); g++) diff_body<3, 0>(T, *P, v, (int)g);
     emul_free(lds);
 }
 
 template <typename R>
 int launch_gsrb_split(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int rb,
-                      int mode, int fly)
+                      int mode, int fly, const ListArgs *P)
 {
     if (dim != 3 || !V.phi_rb || !V.rhs_rb || per_cell_only(T, V)) return 0;
     if (L.n <= 0) return 1;
     if (L.nc != 8 && L.nc != 16) return 0;
+    const bool has_p = P != nullptr;
+    const ListArgs Pc = has_p ? *P : ListArgs{};
     emul_submit(s, [=] {
+        const ListArgs *pp = has_p ? &Pc : nullptr;
         auto go = [&](auto ncv) {
             constexpr int NC = decltype(ncv)::value;
             if (L.tile == 1)
-                gsrb_split_v<NC, R, 1>(T, L, V, rb, mode, fly);
+                gsrb_split_v<NC, R, 1>(T, L, V, rb, mode, fly, pp);
             else if (L.tile == 2)
-                gsrb_split_v<NC, R, 2>(T, L, V, rb, mode, fly);
+                gsrb_split_v<NC, R, 2>(T, L, V, rb, mode, fly, pp);
             else
-                gsrb_split_v<NC, R, 0>(T, L, V, rb, mode, fly);
+                gsrb_split_v<NC, R, 0>(T, L, V, rb, mode, fly, pp);
         };
         if (L.nc == 8)
             go(IC<8>());
@@ -748,7 +754,7 @@ void launch_residual_levels(int dim, hipStream_t s, const DevTree &T, const MgVa
 
 #define AFG_INSTANTIATE(R)                                                                               \
     template int launch_gsrb_split<R>(int, hipStream_t, const DevTree &, const LevelArgs &,            \
-                                      const MgVarsT<R> &, int, int, int);                              \
+                                      const MgVarsT<R> &, int, int, int, const ListArgs *);                              \
     template int launch_gsrb_pair<R>(int, hipStream_t, const DevTree &, const LevelArgs &,             \
                                      const MgVarsT<R> &, const R *, R *, int, int);                    \
     template void launch_fill_gc<R>(int, hipStream_t, const DevTree &, const LevelArgs &, R *);         \
