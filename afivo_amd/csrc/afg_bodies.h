@@ -1361,6 +1361,50 @@ AFG_DEV void uc_tile_a(const DevTree &T, const LevelArgs &L, const LevelViewT<R>
     }
 }
 
+// uc_tile_a for a level whose phi is in the colour-split copy `rbp` (3D, the
+// relaxation block before it left it there): the interiors from the copy, every
+// face ghost computed with ghost_value's expressions from the copy
+// (ghost_value_split) and the parent level.
+template <int DIM, int NC, typename R>
+AFG_DEV void uc_tile_a_split(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &v, const R *rbp, R *lds,
+                             int box0, int tid)
+{
+    using G = UcTile<DIM, NC>;
+    constexpr int S = G::S;
+    constexpr int HALF = G::CELLS / 2;
+    for (int e = tid; e < G::BOXES * G::CELLS; e += G::THREADS) {
+        const int b = e / G::CELLS;
+        if (box0 + b >= L.n) break;
+        const int c = e - b * G::CELLS;
+        const int hi = c >= HALF;
+        int i, j, k;
+        colour_cell<DIM>(c - hi * HALF, NC, hi ? 2 : 1, i, j, k);
+        lds[b * G::LEN + i + S * (j + S * k)] = ldv(rbp + (size_t)(L.slot0 + box0 + b) * v.rb_stride + c);
+    }
+    for (int e = tid; e < G::BOXES * G::GHOSTS; e += G::THREADS) {
+        const int b = e / G::GHOSTS;
+        if (box0 + b >= L.n) break;
+        const int r = e - b * G::GHOSTS;
+        const int f = r / G::FACE, t = r - f * G::FACE;
+        const int dn = f >> 1, high = f & 1;
+        const int m = high ? NC : 1;
+        const int t1 = DIM == 2 ? t + 1 : t % NC + 1;
+        const int t2 = DIM == 2 ? 0 : t / NC + 1;
+        const int i = dn == 0 ? m : t1;
+        const int j = dn == 1 ? m : (dn == 0 ? t1 : t2);
+        const int k = DIM == 3 ? (dn == 2 ? m : t2) : 0;
+        const int gi = dn == 0 ? (high ? NC + 1 : 0) : i;
+        const int gj = dn == 1 ? (high ? NC + 1 : 0) : j;
+        const int gk = dn == 2 ? (high ? NC + 1 : 0) : k;
+        const int slot = L.slot0 + box0 + b;
+        const int nbv = T.nb[slot * (2 * DIM) + f];
+        const R center = nbv >= 0 ? R(0) : ldv(rbp + (size_t)slot * v.rb_stride + colour_index<DIM>(i, j, k, NC));
+        lds[b * G::LEN + gi + S * (gj + S * gk)] =
+            ghost_value_split<DIM>(T, nbv, T.parent[slot], T.choff[slot], rbp, v.rb_stride, v, L.pnc, L.dr, slot, f,
+                                   NC, i, j, k, center);
+    }
+}
+
 // fine cell of the tile -> (box, LDS index)
 template <int DIM, int NC>
 AFG_DEV int uc_tile_cell(int e, int &b)
@@ -1557,9 +1601,11 @@ struct Prolong2 {
 // tmp after tmp = phi - tmp; with fly_diff = 1 that difference is formed
 // here from the parent's phi and tmp (the parent's tmp is then updated by a
 // later stage).  One thread per coarse cell of the child's octant.
+// With rbp, the level's phi is the colour-split copy (rbp, own.rb_stride per
+// slot) in place of the box arrays.
 template <int DIM, int NC, typename R>
 AFG_DEV void correct_body(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &own,
-                          const LevelViewT<R> &par, int op_auto, int fly_diff, int gid)
+                          const LevelViewT<R> &par, int op_auto, int fly_diff, int gid, R *rbp = nullptr)
 {
     const int nc = NC ? NC : L.nc;
     const int hn = nc >> 1;
@@ -1582,6 +1628,11 @@ AFG_DEV void correct_body(const DevTree &T, const LevelArgs &L, const LevelViewT
     const R *pp = par.phi + pb;
     auto corr = [&](int q) -> R { return fly_diff ? ldv(pp + q) - ldv(pt + q) : ldv(pt + q); };
     R *cph = own.phi + own.off(slot);
+    R *crb = rbp ? rbp + (size_t)slot * own.rb_stride : nullptr;
+    // fine cell (fi, fj, fk) of the box
+    auto cell = [&](int fi, int fj, int fk) -> R * {
+        return crb ? crb + colour_index<DIM>(fi, fj, fk, nc) : cph + fi + s * (fj + s * fk);
+    };
     if (lsf_box(T, own, slot)) {
         // mg$D_box_corr_lpllsf (m_aX_mg.f90:1019-1102): the correction from the
         // closest coarse cell and the one-but-closest in each direction, with
@@ -1630,7 +1681,7 @@ AFG_DEV void correct_body(const DevTree &T, const LevelArgs &L, const LevelViewT
             // (m_aX_mg.f90:944-958)
             const int qn[3] = {pq + (sx ? 1 : -1), pq + (sy ? ps : -ps),
                                pq + (sz ? ps * ps : -ps * ps)};
-            R *t = &cph[fi + s * (fj + s * fk)];
+            R *t = cell(fi, fj, fk);
             if (DIM == 2) {
                 R sum = (a0 * u0 + pe[qn[0]] * corr(qn[0])) / (a0 + pe[qn[0]]);
                 sum += (a0 * u0 + pe[qn[1]] * corr(qn[1])) / (a0 + pe[qn[1]]);
@@ -1652,7 +1703,7 @@ AFG_DEV void correct_body(const DevTree &T, const LevelArgs &L, const LevelViewT
         for (int m = 0; m < (1 << DIM); m++) {
             const int fi = 2 * ii - 1 + (m & 1), fj = 2 * jj - 1 + ((m >> 1) & 1);
             const int fk = DIM == 3 ? 2 * kk - 1 + ((m >> 2) & 1) : 0;
-            R *t = &cph[fi + s * (fj + s * fk)];
+            R *t = cell(fi, fj, fk);
             *t = P2.at(m) + ldv(t);
         }
     }

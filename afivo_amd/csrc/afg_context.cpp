@@ -586,12 +586,17 @@ double fill_bytes(const afg_ctx *c, const LevelArgs &L)
 
 // Consecutive colour passes of one level, bracketed by one pair of HIP
 // events on the context stream when that level is timed (a timed block holds
-// n launches and their summed algorithmic bytes).  With `fill`, the level's
-// ghost fill after the last pass is done by that pass where the passes run on
-// the colour-split copy of phi; returns whether it was.
+// n launches and their summed algorithmic bytes).  Where the passes run on
+// the colour-split copy of phi: with `fill` the last pass also does the
+// level's ghost fill (and the parents' diff with `diff`); with `keep` the
+// block leaves phi in the copy (the box arrays keep the state before it) and
+// *in_copy says so; with `from_copy` the block starts from the copy (the
+// state the block before left there, as corrected since).  Returns whether
+// the fill was done.
 template <typename R>
 bool timed_passes(afg_ctx *c, const DevTree &T, const LevelArgs &L, int lvl, const MgVarsT<R> &V, R *phi2,
-                  int n_pass, int first_fly, int fill = 0, const ListArgs *diff = nullptr)
+                  int n_pass, int first_fly, int fill = 0, const ListArgs *diff = nullptr, int keep = 0,
+                  int from_copy = 0, bool *in_copy = nullptr)
 {
     const bool on = c->timing && n_pass > 0 && (c->timing_lvl < c->min_lvl || lvl == c->timing_lvl);
     std::pair<hipEvent_t, hipEvent_t> ev;
@@ -615,19 +620,21 @@ bool timed_passes(afg_ctx *c, const DevTree &T, const LevelArgs &L, int lvl, con
     // one: on the colour-split copy of phi when two or more are left (first,
     // middle and last passes), else on the box arrays.
     int rb = 1, launches = 0;
-    if (phi2 && Lv.pairs[sizeof(R) == 4])
+    if (in_copy) *in_copy = false;
+    if (phi2 && Lv.pairs[sizeof(R) == 4] && !from_copy)
         for (; rb + 3 <= n_pass; rb += 4) {
             if (!launch_gsrb_pair(c->dim, c->stream, T, Lt, V, V.phi, phi2, rb, rb == 1 ? first_fly : 1)) break;
             launch_gsrb_pair(c->dim, c->stream, T, Lt, V, phi2, V.phi, rb + 2, 1);
             launches += 2;
         }
     const int first = rb;
-    const bool split = n_pass - first >= 1 && launch_gsrb_split(c->dim, c->stream, T, Lt, V, rb,
-                                                                 0, rb == 1 ? first_fly : 1);
+    const bool split = n_pass - first >= 1 && launch_gsrb_split(c->dim, c->stream, T, Lt, V, rb, from_copy ? 1 : 0,
+                                                                 rb == 1 ? first_fly : 1);
     if (split) {
         for (rb++, launches++; rb <= n_pass; rb++, launches++)
-            launch_gsrb_split(c->dim, c->stream, T, Lt, V, rb, rb < n_pass ? 1 : fill ? 3 : 2, 1,
+            launch_gsrb_split(c->dim, c->stream, T, Lt, V, rb, rb < n_pass ? 1 : fill ? 3 : keep ? 1 : 2, 1,
                               rb < n_pass ? nullptr : diff);
+        if (in_copy) *in_copy = keep && !fill;
     }
     for (; rb <= n_pass; rb++, launches++)
         launch_gsrb_pass(c->dim, c->stream, T, Lt, V, rb, rb == 1 ? first_fly : 1);
@@ -875,10 +882,12 @@ MgVarsT<R> level_vars(const MgT<R> &m, int lvl)
 }
 
 template <typename R>
-bool passes(MgT<R> &m, int lvl, int n_cycle, int first_fly, int fill = 0, const ListArgs *diff = nullptr)
+bool passes(MgT<R> &m, int lvl, int n_cycle, int first_fly, int fill = 0, const ListArgs *diff = nullptr,
+            int keep = 0, int from_copy = 0, bool *in_copy = nullptr)
 {
     const LevelArgs L = level_args(m.c, lvl);
-    return timed_passes(m.c, m.T, L, lvl, level_vars(m, lvl), m.phi2, 2 * n_cycle, first_fly, fill, diff);
+    return timed_passes(m.c, m.T, L, lvl, level_vars(m, lvl), m.phi2, 2 * n_cycle, first_fly, fill, diff, keep,
+                        from_copy, in_copy);
 }
 
 // gsrb_boxes as the reference has it: passes, then the last ghost fill.
@@ -895,9 +904,10 @@ void gsrb_level(MgT<R> &m, int lvl, int n_cycle)
 // before the upward sweep fills lvl again), and the ghost fill of lvl-1 runs
 // inside the parent stage.
 template <typename R>
-void update_coarse(MgT<R> &m, int lvl, int relaxed)
+void update_coarse(MgT<R> &m, int lvl, int relaxed, bool in_copy = false)
 {
-    launch_uc_child(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), level_vars(m, lvl), 1, relaxed ? 1 : 0);
+    launch_uc_child(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), level_vars(m, lvl), 1, relaxed ? 1 : 0,
+                    in_copy ? 1 : 0);
     launch_uc_parent(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl - 1), m.V, 1);
 }
 
@@ -927,12 +937,14 @@ void init_phi_rhs(MgT<R> &m)
 // fly from the parents' phi and tmp, and the parents' tmp = phi - tmp is
 // written with the final ghost fill of lvl (nothing reads it in between).
 template <typename R>
-void correct_relax(MgT<R> &m, int lvl, int n_cycle)
+void correct_relax(MgT<R> &m, int lvl, int n_cycle, bool in_copy = false)
 {
-    launch_correct(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), m.V, 1);
+    // a level whose down-sweep block left phi in the colour-split copy is
+    // corrected there and its up-sweep block starts from it
+    launch_correct(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), level_vars(m, lvl), 1, in_copy ? 1 : 0);
     const ListArgs P = list_args(m.c, lvl - 1, 0);
     // the last pass on the colour-split copy fills the ghosts and forms the parents' tmp
-    if (!passes(m, lvl, n_cycle, 1, 1, &P))
+    if (!passes(m, lvl, n_cycle, 1, 1, &P, 0, in_copy ? 1 : 0))
         launch_fill_diff(m.c->dim, m.c->stream, m.T, level_args(m.c, lvl), P, m.V);
 }
 
@@ -1004,9 +1016,16 @@ void vcycle_body(MgT<R> &m, int highest_lvl, int set_residual)
     // it holds the whole V-cycle (above it, the up-sweep still writes tmp on
     // the parents of level lb + 1)
     const bool res_inside = persist && set_residual && lb == highest_lvl && c->res_upper > c->highest_lvl;
+    // a level's phi may stay in the colour-split copy from its down-sweep block to
+    // its up-sweep block (update_coarse and the correction work there), when an
+    // up-sweep block will write it back; the FP64-tau stages read the box arrays
+    const bool keep = m.o->n_cycle_up > 0 && !m.V.tau64;
+    std::vector<char> in_copy(c->levels.size(), 0);
     for (int lvl = highest_lvl; lvl > low; lvl--) {
-        passes(m, lvl, m.o->n_cycle_down, 0);
-        update_coarse(m, lvl, m.o->n_cycle_down > 0);
+        bool ic = false;
+        passes(m, lvl, m.o->n_cycle_down, 0, 0, nullptr, keep ? 1 : 0, 0, &ic);
+        update_coarse(m, lvl, m.o->n_cycle_down > 0, ic);
+        in_copy[lvl - c->min_lvl] = ic;
     }
     if (lb >= c->min_lvl && persist) {
         PersistArgs P;
@@ -1024,7 +1043,7 @@ void vcycle_body(MgT<R> &m, int highest_lvl, int set_residual)
     } else {
         gsrb_level(m, c->min_lvl, m.o->n_cycle_base);
     }
-    for (int lvl = low + 1; lvl <= highest_lvl; lvl++) correct_relax(m, lvl, m.o->n_cycle_up);
+    for (int lvl = low + 1; lvl <= highest_lvl; lvl++) correct_relax(m, lvl, m.o->n_cycle_up, in_copy[lvl - c->min_lvl]);
     if (set_residual) {
         if (lb >= c->min_lvl && c->res_upper > c->highest_lvl) {
             if (persist && !res_inside) {

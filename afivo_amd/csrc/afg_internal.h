@@ -43,16 +43,18 @@ void launch_residual(int dim, hipStream_t s, const DevTree &T, const LevelArgs &
 // into r32 (the FP32 rhs pool) and r32_rb (its colour-split copy, or null)
 void launch_residual32(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const MgVarsT<double> &V,
                        float *r32, float *r32_rb, int rb_stride);
+// from_rb: the level's phi is in the colour-split copy V.phi_rb (3D tiles)
 template <typename R>
 void launch_uc_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                     const MgVarsT<R> &V, int keep_child_tmp, int fly);
+                     const MgVarsT<R> &V, int keep_child_tmp, int fly, int from_rb = 0);
 // L: the parent level (all its boxes get their ghosts filled)
 template <typename R>
 void launch_uc_parent(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
                       const MgVarsT<R> &V, int copy_tmp);
+// to_rb: the level's phi is in the colour-split copy V.phi_rb
 template <typename R>
 void launch_correct(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                    const MgVarsT<R> &V, int fly_diff);
+                    const MgVarsT<R> &V, int fly_diff, int to_rb = 0);
 // rhs by colour (MgVarsT::rhs_rb) of every box of a level, from rhs
 template <typename R>
 void launch_split_rhs(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L, const R *rhs, R *rhs_rb,

@@ -195,17 +195,21 @@ __global__ __launch_bounds__(BLOCK) void k_uc_parent64(DevTree T, LevelArgs L, L
         uc_parent_body<DIM, NC, R, double>(T, L, v, v, op_auto, copy_tmp, gid);
 }
 
-// update_coarse, child part, by tiles in LDS (NC 8 and 16)
+// update_coarse, child part, by tiles in LDS (NC 8 and 16); FLY 2: the
+// level's phi in the colour-split copy rbp (ghosts computed from it)
 template <int DIM, int NC, int FLY, typename R>
 __global__ __launch_bounds__((UcTile<DIM, NC>::THREADS)) void k_uc_child_tile(DevTree T, LevelArgs L,
                                                                              LevelViewT<R> v, int op_auto,
-                                                                             int keep_child_tmp)
+                                                                             int keep_child_tmp, const R *rbp)
 {
     using G = UcTile<DIM, NC>;
     __shared__ R lds[G::BOXES * G::LEN];
     R res[G::PER];
     const int box0 = xcd_tile(blockIdx.x, gridDim.x) * G::BOXES;
-    uc_tile_a<DIM, NC, FLY>(T, L, v, lds, box0, threadIdx.x);
+    if constexpr (FLY == 2)
+        uc_tile_a_split<DIM, NC>(T, L, v, rbp, lds, box0, threadIdx.x);
+    else
+        uc_tile_a<DIM, NC, FLY>(T, L, v, lds, box0, threadIdx.x);
     __syncthreads();
     uc_tile_b<DIM, NC>(T, L, v, lds, box0, op_auto, keep_child_tmp, threadIdx.x, res);
     __syncthreads();
@@ -223,13 +227,14 @@ __global__ __launch_bounds__(BLOCK) void k_uc_parent(DevTree T, LevelArgs L, Lev
         uc_parent_body<DIM, NC>(T, L, v, v, op_auto, copy_tmp, gid);
 }
 
+// rbp: the level's phi in the colour-split copy (null: the box arrays)
 template <int DIM, int NC, typename R>
 __global__ __launch_bounds__(BLOCK) void k_correct(DevTree T, LevelArgs L, LevelViewT<R> v,
-                                                   int op_auto, int fly_diff)
+                                                   int op_auto, int fly_diff, R *rbp)
 {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid < L.n * ipow((NC ? NC : L.nc) / 2, DIM))
-        correct_body<DIM, NC>(T, L, v, v, op_auto, fly_diff, gid);
+        correct_body<DIM, NC>(T, L, v, v, op_auto, fly_diff, gid, rbp);
 }
 
 // rhs by colour from rhs over the boxes of a level
@@ -744,7 +749,7 @@ void launch_residual32(int dim, hipStream_t s, const DevTree &T, const LevelArgs
 
 template <typename R>
 void launch_uc_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                     const MgVarsT<R> &V, int keep_child_tmp, int fly)
+                     const MgVarsT<R> &V, int keep_child_tmp, int fly, int from_rb)
 {
     if (L.n <= 0) return;
     if constexpr (sizeof(R) == 4) {
@@ -760,12 +765,19 @@ void launch_uc_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &
             constexpr int D = decltype(dv)::value, N = decltype(nv)::value;
             using G = UcTile<D, N>;
             const int grid = (L.n + G::BOXES - 1) / G::BOXES;
+            if constexpr (D == 3) {
+                if (from_rb) {
+                    k_uc_child_tile<D, N, 2, R><<<grid, G::THREADS, 0, s>>>(T, L, pool_view(T, V), V.op_auto,
+                                                                            keep_child_tmp, V.phi_rb);
+                    return;
+                }
+            }
             if (fly)
                 k_uc_child_tile<D, N, 1, R><<<grid, G::THREADS, 0, s>>>(T, L, pool_view(T, V), V.op_auto,
-                                                                        keep_child_tmp);
+                                                                        keep_child_tmp, nullptr);
             else
                 k_uc_child_tile<D, N, 0, R><<<grid, G::THREADS, 0, s>>>(T, L, pool_view(T, V), V.op_auto,
-                                                                        keep_child_tmp);
+                                                                        keep_child_tmp, nullptr);
         };
         if (dim == 2)
             L.nc == 8 ? go(IC<2>(), IC<8>()) : go(IC<2>(), IC<16>());
@@ -795,11 +807,12 @@ void launch_uc_parent(int dim, hipStream_t s, const DevTree &T, const LevelArgs 
 
 template <typename R>
 void launch_correct(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                    const MgVarsT<R> &V, int fly_diff)
+                    const MgVarsT<R> &V, int fly_diff, int to_rb)
 {
     if (L.n <= 0) return;
     const long long th = (long long)L.n * ipow(L.nc / 2, dim);
-    AFG_DISPATCH(dim, L.nc, k_correct, n_blocks(th), T, L, pool_view(T, V), V.op_auto, fly_diff);
+    R *rbp = to_rb ? V.phi_rb : nullptr;
+    AFG_DISPATCH(dim, L.nc, k_correct, n_blocks(th), T, L, pool_view(T, V), V.op_auto, fly_diff, rbp);
 }
 
 template <typename R>
@@ -1097,11 +1110,11 @@ void launch_bottom_residual(int dim, hipStream_t s, const DevTree &T, const MgVa
     template void launch_residual<R>(int, hipStream_t, const DevTree &, const LevelArgs &,             \
                                      const MgVarsT<R> &);                                              \
     template void launch_uc_child<R>(int, hipStream_t, const DevTree &, const LevelArgs &,             \
-                                     const MgVarsT<R> &, int, int);                                    \
+                                     const MgVarsT<R> &, int, int, int);                               \
     template void launch_uc_parent<R>(int, hipStream_t, const DevTree &, const LevelArgs &,            \
                                       const MgVarsT<R> &, int);                                        \
     template void launch_correct<R>(int, hipStream_t, const DevTree &, const LevelArgs &,              \
-                                    const MgVarsT<R> &, int);                                          \
+                                    const MgVarsT<R> &, int, int);                                     \
     template void launch_restrict<R>(int, hipStream_t, const DevTree &, const LevelArgs &, const R *, R *, const R *, int); \
     template void launch_split_rhs<R>(int, hipStream_t, const DevTree &, const LevelArgs &, const R *, R *, int); \
     template void launch_prolong2<R>(int, hipStream_t, const DevTree &, const LevelArgs &, const R *, R *, \

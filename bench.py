@@ -160,7 +160,8 @@ def pass_instances(dim, nc, shape, pairs, n_down, n_up, split=True):
     two passes per launch; on 3D levels of 8^3 or 16^3 boxes a block of two
     or more single passes runs on the colour-split copy of phi (k_gsrb_split:
     mode 0 the first pass, 1 the middle ones, 2 the last; 3 the last of an
-    up-sweep block, which also fills the level's ghosts)."""
+    up-sweep block, which also fills the level's ghosts; phi stays in the
+    copy from the down-sweep's block to the up-sweep's)."""
     out = {}
 
     def add(name, k):
@@ -171,13 +172,21 @@ def pass_instances(dim, nc, shape, pairs, n_down, n_up, split=True):
         add("k_gsrb_pair<%d, %d, 0, double, %d>" % (dim, nc, shape), 1)
         add("k_gsrb_pair<%d, %d, 1, double, %d>" % (dim, nc, shape), (2 * n_down) // 2 - 1 + (2 * n_up) // 2)
         return out
-    for n, fly in ((2 * n_down, 0), (2 * n_up, 1)):
+    # with an up-sweep block to write it back, the down-sweep's block leaves
+    # phi in the copy (its last pass a middle one) and the up-sweep's starts
+    # from it (its first pass a middle one)
+    keep = n_up > 0
+    for n, up in ((2 * n_down, 0), (2 * n_up, 1)):
         if split and dim == 3 and nc in (8, 16) and n >= 2:
-            add("k_gsrb_split<%d, %d, 0, %d, double, %d>" % (dim, nc, fly, shape), 1)
+            if up and keep:
+                add("k_gsrb_split<%d, %d, 1, 1, double, %d>" % (dim, nc, shape), 1)
+            else:
+                add("k_gsrb_split<%d, %d, 0, %d, double, %d>" % (dim, nc, up, shape), 1)
             add("k_gsrb_split<%d, %d, 1, 1, double, %d>" % (dim, nc, shape), n - 2)
-            add("k_gsrb_split<%d, %d, %d, 1, double, %d>" % (dim, nc, 3 if fly else 2, shape), 1)
+            last = 3 if up else (1 if keep else 2)
+            add("k_gsrb_split<%d, %d, %d, 1, double, %d>" % (dim, nc, last, shape), 1)
         else:
-            add("k_gsrb_tile<%d, %d, %d, double, %d>" % (dim, nc, fly, shape), 1)
+            add("k_gsrb_tile<%d, %d, %d, double, %d>" % (dim, nc, up, shape), 1)
             add("k_gsrb_tile<%d, %d, 1, double, %d>" % (dim, nc, shape), n - 1)
     return out
 

@@ -202,7 +202,7 @@ int launch_gsrb_split(int dim, hipStream_t s, const DevTree &T, const LevelArgs 
 // the tiled child stage of the device: workgroups one after the other, each
 // phase over all threads of the workgroup before the next
 template <int DIM, int NC, typename R>
-void uc_tiles(const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int keep_child_tmp, int fly)
+void uc_tiles(const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int keep_child_tmp, int fly, int from_rb)
 {
     using G = UcTile<DIM, NC>;
     const LevelViewT<R> v = pool_view(T, V);
@@ -212,7 +212,9 @@ void uc_tiles(const DevTree &T, const LevelArgs &L, const MgVarsT<R> &V, int kee
     for (int blk = 0; blk < grid; blk++) {
         const int box0 = xcd_tile(blk, grid) * G::BOXES;
         for (int t = 0; t < G::THREADS; t++) {
-            if (fly)
+            if (DIM == 3 && from_rb)
+                uc_tile_a_split<DIM, NC>(T, L, v, V.phi_rb, lds, box0, t);
+            else if (fly)
                 uc_tile_a<DIM, NC, 1>(T, L, v, lds, box0, t);
             else
                 uc_tile_a<DIM, NC, 0>(T, L, v, lds, box0, t);
@@ -333,7 +335,7 @@ void launch_residual32(int dim, hipStream_t s, const DevTree &T, const LevelArgs
 
 template <typename R>
 void launch_uc_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                     const MgVarsT<R> &V, int keep_child_tmp, int fly)
+                     const MgVarsT<R> &V, int keep_child_tmp, int fly, int from_rb)
 {
     if (L.n <= 0) return;
     if constexpr (sizeof(R) == 4) {
@@ -351,9 +353,10 @@ void launch_uc_child(int dim, hipStream_t s, const DevTree &T, const LevelArgs &
     if ((L.nc == 8 || L.nc == 16) && !per_cell_only(T, V)) {
         emul_submit(s, [=] {
             if (dim == 2)
-                L.nc == 8 ? uc_tiles<2, 8, R>(T, L, V, keep_child_tmp, fly) : uc_tiles<2, 16, R>(T, L, V, keep_child_tmp, fly);
+                L.nc == 8 ? uc_tiles<2, 8, R>(T, L, V, keep_child_tmp, fly, 0) : uc_tiles<2, 16, R>(T, L, V, keep_child_tmp, fly, 0);
             else
-                L.nc == 8 ? uc_tiles<3, 8, R>(T, L, V, keep_child_tmp, fly) : uc_tiles<3, 16, R>(T, L, V, keep_child_tmp, fly);
+                L.nc == 8 ? uc_tiles<3, 8, R>(T, L, V, keep_child_tmp, fly, from_rb)
+                          : uc_tiles<3, 16, R>(T, L, V, keep_child_tmp, fly, from_rb);
         });
         return;
     }
@@ -388,13 +391,14 @@ void launch_uc_parent(int dim, hipStream_t s, const DevTree &T, const LevelArgs 
 
 template <typename R>
 void launch_correct(int dim, hipStream_t s, const DevTree &T, const LevelArgs &L,
-                    const MgVarsT<R> &V, int fly_diff)
+                    const MgVarsT<R> &V, int fly_diff, int to_rb)
 {
     if (L.n <= 0) return;
+    R *rbp = to_rb ? V.phi_rb : nullptr;
     emul_submit(s, [=] {
         const LevelViewT<R> v = pool_view(T, V);
         run_stage(dim, L.nc, [&](auto d) { return (long long)L.n * ipow(L.nc / 2, DIMV); },
-                  [&](auto d, auto c, int g) { correct_body<DIMV, NCV>(T, L, v, v, V.op_auto, fly_diff, g); });
+                  [&](auto d, auto c, int g) { correct_body<DIMV, NCV>(T, L, v, v, V.op_auto, fly_diff, g, rbp); });
     });
 }
 
@@ -767,11 +771,11 @@ void launch_residual_levels(int dim, hipStream_t s, const DevTree &T, const MgVa
     template void launch_residual<R>(int, hipStream_t, const DevTree &, const LevelArgs &,             \
                                      const MgVarsT<R> &);                                              \
     template void launch_uc_child<R>(int, hipStream_t, const DevTree &, const LevelArgs &,             \
-                                     const MgVarsT<R> &, int, int);                                    \
+                                     const MgVarsT<R> &, int, int, int);                               \
     template void launch_uc_parent<R>(int, hipStream_t, const DevTree &, const LevelArgs &,            \
                                       const MgVarsT<R> &, int);                                        \
     template void launch_correct<R>(int, hipStream_t, const DevTree &, const LevelArgs &,              \
-                                    const MgVarsT<R> &, int);                                          \
+                                    const MgVarsT<R> &, int, int);                                     \
     template void launch_restrict<R>(int, hipStream_t, const DevTree &, const LevelArgs &, const R *, R *, const R *, int); \
     template void launch_split_rhs<R>(int, hipStream_t, const DevTree &, const LevelArgs &, const R *, R *, int); \
     template void launch_prolong2<R>(int, hipStream_t, const DevTree &, const LevelArgs &, const R *, R *, \
