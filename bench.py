@@ -493,7 +493,7 @@ def bench_c4(a):
             times[name] += (float(np.abs(phi_mixed[mask] - phi64[mask]).max()) / scale,)
         else:
             times[name] += (None,)
-    # the north star's precision split: the FP32 corrections' coarsest-grid
+    # the precision split of the mixed solver: the FP32 corrections' coarsest-grid
     # solve, their update_coarse (the FAS tau), and both, in FP64 arithmetic,
     # at the fastest cycles-per-correction count
     splits = {}
