@@ -9,6 +9,7 @@
 #include "afivo_amd.h"
 #include "afg_internal.h"
 #include "afg_bodies.h"
+#include "afg_lane.h"
 
 namespace afg {
 
@@ -526,7 +527,7 @@ __global__ __launch_bounds__(BOTTOM_BLOCK) void k_bottom(DevTree T, MgVarsT<R> V
                                  __syncthreads();
                                  stamp();
                              },
-                             wave_sync, [](auto v, int mask) { return __shfl_xor(v, mask); });
+                             wave_sync, [](auto v, int mask) { return lane_xor(v, mask); });
     stamp();
 }
 

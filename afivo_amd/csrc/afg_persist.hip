@@ -18,6 +18,7 @@
 #include "afivo_amd.h"
 #include "afg_internal.h"
 #include "afg_bodies.h"
+#include "afg_lane.h"
 
 namespace afg {
 
@@ -75,7 +76,7 @@ __global__ __launch_bounds__(PERSIST_BLOCK) void k_persist(DevTree T, MgVarsT<R>
         __hip_atomic_load(P.sync + 4 + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != xcc + 1)
         __hip_atomic_store(err, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     persist_run<DIM, AUTO, R>(T, V, *B, P.A, P.set_residual, rank, P.n_rank, threadIdx.x, blockDim.x, warpSize,
-                              gsync, wgsync, [](auto x, int mask) { return __shfl_xor(x, mask); });
+                              gsync, wgsync, [](auto x, int mask) { return lane_xor(x, mask); });
     // the last workgroup to leave clears the counters and the XCD slots for
     // the next launch: every other one has passed its last wait (and read the
     // slots) when it counts its departure
