@@ -2136,10 +2136,13 @@ AFG_DEV void nc_dispatch(int nc, F &&f)
 //   would store) and the up-sweep block none before its first pass.
 // At level 0 the block is one step for all 2n passes (from the cell list,
 // computing the face neighbours) and the ghost fill.
-enum BottomOp { BOP_GSRB, BOP_FILL, BOP_UC_CHILD, BOP_UC_PARENT, BOP_CORRECT, BOP_DIFF, BOP_BASE };
+enum BottomOp { BOP_GSRB, BOP_FILL, BOP_UC_CHILD, BOP_UC_PARENT, BOP_CORRECT, BOP_DIFF, BOP_BASE, BOP_BLOCK };
 
+// op on level li; a colour pass has its colour rb and `fly` (whether it
+// computes its face ghosts); BOP_BLOCK stands for the m steps of a relaxation
+// block of rb sweeps whose first pass has fly (relax_step)
 struct BottomStep {
-    int op, li, rb, fly;
+    int op, li, rb, fly, m;
 };
 
 // The form is fixed per dimension at compile time (measured: computing the
@@ -2159,43 +2162,101 @@ template <int FLY>
 AFG_HD inline BottomStep relax_step(int li, int r, int n, int first_fly)
 {
     if (FLY) {
-        if (r == 2 * n) return {BOP_FILL, li, 0, 0};
-        return {BOP_GSRB, li, r + 1, r > 0 ? 1 : first_fly};
+        if (r == 2 * n) return {BOP_FILL, li, 0, 0, 0};
+        return {BOP_GSRB, li, r + 1, r > 0 ? 1 : first_fly, 0};
     }
-    if (r == 0) return {BOP_GSRB, li, 1, 0};
-    if (r == 4 * n - 1 || (r & 1)) return {BOP_FILL, li, 0, 0};
-    return {BOP_GSRB, li, r / 2 + 1, 0};
+    if (r == 0) return {BOP_GSRB, li, 1, 0, 0};
+    if (r == 4 * n - 1 || (r & 1)) return {BOP_FILL, li, 0, 0, 0};
+    return {BOP_GSRB, li, r / 2 + 1, 0, 0};
 }
 
-// Step k of the sequence above; every block of steps has a fixed length.
+// The sequence above with each relaxation block as one step (BOP_BLOCK), walked
+// by a position (part, li, r) that advances without divisions: part 0 the
+// down-sweep of level li from hl to 1 (r: block, child part, parent part), 1
+// the base level (r: base relaxation, fill), 2 the up-sweep of level li from 1
+// to hl (r: correction, diff, the fill when FLY = 0, block), 3 the end.  Blocks
+// of no steps are passed over.
+struct BottomSeq {
+    int part, li, r;
+};
+
 template <int FLY>
-AFG_HD inline BottomStep bottom_step(const BottomArgs &A, int k)
+AFG_HD inline int down_block_steps(const BottomArgs &A)
 {
-    const int rd = FLY ? 2 * A.n_down : 4 * A.n_down;   // the down block has no fill when FLY
-    const int nd = rd + 2;
-    const int nb = A.n_base > 0 ? 2 : 0;
-    const int nu = (FLY ? 2 : 3) + relax_steps<FLY>(A.n_up);
-    if (k < A.hl * nd) {
-        const int li = A.hl - k / nd, r = k % nd;
-        if (r < rd) return relax_step<FLY>(li, r, A.n_down, 0);
-        return {r == rd ? BOP_UC_CHILD : BOP_UC_PARENT, li, 0, 0};
-    }
-    k -= A.hl * nd;
-    if (k < nb) return {k == 0 ? BOP_BASE : BOP_FILL, 0, 0, 0};
-    k -= nb;
-    const int li = 1 + k / nu, r = k % nu;
-    if (r == 0) return {BOP_CORRECT, li, 0, 0};
-    if (r == 1) return {BOP_DIFF, li, 0, 0};
-    if (FLY) return relax_step<FLY>(li, r - 2, A.n_up, 1);
-    if (r == 2) return {BOP_FILL, li, 0, 0};
-    return relax_step<FLY>(li, r - 3, A.n_up, 0);
+    return FLY ? 2 * A.n_down : 4 * A.n_down;   // the down block has no fill when FLY
 }
 
 template <int FLY>
-AFG_HD inline int bottom_steps(const BottomArgs &A)
+AFG_HD inline void seq_settle(const BottomArgs &A, BottomSeq &q)
 {
-    const int rd = FLY ? 2 * A.n_down : 4 * A.n_down;
-    return A.hl * (rd + 2) + (A.n_base > 0 ? 2 : 0) + A.hl * ((FLY ? 2 : 3) + relax_steps<FLY>(A.n_up));
+    constexpr int NU = FLY ? 3 : 4;
+    for (;;) {
+        if (q.part == 0) {
+            if (q.li < 1) {
+                q.part = 1;
+                q.r = 0;
+            } else if (q.r >= 3) {
+                q.li--;
+                q.r = 0;
+            } else if (q.r == 0 && down_block_steps<FLY>(A) == 0) {
+                q.r = 1;
+            } else {
+                return;
+            }
+        } else if (q.part == 1) {
+            if (A.n_base > 0 && q.r < 2) return;
+            q.part = 2;
+            q.li = 1;
+            q.r = 0;
+        } else if (q.part == 2) {
+            if (q.li > A.hl) {
+                q.part = 3;
+                return;
+            }
+            if (q.r >= NU) {
+                q.li++;
+                q.r = 0;
+            } else if (q.r == NU - 1 && relax_steps<FLY>(A.n_up) == 0) {
+                q.r = NU;
+            } else {
+                return;
+            }
+        } else {
+            return;
+        }
+    }
+}
+
+template <int FLY>
+AFG_HD inline BottomSeq seq_first(const BottomArgs &A)
+{
+    BottomSeq q{0, A.hl, 0};
+    seq_settle<FLY>(A, q);
+    return q;
+}
+
+template <int FLY>
+AFG_HD inline void seq_next(const BottomArgs &A, BottomSeq &q)
+{
+    q.r++;
+    seq_settle<FLY>(A, q);
+}
+
+template <int FLY>
+AFG_HD inline BottomStep seq_step(const BottomArgs &A, const BottomSeq &q)
+{
+    if (q.part == 0) {
+        if (q.r == 0) return {BOP_BLOCK, q.li, A.n_down, 0, down_block_steps<FLY>(A)};
+        return {q.r == 1 ? BOP_UC_CHILD : BOP_UC_PARENT, q.li, 0, 0, 0};
+    }
+    if (q.part == 1) return {q.r == 0 ? BOP_BASE : BOP_FILL, 0, 0, 0, 0};
+    if (q.part == 2) {
+        if (q.r == 0) return {BOP_CORRECT, q.li, 0, 0, 0};
+        if (q.r == 1) return {BOP_DIFF, q.li, 0, 0, 0};
+        if (!FLY && q.r == 2) return {BOP_FILL, q.li, 0, 0, 0};
+        return {BOP_BLOCK, q.li, A.n_up, FLY ? 1 : 0, relax_steps<FLY>(A.n_up)};
+    }
+    return {0, A.hl + 1, 0, 0, 0};
 }
 
 // cell counts with their own code in the bottom kernel (others: generic)
@@ -2362,7 +2423,7 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLeve
                         WaveSync wsync, Xshfl xshfl)
 {
     constexpr int NNB = 2 * DIM;
-    constexpr int BFLY = DIM == 2 ? 1 : 0;   // the relaxation blocks' form (bottom_step)
+    constexpr int BFLY = DIM == 2 ? 1 : 0;   // the relaxation blocks' form (relax_step)
     // topology tables of the bottom slots, in LDS
     char *tb = (char *)lds;
     int32_t *t_nb = (int32_t *)tb;
@@ -2460,7 +2521,9 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLeve
     // count first, then on the operation): a level's steps then run from one
     // compact range of code, which stays in the instruction cache.
     // One step on threads t of n; `in_wave` picks the barrier that orders the
-    // passes of BOP_BASE.
+    // passes of BOP_BASE and the steps of a block.  A relaxation block runs
+    // as one step, its level's arguments read once for all its passes and
+    // fills.
     auto exec = [&](const BottomStep st, int t, int n, bool in_wave) {
         const int li = st.li;
         const LevelViewT<R> own = view(li), par = view(li > 0 ? li - 1 : 0);
@@ -2496,10 +2559,25 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLeve
                         sync();
                 });
             } break;
-            case BOP_GSRB: {
+            case BOP_BLOCK: {
                 const LevelArgs L = B.lv[li];
-                const int tot = gsrb_threads<DIM>(L.n, NC ? NC : L.nc);
-                for (int g = t; g < tot; g += n) gsrb_body<DIM, NC>(T, L, own, par, AUTO, st.rb, BFLY ? st.fly : 0, g);
+                const int tg = gsrb_threads<DIM>(L.n, NC ? NC : L.nc);
+                const int tf = fill_threads<DIM>(L.n, NC ? NC : L.nc);
+                for (int r = 0; r < st.m; r++) {
+                    const BottomStep p = relax_step<BFLY>(li, r, st.rb, st.fly);
+                    if (p.op == BOP_GSRB) {
+                        for (int g = t; g < tg; g += n)
+                            gsrb_body<DIM, NC>(T, L, own, par, AUTO, p.rb, BFLY ? p.fly : 0, g);
+                    } else {
+                        for (int g = t; g < tf; g += n) fill_body<DIM, NC>(T, L, own, par, g);
+                    }
+                    if (r + 1 < st.m) {
+                        if (in_wave)
+                            wsync();
+                        else
+                            sync();
+                    }
+                }
             } break;
             case BOP_FILL: {
                 const LevelArgs L = B.lv[li];
@@ -2542,12 +2620,13 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLeve
     };
 
     const int W = A.hl < B.wave_top ? A.hl : B.wave_top;
-    const int n_steps = bottom_steps<BFLY>(A);
-    BottomStep st = bottom_step<BFLY>(A, 0);
-    for (int k = 0; k < n_steps; k++) {
+    BottomSeq q = seq_first<BFLY>(A);
+    BottomStep st = seq_step<BFLY>(A, q);
+    while (q.part < 3) {
         const bool in_wave = st.li <= W;
         if (!in_wave || tid < wave) exec(st, tid, in_wave ? wave : nt, in_wave);
-        const BottomStep nx = k + 1 < n_steps ? bottom_step<BFLY>(A, k + 1) : BottomStep{0, A.hl + 1, 0, 0};
+        seq_next<BFLY>(A, q);
+        const BottomStep nx = seq_step<BFLY>(A, q);
         if (in_wave && nx.li <= W)
             wsync();
         else
