@@ -1498,10 +1498,11 @@ AFG_DEV void uc_tile_d(const LevelArgs &L, R *lds, int box0, int tid, const R *r
 // rhs = rhs + tmp elsewhere (a$D_box_add_cc over the whole array), then
 // tmp = phi (a$D_box_copy_cc) if copy_tmp.  A phi uses the filled ghost
 // values, computed on the fly.  L is the parent level, own its view, par the
-// view of the level below it.
+// view of the level below it.  fly = 0: the face ghosts are already filled
+// (fill_body), and A phi reads them.
 template <int DIM, int NC, typename R, typename W = R>
 AFG_DEV void uc_parent_body(const DevTree &T, const LevelArgs &L, const LevelViewT<R> &own,
-                            const LevelViewT<R> &par, int op_auto, int copy_tmp, int gid)
+                            const LevelViewT<R> &par, int op_auto, int copy_tmp, int gid, int fly = 1)
 {
     const int nc = NC ? NC : L.nc;
     const int s = nc + 2;
@@ -1521,7 +1522,7 @@ AFG_DEV void uc_parent_body(const DevTree &T, const LevelArgs &L, const LevelVie
         if (!is_par) return;
         const bool lpld = lpld_box(T, op_auto, slot);
         R u[6];
-        stencil<DIM>(T, L, own, par, slot, box, nc, i, j, k, q, 1, ldv(box + q), u);
+        stencil<DIM>(T, L, own, par, slot, box, nc, i, j, k, q, fly, ldv(box + q), u);
         R tau;
         if constexpr (std::is_same<R, W>::value) {
             const R a = lsf_box(T, own, slot)
@@ -1545,7 +1546,7 @@ AFG_DEV void uc_parent_body(const DevTree &T, const LevelArgs &L, const LevelVie
         return;
     }
     R v = ldv(box + q);
-    if (n_out == 1) {   // face ghost: fill it
+    if (n_out == 1 && fly) {   // face ghost: fill it
         const int f = oi ? (i > nc) : oj ? 2 + (j > nc) : 4 + (k > nc);
         const int ci = oi ? (i > nc ? nc : 1) : i;
         const int cj = oj ? (j > nc ? nc : 1) : j;
@@ -2598,11 +2599,23 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLeve
                 const LevelArgs Lp = B.lv[li - 1];
                 const LevelViewT<R> ppar = view(li > 1 ? li - 2 : 0);
                 const int tot = Lp.n * ipow((NC ? NC : Lp.nc) + 2, DIM);
+                if constexpr (!BFLY) {
+                    // 3D: the level's ghost fill first, then the parent part
+                    // reads the stored ghosts (the same values as computing
+                    // them in its stencils, with no divergent ghost code there)
+                    const int tf = fill_threads<DIM>(Lp.n, NC ? NC : Lp.nc);
+                    for (int g = t; g < tf; g += n) fill_body<DIM, NC>(T, Lp, par, ppar, g);
+                    if (in_wave)
+                        wsync();
+                    else
+                        sync();
+                }
                 if constexpr (sizeof(R) == 4 && TAU) {
-                    for (int g = t; g < tot; g += n) uc_parent_body<DIM, NC, R, double>(T, Lp, par, ppar, AUTO, 1, g);
+                    for (int g = t; g < tot; g += n)
+                        uc_parent_body<DIM, NC, R, double>(T, Lp, par, ppar, AUTO, 1, g, BFLY);
                     break;
                 }
-                for (int g = t; g < tot; g += n) uc_parent_body<DIM, NC>(T, Lp, par, ppar, AUTO, 1, g);
+                for (int g = t; g < tot; g += n) uc_parent_body<DIM, NC>(T, Lp, par, ppar, AUTO, 1, g, BFLY);
             } break;
             case BOP_CORRECT: {
                 const LevelArgs L = B.lv[li];
