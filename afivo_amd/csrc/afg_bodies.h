@@ -2141,9 +2141,10 @@ enum BottomOp { BOP_GSRB, BOP_FILL, BOP_UC_CHILD, BOP_UC_PARENT, BOP_CORRECT, BO
 
 // op on level li; a colour pass has its colour rb and `fly` (whether it
 // computes its face ghosts); BOP_BLOCK stands for the m steps of a relaxation
-// block of rb sweeps whose first pass has fly (relax_step)
+// block of rb sweeps whose first pass has fly (relax_step), after a ghost fill
+// of the level when `lead` is set
 struct BottomStep {
-    int op, li, rb, fly, m;
+    int op, li, rb, fly, m, lead;
 };
 
 // The form is fixed per dimension at compile time (measured: computing the
@@ -2163,20 +2164,21 @@ template <int FLY>
 AFG_HD inline BottomStep relax_step(int li, int r, int n, int first_fly)
 {
     if (FLY) {
-        if (r == 2 * n) return {BOP_FILL, li, 0, 0, 0};
-        return {BOP_GSRB, li, r + 1, r > 0 ? 1 : first_fly, 0};
+        if (r == 2 * n) return {BOP_FILL, li, 0, 0, 0, 0};
+        return {BOP_GSRB, li, r + 1, r > 0 ? 1 : first_fly, 0, 0};
     }
-    if (r == 0) return {BOP_GSRB, li, 1, 0, 0};
-    if (r == 4 * n - 1 || (r & 1)) return {BOP_FILL, li, 0, 0, 0};
-    return {BOP_GSRB, li, r / 2 + 1, 0, 0};
+    if (r == 0) return {BOP_GSRB, li, 1, 0, 0, 0};
+    if (r == 4 * n - 1 || (r & 1)) return {BOP_FILL, li, 0, 0, 0, 0};
+    return {BOP_GSRB, li, r / 2 + 1, 0, 0, 0};
 }
 
-// The sequence above with each relaxation block as one step (BOP_BLOCK), walked
-// by a position (part, li, r) that advances without divisions: part 0 the
-// down-sweep of level li from hl to 1 (r: block, child part, parent part), 1
-// the base level (r: base relaxation, fill), 2 the up-sweep of level li from 1
-// to hl (r: correction, diff, the fill when FLY = 0, block), 3 the end.  Blocks
-// of no steps are passed over.
+// The sequence above with each relaxation block as one step (BOP_BLOCK, which
+// also holds the up-sweep's fill before its first pass when FLY = 0) and the
+// base relaxation with its ghost fill as one step, walked by a position
+// (part, li, r) that advances without divisions: part 0 the down-sweep of
+// level li from hl to 1 (r: block, child part, parent part), 1 the base level
+// (r: base relaxation and fill), 2 the up-sweep of level li from 1 to hl (r:
+// correction, diff, block), 3 the end.  Blocks of no steps are passed over.
 struct BottomSeq {
     int part, li, r;
 };
@@ -2190,7 +2192,7 @@ AFG_HD inline int down_block_steps(const BottomArgs &A)
 template <int FLY>
 AFG_HD inline void seq_settle(const BottomArgs &A, BottomSeq &q)
 {
-    constexpr int NU = FLY ? 3 : 4;
+    constexpr int NU = 3;
     for (;;) {
         if (q.part == 0) {
             if (q.li < 1) {
@@ -2205,7 +2207,7 @@ AFG_HD inline void seq_settle(const BottomArgs &A, BottomSeq &q)
                 return;
             }
         } else if (q.part == 1) {
-            if (A.n_base > 0 && q.r < 2) return;
+            if (A.n_base > 0 && q.r < 1) return;
             q.part = 2;
             q.li = 1;
             q.r = 0;
@@ -2217,7 +2219,7 @@ AFG_HD inline void seq_settle(const BottomArgs &A, BottomSeq &q)
             if (q.r >= NU) {
                 q.li++;
                 q.r = 0;
-            } else if (q.r == NU - 1 && relax_steps<FLY>(A.n_up) == 0) {
+            } else if (q.r == NU - 1 && FLY && relax_steps<FLY>(A.n_up) == 0) {
                 q.r = NU;
             } else {
                 return;
@@ -2247,17 +2249,16 @@ template <int FLY>
 AFG_HD inline BottomStep seq_step(const BottomArgs &A, const BottomSeq &q)
 {
     if (q.part == 0) {
-        if (q.r == 0) return {BOP_BLOCK, q.li, A.n_down, 0, down_block_steps<FLY>(A)};
-        return {q.r == 1 ? BOP_UC_CHILD : BOP_UC_PARENT, q.li, 0, 0, 0};
+        if (q.r == 0) return {BOP_BLOCK, q.li, A.n_down, 0, down_block_steps<FLY>(A), 0};
+        return {q.r == 1 ? BOP_UC_CHILD : BOP_UC_PARENT, q.li, 0, 0, 0, 0};
     }
-    if (q.part == 1) return {q.r == 0 ? BOP_BASE : BOP_FILL, 0, 0, 0, 0};
+    if (q.part == 1) return {BOP_BASE, 0, 0, 0, 0, 0};
     if (q.part == 2) {
-        if (q.r == 0) return {BOP_CORRECT, q.li, 0, 0, 0};
-        if (q.r == 1) return {BOP_DIFF, q.li, 0, 0, 0};
-        if (!FLY && q.r == 2) return {BOP_FILL, q.li, 0, 0, 0};
-        return {BOP_BLOCK, q.li, A.n_up, FLY ? 1 : 0, relax_steps<FLY>(A.n_up)};
+        if (q.r == 0) return {BOP_CORRECT, q.li, 0, 0, 0, 0};
+        if (q.r == 1) return {BOP_DIFF, q.li, 0, 0, 0, 0};
+        return {BOP_BLOCK, q.li, A.n_up, FLY ? 1 : 0, relax_steps<FLY>(A.n_up), FLY ? 0 : 1};
     }
-    return {0, A.hl + 1, 0, 0, 0};
+    return {0, A.hl + 1, 0, 0, 0, 0};
 }
 
 // cell counts with their own code in the bottom kernel (others: generic)
@@ -2536,34 +2537,53 @@ AFG_DEV void bottom_run(const DevTree &Tg, const MgVarsT<R> &V, const BottomLeve
                 // 2 n_base colour passes of gsrb_boxes on level 0: the first
                 // reads the stored ghosts, the others the neighbours the ghost
                 // fill after the previous pass would store (cells of the other
-                // colour, or K + c1 phi at physical boundaries)
+                // colour, or K + c1 phi at physical boundaries); then the
+                // level's ghost fill
                 const LevelArgs L = B.lv[0];
-                if constexpr (NC == 2) {
-                    if (B.base_fast && wave == 64) {
-                        // cells of a box on the lanes of a wavefront, the boxes' groups
-                        // over every wavefront of the threads of this step
-                        const int lane = t & 63, w = t >> 6, nw = n >> 6;
-                        if constexpr (sizeof(R) == 4) {
-                            if (A.base64) {
-                                base_wave_run<DIM, AUTO, R, double>(T, own, B, A, L.dx2, L.n, lane, xshfl, w, nw);
-                                break;
+                auto relax = [&] {
+                    if constexpr (NC == 2) {
+                        if (B.base_fast && wave == 64) {
+                            // cells of a box on the lanes of a wavefront, the boxes' groups
+                            // over every wavefront of the threads of this step
+                            const int lane = t & 63, w = t >> 6, nw = n >> 6;
+                            if constexpr (sizeof(R) == 4) {
+                                if (A.base64) {
+                                    base_wave_run<DIM, AUTO, R, double>(T, own, B, A, L.dx2, L.n, lane, xshfl, w, nw);
+                                    return;
+                                }
                             }
+                            base_wave_run<DIM, AUTO, R, R>(T, own, B, A, L.dx2, L.n, lane, xshfl, w, nw);
+                            return;
                         }
-                        base_wave_run<DIM, AUTO, R, R>(T, own, B, A, L.dx2, L.n, lane, xshfl, w, nw);
-                        break;
                     }
-                }
-                base_list_run<DIM, AUTO, R>(T, own, B, A, L, t, n, [&] {
-                    if (in_wave)
-                        wsync();
-                    else
-                        sync();
-                });
+                    base_list_run<DIM, AUTO, R>(T, own, B, A, L, t, n, [&] {
+                        if (in_wave)
+                            wsync();
+                        else
+                            sync();
+                    });
+                };
+                relax();
+                if (in_wave)
+                    wsync();
+                else
+                    sync();
+                const int tf = fill_threads<DIM>(L.n, NC ? NC : L.nc);
+                for (int g = t; g < tf; g += n) fill_body<DIM, NC>(T, L, own, par, g);
             } break;
             case BOP_BLOCK: {
                 const LevelArgs L = B.lv[li];
                 const int tg = gsrb_threads<DIM>(L.n, NC ? NC : L.nc);
                 const int tf = fill_threads<DIM>(L.n, NC ? NC : L.nc);
+                if (st.lead) {
+                    for (int g = t; g < tf; g += n) fill_body<DIM, NC>(T, L, own, par, g);
+                    if (st.m > 0) {
+                        if (in_wave)
+                            wsync();
+                        else
+                            sync();
+                    }
+                }
                 for (int r = 0; r < st.m; r++) {
                     const BottomStep p = relax_step<BFLY>(li, r, st.rb, st.fly);
                     if (p.op == BOP_GSRB) {
